@@ -1,0 +1,275 @@
+/*
+ * phipict_hip.h — C ABI of the MI355X (gfx950) PISO-step and linear-solver library.
+ *
+ * Every entry point takes plain pointers and sizes: device pointers for field data,
+ * host pointers for descriptors and result records, and a hipStream_t passed as
+ * `void *stream` (NULL = the null stream). Scalars that the reference takes as
+ * 1-element host tensors (timeStep, maxit, tol) are plain values or host arrays here.
+ * All entry points return 0 on success and a negative PP_ERR_* code otherwise;
+ * pp_last_error() returns the message of the last failure on the calling thread.
+ *
+ * Each declaration names the reference interface it replaces (paths relative to the
+ * phiPICT source tree). The reference binds those as pybind11 members of `phipict._C`
+ * (csrc/phipict/bindings.cpp:44-874); a maintainer binds these with ctypes, see the
+ * stub in INTEGRATION.md.
+ *
+ * Layout conventions (identical to the reference, csrc/phipict/piso/device/launch.cuh:134-140,
+ * csrc/phipict/domain/domain_structs_gpu.h:186-194):
+ *  - block fields are [batch][channel][z][y][x] (x fastest), a block's cells are numbered
+ *    x + sx*(y + sy*z);
+ *  - flat solve vectors are [batch][channel][num_cells], block b's cells start at
+ *    its global_offset;
+ *  - CSR matrices: int32 row[n+1], int32 col[nnz] (ascending column per row),
+ *    values [num_matrices][nnz];
+ *  - a cell transform is {M[d][d], Minv[d][d], det} row-major, 2d^2+1 scalars.
+ */
+#ifndef PHIPICT_HIP_H
+#define PHIPICT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- enums (values are the reference's, csrc/phipict/domain/domain_structs_gpu.h:146-223) ---- */
+enum { PP_F32 = 4, PP_F64 = 8 };
+enum { PP_BOUND_CONNECTED = 20, PP_BOUND_PERIODIC = 21, PP_BOUND_FIXED = 30 };
+enum { PP_BC_DIRICHLET = 0, PP_BC_NEUMANN = 1 };
+enum { PP_POT_INSULATING = 0, PP_POT_OPEN = 1, PP_POT_DIRICHLET = 2, PP_POT_THIN_WALL = 3 };
+enum { PP_CONV_NORM2 = 0, PP_CONV_NORM2_NORMALIZED = 1 };
+enum {
+    PP_OK = 0,
+    PP_ERR_ARG = -1,      /* invalid argument / inconsistent sizes */
+    PP_ERR_HIP = -2,      /* HIP runtime error */
+    PP_ERR_UNSUPPORTED = -3,
+    PP_ERR_NOMEM = -4,
+};
+
+/* ---- domain descriptors ---- */
+
+/* One face of a block, 88 bytes. Face order [-x,+x,-y,+y,-z,+z]. */
+typedef struct pp_boundary_desc {
+    int32_t type;          /* PP_BOUND_* */
+    int32_t vel_type;      /* FIXED: PP_BC_DIRICHLET or PP_BC_NEUMANN (velocity) */
+    int32_t vel_static;    /* FIXED: 1 = velocity [vb][d]; 0 = [vb][d][face cells] */
+    int32_t vel_batched;   /* FIXED: 1 = velocity has one slice per environment (vb = batch) */
+    int32_t has_transform; /* FIXED: face transforms [face cells][2d^2+1] given */
+    int32_t pot_type;      /* FIXED: PP_POT_* (used when pot_types == 0) */
+    int32_t conn_block;    /* CONNECTED: index of the connected block */
+    int32_t conn_axes[3];  /* CONNECTED: face-format axes of the connection */
+    int32_t pad0, pad1;
+    uint64_t vel;          /* device pointer (dtype) */
+    uint64_t transform;    /* device pointer (dtype) or 0 */
+    uint64_t pot_types;    /* device pointer int8 [face cells] or 0 */
+    uint64_t pot_values;   /* device pointer (dtype) [face cells] or 0 */
+    double   pot_cw;       /* thin-wall conductance ratio */
+} pp_boundary_desc;
+
+/* One block, 96 + 6*88 = 624 bytes. */
+typedef struct pp_block_desc {
+    int32_t size[3];        /* cells in x, y, z (1 for unused dimensions) */
+    int32_t cells;          /* size[0]*size[1]*size[2] */
+    int32_t global_offset;  /* first global cell index */
+    int32_t csr_offset;     /* first CSR entry of the block's rows */
+    int32_t has_transform;
+    int32_t vsource_mode;   /* 0 none, 1 static [vb][d], 2 per cell [vb][d][cells] */
+    int32_t vsource_batched;/* source has one slice per environment */
+    int32_t visc_mode;      /* 0 domain viscosity, 1 static [1], 2 per cell [cells] */
+    int32_t pad0, pad1;
+    uint64_t transform;     /* [cells][2d^2+1] */
+    uint64_t velocity;      /* [batch][d][cells] */
+    uint64_t pressure;      /* [batch][cells] */
+    uint64_t epot;          /* [batch][cells] or 0 */
+    uint64_t vsource;       /* see vsource_mode */
+    uint64_t viscosity;     /* see visc_mode */
+    pp_boundary_desc bnd[6];
+} pp_block_desc;
+
+typedef struct pp_csr {
+    const int32_t *row;     /* [n+1] */
+    const int32_t *col;     /* [nnz] */
+    void *val;              /* [num_matrices][nnz] */
+    int32_t n;
+    int32_t nnz;
+    int32_t num_matrices;   /* 1 = shared by all right-hand sides */
+    int32_t pad;
+} pp_csr;
+
+/* Host-side view of a domain (the reference's DomainGPU, domain_structs_gpu.h:334-445). */
+typedef struct pp_domain {
+    int32_t dims;           /* 2 or 3 */
+    int32_t num_blocks;
+    int32_t num_cells;      /* per environment */
+    int32_t batch;          /* environments */
+    int32_t dtype;          /* PP_F32 / PP_F64 */
+    int32_t max_block_cells;
+    double  viscosity;      /* domain viscosity */
+    const pp_block_desc *blocks; /* DEVICE pointer to num_blocks descriptors */
+    const void *time_step;  /* DEVICE pointer, [batch] (dtype) */
+    void *A;                /* [batch][n] advection diagonal */
+    void *velocity_rhs;     /* [batch][d][n] */
+    void *velocity_result;  /* [batch][d][n] */
+    void *pressure_rhs;     /* [batch][d][n] */
+    void *pressure_rhs_div; /* [batch][n] */
+    void *pressure_result;  /* [batch][n] */
+    void *epot_rhs;         /* [batch][n] */
+    void *epot_result;      /* [batch][n] */
+    pp_csr C;               /* advection matrix, values [batch][nnz] */
+    pp_csr P;               /* pressure matrix, values [batch][nnz] */
+    pp_csr E;               /* electric potential matrix, values [nnz] */
+} pp_domain;
+
+/* The reference's LinearSolverResultInfo (csrc/phipict/solvers/linear_solvers.h:180-185). */
+typedef struct pp_solve_result {
+    double  final_residual;
+    int32_t used_iterations;
+    int32_t converged;
+    int32_t is_finite_residual;
+    int32_t pad;
+} pp_solve_result;
+
+/* One AMG level above the coarsest (reference layout of _C.AMGPCGSolve levels,
+ * csrc/phipict/bindings.cpp:764-776: A row,col,val,dinv, R row,col,val, P row,col,val). */
+typedef struct pp_amg_level {
+    int32_t n, nnz_a, n_coarse, nnz_r, nnz_p, a_batched; /* a_batched: A values [B][nnz_a], dinv [B][n] */
+    const int32_t *a_row, *a_col; const void *a_val; const void *dinv;
+    const int32_t *r_row, *r_col; const void *r_val;
+    const int32_t *p_row, *p_col; const void *p_val;
+} pp_amg_level;
+
+/* ---- library ---- */
+const char *pp_last_error(void);
+int pp_version(void);
+int pp_device_sync(void);
+
+/* ---- grid metrics ----
+ * replaces CoordsToTransforms (csrc/phipict/grid/grid_gen.cu:356-391) and
+ * CoordsToFaceTransforms (grid_gen.cu:529-): coords [d][(sz+1)][sy+1][sx+1] vertices ->
+ * out [cells][2d^2+1] resp. [d][(sz+1)][sy+1][sx+1][2d^2+1]. */
+int pp_coords_to_transforms(int32_t dims, const int32_t *cells, const void *coords, void *out,
+                            int32_t dtype, void *stream);
+int pp_coords_to_face_transforms(int32_t dims, const int32_t *cells, const void *coords, void *out,
+                                 int32_t dtype, void *stream);
+
+/* ---- PISO assembly (non-orthogonal correction flags accepted; the corner terms are
+ * zero on the rectilinear grids these entry points are validated on) ---- */
+
+/* CSR pattern shared by C, P and E: row[n+1] and ascending columns (the row layout the
+ * reference's matrix kernels write, csrc/phipict/piso/device/topology.cuh:168-233, and
+ * the block offsets of Domain::PrepareSolve, csrc/phipict/domain/domain.cpp:426-561).
+ * Needs dom->blocks with csr_offset set; written once per domain. */
+int pp_csr_pattern(const pp_domain *dom, int32_t *row, int32_t *col, void *stream);
+
+/* replaces SetupAdvectionMatrixEulerImplicit (csrc/phipict/piso/advection.cu:1230-1250),
+ * kernel PISO_build_matrix advection.cu:26-306: writes C (row, col, values) and A. */
+int pp_setup_advection_matrix(const pp_domain *dom, int32_t non_ortho_flags, void *stream);
+/* replaces SetupAdvectionVelocityEulerImplicitRHS (advection.cu:1375-1406),
+ * kernel kPISO_build_advection_RHS advection.cu:799-955: writes velocity_rhs. */
+int pp_setup_advection_velocity(const pp_domain *dom, int32_t non_ortho_flags,
+                                int32_t apply_pressure_gradient, void *stream);
+/* replaces SetupPressureMatrix (csrc/phipict/piso/pressure.cu:947-975),
+ * kernel PISO_build_pressure_matrix pressure.cu:25-193: writes P. */
+int pp_setup_pressure_matrix(const pp_domain *dom, int32_t non_ortho_flags, void *stream);
+/* replaces SetupPressureRHS (pressure.cu:978-1027): kernels PISO_build_pressure_rhs
+ * pressure.cu:349-522 and k_computePressureRHSdivergenceFromFlux :755-803. */
+int pp_setup_pressure_rhs(const pp_domain *dom, int32_t non_ortho_flags, int32_t time_step_norm,
+                          void *stream);
+/* replaces SetupPressureRHSdiv (pressure.cu:1030-1070): divergence only. */
+int pp_setup_pressure_rhs_div(const pp_domain *dom, int32_t non_ortho_flags, int32_t time_step_norm,
+                              void *stream);
+/* replaces CorrectVelocity version 1 (csrc/phipict/piso/velocity_correction.cu:216-271),
+ * kernel PISO_update_velocity velocity_correction.cu:26-60. */
+int pp_correct_velocity(const pp_domain *dom, int32_t time_step_norm, void *stream);
+/* replaces SetupEpotMatrix (csrc/phipict/mhd/potential.cu:250-262),
+ * kernel PISO_build_epot_matrix potential.cu:29-233: writes E. */
+int pp_setup_epot_matrix(const pp_domain *dom, int32_t non_ortho_flags, void *stream);
+/* replaces ComputeEpotRHS (potential.cu:325-370), kernel k_computeEpotRHS potential.cu:272-322:
+ * vector_field [batch][d][n] -> divergence [batch][n]. */
+int pp_compute_epot_rhs(const pp_domain *dom, const void *vector_field, void *divergence, void *stream);
+/* replaces ComputeCurrentDensityFaceBased (potential.cu:918-960), kernel
+ * k_computeCurrentDensityFaceBased potential.cu:888-916: phi [batch][n],
+ * u x B [batch][3][n] (2D: 3 components) -> J [batch][3][n]. */
+int pp_compute_current_density(const pp_domain *dom, const void *phi, const void *uxb, void *j_out,
+                               void *stream);
+/* replaces Copy{Velocity,Pressure,Epot}Result{To,From}Blocks (csrc/phipict/piso/result_copy.cu:19-475).
+ * field: 0 velocity, 1 pressure, 2 epot; to_blocks: 1 flat->blocks, 0 blocks->flat. */
+int pp_copy_result(const pp_domain *dom, int32_t field, int32_t to_blocks, void *stream);
+
+/* ---- linear solvers ---- */
+
+/* replaces SolveLinear (csrc/phipict/solvers/linear_solve.cu:131-164) with the fused
+ * solvers krylov::cgSolve / krylov::bicgstabSolve (csrc/phipict/solvers/krylov/krylov.cu:1058-1186).
+ * rhs, x: [nrhs][n] device; RHS k uses matrix k / (nrhs / A->num_matrices).
+ * tols: host array of 1 or nrhs absolute tolerances. matrix_rank_deficient: the operator
+ * has the constant nullspace; the solve keeps the residual mean-free (projection).
+ * results: host array [nrhs]. */
+int pp_solve_linear(const pp_csr *A, const void *rhs, void *x, int32_t nrhs, int32_t dtype,
+                    int32_t maxit, const double *tols, int32_t ntols, int32_t criterion,
+                    int32_t use_bicg, int32_t matrix_rank_deficient, int32_t residual_reset_steps,
+                    int32_t transpose_a, int32_t return_best, void *stream, pp_solve_result *results);
+
+/* replaces AMGPCGSolve (csrc/phipict/bindings.cpp:764-776 -> krylov::amgPcgSolveTorch,
+ * krylov.cu:1565-1612): AMG-preconditioned CG with one V-cycle per iteration.
+ * coarse_pinv: [nc][nc] or [nrhs][nc][nc] (coarse_batched) in hdtype. */
+int pp_amg_pcg_solve(const pp_csr *A, const pp_amg_level *levels, int32_t num_levels,
+                     const void *coarse_pinv, int32_t nc, int32_t coarse_batched, int32_t hdtype,
+                     const void *rhs, void *x, int32_t nrhs, int32_t dtype, int32_t maxit,
+                     double tol, int32_t normalized, int32_t return_best, int32_t project_constant,
+                     int32_t pre_sweeps, int32_t post_sweeps, double omega,
+                     const double *tols, int32_t ntols, void *stream, pp_solve_result *results);
+
+/* replaces ReleaseKrylovWorkspaces (bindings.cpp:777, krylov.cu:921-924). */
+int pp_release_workspaces(void);
+
+/* replaces SparseOuterProduct (linear_solve.cu:166-190, SDDMM legacy_cg.cu:888-927):
+ * out.val[k] = a[row(k)] * b[col(k)] on the pattern of out. */
+int pp_sparse_outer_product(const void *a, const void *b, const pp_csr *out, int32_t dtype, void *stream);
+
+/* replaces krylov::csrTranspose (krylov.cu:1620-1652): pattern and values of A^T into
+ * t_row[n+1], t_col[nnz], t_val[num_matrices][nnz] (caller-allocated, device). */
+int pp_csr_transpose(const pp_csr *A, int32_t dtype, int32_t *t_row, int32_t *t_col, void *t_val,
+                     void *stream);
+
+/* Sparse matrix-vector product y = A x for nrhs right-hand sides (x, y: [nrhs][n]). */
+int pp_spmv(const pp_csr *A, const void *x, void *y, int32_t nrhs, int32_t dtype, void *stream);
+
+/* ---- AMG setup (host) ----
+ * replaces build_amg_hierarchy (src/phipict/solvers/amg.py:450-598), whose classical
+ * Ruge-Stueben coarsening comes from pyamg (ruge_stuben_solver, strength
+ * ('classical', theta)). Host CSR in double precision. Returns a handle. */
+int pp_amg_setup(int32_t n, const int32_t *row, const int32_t *col, const double *val,
+                 double theta, int32_t max_levels, int32_t max_coarse, void **handle);
+/* number of levels including the coarsest; sizes of level l (l < levels-1 has R, P). */
+int pp_amg_num_levels(void *handle);
+int pp_amg_level_sizes(void *handle, int32_t level, int32_t *n, int32_t *nnz_a, int32_t *nnz_p,
+                       int32_t *nnz_r);
+/* copy level l into caller buffers (host); any pointer may be NULL. */
+int pp_amg_level_copy(void *handle, int32_t level, int32_t *a_row, int32_t *a_col, double *a_val,
+                      int32_t *p_row, int32_t *p_col, double *p_val, int32_t *r_row, int32_t *r_col,
+                      double *r_val, int32_t *splitting);
+int pp_amg_free(void *handle);
+
+/* Galerkin refresh on the device (replaces galerkin_refresh, amg.py:749-847):
+ * a precomputed product map turns fine operator values into coarse operator values,
+ * out[e] = sum_{t in seg(e)} w[t] * in[src[t]]. seg: [n_out+1]. */
+int pp_segmented_gather(const int32_t *seg, const int32_t *src, const void *w, int32_t n_out,
+                        const void *in, void *out, int32_t nmat, int32_t in_stride,
+                        int32_t out_stride, int32_t dtype, void *stream);
+/* Galerkin product C = R A P into the fixed pattern (c_row, c_col) of C, accumulated in
+ * double (galerkin_refresh's _galerkin_product, src/phipict/solvers/amg.py:662-696, 749-847).
+ * R and P values are double; A holds A->num_matrices value arrays (a_dtype); C gets one
+ * value array per A matrix (c_dtype). */
+int pp_galerkin_rap(const int32_t *r_row, const int32_t *r_col, const double *r_val, const pp_csr *A, int32_t a_dtype,
+                    const int32_t *p_row, const int32_t *p_col, const double *p_val, const int32_t *c_row,
+                    const int32_t *c_col, void *c_val, int32_t c_n, int32_t c_nnz, int32_t c_dtype, void *stream);
+/* inverse diagonal of a CSR matrix (rows without a usable diagonal get 1):
+ * dinv [num_matrices][n] in dtype_out from values in dtype. */
+int pp_inverse_diagonal(const pp_csr *A, int32_t dtype, void *dinv, int32_t dtype_out, void *stream);
+/* values cast: out[i] = (dtype_out) in[i] */
+int pp_cast(const void *in, int32_t dtype_in, void *out, int32_t dtype_out, int64_t count, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PHIPICT_HIP_H */

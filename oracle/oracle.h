@@ -1,0 +1,68 @@
+/*
+ * ORACLE — test infrastructure only. A plain-C, single-threaded restatement of the
+ * phiPICT reference algorithms on the PISO hot path (per-cell assembly kernels, fused
+ * CG/BiCGStab recurrences, Ruge-Stueben AMG and its V-cycle PCG). Only tests/,
+ * __graft_entry__.smoke() and bench.py's CPU baseline may load it, and only as the
+ * checker; the product path never calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to the
+ * phiPICT source tree). Descriptors here are the oracle's own (host pointers, one
+ * environment), independent of the product's include/phipict_hip.h.
+ */
+#ifndef PHIPICT_ORACLE_H
+#define PHIPICT_ORACLE_H
+#include <stdint.h>
+
+enum { OB_CONNECTED = 20, OB_PERIODIC = 21, OB_FIXED = 30 };
+enum { OB_POT_INSULATING = 0, OB_POT_OPEN = 1, OB_POT_DIRICHLET = 2, OB_POT_THIN_WALL = 3 };
+
+typedef struct ob_bound {
+    int32_t type;          /* OB_* */
+    int32_t vel_neumann;   /* FIXED: velocity BC type is NEUMANN */
+    int32_t vel_static;    /* FIXED: velocity [d] (1) or [d][face cells] (0) */
+    int32_t has_transform; /* FIXED: face transform [face cells][2d^2+1] */
+    int32_t pot_type;      /* FIXED: potential BC (when pot_types is NULL) */
+    int32_t conn_block;    /* CONNECTED */
+    int32_t conn_axes[3];  /* CONNECTED: face-format axes */
+    int32_t pad;
+    const void *vel;
+    const void *transform;
+    const int8_t *pot_types;
+    const void *pot_values;
+    double pot_cw;
+} ob_bound;
+
+typedef struct ob_block {
+    int32_t size[3];
+    int32_t global_offset;
+    int32_t has_transform;
+    int32_t vsource_mode;  /* 0 none, 1 [d], 2 [d][cells] */
+    int32_t visc_mode;     /* 0 domain, 1 [1], 2 [cells] */
+    int32_t pad;
+    const void *transform; /* [cells][2d^2+1] */
+    const void *velocity;  /* [d][cells] */
+    const void *pressure;  /* [cells] */
+    const void *vsource;
+    const void *viscosity;
+    ob_bound b[6];
+} ob_block;
+
+typedef struct ob_domain {
+    int32_t dims;
+    int32_t nblocks;
+    int32_t ncells;
+    int32_t pad;
+    double viscosity;
+    double dt;
+    const ob_block *blocks;
+} ob_domain;
+
+typedef struct ob_result {
+    double final_residual;
+    int32_t used_iterations;
+    int32_t converged;
+    int32_t is_finite;
+    int32_t pad;
+} ob_result;
+
+#endif

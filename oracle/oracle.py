@@ -1,0 +1,492 @@
+"""ORACLE — test infrastructure only.
+
+ctypes wrapper of the C restatement (``oracle/oracle.c``, ``oracle/oracle_amg.c``) of the
+reference's PISO kernels, Krylov recurrences and AMG, plus an independent host-side domain
+description (vertex coordinates, face specifications) from which oracle domains are built.
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU baseline use this module, and only
+as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle.so"
+
+OB_CONNECTED, OB_PERIODIC, OB_FIXED = 20, 21, 30
+POT_INSULATING, POT_OPEN, POT_DIRICHLET, POT_THIN_WALL = 0, 1, 2, 3
+
+
+def build() -> Path:
+    """Compile the oracle with its Makefile (gcc)."""
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+class OBound(C.Structure):
+    _fields_ = [("type", C.c_int32), ("vel_neumann", C.c_int32), ("vel_static", C.c_int32),
+                ("has_transform", C.c_int32), ("pot_type", C.c_int32), ("conn_block", C.c_int32),
+                ("conn_axes", C.c_int32 * 3), ("pad", C.c_int32), ("vel", C.c_void_p), ("transform", C.c_void_p),
+                ("pot_types", C.c_void_p), ("pot_values", C.c_void_p), ("pot_cw", C.c_double)]
+
+
+class OBlock(C.Structure):
+    _fields_ = [("size", C.c_int32 * 3), ("global_offset", C.c_int32), ("has_transform", C.c_int32),
+                ("vsource_mode", C.c_int32), ("visc_mode", C.c_int32), ("pad", C.c_int32),
+                ("transform", C.c_void_p), ("velocity", C.c_void_p), ("pressure", C.c_void_p),
+                ("vsource", C.c_void_p), ("viscosity", C.c_void_p), ("b", OBound * 6)]
+
+
+class ODomain(C.Structure):
+    _fields_ = [("dims", C.c_int32), ("nblocks", C.c_int32), ("ncells", C.c_int32), ("pad", C.c_int32),
+                ("viscosity", C.c_double), ("dt", C.c_double), ("blocks", C.POINTER(OBlock))]
+
+
+class OResult(C.Structure):
+    _fields_ = [("final_residual", C.c_double), ("used_iterations", C.c_int32), ("converged", C.c_int32),
+                ("is_finite", C.c_int32), ("pad", C.c_int32)]
+
+
+_LIB = None
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        L.o_amg_setup.restype = C.c_void_p
+        L.o_amg_setup.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int]
+        L.o_amg_num_levels.argtypes = [C.c_void_p]
+        L.o_amg_level_sizes.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.o_amg_level_copy.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 10
+        L.o_amg_free.argtypes = [C.c_void_p]
+        L.o_galerkin.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int] + [C.c_void_p] * 9 + [C.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _p(a: Optional[np.ndarray]) -> Optional[C.c_void_p]:
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def sfx(dtype) -> str:
+    return "_f64" if np.dtype(dtype) == np.float64 else "_f32"
+
+
+# ----------------------------------------------------------------------------- case spec
+@dataclass
+class FaceSpec:
+    """One face of a block: 'periodic', 'fixed' (Dirichlet/Neumann velocity, potential BC) or
+    'connect' (to face ``other_face`` of block ``other`` with tangential target faces)."""
+
+    kind: str = "periodic"
+    velocity: Optional[np.ndarray] = None      # [d] static or [d, (z), y, x] face-shaped
+    neumann: bool = False
+    pot: int = POT_INSULATING
+    pot_types: Optional[np.ndarray] = None     # int8 per face cell
+    pot_values: Optional[np.ndarray] = None    # per face cell
+    cw: float = 0.0
+    other: int = -1
+    other_face: int = -1
+    axis1: int = -1
+    axis2: int = -1
+
+
+@dataclass
+class BlockSpec:
+    vertices: np.ndarray                       # [d, (z+1), y+1, x+1]
+    faces: list = field(default_factory=list)  # 2d FaceSpec
+    velocity: Optional[np.ndarray] = None      # [d, (z), y, x]
+    pressure: Optional[np.ndarray] = None      # [(z), y, x]
+    vsource: Optional[np.ndarray] = None       # [d] or [d, (z), y, x]
+    viscosity: Optional[np.ndarray] = None     # [1] or [(z), y, x]
+
+    @property
+    def dims(self) -> int:
+        return self.vertices.shape[0]
+
+    @property
+    def cells_zyx(self) -> list:
+        return [s - 1 for s in self.vertices.shape[1:]]
+
+    @property
+    def size_xyz(self) -> list:
+        s = list(reversed(self.cells_zyx))
+        return s + [1] * (3 - len(s))
+
+    @property
+    def ncells(self) -> int:
+        return int(np.prod(self.cells_zyx))
+
+
+def connection_axes(dims: int, f1: int, f2: int, c1: int, c2: int):
+    """Face-format axes of both sides of a block connection (boundaries.cpp:857-891)."""
+    if c1 < 0:
+        c1 = (((f2 >> 1) + 1) % dims) << 1
+    if c2 < 0 and dims > 2:
+        c2 = (((f2 >> 1) + 2) % dims) << 1
+    a1, a2 = [f2], [f1]
+    if dims > 1:
+        a1.append(c1)
+        d1, d2 = f1 >> 1, f2 >> 1
+        swapped = False
+        if dims == 2 or (c1 >> 1) == (d2 + 1) % dims:
+            a2.append((((d1 + 1) % dims) << 1) | (c1 & 1))
+        else:
+            a2.append((((d1 + 2) % dims) << 1) | (c2 & 1))
+            swapped = True
+        if dims > 2:
+            a1.append(c2)
+            a2.append((((d1 + 1) % dims) << 1) | (c1 & 1) if swapped else (((d1 + 2) % dims) << 1) | (c2 & 1))
+    return a1 + [0] * (3 - len(a1)), a2 + [0] * (3 - len(a2))
+
+
+def transforms(coords: np.ndarray, dtype=np.float64) -> np.ndarray:
+    """Cell transforms [cells][2d^2+1] of vertex coordinates [d, (z+1), y+1, x+1]."""
+    d = coords.shape[0]
+    cells = [s - 1 for s in reversed(coords.shape[1:])] + [1] * (3 - d)
+    c = np.ascontiguousarray(coords, dtype=dtype)
+    out = np.zeros((int(np.prod(cells)), 2 * d * d + 1), dtype=dtype)
+    getattr(lib(), "o_transforms" + sfx(dtype))(d, (C.c_int32 * 3)(*cells), _p(c), _p(out))
+    return out
+
+
+def face_transforms(coords: np.ndarray, dtype=np.float64) -> np.ndarray:
+    """Face transforms [d, (z+1), y+1, x+1, 2d^2+1]."""
+    d = coords.shape[0]
+    cells = [s - 1 for s in reversed(coords.shape[1:])] + [1] * (3 - d)
+    c = np.ascontiguousarray(coords, dtype=dtype)
+    out = np.zeros((d, *coords.shape[1:], 2 * d * d + 1), dtype=dtype)
+    getattr(lib(), "o_face_transforms" + sfx(dtype))(d, (C.c_int32 * 3)(*cells), _p(c), _p(out))
+    return out
+
+
+def boundary_slice(ft: np.ndarray, face: int) -> np.ndarray:
+    """Face transforms of one block face (Block::GetFaceTransformBoundarySlice, block.cpp:790-816)."""
+    d = ft.shape[0]
+    axis, upper = face >> 1, face & 1
+    a = ft[axis]
+    idx = []
+    for dd in range(d - 1, -1, -1):
+        if dd == axis:
+            idx.append(slice(-1, None) if upper else slice(0, 1))
+        else:
+            idx.append(slice(0, -1))
+    idx.append(slice(None))
+    return np.ascontiguousarray(a[tuple(idx)])
+
+
+class OracleDomain:
+    """Host domain for the oracle kernels, built from BlockSpecs in one dtype."""
+
+    def __init__(self, specs: Sequence[BlockSpec], viscosity: float, dt: float, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.specs = list(specs)
+        self.dims = specs[0].dims
+        self.keep = []
+        self.blocks = (OBlock * len(specs))()
+        off = 0
+        self.offsets = []
+        self.velocity, self.pressure, self.vsource = [], [], []
+        for bi, s in enumerate(specs):
+            b = self.blocks[bi]
+            for i in range(3):
+                b.size[i] = s.size_xyz[i]
+            b.global_offset = off
+            self.offsets.append(off)
+            off += s.ncells
+            T = transforms(s.vertices, self.dtype)
+            FT = face_transforms(s.vertices, self.dtype)
+            self.keep += [T, FT]
+            b.has_transform = 1
+            b.transform = _p(T)
+            vel = np.zeros((self.dims, s.ncells), self.dtype) if s.velocity is None else \
+                np.ascontiguousarray(s.velocity.reshape(self.dims, -1), self.dtype)
+            prs = np.zeros(s.ncells, self.dtype) if s.pressure is None else \
+                np.ascontiguousarray(s.pressure.reshape(-1), self.dtype)
+            self.velocity.append(vel)
+            self.pressure.append(prs)
+            b.velocity, b.pressure = _p(vel), _p(prs)
+            if s.vsource is None:
+                b.vsource_mode = 0
+                self.vsource.append(None)
+            else:
+                vs = np.ascontiguousarray(s.vsource, self.dtype)
+                b.vsource_mode = 1 if vs.ndim == 1 else 2
+                vs = vs.reshape(-1)
+                self.vsource.append(vs)
+                b.vsource = _p(vs)
+            if s.viscosity is None:
+                b.visc_mode = 0
+            else:
+                vv = np.ascontiguousarray(np.asarray(s.viscosity, self.dtype).reshape(-1))
+                b.visc_mode = 1 if vv.size == 1 else 2
+                self.keep.append(vv)
+                b.viscosity = _p(vv)
+            for f in range(2 * self.dims):
+                fs = s.faces[f]
+                ob = b.b[f]
+                if fs.kind == "periodic":
+                    ob.type = OB_PERIODIC
+                elif fs.kind == "connect":
+                    ob.type = OB_CONNECTED
+                    ob.conn_block = fs.other
+                    if (bi, f) < (fs.other, fs.other_face):
+                        axes, _ = connection_axes(self.dims, f, fs.other_face, fs.axis1, fs.axis2)
+                    else:
+                        # the connection is made from the partner's side with the partner's axes
+                        ps = specs[fs.other].faces[fs.other_face]
+                        _, axes = connection_axes(self.dims, fs.other_face, f, ps.axis1, ps.axis2)
+                    for i in range(3):
+                        ob.conn_axes[i] = axes[i]
+                else:
+                    ob.type = OB_FIXED
+                    ob.vel_neumann = 1 if fs.neumann else 0
+                    vel_b = np.zeros(self.dims, self.dtype) if fs.velocity is None else \
+                        np.ascontiguousarray(fs.velocity, self.dtype)
+                    ob.vel_static = 1 if vel_b.ndim == 1 else 0
+                    vel_b = vel_b.reshape(-1)
+                    bt = boundary_slice(FT, f).reshape(-1)
+                    self.keep += [vel_b, bt]
+                    ob.vel = _p(vel_b)
+                    ob.has_transform = 1
+                    ob.transform = _p(bt)
+                    ob.pot_type = fs.pot
+                    if fs.pot_types is not None:
+                        pt = np.ascontiguousarray(fs.pot_types.reshape(-1), np.int8)
+                        self.keep.append(pt)
+                        ob.pot_types = _p(pt)
+                    if fs.pot_values is not None:
+                        pv = np.ascontiguousarray(fs.pot_values.reshape(-1), self.dtype)
+                        self.keep.append(pv)
+                        ob.pot_values = _p(pv)
+                    ob.pot_cw = fs.cw
+        self.ncells = off
+        self.dom = ODomain(self.dims, len(specs), off, 0, float(viscosity), float(dt), self.blocks)
+        self.row = np.zeros(off + 1, np.int32)
+        rc = lib().o_csr_rows_f64(C.byref(self.dom), _p(self.row))
+        assert rc == 0
+        self.nnz = int(self.row[-1])
+
+    def set_dt(self, dt: float) -> None:
+        self.dom.dt = float(dt)
+
+    def set_source(self, bi: int, src: Optional[np.ndarray]) -> None:
+        b = self.blocks[bi]
+        if src is None:
+            b.vsource_mode = 0
+            self.vsource[bi] = None
+            return
+        vs = np.ascontiguousarray(src, self.dtype)
+        b.vsource_mode = 1 if vs.ndim == 1 else 2
+        vs = vs.reshape(-1)
+        self.vsource[bi] = vs
+        b.vsource = _p(vs)
+
+    def set_velocity_block(self, bi: int, vel: np.ndarray) -> None:
+        self.velocity[bi][...] = vel.reshape(self.dims, -1)
+
+    def flat_from_blocks(self, fields, channels) -> np.ndarray:
+        out = np.zeros((channels, self.ncells), self.dtype)
+        for bi, f in enumerate(fields):
+            o = self.offsets[bi]
+            out[:, o:o + f.size // channels] = f.reshape(channels, -1)
+        return out.reshape(-1)
+
+    def blocks_from_flat(self, flat: np.ndarray, fields, channels) -> None:
+        v = flat.reshape(channels, self.ncells)
+        for bi, f in enumerate(fields):
+            o = self.offsets[bi]
+            n = f.size // channels
+            f.reshape(channels, -1)[...] = v[:, o:o + n]
+
+    # --- kernels
+    def _fn(self, name):
+        return getattr(lib(), name + sfx(self.dtype))
+
+    def advection_matrix(self):
+        col = np.zeros(self.nnz, np.int32)
+        val = np.zeros(self.nnz, self.dtype)
+        A = np.zeros(self.ncells, self.dtype)
+        assert self._fn("o_advection_matrix")(C.byref(self.dom), _p(self.row), _p(col), _p(val), _p(A)) == 0
+        return col, val, A
+
+    def advection_rhs(self, apply_pressure_gradient: bool = False) -> np.ndarray:
+        rhs = np.zeros(self.dims * self.ncells, self.dtype)
+        assert self._fn("o_advection_rhs")(C.byref(self.dom), int(apply_pressure_gradient), _p(rhs)) == 0
+        return rhs
+
+    def pressure_matrix(self, A: np.ndarray):
+        col = np.zeros(self.nnz, np.int32)
+        val = np.zeros(self.nnz, self.dtype)
+        assert self._fn("o_pressure_matrix")(C.byref(self.dom), _p(A), _p(self.row), _p(col), _p(val)) == 0
+        return col, val
+
+    def pressure_rhs(self, A, ccol, cval, vel_result) -> np.ndarray:
+        prhs = np.zeros(self.dims * self.ncells, self.dtype)
+        assert self._fn("o_pressure_rhs")(C.byref(self.dom), _p(A), _p(self.row), _p(ccol), _p(cval),
+                                          _p(vel_result), _p(prhs)) == 0
+        return prhs
+
+    def pressure_div(self, prhs, time_step_norm: bool = False) -> np.ndarray:
+        div = np.zeros(self.ncells, self.dtype)
+        assert self._fn("o_pressure_div")(C.byref(self.dom), _p(prhs), int(time_step_norm), _p(div)) == 0
+        return div
+
+    def correct_velocity(self, A, prhs, time_step_norm: bool = False) -> np.ndarray:
+        vel = np.zeros(self.dims * self.ncells, self.dtype)
+        assert self._fn("o_correct_velocity")(C.byref(self.dom), _p(A), _p(prhs), int(time_step_norm), _p(vel)) == 0
+        return vel
+
+    def epot_matrix(self):
+        col = np.zeros(self.nnz, np.int32)
+        val = np.zeros(self.nnz, self.dtype)
+        assert self._fn("o_epot_matrix")(C.byref(self.dom), _p(self.row), _p(col), _p(val)) == 0
+        return col, val
+
+    def epot_rhs(self, vf: np.ndarray) -> np.ndarray:
+        div = np.zeros(self.ncells, self.dtype)
+        v = np.ascontiguousarray(vf, self.dtype)
+        assert self._fn("o_epot_rhs")(C.byref(self.dom), _p(v), _p(div)) == 0
+        return div
+
+    def current_density(self, phi, uxb) -> np.ndarray:
+        J = np.zeros(3 * self.ncells, self.dtype)
+        assert self._fn("o_current_density")(C.byref(self.dom), _p(np.ascontiguousarray(phi, self.dtype)),
+                                             _p(np.ascontiguousarray(uxb, self.dtype)), _p(J)) == 0
+        return J
+
+
+# ----------------------------------------------------------------------------- solvers
+def spmv(row, col, val, x):
+    y = np.zeros(row.size - 1, val.dtype)
+    getattr(lib(), "o_spmv" + sfx(val.dtype))(row.size - 1, _p(row), _p(col), _p(val), _p(x), _p(y))
+    return y
+
+
+def cg(row, col, val, b, x, tol, maxit=5000, normalized=True, reset_every=0, return_best=False, project=False):
+    """Fused CG of the reference (returns the OResult; x updated in place)."""
+    r = OResult()
+    getattr(lib(), "o_cg" + sfx(val.dtype))(row.size - 1, _p(row), _p(col), _p(val), _p(b), _p(x), int(maxit),
+                                            C.c_double(tol), int(normalized), int(reset_every), int(return_best),
+                                            int(project), C.byref(r))
+    return r
+
+
+def bicgstab(row, col, val, b, x, tol, maxit=5000, normalized=True):
+    r = OResult()
+    getattr(lib(), "o_bicgstab" + sfx(val.dtype))(row.size - 1, _p(row), _p(col), _p(val), _p(b), _p(x), int(maxit),
+                                                  C.c_double(tol), int(normalized), C.byref(r))
+    return r
+
+
+def csr_transpose(row, col, val):
+    n = row.size - 1
+    tr = np.zeros(n + 1, np.int32)
+    tc = np.zeros(col.size, np.int32)
+    tv = np.zeros(val.size, val.dtype)
+    getattr(lib(), "o_csr_transpose" + sfx(val.dtype))(n, col.size, _p(row), _p(col), _p(val), _p(tr), _p(tc), _p(tv))
+    return tr, tc, tv
+
+
+class OracleHierarchy:
+    """Ruge-Stueben hierarchy of the oracle (double precision)."""
+
+    def __init__(self, row, col, val, theta=0.25, max_levels=20, max_coarse=64):
+        L = lib()
+        r = np.ascontiguousarray(row, np.int32)
+        c = np.ascontiguousarray(col, np.int32)
+        v = np.ascontiguousarray(val, np.float64)
+        h = L.o_amg_setup(r.size - 1, _p(r), _p(c), _p(v), float(theta), int(max_levels), int(max_coarse))
+        self.levels = []
+        nl = L.o_amg_num_levels(h)
+        for l in range(nl):
+            sz = np.zeros(4, np.int32)
+            L.o_amg_level_sizes(h, l, _p(sz))
+            n, na, npp, nr = [int(s) for s in sz]
+            ar, ac, av = np.zeros(n + 1, np.int32), np.zeros(max(na, 1), np.int32), np.zeros(max(na, 1))
+            lv = {"n": n}
+            if l < nl - 1:
+                nn = np.zeros(4, np.int32)
+                L.o_amg_level_sizes(h, l + 1, _p(nn))
+                pr, pc, pv = np.zeros(n + 1, np.int32), np.zeros(max(npp, 1), np.int32), np.zeros(max(npp, 1))
+                rr, rc, rv = np.zeros(nn[0] + 1, np.int32), np.zeros(max(nr, 1), np.int32), np.zeros(max(nr, 1))
+                sp = np.zeros(n, np.int32)
+                L.o_amg_level_copy(h, l, _p(ar), _p(ac), _p(av), _p(pr), _p(pc), _p(pv), _p(rr), _p(rc), _p(rv), _p(sp))
+                lv.update(P=(pr, pc[:npp], pv[:npp]), R=(rr, rc[:nr], rv[:nr]), split=sp)
+            else:
+                L.o_amg_level_copy(h, l, _p(ar), _p(ac), _p(av), None, None, None, None, None, None, None)
+            lv["A"] = (ar, ac[:na], av[:na])
+            self.levels.append(lv)
+        L.o_amg_free(h)
+        An = self.levels[-1]["A"]
+        nc = self.levels[-1]["n"]
+        D = np.zeros((nc, nc))
+        for i in range(nc):
+            for k in range(An[0][i], An[0][i + 1]):
+                D[i, An[1][k]] += An[2][k]
+        self.coarse_pinv = np.linalg.pinv(D)
+
+
+def galerkin(A, P, R, nc):
+    """R A P in double with the oracle's product order."""
+    ar, ac, av = A
+    pr, pc, pv = P
+    rr, rc, rv = R
+    cap = ac.size * 8 + 1024
+    cr, cc, cv = np.zeros(nc + 1, np.int32), np.zeros(cap, np.int32), np.zeros(cap)
+    nnz = lib().o_galerkin(ar.size - 1, _p(ar), _p(ac), _p(np.ascontiguousarray(av, np.float64)), nc, _p(pr), _p(pc),
+                           _p(pv), _p(rr), _p(rc), _p(rv), _p(cr), _p(cc), _p(cv), cap)
+    assert nnz >= 0
+    return cr, cc[:nnz], cv[:nnz]
+
+
+def amg_pcg(A_fine, levels_h, pinv_h, b, x, tol, maxit=5000, return_best=False, project=False, pre=1, post=1,
+            omega=2.0 / 3.0, hdtype=np.float32):
+    """AMG-PCG of the reference with a given hierarchy (levels_h: list of dicts holding A, dinv,
+    R, P arrays in the hierarchy dtype; the coarsest level enters only through pinv_h)."""
+    fr, fc, fv = A_fine
+    T = fv.dtype
+    hd = np.dtype(hdtype)
+    name = {("float64", "float64"): "o_amg_pcg_f64_f64", ("float64", "float32"): "o_amg_pcg_f64_f32",
+            ("float32", "float32"): "o_amg_pcg_f32_f32"}[(str(T), str(hd))]
+    fn = getattr(lib(), name)
+    L = len(levels_h)
+    keep = []
+
+    def arr_ptrs(key, idx, dt=None):
+        a = (C.c_void_p * L)()
+        for l, lv in enumerate(levels_h):
+            v = lv[key][idx] if idx is not None else lv[key]
+            v = np.ascontiguousarray(v, dt) if dt is not None else np.ascontiguousarray(v)
+            keep.append(v)
+            a[l] = v.ctypes.data
+        return a
+
+    ln = (C.c_int32 * L)(*[lv["n"] for lv in levels_h])
+    pinv = np.ascontiguousarray(pinv_h, hd)
+    res = OResult()
+    fn(fr.size - 1, _p(fr), _p(fc), _p(fv), L, ln, arr_ptrs("A", 0, np.int32), arr_ptrs("A", 1, np.int32),
+       arr_ptrs("A", 2, hd), arr_ptrs("dinv", None, hd), arr_ptrs("R", 0, np.int32), arr_ptrs("R", 1, np.int32),
+       arr_ptrs("R", 2, hd), arr_ptrs("P", 0, np.int32), arr_ptrs("P", 1, np.int32), arr_ptrs("P", 2, hd),
+       _p(pinv), pinv.shape[0], _p(b), _p(x), int(maxit), C.c_double(tol), 1, int(return_best), int(project), int(pre),
+       int(post), C.c_double(omega), C.byref(res))
+    return res
+
+
+def inverse_diagonal(row, col, val):
+    n = row.size - 1
+    d = np.zeros(n, val.dtype)
+    for i in range(n):
+        for k in range(row[i], row[i + 1]):
+            if col[k] == i:
+                d[i] += val[k]
+    return np.where(np.abs(d) > 1e-30, 1.0 / np.where(d == 0, 1, d), 1.0).astype(val.dtype)

@@ -1,0 +1,1135 @@
+/*
+ * ORACLE — test infrastructure only (see oracle.h). Typed body, included by oracle.c once
+ * with REAL=double (suffix _f64) and once with REAL=float (suffix _f32).
+ *
+ * Restates, for one environment and CENTRAL advection, the reference kernels:
+ *   grid metrics      csrc/phipict/grid/grid_gen.cu:298-354 (cells), :398-527 (faces)
+ *   advection matrix  csrc/phipict/piso/advection.cu:26-306
+ *   advection RHS     csrc/phipict/piso/advection.cu:799-955
+ *   pressure matrix   csrc/phipict/piso/pressure.cu:25-193
+ *   pressure RHS      csrc/phipict/piso/pressure.cu:349-522
+ *   divergence        csrc/phipict/piso/pressure.cu:755-803
+ *   correction        csrc/phipict/piso/velocity_correction.cu:26-60
+ *   Epot matrix       csrc/phipict/mhd/potential.cu:29-233
+ *   Epot RHS          csrc/phipict/mhd/potential.cu:272-322
+ *   current density   csrc/phipict/mhd/potential.cu:708-885
+ *   fused CG          csrc/phipict/solvers/krylov/krylov.cu:291-568, 1058-1125
+ *   fused BiCGStab    csrc/phipict/solvers/krylov/krylov.cu:598-824, 1129-1186
+ * Non-orthogonal corner terms (piso/device/nonortho_*.cuh) vanish identically on
+ * rectilinear grids and are not restated.
+ */
+
+#define FN2(a, b) a##b
+#define FN1(a, b) FN2(a, b)
+#define FN(name) FN1(name, SFX)
+
+typedef struct {
+    const ob_domain *dom;
+    int d;
+    int ts; /* transform size 2d^2+1 */
+} FN(octx);
+
+static int FN(ncells)(const ob_block *b) { return b->size[0] * b->size[1] * b->size[2]; }
+static int FN(flat)(const int *p, const int *s) { return p[0] + s[0] * (p[1] + s[1] * p[2]); }
+static void FN(unflat)(int idx, const int *s, int *p) {
+    p[0] = idx % s[0];
+    p[1] = (idx / s[0]) % s[1];
+    p[2] = idx / (s[0] * s[1]);
+}
+static int FN(at_bound)(const int *p, int bound, const ob_block *b) {
+    int axis = bound >> 1;
+    return (bound & 1) ? p[axis] == b->size[axis] - 1 : p[axis] == 0;
+}
+/* isEmptyBound, piso/device/topology.cuh:24-31: only FIXED among the prescribed types */
+static int FN(is_empty)(const ob_block *b, int bound) { return b->b[bound].type == OB_FIXED; }
+static int FN(face_sign)(int bound) { return (bound & 1) * 2 - 1; }
+/* face-cell index of cell p on face `bound` (normal coordinate 0, face grid size 1) */
+static int FN(face_flat)(const int *p, int bound, const ob_block *b) {
+    int s[3] = {b->size[0], b->size[1], b->size[2]};
+    int q[3] = {p[0], p[1], p[2]};
+    s[bound >> 1] = 1;
+    q[bound >> 1] = 0;
+    return FN(flat)(q, s);
+}
+static int FN(face_cells)(int bound, const ob_block *b) {
+    int s[3] = {b->size[0], b->size[1], b->size[2]};
+    s[bound >> 1] = 1;
+    return s[0] * s[1] * s[2];
+}
+static int FN(pos_mod)(int a, int m) { int r = a % m; return r < 0 ? r + m : r; }
+
+/* computeConnectedPos, topology.cuh:246-266 */
+static void FN(conn_pos)(const FN(octx) *c, const int *pos, int bdim, const ob_bound *cb, int border, int *out) {
+    const ob_block *ob = &c->dom->blocks[cb->conn_block];
+    out[0] = pos[0]; out[1] = pos[1]; out[2] = pos[2];
+    int ca = cb->conn_axes[0] >> 1;
+    out[ca] = (cb->conn_axes[0] & 1) ? ob->size[ca] - 1 - border : border;
+    if (c->d > 1) {
+        int axis = (bdim + 1) % c->d;
+        ca = cb->conn_axes[1] >> 1;
+        out[ca] = (cb->conn_axes[1] & 1) ? ob->size[ca] - 1 - pos[axis] : pos[axis];
+        if (c->d > 2) {
+            axis = (bdim + 2) % c->d;
+            ca = cb->conn_axes[2] >> 1;
+            out[ca] = (cb->conn_axes[2] & 1) ? ob->size[ca] - 1 - pos[axis] : pos[axis];
+        }
+    }
+}
+/* channel of computeConnectedPosWithChannel, topology.cuh:270-281 */
+static int FN(conn_channel)(const FN(octx) *c, int w, int bdim, const ob_bound *cb) {
+    return cb->conn_axes[FN(pos_mod)(w - bdim, c->d)] >> 1;
+}
+
+static const REAL *FN(T)(const ob_block *b, const int *p, int ts) {
+    return (const REAL *)b->transform + (size_t)FN(flat)(p, b->size) * ts;
+}
+/* getDeterminant, discretization.cuh:279-289 */
+static REAL FN(det)(const FN(octx) *c, const ob_block *b, const int *p) {
+    if (!b->has_transform) return (REAL)1;
+    return FN(T)(b, p, c->ts)[2 * c->d * c->d];
+}
+/* getLaplaceCoefficientOrthogonal, discretization.cuh:359-374: det * |Minv[w]|^2 */
+static REAL FN(alpha)(const FN(octx) *c, const ob_block *b, const int *p, int w) {
+    if (!b->has_transform) return (REAL)1;
+    const REAL *t = FN(T)(b, p, c->ts);
+    const REAL *mi = t + c->d * c->d + w * c->d;
+    REAL r = 0;
+    for (int i = 0; i < c->d; ++i) r += mi[i] * mi[i];
+    return t[2 * c->d * c->d] * r;
+}
+/* getMinvNorm, discretization.cuh:677-686 */
+static REAL FN(minv_norm)(const FN(octx) *c, const ob_block *b, const int *p, int axis) {
+    if (!b->has_transform) return (REAL)1;
+    const REAL *t = FN(T)(b, p, c->ts);
+    const REAL *mi = t + c->d * c->d + axis * c->d;
+    REAL r = 0;
+    for (int i = 0; i < c->d; ++i) r += mi[i] * mi[i];
+    return sizeof(REAL) == 4 ? (REAL)sqrtf((float)r) : (REAL)sqrt((double)r);
+}
+/* getViscosityBlock, discretization.cuh:1152-1172 */
+static REAL FN(visc)(const FN(octx) *c, const ob_block *b, const int *p) {
+    if (b->visc_mode == 0) return (REAL)c->dom->viscosity;
+    if (b->visc_mode == 1) return ((const REAL *)b->viscosity)[0];
+    return ((const REAL *)b->viscosity)[FN(flat)(p, b->size)];
+}
+/* velocity component k of a cell, from the block field or a flat global vector */
+static REAL FN(vel_at)(const FN(octx) *c, const ob_block *b, const int *p, int k, const REAL *glob) {
+    int f = FN(flat)(p, b->size);
+    if (glob) return glob[f + b->global_offset + (size_t)c->dom->ncells * k];
+    return ((const REAL *)b->velocity)[(size_t)k * FN(ncells)(b) + f];
+}
+/* getContravariantComponent, velocity.cuh:222-234 with VelocityToContravariantComponent :24-39 */
+static REAL FN(contra)(const FN(octx) *c, const ob_block *b, const int *p, int w, const REAL *glob) {
+    if (!b->has_transform) return FN(vel_at)(c, b, p, w, glob);
+    REAL v[3];
+    for (int k = 0; k < c->d; ++k) v[k] = FN(vel_at)(c, b, p, k, glob);
+    const REAL *t = FN(T)(b, p, c->ts);
+    const REAL *mi = t + c->d * c->d + w * c->d;
+    REAL r = 0;
+    for (int k = 0; k < c->d; ++k) r += mi[k] * v[k];
+    return t[2 * c->d * c->d] * r;
+}
+/* getFixedBoundaryData (VELOCITY), block_data.cuh:123-196 */
+static REAL FN(fixed_vel)(const FN(octx) *c, const ob_block *b, int bound, const int *p, int k) {
+    const ob_bound *fb = &b->b[bound];
+    const REAL *data = (const REAL *)fb->vel;
+    if (fb->vel_static) return data[k];
+    return data[(size_t)k * FN(face_cells)(bound, b) + FN(face_flat)(p, bound, b)];
+}
+/* getContravariantComponentBoundaryFixed, velocity.cuh:245-265 with getVelocityFromBoundaryFixed :134-153 */
+static REAL FN(contra_fixed)(const FN(octx) *c, const ob_block *b, int bound, const int *p, int w) {
+    const ob_bound *fb = &b->b[bound];
+    if (!fb->has_transform) return FN(fixed_vel)(c, b, bound, p, w);
+    REAL v[3] = {0, 0, 0};
+    if (!fb->vel_neumann)
+        for (int k = 0; k < c->d; ++k) v[k] = FN(fixed_vel)(c, b, bound, p, k);
+    const REAL *t = (const REAL *)fb->transform + (size_t)FN(face_flat)(p, bound, b) * c->ts;
+    const REAL *mi = t + c->d * c->d + w * c->d;
+    REAL r = 0;
+    for (int k = 0; k < c->d; ++k) r += mi[k] * v[k];
+    return t[2 * c->d * c->d] * r;
+}
+/* getLaplaceCoefficientOrthogonalBoundary, discretization.cuh:603-618 */
+static REAL FN(alpha_fixed)(const FN(octx) *c, const ob_block *b, int bound, const int *p, int w) {
+    const ob_bound *fb = &b->b[bound];
+    if (!fb->has_transform) return (REAL)1;
+    const REAL *t = (const REAL *)fb->transform + (size_t)FN(face_flat)(p, bound, b) * c->ts;
+    const REAL *mi = t + c->d * c->d + w * c->d;
+    REAL r = 0;
+    for (int i = 0; i < c->d; ++i) r += mi[i] * mi[i];
+    return t[2 * c->d * c->d] * r;
+}
+/* getBlockVelocitySource, nonortho_laplace.cuh:382-393 */
+static REAL FN(source)(const ob_block *b, const int *p, int k) {
+    if (b->vsource_mode == 0) return 0;
+    if (b->vsource_mode == 1) return ((const REAL *)b->vsource)[k];
+    return ((const REAL *)b->vsource)[(size_t)k * FN(ncells)(b) + FN(flat)(p, b->size)];
+}
+static int FN(pot_bc)(const ob_block *b, int bound, const int *p) {
+    const ob_bound *fb = &b->b[bound];
+    if (!fb->pot_types) return fb->pot_type;
+    return fb->pot_types[FN(face_flat)(p, bound, b)];
+}
+static REAL FN(pot_value)(const ob_block *b, int bound, const int *p) {
+    const ob_bound *fb = &b->b[bound];
+    if (!fb->pot_values) return 0;
+    return ((const REAL *)fb->pot_values)[FN(face_flat)(p, bound, b)];
+}
+static int FN(is_epot_dirichlet)(const ob_block *b, int bound, const int *p) {
+    return b->b[bound].type == OB_FIXED && FN(pot_bc)(b, bound, p) == OB_POT_DIRICHLET;
+}
+static int FN(is_insulating_wall)(const ob_block *b, int bound, const int *p) {
+    if (b->b[bound].type != OB_FIXED) return 0;
+    int t = FN(pot_bc)(b, bound, p);
+    return t == OB_POT_INSULATING || t == OB_POT_THIN_WALL;
+}
+static int FN(is_thin_wall)(const ob_block *b, int bound, const int *p) {
+    return b->b[bound].type == OB_FIXED && FN(pot_bc)(b, bound, p) == OB_POT_THIN_WALL;
+}
+
+/* Neighbour of cell p across face `bound` (interior, periodic or connected): block, position
+ * and the channel of the face-normal direction in the neighbour block. Used by the matrix
+ * kernels (advection.cu:148-169, pressure.cu:53-67). */
+static const ob_block *FN(neighbor)(const FN(octx) *c, const ob_block *b, const int *p, int bound, int *np, int *nw) {
+    int dim = bound >> 1, up = bound & 1;
+    np[0] = p[0]; np[1] = p[1]; np[2] = p[2];
+    *nw = dim;
+    if (FN(at_bound)(p, bound, b) && b->b[bound].type == OB_CONNECTED) {
+        const ob_bound *cb = &b->b[bound];
+        FN(conn_pos)(c, p, dim, cb, 0, np);
+        *nw = FN(conn_channel)(c, dim, dim, cb);
+        return &c->dom->blocks[cb->conn_block];
+    }
+    if (FN(at_bound)(p, bound, b) && b->b[bound].type == OB_PERIODIC) np[dim] = up ? 0 : b->size[dim] - 1;
+    else np[dim] = p[dim] + FN(face_sign)(bound);
+    return b;
+}
+
+/* computeFluxesNDLoop, discretization.cuh:758-852 (fluxes not multiplied by the face sign).
+ * epot: computeEpotFluxesNDLoop :856-912 (prescribed faces: insulating 0, open velC). */
+static void FN(fluxes)(const FN(octx) *c, const ob_block *b, const int *p, const REAL *glob, int epot, REAL *fl) {
+    for (int bound = 0; bound < 2 * c->d; ++bound) {
+        int dim = bound >> 1, up = bound & 1;
+        REAL velC = FN(contra)(c, b, p, dim, glob);
+        if (FN(at_bound)(p, bound, b)) {
+            const ob_bound *bd = &b->b[bound];
+            if (bd->type == OB_FIXED) {
+                if (epot) fl[bound] = FN(is_insulating_wall)(b, bound, p) ? 0 : velC;
+                else fl[bound] = FN(contra_fixed)(c, b, bound, p, dim);
+            } else if (bd->type == OB_CONNECTED) {
+                int np[3];
+                const ob_block *nb = &c->dom->blocks[bd->conn_block];
+                FN(conn_pos)(c, p, dim, bd, 0, np);
+                int nw = bd->conn_axes[0] >> 1;
+                REAL velN = FN(contra)(c, nb, np, nw, glob);
+                if ((bd->conn_axes[0] & 1) == up) velN = -velN;
+                fl[bound] = (velN + velC) * (REAL)0.5f;
+            } else if (bd->type == OB_PERIODIC) {
+                int np[3] = {p[0], p[1], p[2]};
+                np[dim] = up ? 0 : b->size[dim] - 1;
+                REAL velN = FN(contra)(c, b, np, dim, glob);
+                fl[bound] = (velN + velC) * (REAL)0.5f;
+            } else {
+                fl[bound] = 0;
+            }
+        } else {
+            int np[3] = {p[0], p[1], p[2]};
+            np[dim] += FN(face_sign)(bound);
+            REAL velN = FN(contra)(c, b, np, dim, glob);
+            fl[bound] = (velN + velC) * (REAL)0.5f;
+        }
+    }
+}
+
+/* CSR row emission, advection.cu:286-300 / csr_rows.cuh:23-33: entries in ascending
+ * column order, first occurrence first among equal columns. Returns entries written. */
+static int FN(emit_row)(int *idx, REAL *vals, int n, int32_t *col, REAL *val, int divide, REAL det) {
+    int written = 0;
+    for (;;) {
+        int best = -1, bv = 0x7fffffff;
+        for (int i = 0; i < n; ++i)
+            if (idx[i] >= 0 && idx[i] < bv) { bv = idx[i]; best = i; }
+        if (best < 0) break;
+        if (divide) vals[best] /= det;
+        col[written] = idx[best];
+        val[written] = vals[best];
+        idx[best] = -1;
+        ++written;
+    }
+    return written;
+}
+
+static int FN(domain_ok)(const ob_domain *dom) { return dom && dom->dims >= 2 && dom->dims <= 3 && dom->nblocks > 0; }
+
+/* CSR pattern row offsets: rows in global cell order, row sizes 2d+1 minus the prescribed
+ * faces of the cell (getCSRMatrixRowEndOffsetFromBlockBoundaries3D, topology.cuh:168-233). */
+int FN(o_csr_rows)(const ob_domain *dom, int32_t *row) {
+    if (!FN(domain_ok)(dom)) return -1;
+    int d = dom->dims;
+    row[0] = 0;
+    for (int bi = 0; bi < dom->nblocks; ++bi) {
+        const ob_block *b = &dom->blocks[bi];
+        int n = FN(ncells)(b);
+        for (int f = 0; f < n; ++f) {
+            int p[3];
+            FN(unflat)(f, b->size, p);
+            int rs = 2 * d + 1;
+            for (int bound = 0; bound < 2 * d; ++bound)
+                if (FN(at_bound)(p, bound, b) && FN(is_empty)(b, bound)) --rs;
+            int g = b->global_offset + f;
+            row[g + 1] = row[g] + rs;
+        }
+    }
+    return 0;
+}
+
+/* PISO_build_matrix, advection.cu:26-306 (CENTRAL scheme, orthogonal part) */
+int FN(o_advection_matrix)(const ob_domain *dom, const int32_t *row, int32_t *col, REAL *val, REAL *adiag) {
+    if (!FN(domain_ok)(dom)) return -1;
+    FN(octx) c = {dom, dom->dims, 2 * dom->dims * dom->dims + 1};
+    const REAL dt = (REAL)dom->dt;
+    for (int bi = 0; bi < dom->nblocks; ++bi) {
+        const ob_block *b = &dom->blocks[bi];
+        int n = FN(ncells)(b);
+        for (int f = 0; f < n; ++f) {
+            int p[3];
+            FN(unflat)(f, b->size, p);
+            REAL fl[6];
+            FN(fluxes)(&c, b, p, NULL, 0, fl);
+            int idx[7];
+            REAL rv[7] = {0, 0, 0, 0, 0, 0, 0};
+            const REAL det = b->has_transform ? FN(det)(&c, b, p) : (REAL)1;
+            REAL diag = det / dt;
+            idx[0] = f + b->global_offset;
+            const REAL vp = FN(visc)(&c, b, p);
+            REAL ap[3] = {1, 1, 1};
+            for (int dd = 0; dd < c.d; ++dd) ap[dd] = FN(alpha)(&c, b, p, dd);
+            for (int bound = 0; bound < 2 * c.d; ++bound) {
+                int dim = bound >> 1;
+                REAL fs = (REAL)FN(face_sign)(bound);
+                int atb = FN(at_bound)(p, bound, b);
+                if (!atb || !FN(is_empty)(b, bound)) {
+                    REAL ffo = fs * fl[bound];
+                    REAL ff = (REAL)0.5 * ffo;
+                    diag += ff;
+                    rv[bound + 1] += ff;
+                    int np[3], nw;
+                    const ob_block *nb = FN(neighbor)(&c, b, p, bound, np, &nw);
+                    REAL an = FN(alpha)(&c, nb, np, nw);
+                    REAL vn = FN(visc)(&c, nb, np);
+                    idx[bound + 1] = FN(flat)(np, nb->size) + nb->global_offset;
+                    REAL vc = (ap[dim] * vp + an * vn) * (REAL)0.5;
+                    diag += vc;
+                    rv[bound + 1] -= vc;
+                } else {
+                    int slip = b->b[bound].vel_neumann ? 1 : 0;
+                    diag += (REAL)(1 - slip) * 2 * vp * ap[dim];
+                    idx[bound + 1] = -1;
+                }
+            }
+            for (int dd = c.d; dd < 3; ++dd) { idx[dd * 2 + 1] = -1; idx[dd * 2 + 2] = -1; }
+            rv[0] = diag;
+            int g = b->global_offset + f;
+            int w = FN(emit_row)(idx, rv, 7, col + row[g], val + row[g], 1, det);
+            if (w != row[g + 1] - row[g]) return -2;
+            adiag[g] = rv[0];
+        }
+    }
+    return 0;
+}
+
+/* kPISO_build_advection_RHS, advection.cu:799-955 (CENTRAL, orthogonal part) */
+int FN(o_advection_rhs)(const ob_domain *dom, int apply_pressure_gradient, REAL *rhs);
+/* getPressureGradient + getPressureAtWithBounds, velocity.cuh:347-382, nonortho_laplace.cuh:302-378 */
+static REAL FN(pressure_at)(const FN(octx) *c, const ob_block *b, const int *pin) {
+    int t[3] = {pin[0], pin[1], pin[2]};
+    for (int dim = 0; dim < c->d; ++dim) {
+        int bound = dim * 2;
+        if (pin[dim] < 0) {
+            const ob_bound *bd = &b->b[bound];
+            if (bd->type == OB_FIXED) { t[dim] = 0; return ((const REAL *)b->pressure)[FN(flat)(t, b->size)]; }
+            if (bd->type == OB_CONNECTED) {
+                int op[3];
+                const ob_block *nb = &c->dom->blocks[bd->conn_block];
+                FN(conn_pos)(c, t, dim, bd, -(pin[dim] + 1), op);
+                return ((const REAL *)nb->pressure)[FN(flat)(op, nb->size)];
+            }
+            if (bd->type == OB_PERIODIC) { t[dim] += b->size[dim]; return ((const REAL *)b->pressure)[FN(flat)(t, b->size)]; }
+            return 0;
+        }
+        bound = dim * 2 + 1;
+        if (pin[dim] >= b->size[dim]) {
+            const ob_bound *bd = &b->b[bound];
+            if (bd->type == OB_FIXED) { t[dim] = b->size[dim] - 1; return ((const REAL *)b->pressure)[FN(flat)(t, b->size)]; }
+            if (bd->type == OB_CONNECTED) {
+                int op[3];
+                const ob_block *nb = &c->dom->blocks[bd->conn_block];
+                FN(conn_pos)(c, t, dim, bd, pin[dim] - b->size[dim], op);
+                return ((const REAL *)nb->pressure)[FN(flat)(op, nb->size)];
+            }
+            if (bd->type == OB_PERIODIC) { t[dim] -= b->size[dim]; return ((const REAL *)b->pressure)[FN(flat)(t, b->size)]; }
+            return 0;
+        }
+    }
+    return ((const REAL *)b->pressure)[FN(flat)(pin, b->size)];
+}
+static void FN(pressure_grad)(const FN(octx) *c, const ob_block *b, const int *p, REAL *g) {
+    REAL gr[3];
+    for (int dim = 0; dim < c->d; ++dim) {
+        int t[3] = {p[0], p[1], p[2]};
+        REAL fac = (REAL)0.5;
+        if (!(p[dim] == 0 && FN(is_empty)(b, dim * 2))) t[dim] = p[dim] - 1;
+        else fac = 1;
+        REAL vn = FN(pressure_at)(c, b, t);
+        if (!(p[dim] == b->size[dim] - 1 && FN(is_empty)(b, dim * 2 + 1))) t[dim] = p[dim] + 1;
+        else { t[dim] = p[dim]; fac = 1; }
+        REAL vpp = FN(pressure_at)(c, b, t);
+        gr[dim] = (vpp - vn) * fac;
+    }
+    if (b->has_transform) {
+        const REAL *mi = FN(T)(b, p, c->ts) + c->d * c->d;
+        REAL r[3] = {0, 0, 0};
+        for (int i = 0; i < c->d; ++i)
+            for (int j = 0; j < c->d; ++j) r[j] += mi[i * c->d + j] * gr[i];
+        for (int j = 0; j < c->d; ++j) gr[j] = r[j];
+    }
+    for (int j = 0; j < c->d; ++j) g[j] = gr[j];
+}
+
+int FN(o_advection_rhs)(const ob_domain *dom, int apply_pressure_gradient, REAL *rhs) {
+    if (!FN(domain_ok)(dom)) return -1;
+    FN(octx) c = {dom, dom->dims, 2 * dom->dims * dom->dims + 1};
+    const REAL dt = (REAL)dom->dt;
+    for (int bi = 0; bi < dom->nblocks; ++bi) {
+        const ob_block *b = &dom->blocks[bi];
+        int n = FN(ncells)(b);
+        for (int f = 0; f < n; ++f) {
+            int p[3];
+            FN(unflat)(f, b->size, p);
+            const REAL det = b->has_transform ? FN(det)(&c, b, p) : (REAL)1;
+            for (int dim = 0; dim < c.d; ++dim) {
+                REAL t = det * ((const REAL *)b->velocity)[(size_t)dim * n + f] / dt;
+                for (int bound = 0; bound < 2 * c.d; ++bound) {
+                    if (!FN(at_bound)(p, bound, b)) continue;
+                    if (b->b[bound].type != OB_FIXED) continue;
+                    REAL fn = (REAL)((bound & 1) * 2 - 1);
+                    REAL vel = FN(fixed_vel)(&c, b, bound, p, dim);
+                    REAL alpha = FN(alpha_fixed)(&c, b, bound, p, bound >> 1);
+                    REAL flux = FN(contra_fixed)(&c, b, bound, p, bound >> 1) * fn;
+                    t -= vel * flux;
+                    REAL bv = FN(visc)(&c, b, p);
+                    REAL slip = b->b[bound].vel_neumann ? 1 : 0;
+                    t += vel * (1 - slip) * bv * 2 * alpha;
+                }
+                t /= det;
+                t += FN(source)(b, p, dim);
+                if (apply_pressure_gradient) {
+                    REAL g[3];
+                    FN(pressure_grad)(&c, b, p, g);
+                    t -= g[dim] * dt;
+                }
+                rhs[(size_t)dim * dom->ncells + b->global_offset + f] = t;
+            }
+        }
+    }
+    return 0;
+}
+
+/* PISO_build_pressure_matrix, pressure.cu:25-193 (orthogonal part, useFaceTransform=false) */
+int FN(o_pressure_matrix)(const ob_domain *dom, const REAL *adiag, const int32_t *row, int32_t *col, REAL *val) {
+    if (!FN(domain_ok)(dom)) return -1;
+    FN(octx) c = {dom, dom->dims, 2 * dom->dims * dom->dims + 1};
+    for (int bi = 0; bi < dom->nblocks; ++bi) {
+        const ob_block *b = &dom->blocks[bi];
+        int n = FN(ncells)(b);
+        for (int f = 0; f < n; ++f) {
+            int p[3];
+            FN(unflat)(f, b->size, p);
+            int g = b->global_offset + f;
+            int idx[7];
+            REAL rv[7] = {0, 0, 0, 0, 0, 0, 0};
+            REAL ap[3];
+            for (int dd = 0; dd < c.d; ++dd) ap[dd] = FN(alpha)(&c, b, p, dd);
+            const REAL raP = (REAL)1.0 / adiag[g];
+            idx[0] = g;
+            for (int bound = 0; bound < 2 * c.d; ++bound) {
+                int dim = bound >> 1;
+                if (!(FN(at_bound)(p, bound, b) && FN(is_empty)(b, bound))) {
+                    int np[3], nw;
+                    const ob_block *nb = FN(neighbor)(&c, b, p, bound, np, &nw);
+                    REAL an = FN(alpha)(&c, nb, np, nw);
+                    int ng = FN(flat)(np, nb->size) + nb->global_offset;
+                    REAL raN = (REAL)1.0 / adiag[ng];
+                    REAL coef = (REAL)0.5 * (ap[dim] * raP + an * raN);
+                    rv[0] -= coef;
+                    rv[bound + 1] += coef;
+                    idx[bound + 1] = ng;
+                } else {
+                    idx[bound + 1] = -1;
+                }
+            }
+            for (int dd = c.d; dd < 3; ++dd) { idx[dd * 2 + 1] = -1; idx[dd * 2 + 2] = -1; }
+            int w = FN(emit_row)(idx, rv, 7, col + row[g], val + row[g], 0, 1);
+            if (w != row[g + 1] - row[g]) return -2;
+        }
+    }
+    return 0;
+}
+
+/* PISO_build_pressure_rhs, pressure.cu:349-522 (CENTRAL, boundary sources on) */
+int FN(o_pressure_rhs)(const ob_domain *dom, const REAL *adiag, const int32_t *crow, const int32_t *ccol,
+                       const REAL *cval, const REAL *vel_result, REAL *prhs) {
+    if (!FN(domain_ok)(dom)) return -1;
+    FN(octx) c = {dom, dom->dims, 2 * dom->dims * dom->dims + 1};
+    const REAL dt = (REAL)dom->dt;
+    for (int bi = 0; bi < dom->nblocks; ++bi) {
+        const ob_block *b = &dom->blocks[bi];
+        int n = FN(ncells)(b);
+        for (int f = 0; f < n; ++f) {
+            int p[3];
+            FN(unflat)(f, b->size, p);
+            int g = b->global_offset + f;
+            const REAL rDiag = 1 / adiag[g];
+            const REAL det = b->has_transform ? FN(det)(&c, b, p) : (REAL)1;
+            int cs = crow[g], rs = crow[g + 1] - cs;
+            for (int dim = 0; dim < c.d; ++dim) {
+                const REAL velOld = ((const REAL *)b->velocity)[(size_t)dim * n + f] / dt;
+                REAL H = 0;
+                for (int i = 0; i < rs && i < 7; ++i) {
+                    int ix = ccol[cs + i];
+                    if (ix != g) H += cval[cs + i] * vel_result[ix + (size_t)dom->ncells * dim];
+                }
+                REAL S = 0;
+                for (int bound = 0; bound < 2 * c.d; ++bound) {
+                    if (!FN(at_bound)(p, bound, b)) continue;
+                    if (b->b[bound].type != OB_FIXED) continue;
+                    REAL fn = (REAL)((bound & 1) * 2 - 1);
+                    REAL vel = FN(fixed_vel)(&c, b, bound, p, dim);
+                    REAL alpha = FN(alpha_fixed)(&c, b, bound, p, bound >> 1);
+                    REAL flux = FN(contra_fixed)(&c, b, bound, p, bound >> 1) * fn;
+                    S -= vel * flux;
+                    REAL bv = FN(visc)(&c, b, p);
+                    REAL slip = b->b[bound].vel_neumann ? 1 : 0;
+                    S += vel * (1 - slip) * bv * 2 * alpha;
+                }
+                S /= det;
+                S += FN(source)(b, p, dim);
+                prhs[g + (size_t)dom->ncells * dim] = rDiag * (velOld - H + S);
+            }
+        }
+    }
+    return 0;
+}
+
+/* k_computePressureRHSdivergenceFromFlux, pressure.cu:755-803 (useFaceTransform=false) */
+int FN(o_pressure_div)(const ob_domain *dom, const REAL *prhs, int time_step_norm, REAL *div) {
+    if (!FN(domain_ok)(dom)) return -1;
+    FN(octx) c = {dom, dom->dims, 2 * dom->dims * dom->dims + 1};
+    for (int bi = 0; bi < dom->nblocks; ++bi) {
+        const ob_block *b = &dom->blocks[bi];
+        int n = FN(ncells)(b);
+        for (int f = 0; f < n; ++f) {
+            int p[3];
+            FN(unflat)(f, b->size, p);
+            REAL fl[6];
+            FN(fluxes)(&c, b, p, prhs, 0, fl);
+            REAL dv = 0;
+            for (int dim = 0; dim < c.d; ++dim) dv += fl[dim * 2 + 1] - fl[dim * 2];
+            if (time_step_norm) dv /= (REAL)dom->dt;
+            div[f + b->global_offset] = dv;
+        }
+    }
+    return 0;
+}
+
+/* PISO_update_velocity, velocity_correction.cu:26-60 */
+int FN(o_correct_velocity)(const ob_domain *dom, const REAL *adiag, const REAL *prhs, int time_step_norm, REAL *vel_result) {
+    if (!FN(domain_ok)(dom)) return -1;
+    FN(octx) c = {dom, dom->dims, 2 * dom->dims * dom->dims + 1};
+    for (int bi = 0; bi < dom->nblocks; ++bi) {
+        const ob_block *b = &dom->blocks[bi];
+        int n = FN(ncells)(b);
+        for (int f = 0; f < n; ++f) {
+            int p[3];
+            FN(unflat)(f, b->size, p);
+            int g = b->global_offset + f;
+            const REAL rDiag = 1 / adiag[g];
+            REAL gr[3];
+            FN(pressure_grad)(&c, b, p, gr);
+            for (int dim = 0; dim < c.d; ++dim) {
+                REAL u = -rDiag * gr[dim];
+                if (time_step_norm) u *= (REAL)dom->dt;
+                size_t o = g + (size_t)dom->ncells * dim;
+                u += prhs[o];
+                vel_result[o] = u;
+            }
+        }
+    }
+    return 0;
+}
+
+/* PISO_build_epot_matrix, potential.cu:29-233 (orthogonal part, useFaceTransform=false) */
+int FN(o_epot_matrix)(const ob_domain *dom, const int32_t *row, int32_t *col, REAL *val) {
+    if (!FN(domain_ok)(dom)) return -1;
+    FN(octx) c = {dom, dom->dims, 2 * dom->dims * dom->dims + 1};
+    for (int bi = 0; bi < dom->nblocks; ++bi) {
+        const ob_block *b = &dom->blocks[bi];
+        int n = FN(ncells)(b);
+        for (int f = 0; f < n; ++f) {
+            int p[3];
+            FN(unflat)(f, b->size, p);
+            int g = b->global_offset + f;
+            int idx[7];
+            REAL rv[7] = {0, 0, 0, 0, 0, 0, 0};
+            REAL ap[3];
+            for (int dd = 0; dd < c.d; ++dd) ap[dd] = FN(alpha)(&c, b, p, dd);
+            const REAL raP = (REAL)1.0;
+            idx[0] = g;
+            for (int bound = 0; bound < 2 * c.d; ++bound) {
+                int dim = bound >> 1;
+                if (!(FN(at_bound)(p, bound, b) && FN(is_empty)(b, bound))) {
+                    int np[3], nw;
+                    const ob_block *nb = FN(neighbor)(&c, b, p, bound, np, &nw);
+                    REAL an = FN(alpha)(&c, nb, np, nw);
+                    int ng = FN(flat)(np, nb->size) + nb->global_offset;
+                    const REAL raN = (REAL)1.0;
+                    REAL coef = (REAL)0.5 * (ap[dim] * raP + an * raN);
+                    rv[0] -= coef;
+                    rv[bound + 1] += coef;
+                    idx[bound + 1] = ng;
+                } else {
+                    idx[bound + 1] = -1;
+                    if (FN(is_epot_dirichlet)(b, bound, p)) rv[0] -= ap[dim] * raP;
+                }
+            }
+            for (int dd = c.d; dd < 3; ++dd) { idx[dd * 2 + 1] = -1; idx[dd * 2 + 2] = -1; }
+            /* thin-wall Robin correction, potential.cu:164-218 */
+            for (int tw = 0; tw < 2 * c.d; ++tw) {
+                int twDim = tw >> 1;
+                if (!FN(at_bound)(p, tw, b)) continue;
+                if (!FN(is_thin_wall)(b, tw, p)) continue;
+                REAL Cw = (REAL)b->b[tw].pot_cw;
+                REAL mnP = FN(minv_norm)(&c, b, p, twDim);
+                for (int tDim = 0; tDim < c.d; ++tDim) {
+                    if (tDim == twDim) continue;
+                    for (int ts = 0; ts < 2; ++ts) {
+                        int tb = tDim * 2 + ts;
+                        if (idx[tb + 1] < 0) continue;
+                        int tfs = FN(face_sign)(tb);
+                        int tn[3] = {p[0], p[1], p[2]};
+                        if (FN(at_bound)(p, tb, b)) {
+                            if (b->b[tb].type == OB_PERIODIC) tn[tDim] = ts == 1 ? 0 : b->size[tDim] - 1;
+                            else continue;
+                        } else {
+                            tn[tDim] = p[tDim] + tfs;
+                        }
+                        if (!FN(is_thin_wall)(b, tw, tn)) continue;
+                        REAL ant = FN(alpha)(&c, b, tn, tDim);
+                        REAL mnN = FN(minv_norm)(&c, b, tn, twDim);
+                        REAL sc = Cw * (REAL)0.5 * (ap[tDim] * mnP + ant * mnN);
+                        rv[0] -= sc;
+                        rv[tb + 1] += sc;
+                    }
+                }
+            }
+            int w = FN(emit_row)(idx, rv, 7, col + row[g], val + row[g], 0, 1);
+            if (w != row[g + 1] - row[g]) return -2;
+        }
+    }
+    return 0;
+}
+
+/* k_computeEpotRHS, potential.cu:272-322 (useFaceTransform=false). vf: [d][n] */
+int FN(o_epot_rhs)(const ob_domain *dom, const REAL *vf, REAL *div) {
+    if (!FN(domain_ok)(dom)) return -1;
+    FN(octx) c = {dom, dom->dims, 2 * dom->dims * dom->dims + 1};
+    for (int bi = 0; bi < dom->nblocks; ++bi) {
+        const ob_block *b = &dom->blocks[bi];
+        int n = FN(ncells)(b);
+        for (int f = 0; f < n; ++f) {
+            int p[3];
+            FN(unflat)(f, b->size, p);
+            REAL fl[6];
+            FN(fluxes)(&c, b, p, vf, 1, fl);
+            REAL dv = 0;
+            for (int dim = 0; dim < c.d; ++dim) dv += fl[dim * 2 + 1] - fl[dim * 2];
+            for (int bound = 0; bound < 2 * c.d; ++bound) {
+                if (!FN(at_bound)(p, bound, b) || !FN(is_empty)(b, bound)) continue;
+                if (!FN(is_epot_dirichlet)(b, bound, p)) continue;
+                REAL gv = FN(pot_value)(b, bound, p);
+                if (gv == 0) continue;
+                dv -= FN(alpha)(&c, b, p, bound >> 1) * gv;
+            }
+            div[f + b->global_offset] = dv;
+        }
+    }
+    return 0;
+}
+
+/* computeCurrentDensityFaceBased, potential.cu:708-885. phi [n], uxb [3][n], J [3][n] */
+int FN(o_current_density)(const ob_domain *dom, const REAL *phi, const REAL *uxb, REAL *J) {
+    if (!FN(domain_ok)(dom)) return -1;
+    FN(octx) c = {dom, dom->dims, 2 * dom->dims * dom->dims + 1};
+    const int nc = dom->ncells;
+    for (int bi = 0; bi < dom->nblocks; ++bi) {
+        const ob_block *b = &dom->blocks[bi];
+        int n = FN(ncells)(b);
+        for (int f = 0; f < n; ++f) {
+            int p[3];
+            FN(unflat)(f, b->size, p);
+            int g = f + b->global_offset;
+            REAL phiP = phi[g];
+            REAL ap[3] = {0, 0, 0}, cvp[3] = {0, 0, 0};
+            for (int dd = 0; dd < c.d; ++dd) {
+                ap[dd] = FN(alpha)(&c, b, p, dd);
+                cvp[dd] = FN(contra)(&c, b, p, dd, uxb);
+            }
+            REAL jf[6] = {0, 0, 0, 0, 0, 0};
+            for (int bound = 0; bound < 2 * c.d; ++bound) {
+                int dim = bound >> 1, up = bound & 1;
+                REAL fs = (REAL)FN(face_sign)(bound);
+                int atb = FN(at_bound)(p, bound, b);
+                if (atb && FN(is_empty)(b, bound)) {
+                    if (FN(is_insulating_wall)(b, bound, p)) jf[bound] = 0;
+                    else if (FN(is_epot_dirichlet)(b, bound, p)) {
+                        REAL gv = FN(pot_value)(b, bound, p);
+                        jf[bound] = ap[dim] * fs * (phiP - gv) + cvp[dim];
+                    } else jf[bound] = cvp[dim];
+                    continue;
+                }
+                REAL phiN, aN, cvN;
+                if (atb) {
+                    const ob_bound *bd = &b->b[bound];
+                    if (bd->type == OB_CONNECTED) {
+                        int np[3];
+                        const ob_block *nb = &dom->blocks[bd->conn_block];
+                        FN(conn_pos)(&c, p, dim, bd, 0, np);
+                        int ca = FN(conn_channel)(&c, dim, dim, bd);
+                        phiN = phi[FN(flat)(np, nb->size) + nb->global_offset];
+                        aN = FN(alpha)(&c, nb, np, ca);
+                        cvN = FN(contra)(&c, nb, np, ca, uxb);
+                        if ((bd->conn_axes[0] & 1) == up) cvN = -cvN;
+                    } else if (bd->type == OB_PERIODIC) {
+                        int np[3] = {p[0], p[1], p[2]};
+                        np[dim] = up ? 0 : b->size[dim] - 1;
+                        phiN = phi[FN(flat)(np, b->size) + b->global_offset];
+                        aN = FN(alpha)(&c, b, np, dim);
+                        cvN = FN(contra)(&c, b, np, dim, uxb);
+                    } else {
+                        jf[bound] = 0;
+                        continue;
+                    }
+                } else {
+                    int np[3] = {p[0], p[1], p[2]};
+                    np[dim] += FN(face_sign)(bound);
+                    phiN = phi[FN(flat)(np, b->size) + b->global_offset];
+                    aN = FN(alpha)(&c, b, np, dim);
+                    cvN = FN(contra)(&c, b, np, dim, uxb);
+                }
+                REAL coef = (REAL)0.5 * (ap[dim] + aN);
+                REAL chi = (REAL)0.5 * (cvp[dim] + cvN);
+                jf[bound] = -coef * fs * (phiN - phiP) + chi;
+            }
+            REAL ja[3] = {0, 0, 0};
+            for (int dd = 0; dd < c.d; ++dd) ja[dd] = (REAL)0.5 * (jf[dd * 2] + jf[dd * 2 + 1]);
+            REAL Jv[3] = {0, 0, 0};
+            if (b->has_transform) {
+                const REAL *t = FN(T)(b, p, c.ts);
+                const REAL *M = t;
+                REAL inv = (REAL)1 / t[2 * c.d * c.d];
+                if (c.d == 2) {
+                    Jv[0] = (M[0] * ja[0] + M[1] * ja[1]) * inv;
+                    Jv[1] = (M[2] * ja[0] + M[3] * ja[1]) * inv;
+                    Jv[2] = uxb[g + (size_t)nc * 2];
+                } else {
+                    Jv[0] = (M[0] * ja[0] + M[1] * ja[1] + M[2] * ja[2]) * inv;
+                    Jv[1] = (M[3] * ja[0] + M[4] * ja[1] + M[5] * ja[2]) * inv;
+                    Jv[2] = (M[6] * ja[0] + M[7] * ja[1] + M[8] * ja[2]) * inv;
+                }
+            } else {
+                for (int dd = 0; dd < c.d; ++dd) Jv[dd] = ja[dd];
+                if (c.d == 2) Jv[2] = uxb[g + (size_t)nc * 2];
+            }
+            J[g] = Jv[0];
+            J[g + nc] = Jv[1];
+            J[g + (size_t)nc * 2] = Jv[2];
+        }
+    }
+    return 0;
+}
+
+/* det / inverse of MatrixSquare, common/transformations.h:437-495, 558-562 */
+static REAL FN(mdet)(int d, const REAL *m) {
+    if (d == 2) return m[0] * m[3] - m[2] * m[1];
+    return m[0] * (m[4] * m[8] - m[7] * m[5]) - m[1] * (m[3] * m[8] - m[6] * m[5]) + m[2] * (m[3] * m[7] - m[6] * m[4]);
+}
+static void FN(minv)(int d, const REAL *m, REAL det, REAL *a) {
+    if (d == 2) {
+        a[0] = m[3]; a[1] = -m[1]; a[2] = -m[2]; a[3] = m[0];
+    } else {
+        a[0] = m[4] * m[8] - m[7] * m[5];
+        a[1] = m[2] * m[7] - m[1] * m[8];
+        a[2] = m[1] * m[5] - m[2] * m[4];
+        a[3] = m[5] * m[6] - m[3] * m[8];
+        a[4] = m[0] * m[8] - m[2] * m[6];
+        a[5] = m[3] * m[2] - m[0] * m[5];
+        a[6] = m[3] * m[7] - m[6] * m[4];
+        a[7] = m[6] * m[1] - m[0] * m[7];
+        a[8] = m[0] * m[4] - m[3] * m[1];
+    }
+    REAL s = (REAL)1 / det;
+    for (int i = 0; i < d * d; ++i) a[i] *= s;
+}
+
+/* k_CoordsToTransforms, grid_gen.cu:298-354. cells[3]: cell counts (1 for unused);
+ * coords [d][vz][vy][vx]; out [cells][2d^2+1]. */
+int FN(o_transforms)(int d, const int32_t *cells, const REAL *coords, REAL *out) {
+    int vs[3] = {cells[0] + 1, d > 1 ? cells[1] + 1 : 1, d > 2 ? cells[2] + 1 : 1};
+    size_t vstride = (size_t)vs[0] * vs[1] * vs[2];
+    int cs[3] = {cells[0], d > 1 ? cells[1] : 1, d > 2 ? cells[2] : 1};
+    int ncell = cs[0] * cs[1] * cs[2];
+    int ts = 2 * d * d + 1;
+    for (int f = 0; f < ncell; ++f) {
+        int p[3];
+        FN(unflat)(f, cs, p);
+        REAL fc[6][3];
+        memset(fc, 0, sizeof(fc));
+        for (int v = 0; v < (1 << d); ++v) {
+            int vp[3] = {p[0] + (v & 1), p[1] + ((v & 2) != 0), p[2] + ((v & 4) != 0)};
+            REAL co[3];
+            for (int k = 0; k < d; ++k) co[k] = coords[k * vstride + FN(flat)(vp, vs)];
+            for (int fi = 0; fi < d; ++fi) {
+                int up = (v & (1 << fi)) != 0;
+                for (int k = 0; k < d; ++k) fc[fi * 2 + up][k] += co[k];
+            }
+        }
+        REAL norm = (REAL)1.0 / (REAL)(1 << (d - 1));
+        for (int fi = 0; fi < 2 * d; ++fi)
+            for (int k = 0; k < d; ++k) fc[fi][k] *= norm;
+        REAL *t = out + (size_t)f * ts;
+        for (int i = 0; i < d; ++i)
+            for (int k = 0; k < d; ++k) t[i * d + k] = fc[k * 2 + 1][i] - fc[k * 2][i];
+        REAL det = FN(mdet)(d, t);
+        FN(minv)(d, t, det, t + d * d);
+        t[2 * d * d] = det;
+    }
+    return 0;
+}
+
+/* k_CoordsToFaceTransforms, grid_gen.cu:398-527. out [d][vz][vy][vx][2d^2+1] (zeros where
+ * no face exists). */
+int FN(o_face_transforms)(int d, const int32_t *cells, const REAL *coords, REAL *out) {
+    int vs[3] = {cells[0] + 1, d > 1 ? cells[1] + 1 : 1, d > 2 ? cells[2] + 1 : 1};
+    size_t vstride = (size_t)vs[0] * vs[1] * vs[2];
+    int ts = 2 * d * d + 1;
+    int nfv = 1 << (d - 1), nev = 1 << (d - 2);
+    memset(out, 0, sizeof(REAL) * vstride * d * ts);
+    for (size_t f = 0; f < vstride; ++f) {
+        int p[3];
+        FN(unflat)((int)f, vs, p);
+        for (int dim = 0; dim < d; ++dim) {
+            int skip = 0;
+            for (int i = 1; i < d; ++i) {
+                int td = (dim + i) % d;
+                if (p[td] == vs[td] - 1) skip = 1;
+            }
+            if (skip) continue;
+            REAL t[9];
+            memset(t, 0, sizeof(t));
+            REAL nw = (REAL)0.5;
+            for (int up = 0; up < 2; ++up) {
+                int fs = up * 2 - 1;
+                int atb = up ? p[dim] == vs[dim] - 1 : p[dim] == 0;
+                int fp[3] = {p[0], p[1], p[2]};
+                if (!atb) fp[dim] += fs;
+                else nw = (REAL)1.0;
+                for (int v = 0; v < nfv; ++v) {
+                    int vp[3] = {fp[0], fp[1], fp[2]};
+                    for (int i = 1; i < d; ++i) vp[(dim + i) % d] += (v & i) != 0;
+                    for (int k = 0; k < d; ++k) t[k * d + dim] += coords[k * vstride + FN(flat)(vp, vs)] * (REAL)fs;
+                }
+            }
+            nw /= (REAL)nfv;
+            for (int k = 0; k < d; ++k) t[k * d + dim] *= nw;
+            REAL tnw = (REAL)1.0 / (REAL)nev;
+            for (int kk = 1; kk < d; ++kk) {
+                int td = (dim + kk) % d;
+                for (int v = 0; v < nfv; ++v) {
+                    int vp[3] = {p[0], p[1], p[2]};
+                    for (int i = 1; i < d; ++i) vp[(dim + i) % d] += (v & i) != 0;
+                    REAL tfs = (v & kk) ? (REAL)1 : (REAL)-1;
+                    for (int k = 0; k < d; ++k) t[k * d + td] += coords[k * vstride + FN(flat)(vp, vs)] * tfs;
+                }
+                for (int k = 0; k < d; ++k) t[k * d + td] *= tnw;
+            }
+            REAL *o = out + ((size_t)dim * vstride + f) * ts;
+            for (int i = 0; i < d * d; ++i) o[i] = t[i];
+            REAL det = FN(mdet)(d, t);
+            FN(minv)(d, t, det, o + d * d);
+            o[2 * d * d] = det;
+        }
+    }
+    return 0;
+}
+
+/* ---------------- sparse linear algebra ---------------- */
+
+int FN(o_spmv)(int n, const int32_t *row, const int32_t *col, const REAL *val, const REAL *x, REAL *y) {
+    for (int i = 0; i < n; ++i) {
+        REAL s = 0;
+        for (int j = row[i]; j < row[i + 1]; ++j) s += val[j] * x[col[j]];
+        y[i] = s;
+    }
+    return 0;
+}
+
+#ifndef ORACLE_ST_ENUM
+#define ORACLE_ST_ENUM
+enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_NONFINITE = 2, ST_MAXIT = 3, ST_RISING = 4 };
+#endif
+
+/* Fused CG of krylov::cgSolve, krylov.cu:1058-1125 (kResidual :291-366, kCgDirection
+ * :427-498, kCgUpdate :501-568, kRestoreBest :369-375, collectResults :958-980), one RHS,
+ * no preconditioner. project: subtract the mean of b first (singular consistent systems). */
+int FN(o_cg)(int n, const int32_t *row, const int32_t *col, const REAL *val, const REAL *b_in, REAL *x,
+             int maxit, double tol, int normalized, int reset_every, int return_best, int project, ob_result *res) {
+    const double ns = normalized ? 1.0 / sqrt((double)n) : 1.0;
+    REAL *r = (REAL *)malloc(sizeof(REAL) * n), *pa = (REAL *)malloc(sizeof(REAL) * n);
+    REAL *pb = (REAL *)malloc(sizeof(REAL) * n), *q = (REAL *)malloc(sizeof(REAL) * n);
+    REAL *bw = (REAL *)malloc(sizeof(REAL) * n), *xbest = (REAL *)malloc(sizeof(REAL) * n);
+    memcpy(bw, b_in, sizeof(REAL) * n);
+    if (project) {
+        double m = 0;
+        for (int i = 0; i < n; ++i) m += (double)bw[i];
+        REAL mm = (REAL)(m / (double)n);
+        for (int i = 0; i < n; ++i) bw[i] -= mm;
+    }
+    double rho = 0, alpha = 0, beta = 0, crit = 0, best = INFINITY, last = 0;
+    int it = 0, bestIt = -1, rising = 0, done = ST_RUNNING, restart = 1, saveBest = 0, usedIt = 0;
+    /* initial residual */
+    {
+        double acc = 0;
+        for (int i = 0; i < n; ++i) {
+            REAL s = 0;
+            for (int j = row[i]; j < row[i + 1]; ++j) s += val[j] * x[col[j]];
+            REAL ri = bw[i] - s;
+            r[i] = ri;
+            acc += (double)ri * (double)ri;
+        }
+        rho = acc;
+        restart = 1;
+        crit = sqrt(acc) * ns;
+        if (!isfinite(crit)) { done = ST_NONFINITE; usedIt = 0; }
+        else if (crit == 0.0) { done = ST_CONVERGED; usedIt = 0; }
+    }
+    REAL *pin = pa, *pout = pb;
+    long guard = 0;
+    while (done == ST_RUNNING && guard++ < (long)maxit + 4) {
+        if (reset_every > 0 && it > 0 && (it % reset_every) == 0) {
+            double acc = 0;
+            for (int i = 0; i < n; ++i) {
+                REAL s = 0;
+                for (int j = row[i]; j < row[i + 1]; ++j) s += val[j] * x[col[j]];
+                REAL ri = bw[i] - s;
+                r[i] = ri;
+                acc += (double)ri * (double)ri;
+            }
+            rho = acc;
+            restart = 1;
+        }
+        /* direction */
+        double pq = 0;
+        const REAL bt = (REAL)beta;
+        for (int i = 0; i < n; ++i) pout[i] = restart ? r[i] : r[i] + bt * pin[i];
+        for (int i = 0; i < n; ++i) {
+            REAL s = 0;
+            for (int j = row[i]; j < row[i + 1]; ++j) s += val[j] * pout[col[j]];
+            q[i] = s;
+            pq += (double)pout[i] * (double)s;
+        }
+        {
+            double a = rho / pq;
+            restart = 0;
+            if (!isfinite(a)) { done = ST_NONFINITE; usedIt = it; break; }
+            alpha = a;
+        }
+        /* update */
+        double rr = 0;
+        const REAL al = (REAL)alpha;
+        for (int i = 0; i < n; ++i) {
+            REAL xi = x[i];
+            if (saveBest && return_best) xbest[i] = xi;
+            x[i] = xi + al * pout[i];
+            REAL ri = r[i] - al * q[i];
+            r[i] = ri;
+            rr += (double)ri * (double)ri;
+        }
+        crit = sqrt(rr) * ns;
+        ++it;
+        saveBest = 0;
+        usedIt = it - 1;
+        if (!isfinite(crit)) done = ST_NONFINITE;
+        else {
+            if (return_best) {
+                if (crit < best) { best = crit; bestIt = it - 1; saveBest = 1; }
+                if (it > 1 && crit >= last) ++rising; else rising = 0;
+                last = crit;
+            }
+            if (crit < tol) done = ST_CONVERGED;
+            else if (it >= maxit) done = ST_MAXIT;
+            else if (return_best && rising >= 100) done = ST_RISING;
+            else {
+                double gamma = rr;
+                beta = gamma / rho;
+                rho = gamma;
+                if (!isfinite(beta)) done = ST_NONFINITE;
+            }
+        }
+        REAL *tmp = pin; pin = pout; pout = tmp;
+    }
+    if (return_best && (done == ST_MAXIT || done == ST_RISING) && bestIt >= 0 && best < crit)
+        memcpy(x, xbest, sizeof(REAL) * n);
+    res->final_residual = crit;
+    res->used_iterations = usedIt;
+    res->converged = done == ST_CONVERGED;
+    res->is_finite = isfinite(crit) ? 1 : 0;
+    if (return_best && (done == ST_MAXIT || done == ST_RISING) && bestIt >= 0 && best < crit) {
+        res->used_iterations = bestIt;
+        res->final_residual = best;
+    }
+    free(r); free(pa); free(pb); free(q); free(bw); free(xbest);
+    return 0;
+}
+
+/* Fused BiCGStab of krylov::bicgstabSolve, krylov.cu:1129-1186 (kResidual with r0,
+ * kBicgP :598-610, kBicgV :613-670, kBicgT :674-750, kBicgUpdate :754-824), one RHS,
+ * no preconditioner. */
+int FN(o_bicgstab)(int n, const int32_t *row, const int32_t *col, const REAL *val, const REAL *b, REAL *x,
+                   int maxit, double tol, int normalized, ob_result *res) {
+    const double ns = normalized ? 1.0 / sqrt((double)n) : 1.0;
+    REAL *r = (REAL *)malloc(sizeof(REAL) * n), *r0 = (REAL *)malloc(sizeof(REAL) * n);
+    REAL *p = (REAL *)calloc(n, sizeof(REAL)), *v = (REAL *)calloc(n, sizeof(REAL));
+    REAL *t = (REAL *)malloc(sizeof(REAL) * n), *s = (REAL *)malloc(sizeof(REAL) * n);
+    double rho = 0, alpha = 0, beta = 0, omega = 0, crit = 0, halfCrit = 0;
+    int it = 0, done = ST_RUNNING, restart = 1, halfDone = 0, usedIt = 0;
+    {
+        double acc = 0;
+        for (int i = 0; i < n; ++i) {
+            REAL sm = 0;
+            for (int j = row[i]; j < row[i + 1]; ++j) sm += val[j] * x[col[j]];
+            REAL ri = b[i] - sm;
+            r[i] = ri;
+            r0[i] = ri;
+            acc += (double)ri * (double)ri;
+        }
+        rho = acc;
+        restart = 1;
+        crit = sqrt(acc) * ns;
+        if (!isfinite(crit)) { done = ST_NONFINITE; usedIt = 0; }
+        else if (crit < tol) { done = ST_CONVERGED; usedIt = -1; }
+    }
+    long guard = 0;
+    while (done == ST_RUNNING && guard++ < (long)maxit + 4) {
+        {
+            const REAL bt = (REAL)beta, om = (REAL)omega;
+            for (int i = 0; i < n; ++i) p[i] = restart ? r[i] : r[i] + bt * (p[i] - om * v[i]);
+        }
+        {
+            double acc = 0;
+            for (int i = 0; i < n; ++i) {
+                REAL sm = 0;
+                for (int j = row[i]; j < row[i + 1]; ++j) sm += val[j] * p[col[j]];
+                v[i] = sm;
+                acc += (double)r0[i] * (double)sm;
+            }
+            double a = rho / acc;
+            restart = 0;
+            if (!isfinite(a)) { done = ST_NONFINITE; usedIt = it; break; }
+            alpha = a;
+        }
+        {
+            const REAL al = (REAL)alpha;
+            for (int i = 0; i < n; ++i) s[i] = r[i] - al * v[i];
+            double ts_ = 0, tt = 0, ss = 0;
+            for (int i = 0; i < n; ++i) {
+                REAL sm = 0;
+                for (int j = row[i]; j < row[i + 1]; ++j) sm += val[j] * s[col[j]];
+                t[i] = sm;
+                ts_ += (double)sm * (double)s[i];
+                tt += (double)sm * (double)sm;
+                ss += (double)s[i] * (double)s[i];
+            }
+            double critS = sqrt(ss) * ns;
+            halfCrit = critS;
+            if (!isfinite(critS)) { done = ST_NONFINITE; crit = critS; usedIt = it; break; }
+            else if (critS < tol) { halfDone = 1; omega = 0.0; }
+            else {
+                double om = ts_ / tt;
+                if (!isfinite(om)) { done = ST_NONFINITE; crit = critS; usedIt = it; break; }
+                omega = om;
+            }
+        }
+        {
+            const REAL al = (REAL)alpha, om = (REAL)omega;
+            double rr0 = 0, rr = 0;
+            for (int i = 0; i < n; ++i) {
+                REAL si = r[i] - al * v[i];
+                if (halfDone) {
+                    x[i] += al * p[i] * (REAL)1;
+                    r[i] = si;
+                } else {
+                    x[i] += (al * p[i] + om * si) * (REAL)1;
+                    REAL ri = si - om * t[i];
+                    r[i] = ri;
+                    rr0 += (double)r0[i] * (double)ri;
+                    rr += (double)ri * (double)ri;
+                }
+            }
+            if (halfDone) { done = ST_CONVERGED; crit = halfCrit; usedIt = it; }
+            else {
+                crit = sqrt(rr) * ns;
+                if (!isfinite(crit)) { done = ST_NONFINITE; usedIt = it; }
+                else if (crit < tol) { done = ST_CONVERGED; usedIt = it + 1; }
+                else {
+                    ++it;
+                    usedIt = it - 1;
+                    double bnew = (rr0 / rho) * (alpha / omega);
+                    rho = rr0;
+                    beta = bnew;
+                    if (!isfinite(beta)) done = ST_NONFINITE;
+                    else if (it >= maxit) done = ST_MAXIT;
+                }
+            }
+        }
+    }
+    res->final_residual = crit;
+    res->used_iterations = usedIt;
+    res->converged = done == ST_CONVERGED;
+    res->is_finite = isfinite(crit) ? 1 : 0;
+    free(r); free(r0); free(p); free(v); free(t); free(s);
+    return 0;
+}
+
+/* CSR transpose (closed form of the transpose krylov::csrTranspose computes with
+ * cusparseCsr2cscEx2, krylov.cu:1620-1652): entries of a column in ascending row order. */
+int FN(o_csr_transpose)(int n, int nnz, const int32_t *row, const int32_t *col, const REAL *val,
+                        int32_t *trow, int32_t *tcol, REAL *tval) {
+    for (int i = 0; i <= n; ++i) trow[i] = 0;
+    for (int k = 0; k < nnz; ++k) trow[col[k] + 1]++;
+    for (int i = 0; i < n; ++i) trow[i + 1] += trow[i];
+    int *fill = (int *)malloc(sizeof(int) * (n + 1));
+    for (int i = 0; i < n; ++i) fill[i] = trow[i];
+    for (int i = 0; i < n; ++i)
+        for (int k = row[i]; k < row[i + 1]; ++k) {
+            int c = col[k];
+            int o = fill[c]++;
+            tcol[o] = i;
+            tval[o] = val[k];
+        }
+    free(fill);
+    return 0;
+}
+
+#undef FN
+#undef FN1
+#undef FN2

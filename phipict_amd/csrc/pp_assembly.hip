@@ -1,0 +1,930 @@
+// PISO assembly kernels for gfx950: one work-item per (cell, environment), the grid's y
+// dimension enumerates batched environments. Each kernel restates one reference kernel
+// (cited at its definition) for the CENTRAL advection scheme on grids whose
+// non-orthogonal corner terms vanish.
+//
+// Matrix kernels write the values of rows whose pattern (row offsets, ascending columns)
+// is fixed per domain; k_csr_pattern writes that pattern once per domain.
+#include "pp_common.hpp"
+
+#include <vector>
+
+namespace pp {
+
+constexpr int ASM_BLOCK = 256;
+
+// Face fluxes of cell f (not multiplied by the face sign): computeFluxesNDLoop,
+// discretization.cuh:758-852; with EPOT the prescribed faces follow
+// computeEpotFluxesNDLoop :856-912 (insulating wall 0, open face velC).
+template <typename T, int D, bool EPOT>
+__device__ void cell_fluxes(const pp_domain &dom, const pp_block_desc *__restrict__ blocks, int bi, const int *p,
+                            int f, const T *glob, int env, T *fl) {
+    const pp_block_desc &b = blocks[bi];
+#pragma unroll
+    for (int bound = 0; bound < 2 * D; ++bound) {
+        const int dim = bound >> 1;
+        const int up = bound & 1;
+        const T velC = contra<T, D>(dom, b, f, dim, glob, env);
+        if (at_bound(p, bound, b)) {
+            const pp_boundary_desc &bd = b.bnd[bound];
+            if (bd.type == PP_BOUND_FIXED) {
+                if (EPOT) {
+                    fl[bound] = is_insulating_wall(b, bound, p) ? T(0) : velC;
+                } else {
+                    fl[bound] = contra_fixed<T, D>(bd, face_cells(b, bound), face_flat(p, bound, b), dim, env);
+                }
+            } else if (bd.type == PP_BOUND_CONNECTED) {
+                int np[3];
+                conn_pos(blocks, D, p, dim, bd, 0, np);
+                const pp_block_desc &nb = blocks[bd.conn_block];
+                T velN = contra<T, D>(dom, nb, flat3(np, nb.size), bd.conn_axes[0] >> 1, glob, env);
+                if ((bd.conn_axes[0] & 1) == up) velN = -velN;
+                fl[bound] = (velN + velC) * T(0.5f);
+            } else if (bd.type == PP_BOUND_PERIODIC) {
+                int np[3] = {p[0], p[1], p[2]};
+                np[dim] = up ? 0 : b.size[dim] - 1;
+                const T velN = contra<T, D>(dom, b, flat3(np, b.size), dim, glob, env);
+                fl[bound] = (velN + velC) * T(0.5f);
+            } else {
+                fl[bound] = T(0);
+            }
+        } else {
+            int np[3] = {p[0], p[1], p[2]};
+            np[dim] += face_sign(bound);
+            const T velN = contra<T, D>(dom, b, flat3(np, b.size), dim, glob, env);
+            fl[bound] = (velN + velC) * T(0.5f);
+        }
+    }
+}
+
+// Emit the 2D+1 candidate entries of a row in ascending column order, first occurrence
+// first among equal columns (advection.cu:286-300). Returns the number of entries.
+template <typename T, int D>
+__device__ __forceinline__ int emit_row(int *idx, T *vals, bool divide, T det, int32_t *col_out, T *val_out) {
+    int written = 0;
+#pragma unroll
+    for (int it = 0; it < 2 * D + 1; ++it) {
+        int best = -1, bv = 0x7fffffff;
+#pragma unroll
+        for (int i = 0; i < 2 * D + 1; ++i)
+            if (idx[i] >= 0 && idx[i] < bv) { bv = idx[i]; best = i; }
+        if (best < 0) break;
+        T v = vals[best];
+        if (divide) { v = v / det; vals[best] = v; }
+        if (col_out) col_out[written] = idx[best];
+        val_out[written] = v;
+        idx[best] = -1;
+        ++written;
+    }
+    return written;
+}
+
+#define CELL_PROLOGUE                                                          \
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;                       \
+    if (g >= dom.num_cells) return;                                            \
+    const int env = blockIdx.y;                                                \
+    const pp_block_desc *__restrict__ blocks = dom.blocks;                     \
+    const int bi = find_block(blocks, dom.num_blocks, g);                      \
+    const pp_block_desc &b = blocks[bi];                                       \
+    const int f = g - b.global_offset;                                         \
+    int p[3];                                                                  \
+    unflat(f, b.size, p);
+
+// CSR pattern: row offsets and ascending neighbour columns of every row (shared by the
+// advection, pressure and potential matrices; row layout of topology.cuh:168-233).
+template <int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_csr_pattern(pp_domain dom, int32_t *row, int32_t *col) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= dom.num_cells) return;
+    const pp_block_desc *__restrict__ blocks = dom.blocks;
+    const int bi = find_block(blocks, dom.num_blocks, g);
+    const pp_block_desc &b = blocks[bi];
+    const int f = g - b.global_offset;
+    int p[3];
+    unflat(f, b.size, p);
+    // row start: block csr offset + sum of the row sizes of the block's earlier cells,
+    // in closed form (each prescribed face removes one entry per face cell before f)
+    int start = f * (2 * D + 1);
+    for (int bound = 0; bound < 2 * D; ++bound) {
+        if (!is_fixed(b, bound)) continue;
+        const int a = bound >> 1;
+        const int up = bound & 1;
+        // cells before f whose coordinate along a equals the face coordinate
+        const int stride_a = a == 0 ? 1 : (a == 1 ? b.size[0] : b.size[0] * b.size[1]);
+        const int outer = f / (stride_a * b.size[a]);                 // full slabs before
+        const int inner = f % stride_a;                               // position inside the slab
+        const int ca = p[a];
+        const int target = up ? b.size[a] - 1 : 0;
+        int cnt = outer * stride_a;
+        if (ca > target) cnt += stride_a;
+        else if (ca == target) cnt += inner;
+        start -= cnt;
+    }
+    start += b.csr_offset;
+    int idx[2 * D + 1];
+    float dummy[2 * D + 1];
+    idx[0] = g;
+    for (int bound = 0; bound < 2 * D; ++bound) {
+        if (at_bound(p, bound, b) && is_fixed(b, bound)) { idx[bound + 1] = -1; continue; }
+        int np[3], nw;
+        const int nbi = neighbor(blocks, bi, D, p, bound, np, &nw);
+        idx[bound + 1] = flat3(np, blocks[nbi].size) + blocks[nbi].global_offset;
+    }
+    for (int i = 0; i < 2 * D + 1; ++i) dummy[i] = 0.f;
+    const int w = emit_row<float, D>(idx, dummy, false, 1.f, col + start, dummy);
+    if (g == 0) row[0] = 0;
+    row[g + 1] = start + w;
+}
+
+// PISO_build_matrix, advection.cu:26-306 (CENTRAL): values of C and the diagonal A.
+template <typename T, int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_advection_matrix(pp_domain dom) {
+    CELL_PROLOGUE
+    const T dt = reinterpret_cast<const T *>(dom.time_step)[env];
+    T fl[2 * D];
+    cell_fluxes<T, D, false>(dom, blocks, bi, p, f, nullptr, env, fl);
+    int idx[2 * D + 1];
+    T rv[2 * D + 1];
+#pragma unroll
+    for (int i = 0; i < 2 * D + 1; ++i) rv[i] = T(0);
+    const T det = Metric<T, D>::det(b, f);
+    T diag = det / dt;
+    idx[0] = g;
+    const T vp = visc_at<T>(dom, b, f);
+    T ap[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) ap[k] = Metric<T, D>::alpha(b, f, k);
+#pragma unroll
+    for (int bound = 0; bound < 2 * D; ++bound) {
+        const int dim = bound >> 1;
+        if (!at_bound(p, bound, b) || !is_fixed(b, bound)) {
+            const T ffo = T(face_sign(bound)) * fl[bound];
+            const T ff = T(0.5) * ffo;
+            diag += ff;
+            rv[bound + 1] += ff;
+            int np[3], nw;
+            const int nbi = neighbor(blocks, bi, D, p, bound, np, &nw);
+            const pp_block_desc &nb = blocks[nbi];
+            const int nf = flat3(np, nb.size);
+            const T an = Metric<T, D>::alpha(nb, nf, nw);
+            const T vn = visc_at<T>(dom, nb, nf);
+            idx[bound + 1] = nf + nb.global_offset;
+            const T vc = (ap[dim] * vp + an * vn) * T(0.5);
+            diag += vc;
+            rv[bound + 1] -= vc;
+        } else {
+            const int slip = b.bnd[bound].vel_type == PP_BC_DIRICHLET ? 0 : 1;
+            diag += T(1 - slip) * 2 * vp * ap[dim];
+            idx[bound + 1] = -1;
+        }
+    }
+    rv[0] = diag;
+    const int32_t start = dom.C.row[g];
+    T *val = reinterpret_cast<T *>(dom.C.val) + (size_t)env * dom.C.nnz + start;
+    emit_row<T, D>(idx, rv, true, det, nullptr, val);
+    reinterpret_cast<T *>(dom.A)[(size_t)env * dom.num_cells + g] = rv[0];
+}
+
+// Pressure at a position that may lie one cell outside the block (getPressureAtWithBounds,
+// nonortho_laplace.cuh:302-378).
+template <typename T>
+__device__ T pressure_at(const pp_domain &dom, const pp_block_desc *__restrict__ blocks, int bi, const int *pin,
+                         int env) {
+    const pp_block_desc &b = blocks[bi];
+    int t[3] = {pin[0], pin[1], pin[2]};
+    const int D = dom.dims;
+    for (int dim = 0; dim < D; ++dim) {
+        const bool lower = pin[dim] < 0;
+        const bool upper = pin[dim] >= b.size[dim];
+        if (!lower && !upper) continue;
+        const pp_boundary_desc &bd = b.bnd[dim * 2 + (upper ? 1 : 0)];
+        if (bd.type == PP_BOUND_FIXED) {
+            t[dim] = lower ? 0 : b.size[dim] - 1;
+            return reinterpret_cast<const T *>(b.pressure)[(size_t)env * b.cells + flat3(t, b.size)];
+        }
+        if (bd.type == PP_BOUND_CONNECTED) {
+            int op[3];
+            conn_pos(blocks, D, t, dim, bd, lower ? -(pin[dim] + 1) : pin[dim] - b.size[dim], op);
+            const pp_block_desc &nb = blocks[bd.conn_block];
+            return reinterpret_cast<const T *>(nb.pressure)[(size_t)env * nb.cells + flat3(op, nb.size)];
+        }
+        if (bd.type == PP_BOUND_PERIODIC) {
+            t[dim] += lower ? b.size[dim] : -b.size[dim];
+            return reinterpret_cast<const T *>(b.pressure)[(size_t)env * b.cells + flat3(t, b.size)];
+        }
+        return T(0);
+    }
+    return reinterpret_cast<const T *>(b.pressure)[(size_t)env * b.cells + flat3(pin, b.size)];
+}
+
+// getPressureGradient, velocity.cuh:347-382 (FD, transformed with Minv^T)
+template <typename T, int D>
+__device__ void pressure_grad(const pp_domain &dom, const pp_block_desc *__restrict__ blocks, int bi, const int *p,
+                              int f, int env, T *g) {
+    const pp_block_desc &b = blocks[bi];
+    T gr[D];
+#pragma unroll
+    for (int dim = 0; dim < D; ++dim) {
+        int t[3] = {p[0], p[1], p[2]};
+        T fac = T(0.5);
+        if (!(p[dim] == 0 && is_fixed(b, dim * 2))) t[dim] = p[dim] - 1;
+        else fac = T(1);
+        const T vn = pressure_at<T>(dom, blocks, bi, t, env);
+        if (!(p[dim] == b.size[dim] - 1 && is_fixed(b, dim * 2 + 1))) t[dim] = p[dim] + 1;
+        else { t[dim] = p[dim]; fac = T(1); }
+        const T vp = pressure_at<T>(dom, blocks, bi, t, env);
+        gr[dim] = (vp - vn) * fac;
+    }
+    if (b.has_transform) {
+        const T *mi = Metric<T, D>::at(b, f) + D * D;
+        T r[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) r[j] = T(0);
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j) r[j] += mi[i * D + j] * gr[i];
+#pragma unroll
+        for (int j = 0; j < D; ++j) gr[j] = r[j];
+    }
+#pragma unroll
+    for (int j = 0; j < D; ++j) g[j] = gr[j];
+}
+
+// Boundary source of the momentum equation at cell f for component dim: advective inflow
+// and wall viscous terms of prescribed faces (advection.cu:831-853, pressure.cu:410-432).
+template <typename T, int D>
+__device__ __forceinline__ T boundary_source(const pp_domain &dom, const pp_block_desc &b, const int *p, int f, int dim,
+                                             int env) {
+    T s = T(0);
+#pragma unroll
+    for (int bound = 0; bound < 2 * D; ++bound) {
+        if (!at_bound(p, bound, b)) continue;
+        const pp_boundary_desc &fb = b.bnd[bound];
+        if (fb.type != PP_BOUND_FIXED) continue;
+        const T fn = T((bound & 1) * 2 - 1);
+        const int fc = face_cells(b, bound), ff = face_flat(p, bound, b);
+        const T vel = fixed_vel<T>(fb, D, fc, ff, dim, env);
+        const T alpha = alpha_fixed<T, D>(fb, ff, bound >> 1);
+        const T flux = contra_fixed<T, D>(fb, fc, ff, bound >> 1, env) * fn;
+        s -= vel * flux;
+        const T bv = visc_at<T>(dom, b, f);
+        const T slip = fb.vel_type == PP_BC_DIRICHLET ? T(0) : T(1);
+        s += vel * (1 - slip) * bv * 2 * alpha;
+    }
+    return s;
+}
+
+// kPISO_build_advection_RHS, advection.cu:799-955 (CENTRAL)
+template <typename T, int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_advection_rhs(pp_domain dom, int apply_pg) {
+    CELL_PROLOGUE
+    const T dt = reinterpret_cast<const T *>(dom.time_step)[env];
+    const T det = Metric<T, D>::det(b, f);
+    const T *vel = reinterpret_cast<const T *>(b.velocity) + (size_t)env * D * b.cells;
+    T *rhs = reinterpret_cast<T *>(dom.velocity_rhs) + (size_t)env * D * dom.num_cells;
+    T pg[D];
+    if (apply_pg) pressure_grad<T, D>(dom, blocks, bi, p, f, env, pg);
+#pragma unroll
+    for (int dim = 0; dim < D; ++dim) {
+        T t = det * vel[(size_t)dim * b.cells + f] / dt;
+        // the reference accumulates the boundary terms one face at a time on t
+#pragma unroll
+        for (int bound = 0; bound < 2 * D; ++bound) {
+            if (!at_bound(p, bound, b)) continue;
+            const pp_boundary_desc &fb = b.bnd[bound];
+            if (fb.type != PP_BOUND_FIXED) continue;
+            const T fn = T((bound & 1) * 2 - 1);
+            const int fc = face_cells(b, bound), ff = face_flat(p, bound, b);
+            const T v = fixed_vel<T>(fb, D, fc, ff, dim, env);
+            const T alpha = alpha_fixed<T, D>(fb, ff, bound >> 1);
+            const T flux = contra_fixed<T, D>(fb, fc, ff, bound >> 1, env) * fn;
+            t -= v * flux;
+            const T bv = visc_at<T>(dom, b, f);
+            const T slip = fb.vel_type == PP_BC_DIRICHLET ? T(0) : T(1);
+            t += v * (1 - slip) * bv * 2 * alpha;
+        }
+        t /= det;
+        t += vsource<T>(dom, b, f, dim, env);
+        if (apply_pg) t -= pg[dim] * dt;
+        rhs[(size_t)dim * dom.num_cells + g] = t;
+    }
+}
+
+// PISO_build_pressure_matrix, pressure.cu:25-193 (useFaceTransform = false)
+template <typename T, int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_pressure_matrix(pp_domain dom) {
+    CELL_PROLOGUE
+    const T *A = reinterpret_cast<const T *>(dom.A) + (size_t)env * dom.num_cells;
+    int idx[2 * D + 1];
+    T rv[2 * D + 1];
+#pragma unroll
+    for (int i = 0; i < 2 * D + 1; ++i) rv[i] = T(0);
+    T ap[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) ap[k] = Metric<T, D>::alpha(b, f, k);
+    const T raP = T(1.0) / A[g];
+    idx[0] = g;
+#pragma unroll
+    for (int bound = 0; bound < 2 * D; ++bound) {
+        const int dim = bound >> 1;
+        if (!(at_bound(p, bound, b) && is_fixed(b, bound))) {
+            int np[3], nw;
+            const int nbi = neighbor(blocks, bi, D, p, bound, np, &nw);
+            const pp_block_desc &nb = blocks[nbi];
+            const int nf = flat3(np, nb.size);
+            const T an = Metric<T, D>::alpha(nb, nf, nw);
+            const int ng = nf + nb.global_offset;
+            const T raN = T(1.0) / A[ng];
+            const T coef = T(0.5) * (ap[dim] * raP + an * raN);
+            rv[0] -= coef;
+            rv[bound + 1] += coef;
+            idx[bound + 1] = ng;
+        } else {
+            idx[bound + 1] = -1;
+        }
+    }
+    const int32_t start = dom.P.row[g];
+    T *val = reinterpret_cast<T *>(dom.P.val) + (size_t)env * dom.P.nnz + start;
+    emit_row<T, D>(idx, rv, false, T(1), nullptr, val);
+}
+
+// PISO_build_pressure_rhs, pressure.cu:349-522 (CENTRAL, boundary sources on)
+template <typename T, int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_pressure_rhs(pp_domain dom) {
+    CELL_PROLOGUE
+    const T dt = reinterpret_cast<const T *>(dom.time_step)[env];
+    const T rDiag = 1 / reinterpret_cast<const T *>(dom.A)[(size_t)env * dom.num_cells + g];
+    const T det = Metric<T, D>::det(b, f);
+    const int32_t cs = dom.C.row[g];
+    const int rs = dom.C.row[g + 1] - cs;
+    const T *cval = reinterpret_cast<const T *>(dom.C.val) + (size_t)env * dom.C.nnz + cs;
+    const int32_t *ccol = dom.C.col + cs;
+    const T *vres = reinterpret_cast<const T *>(dom.velocity_result) + (size_t)env * D * dom.num_cells;
+    const T *vel = reinterpret_cast<const T *>(b.velocity) + (size_t)env * D * b.cells;
+    T *prhs = reinterpret_cast<T *>(dom.pressure_rhs) + (size_t)env * D * dom.num_cells;
+#pragma unroll
+    for (int dim = 0; dim < D; ++dim) {
+        const T velOld = vel[(size_t)dim * b.cells + f] / dt;
+        T H = T(0);
+        for (int i = 0; i < rs && i < 2 * D + 1; ++i) {
+            const int ix = ccol[i];
+            if (ix != g) H += cval[i] * vres[ix + (size_t)dom.num_cells * dim];
+        }
+        T S = boundary_source<T, D>(dom, b, p, f, dim, env);
+        S /= det;
+        S += vsource<T>(dom, b, f, dim, env);
+        prhs[g + (size_t)dom.num_cells * dim] = rDiag * (velOld - H + S);
+    }
+}
+
+// k_computePressureRHSdivergenceFromFlux, pressure.cu:755-803 (useFaceTransform = false)
+template <typename T, int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_pressure_div(pp_domain dom, int time_step_norm) {
+    CELL_PROLOGUE
+    const T *prhs = reinterpret_cast<const T *>(dom.pressure_rhs) + (size_t)env * D * dom.num_cells;
+    T fl[2 * D];
+    cell_fluxes<T, D, false>(dom, blocks, bi, p, f, prhs, env, fl);
+    T dv = T(0);
+#pragma unroll
+    for (int dim = 0; dim < D; ++dim) dv += fl[dim * 2 + 1] - fl[dim * 2];
+    if (time_step_norm) dv /= reinterpret_cast<const T *>(dom.time_step)[env];
+    reinterpret_cast<T *>(dom.pressure_rhs_div)[(size_t)env * dom.num_cells + g] = dv;
+}
+
+// PISO_update_velocity, velocity_correction.cu:26-60
+template <typename T, int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_correct_velocity(pp_domain dom, int time_step_norm) {
+    CELL_PROLOGUE
+    const T rDiag = 1 / reinterpret_cast<const T *>(dom.A)[(size_t)env * dom.num_cells + g];
+    T gr[D];
+    pressure_grad<T, D>(dom, blocks, bi, p, f, env, gr);
+    const T dt = reinterpret_cast<const T *>(dom.time_step)[env];
+    const T *prhs = reinterpret_cast<const T *>(dom.pressure_rhs) + (size_t)env * D * dom.num_cells;
+    T *vres = reinterpret_cast<T *>(dom.velocity_result) + (size_t)env * D * dom.num_cells;
+#pragma unroll
+    for (int dim = 0; dim < D; ++dim) {
+        T u = -rDiag * gr[dim];
+        if (time_step_norm) u *= dt;
+        const size_t o = g + (size_t)dom.num_cells * dim;
+        u += prhs[o];
+        vres[o] = u;
+    }
+}
+
+// PISO_build_epot_matrix, potential.cu:29-233 (useFaceTransform = false)
+template <typename T, int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_epot_matrix(pp_domain dom) {
+    CELL_PROLOGUE
+    if (env != 0) return; // one matrix shared by all environments
+    int idx[2 * D + 1];
+    T rv[2 * D + 1];
+#pragma unroll
+    for (int i = 0; i < 2 * D + 1; ++i) rv[i] = T(0);
+    T ap[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) ap[k] = Metric<T, D>::alpha(b, f, k);
+    const T raP = T(1.0);
+    idx[0] = g;
+#pragma unroll
+    for (int bound = 0; bound < 2 * D; ++bound) {
+        const int dim = bound >> 1;
+        if (!(at_bound(p, bound, b) && is_fixed(b, bound))) {
+            int np[3], nw;
+            const int nbi = neighbor(blocks, bi, D, p, bound, np, &nw);
+            const pp_block_desc &nb = blocks[nbi];
+            const int nf = flat3(np, nb.size);
+            const T an = Metric<T, D>::alpha(nb, nf, nw);
+            const T raN = T(1.0);
+            const T coef = T(0.5) * (ap[dim] * raP + an * raN);
+            rv[0] -= coef;
+            rv[bound + 1] += coef;
+            idx[bound + 1] = nf + nb.global_offset;
+        } else {
+            idx[bound + 1] = -1;
+            if (is_epot_dirichlet(b, bound, p)) rv[0] -= ap[dim] * raP;
+        }
+    }
+    // thin-wall Robin correction, potential.cu:164-218
+    for (int tw = 0; tw < 2 * D; ++tw) {
+        const int twDim = tw >> 1;
+        if (!at_bound(p, tw, b)) continue;
+        if (!is_thin_wall(b, tw, p)) continue;
+        const T Cw = T(b.bnd[tw].pot_cw);
+        const T mnP = Metric<T, D>::minv_norm(b, f, twDim);
+        for (int tDim = 0; tDim < D; ++tDim) {
+            if (tDim == twDim) continue;
+            for (int ts = 0; ts < 2; ++ts) {
+                const int tb = tDim * 2 + ts;
+                if (idx[tb + 1] < 0) continue;
+                int tn[3] = {p[0], p[1], p[2]};
+                if (at_bound(p, tb, b)) {
+                    if (b.bnd[tb].type == PP_BOUND_PERIODIC) tn[tDim] = ts == 1 ? 0 : b.size[tDim] - 1;
+                    else continue;
+                } else {
+                    tn[tDim] = p[tDim] + face_sign(tb);
+                }
+                if (!is_thin_wall(b, tw, tn)) continue;
+                const int tf = flat3(tn, b.size);
+                const T ant = Metric<T, D>::alpha(b, tf, tDim);
+                const T mnN = Metric<T, D>::minv_norm(b, tf, twDim);
+                const T sc = Cw * T(0.5) * (ap[tDim] * mnP + ant * mnN);
+                rv[0] -= sc;
+                rv[tb + 1] += sc;
+            }
+        }
+    }
+    const int32_t start = dom.E.row[g];
+    T *val = reinterpret_cast<T *>(dom.E.val) + start;
+    emit_row<T, D>(idx, rv, false, T(1), nullptr, val);
+}
+
+// k_computeEpotRHS, potential.cu:272-322. vf: [batch][d][n] -> div [batch][n]
+template <typename T, int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_epot_rhs(pp_domain dom, const T *__restrict__ vf, T *__restrict__ div) {
+    CELL_PROLOGUE
+    const T *v = vf + (size_t)env * D * dom.num_cells;
+    T fl[2 * D];
+    cell_fluxes<T, D, true>(dom, blocks, bi, p, f, v, env, fl);
+    T dv = T(0);
+#pragma unroll
+    for (int dim = 0; dim < D; ++dim) dv += fl[dim * 2 + 1] - fl[dim * 2];
+#pragma unroll
+    for (int bound = 0; bound < 2 * D; ++bound) {
+        if (!at_bound(p, bound, b) || !is_fixed(b, bound)) continue;
+        if (!is_epot_dirichlet(b, bound, p)) continue;
+        const T gv = pot_value<T>(b, bound, p);
+        if (gv == T(0)) continue;
+        dv -= Metric<T, D>::alpha(b, f, bound >> 1) * gv;
+    }
+    div[(size_t)env * dom.num_cells + g] = dv;
+}
+
+// computeCurrentDensityFaceBased, potential.cu:708-885. phi [batch][n], uxb [batch][3][n]
+// (the in-plane components are the first D), J [batch][3][n].
+template <typename T, int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_current_density(pp_domain dom, const T *__restrict__ phi_all,
+                                                             const T *__restrict__ uxb_all, T *__restrict__ J) {
+    CELL_PROLOGUE
+    const int nc = dom.num_cells;
+    const T *phi = phi_all + (size_t)env * nc;
+    const T *uxb = uxb_all + (size_t)env * 3 * nc;
+    const T phiP = phi[g];
+    T ap[D], cvp[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        ap[k] = Metric<T, D>::alpha(b, f, k);
+        cvp[k] = contra<T, D>(dom, b, f, k, uxb, env);
+    }
+    T jf[2 * D];
+#pragma unroll
+    for (int bound = 0; bound < 2 * D; ++bound) {
+        const int dim = bound >> 1;
+        const int up = bound & 1;
+        const T fs = T(face_sign(bound));
+        const bool atb = at_bound(p, bound, b);
+        if (atb && is_fixed(b, bound)) {
+            if (is_insulating_wall(b, bound, p)) jf[bound] = T(0);
+            else if (is_epot_dirichlet(b, bound, p)) {
+                const T gv = pot_value<T>(b, bound, p);
+                jf[bound] = ap[dim] * fs * (phiP - gv) + cvp[dim];
+            } else jf[bound] = cvp[dim];
+            continue;
+        }
+        T phiN, aN, cvN;
+        if (atb) {
+            const pp_boundary_desc &bd = b.bnd[bound];
+            if (bd.type == PP_BOUND_CONNECTED) {
+                int np[3];
+                conn_pos(blocks, D, p, dim, bd, 0, np);
+                const int ca = conn_channel(D, dim, dim, bd);
+                const pp_block_desc &nb = blocks[bd.conn_block];
+                const int nf = flat3(np, nb.size);
+                phiN = phi[nf + nb.global_offset];
+                aN = Metric<T, D>::alpha(nb, nf, ca);
+                cvN = contra<T, D>(dom, nb, nf, ca, uxb, env);
+                if ((bd.conn_axes[0] & 1) == up) cvN = -cvN;
+            } else if (bd.type == PP_BOUND_PERIODIC) {
+                int np[3] = {p[0], p[1], p[2]};
+                np[dim] = up ? 0 : b.size[dim] - 1;
+                const int nf = flat3(np, b.size);
+                phiN = phi[nf + b.global_offset];
+                aN = Metric<T, D>::alpha(b, nf, dim);
+                cvN = contra<T, D>(dom, b, nf, dim, uxb, env);
+            } else {
+                jf[bound] = T(0);
+                continue;
+            }
+        } else {
+            int np[3] = {p[0], p[1], p[2]};
+            np[dim] += face_sign(bound);
+            const int nf = flat3(np, b.size);
+            phiN = phi[nf + b.global_offset];
+            aN = Metric<T, D>::alpha(b, nf, dim);
+            cvN = contra<T, D>(dom, b, nf, dim, uxb, env);
+        }
+        const T coef = T(0.5) * (ap[dim] + aN);
+        const T chi = T(0.5) * (cvp[dim] + cvN);
+        jf[bound] = -coef * fs * (phiN - phiP) + chi;
+    }
+    T ja[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) ja[k] = T(0.5) * (jf[k * 2] + jf[k * 2 + 1]);
+    T Jv[3] = {T(0), T(0), T(0)};
+    if (b.has_transform) {
+        const T *t = Metric<T, D>::at(b, f);
+        const T inv = T(1) / t[2 * D * D];
+        if constexpr (D == 2) {
+            Jv[0] = (t[0] * ja[0] + t[1] * ja[1]) * inv;
+            Jv[1] = (t[2] * ja[0] + t[3] * ja[1]) * inv;
+            Jv[2] = uxb[g + (size_t)nc * 2];
+        } else {
+            Jv[0] = (t[0] * ja[0] + t[1] * ja[1] + t[2] * ja[2]) * inv;
+            Jv[1] = (t[3] * ja[0] + t[4] * ja[1] + t[5] * ja[2]) * inv;
+            Jv[2] = (t[6] * ja[0] + t[7] * ja[1] + t[8] * ja[2]) * inv;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < D; ++k) Jv[k] = ja[k];
+        if constexpr (D == 2) Jv[2] = uxb[g + (size_t)nc * 2];
+    }
+    T *Jo = J + (size_t)env * 3 * nc;
+    Jo[g] = Jv[0];
+    Jo[g + nc] = Jv[1];
+    Jo[g + (size_t)nc * 2] = Jv[2];
+}
+
+// flat [batch][ch][n] <-> block fields [batch][ch][cells]
+template <typename T>
+__global__ void __launch_bounds__(ASM_BLOCK) k_copy_result(pp_domain dom, int field, int ch, int to_blocks,
+                                                         const T *__restrict__ flat_in, T *__restrict__ flat_out) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= dom.num_cells) return;
+    const int env = blockIdx.y;
+    const pp_block_desc *__restrict__ blocks = dom.blocks;
+    const int bi = find_block(blocks, dom.num_blocks, g);
+    const pp_block_desc &b = blocks[bi];
+    const int f = g - b.global_offset;
+    T *bf = reinterpret_cast<T *>(field == 0 ? b.velocity : (field == 1 ? b.pressure : b.epot));
+    for (int c = 0; c < ch; ++c) {
+        const size_t fo = ((size_t)env * ch + c) * dom.num_cells + g;
+        const size_t bo = ((size_t)env * ch + c) * b.cells + f;
+        if (to_blocks) bf[bo] = flat_in[fo];
+        else flat_out[fo] = bf[bo];
+    }
+}
+
+// det and adjugate*(1/det) of MatrixSquare (common/transformations.h:437-495, 558-562)
+template <typename T, int D>
+__device__ __forceinline__ void det_inverse(const T *m, T &det, T *inv) {
+    T a[D * D];
+    if constexpr (D == 2) {
+        det = m[0] * m[3] - m[2] * m[1];
+        a[0] = m[3]; a[1] = -m[1]; a[2] = -m[2]; a[3] = m[0];
+    } else {
+        det = m[0] * (m[4] * m[8] - m[7] * m[5]) - m[1] * (m[3] * m[8] - m[6] * m[5]) + m[2] * (m[3] * m[7] - m[6] * m[4]);
+        a[0] = m[4] * m[8] - m[7] * m[5];
+        a[1] = m[2] * m[7] - m[1] * m[8];
+        a[2] = m[1] * m[5] - m[2] * m[4];
+        a[3] = m[5] * m[6] - m[3] * m[8];
+        a[4] = m[0] * m[8] - m[2] * m[6];
+        a[5] = m[3] * m[2] - m[0] * m[5];
+        a[6] = m[3] * m[7] - m[6] * m[4];
+        a[7] = m[6] * m[1] - m[0] * m[7];
+        a[8] = m[0] * m[4] - m[3] * m[1];
+    }
+    const T s = T(1) / det;
+#pragma unroll
+    for (int i = 0; i < D * D; ++i) inv[i] = a[i] * s;
+}
+
+// k_CoordsToTransforms, grid_gen.cu:298-354
+template <typename T, int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_coords_to_transforms(const T *__restrict__ coords, int cx, int cy, int cz,
+                                                                  T *__restrict__ out) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncell = cx * cy * cz;
+    if (f >= ncell) return;
+    const int32_t cs[3] = {cx, cy, cz};
+    const int32_t vs[3] = {cx + 1, D > 1 ? cy + 1 : 1, D > 2 ? cz + 1 : 1};
+    const size_t vstride = (size_t)vs[0] * vs[1] * vs[2];
+    int p[3];
+    unflat(f, cs, p);
+    T fc[2 * D][D];
+#pragma unroll
+    for (int i = 0; i < 2 * D; ++i)
+#pragma unroll
+        for (int k = 0; k < D; ++k) fc[i][k] = T(0);
+#pragma unroll
+    for (int v = 0; v < (1 << D); ++v) {
+        int vp[3] = {p[0] + (v & 1), p[1] + ((v & 2) != 0), p[2] + ((v & 4) != 0)};
+        T co[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) co[k] = coords[k * vstride + flat3(vp, vs)];
+#pragma unroll
+        for (int fi = 0; fi < D; ++fi) {
+            const int up = (v & (1 << fi)) != 0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) fc[fi * 2 + up][k] += co[k];
+        }
+    }
+    const T norm = T(1.0) / T(1 << (D - 1));
+#pragma unroll
+    for (int i = 0; i < 2 * D; ++i)
+#pragma unroll
+        for (int k = 0; k < D; ++k) fc[i][k] *= norm;
+    T *t = out + (size_t)f * (2 * D * D + 1);
+    T m[D * D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int k = 0; k < D; ++k) m[i * D + k] = fc[k * 2 + 1][i] - fc[k * 2][i];
+    T det;
+    det_inverse<T, D>(m, det, t + D * D);
+#pragma unroll
+    for (int i = 0; i < D * D; ++i) t[i] = m[i];
+    t[2 * D * D] = det;
+}
+
+// k_CoordsToFaceTransforms, grid_gen.cu:398-527; out [d][vertex grid][2d^2+1]
+template <typename T, int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_coords_to_face_transforms(const T *__restrict__ coords, int cx, int cy,
+                                                                       int cz, T *__restrict__ out) {
+    const int32_t vs[3] = {cx + 1, D > 1 ? cy + 1 : 1, D > 2 ? cz + 1 : 1};
+    const int vstride = vs[0] * vs[1] * vs[2];
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= vstride) return;
+    int p[3];
+    unflat(f, vs, p);
+    constexpr int NFV = 1 << (D - 1), NEV = 1 << (D - 2);
+    constexpr int TS = 2 * D * D + 1;
+#pragma unroll
+    for (int dim = 0; dim < D; ++dim) {
+        T *o = out + ((size_t)dim * vstride + f) * TS;
+        bool skip = false;
+#pragma unroll
+        for (int i = 1; i < D; ++i) {
+            const int td = (dim + i) % D;
+            if (p[td] == vs[td] - 1) skip = true;
+        }
+        if (skip) {
+            for (int i = 0; i < TS; ++i) o[i] = T(0);
+            continue;
+        }
+        T t[D * D];
+#pragma unroll
+        for (int i = 0; i < D * D; ++i) t[i] = T(0);
+        T nw = T(0.5);
+#pragma unroll
+        for (int up = 0; up < 2; ++up) {
+            const int fs = up * 2 - 1;
+            const bool atb = up ? p[dim] == vs[dim] - 1 : p[dim] == 0;
+            int fp[3] = {p[0], p[1], p[2]};
+            if (!atb) fp[dim] += fs;
+            else nw = T(1.0);
+#pragma unroll
+            for (int v = 0; v < NFV; ++v) {
+                int vp[3] = {fp[0], fp[1], fp[2]};
+#pragma unroll
+                for (int i = 1; i < D; ++i) vp[(dim + i) % D] += (v & i) != 0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) t[k * D + dim] += coords[(size_t)k * vstride + flat3(vp, vs)] * T(fs);
+            }
+        }
+        nw /= T(NFV);
+#pragma unroll
+        for (int k = 0; k < D; ++k) t[k * D + dim] *= nw;
+        const T tnw = T(1.0) / T(NEV);
+#pragma unroll
+        for (int kk = 1; kk < D; ++kk) {
+            const int td = (dim + kk) % D;
+#pragma unroll
+            for (int v = 0; v < NFV; ++v) {
+                int vp[3] = {p[0], p[1], p[2]};
+#pragma unroll
+                for (int i = 1; i < D; ++i) vp[(dim + i) % D] += (v & i) != 0;
+                const T tfs = (v & kk) ? T(1) : T(-1);
+#pragma unroll
+                for (int k = 0; k < D; ++k) t[k * D + td] += coords[(size_t)k * vstride + flat3(vp, vs)] * tfs;
+            }
+#pragma unroll
+            for (int k = 0; k < D; ++k) t[k * D + td] *= tnw;
+        }
+        T det;
+        det_inverse<T, D>(t, det, o + D * D);
+#pragma unroll
+        for (int i = 0; i < D * D; ++i) o[i] = t[i];
+        o[2 * D * D] = det;
+    }
+}
+
+static int check_domain(const pp_domain *dom, const char *fn) {
+    if (!dom || !dom->blocks || dom->num_blocks <= 0 || dom->num_cells <= 0 || dom->batch <= 0 ||
+        (dom->dims != 2 && dom->dims != 3) || (dom->dtype != PP_F32 && dom->dtype != PP_F64)) {
+        set_error(std::string(fn) + ": invalid domain descriptor");
+        return PP_ERR_ARG;
+    }
+    if (dom->batch > 65535) {
+        set_error(std::string(fn) + ": batch exceeds the grid limit");
+        return PP_ERR_ARG;
+    }
+    return PP_OK;
+}
+
+static inline dim3 cell_grid(const pp_domain *dom, int envs) {
+    return dim3((dom->num_cells + ASM_BLOCK - 1) / ASM_BLOCK, envs);
+}
+
+#define DIMS_DISPATCH(DIMV, ...)                                   \
+    if ((DIMV) == 2) { constexpr int D = 2; __VA_ARGS__; }         \
+    else { constexpr int D = 3; __VA_ARGS__; }
+
+} // namespace pp
+
+using namespace pp;
+
+extern "C" {
+
+int pp_csr_pattern(const pp_domain *dom, int32_t *row, int32_t *col, void *stream) {
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(row && col, "row/col must not be null");
+    DIMS_DISPATCH(dom->dims,
+        hipLaunchKernelGGL((k_csr_pattern<D>), cell_grid(dom, 1), dim3(ASM_BLOCK), 0, as_stream(stream), *dom, row, col));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_setup_advection_matrix(const pp_domain *dom, int32_t non_ortho_flags, void *stream) {
+    (void)non_ortho_flags;
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(dom->C.val && dom->C.row && dom->A && dom->time_step, "C, A and time_step must be set");
+    PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
+        hipLaunchKernelGGL((k_advection_matrix<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom)));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_setup_advection_velocity(const pp_domain *dom, int32_t non_ortho_flags, int32_t apply_pressure_gradient, void *stream) {
+    (void)non_ortho_flags;
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(dom->velocity_rhs && dom->time_step, "velocity_rhs and time_step must be set");
+    PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
+        hipLaunchKernelGGL((k_advection_rhs<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+                           (int)apply_pressure_gradient)));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_setup_pressure_matrix(const pp_domain *dom, int32_t non_ortho_flags, void *stream) {
+    (void)non_ortho_flags;
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(dom->P.val && dom->P.row && dom->A, "P and A must be set");
+    PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
+        hipLaunchKernelGGL((k_pressure_matrix<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom)));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_setup_pressure_rhs(const pp_domain *dom, int32_t non_ortho_flags, int32_t time_step_norm, void *stream) {
+    (void)non_ortho_flags;
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(dom->pressure_rhs && dom->pressure_rhs_div && dom->C.val && dom->velocity_result, "pressure fields must be set");
+    PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
+        hipLaunchKernelGGL((k_pressure_rhs<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom);
+        hipLaunchKernelGGL((k_pressure_div<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+                           (int)time_step_norm)));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_setup_pressure_rhs_div(const pp_domain *dom, int32_t non_ortho_flags, int32_t time_step_norm, void *stream) {
+    (void)non_ortho_flags;
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(dom->pressure_rhs && dom->pressure_rhs_div, "pressure fields must be set");
+    PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
+        hipLaunchKernelGGL((k_pressure_div<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+                           (int)time_step_norm)));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_correct_velocity(const pp_domain *dom, int32_t time_step_norm, void *stream) {
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(dom->velocity_result && dom->pressure_rhs && dom->A, "velocity_result, pressure_rhs and A must be set");
+    PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
+        hipLaunchKernelGGL((k_correct_velocity<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+                           (int)time_step_norm)));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_setup_epot_matrix(const pp_domain *dom, int32_t non_ortho_flags, void *stream) {
+    (void)non_ortho_flags;
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(dom->E.val && dom->E.row, "E must be set");
+    PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
+        hipLaunchKernelGGL((k_epot_matrix<T, D>), cell_grid(dom, 1), dim3(ASM_BLOCK), 0, as_stream(stream), *dom)));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_compute_epot_rhs(const pp_domain *dom, const void *vector_field, void *divergence, void *stream) {
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(vector_field && divergence, "vector_field and divergence must not be null");
+    PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
+        hipLaunchKernelGGL((k_epot_rhs<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+                           reinterpret_cast<const T *>(vector_field), reinterpret_cast<T *>(divergence))));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_compute_current_density(const pp_domain *dom, const void *phi, const void *uxb, void *j_out, void *stream) {
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(phi && uxb && j_out, "phi, uxb and j_out must not be null");
+    PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
+        hipLaunchKernelGGL((k_current_density<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+                           reinterpret_cast<const T *>(phi), reinterpret_cast<const T *>(uxb), reinterpret_cast<T *>(j_out))));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_copy_result(const pp_domain *dom, int32_t field, int32_t to_blocks, void *stream) {
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(field >= 0 && field <= 2, "field must be 0, 1 or 2");
+    void *flat = field == 0 ? dom->velocity_result : (field == 1 ? dom->pressure_result : dom->epot_result);
+    PP_CHECK_ARG(flat != nullptr, "result vector not set");
+    const int ch = field == 0 ? dom->dims : 1;
+    PP_DISPATCH(dom->dtype,
+        hipLaunchKernelGGL((k_copy_result<T>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom, (int)field, ch,
+                           (int)to_blocks, reinterpret_cast<const T *>(flat), reinterpret_cast<T *>(flat)));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_coords_to_transforms(int32_t dims, const int32_t *cells, const void *coords, void *out, int32_t dtype, void *stream) {
+    PP_CHECK_ARG(dims == 2 || dims == 3, "dims must be 2 or 3");
+    PP_CHECK_ARG(cells && coords && out, "null argument");
+    const int cx = cells[0], cy = dims > 1 ? cells[1] : 1, cz = dims > 2 ? cells[2] : 1;
+    PP_CHECK_ARG(cx > 0 && cy > 0 && cz > 0, "cell counts must be positive");
+    const int n = cx * cy * cz;
+    PP_DISPATCH(dtype, DIMS_DISPATCH(dims,
+        hipLaunchKernelGGL((k_coords_to_transforms<T, D>), dim3((n + ASM_BLOCK - 1) / ASM_BLOCK), dim3(ASM_BLOCK), 0, as_stream(stream),
+                           reinterpret_cast<const T *>(coords), cx, cy, cz, reinterpret_cast<T *>(out))));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_coords_to_face_transforms(int32_t dims, const int32_t *cells, const void *coords, void *out, int32_t dtype, void *stream) {
+    PP_CHECK_ARG(dims == 2 || dims == 3, "dims must be 2 or 3");
+    PP_CHECK_ARG(cells && coords && out, "null argument");
+    const int cx = cells[0], cy = dims > 1 ? cells[1] : 1, cz = dims > 2 ? cells[2] : 1;
+    PP_CHECK_ARG(cx > 0 && cy > 0 && cz > 0, "cell counts must be positive");
+    const int nv = (cx + 1) * (dims > 1 ? cy + 1 : 1) * (dims > 2 ? cz + 1 : 1);
+    PP_DISPATCH(dtype, DIMS_DISPATCH(dims,
+        hipLaunchKernelGGL((k_coords_to_face_transforms<T, D>), dim3((nv + ASM_BLOCK - 1) / ASM_BLOCK), dim3(ASM_BLOCK), 0,
+                           as_stream(stream), reinterpret_cast<const T *>(coords), cx, cy, cz, reinterpret_cast<T *>(out))));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+} // extern "C"

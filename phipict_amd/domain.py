@@ -1,0 +1,1000 @@
+"""Multi-block domain data model: ``Domain``, ``Block``, boundaries and ``CSRmatrix``.
+
+Mirrors the classes the reference binds in ``phipict._C`` (csrc/phipict/bindings.cpp:92-625,
+csrc/phipict/domain/domain_structs.h) for the PISO path: the same names, tensor layouts and
+methods, with torch tensors owning every field. ``Domain.desc()`` packs the device-side
+descriptor table (include/phipict_hip.h) that the gfx950 kernels read; it is rebuilt only
+when a tensor of the domain changed (``UpdateDomainData``), like the reference's atlas
+(csrc/phipict/domain/domain.cpp:1053-1342).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from enum import IntEnum
+from typing import Optional, Sequence
+
+import torch
+
+from . import _lib
+
+
+class BoundaryType(IntEnum):
+    """Face types (csrc/phipict/domain/domain_structs_gpu.h:146-161)."""
+
+    CONNECTED_GRID = 20
+    PERIODIC = 21
+    FIXED = 30
+
+
+class BoundaryConditionType(IntEnum):
+    DIRICHLET = 0
+    NEUMANN = 1
+
+
+class PotentialBC(IntEnum):
+    """Electric-potential face types (domain_structs_gpu.h:217-223)."""
+
+    INSULATING = 0
+    OPEN = 1
+    DIRICHLET = 2
+    THIN_WALL = 3
+
+
+class ConvergenceCriterion(IntEnum):
+    NORM2 = 0
+    NORM2_NORMALIZED = 1
+
+
+class AdvectionScheme(IntEnum):
+    CENTRAL = 0
+    LINEAR_UPWIND = 1
+
+
+_FACES = {"-x": 0, "+x": 1, "-y": 2, "+y": 3, "-z": 4, "+z": 5}
+_AXES = {"x": 0, "y": 1, "z": 2}
+
+
+def face_index(face) -> int:
+    """Face string ('-x', '+y', ...) or index -> index [-x,+x,-y,+y,-z,+z]."""
+    if isinstance(face, str):
+        if face not in _FACES:
+            raise ValueError(f"unknown face {face!r}")
+        return _FACES[face]
+    return int(face)
+
+
+def axis_index(axis) -> int:
+    if isinstance(axis, str):
+        return _AXES[axis]
+    return int(axis)
+
+
+def transform_size(dims: int) -> int:
+    return 2 * dims * dims + 1
+
+
+# ------------------------------------------------------------------------------------------
+class CSRmatrix:
+    """CSR matrix with int32 ``row``/``index`` and one or more value arrays of one pattern
+    (csrc/phipict/domain/domain_structs.h:76-110; ``getBatchSize() = |value| / nnz``)."""
+
+    def __init__(self, value, index=None, row=None, dtype=None, device=None):
+        if isinstance(value, int):
+            # CSRmatrix(numValues, numRows, dtype, device)
+            nnz, n = value, index
+            self.value = torch.zeros(nnz, dtype=dtype, device=device)
+            self.index = torch.zeros(nnz, dtype=torch.int32, device=device)
+            self.row = torch.zeros(n + 1, dtype=torch.int32, device=device)
+        else:
+            if value.dim() != 1 or index.dim() != 1 or row.dim() != 1:
+                raise ValueError("value, index and row must be 1D")
+            if index.dtype != torch.int32 or row.dtype != torch.int32:
+                raise TypeError("index and row must be int32")
+            if index.numel() > 0 and value.numel() % index.numel() != 0:
+                raise ValueError("value size must be a multiple of the index size")
+            self.value, self.index, self.row = value, index, row
+
+    def getRows(self) -> int:
+        return self.row.numel() - 1
+
+    def getNnz(self) -> int:
+        return self.index.numel()
+
+    def getSize(self) -> int:
+        return self.index.numel()
+
+    def getBatchSize(self) -> int:
+        return max(1, self.value.numel() // max(1, self.index.numel()))
+
+    def getDevice(self):
+        return self.value.device
+
+    def getDtype(self):
+        return self.value.dtype
+
+    def setValue(self, t: torch.Tensor) -> None:
+        if t.numel() != self.value.numel():
+            raise ValueError("value must have the correct size")
+        self.value = t
+
+    def CreateValue(self) -> None:
+        self.value = torch.zeros_like(self.value)
+
+    def WithZeroValue(self) -> "CSRmatrix":
+        return CSRmatrix(torch.zeros_like(self.value), self.index, self.row)
+
+    def toType(self, dtype) -> "CSRmatrix":
+        return CSRmatrix(self.value.to(dtype), self.index, self.row)
+
+    def clone(self) -> "CSRmatrix":
+        return CSRmatrix(self.value.clone(), self.index.clone(), self.row.clone())
+
+    def copy(self) -> "CSRmatrix":
+        return CSRmatrix(self.value, self.index, self.row)
+
+    def detach(self) -> None:
+        self.value = self.value.detach()
+
+    def desc(self) -> _lib.CsrDesc:
+        d = _lib.CsrDesc()
+        d.row = _lib.ptr(self.row)
+        d.col = _lib.ptr(self.index)
+        d.val = _lib.ptr(self.value)
+        d.n = self.getRows()
+        d.nnz = self.getNnz()
+        d.num_matrices = self.getBatchSize()
+        return d
+
+    def __str__(self) -> str:
+        return f"CSRmatrix(rows={self.getRows()}, nnz={self.getNnz()}, batch={self.getBatchSize()}, dtype={self.value.dtype})"
+
+
+# ------------------------------------------------------------------------------------------
+class Boundary:
+    type: BoundaryType
+
+    def __init__(self, domain: "Domain"):
+        self._domain = domain
+
+    def getParentDomain(self) -> "Domain":
+        return self._domain
+
+
+class PeriodicBoundary(Boundary):
+    type = BoundaryType.PERIODIC
+
+
+class ConnectedBoundary(Boundary):
+    """Connection to a face of another block; ``axes`` in the reference's face format
+    (csrc/phipict/domain/boundaries.cpp:857-891)."""
+
+    type = BoundaryType.CONNECTED_GRID
+
+    def __init__(self, domain, block: "Block", axes: Sequence[int]):
+        super().__init__(domain)
+        self._block = block
+        self.axes = [int(a) for a in axes]
+
+    def getConnectedBlock(self) -> "Block":
+        return self._block
+
+
+class FixedBoundary(Boundary):
+    """Prescribed face: velocity (Dirichlet or Neumann, static ``[1|B, d]`` or varying
+    ``[1|B, d, (z), y, x]`` with size 1 along the normal), the face transform and the
+    electric-potential condition (csrc/phipict/domain/boundaries.cpp)."""
+
+    type = BoundaryType.FIXED
+
+    def __init__(self, domain, block: "Block", face: int, velocity: Optional[torch.Tensor] = None,
+                 velocity_type: BoundaryConditionType = BoundaryConditionType.DIRICHLET):
+        super().__init__(domain)
+        self._block = block
+        self.face = face
+        dims = domain.getSpatialDims()
+        if velocity is None:
+            velocity = torch.zeros(1, dims, dtype=domain.getDtype(), device=domain.getDevice())
+        self.setVelocity(velocity)
+        self.velocityType = BoundaryConditionType(velocity_type)
+        self.transform: Optional[torch.Tensor] = None
+        self._potentialType = PotentialBC.INSULATING
+        self._potentialCw = 0.0
+        self.potentialTypes: Optional[torch.Tensor] = None
+        self.potentialValues: Optional[torch.Tensor] = None
+
+    # velocity
+    def setVelocity(self, velocity: torch.Tensor) -> None:
+        dims = self._domain.getSpatialDims()
+        if velocity.dim() not in (2, dims + 2) or velocity.size(1) != dims:
+            raise ValueError("boundary velocity must be [1|B, d] or [1|B, d, (z), y, x]")
+        if velocity.dim() == dims + 2:
+            expect = self.face_shape()
+            if list(velocity.shape[2:]) != expect:
+                raise ValueError(f"varying boundary velocity must have spatial shape {expect}")
+        self.velocity = velocity.to(device=self._domain.getDevice(), dtype=self._domain.getDtype()).contiguous()
+        self._domain._changed = True
+
+    def isVelocityStatic(self) -> bool:
+        return self.velocity.dim() == 2
+
+    def makeVelocityVarying(self) -> None:
+        if self.isVelocityStatic():
+            shape = [self.velocity.size(0), self.velocity.size(1)] + self.face_shape()
+            v = self.velocity.reshape(*self.velocity.shape, *([1] * (len(shape) - 2))).expand(*shape)
+            self.setVelocity(v.contiguous())
+
+    def getVelocity(self, computational: bool = False) -> torch.Tensor:
+        return self.velocity
+
+    def face_shape(self) -> list:
+        """Spatial shape [(z), y, x] of the face (size 1 along the normal)."""
+        sizes = list(self._block.getSizesZYX())
+        dims = self._domain.getSpatialDims()
+        axis = self.face >> 1
+        sizes[dims - 1 - axis] = 1
+        return sizes
+
+    def face_cells(self) -> int:
+        n = 1
+        for s in self.face_shape():
+            n *= s
+        return n
+
+    def setTransform(self, t: Optional[torch.Tensor]) -> None:
+        self.transform = t
+        self._domain._changed = True
+
+    def hasTransform(self) -> bool:
+        return self.transform is not None
+
+    # electric potential (csrc/phipict/domain/boundaries.cpp:896-993)
+    def setPotentialBC(self, type: PotentialBC, cw: Optional[float] = None) -> None:
+        type = PotentialBC(type)
+        if type == PotentialBC.THIN_WALL:
+            if cw is None or not cw > 0:
+                raise ValueError("A THIN_WALL potential BC needs a wall conductance ratio cw > 0.")
+            self._potentialCw = float(cw)
+        else:
+            if cw is not None and cw != 0.0:
+                raise ValueError("cw is only valid for a THIN_WALL potential BC.")
+            self._potentialCw = 0.0
+        self._potentialType = type
+        self.potentialTypes = None
+        self._domain._changed = True
+
+    def getPotentialBC(self) -> PotentialBC:
+        return self._potentialType
+
+    def getPotentialCw(self) -> float:
+        return self._potentialCw
+
+    def setPotentialTypes(self, types: torch.Tensor, cw: Optional[float] = None) -> None:
+        if types.is_floating_point():
+            raise TypeError("Potential BC types must be an integer tensor of PotentialBC values.")
+        if types.numel() != self.face_cells():
+            raise ValueError("Potential BC types must have one entry per face cell")
+        if int(types.min()) < 0 or int(types.max()) > 3:
+            raise ValueError("Potential BC types contain values that are not PotentialBC members.")
+        if bool((types == int(PotentialBC.THIN_WALL)).any()):
+            if cw is None or not cw > 0:
+                raise ValueError("Potential BC types contain THIN_WALL cells but no wall conductance ratio cw was given.")
+            self._potentialCw = float(cw)
+        else:
+            self._potentialCw = 0.0
+        self.potentialTypes = types.to(device=self._domain.getDevice(), dtype=torch.int8).contiguous().reshape(-1)
+        self._domain._changed = True
+
+    def clearPotentialTypes(self) -> None:
+        self.potentialTypes = None
+        self._domain._changed = True
+
+    def setPotentialValues(self, values: torch.Tensor) -> None:
+        if values.numel() != self.face_cells():
+            raise ValueError("Potential values must have one entry per face cell")
+        self.potentialValues = values.to(device=self._domain.getDevice(), dtype=self._domain.getDtype()).contiguous().reshape(-1)
+        self._domain._changed = True
+
+    def clearPotentialValues(self) -> None:
+        self.potentialValues = None
+        self._domain._changed = True
+
+    def hasPotentialDirichlet(self) -> bool:
+        if self.potentialTypes is not None:
+            return bool((self.potentialTypes == int(PotentialBC.DIRICHLET)).any())
+        return self._potentialType == PotentialBC.DIRICHLET
+
+    def hasPotentialThinWall(self) -> bool:
+        if self.potentialTypes is not None:
+            return bool((self.potentialTypes == int(PotentialBC.THIN_WALL)).any())
+        return self._potentialType == PotentialBC.THIN_WALL
+
+    def GetFluxes(self) -> torch.Tensor:
+        """Contravariant boundary fluxes det*Minv[axis].u per face cell, [vb, face cells]."""
+        dims = self._domain.getSpatialDims()
+        axis = self.face >> 1
+        vb = self.velocity.size(0)
+        fc = self.face_cells()
+        if self.isVelocityStatic():
+            vel = self.velocity.reshape(vb, dims, 1).expand(vb, dims, fc)
+        else:
+            vel = self.velocity.reshape(vb, dims, fc)
+        if self.velocityType != BoundaryConditionType.DIRICHLET and self.transform is not None:
+            vel = torch.zeros_like(vel)
+        if self.transform is None:
+            return vel[:, axis, :]
+        t = self.transform.reshape(fc, transform_size(dims))
+        minv = t[:, dims * dims:2 * dims * dims].reshape(fc, dims, dims)[:, axis, :]  # [fc, d]
+        det = t[:, 2 * dims * dims]
+        return det[None, :] * (minv.t()[None, :, :] * vel).sum(dim=1)
+
+
+# ------------------------------------------------------------------------------------------
+class Block:
+    """One structured block: fields ``velocity [B, d, (z), y, x]``, ``pressure [B, 1, ...]``,
+    optional ``epot``, ``velocitySource`` and ``viscosity``, metrics from vertex coordinates,
+    and one boundary per face (csrc/phipict/domain/block.cpp)."""
+
+    def __init__(self, domain: "Domain", velocity: torch.Tensor, pressure: Optional[torch.Tensor], name: str):
+        self._domain = domain
+        self.name = name
+        dims = domain.getSpatialDims()
+        if velocity.dim() != dims + 2 or velocity.size(1) != dims:
+            raise ValueError(f"velocity must have shape [B, {dims}, (z), y, x]")
+        self.velocity = velocity.to(device=domain.getDevice(), dtype=domain.getDtype()).contiguous()
+        spatial = list(self.velocity.shape[2:])
+        if pressure is None:
+            pressure = torch.zeros(self.velocity.size(0), 1, *spatial, dtype=domain.getDtype(), device=domain.getDevice())
+        self.pressure = pressure.to(device=domain.getDevice(), dtype=domain.getDtype()).contiguous()
+        self.epot: Optional[torch.Tensor] = None
+        self.velocitySource: Optional[torch.Tensor] = None
+        self.viscosity: Optional[torch.Tensor] = None
+        self.vertexCoordinates: Optional[torch.Tensor] = None
+        self.transform: Optional[torch.Tensor] = None
+        self.faceTransform: Optional[torch.Tensor] = None
+        self.globalOffset = 0
+        self.csrOffset = 0
+        self.boundaries: list[Boundary] = [PeriodicBoundary(domain) for _ in range(2 * dims)]
+
+    # sizes
+    def getSpatialDims(self) -> int:
+        return self._domain.getSpatialDims()
+
+    def getSizesZYX(self) -> list:
+        return list(self.velocity.shape[2:])
+
+    def getSizes(self) -> list:
+        """[x, y, z] cell counts (1 for unused dimensions)."""
+        s = list(reversed(self.getSizesZYX()))
+        return s + [1] * (3 - len(s))
+
+    def getStrides(self) -> int:
+        n = 1
+        for s in self.getSizesZYX():
+            n *= s
+        return n
+
+    def getCells(self) -> int:
+        return self.getStrides()
+
+    def getBatchSize(self) -> int:
+        return self.velocity.size(0)
+
+    def getDevice(self):
+        return self._domain.getDevice()
+
+    def getParentDomain(self) -> "Domain":
+        return self._domain
+
+    # fields
+    def setVelocity(self, v: torch.Tensor) -> None:
+        if v.shape != self.velocity.shape:
+            raise ValueError("velocity shape mismatch")
+        self.velocity = v.contiguous()
+        self._domain._changed = True
+
+    def setPressure(self, p: torch.Tensor) -> None:
+        if p.shape != self.pressure.shape:
+            raise ValueError("pressure shape mismatch")
+        self.pressure = p.contiguous()
+        self._domain._changed = True
+
+    def CreateEpot(self) -> None:
+        self.epot = torch.zeros_like(self.pressure)
+        self._domain._changed = True
+
+    def setEpot(self, e: torch.Tensor) -> None:
+        self.epot = e.contiguous()
+        self._domain._changed = True
+
+    def hasEpot(self) -> bool:
+        return self.epot is not None
+
+    def setVelocitySource(self, s: torch.Tensor) -> None:
+        """Static ``[1|B, d]`` or per-cell ``[1|B, d, (z), y, x]`` source (block.cpp:275-300)."""
+        dims = self.getSpatialDims()
+        s = s.to(device=self._domain.getDevice(), dtype=self._domain.getDtype())
+        if s.dim() == 2:
+            if s.size(1) != dims:
+                raise ValueError("static velocity source must be [1|B, d]")
+        elif s.dim() == dims + 2:
+            if list(s.shape[1:]) != [dims] + self.getSizesZYX():
+                raise ValueError("per-cell velocity source must be [1|B, d, (z), y, x]")
+        else:
+            raise ValueError("velocity source must be [1|B, d] or [1|B, d, (z), y, x]")
+        if s.size(0) not in (1, self.getBatchSize()):
+            raise ValueError("velocity source batch must be 1 or the batch size")
+        self.velocitySource = s.contiguous()
+        self._domain._changed = True
+
+    def clearVelocitySource(self) -> None:
+        self.velocitySource = None
+        self._domain._changed = True
+
+    def hasVelocitySource(self) -> bool:
+        return self.velocitySource is not None
+
+    @property
+    def isVelocitySourceStatic(self) -> bool:
+        return self.velocitySource is not None and self.velocitySource.dim() == 2
+
+    def setViscosity(self, v: torch.Tensor) -> None:
+        v = v.to(device=self._domain.getDevice(), dtype=self._domain.getDtype()).contiguous()
+        if v.numel() not in (1, self.getCells()):
+            raise ValueError("block viscosity must be static [1] or per cell")
+        self.viscosity = v
+        self._domain._changed = True
+
+    def clearViscosity(self) -> None:
+        self.viscosity = None
+        self._domain._changed = True
+
+    def hasViscosity(self) -> bool:
+        return self.viscosity is not None
+
+    def isViscosityStatic(self) -> bool:
+        return self.viscosity is not None and self.viscosity.numel() == 1
+
+    # metrics
+    def setVertexCoordinates(self, coords: torch.Tensor) -> None:
+        """Vertex coordinates ``[1, d, (z+1), y+1, x+1]`` -> cell transforms ``[1, (z), y, x, 2d^2+1]``
+        and face transforms ``[1, d, (z+1), y+1, x+1, 2d^2+1]`` (Block::setVertexCoordinates,
+        block.cpp:995-1011, kernels grid_gen.cu:298-527)."""
+        dims = self.getSpatialDims()
+        if coords.dim() != dims + 2 or coords.size(0) != 1 or coords.size(1) != dims:
+            raise ValueError("vertex coordinates must be [1, d, (z+1), y+1, x+1]")
+        if [s - 1 for s in coords.shape[2:]] != self.getSizesZYX():
+            raise ValueError("vertex coordinates spatial dimensions must match velocity's +1")
+        dev = self._domain.getDevice()
+        if dev.type != "cuda":
+            raise RuntimeError("metrics are computed by the gfx950 library; the domain must live on a GPU")
+        coords = coords.to(device=dev, dtype=self._domain.getDtype()).contiguous()
+        self.vertexCoordinates = coords
+        ts = transform_size(dims)
+        cells = (C.c_int32 * 3)(*self.getSizes())
+        self.transform = torch.empty(1, *self.getSizesZYX(), ts, dtype=coords.dtype, device=dev)
+        vshape = list(coords.shape[2:])
+        self.faceTransform = torch.empty(1, dims, *vshape, ts, dtype=coords.dtype, device=dev)
+        code = _lib.dtype_code(coords.dtype)
+        s = _lib.stream_ptr(dev)
+        _lib.call("pp_coords_to_transforms", dims, cells, _lib.ptr(coords), _lib.ptr(self.transform), code, s)
+        _lib.call("pp_coords_to_face_transforms", dims, cells, _lib.ptr(coords), _lib.ptr(self.faceTransform), code, s)
+        for b in self.boundaries:
+            if isinstance(b, FixedBoundary):
+                b.setTransform(self.faceTransformBoundarySlice(b.face))
+        self._domain._changed = True
+
+    def setTransform(self, transform: torch.Tensor, faceTransform: Optional[torch.Tensor] = None) -> None:
+        dims = self.getSpatialDims()
+        if list(transform.shape) != [1] + self.getSizesZYX() + [transform_size(dims)]:
+            raise ValueError("transform must be [1, (z), y, x, 2d^2+1]")
+        self.vertexCoordinates = None
+        self.transform = transform.to(device=self._domain.getDevice(), dtype=self._domain.getDtype()).contiguous()
+        self.faceTransform = None if faceTransform is None else faceTransform.contiguous()
+        for b in self.boundaries:
+            if isinstance(b, FixedBoundary):
+                b.setTransform(self.faceTransformBoundarySlice(b.face) if self.faceTransform is not None else None)
+        self._domain._changed = True
+
+    def hasTransform(self) -> bool:
+        return self.transform is not None
+
+    def hasFaceTransform(self) -> bool:
+        return self.faceTransform is not None
+
+    def isOrthogonal(self) -> bool:
+        """True when the metric ``Minv Minv^T`` of every cell is diagonal, i.e. the
+        non-orthogonal correction terms of the reference's assembly vanish (one device sync per
+        new transform)."""
+        t = self.transform
+        if t is None:
+            return True
+        key = (t.data_ptr(), t._version)
+        if getattr(self, "_ortho_key", None) == key:
+            return self._ortho
+        dims = self.getSpatialDims()
+        minv = t.reshape(-1, transform_size(dims))[:, dims * dims:2 * dims * dims].reshape(-1, dims, dims)
+        g = (minv.to(torch.float64) @ minv.to(torch.float64).transpose(1, 2))
+        diag = torch.diagonal(g, dim1=1, dim2=2)
+        off = g - torch.diag_embed(diag)
+        scale = diag.abs().amax(dim=1, keepdim=True).clamp_min(1e-300)
+        self._ortho = bool(((off.abs().amax(dim=2) / scale) <= 1e-10).all())
+        self._ortho_key = key
+        return self._ortho
+
+    def faceTransformBoundarySlice(self, face: int) -> torch.Tensor:
+        """Face transforms of one block face, ``[1, (z), y, x, T]`` with size 1 along the
+        normal (Block::GetFaceTransformBoundarySlice, block.cpp:790-816)."""
+        dims = self.getSpatialDims()
+        axis, upper = face >> 1, face & 1
+        ft = self.faceTransform[:, axis]
+        idx = [slice(None)]
+        for d in range(dims - 1, -1, -1):
+            if d == axis:
+                idx.append(slice(-1, None) if upper else slice(0, 1))
+            else:
+                idx.append(slice(0, -1))
+        idx.append(slice(None))
+        return ft[tuple(idx)].contiguous()
+
+    # boundaries
+    def getBoundary(self, face) -> Boundary:
+        return self.boundaries[face_index(face)]
+
+    def getBoundaries(self) -> list:
+        return list(self.boundaries)
+
+    def getFixedBoundaries(self) -> list:
+        return [(i, b) for i, b in enumerate(self.boundaries) if isinstance(b, FixedBoundary)]
+
+    def _setBoundary(self, face, boundary: Boundary) -> None:
+        self.boundaries[face_index(face)] = boundary
+        self._domain._changed = True
+
+    def _make_fixed(self, face: int, velocity, vtype) -> FixedBoundary:
+        fb = FixedBoundary(self._domain, self, face, velocity, vtype)
+        if self.faceTransform is not None:
+            fb.setTransform(self.faceTransformBoundarySlice(face))
+        self._setBoundary(face, fb)
+        return fb
+
+    def _disconnect(self, face: int) -> None:
+        b = self.boundaries[face]
+        if isinstance(b, ConnectedBoundary):
+            other = b.getConnectedBlock()
+            of = b.axes[0]
+            if isinstance(other.boundaries[of], ConnectedBoundary) and other.boundaries[of].getConnectedBlock() is self:
+                other.boundaries[of] = PeriodicBoundary(self._domain)
+                other._make_fixed(of, None, BoundaryConditionType.DIRICHLET)
+
+    def CloseBoundary(self, face, velocity: Optional[torch.Tensor] = None, passiveScalar=None) -> FixedBoundary:
+        """No-slip (or prescribed-velocity) Dirichlet face (Block::CloseBoundary, block.cpp:960-964)."""
+        f = face_index(face)
+        self._disconnect(f)
+        return self._make_fixed(f, velocity, BoundaryConditionType.DIRICHLET)
+
+    def OpenBoundary(self, face, passiveScalar=None) -> FixedBoundary:
+        """Zero-gradient (Neumann) velocity face (Block::OpenBoundary, block.cpp:969-977)."""
+        f = face_index(face)
+        self._disconnect(f)
+        return self._make_fixed(f, None, BoundaryConditionType.NEUMANN)
+
+    def CloseAllBoundaries(self) -> None:
+        for f in range(2 * self.getSpatialDims()):
+            self.CloseBoundary(f)
+
+    def MakePeriodic(self, axis) -> None:
+        a = axis_index(axis)
+        for f in (2 * a, 2 * a + 1):
+            self._disconnect(f)
+            if not isinstance(self.boundaries[f], PeriodicBoundary):
+                self._setBoundary(f, PeriodicBoundary(self._domain))
+
+    def MakeAllPeriodic(self) -> None:
+        for a in range(self.getSpatialDims()):
+            self.MakePeriodic(a)
+
+    def ConnectBlock(self, face, other: "Block", otherFace, axis1="", axis2="") -> None:
+        """Connect face ``face`` to ``otherFace`` of ``other``; ``axis1``/``axis2`` name the
+        faces of ``other`` the tangential axes map to (ConnectBlocks, boundaries.cpp:857-891)."""
+        f1, f2 = face_index(face), face_index(otherFace)
+        dims = self.getSpatialDims()
+
+        def ax(a, k):
+            if a == "" or a is None or (isinstance(a, int) and a < 0):
+                # default: the k-th tangential axis of other's face, lower side
+                return (((f2 >> 1) + k) % dims) << 1
+            return face_index(a)
+
+        c1 = ax(axis1, 1)
+        c2 = ax(axis2, 2) if dims > 2 else 0
+        self._disconnect(f1)
+        other._disconnect(f2)
+        axes1 = [f2]
+        axes2 = [f1]
+        if dims > 1:
+            axes1.append(c1)
+            d1, d2 = f1 >> 1, f2 >> 1
+            swapped = False
+            if dims == 2 or (c1 >> 1) == (d2 + 1) % dims:
+                axes2.append((((d1 + 1) % dims) << 1) | (c1 & 1))
+            else:
+                if (c2 >> 1) != (d2 + 1) % dims:
+                    raise ValueError("Invalid connection.")
+                axes2.append((((d1 + 2) % dims) << 1) | (c2 & 1))
+                swapped = True
+            if dims > 2:
+                axes1.append(c2)
+                if not swapped:
+                    axes2.append((((d1 + 2) % dims) << 1) | (c2 & 1))
+                else:
+                    axes2.append((((d1 + 1) % dims) << 1) | (c1 & 1))
+        self._setBoundary(f1, ConnectedBoundary(self._domain, other, axes1))
+        other._setBoundary(f2, ConnectedBoundary(self._domain, self, axes2))
+
+    def ComputeCSRSize(self) -> int:
+        """Non-zeros of the block's rows (Block::ComputeCSRSize, block.cpp:1145-1155)."""
+        dims = self.getSpatialDims()
+        size = self.getSizes()
+        n = self.getCells() * (2 * dims + 1)
+        for d in range(dims):
+            area = size[(d + 1) % 3] * size[(d + 2) % 3]
+            if isinstance(self.boundaries[2 * d], FixedBoundary):
+                n -= area
+            if isinstance(self.boundaries[2 * d + 1], FixedBoundary):
+                n -= area
+        return n
+
+    def __str__(self) -> str:
+        return f"Block[{self.name!r} {self.getSpatialDims()}D, size={self.getSizes()}, batch={self.getBatchSize()}]"
+
+
+# ------------------------------------------------------------------------------------------
+class Domain:
+    """Multi-block domain (csrc/phipict/domain/domain.cpp): blocks, the flat solve vectors
+    ``[B][channel][totalSize]`` and the matrices C (advection), P (pressure) and Epot."""
+
+    def __init__(self, spatialDims: int, viscosity, name: str = "Domain", dtype=torch.float32, device=None,
+                 passiveScalarChannels: int = 0):
+        if spatialDims not in (2, 3):
+            raise ValueError("spatialDims must be 2 or 3")
+        self._dims = int(spatialDims)
+        self._dtype = dtype
+        self._device = torch.device(device) if device is not None else torch.device("cuda")
+        if self._device.type == "cuda" and self._device.index is None:
+            self._device = torch.device("cuda", torch.cuda.current_device())
+        if not isinstance(viscosity, torch.Tensor):
+            viscosity = torch.tensor([float(viscosity)], dtype=dtype)
+        if viscosity.dtype != dtype:
+            raise TypeError("Data type of viscosity does not match.")
+        self.viscosity = viscosity.detach().cpu().reshape(-1)
+        self.name = name
+        self._blocks: list[Block] = []
+        self._batch = 1
+        self._initialized = False
+        self._changed = True
+        self.advectionScheme = AdvectionScheme.CENTRAL
+        self.totalSize = 0
+        self.C: Optional[CSRmatrix] = None
+        self.P: Optional[CSRmatrix] = None
+        self.Epot: Optional[CSRmatrix] = None
+        self.A: Optional[torch.Tensor] = None
+        self.velocityRHS = self.velocityResult = None
+        self.pressureRHS = self.pressureRHSdiv = self.pressureResult = None
+        self.epotRHS: Optional[torch.Tensor] = None
+        self.epotResult: Optional[torch.Tensor] = None
+        self._epot_enabled = False
+        self._epot_flags = (0, False)
+        self._time_step: Optional[torch.Tensor] = None
+        self._time_step_host: Optional[torch.Tensor] = None
+        self._atlas: Optional[torch.Tensor] = None
+        self._atlas_bytes: Optional[bytes] = None
+        self._desc = _lib.DomainDesc()
+        # bumped by every assembly of the advection matrix: the pressure operator depends on
+        # the advection diagonal only, so solvers key their pressure caches on it
+        self._a_version = 0
+
+    # --- basic accessors
+    def getSpatialDims(self) -> int:
+        return self._dims
+
+    def getDtype(self):
+        return self._dtype
+
+    def getDevice(self):
+        return self._device
+
+    def getBatchSize(self) -> int:
+        return self._batch
+
+    def getTotalSize(self) -> int:
+        return self.totalSize
+
+    def getBlocks(self) -> list:
+        return list(self._blocks)
+
+    def getBlock(self, i: int) -> Block:
+        return self._blocks[i]
+
+    def getNumBlocks(self) -> int:
+        return len(self._blocks)
+
+    def IsInitialized(self) -> bool:
+        return self._initialized
+
+    def hasEpot(self) -> bool:
+        return self.Epot is not None and self.epotRHS is not None and self.epotResult is not None
+
+    def setAdvectionScheme(self, scheme) -> None:
+        if AdvectionScheme(scheme) != AdvectionScheme.CENTRAL:
+            raise NotImplementedError("only the CENTRAL advection scheme is implemented")
+        self.advectionScheme = AdvectionScheme(scheme)
+
+    def getAdvectionScheme(self) -> AdvectionScheme:
+        return self.advectionScheme
+
+    # --- blocks
+    def CreateBlock(self, velocity: Optional[torch.Tensor] = None, pressure: Optional[torch.Tensor] = None,
+                    passiveScalar=None, vertexCoordinates: Optional[torch.Tensor] = None, name: str = "Block") -> Block:
+        if velocity is None:
+            if vertexCoordinates is None:
+                raise ValueError("velocity or vertexCoordinates is required")
+            spatial = [s - 1 for s in vertexCoordinates.shape[2:]]
+            velocity = torch.zeros(self._batch, self._dims, *spatial, dtype=self._dtype, device=self._device)
+        block = Block(self, velocity, pressure, name)
+        if vertexCoordinates is not None:
+            block.setVertexCoordinates(vertexCoordinates)
+        self._blocks.append(block)
+        self._initialized = False
+        self._changed = True
+        return block
+
+    def CreateBlockWithSize(self, size, name: str = "Block") -> Block:
+        sx, sy, sz = (list(size) + [1, 1, 1])[:3]
+        spatial = [sz, sy, sx][3 - self._dims:]
+        return self.CreateBlock(velocity=torch.zeros(self._batch, self._dims, *spatial, dtype=self._dtype,
+                                                     device=self._device), name=name)
+
+    # --- batching (Domain::setBatchSize, domain.cpp:1348-1368)
+    def setBatchSize(self, batchSize: int) -> None:
+        if batchSize < 1:
+            raise ValueError("batch size must be positive")
+        if batchSize == self._batch:
+            return
+        for b in self._blocks:
+            if b.velocity.size(0) != 1 and b.velocity.size(0) != batchSize:
+                raise ValueError("cannot change a batch size > 1 to a different one")
+            b.velocity = b.velocity[:1].repeat(batchSize, *([1] * (b.velocity.dim() - 1))).contiguous()
+            b.pressure = b.pressure[:1].repeat(batchSize, *([1] * (b.pressure.dim() - 1))).contiguous()
+            if b.epot is not None:
+                b.epot = b.epot[:1].repeat(batchSize, *([1] * (b.epot.dim() - 1))).contiguous()
+        self._batch = batchSize
+        self._initialized = False
+        self._changed = True
+
+    # --- setup (Domain::PrepareSolve, domain.cpp:426-561)
+    def PrepareSolve(self) -> None:
+        if not self._blocks:
+            raise RuntimeError("Domain does not contain any blocks.")
+        dims, B = self._dims, self._batch
+        csr, total = 0, 0
+        for b in self._blocks:
+            b.csrOffset = csr
+            b.globalOffset = total
+            csr += b.ComputeCSRSize()
+            total += b.getCells()
+            if b.getBatchSize() != B:
+                raise RuntimeError("All blocks must have the same batch size.")
+        for bi, b in enumerate(self._blocks):
+            for f, bound in enumerate(b.boundaries):
+                if isinstance(bound, ConnectedBoundary):
+                    other = bound.getConnectedBlock()
+                    if other not in self._blocks:
+                        raise RuntimeError("Connected block is not part of this domain.")
+                    ob = other.boundaries[bound.axes[0]]
+                    if not (isinstance(ob, ConnectedBoundary) and ob.getConnectedBlock() is b and ob.axes[0] == f):
+                        raise RuntimeError("Mismatch in block connection.")
+                if isinstance(bound, PeriodicBoundary) and not isinstance(b.boundaries[f ^ 1], PeriodicBoundary):
+                    raise RuntimeError("Opposite boundary must also be periodic.")
+        self.totalSize = total
+        dev, dt = self._device, self._dtype
+        row = torch.zeros(total + 1, dtype=torch.int32, device=dev)
+        index = torch.zeros(csr, dtype=torch.int32, device=dev)
+        self.C = CSRmatrix(torch.zeros(csr * B, dtype=dt, device=dev), index, row)
+        self.P = CSRmatrix(torch.zeros(csr * B, dtype=dt, device=dev), index, row)
+        self.A = torch.zeros(B * total, dtype=dt, device=dev)
+        self.velocityRHS = torch.zeros(B * total * dims, dtype=dt, device=dev)
+        self.velocityResult = torch.zeros(B * total * dims, dtype=dt, device=dev)
+        self.pressureRHS = torch.zeros(B * total * dims, dtype=dt, device=dev)
+        self.pressureRHSdiv = torch.zeros(B * total, dtype=dt, device=dev)
+        self.pressureResult = torch.zeros(B * total, dtype=dt, device=dev)
+        if self._time_step is None or self._time_step.numel() != B:
+            self.setTimeStep(torch.ones(B, dtype=dt))
+        self._initialized = True
+        self._changed = True
+        self.UpdateDomainData()
+        _lib.call("pp_csr_pattern", C.byref(self._desc), _lib.ptr(row), _lib.ptr(index), _lib.stream_ptr(dev))
+        if self._epot_enabled:
+            self.SetupEpotOnDomain(*self._epot_flags)
+
+    # --- flat vectors
+    def _check_flat(self, t: torch.Tensor, channels: int, name: str) -> torch.Tensor:
+        if t.dim() != 1 or t.numel() != self._batch * self.totalSize * channels:
+            raise ValueError(f"{name} must be flat with {self._batch * self.totalSize * channels} entries")
+        if t.dtype != self._dtype:
+            raise TypeError(f"{name} dtype mismatch")
+        self._changed = True
+        return t.contiguous()
+
+    def setVelocityRHS(self, t): self.velocityRHS = self._check_flat(t, self._dims, "velocityRHS")
+    def setVelocityResult(self, t): self.velocityResult = self._check_flat(t, self._dims, "velocityResult")
+    def setPressureRHS(self, t): self.pressureRHS = self._check_flat(t, self._dims, "pressureRHS")
+    def setPressureRHSdiv(self, t): self.pressureRHSdiv = self._check_flat(t, 1, "pressureRHSdiv")
+    def setPressureResult(self, t): self.pressureResult = self._check_flat(t, 1, "pressureResult")
+    def setEpotRHS(self, t): self.epotRHS = self._check_flat(t, 1, "epotRHS")
+    def setEpotResult(self, t): self.epotResult = self._check_flat(t, 1, "epotResult")
+
+    def setA(self, t):
+        self.A = self._check_flat(t, 1, "A")
+
+    def CreateVelocityResult(self):
+        self.velocityResult = torch.zeros(self._batch * self.totalSize * self._dims, dtype=self._dtype, device=self._device)
+        self._changed = True
+
+    def CreatePressureResult(self):
+        self.pressureResult = torch.zeros(self._batch * self.totalSize, dtype=self._dtype, device=self._device)
+        self._changed = True
+
+    # --- time step (Domain::setTimeStep, domain.cpp:370-400): one value or one per environment
+    def setTimeStep(self, timeStep) -> None:
+        if not isinstance(timeStep, torch.Tensor):
+            timeStep = torch.tensor([float(timeStep)], dtype=self._dtype)
+        ts = timeStep.detach().to("cpu", self._dtype).reshape(-1)
+        if ts.numel() == 1:
+            ts = ts.repeat(self._batch)
+        if ts.numel() != self._batch:
+            raise ValueError("time step must have 1 or batch-size entries")
+        if self._time_step_host is not None and torch.equal(ts, self._time_step_host):
+            return
+        self._time_step_host = ts.clone()
+        dev_ts = ts.to(self._device)
+        if self._time_step is not None and self._time_step.numel() == dev_ts.numel():
+            self._time_step.copy_(dev_ts)
+        else:
+            self._time_step = dev_ts
+            self._changed = True
+
+    def getTimeStep(self) -> Optional[torch.Tensor]:
+        return self._time_step_host
+
+    # --- electric potential (Domain::SetupEpotOnDomain, domain.cpp:756-781)
+    def SetupEpotOnDomain(self, nonOrthoFlags: int = 0, useFaceTransform: bool = False) -> None:
+        if not self._initialized:
+            raise RuntimeError("Domain is not initialized. Run domain.PrepareSolve() first.")
+        if useFaceTransform:
+            raise NotImplementedError("face-transform Laplacians are not implemented")
+        if nonOrthoFlags and not self.isOrthogonal():
+            raise NotImplementedError("non-orthogonal correction terms on skewed grids are not implemented")
+        self._epot_enabled = True
+        self._epot_flags = (nonOrthoFlags, useFaceTransform)
+        if self.epotRHS is None or self.epotRHS.numel() != self._batch * self.totalSize:
+            self.epotRHS = torch.zeros(self._batch * self.totalSize, dtype=self._dtype, device=self._device)
+        if self.epotResult is None or self.epotResult.numel() != self._batch * self.totalSize:
+            self.epotResult = torch.zeros(self._batch * self.totalSize, dtype=self._dtype, device=self._device)
+        self.Epot = CSRmatrix(torch.zeros(self.P.getNnz(), dtype=self._dtype, device=self._device), self.P.index, self.P.row)
+        for b in self._blocks:
+            if b.epot is None:
+                b.CreateEpot()
+        self._changed = True
+        self.UpdateDomainData()
+        _lib.call("pp_setup_epot_matrix", C.byref(self._desc), int(nonOrthoFlags), _lib.stream_ptr(self._device))
+
+    # --- descriptors
+    def _block_desc(self, b: Block) -> _lib.BlockDesc:
+        d = _lib.BlockDesc()
+        size = b.getSizes()
+        for i in range(3):
+            d.size[i] = size[i]
+        d.cells = b.getCells()
+        d.global_offset = b.globalOffset
+        d.csr_offset = b.csrOffset
+        d.has_transform = 1 if b.transform is not None else 0
+        d.transform = _lib.ptr(b.transform)
+        d.velocity = _lib.ptr(b.velocity)
+        d.pressure = _lib.ptr(b.pressure)
+        d.epot = _lib.ptr(b.epot)
+        if b.velocitySource is None:
+            d.vsource_mode = 0
+        else:
+            d.vsource_mode = 1 if b.velocitySource.dim() == 2 else 2
+            d.vsource_batched = 1 if b.velocitySource.size(0) > 1 else 0
+            d.vsource = _lib.ptr(b.velocitySource)
+        if b.viscosity is None:
+            d.visc_mode = 0
+        else:
+            d.visc_mode = 1 if b.viscosity.numel() == 1 else 2
+            d.viscosity = _lib.ptr(b.viscosity)
+        for f in range(2 * self._dims):
+            bound = b.boundaries[f]
+            bd = d.bnd[f]
+            bd.type = int(bound.type)
+            if isinstance(bound, FixedBoundary):
+                bd.vel_type = int(bound.velocityType)
+                bd.vel_static = 1 if bound.isVelocityStatic() else 0
+                bd.vel_batched = 1 if bound.velocity.size(0) > 1 else 0
+                bd.vel = _lib.ptr(bound.velocity)
+                bd.has_transform = 1 if bound.transform is not None else 0
+                bd.transform = _lib.ptr(bound.transform)
+                bd.pot_type = int(bound.getPotentialBC())
+                bd.pot_types = _lib.ptr(bound.potentialTypes)
+                bd.pot_values = _lib.ptr(bound.potentialValues)
+                bd.pot_cw = bound.getPotentialCw()
+            elif isinstance(bound, ConnectedBoundary):
+                bd.conn_block = self._blocks.index(bound.getConnectedBlock())
+                for i in range(3):
+                    bd.conn_axes[i] = bound.axes[i] if i < len(bound.axes) else 0
+        return d
+
+    def UpdateDomainData(self) -> None:
+        """Repack the device descriptors if any tensor of the domain changed."""
+        if not self._initialized:
+            raise RuntimeError("Domain is not initialized.")
+        arr = (_lib.BlockDesc * len(self._blocks))(*[self._block_desc(b) for b in self._blocks])
+        raw = bytes(arr)
+        if raw != self._atlas_bytes:
+            host = torch.frombuffer(bytearray(raw), dtype=torch.uint8)
+            if self._atlas is not None and self._atlas.numel() == host.numel():
+                self._atlas.copy_(host)
+            else:
+                self._atlas = host.to(self._device)
+            self._atlas_bytes = raw
+        d = self._desc
+        d.dims = self._dims
+        d.num_blocks = len(self._blocks)
+        d.num_cells = self.totalSize
+        d.batch = self._batch
+        d.dtype = _lib.dtype_code(self._dtype)
+        d.max_block_cells = max(b.getCells() for b in self._blocks)
+        d.viscosity = float(self.viscosity[0])
+        d.blocks = _lib.ptr(self._atlas)
+        d.time_step = _lib.ptr(self._time_step)
+        d.A = _lib.ptr(self.A)
+        d.velocity_rhs = _lib.ptr(self.velocityRHS)
+        d.velocity_result = _lib.ptr(self.velocityResult)
+        d.pressure_rhs = _lib.ptr(self.pressureRHS)
+        d.pressure_rhs_div = _lib.ptr(self.pressureRHSdiv)
+        d.pressure_result = _lib.ptr(self.pressureResult)
+        d.epot_rhs = _lib.ptr(self.epotRHS)
+        d.epot_result = _lib.ptr(self.epotResult)
+        d.C = self.C.desc() if self.C is not None else _lib.CsrDesc()
+        d.P = self.P.desc() if self.P is not None else _lib.CsrDesc()
+        d.E = self.Epot.desc() if self.Epot is not None else _lib.CsrDesc()
+        self._changed = False
+
+    def IsTensorChanged(self) -> bool:
+        return self._changed
+
+    def desc(self) -> _lib.DomainDesc:
+        if self._changed:
+            self.UpdateDomainData()
+        return self._desc
+
+    # --- diagnostics
+    def GetBoundaryFluxBalance(self) -> torch.Tensor:
+        """Net outward flux through all prescribed faces, one value per environment
+        (Domain::GetGlobalFluxBalance, domain.cpp:296-331)."""
+        total = torch.zeros(self._batch, dtype=torch.float64)
+        for b in self._blocks:
+            for f, bound in b.getFixedBoundaries():
+                sign = 1.0 if (f & 1) else -1.0
+                fl = bound.GetFluxes().to(torch.float64).sum(dim=-1).cpu()
+                total += sign * (fl if fl.numel() == self._batch else fl.expand(self._batch))
+        return total
+
+    def isOrthogonal(self) -> bool:
+        return all(b.isOrthogonal() for b in self._blocks)
+
+    def getVertexCoordinates(self) -> list:
+        return [b.vertexCoordinates for b in self._blocks]
+
+    def __str__(self) -> str:
+        return f"Domain[{self.name!r} {self._dims}D, blocks={len(self._blocks)}, cells={self.totalSize}, batch={self._batch}]"

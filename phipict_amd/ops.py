@@ -1,0 +1,337 @@
+"""The reference's ``phipict._C`` operator functions for the PISO path, backed by the gfx950
+library. Names, argument order, defaults and return values follow
+csrc/phipict/bindings.cpp:639-821; every call runs on the current torch stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import torch
+
+from . import _lib
+from .domain import ConvergenceCriterion, CSRmatrix, Domain
+
+# non-orthogonal flag bits (csrc/phipict/piso/non_ortho_flags.h)
+NON_ORTHO_DIRECT_MATRIX = 1
+NON_ORTHO_DIRECT_RHS = 2
+NON_ORTHO_DIAGONAL_MATRIX = 4
+NON_ORTHO_DIAGONAL_RHS = 8
+NON_ORTHO_CENTER_MATRIX = 16
+
+
+def _prep(domain: Domain, timeStep=None, nonOrthoFlags: int = 0) -> _lib.DomainDesc:
+    if not domain.IsInitialized():
+        raise RuntimeError("Domain is not initialized.")
+    if domain.getNumBlocks() == 0:
+        raise RuntimeError("Domain does not contain blocks.")
+    if nonOrthoFlags and not domain.isOrthogonal():
+        # the corner/diagonal correction terms of skewed grids (piso/device/nonortho_*.cuh) are
+        # not assembled; on grids whose metric is diagonal they vanish and the flags are no-ops
+        raise NotImplementedError("non-orthogonal correction terms on skewed grids are not implemented")
+    if timeStep is not None:
+        domain.setTimeStep(timeStep)
+    return domain.desc()
+
+
+def _stream(domain: Domain) -> int:
+    return _lib.stream_ptr(domain.getDevice())
+
+
+def SetupAdvectionMatrix(domain: Domain, timeStep, nonOrthoFlags: int = 0, forPassiveScalar: bool = False,
+                         passiveScalarChannel: int = 0) -> None:
+    """Implicit advection-diffusion matrix C and its diagonal A (bindings.cpp:639-641)."""
+    if forPassiveScalar:
+        raise NotImplementedError("passive-scalar advection is not implemented")
+    d = _prep(domain, timeStep, nonOrthoFlags)
+    _lib.call("pp_setup_advection_matrix", C.byref(d), int(nonOrthoFlags), _stream(domain))
+
+
+def SetupAdvectionVelocity(domain: Domain, timeStep, nonOrthoFlags: int = 0, applyPressureGradient: bool = False) -> None:
+    """Momentum right-hand side into ``domain.velocityRHS`` (bindings.cpp:645-647)."""
+    d = _prep(domain, timeStep, nonOrthoFlags)
+    _lib.call("pp_setup_advection_velocity", C.byref(d), int(nonOrthoFlags), int(bool(applyPressureGradient)), _stream(domain))
+
+
+def SetupPressureMatrix(domain: Domain, timeStep, nonOrthoMode: int = 0, useFaceTransform: bool = False) -> None:
+    if useFaceTransform:
+        raise NotImplementedError("face-transform pressure Laplacians are not implemented")
+    d = _prep(domain, timeStep, nonOrthoMode)
+    _lib.call("pp_setup_pressure_matrix", C.byref(d), int(nonOrthoMode), _stream(domain))
+
+
+def SetupPressureRHS(domain: Domain, timeStep, nonOrthoMode: int = 0, useFaceTransform: bool = False,
+                     timeStepNorm: bool = False) -> None:
+    if useFaceTransform:
+        raise NotImplementedError("face-transform fluxes are not implemented")
+    d = _prep(domain, timeStep, nonOrthoMode)
+    _lib.call("pp_setup_pressure_rhs", C.byref(d), int(nonOrthoMode), int(bool(timeStepNorm)), _stream(domain))
+
+
+def SetupPressureRHSdiv(domain: Domain, timeStep, nonOrthoMode: int = 0, useFaceTransform: bool = False,
+                        timeStepNorm: bool = False) -> None:
+    if useFaceTransform:
+        raise NotImplementedError("face-transform fluxes are not implemented")
+    d = _prep(domain, timeStep, nonOrthoMode)
+    _lib.call("pp_setup_pressure_rhs_div", C.byref(d), int(nonOrthoMode), int(bool(timeStepNorm)), _stream(domain))
+
+
+def SetupPressureCorrection(domain: Domain, timeStep, nonOrthoMode: int = 0, useFaceTransform: bool = False,
+                            timeStepNorm: bool = False) -> None:
+    """Matrix, HbyA right-hand side and its divergence (pressure.cu:882-945)."""
+    SetupPressureMatrix(domain, timeStep, nonOrthoMode, useFaceTransform)
+    SetupPressureRHS(domain, timeStep, nonOrthoMode, useFaceTransform, timeStepNorm)
+
+
+def CorrectVelocity(domain: Domain, timeStep, version: int = 0, timeStepNorm: bool = False) -> None:
+    """u = HbyA - grad(p)/A with finite-difference gradients (versions 0 and 1,
+    velocity_correction.cu:216-271)."""
+    if version not in (0, 1):
+        raise NotImplementedError("only the finite-difference velocity corrector (version 0/1) is implemented")
+    d = _prep(domain, timeStep)
+    _lib.call("pp_correct_velocity", C.byref(d), int(bool(timeStepNorm)), _stream(domain))
+
+
+def SetupEpotMatrix(domain: Domain, nonOrthoFlags: int = 0, useFaceTransform: bool = False) -> None:
+    if domain.Epot is None:
+        raise RuntimeError("domain.Epot is not allocated. Call domain.SetupEpotOnDomain() first.")
+    d = _prep(domain, None, nonOrthoFlags)
+    _lib.call("pp_setup_epot_matrix", C.byref(d), int(nonOrthoFlags), _stream(domain))
+
+
+def ComputeEpotRHS(domain: Domain, vectorField: torch.Tensor, useFaceTransform: bool = False) -> torch.Tensor:
+    """div(u x B) per cell, ``[B * n]`` (potential.cu:325-370); ``vectorField`` is ``[B][d][n]``."""
+    n, B, dims = domain.getTotalSize(), domain.getBatchSize(), domain.getSpatialDims()
+    if vectorField.dim() != 1 or vectorField.numel() != n * dims * B:
+        raise ValueError("vectorField must be flat [batch][dims][totalSize]")
+    if vectorField.dtype != domain.getDtype():
+        raise TypeError("vectorField dtype must match the domain dtype")
+    vf = vectorField.contiguous()
+    out = torch.empty(n * B, dtype=domain.getDtype(), device=domain.getDevice())
+    d = _prep(domain)
+    _lib.call("pp_compute_epot_rhs", C.byref(d), _lib.ptr(vf), _lib.ptr(out), _stream(domain))
+    return out
+
+
+def ComputeCurrentDensityFaceBased(domain: Domain, epotField: torch.Tensor, uCrossBField: torch.Tensor) -> torch.Tensor:
+    """Face-based current density J, ``[B][3][n]`` (potential.cu:918-960). ``uCrossBField`` is
+    ``[B][3][n]`` (3 components also in 2D)."""
+    n, B, dims = domain.getTotalSize(), domain.getBatchSize(), domain.getSpatialDims()
+    if epotField.dim() != 1 or epotField.numel() != n * B:
+        raise ValueError("epotField size must equal domain totalSize (* batch size).")
+    if uCrossBField.numel() == n * dims * B and dims == 3:
+        uxb = uCrossBField.contiguous()
+    elif uCrossBField.numel() == n * 3 * B:
+        uxb = uCrossBField.contiguous()
+    else:
+        raise ValueError("uCrossBField size must be totalSize * 3 (* batch size)")
+    if dims == 2 and uxb.numel() != n * 3 * B:
+        raise ValueError("2D domains need the 3-component u x B field")
+    phi = epotField.contiguous()
+    out = torch.zeros(n * 3 * B, dtype=domain.getDtype(), device=domain.getDevice())
+    d = _prep(domain)
+    _lib.call("pp_compute_current_density", C.byref(d), _lib.ptr(phi), _lib.ptr(uxb), _lib.ptr(out), _stream(domain))
+    return out
+
+
+def _copy(domain: Domain, field: int, to_blocks: int) -> None:
+    d = _prep(domain)
+    _lib.call("pp_copy_result", C.byref(d), field, to_blocks, _stream(domain))
+
+
+def CopyVelocityResultToBlocks(domain: Domain) -> None: _copy(domain, 0, 1)
+def CopyVelocityResultFromBlocks(domain: Domain) -> None: _copy(domain, 0, 0)
+def CopyPressureResultToBlocks(domain: Domain) -> None: _copy(domain, 1, 1)
+def CopyPressureResultFromBlocks(domain: Domain) -> None: _copy(domain, 1, 0)
+def CopyEpotResultToBlocks(domain: Domain) -> None: _copy(domain, 2, 1)
+def CopyEpotResultFromBlocks(domain: Domain) -> None: _copy(domain, 2, 0)
+
+
+def CoordsToTransforms(coords: torch.Tensor) -> torch.Tensor:
+    dims = coords.dim() - 2
+    cells = [s - 1 for s in reversed(coords.shape[2:])] + [1] * (3 - dims)
+    out = torch.empty(1, *[s - 1 for s in coords.shape[2:]], 2 * dims * dims + 1, dtype=coords.dtype, device=coords.device)
+    c = coords.contiguous()
+    _lib.call("pp_coords_to_transforms", dims, (C.c_int32 * 3)(*cells), _lib.ptr(c), _lib.ptr(out),
+              _lib.dtype_code(c.dtype), _lib.stream_ptr(c.device))
+    return out
+
+
+def CoordsToFaceTransforms(coords: torch.Tensor) -> torch.Tensor:
+    dims = coords.dim() - 2
+    cells = [s - 1 for s in reversed(coords.shape[2:])] + [1] * (3 - dims)
+    out = torch.empty(1, dims, *coords.shape[2:], 2 * dims * dims + 1, dtype=coords.dtype, device=coords.device)
+    c = coords.contiguous()
+    _lib.call("pp_coords_to_face_transforms", dims, (C.c_int32 * 3)(*cells), _lib.ptr(c), _lib.ptr(out),
+              _lib.dtype_code(c.dtype), _lib.stream_ptr(c.device))
+    return out
+
+
+# ---------------------------------------------------------------- linear solvers
+@dataclass
+class LinearSolverResultInfo:
+    """csrc/phipict/solvers/linear_solvers.h:180-185"""
+
+    finalResidual: float
+    usedIterations: int
+    converged: bool
+    isFiniteResidual: bool
+
+    def __str__(self) -> str:
+        return (f"LinearSolverResultInfo(finalResidual={self.finalResidual:.3e}, usedIterations={self.usedIterations}, "
+                f"converged={self.converged}, isFiniteResidual={self.isFiniteResidual})")
+
+
+@dataclass
+class KrylovSettings:
+    """The reference's krylov::GlobalSettings (krylov.h:68-74)."""
+
+    enabled: bool = True
+    checkInterval: int = 8
+    useGraphs: bool = True
+    cgPreconditioner: int = 0
+    bicgPreconditioner: int = 0
+
+
+_KRYLOV = KrylovSettings()
+
+
+def SetKrylovSettings(enabled: bool = True, checkInterval: int = 8, useGraphs: bool = True, cgPreconditioner: int = 0,
+                      bicgPreconditioner: int = 0) -> None:
+    if cgPreconditioner or bicgPreconditioner:
+        raise NotImplementedError("Jacobi-preconditioned fused solvers are not implemented")
+    _KRYLOV.enabled, _KRYLOV.checkInterval, _KRYLOV.useGraphs = bool(enabled), int(checkInterval), bool(useGraphs)
+
+
+def GetKrylovSettings() -> KrylovSettings:
+    return KrylovSettings(**_KRYLOV.__dict__)
+
+
+def _results(res, n) -> list:
+    return [LinearSolverResultInfo(float(r.final_residual), int(r.used_iterations), bool(r.converged),
+                                   bool(r.is_finite_residual)) for r in res[:n]]
+
+
+def _host_scalar_list(t, dtype) -> list:
+    if isinstance(t, torch.Tensor):
+        if t.device.type != "cpu":
+            raise ValueError("tolerance and maxit must be host tensors")
+        return [float(v) for v in t.reshape(-1).tolist()]
+    if isinstance(t, (list, tuple)):
+        return [float(v) for v in t]
+    return [float(t)]
+
+
+def SolveLinear(A: CSRmatrix, RHS: torch.Tensor, x: torch.Tensor, maxIterations=1000, tolerance=1e-8,
+                convergenceCriterion=ConvergenceCriterion.NORM2_NORMALIZED, useBiCG: bool = False,
+                matrixRankDeficient: bool = False, residualResetSteps: int = 0, transposeA: bool = False,
+                printResidual: bool = False, returnBestResult: bool = False, BiCGwithPreconditioner: bool = True) -> list:
+    """Solve A x = RHS in place in ``x`` (linear_solve.cu:131-164) with the fused CG or BiCGStab.
+
+    ``RHS``/``x`` hold ``k * n`` entries; RHS k uses matrix ``k / (k_total / A.getBatchSize())``.
+    ``matrixRankDeficient`` marks a singular operator with the constant nullspace: the solve is
+    run on the mean-free right-hand side (the reference routes this flag to its legacy cuBLAS
+    CG instead). ``BiCGwithPreconditioner`` is accepted for signature compatibility; the fused
+    BiCGStab is unpreconditioned, as the reference's default settings select."""
+    n = A.getRows()
+    if RHS.dim() != 1 or x.dim() != 1:
+        raise ValueError("RHS and x must be vectors.")
+    if RHS.numel() % n != 0:
+        raise ValueError("Size of RHS must be a multiple of the matrix size.")
+    if x.numel() != RHS.numel():
+        raise ValueError("Size of x must match RHS.")
+    if RHS.dtype != A.getDtype() or x.dtype != A.getDtype():
+        raise TypeError("Data type of RHS and x must match A.")
+    if not x.is_contiguous():
+        raise ValueError("x must be contiguous (the solve writes it in place)")
+    if convergenceCriterion not in (ConvergenceCriterion.NORM2, ConvergenceCriterion.NORM2_NORMALIZED):
+        raise NotImplementedError("only the NORM2 and NORM2_NORMALIZED criteria are implemented")
+    nrhs = RHS.numel() // n
+    maxit = int(_host_scalar_list(maxIterations, torch.int32)[0])
+    tols = _host_scalar_list(tolerance, A.getDtype())
+    if len(tols) not in (1, nrhs):
+        raise ValueError("tol must be a scalar or one value per right-hand side.")
+    rhs = RHS.contiguous()
+    res = (_lib.SolveResult * nrhs)()
+    tarr = (C.c_double * len(tols))(*tols)
+    desc = A.desc()
+    _lib.call("pp_solve_linear", C.byref(desc), _lib.ptr(rhs), _lib.ptr(x), nrhs, _lib.dtype_code(A.getDtype()), maxit,
+              tarr, len(tols), int(convergenceCriterion), int(bool(useBiCG)), int(bool(matrixRankDeficient)),
+              int(residualResetSteps), int(bool(transposeA)), int(bool(returnBestResult)),
+              _lib.stream_ptr(x.device), res)
+    return _results(res, nrhs)
+
+
+def AMGPCGSolve(aRow, aCol, aVal, levels: Sequence[Sequence[torch.Tensor]], coarsePinv: torch.Tensor, rhs: torch.Tensor,
+                x: torch.Tensor, maxIterations: int, tolerance: float, normalized: bool = True, returnBest: bool = False,
+                projectConstant: bool = False, preSweeps: int = 1, postSweeps: int = 1, omega: float = 2.0 / 3.0,
+                tolerances: Sequence[float] = ()) -> list:
+    """AMG-preconditioned CG (bindings.cpp:764-776 -> krylov.cu:1565-1612). ``levels`` holds
+    per level ``(A row, col, val, dinv, R row, col, val, P row, col, val)``; the hierarchy
+    dtype is that of ``coarsePinv``; ``x`` is updated in place."""
+    n = aRow.numel() - 1
+    if rhs.numel() % n != 0 or rhs.numel() != x.numel():
+        raise ValueError("rhs and x must be contiguous and of equal size, a multiple of the matrix size")
+    nrhs = rhs.numel() // n
+    hd = coarsePinv.dtype
+    lv = (_lib.AmgLevel * len(levels))()
+    keep = []
+    for i, t in enumerate(levels):
+        if len(t) != 10:
+            raise ValueError("each AMG level needs (A row, col, val, dinv, R row, col, val, P row, col, val)")
+        if t[2].dtype != hd or t[3].dtype != hd or t[6].dtype != hd or t[9].dtype != hd:
+            raise TypeError("all hierarchy values must have the dtype of the coarse pseudo-inverse")
+        L = lv[i]
+        L.n = t[0].numel() - 1
+        L.nnz_a = t[1].numel()
+        L.n_coarse = t[4].numel() - 1
+        L.nnz_r = t[5].numel()
+        L.nnz_p = t[8].numel()
+        L.a_batched = 1 if t[2].numel() != t[1].numel() else 0
+        for name, tt in zip(("a_row", "a_col", "a_val", "dinv", "r_row", "r_col", "r_val", "p_row", "p_col", "p_val"), t):
+            tt = tt.contiguous()
+            keep.append(tt)
+            setattr(L, name, _lib.ptr(tt))
+    nc = coarsePinv.size(-1)
+    batched_coarse = 1 if (coarsePinv.dim() == 3 and nrhs > 1) else 0
+    pinv = coarsePinv.contiguous()
+    A = CSRmatrix(aVal, aCol, aRow)
+    desc = A.desc()
+    res = (_lib.SolveResult * nrhs)()
+    tl = list(tolerances)
+    tarr = (C.c_double * max(1, len(tl)))(*(tl if tl else [float(tolerance)]))
+    rhs_c = rhs.contiguous()
+    if not x.is_contiguous():
+        raise ValueError("x must be contiguous")
+    _lib.call("pp_amg_pcg_solve", C.byref(desc), lv, len(levels), _lib.ptr(pinv), nc, batched_coarse, _lib.dtype_code(hd),
+              _lib.ptr(rhs_c), _lib.ptr(x), nrhs, _lib.dtype_code(aVal.dtype), int(maxIterations), float(tolerance),
+              int(bool(normalized)), int(bool(returnBest)), int(bool(projectConstant)), int(preSweeps), int(postSweeps),
+              float(omega), tarr, len(tl), _lib.stream_ptr(x.device), res)
+    del keep
+    return _results(res, nrhs)
+
+
+def ReleaseKrylovWorkspaces() -> None:
+    _lib.call("pp_release_workspaces")
+
+
+def SparseOuterProduct(a: torch.Tensor, b: torch.Tensor, out_pattern: CSRmatrix) -> None:
+    """out.value[k] = a[row(k)] * b[col(k)] on the pattern of ``out_pattern`` (linear_solve.cu:166-190)."""
+    n = out_pattern.getRows()
+    if a.numel() != n or b.numel() != n:
+        raise ValueError("Size of a and b must match matrix size.")
+    desc = out_pattern.desc()
+    _lib.call("pp_sparse_outer_product", _lib.ptr(a.contiguous()), _lib.ptr(b.contiguous()), C.byref(desc),
+              _lib.dtype_code(out_pattern.getDtype()), _lib.stream_ptr(a.device))
+
+
+def spmv(A: CSRmatrix, x: torch.Tensor) -> torch.Tensor:
+    """y = A x for every right-hand side held in ``x``."""
+    n = A.getRows()
+    nrhs = x.numel() // n
+    y = torch.empty_like(x)
+    desc = A.desc()
+    _lib.call("pp_spmv", C.byref(desc), _lib.ptr(x.contiguous()), _lib.ptr(y), nrhs, _lib.dtype_code(A.getDtype()),
+              _lib.stream_ptr(x.device))
+    return y

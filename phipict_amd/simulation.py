@@ -1,0 +1,395 @@
+"""PISO time step driver.
+
+Restates the control flow of the reference's ``Simulation._PISO_split_step``
+(src/phipict/core/piso_simulation.py:3525-4169) and ``single_step``
+(src/phipict/simulation/simulation.py:262-333): hooks, momentum prediction (BiCGStab),
+pressure correctors (CG or AMG-PCG), velocity correction, mean projection of the pressure,
+tolerance resolution and the pressure AMG refresh policy (:2222-2392). Every kernel and solve
+runs in the gfx950 library through :mod:`phipict_amd.ops`.
+"""
+from __future__ import annotations
+
+import logging
+from typing import Any, Optional
+
+import torch
+
+from . import ops
+from .domain import Domain
+from .hooks import Hook, Hooks
+from .solvers import amg as amg_mod
+from .solvers.tolerance import SolverTolerance, default_tolerance, resolve_tolerance
+
+_LOG = logging.getLogger("phipict_amd.piso")
+
+
+class LinsolveError(RuntimeError):
+    """A linear solve did not converge or produced a non-finite residual
+    (src/phipict/core/piso_diff.py:655-791)."""
+
+
+def subtract_env_mean(flat: torch.Tensor, batch: int) -> torch.Tensor:
+    """Subtract each batched environment's mean (piso_simulation.py:68-89)."""
+    if batch == 1:
+        return flat - torch.mean(flat)
+    per_env = flat.reshape(batch, -1)
+    return (per_env - per_env.mean(dim=1, keepdim=True)).reshape(flat.shape)
+
+
+def check_solver_return_infos(infos, tol, max_iter, return_best_result: bool, tag: str = "") -> None:
+    """Raise LinsolveError like the reference (piso_diff.py:659-791): always for a non-finite
+    residual, for non-convergence unless the best iterate is returned."""
+    if any(not i.isFiniteResidual for i in infos):
+        bad = [k for k, i in enumerate(infos) if not i.isFiniteResidual]
+        raise LinsolveError(f"Linear solve ({tag}, tol:{tol:.03e}) reported non-finite residual for RHS {bad} of "
+                            f"{len(infos)}: {[str(i) for i in infos]}")
+    if any(not i.converged for i in infos):
+        bad = [k for k, i in enumerate(infos) if not i.converged]
+        msg = f"Linear solve ({tag}, tol:{tol:.03e}) RHS {bad} of {len(infos)} did not converge after {max_iter} iterations."
+        if return_best_result:
+            _LOG.warning(msg)
+        else:
+            raise LinsolveError(msg + f" Solver infos: {[str(i) for i in infos]}")
+
+
+class Simulation:
+    """PISO simulation of an incompressible flow on a multi-block domain (the reference's
+    ``phipict.Simulation``; constructor arguments as simulation.py:166-248)."""
+
+    NON_ORTHO_DIRECT_MATRIX = 1
+    NON_ORTHO_DIRECT_RHS = 2
+    NON_ORTHO_DIAGONAL_MATRIX = 4
+    NON_ORTHO_DIAGONAL_RHS = 8
+    NON_ORTHO_CENTER_MATRIX = 16
+    PRESSURE_AMG_AUTO_MIN_ROWS = 200_000
+    _NON_ORTHO_MODE = NON_ORTHO_CENTER_MATRIX | NON_ORTHO_DIRECT_MATRIX | NON_ORTHO_DIAGONAL_RHS
+
+    def __init__(self, domain: Optional[Domain] = None, dt: float = 1.0, substeps: int = 1, corrector_steps: int = 2,
+                 advection_use_BiCG: bool = True, pressure_use_BiCG: bool = False, advection_tol=None,
+                 pressure_tol=None, pressure_tol_intermediate=None, pressure_warm_start: bool = False,
+                 pressure_use_amg: Optional[bool] = None, pressure_amg_options: Optional[dict] = None,
+                 flux_balance_tol: float = 1e-5, advect_non_ortho_steps: int = 1, pressure_non_ortho_steps: int = 1,
+                 normalize_pressure_result: bool = True, pressure_return_best_result: bool = False,
+                 pressure_time_step_normalized: bool = False, velocity_corrector: str = "FD",
+                 non_orthogonal: bool = True, differentiable: bool = False, hooks=None, prep_fn=None,
+                 time_step: Optional[float] = None, **unsupported: Any) -> None:
+        for k, v in unsupported.items():
+            if k in ("adaptive_CFL", "adaptive_CFL_per_env", "density_viscosity", "scipy_solve_advection",
+                     "scipy_solve_pressure", "preconditionBiCG", "BiCG_precondition_fallback", "convergence_tol",
+                     "solver_double_fallback", "advect_passive_scalar", "exclude_advection_solve_gradients",
+                     "exclude_pressure_solve_gradients", "output_resampling_shape", "output_resampling_fill_max_steps"):
+                if v not in (None, False, 0, 0.8, True) or k in ("scipy_solve_advection", "scipy_solve_pressure") and v:
+                    raise NotImplementedError(f"option {k}={v!r} is not implemented")
+                continue
+            raise TypeError(f"unexpected argument {k!r}")
+        if hooks is not None and prep_fn is not None:
+            raise ValueError("Pass hooks or its former name prep_fn, not both.")
+        if differentiable:
+            raise NotImplementedError("the differentiable (adjoint) PISO step is not implemented")
+        if substeps == "ADAPTIVE" or (isinstance(substeps, int) and substeps < 1):
+            raise NotImplementedError("adaptive CFL substeps are not implemented; use a fixed number of substeps")
+        self.domain = domain
+        self.hooks = Hooks.coerce(hooks if hooks is not None else prep_fn)
+        self.time_step = float(time_step if time_step is not None else dt)
+        self.substeps = int(substeps)
+        self.corrector_steps = int(corrector_steps)
+        self.advection_use_BiCG = advection_use_BiCG
+        self.pressure_use_BiCG = pressure_use_BiCG
+        self.advection_tol = advection_tol
+        self.pressure_tol = pressure_tol
+        self.pressure_tol_intermediate = pressure_tol_intermediate
+        self.pressure_warm_start = pressure_warm_start
+        self.pressure_use_amg = pressure_use_amg
+        self._pressure_amg_options = dict(pressure_amg_options or {})
+        self.flux_balance_tol = flux_balance_tol
+        self.advect_non_ortho_steps = advect_non_ortho_steps
+        self.pressure_non_ortho_steps = pressure_non_ortho_steps
+        self.normalize_pressure_result = normalize_pressure_result
+        self.pressure_return_best_result = pressure_return_best_result
+        self.pressure_time_step_normalized = pressure_time_step_normalized
+        self._velocity_corrector_versions = {"FD": 1, "FVM_CENTER": 5, "FVM_FACE": 6}
+        if velocity_corrector not in self._velocity_corrector_versions:
+            raise ValueError(f"unknown velocity corrector {velocity_corrector!r}")
+        self._velocity_corrector_version = self._velocity_corrector_versions[velocity_corrector]
+        self.non_orthogonal = non_orthogonal
+        self._non_ortho_flags = self._NON_ORTHO_MODE if non_orthogonal else 0
+        self.linear_solve_max_iterations = 5000
+        self.total_step = 0
+        self.total_time = 0.0
+        self._pressure_guess: dict = {}
+        self._pressure_amg_hierarchy = None
+        self._pressure_amg_version = None
+        self._pressure_amg_refreshes = 0
+        self._pressure_amg_baseline_iters: Optional[int] = None
+        self._pressure_amg_stale = False
+        self._pressure_rank_deficient_cache = None
+        self.solver_log: list = []
+        self.record_solver_stats = False
+
+    # ---- helpers
+    def _time_step_tensor(self) -> torch.Tensor:
+        return torch.tensor([self.time_step], dtype=self.domain.getDtype())
+
+    def _run_hook(self, hook: Hook, **kw) -> None:
+        self.hooks.run(hook, domain=self.domain, total_step=self.total_step, total_time=self.total_time, **kw)
+
+    def end_step(self, time_step) -> None:
+        self.total_step += 1
+        self.total_time += float(time_step.reshape(-1)[0]) if isinstance(time_step, torch.Tensor) else float(time_step)
+
+    def _pressure_tol_for(self, cstep: int, pstep: int, corrector_steps: int):
+        """Final corrector: pressure_tol, earlier ones: the intermediate tolerance (:2078-2105)."""
+        if self.pressure_tol_intermediate is None:
+            return self.pressure_tol
+        final = cstep == corrector_steps - 1 and pstep == self.pressure_non_ortho_steps - 1
+        return self.pressure_tol if final else self.pressure_tol_intermediate
+
+    def _pressure_initial_guess(self, cstep: int, reference: torch.Tensor):
+        if not self.pressure_warm_start:
+            return None
+        g = self._pressure_guess.get(cstep)
+        if g is None or g.shape != reference.shape or not bool(torch.isfinite(g).all()):
+            return None
+        return g.to(dtype=reference.dtype, device=reference.device).clone()
+
+    def _store_pressure_guess(self, cstep: int, result: torch.Tensor) -> None:
+        if self.pressure_warm_start:
+            self._pressure_guess[cstep] = result.detach().clone()
+
+    def _pressure_rank_deficient(self, P) -> bool:
+        """Pure-Neumann pressure operator: every row sum vanishes (:2165-2220)."""
+        if P is None:
+            return False
+        cached = self._pressure_rank_deficient_cache
+        if cached is not None and cached[0] is P.index and cached[2] == P.getRows():
+            return cached[1]
+        value = P.value.reshape(-1)[: P.getNnz()]
+        n = P.getRows()
+        counts = (P.row[1:] - P.row[:-1]).to(torch.int64)
+        seg = torch.repeat_interleave(torch.arange(n, device=value.device), counts)
+        row_sum = torch.zeros(n, dtype=value.dtype, device=value.device).index_add_(0, seg, value)
+        deficient = bool((row_sum.abs() > 1e-10 * value.abs().max()).sum().item() == 0)
+        self._pressure_rank_deficient_cache = (P.index, deficient, n)
+        return deficient
+
+    def _get_pressure_amg_hierarchy(self, P):
+        """Pressure AMG hierarchy, refreshed once per change of the advection diagonal (the
+        only input of P that changes between steps): the finest level every change, the
+        coarse Galerkin operators every ``coarse_refresh_interval`` changes or when the
+        iteration counts rose (:2222-2392)."""
+        if self.pressure_use_BiCG:
+            return None
+        if P.getBatchSize() > 1:
+            if self.pressure_use_amg:
+                raise NotImplementedError("batched pressure AMG is not implemented")
+            return None
+        use_amg = self.pressure_use_amg
+        if use_amg is None:
+            if P.getRows() < self.PRESSURE_AMG_AUTO_MIN_ROWS:
+                return None
+        elif not use_amg:
+            return None
+        version = self.domain._a_version
+        if self._pressure_amg_hierarchy is not None and self._pressure_amg_version == version:
+            return self._pressure_amg_hierarchy
+        options = dict(self._pressure_amg_options)
+        interval = int(options.pop("coarse_refresh_interval", 4))
+        refreshes = self._pressure_amg_refreshes
+        previous = self._pressure_amg_hierarchy
+        if previous is not None and interval > 1 and refreshes % interval != 0 and not self._pressure_amg_stale:
+            self._pressure_amg_hierarchy = amg_mod.galerkin_refresh(previous, P, coarse=False)
+        else:
+            self._pressure_amg_hierarchy = None
+            self._pressure_amg_hierarchy = amg_mod.hierarchy_for(P, project_constant=self._pressure_rank_deficient(P),
+                                                                 **options)
+            self._pressure_amg_refreshes = 0
+            self._pressure_amg_stale = False
+            self._pressure_amg_baseline_iters = None
+            refreshes = 0
+        self._pressure_amg_refreshes = refreshes + 1
+        self._pressure_amg_version = version
+        return self._pressure_amg_hierarchy
+
+    def _note_pressure_amg_iterations(self, infos) -> None:
+        iters = max(int(i.usedIterations) for i in infos)
+        base = self._pressure_amg_baseline_iters
+        if base is None or self._pressure_amg_refreshes <= 1:
+            self._pressure_amg_baseline_iters = iters if base is None else max(base, iters)
+        elif iters > 2 * base + 5:
+            self._pressure_amg_stale = True
+
+    def _record(self, tag: str, infos) -> None:
+        if self.record_solver_stats:
+            self.solver_log.append((self.total_step, tag, [(i.usedIterations, i.finalResidual, i.converged) for i in infos]))
+
+    # ---- linear solves (piso_simulation.py:2834-3199)
+    def linear_solve(self, csrMat, rhs: torch.Tensor, x: Optional[torch.Tensor] = None, use_BiCG: bool = False,
+                     tol=None, max_iter: Optional[int] = None, matrix_rank_deficient: bool = False,
+                     residual_reset_step: int = 0, return_best_result: bool = False, amg_hierarchy=None,
+                     tag: str = "unknown"):
+        if max_iter is None:
+            max_iter = self.linear_solve_max_iterations
+        batch = self.domain.getBatchSize() if self.domain is not None else 1
+        abs_tol = resolve_tolerance(tol, rhs, batch_size=batch)
+        if isinstance(abs_tol, list):
+            rows = rhs.numel() // csrMat.getRows()
+            abs_tol = [t for t in abs_tol for _ in range(rows // batch)]
+        if abs_tol is None:
+            abs_tol = default_tolerance(rhs.dtype)
+        if amg_hierarchy is not None:
+            if x is None:
+                x = torch.zeros_like(rhs)
+            infos = amg_mod.amg_pcg_solve(rhs, x, amg_hierarchy, tol=abs_tol, max_iter=max_iter,
+                                          return_best_result=return_best_result)
+            if amg_hierarchy is self._pressure_amg_hierarchy:
+                self._note_pressure_amg_iterations(infos)
+            self._record(tag, infos)
+            tv = max(abs_tol) if isinstance(abs_tol, list) else float(abs_tol)
+            check_solver_return_infos(infos, tv, max_iter, return_best_result, tag)
+            return x, True
+        if x is None:
+            x = torch.zeros_like(rhs)
+        if not bool(rhs.eq(0).all()):
+            infos = ops.SolveLinear(csrMat, rhs, x, maxIterations=max_iter, tolerance=abs_tol,
+                                    convergenceCriterion=ops.ConvergenceCriterion.NORM2_NORMALIZED, useBiCG=use_BiCG,
+                                    matrixRankDeficient=matrix_rank_deficient, residualResetSteps=residual_reset_step,
+                                    transposeA=False, returnBestResult=return_best_result)
+            self._record(tag, infos)
+            tv = max(abs_tol) if isinstance(abs_tol, list) else float(abs_tol)
+            check_solver_return_infos(infos, tv, max_iter, return_best_result, tag)
+        else:
+            x.zero_()
+        return x, True
+
+    # ---- the step
+    def single_step(self, static: bool = False) -> bool:
+        """One step: flux-balance check, then ``substeps`` PISO substeps (simulation.py:262-333)."""
+        balance = self.domain.GetBoundaryFluxBalance()
+        if float(balance.abs().max()) > self.flux_balance_tol:
+            raise RuntimeError(f"Domain boundary fluxes not balanced, cannot proceed with simulation step. "
+                               f"Flux balance: {balance}, flux_balance_tol: {self.flux_balance_tol}")
+        try:
+            return self._PISO_split_step(self.substeps)
+        except LinsolveError:
+            _LOG.exception("Simulation failed in step (total step %d):", self.total_step)
+            return False
+
+    def _PISO_split_step(self, iterations: int, time_step: Optional[torch.Tensor] = None) -> bool:
+        domain = self.domain
+        if domain is None or not domain.IsInitialized():
+            raise RuntimeError("the simulation needs a prepared domain (domain.PrepareSolve())")
+        if time_step is None:
+            time_step = self._time_step_tensor()
+        solve_ok = True
+        flags = self._non_ortho_flags
+        vcv = self._velocity_corrector_version
+        if vcv != 1:
+            raise NotImplementedError("only the FD velocity corrector is implemented")
+        for step in range(int(iterations)):
+            self._run_hook(Hook.PRE, local_step=step, time_step=time_step)
+            domain.UpdateDomainData()
+            # momentum prediction
+            self._run_hook(Hook.PRE_VELOCITY_SETUP, local_step=step, time_step=time_step)
+            ops.SetupAdvectionMatrix(domain, time_step, flags)
+            domain._a_version += 1
+            if not self.non_orthogonal:
+                ops.SetupAdvectionVelocity(domain, time_step, 0, False)
+                self._run_hook(Hook.POST_VELOCITY_SETUP, no_step=0, local_step=step, time_step=time_step)
+                vel, solve_ok = self.linear_solve(domain.C, domain.velocityRHS, x=domain.velocityResult,
+                                                  use_BiCG=self.advection_use_BiCG, tol=self.advection_tol,
+                                                  tag="velocity")
+                domain.setVelocityResult(vel)
+                domain.UpdateDomainData()
+            else:
+                ops.CopyVelocityResultFromBlocks(domain)
+                for no_step in range(self.advect_non_ortho_steps):
+                    ops.SetupAdvectionVelocity(domain, time_step, flags, False)
+                    self._run_hook(Hook.POST_VELOCITY_SETUP, no_step=no_step, local_step=step, time_step=time_step)
+                    x = None if no_step == 0 else domain.velocityResult
+                    vel, solve_ok = self.linear_solve(domain.C, domain.velocityRHS, x=x,
+                                                      use_BiCG=self.advection_use_BiCG, tol=self.advection_tol,
+                                                      tag="velocity")
+                    domain.setVelocityResult(vel)
+                    domain.UpdateDomainData()
+            if not solve_ok:
+                return solve_ok
+            self._run_hook(Hook.POST_PREDICTION, local_step=step, time_step=time_step)
+            # pressure correctors
+            for cstep in range(self.corrector_steps):
+                if not self.non_orthogonal:
+                    ops.SetupPressureCorrection(domain, time_step, 0, False, timeStepNorm=self.pressure_time_step_normalized)
+                    self._run_hook(Hook.POST_PRESSURE_SETUP, local_step=step, time_step=time_step)
+                    p, solve_ok = self.linear_solve(domain.P, domain.pressureRHSdiv,
+                                                    x=self._pressure_initial_guess(cstep, domain.pressureRHSdiv),
+                                                    use_BiCG=self.pressure_use_BiCG,
+                                                    tol=self._pressure_tol_for(cstep, 0, self.corrector_steps),
+                                                    return_best_result=self.pressure_return_best_result,
+                                                    amg_hierarchy=self._get_pressure_amg_hierarchy(domain.P),
+                                                    tag="pressure")
+                    if not solve_ok:
+                        return solve_ok
+                    if self.normalize_pressure_result:
+                        p = subtract_env_mean(p, domain.getBatchSize())
+                    self._store_pressure_guess(cstep, p)
+                    domain.setPressureResult(p)
+                    domain.UpdateDomainData()
+                    self._run_hook(Hook.POST_PRESSURE_RESULT, local_step=step, time_step=time_step)
+                else:
+                    ops.SetupPressureMatrix(domain, time_step, flags, False)
+                    for pstep in range(self.pressure_non_ortho_steps):
+                        if pstep == 0:
+                            ops.SetupPressureRHS(domain, time_step, flags, False,
+                                                 timeStepNorm=self.pressure_time_step_normalized)
+                        else:
+                            ops.SetupPressureRHSdiv(domain, time_step, flags, False,
+                                                    timeStepNorm=self.pressure_time_step_normalized)
+                        self._run_hook(Hook.POST_PRESSURE_SETUP, local_step=step, time_step=time_step)
+                        x = (domain.pressureResult.clone() if pstep > 0
+                             else self._pressure_initial_guess(cstep, domain.pressureRHSdiv))
+                        p, solve_ok = self.linear_solve(domain.P, domain.pressureRHSdiv, x=x,
+                                                        matrix_rank_deficient=False, residual_reset_step=100,
+                                                        use_BiCG=self.pressure_use_BiCG,
+                                                        tol=self._pressure_tol_for(cstep, pstep, self.corrector_steps),
+                                                        return_best_result=self.pressure_return_best_result,
+                                                        amg_hierarchy=self._get_pressure_amg_hierarchy(domain.P),
+                                                        tag="pressure")
+                        if self.normalize_pressure_result:
+                            p = subtract_env_mean(p, domain.getBatchSize())
+                        if pstep == self.pressure_non_ortho_steps - 1:
+                            self._store_pressure_guess(cstep, p)
+                        domain.setPressureResult(p)
+                        domain.UpdateDomainData()
+                        self._run_hook(Hook.POST_PRESSURE_RESULT, local_step=step, time_step=time_step)
+                        if not solve_ok:
+                            return solve_ok
+                    self._run_hook(Hook.POST_PRESSURE_NON_ORTHO, local_step=step, time_step=time_step)
+                ops.CopyPressureResultToBlocks(domain)
+                ops.CorrectVelocity(domain, time_step, version=vcv, timeStepNorm=self.pressure_time_step_normalized)
+                self._run_hook(Hook.POST_VELOCITY_CORRECTION, local_step=step, time_step=time_step)
+            ops.CopyVelocityResultToBlocks(domain)
+            self._run_hook(Hook.POST, local_step=step, time_step=time_step)
+            if not solve_ok:
+                break
+            self.end_step(time_step)
+        return solve_ok
+
+    def make_divergence_free(self, max_iter: int = 1000) -> bool:
+        """Project the current velocity onto a divergence-free field: one pressure solve and
+        correction with the current advection diagonal (piso_simulation.py:3369-3401)."""
+        domain = self.domain
+        ts = self._time_step_tensor()
+        ops.CopyVelocityResultFromBlocks(domain)
+        ops.SetupAdvectionMatrix(domain, ts, 0)
+        domain._a_version += 1
+        # project the block velocity: HbyA = u (pressureRHS = velocityResult), div, solve, correct
+        domain.setPressureRHS(domain.velocityResult.clone())
+        ops.SetupPressureMatrix(domain, ts, 0)
+        ops.SetupPressureRHSdiv(domain, ts, 0)
+        p, ok = self.linear_solve(domain.P, domain.pressureRHSdiv, use_BiCG=self.pressure_use_BiCG, tol=self.pressure_tol,
+                                  max_iter=max_iter, tag="pressure")
+        p = subtract_env_mean(p, domain.getBatchSize())
+        domain.setPressureResult(p)
+        ops.CopyPressureResultToBlocks(domain)
+        ops.CorrectVelocity(domain, ts, version=1)
+        ops.CopyVelocityResultToBlocks(domain)
+        return ok

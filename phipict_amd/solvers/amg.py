@@ -1,0 +1,304 @@
+"""Algebraic multigrid for the pressure and electric-potential Poisson systems.
+
+Mirrors src/phipict/solvers/amg.py: a classical Ruge-Stueben setup on the host
+(``build_amg_hierarchy``, amg.py:450-598; here the library's own C++ setup instead of
+pyamg), an interpolation cache per sparsity pattern (``hierarchy_for``, :952-1045), Galerkin
+refreshes of the coarse operators on the device (``galerkin_refresh``, :749-847) and the
+AMG-preconditioned CG (``amg_pcg_solve``, :1544-1627) with the V-cycle on the device in the
+hierarchy's native dtype (float32 by default, :956).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+from typing import Any, Optional, Sequence
+
+import numpy as np
+import torch
+
+from .. import _lib
+from ..domain import CSRmatrix
+from .. import ops
+
+JACOBI_OMEGA = 2.0 / 3.0
+MIN_ABS_DIAG = 1e-30
+
+
+def _host_csr(csr: CSRmatrix):
+    row = csr.row.detach().cpu().numpy().astype(np.int32)
+    col = csr.index.detach().cpu().numpy().astype(np.int32)
+    val = csr.value.detach().reshape(-1)[: csr.getNnz()].to(torch.float64).cpu().numpy()
+    return row, col, val
+
+
+@dataclass
+class Level:
+    """One level of a hierarchy. ``A`` holds the operator in the hierarchy dtype (for the
+    finest level: the system matrix itself); R and P are kept in double for refreshes and in
+    the native dtype for the V-cycle. The coarsest level has no R/P."""
+
+    n: int
+    A: CSRmatrix
+    R_row: Optional[torch.Tensor] = None
+    R_col: Optional[torch.Tensor] = None
+    R_val: Optional[torch.Tensor] = None  # float64
+    P_row: Optional[torch.Tensor] = None
+    P_col: Optional[torch.Tensor] = None
+    P_val: Optional[torch.Tensor] = None  # float64
+    native: dict = field(default_factory=dict)
+
+
+class AMGHierarchy:
+    """A V-cycle hierarchy: levels, the dense pseudo-inverse of the coarsest operator, the
+    smoothing sweeps and whether the constant mode is projected out (amg.py:260-448)."""
+
+    def __init__(self, levels: list, coarse_A: torch.Tensor, project_constant: bool = False, num_pre_smooth: int = 1,
+                 num_post_smooth: int = 1, native_dtype=torch.float32, setup_seconds: float = 0.0):
+        self.levels = levels
+        self.project_constant = bool(project_constant)
+        self.num_pre_smooth = num_pre_smooth
+        self.num_post_smooth = num_post_smooth
+        self.native_dtype = native_dtype
+        self.setup_seconds = setup_seconds
+        self.set_coarse(coarse_A)
+
+    def set_coarse(self, coarse_A: torch.Tensor) -> None:
+        """Dense pseudo-inverse of the coarsest operator (numpy, float64; amg.py:583-587)."""
+        dense = coarse_A.detach().to(torch.float64).cpu().numpy()
+        self.coarse_pinv64 = torch.from_numpy(np.linalg.pinv(dense)).to(coarse_A.device)
+        self._pinv_native = {}
+
+    @property
+    def dtype(self):
+        return self.levels[0].A.getDtype()
+
+    def coarse_pinv(self, dtype) -> torch.Tensor:
+        if dtype not in self._pinv_native:
+            self._pinv_native[dtype] = self.coarse_pinv64.to(dtype).contiguous()
+        return self._pinv_native[dtype]
+
+    def native_levels(self, dtype) -> list:
+        """Per level above the coarsest: (A row, col, val, dinv, R row, col, val, P row, col, val)
+        in ``dtype`` (the layout _C.AMGPCGSolve takes, bindings.cpp:764-776)."""
+        out = []
+        for lvl in self.levels[:-1]:
+            nat = lvl.native.get(dtype)
+            if nat is None:
+                A = lvl.A
+                aval = A.value if A.value.dtype == dtype else A.value.to(dtype)
+                dinv = torch.empty(A.getRows() * A.getBatchSize(), dtype=dtype, device=A.value.device)
+                d = A.desc()
+                _lib.call("pp_inverse_diagonal", C.byref(d), _lib.dtype_code(A.getDtype()), _lib.ptr(dinv),
+                          _lib.dtype_code(dtype), _lib.stream_ptr(A.value.device))
+                nat = (A.row, A.index, aval, dinv, lvl.R_row, lvl.R_col, lvl.R_val.to(dtype), lvl.P_row, lvl.P_col,
+                       lvl.P_val.to(dtype))
+                lvl.native[dtype] = nat
+            out.append(nat)
+        return out
+
+    def operator_complexity(self) -> float:
+        fine = self.levels[0].A.getNnz()
+        return sum(l.A.getNnz() for l in self.levels) / max(1, fine)
+
+    def describe(self) -> str:
+        rows = [f"level {i}: n={l.n}, nnz={l.A.getNnz()}" for i, l in enumerate(self.levels)]
+        return "\n".join(rows + [f"operator complexity {self.operator_complexity():.3f}"])
+
+
+class _SetupHandle:
+    def __init__(self, n, row, col, val, theta, max_levels, max_coarse):
+        h = C.c_void_p()
+        _lib.call("pp_amg_setup", int(n), row.ctypes.data, col.ctypes.data, val.ctypes.data, float(theta),
+                  int(max_levels), int(max_coarse), C.byref(h))
+        self.h = h
+
+    def levels(self):
+        L = _lib.LIB.pp_amg_num_levels(self.h)
+        out = []
+        for l in range(L):
+            n, na, npp, nr = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+            _lib.call("pp_amg_level_sizes", self.h, l, C.byref(n), C.byref(na), C.byref(npp), C.byref(nr))
+            ar = np.empty(n.value + 1, np.int32); ac = np.empty(max(na.value, 1), np.int32); av = np.empty(max(na.value, 1))
+            has_t = l < L - 1
+            if has_t:
+                pr = np.empty(n.value + 1, np.int32); pc = np.empty(max(npp.value, 1), np.int32); pv = np.empty(max(npp.value, 1))
+            args = [ar.ctypes.data, ac.ctypes.data, av.ctypes.data]
+            if has_t:
+                # R has n_coarse rows: size known from the next level
+                nn = C.c_int32()
+                _lib.call("pp_amg_level_sizes", self.h, l + 1, C.byref(nn), None, None, None)
+                rr = np.empty(nn.value + 1, np.int32); rc = np.empty(max(nr.value, 1), np.int32); rv = np.empty(max(nr.value, 1))
+                split = np.empty(n.value, np.int32)
+                args += [pr.ctypes.data, pc.ctypes.data, pv.ctypes.data, rr.ctypes.data, rc.ctypes.data, rv.ctypes.data,
+                         split.ctypes.data]
+            else:
+                args += [None] * 7
+            _lib.call("pp_amg_level_copy", self.h, l, *args)
+            lv = {"n": n.value, "A": (ar, ac[: na.value], av[: na.value])}
+            if has_t:
+                lv["P"] = (pr, pc[: npp.value], pv[: npp.value])
+                lv["R"] = (rr, rc[: nr.value], rv[: nr.value])
+                lv["split"] = split
+            out.append(lv)
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _lib.LIB.pp_amg_free(self.h)
+            self.h = None
+
+
+def rs_setup_host(row: np.ndarray, col: np.ndarray, val: np.ndarray, theta: float = 0.25, max_levels: int = 20,
+                  max_coarse: int = 64) -> list:
+    """Run the library's Ruge-Stueben setup on a host CSR matrix (float64); returns per level
+    dicts with numpy arrays (A, P, R, C/F splitting)."""
+    n = row.size - 1
+    return _SetupHandle(n, np.ascontiguousarray(row, np.int32), np.ascontiguousarray(col, np.int32),
+                        np.ascontiguousarray(val, np.float64), theta, max_levels, max_coarse).levels()
+
+
+def _dev(a: np.ndarray, device, dtype=None) -> torch.Tensor:
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    return t.to(device=device, dtype=dtype) if dtype is not None else t.to(device)
+
+
+def build_amg_hierarchy(csr_mat: CSRmatrix, *, method: str = "ruge_stuben", max_levels: int = 20, max_coarse: int = 64,
+                        strength: float = 0.25, num_pre_smooth: int = 1, num_post_smooth: int = 1,
+                        project_constant: bool = False, native_dtype=torch.float32) -> AMGHierarchy:
+    """Classical Ruge-Stueben hierarchy for ``csr_mat`` (first matrix of a batch) built on the
+    host and uploaded (amg.py:450-598)."""
+    if method != "ruge_stuben":
+        raise NotImplementedError("only the Ruge-Stueben coarsening is implemented")
+    t0 = time.perf_counter()
+    device, dtype = csr_mat.value.device, csr_mat.value.dtype
+    row, col, val = _host_csr(csr_mat)
+    host = rs_setup_host(row, col, val, strength, max_levels, max_coarse)
+    levels = []
+    for i, lv in enumerate(host):
+        if i == 0:
+            A = CSRmatrix(csr_mat.value.reshape(-1)[: csr_mat.getNnz()], csr_mat.index, csr_mat.row)
+        else:
+            ar, ac, av = lv["A"]
+            A = CSRmatrix(_dev(av, device, dtype), _dev(ac, device), _dev(ar, device))
+        level = Level(n=lv["n"], A=A)
+        if "P" in lv:
+            pr, pc, pv = lv["P"]
+            rr, rc, rv = lv["R"]
+            level.P_row, level.P_col, level.P_val = _dev(pr, device), _dev(pc, device), _dev(pv, device, torch.float64)
+            level.R_row, level.R_col, level.R_val = _dev(rr, device), _dev(rc, device), _dev(rv, device, torch.float64)
+        levels.append(level)
+    return AMGHierarchy(levels, _dense(levels[-1].A), project_constant, num_pre_smooth, num_post_smooth, native_dtype,
+                        time.perf_counter() - t0)
+
+
+def _dense(A: CSRmatrix) -> torch.Tensor:
+    n = A.getRows()
+    row = A.row.to(torch.int64)
+    counts = row[1:] - row[:-1]
+    rows = torch.repeat_interleave(torch.arange(n, device=row.device), counts)
+    D = torch.zeros(n, n, dtype=torch.float64, device=A.value.device)
+    D.index_put_((rows, A.index.to(torch.int64)), A.value.reshape(-1)[: A.getNnz()].to(torch.float64), accumulate=True)
+    return D
+
+
+def galerkin_refresh(hierarchy: AMGHierarchy, csr_mat: CSRmatrix, *, project_constant: Optional[bool] = None,
+                     coarse: bool = True) -> AMGHierarchy:
+    """Keep P and R, replace the finest operator by ``csr_mat`` and (with ``coarse``) recompute
+    the coarse operators as device Galerkin products (amg.py:749-847)."""
+    t0 = time.perf_counter()
+    fine = hierarchy.levels[0]
+    if csr_mat.getRows() != fine.n:
+        raise ValueError("operator does not match the finest AMG level")
+    A = CSRmatrix(csr_mat.value.reshape(-1)[: csr_mat.getNnz()], csr_mat.index, csr_mat.row)
+    levels = [Level(n=fine.n, A=A, R_row=fine.R_row, R_col=fine.R_col, R_val=fine.R_val, P_row=fine.P_row,
+                    P_col=fine.P_col, P_val=fine.P_val)]
+    if coarse:
+        for i in range(1, len(hierarchy.levels)):
+            prev = levels[-1]
+            old = hierarchy.levels[i]
+            cval = torch.empty(old.A.getNnz(), dtype=A.getDtype(), device=A.value.device)
+            d = prev.A.desc()
+            _lib.call("pp_galerkin_rap", _lib.ptr(prev.R_row), _lib.ptr(prev.R_col), _lib.ptr(prev.R_val), C.byref(d),
+                      _lib.dtype_code(prev.A.getDtype()), _lib.ptr(prev.P_row), _lib.ptr(prev.P_col), _lib.ptr(prev.P_val),
+                      _lib.ptr(old.A.row), _lib.ptr(old.A.index), _lib.ptr(cval), old.n, old.A.getNnz(),
+                      _lib.dtype_code(A.getDtype()), _lib.stream_ptr(A.value.device))
+            levels.append(Level(n=old.n, A=CSRmatrix(cval, old.A.index, old.A.row), R_row=old.R_row, R_col=old.R_col,
+                                R_val=old.R_val, P_row=old.P_row, P_col=old.P_col, P_val=old.P_val))
+        coarse_A = _dense(levels[-1].A)
+    else:
+        for old in hierarchy.levels[1:]:
+            levels.append(old)
+        coarse_A = None
+    h = AMGHierarchy.__new__(AMGHierarchy)
+    h.levels = levels
+    h.project_constant = hierarchy.project_constant if project_constant is None else bool(project_constant)
+    h.num_pre_smooth, h.num_post_smooth = hierarchy.num_pre_smooth, hierarchy.num_post_smooth
+    h.native_dtype = hierarchy.native_dtype
+    h.setup_seconds = time.perf_counter() - t0
+    if coarse_A is not None:
+        h.set_coarse(coarse_A)
+    else:
+        h.coarse_pinv64 = hierarchy.coarse_pinv64
+        h._pinv_native = hierarchy._pinv_native
+    return h
+
+
+# interpolation cache per sparsity pattern (amg.py:857-868, 952-1045)
+_INTERPOLATION_CACHE: dict = {}
+
+
+def _sparsity_key(csr: CSRmatrix) -> tuple:
+    return (str(csr.value.device), csr.getRows(), csr.getNnz(), csr.row.data_ptr(), csr.index.data_ptr())
+
+
+def clear_interpolation_cache() -> None:
+    _INTERPOLATION_CACHE.clear()
+
+
+def hierarchy_for(csr_mat: CSRmatrix, *, project_constant: bool = False, reuse_interpolation: bool = True,
+                  native_dtype=torch.float32, **build_kwargs) -> AMGHierarchy:
+    """Hierarchy for ``csr_mat``: a new host setup for the first operator of a pattern, a
+    device Galerkin refresh with the cached interpolation for every further one."""
+    if not reuse_interpolation:
+        return build_amg_hierarchy(csr_mat, project_constant=project_constant, native_dtype=native_dtype, **build_kwargs)
+    key = _sparsity_key(csr_mat)
+    base = _INTERPOLATION_CACHE.get(key)
+    if base is None or base.dtype != csr_mat.getDtype():
+        h = build_amg_hierarchy(csr_mat, project_constant=project_constant, native_dtype=native_dtype, **build_kwargs)
+        _INTERPOLATION_CACHE[key] = h
+        return h
+    h = galerkin_refresh(base, csr_mat, project_constant=project_constant)
+    h.native_dtype = native_dtype
+    return h
+
+
+@dataclass
+class AMGSolveInfo:
+    finalResidual: float
+    usedIterations: int
+    converged: bool
+    isFiniteResidual: bool
+
+
+def amg_pcg_solve(rhs: torch.Tensor, x: torch.Tensor, hierarchy: AMGHierarchy, *, tol, max_iter: int,
+                  return_best_result: bool = False) -> list:
+    """Solve ``A x = rhs`` in place with AMG-preconditioned CG; ``A`` is the finest level
+    (amg.py:1544-1627). Non-finite results are zeroed, as the reference's contract."""
+    A = hierarchy.levels[0].A
+    n = A.getRows()
+    if rhs.numel() % n:
+        raise ValueError(f"RHS size {rhs.numel()} is not a multiple of the matrix size {n}.")
+    pdtype = hierarchy.native_dtype or rhs.dtype
+    levels = hierarchy.native_levels(pdtype)
+    pinv = hierarchy.coarse_pinv(pdtype)
+    tols = [float(t) for t in tol] if isinstance(tol, (list, tuple)) else []
+    tol_scalar = max(tols) if tols else float(tol)
+    infos = ops.AMGPCGSolve(A.row, A.index, A.value.to(rhs.dtype) if A.value.dtype != rhs.dtype else A.value, levels,
+                            pinv, rhs.reshape(-1), x.view(-1), int(max_iter), tol_scalar, True, bool(return_best_result),
+                            hierarchy.project_constant, hierarchy.num_pre_smooth, hierarchy.num_post_smooth,
+                            JACOBI_OMEGA, tols)
+    for i, info in enumerate(infos):
+        if not info.isFiniteResidual:
+            x.view(len(infos), -1)[i].zero_()
+    return infos

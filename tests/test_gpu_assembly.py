@@ -1,0 +1,179 @@
+"""GPU parity of the assembly kernels (through the C ABI) with the oracle on the same seeded
+domains: CSR pattern (exact), advection matrix and diagonal, momentum RHS, pressure matrix,
+HbyA RHS and divergence, FD velocity correction, Epot matrix and RHS, face-based current
+density, result copies and grid metrics; FP64 and FP32; single and batched environments."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import cases
+from cases import build_domain, rel_l2, to_np
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+NU, DT = 0.013, 0.021
+TOL = {np.float64: 1e-12, np.float32: 2e-6}
+
+
+def _tdt(dtype):
+    return torch.float64 if dtype == np.float64 else torch.float32
+
+
+def close(a, b, dtype, what):
+    a = np.asarray(a, np.float64).reshape(-1)
+    b = np.asarray(b, np.float64).reshape(-1)
+    assert a.shape == b.shape, what
+    err = rel_l2(a, b)
+    assert err <= TOL[dtype], f"{what}: rel L2 {err:.3e}"
+    scale = max(np.abs(b).max(), 1e-30)
+    bad = np.abs(a - b) > (TOL[dtype] * 50) * scale + (TOL[dtype] * 10) * np.abs(b)
+    assert not bad.any(), f"{what}: {bad.sum()} entries differ, worst {np.abs(a - b).max():.3e} of scale {scale:.3e}"
+
+
+@pytest.fixture(params=[np.float64, np.float32], ids=["f64", "f32"])
+def dtype(request):
+    return request.param
+
+
+@pytest.mark.parametrize("name", list(cases.CASES))
+def test_assembly_matches_oracle(name, dtype):
+    print(f"[assembly] case={name} dtype={np.dtype(dtype).name}", flush=True)
+    import phipict_amd as pa
+
+    specs = cases.CASES[name]()
+    od = O.OracleDomain(specs, NU, DT, dtype)
+    dom = build_domain(specs, NU, DT, _tdt(dtype), epot=True)
+    ts = torch.tensor([DT], dtype=_tdt(dtype))
+    n, d = od.ncells, od.dims
+    # pattern
+    np.testing.assert_array_equal(to_np(dom.C.row), od.row)
+    pa.SetupAdvectionMatrix(dom, ts, 0)
+    ccol, cval, A = od.advection_matrix()
+    np.testing.assert_array_equal(to_np(dom.C.index), ccol)
+    close(to_np(dom.C.value), cval, dtype, "advection matrix")
+    close(to_np(dom.A), A, dtype, "advection diagonal")
+    for apg in (False, True):
+        pa.SetupAdvectionVelocity(dom, ts, 0, apg)
+        close(to_np(dom.velocityRHS), od.advection_rhs(apg), dtype, f"momentum rhs (pressure gradient {apg})")
+    pa.SetupPressureMatrix(dom, ts, 0)
+    pcol, pval = od.pressure_matrix(A)
+    np.testing.assert_array_equal(to_np(dom.P.index), pcol)
+    close(to_np(dom.P.value), pval, dtype, "pressure matrix")
+    rng = np.random.default_rng(11)
+    vres = rng.standard_normal(d * n).astype(dtype)
+    dom.setVelocityResult(torch.as_tensor(vres, device=dom.getDevice()))
+    pa.SetupPressureRHS(dom, ts, 0)
+    prhs = od.pressure_rhs(A, ccol, cval, vres)
+    close(to_np(dom.pressureRHS), prhs, dtype, "HbyA")
+    close(to_np(dom.pressureRHSdiv), od.pressure_div(prhs), dtype, "HbyA divergence")
+    for tsn in (False, True):
+        pa.SetupPressureRHSdiv(dom, ts, 0, False, timeStepNorm=tsn)
+        close(to_np(dom.pressureRHSdiv), od.pressure_div(prhs, tsn), dtype, f"divergence (time-step norm {tsn})")
+        pa.CorrectVelocity(dom, ts, version=1, timeStepNorm=tsn)
+        close(to_np(dom.velocityResult), od.correct_velocity(A, prhs, tsn), dtype, f"velocity correction ({tsn})")
+    ecol, evalv = od.epot_matrix()
+    np.testing.assert_array_equal(to_np(dom.Epot.index), ecol)
+    close(to_np(dom.Epot.value), evalv, dtype, "Epot matrix")
+    vf = rng.standard_normal(d * n).astype(dtype)
+    close(to_np(pa.ComputeEpotRHS(dom, torch.as_tensor(vf, device=dom.getDevice()))), od.epot_rhs(vf), dtype, "Epot rhs")
+    phi = rng.standard_normal(n).astype(dtype)
+    uxb = rng.standard_normal(3 * n).astype(dtype)
+    J = pa.ComputeCurrentDensityFaceBased(dom, torch.as_tensor(phi, device=dom.getDevice()),
+                                          torch.as_tensor(uxb, device=dom.getDevice()))
+    close(to_np(J), od.current_density(phi, uxb), dtype, "current density")
+
+
+def test_result_copies(dtype):
+    import phipict_amd as pa
+
+    specs = cases.rotated_pair3d()
+    od = O.OracleDomain(specs, NU, DT, dtype)
+    dom = build_domain(specs, NU, DT, _tdt(dtype), epot=True)
+    n, d = od.ncells, od.dims
+    flat_v = od.flat_from_blocks(od.velocity, d)
+    pa.CopyVelocityResultFromBlocks(dom)
+    np.testing.assert_array_equal(to_np(dom.velocityResult), flat_v)
+    flat_p = od.flat_from_blocks(od.pressure, 1)
+    pa.CopyPressureResultFromBlocks(dom)
+    np.testing.assert_array_equal(to_np(dom.pressureResult), flat_p)
+    rng = np.random.default_rng(3)
+    new_v = rng.standard_normal(d * n).astype(dtype)
+    dom.setVelocityResult(torch.as_tensor(new_v, device=dom.getDevice()))
+    pa.CopyVelocityResultToBlocks(dom)
+    got = np.concatenate([to_np(b.velocity).reshape(d, -1) for b in dom.getBlocks()], axis=1).reshape(-1)
+    np.testing.assert_array_equal(got, new_v)
+    new_e = rng.standard_normal(n).astype(dtype)
+    dom.setEpotResult(torch.as_tensor(new_e, device=dom.getDevice()))
+    pa.CopyEpotResultToBlocks(dom)
+    got = np.concatenate([to_np(b.epot).reshape(-1) for b in dom.getBlocks()])
+    np.testing.assert_array_equal(got, new_e)
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_metrics_match_oracle(d, dtype):
+    import phipict_amd as pa
+
+    axes = [np.linspace(0, 1, 7), cases.tanh_coords(5, 0, 2, 1.3), np.linspace(0, 0.5, 4)][:d]
+    v = cases.skewed_vertices(axes, 0.2, seed=d)
+    dev = torch.device("cuda", 0)
+    coords = torch.as_tensor(v, dtype=_tdt(dtype), device=dev).unsqueeze(0)
+    close(to_np(pa.CoordsToTransforms(coords)).reshape(-1), O.transforms(v, dtype).reshape(-1), dtype, "transforms")
+    close(to_np(pa.CoordsToFaceTransforms(coords)).reshape(-1), O.face_transforms(v, dtype).reshape(-1), dtype,
+          "face transforms")
+
+
+def test_batched_assembly_per_env(dtype):
+    """Two environments with different velocities, sources and time steps assemble what two
+    separate oracle domains assemble (a shared pattern, per-environment values)."""
+    import phipict_amd as pa
+
+    specs = cases.channel2d(12, 9, per_cell_source=True)
+    rng = np.random.default_rng(21)
+    specs2 = cases.channel2d(12, 9, per_cell_source=True, seed=5)
+    dts = [DT, 1.7 * DT]
+    dom = build_domain(specs, NU, DT, _tdt(dtype), batch=2)
+    blk = dom.getBlocks()[0]
+    v = blk.velocity.clone()
+    v[1] = torch.as_tensor(specs2[0].velocity, dtype=v.dtype)
+    blk.setVelocity(v)
+    src = torch.stack([torch.as_tensor(specs[0].vsource), torch.as_tensor(specs2[0].vsource)]).to(v)
+    blk.setVelocitySource(src)
+    dom.setTimeStep(torch.tensor(dts, dtype=_tdt(dtype)))
+    pa.SetupAdvectionMatrix(dom, None, 0)
+    pa.SetupAdvectionVelocity(dom, None, 0, False)
+    pa.SetupPressureMatrix(dom, None, 0)
+    n = dom.getTotalSize()
+    vres = rng.standard_normal(2 * 2 * n).astype(dtype)
+    dom.setVelocityResult(torch.as_tensor(vres, device=dom.getDevice()))
+    pa.SetupPressureRHS(dom, None, 0)
+    nnz = dom.C.getNnz()
+    for e, (sp_, dt_) in enumerate(zip([specs, specs2], dts)):
+        od = O.OracleDomain(sp_, NU, dt_, dtype)
+        ccol, cval, A = od.advection_matrix()
+        close(to_np(dom.C.value)[e * nnz:(e + 1) * nnz], cval, dtype, f"env {e} advection matrix")
+        close(to_np(dom.A)[e * n:(e + 1) * n], A, dtype, f"env {e} diagonal")
+        close(to_np(dom.velocityRHS)[e * 2 * n:(e + 1) * 2 * n], od.advection_rhs(), dtype, f"env {e} rhs")
+        pcol, pval = od.pressure_matrix(A)
+        close(to_np(dom.P.value)[e * nnz:(e + 1) * nnz], pval, dtype, f"env {e} pressure matrix")
+        vr = vres[e * 2 * n:(e + 1) * 2 * n]
+        prhs = od.pressure_rhs(A, ccol, cval, vr)
+        close(to_np(dom.pressureRHS)[e * 2 * n:(e + 1) * 2 * n], prhs, dtype, f"env {e} HbyA")
+        close(to_np(dom.pressureRHSdiv)[e * n:(e + 1) * n], od.pressure_div(prhs), dtype, f"env {e} divergence")
+
+
+def test_skewed_grid_rejects_non_orthogonal_flags():
+    import phipict_amd as pa
+
+    specs = cases.channel2d(8, 6)
+    axes = [np.linspace(0, 2, 9), cases.tanh_coords(6, -1, 1, 1.5)]
+    specs[0].vertices = cases.skewed_vertices(axes, 0.2, seed=1)
+    dom = build_domain(specs, NU, DT, torch.float64)
+    assert not dom.isOrthogonal()
+    with pytest.raises(NotImplementedError):
+        pa.SetupAdvectionMatrix(dom, torch.tensor([DT], dtype=torch.float64), 1)
+    pa.SetupAdvectionMatrix(dom, torch.tensor([DT], dtype=torch.float64), 0)
+    rect = build_domain(cases.channel2d(8, 6), NU, DT, torch.float64)
+    assert rect.isOrthogonal()

@@ -1,0 +1,175 @@
+"""Whole PISO (+ inductionless MHD) steps on the GPU against the oracle's step driver on the same
+seeded domains: velocity, pressure and potential after N steps agree within relative L2 1e-10
+(FP64) and 1e-5 (FP32) at the same Krylov tolerances; CG and AMG-PCG pressure/potential solves;
+multi-block domains; the steady Hartmann profile against its analytic solution."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import cases
+from cases import build_domain, rel_l2, to_np
+from oracle import oracle as O
+from oracle.piso_oracle import MHDSpec, OraclePISO
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL = {np.float64: 1e-10, np.float32: 1e-5}
+
+
+def _tdt(dtype):
+    return torch.float64 if dtype == np.float64 else torch.float32
+
+
+def run_pair(specs, dtype, steps, nu, dt, mhd=None, amg=False, tol=None, pot_amg=False):
+    """Run ``steps`` steps on the GPU and in the oracle; return both final states."""
+    import phipict_amd as pa
+
+    tol = tol if tol is not None else (1e-10 if dtype == np.float64 else 1e-5)
+    epot = mhd is not None
+    dom = build_domain(specs, nu, dt, _tdt(dtype))
+    base = [None if b.velocitySource is None else b.velocitySource.clone() for b in dom.getBlocks()]
+
+    def reset_source(domain, **kw):
+        for b, s in zip(domain.getBlocks(), base):
+            if s is not None:
+                b.setVelocitySource(s.clone())
+        domain.UpdateDomainData()
+
+    common = dict(domain=dom, dt=dt, non_orthogonal=False, advection_tol=tol, pressure_tol=tol,
+                  pressure_use_amg=amg, hooks={"PRE_VELOCITY_SETUP": reset_source})
+    if epot:
+        sim = pa.MHDSimulation(stuart_number=torch.tensor(mhd.stuart), e_b=torch.tensor(mhd.e_b, dtype=_tdt(dtype)),
+                               potential_tol=tol, potential_use_preconditioner=pot_amg, **common)
+    else:
+        sim = pa.Simulation(**common)
+    sim.record_solver_stats = True
+    od = O.OracleDomain(specs, nu, dt, dtype)
+    obase = [None if s is None else s.copy() for s in od.vsource]
+
+    def oreset(o):
+        for bi, s in enumerate(obase):
+            o.od.set_source(bi, None if s is None else s.copy())
+
+    osim = OraclePISO(od, dt, advection_tol=tol, pressure_tol=tol, pressure_use_amg=amg,
+                      mhd=None if mhd is None else MHDSpec(mhd.stuart, mhd.e_b, potential_tol=tol, use_amg=pot_amg),
+                      pre_velocity_setup=oreset)
+    for s in range(steps):
+        assert sim.single_step(), f"step {s} failed"
+        osim.step()
+    return dom, sim, od, osim
+
+
+def compare(dom, od, osim, dtype, epot):
+    d = od.dims
+    vel = np.concatenate([to_np(b.velocity).reshape(d, -1) for b in dom.getBlocks()], axis=1)
+    ovel = od.flat_from_blocks(od.velocity, d).reshape(d, -1)
+    errs = {"velocity": rel_l2(vel, ovel)}
+    p = np.concatenate([to_np(b.pressure).reshape(-1) for b in dom.getBlocks()])
+    errs["pressure"] = rel_l2(p, od.flat_from_blocks(od.pressure, 1))
+    if epot:
+        errs["potential"] = rel_l2(to_np(dom.epotResult), osim.epot_result)
+    print("rel L2:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= FIELD_TOL[dtype], f"{k}: rel L2 {v:.3e} > {FIELD_TOL[dtype]}"
+
+
+@pytest.fixture(params=[np.float64, np.float32], ids=["f64", "f32"])
+def dtype(request):
+    return request.param
+
+
+def test_hartmann_channel_mhd_cg(dtype):
+    print(f"[piso] hartmann 2D CG dtype={np.dtype(dtype).name}", flush=True)
+    specs = cases.channel2d(nx=24, ny=20, gradient=16.0, noise=1e-2)
+    dom, sim, od, osim = run_pair(specs, dtype, 5, 0.1, 5e-3, mhd=MHDSpec(16.0))
+    compare(dom, od, osim, dtype, True)
+
+
+def test_duct3d_mhd_amg(dtype):
+    print(f"[piso] duct 3D AMG dtype={np.dtype(dtype).name}", flush=True)
+    specs = cases.duct3d(nx=16, ny=12, nz=14, gradient=50.0, noise=1e-2)
+    dom, sim, od, osim = run_pair(specs, dtype, 3, 1.0 / 50, 5e-4, mhd=MHDSpec(50.0), amg=True, pot_amg=True)
+    compare(dom, od, osim, dtype, True)
+
+
+def test_inflow_outflow_plug(dtype):
+    print(f"[piso] inflow/outflow dtype={np.dtype(dtype).name}", flush=True)
+    specs = cases.inflow2d(nx=24, ny=12)
+    dom, sim, od, osim = run_pair(specs, dtype, 4, 0.01, 1e-2)
+    compare(dom, od, osim, dtype, False)
+
+
+def test_two_block_channel_mhd(dtype):
+    print(f"[piso] two-block channel dtype={np.dtype(dtype).name}", flush=True)
+    _, pair = cases.split_channel2d(20, 14, 8, gradient=4.0)
+    dom, sim, od, osim = run_pair(pair, dtype, 4, 0.05, 1e-2, mhd=MHDSpec(4.0))
+    compare(dom, od, osim, dtype, True)
+
+
+def test_rotated_connection_3d():
+    print("[piso] rotated 3D pair f64", flush=True)
+    specs = cases.rotated_pair3d()
+    dom, sim, od, osim = run_pair(specs, np.float64, 3, 0.02, 2e-2)
+    compare(dom, od, osim, np.float64, False)
+
+
+def test_batched_environments_match_single():
+    """Two environments in one batched domain step exactly like two separate domains."""
+    import phipict_amd as pa
+
+    specs_a = cases.channel2d(16, 12, gradient=3.0, seed=1, noise=1e-2)
+    specs_b = cases.channel2d(16, 12, gradient=3.0, seed=2, noise=1e-2)
+    dt = 1e-2
+    runs = []
+    for specs in (specs_a, specs_b):
+        dom = build_domain(specs, 0.05, dt, torch.float64)
+        sim = pa.Simulation(domain=dom, dt=dt, non_orthogonal=False, advection_tol=1e-11, pressure_tol=1e-11)
+        for _ in range(3):
+            assert sim.single_step()
+        runs.append(to_np(dom.getBlocks()[0].velocity)[0])
+    dom = build_domain(specs_a, 0.05, dt, torch.float64, batch=2)
+    blk = dom.getBlocks()[0]
+    v = blk.velocity.clone()
+    v[1] = torch.as_tensor(specs_b[0].velocity, dtype=torch.float64)
+    blk.setVelocity(v)
+    sim = pa.Simulation(domain=dom, dt=dt, non_orthogonal=False, advection_tol=1e-11, pressure_tol=1e-11)
+    for _ in range(3):
+        assert sim.single_step()
+    got = to_np(blk.velocity)
+    for e in range(2):
+        assert rel_l2(got[e], runs[e]) < 1e-10
+
+
+def test_hartmann_profile_error():
+    """Steady Hartmann flow: the computed profile against u(y) = G/N (1 - cosh(Ha y)/cosh Ha)
+    (examples/hartmann.py:85); the maximum relative error is printed and bounded."""
+    import phipict_amd as pa
+
+    Ha, Re = 10.0, 1.0
+    N = Ha ** 2 / Re
+    ny = 48
+    specs = cases.channel2d(nx=4, ny=ny, stretch=1.5, noise=0.0, gradient=N)
+    dom = build_domain(specs, 1.0 / Re, 0.02, torch.float64)
+    src = dom.getBlocks()[0].velocitySource.clone()
+
+    def forcing(domain, **kw):
+        domain.getBlocks()[0].setVelocitySource(src.clone())
+        domain.UpdateDomainData()
+
+    sim = pa.MHDSimulation(domain=dom, dt=0.02, stuart_number=torch.tensor(N),
+                           e_b=torch.tensor([0.0, 1.0, 0.0], dtype=torch.float64), non_orthogonal=False,
+                           hooks={"PRE_VELOCITY_SETUP": forcing}, advection_tol=1e-12, pressure_tol=1e-12,
+                           potential_tol=1e-12)
+    for _ in range(100):
+        assert sim.single_step()
+    v = specs[0].vertices
+    y = 0.5 * (v[1, 1:, 0] + v[1, :-1, 0])
+    u = to_np(dom.getBlocks()[0].velocity)[0, 0].mean(axis=-1)
+    exact = (1 - np.cosh(Ha * y) / math.cosh(Ha))
+    err = float(np.abs(u - exact).max() / np.abs(exact).max())
+    print(f"Hartmann Ha={Ha} ny={ny}: max relative error {err:.3e}", flush=True)
+    assert err < 2e-2

@@ -1,0 +1,345 @@
+"""GPU parity of the fused Krylov solvers and the AMG path (through the C ABI) with the oracle:
+CG and BiCGStab on both engines (on-chip for small systems, kernel-per-phase for large ones),
+several right-hand sides per matrix, batched matrices with per-RHS tolerances, the result
+contract (iterations, converged start, zero RHS, non-finite input, return-best), singular
+systems, the AMG hierarchy upload, device Galerkin refresh and AMG-PCG; plus the helper
+kernels (SpMV, transpose, sparse outer product)."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import cases
+from cases import rel_l2, to_np
+from oracle import oracle as O
+from oracle.piso_oracle import OracleHierarchyState
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda", 0)
+SMALL_N = 32768  # systems up to this size run on the on-chip engine
+
+
+def _tdt(dtype):
+    return torch.float64 if dtype == np.float64 else torch.float32
+
+
+def poisson(nx, ny, shift=0.0, seed=0):
+    """Pressure operator of a wall-bounded channel (negative semidefinite), optionally shifted."""
+    od = O.OracleDomain(cases.channel2d(nx, ny, seed=seed), 0.02, 0.01)
+    _, _, A = od.advection_matrix()
+    col, val = od.pressure_matrix(A)
+    val = val.copy()
+    if shift:
+        diag = np.flatnonzero(col == np.repeat(np.arange(od.ncells), np.diff(od.row)))
+        val[diag] -= shift
+    return od.row, col, val
+
+
+def advection(nx, ny, u0=2.0, seed=0):
+    od = O.OracleDomain(cases.channel2d(nx, ny, u0=u0, seed=seed), 0.02, 0.05)
+    col, val, _ = od.advection_matrix()
+    return od.row, col, val
+
+
+def csr(row, col, val, dtype):
+    import phipict_amd as pa
+
+    return pa.CSRmatrix(torch.as_tensor(np.ascontiguousarray(val), dtype=_tdt(dtype), device=DEV),
+                        torch.as_tensor(col, device=DEV), torch.as_tensor(row, device=DEV))
+
+
+def check_info(info, ref, dtype, what, it_slack=0):
+    assert info.converged == bool(ref.converged), what
+    assert info.isFiniteResidual == bool(ref.is_finite), what
+    assert abs(info.usedIterations - ref.used_iterations) <= it_slack, \
+        f"{what}: iterations {info.usedIterations} vs oracle {ref.used_iterations}"
+    if ref.is_finite and ref.final_residual > 0:
+        assert abs(info.finalResidual - ref.final_residual) <= 1e-6 * ref.final_residual + \
+            (1e-12 if dtype == np.float64 else 1e-5) * abs(ref.final_residual) + (0 if it_slack == 0 else ref.final_residual)
+
+
+SOLVE_TOL = {np.float64: 1e-10, np.float32: 2e-5}
+
+
+@pytest.fixture(params=[np.float64, np.float32], ids=["f64", "f32"])
+def dtype(request):
+    return request.param
+
+
+@pytest.mark.parametrize("size", ["small", "large"])
+def test_cg_matches_oracle(size, dtype):
+    nx, ny = (40, 30) if size == "small" else (260, 160)
+    assert (nx * ny <= SMALL_N) == (size == "small")
+    print(f"[cg] size={size} n={nx * ny} dtype={np.dtype(dtype).name}", flush=True)
+    import phipict_amd as pa
+
+    row, col, val = poisson(nx, ny, shift=0.05)
+    n = row.size - 1
+    rng = np.random.default_rng(1)
+    k = 3
+    b = rng.standard_normal(k * n).astype(dtype)
+    tol = 1e-9 if dtype == np.float64 else 1e-4
+    x = torch.zeros(k * n, dtype=_tdt(dtype), device=DEV)
+    infos = pa.SolveLinear(csr(row, col, val, dtype), torch.as_tensor(b, device=DEV), x, maxIterations=5000,
+                           tolerance=tol)
+    xs = to_np(x).reshape(k, n)
+    slack = 0 if dtype == np.float64 else 3
+    for r in range(k):
+        xo = np.zeros(n, dtype)
+        ref = O.cg(row, col, val.astype(dtype), b[r * n:(r + 1) * n].copy(), xo, tol)
+        check_info(infos[r], ref, dtype, f"rhs {r}", slack)
+        assert rel_l2(xs[r], xo) < SOLVE_TOL[dtype] * (1 if slack == 0 else 100), rel_l2(xs[r], xo)
+
+
+@pytest.mark.parametrize("size", ["small", "large"])
+def test_bicgstab_matches_oracle(size, dtype):
+    nx, ny = (40, 30) if size == "small" else (260, 160)
+    print(f"[bicg] size={size} n={nx * ny} dtype={np.dtype(dtype).name}", flush=True)
+    import phipict_amd as pa
+
+    row, col, val = advection(nx, ny)
+    n = row.size - 1
+    rng = np.random.default_rng(2)
+    k = 2
+    b = rng.standard_normal(k * n).astype(dtype)
+    tol = 1e-10 if dtype == np.float64 else 1e-5
+    x0 = (0.1 * rng.standard_normal(k * n)).astype(dtype)
+    x = torch.as_tensor(x0.copy(), device=DEV)
+    infos = pa.SolveLinear(csr(row, col, val, dtype), torch.as_tensor(b, device=DEV), x, maxIterations=5000,
+                           tolerance=tol, useBiCG=True)
+    xs = to_np(x).reshape(k, n)
+    slack = 0 if dtype == np.float64 else 2
+    for r in range(k):
+        xo = x0[r * n:(r + 1) * n].copy()
+        ref = O.bicgstab(row, col, val.astype(dtype), b[r * n:(r + 1) * n].copy(), xo, tol)
+        check_info(infos[r], ref, dtype, f"rhs {r}", slack)
+        assert rel_l2(xs[r], xo) < SOLVE_TOL[dtype] * (1 if slack == 0 else 100)
+
+
+@pytest.mark.parametrize("use_bicg", [False, True])
+def test_batched_matrices_per_rhs_tolerance(use_bicg, dtype):
+    """value holds two matrices; 2 RHS per matrix; one tolerance per RHS."""
+    import phipict_amd as pa
+
+    mats = [poisson(24, 20, shift=0.1, seed=s) if not use_bicg else advection(24, 20, seed=s) for s in (0, 1)]
+    row, col = mats[0][0], mats[0][1]
+    n = row.size - 1
+    vals = np.concatenate([m[2] for m in mats])
+    A = pa.CSRmatrix(torch.as_tensor(vals, dtype=_tdt(dtype), device=DEV), torch.as_tensor(col, device=DEV),
+                     torch.as_tensor(row, device=DEV))
+    assert A.getBatchSize() == 2
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(4 * n).astype(dtype)
+    base = 1e-9 if dtype == np.float64 else 1e-4
+    tols = [base, 10 * base, 100 * base, base]
+    x = torch.zeros(4 * n, dtype=_tdt(dtype), device=DEV)
+    infos = pa.SolveLinear(A, torch.as_tensor(b, device=DEV), x, maxIterations=3000,
+                           tolerance=torch.tensor(tols, dtype=torch.float64), useBiCG=use_bicg)
+    xs = to_np(x).reshape(4, n)
+    slack = 0 if dtype == np.float64 else 3
+    for r in range(4):
+        m = mats[r // 2]
+        xo = np.zeros(n, dtype)
+        fn = O.bicgstab if use_bicg else O.cg
+        ref = fn(row, col, m[2].astype(dtype), b[r * n:(r + 1) * n].copy(), xo, tols[r])
+        check_info(infos[r], ref, dtype, f"rhs {r}", slack)
+        assert rel_l2(xs[r], xo) < (1e-9 if slack == 0 else 1e-3)
+
+
+@pytest.mark.parametrize("size", ["small", "large"])
+def test_result_contract_edge_cases(size):
+    import phipict_amd as pa
+
+    nx, ny = (20, 16) if size == "small" else (200, 180)
+    row, col, val = poisson(nx, ny, shift=0.2)
+    n = row.size - 1
+    A = csr(row, col, val, np.float64)
+    rng = np.random.default_rng(4)
+    b = rng.standard_normal(n)
+    # zero RHS with zero start: converged at once, zero iterations
+    x = torch.zeros(n, dtype=torch.float64, device=DEV)
+    info = pa.SolveLinear(A, torch.zeros(n, dtype=torch.float64, device=DEV), x, tolerance=1e-10)[0]
+    assert info.converged and info.usedIterations == 0 and float(x.abs().max()) == 0
+    # exact start
+    xo = np.zeros(n)
+    O.cg(row, col, val, b.copy(), xo, 1e-13)
+    x = torch.as_tensor(xo.copy(), device=DEV)
+    for bicg in (False, True):
+        info = pa.SolveLinear(A, torch.as_tensor(b, device=DEV), x.clone(), tolerance=1e-8, useBiCG=bicg)[0]
+        assert info.converged
+        assert info.usedIterations == (-1 if bicg else 0)
+    # a non-finite right-hand side: not converged, non-finite residual flagged, other RHS unaffected
+    bb = np.concatenate([b, b])
+    bb[5] = np.nan
+    x = torch.zeros(2 * n, dtype=torch.float64, device=DEV)
+    infos = pa.SolveLinear(A, torch.as_tensor(bb, device=DEV), x, tolerance=1e-8)
+    assert not infos[0].converged and not infos[0].isFiniteResidual
+    assert infos[1].converged and infos[1].isFiniteResidual
+    # return-best with too few iterations reports the residual of the returned iterate
+    x = torch.zeros(n, dtype=torch.float64, device=DEV)
+    xo = np.zeros(n)
+    ref = O.cg(row, col, val, b.copy(), xo, 1e-14, maxit=9, return_best=True)
+    info = pa.SolveLinear(A, torch.as_tensor(b, device=DEV), x, maxIterations=9, tolerance=1e-14,
+                          returnBestResult=True)[0]
+    assert not info.converged and not ref.converged
+    M = O.spmv(row, col, val, to_np(x))
+    true_res = np.linalg.norm(b - M) / math.sqrt(n)
+    assert abs(info.finalResidual - true_res) < 1e-9 * true_res
+    assert abs(info.finalResidual - ref.final_residual) < 1e-9 * ref.final_residual
+    assert rel_l2(to_np(x), xo) < 1e-10
+
+
+@pytest.mark.parametrize("size", ["small", "large"])
+def test_singular_cg_projected(size, dtype):
+    import phipict_amd as pa
+
+    nx, ny = (30, 20) if size == "small" else (220, 170)
+    row, col, val = poisson(nx, ny)
+    n = row.size - 1
+    b = np.random.default_rng(5).standard_normal(n)
+    b = (b - b.mean()).astype(dtype)
+    tol = 1e-9 if dtype == np.float64 else 1e-4
+    x = torch.zeros(n, dtype=_tdt(dtype), device=DEV)
+    info = pa.SolveLinear(csr(row, col, val, dtype), torch.as_tensor(b, device=DEV), x, maxIterations=5000,
+                          tolerance=tol, matrixRankDeficient=True, returnBestResult=True)[0]
+    xo = np.zeros(n, dtype)
+    ref = O.cg(row, col, val.astype(dtype), b.copy(), xo, tol, maxit=5000, return_best=True, project=True)
+    check_info(info, ref, dtype, "singular", 0 if dtype == np.float64 else 5)
+    xg = to_np(x)
+    assert rel_l2(xg - xg.mean(), xo - xo.mean()) < (1e-9 if dtype == np.float64 else 1e-2)
+
+
+def test_residual_reset_matches_oracle():
+    import phipict_amd as pa
+
+    row, col, val = poisson(30, 24, shift=0.01)
+    n = row.size - 1
+    b = np.random.default_rng(6).standard_normal(n)
+    x = torch.zeros(n, dtype=torch.float64, device=DEV)
+    info = pa.SolveLinear(csr(row, col, val, np.float64), torch.as_tensor(b, device=DEV), x, tolerance=1e-11,
+                          residualResetSteps=7)[0]
+    xo = np.zeros(n)
+    ref = O.cg(row, col, val, b.copy(), xo, 1e-11, reset_every=7)
+    check_info(info, ref, np.float64, "reset")
+    assert rel_l2(to_np(x), xo) < 1e-10
+
+
+def test_transposed_solve():
+    import phipict_amd as pa
+
+    row, col, val = advection(20, 14)
+    n = row.size - 1
+    b = np.random.default_rng(7).standard_normal(n)
+    x = torch.zeros(n, dtype=torch.float64, device=DEV)
+    info = pa.SolveLinear(csr(row, col, val, np.float64), torch.as_tensor(b, device=DEV), x, tolerance=1e-11,
+                          useBiCG=True, transposeA=True)[0]
+    tr, tc, tv = O.csr_transpose(row, col, val)
+    xo = np.zeros(n)
+    ref = O.bicgstab(tr, tc, tv, b.copy(), xo, 1e-11)
+    check_info(info, ref, np.float64, "transposed")
+    assert rel_l2(to_np(x), xo) < 1e-9
+
+
+def test_helpers_spmv_transpose_outer(dtype):
+    import phipict_amd as pa
+    from phipict_amd import _lib
+    import ctypes as C
+
+    row, col, val = advection(17, 13)
+    n = row.size - 1
+    A = csr(row, col, val, dtype)
+    xv = np.random.default_rng(8).standard_normal(2 * n).astype(dtype)
+    y = pa.ops.spmv(A, torch.as_tensor(xv, device=DEV))
+    yo = np.concatenate([O.spmv(row, col, val.astype(dtype), xv[:n].copy()),
+                         O.spmv(row, col, val.astype(dtype), xv[n:].copy())])
+    assert rel_l2(to_np(y), yo) < (1e-14 if dtype == np.float64 else 1e-6)
+    tr = torch.empty(n + 1, dtype=torch.int32, device=DEV)
+    tc = torch.empty(col.size, dtype=torch.int32, device=DEV)
+    tv = torch.empty(col.size, dtype=_tdt(dtype), device=DEV)
+    d = A.desc()
+    _lib.call("pp_csr_transpose", C.byref(d), _lib.dtype_code(_tdt(dtype)), _lib.ptr(tr), _lib.ptr(tc), _lib.ptr(tv),
+              _lib.stream_ptr(DEV))
+    orow, ocol, oval = O.csr_transpose(row, col, val.astype(dtype))
+    np.testing.assert_array_equal(to_np(tr), orow)
+    np.testing.assert_array_equal(to_np(tc), ocol)
+    np.testing.assert_array_equal(to_np(tv), oval)
+    a = np.random.default_rng(9).standard_normal(n).astype(dtype)
+    bvec = np.random.default_rng(10).standard_normal(n).astype(dtype)
+    out = A.WithZeroValue()
+    pa.SparseOuterProduct(torch.as_tensor(a, device=DEV), torch.as_tensor(bvec, device=DEV), out)
+    rows = np.repeat(np.arange(n), np.diff(row))
+    np.testing.assert_allclose(to_np(out.value), a[rows] * bvec[col], rtol=1e-6 if dtype == np.float32 else 1e-15)
+
+
+# ----------------------------------------------------------------------------- AMG
+def _hier_pair(row, col, val, dtype, project, native):
+    import phipict_amd as pa
+    from phipict_amd.solvers import amg
+
+    A = csr(row, col, val, dtype)
+    h = amg.build_amg_hierarchy(A, project_constant=project, native_dtype=_tdt(native))
+    hs = OracleHierarchyState(row, col, val.astype(dtype), project, hdtype=native)
+    return A, h, hs
+
+
+@pytest.mark.parametrize("combo", ["f64/f32", "f64/f64", "f32/f32"])
+@pytest.mark.parametrize("project", [False, True])
+def test_amg_pcg_matches_oracle(combo, project):
+    print(f"[amg] combo={combo} project={project}", flush=True)
+    from phipict_amd.solvers import amg
+
+    dt_s, dt_h = {"f64/f32": (np.float64, np.float32), "f64/f64": (np.float64, np.float64),
+                  "f32/f32": (np.float32, np.float32)}[combo]
+    row, col, val = poisson(96, 80, shift=0.0 if project else 1e-3)
+    n = row.size - 1
+    A, h, hs = _hier_pair(row, col, val, dt_s, project, dt_h)
+    assert len(h.levels) == len(hs.h.levels)
+    b = np.random.default_rng(11).standard_normal(2 * n)
+    if project:
+        b[:n] -= b[:n].mean()
+        b[n:] -= b[n:].mean()
+    b = b.astype(dt_s)
+    tol = 1e-9 if dt_s == np.float64 else 1e-4
+    x = torch.zeros(2 * n, dtype=_tdt(dt_s), device=DEV)
+    infos = amg.amg_pcg_solve(torch.as_tensor(b, device=DEV), x, h, tol=tol, max_iter=500)
+    xs = to_np(x).reshape(2, n)
+    exact_h = dt_h == np.float64
+    for r in range(2):
+        xo = np.zeros(n, dt_s)
+        ref = O.amg_pcg((row, col, val.astype(dt_s)), hs.native, hs.pinv, b[r * n:(r + 1) * n].copy(), xo, tol,
+                        maxit=500, project=project, hdtype=dt_h)
+        assert ref.converged
+        check_info(infos[r], ref, dt_s, f"rhs {r}", 0 if exact_h else 2)
+        a, o = xs[r], xo
+        if project:
+            a, o = a - a.mean(), o - o.mean()
+        assert rel_l2(a, o) < (1e-9 if exact_h else (1e-4 if dt_s == np.float64 else 1e-3)), rel_l2(a, o)
+
+
+def test_galerkin_refresh_matches_oracle():
+    """Device R A P of a new fine operator on a cached interpolation equals the oracle's
+    double-precision Galerkin product; the finest-only refresh keeps the coarse levels."""
+    from phipict_amd.solvers import amg
+
+    row, col, val = poisson(64, 48, shift=0.0)
+    _, _, val2 = poisson(64, 48, shift=0.0, seed=9)
+    val2 = val2 * 1.3
+    A, h, hs = _hier_pair(row, col, val, np.float64, True, np.float32)
+    A2 = csr(row, col, val2, np.float64)
+    h2 = amg.galerkin_refresh(h, A2)
+    lv = hs.h.levels
+    prev = (row, col, val2)
+    for l in range(1, len(lv)):
+        ref = O.galerkin(prev, lv[l - 1]["P"], lv[l - 1]["R"], lv[l]["n"])
+        got = h2.levels[l].A
+        np.testing.assert_array_equal(to_np(got.row), ref[0])
+        np.testing.assert_array_equal(to_np(got.index), ref[1])
+        np.testing.assert_allclose(to_np(got.value), ref[2], rtol=1e-12, atol=1e-14 * np.abs(ref[2]).max())
+        prev = ref
+    h3 = amg.galerkin_refresh(h, A2, coarse=False)
+    assert h3.levels[1] is h.levels[1]
+    assert h3.coarse_pinv64 is h.coarse_pinv64

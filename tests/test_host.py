@@ -1,0 +1,231 @@
+"""Host-side checks of the product (CPU only, no kernel launches): the C-ABI library loads and
+exports every entry point the header declares, the ctypes descriptors match the C layout, the
+host Ruge-Stueben setup equals the oracle's, and the host logic mirrors the reference
+(connection axes, grid weights, tolerances, hooks, option validation)."""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+import cases
+from oracle import oracle as O
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "phipict_hip.h"
+
+
+def declared_entry_points() -> list:
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pp_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from phipict_amd import _lib
+
+    names = declared_entry_points()
+    assert len(names) >= 30
+    for name in names:
+        assert hasattr(_lib.LIB, name), name
+    # every declared entry point has a ctypes signature (pp_last_error is typed separately)
+    assert set(names) - {"pp_last_error"} == set(_lib.SIGNATURES)
+
+
+def test_version_and_error_reporting_without_gpu():
+    from phipict_amd import _lib
+
+    assert _lib.LIB.pp_version() > 0
+    # argument checks fail before any device call
+    rc = _lib.LIB.pp_cast(None, 8, None, 4, 10, None)
+    assert rc != 0
+    assert b"null" in _lib.LIB.pp_last_error()
+
+
+def test_descriptor_layout_matches_header(tmp_path):
+    from phipict_amd import _lib
+
+    structs = {"pp_boundary_desc": _lib.BoundaryDesc, "pp_block_desc": _lib.BlockDesc, "pp_csr": _lib.CsrDesc,
+               "pp_domain": _lib.DomainDesc, "pp_solve_result": _lib.SolveResult, "pp_amg_level": _lib.AmgLevel}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, ct in structs.items():
+        lines.append(f'printf("{cname} __sizeof__ %zu\\n", sizeof({cname}));')
+        for fname, _ in ct._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            s, f, v = line.split()
+            got[(s, f)] = int(v)
+    for cname, ct in structs.items():
+        assert got[(cname, "__sizeof__")] == C.sizeof(ct), cname
+        for fname, _ in ct._fields_:
+            assert got[(cname, fname)] == getattr(ct, fname).offset, (cname, fname)
+
+
+@pytest.mark.parametrize("which", ["pressure2d", "epot3d", "advection2d"])
+def test_host_rs_setup_equals_oracle(which):
+    """The library's C++ Ruge-Stueben setup makes the same splittings and the same P, R and
+    coarse operators as the oracle's restatement."""
+    from phipict_amd.solvers.amg import rs_setup_host
+
+    if which == "pressure2d":
+        od = O.OracleDomain(cases.channel2d(40, 30), 0.02, 0.01)
+        _, _, A = od.advection_matrix()
+        col, val = od.pressure_matrix(A)
+    elif which == "epot3d":
+        od = O.OracleDomain(cases.duct3d(14, 10, 12), 0.01, 0.02)
+        col, val = od.epot_matrix()
+    else:
+        od = O.OracleDomain(cases.inflow2d(30, 20), 0.02, 0.05)
+        col, val, _ = od.advection_matrix()
+    row = od.row
+    mine = rs_setup_host(row, col, val)
+    ref = O.OracleHierarchy(row, col, val).levels
+    assert len(mine) == len(ref)
+    for a, b in zip(mine, ref):
+        assert a["n"] == b["n"]
+        for key in ("A", "P", "R"):
+            if key not in b:
+                continue
+            for x, y in zip(a[key][:2], b[key][:2]):
+                np.testing.assert_array_equal(x, y)
+            np.testing.assert_allclose(a[key][2], b[key][2], rtol=1e-13, atol=1e-300)
+        if "split" in b:
+            np.testing.assert_array_equal(a["split"], b["split"])
+
+
+def _cpu_domain(dims):
+    import phipict_amd as pa
+
+    return pa.Domain(dims, torch.tensor([0.01], dtype=torch.float64), dtype=torch.float64, device="cpu")
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_connection_axes_match_restatement(dims):
+    """Block.ConnectBlock stores the same face-format axes as the oracle's restatement of
+    ConnectBlocks (boundaries.cpp:857-891) for every face pair and valid axis choice."""
+    tested = 0
+    for f1 in range(2 * dims):
+        for f2 in range(2 * dims):
+            d2 = f2 >> 1
+            tang = [((d2 + k) % dims) for k in range(1, dims)]
+            choices = [(-1, -1)]
+            if dims == 2:
+                choices += [((tang[0] << 1) | s, -1) for s in (0, 1)]
+            else:
+                for s1 in (0, 1):
+                    for s2 in (0, 1):
+                        choices.append(((tang[0] << 1) | s1, (tang[1] << 1) | s2))
+                        choices.append(((tang[1] << 1) | s1, (tang[0] << 1) | s2))
+            for c1, c2 in choices:
+                dom = _cpu_domain(dims)
+                n = [4] * dims
+                b0 = dom.CreateBlockWithSize(n)
+                b1 = dom.CreateBlockWithSize(n)
+                b0.ConnectBlock(f1, b1, f2, "" if c1 < 0 else c1, "" if c2 < 0 else c2)
+                a1, a2 = O.connection_axes(dims, f1, f2, c1, c2)
+                assert b0.getBoundary(f1).axes == a1[:dims], (f1, f2, c1, c2)
+                assert b1.getBoundary(f2).axes == a2[:dims], (f1, f2, c1, c2)
+                tested += 1
+    assert tested > 10
+
+
+def test_csr_size_formula():
+    """nnz = n (2d+1) - sum over prescribed faces of the face area (block.cpp:1145-1155)."""
+    for name, make in cases.CASES.items():
+        specs = make()
+        od = O.OracleDomain(specs, 0.01, 0.01)
+        dom = _cpu_domain(specs[0].dims)
+        total = 0
+        for s in specs:
+            b = dom.CreateBlockWithSize(s.size_xyz[: s.dims])
+            for f, fs in enumerate(s.faces):
+                if fs.kind == "fixed":
+                    b.CloseBoundary(f)
+            total += b.ComputeCSRSize()
+        assert total == od.nnz, name
+
+
+def test_grid_weights():
+    from phipict_amd import grid
+
+    w = np.array(grid.make_weights_tanh(20, 1.5, "BOTH"))
+    assert w[0] == 0 and w[-1] == 1 and np.all(np.diff(w) > 0)
+    np.testing.assert_allclose(w + w[::-1], 1.0, atol=1e-15)
+    d = np.diff(w)
+    assert d[0] < d[10] and d[-1] < d[10]
+    np.testing.assert_allclose(grid.make_weights_tanh(8, 0.0, "BOTH"), np.linspace(0, 1, 9))
+    s = np.array(grid.make_weights_simple_grading(10, 4.0, "START"))
+    ds = np.diff(s)
+    np.testing.assert_allclose(ds[-1] / ds[0], 4.0, rtol=1e-12)
+    np.testing.assert_allclose(s[-1], 1.0)
+    v = grid.rectilinear_vertices([np.linspace(0, 1, 4), np.linspace(0, 2, 3)])
+    assert list(v.shape) == [1, 2, 3, 4]
+    np.testing.assert_allclose(v[0, 1, :, 0].numpy(), [0, 1, 2])
+
+
+def test_solver_tolerance_resolution():
+    from phipict_amd.solvers.tolerance import SolverTolerance, default_tolerance, resolve_tolerance
+
+    b = torch.full((100,), 2.0, dtype=torch.float64)
+    t = SolverTolerance(rtol=1e-3, atol=1e-9)
+    assert math.isclose(resolve_tolerance(t, b), 2e-3)
+    assert math.isclose(SolverTolerance(atol=1e-4).resolve(b), 1e-4)
+    # the precision floor: 8 eps of the RHS scale
+    assert math.isclose(SolverTolerance(rtol=1e-30).resolve(b), 8 * torch.finfo(torch.float64).eps * 2.0)
+    bb = torch.cat([torch.full((50,), 1.0), torch.full((50,), 4.0)]).double()
+    assert resolve_tolerance(t, bb, batch_size=2) == pytest.approx([1e-3, 4e-3])
+    assert resolve_tolerance(1e-7, b) == 1e-7
+    assert default_tolerance(torch.float32) == 1e-5 and default_tolerance(torch.float64) == 1e-8
+    with pytest.raises(ValueError):
+        SolverTolerance()
+    with pytest.raises(ValueError):
+        SolverTolerance(rtol=-1.0)
+
+
+def test_hooks_order_and_validation():
+    from phipict_amd.hooks import Hook, Hooks
+
+    seen = []
+    h = Hooks({"PRE": lambda **k: seen.append("a")})
+    h.append(Hook.PRE, lambda **k: seen.append("b")).prepend("PRE", lambda **k: seen.append("c"))
+    h.run(Hook.PRE)
+    assert seen == ["c", "a", "b"]
+    assert Hook.PRE in h and Hook.POST not in h
+    with pytest.raises(ValueError):
+        h.append("NOT_A_HOOK", print)
+
+
+def test_simulation_rejects_unimplemented_options():
+    import phipict_amd as pa
+
+    dom = _cpu_domain(2)
+    with pytest.raises(NotImplementedError):
+        pa.Simulation(domain=dom, dt=0.1, differentiable=True)
+    with pytest.raises(NotImplementedError):
+        pa.Simulation(domain=dom, dt=0.1, substeps="ADAPTIVE")
+    with pytest.raises(TypeError):
+        pa.Simulation(domain=dom, dt=0.1, no_such_option=1)
+    with pytest.raises(ValueError):
+        pa.Simulation(domain=dom, dt=0.1, velocity_corrector="SPECTRAL")
+    pa.Simulation(domain=dom, dt=0.1, non_orthogonal=False)
+
+
+def test_product_does_not_import_oracle():
+    """The package never reaches the oracle: no import of it anywhere under phipict_amd/."""
+    for p in (ROOT / "phipict_amd").rglob("*.py"):
+        text = p.read_text()
+        assert "oracle" not in re.sub(r"#.*", "", text).replace('"""', ""), p

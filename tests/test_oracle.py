@@ -1,0 +1,496 @@
+"""Pins the oracle against what the mathematics fixes: closed forms on uniform periodic grids
+(metrics, Fourier-mode actions of the assembled operators), invariants the reference's own tests
+check (zero row sums and symmetry of the Poisson operators, test_amg.py:73-86; constant
+potential with a Dirichlet face, test_potential_bc.py:456-471; a block split and reconnected,
+test_batching.py:22-37) and direct solves for the Krylov and AMG paths (test_krylov.py:13-35,
+test_amg.py:88-131). CPU only."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import cases
+from oracle import oracle as O
+
+
+def dense(row, col, val, n=None):
+    n = n or row.size - 1
+    return sp.csr_matrix((np.asarray(val, np.float64), col, row), shape=(n, n)).toarray()
+
+
+def periodic_box(n, lengths, vel=None, dims=None):
+    """Uniform periodic block with cell counts n=[nx, ny(, nz)]."""
+    axes = [np.linspace(0.0, L, k + 1) for k, L in zip(n, lengths)]
+    v = cases.vertices(axes)
+    d = len(n)
+    cz = cases.cell_shape(v)
+    velocity = np.zeros([d] + cz)
+    if vel is not None:
+        velocity[...] = np.asarray(vel).reshape([d] + [1] * d)
+    return [O.BlockSpec(v, [O.FaceSpec("periodic") for _ in range(2 * d)], velocity=velocity, pressure=np.zeros(cz))]
+
+
+def centers_x(spec):
+    v = spec.vertices
+    d = v.shape[0]
+    x = v[0]
+    for ax in range(d):
+        x = 0.5 * (np.take(x, range(x.shape[ax] - 1), axis=ax) + np.take(x, range(1, x.shape[ax]), axis=ax))
+    return x.reshape(-1)
+
+
+# ----------------------------------------------------------------------------- metrics
+@pytest.mark.parametrize("d", [2, 3])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_transforms_affine_closed_form(d, dtype):
+    """For vertices x = L (h * xi) the cell and face Jacobians are L diag(h) everywhere."""
+    rng = np.random.default_rng(7)
+    n = [4, 3, 5][:d]
+    h = np.array([0.3, 0.7, 0.45][:d])
+    L = np.eye(d) + 0.2 * rng.standard_normal((d, d))
+    base = cases.vertices([np.arange(k + 1, dtype=np.float64) for k in n])
+    v = np.einsum("ij,j...->i...", L @ np.diag(h), base)
+    M = L @ np.diag(h)
+    T = O.transforms(v, dtype)
+    tol = 1e-12 if dtype == np.float64 else 2e-5
+    np.testing.assert_allclose(T[:, :d * d], np.tile(M.reshape(-1), (T.shape[0], 1)), atol=tol)
+    np.testing.assert_allclose(T[:, d * d:2 * d * d], np.tile(np.linalg.inv(M).reshape(-1), (T.shape[0], 1)),
+                               atol=tol * 10)
+    np.testing.assert_allclose(T[:, -1], np.linalg.det(M), rtol=tol * 10)
+    FT = O.face_transforms(v, dtype)
+    for axis in range(d):
+        # valid face positions: every tangential vertex index below the last
+        idx = [axis] + [slice(None)] * d
+        for dd in range(d):
+            if dd != axis:
+                idx[1 + (d - 1 - dd)] = slice(0, -1)
+        ft = FT[tuple(idx)].reshape(-1, 2 * d * d + 1)
+        np.testing.assert_allclose(ft[:, :d * d], np.tile(M.reshape(-1), (ft.shape[0], 1)), atol=tol)
+        np.testing.assert_allclose(ft[:, -1], np.linalg.det(M), rtol=tol * 10)
+
+
+def test_transforms_rectilinear_tanh():
+    """Rectilinear graded grid: M = diag(dx, dy), det = dx dy, Minv = diag(1/dx, 1/dy)."""
+    xs = np.linspace(0, 2, 9)
+    ys = cases.tanh_coords(6, -1, 1, 1.5)
+    T = O.transforms(cases.vertices([xs, ys]))
+    dx = np.tile(np.diff(xs), 6)
+    dy = np.repeat(np.diff(ys), 8)
+    np.testing.assert_allclose(T[:, 0], dx, rtol=1e-14)
+    np.testing.assert_allclose(T[:, 3], dy, rtol=1e-14)
+    np.testing.assert_allclose(T[:, [1, 2, 5, 6]], 0, atol=0)
+    np.testing.assert_allclose(T[:, 4], 1 / dx, rtol=1e-14)
+    np.testing.assert_allclose(T[:, 7], 1 / dy, rtol=1e-14)
+    np.testing.assert_allclose(T[:, 8], dx * dy, rtol=1e-14)
+
+
+# ----------------------------------------------------------------------------- assembly closed forms
+@pytest.mark.parametrize("d", [2, 3])
+def test_advection_matrix_fourier(d):
+    """Constant velocity (a, b, c) on a uniform periodic grid: C cos(k x) =
+    (1/dt + 2 nu (1 - cos kh)/h^2) cos(k x) - a sin(kh)/h sin(k x)  (central fluxes)."""
+    n = [16, 6, 5][:d]
+    lengths = [2.0, 1.3, 0.9][:d]
+    vel = [0.7, -0.4, 0.25][:d]
+    nu, dt = 0.03, 0.05
+    specs = periodic_box(n, lengths, vel)
+    od = O.OracleDomain(specs, nu, dt)
+    col, val, A = od.advection_matrix()
+    D = dense(od.row, col, val)
+    h = lengths[0] / n[0]
+    k = 2 * math.pi * 3 / lengths[0]
+    x = centers_x(specs[0])
+    phi = np.cos(k * x)
+    expect = (1 / dt + 2 * nu * (1 - math.cos(k * h)) / h ** 2) * phi - vel[0] * math.sin(k * h) / h * np.sin(k * x)
+    np.testing.assert_allclose(D @ phi, expect, atol=1e-11 * np.abs(expect).max())
+    hs = [L / m for L, m in zip(lengths, n)]
+    np.testing.assert_allclose(A, 1 / dt + sum(2 * nu / hh ** 2 for hh in hs), rtol=1e-13)
+    np.testing.assert_allclose(np.diag(D), A, rtol=0)
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_pressure_and_epot_matrix_fourier(d):
+    """P cos(kx) = -(alpha_x/A)(2 - 2cos kh) cos(kx), Epot cos(kx) = -alpha_x (2 - 2cos kh) cos(kx)
+    with alpha_x = det/h_x^2."""
+    n = [12, 5, 4][:d]
+    lengths = [1.5, 1.0, 0.8][:d]
+    specs = periodic_box(n, lengths, [0.2, 0.1, -0.3][:d])
+    od = O.OracleDomain(specs, 0.02, 0.01)
+    _, _, A = od.advection_matrix()
+    pc, pv = od.pressure_matrix(A)
+    ec, ev = od.epot_matrix()
+    hs = [L / m for L, m in zip(lengths, n)]
+    det = float(np.prod(hs))
+    alpha = det / hs[0] ** 2
+    k = 2 * math.pi * 2 / lengths[0]
+    x = centers_x(specs[0])
+    phi = np.cos(k * x)
+    lam = -alpha * (2 - 2 * math.cos(k * hs[0]))
+    np.testing.assert_allclose(dense(od.row, pc, pv) @ phi, lam / A[0] * phi, atol=1e-12 * abs(lam / A[0]))
+    np.testing.assert_allclose(dense(od.row, ec, ev) @ phi, lam * phi, atol=1e-12 * abs(lam))
+
+
+def test_advection_rhs_and_hbya_closed_form():
+    """Periodic, constant fields: rhs = u/dt + S; HbyA = (u^n/dt - (1/dt - A) u* + S)/A."""
+    n, lengths = [8, 6], [1.0, 2.0]
+    un = np.array([0.3, -0.2])
+    specs = periodic_box(n, lengths, un)
+    S = np.array([0.5, 1.5])
+    specs[0].vsource = S
+    dt = 0.02
+    od = O.OracleDomain(specs, 0.01, dt)
+    col, val, A = od.advection_matrix()
+    rhs = od.advection_rhs().reshape(2, -1)
+    np.testing.assert_allclose(rhs, (un / dt + S)[:, None] * np.ones_like(rhs), rtol=1e-13)
+    ustar = np.array([0.25, 0.05])
+    vres = np.repeat(ustar, od.ncells)
+    prhs = od.pressure_rhs(A, col, val, vres).reshape(2, -1)
+    expect = (un[:, None] / dt - (1 / dt - A[None, :]) * ustar[:, None] + S[:, None]) / A[None, :]
+    np.testing.assert_allclose(prhs, expect, rtol=1e-12)
+
+
+def test_divergence_gradient_current_closed_form():
+    """Face-interpolated divergence, FD gradient correction and face-based current density of
+    Fourier modes on a uniform periodic grid."""
+    n, lengths = [16, 4], [2.0, 1.0]
+    specs = periodic_box(n, lengths)
+    od = O.OracleDomain(specs, 0.01, 0.1)
+    _, _, A = od.advection_matrix()
+    hx, hy = lengths[0] / n[0], lengths[1] / n[1]
+    k = 2 * math.pi / lengths[0]
+    x = centers_x(specs[0])
+    N = od.ncells
+    h = np.concatenate([np.sin(k * x), np.zeros(N)])
+    div = od.pressure_div(h)
+    np.testing.assert_allclose(div, hy * np.cos(k * x) * math.sin(k * hx), atol=1e-13)
+    # epot rhs of chi = (sin kx, 0) is the same face divergence
+    np.testing.assert_allclose(od.epot_rhs(h), hy * np.cos(k * x) * math.sin(k * hx), atol=1e-13)
+    # velocity correction: u = HbyA - dp/dx / A with central differences
+    p = np.sin(k * x)
+    od.pressure[0][...] = p
+    hb = np.concatenate([np.full(N, 0.3), np.full(N, -0.1)])
+    u = od.correct_velocity(A, hb).reshape(2, -1)
+    np.testing.assert_allclose(u[0], 0.3 - np.cos(k * x) * math.sin(k * hx) / hx / A, atol=1e-12)
+    np.testing.assert_allclose(u[1], -0.1, atol=1e-13)
+    # current density: J = -grad_h(phi) + chi
+    chi = np.concatenate([np.full(N, 0.2), np.full(N, -0.7), np.full(N, 0.4)])
+    J = od.current_density(p, chi).reshape(3, -1)
+    np.testing.assert_allclose(J[0], 0.2 - np.cos(k * x) * math.sin(k * hx) / hx, atol=1e-12)
+    np.testing.assert_allclose(J[1], -0.7, atol=1e-12)
+    np.testing.assert_allclose(J[2], 0.4, atol=1e-13)
+
+
+# ----------------------------------------------------------------------------- invariants
+@pytest.mark.parametrize("name", list(cases.CASES))
+def test_poisson_operators_symmetric(name):
+    specs = cases.CASES[name]()
+    od = O.OracleDomain(specs, 0.01, 0.02)
+    _, _, A = od.advection_matrix()
+    pc, pv = od.pressure_matrix(A)
+    P = dense(od.row, pc, pv)
+    np.testing.assert_allclose(P, P.T, atol=1e-14 * np.abs(P).max())
+    # the pressure operator is pure Neumann: every row sums to zero
+    np.testing.assert_allclose(P.sum(axis=1), 0, atol=1e-13 * np.abs(P).max())
+    ec, ev = od.epot_matrix()
+    E = dense(od.row, ec, ev)
+    np.testing.assert_allclose(E, E.T, atol=1e-14 * np.abs(E).max())
+    assert np.all(np.diag(E) <= 0) and np.all(np.diag(P) < 0)
+
+
+def test_epot_row_sums_only_at_anchored_faces():
+    """Rows sum to zero except cells on Dirichlet or thin-wall faces, whose row sums are negative."""
+    for specs in (cases.potential_box3d(), cases.potential_box2d()):
+        od = O.OracleDomain(specs, 0.01, 0.02)
+        ec, ev = od.epot_matrix()
+        E = dense(od.row, ec, ev)
+        rs = E.sum(axis=1)
+        assert np.all(rs <= 1e-12)
+        assert np.any(rs < -1e-6)
+    od = O.OracleDomain(cases.duct3d(), 0.01, 0.02)
+    ec, ev = od.epot_matrix()
+    np.testing.assert_allclose(dense(od.row, ec, ev).sum(axis=1), 0, atol=1e-12)
+
+
+def test_constant_potential_with_dirichlet_face():
+    """phi = g everywhere with Dirichlet value g on a face and chi = 0: Epot phi == rhs and
+    J == 0 (test_potential_bc.py:456-471)."""
+    specs = cases.channel2d(nx=6, ny=5, gradient=None)
+    g = 0.75
+    fs = specs[0].faces[2]
+    fs.pot = O.POT_DIRICHLET
+    fs.pot_values = np.full(cases.face_shape(specs[0].cells_zyx, 2), g)
+    od = O.OracleDomain(specs, 0.01, 0.02)
+    ec, ev = od.epot_matrix()
+    N = od.ncells
+    phi = np.full(N, g)
+    rhs = od.epot_rhs(np.zeros(2 * N))
+    np.testing.assert_allclose(dense(od.row, ec, ev) @ phi, rhs, atol=1e-13)
+    J = od.current_density(phi, np.zeros(3 * N))
+    np.testing.assert_allclose(J, 0, atol=1e-13)
+    # Dirichlet values enter the RHS linearly and never the matrix
+    fs.pot_values = np.full_like(fs.pot_values, 2 * g)
+    od2 = O.OracleDomain(specs, 0.01, 0.02)
+    ec2, ev2 = od2.epot_matrix()
+    np.testing.assert_array_equal(ev2, ev)
+    np.testing.assert_allclose(od2.epot_rhs(np.zeros(2 * N)), 2 * rhs, rtol=1e-14)
+
+
+def _perm_split(nx, ny, split):
+    perm = np.zeros(nx * ny, np.int64)
+    for j in range(ny):
+        for i in range(nx):
+            if i < split:
+                m = j * split + i
+            else:
+                m = ny * split + j * (nx - split) + (i - split)
+            perm[j * nx + i] = m
+    return perm
+
+
+def test_split_block_equals_single_block():
+    """A channel split into two connected blocks assembles the same operators and fields."""
+    nx, ny, split = 16, 12, 6
+    single, pair = cases.split_channel2d(nx, ny, split)
+    o1 = O.OracleDomain(single, 0.02, 0.03)
+    o2 = O.OracleDomain(pair, 0.02, 0.03)
+    perm = _perm_split(nx, ny, split)
+    c1, v1, A1 = o1.advection_matrix()
+    c2, v2, A2 = o2.advection_matrix()
+    D1, D2 = dense(o1.row, c1, v1), dense(o2.row, c2, v2)
+    np.testing.assert_allclose(D2[np.ix_(perm, perm)], D1, atol=1e-13 * np.abs(D1).max())
+    np.testing.assert_allclose(A2[perm], A1, rtol=1e-14)
+    np.testing.assert_allclose(o2.advection_rhs().reshape(2, -1)[:, perm], o1.advection_rhs().reshape(2, -1), rtol=1e-13)
+    p1c, p1v = o1.pressure_matrix(A1)
+    p2c, p2v = o2.pressure_matrix(A2)
+    np.testing.assert_allclose(dense(o2.row, p2c, p2v)[np.ix_(perm, perm)], dense(o1.row, p1c, p1v), atol=1e-15)
+    e1c, e1v = o1.epot_matrix()
+    e2c, e2v = o2.epot_matrix()
+    np.testing.assert_allclose(dense(o2.row, e2c, e2v)[np.ix_(perm, perm)], dense(o1.row, e1c, e1v), atol=1e-13)
+    rng = np.random.default_rng(3)
+    vr = rng.standard_normal(2 * nx * ny)
+    vr2 = np.zeros_like(vr)
+    vr2.reshape(2, -1)[:, perm] = vr.reshape(2, -1)
+    pr1 = o1.pressure_rhs(A1, c1, v1, vr)
+    pr2 = o2.pressure_rhs(A2, c2, v2, vr2)
+    np.testing.assert_allclose(pr2.reshape(2, -1)[:, perm], pr1.reshape(2, -1), rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(o2.pressure_div(pr2)[perm], o1.pressure_div(pr1), atol=1e-12)
+
+
+# ----------------------------------------------------------------------------- Krylov
+def _poisson(nx, ny, shift=0.0):
+    od = O.OracleDomain(cases.channel2d(nx, ny), 0.02, 0.01)
+    _, _, A = od.advection_matrix()
+    pc, pv = od.pressure_matrix(A)
+    if shift:
+        pv = pv.copy()
+        for i in range(od.ncells):
+            for k in range(od.row[i], od.row[i + 1]):
+                if pc[k] == i:
+                    pv[k] -= shift
+    return od.row, pc, pv
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_cg_matches_direct_solve(dtype):
+    row, col, val = _poisson(12, 10, shift=0.5)
+    n = row.size - 1
+    rng = np.random.default_rng(0)
+    b = rng.standard_normal(n).astype(dtype)
+    x = np.zeros(n, dtype)
+    tol = 1e-10 if dtype == np.float64 else 1e-5
+    r = O.cg(row, col, val.astype(dtype), b, x, tol)
+    assert r.converged and r.is_finite
+    M = sp.csr_matrix((val, col, row), shape=(n, n))
+    xd = spla.spsolve(M.tocsc(), b.astype(np.float64))
+    res = np.linalg.norm(b - M @ x) / math.sqrt(n)
+    assert res < 10 * tol
+    assert abs(res - r.final_residual) < 2 * tol + 1e-3 * res
+    assert cases.rel_l2(x, xd) < (1e-8 if dtype == np.float64 else 1e-3)
+
+
+def test_cg_distinct_eigenvalues_and_exact_start():
+    """CG on a diagonal matrix with 3 distinct eigenvalues converges within 3 iterations; an
+    exact initial guess stops before the first iteration."""
+    n = 30
+    d = np.repeat([1.0, 3.0, 7.0], 10)
+    row = np.arange(n + 1, dtype=np.int32)
+    col = np.arange(n, dtype=np.int32)
+    b = np.random.default_rng(1).standard_normal(n)
+    x = np.zeros(n)
+    r = O.cg(row, col, d, b, x, 1e-12)
+    assert r.converged and r.used_iterations <= 3
+    np.testing.assert_allclose(x, b / d, rtol=1e-12)
+    x2 = b / d
+    r2 = O.cg(row, col, d, b, x2, 1e-12)
+    assert r2.converged and r2.used_iterations == 0
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_bicgstab_matches_direct_solve(dtype):
+    od = O.OracleDomain(cases.channel2d(12, 10, u0=3.0), 0.02, 0.05)
+    col, val, A = od.advection_matrix()
+    n = od.ncells
+    M = sp.csr_matrix((val, col, od.row), shape=(n, n))
+    assert abs(M - M.T).max() > 1e-3  # non-symmetric
+    b = np.random.default_rng(2).standard_normal(n).astype(dtype)
+    x = np.zeros(n, dtype)
+    tol = 1e-10 if dtype == np.float64 else 1e-5
+    r = O.bicgstab(od.row, col, val.astype(dtype), b, x, tol)
+    assert r.converged
+    xd = spla.spsolve(M.tocsc(), b.astype(np.float64))
+    assert np.linalg.norm(b - M @ x) / math.sqrt(n) < 10 * tol
+    assert cases.rel_l2(x, xd) < (1e-8 if dtype == np.float64 else 1e-4)
+    # converged start reports -1 iterations (krylov.cu:800-805)
+    r2 = O.bicgstab(od.row, col, val.astype(dtype), b, x.copy(), 1e3)
+    assert r2.converged and r2.used_iterations == -1
+
+
+def test_cg_return_best_reports_its_residual():
+    row, col, val = _poisson(16, 12)
+    n = row.size - 1
+    b = np.random.default_rng(4).standard_normal(n)
+    b -= b.mean()
+    x = np.zeros(n)
+    r = O.cg(row, col, val, b, x, 1e-14, maxit=7, return_best=True)
+    assert not r.converged
+    M = sp.csr_matrix((val, col, row), shape=(n, n))
+    true_res = np.linalg.norm(b - M @ x) / math.sqrt(n)
+    assert abs(true_res - r.final_residual) < 1e-10 * max(1.0, true_res)
+
+
+def test_projected_cg_on_singular_system():
+    """Pure-Neumann operator with a mean-free RHS: projected CG returns the least-squares
+    solution up to a constant."""
+    row, col, val = _poisson(10, 8)
+    n = row.size - 1
+    b = np.random.default_rng(5).standard_normal(n)
+    b -= b.mean()
+    x = np.zeros(n)
+    r = O.cg(row, col, val, b, x, 1e-11, project=True)
+    assert r.converged
+    M = dense(row, col, val)
+    xl = np.linalg.lstsq(M, b, rcond=None)[0]
+    np.testing.assert_allclose(x - x.mean(), xl - xl.mean(), atol=1e-7 * np.abs(xl).max())
+
+
+def test_csr_transpose_matches_scipy():
+    od = O.OracleDomain(cases.rotated_pair3d(), 0.02, 0.05)
+    col, val, _ = od.advection_matrix()
+    tr, tc, tv = O.csr_transpose(od.row, col, val)
+    M = sp.csr_matrix((val, col, od.row), shape=(od.ncells,) * 2)
+    np.testing.assert_array_equal(sp.csr_matrix((tv, tc, tr), shape=M.shape).toarray(), M.T.toarray())
+
+
+# ----------------------------------------------------------------------------- AMG
+def _as_sp(t, shape):
+    r, c, v = t
+    return sp.csr_matrix((v, c, r), shape=shape)
+
+
+@pytest.mark.parametrize("which", ["pressure2d", "epot3d"])
+def test_amg_hierarchy_invariants(which):
+    if which == "pressure2d":
+        row, col, val = _poisson(40, 32)
+    else:
+        od = O.OracleDomain(cases.duct3d(16, 12, 12), 0.01, 0.02)
+        col, val = od.epot_matrix()
+        row = od.row
+    n = row.size - 1
+    H = O.OracleHierarchy(row, col, val)
+    L = H.levels
+    assert len(L) >= 2 and L[-1]["n"] <= 64
+    nnz = [lv["A"][1].size for lv in L]
+    assert sum(nnz) / nnz[0] < 3.0  # operator complexity (test_amg.py:119)
+    for l in range(len(L) - 1):
+        nf, nc = L[l]["n"], L[l + 1]["n"]
+        A = _as_sp(L[l]["A"], (nf, nf))
+        P = _as_sp(L[l]["P"], (nf, nc))
+        R = _as_sp(L[l]["R"], (nc, nf))
+        assert abs(R - P.T).max() == 0
+        Ac = _as_sp(L[l + 1]["A"], (nc, nc))
+        np.testing.assert_allclose((R @ A @ P).toarray(), Ac.toarray(), atol=1e-12 * abs(A).max())
+        split = L[l]["split"]
+        assert split.sum() == nc and nc < nf
+        # C points are injected
+        cpts = np.flatnonzero(split == 1)
+        Pd = P.toarray()
+        np.testing.assert_array_equal(Pd[cpts], np.eye(nc))
+        # constants are interpolated exactly where A has zero row sums
+        zero_rows = np.abs(np.asarray(A.sum(axis=1)).ravel()) < 1e-12 * abs(A).max()
+        if zero_rows.all():
+            np.testing.assert_allclose(Pd.sum(axis=1), 1.0, atol=1e-12)
+        # every F point with strong connections has a strong C neighbour (RS first pass)
+        Ad = A.toarray()
+        for i in np.flatnonzero(split == 0):
+            off = -Ad[i].copy()
+            off[i] = 0
+            if np.abs(off).max() == 0:
+                continue
+            strong = np.abs(off) >= 0.25 * np.abs(off).max()
+            assert (strong & (split == 1)).any(), i
+
+
+def test_amg_pcg_matches_direct_and_converges_fast():
+    row, col, val = _poisson(48, 40, shift=1e-3)
+    n = row.size - 1
+    from oracle.piso_oracle import OracleHierarchyState
+
+    hs = OracleHierarchyState(row, col, val, project=False, hdtype=np.float64)
+    b = np.random.default_rng(6).standard_normal(n)
+    x = np.zeros(n)
+    r = O.amg_pcg((row, col, val), hs.native, hs.pinv, b, x, 1e-10, hdtype=np.float64)
+    assert r.converged and r.used_iterations < 40
+    M = sp.csr_matrix((val, col, row), shape=(n, n))
+    xd = spla.spsolve(M.tocsc(), b)
+    assert cases.rel_l2(x, xd) < 1e-6
+    # plain CG needs many more iterations
+    x2 = np.zeros(n)
+    r2 = O.cg(row, col, val, b, x2, 1e-10)
+    assert r2.used_iterations > 2 * r.used_iterations
+
+
+def test_amg_pcg_singular_projected():
+    row, col, val = _poisson(32, 24)
+    n = row.size - 1
+    from oracle.piso_oracle import OracleHierarchyState
+
+    hs = OracleHierarchyState(row, col, val, project=True)
+    b = np.random.default_rng(8).standard_normal(n)
+    b -= b.mean()
+    x = np.zeros(n)
+    r = O.amg_pcg((row, col, val), hs.native, hs.pinv, b, x, 1e-9, project=True, hdtype=np.float32)
+    assert r.converged
+    xl = np.linalg.lstsq(dense(row, col, val), b, rcond=None)[0]
+    np.testing.assert_allclose(x - x.mean(), xl - xl.mean(), atol=1e-6 * np.abs(xl).max())
+
+
+# ----------------------------------------------------------------------------- PISO
+def test_oracle_hartmann_profile():
+    """Steady Hartmann flow between insulating walls: u(y) = G/N (1 - cosh(Ha y)/cosh Ha)
+    (examples/hartmann.py:85). Coarse 2D grid, periodic in x; the error is the discretisation
+    error of the wall-refined grid."""
+    from oracle.piso_oracle import MHDSpec, OraclePISO
+
+    Ha, Re = 4.0, 1.0
+    N = Ha ** 2 / Re
+    G = N
+    specs = cases.channel2d(nx=4, ny=32, stretch=1.5, noise=0.0, gradient=G)
+    od = O.OracleDomain(specs, 1.0 / Re, 0.05)
+    base = specs[0].vsource.copy()
+
+    def reset_source(sim):
+        sim.od.set_source(0, base)
+
+    sim = OraclePISO(od, 0.05, mhd=MHDSpec(stuart=N, e_b=(0.0, 1.0, 0.0), potential_tol=1e-11),
+                     pre_velocity_setup=reset_source, advection_tol=1e-11, pressure_tol=1e-11)
+    for _ in range(150):
+        sim.step()
+    y = 0.5 * (specs[0].vertices[1, 1:, 0] + specs[0].vertices[1, :-1, 0])
+    u = od.velocity[0].reshape(2, 32, 4)[0].mean(axis=1)
+    exact = G / N * (1 - np.cosh(Ha * y) / math.cosh(Ha))
+    err = np.abs(u - exact).max() / np.abs(exact).max()
+    assert err < 5e-3, err
