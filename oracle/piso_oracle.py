@@ -26,6 +26,18 @@ class MHDSpec:
     use_amg: bool = False
 
 
+def resolve_tol(tol, b: np.ndarray) -> float:
+    """Absolute tolerance of a solve: a float is absolute; ``("rtol", r)`` is relative to the RHS
+    scale ||b||/sqrt(n), floored at 8 machine epsilons of that scale (tolerance.py:180-215)."""
+    if not isinstance(tol, tuple):
+        return float(tol)
+    scale = float(np.linalg.norm(b.astype(np.float64))) / np.sqrt(max(b.size, 1))
+    t = max(tol[1] * scale, 8.0 * float(np.finfo(b.dtype).eps) * scale)
+    if not t > 0:
+        t = 1e-8 if b.dtype == np.float64 else 1e-5
+    return t
+
+
 def cross(v: np.ndarray, e: np.ndarray) -> np.ndarray:
     """Row-wise v x e for v [3, n], e [3]."""
     return np.stack([v[1] * e[2] - v[2] * e[1], v[2] * e[0] - v[0] * e[2], v[0] * e[1] - v[1] * e[0]])
@@ -124,6 +136,7 @@ class OraclePISO:
         if not np.any(b != 0):
             x[...] = 0
             return None
+        tol = resolve_tol(tol, b)
         if hier is not None:
             r = O.amg_pcg((row, col, val), hier.native, hier.pinv, b, x, tol, self.max_iter, return_best,
                           hier.project, hdtype=hier.hdtype)
@@ -196,9 +209,10 @@ class OraclePISO:
             rhs = od.advection_rhs()
             x = np.zeros_like(self.vel_result)
         if np.any(rhs != 0):
+            adv_tol = resolve_tol(self.adv_tol, rhs)
             for k in range(d):
                 xk = np.ascontiguousarray(x[k * n:(k + 1) * n])
-                r = O.bicgstab(od.row, ccol, cval, np.ascontiguousarray(rhs[k * n:(k + 1) * n]), xk, self.adv_tol,
+                r = O.bicgstab(od.row, ccol, cval, np.ascontiguousarray(rhs[k * n:(k + 1) * n]), xk, adv_tol,
                                self.max_iter)
                 self.iters.append(("bicg", r.used_iterations, r.final_residual, r.converged))
                 if not r.converged:
@@ -248,6 +262,9 @@ class OraclePISO:
             self._magnetic_step()
 
     def _p_singular(self, pval):
+        """Pure-Neumann pressure operator: every row sum vanishes to within the rounding of the
+        value dtype (piso_simulation.py:2165-2220 uses 1e-10 of the largest entry)."""
         row = self.od.row
         sums = np.add.reduceat(pval.astype(np.float64), row[:-1]) if pval.size else np.zeros(0)
-        return bool((np.abs(sums) > 1e-10 * np.abs(pval).max()).sum() == 0)
+        rel = max(1e-10, 64.0 * float(np.finfo(pval.dtype).eps))
+        return bool((np.abs(sums) > rel * np.abs(pval).max()).sum() == 0)

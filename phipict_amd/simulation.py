@@ -168,7 +168,10 @@ class Simulation:
         counts = (P.row[1:] - P.row[:-1]).to(torch.int64)
         seg = torch.repeat_interleave(torch.arange(n, device=value.device), counts)
         row_sum = torch.zeros(n, dtype=value.dtype, device=value.device).index_add_(0, seg, value)
-        deficient = bool((row_sum.abs() > 1e-10 * value.abs().max()).sum().item() == 0)
+        # the reference's threshold of 1e-10 of the largest entry is below FP32 rounding, which
+        # would call every FP32 pressure operator regular; scale it with the dtype instead
+        rel = max(1e-10, 64.0 * torch.finfo(value.dtype).eps)
+        deficient = bool((row_sum.abs() > rel * value.abs().max()).sum().item() == 0)
         self._pressure_rank_deficient_cache = (P.index, deficient, n)
         return deficient
 
