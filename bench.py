@@ -1,0 +1,304 @@
+#!/usr/bin/env python3
+"""PISO + inductionless-MHD step throughput on one MI355X.
+
+Workload (default): the 3D rectangular MHD duct, one block of 192 x 96 x 96 cells (x streamwise,
+periodic, driven by a uniform pressure-gradient source; y and z wall-refined tanh grids with
+factor 1.5 on [-1, 1]; insulating no-slip walls), Re = 500, Ha = 100 (N = Ha^2/Re = 20),
+e_b = (0, 1, 0), dt = 5e-4, two pressure correctors, BiCGStab momentum solves, AMG-preconditioned
+CG pressure and potential solves to the absolute tolerance 1e-8; initial velocity N(0, 1e-3)
+(seed 1). A step is one full PISO + magnetic step of the library (``MHDSimulation.single_step``)
+with every field resident in HBM.
+
+Printed: one JSON line with steps/s, the roofline of the kernel with the largest share of the
+timed region (algorithmic bytes per launch / average launch time, from HIP events recorded on
+the kernel's own stream for every k-th launch during the timed region) and the oracle's CPU
+time for a bounded sample of the same physics.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+RE, HA = 500.0, 100.0
+DT = 5e-4
+TOL = 1e-8
+STRETCH = 1.5
+LENGTH = 4.0
+
+
+def parse_args(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--dtype", choices=["f64", "f32"], default="f64")
+    ap.add_argument("--size", default="192x96x96", help="cells nx x ny x nz")
+    ap.add_argument("--timer-every", type=int, default=4, help="time every k-th launch of each kernel")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-size", default="96x48x48", help="cells of the CPU sample")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--details", action="store_true", help="also print the per-kernel table to stderr")
+    return ap.parse_args(argv)
+
+
+def duct_vertices(nx, ny, nz):
+    import numpy as np
+
+    from phipict_amd import grid
+
+    xs = np.linspace(0.0, LENGTH, nx + 1)
+    ys = np.array(grid.axis_coords(-1.0, 1.0, grid.make_weights_tanh(ny, STRETCH, "BOTH")))
+    zs = np.array(grid.axis_coords(-1.0, 1.0, grid.make_weights_tanh(nz, STRETCH, "BOTH")))
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    return np.stack([X, Y, Z])
+
+
+def make_sim(nx, ny, nz, dtype, device):
+    import torch
+
+    import phipict_amd as pa
+
+    N = HA ** 2 / RE
+    G = N  # core velocity of order one
+    v = duct_vertices(nx, ny, nz)
+    gen = torch.Generator("cpu").manual_seed(1)
+    vel = 1e-3 * torch.randn(1, 3, nz, ny, nx, generator=gen, dtype=torch.float64)
+    dom = pa.Domain(3, torch.tensor([1.0 / RE], dtype=dtype), dtype=dtype, device=device)
+    blk = dom.CreateBlock(velocity=vel.to(dtype), vertexCoordinates=torch.as_tensor(v, dtype=dtype).unsqueeze(0))
+    for f in (2, 3, 4, 5):
+        blk.CloseBoundary(f)
+    source = torch.tensor([[G, 0.0, 0.0]], dtype=dtype, device=device)
+    blk.setVelocitySource(source)
+    dom.PrepareSolve()
+
+    def forcing(domain, **kw):
+        domain.getBlocks()[0].setVelocitySource(source)
+        domain.UpdateDomainData()
+
+    sim = pa.MHDSimulation(domain=dom, dt=DT, stuart_number=torch.tensor(N),
+                           e_b=torch.tensor([0.0, 1.0, 0.0], dtype=dtype), non_orthogonal=False,
+                           pressure_use_amg=True, potential_use_preconditioner=True, pressure_tol=TOL,
+                           potential_tol=TOL, hooks={"PRE_VELOCITY_SETUP": forcing})
+    sim.record_solver_stats = True
+    return dom, sim
+
+
+# ----------------------------------------------------------------------------- algorithmic bytes
+def kernel_bytes(name, rows, s, sh, fine, levels):
+    """Algorithmic HBM bytes of one launch (neighbour reads counted once): ``fine`` = (n, nnz)
+    of the system matrices, ``levels`` maps level rows -> (nnz_A, n_coarse, nnz_R, nnz_P).
+    s / sh: bytes per value of the system / the AMG hierarchy."""
+    n, nnz = fine
+
+    def M(nz, rws, val):
+        return nz * (val + 4) + (rws + 1) * 4
+
+    d = 3
+    if name in ("cg_direction", "pcg_direction"):
+        return M(nnz, n, s) + 4 * n * s
+    if name in ("cg_update", "pcg_update"):
+        return 6 * n * s
+    if name == "pcg_dots":
+        return 2 * n * s
+    if name == "bicg_p":
+        return 4 * n * s
+    if name in ("bicg_v", "bicg_t"):
+        return M(nnz, n, s) + 3 * n * s
+    if name == "bicg_update":
+        return 8 * n * s
+    if name in ("mean",):
+        return n * s
+    if name == "subtract_mean":
+        return 2 * n * s
+    lv = levels.get(rows)
+    if name == "amg_presmooth" and lv:
+        vb = s if rows == n else sh
+        return M(lv[0], rows, sh) + rows * (sh + vb + 2 * sh)
+    if name in ("amg_jacobi", "amg_post_jacobi") and lv:
+        vb = s if rows == n else sh
+        out = s if (rows == n and name == "amg_post_jacobi") else sh
+        return M(lv[0], rows, sh) + rows * (sh + vb + sh + out)
+    if name == "amg_restrict":
+        for nf, (na, nc, nr, npp) in levels.items():
+            if nc == rows:
+                return M(nr, nc, sh) + nf * sh + nc * sh
+    if name == "amg_prolong" and lv:
+        return M(lv[3], rows, sh) + lv[1] * sh + 2 * rows * sh
+    if name == "advection_matrix":
+        return n * (d * s + (2 * d * d + 1) * s + (2 * d + 1) * s + s)
+    if name == "advection_rhs":
+        return n * ((2 * d + 2) * s)
+    if name == "pressure_matrix":
+        return n * (s + (d * d + 1) * s + (2 * d + 1) * s)
+    if name == "pressure_rhs":
+        return n * ((2 * d + 1) * (s + 4) + 4 + 3 * d * s + s)
+    if name == "pressure_div":
+        return n * (d * s + d * d * s + s)
+    if name == "correct_velocity":
+        return n * (2 * s + 2 * d * s)
+    if name == "epot_rhs":
+        return n * (3 * s + d * d * s + s)
+    if name == "current_density":
+        return n * (s + 3 * s + (2 * d * d + 1) * s + 3 * s)
+    return None
+
+
+def hierarchy_levels(sim):
+    out = {}
+    for h in (getattr(sim, "_pressure_amg_hierarchy", None), getattr(sim, "_epot_amg_hierarchy", None)):
+        if h is None:
+            continue
+        for i, lvl in enumerate(h.levels[:-1]):
+            nxt = h.levels[i + 1]
+            out[lvl.n] = (lvl.A.getNnz(), nxt.n, int(lvl.R_col.numel()), int(lvl.P_col.numel()))
+    return out
+
+
+# ----------------------------------------------------------------------------- CPU baseline
+def cpu_baseline(size, steps, full_cells, dtype_name):
+    """The oracle (single-threaded C restatement) on a bounded sample: the same physics and
+    solver settings on a coarser grid; steps/s scaled to the full grid by the cell ratio."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from oracle.piso_oracle import MHDSpec, OraclePISO
+
+    nx, ny, nz = size
+    dtype = np.float64 if dtype_name == "f64" else np.float32
+    v = duct_vertices(nx, ny, nz)
+    rng = np.random.default_rng(1)
+    cz = [nz, ny, nx]
+    faces = [O.FaceSpec("periodic"), O.FaceSpec("periodic")] + [O.FaceSpec("fixed") for _ in range(4)]
+    N = HA ** 2 / RE
+    src = np.array([N, 0.0, 0.0])
+    spec = O.BlockSpec(v, faces, velocity=1e-3 * rng.standard_normal([3] + cz), pressure=np.zeros(cz), vsource=src)
+    od = O.OracleDomain([spec], 1.0 / RE, DT, dtype)
+    atol = 1e-8 if dtype == np.float64 else 1e-5
+    sim = OraclePISO(od, DT, advection_tol=atol, pressure_tol=TOL, pressure_use_amg=True,
+                     mhd=MHDSpec(N, potential_tol=TOL, use_amg=True),
+                     pre_velocity_setup=lambda o: o.od.set_source(0, src.astype(dtype)))
+    sim.step()  # AMG setup and first assembly
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+    el = time.perf_counter() - t0
+    cells = nx * ny * nz
+    sps_sample = steps / el
+    return {
+        "value": sps_sample * cells / full_cells,
+        "unit": "steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle (C, 1 thread) on the same duct physics at {nx}x{ny}x{nz} cells, {steps} steps after one "
+                   f"warm-up step ({el:.2f} s, {sps_sample:.3f} steps/s, {sps_sample * cells:.3e} cell-updates/s); "
+                   f"value scaled to {full_cells} cells by the cell ratio"),
+        "cell_updates_per_s": sps_sample * cells,
+    }
+
+
+def main(argv=None) -> int:
+    args = parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (the PISO path would "
+              "shard by blocks across GPUs, which is not built)", file=sys.stderr)
+        return 2
+    import numpy as np
+    import torch
+
+    import phipict_amd as pa
+    from phipict_amd import profiling
+
+    nx, ny, nz = (int(v) for v in args.size.lower().split("x"))
+    dtype = torch.float64 if args.dtype == "f64" else torch.float32
+    device = torch.device("cuda", 0)
+    print(f"[bench] building the {nx}x{ny}x{nz} duct ({args.dtype})", file=sys.stderr, flush=True)
+    dom, sim = make_sim(nx, ny, nz, dtype, device)
+    for i in range(args.warmup):
+        if not sim.single_step():
+            raise RuntimeError(f"warm-up step {i} failed")
+    torch.cuda.synchronize()
+    print(f"[bench] warm-up done ({args.warmup} steps)", file=sys.stderr, flush=True)
+    log_start = len(sim.solver_log)
+    profiling.start(args.timer_every)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        if not sim.single_step():
+            raise RuntimeError(f"step {i} failed")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    profiling.stop()
+    timings = profiling.read()
+    print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    n = dom.getTotalSize()
+    s = 8 if dtype == torch.float64 else 4
+    sh = 4  # hierarchy values are float32
+    fine = (n, dom.C.getNnz())
+    levels = hierarchy_levels(sim)
+    rows = []
+    for t in timings:
+        b = kernel_bytes(t.name, t.rows, s, sh, fine, levels)
+        rows.append((t, b))
+    rows.sort(key=lambda tb: -tb[0].est_total_ms)
+    if args.details:
+        tot = sum(t.est_total_ms for t, _ in rows)
+        print(f"{'kernel':22s} {'rows':>9s} {'launches':>9s} {'avg us':>9s} {'share':>7s} {'GB/s':>8s}", file=sys.stderr)
+        for t, b in rows:
+            gbs = b / (t.avg_ms * 1e-3) / 1e9 if (b and t.sampled) else float("nan")
+            print(f"{t.name:22s} {t.rows:9d} {t.launches:9d} {t.avg_ms * 1e3:9.1f} {t.est_total_ms / max(tot, 1e-12):7.1%} "
+                  f"{gbs:8.1f}", file=sys.stderr)
+    dom_k = next(((t, b) for t, b in rows if b is not None and t.sampled > 0), None)
+    roofline = None
+    if dom_k is not None:
+        t, b = dom_k
+        achieved = b / (t.avg_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": t.name, "rows": t.rows,
+                    "bytes_per_launch": int(b), "avg_us": round(t.avg_ms * 1e3, 2), "launches": t.launches,
+                    "share_of_timed_kernels": round(t.est_total_ms / max(sum(x.est_total_ms for x, _ in rows), 1e-12), 3)}
+    iters = {}
+    for _, tag, infos in sim.solver_log[log_start:]:
+        iters.setdefault(tag, []).append(max(i[0] for i in infos))
+    cpu = None
+    if not args.no_cpu_baseline:
+        cx, cy, cz = (int(v) for v in args.cpu_size.lower().split("x"))
+        print(f"[bench] CPU baseline sample {cx}x{cy}x{cz}", file=sys.stderr, flush=True)
+        cpu = cpu_baseline((cx, cy, cz), args.cpu_steps, n, args.dtype)
+    sps = args.steps / elapsed
+    line = {
+        "metric": "PISO steps/s",
+        "value": round(sps, 4),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic",
+        "config": {"workload": f"3D MHD duct {nx}x{ny}x{nz} cells (periodic x, pressure-gradient source), Re=500, "
+                               f"Ha=100, dt=5e-4, AMG-PCG pressure+potential tol 1e-8, BiCGStab momentum",
+                   "cells": n, "nnz": dom.C.getNnz(), "correctors": 2},
+        "cell_updates_per_s": round(sps * n, 1),
+        "solver_iterations_mean": {k: round(float(np.mean(v)), 2) for k, v in iters.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

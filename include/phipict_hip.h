@@ -143,6 +143,28 @@ const char *pp_last_error(void);
 int pp_version(void);
 int pp_device_sync(void);
 
+/* ---- measurement (no counterpart in the reference) ----
+ * A sampled per-kernel timer: while started, every sample_every-th launch of each
+ * instrumented kernel is bracketed by HIP events on the stream it is launched on.
+ * pp_kernel_timer_read returns, per (kernel id, rows per launch), the summed duration of the
+ * sampled launches, their number and the number of all launches; *count is the number of
+ * entries (may exceed capacity). */
+enum pp_kernel_id {
+    PP_K_ADVECTION_MATRIX = 1, PP_K_ADVECTION_RHS = 2, PP_K_PRESSURE_MATRIX = 3, PP_K_PRESSURE_RHS = 4,
+    PP_K_PRESSURE_DIV = 5, PP_K_CORRECT_VELOCITY = 6, PP_K_EPOT_RHS = 7, PP_K_CURRENT_DENSITY = 8,
+    PP_K_COPY_RESULT = 9,
+    PP_K_RESIDUAL = 20, PP_K_CG_DIRECTION = 21, PP_K_CG_UPDATE = 22, PP_K_BICG_P = 23, PP_K_BICG_V = 24,
+    PP_K_BICG_T = 25, PP_K_BICG_UPDATE = 26, PP_K_CG_SMALL = 27, PP_K_BICG_SMALL = 28,
+    PP_K_PCG_DOTS = 30, PP_K_PCG_DIRECTION = 31, PP_K_PCG_UPDATE = 32, PP_K_AMG_PRESMOOTH = 33,
+    PP_K_AMG_JACOBI = 34, PP_K_AMG_LEVEL_RESIDUAL = 35, PP_K_AMG_RESTRICT = 36, PP_K_AMG_PROLONG = 37,
+    PP_K_AMG_COARSE = 38, PP_K_MEAN = 39, PP_K_SUBTRACT_MEAN = 40, PP_K_AMG_POST_JACOBI = 41,
+    PP_K_SPMV = 50, PP_K_GALERKIN = 51
+};
+int pp_kernel_timer_start(int32_t sample_every);
+int pp_kernel_timer_stop(void);
+int pp_kernel_timer_read(int32_t capacity, int32_t *kernel_ids, int32_t *rows, double *total_ms, int64_t *sampled,
+                         int64_t *launches, int32_t *count);
+
 /* ---- grid metrics ----
  * replaces CoordsToTransforms (csrc/phipict/grid/grid_gen.cu:356-391) and
  * CoordsToFaceTransforms (grid_gen.cu:529-): coords [d][(sz+1)][sy+1][sx+1] vertices ->

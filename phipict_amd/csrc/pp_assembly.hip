@@ -799,7 +799,7 @@ int pp_setup_advection_matrix(const pp_domain *dom, int32_t non_ortho_flags, voi
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(dom->C.val && dom->C.row && dom->A && dom->time_step, "C, A and time_step must be set");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        hipLaunchKernelGGL((k_advection_matrix<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom)));
+        PP_TIMED(PP_K_ADVECTION_MATRIX, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_advection_matrix<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom))));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
@@ -809,8 +809,8 @@ int pp_setup_advection_velocity(const pp_domain *dom, int32_t non_ortho_flags, i
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(dom->velocity_rhs && dom->time_step, "velocity_rhs and time_step must be set");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        hipLaunchKernelGGL((k_advection_rhs<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
-                           (int)apply_pressure_gradient)));
+        PP_TIMED(PP_K_ADVECTION_RHS, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_advection_rhs<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+                           (int)apply_pressure_gradient))));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
@@ -820,7 +820,7 @@ int pp_setup_pressure_matrix(const pp_domain *dom, int32_t non_ortho_flags, void
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(dom->P.val && dom->P.row && dom->A, "P and A must be set");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        hipLaunchKernelGGL((k_pressure_matrix<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom)));
+        PP_TIMED(PP_K_PRESSURE_MATRIX, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_pressure_matrix<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom))));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
@@ -830,9 +830,9 @@ int pp_setup_pressure_rhs(const pp_domain *dom, int32_t non_ortho_flags, int32_t
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(dom->pressure_rhs && dom->pressure_rhs_div && dom->C.val && dom->velocity_result, "pressure fields must be set");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        hipLaunchKernelGGL((k_pressure_rhs<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom);
-        hipLaunchKernelGGL((k_pressure_div<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
-                           (int)time_step_norm)));
+        PP_TIMED(PP_K_PRESSURE_RHS, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_pressure_rhs<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom));
+        PP_TIMED(PP_K_PRESSURE_DIV, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_pressure_div<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+                           (int)time_step_norm))));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
@@ -842,8 +842,8 @@ int pp_setup_pressure_rhs_div(const pp_domain *dom, int32_t non_ortho_flags, int
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(dom->pressure_rhs && dom->pressure_rhs_div, "pressure fields must be set");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        hipLaunchKernelGGL((k_pressure_div<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
-                           (int)time_step_norm)));
+        PP_TIMED(PP_K_PRESSURE_DIV, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_pressure_div<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+                           (int)time_step_norm))));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
@@ -852,8 +852,8 @@ int pp_correct_velocity(const pp_domain *dom, int32_t time_step_norm, void *stre
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(dom->velocity_result && dom->pressure_rhs && dom->A, "velocity_result, pressure_rhs and A must be set");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        hipLaunchKernelGGL((k_correct_velocity<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
-                           (int)time_step_norm)));
+        PP_TIMED(PP_K_CORRECT_VELOCITY, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_correct_velocity<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+                           (int)time_step_norm))));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
@@ -872,8 +872,8 @@ int pp_compute_epot_rhs(const pp_domain *dom, const void *vector_field, void *di
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(vector_field && divergence, "vector_field and divergence must not be null");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        hipLaunchKernelGGL((k_epot_rhs<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
-                           reinterpret_cast<const T *>(vector_field), reinterpret_cast<T *>(divergence))));
+        PP_TIMED(PP_K_EPOT_RHS, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_epot_rhs<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+                           reinterpret_cast<const T *>(vector_field), reinterpret_cast<T *>(divergence)))));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
@@ -882,8 +882,8 @@ int pp_compute_current_density(const pp_domain *dom, const void *phi, const void
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(phi && uxb && j_out, "phi, uxb and j_out must not be null");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        hipLaunchKernelGGL((k_current_density<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
-                           reinterpret_cast<const T *>(phi), reinterpret_cast<const T *>(uxb), reinterpret_cast<T *>(j_out))));
+        PP_TIMED(PP_K_CURRENT_DENSITY, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_current_density<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+                           reinterpret_cast<const T *>(phi), reinterpret_cast<const T *>(uxb), reinterpret_cast<T *>(j_out)))));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
@@ -895,8 +895,8 @@ int pp_copy_result(const pp_domain *dom, int32_t field, int32_t to_blocks, void 
     PP_CHECK_ARG(flat != nullptr, "result vector not set");
     const int ch = field == 0 ? dom->dims : 1;
     PP_DISPATCH(dom->dtype,
-        hipLaunchKernelGGL((k_copy_result<T>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom, (int)field, ch,
-                           (int)to_blocks, reinterpret_cast<const T *>(flat), reinterpret_cast<T *>(flat)));
+        PP_TIMED(PP_K_COPY_RESULT, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_copy_result<T>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom, (int)field, ch,
+                           (int)to_blocks, reinterpret_cast<const T *>(flat), reinterpret_cast<T *>(flat))));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }

@@ -50,6 +50,20 @@ const char *get_error();
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// ---- sampled kernel timer ---------------------------------------------------------------
+// When enabled (pp_kernel_timer_start), every k-th launch of an instrumented kernel is
+// bracketed by two HIP events on its own stream; durations are summed per (kernel, rows).
+struct TimerSlot;
+TimerSlot *timer_begin(int kernel_id, int rows, hipStream_t s);
+void timer_end(TimerSlot *slot, hipStream_t s);
+
+#define PP_TIMED(ID, ROWS, STREAM, ...)                                       \
+    do {                                                                      \
+        ::pp::TimerSlot *ts_ = ::pp::timer_begin((ID), (ROWS), (STREAM));     \
+        __VA_ARGS__;                                                          \
+        if (ts_) ::pp::timer_end(ts_, (STREAM));                              \
+    } while (0)
+
 // ---- device helpers ------------------------------------------------------------------
 template <typename T> __device__ __forceinline__ T dsqrt(T x);
 template <> __device__ __forceinline__ float dsqrt<float>(float x) { return sqrtf(x); }

@@ -579,29 +579,23 @@ __global__ void __launch_bounds__(SMALL_BLOCK) k_bicg_small(Csr<T> A, const T *_
 // ======================================================================================
 constexpr int BLOCK = 256;
 
-// Per-workgroup partials and fixed-order reduction by the last workgroup to finish.
+// Per-workgroup partial sums, written with plain stores. The one-workgroup finish kernel that
+// follows on the same stream combines them in a fixed order (deterministic); the kernel
+// boundary makes them visible across XCDs. (A last-workgroup-done scheme would need a
+// device-scope release fence per workgroup, which on gfx950 writes back the XCD's L2.)
 template <int NB, int NQ>
-__device__ __forceinline__ bool store_partials_last(double (&acc)[NB][NQ], int b0, int nb, double *partial,
-                                                    Control *ctrl, double *sm) {
+__device__ __forceinline__ void store_partials(double (&acc)[NB][NQ], int b0, int nb, double *partial, double *sm) {
     double flat[NB * NQ];
 #pragma unroll
     for (int k = 0; k < NB; ++k)
 #pragma unroll
         for (int q = 0; q < NQ; ++q) flat[k * NQ + q] = acc[k][q];
     block_sum<NB * NQ>(flat, sm);
-    __shared__ bool isLast;
     if (threadIdx.x == 0) {
         const int G = gridDim.x;
         for (int k = 0; k < nb; ++k)
             for (int q = 0; q < NQ; ++q) partial[((size_t)(b0 + k) * G + blockIdx.x) * NQ + q] = flat[k * NQ + q];
-        __threadfence();
-        const unsigned int total = gridDim.x * gridDim.y;
-        const unsigned int ticket = atomicAdd(&ctrl->counter, 1u);
-        isLast = ticket == total - 1;
     }
-    __syncthreads();
-    if (isLast) __threadfence();
-    return isLast;
 }
 
 template <int NQ>
@@ -609,7 +603,7 @@ __device__ __forceinline__ void reduce_batch(const double *partial, int b, int G
 #pragma unroll
     for (int q = 0; q < NQ; ++q) out[q] = 0.0;
     for (int g = threadIdx.x; g < G; g += blockDim.x) {
-        const volatile double *pp = partial + ((size_t)b * G + g) * NQ;
+        const double *pp = partial + ((size_t)b * G + g) * NQ;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) out[q] += pp[q];
     }
@@ -621,7 +615,6 @@ __device__ __forceinline__ void finish_last(BatchState *st, int B, Control *ctrl
         int nd = 0;
         for (int b = 0; b < B; ++b) nd += st[b].done != RUNNING;
         ctrl->allDone = nd == B;
-        ctrl->counter = 0;
     }
 }
 
@@ -656,6 +649,7 @@ __global__ void __launch_bounds__(BLOCK) k_residual(Csr<T> A, const T *__restric
                                                     T *__restrict__ r, T *__restrict__ r0, BatchState *st, int B,
                                                     double *partial, Control *ctrl, double normScale, int resetEvery) {
     __shared__ double sm[16 * NB * 2];
+    if (ctrl->allDone) return;
     const int n = A.n;
     const Group grp = batch_group<NB>(B, A.rpm);
     auto isActive = [&](const BatchState &s) {
@@ -692,29 +686,36 @@ __global__ void __launch_bounds__(BLOCK) k_residual(Csr<T> A, const T *__restric
                 }
         }
     }
-    if (store_partials_last<NB, 1>(acc, grp.b0, grp.nb, partial, ctrl, sm)) {
-        for (int bb = 0; bb < B; ++bb) {
-            if (!isActive(st[bb])) continue;
-            double red[1];
-            reduce_batch<1>(partial, bb, gridDim.x, red, sm);
-            if (threadIdx.x == 0) {
-                BatchState &s = st[bb];
-                s.rho = red[0];
-                s.restart = 1;
-                if (resetEvery <= 0) {
-                    const double crit = sqrt(red[0]) * normScale;
-                    s.crit = crit;
-                    if (!finite_d(crit)) { s.done = NONFINITE; s.usedIt = 0; }
-                    else if (r0 != nullptr ? crit < s.tol : crit == 0.0) {
-                        s.done = CONVERGED;
-                        s.usedIt = r0 != nullptr ? -1 : 0;
-                    }
+    store_partials<NB, 1>(acc, grp.b0, grp.nb, partial, sm);
+}
+
+// Finish of k_residual: rho, restart and (initial call) the converged-start test.
+__global__ void __launch_bounds__(BLOCK) k_fin_residual(BatchState *st, int B, const double *partial, int G,
+                                                        Control *ctrl, double normScale, int resetEvery, int bicg) {
+    __shared__ double sm[16];
+    if (ctrl->allDone) return;
+    for (int bb = 0; bb < B; ++bb) {
+        const BatchState &sb = st[bb];
+        if (!(sb.done == RUNNING && (resetEvery <= 0 || (sb.it > 0 && (sb.it % resetEvery) == 0)))) continue;
+        double red[1];
+        reduce_batch<1>(partial, bb, G, red, sm);
+        if (threadIdx.x == 0) {
+            BatchState &s = st[bb];
+            s.rho = red[0];
+            s.restart = 1;
+            if (resetEvery <= 0) {
+                const double crit = sqrt(red[0]) * normScale;
+                s.crit = crit;
+                if (!finite_d(crit)) { s.done = NONFINITE; s.usedIt = 0; }
+                else if (bicg ? crit < s.tol : crit == 0.0) {
+                    s.done = CONVERGED;
+                    s.usedIt = bicg ? -1 : 0;
                 }
             }
-            __syncthreads();
         }
-        finish_last(st, B, ctrl);
+        __syncthreads();
     }
+    finish_last(st, B, ctrl);
 }
 
 enum PrecondMode { PM_NONE = 0, PM_VECTOR = 2 };
@@ -726,6 +727,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_direction(Csr<T> A, const T *__res
                                                         T *__restrict__ q, BatchState *st, int B, double *partial,
                                                         Control *ctrl) {
     __shared__ double sm[16 * NB];
+    if (ctrl->allDone) return;
     const int n = A.n;
     const Group grp = batch_group<NB>(B, A.rpm);
     bool active[NB], restart[NB];
@@ -772,22 +774,28 @@ __global__ void __launch_bounds__(BLOCK) k_cg_direction(Csr<T> A, const T *__res
                 }
         }
     }
-    if (store_partials_last<NB, 1>(acc, grp.b0, grp.nb, partial, ctrl, sm)) {
-        for (int bb = 0; bb < B; ++bb) {
-            if (st[bb].done != RUNNING) continue;
-            double red[1];
-            reduce_batch<1>(partial, bb, gridDim.x, red, sm);
-            if (threadIdx.x == 0) {
-                BatchState &s = st[bb];
-                const double alpha = s.rho / red[0];
-                s.restart = 0;
-                if (!finite_d(alpha)) { s.done = NONFINITE; s.usedIt = s.it; }
-                else s.alpha = alpha;
-            }
-            __syncthreads();
+    store_partials<NB, 1>(acc, grp.b0, grp.nb, partial, sm);
+}
+
+// Finish of a direction phase (CG and BiCGStab's v = A p): alpha = rho / (d, A p).
+__global__ void __launch_bounds__(BLOCK) k_fin_alpha(BatchState *st, int B, const double *partial, int G,
+                                                     Control *ctrl) {
+    __shared__ double sm[16];
+    if (ctrl->allDone) return;
+    for (int bb = 0; bb < B; ++bb) {
+        if (st[bb].done != RUNNING) continue;
+        double red[1];
+        reduce_batch<1>(partial, bb, G, red, sm);
+        if (threadIdx.x == 0) {
+            BatchState &s = st[bb];
+            const double alpha = s.rho / red[0];
+            s.restart = 0;
+            if (!finite_d(alpha)) { s.done = NONFINITE; s.usedIt = s.it; }
+            else s.alpha = alpha;
         }
-        finish_last(st, B, ctrl);
+        __syncthreads();
     }
+    finish_last(st, B, ctrl);
 }
 
 // x += alpha p; r -= alpha q; (r, r). Last workgroup: convergence and beta.
@@ -797,6 +805,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update(int n, int rpm, T *__restri
                                                      BatchState *st, int B, double *partial, Control *ctrl,
                                                      double normScale, int returnBest) {
     __shared__ double sm[16 * NB];
+    if (ctrl->allDone) return;
     const Group grp = batch_group<NB>(B, rpm);
     double acc[NB][1];
 #pragma unroll
@@ -819,11 +828,20 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update(int n, int rpm, T *__restri
             acc[k][0] += double(ri) * double(ri);
         }
     }
-    if (store_partials_last<NB, 1>(acc, grp.b0, grp.nb, partial, ctrl, sm)) {
+    store_partials<NB, 1>(acc, grp.b0, grp.nb, partial, sm);
+}
+
+// Finish of k_cg_update: iteration count, convergence, best iterate, beta (unpreconditioned).
+template <int PM>
+__global__ void __launch_bounds__(BLOCK) k_fin_cg_update(BatchState *st, int B, const double *partial, int G,
+                                                         Control *ctrl, double normScale, int returnBest) {
+    __shared__ double sm[16];
+    if (ctrl->allDone) return;
+    {
         for (int bb = 0; bb < B; ++bb) {
             if (st[bb].done != RUNNING) continue;
             double red[1];
-            reduce_batch<1>(partial, bb, gridDim.x, red, sm);
+            reduce_batch<1>(partial, bb, G, red, sm);
             if (threadIdx.x == 0) {
                 BatchState &s = st[bb];
                 const double crit = sqrt(red[0]) * normScale;
@@ -860,16 +878,25 @@ template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_cg_dots(int n, const T *__restrict__ r, const T *__restrict__ z,
                                                    BatchState *st, int B, double *partial, Control *ctrl) {
     __shared__ double sm[16];
+    if (ctrl->allDone) return;
     const int b = blockIdx.y;
     double acc[1][1] = {{0.0}};
     if (st[b].done == RUNNING)
         for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK)
             acc[0][0] += double(r[(size_t)b * n + i]) * double(z[(size_t)b * n + i]);
-    if (store_partials_last<1, 1>(acc, b, 1, partial, ctrl, sm)) {
+    store_partials<1, 1>(acc, b, 1, partial, sm);
+}
+
+// Finish of k_cg_dots: rho = (r, z) and beta.
+__global__ void __launch_bounds__(BLOCK) k_fin_cg_dots(BatchState *st, int B, const double *partial, int G,
+                                                       Control *ctrl) {
+    __shared__ double sm[16];
+    if (ctrl->allDone) return;
+    {
         for (int bb = 0; bb < B; ++bb) {
             if (st[bb].done != RUNNING) continue;
             double red[1];
-            reduce_batch<1>(partial, bb, gridDim.x, red, sm);
+            reduce_batch<1>(partial, bb, G, red, sm);
             if (threadIdx.x == 0) {
                 BatchState &s = st[bb];
                 if (!s.restart) s.beta = red[0] / s.rho;
@@ -902,6 +929,7 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_v(Csr<T> A, const T *__restrict_
                                                   const T *__restrict__ r0, BatchState *st, int B, double *partial,
                                                   Control *ctrl) {
     __shared__ double sm[16 * NB];
+    if (ctrl->allDone) return;
     const int n = A.n;
     const Group grp = batch_group<NB>(B, A.rpm);
     bool active[NB], any = false;
@@ -933,22 +961,7 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_v(Csr<T> A, const T *__restrict_
                 }
         }
     }
-    if (store_partials_last<NB, 1>(acc, grp.b0, grp.nb, partial, ctrl, sm)) {
-        for (int bb = 0; bb < B; ++bb) {
-            if (st[bb].done != RUNNING) continue;
-            double red[1];
-            reduce_batch<1>(partial, bb, gridDim.x, red, sm);
-            if (threadIdx.x == 0) {
-                BatchState &s = st[bb];
-                const double alpha = s.rho / red[0];
-                s.restart = 0;
-                if (!finite_d(alpha)) { s.done = NONFINITE; s.usedIt = s.it; }
-                else s.alpha = alpha;
-            }
-            __syncthreads();
-        }
-        finish_last(st, B, ctrl);
-    }
+    store_partials<NB, 1>(acc, grp.b0, grp.nb, partial, sm);
 }
 
 template <typename T, int NB>
@@ -956,6 +969,7 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_t(Csr<T> A, const T *__restrict_
                                                   T *__restrict__ t, BatchState *st, int B, double *partial,
                                                   Control *ctrl, double normScale) {
     __shared__ double sm[16 * NB * 3];
+    if (ctrl->allDone) return;
     const int n = A.n;
     const Group grp = batch_group<NB>(B, A.rpm);
     bool active[NB], any = false;
@@ -998,11 +1012,19 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_t(Csr<T> A, const T *__restrict_
                 }
         }
     }
-    if (store_partials_last<NB, 3>(acc, grp.b0, grp.nb, partial, ctrl, sm)) {
+    store_partials<NB, 3>(acc, grp.b0, grp.nb, partial, sm);
+}
+
+// Finish of k_bicg_t: half-step convergence and omega.
+__global__ void __launch_bounds__(BLOCK) k_fin_bicg_t(BatchState *st, int B, const double *partial, int G,
+                                                      Control *ctrl, double normScale) {
+    __shared__ double sm[16 * 3];
+    if (ctrl->allDone) return;
+    {
         for (int bb = 0; bb < B; ++bb) {
             if (st[bb].done != RUNNING) continue;
             double red[3];
-            reduce_batch<3>(partial, bb, gridDim.x, red, sm);
+            reduce_batch<3>(partial, bb, G, red, sm);
             if (threadIdx.x == 0) {
                 BatchState &s = st[bb];
                 const double critS = sqrt(red[2]) * normScale;
@@ -1027,6 +1049,7 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_update(int n, int rpm, T *__rest
                                                        const T *__restrict__ t, T *__restrict__ x, BatchState *st,
                                                        int B, double *partial, Control *ctrl, double normScale) {
     __shared__ double sm[16 * NB * 2];
+    if (ctrl->allDone) return;
     const Group grp = batch_group<NB>(B, rpm);
     double acc[NB][2];
 #pragma unroll
@@ -1054,11 +1077,19 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_update(int n, int rpm, T *__rest
             }
         }
     }
-    if (store_partials_last<NB, 2>(acc, grp.b0, grp.nb, partial, ctrl, sm)) {
+    store_partials<NB, 2>(acc, grp.b0, grp.nb, partial, sm);
+}
+
+// Finish of k_bicg_update: convergence, iteration count and beta.
+__global__ void __launch_bounds__(BLOCK) k_fin_bicg_update(BatchState *st, int B, const double *partial, int G,
+                                                           Control *ctrl, double normScale) {
+    __shared__ double sm[16 * 2];
+    if (ctrl->allDone) return;
+    {
         for (int bb = 0; bb < B; ++bb) {
             if (st[bb].done != RUNNING) continue;
             double red[2];
-            reduce_batch<2>(partial, bb, gridDim.x, red, sm);
+            reduce_batch<2>(partial, bb, G, red, sm);
             if (threadIdx.x == 0) {
                 BatchState &s = st[bb];
                 if (s.halfDone) {
@@ -1101,18 +1132,24 @@ template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_mean(const T *__restrict__ v, int n, int B, double *means, double *partial,
                                                 Control *ctrl, const BatchState *st, int onlyRunning) {
     __shared__ double sm[16];
+    if (onlyRunning && ctrl->allDone) return;
     const int batch = blockIdx.y;
     double acc[1][1] = {{0.0}};
     if (!onlyRunning || st[batch].done == RUNNING)
         for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) acc[0][0] += double(v[(size_t)batch * n + i]);
-    if (store_partials_last<1, 1>(acc, batch, 1, partial, ctrl, sm)) {
-        for (int b = 0; b < B; ++b) {
-            double red[1];
-            reduce_batch<1>(partial, b, gridDim.x, red, sm);
-            if (threadIdx.x == 0) means[b] = red[0] / double(n);
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) ctrl->counter = 0;
+    store_partials<1, 1>(acc, batch, 1, partial, sm);
+}
+
+// Finish of k_mean: means[b] = sum / n.
+__global__ void __launch_bounds__(BLOCK) k_fin_mean(const double *partial, int G, int n, int B, double *means,
+                                                    const Control *ctrl, int onlyRunning) {
+    __shared__ double sm[16];
+    if (onlyRunning && ctrl->allDone) return;
+    for (int b = 0; b < B; ++b) {
+        double red[1];
+        reduce_batch<1>(partial, b, G, red, sm);
+        if (threadIdx.x == 0) means[b] = red[0] / double(n);
+        __syncthreads();
     }
 }
 
@@ -1353,16 +1390,18 @@ static int collect(Workspace &ws, int B, int returnBest, hipStream_t s, pp_solve
     return PP_OK;
 }
 
-// Host loop with lagged asynchronous polling of ctrl->allDone.
+// Host loop with lagged asynchronous polling of ctrl->allDone. Chunks of iterations grow
+// 0, 1, 2, 4, ... up to `check`: a solve that converges at once or in a few iterations costs
+// few extra launches, and those find allDone set and return at once.
 template <typename Body>
 static int run_loop(Workspace &ws, hipStream_t s, int maxLoops, int check, Body body) {
-    int chunk = 0;
+    int chunk = 0, size = 0;
     hipEvent_t ev[2];
     PP_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     PP_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     int rc = PP_OK;
-    for (int loops = 0; loops < maxLoops;) {
-        for (int k = 0; k < check && loops < maxLoops; ++k, ++loops) body(s);
+    for (int loops = 0; loops <= maxLoops;) {
+        for (int k = 0; k < size && loops < maxLoops; ++k, ++loops) body(s);
         if (hipGetLastError() != hipSuccess) { set_error("run_loop: kernel launch failed"); rc = PP_ERR_HIP; break; }
         if (hipMemcpyAsync(ws.hostFlags + (chunk & 1), &ctrl_of(ws)->allDone, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipEventRecord(ev[chunk & 1], s) != hipSuccess) { set_error("run_loop: poll failed"); rc = PP_ERR_HIP; break; }
@@ -1370,7 +1409,9 @@ static int run_loop(Workspace &ws, hipStream_t s, int maxLoops, int check, Body 
             if (hipEventSynchronize(ev[(chunk - 1) & 1]) != hipSuccess) { set_error("run_loop: sync failed"); rc = PP_ERR_HIP; break; }
             if (ws.hostFlags[(chunk - 1) & 1]) break;
         }
+        if (loops >= maxLoops) break;
         ++chunk;
+        size = size == 0 ? 1 : std::min(2 * size, std::max(1, check));
     }
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
@@ -1405,23 +1446,33 @@ static int cg_grid(const Csr<T> &A, const T *b, T *x, int B, int maxit, const do
     PP_HIP(hipMemcpyAsync(bw, b, sizeof(T) * (size_t)n * B, hipMemcpyDeviceToDevice, s));
     if (project) {
         hipLaunchKernelGGL((k_mean<T>), dim3(Gm, B), dim3(BLOCK), 0, s, bw, n, B, means, partial, ctrl, st, 0);
+        hipLaunchKernelGGL(k_fin_mean, dim3(1), dim3(BLOCK), 0, s, partial, Gm, n, B, means, ctrl, 0);
         hipLaunchKernelGGL((k_subtract_mean<T>), dim3(Gm, B), dim3(BLOCK), 0, s, bw, n, means, st, 0);
     }
     NB_DISPATCH(NBv,
         hipLaunchKernelGGL((k_residual<T, NB>), grid, dim3(BLOCK), 0, s, A, bw, x, r, (T *)nullptr, st, B, partial, ctrl,
                            normScale, 0));
+    hipLaunchKernelGGL(k_fin_residual, dim3(1), dim3(BLOCK), 0, s, st, B, partial, G, ctrl, normScale, 0, 0);
     PP_LAUNCH_CHECK();
     int parity = 0;
     auto body = [&](hipStream_t strm) {
         T *pIn = parity ? p1 : p0, *pOut = parity ? p0 : p1;
         NB_DISPATCH(NBv,
-            if (resetEvery > 0)
+            if (resetEvery > 0) {
                 hipLaunchKernelGGL((k_residual<T, NB>), grid, dim3(BLOCK), 0, strm, A, bw, x, r, (T *)nullptr, st, B, partial,
                                    ctrl, normScale, resetEvery);
-            hipLaunchKernelGGL((k_cg_direction<T, NB, PM_NONE>), grid, dim3(BLOCK), 0, strm, A, r, (const T *)nullptr, pIn,
-                               pOut, q, st, B, partial, ctrl);
-            hipLaunchKernelGGL((k_cg_update<T, NB, PM_NONE>), grid, dim3(BLOCK), 0, strm, n, A.rpm, r, pOut, q, x, xbest,
-                               st, B, partial, ctrl, normScale, returnBest));
+                hipLaunchKernelGGL(k_fin_residual, dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl, normScale,
+                                   resetEvery, 0);
+            }
+            PP_TIMED(PP_K_CG_DIRECTION, n, strm,
+                hipLaunchKernelGGL((k_cg_direction<T, NB, PM_NONE>), grid, dim3(BLOCK), 0, strm, A, r, (const T *)nullptr,
+                                   pIn, pOut, q, st, B, partial, ctrl));
+            hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl);
+            PP_TIMED(PP_K_CG_UPDATE, n, strm,
+                hipLaunchKernelGGL((k_cg_update<T, NB, PM_NONE>), grid, dim3(BLOCK), 0, strm, n, A.rpm, r, pOut, q, x, xbest,
+                                   st, B, partial, ctrl, normScale, returnBest));
+            hipLaunchKernelGGL((k_fin_cg_update<PM_NONE>), dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl,
+                               normScale, returnBest));
         parity ^= 1;
     };
     if (int e = run_loop(ws, s, maxit + 2, g_check_interval, body)) return e;
@@ -1457,14 +1508,22 @@ static int bicg_grid(const Csr<T> &A, const T *b, T *x, int B, int maxit, const 
     PP_HIP(hipMemsetAsync(v, 0, sizeof(T) * (size_t)n * B, s));
     NB_DISPATCH(NBv,
         hipLaunchKernelGGL((k_residual<T, NB>), grid, dim3(BLOCK), 0, s, A, b, x, r, r0, st, B, partial, ctrl, normScale, 0));
+    hipLaunchKernelGGL(k_fin_residual, dim3(1), dim3(BLOCK), 0, s, st, B, partial, G, ctrl, normScale, 0, 1);
     PP_LAUNCH_CHECK();
     auto body = [&](hipStream_t strm) {
-        hipLaunchKernelGGL((k_bicg_p<T>), gridP, dim3(BLOCK), 0, strm, n, r, p, v, st);
+        PP_TIMED(PP_K_BICG_P, n, strm, hipLaunchKernelGGL((k_bicg_p<T>), gridP, dim3(BLOCK), 0, strm, n, r, p, v, st));
         NB_DISPATCH(NBv,
-            hipLaunchKernelGGL((k_bicg_v<T, NB>), grid, dim3(BLOCK), 0, strm, A, p, v, r0, st, B, partial, ctrl);
-            hipLaunchKernelGGL((k_bicg_t<T, NB>), grid, dim3(BLOCK), 0, strm, A, r, v, t, st, B, partial, ctrl, normScale);
-            hipLaunchKernelGGL((k_bicg_update<T, NB>), grid, dim3(BLOCK), 0, strm, n, A.rpm, r, r0, p, v, t, x, st, B, partial,
-                               ctrl, normScale));
+            PP_TIMED(PP_K_BICG_V, n, strm,
+                hipLaunchKernelGGL((k_bicg_v<T, NB>), grid, dim3(BLOCK), 0, strm, A, p, v, r0, st, B, partial, ctrl));
+            hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl);
+            PP_TIMED(PP_K_BICG_T, n, strm,
+                hipLaunchKernelGGL((k_bicg_t<T, NB>), grid, dim3(BLOCK), 0, strm, A, r, v, t, st, B, partial, ctrl,
+                                   normScale));
+            hipLaunchKernelGGL(k_fin_bicg_t, dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl, normScale);
+            PP_TIMED(PP_K_BICG_UPDATE, n, strm,
+                hipLaunchKernelGGL((k_bicg_update<T, NB>), grid, dim3(BLOCK), 0, strm, n, A.rpm, r, r0, p, v, t, x, st, B,
+                                   partial, ctrl, normScale));
+            hipLaunchKernelGGL(k_fin_bicg_update, dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl, normScale));
     };
     if (int e = run_loop(ws, s, maxit + 2, g_check_interval, body)) return e;
     return collect(ws, B, 0, s, res);
@@ -1490,8 +1549,10 @@ static int cg_small(const Csr<T> &A, const T *b, T *x, int B, int maxit, const d
     const int numMat = (B + A.rpm - 1) / A.rpm;
     const int groups = numMat * gpm;
     NB_DISPATCH(NBv,
-        if (bicg) hipLaunchKernelGGL((k_bicg_small<T, NB>), dim3(groups), dim3(SMALL_BLOCK), 0, s, A, b, x, W, st, a);
-        else hipLaunchKernelGGL((k_cg_small<T, NB>), dim3(groups), dim3(SMALL_BLOCK), 0, s, A, b, x, W, st, a));
+        if (bicg) PP_TIMED(PP_K_BICG_SMALL, n, s,
+                           hipLaunchKernelGGL((k_bicg_small<T, NB>), dim3(groups), dim3(SMALL_BLOCK), 0, s, A, b, x, W, st, a));
+        else PP_TIMED(PP_K_CG_SMALL, n, s,
+                      hipLaunchKernelGGL((k_cg_small<T, NB>), dim3(groups), dim3(SMALL_BLOCK), 0, s, A, b, x, W, st, a)));
     PP_LAUNCH_CHECK();
     return collect(ws, B, returnBest, s, res);
 }
@@ -1535,9 +1596,11 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
     PP_HIP(hipMemcpyAsync(bw, b, sizeof(T) * (size_t)n * B, hipMemcpyDeviceToDevice, s));
     if (project) {
         hipLaunchKernelGGL((k_mean<T>), grid, dim3(BLOCK), 0, s, bw, n, B, meanR, partial, ctrl, st, 1);
+        hipLaunchKernelGGL(k_fin_mean, dim3(1), dim3(BLOCK), 0, s, partial, G, n, B, meanR, ctrl, 1);
         hipLaunchKernelGGL((k_subtract_mean<T>), grid, dim3(BLOCK), 0, s, bw, n, meanR, st, 1);
     }
     hipLaunchKernelGGL((k_residual<T, 1>), grid, dim3(BLOCK), 0, s, Af, bw, x, r, (T *)nullptr, st, B, partial, ctrl, normScale, 0);
+    hipLaunchKernelGGL(k_fin_residual, dim3(1), dim3(BLOCK), 0, s, st, B, partial, G, ctrl, normScale, 0, 0);
     PP_LAUNCH_CHECK();
     const int cus = num_cus();
     auto lgrid = [&](int rows, int W) {
@@ -1557,7 +1620,9 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
     auto vcycle = [&](hipStream_t strm) {
         const double *shift = nullptr;
         if (project) {
-            hipLaunchKernelGGL((k_mean<T>), grid, dim3(BLOCK), 0, strm, r, n, B, meanR, partial, ctrl, st, 1);
+            PP_TIMED(PP_K_MEAN, n, strm,
+                hipLaunchKernelGGL((k_mean<T>), grid, dim3(BLOCK), 0, strm, r, n, B, meanR, partial, ctrl, st, 1));
+            hipLaunchKernelGGL(k_fin_mean, dim3(1), dim3(BLOCK), 0, strm, partial, G, n, B, meanR, ctrl, 1);
             shift = meanR;
         }
         std::vector<H *> cur(L), oth(L);
@@ -1568,8 +1633,8 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
             H *resl = lres(l);
             const H *dinv = reinterpret_cast<const H *>(lv[l].dinv);
             W_DISPATCH(wa[l],
-                if (l == 0) hipLaunchKernelGGL((k_presmooth_residual<W, T, H>), g, dim3(BLOCK), 0, strm, LA[l], dinv, r, shift, cur[l], resl, omega, st);
-                else hipLaunchKernelGGL((k_presmooth_residual<W, H, H>), g, dim3(BLOCK), 0, strm, LA[l], dinv, lb(l), (const double *)nullptr, cur[l], resl, omega, st);
+                if (l == 0) PP_TIMED(PP_K_AMG_PRESMOOTH, lv[l].n, strm, hipLaunchKernelGGL((k_presmooth_residual<W, T, H>), g, dim3(BLOCK), 0, strm, LA[l], dinv, r, shift, cur[l], resl, omega, st));
+                else PP_TIMED(PP_K_AMG_PRESMOOTH, lv[l].n, strm, hipLaunchKernelGGL((k_presmooth_residual<W, H, H>), g, dim3(BLOCK), 0, strm, LA[l], dinv, lb(l), (const double *)nullptr, cur[l], resl, omega, st));
                 for (int k = 1; k < pre; ++k) {
                     if (l == 0) hipLaunchKernelGGL((k_jacobi_sweep<W, T, H, H>), g, dim3(BLOCK), 0, strm, LA[l], dinv, r, shift, cur[l], oth[l], omega, st);
                     else hipLaunchKernelGGL((k_jacobi_sweep<W, H, H, H>), g, dim3(BLOCK), 0, strm, LA[l], dinv, lb(l), (const double *)nullptr, cur[l], oth[l], omega, st);
@@ -1580,40 +1645,51 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
                     else hipLaunchKernelGGL((k_level_residual<W, H, H>), g, dim3(BLOCK), 0, strm, LA[l], lb(l), (const double *)nullptr, cur[l], resl, st);
                 });
             W_DISPATCH(wr[l],
-                hipLaunchKernelGGL((k_spmv_rect<W, H>), lgrid(lv[l].n_coarse, wr[l]), dim3(BLOCK), 0, strm, LR[l], resl, lv[l].n, lb(l + 1), 0, st));
+                PP_TIMED(PP_K_AMG_RESTRICT, lv[l].n_coarse, strm, hipLaunchKernelGGL((k_spmv_rect<W, H>), lgrid(lv[l].n_coarse, wr[l]), dim3(BLOCK), 0, strm, LR[l], resl, lv[l].n, lb(l + 1), 0, st)));
         }
-        hipLaunchKernelGGL((k_coarse_solve<H>), dim3((nc + 127) / 128, B), dim3(128), 0, strm, pinv, nc, lb(L), cx, st, coarseBatched);
+        PP_TIMED(PP_K_AMG_COARSE, nc, strm, hipLaunchKernelGGL((k_coarse_solve<H>), dim3((nc + 127) / 128, B), dim3(128), 0, strm, pinv, nc, lb(L), cx, st, coarseBatched));
         for (int l = L - 1; l >= 0; --l) {
             const dim3 g = lgrid(lv[l].n, wa[l]);
             const H *xc = l + 1 < L ? cur[l + 1] : cx;
             const int ncoarse = l + 1 < L ? lv[l + 1].n : nc;
             const H *dinv = reinterpret_cast<const H *>(lv[l].dinv);
             W_DISPATCH(wp[l],
-                hipLaunchKernelGGL((k_spmv_rect<W, H>), lgrid(lv[l].n, wp[l]), dim3(BLOCK), 0, strm, LP[l], xc, ncoarse, cur[l], 1, st));
+                PP_TIMED(PP_K_AMG_PROLONG, lv[l].n, strm, hipLaunchKernelGGL((k_spmv_rect<W, H>), lgrid(lv[l].n, wp[l]), dim3(BLOCK), 0, strm, LP[l], xc, ncoarse, cur[l], 1, st)));
             W_DISPATCH(wa[l],
                 for (int k = 0; k < post; ++k) {
                     if (l == 0) {
-                        if (k == post - 1) hipLaunchKernelGGL((k_jacobi_sweep<W, T, H, T>), g, dim3(BLOCK), 0, strm, LA[l], dinv, r, shift, cur[l], u, omega, st);
-                        else hipLaunchKernelGGL((k_jacobi_sweep<W, T, H, H>), g, dim3(BLOCK), 0, strm, LA[l], dinv, r, shift, cur[l], oth[l], omega, st);
+                        if (k == post - 1) PP_TIMED(PP_K_AMG_POST_JACOBI, lv[l].n, strm, hipLaunchKernelGGL((k_jacobi_sweep<W, T, H, T>), g, dim3(BLOCK), 0, strm, LA[l], dinv, r, shift, cur[l], u, omega, st));
+                        else PP_TIMED(PP_K_AMG_JACOBI, lv[l].n, strm, hipLaunchKernelGGL((k_jacobi_sweep<W, T, H, H>), g, dim3(BLOCK), 0, strm, LA[l], dinv, r, shift, cur[l], oth[l], omega, st));
                     } else {
-                        hipLaunchKernelGGL((k_jacobi_sweep<W, H, H, H>), g, dim3(BLOCK), 0, strm, LA[l], dinv, lb(l), (const double *)nullptr, cur[l], oth[l], omega, st);
+                        PP_TIMED(PP_K_AMG_POST_JACOBI, lv[l].n, strm, hipLaunchKernelGGL((k_jacobi_sweep<W, H, H, H>), g, dim3(BLOCK), 0, strm, LA[l], dinv, lb(l), (const double *)nullptr, cur[l], oth[l], omega, st));
                     }
                     std::swap(cur[l], oth[l]);
                 });
         }
         if (project) {
-            hipLaunchKernelGGL((k_mean<T>), grid, dim3(BLOCK), 0, strm, u, n, B, meanU, partial, ctrl, st, 1);
-            hipLaunchKernelGGL((k_subtract_mean<T>), grid, dim3(BLOCK), 0, strm, u, n, meanU, st, 1);
+            PP_TIMED(PP_K_MEAN, n, strm,
+                hipLaunchKernelGGL((k_mean<T>), grid, dim3(BLOCK), 0, strm, u, n, B, meanU, partial, ctrl, st, 1));
+            hipLaunchKernelGGL(k_fin_mean, dim3(1), dim3(BLOCK), 0, strm, partial, G, n, B, meanU, ctrl, 1);
+            PP_TIMED(PP_K_SUBTRACT_MEAN, n, strm,
+                hipLaunchKernelGGL((k_subtract_mean<T>), grid, dim3(BLOCK), 0, strm, u, n, meanU, st, 1));
         }
     };
     int parity = 0;
     auto body = [&](hipStream_t strm) {
         T *pIn = parity ? p1 : p0, *pOut = parity ? p0 : p1;
         vcycle(strm);
-        hipLaunchKernelGGL((k_cg_dots<T>), grid, dim3(BLOCK), 0, strm, n, r, u, st, B, partial, ctrl);
-        hipLaunchKernelGGL((k_cg_direction<T, 1, PM_VECTOR>), grid, dim3(BLOCK), 0, strm, Af, r, u, pIn, pOut, q, st, B, partial, ctrl);
-        hipLaunchKernelGGL((k_cg_update<T, 1, PM_VECTOR>), grid, dim3(BLOCK), 0, strm, n, 1, r, pOut, q, x, xbest, st, B, partial, ctrl,
-                           normScale, returnBest);
+        PP_TIMED(PP_K_PCG_DOTS, n, strm,
+            hipLaunchKernelGGL((k_cg_dots<T>), grid, dim3(BLOCK), 0, strm, n, r, u, st, B, partial, ctrl));
+        hipLaunchKernelGGL(k_fin_cg_dots, dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl);
+        PP_TIMED(PP_K_PCG_DIRECTION, n, strm,
+            hipLaunchKernelGGL((k_cg_direction<T, 1, PM_VECTOR>), grid, dim3(BLOCK), 0, strm, Af, r, u, pIn, pOut, q, st, B,
+                               partial, ctrl));
+        hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl);
+        PP_TIMED(PP_K_PCG_UPDATE, n, strm,
+            hipLaunchKernelGGL((k_cg_update<T, 1, PM_VECTOR>), grid, dim3(BLOCK), 0, strm, n, 1, r, pOut, q, x, xbest, st, B,
+                               partial, ctrl, normScale, returnBest));
+        hipLaunchKernelGGL((k_fin_cg_update<PM_VECTOR>), dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl, normScale,
+                           returnBest);
         parity ^= 1;
     };
     if (int e = run_loop(ws, s, maxit + 2, std::max(1, g_check_interval / 4), body)) return e;

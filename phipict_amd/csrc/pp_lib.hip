@@ -1,10 +1,84 @@
-// Library-wide entry points: error reporting, version, device synchronisation.
+// Library-wide entry points: error reporting, version, device synchronisation and the
+// sampled kernel timer used for live roofline measurements.
 #include "pp_common.hpp"
+
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
 
 namespace pp {
 static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
 const char *get_error() { return g_last_error.c_str(); }
+
+struct TimerSlot {
+    hipEvent_t start = nullptr, stop = nullptr;
+    int kernel = 0, rows = 0;
+};
+
+struct TimerStats {
+    double total_ms = 0.0;
+    int64_t sampled = 0, launches = 0;
+};
+
+static struct {
+    std::mutex mu;
+    bool enabled = false;
+    int every = 1;
+    std::map<std::pair<int, int>, TimerStats> stats;
+    std::vector<TimerSlot *> pending, pool;
+} g_timer;
+
+TimerSlot *timer_begin(int kernel_id, int rows, hipStream_t s) {
+    if (!g_timer.enabled) return nullptr;
+    std::lock_guard<std::mutex> lock(g_timer.mu);
+    TimerStats &st = g_timer.stats[{kernel_id, rows}];
+    const bool sample = (st.launches++ % g_timer.every) == 0;
+    if (!sample) return nullptr;
+    TimerSlot *slot;
+    if (!g_timer.pool.empty()) {
+        slot = g_timer.pool.back();
+        g_timer.pool.pop_back();
+    } else {
+        slot = new TimerSlot;
+        if (hipEventCreate(&slot->start) != hipSuccess || hipEventCreate(&slot->stop) != hipSuccess) {
+            delete slot;
+            return nullptr;
+        }
+    }
+    slot->kernel = kernel_id;
+    slot->rows = rows;
+    if (hipEventRecord(slot->start, s) != hipSuccess) {
+        g_timer.pool.push_back(slot);
+        return nullptr;
+    }
+    return slot;
+}
+
+void timer_end(TimerSlot *slot, hipStream_t s) {
+    std::lock_guard<std::mutex> lock(g_timer.mu);
+    if (hipEventRecord(slot->stop, s) != hipSuccess) {
+        g_timer.pool.push_back(slot);
+        return;
+    }
+    g_timer.pending.push_back(slot);
+}
+
+// Fold completed samples into the statistics (blocks until every recorded stop event is done).
+static int timer_drain() {
+    for (TimerSlot *slot : g_timer.pending) {
+        PP_HIP(hipEventSynchronize(slot->stop));
+        float ms = 0.0f;
+        PP_HIP(hipEventElapsedTime(&ms, slot->start, slot->stop));
+        TimerStats &st = g_timer.stats[{slot->kernel, slot->rows}];
+        st.total_ms += ms;
+        st.sampled += 1;
+        g_timer.pool.push_back(slot);
+    }
+    g_timer.pending.clear();
+    return PP_OK;
+}
 } // namespace pp
 
 extern "C" {
@@ -12,6 +86,42 @@ const char *pp_last_error(void) { return pp::get_error(); }
 int pp_version(void) { return 1; }
 int pp_device_sync(void) {
     PP_HIP(hipDeviceSynchronize());
+    return PP_OK;
+}
+
+int pp_kernel_timer_start(int32_t sample_every) {
+    PP_CHECK_ARG(sample_every >= 1, "sample_every must be >= 1");
+    std::lock_guard<std::mutex> lock(pp::g_timer.mu);
+    if (int e = pp::timer_drain()) return e;
+    pp::g_timer.stats.clear();
+    pp::g_timer.every = sample_every;
+    pp::g_timer.enabled = true;
+    return PP_OK;
+}
+
+int pp_kernel_timer_stop(void) {
+    std::lock_guard<std::mutex> lock(pp::g_timer.mu);
+    pp::g_timer.enabled = false;
+    return pp::timer_drain();
+}
+
+int pp_kernel_timer_read(int32_t capacity, int32_t *kernel_ids, int32_t *rows, double *total_ms, int64_t *sampled,
+                         int64_t *launches, int32_t *count) {
+    PP_CHECK_ARG(count != nullptr && capacity >= 0, "bad arguments");
+    std::lock_guard<std::mutex> lock(pp::g_timer.mu);
+    if (int e = pp::timer_drain()) return e;
+    int i = 0;
+    for (const auto &kv : pp::g_timer.stats) {
+        if (i < capacity) {
+            if (kernel_ids) kernel_ids[i] = kv.first.first;
+            if (rows) rows[i] = kv.first.second;
+            if (total_ms) total_ms[i] = kv.second.total_ms;
+            if (sampled) sampled[i] = kv.second.sampled;
+            if (launches) launches[i] = kv.second.launches;
+        }
+        ++i;
+    }
+    *count = i;
     return PP_OK;
 }
 }

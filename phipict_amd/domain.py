@@ -213,6 +213,7 @@ class FixedBoundary(Boundary):
                 raise ValueError(f"varying boundary velocity must have spatial shape {expect}")
         self.velocity = velocity.to(device=self._domain.getDevice(), dtype=self._domain.getDtype()).contiguous()
         self._domain._changed = True
+        self._domain._bc_generation += 1
 
     def isVelocityStatic(self) -> bool:
         return self.velocity.dim() == 2
@@ -243,6 +244,7 @@ class FixedBoundary(Boundary):
     def setTransform(self, t: Optional[torch.Tensor]) -> None:
         self.transform = t
         self._domain._changed = True
+        self._domain._bc_generation += 1
 
     def hasTransform(self) -> bool:
         return self.transform is not None
@@ -549,6 +551,7 @@ class Block:
     def _setBoundary(self, face, boundary: Boundary) -> None:
         self.boundaries[face_index(face)] = boundary
         self._domain._changed = True
+        self._domain._bc_generation += 1
 
     def _make_fixed(self, face: int, velocity, vtype) -> FixedBoundary:
         fb = FixedBoundary(self._domain, self, face, velocity, vtype)
@@ -688,6 +691,12 @@ class Domain:
         self._time_step_host: Optional[torch.Tensor] = None
         self._atlas: Optional[torch.Tensor] = None
         self._atlas_bytes: Optional[bytes] = None
+        # device copies of the block descriptors: a ring of slots filled by asynchronous copies
+        # from pinned staging buffers, so that a changed descriptor never waits for the GPU
+        self._atlas_ring: Optional[list] = None
+        self._atlas_slot = -1
+        self._flux_cache: Optional[tuple] = None
+        self._bc_generation = 0  # bumped by every change of a boundary object, velocity or transform
         self._desc = _lib.DomainDesc()
         # bumped by every assembly of the advection matrix: the pressure operator depends on
         # the advection diagonal only, so solvers key their pressure caches on it
@@ -941,11 +950,7 @@ class Domain:
         arr = (_lib.BlockDesc * len(self._blocks))(*[self._block_desc(b) for b in self._blocks])
         raw = bytes(arr)
         if raw != self._atlas_bytes:
-            host = torch.frombuffer(bytearray(raw), dtype=torch.uint8)
-            if self._atlas is not None and self._atlas.numel() == host.numel():
-                self._atlas.copy_(host)
-            else:
-                self._atlas = host.to(self._device)
+            self._upload_atlas(raw)
             self._atlas_bytes = raw
         d = self._desc
         d.dims = self._dims
@@ -970,6 +975,40 @@ class Domain:
         d.E = self.Epot.desc() if self.Epot is not None else _lib.CsrDesc()
         self._changed = False
 
+    ATLAS_RING = 16
+
+    def _upload_atlas(self, raw: bytes) -> None:
+        """Copy the packed block descriptors to the next device slot of the ring. The copy is
+        asynchronous on the current stream; a slot is reused only after the work that was
+        enqueued while it was current has finished (an event recorded when it was retired)."""
+        import numpy as np
+
+        size = len(raw)
+        if self._device.type != "cuda":
+            self._atlas = torch.frombuffer(bytearray(raw), dtype=torch.uint8).clone()
+            return
+        stream = torch.cuda.current_stream(self._device)
+        if self._atlas_ring is None or self._atlas_ring[0][0].numel() != size:
+            if self._atlas_ring is not None:
+                torch.cuda.synchronize(self._device)
+            self._atlas_ring = [[torch.empty(size, dtype=torch.uint8, pin_memory=True),
+                                 torch.empty(size, dtype=torch.uint8, device=self._device), None]
+                                for _ in range(self.ATLAS_RING)]
+            self._atlas_slot = -1
+        if self._atlas_slot >= 0:
+            retired = self._atlas_ring[self._atlas_slot]
+            if retired[2] is None:
+                retired[2] = torch.cuda.Event()
+            retired[2].record(stream)
+        slot = (self._atlas_slot + 1) % self.ATLAS_RING
+        host, dev, ev = self._atlas_ring[slot]
+        if ev is not None:
+            ev.synchronize()
+        host.numpy()[:] = np.frombuffer(raw, dtype=np.uint8)
+        dev.copy_(host, non_blocking=True)
+        self._atlas = dev
+        self._atlas_slot = slot
+
     def IsTensorChanged(self) -> bool:
         return self._changed
 
@@ -982,12 +1021,22 @@ class Domain:
     def GetBoundaryFluxBalance(self) -> torch.Tensor:
         """Net outward flux through all prescribed faces, one value per environment
         (Domain::GetGlobalFluxBalance, domain.cpp:296-331)."""
+        # cached while no boundary was replaced or written (in-place writes bump _version)
+        key = [self._batch, self._bc_generation, len(self._blocks)]
+        for b in self._blocks:
+            for f, bound in b.getFixedBoundaries():
+                t = bound.transform
+                key.append((f, bound.velocity._version, int(bound.velocityType), None if t is None else t._version))
+        key = tuple(key)
+        if self._flux_cache is not None and self._flux_cache[0] == key:
+            return self._flux_cache[1].clone()
         total = torch.zeros(self._batch, dtype=torch.float64)
         for b in self._blocks:
             for f, bound in b.getFixedBoundaries():
                 sign = 1.0 if (f & 1) else -1.0
                 fl = bound.GetFluxes().to(torch.float64).sum(dim=-1).cpu()
                 total += sign * (fl if fl.numel() == self._batch else fl.expand(self._batch))
+        self._flux_cache = (key, total.clone())
         return total
 
     def isOrthogonal(self) -> bool:

@@ -42,9 +42,13 @@ def run_pair(specs, dtype, steps, nu, dt, mhd=None, amg=False, tol=None, pot_amg
     base = [None if b.velocitySource is None else b.velocitySource.clone() for b in dom.getBlocks()]
 
     def reset_source(domain, **kw):
+        # the Lorentz force is added to the current source, so a driving source is reset before
+        # every step (examples/hartmann.py:60-65); blocks without one start from none
         for b, s in zip(domain.getBlocks(), base):
             if s is not None:
                 b.setVelocitySource(s.clone())
+            else:
+                b.clearVelocitySource()
         domain.UpdateDomainData()
 
     common = dict(domain=dom, dt=dt, non_orthogonal=False, advection_tol=tol, pressure_tol=tol,

