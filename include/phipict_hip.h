@@ -285,6 +285,19 @@ int pp_segmented_gather(const int32_t *seg, const int32_t *src, const void *w, i
 int pp_galerkin_rap(const int32_t *r_row, const int32_t *r_col, const double *r_val, const pp_csr *A, int32_t a_dtype,
                     const int32_t *p_row, const int32_t *p_col, const double *p_val, const int32_t *c_row,
                     const int32_t *c_col, void *c_val, int32_t c_n, int32_t c_nnz, int32_t c_dtype, void *stream);
+/* Galerkin map (the precomputed product map of _galerkin_map, src/phipict/solvers/amg.py:
+ * 1114-1180, used here for every refresh): host construction for fixed R, P, fine pattern A
+ * and coarse pattern C -- c[e] = sum_{t in seg(e)} w[t] * a[src[t]], seg [nnz_c+1] -- and its
+ * device application in double accumulation (a: nmat value arrays of stride a_stride). */
+int pp_galerkin_map_build(int32_t n, const int32_t *a_row, const int32_t *a_col, int32_t nc, const int32_t *p_row,
+                          const int32_t *p_col, const double *p_val, const int32_t *r_row, const int32_t *r_col,
+                          const double *r_val, const int32_t *c_row, const int32_t *c_col, void **handle);
+int pp_galerkin_map_size(void *handle, int64_t *nterms);
+int pp_galerkin_map_copy(void *handle, int32_t *seg, int32_t *src, double *w);
+int pp_galerkin_map_free(void *handle);
+int pp_galerkin_apply(const int32_t *seg, const int32_t *src, const double *w, int32_t n_out, const void *a_val,
+                      int32_t a_dtype, void *c_val, int32_t c_dtype, int32_t nmat, int32_t a_stride, int32_t c_stride,
+                      void *stream);
 /* inverse diagonal of a CSR matrix (rows without a usable diagonal get 1):
  * dinv [num_matrices][n] in dtype_out from values in dtype. */
 int pp_inverse_diagonal(const pp_csr *A, int32_t dtype, void *dinv, int32_t dtype_out, void *stream);

@@ -357,6 +357,98 @@ int pp_amg_level_copy(void *handle, int32_t level, int32_t *a_row, int32_t *a_co
     return PP_OK;
 }
 
+// ---- Galerkin product map ----------------------------------------------------------
+// For fixed R, P and fine/coarse patterns, C = R A P is linear in A's values:
+// C[e] = sum_t w[t] * A[src[t]] with w = R(I,i) P(k,J) for every path I -> i -> k -> J
+// (A(i,k) at position src). Terms of an entry are kept in traversal order (i ascending in
+// R's row, then A's row, then P's row), so every refresh sums them in the same order.
+struct GalerkinMap {
+    std::vector<int32_t> seg, src;
+    std::vector<double> w;
+};
+
+int pp_galerkin_map_build(int32_t n, const int32_t *a_row, const int32_t *a_col, int32_t nc, const int32_t *p_row,
+                          const int32_t *p_col, const double *p_val, const int32_t *r_row, const int32_t *r_col,
+                          const double *r_val, const int32_t *c_row, const int32_t *c_col, void **handle) {
+    if (!a_row || !a_col || !p_row || !p_col || !p_val || !r_row || !r_col || !r_val || !c_row || !c_col || !handle ||
+        n < 0 || nc < 0) {
+        pp::set_error("pp_galerkin_map_build: null argument");
+        return PP_ERR_ARG;
+    }
+    auto m = std::make_unique<GalerkinMap>();
+    const int32_t nnzc = c_row[nc];
+    m->seg.assign((size_t)nnzc + 1, 0);
+    std::vector<int32_t> pos((size_t)nc, -1);
+    // pass 1: terms per coarse entry
+    for (int I = 0; I < nc; ++I) {
+        for (int e = c_row[I]; e < c_row[I + 1]; ++e) pos[c_col[e]] = e;
+        for (int ri = r_row[I]; ri < r_row[I + 1]; ++ri) {
+            const int i = r_col[ri];
+            if (i < 0 || i >= n) { pp::set_error("pp_galerkin_map_build: R column out of range"); return PP_ERR_ARG; }
+            for (int a = a_row[i]; a < a_row[i + 1]; ++a) {
+                const int k = a_col[a];
+                for (int pk = p_row[k]; pk < p_row[k + 1]; ++pk) {
+                    const int e = pos[p_col[pk]];
+                    if (e < 0) {
+                        pp::set_error("pp_galerkin_map_build: coarse pattern misses an entry of R A P");
+                        return PP_ERR_ARG;
+                    }
+                    ++m->seg[(size_t)e + 1];
+                }
+            }
+        }
+        for (int e = c_row[I]; e < c_row[I + 1]; ++e) pos[c_col[e]] = -1;
+    }
+    for (size_t e = 0; e < (size_t)nnzc; ++e) {
+        m->seg[e + 1] += m->seg[e];
+        if (m->seg[e + 1] < 0) { pp::set_error("pp_galerkin_map_build: more than 2^31 terms"); return PP_ERR_ARG; }
+    }
+    const size_t nt = (size_t)m->seg[nnzc];
+    m->src.resize(nt);
+    m->w.resize(nt);
+    std::vector<int32_t> fill(m->seg.begin(), m->seg.end() - 1);
+    // pass 2: the terms
+    for (int I = 0; I < nc; ++I) {
+        for (int e = c_row[I]; e < c_row[I + 1]; ++e) pos[c_col[e]] = e;
+        for (int ri = r_row[I]; ri < r_row[I + 1]; ++ri) {
+            const int i = r_col[ri];
+            const double rv = r_val[ri];
+            for (int a = a_row[i]; a < a_row[i + 1]; ++a) {
+                const int k = a_col[a];
+                for (int pk = p_row[k]; pk < p_row[k + 1]; ++pk) {
+                    const int e = pos[p_col[pk]];
+                    const int t = fill[e]++;
+                    m->src[t] = a;
+                    m->w[t] = rv * p_val[pk];
+                }
+            }
+        }
+        for (int e = c_row[I]; e < c_row[I + 1]; ++e) pos[c_col[e]] = -1;
+    }
+    *handle = m.release();
+    return PP_OK;
+}
+
+int pp_galerkin_map_size(void *handle, int64_t *nterms) {
+    if (!handle || !nterms) { pp::set_error("pp_galerkin_map_size: null argument"); return PP_ERR_ARG; }
+    *nterms = (int64_t)static_cast<GalerkinMap *>(handle)->src.size();
+    return PP_OK;
+}
+
+int pp_galerkin_map_copy(void *handle, int32_t *seg, int32_t *src, double *w) {
+    if (!handle || !seg || !src || !w) { pp::set_error("pp_galerkin_map_copy: null argument"); return PP_ERR_ARG; }
+    const GalerkinMap *m = static_cast<GalerkinMap *>(handle);
+    std::memcpy(seg, m->seg.data(), m->seg.size() * sizeof(int32_t));
+    std::memcpy(src, m->src.data(), m->src.size() * sizeof(int32_t));
+    std::memcpy(w, m->w.data(), m->w.size() * sizeof(double));
+    return PP_OK;
+}
+
+int pp_galerkin_map_free(void *handle) {
+    delete static_cast<GalerkinMap *>(handle);
+    return PP_OK;
+}
+
 int pp_amg_free(void *handle) {
     delete static_cast<Hierarchy *>(handle);
     return PP_OK;

@@ -1753,6 +1753,19 @@ __global__ void k_segmented_gather(const int32_t *__restrict__ seg, const int32_
 }
 
 
+// out[e] = sum_{t in seg(e)} w[t] * in[src[t]] accumulated in double (Galerkin map).
+template <typename TA, typename TO>
+__global__ void k_galerkin_apply(const int32_t *__restrict__ seg, const int32_t *__restrict__ src,
+                                 const double *__restrict__ w, int nOut, const TA *__restrict__ in,
+                                 TO *__restrict__ out, int inStride, int outStride) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nOut) return;
+    const TA *vin = in + (size_t)blockIdx.y * inStride;
+    double acc = 0.0;
+    for (int t = seg[e]; t < seg[e + 1]; ++t) acc += w[t] * double(vin[src[t]]);
+    out[(size_t)blockIdx.y * outStride + e] = TO(acc);
+}
+
 // Galerkin product C = R A P into the fixed pattern of C (one thread per coarse row,
 // accumulation in double; A may hold nmat value arrays of one pattern).
 template <typename TA, typename TO>
@@ -1979,6 +1992,26 @@ int pp_segmented_gather(const int32_t *seg, const int32_t *src, const void *w, i
     return PP_OK;
 }
 
+
+int pp_galerkin_apply(const int32_t *seg, const int32_t *src, const double *w, int32_t n_out, const void *a_val,
+                      int32_t a_dtype, void *c_val, int32_t c_dtype, int32_t nmat, int32_t a_stride, int32_t c_stride,
+                      void *stream) {
+    PP_CHECK_ARG(seg && src && w && a_val && c_val && n_out >= 0 && nmat >= 1, "null argument");
+    if (n_out == 0) return PP_OK;
+    const dim3 g((n_out + 255) / 256, nmat);
+    hipStream_t s = as_stream(stream);
+    if (a_dtype == PP_F64 && c_dtype == PP_F64)
+        PP_TIMED(PP_K_GALERKIN, n_out, s, hipLaunchKernelGGL((k_galerkin_apply<double, double>), g, dim3(256), 0, s, seg, src, w, n_out, (const double *)a_val, (double *)c_val, a_stride, c_stride));
+    else if (a_dtype == PP_F64 && c_dtype == PP_F32)
+        PP_TIMED(PP_K_GALERKIN, n_out, s, hipLaunchKernelGGL((k_galerkin_apply<double, float>), g, dim3(256), 0, s, seg, src, w, n_out, (const double *)a_val, (float *)c_val, a_stride, c_stride));
+    else if (a_dtype == PP_F32 && c_dtype == PP_F32)
+        PP_TIMED(PP_K_GALERKIN, n_out, s, hipLaunchKernelGGL((k_galerkin_apply<float, float>), g, dim3(256), 0, s, seg, src, w, n_out, (const float *)a_val, (float *)c_val, a_stride, c_stride));
+    else if (a_dtype == PP_F32 && c_dtype == PP_F64)
+        PP_TIMED(PP_K_GALERKIN, n_out, s, hipLaunchKernelGGL((k_galerkin_apply<float, double>), g, dim3(256), 0, s, seg, src, w, n_out, (const float *)a_val, (double *)c_val, a_stride, c_stride));
+    else { set_error("pp_galerkin_apply: dtype"); return PP_ERR_ARG; }
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
 
 int pp_galerkin_rap(const int32_t *r_row, const int32_t *r_col, const double *r_val, const pp_csr *A, int32_t a_dtype,
                     const int32_t *p_row, const int32_t *p_col, const double *p_val, const int32_t *c_row,

@@ -46,6 +46,8 @@ class Level:
     P_row: Optional[torch.Tensor] = None
     P_col: Optional[torch.Tensor] = None
     P_val: Optional[torch.Tensor] = None  # float64
+    # map from this level's operator values to the next level's (seg, src, w): Galerkin refresh
+    gmap: Optional[tuple] = None
     native: dict = field(default_factory=dict)
 
 
@@ -158,6 +160,28 @@ def rs_setup_host(row: np.ndarray, col: np.ndarray, val: np.ndarray, theta: floa
                         np.ascontiguousarray(val, np.float64), theta, max_levels, max_coarse).levels()
 
 
+def galerkin_map_host(fine_row, fine_col, P, R, coarse_row, coarse_col) -> tuple:
+    """Host Galerkin map of one level: (seg, src, w) with C[e] = sum_t w[t] A[src[t]]."""
+    pr, pc, rr, rc = (np.ascontiguousarray(a, np.int32) for a in (P[0], P[1], R[0], R[1]))
+    pv, rv = np.ascontiguousarray(P[2], np.float64), np.ascontiguousarray(R[2], np.float64)
+    fr, fc = np.ascontiguousarray(fine_row, np.int32), np.ascontiguousarray(fine_col, np.int32)
+    cr, cc = np.ascontiguousarray(coarse_row, np.int32), np.ascontiguousarray(coarse_col, np.int32)
+    h = C.c_void_p()
+    _lib.call("pp_galerkin_map_build", fr.size - 1, fr.ctypes.data, fc.ctypes.data, cr.size - 1, pr.ctypes.data,
+              pc.ctypes.data, pv.ctypes.data, rr.ctypes.data, rc.ctypes.data, rv.ctypes.data, cr.ctypes.data,
+              cc.ctypes.data, C.byref(h))
+    try:
+        nt = C.c_int64()
+        _lib.call("pp_galerkin_map_size", h, C.byref(nt))
+        seg = np.empty(cc.size + 1, np.int32)
+        src = np.empty(max(nt.value, 1), np.int32)
+        w = np.empty(max(nt.value, 1), np.float64)
+        _lib.call("pp_galerkin_map_copy", h, seg.ctypes.data, src.ctypes.data, w.ctypes.data)
+    finally:
+        _lib.LIB.pp_galerkin_map_free(h)
+    return seg, src[: nt.value], w[: nt.value]
+
+
 def _dev(a: np.ndarray, device, dtype=None) -> torch.Tensor:
     t = torch.from_numpy(np.ascontiguousarray(a))
     return t.to(device=device, dtype=dtype) if dtype is not None else t.to(device)
@@ -187,6 +211,9 @@ def build_amg_hierarchy(csr_mat: CSRmatrix, *, method: str = "ruge_stuben", max_
             rr, rc, rv = lv["R"]
             level.P_row, level.P_col, level.P_val = _dev(pr, device), _dev(pc, device), _dev(pv, device, torch.float64)
             level.R_row, level.R_col, level.R_val = _dev(rr, device), _dev(rc, device), _dev(rv, device, torch.float64)
+            fine_row, fine_col = (row, col) if i == 0 else lv["A"][:2]
+            seg, src, w = galerkin_map_host(fine_row, fine_col, lv["P"], lv["R"], host[i + 1]["A"][0], host[i + 1]["A"][1])
+            level.gmap = (_dev(seg, device), _dev(src, device), _dev(w, device, torch.float64))
         levels.append(level)
     return AMGHierarchy(levels, _dense(levels[-1].A), project_constant, num_pre_smooth, num_post_smooth, native_dtype,
                         time.perf_counter() - t0)
@@ -213,18 +240,26 @@ def galerkin_refresh(hierarchy: AMGHierarchy, csr_mat: CSRmatrix, *, project_con
     A = CSRmatrix(csr_mat.value.reshape(-1)[: csr_mat.getNnz()], csr_mat.index, csr_mat.row)
     levels = [Level(n=fine.n, A=A, R_row=fine.R_row, R_col=fine.R_col, R_val=fine.R_val, P_row=fine.P_row,
                     P_col=fine.P_col, P_val=fine.P_val)]
+    levels[0].gmap = fine.gmap
     if coarse:
         for i in range(1, len(hierarchy.levels)):
             prev = levels[-1]
             old = hierarchy.levels[i]
             cval = torch.empty(old.A.getNnz(), dtype=A.getDtype(), device=A.value.device)
-            d = prev.A.desc()
-            _lib.call("pp_galerkin_rap", _lib.ptr(prev.R_row), _lib.ptr(prev.R_col), _lib.ptr(prev.R_val), C.byref(d),
-                      _lib.dtype_code(prev.A.getDtype()), _lib.ptr(prev.P_row), _lib.ptr(prev.P_col), _lib.ptr(prev.P_val),
-                      _lib.ptr(old.A.row), _lib.ptr(old.A.index), _lib.ptr(cval), old.n, old.A.getNnz(),
-                      _lib.dtype_code(A.getDtype()), _lib.stream_ptr(A.value.device))
+            stream = _lib.stream_ptr(A.value.device)
+            if hierarchy.levels[i - 1].gmap is not None:
+                seg, src, w = hierarchy.levels[i - 1].gmap
+                _lib.call("pp_galerkin_apply", _lib.ptr(seg), _lib.ptr(src), _lib.ptr(w), old.A.getNnz(),
+                          _lib.ptr(prev.A.value), _lib.dtype_code(prev.A.getDtype()), _lib.ptr(cval),
+                          _lib.dtype_code(A.getDtype()), 1, 0, 0, stream)
+            else:
+                d = prev.A.desc()
+                _lib.call("pp_galerkin_rap", _lib.ptr(prev.R_row), _lib.ptr(prev.R_col), _lib.ptr(prev.R_val),
+                          C.byref(d), _lib.dtype_code(prev.A.getDtype()), _lib.ptr(prev.P_row), _lib.ptr(prev.P_col),
+                          _lib.ptr(prev.P_val), _lib.ptr(old.A.row), _lib.ptr(old.A.index), _lib.ptr(cval), old.n,
+                          old.A.getNnz(), _lib.dtype_code(A.getDtype()), stream)
             levels.append(Level(n=old.n, A=CSRmatrix(cval, old.A.index, old.A.row), R_row=old.R_row, R_col=old.R_col,
-                                R_val=old.R_val, P_row=old.P_row, P_col=old.P_col, P_val=old.P_val))
+                                R_val=old.R_val, P_row=old.P_row, P_col=old.P_col, P_val=old.P_val, gmap=old.gmap))
         coarse_A = _dense(levels[-1].A)
     else:
         for old in hierarchy.levels[1:]:

@@ -229,3 +229,33 @@ def test_product_does_not_import_oracle():
     for p in (ROOT / "phipict_amd").rglob("*.py"):
         text = p.read_text()
         assert "oracle" not in re.sub(r"#.*", "", text).replace('"""', ""), p
+
+
+@pytest.mark.parametrize("which", ["pressure2d", "epot3d"])
+def test_galerkin_map_reproduces_rap(which):
+    """The host Galerkin map applied to new fine values gives R A P of those values (the
+    oracle's double-precision product) on every level of the hierarchy."""
+    from phipict_amd.solvers.amg import galerkin_map_host, rs_setup_host
+
+    if which == "pressure2d":
+        od = O.OracleDomain(cases.channel2d(40, 30), 0.02, 0.01)
+        _, _, A = od.advection_matrix()
+        col, val = od.pressure_matrix(A)
+    else:
+        od = O.OracleDomain(cases.duct3d(14, 10, 12), 0.01, 0.02)
+        col, val = od.epot_matrix()
+    row = od.row
+    levels = rs_setup_host(row, col, val)
+    rng = np.random.default_rng(0)
+    fine = (row, col, val * (1 + 0.3 * rng.uniform(size=val.size)))
+    for l in range(len(levels) - 1):
+        lv, nxt = levels[l], levels[l + 1]
+        frow, fcol = (row, col) if l == 0 else lv["A"][:2]
+        seg, src, w = galerkin_map_host(frow, fcol, lv["P"], lv["R"], nxt["A"][0], nxt["A"][1])
+        got = np.add.reduceat(w * fine[2][src], seg[:-1]) if src.size else np.zeros(seg.size - 1)
+        got[np.diff(seg) == 0] = 0.0
+        ref = O.galerkin(fine, lv["P"], lv["R"], nxt["n"])
+        np.testing.assert_array_equal(ref[0], nxt["A"][0])
+        np.testing.assert_array_equal(ref[1], nxt["A"][1])
+        np.testing.assert_allclose(got, ref[2], rtol=1e-12, atol=1e-14 * np.abs(ref[2]).max())
+        fine = ref
