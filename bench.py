@@ -48,6 +48,8 @@ def parse_args(argv=None):
     ap.add_argument("--cpu-size", default="96x48x48", help="cells of the CPU sample")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--details", action="store_true", help="also print the per-kernel table to stderr")
+    ap.add_argument("--host-profile", action="store_true",
+                    help="profile the host (Python) side of the timed steps with cProfile; the result goes to stderr")
     return ap.parse_args(argv)
 
 
@@ -230,14 +232,30 @@ def main(argv=None) -> int:
     print(f"[bench] warm-up done ({args.warmup} steps)", file=sys.stderr, flush=True)
     log_start = len(sim.solver_log)
     profiling.start(args.timer_every)
+    prof = None
+    if args.host_profile:
+        import cProfile
+
+        prof = cProfile.Profile()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    if prof is not None:
+        prof.enable()
     for i in range(args.steps):
         if not sim.single_step():
             raise RuntimeError(f"step {i} failed")
+    if prof is not None:
+        prof.disable()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     profiling.stop()
+    if prof is not None:
+        import io
+        import pstats
+
+        buf = io.StringIO()
+        pstats.Stats(prof, stream=buf).sort_stats("tottime").print_stats(45)
+        print(buf.getvalue(), file=sys.stderr)
     timings = profiling.read()
     print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
     n = dom.getTotalSize()

@@ -77,7 +77,7 @@ typedef struct pp_block_desc {
     int32_t vsource_batched;/* source has one slice per environment */
     int32_t visc_mode;      /* 0 domain viscosity, 1 static [1], 2 per cell [cells] */
     int32_t pad0, pad1;
-    uint64_t transform;     /* [cells][2d^2+1] */
+    uint64_t transform;     /* component-major [2d^2+1][cells]: Block.transform transposed */
     uint64_t velocity;      /* [batch][d][cells] */
     uint64_t pressure;      /* [batch][cells] */
     uint64_t epot;          /* [batch][cells] or 0 */
@@ -176,6 +176,10 @@ int pp_coords_to_face_transforms(int32_t dims, const int32_t *cells, const void 
 
 /* ---- PISO assembly (non-orthogonal correction flags accepted; the corner terms are
  * zero on the rectilinear grids these entry points are validated on) ---- */
+
+/* out[c][r] = in[r][c] for a row-major [rows][cols] array (Block.transform [cells][2d^2+1]
+ * to the component-major layout the block descriptors take); no reference counterpart. */
+int pp_transpose_2d(const void *in, int32_t rows, int32_t cols, int32_t dtype, void *out, void *stream);
 
 /* CSR pattern shared by C, P and E: row[n+1] and ascending columns (the row layout the
  * reference's matrix kernels write, csrc/phipict/piso/device/topology.cuh:168-233, and

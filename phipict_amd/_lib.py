@@ -139,6 +139,7 @@ SIGNATURES = {
     "pp_device_sync": [],
     "pp_coords_to_transforms": [_i32, _P(_i32), _vp, _vp, _i32, _vp],
     "pp_coords_to_face_transforms": [_i32, _P(_i32), _vp, _vp, _i32, _vp],
+    "pp_transpose_2d": [_vp, _i32, _i32, _i32, _vp, _vp],
     "pp_csr_pattern": [_P(DomainDesc), _vp, _vp, _vp],
     "pp_setup_advection_matrix": [_P(DomainDesc), _i32, _vp],
     "pp_setup_advection_velocity": [_P(DomainDesc), _i32, _i32, _vp],

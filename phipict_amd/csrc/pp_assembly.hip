@@ -20,11 +20,13 @@ template <typename T, int D, bool EPOT>
 __device__ void cell_fluxes(const pp_domain &dom, const pp_block_desc *__restrict__ blocks, int bi, const int *p,
                             int f, const T *glob, int env, T *fl) {
     const pp_block_desc &b = blocks[bi];
+    T velCd[D];
+    contra_all<T, D>(dom, b, f, glob, env, velCd);
 #pragma unroll
     for (int bound = 0; bound < 2 * D; ++bound) {
         const int dim = bound >> 1;
         const int up = bound & 1;
-        const T velC = contra<T, D>(dom, b, f, dim, glob, env);
+        const T velC = velCd[dim];
         if (at_bound(p, bound, b)) {
             const pp_boundary_desc &bd = b.bnd[bound];
             if (bd.type == PP_BOUND_FIXED) {
@@ -79,14 +81,17 @@ __device__ __forceinline__ int emit_row(int *idx, T *vals, bool divide, T det, i
     return written;
 }
 
+// Workgroups never straddle blocks: blockIdx.x = block * wpb + tile, so the block index and
+// its descriptor are uniform over the workgroup (scalar loads).
 #define CELL_PROLOGUE                                                          \
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;                       \
-    if (g >= dom.num_cells) return;                                            \
-    const int env = blockIdx.y;                                                \
+    const int wpb_ = (dom.max_block_cells + ASM_BLOCK - 1) / ASM_BLOCK;        \
+    const int bi = __builtin_amdgcn_readfirstlane(blockIdx.x / wpb_);           \
+    const int f = (blockIdx.x - bi * wpb_) * ASM_BLOCK + threadIdx.x;           \
     const pp_block_desc *__restrict__ blocks = dom.blocks;                     \
-    const int bi = find_block(blocks, dom.num_blocks, g);                      \
     const pp_block_desc &b = blocks[bi];                                       \
-    const int f = g - b.global_offset;                                         \
+    if (f >= b.cells) return;                                                  \
+    const int g = b.global_offset + f;                                         \
+    const int env = blockIdx.y;                                                \
     int p[3];                                                                  \
     unflat(f, b.size, p);
 
@@ -236,14 +241,14 @@ __device__ void pressure_grad(const pp_domain &dom, const pp_block_desc *__restr
         gr[dim] = (vp - vn) * fac;
     }
     if (b.has_transform) {
-        const T *mi = Metric<T, D>::at(b, f) + D * D;
+        const TView<T> t = Metric<T, D>::at(b, f);
         T r[D];
 #pragma unroll
         for (int j = 0; j < D; ++j) r[j] = T(0);
 #pragma unroll
         for (int i = 0; i < D; ++i)
 #pragma unroll
-            for (int j = 0; j < D; ++j) r[j] += mi[i * D + j] * gr[i];
+            for (int j = 0; j < D; ++j) r[j] += t[D * D + i * D + j] * gr[i];
 #pragma unroll
         for (int j = 0; j < D; ++j) gr[j] = r[j];
     }
@@ -572,7 +577,7 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_current_density(pp_domain dom, co
     for (int k = 0; k < D; ++k) ja[k] = T(0.5) * (jf[k * 2] + jf[k * 2 + 1]);
     T Jv[3] = {T(0), T(0), T(0)};
     if (b.has_transform) {
-        const T *t = Metric<T, D>::at(b, f);
+        const TView<T> t = Metric<T, D>::at(b, f);
         const T inv = T(1) / t[2 * D * D];
         if constexpr (D == 2) {
             Jv[0] = (t[0] * ja[0] + t[1] * ja[1]) * inv;
@@ -598,13 +603,13 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_current_density(pp_domain dom, co
 template <typename T>
 __global__ void __launch_bounds__(ASM_BLOCK) k_copy_result(pp_domain dom, int field, int ch, int to_blocks,
                                                          const T *__restrict__ flat_in, T *__restrict__ flat_out) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= dom.num_cells) return;
+    const int wpb_ = (dom.max_block_cells + ASM_BLOCK - 1) / ASM_BLOCK;
+    const int bi = __builtin_amdgcn_readfirstlane(blockIdx.x / wpb_);
+    const int f = (blockIdx.x - bi * wpb_) * ASM_BLOCK + threadIdx.x;
+    const pp_block_desc &b = dom.blocks[bi];
+    if (f >= b.cells) return;
+    const int g = b.global_offset + f;
     const int env = blockIdx.y;
-    const pp_block_desc *__restrict__ blocks = dom.blocks;
-    const int bi = find_block(blocks, dom.num_blocks, g);
-    const pp_block_desc &b = blocks[bi];
-    const int f = g - b.global_offset;
     T *bf = reinterpret_cast<T *>(field == 0 ? b.velocity : (field == 1 ? b.pressure : b.epot));
     for (int c = 0; c < ch; ++c) {
         const size_t fo = ((size_t)env * ch + c) * dom.num_cells + g;
@@ -758,8 +763,26 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_coords_to_face_transforms(const T
     }
 }
 
+// out[c][r] = in[r][c] through 32x32 LDS tiles (row-major [rows][cols] -> [cols][rows]).
+template <typename T>
+__global__ void __launch_bounds__(256) k_transpose_2d(const T *__restrict__ in, int rows, int cols, T *__restrict__ out) {
+    __shared__ T tile[32][33];
+    const int r0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5; // 32 x 8
+    for (int k = ty; k < 32; k += 8) {
+        const int r = r0 + k, c = c0 + tx;
+        if (r < rows && c < cols) tile[k][tx] = in[(size_t)r * cols + c];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int c = c0 + k, r = r0 + tx;
+        if (r < rows && c < cols) out[(size_t)c * rows + r] = tile[tx][k];
+    }
+}
+
 static int check_domain(const pp_domain *dom, const char *fn) {
     if (!dom || !dom->blocks || dom->num_blocks <= 0 || dom->num_cells <= 0 || dom->batch <= 0 ||
+        dom->max_block_cells <= 0 ||
         (dom->dims != 2 && dom->dims != 3) || (dom->dtype != PP_F32 && dom->dtype != PP_F64)) {
         set_error(std::string(fn) + ": invalid domain descriptor");
         return PP_ERR_ARG;
@@ -774,6 +797,11 @@ static int check_domain(const pp_domain *dom, const char *fn) {
 static inline dim3 cell_grid(const pp_domain *dom, int envs) {
     return dim3((dom->num_cells + ASM_BLOCK - 1) / ASM_BLOCK, envs);
 }
+// grid of CELL_PROLOGUE kernels: ceil(max_block_cells / ASM_BLOCK) workgroups per block
+static inline dim3 block_grid(const pp_domain *dom, int envs) {
+    const int wpb = (dom->max_block_cells + ASM_BLOCK - 1) / ASM_BLOCK;
+    return dim3(wpb * dom->num_blocks, envs);
+}
 
 #define DIMS_DISPATCH(DIMV, ...)                                   \
     if ((DIMV) == 2) { constexpr int D = 2; __VA_ARGS__; }         \
@@ -784,6 +812,17 @@ static inline dim3 cell_grid(const pp_domain *dom, int envs) {
 using namespace pp;
 
 extern "C" {
+
+int pp_transpose_2d(const void *in, int32_t rows, int32_t cols, int32_t dtype, void *out, void *stream) {
+    PP_CHECK_ARG(in && out && rows >= 0 && cols >= 0, "bad arguments");
+    if (rows == 0 || cols == 0) return PP_OK;
+    PP_CHECK_ARG((cols + 31) / 32 <= 65535, "too many columns for the grid");
+    const dim3 g((rows + 31) / 32, (cols + 31) / 32);
+    PP_DISPATCH(dtype, hipLaunchKernelGGL((k_transpose_2d<T>), g, dim3(256), 0, as_stream(stream),
+                                          reinterpret_cast<const T *>(in), rows, cols, reinterpret_cast<T *>(out)));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
 
 int pp_csr_pattern(const pp_domain *dom, int32_t *row, int32_t *col, void *stream) {
     if (int e = check_domain(dom, __func__)) return e;
@@ -799,7 +838,7 @@ int pp_setup_advection_matrix(const pp_domain *dom, int32_t non_ortho_flags, voi
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(dom->C.val && dom->C.row && dom->A && dom->time_step, "C, A and time_step must be set");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        PP_TIMED(PP_K_ADVECTION_MATRIX, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_advection_matrix<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom))));
+        PP_TIMED(PP_K_ADVECTION_MATRIX, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_advection_matrix<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom))));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
@@ -809,7 +848,7 @@ int pp_setup_advection_velocity(const pp_domain *dom, int32_t non_ortho_flags, i
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(dom->velocity_rhs && dom->time_step, "velocity_rhs and time_step must be set");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        PP_TIMED(PP_K_ADVECTION_RHS, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_advection_rhs<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+        PP_TIMED(PP_K_ADVECTION_RHS, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_advection_rhs<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
                            (int)apply_pressure_gradient))));
     PP_LAUNCH_CHECK();
     return PP_OK;
@@ -820,7 +859,7 @@ int pp_setup_pressure_matrix(const pp_domain *dom, int32_t non_ortho_flags, void
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(dom->P.val && dom->P.row && dom->A, "P and A must be set");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        PP_TIMED(PP_K_PRESSURE_MATRIX, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_pressure_matrix<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom))));
+        PP_TIMED(PP_K_PRESSURE_MATRIX, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_pressure_matrix<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom))));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
@@ -830,8 +869,8 @@ int pp_setup_pressure_rhs(const pp_domain *dom, int32_t non_ortho_flags, int32_t
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(dom->pressure_rhs && dom->pressure_rhs_div && dom->C.val && dom->velocity_result, "pressure fields must be set");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        PP_TIMED(PP_K_PRESSURE_RHS, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_pressure_rhs<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom));
-        PP_TIMED(PP_K_PRESSURE_DIV, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_pressure_div<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+        PP_TIMED(PP_K_PRESSURE_RHS, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_pressure_rhs<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom));
+        PP_TIMED(PP_K_PRESSURE_DIV, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_pressure_div<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
                            (int)time_step_norm))));
     PP_LAUNCH_CHECK();
     return PP_OK;
@@ -842,7 +881,7 @@ int pp_setup_pressure_rhs_div(const pp_domain *dom, int32_t non_ortho_flags, int
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(dom->pressure_rhs && dom->pressure_rhs_div, "pressure fields must be set");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        PP_TIMED(PP_K_PRESSURE_DIV, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_pressure_div<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+        PP_TIMED(PP_K_PRESSURE_DIV, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_pressure_div<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
                            (int)time_step_norm))));
     PP_LAUNCH_CHECK();
     return PP_OK;
@@ -852,7 +891,7 @@ int pp_correct_velocity(const pp_domain *dom, int32_t time_step_norm, void *stre
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(dom->velocity_result && dom->pressure_rhs && dom->A, "velocity_result, pressure_rhs and A must be set");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        PP_TIMED(PP_K_CORRECT_VELOCITY, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_correct_velocity<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+        PP_TIMED(PP_K_CORRECT_VELOCITY, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_correct_velocity<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
                            (int)time_step_norm))));
     PP_LAUNCH_CHECK();
     return PP_OK;
@@ -863,7 +902,7 @@ int pp_setup_epot_matrix(const pp_domain *dom, int32_t non_ortho_flags, void *st
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(dom->E.val && dom->E.row, "E must be set");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        hipLaunchKernelGGL((k_epot_matrix<T, D>), cell_grid(dom, 1), dim3(ASM_BLOCK), 0, as_stream(stream), *dom)));
+        hipLaunchKernelGGL((k_epot_matrix<T, D>), block_grid(dom, 1), dim3(ASM_BLOCK), 0, as_stream(stream), *dom)));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
@@ -872,7 +911,7 @@ int pp_compute_epot_rhs(const pp_domain *dom, const void *vector_field, void *di
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(vector_field && divergence, "vector_field and divergence must not be null");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        PP_TIMED(PP_K_EPOT_RHS, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_epot_rhs<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+        PP_TIMED(PP_K_EPOT_RHS, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_epot_rhs<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
                            reinterpret_cast<const T *>(vector_field), reinterpret_cast<T *>(divergence)))));
     PP_LAUNCH_CHECK();
     return PP_OK;
@@ -882,7 +921,7 @@ int pp_compute_current_density(const pp_domain *dom, const void *phi, const void
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(phi && uxb && j_out, "phi, uxb and j_out must not be null");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        PP_TIMED(PP_K_CURRENT_DENSITY, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_current_density<T, D>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+        PP_TIMED(PP_K_CURRENT_DENSITY, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_current_density<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
                            reinterpret_cast<const T *>(phi), reinterpret_cast<const T *>(uxb), reinterpret_cast<T *>(j_out)))));
     PP_LAUNCH_CHECK();
     return PP_OK;
@@ -895,7 +934,7 @@ int pp_copy_result(const pp_domain *dom, int32_t field, int32_t to_blocks, void 
     PP_CHECK_ARG(flat != nullptr, "result vector not set");
     const int ch = field == 0 ? dom->dims : 1;
     PP_DISPATCH(dom->dtype,
-        PP_TIMED(PP_K_COPY_RESULT, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_copy_result<T>), cell_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom, (int)field, ch,
+        PP_TIMED(PP_K_COPY_RESULT, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_copy_result<T>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom, (int)field, ch,
                            (int)to_blocks, reinterpret_cast<const T *>(flat), reinterpret_cast<T *>(flat))));
     PP_LAUNCH_CHECK();
     return PP_OK;

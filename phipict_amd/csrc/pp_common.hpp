@@ -153,23 +153,37 @@ __device__ __forceinline__ int neighbor(const pp_block_desc *__restrict__ blocks
     return bi;
 }
 
-// Per-cell metric access. TS = 2*D*D+1 scalars {M, Minv, det}.
+// Strided view of the 2*D*D+1 scalars {M, Minv, det} of one cell or face cell.
+template <typename T>
+struct TView {
+    const T *p;
+    int stride;
+    __device__ __forceinline__ T operator[](int k) const { return p[(size_t)k * stride]; }
+};
+
+// Per-cell metric access. Cell transforms are component-major, [2*D*D+1][cells] (neighbouring
+// work-items read neighbouring addresses); boundary face transforms are [face cells][2*D*D+1].
 template <typename T, int D>
 struct Metric {
     static constexpr int TS = 2 * D * D + 1;
-    __device__ static __forceinline__ const T *at(const pp_block_desc &b, int f) {
-        return reinterpret_cast<const T *>(b.transform) + (size_t)f * TS;
+    __device__ static __forceinline__ TView<T> at(const pp_block_desc &b, int f) {
+        return TView<T>{reinterpret_cast<const T *>(b.transform) + f, b.cells};
+    }
+    __device__ static __forceinline__ TView<T> face(const pp_boundary_desc &fb, int ff) {
+        return TView<T>{reinterpret_cast<const T *>(fb.transform) + (size_t)ff * TS, 1};
     }
     __device__ static __forceinline__ T det(const pp_block_desc &b, int f) {
         if (!b.has_transform) return T(1);
         return at(b, f)[2 * D * D];
     }
     // det * |Minv[w]|^2 (getLaplaceCoefficientOrthogonal, discretization.cuh:359-374)
-    __device__ static __forceinline__ T alpha_t(const T *t, int w) {
-        const T *mi = t + D * D + w * D;
+    __device__ static __forceinline__ T alpha_t(const TView<T> &t, int w) {
         T r = 0;
 #pragma unroll
-        for (int i = 0; i < D; ++i) r += mi[i] * mi[i];
+        for (int i = 0; i < D; ++i) {
+            const T m = t[D * D + w * D + i];
+            r += m * m;
+        }
         return t[2 * D * D] * r;
     }
     __device__ static __forceinline__ T alpha(const pp_block_desc &b, int f, int w) {
@@ -177,19 +191,21 @@ struct Metric {
         return alpha_t(at(b, f), w);
     }
     // det * Minv[w] . v (VelocityToContravariantComponent, velocity.cuh:24-39)
-    __device__ static __forceinline__ T contra_t(const T *t, int w, const T *v) {
-        const T *mi = t + D * D + w * D;
+    __device__ static __forceinline__ T contra_t(const TView<T> &t, int w, const T *v) {
         T r = 0;
 #pragma unroll
-        for (int k = 0; k < D; ++k) r += mi[k] * v[k];
+        for (int k = 0; k < D; ++k) r += t[D * D + w * D + k] * v[k];
         return t[2 * D * D] * r;
     }
     __device__ static __forceinline__ T minv_norm(const pp_block_desc &b, int f, int axis) {
         if (!b.has_transform) return T(1);
-        const T *mi = at(b, f) + D * D + axis * D;
+        const TView<T> t = at(b, f);
         T r = 0;
 #pragma unroll
-        for (int i = 0; i < D; ++i) r += mi[i] * mi[i];
+        for (int i = 0; i < D; ++i) {
+            const T m = t[D * D + axis * D + i];
+            r += m * m;
+        }
         return dsqrt<T>(r);
     }
 };
@@ -219,6 +235,23 @@ __device__ __forceinline__ T contra(const pp_domain &dom, const pp_block_desc &b
     return Metric<T, D>::contra_t(Metric<T, D>::at(b, f), w, v);
 }
 
+// All D contravariant components of cell f (one read of the velocity and the metric).
+template <typename T, int D>
+__device__ __forceinline__ void contra_all(const pp_domain &dom, const pp_block_desc &b, int f, const T *glob, int env,
+                                           T *out) {
+    T v[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) v[k] = vel_at<T>(dom, b, f, k, glob, env);
+    if (!b.has_transform) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) out[k] = v[k];
+        return;
+    }
+    const TView<T> t = Metric<T, D>::at(b, f);
+#pragma unroll
+    for (int w = 0; w < D; ++w) out[w] = Metric<T, D>::contra_t(t, w, v);
+}
+
 // FIXED boundary velocity component k at face cell ff (getFixedBoundaryData, block_data.cuh:123-196)
 template <typename T>
 __device__ __forceinline__ T fixed_vel(const pp_boundary_desc &fb, int d, int fc, int ff, int k, int env) {
@@ -234,14 +267,12 @@ __device__ __forceinline__ T contra_fixed(const pp_boundary_desc &fb, int fc, in
     T v[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) v[k] = fb.vel_type == PP_BC_DIRICHLET ? fixed_vel<T>(fb, D, fc, ff, k, env) : T(0);
-    const T *t = reinterpret_cast<const T *>(fb.transform) + (size_t)ff * Metric<T, D>::TS;
-    return Metric<T, D>::contra_t(t, w, v);
+    return Metric<T, D>::contra_t(Metric<T, D>::face(fb, ff), w, v);
 }
 template <typename T, int D>
 __device__ __forceinline__ T alpha_fixed(const pp_boundary_desc &fb, int ff, int w) {
     if (!fb.has_transform) return T(1);
-    const T *t = reinterpret_cast<const T *>(fb.transform) + (size_t)ff * Metric<T, D>::TS;
-    return Metric<T, D>::alpha_t(t, w);
+    return Metric<T, D>::alpha_t(Metric<T, D>::face(fb, ff), w);
 }
 template <typename T>
 __device__ __forceinline__ T vsource(const pp_domain &dom, const pp_block_desc &b, int f, int k, int env) {

@@ -500,6 +500,24 @@ class Block:
     def hasTransform(self) -> bool:
         return self.transform is not None
 
+    def _transform_component_major(self) -> Optional[torch.Tensor]:
+        """The cell transforms as [2d^2+1][cells] for the library's descriptors; rebuilt when
+        the transform tensor is replaced or written."""
+        t = self.transform
+        if t is None:
+            return None
+        key = (t.data_ptr(), t._version, t.dtype)
+        cached = getattr(self, "_tcm", None)
+        if cached is not None and cached[0] == key:
+            return cached[1]
+        ts = transform_size(self.getSpatialDims())
+        out = torch.empty(ts, self.getCells(), dtype=t.dtype, device=t.device)
+        src = t.contiguous()
+        _lib.call("pp_transpose_2d", _lib.ptr(src), self.getCells(), ts, _lib.dtype_code(t.dtype), _lib.ptr(out),
+                  _lib.stream_ptr(t.device))
+        self._tcm = (key, out, src)
+        return out
+
     def hasFaceTransform(self) -> bool:
         return self.faceTransform is not None
 
@@ -907,7 +925,7 @@ class Domain:
         d.global_offset = b.globalOffset
         d.csr_offset = b.csrOffset
         d.has_transform = 1 if b.transform is not None else 0
-        d.transform = _lib.ptr(b.transform)
+        d.transform = _lib.ptr(b._transform_component_major())
         d.velocity = _lib.ptr(b.velocity)
         d.pressure = _lib.ptr(b.pressure)
         d.epot = _lib.ptr(b.epot)

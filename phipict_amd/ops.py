@@ -263,18 +263,9 @@ def SolveLinear(A: CSRmatrix, RHS: torch.Tensor, x: torch.Tensor, maxIterations=
     return _results(res, nrhs)
 
 
-def AMGPCGSolve(aRow, aCol, aVal, levels: Sequence[Sequence[torch.Tensor]], coarsePinv: torch.Tensor, rhs: torch.Tensor,
-                x: torch.Tensor, maxIterations: int, tolerance: float, normalized: bool = True, returnBest: bool = False,
-                projectConstant: bool = False, preSweeps: int = 1, postSweeps: int = 1, omega: float = 2.0 / 3.0,
-                tolerances: Sequence[float] = ()) -> list:
-    """AMG-preconditioned CG (bindings.cpp:764-776 -> krylov.cu:1565-1612). ``levels`` holds
-    per level ``(A row, col, val, dinv, R row, col, val, P row, col, val)``; the hierarchy
-    dtype is that of ``coarsePinv``; ``x`` is updated in place."""
-    n = aRow.numel() - 1
-    if rhs.numel() % n != 0 or rhs.numel() != x.numel():
-        raise ValueError("rhs and x must be contiguous and of equal size, a multiple of the matrix size")
-    nrhs = rhs.numel() // n
-    hd = coarsePinv.dtype
+def pack_amg_levels(levels: Sequence[Sequence[torch.Tensor]], hd) -> tuple:
+    """ctypes level descriptors for ``levels`` (per level: A row, col, val, dinv, R row, col,
+    val, P row, col, val, all values in ``hd``); returns (array, tensors kept alive)."""
     lv = (_lib.AmgLevel * len(levels))()
     keep = []
     for i, t in enumerate(levels):
@@ -293,23 +284,50 @@ def AMGPCGSolve(aRow, aCol, aVal, levels: Sequence[Sequence[torch.Tensor]], coar
             tt = tt.contiguous()
             keep.append(tt)
             setattr(L, name, _lib.ptr(tt))
+    return lv, keep
+
+
+def amg_pcg_packed(aRow, aCol, aVal, packed: tuple, coarsePinv: torch.Tensor, rhs: torch.Tensor, x: torch.Tensor,
+                   maxIterations: int, tolerance: float, normalized: bool = True, returnBest: bool = False,
+                   projectConstant: bool = False, preSweeps: int = 1, postSweeps: int = 1, omega: float = 2.0 / 3.0,
+                   tolerances: Sequence[float] = ()) -> list:
+    """AMGPCGSolve on levels already packed by ``pack_amg_levels``."""
+    lv, _keep = packed
+    n = aRow.numel() - 1
+    if rhs.numel() % n != 0 or rhs.numel() != x.numel():
+        raise ValueError("rhs and x must be contiguous and of equal size, a multiple of the matrix size")
+    if not x.is_contiguous():
+        raise ValueError("x must be contiguous")
+    nrhs = rhs.numel() // n
+    hd = coarsePinv.dtype
     nc = coarsePinv.size(-1)
     batched_coarse = 1 if (coarsePinv.dim() == 3 and nrhs > 1) else 0
     pinv = coarsePinv.contiguous()
-    A = CSRmatrix(aVal, aCol, aRow)
-    desc = A.desc()
+    desc = _lib.CsrDesc()
+    desc.row, desc.col, desc.val = _lib.ptr(aRow), _lib.ptr(aCol), _lib.ptr(aVal)
+    desc.n, desc.nnz = n, aCol.numel()
+    desc.num_matrices = max(1, aVal.numel() // max(1, aCol.numel()))
     res = (_lib.SolveResult * nrhs)()
     tl = list(tolerances)
     tarr = (C.c_double * max(1, len(tl)))(*(tl if tl else [float(tolerance)]))
     rhs_c = rhs.contiguous()
-    if not x.is_contiguous():
-        raise ValueError("x must be contiguous")
-    _lib.call("pp_amg_pcg_solve", C.byref(desc), lv, len(levels), _lib.ptr(pinv), nc, batched_coarse, _lib.dtype_code(hd),
+    _lib.call("pp_amg_pcg_solve", C.byref(desc), lv, len(lv), _lib.ptr(pinv), nc, batched_coarse, _lib.dtype_code(hd),
               _lib.ptr(rhs_c), _lib.ptr(x), nrhs, _lib.dtype_code(aVal.dtype), int(maxIterations), float(tolerance),
               int(bool(normalized)), int(bool(returnBest)), int(bool(projectConstant)), int(preSweeps), int(postSweeps),
               float(omega), tarr, len(tl), _lib.stream_ptr(x.device), res)
-    del keep
     return _results(res, nrhs)
+
+
+def AMGPCGSolve(aRow, aCol, aVal, levels: Sequence[Sequence[torch.Tensor]], coarsePinv: torch.Tensor, rhs: torch.Tensor,
+                x: torch.Tensor, maxIterations: int, tolerance: float, normalized: bool = True, returnBest: bool = False,
+                projectConstant: bool = False, preSweeps: int = 1, postSweeps: int = 1, omega: float = 2.0 / 3.0,
+                tolerances: Sequence[float] = ()) -> list:
+    """AMG-preconditioned CG (bindings.cpp:764-776 -> krylov.cu:1565-1612). ``levels`` holds
+    per level ``(A row, col, val, dinv, R row, col, val, P row, col, val)``; the hierarchy
+    dtype is that of ``coarsePinv``; ``x`` is updated in place."""
+    packed = pack_amg_levels(levels, coarsePinv.dtype)
+    return amg_pcg_packed(aRow, aCol, aVal, packed, coarsePinv, rhs, x, maxIterations, tolerance, normalized,
+                          returnBest, projectConstant, preSweeps, postSweeps, omega, tolerances)
 
 
 def ReleaseKrylovWorkspaces() -> None:

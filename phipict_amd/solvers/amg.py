@@ -99,6 +99,24 @@ class AMGHierarchy:
             out.append(nat)
         return out
 
+    def packed_levels(self, dtype) -> tuple:
+        """Level descriptors in ``dtype`` packed for the library (cached per dtype)."""
+        cache = self.__dict__.setdefault("_packed", {})
+        if dtype not in cache:
+            parent = self.__dict__.get("_packed_parent")
+            pcache = getattr(parent, "_packed", {}) if parent is not None else {}
+            if dtype in pcache and len(parent.levels) == len(self.levels):
+                # a finest-level refresh: copy the parent's descriptors, replace level 0
+                arr, keep = pcache[dtype]
+                arr = type(arr).from_buffer_copy(arr)
+                first, keep0 = ops.pack_amg_levels(self.native_levels(dtype)[:1], dtype)
+                arr[0] = first[0]
+                cache[dtype] = (arr, list(keep) + keep0)
+            else:
+                cache[dtype] = ops.pack_amg_levels(self.native_levels(dtype), dtype)
+            self.__dict__.pop("_packed_parent", None)
+        return cache[dtype]
+
     def operator_complexity(self) -> float:
         fine = self.levels[0].A.getNnz()
         return sum(l.A.getNnz() for l in self.levels) / max(1, fine)
@@ -219,11 +237,20 @@ def build_amg_hierarchy(csr_mat: CSRmatrix, *, method: str = "ruge_stuben", max_
                         time.perf_counter() - t0)
 
 
+_DENSE_ROWS: dict = {}
+
+
 def _dense(A: CSRmatrix) -> torch.Tensor:
     n = A.getRows()
-    row = A.row.to(torch.int64)
-    counts = row[1:] - row[:-1]
-    rows = torch.repeat_interleave(torch.arange(n, device=row.device), counts)
+    key = (A.row.data_ptr(), n, A.getNnz())
+    rows = _DENSE_ROWS.get(key)
+    if rows is None or rows[0] is not A.row:
+        row = A.row.to(torch.int64)
+        counts = row[1:] - row[:-1]
+        # row index per entry; computed once per pattern (repeat_interleave waits for the GPU)
+        rows = (A.row, torch.repeat_interleave(torch.arange(n, device=row.device), counts))
+        _DENSE_ROWS[key] = rows
+    rows = rows[1]
     D = torch.zeros(n, n, dtype=torch.float64, device=A.value.device)
     D.index_put_((rows, A.index.to(torch.int64)), A.value.reshape(-1)[: A.getNnz()].to(torch.float64), accumulate=True)
     return D
@@ -276,6 +303,7 @@ def galerkin_refresh(hierarchy: AMGHierarchy, csr_mat: CSRmatrix, *, project_con
     else:
         h.coarse_pinv64 = hierarchy.coarse_pinv64
         h._pinv_native = hierarchy._pinv_native
+        h._packed_parent = hierarchy
     return h
 
 
@@ -325,14 +353,14 @@ def amg_pcg_solve(rhs: torch.Tensor, x: torch.Tensor, hierarchy: AMGHierarchy, *
     if rhs.numel() % n:
         raise ValueError(f"RHS size {rhs.numel()} is not a multiple of the matrix size {n}.")
     pdtype = hierarchy.native_dtype or rhs.dtype
-    levels = hierarchy.native_levels(pdtype)
+    packed = hierarchy.packed_levels(pdtype)
     pinv = hierarchy.coarse_pinv(pdtype)
     tols = [float(t) for t in tol] if isinstance(tol, (list, tuple)) else []
     tol_scalar = max(tols) if tols else float(tol)
-    infos = ops.AMGPCGSolve(A.row, A.index, A.value.to(rhs.dtype) if A.value.dtype != rhs.dtype else A.value, levels,
-                            pinv, rhs.reshape(-1), x.view(-1), int(max_iter), tol_scalar, True, bool(return_best_result),
-                            hierarchy.project_constant, hierarchy.num_pre_smooth, hierarchy.num_post_smooth,
-                            JACOBI_OMEGA, tols)
+    infos = ops.amg_pcg_packed(A.row, A.index, A.value.to(rhs.dtype) if A.value.dtype != rhs.dtype else A.value,
+                               packed, pinv, rhs.reshape(-1), x.view(-1), int(max_iter), tol_scalar, True,
+                               bool(return_best_result), hierarchy.project_constant, hierarchy.num_pre_smooth,
+                               hierarchy.num_post_smooth, JACOBI_OMEGA, tols)
     for i, info in enumerate(infos):
         if not info.isFiniteResidual:
             x.view(len(infos), -1)[i].zero_()
