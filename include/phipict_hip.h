@@ -158,6 +158,7 @@ enum pp_kernel_id {
     PP_K_PCG_DOTS = 30, PP_K_PCG_DIRECTION = 31, PP_K_PCG_UPDATE = 32, PP_K_AMG_PRESMOOTH = 33,
     PP_K_AMG_JACOBI = 34, PP_K_AMG_LEVEL_RESIDUAL = 35, PP_K_AMG_RESTRICT = 36, PP_K_AMG_PROLONG = 37,
     PP_K_AMG_COARSE = 38, PP_K_MEAN = 39, PP_K_SUBTRACT_MEAN = 40, PP_K_AMG_POST_JACOBI = 41,
+    PP_K_AMG_TAIL = 42,
     PP_K_SPMV = 50, PP_K_GALERKIN = 51
 };
 int pp_kernel_timer_start(int32_t sample_every);
@@ -299,7 +300,8 @@ int pp_galerkin_map_build(int32_t n, const int32_t *a_row, const int32_t *a_col,
 int pp_galerkin_map_size(void *handle, int64_t *nterms);
 int pp_galerkin_map_copy(void *handle, int32_t *seg, int32_t *src, double *w);
 int pp_galerkin_map_free(void *handle);
-int pp_galerkin_apply(const int32_t *seg, const int32_t *src, const double *w, int32_t n_out, const void *a_val,
+int pp_galerkin_apply(const int32_t *seg, const int32_t *src, const double *w, int32_t n_out, int64_t nterms,
+                      const void *a_val,
                       int32_t a_dtype, void *c_val, int32_t c_dtype, int32_t nmat, int32_t a_stride, int32_t c_stride,
                       void *stream);
 /* inverse diagonal of a CSR matrix (rows without a usable diagonal get 1):

@@ -172,7 +172,7 @@ SIGNATURES = {
     "pp_galerkin_map_size": [_vp, _P(_i64)],
     "pp_galerkin_map_copy": [_vp, _vp, _vp, _vp],
     "pp_galerkin_map_free": [_vp],
-    "pp_galerkin_apply": [_vp, _vp, _vp, _i32, _vp, _i32, _vp, _i32, _i32, _i32, _i32, _vp],
+    "pp_galerkin_apply": [_vp, _vp, _vp, _i32, _i64, _vp, _i32, _vp, _i32, _i32, _i32, _i32, _vp],
     "pp_kernel_timer_start": [_i32],
     "pp_kernel_timer_stop": [],
     "pp_kernel_timer_read": [_i32, _vp, _vp, _vp, _vp, _vp, _P(_i32)],

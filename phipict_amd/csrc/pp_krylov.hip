@@ -1275,6 +1275,144 @@ __global__ void k_coarse_solve(const H *__restrict__ pinv, int nc, const H *__re
     }
 }
 
+// ---- coarse tail of the V-cycle: all levels below a size threshold in one workgroup ----
+// The levels lt..L-1, the dense coarse solve and the way back up run as phases of a single
+// workgroup per batch (barriers between phases), instead of ~4 latency-bound launches per
+// level. Row sums use the same lane groups and shuffle trees as the per-level kernels.
+constexpr int TAIL_MAX = 12;
+constexpr int TAIL_ROWS = 8192;
+constexpr int TAIL_BLOCK = 1024;
+
+template <typename H>
+struct TailArgs {
+    int nl, nc, pre, post, coarseBatched; // nl tail levels; index k = level - first tail level
+    H omega;
+    const H *pinv;
+    H *cb, *cx;
+    int n[TAIL_MAX], wa[TAIL_MAX], wr[TAIL_MAX], wp[TAIL_MAX];
+    Csr<H> A[TAIL_MAX], R[TAIL_MAX], P[TAIL_MAX];
+    const H *dinv[TAIL_MAX];
+    H *x[TAIL_MAX], *y[TAIL_MAX], *res[TAIL_MAX], *b[TAIL_MAX];
+};
+
+template <typename H, typename G>
+__device__ __forceinline__ H tail_row_sum(const Csr<H> &A, const H *val, int i, bool valid, int lane, int W, G gather) {
+    H sum = H(0);
+    if (valid) {
+        const int e = A.row[i + 1];
+        for (int j = A.row[i] + lane; j < e; j += W) sum += val[j] * gather(A.col[j]);
+    }
+    for (int o = W / 2; o > 0; o >>= 1) sum += __shfl_down(sum, o, W);
+    return sum;
+}
+
+template <typename H>
+__global__ void __launch_bounds__(TAIL_BLOCK) k_vcycle_tail(TailArgs<H> a, const BatchState *__restrict__ st) {
+    const int batch = blockIdx.y;
+    if (st[batch].done != RUNNING) return;
+    H *cur[TAIL_MAX], *oth[TAIL_MAX];
+    for (int k = 0; k < a.nl; ++k) {
+        const int n = a.n[k], W = a.wa[k];
+        const size_t off = (size_t)batch * n;
+        const Csr<H> &A = a.A[k];
+        const H *val = A.val + (size_t)batch * A.valStride;
+        const H *dv = a.dinv[k] + (A.valStride ? off : 0);
+        const H *bl = a.b[k] + off;
+        cur[k] = a.x[k];
+        oth[k] = a.y[k];
+        H *res = a.res[k] + off;
+        const int lane = threadIdx.x % W, grp = threadIdx.x / W, gpb = TAIL_BLOCK / W;
+        for (int base = 0; base < n; base += gpb) {
+            const int i = base + grp;
+            const bool valid = i < n;
+            const H sum = tail_row_sum<H>(A, val, i, valid, lane, W, [&](int c) { return dv[c] * bl[c]; });
+            if (valid && lane == 0) {
+                cur[k][off + i] = a.omega * dv[i] * bl[i];
+                res[i] = bl[i] - a.omega * sum;
+            }
+        }
+        __syncthreads();
+        for (int sw = 1; sw < a.pre; ++sw) {
+            const H *xc = cur[k] + off;
+            for (int base = 0; base < n; base += gpb) {
+                const int i = base + grp;
+                const bool valid = i < n;
+                const H sum = tail_row_sum<H>(A, val, i, valid, lane, W, [&](int c) { return xc[c]; });
+                if (valid && lane == 0) oth[k][off + i] = xc[i] + a.omega * dv[i] * (bl[i] - sum);
+            }
+            __syncthreads();
+            H *t = cur[k]; cur[k] = oth[k]; oth[k] = t;
+        }
+        if (a.pre > 1) {
+            const H *xc = cur[k] + off;
+            for (int base = 0; base < n; base += gpb) {
+                const int i = base + grp;
+                const bool valid = i < n;
+                const H sum = tail_row_sum<H>(A, val, i, valid, lane, W, [&](int c) { return xc[c]; });
+                if (valid && lane == 0) res[i] = bl[i] - sum;
+            }
+            __syncthreads();
+        }
+        // restriction into the next level's right-hand side (or the coarsest one)
+        const Csr<H> &R = a.R[k];
+        const int ncR = R.n, WR = a.wr[k];
+        H *bn = (k + 1 < a.nl ? a.b[k + 1] : a.cb) + (size_t)batch * ncR;
+        const int laneR = threadIdx.x % WR, grpR = threadIdx.x / WR, gpbR = TAIL_BLOCK / WR;
+        for (int base = 0; base < ncR; base += gpbR) {
+            const int i = base + grpR;
+            const bool valid = i < ncR;
+            const H sum = tail_row_sum<H>(R, R.val, i, valid, laneR, WR, [&](int c) { return res[c]; });
+            if (valid && laneR == 0) bn[i] = sum;
+        }
+        __syncthreads();
+    }
+    {
+        const H *pv = a.pinv + (a.coarseBatched ? (size_t)batch * a.nc * a.nc : 0);
+        const H *cb = a.cb + (size_t)batch * a.nc;
+        for (int i = threadIdx.x; i < a.nc; i += TAIL_BLOCK) {
+            double sum = 0;
+            for (int j = 0; j < a.nc; ++j) sum += double(pv[(size_t)i * a.nc + j]) * double(cb[j]);
+            a.cx[(size_t)batch * a.nc + i] = H(sum);
+        }
+    }
+    __syncthreads();
+    for (int k = a.nl - 1; k >= 0; --k) {
+        const int n = a.n[k];
+        const size_t off = (size_t)batch * n;
+        const int ncoarse = k + 1 < a.nl ? a.n[k + 1] : a.nc;
+        const H *xc = (k + 1 < a.nl ? cur[k + 1] : a.cx) + (size_t)batch * ncoarse;
+        const Csr<H> &P = a.P[k];
+        const int WP = a.wp[k];
+        {
+            const int lane = threadIdx.x % WP, grp = threadIdx.x / WP, gpb = TAIL_BLOCK / WP;
+            for (int base = 0; base < n; base += gpb) {
+                const int i = base + grp;
+                const bool valid = i < n;
+                const H sum = tail_row_sum<H>(P, P.val, i, valid, lane, WP, [&](int c) { return xc[c]; });
+                if (valid && lane == 0) cur[k][off + i] += sum;
+            }
+        }
+        __syncthreads();
+        const Csr<H> &A = a.A[k];
+        const H *val = A.val + (size_t)batch * A.valStride;
+        const H *dv = a.dinv[k] + (A.valStride ? off : 0);
+        const H *bl = a.b[k] + off;
+        const int W = a.wa[k];
+        const int lane = threadIdx.x % W, grp = threadIdx.x / W, gpb = TAIL_BLOCK / W;
+        for (int sw = 0; sw < a.post; ++sw) {
+            const H *xcur = cur[k] + off;
+            for (int base = 0; base < n; base += gpb) {
+                const int i = base + grp;
+                const bool valid = i < n;
+                const H sum = tail_row_sum<H>(A, val, i, valid, lane, W, [&](int c) { return xcur[c]; });
+                if (valid && lane == 0) oth[k][off + i] = xcur[i] + a.omega * dv[i] * (bl[i] - sum);
+            }
+            __syncthreads();
+            H *t = cur[k]; cur[k] = oth[k]; oth[k] = t;
+        }
+    }
+}
+
 static int row_group_width(double nnzPerRow) {
     if (nnzPerRow <= 3.0) return 1;
     if (nnzPerRow <= 6.0) return 2;
@@ -1617,6 +1755,22 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
         wr[l] = row_group_width(double(lv[l].nnz_r) / std::max(1, lv[l].n_coarse));
         wp[l] = row_group_width(double(lv[l].nnz_p) / std::max(1, lv[l].n));
     }
+    // first level handled by the one-workgroup tail (never the finest level)
+    int lt = L;
+    for (int l = 1; l < L; ++l)
+        if (lv[l].n <= TAIL_ROWS && L - l <= TAIL_MAX) { lt = l; break; }
+    TailArgs<H> tail{};
+    if (lt < L) {
+        tail.nl = L - lt; tail.nc = nc; tail.pre = pre; tail.post = post; tail.coarseBatched = coarseBatched;
+        tail.omega = omega; tail.pinv = pinv; tail.cb = lb(L); tail.cx = cx;
+        for (int l = lt; l < L; ++l) {
+            const int k = l - lt;
+            tail.n[k] = lv[l].n; tail.wa[k] = wa[l]; tail.wr[k] = wr[l]; tail.wp[k] = wp[l];
+            tail.A[k] = LA[l]; tail.R[k] = LR[l]; tail.P[k] = LP[l];
+            tail.dinv[k] = reinterpret_cast<const H *>(lv[l].dinv);
+            tail.x[k] = lx(l); tail.y[k] = ly(l); tail.res[k] = lres(l); tail.b[k] = lb(l);
+        }
+    }
     auto vcycle = [&](hipStream_t strm) {
         const double *shift = nullptr;
         if (project) {
@@ -1627,6 +1781,7 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
         }
         std::vector<H *> cur(L), oth(L);
         for (int l = 0; l < L; ++l) {
+            if (l == lt) break; // levels lt.. run in the one-workgroup tail
             const dim3 g = lgrid(lv[l].n, wa[l]);
             cur[l] = lx(l);
             oth[l] = ly(l);
@@ -1647,8 +1802,19 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
             W_DISPATCH(wr[l],
                 PP_TIMED(PP_K_AMG_RESTRICT, lv[l].n_coarse, strm, hipLaunchKernelGGL((k_spmv_rect<W, H>), lgrid(lv[l].n_coarse, wr[l]), dim3(BLOCK), 0, strm, LR[l], resl, lv[l].n, lb(l + 1), 0, st)));
         }
-        PP_TIMED(PP_K_AMG_COARSE, nc, strm, hipLaunchKernelGGL((k_coarse_solve<H>), dim3((nc + 127) / 128, B), dim3(128), 0, strm, pinv, nc, lb(L), cx, st, coarseBatched));
-        for (int l = L - 1; l >= 0; --l) {
+        if (lt < L) {
+            PP_TIMED(PP_K_AMG_TAIL, lv[lt].n, strm,
+                hipLaunchKernelGGL((k_vcycle_tail<H>), dim3(1, B), dim3(TAIL_BLOCK), 0, strm, tail, st));
+            for (int l = lt; l < L; ++l) {
+                // the tail leaves level l's correction in x after an even number of sweeps
+                const bool inX = ((pre - 1) + post) % 2 == 0;
+                cur[l] = inX ? lx(l) : ly(l);
+                oth[l] = inX ? ly(l) : lx(l);
+            }
+        } else {
+            PP_TIMED(PP_K_AMG_COARSE, nc, strm, hipLaunchKernelGGL((k_coarse_solve<H>), dim3((nc + 127) / 128, B), dim3(128), 0, strm, pinv, nc, lb(L), cx, st, coarseBatched));
+        }
+        for (int l = std::min(L, lt) - 1; l >= 0; --l) {
             const dim3 g = lgrid(lv[l].n, wa[l]);
             const H *xc = l + 1 < L ? cur[l + 1] : cx;
             const int ncoarse = l + 1 < L ? lv[l + 1].n : nc;
@@ -1753,17 +1919,26 @@ __global__ void k_segmented_gather(const int32_t *__restrict__ seg, const int32_
 }
 
 
-// out[e] = sum_{t in seg(e)} w[t] * in[src[t]] accumulated in double (Galerkin map).
-template <typename TA, typename TO>
-__global__ void k_galerkin_apply(const int32_t *__restrict__ seg, const int32_t *__restrict__ src,
-                                 const double *__restrict__ w, int nOut, const TA *__restrict__ in,
-                                 TO *__restrict__ out, int inStride, int outStride) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nOut) return;
+// out[e] = sum_{t in seg(e)} w[t] * in[src[t]] accumulated in double (Galerkin map). W lanes
+// share an entry: lane j takes terms j, j+W, ... (neighbouring lanes read neighbouring terms),
+// then a fixed shuffle tree combines the W partial sums.
+template <int W, typename TA, typename TO>
+__global__ void __launch_bounds__(256) k_galerkin_apply(const int32_t *__restrict__ seg, const int32_t *__restrict__ src,
+                                                        const double *__restrict__ w, int nOut,
+                                                        const TA *__restrict__ in, TO *__restrict__ out, int inStride,
+                                                        int outStride) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = gid / W;
+    const int lane = gid % W;
     const TA *vin = in + (size_t)blockIdx.y * inStride;
     double acc = 0.0;
-    for (int t = seg[e]; t < seg[e + 1]; ++t) acc += w[t] * double(vin[src[t]]);
-    out[(size_t)blockIdx.y * outStride + e] = TO(acc);
+    if (e < nOut) {
+        const int end = seg[e + 1];
+        for (int t = seg[e] + lane; t < end; t += W) acc += w[t] * double(vin[src[t]]);
+    }
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) acc += __shfl_down(acc, o, W);
+    if (e < nOut && lane == 0) out[(size_t)blockIdx.y * outStride + e] = TO(acc);
 }
 
 // Galerkin product C = R A P into the fixed pattern of C (one thread per coarse row,
@@ -1993,22 +2168,36 @@ int pp_segmented_gather(const int32_t *seg, const int32_t *src, const void *w, i
 }
 
 
-int pp_galerkin_apply(const int32_t *seg, const int32_t *src, const double *w, int32_t n_out, const void *a_val,
+int pp_galerkin_apply(const int32_t *seg, const int32_t *src, const double *w, int32_t n_out, int64_t nterms,
+                      const void *a_val,
                       int32_t a_dtype, void *c_val, int32_t c_dtype, int32_t nmat, int32_t a_stride, int32_t c_stride,
                       void *stream) {
     PP_CHECK_ARG(seg && src && w && a_val && c_val && n_out >= 0 && nmat >= 1, "null argument");
     if (n_out == 0) return PP_OK;
-    const dim3 g((n_out + 255) / 256, nmat);
     hipStream_t s = as_stream(stream);
-    if (a_dtype == PP_F64 && c_dtype == PP_F64)
-        PP_TIMED(PP_K_GALERKIN, n_out, s, hipLaunchKernelGGL((k_galerkin_apply<double, double>), g, dim3(256), 0, s, seg, src, w, n_out, (const double *)a_val, (double *)c_val, a_stride, c_stride));
-    else if (a_dtype == PP_F64 && c_dtype == PP_F32)
-        PP_TIMED(PP_K_GALERKIN, n_out, s, hipLaunchKernelGGL((k_galerkin_apply<double, float>), g, dim3(256), 0, s, seg, src, w, n_out, (const double *)a_val, (float *)c_val, a_stride, c_stride));
-    else if (a_dtype == PP_F32 && c_dtype == PP_F32)
-        PP_TIMED(PP_K_GALERKIN, n_out, s, hipLaunchKernelGGL((k_galerkin_apply<float, float>), g, dim3(256), 0, s, seg, src, w, n_out, (const float *)a_val, (float *)c_val, a_stride, c_stride));
-    else if (a_dtype == PP_F32 && c_dtype == PP_F64)
-        PP_TIMED(PP_K_GALERKIN, n_out, s, hipLaunchKernelGGL((k_galerkin_apply<float, double>), g, dim3(256), 0, s, seg, src, w, n_out, (const float *)a_val, (double *)c_val, a_stride, c_stride));
+    // lanes per entry from the mean number of terms
+    const double mean = double(nterms) / double(n_out);
+    const int W = mean > 24 ? 16 : (mean > 12 ? 8 : (mean > 6 ? 4 : (mean > 3 ? 2 : 1)));
+    const dim3 g((unsigned)(((size_t)n_out * W + 255) / 256), nmat);
+#define GAL_LAUNCH(WV, TA, TO)                                                                                  \
+    PP_TIMED(PP_K_GALERKIN, n_out, s,                                                                          \
+             hipLaunchKernelGGL((k_galerkin_apply<WV, TA, TO>), g, dim3(256), 0, s, seg, src, w, n_out,          \
+                                (const TA *)a_val, (TO *)c_val, a_stride, c_stride))
+#define GAL_W(TA, TO)                                                                                           \
+    switch (W) {                                                                                                \
+    case 1: GAL_LAUNCH(1, TA, TO); break;                                                                      \
+    case 2: GAL_LAUNCH(2, TA, TO); break;                                                                      \
+    case 4: GAL_LAUNCH(4, TA, TO); break;                                                                      \
+    case 8: GAL_LAUNCH(8, TA, TO); break;                                                                      \
+    default: GAL_LAUNCH(16, TA, TO); break;                                                                    \
+    }
+    if (a_dtype == PP_F64 && c_dtype == PP_F64) { GAL_W(double, double) }
+    else if (a_dtype == PP_F64 && c_dtype == PP_F32) { GAL_W(double, float) }
+    else if (a_dtype == PP_F32 && c_dtype == PP_F32) { GAL_W(float, float) }
+    else if (a_dtype == PP_F32 && c_dtype == PP_F64) { GAL_W(float, double) }
     else { set_error("pp_galerkin_apply: dtype"); return PP_ERR_ARG; }
+#undef GAL_W
+#undef GAL_LAUNCH
     PP_LAUNCH_CHECK();
     return PP_OK;
 }

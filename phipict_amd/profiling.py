@@ -16,7 +16,7 @@ KERNEL_NAMES = {
     27: "cg_small", 28: "bicg_small",
     30: "pcg_dots", 31: "pcg_direction", 32: "pcg_update", 33: "amg_presmooth", 34: "amg_jacobi",
     35: "amg_level_residual", 36: "amg_restrict", 37: "amg_prolong", 38: "amg_coarse", 39: "mean", 40: "subtract_mean",
-    41: "amg_post_jacobi", 50: "spmv", 51: "galerkin",
+    41: "amg_post_jacobi", 42: "amg_tail", 50: "spmv", 51: "galerkin",
 }
 
 

@@ -276,7 +276,7 @@ def galerkin_refresh(hierarchy: AMGHierarchy, csr_mat: CSRmatrix, *, project_con
             stream = _lib.stream_ptr(A.value.device)
             if hierarchy.levels[i - 1].gmap is not None:
                 seg, src, w = hierarchy.levels[i - 1].gmap
-                _lib.call("pp_galerkin_apply", _lib.ptr(seg), _lib.ptr(src), _lib.ptr(w), old.A.getNnz(),
+                _lib.call("pp_galerkin_apply", _lib.ptr(seg), _lib.ptr(src), _lib.ptr(w), old.A.getNnz(), src.numel(),
                           _lib.ptr(prev.A.value), _lib.dtype_code(prev.A.getDtype()), _lib.ptr(cval),
                           _lib.dtype_code(A.getDtype()), 1, 0, 0, stream)
             else:
