@@ -254,7 +254,9 @@ def main(argv=None) -> int:
         import pstats
 
         buf = io.StringIO()
-        pstats.Stats(prof, stream=buf).sort_stats("tottime").print_stats(45)
+        st = pstats.Stats(prof, stream=buf)
+        st.sort_stats("tottime").print_stats(30)
+        st.sort_stats("cumulative").print_stats(30)
         print(buf.getvalue(), file=sys.stderr)
     timings = profiling.read()
     print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)

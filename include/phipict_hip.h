@@ -93,7 +93,12 @@ typedef struct pp_csr {
     int32_t n;
     int32_t nnz;
     int32_t num_matrices;   /* 1 = shared by all right-hand sides */
-    int32_t pad;
+    int32_t ell_width;      /* > 0: the ELL copy below holds the same matrix */
+    /* Optional ELL copy (no reference counterpart; the library's own coalesced layout of the
+     * same entries): entry k of row i at [k][i], rows padded with zero values on the row's
+     * own column. Column order within a row equals the CSR order. */
+    const int32_t *ell_col; /* [ell_width][n] */
+    void *ell_val;          /* [num_matrices][ell_width][n] */
 } pp_csr;
 
 /* Host-side view of a domain (the reference's DomainGPU, domain_structs_gpu.h:334-445). */
@@ -290,6 +295,12 @@ int pp_segmented_gather(const int32_t *seg, const int32_t *src, const void *w, i
 int pp_galerkin_rap(const int32_t *r_row, const int32_t *r_col, const double *r_val, const pp_csr *A, int32_t a_dtype,
                     const int32_t *p_row, const int32_t *p_col, const double *p_val, const int32_t *c_row,
                     const int32_t *c_col, void *c_val, int32_t c_n, int32_t c_nnz, int32_t c_dtype, void *stream);
+/* ELL copy of a CSR matrix (the layout of pp_csr.ell_col / ell_val): entry k of row i at
+ * [k][i], rows padded to `width` entries with the row's own column and zero values. Either
+ * output may be NULL; no reference counterpart. */
+int pp_ell_from_csr(const int32_t *row, const int32_t *col, const void *val, int32_t n, int32_t nnz, int32_t nmat,
+                    int32_t width, int32_t dtype, int32_t *ell_col, void *ell_val, void *stream);
+
 /* Galerkin map (the precomputed product map of _galerkin_map, src/phipict/solvers/amg.py:
  * 1114-1180, used here for every refresh): host construction for fixed R, P, fine pattern A
  * and coarse pattern C -- c[e] = sum_{t in seg(e)} w[t] * a[src[t]], seg [nnz_c+1] -- and its

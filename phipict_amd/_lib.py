@@ -67,7 +67,9 @@ class CsrDesc(C.Structure):
         ("n", C.c_int32),
         ("nnz", C.c_int32),
         ("num_matrices", C.c_int32),
-        ("pad", C.c_int32),
+        ("ell_width", C.c_int32),
+        ("ell_col", C.c_void_p),
+        ("ell_val", C.c_void_p),
     ]
 
 
@@ -168,6 +170,7 @@ SIGNATURES = {
     "pp_galerkin_rap": [_vp, _vp, _vp, _P(CsrDesc), _i32, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _vp],
     "pp_inverse_diagonal": [_P(CsrDesc), _i32, _vp, _i32, _vp],
     "pp_cast": [_vp, _i32, _vp, _i32, _i64, _vp],
+    "pp_ell_from_csr": [_vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _vp],
     "pp_galerkin_map_build": [_i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _P(_vp)],
     "pp_galerkin_map_size": [_vp, _P(_i64)],
     "pp_galerkin_map_copy": [_vp, _vp, _vp, _vp],

@@ -60,9 +60,12 @@ __device__ void cell_fluxes(const pp_domain &dom, const pp_block_desc *__restric
 }
 
 // Emit the 2D+1 candidate entries of a row in ascending column order, first occurrence
-// first among equal columns (advection.cu:286-300). Returns the number of entries.
+// first among equal columns (advection.cu:286-300). Returns the number of entries. With
+// ell_out the values are also written to the ELL copy (entry k at ell_out[k * n]), the row
+// padded with zeros to 2D+1 entries.
 template <typename T, int D>
-__device__ __forceinline__ int emit_row(int *idx, T *vals, bool divide, T det, int32_t *col_out, T *val_out) {
+__device__ __forceinline__ int emit_row(int *idx, T *vals, bool divide, T det, int32_t *col_out, T *val_out,
+                                        T *ell_out = nullptr, int n = 0) {
     int written = 0;
 #pragma unroll
     for (int it = 0; it < 2 * D + 1; ++it) {
@@ -75,9 +78,12 @@ __device__ __forceinline__ int emit_row(int *idx, T *vals, bool divide, T det, i
         if (divide) { v = v / det; vals[best] = v; }
         if (col_out) col_out[written] = idx[best];
         val_out[written] = v;
+        if (ell_out) ell_out[(size_t)written * n] = v;
         idx[best] = -1;
         ++written;
     }
+    if (ell_out)
+        for (int k = written; k < 2 * D + 1; ++k) ell_out[(size_t)k * n] = T(0);
     return written;
 }
 
@@ -186,7 +192,9 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_advection_matrix(pp_domain dom) {
     rv[0] = diag;
     const int32_t start = dom.C.row[g];
     T *val = reinterpret_cast<T *>(dom.C.val) + (size_t)env * dom.C.nnz + start;
-    emit_row<T, D>(idx, rv, true, det, nullptr, val);
+    T *ell = dom.C.ell_val ? reinterpret_cast<T *>(dom.C.ell_val) + ((size_t)env * (2 * D + 1)) * dom.num_cells + g
+                           : nullptr;
+    emit_row<T, D>(idx, rv, true, det, nullptr, val, ell, dom.num_cells);
     reinterpret_cast<T *>(dom.A)[(size_t)env * dom.num_cells + g] = rv[0];
 }
 
@@ -351,7 +359,9 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_pressure_matrix(pp_domain dom) {
     }
     const int32_t start = dom.P.row[g];
     T *val = reinterpret_cast<T *>(dom.P.val) + (size_t)env * dom.P.nnz + start;
-    emit_row<T, D>(idx, rv, false, T(1), nullptr, val);
+    T *ell = dom.P.ell_val ? reinterpret_cast<T *>(dom.P.ell_val) + ((size_t)env * (2 * D + 1)) * dom.num_cells + g
+                           : nullptr;
+    emit_row<T, D>(idx, rv, false, T(1), nullptr, val, ell, dom.num_cells);
 }
 
 // PISO_build_pressure_rhs, pressure.cu:349-522 (CENTRAL, boundary sources on)
@@ -361,21 +371,41 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_pressure_rhs(pp_domain dom) {
     const T dt = reinterpret_cast<const T *>(dom.time_step)[env];
     const T rDiag = 1 / reinterpret_cast<const T *>(dom.A)[(size_t)env * dom.num_cells + g];
     const T det = Metric<T, D>::det(b, f);
-    const int32_t cs = dom.C.row[g];
-    const int rs = dom.C.row[g + 1] - cs;
-    const T *cval = reinterpret_cast<const T *>(dom.C.val) + (size_t)env * dom.C.nnz + cs;
-    const int32_t *ccol = dom.C.col + cs;
     const T *vres = reinterpret_cast<const T *>(dom.velocity_result) + (size_t)env * D * dom.num_cells;
     const T *vel = reinterpret_cast<const T *>(b.velocity) + (size_t)env * D * b.cells;
     T *prhs = reinterpret_cast<T *>(dom.pressure_rhs) + (size_t)env * D * dom.num_cells;
+    // off-diagonal part of C times the predicted velocity, all components in one pass over the row
+    T Hs[D];
+#pragma unroll
+    for (int dim = 0; dim < D; ++dim) Hs[dim] = T(0);
+    if (dom.C.ell_val) {
+        const int n = dom.num_cells;
+        const T *ev = reinterpret_cast<const T *>(dom.C.ell_val) + (size_t)env * (2 * D + 1) * n + g;
+        const int32_t *ec = dom.C.ell_col + g;
+#pragma unroll
+        for (int k = 0; k < 2 * D + 1; ++k) {
+            const int ix = ec[(size_t)k * n];
+            const T cv = ev[(size_t)k * n];
+            if (ix != g)
+#pragma unroll
+                for (int dim = 0; dim < D; ++dim) Hs[dim] += cv * vres[ix + (size_t)n * dim];
+        }
+    } else {
+        const int32_t cs = dom.C.row[g];
+        const int rs = dom.C.row[g + 1] - cs;
+        const T *cval = reinterpret_cast<const T *>(dom.C.val) + (size_t)env * dom.C.nnz + cs;
+        const int32_t *ccol = dom.C.col + cs;
+        for (int i = 0; i < rs && i < 2 * D + 1; ++i) {
+            const int ix = ccol[i];
+            if (ix != g)
+#pragma unroll
+                for (int dim = 0; dim < D; ++dim) Hs[dim] += cval[i] * vres[ix + (size_t)dom.num_cells * dim];
+        }
+    }
 #pragma unroll
     for (int dim = 0; dim < D; ++dim) {
         const T velOld = vel[(size_t)dim * b.cells + f] / dt;
-        T H = T(0);
-        for (int i = 0; i < rs && i < 2 * D + 1; ++i) {
-            const int ix = ccol[i];
-            if (ix != g) H += cval[i] * vres[ix + (size_t)dom.num_cells * dim];
-        }
+        const T H = Hs[dim];
         T S = boundary_source<T, D>(dom, b, p, f, dim, env);
         S /= det;
         S += vsource<T>(dom, b, f, dim, env);
@@ -481,7 +511,8 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_epot_matrix(pp_domain dom) {
     }
     const int32_t start = dom.E.row[g];
     T *val = reinterpret_cast<T *>(dom.E.val) + start;
-    emit_row<T, D>(idx, rv, false, T(1), nullptr, val);
+    T *ell = dom.E.ell_val ? reinterpret_cast<T *>(dom.E.ell_val) + g : nullptr;
+    emit_row<T, D>(idx, rv, false, T(1), nullptr, val, ell, dom.num_cells);
 }
 
 // k_computeEpotRHS, potential.cu:272-322. vf: [batch][d][n] -> div [batch][n]

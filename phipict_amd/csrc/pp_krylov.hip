@@ -45,7 +45,23 @@ struct Csr {
     int n;
     int valStride; // 0: matrix shared by all right-hand sides
     int rpm;       // right-hand sides per matrix
+    int ell_w;     // > 0: ELL copy of the same entries (entry k of row i at [k][i])
+    const int32_t *__restrict__ ell_col;
+    const T *__restrict__ ell_val;
 };
+
+// Matrix view of a descriptor; the ELL copy is used when present.
+template <typename T>
+static Csr<T> csr_view(const pp_csr *A, int rpm) {
+    Csr<T> C{A->row, A->col, reinterpret_cast<const T *>(A->val), A->n, A->num_matrices > 1 ? A->nnz : 0, rpm, 0,
+             nullptr, nullptr};
+    if (A->ell_width > 0 && A->ell_col && A->ell_val) {
+        C.ell_w = A->ell_width;
+        C.ell_col = A->ell_col;
+        C.ell_val = reinterpret_cast<const T *>(A->ell_val);
+    }
+    return C;
+}
 
 // ---------------- reductions (wave64 shuffles + LDS tree) ----------------
 __device__ __forceinline__ double wave_sum(double v) {
@@ -663,14 +679,17 @@ __global__ void __launch_bounds__(BLOCK) k_residual(Csr<T> A, const T *__restric
     for (int k = 0; k < NB; ++k) acc[k][0] = 0.0;
     if (any) {
         const T *val = A.val + (size_t)grp.m * A.valStride;
+        const T *ellv = A.ell_val + (A.valStride ? (size_t)grp.m * A.ell_w * n : 0);
+        const int ew = A.ell_w;
         for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
             T sum[NB];
 #pragma unroll
             for (int k = 0; k < NB; ++k) sum[k] = T(0);
-            const int e = A.row[i + 1];
-            for (int j = A.row[i]; j < e; ++j) {
-                const int c = A.col[j];
-                const T a = val[j];
+            const int jb = ew ? 0 : A.row[i], je = ew ? ew : A.row[i + 1];
+            for (int j = jb; j < je; ++j) {
+                const size_t eo = (size_t)j * n + i;
+                const int c = ew ? A.ell_col[eo] : A.col[j];
+                const T a = ew ? ellv[eo] : val[j];
 #pragma unroll
                 for (int k = 0; k < NB; ++k)
                     if (active[k]) sum[k] += a * x[(size_t)(grp.b0 + k) * n + c];
@@ -746,14 +765,17 @@ __global__ void __launch_bounds__(BLOCK) k_cg_direction(Csr<T> A, const T *__res
     for (int k = 0; k < NB; ++k) acc[k][0] = 0.0;
     if (any) {
         const T *val = A.val + (size_t)grp.m * A.valStride;
+        const T *ellv = A.ell_val + (A.valStride ? (size_t)grp.m * A.ell_w * n : 0);
+        const int ew = A.ell_w;
         for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
             T sum[NB];
 #pragma unroll
             for (int k = 0; k < NB; ++k) sum[k] = T(0);
-            const int e = A.row[i + 1];
-            for (int j = A.row[i]; j < e; ++j) {
-                const int c = A.col[j];
-                const T a = val[j];
+            const int jb = ew ? 0 : A.row[i], je = ew ? ew : A.row[i + 1];
+            for (int j = jb; j < je; ++j) {
+                const size_t eo = (size_t)j * n + i;
+                const int c = ew ? A.ell_col[eo] : A.col[j];
+                const T a = ew ? ellv[eo] : val[j];
 #pragma unroll
                 for (int k = 0; k < NB; ++k)
                     if (active[k]) {
@@ -940,14 +962,17 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_v(Csr<T> A, const T *__restrict_
     for (int k = 0; k < NB; ++k) acc[k][0] = 0.0;
     if (any) {
         const T *val = A.val + (size_t)grp.m * A.valStride;
+        const T *ellv = A.ell_val + (A.valStride ? (size_t)grp.m * A.ell_w * n : 0);
+        const int ew = A.ell_w;
         for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
             T sum[NB];
 #pragma unroll
             for (int k = 0; k < NB; ++k) sum[k] = T(0);
-            const int e = A.row[i + 1];
-            for (int j = A.row[i]; j < e; ++j) {
-                const int c = A.col[j];
-                const T a = val[j];
+            const int jb = ew ? 0 : A.row[i], je = ew ? ew : A.row[i + 1];
+            for (int j = jb; j < je; ++j) {
+                const size_t eo = (size_t)j * n + i;
+                const int c = ew ? A.ell_col[eo] : A.col[j];
+                const T a = ew ? ellv[eo] : val[j];
 #pragma unroll
                 for (int k = 0; k < NB; ++k)
                     if (active[k]) sum[k] += a * p[(size_t)(grp.b0 + k) * n + c];
@@ -985,14 +1010,17 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_t(Csr<T> A, const T *__restrict_
     for (int k = 0; k < NB; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.0;
     if (any) {
         const T *val = A.val + (size_t)grp.m * A.valStride;
+        const T *ellv = A.ell_val + (A.valStride ? (size_t)grp.m * A.ell_w * n : 0);
+        const int ew = A.ell_w;
         for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
             T sum[NB];
 #pragma unroll
             for (int k = 0; k < NB; ++k) sum[k] = T(0);
-            const int e = A.row[i + 1];
-            for (int j = A.row[i]; j < e; ++j) {
-                const int c = A.col[j];
-                const T a = val[j];
+            const int jb = ew ? 0 : A.row[i], je = ew ? ew : A.row[i + 1];
+            for (int j = jb; j < je; ++j) {
+                const size_t eo = (size_t)j * n + i;
+                const int c = ew ? A.ell_col[eo] : A.col[j];
+                const T a = ew ? ellv[eo] : val[j];
 #pragma unroll
                 for (int k = 0; k < NB; ++k)
                     if (active[k]) {
@@ -1275,13 +1303,14 @@ __global__ void k_coarse_solve(const H *__restrict__ pinv, int nc, const H *__re
     }
 }
 
-// ---- coarse tail of the V-cycle: all levels below a size threshold in one workgroup ----
+// ---- coarse tail of the V-cycle: the smallest levels in one workgroup ----
 // The levels lt..L-1, the dense coarse solve and the way back up run as phases of a single
 // workgroup per batch (barriers between phases), instead of ~4 latency-bound launches per
-// level. Row sums use the same lane groups and shuffle trees as the per-level kernels.
+// level. A level qualifies when one pass of the workgroup covers it (one row per work-item),
+// so every phase is a single dependent load chain.
 constexpr int TAIL_MAX = 12;
-constexpr int TAIL_ROWS = 8192;
 constexpr int TAIL_BLOCK = 1024;
+constexpr int TAIL_ROWS = TAIL_BLOCK;
 
 template <typename H>
 struct TailArgs {
@@ -1765,7 +1794,7 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
         tail.omega = omega; tail.pinv = pinv; tail.cb = lb(L); tail.cx = cx;
         for (int l = lt; l < L; ++l) {
             const int k = l - lt;
-            tail.n[k] = lv[l].n; tail.wa[k] = wa[l]; tail.wr[k] = wr[l]; tail.wp[k] = wp[l];
+            tail.n[k] = lv[l].n; tail.wa[k] = 1; tail.wr[k] = 1; tail.wp[k] = 1;
             tail.A[k] = LA[l]; tail.R[k] = LR[l]; tail.P[k] = LP[l];
             tail.dinv[k] = reinterpret_cast<const H *>(lv[l].dinv);
             tail.x[k] = lx(l); tail.y[k] = ly(l); tail.res[k] = lres(l); tail.b[k] = lb(l);
@@ -1874,6 +1903,25 @@ __global__ void k_spmv(Csr<T> A, const T *__restrict__ x, T *__restrict__ y, int
         T s = T(0);
         for (int j = A.row[i]; j < A.row[i + 1]; ++j) s += val[j] * x[(size_t)b * A.n + A.col[j]];
         y[(size_t)b * A.n + i] = s;
+    }
+}
+
+// ELL copy of a CSR matrix: entry k of row i at [k][i], rows padded to `width` with the
+// row's own column and zero values; rows longer than width are an error left to the caller
+// (the library's domain matrices have at most 2d+1 entries per row).
+template <typename T>
+__global__ void k_ell_from_csr(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
+                               const T *__restrict__ val, int n, int nnz, int width, int32_t *__restrict__ ellCol,
+                               T *__restrict__ ellVal) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int m = blockIdx.y;
+    const int b = row[i], e = min(row[i + 1], b + width);
+    for (int k = 0; k < width; ++k) {
+        const size_t o = (size_t)k * n + i;
+        const bool has = b + k < e;
+        if (ellCol && m == 0) ellCol[o] = has ? col[b + k] : i;
+        if (ellVal) ellVal[(size_t)m * width * n + o] = has ? val[(size_t)m * nnz + b + k] : T(0);
     }
 }
 
@@ -2032,8 +2080,9 @@ int pp_solve_linear(const pp_csr *A, const void *rhs, void *x, int32_t nrhs, int
     static size_t tcap = 0;
     static int *tbad = nullptr;
     PP_DISPATCH(dtype, {
-        Csr<T> C{A->row, A->col, reinterpret_cast<const T *>(A->val), A->n, A->num_matrices > 1 ? A->nnz : 0, rpm};
+        Csr<T> C = csr_view<T>(A, rpm);
         if (transpose_a && use_bicg) {
+            C.ell_w = 0;
             const size_t bytes = sizeof(T) * (size_t)A->nnz * A->num_matrices;
             if (int e = reserve(&tbuf, &tcap, std::max<size_t>(bytes, 8))) return e;
             if (!tbad) PP_HIP(hipMalloc((void **)&tbad, sizeof(int)));
@@ -2078,19 +2127,19 @@ int pp_amg_pcg_solve(const pp_csr *A, const pp_amg_level *levels, int32_t num_le
     const double *tp = ntols > 1 ? tols : &t1;
     const int nt = ntols > 1 ? ntols : 1;
     if (dtype == PP_F64 && hdtype == PP_F32) {
-        Csr<double> C{A->row, A->col, reinterpret_cast<const double *>(A->val), A->n, A->num_matrices > 1 ? A->nnz : 0, 1};
+        const Csr<double> C = csr_view<double>(A, 1);
         return amg_pcg<double, float>(C, levels, num_levels, reinterpret_cast<const float *>(coarse_pinv), nc, coarse_batched,
                                       reinterpret_cast<const double *>(rhs), reinterpret_cast<double *>(x), nrhs, maxit, tp, nt,
                                       normScale, return_best, project_constant, pre_sweeps, post_sweeps, omega, s, results);
     }
     if (dtype == PP_F64 && hdtype == PP_F64) {
-        Csr<double> C{A->row, A->col, reinterpret_cast<const double *>(A->val), A->n, A->num_matrices > 1 ? A->nnz : 0, 1};
+        const Csr<double> C = csr_view<double>(A, 1);
         return amg_pcg<double, double>(C, levels, num_levels, reinterpret_cast<const double *>(coarse_pinv), nc, coarse_batched,
                                        reinterpret_cast<const double *>(rhs), reinterpret_cast<double *>(x), nrhs, maxit, tp, nt,
                                        normScale, return_best, project_constant, pre_sweeps, post_sweeps, omega, s, results);
     }
     if (dtype == PP_F32 && hdtype == PP_F32) {
-        Csr<float> C{A->row, A->col, reinterpret_cast<const float *>(A->val), A->n, A->num_matrices > 1 ? A->nnz : 0, 1};
+        const Csr<float> C = csr_view<float>(A, 1);
         return amg_pcg<float, float>(C, levels, num_levels, reinterpret_cast<const float *>(coarse_pinv), nc, coarse_batched,
                                      reinterpret_cast<const float *>(rhs), reinterpret_cast<float *>(x), nrhs, maxit, tp, nt,
                                      normScale, return_best, project_constant, pre_sweeps, post_sweeps, omega, s, results);
@@ -2167,6 +2216,20 @@ int pp_segmented_gather(const int32_t *seg, const int32_t *src, const void *w, i
     return PP_OK;
 }
 
+
+int pp_ell_from_csr(const int32_t *row, const int32_t *col, const void *val, int32_t n, int32_t nnz, int32_t nmat,
+                    int32_t width, int32_t dtype, int32_t *ell_col, void *ell_val, void *stream) {
+    PP_CHECK_ARG(row && n >= 0 && width > 0 && nmat >= 1, "bad arguments");
+    PP_CHECK_ARG(!ell_col || col, "ell_col needs col");
+    PP_CHECK_ARG(!ell_val || val, "ell_val needs val");
+    if (n == 0) return PP_OK;
+    const dim3 g((n + 255) / 256, nmat);
+    PP_DISPATCH(dtype, hipLaunchKernelGGL((k_ell_from_csr<T>), g, dim3(256), 0, as_stream(stream), row, col,
+                                          reinterpret_cast<const T *>(val), n, nnz, width, ell_col,
+                                          reinterpret_cast<T *>(ell_val)));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
 
 int pp_galerkin_apply(const int32_t *seg, const int32_t *src, const double *w, int32_t n_out, int64_t nterms,
                       const void *a_val,

@@ -93,6 +93,29 @@ class CSRmatrix:
             if index.numel() > 0 and value.numel() % index.numel() != 0:
                 raise ValueError("value size must be a multiple of the index size")
             self.value, self.index, self.row = value, index, row
+        # optional ELL copy written by the library's assembly kernels (coalesced SpMV layout):
+        # valid while the value tensor is the one it was written with and was not modified
+        self.ell_col: Optional[torch.Tensor] = None
+        self.ell_val: Optional[torch.Tensor] = None
+        self.ell_width = 0
+        self._ell_key = None
+
+    def _value_key(self) -> tuple:
+        return (self.value.data_ptr(), self.value._version, self.value.numel())
+
+    def mark_ell_valid(self) -> None:
+        """The ELL copy now holds the current values (called after the library wrote both)."""
+        if self.ell_val is not None:
+            self._ell_key = self._value_key()
+
+    def ell_valid(self) -> bool:
+        return self.ell_val is not None and self._ell_key == self._value_key()
+
+    def share_ell_from(self, other: "CSRmatrix") -> None:
+        """Use ``other``'s ELL copy for this matrix (same pattern and value storage)."""
+        if other.ell_valid() and other.value.data_ptr() == self.value.data_ptr():
+            self.ell_col, self.ell_val, self.ell_width = other.ell_col, other.ell_val, other.ell_width
+            self._ell_key = self._value_key()
 
     def getRows(self) -> int:
         return self.row.numel() - 1
@@ -135,7 +158,9 @@ class CSRmatrix:
     def detach(self) -> None:
         self.value = self.value.detach()
 
-    def desc(self) -> _lib.CsrDesc:
+    def desc(self, ell_output: bool = False) -> _lib.CsrDesc:
+        """Library descriptor. The ELL copy is included when valid, or always with
+        ``ell_output`` (assembly kernels that write it)."""
         d = _lib.CsrDesc()
         d.row = _lib.ptr(self.row)
         d.col = _lib.ptr(self.index)
@@ -143,6 +168,10 @@ class CSRmatrix:
         d.n = self.getRows()
         d.nnz = self.getNnz()
         d.num_matrices = self.getBatchSize()
+        if self.ell_val is not None and (ell_output or self.ell_valid()):
+            d.ell_width = self.ell_width
+            d.ell_col = _lib.ptr(self.ell_col)
+            d.ell_val = _lib.ptr(self.ell_val)
         return d
 
     def __str__(self) -> str:
@@ -828,6 +857,11 @@ class Domain:
         index = torch.zeros(csr, dtype=torch.int32, device=dev)
         self.C = CSRmatrix(torch.zeros(csr * B, dtype=dt, device=dev), index, row)
         self.P = CSRmatrix(torch.zeros(csr * B, dtype=dt, device=dev), index, row)
+        width = 2 * dims + 1
+        self._ell_col = torch.empty(width * total, dtype=torch.int32, device=dev)
+        for M in (self.C, self.P):
+            M.ell_col, M.ell_width = self._ell_col, width
+            M.ell_val = torch.zeros(B * width * total, dtype=dt, device=dev)
         self.A = torch.zeros(B * total, dtype=dt, device=dev)
         self.velocityRHS = torch.zeros(B * total * dims, dtype=dt, device=dev)
         self.velocityResult = torch.zeros(B * total * dims, dtype=dt, device=dev)
@@ -840,6 +874,8 @@ class Domain:
         self._changed = True
         self.UpdateDomainData()
         _lib.call("pp_csr_pattern", C.byref(self._desc), _lib.ptr(row), _lib.ptr(index), _lib.stream_ptr(dev))
+        _lib.call("pp_ell_from_csr", _lib.ptr(row), _lib.ptr(index), None, total, csr, 1, width, _lib.dtype_code(dt),
+                  _lib.ptr(self._ell_col), None, _lib.stream_ptr(dev))
         if self._epot_enabled:
             self.SetupEpotOnDomain(*self._epot_flags)
 
@@ -908,12 +944,15 @@ class Domain:
         if self.epotResult is None or self.epotResult.numel() != self._batch * self.totalSize:
             self.epotResult = torch.zeros(self._batch * self.totalSize, dtype=self._dtype, device=self._device)
         self.Epot = CSRmatrix(torch.zeros(self.P.getNnz(), dtype=self._dtype, device=self._device), self.P.index, self.P.row)
+        self.Epot.ell_col, self.Epot.ell_width = self._ell_col, self.P.ell_width
+        self.Epot.ell_val = torch.zeros(self.P.ell_width * self.totalSize, dtype=self._dtype, device=self._device)
         for b in self._blocks:
             if b.epot is None:
                 b.CreateEpot()
         self._changed = True
         self.UpdateDomainData()
         _lib.call("pp_setup_epot_matrix", C.byref(self._desc), int(nonOrthoFlags), _lib.stream_ptr(self._device))
+        self.Epot.mark_ell_valid()
 
     # --- descriptors
     def _block_desc(self, b: Block) -> _lib.BlockDesc:
@@ -988,9 +1027,9 @@ class Domain:
         d.pressure_result = _lib.ptr(self.pressureResult)
         d.epot_rhs = _lib.ptr(self.epotRHS)
         d.epot_result = _lib.ptr(self.epotResult)
-        d.C = self.C.desc() if self.C is not None else _lib.CsrDesc()
-        d.P = self.P.desc() if self.P is not None else _lib.CsrDesc()
-        d.E = self.Epot.desc() if self.Epot is not None else _lib.CsrDesc()
+        d.C = self.C.desc(ell_output=True) if self.C is not None else _lib.CsrDesc()
+        d.P = self.P.desc(ell_output=True) if self.P is not None else _lib.CsrDesc()
+        d.E = self.Epot.desc(ell_output=True) if self.Epot is not None else _lib.CsrDesc()
         self._changed = False
 
     ATLAS_RING = 16
@@ -1031,7 +1070,10 @@ class Domain:
         return self._changed
 
     def desc(self) -> _lib.DomainDesc:
-        if self._changed:
+        d = self._desc
+        if (self._changed or (self.C is not None and d.C.val != self.C.value.data_ptr())
+                or (self.P is not None and d.P.val != self.P.value.data_ptr())
+                or (self.Epot is not None and d.E.val != self.Epot.value.data_ptr())):
             self.UpdateDomainData()
         return self._desc
 

@@ -46,6 +46,7 @@ def SetupAdvectionMatrix(domain: Domain, timeStep, nonOrthoFlags: int = 0, forPa
         raise NotImplementedError("passive-scalar advection is not implemented")
     d = _prep(domain, timeStep, nonOrthoFlags)
     _lib.call("pp_setup_advection_matrix", C.byref(d), int(nonOrthoFlags), _stream(domain))
+    domain.C.mark_ell_valid()
 
 
 def SetupAdvectionVelocity(domain: Domain, timeStep, nonOrthoFlags: int = 0, applyPressureGradient: bool = False) -> None:
@@ -59,6 +60,7 @@ def SetupPressureMatrix(domain: Domain, timeStep, nonOrthoMode: int = 0, useFace
         raise NotImplementedError("face-transform pressure Laplacians are not implemented")
     d = _prep(domain, timeStep, nonOrthoMode)
     _lib.call("pp_setup_pressure_matrix", C.byref(d), int(nonOrthoMode), _stream(domain))
+    domain.P.mark_ell_valid()
 
 
 def SetupPressureRHS(domain: Domain, timeStep, nonOrthoMode: int = 0, useFaceTransform: bool = False,
@@ -98,6 +100,7 @@ def SetupEpotMatrix(domain: Domain, nonOrthoFlags: int = 0, useFaceTransform: bo
         raise RuntimeError("domain.Epot is not allocated. Call domain.SetupEpotOnDomain() first.")
     d = _prep(domain, None, nonOrthoFlags)
     _lib.call("pp_setup_epot_matrix", C.byref(d), int(nonOrthoFlags), _stream(domain))
+    domain.Epot.mark_ell_valid()
 
 
 def ComputeEpotRHS(domain: Domain, vectorField: torch.Tensor, useFaceTransform: bool = False) -> torch.Tensor:
@@ -290,8 +293,9 @@ def pack_amg_levels(levels: Sequence[Sequence[torch.Tensor]], hd) -> tuple:
 def amg_pcg_packed(aRow, aCol, aVal, packed: tuple, coarsePinv: torch.Tensor, rhs: torch.Tensor, x: torch.Tensor,
                    maxIterations: int, tolerance: float, normalized: bool = True, returnBest: bool = False,
                    projectConstant: bool = False, preSweeps: int = 1, postSweeps: int = 1, omega: float = 2.0 / 3.0,
-                   tolerances: Sequence[float] = ()) -> list:
-    """AMGPCGSolve on levels already packed by ``pack_amg_levels``."""
+                   tolerances: Sequence[float] = (), ell: Optional[tuple] = None) -> list:
+    """AMGPCGSolve on levels already packed by ``pack_amg_levels``; ``ell`` = (columns, values,
+    width) of an ELL copy of the fine matrix, if one is valid."""
     lv, _keep = packed
     n = aRow.numel() - 1
     if rhs.numel() % n != 0 or rhs.numel() != x.numel():
@@ -307,6 +311,8 @@ def amg_pcg_packed(aRow, aCol, aVal, packed: tuple, coarsePinv: torch.Tensor, rh
     desc.row, desc.col, desc.val = _lib.ptr(aRow), _lib.ptr(aCol), _lib.ptr(aVal)
     desc.n, desc.nnz = n, aCol.numel()
     desc.num_matrices = max(1, aVal.numel() // max(1, aCol.numel()))
+    if ell is not None:
+        desc.ell_col, desc.ell_val, desc.ell_width = _lib.ptr(ell[0]), _lib.ptr(ell[1]), int(ell[2])
     res = (_lib.SolveResult * nrhs)()
     tl = list(tolerances)
     tarr = (C.c_double * max(1, len(tl)))(*(tl if tl else [float(tolerance)]))

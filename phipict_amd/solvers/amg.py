@@ -220,6 +220,7 @@ def build_amg_hierarchy(csr_mat: CSRmatrix, *, method: str = "ruge_stuben", max_
     for i, lv in enumerate(host):
         if i == 0:
             A = CSRmatrix(csr_mat.value.reshape(-1)[: csr_mat.getNnz()], csr_mat.index, csr_mat.row)
+            A.share_ell_from(csr_mat)
         else:
             ar, ac, av = lv["A"]
             A = CSRmatrix(_dev(av, device, dtype), _dev(ac, device), _dev(ar, device))
@@ -265,6 +266,7 @@ def galerkin_refresh(hierarchy: AMGHierarchy, csr_mat: CSRmatrix, *, project_con
     if csr_mat.getRows() != fine.n:
         raise ValueError("operator does not match the finest AMG level")
     A = CSRmatrix(csr_mat.value.reshape(-1)[: csr_mat.getNnz()], csr_mat.index, csr_mat.row)
+    A.share_ell_from(csr_mat)
     levels = [Level(n=fine.n, A=A, R_row=fine.R_row, R_col=fine.R_col, R_val=fine.R_val, P_row=fine.P_row,
                     P_col=fine.P_col, P_val=fine.P_val)]
     levels[0].gmap = fine.gmap
@@ -357,10 +359,12 @@ def amg_pcg_solve(rhs: torch.Tensor, x: torch.Tensor, hierarchy: AMGHierarchy, *
     pinv = hierarchy.coarse_pinv(pdtype)
     tols = [float(t) for t in tol] if isinstance(tol, (list, tuple)) else []
     tol_scalar = max(tols) if tols else float(tol)
-    infos = ops.amg_pcg_packed(A.row, A.index, A.value.to(rhs.dtype) if A.value.dtype != rhs.dtype else A.value,
+    same_dtype = A.value.dtype == rhs.dtype
+    ell = (A.ell_col, A.ell_val, A.ell_width) if (same_dtype and A.ell_valid()) else None
+    infos = ops.amg_pcg_packed(A.row, A.index, A.value if same_dtype else A.value.to(rhs.dtype),
                                packed, pinv, rhs.reshape(-1), x.view(-1), int(max_iter), tol_scalar, True,
                                bool(return_best_result), hierarchy.project_constant, hierarchy.num_pre_smooth,
-                               hierarchy.num_post_smooth, JACOBI_OMEGA, tols)
+                               hierarchy.num_post_smooth, JACOBI_OMEGA, tols, ell=ell)
     for i, info in enumerate(infos):
         if not info.isFiniteResidual:
             x.view(len(infos), -1)[i].zero_()

@@ -382,3 +382,44 @@ def test_galerkin_refresh_matches_oracle():
     h3 = amg.galerkin_refresh(h, A2, coarse=False)
     assert h3.levels[1] is h.levels[1]
     assert h3.coarse_pinv64 is h.coarse_pinv64
+
+
+@pytest.mark.parametrize("use_bicg", [False, True])
+def test_ell_copy_gives_identical_solves(use_bicg):
+    """The domain matrices carry an ELL copy written by the assembly kernels; solves through
+    it are bit-identical to solves through the CSR arrays alone (same entry order; padding
+    entries are zeros on the row's own column)."""
+    import phipict_amd as pa
+    from cases import build_domain
+
+    specs = cases.channel2d(260, 160, u0=1.0)
+    dom = build_domain(specs, 0.02, 0.01, torch.float64)
+    assert dom.getTotalSize() > SMALL_N
+    ts = torch.tensor([0.01], dtype=torch.float64)
+    pa.SetupAdvectionMatrix(dom, ts, 0)
+    pa.SetupPressureMatrix(dom, ts, 0)
+    A = dom.C if use_bicg else dom.P
+    assert A.ell_valid()
+    n = A.getRows()
+    width = A.ell_width
+    ev = to_np(A.ell_val).reshape(width, n)
+    ec = to_np(A.ell_col).reshape(width, n)
+    row, col, val = to_np(A.row), to_np(A.index), to_np(A.value)
+    for i in range(0, n, 997):
+        k = row[i + 1] - row[i]
+        np.testing.assert_array_equal(ec[:k, i], col[row[i]:row[i + 1]])
+        np.testing.assert_array_equal(ev[:k, i], val[row[i]:row[i + 1]])
+        assert (ec[k:, i] == i).all() and (ev[k:, i] == 0).all()
+    plain = pa.CSRmatrix(A.value.clone(), A.index, A.row)
+    b = torch.as_tensor(np.random.default_rng(13).standard_normal(n), device=DEV)
+    if not use_bicg:
+        b = b - b.mean()
+    xs = []
+    for M in (A, plain):
+        x = torch.zeros(n, dtype=torch.float64, device=DEV)
+        pa.SolveLinear(M, b, x, maxIterations=400, tolerance=1e-9, useBiCG=use_bicg, returnBestResult=not use_bicg)
+        xs.append(to_np(x))
+    np.testing.assert_array_equal(xs[0], xs[1])
+    # a modified value tensor invalidates the copy
+    A.value.mul_(1.0)
+    assert not A.ell_valid()
