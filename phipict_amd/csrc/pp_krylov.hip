@@ -1306,11 +1306,17 @@ __global__ void k_coarse_solve(const H *__restrict__ pinv, int nc, const H *__re
 // ---- coarse tail of the V-cycle: the smallest levels in one workgroup ----
 // The levels lt..L-1, the dense coarse solve and the way back up run as phases of a single
 // workgroup per batch (barriers between phases), instead of ~4 latency-bound launches per
-// level. A level qualifies when one pass of the workgroup covers it (one row per work-item),
-// so every phase is a single dependent load chain.
+// level. A workgroup has little memory-level parallelism, so only levels that one pass covers
+// with at least 8 lanes per row qualify (coarse rows are long: 40-50 entries).
 constexpr int TAIL_MAX = 12;
 constexpr int TAIL_BLOCK = 1024;
-constexpr int TAIL_ROWS = TAIL_BLOCK;
+constexpr int TAIL_ROWS = TAIL_BLOCK / 8;
+// lanes per row in the tail: the widest power of two (<= 32) that covers the level in one pass
+static int tail_width(int rows) {
+    int w = 32;
+    while (w > 1 && rows * w > TAIL_BLOCK) w >>= 1;
+    return w;
+}
 
 template <typename H>
 struct TailArgs {
@@ -1794,7 +1800,10 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
         tail.omega = omega; tail.pinv = pinv; tail.cb = lb(L); tail.cx = cx;
         for (int l = lt; l < L; ++l) {
             const int k = l - lt;
-            tail.n[k] = lv[l].n; tail.wa[k] = 1; tail.wr[k] = 1; tail.wp[k] = 1;
+            tail.n[k] = lv[l].n;
+            tail.wa[k] = tail_width(lv[l].n);
+            tail.wr[k] = tail_width(lv[l].n_coarse);
+            tail.wp[k] = tail_width(lv[l].n);
             tail.A[k] = LA[l]; tail.R[k] = LR[l]; tail.P[k] = LP[l];
             tail.dinv[k] = reinterpret_cast<const H *>(lv[l].dinv);
             tail.x[k] = lx(l); tail.y[k] = ly(l); tail.res[k] = lres(l); tail.b[k] = lb(l);

@@ -106,12 +106,16 @@ class AMGHierarchy:
             parent = self.__dict__.get("_packed_parent")
             pcache = getattr(parent, "_packed", {}) if parent is not None else {}
             if dtype in pcache and len(parent.levels) == len(self.levels):
-                # a finest-level refresh: copy the parent's descriptors, replace level 0
+                # a finest-level refresh: copy the parent's descriptors, point level 0 at the
+                # new operator values and inverse diagonal
                 arr, keep = pcache[dtype]
                 arr = type(arr).from_buffer_copy(arr)
-                first, keep0 = ops.pack_amg_levels(self.native_levels(dtype)[:1], dtype)
-                arr[0] = first[0]
-                cache[dtype] = (arr, list(keep) + keep0)
+                nat = self.native_levels(dtype)[0]
+                arr[0].a_val = nat[2].data_ptr()
+                arr[0].dinv = nat[3].data_ptr()
+                # keep the tensors of levels >= 1 (shared with the parent) and the new level 0
+                base = keep[0] if isinstance(keep, tuple) else keep
+                cache[dtype] = (arr, (base, [nat[2], nat[3]]))
             else:
                 cache[dtype] = ops.pack_amg_levels(self.native_levels(dtype), dtype)
             self.__dict__.pop("_packed_parent", None)
