@@ -34,7 +34,8 @@ struct BatchState {
 struct Control {
     unsigned int counter;
     int allDone;
-    int pad[2];
+    int zeroRhs; // every right-hand side is exactly zero: x = 0 without iterating
+    int pad;
 };
 
 template <typename T>
@@ -656,7 +657,7 @@ __global__ void k_init_state(BatchState *st, int B, double tol, int maxit, Contr
         s.bestIt = -1;
         st[b] = s;
     }
-    if (threadIdx.x == 0) { ctrl->counter = 0; ctrl->allDone = 0; }
+    if (threadIdx.x == 0) { ctrl->counter = 0; ctrl->allDone = 0; ctrl->zeroRhs = 0; }
 }
 
 // r = b - A x (+ r0 = r for BiCGStab); reset mode only acts at multiples of resetEvery.
@@ -674,9 +675,9 @@ __global__ void __launch_bounds__(BLOCK) k_residual(Csr<T> A, const T *__restric
     bool active[NB], any = false;
 #pragma unroll
     for (int k = 0; k < NB; ++k) { active[k] = k < grp.nb && isActive(st[grp.b0 + k]); any |= active[k]; }
-    double acc[NB][1];
+    double acc[NB][2];
 #pragma unroll
-    for (int k = 0; k < NB; ++k) acc[k][0] = 0.0;
+    for (int k = 0; k < NB; ++k) acc[k][0] = acc[k][1] = 0.0;
     if (any) {
         const T *val = A.val + (size_t)grp.m * A.valStride;
         const T *ellv = A.ell_val + (A.valStride ? (size_t)grp.m * A.ell_w * n : 0);
@@ -698,26 +699,53 @@ __global__ void __launch_bounds__(BLOCK) k_residual(Csr<T> A, const T *__restric
             for (int k = 0; k < NB; ++k)
                 if (active[k]) {
                     const size_t o = (size_t)(grp.b0 + k) * n + i;
-                    const T ri = b[o] - sum[k];
+                    const T bo = b[o];
+                    const T ri = bo - sum[k];
                     r[o] = ri;
                     if (r0) r0[o] = ri;
                     acc[k][0] += double(ri) * double(ri);
+                    acc[k][1] += double(bo) * double(bo);
                 }
         }
     }
-    store_partials<NB, 1>(acc, grp.b0, grp.nb, partial, sm);
+    store_partials<NB, 2>(acc, grp.b0, grp.nb, partial, sm);
 }
 
-// Finish of k_residual: rho, restart and (initial call) the converged-start test.
+// Finish of k_residual: rho, restart and (initial call) the converged-start test. In the
+// initial call with zeroShortcut, right-hand sides that are all exactly zero finish at once
+// and ctrl->zeroRhs asks for x = 0 (the reference's zero-RHS shortcut, piso_diff.py:867).
 __global__ void __launch_bounds__(BLOCK) k_fin_residual(BatchState *st, int B, const double *partial, int G,
-                                                        Control *ctrl, double normScale, int resetEvery, int bicg) {
-    __shared__ double sm[16];
+                                                        Control *ctrl, double normScale, int resetEvery, int bicg,
+                                                        int zeroShortcut = 0) {
+    __shared__ double sm[16 * 2];
     if (ctrl->allDone) return;
+    if (zeroShortcut && resetEvery <= 0) {
+        __shared__ int allZero; // reductions are valid in thread 0 only: broadcast the verdict
+        if (threadIdx.x == 0) allZero = 1;
+        for (int bb = 0; bb < B; ++bb) {
+            double red[2];
+            reduce_batch<2>(partial, bb, G, red, sm);
+            if (threadIdx.x == 0 && red[1] != 0.0) allZero = 0;
+        }
+        __syncthreads();
+        if (allZero) {
+            if (threadIdx.x == 0) {
+                for (int bb = 0; bb < B; ++bb) {
+                    st[bb].done = CONVERGED;
+                    st[bb].crit = 0.0;
+                    st[bb].usedIt = 0;
+                }
+                ctrl->zeroRhs = 1;
+                ctrl->allDone = 1;
+            }
+            return;
+        }
+    }
     for (int bb = 0; bb < B; ++bb) {
         const BatchState &sb = st[bb];
         if (!(sb.done == RUNNING && (resetEvery <= 0 || (sb.it > 0 && (sb.it % resetEvery) == 0)))) continue;
-        double red[1];
-        reduce_batch<1>(partial, bb, G, red, sm);
+        double red[2];
+        reduce_batch<2>(partial, bb, G, red, sm);
         if (threadIdx.x == 0) {
             BatchState &s = st[bb];
             s.rho = red[0];
@@ -1145,6 +1173,13 @@ __global__ void __launch_bounds__(BLOCK) k_fin_bicg_update(BatchState *st, int B
         }
         finish_last(st, B, ctrl);
     }
+}
+
+template <typename T>
+__global__ void k_zero_if_zero_rhs(T *x, size_t count, const Control *ctrl) {
+    if (!ctrl->zeroRhs) return;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        x[i] = T(0);
 }
 
 template <typename T>
@@ -1625,7 +1660,7 @@ static int cg_grid(const Csr<T> &A, const T *b, T *x, int B, int maxit, const do
     NB_DISPATCH(NBv,
         hipLaunchKernelGGL((k_residual<T, NB>), grid, dim3(BLOCK), 0, s, A, bw, x, r, (T *)nullptr, st, B, partial, ctrl,
                            normScale, 0));
-    hipLaunchKernelGGL(k_fin_residual, dim3(1), dim3(BLOCK), 0, s, st, B, partial, G, ctrl, normScale, 0, 0);
+    hipLaunchKernelGGL(k_fin_residual, dim3(1), dim3(BLOCK), 0, s, st, B, partial, G, ctrl, normScale, 0, 0, 1);
     PP_LAUNCH_CHECK();
     int parity = 0;
     auto body = [&](hipStream_t strm) {
@@ -1651,6 +1686,8 @@ static int cg_grid(const Csr<T> &A, const T *b, T *x, int B, int maxit, const do
     if (int e = run_loop(ws, s, maxit + 2, g_check_interval, body)) return e;
     if (returnBest)
         hipLaunchKernelGGL((k_restore_best<T>), dim3(std::min(4096, (n * B + 255) / 256)), dim3(256), 0, s, x, xbest, st, n, B);
+    hipLaunchKernelGGL((k_zero_if_zero_rhs<T>), dim3(std::min(4096, (n * B + 255) / 256)), dim3(256), 0, s, x,
+                       (size_t)n * B, ctrl);
     PP_LAUNCH_CHECK();
     return collect(ws, B, returnBest, s, res);
 }
@@ -1681,7 +1718,7 @@ static int bicg_grid(const Csr<T> &A, const T *b, T *x, int B, int maxit, const 
     PP_HIP(hipMemsetAsync(v, 0, sizeof(T) * (size_t)n * B, s));
     NB_DISPATCH(NBv,
         hipLaunchKernelGGL((k_residual<T, NB>), grid, dim3(BLOCK), 0, s, A, b, x, r, r0, st, B, partial, ctrl, normScale, 0));
-    hipLaunchKernelGGL(k_fin_residual, dim3(1), dim3(BLOCK), 0, s, st, B, partial, G, ctrl, normScale, 0, 1);
+    hipLaunchKernelGGL(k_fin_residual, dim3(1), dim3(BLOCK), 0, s, st, B, partial, G, ctrl, normScale, 0, 1, 1);
     PP_LAUNCH_CHECK();
     auto body = [&](hipStream_t strm) {
         PP_TIMED(PP_K_BICG_P, n, strm, hipLaunchKernelGGL((k_bicg_p<T>), gridP, dim3(BLOCK), 0, strm, n, r, p, v, st));
@@ -1699,6 +1736,9 @@ static int bicg_grid(const Csr<T> &A, const T *b, T *x, int B, int maxit, const 
             hipLaunchKernelGGL(k_fin_bicg_update, dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl, normScale));
     };
     if (int e = run_loop(ws, s, maxit + 2, g_check_interval, body)) return e;
+    hipLaunchKernelGGL((k_zero_if_zero_rhs<T>), dim3(std::min(4096, (n * B + 255) / 256)), dim3(256), 0, s, x,
+                       (size_t)n * B, ctrl);
+    PP_LAUNCH_CHECK();
     return collect(ws, B, 0, s, res);
 }
 

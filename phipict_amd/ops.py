@@ -13,6 +13,9 @@ import torch
 from . import _lib
 from .domain import ConvergenceCriterion, CSRmatrix, Domain
 
+# systems up to this many rows run the whole solve in one workgroup per RHS group
+SMALL_SYSTEM_ROWS = 32768
+
 # non-orthogonal flag bits (csrc/phipict/piso/non_ortho_flags.h)
 NON_ORTHO_DIRECT_MATRIX = 1
 NON_ORTHO_DIRECT_RHS = 2

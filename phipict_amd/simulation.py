@@ -252,7 +252,9 @@ class Simulation:
             return x, True
         if x is None:
             x = torch.zeros_like(rhs)
-        if not bool(rhs.eq(0).all()):
+        # the zero-RHS shortcut (piso_diff.py:867): systems above the on-chip size take it on the
+        # device (no host round trip); small ones are checked here
+        if csrMat.getRows() > ops.SMALL_SYSTEM_ROWS or not bool(rhs.eq(0).all()):
             infos = ops.SolveLinear(csrMat, rhs, x, maxIterations=max_iter, tolerance=abs_tol,
                                     convergenceCriterion=ops.ConvergenceCriterion.NORM2_NORMALIZED, useBiCG=use_BiCG,
                                     matrixRankDeficient=matrix_rank_deficient, residualResetSteps=residual_reset_step,

@@ -423,3 +423,27 @@ def test_ell_copy_gives_identical_solves(use_bicg):
     # a modified value tensor invalidates the copy
     A.value.mul_(1.0)
     assert not A.ell_valid()
+
+
+@pytest.mark.parametrize("use_bicg", [False, True])
+def test_zero_rhs_shortcut_on_device(use_bicg):
+    """Large systems take the zero right-hand-side shortcut on the device: every RHS zero gives
+    x = 0 at once (also from a non-zero start); one non-zero RHS solves all of them."""
+    import phipict_amd as pa
+
+    row, col, val = poisson(220, 170, shift=0.1) if not use_bicg else advection(220, 170)
+    n = row.size - 1
+    assert n > SMALL_N
+    A = csr(row, col, val, np.float64)
+    x = torch.ones(2 * n, dtype=torch.float64, device=DEV)
+    infos = pa.SolveLinear(A, torch.zeros(2 * n, dtype=torch.float64, device=DEV), x, tolerance=1e-10,
+                           useBiCG=use_bicg)
+    assert all(i.converged and i.usedIterations == 0 for i in infos)
+    assert float(x.abs().max()) == 0.0
+    b = torch.zeros(2 * n, dtype=torch.float64, device=DEV)
+    b[n + 5] = 1.0
+    x = torch.ones(2 * n, dtype=torch.float64, device=DEV)
+    infos = pa.SolveLinear(A, b, x, tolerance=1e-10, useBiCG=use_bicg, maxIterations=3000)
+    assert infos[0].converged and infos[1].converged and infos[1].usedIterations > 0
+    # the all-zero RHS started from ones converges to (nearly) zero, not set to zero
+    assert float(x[:n].abs().max()) < 1e-6
