@@ -17,6 +17,7 @@ time for a bounded sample of the same physics.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import math
 import os
@@ -29,6 +30,7 @@ if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+MARK_BYTES = 256  # k_copy_bytes launches that bracket the timed steps (one workgroup)
 RE, HA = 500.0, 100.0
 DT = 5e-4
 TOL = 1e-8
@@ -50,6 +52,18 @@ def parse_args(argv=None):
     ap.add_argument("--details", action="store_true", help="also print the per-kernel table to stderr")
     ap.add_argument("--host-profile", action="store_true",
                     help="profile the host (Python) side of the timed steps with cProfile; the result goes to stderr")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="before the warm-up, copy this many MiB three times with k_copy_bytes (the reference "
+                         "point for memory counters)")
+    ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="no GPU run: summarize two rocprofv3 counter_collection.csv files (FETCH_SIZE, WRITE_SIZE) "
+                         "of runs made with --copy-probe-mib into a markdown table on stdout")
+    ap.add_argument("--trace-summary", metavar="KERNEL_TRACE_CSV",
+                    help="no GPU run: per-kernel table of the timed steps (between the two marker launches) of a "
+                         "rocprofv3 --kernel-trace csv of this script, as markdown on stdout")
+    ap.add_argument("--traffic-file", default=None,
+                    help="measured HBM bytes per launch (a --traffic-summary output) for roofline.traffic; default: "
+                         "profiles/hbm_traffic_duct<size>_<dtype>.md when present")
     return ap.parse_args(argv)
 
 
@@ -166,6 +180,139 @@ def hierarchy_levels(sim):
     return out
 
 
+# ----------------------------------------------------------------------------- measured traffic
+TRAFFIC_BEGIN, TRAFFIC_END = "<!-- traffic-json", "traffic-json -->"
+
+
+def traffic_summary(fetch_csv, write_csv, copy_mib, command_note):
+    """Markdown summary of the two counter runs; the figures also go into a JSON comment block
+    that ``load_traffic`` reads back."""
+    from phipict_amd import profiling
+
+    tab = profiling.traffic_table(fetch_csv, write_csv, copy_mib << 20)
+    entries = []
+    for (base, targs, grid), r in sorted(tab["kernels"].items(), key=lambda kv: -kv[1]["fetch_bytes"] * kv[1]["dispatches"]):
+        entries.append({"kernel": base, "targs": list(targs), "grid": grid, **r})
+    lines = [command_note, "",
+             f"Counter scale from `k_copy_bytes` moving {copy_mib} MiB: FETCH_SIZE reads {tab['fetch_scale']:.4f} x the "
+             f"bytes read, WRITE_SIZE {tab['write_scale']:.4f} x the bytes written; the figures below are divided by "
+             "these factors. Early exits: dispatches below 2% of the group's largest fetch (launches after the solve "
+             "had converged); the means are over the other dispatches.", "",
+             "| Kernel | Template | Grid | Dispatches | Early exits | Fetch MB | Write MB |",
+             "|---|---|---:|---:|---:|---:|---:|"]
+    for e in entries[:40]:
+        lines.append(f"| `{e['kernel']}` | `{', '.join(e['targs'])}` | {e['grid']} | {e['dispatches']} | "
+                     f"{e['early_exits']} | {e['fetch_bytes'] / 1e6:.3f} | {e['write_bytes'] / 1e6:.3f} |")
+    lines += ["", TRAFFIC_BEGIN, json.dumps({"fetch_scale": tab["fetch_scale"], "write_scale": tab["write_scale"],
+                                             "kernels": entries}), TRAFFIC_END]
+    return "\n".join(lines) + "\n"
+
+
+def trace_summary(path, steps):
+    """Kernels of the timed region of a rocprofv3 kernel trace of this script: the dispatches
+    between the two MARK_BYTES k_copy_bytes markers."""
+    import csv
+
+    import numpy as np
+
+    from phipict_amd import profiling
+
+    with open(path, newline="") as f:
+        rows = sorted(csv.DictReader(f), key=lambda r: int(r["Start_Timestamp"]))
+    mark_grid = 256
+    marks = [i for i, r in enumerate(rows) if profiling.split_kernel_name(r["Kernel_Name"])[0] == "k_copy_bytes"
+             and int(r.get("Grid_Size_X") or r.get("Grid_Size") or 0) == mark_grid]
+    if len(marks) < 2:
+        raise ValueError("the trace holds no pair of timed-region markers")
+    a, b = marks[-2], marks[-1]
+    win = rows[a + 1:b]
+    t_lo, t_hi = int(rows[a]["End_Timestamp"]), int(rows[b]["Start_Timestamp"])
+    stats = {}
+    for r in win:
+        base, targs = profiling.split_kernel_name(r["Kernel_Name"])
+        key = base + (f"<{', '.join(targs)}>" if targs else "")
+        d = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+        s = stats.setdefault(key, [0, 0, None, 0])
+        s[0] += 1
+        s[1] += d
+        s[2] = d if s[2] is None else min(s[2], d)
+        s[3] = max(s[3], d)
+    busy = sum(s[1] for s in stats.values())
+    span = t_hi - t_lo
+    # idle time between consecutive dispatches (start of one minus the latest end before it)
+    gaps, last_end = [], t_lo
+    for r in win:
+        st, en = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+        gaps.append(max(0, st - last_end))
+        last_end = max(last_end, en)
+    gaps = np.asarray(gaps, dtype=np.float64) / 1e3 if gaps else np.zeros(1)
+    lines = [f"Timed region: {steps} steps, {span / 1e6:.2f} ms from the first marker's end to the second marker's "
+             f"start ({span / 1e6 / steps:.3f} ms per step); {len(win)} dispatches ({len(win) / steps:.0f} per step); "
+             f"summed kernel time {busy / 1e6:.2f} ms ({busy / max(span, 1):.1%} of the span). Idle gap before a "
+             f"dispatch: median {np.median(gaps):.2f} us, 90th percentile {np.percentile(gaps, 90):.2f} us, total "
+             f"{gaps.sum() / 1e3:.2f} ms.", "",
+             "| Kernel | Calls | Per step | Total ms | Avg us | Min us | Max us | % of kernel time |",
+             "|---|---:|---:|---:|---:|---:|---:|---:|"]
+    for key, (c, tot, lo, hi) in sorted(stats.items(), key=lambda kv: -kv[1][1])[:45]:
+        lines.append(f"| `{key}` | {c} | {c / steps:.1f} | {tot / 1e6:.3f} | {tot / c / 1e3:.2f} | {lo / 1e3:.2f} | "
+                     f"{hi / 1e3:.2f} | {tot / max(busy, 1):.2%} |")
+    return "\n".join(lines) + "\n"
+
+
+def load_traffic(path):
+    try:
+        text = Path(path).read_text()
+    except OSError:
+        return None
+    if TRAFFIC_BEGIN not in text:
+        return None
+    body = text.split(TRAFFIC_BEGIN, 1)[1].split(TRAFFIC_END, 1)[0]
+    return json.loads(body)
+
+
+def row_group_width(nnz_per_row):
+    """Lanes per row of the AMG level kernels (the library's rule for CSR levels)."""
+    for w, lim in ((1, 3.0), (2, 6.0), (4, 12.0), (8, 24.0), (16, 48.0)):
+        if nnz_per_row <= lim:
+            return w
+    return 32
+
+
+def traffic_for(table, name, rows, n, dtype_name, levels, cus):
+    """Measured bytes per launch of the timer's kernel ``name`` at ``rows`` rows, or None.
+    AMG level kernels are told apart by their lanes per row (1 on the finest level, which runs
+    on its ELL copy) and their grid (min(rows * lanes / 256, 8 * CUs) workgroups of 256)."""
+    from phipict_amd import profiling
+
+    if table is None:
+        return None
+    base = profiling.TIMER_TO_KERNEL.get(name)
+    cands = [e for e in table["kernels"] if e["kernel"] == base]
+    if name.startswith("amg_"):
+        if name not in ("amg_presmooth", "amg_jacobi", "amg_post_jacobi"):
+            return None
+        if rows == n:
+            w = 1
+        elif rows in levels:
+            w = row_group_width(levels[rows][0] / rows)
+        else:
+            return None
+        grid = min(-(-rows * w // 256), 8 * cus) * 256
+        cands = [e for e in cands if e["targs"] and e["targs"][0] == str(w) and e["grid"] == grid]
+        sys_t = "double" if dtype_name == "f64" else "float"
+        if rows == n:
+            want_out = sys_t if name == "amg_post_jacobi" else "float"
+            cands = [e for e in cands if e["kernel"] != "k_jacobi_sweep" or e["targs"][-1] == want_out]
+    elif rows != n:
+        return None
+    if not cands:
+        return None
+    e = max(cands, key=lambda e: e["fetch_bytes"])
+    return {"bytes": int(e["fetch_bytes"] + e["write_bytes"]), "fetch": int(e["fetch_bytes"]),
+            "write": int(e["write_bytes"]), "dispatches": e["dispatches"], "early_exits": e["early_exits"],
+            "kernel": f"{e['kernel']}<{', '.join(e['targs'])}>", "grid": e["grid"]}
+
+
 # ----------------------------------------------------------------------------- CPU baseline
 def cpu_baseline(size, steps, full_cells, dtype_name):
     """The oracle (single-threaded C restatement) on a bounded sample: the same physics and
@@ -210,6 +357,15 @@ def cpu_baseline(size, steps, full_cells, dtype_name):
 
 def main(argv=None) -> int:
     args = parse_args(argv)
+    if args.trace_summary:
+        sys.stdout.write(trace_summary(args.trace_summary, args.steps))
+        return 0
+    if args.traffic_summary:
+        note = (f"Counter runs: `rocprofv3 --pmc FETCH_SIZE` and `rocprofv3 --pmc WRITE_SIZE` over `python3 bench.py "
+                f"... --copy-probe-mib {args.copy_probe_mib}`; summary by `python3 bench.py --copy-probe-mib "
+                f"{args.copy_probe_mib} --traffic-summary <fetch csv> <write csv>`.")
+        sys.stdout.write(traffic_summary(args.traffic_summary[0], args.traffic_summary[1], args.copy_probe_mib, note))
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (the PISO path would "
               "shard by blocks across GPUs, which is not built)", file=sys.stderr)
@@ -225,11 +381,30 @@ def main(argv=None) -> int:
     device = torch.device("cuda", 0)
     print(f"[bench] building the {nx}x{ny}x{nz} duct ({args.dtype})", file=sys.stderr, flush=True)
     dom, sim = make_sim(nx, ny, nz, dtype, device)
+    import ctypes
+
+    from phipict_amd import _lib
+
+    if args.copy_probe_mib > 0:
+        nb = args.copy_probe_mib << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device=device)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            _lib.call("pp_copy_bytes", _lib.ptr(src), _lib.ptr(dst), ctypes.c_int64(nb), _lib.stream_ptr(device))
+        torch.cuda.synchronize()
+        if not torch.equal(src, dst):
+            raise RuntimeError("k_copy_bytes copy mismatch")
+        del src, dst
     for i in range(args.warmup):
         if not sim.single_step():
             raise RuntimeError(f"warm-up step {i} failed")
     torch.cuda.synchronize()
     print(f"[bench] warm-up done ({args.warmup} steps)", file=sys.stderr, flush=True)
+    # A long-running driver keeps its setup objects (torch, the AMG hierarchy) out of the cyclic
+    # collector's generations; otherwise a full collection (about 50 ms with torch loaded)
+    # lands in the timed steps whenever the allocation counters say so.
+    gc.collect()
+    gc.freeze()
     log_start = len(sim.solver_log)
     profiling.start(args.timer_every)
     prof = None
@@ -237,6 +412,14 @@ def main(argv=None) -> int:
         import cProfile
 
         prof = cProfile.Profile()
+    # two 256-byte k_copy_bytes launches bracket the timed steps in kernel traces
+    mark = torch.zeros(64, dtype=torch.float64, device=device)
+
+    def marker():
+        _lib.call("pp_copy_bytes", _lib.ptr(mark), _lib.ptr(mark[32:]), ctypes.c_int64(MARK_BYTES),
+                  _lib.stream_ptr(device))
+
+    marker()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     if prof is not None:
@@ -248,6 +431,7 @@ def main(argv=None) -> int:
         prof.disable()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    marker()
     profiling.stop()
     if prof is not None:
         import io
@@ -286,6 +470,12 @@ def main(argv=None) -> int:
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": t.name, "rows": t.rows,
                     "bytes_per_launch": int(b), "avg_us": round(t.avg_ms * 1e3, 2), "launches": t.launches,
                     "share_of_timed_kernels": round(t.est_total_ms / max(sum(x.est_total_ms for x, _ in rows), 1e-12), 3)}
+        tfile = args.traffic_file or str(ROOT / "profiles" / f"hbm_traffic_duct{nx}x{ny}x{nz}_{args.dtype}.md")
+        cus = torch.cuda.get_device_properties(device).multi_processor_count
+        tr = traffic_for(load_traffic(tfile), t.name, t.rows, n, args.dtype, levels, cus)
+        if tr is not None:
+            roofline["traffic"] = tr["bytes"]
+            roofline["traffic_detail"] = {**tr, "source": os.path.relpath(tfile, ROOT)}
     iters = {}
     for _, tag, infos in sim.solver_log[log_start:]:
         iters.setdefault(tag, []).append(max(i[0] for i in infos))
@@ -310,7 +500,8 @@ def main(argv=None) -> int:
         "data": "synthetic",
         "config": {"workload": f"3D MHD duct {nx}x{ny}x{nz} cells (periodic x, pressure-gradient source), Re=500, "
                                f"Ha=100, dt=5e-4, AMG-PCG pressure+potential tol 1e-8, BiCGStab momentum",
-                   "cells": n, "nnz": dom.C.getNnz(), "correctors": 2},
+                   "cells": n, "nnz": dom.C.getNnz(), "correctors": 2,
+                   "host": "Python driver; gc.collect() + gc.freeze() after the warm-up"},
         "cell_updates_per_s": round(sps * n, 1),
         "solver_iterations_mean": {k: round(float(np.mean(v)), 2) for k, v in iters.items()},
         "roofline": roofline,

@@ -141,12 +141,30 @@ typedef struct pp_amg_level {
     const int32_t *a_row, *a_col; const void *a_val; const void *dinv;
     const int32_t *r_row, *r_col; const void *r_val;
     const int32_t *p_row, *p_col; const void *p_val;
+    /* optional ELL copy of A (layout of pp_csr.ell_col/ell_val; no reference counterpart) */
+    int32_t a_ell_width, pad;
+    const int32_t *a_ell_col; const void *a_ell_val;
 } pp_amg_level;
 
 /* ---- library ---- */
 const char *pp_last_error(void);
 int pp_version(void);
 int pp_device_sync(void);
+
+/* ---- inductionless MHD cross products ----
+ * pp_cross_eb replaces _compute_u_cross_eb_flat (src/phipict/simulation/mhd.py:309-355):
+ * v [batch][vdims][n] (2D lifted to 3D with a zero z component), e_b [3] or [batch][3]
+ * (eb_per_batch) -> out [batch][3][n] = v x e_b.
+ * pp_lorentz_source replaces the force part of add_lorentz_force (mhd.py:543-594) for one
+ * block: out [batch][dims][block_cells] = N (J x e_b)[:dims] over the block's cells
+ * (block_offset .. + block_cells of J [batch][3][total]) plus the block's current velocity
+ * source: src_kind 0 none, 1 static [1|batch][dims], 2 per cell [1|batch][dims][block_cells]
+ * (src_batched: one per environment). stuart: N, one value or one per environment. */
+int pp_cross_eb(const void *v, int32_t vdims, int32_t n, int32_t batch, const void *e_b, int32_t eb_per_batch, void *out,
+                int32_t dtype, void *stream);
+int pp_lorentz_source(const void *J, int32_t total, int32_t batch, int32_t dims, const void *e_b, int32_t eb_per_batch,
+                      const void *stuart, int32_t stuart_per_batch, int32_t block_offset, int32_t block_cells,
+                      const void *src, int32_t src_kind, int32_t src_batched, void *out, int32_t dtype, void *stream);
 
 /* ---- measurement (no counterpart in the reference) ----
  * A sampled per-kernel timer: while started, every sample_every-th launch of each
@@ -170,6 +188,9 @@ int pp_kernel_timer_start(int32_t sample_every);
 int pp_kernel_timer_stop(void);
 int pp_kernel_timer_read(int32_t capacity, int32_t *kernel_ids, int32_t *rows, double *total_ms, int64_t *sampled,
                          int64_t *launches, int32_t *count);
+/* A plain device copy of `bytes` (a multiple of 16) bytes by the kernel k_copy_bytes: the
+ * reference point that scales hardware memory counters (FETCH_SIZE, WRITE_SIZE) to bytes. */
+int pp_copy_bytes(const void *src, void *dst, int64_t bytes, void *stream);
 
 /* ---- grid metrics ----
  * replaces CoordsToTransforms (csrc/phipict/grid/grid_gen.cu:356-391) and
@@ -299,7 +320,7 @@ int pp_galerkin_rap(const int32_t *r_row, const int32_t *r_col, const double *r_
  * [k][i], rows padded to `width` entries with the row's own column and zero values. Either
  * output may be NULL; no reference counterpart. */
 int pp_ell_from_csr(const int32_t *row, const int32_t *col, const void *val, int32_t n, int32_t nnz, int32_t nmat,
-                    int32_t width, int32_t dtype, int32_t *ell_col, void *ell_val, void *stream);
+                    int32_t width, int32_t dtype, int32_t *ell_col, void *ell_val, int32_t ell_dtype, void *stream);
 
 /* Galerkin map (the precomputed product map of _galerkin_map, src/phipict/solvers/amg.py:
  * 1114-1180, used here for every refresh): host construction for fixed R, P, fine pattern A

@@ -126,6 +126,10 @@ class AmgLevel(C.Structure):
         ("p_row", C.c_void_p),
         ("p_col", C.c_void_p),
         ("p_val", C.c_void_p),
+        ("a_ell_width", C.c_int32),
+        ("pad", C.c_int32),
+        ("a_ell_col", C.c_void_p),
+        ("a_ell_val", C.c_void_p),
     ]
 
 
@@ -170,7 +174,7 @@ SIGNATURES = {
     "pp_galerkin_rap": [_vp, _vp, _vp, _P(CsrDesc), _i32, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _vp],
     "pp_inverse_diagonal": [_P(CsrDesc), _i32, _vp, _i32, _vp],
     "pp_cast": [_vp, _i32, _vp, _i32, _i64, _vp],
-    "pp_ell_from_csr": [_vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _vp],
+    "pp_ell_from_csr": [_vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32, _vp],
     "pp_galerkin_map_build": [_i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _P(_vp)],
     "pp_galerkin_map_size": [_vp, _P(_i64)],
     "pp_galerkin_map_copy": [_vp, _vp, _vp, _vp],
@@ -179,6 +183,9 @@ SIGNATURES = {
     "pp_kernel_timer_start": [_i32],
     "pp_kernel_timer_stop": [],
     "pp_kernel_timer_read": [_i32, _vp, _vp, _vp, _vp, _vp, _P(_i32)],
+    "pp_copy_bytes": [_vp, _vp, _i64, _vp],
+    "pp_cross_eb": [_vp, _i32, _i32, _i32, _vp, _i32, _vp, _i32, _vp],
+    "pp_lorentz_source": [_vp, _i32, _i32, _i32, _vp, _i32, _vp, _i32, _i32, _i32, _vp, _i32, _i32, _vp, _i32, _vp],
 }
 
 

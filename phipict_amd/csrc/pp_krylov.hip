@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -1239,6 +1240,22 @@ __device__ __forceinline__ H group_row_sum(const int32_t *__restrict__ row, cons
     return sum;
 }
 
+// Row sum through the ELL copy (entry k of row i at [k][i]) when the level has one.
+template <int W, typename H, typename G>
+__device__ __forceinline__ H level_row_sum(const Csr<H> &A, const H *__restrict__ val, const H *__restrict__ ellv,
+                                           int i, bool valid, int lane, G gather) {
+    if (!A.ell_w) return group_row_sum<W, H>(A.row, A.col, val, i, valid, lane, gather);
+    H sum = H(0);
+    if (valid)
+        for (int k = lane; k < A.ell_w; k += W) {
+            const size_t o = (size_t)k * A.n + i;
+            sum += ellv[o] * gather(A.ell_col[o]);
+        }
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) sum += __shfl_down(sum, o, W);
+    return sum;
+}
+
 #define GROUP_ROW_LOOP(W, n)                                                                  \
     const int lane = threadIdx.x % (W);                                                       \
     const int numGroups = gridDim.x * (BLOCK / (W));                                          \
@@ -1259,11 +1276,12 @@ __global__ void __launch_bounds__(BLOCK) k_presmooth_residual(Csr<H> A, const H 
     const int n = A.n, batch = blockIdx.y;
     const H *val = A.val + (size_t)batch * A.valStride;
     const H *dv = dinv + (A.valStride ? (size_t)batch * n : 0);
+    const H *ev = A.ell_val + (A.valStride ? (size_t)batch * A.ell_w * n : 0);
     const size_t off = (size_t)batch * n;
     GROUP_ROW_LOOP(W, n) {
         const int i = base + groupInWave;
         const bool valid = i < n;
-        const H sum = group_row_sum<W, H>(A.row, A.col, val, i, valid, lane,
+        const H sum = level_row_sum<W, H>(A, val, ev, i, valid, lane,
                                           [&](int c) { return dv[c] * load_rhs<TB, H>(b, off + c, shift, batch); });
         if (valid && lane == 0) {
             const H bi = load_rhs<TB, H>(b, off + i, shift, batch);
@@ -1281,11 +1299,12 @@ __global__ void __launch_bounds__(BLOCK) k_jacobi_sweep(Csr<H> A, const H *__res
     const int n = A.n, batch = blockIdx.y;
     const H *val = A.val + (size_t)batch * A.valStride;
     const H *dv = dinv + (A.valStride ? (size_t)batch * n : 0);
+    const H *ev = A.ell_val + (A.valStride ? (size_t)batch * A.ell_w * n : 0);
     const size_t off = (size_t)batch * n;
     GROUP_ROW_LOOP(W, n) {
         const int i = base + groupInWave;
         const bool valid = i < n;
-        const H sum = group_row_sum<W, H>(A.row, A.col, val, i, valid, lane, [&](int c) { return x[off + c]; });
+        const H sum = level_row_sum<W, H>(A, val, ev, i, valid, lane, [&](int c) { return x[off + c]; });
         if (valid && lane == 0) {
             const H bi = load_rhs<TB, H>(b, off + i, shift, batch);
             out[off + i] = TO(x[off + i] + omega * dv[i] * (bi - sum));
@@ -1300,11 +1319,12 @@ __global__ void __launch_bounds__(BLOCK) k_level_residual(Csr<H> A, const TB *__
     if (st[blockIdx.y].done != RUNNING) return;
     const int n = A.n, batch = blockIdx.y;
     const H *val = A.val + (size_t)batch * A.valStride;
+    const H *ev = A.ell_val + (A.valStride ? (size_t)batch * A.ell_w * n : 0);
     const size_t off = (size_t)batch * n;
     GROUP_ROW_LOOP(W, n) {
         const int i = base + groupInWave;
         const bool valid = i < n;
-        const H sum = group_row_sum<W, H>(A.row, A.col, val, i, valid, lane, [&](int c) { return x[off + c]; });
+        const H sum = level_row_sum<W, H>(A, val, ev, i, valid, lane, [&](int c) { return x[off + c]; });
         if (valid && lane == 0) res[off + i] = load_rhs<TB, H>(b, off + i, shift, batch) - sum;
     }
 }
@@ -1326,16 +1346,18 @@ __global__ void __launch_bounds__(BLOCK) k_spmv_rect(Csr<H> M, const H *__restri
 }
 
 template <typename H>
-__global__ void k_coarse_solve(const H *__restrict__ pinv, int nc, const H *__restrict__ b, H *__restrict__ x,
-                               const BatchState *__restrict__ st, int batchedPinv) {
+__global__ void __launch_bounds__(256) k_coarse_solve(const H *__restrict__ pinv, int nc, const H *__restrict__ b, H *__restrict__ x,
+                                                      const BatchState *__restrict__ st, int batchedPinv) {
+    // one wavefront per row of the dense pseudo-inverse, lanes over consecutive columns
     if (st[blockIdx.y].done != RUNNING) return;
     const int batch = blockIdx.y;
     if (batchedPinv) pinv += (size_t)batch * nc * nc;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
-        double sum = 0;
-        for (int j = 0; j < nc; ++j) sum += double(pinv[(size_t)i * nc + j]) * double(b[(size_t)batch * nc + j]);
-        x[(size_t)batch * nc + i] = H(sum);
-    }
+    const int i = blockIdx.x * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
+    double sum = 0;
+    if (i < nc)
+        for (int j = lane; j < nc; j += 64) sum += double(pinv[(size_t)i * nc + j]) * double(b[(size_t)batch * nc + j]);
+    sum = wave_sum(sum);
+    if (i < nc && lane == 0) x[(size_t)batch * nc + i] = H(sum);
 }
 
 // ---- coarse tail of the V-cycle: the smallest levels in one workgroup ----
@@ -1346,6 +1368,7 @@ __global__ void k_coarse_solve(const H *__restrict__ pinv, int nc, const H *__re
 constexpr int TAIL_MAX = 12;
 constexpr int TAIL_BLOCK = 1024;
 constexpr int TAIL_ROWS = TAIL_BLOCK / 8;
+constexpr int TAIL_CW = 16; // lanes per row of the dense coarse solve
 // lanes per row in the tail: the widest power of two (<= 32) that covers the level in one pass
 static int tail_width(int rows) {
     int w = 32;
@@ -1376,45 +1399,53 @@ __device__ __forceinline__ H tail_row_sum(const Csr<H> &A, const H *val, int i, 
     return sum;
 }
 
+// The level table lives in device memory (written once per solve): indexing a by-value kernel
+// argument with the runtime level index would copy the whole table into per-lane scratch.
+// Level k's iterate after j Jacobi updates is x[k] for even j and y[k] for odd j.
 template <typename H>
-__global__ void __launch_bounds__(TAIL_BLOCK) k_vcycle_tail(TailArgs<H> a, const BatchState *__restrict__ st) {
+__global__ void __launch_bounds__(TAIL_BLOCK) k_vcycle_tail(const TailArgs<H> *__restrict__ ap,
+                                                            const BatchState *__restrict__ st) {
     const int batch = blockIdx.y;
     if (st[batch].done != RUNNING) return;
-    H *cur[TAIL_MAX], *oth[TAIL_MAX];
-    for (int k = 0; k < a.nl; ++k) {
+    const TailArgs<H> &a = *ap;
+    const int nl = a.nl, pre = a.pre, post = a.post;
+    const H omega = a.omega;
+    auto iterate = [&](int k, int updates) { return (updates & 1) ? a.y[k] : a.x[k]; };
+    for (int k = 0; k < nl; ++k) {
         const int n = a.n[k], W = a.wa[k];
         const size_t off = (size_t)batch * n;
         const Csr<H> &A = a.A[k];
         const H *val = A.val + (size_t)batch * A.valStride;
         const H *dv = a.dinv[k] + (A.valStride ? off : 0);
         const H *bl = a.b[k] + off;
-        cur[k] = a.x[k];
-        oth[k] = a.y[k];
         H *res = a.res[k] + off;
         const int lane = threadIdx.x % W, grp = threadIdx.x / W, gpb = TAIL_BLOCK / W;
-        for (int base = 0; base < n; base += gpb) {
-            const int i = base + grp;
-            const bool valid = i < n;
-            const H sum = tail_row_sum<H>(A, val, i, valid, lane, W, [&](int c) { return dv[c] * bl[c]; });
-            if (valid && lane == 0) {
-                cur[k][off + i] = a.omega * dv[i] * bl[i];
-                res[i] = bl[i] - a.omega * sum;
+        {
+            H *xo = a.x[k] + off;
+            for (int base = 0; base < n; base += gpb) {
+                const int i = base + grp;
+                const bool valid = i < n;
+                const H sum = tail_row_sum<H>(A, val, i, valid, lane, W, [&](int c) { return dv[c] * bl[c]; });
+                if (valid && lane == 0) {
+                    xo[i] = omega * dv[i] * bl[i];
+                    res[i] = bl[i] - omega * sum;
+                }
             }
         }
         __syncthreads();
-        for (int sw = 1; sw < a.pre; ++sw) {
-            const H *xc = cur[k] + off;
+        for (int sw = 1; sw < pre; ++sw) {
+            const H *xc = iterate(k, sw - 1) + off;
+            H *xo = iterate(k, sw) + off;
             for (int base = 0; base < n; base += gpb) {
                 const int i = base + grp;
                 const bool valid = i < n;
                 const H sum = tail_row_sum<H>(A, val, i, valid, lane, W, [&](int c) { return xc[c]; });
-                if (valid && lane == 0) oth[k][off + i] = xc[i] + a.omega * dv[i] * (bl[i] - sum);
+                if (valid && lane == 0) xo[i] = xc[i] + omega * dv[i] * (bl[i] - sum);
             }
             __syncthreads();
-            H *t = cur[k]; cur[k] = oth[k]; oth[k] = t;
         }
-        if (a.pre > 1) {
-            const H *xc = cur[k] + off;
+        if (pre > 1) {
+            const H *xc = iterate(k, pre - 1) + off;
             for (int base = 0; base < n; base += gpb) {
                 const int i = base + grp;
                 const bool valid = i < n;
@@ -1426,7 +1457,7 @@ __global__ void __launch_bounds__(TAIL_BLOCK) k_vcycle_tail(TailArgs<H> a, const
         // restriction into the next level's right-hand side (or the coarsest one)
         const Csr<H> &R = a.R[k];
         const int ncR = R.n, WR = a.wr[k];
-        H *bn = (k + 1 < a.nl ? a.b[k + 1] : a.cb) + (size_t)batch * ncR;
+        H *bn = (k + 1 < nl ? a.b[k + 1] : a.cb) + (size_t)batch * ncR;
         const int laneR = threadIdx.x % WR, grpR = threadIdx.x / WR, gpbR = TAIL_BLOCK / WR;
         for (int base = 0; base < ncR; base += gpbR) {
             const int i = base + grpR;
@@ -1437,29 +1468,38 @@ __global__ void __launch_bounds__(TAIL_BLOCK) k_vcycle_tail(TailArgs<H> a, const
         __syncthreads();
     }
     {
-        const H *pv = a.pinv + (a.coarseBatched ? (size_t)batch * a.nc * a.nc : 0);
-        const H *cb = a.cb + (size_t)batch * a.nc;
-        for (int i = threadIdx.x; i < a.nc; i += TAIL_BLOCK) {
+        // dense coarse solve: TAIL_CW lanes per row read consecutive entries of the row
+        const int nc = a.nc;
+        const H *pv = a.pinv + (a.coarseBatched ? (size_t)batch * nc * nc : 0);
+        const H *cb = a.cb + (size_t)batch * nc;
+        H *cx = a.cx + (size_t)batch * nc;
+        const int lane = threadIdx.x % TAIL_CW, grp = threadIdx.x / TAIL_CW;
+        for (int base = 0; base < nc; base += TAIL_BLOCK / TAIL_CW) {
+            const int i = base + grp;
             double sum = 0;
-            for (int j = 0; j < a.nc; ++j) sum += double(pv[(size_t)i * a.nc + j]) * double(cb[j]);
-            a.cx[(size_t)batch * a.nc + i] = H(sum);
+            if (i < nc)
+                for (int j = lane; j < nc; j += TAIL_CW) sum += double(pv[(size_t)i * nc + j]) * double(cb[j]);
+#pragma unroll
+            for (int o = TAIL_CW / 2; o > 0; o >>= 1) sum += __shfl_down(sum, o, TAIL_CW);
+            if (i < nc && lane == 0) cx[i] = H(sum);
         }
     }
     __syncthreads();
-    for (int k = a.nl - 1; k >= 0; --k) {
+    for (int k = nl - 1; k >= 0; --k) {
         const int n = a.n[k];
         const size_t off = (size_t)batch * n;
-        const int ncoarse = k + 1 < a.nl ? a.n[k + 1] : a.nc;
-        const H *xc = (k + 1 < a.nl ? cur[k + 1] : a.cx) + (size_t)batch * ncoarse;
+        const int ncoarse = k + 1 < nl ? a.n[k + 1] : a.nc;
+        const H *xc = (k + 1 < nl ? iterate(k + 1, pre - 1 + post) : a.cx) + (size_t)batch * ncoarse;
         const Csr<H> &P = a.P[k];
         const int WP = a.wp[k];
         {
+            H *xk = iterate(k, pre - 1) + off;
             const int lane = threadIdx.x % WP, grp = threadIdx.x / WP, gpb = TAIL_BLOCK / WP;
             for (int base = 0; base < n; base += gpb) {
                 const int i = base + grp;
                 const bool valid = i < n;
                 const H sum = tail_row_sum<H>(P, P.val, i, valid, lane, WP, [&](int c) { return xc[c]; });
-                if (valid && lane == 0) cur[k][off + i] += sum;
+                if (valid && lane == 0) xk[i] += sum;
             }
         }
         __syncthreads();
@@ -1469,16 +1509,16 @@ __global__ void __launch_bounds__(TAIL_BLOCK) k_vcycle_tail(TailArgs<H> a, const
         const H *bl = a.b[k] + off;
         const int W = a.wa[k];
         const int lane = threadIdx.x % W, grp = threadIdx.x / W, gpb = TAIL_BLOCK / W;
-        for (int sw = 0; sw < a.post; ++sw) {
-            const H *xcur = cur[k] + off;
+        for (int sw = 0; sw < post; ++sw) {
+            const H *xcur = iterate(k, pre - 1 + sw) + off;
+            H *xo = iterate(k, pre + sw) + off;
             for (int base = 0; base < n; base += gpb) {
                 const int i = base + grp;
                 const bool valid = i < n;
                 const H sum = tail_row_sum<H>(A, val, i, valid, lane, W, [&](int c) { return xcur[c]; });
-                if (valid && lane == 0) oth[k][off + i] = xcur[i] + a.omega * dv[i] * (bl[i] - sum);
+                if (valid && lane == 0) xo[i] = xcur[i] + omega * dv[i] * (bl[i] - sum);
             }
             __syncthreads();
-            H *t = cur[k]; cur[k] = oth[k]; oth[k] = t;
         }
     }
 }
@@ -1524,7 +1564,11 @@ struct Workspace {
     size_t tolCount = 0;
     BatchState *hostState = nullptr;
     int hostCap = 0;
-    int *hostFlags = nullptr;
+    char *hostSnap = nullptr; // two pinned snapshots of Control + BatchState[] for the host loop
+    size_t snapBytes = 0;
+    void *tailDev = nullptr;  // V-cycle tail level table (device copy and pinned staging)
+    size_t tailBytes = 0;
+    void *tailHost = nullptr;
     int device = -1;
 };
 
@@ -1566,7 +1610,13 @@ static int prepare_workspace(Workspace &ws, int nvec, size_t n, int B, size_t el
         PP_HIP(hipHostMalloc((void **)&ws.hostState, sizeof(BatchState) * B, hipHostMallocDefault));
         ws.hostCap = B;
     }
-    if (!ws.hostFlags) PP_HIP(hipHostMalloc((void **)&ws.hostFlags, sizeof(int) * 4, hipHostMallocDefault));
+    if (ws.snapBytes < stateBytes) {
+        if (ws.hostSnap) (void)hipHostFree(ws.hostSnap);
+        ws.hostSnap = nullptr;
+        ws.snapBytes = 0;
+        PP_HIP(hipHostMalloc((void **)&ws.hostSnap, 2 * stateBytes, hipHostMallocDefault));
+        ws.snapBytes = stateBytes;
+    }
     if (ntols > 1) {
         if (int e = reserve((void **)&ws.tols, &ws.tolCount, sizeof(double) * ntols)) return e;
         PP_HIP(hipMemcpyAsync(ws.tols, tols, sizeof(double) * ntols, hipMemcpyHostToDevice, s));
@@ -1577,11 +1627,9 @@ static int prepare_workspace(Workspace &ws, int nvec, size_t n, int B, size_t el
 static Control *ctrl_of(Workspace &ws) { return reinterpret_cast<Control *>(ws.state); }
 static BatchState *state_of(Workspace &ws) { return reinterpret_cast<BatchState *>(reinterpret_cast<char *>(ws.state) + 128); }
 
-static int collect(Workspace &ws, int B, int returnBest, hipStream_t s, pp_solve_result *out) {
-    PP_HIP(hipMemcpyAsync(ws.hostState, state_of(ws), sizeof(BatchState) * B, hipMemcpyDeviceToHost, s));
-    PP_HIP(hipStreamSynchronize(s));
+static void fill_results(const BatchState *states, int B, int returnBest, pp_solve_result *out) {
     for (int b = 0; b < B; ++b) {
-        const BatchState &st = ws.hostState[b];
+        const BatchState &st = states[b];
         pp_solve_result r;
         r.final_residual = st.crit;
         r.used_iterations = st.usedIt;
@@ -1595,32 +1643,125 @@ static int collect(Workspace &ws, int B, int returnBest, hipStream_t s, pp_solve
         }
         out[b] = r;
     }
+}
+
+// Results from the device states (one copy and a stream synchronisation).
+static int collect(Workspace &ws, int B, int returnBest, hipStream_t s, pp_solve_result *out) {
+    PP_HIP(hipMemcpyAsync(ws.hostState, state_of(ws), sizeof(BatchState) * B, hipMemcpyDeviceToHost, s));
+    PP_HIP(hipStreamSynchronize(s));
+    fill_results(ws.hostState, B, returnBest, out);
     return PP_OK;
 }
 
-// Host loop with lagged asynchronous polling of ctrl->allDone. Chunks of iterations grow
-// 0, 1, 2, 4, ... up to `check`: a solve that converges at once or in a few iterations costs
-// few extra launches, and those find allDone set and return at once.
+// Results from the host loop's last observed snapshot: once every right-hand side has stopped
+// (or the iteration budget is spent and every chunk observed) no later launch changes a state,
+// so no further copy or synchronisation is needed; the launches still queued (restoring the
+// best iterate, zeroing x for a zero right-hand side) only write x, in stream order.
+static int collect_snapshot(Workspace &ws, int B, int returnBest, int slot, pp_solve_result *out) {
+    fill_results(reinterpret_cast<const BatchState *>(ws.hostSnap + slot * ws.snapBytes + 128), B, returnBest, out);
+    return PP_OK;
+}
+
+// Host loop over chunks of iterations. After each chunk the stream copies Control and the
+// per-RHS states into a pinned snapshot; the host observes the snapshots in order. The initial
+// residual is observed synchronously, so a converged start costs no iteration. From two
+// observations of a right-hand side's residual the host estimates the total iteration count
+// (log-linear decay) and keeps that many iterations, less `margin`, enqueued ahead of the GPU,
+// at most `cap` per chunk and at most two unobserved chunks; once the estimate is reached it
+// waits for the next snapshot instead of enqueuing launches that would only return at once.
+// Without an estimate, `speculate` (cheap iterations) doubles the chunks 1, 2, 4, ... up to
+// `cap` one chunk ahead, otherwise (expensive iterations) one iteration is enqueued and
+// observed before the next. The estimate only schedules launches: every launch checks the
+// device state itself, so a wrong estimate costs time, never results.
+struct LoopPolicy {
+    int cap;        // iterations per chunk at most
+    int margin;     // iterations short of the estimate to stop enqueuing at
+    bool speculate; // enqueue ahead without an estimate
+};
+
 template <typename Body>
-static int run_loop(Workspace &ws, hipStream_t s, int maxLoops, int check, Body body) {
-    int chunk = 0, size = 0;
+static int run_loop(Workspace &ws, hipStream_t s, int B, int maxLoops, LoopPolicy pol, int *finalSlot, Body body) {
     hipEvent_t ev[2];
     PP_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     PP_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    const size_t bytes = 128 + sizeof(BatchState) * (size_t)B;
+    const int cap = std::max(1, pol.cap);
+    auto snapCtrl = [&](int slot) { return reinterpret_cast<const Control *>(ws.hostSnap + slot * ws.snapBytes); };
+    auto snapState = [&](int slot) {
+        return reinterpret_cast<const BatchState *>(ws.hostSnap + slot * ws.snapBytes + 128);
+    };
     int rc = PP_OK;
-    for (int loops = 0; loops <= maxLoops;) {
-        for (int k = 0; k < size && loops < maxLoops; ++k, ++loops) body(s);
-        if (hipGetLastError() != hipSuccess) { set_error("run_loop: kernel launch failed"); rc = PP_ERR_HIP; break; }
-        if (hipMemcpyAsync(ws.hostFlags + (chunk & 1), &ctrl_of(ws)->allDone, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipEventRecord(ev[chunk & 1], s) != hipSuccess) { set_error("run_loop: poll failed"); rc = PP_ERR_HIP; break; }
-        if (chunk > 0) {
-            if (hipEventSynchronize(ev[(chunk - 1) & 1]) != hipSuccess) { set_error("run_loop: sync failed"); rc = PP_ERR_HIP; break; }
-            if (ws.hostFlags[(chunk - 1) & 1]) break;
+    auto snap = [&](int slot) -> int {
+        if (hipMemcpyAsync(ws.hostSnap + slot * ws.snapBytes, ws.state, bytes, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipEventRecord(ev[slot], s) != hipSuccess) { set_error("run_loop: poll failed"); return PP_ERR_HIP; }
+        return PP_OK;
+    };
+    auto wait = [&](int slot) -> int {
+        if (hipEventSynchronize(ev[slot]) != hipSuccess) { set_error("run_loop: sync failed"); return PP_ERR_HIP; }
+        return PP_OK;
+    };
+    std::vector<int> prevIt(B, -1);
+    std::vector<double> prevCrit(B, 0.0);
+    // iterations still needed after the observed snapshot (-1: no estimate yet)
+    auto remaining = [&](int slot) -> int {
+        const BatchState *st = snapState(slot);
+        int rem = 0;
+        bool known = true;
+        for (int b = 0; b < B; ++b) {
+            const BatchState &x = st[b];
+            if (x.done != RUNNING) continue;
+            int r = -1;
+            if (prevIt[b] >= 0 && x.it > prevIt[b] && prevCrit[b] > 0.0 && x.crit > 0.0 && x.tol > 0.0 &&
+                std::isfinite(x.crit) && x.crit < prevCrit[b]) {
+                const double rate = std::log(x.crit / prevCrit[b]) / double(x.it - prevIt[b]);
+                const double need = std::log(x.tol / x.crit) / rate;
+                if (std::isfinite(need)) r = std::max(1, (int)std::ceil(need - 1e-9));
+            }
+            if (x.it != prevIt[b]) { prevIt[b] = x.it; prevCrit[b] = x.crit; }
+            if (r < 0) known = false;
+            else rem = std::max(rem, std::min(r, std::max(1, x.maxit - x.it)));
         }
-        if (loops >= maxLoops) break;
-        ++chunk;
-        size = size == 0 ? 1 : std::min(2 * size, std::max(1, check));
-    }
+        return known ? rem : -1;
+    };
+    do {
+        if ((rc = snap(0)) || (rc = wait(0))) break;
+        *finalSlot = 0;
+        if (snapCtrl(0)->allDone) break;
+        (void)remaining(0);
+        int loops = 0, estTotal = -1, grow = 1, slot = 0;
+        int qSlot[2], qLoops[2], q = 0; // unobserved snapshots (oldest first) and the loops they follow
+        while (true) {
+            int next;
+            if (estTotal >= 0) next = std::min(estTotal - pol.margin - loops, cap);
+            else if (pol.speculate) next = q < 2 ? grow : 0;
+            else next = q == 0 ? 1 : 0;
+            if (next <= 0) next = q == 0 ? 1 : 0; // nothing in flight: make progress
+            next = std::min(next, maxLoops - loops);
+            if (next > 0) {
+                for (int k = 0; k < next; ++k) body(s);
+                loops += next;
+                if (hipGetLastError() != hipSuccess) { set_error("run_loop: kernel launch failed"); rc = PP_ERR_HIP; break; }
+                slot ^= 1;
+                if ((rc = snap(slot))) break;
+                qSlot[q] = slot;
+                qLoops[q] = loops;
+                ++q;
+                if (estTotal < 0 && pol.speculate) grow = std::min(2 * grow, cap);
+            }
+            if (q == 0) break; // iteration budget spent, everything observed
+            if (q == 2 || next == 0) {
+                if ((rc = wait(qSlot[0]))) break;
+                const int obs = qSlot[0], obsLoops = qLoops[0];
+                *finalSlot = obs;
+                qSlot[0] = qSlot[1];
+                qLoops[0] = qLoops[1];
+                --q;
+                if (snapCtrl(obs)->allDone) break;
+                const int r = remaining(obs);
+                estTotal = r >= 0 ? obsLoops + r : -1;
+            }
+        }
+    } while (false);
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
     return rc;
@@ -1683,13 +1824,14 @@ static int cg_grid(const Csr<T> &A, const T *b, T *x, int B, int maxit, const do
                                normScale, returnBest));
         parity ^= 1;
     };
-    if (int e = run_loop(ws, s, maxit + 2, g_check_interval, body)) return e;
+    int fin = 0;
+    if (int e = run_loop(ws, s, B, maxit + 2, LoopPolicy{g_check_interval, 0, true}, &fin, body)) return e;
     if (returnBest)
         hipLaunchKernelGGL((k_restore_best<T>), dim3(std::min(4096, (n * B + 255) / 256)), dim3(256), 0, s, x, xbest, st, n, B);
     hipLaunchKernelGGL((k_zero_if_zero_rhs<T>), dim3(std::min(4096, (n * B + 255) / 256)), dim3(256), 0, s, x,
                        (size_t)n * B, ctrl);
     PP_LAUNCH_CHECK();
-    return collect(ws, B, returnBest, s, res);
+    return collect_snapshot(ws, B, returnBest, fin, res);
 }
 
 template <typename T>
@@ -1735,11 +1877,12 @@ static int bicg_grid(const Csr<T> &A, const T *b, T *x, int B, int maxit, const 
                                    partial, ctrl, normScale));
             hipLaunchKernelGGL(k_fin_bicg_update, dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl, normScale));
     };
-    if (int e = run_loop(ws, s, maxit + 2, g_check_interval, body)) return e;
+    int fin = 0;
+    if (int e = run_loop(ws, s, B, maxit + 2, LoopPolicy{g_check_interval, 0, true}, &fin, body)) return e;
     hipLaunchKernelGGL((k_zero_if_zero_rhs<T>), dim3(std::min(4096, (n * B + 255) / 256)), dim3(256), 0, s, x,
                        (size_t)n * B, ctrl);
     PP_LAUNCH_CHECK();
-    return collect(ws, B, 0, s, res);
+    return collect_snapshot(ws, B, 0, fin, res);
 }
 
 template <typename T>
@@ -1824,9 +1967,15 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
     std::vector<int> wa(L), wr(L), wp(L);
     for (int l = 0; l < L; ++l) {
         LA[l] = Csr<H>{lv[l].a_row, lv[l].a_col, reinterpret_cast<const H *>(lv[l].a_val), lv[l].n, lv[l].a_batched ? lv[l].nnz_a : 0, 1};
+        if (lv[l].a_ell_width > 0) {
+            LA[l].ell_w = lv[l].a_ell_width;
+            LA[l].ell_col = lv[l].a_ell_col;
+            LA[l].ell_val = reinterpret_cast<const H *>(lv[l].a_ell_val);
+        }
         LR[l] = Csr<H>{lv[l].r_row, lv[l].r_col, reinterpret_cast<const H *>(lv[l].r_val), lv[l].n_coarse, 0, 1};
         LP[l] = Csr<H>{lv[l].p_row, lv[l].p_col, reinterpret_cast<const H *>(lv[l].p_val), lv[l].n, 0, 1};
-        wa[l] = row_group_width(double(lv[l].nnz_a) / std::max(1, lv[l].n));
+        // an ELL level reads one coalesced column of entries per lane: one lane per row
+        wa[l] = LA[l].ell_w ? 1 : row_group_width(double(lv[l].nnz_a) / std::max(1, lv[l].n));
         wr[l] = row_group_width(double(lv[l].nnz_r) / std::max(1, lv[l].n_coarse));
         wp[l] = row_group_width(double(lv[l].nnz_p) / std::max(1, lv[l].n));
     }
@@ -1848,7 +1997,14 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
             tail.dinv[k] = reinterpret_cast<const H *>(lv[l].dinv);
             tail.x[k] = lx(l); tail.y[k] = ly(l); tail.res[k] = lres(l); tail.b[k] = lb(l);
         }
+        // the staging buffer is rewritten only by the next solve, which starts after this
+        // solve's final stream synchronisation
+        if (!ws.tailHost) PP_HIP(hipHostMalloc(&ws.tailHost, sizeof(TailArgs<double>), hipHostMallocDefault));
+        if (int e = reserve(&ws.tailDev, &ws.tailBytes, sizeof(TailArgs<double>))) return e;
+        std::memcpy(ws.tailHost, &tail, sizeof(tail));
+        PP_HIP(hipMemcpyAsync(ws.tailDev, ws.tailHost, sizeof(tail), hipMemcpyHostToDevice, s));
     }
+    const TailArgs<H> *tailDev = reinterpret_cast<const TailArgs<H> *>(ws.tailDev);
     auto vcycle = [&](hipStream_t strm) {
         const double *shift = nullptr;
         if (project) {
@@ -1882,7 +2038,7 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
         }
         if (lt < L) {
             PP_TIMED(PP_K_AMG_TAIL, lv[lt].n, strm,
-                hipLaunchKernelGGL((k_vcycle_tail<H>), dim3(1, B), dim3(TAIL_BLOCK), 0, strm, tail, st));
+                hipLaunchKernelGGL((k_vcycle_tail<H>), dim3(1, B), dim3(TAIL_BLOCK), 0, strm, tailDev, st));
             for (int l = lt; l < L; ++l) {
                 // the tail leaves level l's correction in x after an even number of sweeps
                 const bool inX = ((pre - 1) + post) % 2 == 0;
@@ -1890,7 +2046,7 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
                 oth[l] = inX ? ly(l) : lx(l);
             }
         } else {
-            PP_TIMED(PP_K_AMG_COARSE, nc, strm, hipLaunchKernelGGL((k_coarse_solve<H>), dim3((nc + 127) / 128, B), dim3(128), 0, strm, pinv, nc, lb(L), cx, st, coarseBatched));
+            PP_TIMED(PP_K_AMG_COARSE, nc, strm, hipLaunchKernelGGL((k_coarse_solve<H>), dim3((nc + 3) / 4, B), dim3(256), 0, strm, pinv, nc, lb(L), cx, st, coarseBatched));
         }
         for (int l = std::min(L, lt) - 1; l >= 0; --l) {
             const dim3 g = lgrid(lv[l].n, wa[l]);
@@ -1936,11 +2092,12 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
                            returnBest);
         parity ^= 1;
     };
-    if (int e = run_loop(ws, s, maxit + 2, std::max(1, g_check_interval / 4), body)) return e;
+    int fin = 0;
+    if (int e = run_loop(ws, s, B, maxit + 2, LoopPolicy{2, 1, false}, &fin, body)) return e;
     if (returnBest)
         hipLaunchKernelGGL((k_restore_best<T>), dim3(std::min(4096, (n * B + 255) / 256)), dim3(256), 0, s, x, xbest, st, n, B);
     PP_LAUNCH_CHECK();
-    return collect(ws, B, returnBest, s, res);
+    return collect_snapshot(ws, B, returnBest, fin, res);
 }
 
 // ---------------- misc kernels ----------------
@@ -1958,10 +2115,10 @@ __global__ void k_spmv(Csr<T> A, const T *__restrict__ x, T *__restrict__ y, int
 // ELL copy of a CSR matrix: entry k of row i at [k][i], rows padded to `width` with the
 // row's own column and zero values; rows longer than width are an error left to the caller
 // (the library's domain matrices have at most 2d+1 entries per row).
-template <typename T>
+template <typename T, typename TO>
 __global__ void k_ell_from_csr(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
                                const T *__restrict__ val, int n, int nnz, int width, int32_t *__restrict__ ellCol,
-                               T *__restrict__ ellVal) {
+                               TO *__restrict__ ellVal) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int m = blockIdx.y;
@@ -1970,7 +2127,7 @@ __global__ void k_ell_from_csr(const int32_t *__restrict__ row, const int32_t *_
         const size_t o = (size_t)k * n + i;
         const bool has = b + k < e;
         if (ellCol && m == 0) ellCol[o] = has ? col[b + k] : i;
-        if (ellVal) ellVal[(size_t)m * width * n + o] = has ? val[(size_t)m * nnz + b + k] : T(0);
+        if (ellVal) ellVal[(size_t)m * width * n + o] = has ? TO(val[(size_t)m * nnz + b + k]) : TO(0);
     }
 }
 
@@ -2167,6 +2324,9 @@ int pp_amg_pcg_solve(const pp_csr *A, const pp_amg_level *levels, int32_t num_le
     PP_CHECK_ARG(nc > 0, "empty coarse level");
     PP_CHECK_ARG(A->num_matrices == 1 || A->num_matrices == nrhs, "system matrix must be shared or one per right-hand side");
     for (int l = 0; l + 1 < num_levels; ++l) PP_CHECK_ARG(levels[l].n_coarse == levels[l + 1].n, "inconsistent level sizes");
+    for (int l = 0; l < num_levels; ++l)
+        PP_CHECK_ARG(levels[l].a_ell_width == 0 || (levels[l].a_ell_width > 0 && levels[l].a_ell_col && levels[l].a_ell_val),
+                     "level ELL copy needs a positive width, columns and values");
     PP_CHECK_ARG(levels[num_levels - 1].n_coarse == nc, "coarsest level size mismatch");
     PP_CHECK_ARG(ntols == 0 || ntols == 1 || ntols == nrhs, "tolerance count");
     std::lock_guard<std::mutex> lock(g_mutex);
@@ -2206,7 +2366,9 @@ int pp_release_workspaces(void) {
         if (ws.aux) (void)hipFree(ws.aux);
         if (ws.tols) (void)hipFree(ws.tols);
         if (ws.hostState) (void)hipHostFree(ws.hostState);
-        if (ws.hostFlags) (void)hipHostFree(ws.hostFlags);
+        if (ws.hostSnap) (void)hipHostFree(ws.hostSnap);
+        if (ws.tailDev) (void)hipFree(ws.tailDev);
+        if (ws.tailHost) (void)hipHostFree(ws.tailHost);
         ws = Workspace();
     }
     return PP_OK;
@@ -2267,15 +2429,22 @@ int pp_segmented_gather(const int32_t *seg, const int32_t *src, const void *w, i
 
 
 int pp_ell_from_csr(const int32_t *row, const int32_t *col, const void *val, int32_t n, int32_t nnz, int32_t nmat,
-                    int32_t width, int32_t dtype, int32_t *ell_col, void *ell_val, void *stream) {
+                    int32_t width, int32_t dtype, int32_t *ell_col, void *ell_val, int32_t ell_dtype, void *stream) {
     PP_CHECK_ARG(row && n >= 0 && width > 0 && nmat >= 1, "bad arguments");
     PP_CHECK_ARG(!ell_col || col, "ell_col needs col");
     PP_CHECK_ARG(!ell_val || val, "ell_val needs val");
     if (n == 0) return PP_OK;
     const dim3 g((n + 255) / 256, nmat);
-    PP_DISPATCH(dtype, hipLaunchKernelGGL((k_ell_from_csr<T>), g, dim3(256), 0, as_stream(stream), row, col,
-                                          reinterpret_cast<const T *>(val), n, nnz, width, ell_col,
-                                          reinterpret_cast<T *>(ell_val)));
+    hipStream_t s = as_stream(stream);
+    if (dtype == PP_F64 && ell_dtype == PP_F64)
+        hipLaunchKernelGGL((k_ell_from_csr<double, double>), g, dim3(256), 0, s, row, col, (const double *)val, n, nnz, width, ell_col, (double *)ell_val);
+    else if (dtype == PP_F64 && ell_dtype == PP_F32)
+        hipLaunchKernelGGL((k_ell_from_csr<double, float>), g, dim3(256), 0, s, row, col, (const double *)val, n, nnz, width, ell_col, (float *)ell_val);
+    else if (dtype == PP_F32 && ell_dtype == PP_F32)
+        hipLaunchKernelGGL((k_ell_from_csr<float, float>), g, dim3(256), 0, s, row, col, (const float *)val, n, nnz, width, ell_col, (float *)ell_val);
+    else if (dtype == PP_F32 && ell_dtype == PP_F64)
+        hipLaunchKernelGGL((k_ell_from_csr<float, double>), g, dim3(256), 0, s, row, col, (const float *)val, n, nnz, width, ell_col, (double *)ell_val);
+    else { set_error("pp_ell_from_csr: dtype"); return PP_ERR_ARG; }
     PP_LAUNCH_CHECK();
     return PP_OK;
 }

@@ -2,6 +2,7 @@
 // sampled kernel timer used for live roofline measurements.
 #include "pp_common.hpp"
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -79,6 +80,11 @@ static int timer_drain() {
     g_timer.pending.clear();
     return PP_OK;
 }
+
+// grid-stride copy of 16-byte words; every byte is read once and written once
+__global__ void __launch_bounds__(256) k_copy_bytes(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t words) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < words; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
 } // namespace pp
 
 extern "C" {
@@ -86,6 +92,17 @@ const char *pp_last_error(void) { return pp::get_error(); }
 int pp_version(void) { return 1; }
 int pp_device_sync(void) {
     PP_HIP(hipDeviceSynchronize());
+    return PP_OK;
+}
+
+int pp_copy_bytes(const void *src, void *dst, int64_t bytes, void *stream) {
+    PP_CHECK_ARG(src && dst && bytes >= 0 && bytes % 16 == 0, "pp_copy_bytes: null pointer or size not a multiple of 16");
+    if (bytes == 0) return PP_OK;
+    const size_t words = (size_t)bytes / 16;
+    const unsigned blocks = (unsigned)std::min<size_t>((words + 255) / 256, 8192);
+    hipLaunchKernelGGL(pp::k_copy_bytes, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), words);
+    PP_LAUNCH_CHECK();
     return PP_OK;
 }
 

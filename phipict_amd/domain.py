@@ -875,7 +875,7 @@ class Domain:
         self.UpdateDomainData()
         _lib.call("pp_csr_pattern", C.byref(self._desc), _lib.ptr(row), _lib.ptr(index), _lib.stream_ptr(dev))
         _lib.call("pp_ell_from_csr", _lib.ptr(row), _lib.ptr(index), None, total, csr, 1, width, _lib.dtype_code(dt),
-                  _lib.ptr(self._ell_col), None, _lib.stream_ptr(dev))
+                  _lib.ptr(self._ell_col), None, _lib.dtype_code(dt), _lib.stream_ptr(dev))
         if self._epot_enabled:
             self.SetupEpotOnDomain(*self._epot_flags)
 

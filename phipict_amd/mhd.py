@@ -12,7 +12,7 @@ from typing import Any, Optional
 
 import torch
 
-from . import ops
+from . import _lib, ops
 from .domain import Domain
 from .hooks import Hook, Hooks
 from .simulation import Simulation, subtract_env_mean
@@ -33,27 +33,16 @@ def epot_matrix_is_anchored(domain: Domain) -> bool:
 
 def compute_u_cross_eb_flat(vel_flat: torch.Tensor, e_b: torch.Tensor, total_size: int, dims: int) -> torch.Tensor:
     """u x e_b for a flat ``[B][d][n]`` velocity, returned as ``[B][3][n]`` (2D lifted to 3D,
-    mhd.py:309-355). ``e_b``: uniform ``(3,)`` or one direction per batch ``(B, 3)``."""
-    v = vel_flat.reshape(-1, dims, total_size)
-    if dims == 2:
-        v = torch.cat([v, torch.zeros_like(v[:, :1])], dim=1)
-    eb = e_b.to(device=vel_flat.device, dtype=vel_flat.dtype)
-    if eb.dim() == 1:
-        eb = eb.reshape(1, 3, 1)
-    else:
-        eb = eb.reshape(-1, 3, 1)
-    # (v x e)_x = v_y e_z - v_z e_y, ...
-    cx = v[:, 1] * eb[:, 2] - v[:, 2] * eb[:, 1]
-    cy = v[:, 2] * eb[:, 0] - v[:, 0] * eb[:, 2]
-    cz = v[:, 0] * eb[:, 1] - v[:, 1] * eb[:, 0]
-    return torch.stack([cx, cy, cz], dim=1).reshape(-1)
-
-
-def _flat_to_block(block, flat: torch.Tensor, dims: int, total: int, batch: int) -> torch.Tensor:
-    """Block view [B, d, (z), y, x] of a flat [B][d][n] field."""
-    off, n = block.globalOffset, block.getCells()
-    f = flat.reshape(batch, dims, total)[:, :, off:off + n]
-    return f.reshape(batch, dims, *block.getSizesZYX())
+    mhd.py:309-355), in one library launch. ``e_b``: uniform ``(3,)`` or one direction per
+    batch ``(B, 3)``."""
+    if not vel_flat.is_contiguous():
+        raise ValueError("the velocity must be contiguous")
+    batch = vel_flat.numel() // (dims * total_size)
+    eb = e_b.to(device=vel_flat.device, dtype=vel_flat.dtype).contiguous()
+    out = torch.empty(batch * 3 * total_size, dtype=vel_flat.dtype, device=vel_flat.device)
+    _lib.call("pp_cross_eb", _lib.ptr(vel_flat), dims, total_size, batch, _lib.ptr(eb), 1 if eb.dim() == 2 else 0,
+              _lib.ptr(out), _lib.dtype_code(vel_flat.dtype), _lib.stream_ptr(vel_flat.device))
+    return out
 
 
 class MHDSimulation(Simulation):
@@ -120,22 +109,26 @@ class MHDSimulation(Simulation):
             self._potential_stale = False
         total, batch = domain.getTotalSize(), domain.getBatchSize()
         J = ops.ComputeCurrentDensityFaceBased(domain, domain.epotResult, self._u_cross_eb_flat)
-        Jv = J.reshape(batch, 3, total)
-        eb = self._e_b_device.to(Jv.dtype)
-        eb = eb.reshape(1, 3, 1) if eb.dim() == 1 else eb.reshape(-1, 3, 1)
-        N = self._stuart(Jv).reshape(-1, 1, 1)
-        Fx = Jv[:, 1] * eb[:, 2] - Jv[:, 2] * eb[:, 1]
-        Fy = Jv[:, 2] * eb[:, 0] - Jv[:, 0] * eb[:, 2]
-        Fz = Jv[:, 0] * eb[:, 1] - Jv[:, 1] * eb[:, 0]
-        FL = (N * torch.stack([Fx, Fy, Fz], dim=1))[:, : self._dims].reshape(-1).contiguous()
+        dtype, dev = J.dtype, J.device
+        eb = self._e_b_device.to(dtype).contiguous()
+        N = self._stuart(J).reshape(-1).contiguous()
+        code, stream = _lib.dtype_code(dtype), _lib.stream_ptr(dev)
         for block in domain.getBlocks():
-            fb = _flat_to_block(block, FL, self._dims, total, batch)
+            # F_L = N (J x e_b)[:d] over the block's cells plus its current source, one launch
+            cells = block.getCells()
+            out = torch.empty(batch, self._dims, *block.getSizesZYX(), dtype=dtype, device=dev)
             S = block.velocitySource
+            kind, S_arg, S_batched = 0, None, 0
             if S is not None:
-                if S.dim() == 2:
-                    S = S.reshape(*S.shape, *([1] * (fb.dim() - 2)))
-                fb = fb + S
-            block.setVelocitySource(fb.contiguous())
+                S = S.to(device=dev, dtype=dtype).contiguous()
+                kind = 1 if S.dim() == 2 else 2
+                if S.numel() != S.size(0) * self._dims * (1 if kind == 1 else cells) or S.size(0) not in (1, batch):
+                    raise ValueError("velocity source must be [1|B, d] or [1|B, d, (z), y, x]")
+                S_arg, S_batched = S, 1 if S.size(0) > 1 else 0
+            _lib.call("pp_lorentz_source", _lib.ptr(J), total, batch, self._dims, _lib.ptr(eb),
+                      1 if eb.dim() == 2 else 0, _lib.ptr(N), 1 if N.numel() > 1 else 0, block.globalOffset, cells,
+                      _lib.ptr(S_arg) if S_arg is not None else None, kind, S_batched, _lib.ptr(out), code, stream)
+            block.setVelocitySource(out)
         domain.UpdateDomainData()
 
     def _epot_initial_guess(self, domain: Domain, rhs: torch.Tensor):

@@ -271,12 +271,21 @@ def SolveLinear(A: CSRmatrix, RHS: torch.Tensor, x: torch.Tensor, maxIterations=
 
 def pack_amg_levels(levels: Sequence[Sequence[torch.Tensor]], hd) -> tuple:
     """ctypes level descriptors for ``levels`` (per level: A row, col, val, dinv, R row, col,
-    val, P row, col, val, all values in ``hd``); returns (array, tensors kept alive)."""
+    val, P row, col, val, all values in ``hd``, optionally followed by an ELL copy of A as
+    (columns, values, width)); returns (array, tensors kept alive)."""
     lv = (_lib.AmgLevel * len(levels))()
     keep = []
     for i, t in enumerate(levels):
-        if len(t) != 10:
-            raise ValueError("each AMG level needs (A row, col, val, dinv, R row, col, val, P row, col, val)")
+        if len(t) not in (10, 13):
+            raise ValueError("each AMG level needs (A row, col, val, dinv, R row, col, val, P row, col, val"
+                             "[, ELL col, ELL val, ELL width])")
+        if len(t) == 13:
+            n = t[0].numel() - 1
+            if t[11].dtype != hd or t[10].numel() != t[12] * n or t[11].numel() % (t[12] * n):
+                raise ValueError("ELL copy does not match the level operator")
+            keep.extend((t[10], t[11]))
+            lv[i].a_ell_width, lv[i].a_ell_col, lv[i].a_ell_val = int(t[12]), _lib.ptr(t[10]), _lib.ptr(t[11])
+        t = t[:10]
         if t[2].dtype != hd or t[3].dtype != hd or t[6].dtype != hd or t[9].dtype != hd:
             raise TypeError("all hierarchy values must have the dtype of the coarse pseudo-inverse")
         L = lv[i]

@@ -60,3 +60,107 @@ def read() -> list:
               launches.ctypes.data, C.byref(count))
     return [KernelTiming(int(ids[i]), KERNEL_NAMES.get(int(ids[i]), f"kernel{ids[i]}"), int(rows[i]), int(launches[i]),
                          int(sampled[i]), float(ms[i])) for i in range(n)]
+
+
+# ----------------------------------------------------------------------------- memory counters
+# rocprofv3 names of the library kernels behind the timer's kernel names
+TIMER_TO_KERNEL = {
+    "advection_matrix": "k_advection_matrix", "advection_rhs": "k_advection_rhs", "pressure_matrix": "k_pressure_matrix",
+    "pressure_rhs": "k_pressure_rhs", "pressure_div": "k_pressure_div", "correct_velocity": "k_correct_velocity",
+    "epot_rhs": "k_epot_rhs", "current_density": "k_current_density", "copy_result": "k_copy_result",
+    "residual": "k_residual", "cg_direction": "k_cg_direction", "cg_update": "k_cg_update", "bicg_p": "k_bicg_p",
+    "bicg_v": "k_bicg_v", "bicg_t": "k_bicg_t", "bicg_update": "k_bicg_update", "pcg_dots": "k_cg_dots",
+    "pcg_direction": "k_cg_direction", "pcg_update": "k_cg_update", "amg_presmooth": "k_presmooth_residual",
+    "amg_jacobi": "k_jacobi_sweep", "amg_post_jacobi": "k_jacobi_sweep", "amg_tail": "k_vcycle_tail",
+    "mean": "k_mean", "subtract_mean": "k_subtract_mean", "galerkin": "k_galerkin_apply",
+}
+
+
+def split_kernel_name(name: str) -> tuple:
+    """'void pp::k_x<1, double, float>(...)' -> ('k_x', ('1', 'double', 'float'))."""
+    head = name.split("(")[0].strip()
+    if head.startswith("void "):
+        head = head[5:]
+    args = ()
+    if "<" in head:
+        base, rest = head.split("<", 1)
+        depth, cur, out = 0, "", []
+        for ch in rest:
+            if ch == "<":
+                depth += 1
+            elif ch == ">":
+                if depth == 0:
+                    break
+                depth -= 1
+            if ch == "," and depth == 0:
+                out.append(cur.strip())
+                cur = ""
+            else:
+                cur += ch
+        out.append(cur.strip())
+        args = tuple(out)
+        head = base
+    return head.split("::")[-1], args
+
+
+def read_counter_csv(path) -> dict:
+    """Per dispatch id: (kernel name, grid size, {counter: value}) from a rocprofv3
+    ``counter_collection.csv`` (values summed over the counter's dimensions)."""
+    import csv
+
+    out = {}
+    with open(path, newline="") as f:
+        for r in csv.DictReader(f):
+            did = int(r["Dispatch_Id"])
+            name = r["Kernel_Name"]
+            grid = int(r.get("Grid_Size", 0) or 0)
+            ent = out.setdefault(did, (name, grid, {}))
+            ent[2][r["Counter_Name"]] = ent[2].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    return out
+
+
+def traffic_table(fetch_csv, write_csv, copy_bytes: int, idle_fraction: float = 0.02) -> dict:
+    """Measured HBM bytes per dispatch from a FETCH_SIZE run and a WRITE_SIZE run (KiB per
+    dispatch each), scaled by what the same counters read for ``k_copy_bytes`` moving
+    ``copy_bytes`` bytes. Dispatches whose fetch is below ``idle_fraction`` of the group's
+    largest are counted as early exits (a solve that had already converged) and kept apart.
+    Keys: (kernel, template arguments, grid size in work-items)."""
+    fetch = read_counter_csv(fetch_csv)
+    write = read_counter_csv(write_csv)
+
+    def counter(d, key):
+        return next((v for k, v in d.items() if k.startswith(key)), None)
+
+    def calib(table, key):
+        copies = [(grid, counter(c, key)) for name, grid, c in table.values()
+                  if split_kernel_name(name)[0] == "k_copy_bytes"]
+        # the calibration copies are the widest k_copy_bytes launches (markers are one workgroup)
+        widest = max((g for g, _ in copies), default=0)
+        vals = [v for g, v in copies if g == widest and v]
+        if not vals:
+            raise ValueError("no k_copy_bytes dispatch in the counter run")
+        return float(np.median(vals)) * 1024.0 / copy_bytes
+
+    ff, fw = calib(fetch, "FETCH_SIZE"), calib(write, "WRITE_SIZE")
+    groups = {}
+    for tab, key, scale, slot in ((fetch, "FETCH_SIZE", ff, 0), (write, "WRITE_SIZE", fw, 1)):
+        for name, grid, c in tab.values():
+            v = counter(c, key)
+            if v is None:
+                continue
+            g = groups.setdefault(split_kernel_name(name) + (grid,), ([], []))
+            g[slot].append(v * 1024.0 / scale)
+    def busy_mean(vals):
+        v = np.asarray(vals, dtype=np.float64)
+        top = v.max()
+        busy = v >= idle_fraction * top if top > 0 else np.ones(v.size, dtype=bool)
+        return float(v[busy].mean()), int((~busy).sum())
+
+    rows = {}
+    for key_, (fb, wb) in groups.items():
+        if not fb or not wb:
+            continue
+        fmean, fidle = busy_mean(fb)
+        wmean, _ = busy_mean(wb)
+        rows[key_] = {"dispatches": len(fb), "early_exits": fidle, "fetch_bytes": fmean, "write_bytes": wmean}
+    return {"fetch_scale": ff, "write_scale": fw, "kernels": rows}

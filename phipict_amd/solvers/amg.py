@@ -95,6 +95,13 @@ class AMGHierarchy:
                           _lib.dtype_code(dtype), _lib.stream_ptr(A.value.device))
                 nat = (A.row, A.index, aval, dinv, lvl.R_row, lvl.R_col, lvl.R_val.to(dtype), lvl.P_row, lvl.P_col,
                        lvl.P_val.to(dtype))
+                if lvl is self.levels[0] and A.ell_col is not None and A.ell_width > 0:
+                    # ELL copy of the finest operator in the hierarchy's dtype (coalesced smoothing sweeps)
+                    ev = torch.empty(A.ell_width * A.getRows() * A.getBatchSize(), dtype=dtype, device=A.value.device)
+                    _lib.call("pp_ell_from_csr", _lib.ptr(A.row), _lib.ptr(A.index), _lib.ptr(A.value), A.getRows(),
+                              A.getNnz(), A.getBatchSize(), A.ell_width, _lib.dtype_code(A.getDtype()), None,
+                              _lib.ptr(ev), _lib.dtype_code(dtype), _lib.stream_ptr(A.value.device))
+                    nat = nat + (A.ell_col, ev, A.ell_width)
                 lvl.native[dtype] = nat
             out.append(nat)
         return out
@@ -113,9 +120,13 @@ class AMGHierarchy:
                 nat = self.native_levels(dtype)[0]
                 arr[0].a_val = nat[2].data_ptr()
                 arr[0].dinv = nat[3].data_ptr()
+                if len(nat) > 10:
+                    arr[0].a_ell_width, arr[0].a_ell_col, arr[0].a_ell_val = nat[12], nat[10].data_ptr(), nat[11].data_ptr()
+                else:
+                    arr[0].a_ell_width, arr[0].a_ell_col, arr[0].a_ell_val = 0, None, None
                 # keep the tensors of levels >= 1 (shared with the parent) and the new level 0
                 base = keep[0] if isinstance(keep, tuple) else keep
-                cache[dtype] = (arr, (base, [nat[2], nat[3]]))
+                cache[dtype] = (arr, (base, list(t for t in nat if isinstance(t, torch.Tensor))))
             else:
                 cache[dtype] = ops.pack_amg_levels(self.native_levels(dtype), dtype)
             self.__dict__.pop("_packed_parent", None)

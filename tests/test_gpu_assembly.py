@@ -177,3 +177,54 @@ def test_skewed_grid_rejects_non_orthogonal_flags():
     pa.SetupAdvectionMatrix(dom, torch.tensor([DT], dtype=torch.float64), 0)
     rect = build_domain(cases.channel2d(8, 6), NU, DT, torch.float64)
     assert rect.isOrthogonal()
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+@pytest.mark.parametrize("per_env", [False, True])
+def test_mhd_cross_products(dtype, dims, per_env):
+    """pp_cross_eb (u x e_b lifted to 3D) and pp_lorentz_source (N (J x e_b)[:d] over a block
+    plus its static, per-cell or absent source) against numpy's cross product."""
+    from phipict_amd import _lib
+
+    td = _tdt(dtype)
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(11)
+    B, total, g0, cells = 2, 1000, 137, 611
+    v = rng.standard_normal((B, dims, total))
+    eb = rng.standard_normal((B, 3)) if per_env else rng.standard_normal(3)
+    ebB = eb if per_env else np.broadcast_to(eb, (B, 3))
+    v3 = np.concatenate([v, np.zeros((B, 3 - dims, total))], axis=1)
+    want = np.stack([np.cross(v3[b].T, ebB[b]).T for b in range(B)])
+    vt = torch.as_tensor(v, dtype=td, device=dev).contiguous()
+    ebt = torch.as_tensor(np.ascontiguousarray(eb), dtype=td, device=dev)
+    out = torch.empty(B * 3 * total, dtype=td, device=dev)
+    s = _lib.stream_ptr(dev)
+    _lib.call("pp_cross_eb", _lib.ptr(vt), dims, total, B, _lib.ptr(ebt), int(per_env), _lib.ptr(out),
+              _lib.dtype_code(td), s)
+    close(to_np(out).reshape(B, 3, total), want, dtype, "u x e_b")
+
+    J = rng.standard_normal((B, 3, total))
+    N = rng.uniform(1, 30, B) if per_env else np.array([17.0])
+    NB = N if per_env else np.repeat(N, B)
+    force = np.stack([NB[b] * np.cross(J[b].T, ebB[b]).T for b in range(B)])[:, :dims, g0:g0 + cells]
+    Jt = torch.as_tensor(J, dtype=td, device=dev).contiguous()
+    Nt = torch.as_tensor(N, dtype=td, device=dev)
+    sources = {0: (None, 0, np.zeros((1, dims, 1))),
+               1: (rng.standard_normal((B if per_env else 1, dims)), 1, None),
+               2: (rng.standard_normal((B if per_env else 1, dims, cells)), 2, None)}
+    for kind, (S, k, _) in sources.items():
+        St = torch.as_tensor(S, dtype=td, device=dev).contiguous() if S is not None else None
+        o = torch.empty(B * dims * cells, dtype=td, device=dev)
+        _lib.call("pp_lorentz_source", _lib.ptr(Jt), total, B, dims, _lib.ptr(ebt), int(per_env), _lib.ptr(Nt),
+                  int(per_env), g0, cells, _lib.ptr(St) if St is not None else None, k,
+                  int(S is not None and S.shape[0] > 1), _lib.ptr(o), _lib.dtype_code(td), s)
+        exp = force.copy()
+        if kind == 1:
+            exp = exp + S[:, :, None]
+        elif kind == 2:
+            exp = exp + S
+        close(to_np(o).reshape(B, dims, cells), exp, dtype, f"Lorentz source kind {kind}")
+    # a block reaching past the domain is refused before any launch
+    with pytest.raises(RuntimeError):
+        _lib.call("pp_lorentz_source", _lib.ptr(Jt), total, B, dims, _lib.ptr(ebt), 0, _lib.ptr(Nt), 0, g0, total,
+                  None, 0, 0, _lib.ptr(out), _lib.dtype_code(td), s)

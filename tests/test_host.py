@@ -259,3 +259,35 @@ def test_galerkin_map_reproduces_rap(which):
         np.testing.assert_array_equal(ref[1], nxt["A"][1])
         np.testing.assert_allclose(got, ref[2], rtol=1e-12, atol=1e-14 * np.abs(ref[2]).max())
         fine = ref
+
+
+def test_counter_traffic_table(tmp_path):
+    """Counter summaries: kernel names split into base and template arguments, counters scaled
+    by the copy kernel's reading, early-exit dispatches kept apart."""
+    from phipict_amd import profiling
+
+    assert profiling.split_kernel_name("void pp::k_jacobi_sweep<1, double, float, double>(pp::Csr<float>, int)") == \
+        ("k_jacobi_sweep", ("1", "double", "float", "double"))
+    assert profiling.split_kernel_name("pp::k_copy_bytes(uint4 const*, uint4*, unsigned long)") == ("k_copy_bytes", ())
+    assert profiling.split_kernel_name("void pp::k_x<pp::Csr<float>, 2>(int)") == ("k_x", ("pp::Csr<float>", "2"))
+
+    def write(path, counter, rows):
+        with open(path, "w") as f:
+            f.write("Dispatch_Id,Kernel_Name,Grid_Size,Counter_Name,Counter_Value\n")
+            for did, name, grid, val in rows:
+                # two dimensions of one counter: summed
+                f.write(f'{did},"{name}",{grid},{counter},{val / 2}\n{did},"{name}",{grid},{counter},{val / 2}\n')
+
+    copy = "pp::k_copy_bytes(uint4 const*, uint4*, unsigned long)"
+    kern = "void pp::k_cg_direction<double, 1, 1>(pp::Csr<double>)"
+    # the copy of 1 MiB reads 2048 KiB and writes 1536 KiB on the counters: scales 2 and 1.5
+    write(tmp_path / "f.csv", "FETCH_SIZE", [(1, copy, 100, 2048), (2, kern, 512, 400), (3, kern, 512, 600),
+                                             (4, kern, 512, 1)])
+    write(tmp_path / "w.csv", "WRITE_SIZE", [(1, copy, 100, 1536), (2, kern, 512, 150), (3, kern, 512, 150),
+                                             (4, kern, 512, 0)])
+    tab = profiling.traffic_table(tmp_path / "f.csv", tmp_path / "w.csv", 1 << 20)
+    assert tab["fetch_scale"] == pytest.approx(2.0) and tab["write_scale"] == pytest.approx(1.5)
+    r = tab["kernels"][("k_cg_direction", ("double", "1", "1"), 512)]
+    assert r["dispatches"] == 3 and r["early_exits"] == 1
+    assert r["fetch_bytes"] == pytest.approx(500 * 1024 / 2)
+    assert r["write_bytes"] == pytest.approx(150 * 1024 / 1.5)
