@@ -110,10 +110,11 @@ def make_sim(nx, ny, nz, dtype, device):
 
 
 # ----------------------------------------------------------------------------- algorithmic bytes
-def kernel_bytes(name, rows, s, sh, fine, levels):
+def kernel_bytes(name, rows, s, sh, fine, levels, m=3):
     """Algorithmic HBM bytes of one launch (neighbour reads counted once): ``fine`` = (n, nnz)
     of the system matrices, ``levels`` maps level rows -> (nnz_A, n_coarse, nnz_R, nnz_P).
-    s / sh: bytes per value of the system / the AMG hierarchy."""
+    s / sh: bytes per value of the system / the AMG hierarchy; m: right-hand sides of the
+    BiCGStab launches (the momentum solve: one per velocity component, sharing the matrix)."""
     n, nnz = fine
 
     def M(nz, rws, val):
@@ -127,11 +128,11 @@ def kernel_bytes(name, rows, s, sh, fine, levels):
     if name == "pcg_dots":
         return 2 * n * s
     if name == "bicg_p":
-        return 4 * n * s
+        return 4 * m * n * s
     if name in ("bicg_v", "bicg_t"):
-        return M(nnz, n, s) + 3 * n * s
+        return M(nnz, n, s) + 3 * m * n * s
     if name == "bicg_update":
-        return 8 * n * s
+        return 8 * m * n * s
     if name in ("mean",):
         return n * s
     if name == "subtract_mean":

@@ -134,6 +134,14 @@ typedef struct pp_solve_result {
     int32_t pad;
 } pp_solve_result;
 
+/* Sliced ELL copy of a CSR matrix (no reference counterpart): rows in slices of 64, slice s
+ * padded to slice_width[s] entries per row; entry k of row 64 s + j at slice_off[s] + 64 k + j
+ * (col, val), padding with column 0 and value 0. All four pointers set, or all null. */
+typedef struct pp_sell {
+    const int32_t *slice_off, *slice_width, *col;
+    const void *val;
+} pp_sell;
+
 /* One AMG level above the coarsest (reference layout of _C.AMGPCGSolve levels,
  * csrc/phipict/bindings.cpp:764-776: A row,col,val,dinv, R row,col,val, P row,col,val). */
 typedef struct pp_amg_level {
@@ -144,6 +152,9 @@ typedef struct pp_amg_level {
     /* optional ELL copy of A (layout of pp_csr.ell_col/ell_val; no reference counterpart) */
     int32_t a_ell_width, pad;
     const int32_t *a_ell_col; const void *a_ell_val;
+    /* optional sliced ELL copies of A and R (values in the hierarchy dtype; A only when shared
+     * by all right-hand sides) */
+    pp_sell a_sell, r_sell;
 } pp_amg_level;
 
 /* ---- library ---- */
@@ -321,6 +332,12 @@ int pp_galerkin_rap(const int32_t *r_row, const int32_t *r_col, const double *r_
  * output may be NULL; no reference counterpart. */
 int pp_ell_from_csr(const int32_t *row, const int32_t *col, const void *val, int32_t n, int32_t nnz, int32_t nmat,
                     int32_t width, int32_t dtype, int32_t *ell_col, void *ell_val, int32_t ell_dtype, void *stream);
+/* Fill a sliced ELL copy (pp_sell layout) of the CSR matrix (row, col, val) with n rows, whose
+ * slice offsets and widths the caller computed from the row lengths: columns (sell_col, may be
+ * null) and values converted from dtype to sell_dtype (sell_val, may be null). */
+int pp_sell_fill(const int32_t *row, const int32_t *col, const void *val, int32_t n, const int32_t *slice_off,
+                 const int32_t *slice_width, int32_t dtype, int32_t *sell_col, void *sell_val, int32_t sell_dtype,
+                 void *stream);
 
 /* Galerkin map (the precomputed product map of _galerkin_map, src/phipict/solvers/amg.py:
  * 1114-1180, used here for every refresh): host construction for fixed R, P, fine pattern A

@@ -9,7 +9,8 @@ import ctypes as C
 import os
 from pathlib import Path
 
-_LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libphipict_hip.so"
+# PHIPICT_HIP_LIBRARY may name another build of the same library (e.g. a tuning variant)
+_LIB_PATH = Path(os.environ.get("PHIPICT_HIP_LIBRARY") or Path(__file__).resolve().parent / "_lib" / "libphipict_hip.so")
 
 PP_F32, PP_F64 = 4, 8
 PP_BOUND_CONNECTED, PP_BOUND_PERIODIC, PP_BOUND_FIXED = 20, 21, 30
@@ -108,6 +109,15 @@ class SolveResult(C.Structure):
     ]
 
 
+class SellDesc(C.Structure):
+    _fields_ = [
+        ("slice_off", C.c_void_p),
+        ("slice_width", C.c_void_p),
+        ("col", C.c_void_p),
+        ("val", C.c_void_p),
+    ]
+
+
 class AmgLevel(C.Structure):
     _fields_ = [
         ("n", C.c_int32),
@@ -130,6 +140,8 @@ class AmgLevel(C.Structure):
         ("pad", C.c_int32),
         ("a_ell_col", C.c_void_p),
         ("a_ell_val", C.c_void_p),
+        ("a_sell", SellDesc),
+        ("r_sell", SellDesc),
     ]
 
 
@@ -184,6 +196,7 @@ SIGNATURES = {
     "pp_kernel_timer_stop": [],
     "pp_kernel_timer_read": [_i32, _vp, _vp, _vp, _vp, _vp, _P(_i32)],
     "pp_copy_bytes": [_vp, _vp, _i64, _vp],
+    "pp_sell_fill": [_vp, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _vp],
     "pp_cross_eb": [_vp, _i32, _i32, _i32, _vp, _i32, _vp, _i32, _vp],
     "pp_lorentz_source": [_vp, _i32, _i32, _i32, _vp, _i32, _vp, _i32, _i32, _i32, _vp, _i32, _i32, _vp, _i32, _vp],
 }

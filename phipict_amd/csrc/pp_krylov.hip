@@ -50,13 +50,18 @@ struct Csr {
     int ell_w;     // > 0: ELL copy of the same entries (entry k of row i at [k][i])
     const int32_t *__restrict__ ell_col;
     const T *__restrict__ ell_val;
+    // sliced ELL copy (pp_sell layout; AMG levels only): used when sell_off is set
+    const int32_t *__restrict__ sell_off;
+    const int32_t *__restrict__ sell_w;
+    const int32_t *__restrict__ sell_col;
+    const T *__restrict__ sell_val;
 };
 
 // Matrix view of a descriptor; the ELL copy is used when present.
 template <typename T>
 static Csr<T> csr_view(const pp_csr *A, int rpm) {
     Csr<T> C{A->row, A->col, reinterpret_cast<const T *>(A->val), A->n, A->num_matrices > 1 ? A->nnz : 0, rpm, 0,
-             nullptr, nullptr};
+             nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     if (A->ell_width > 0 && A->ell_col && A->ell_val) {
         C.ell_w = A->ell_width;
         C.ell_col = A->ell_col;
@@ -597,6 +602,12 @@ __global__ void __launch_bounds__(SMALL_BLOCK) k_bicg_small(Csr<T> A, const T *_
 // ======================================================================================
 constexpr int BLOCK = 256;
 
+// Rows of a workgroup: a grid-wide sweep (workgroup b takes rows b*BLOCK + k*G*BLOCK). Giving
+// each XCD a contiguous span of rows (so that stencil neighbours hit its own L2) measured
+// slower: the sweep's DRAM locality matters more than the cross-XCD reads, which the
+// Infinity Cache serves.
+#define FOR_BLOCK_ROWS(i, n) for (int i = blockIdx.x * BLOCK + (int)threadIdx.x; i < (n); i += gridDim.x * BLOCK)
+
 // Per-workgroup partial sums, written with plain stores. The one-workgroup finish kernel that
 // follows on the same stream combines them in a fixed order (deterministic); the kernel
 // boundary makes them visible across XCDs. (A last-workgroup-done scheme would need a
@@ -683,7 +694,7 @@ __global__ void __launch_bounds__(BLOCK) k_residual(Csr<T> A, const T *__restric
         const T *val = A.val + (size_t)grp.m * A.valStride;
         const T *ellv = A.ell_val + (A.valStride ? (size_t)grp.m * A.ell_w * n : 0);
         const int ew = A.ell_w;
-        for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        FOR_BLOCK_ROWS(i, n) {
             T sum[NB];
 #pragma unroll
             for (int k = 0; k < NB; ++k) sum[k] = T(0);
@@ -796,7 +807,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_direction(Csr<T> A, const T *__res
         const T *val = A.val + (size_t)grp.m * A.valStride;
         const T *ellv = A.ell_val + (A.valStride ? (size_t)grp.m * A.ell_w * n : 0);
         const int ew = A.ell_w;
-        for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        FOR_BLOCK_ROWS(i, n) {
             T sum[NB];
 #pragma unroll
             for (int k = 0; k < NB; ++k) sum[k] = T(0);
@@ -869,7 +880,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update(int n, int rpm, T *__restri
         const T alpha = T(bs.alpha);
         const bool saveBest = bs.saveBest && xbest != nullptr;
         const size_t off = (size_t)(grp.b0 + k) * n;
-        for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        FOR_BLOCK_ROWS(i, n) {
             const size_t o = off + i;
             const T xi = x[o];
             if (saveBest) xbest[o] = xi;
@@ -933,7 +944,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_dots(int n, const T *__restrict__ 
     const int b = blockIdx.y;
     double acc[1][1] = {{0.0}};
     if (st[b].done == RUNNING)
-        for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK)
+        FOR_BLOCK_ROWS(i, n)
             acc[0][0] += double(r[(size_t)b * n + i]) * double(z[(size_t)b * n + i]);
     store_partials<1, 1>(acc, b, 1, partial, sm);
 }
@@ -969,7 +980,7 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_p(int n, const T *__restrict__ r
     const bool restart = bs.restart;
     const T beta = T(bs.beta), omega = T(bs.omega);
     const size_t off = (size_t)b * n;
-    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+    FOR_BLOCK_ROWS(i, n) {
         const size_t o = off + i;
         p[o] = restart ? r[o] : r[o] + beta * (p[o] - omega * v[o]);
     }
@@ -993,7 +1004,7 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_v(Csr<T> A, const T *__restrict_
         const T *val = A.val + (size_t)grp.m * A.valStride;
         const T *ellv = A.ell_val + (A.valStride ? (size_t)grp.m * A.ell_w * n : 0);
         const int ew = A.ell_w;
-        for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        FOR_BLOCK_ROWS(i, n) {
             T sum[NB];
 #pragma unroll
             for (int k = 0; k < NB; ++k) sum[k] = T(0);
@@ -1041,7 +1052,7 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_t(Csr<T> A, const T *__restrict_
         const T *val = A.val + (size_t)grp.m * A.valStride;
         const T *ellv = A.ell_val + (A.valStride ? (size_t)grp.m * A.ell_w * n : 0);
         const int ew = A.ell_w;
-        for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        FOR_BLOCK_ROWS(i, n) {
             T sum[NB];
 #pragma unroll
             for (int k = 0; k < NB; ++k) sum[k] = T(0);
@@ -1119,7 +1130,7 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_update(int n, int rpm, T *__rest
         const T alpha = T(bs.alpha), omega = T(bs.omega);
         const bool half = bs.halfDone;
         const size_t off = (size_t)(grp.b0 + k) * n;
-        for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        FOR_BLOCK_ROWS(i, n) {
             const size_t o = off + i;
             const T si = r[o] - alpha * v[o];
             if (half) {
@@ -1200,7 +1211,7 @@ __global__ void __launch_bounds__(BLOCK) k_mean(const T *__restrict__ v, int n, 
     const int batch = blockIdx.y;
     double acc[1][1] = {{0.0}};
     if (!onlyRunning || st[batch].done == RUNNING)
-        for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) acc[0][0] += double(v[(size_t)batch * n + i]);
+        FOR_BLOCK_ROWS(i, n) acc[0][0] += double(v[(size_t)batch * n + i]);
     store_partials<1, 1>(acc, batch, 1, partial, sm);
 }
 
@@ -1222,7 +1233,7 @@ __global__ void k_subtract_mean(T *__restrict__ v, int n, const double *means, c
     const int batch = blockIdx.y;
     if (onlyRunning && st[batch].done != RUNNING) return;
     const T m = T(means[batch]);
-    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) v[(size_t)batch * n + i] -= m;
+    FOR_BLOCK_ROWS(i, n) v[(size_t)batch * n + i] -= m;
 }
 
 // ---------------- AMG V-cycle level kernels (krylov.cu:1247-1360) ----------------
@@ -1241,9 +1252,22 @@ __device__ __forceinline__ H group_row_sum(const int32_t *__restrict__ row, cons
 }
 
 // Row sum through the ELL copy (entry k of row i at [k][i]) when the level has one.
+// Row sum of a level matrix through its sliced ELL copy, its ELL copy or its CSR arrays (in
+// this order of preference); the copies are read with one lane per row (W = 1).
 template <int W, typename H, typename G>
 __device__ __forceinline__ H level_row_sum(const Csr<H> &A, const H *__restrict__ val, const H *__restrict__ ellv,
                                            int i, bool valid, int lane, G gather) {
+    if (A.sell_off) {
+        H sum = H(0);
+        if (valid) {
+            const int sl = i >> 6;
+            const int w = A.sell_w[sl];
+            const int base = A.sell_off[sl] + (i & 63);
+#pragma unroll 4
+            for (int k = 0; k < w; ++k) sum += A.sell_val[base + 64 * k] * gather(A.sell_col[base + 64 * k]);
+        }
+        return sum;
+    }
     if (!A.ell_w) return group_row_sum<W, H>(A.row, A.col, val, i, valid, lane, gather);
     H sum = H(0);
     if (valid)
@@ -1256,6 +1280,8 @@ __device__ __forceinline__ H level_row_sum(const Csr<H> &A, const H *__restrict_
     return sum;
 }
 
+// Rows in groups of W lanes, swept over the whole grid; each wavefront takes 64 / W
+// consecutive rows per pass.
 #define GROUP_ROW_LOOP(W, n)                                                                  \
     const int lane = threadIdx.x % (W);                                                       \
     const int numGroups = gridDim.x * (BLOCK / (W));                                          \
@@ -1340,7 +1366,7 @@ __global__ void __launch_bounds__(BLOCK) k_spmv_rect(Csr<H> M, const H *__restri
     GROUP_ROW_LOOP(W, n) {
         const int i = base + groupInWave;
         const bool valid = i < n;
-        const H sum = group_row_sum<W, H>(M.row, M.col, val, i, valid, lane, [&](int c) { return vin[c]; });
+        const H sum = level_row_sum<W, H>(M, val, (const H *)nullptr, i, valid, lane, [&](int c) { return vin[c]; });
         if (valid && lane == 0) out[off + i] = accumulate ? out[off + i] + sum : sum;
     }
 }
@@ -1974,9 +2000,18 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
         }
         LR[l] = Csr<H>{lv[l].r_row, lv[l].r_col, reinterpret_cast<const H *>(lv[l].r_val), lv[l].n_coarse, 0, 1};
         LP[l] = Csr<H>{lv[l].p_row, lv[l].p_col, reinterpret_cast<const H *>(lv[l].p_val), lv[l].n, 0, 1};
-        // an ELL level reads one coalesced column of entries per lane: one lane per row
-        wa[l] = LA[l].ell_w ? 1 : row_group_width(double(lv[l].nnz_a) / std::max(1, lv[l].n));
-        wr[l] = row_group_width(double(lv[l].nnz_r) / std::max(1, lv[l].n_coarse));
+        auto attach = [](Csr<H> &M, const pp_sell &sd) {
+            if (!sd.slice_off) return;
+            M.sell_off = sd.slice_off;
+            M.sell_w = sd.slice_width;
+            M.sell_col = sd.col;
+            M.sell_val = reinterpret_cast<const H *>(sd.val);
+        };
+        attach(LA[l], lv[l].a_sell);
+        attach(LR[l], lv[l].r_sell);
+        // ELL and sliced ELL copies are read with one lane per row (coalesced across the wavefront)
+        wa[l] = (LA[l].ell_w || LA[l].sell_off) ? 1 : row_group_width(double(lv[l].nnz_a) / std::max(1, lv[l].n));
+        wr[l] = LR[l].sell_off ? 1 : row_group_width(double(lv[l].nnz_r) / std::max(1, lv[l].n_coarse));
         wp[l] = row_group_width(double(lv[l].nnz_p) / std::max(1, lv[l].n));
     }
     // first level handled by the one-workgroup tail (never the finest level)
@@ -2109,6 +2144,23 @@ __global__ void k_spmv(Csr<T> A, const T *__restrict__ x, T *__restrict__ y, int
         T s = T(0);
         for (int j = A.row[i]; j < A.row[i + 1]; ++j) s += val[j] * x[(size_t)b * A.n + A.col[j]];
         y[(size_t)b * A.n + i] = s;
+    }
+}
+
+// Sliced ELL copy (pp_sell layout) of a CSR matrix: one work-item per row slot of the padded
+// slices; padding entries get column 0 and value 0.
+template <typename T, typename TO>
+__global__ void k_sell_fill(const int32_t *__restrict__ row, const int32_t *__restrict__ col, const T *__restrict__ val,
+                            int n, const int32_t *__restrict__ sliceOff, const int32_t *__restrict__ sliceW,
+                            int32_t *__restrict__ sellCol, TO *__restrict__ sellVal) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (n + 63) / 64 * 64) return;
+    const int sl = i >> 6, w = sliceW[sl], base = sliceOff[sl] + (i & 63);
+    const int b = i < n ? row[i] : 0, len = i < n ? row[i + 1] - b : 0;
+    for (int k = 0; k < w; ++k) {
+        const bool has = k < len;
+        if (sellCol) sellCol[base + 64 * k] = has ? col[b + k] : 0;
+        if (sellVal) sellVal[base + 64 * k] = has ? TO(val[b + k]) : TO(0);
     }
 }
 
@@ -2324,9 +2376,16 @@ int pp_amg_pcg_solve(const pp_csr *A, const pp_amg_level *levels, int32_t num_le
     PP_CHECK_ARG(nc > 0, "empty coarse level");
     PP_CHECK_ARG(A->num_matrices == 1 || A->num_matrices == nrhs, "system matrix must be shared or one per right-hand side");
     for (int l = 0; l + 1 < num_levels; ++l) PP_CHECK_ARG(levels[l].n_coarse == levels[l + 1].n, "inconsistent level sizes");
-    for (int l = 0; l < num_levels; ++l)
+    auto sell_ok = [](const pp_sell &sd) {
+        const bool any = sd.slice_off || sd.slice_width || sd.col || sd.val;
+        return !any || (sd.slice_off && sd.slice_width && sd.col && sd.val);
+    };
+    for (int l = 0; l < num_levels; ++l) {
         PP_CHECK_ARG(levels[l].a_ell_width == 0 || (levels[l].a_ell_width > 0 && levels[l].a_ell_col && levels[l].a_ell_val),
                      "level ELL copy needs a positive width, columns and values");
+        PP_CHECK_ARG(sell_ok(levels[l].a_sell) && sell_ok(levels[l].r_sell), "sliced ELL copy needs all four arrays");
+        PP_CHECK_ARG(!levels[l].a_sell.slice_off || !levels[l].a_batched, "sliced ELL copy of a batched level operator");
+    }
     PP_CHECK_ARG(levels[num_levels - 1].n_coarse == nc, "coarsest level size mismatch");
     PP_CHECK_ARG(ntols == 0 || ntols == 1 || ntols == nrhs, "tolerance count");
     std::lock_guard<std::mutex> lock(g_mutex);
@@ -2445,6 +2504,35 @@ int pp_ell_from_csr(const int32_t *row, const int32_t *col, const void *val, int
     else if (dtype == PP_F32 && ell_dtype == PP_F64)
         hipLaunchKernelGGL((k_ell_from_csr<float, double>), g, dim3(256), 0, s, row, col, (const float *)val, n, nnz, width, ell_col, (double *)ell_val);
     else { set_error("pp_ell_from_csr: dtype"); return PP_ERR_ARG; }
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_sell_fill(const int32_t *row, const int32_t *col, const void *val, int32_t n, const int32_t *slice_off,
+                 const int32_t *slice_width, int32_t dtype, int32_t *sell_col, void *sell_val, int32_t sell_dtype,
+                 void *stream) {
+    PP_CHECK_ARG(row && slice_off && slice_width && n >= 0, "bad arguments");
+    PP_CHECK_ARG(!sell_col || col, "sell_col needs col");
+    PP_CHECK_ARG(!sell_val || val, "sell_val needs val");
+    if (n == 0) return PP_OK;
+    const dim3 g(((n + 63) / 64 * 64 + 255) / 256);
+    hipStream_t s = as_stream(stream);
+    if (!sell_val)
+        hipLaunchKernelGGL((k_sell_fill<float, float>), g, dim3(256), 0, s, row, col, (const float *)nullptr, n, slice_off,
+                           slice_width, sell_col, (float *)nullptr);
+    else if (dtype == PP_F64 && sell_dtype == PP_F64)
+        hipLaunchKernelGGL((k_sell_fill<double, double>), g, dim3(256), 0, s, row, col, (const double *)val, n, slice_off,
+                           slice_width, sell_col, (double *)sell_val);
+    else if (dtype == PP_F64 && sell_dtype == PP_F32)
+        hipLaunchKernelGGL((k_sell_fill<double, float>), g, dim3(256), 0, s, row, col, (const double *)val, n, slice_off,
+                           slice_width, sell_col, (float *)sell_val);
+    else if (dtype == PP_F32 && sell_dtype == PP_F32)
+        hipLaunchKernelGGL((k_sell_fill<float, float>), g, dim3(256), 0, s, row, col, (const float *)val, n, slice_off,
+                           slice_width, sell_col, (float *)sell_val);
+    else if (dtype == PP_F32 && sell_dtype == PP_F64)
+        hipLaunchKernelGGL((k_sell_fill<float, double>), g, dim3(256), 0, s, row, col, (const float *)val, n, slice_off,
+                           slice_width, sell_col, (double *)sell_val);
+    else { set_error("pp_sell_fill: dtype"); return PP_ERR_ARG; }
     PP_LAUNCH_CHECK();
     return PP_OK;
 }

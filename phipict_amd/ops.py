@@ -269,23 +269,16 @@ def SolveLinear(A: CSRmatrix, RHS: torch.Tensor, x: torch.Tensor, maxIterations=
     return _results(res, nrhs)
 
 
-def pack_amg_levels(levels: Sequence[Sequence[torch.Tensor]], hd) -> tuple:
+def pack_amg_levels(levels: Sequence[Sequence[torch.Tensor]], hd, extras: Optional[Sequence[dict]] = None) -> tuple:
     """ctypes level descriptors for ``levels`` (per level: A row, col, val, dinv, R row, col,
-    val, P row, col, val, all values in ``hd``, optionally followed by an ELL copy of A as
-    (columns, values, width)); returns (array, tensors kept alive)."""
+    val, P row, col, val, all values in ``hd``) and optional per-level coalesced copies
+    (``extras[i]``: "ell" = (columns, values, width) of A, "a_sell" / "r_sell" = (slice
+    offsets, slice widths, columns, values) of A / R); returns (array, tensors kept alive)."""
     lv = (_lib.AmgLevel * len(levels))()
     keep = []
     for i, t in enumerate(levels):
-        if len(t) not in (10, 13):
-            raise ValueError("each AMG level needs (A row, col, val, dinv, R row, col, val, P row, col, val"
-                             "[, ELL col, ELL val, ELL width])")
-        if len(t) == 13:
-            n = t[0].numel() - 1
-            if t[11].dtype != hd or t[10].numel() != t[12] * n or t[11].numel() % (t[12] * n):
-                raise ValueError("ELL copy does not match the level operator")
-            keep.extend((t[10], t[11]))
-            lv[i].a_ell_width, lv[i].a_ell_col, lv[i].a_ell_val = int(t[12]), _lib.ptr(t[10]), _lib.ptr(t[11])
-        t = t[:10]
+        if len(t) != 10:
+            raise ValueError("each AMG level needs (A row, col, val, dinv, R row, col, val, P row, col, val)")
         if t[2].dtype != hd or t[3].dtype != hd or t[6].dtype != hd or t[9].dtype != hd:
             raise TypeError("all hierarchy values must have the dtype of the coarse pseudo-inverse")
         L = lv[i]
@@ -299,6 +292,20 @@ def pack_amg_levels(levels: Sequence[Sequence[torch.Tensor]], hd) -> tuple:
             tt = tt.contiguous()
             keep.append(tt)
             setattr(L, name, _lib.ptr(tt))
+        ex = extras[i] if extras is not None else {}
+        if "ell" in ex:
+            col, val, width = ex["ell"]
+            if val.dtype != hd or col.numel() != width * L.n or val.numel() % (width * L.n):
+                raise ValueError("ELL copy does not match the level operator")
+            keep.extend((col, val))
+            L.a_ell_width, L.a_ell_col, L.a_ell_val = int(width), _lib.ptr(col), _lib.ptr(val)
+        for key, field_ in (("a_sell", "a_sell"), ("r_sell", "r_sell")):
+            if key in ex:
+                off, width, col, val = ex[key]
+                if val.dtype != hd or col.numel() != val.numel():
+                    raise ValueError("sliced ELL copy does not match the level")
+                keep.extend((off, width, col, val))
+                setattr(L, field_, _lib.SellDesc(_lib.ptr(off), _lib.ptr(width), _lib.ptr(col), _lib.ptr(val)))
     return lv, keep
 
 

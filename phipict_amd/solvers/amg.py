@@ -49,6 +49,59 @@ class Level:
     # map from this level's operator values to the next level's (seg, src, w): Galerkin refresh
     gmap: Optional[tuple] = None
     native: dict = field(default_factory=dict)
+    # sliced ELL layouts of the operator ("A") and of R ("R"), per sparsity pattern: SellLayout
+    sell: dict = field(default_factory=dict)
+    native_extra: dict = field(default_factory=dict)
+
+
+# Levels and restrictions with at least this many rows get a sliced ELL copy read with one lane
+# per row (coalesced); with fewer rows one lane per row leaves too few wavefronts per SIMD to
+# hide memory latency and the CSR kernels with lane groups per row are used instead.
+SELL_MIN_ROWS = 262144
+
+
+class SellLayout:
+    """Slices of 64 rows of a CSR pattern padded to the slice's longest row: device slice
+    offsets, widths and columns (the pp_sell layout), plus value copies per dtype for
+    matrices whose values never change (R)."""
+
+    def __init__(self, row_host: np.ndarray, row_dev: torch.Tensor, col_dev: torch.Tensor):
+        n = row_host.size - 1
+        lengths = np.diff(row_host.astype(np.int64))
+        slices = (n + 63) // 64
+        padded = np.zeros(slices * 64, np.int64)
+        padded[:n] = lengths
+        width = padded.reshape(slices, 64).max(axis=1)
+        off = np.zeros(slices, np.int64)
+        off[1:] = np.cumsum(64 * width[:-1])
+        self.total = int(64 * width.sum())
+        if self.total >= 2 ** 31:
+            raise ValueError("sliced ELL copy too large for 32-bit offsets")
+        dev = row_dev.device
+        self.n = n
+        self.slice_off = _dev(off.astype(np.int32), dev)
+        self.slice_width = _dev(width.astype(np.int32), dev)
+        self.col = torch.empty(max(self.total, 1), dtype=torch.int32, device=dev)
+        self.row, self.src_col = row_dev, col_dev
+        _lib.call("pp_sell_fill", _lib.ptr(row_dev), _lib.ptr(col_dev), None, n, _lib.ptr(self.slice_off),
+                  _lib.ptr(self.slice_width), _lib.PP_F32, _lib.ptr(self.col), None, _lib.PP_F32,
+                  _lib.stream_ptr(dev))
+        self.values: dict = {}
+
+    def fill(self, val: torch.Tensor, dtype) -> torch.Tensor:
+        out = torch.empty(max(self.total, 1), dtype=dtype, device=val.device)
+        _lib.call("pp_sell_fill", _lib.ptr(self.row), _lib.ptr(self.src_col), _lib.ptr(val), self.n,
+                  _lib.ptr(self.slice_off), _lib.ptr(self.slice_width), _lib.dtype_code(val.dtype), None, _lib.ptr(out),
+                  _lib.dtype_code(dtype), _lib.stream_ptr(val.device))
+        return out
+
+    def cached_values(self, val: torch.Tensor, dtype) -> torch.Tensor:
+        if dtype not in self.values:
+            self.values[dtype] = self.fill(val, dtype)
+        return self.values[dtype]
+
+    def desc(self, val: torch.Tensor) -> tuple:
+        return (self.slice_off, self.slice_width, self.col, val)
 
 
 class AMGHierarchy:
@@ -95,15 +148,36 @@ class AMGHierarchy:
                           _lib.dtype_code(dtype), _lib.stream_ptr(A.value.device))
                 nat = (A.row, A.index, aval, dinv, lvl.R_row, lvl.R_col, lvl.R_val.to(dtype), lvl.P_row, lvl.P_col,
                        lvl.P_val.to(dtype))
-                if lvl is self.levels[0] and A.ell_col is not None and A.ell_width > 0:
-                    # ELL copy of the finest operator in the hierarchy's dtype (coalesced smoothing sweeps)
-                    ev = torch.empty(A.ell_width * A.getRows() * A.getBatchSize(), dtype=dtype, device=A.value.device)
-                    _lib.call("pp_ell_from_csr", _lib.ptr(A.row), _lib.ptr(A.index), _lib.ptr(A.value), A.getRows(),
-                              A.getNnz(), A.getBatchSize(), A.ell_width, _lib.dtype_code(A.getDtype()), None,
-                              _lib.ptr(ev), _lib.dtype_code(dtype), _lib.stream_ptr(A.value.device))
-                    nat = nat + (A.ell_col, ev, A.ell_width)
                 lvl.native[dtype] = nat
             out.append(nat)
+        return out
+
+    def native_extras(self, dtype) -> list:
+        """Per level above the coarsest: coalesced copies for the V-cycle kernels in ``dtype``
+        -- "ell": (columns, values, width) of the finest operator, "a_sell" / "r_sell":
+        sliced ELL copies (slice offsets, widths, columns, values) of large operators and
+        restrictions."""
+        out = []
+        for i, lvl in enumerate(self.levels[:-1]):
+            ex = lvl.native_extra.get(dtype)
+            if ex is None:
+                ex = {}
+                A = lvl.A
+                dev = A.value.device
+                if i == 0 and A.ell_col is not None and A.ell_width > 0:
+                    ev = torch.empty(A.ell_width * A.getRows() * A.getBatchSize(), dtype=dtype, device=dev)
+                    _lib.call("pp_ell_from_csr", _lib.ptr(A.row), _lib.ptr(A.index), _lib.ptr(A.value), A.getRows(),
+                              A.getNnz(), A.getBatchSize(), A.ell_width, _lib.dtype_code(A.getDtype()), None,
+                              _lib.ptr(ev), _lib.dtype_code(dtype), _lib.stream_ptr(dev))
+                    ex["ell"] = (A.ell_col, ev, A.ell_width)
+                layout = lvl.sell.get("A")
+                if i > 0 and layout is not None and A.getBatchSize() == 1:
+                    ex["a_sell"] = layout.desc(layout.fill(A.value.reshape(-1)[: A.getNnz()], dtype))
+                layout = lvl.sell.get("R")
+                if layout is not None:
+                    ex["r_sell"] = layout.desc(layout.cached_values(lvl.R_val, dtype))
+                lvl.native_extra[dtype] = ex
+            out.append(ex)
         return out
 
     def packed_levels(self, dtype) -> tuple:
@@ -120,15 +194,20 @@ class AMGHierarchy:
                 nat = self.native_levels(dtype)[0]
                 arr[0].a_val = nat[2].data_ptr()
                 arr[0].dinv = nat[3].data_ptr()
-                if len(nat) > 10:
-                    arr[0].a_ell_width, arr[0].a_ell_col, arr[0].a_ell_val = nat[12], nat[10].data_ptr(), nat[11].data_ptr()
+                ex = self.native_extras(dtype)[0]
+                if "ell" in ex:
+                    arr[0].a_ell_col, arr[0].a_ell_val = ex["ell"][0].data_ptr(), ex["ell"][1].data_ptr()
+                    arr[0].a_ell_width = ex["ell"][2]
                 else:
                     arr[0].a_ell_width, arr[0].a_ell_col, arr[0].a_ell_val = 0, None, None
-                # keep the tensors of levels >= 1 (shared with the parent) and the new level 0
+                # keep the tensors of levels >= 1 and of level 0's R copy (shared with the
+                # parent) and the new level 0
                 base = keep[0] if isinstance(keep, tuple) else keep
-                cache[dtype] = (arr, (base, list(t for t in nat if isinstance(t, torch.Tensor))))
+                new = [t for t in nat if isinstance(t, torch.Tensor)]
+                new += [t for v in ex.values() for t in v if isinstance(t, torch.Tensor)]
+                cache[dtype] = (arr, (base, new))
             else:
-                cache[dtype] = ops.pack_amg_levels(self.native_levels(dtype), dtype)
+                cache[dtype] = ops.pack_amg_levels(self.native_levels(dtype), dtype, self.native_extras(dtype))
             self.__dict__.pop("_packed_parent", None)
         return cache[dtype]
 
@@ -248,6 +327,10 @@ def build_amg_hierarchy(csr_mat: CSRmatrix, *, method: str = "ruge_stuben", max_
             fine_row, fine_col = (row, col) if i == 0 else lv["A"][:2]
             seg, src, w = galerkin_map_host(fine_row, fine_col, lv["P"], lv["R"], host[i + 1]["A"][0], host[i + 1]["A"][1])
             level.gmap = (_dev(seg, device), _dev(src, device), _dev(w, device, torch.float64))
+            if rr.size - 1 >= SELL_MIN_ROWS:
+                level.sell["R"] = SellLayout(rr, level.R_row, level.R_col)
+        if i > 0 and lv["n"] >= SELL_MIN_ROWS and "P" in lv:
+            level.sell["A"] = SellLayout(lv["A"][0], A.row, A.index)
         levels.append(level)
     return AMGHierarchy(levels, _dense(levels[-1].A), project_constant, num_pre_smooth, num_post_smooth, native_dtype,
                         time.perf_counter() - t0)
@@ -283,7 +366,7 @@ def galerkin_refresh(hierarchy: AMGHierarchy, csr_mat: CSRmatrix, *, project_con
     A = CSRmatrix(csr_mat.value.reshape(-1)[: csr_mat.getNnz()], csr_mat.index, csr_mat.row)
     A.share_ell_from(csr_mat)
     levels = [Level(n=fine.n, A=A, R_row=fine.R_row, R_col=fine.R_col, R_val=fine.R_val, P_row=fine.P_row,
-                    P_col=fine.P_col, P_val=fine.P_val)]
+                    P_col=fine.P_col, P_val=fine.P_val, sell=fine.sell)]
     levels[0].gmap = fine.gmap
     if coarse:
         for i in range(1, len(hierarchy.levels)):
@@ -303,7 +386,8 @@ def galerkin_refresh(hierarchy: AMGHierarchy, csr_mat: CSRmatrix, *, project_con
                           _lib.ptr(prev.P_val), _lib.ptr(old.A.row), _lib.ptr(old.A.index), _lib.ptr(cval), old.n,
                           old.A.getNnz(), _lib.dtype_code(A.getDtype()), stream)
             levels.append(Level(n=old.n, A=CSRmatrix(cval, old.A.index, old.A.row), R_row=old.R_row, R_col=old.R_col,
-                                R_val=old.R_val, P_row=old.P_row, P_col=old.P_col, P_val=old.P_val, gmap=old.gmap))
+                                R_val=old.R_val, P_row=old.P_row, P_col=old.P_col, P_val=old.P_val, gmap=old.gmap,
+                                sell=old.sell))
         coarse_A = _dense(levels[-1].A)
     else:
         for old in hierarchy.levels[1:]:

@@ -359,6 +359,66 @@ def test_amg_pcg_matches_oracle(combo, project):
         assert rel_l2(a, o) < (1e-9 if exact_h else (1e-4 if dt_s == np.float64 else 1e-3)), rel_l2(a, o)
 
 
+def test_sliced_ell_layout_round_trip():
+    """pp_sell_fill: every CSR entry lands at slice_off[s] + 64 k + j of row 64 s + j, padding
+    holds column 0 and value 0, and the slice widths are the slices' longest rows."""
+    from phipict_amd.solvers import amg
+
+    row, col, val = poisson(23, 17, shift=1e-3)
+    keep = np.random.default_rng(3).random(col.size) < 0.8  # ragged rows
+    keep[row[:-1]] = True
+    rows = np.repeat(np.arange(row.size - 1), np.diff(row))
+    col, val, rows = col[keep], val[keep], rows[keep]
+    row = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=row.size - 1))]).astype(np.int32)
+    n = row.size - 1
+    lay = amg.SellLayout(row, torch.as_tensor(row, device=DEV), torch.as_tensor(col.astype(np.int32), device=DEV))
+    sv = to_np(lay.fill(torch.as_tensor(val, device=DEV), torch.float64))
+    sc, off, wid = to_np(lay.col), to_np(lay.slice_off), to_np(lay.slice_width)
+    lengths = np.diff(row)
+    for s in range((n + 63) // 64):
+        rs = range(64 * s, min(n, 64 * s + 64))
+        assert wid[s] == max(lengths[i] for i in rs)
+        for i in rs:
+            j = i - 64 * s
+            pos = off[s] + j + 64 * np.arange(wid[s])
+            np.testing.assert_array_equal(sc[pos[: lengths[i]]], col[row[i]:row[i + 1]])
+            np.testing.assert_array_equal(sv[pos[: lengths[i]]], val[row[i]:row[i + 1]])
+            assert (sc[pos[lengths[i]:]] == 0).all() and (sv[pos[lengths[i]:]] == 0).all()
+
+
+@pytest.mark.parametrize("combo", ["f64/f32", "f64/f64"])
+@pytest.mark.parametrize("project", [False, True])
+def test_amg_pcg_sliced_ell_matches_oracle(monkeypatch, combo, project):
+    """The V-cycle on sliced ELL copies of the coarse operators and restrictions (used for
+    large levels; here forced on every level) gives the oracle's solves."""
+    print(f"[amg sell] combo={combo} project={project}", flush=True)
+    from phipict_amd.solvers import amg
+
+    monkeypatch.setattr(amg, "SELL_MIN_ROWS", 1)
+    dt_s, dt_h = {"f64/f32": (np.float64, np.float32), "f64/f64": (np.float64, np.float64)}[combo]
+    row, col, val = poisson(96, 80, shift=0.0 if project else 1e-3)
+    n = row.size - 1
+    A, h, hs = _hier_pair(row, col, val, dt_s, project, dt_h)
+    assert all("R" in lv.sell for lv in h.levels[:-1]) and all("A" in lv.sell for lv in h.levels[1:-1])
+    b = np.random.default_rng(12).standard_normal(n)
+    if project:
+        b -= b.mean()
+    b = b.astype(dt_s)
+    tol = 1e-9
+    x = torch.zeros(n, dtype=_tdt(dt_s), device=DEV)
+    infos = amg.amg_pcg_solve(torch.as_tensor(b, device=DEV), x, h, tol=tol, max_iter=500)
+    xo = np.zeros(n, dt_s)
+    ref = O.amg_pcg((row, col, val.astype(dt_s)), hs.native, hs.pinv, b.copy(), xo, tol, maxit=500, project=project,
+                    hdtype=dt_h)
+    assert ref.converged
+    exact_h = dt_h == np.float64
+    check_info(infos[0], ref, dt_s, "sell", 0 if exact_h else 2)
+    a, o = to_np(x), xo
+    if project:
+        a, o = a - a.mean(), o - o.mean()
+    assert rel_l2(a, o) < (1e-9 if exact_h else 1e-4), rel_l2(a, o)
+
+
 def test_galerkin_refresh_matches_oracle():
     """Device R A P of a new fine operator on a cached interpolation equals the oracle's
     double-precision Galerkin product; the finest-only refresh keeps the coarse levels."""

@@ -52,7 +52,8 @@ def test_descriptor_layout_matches_header(tmp_path):
     from phipict_amd import _lib
 
     structs = {"pp_boundary_desc": _lib.BoundaryDesc, "pp_block_desc": _lib.BlockDesc, "pp_csr": _lib.CsrDesc,
-               "pp_domain": _lib.DomainDesc, "pp_solve_result": _lib.SolveResult, "pp_amg_level": _lib.AmgLevel}
+               "pp_domain": _lib.DomainDesc, "pp_solve_result": _lib.SolveResult, "pp_amg_level": _lib.AmgLevel,
+               "pp_sell": _lib.SellDesc}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
     for cname, ct in structs.items():
         lines.append(f'printf("{cname} __sizeof__ %zu\\n", sizeof({cname}));')
