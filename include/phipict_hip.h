@@ -178,8 +178,9 @@ int pp_lorentz_source(const void *J, int32_t total, int32_t batch, int32_t dims,
                       const void *src, int32_t src_kind, int32_t src_batched, void *out, int32_t dtype, void *stream);
 
 /* ---- measurement (no counterpart in the reference) ----
- * A sampled per-kernel timer: while started, every sample_every-th launch of each
- * instrumented kernel is bracketed by HIP events on the stream it is launched on.
+ * A sampled per-kernel timer: while started, a pseudo-random 1/sample_every of the launches of
+ * each instrumented kernel (all of them for 1) is bracketed by HIP events on the stream it is
+ * launched on.
  * pp_kernel_timer_read returns, per (kernel id, rows per launch), the summed duration of the
  * sampled launches, their number and the number of all launches; *count is the number of
  * entries (may exceed capacity). */

@@ -35,7 +35,10 @@ TimerSlot *timer_begin(int kernel_id, int rows, hipStream_t s) {
     if (!g_timer.enabled) return nullptr;
     std::lock_guard<std::mutex> lock(g_timer.mu);
     TimerStats &st = g_timer.stats[{kernel_id, rows}];
-    const bool sample = (st.launches++ % g_timer.every) == 0;
+    // sample a pseudo-random 1/every of the launches (a multiplicative hash of the launch count):
+    // a fixed stride would alias with the launch pattern of a solve (e.g. 4 phases per solve)
+    const uint32_t h = (uint32_t)(st.launches++) * 2654435761u;
+    const bool sample = g_timer.every == 1 || (h >> 16) % (uint32_t)g_timer.every == 0;
     if (!sample) return nullptr;
     TimerSlot *slot;
     if (!g_timer.pool.empty()) {

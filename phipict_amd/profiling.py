@@ -1,5 +1,6 @@
-"""Sampled per-kernel timing through the library's event timer (``pp_kernel_timer_*``): every
-k-th launch of each instrumented kernel is bracketed by HIP events on the stream it runs on."""
+"""Sampled per-kernel timing through the library's event timer (``pp_kernel_timer_*``): a
+pseudo-random 1/k of the launches of each instrumented kernel is bracketed by HIP events on the
+stream it runs on; and summaries of rocprofv3 memory-counter runs."""
 from __future__ import annotations
 
 import ctypes as C
