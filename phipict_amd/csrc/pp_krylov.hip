@@ -777,29 +777,32 @@ __global__ void __launch_bounds__(BLOCK) k_fin_residual(BatchState *st, int B, c
     finish_last(st, B, ctrl);
 }
 
-enum PrecondMode { PM_NONE = 0, PM_VECTOR = 2 };
+// PM_VECTOR: z given; PM_VECTOR_SHIFTED: z = u - mean(u) formed from u as it is read
+enum PrecondMode { PM_NONE = 0, PM_VECTOR = 2, PM_VECTOR_SHIFTED = 3 };
 
 // CG direction: p' = z + beta p (z = r or the preconditioned vector u); q = A p'; (p', q).
 template <typename T, int NB, int PM>
 __global__ void __launch_bounds__(BLOCK) k_cg_direction(Csr<T> A, const T *__restrict__ r, const T *__restrict__ z,
-                                                        const T *__restrict__ pIn, T *__restrict__ pOut,
-                                                        T *__restrict__ q, BatchState *st, int B, double *partial,
-                                                        Control *ctrl) {
+                                                        const double *__restrict__ zShift, const T *__restrict__ pIn,
+                                                        T *__restrict__ pOut, T *__restrict__ q, BatchState *st, int B,
+                                                        double *partial, Control *ctrl) {
     __shared__ double sm[16 * NB];
     if (ctrl->allDone) return;
     const int n = A.n;
     const Group grp = batch_group<NB>(B, A.rpm);
     bool active[NB], restart[NB];
-    T beta[NB];
+    T beta[NB], shift[NB];
     bool any = false;
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
         active[k] = k < grp.nb && st[grp.b0 + k].done == RUNNING;
         restart[k] = active[k] && st[grp.b0 + k].restart;
         beta[k] = active[k] ? T(st[grp.b0 + k].beta) : T(0);
+        shift[k] = (active[k] && PM == PM_VECTOR_SHIFTED) ? T(zShift[grp.b0 + k]) : T(0);
         any |= active[k];
     }
-    const T *zz = PM == PM_VECTOR ? z : r;
+    // z = u - mean(u) (the projected preconditioner output) when zShift is given
+    const T *zz = PM == PM_NONE ? r : z;
     double acc[NB][1];
 #pragma unroll
     for (int k = 0; k < NB; ++k) acc[k][0] = 0.0;
@@ -820,7 +823,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_direction(Csr<T> A, const T *__res
                 for (int k = 0; k < NB; ++k)
                     if (active[k]) {
                         const size_t o = (size_t)(grp.b0 + k) * n + c;
-                        const T zc = zz[o];
+                        const T zc = PM == PM_VECTOR_SHIFTED ? zz[o] - shift[k] : zz[o];
                         sum[k] += a * (restart[k] ? zc : zc + beta[k] * pIn[o]);
                     }
             }
@@ -828,7 +831,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_direction(Csr<T> A, const T *__res
             for (int k = 0; k < NB; ++k)
                 if (active[k]) {
                     const size_t o = (size_t)(grp.b0 + k) * n + i;
-                    const T zi = zz[o];
+                    const T zi = PM == PM_VECTOR_SHIFTED ? zz[o] - shift[k] : zz[o];
                     const T pi = restart[k] ? zi : zi + beta[k] * pIn[o];
                     pOut[o] = pi;
                     q[o] = sum[k];
@@ -865,13 +868,14 @@ template <typename T, int NB, int PM>
 __global__ void __launch_bounds__(BLOCK) k_cg_update(int n, int rpm, T *__restrict__ r, const T *__restrict__ p,
                                                      const T *__restrict__ q, T *__restrict__ x, T *__restrict__ xbest,
                                                      BatchState *st, int B, double *partial, Control *ctrl,
-                                                     double normScale, int returnBest) {
+                                                     double normScale, int returnBest, double *sumPartial) {
+    // sumPartial (optional): per-workgroup sums of the new r (its mean for the next projection)
     __shared__ double sm[16 * NB];
     if (ctrl->allDone) return;
     const Group grp = batch_group<NB>(B, rpm);
-    double acc[NB][1];
+    double acc[NB][1], rsum[NB][1];
 #pragma unroll
-    for (int k = 0; k < NB; ++k) acc[k][0] = 0.0;
+    for (int k = 0; k < NB; ++k) acc[k][0] = rsum[k][0] = 0.0;
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
         if (k >= grp.nb) break;
@@ -888,20 +892,29 @@ __global__ void __launch_bounds__(BLOCK) k_cg_update(int n, int rpm, T *__restri
             const T ri = r[o] - alpha * q[o];
             r[o] = ri;
             acc[k][0] += double(ri) * double(ri);
+            rsum[k][0] += double(ri);
         }
     }
     store_partials<NB, 1>(acc, grp.b0, grp.nb, partial, sm);
+    if (sumPartial) store_partials<NB, 1>(rsum, grp.b0, grp.nb, sumPartial, sm);
 }
 
 // Finish of k_cg_update: iteration count, convergence, best iterate, beta (unpreconditioned).
 template <int PM>
 __global__ void __launch_bounds__(BLOCK) k_fin_cg_update(BatchState *st, int B, const double *partial, int G,
-                                                         Control *ctrl, double normScale, int returnBest) {
+                                                         Control *ctrl, double normScale, int returnBest,
+                                                         const double *sumPartial, int n, double *meanOut) {
+    // with sumPartial: also meanOut[b] = mean of the new r (for the next projected V-cycle)
     __shared__ double sm[16];
     if (ctrl->allDone) return;
     {
         for (int bb = 0; bb < B; ++bb) {
             if (st[bb].done != RUNNING) continue;
+            if (sumPartial) {
+                double rs[1];
+                reduce_batch<1>(sumPartial, bb, G, rs, sm);
+                if (threadIdx.x == 0) meanOut[bb] = rs[0] / double(n);
+            }
             double red[1];
             reduce_batch<1>(partial, bb, G, red, sm);
             if (threadIdx.x == 0) {
@@ -921,7 +934,7 @@ __global__ void __launch_bounds__(BLOCK) k_fin_cg_update(BatchState *st, int B, 
                     if (crit < s.tol) s.done = CONVERGED;
                     else if (s.it >= s.maxit) s.done = MAXIT;
                     else if (returnBest && s.rising >= RISING_CUTOFF) s.done = RISING;
-                    else if (PM != PM_VECTOR) {
+                    else if (PM == PM_NONE) {
                         const double gamma = red[0];
                         s.beta = gamma / s.rho;
                         s.rho = gamma;
@@ -936,16 +949,22 @@ __global__ void __launch_bounds__(BLOCK) k_fin_cg_update(BatchState *st, int B, 
 }
 
 // (r, z) for an explicit preconditioned vector z; last workgroup: rho and beta (kCgDots).
-template <typename T>
+template <typename T, bool SHIFT>
 __global__ void __launch_bounds__(BLOCK) k_cg_dots(int n, const T *__restrict__ r, const T *__restrict__ z,
-                                                   BatchState *st, int B, double *partial, Control *ctrl) {
+                                                   const double *__restrict__ zShift, BatchState *st, int B,
+                                                   double *partial, Control *ctrl) {
+    // (r, z); with SHIFT z = u - mean(u), zShift holding the means (projected preconditioner output)
     __shared__ double sm[16];
     if (ctrl->allDone) return;
     const int b = blockIdx.y;
     double acc[1][1] = {{0.0}};
-    if (st[b].done == RUNNING)
-        FOR_BLOCK_ROWS(i, n)
-            acc[0][0] += double(r[(size_t)b * n + i]) * double(z[(size_t)b * n + i]);
+    if (st[b].done == RUNNING) {
+        const T sh = SHIFT ? T(zShift[b]) : T(0);
+        FOR_BLOCK_ROWS(i, n) {
+            const T zi = SHIFT ? z[(size_t)b * n + i] - sh : z[(size_t)b * n + i];
+            acc[0][0] += double(r[(size_t)b * n + i]) * double(zi);
+        }
+    }
     store_partials<1, 1>(acc, b, 1, partial, sm);
 }
 
@@ -1320,8 +1339,12 @@ __global__ void __launch_bounds__(BLOCK) k_presmooth_residual(Csr<H> A, const H 
 template <int W, typename TB, typename H, typename TO>
 __global__ void __launch_bounds__(BLOCK) k_jacobi_sweep(Csr<H> A, const H *__restrict__ dinv, const TB *__restrict__ b,
                                                         const double *shift, const H *__restrict__ x, TO *__restrict__ out,
-                                                        H omega, const BatchState *__restrict__ st) {
+                                                        H omega, const BatchState *__restrict__ st,
+                                                        double *outSumPartial) {
+    // outSumPartial (optional): per-workgroup sums of the written values (their mean follows)
+    __shared__ double sm[16];
     if (st[blockIdx.y].done != RUNNING) return;
+    double osum[1][1] = {{0.0}};
     const int n = A.n, batch = blockIdx.y;
     const H *val = A.val + (size_t)batch * A.valStride;
     const H *dv = dinv + (A.valStride ? (size_t)batch * n : 0);
@@ -1333,9 +1356,12 @@ __global__ void __launch_bounds__(BLOCK) k_jacobi_sweep(Csr<H> A, const H *__res
         const H sum = level_row_sum<W, H>(A, val, ev, i, valid, lane, [&](int c) { return x[off + c]; });
         if (valid && lane == 0) {
             const H bi = load_rhs<TB, H>(b, off + i, shift, batch);
-            out[off + i] = TO(x[off + i] + omega * dv[i] * (bi - sum));
+            const TO o = TO(x[off + i] + omega * dv[i] * (bi - sum));
+            out[off + i] = o;
+            osum[0][0] += double(o);
         }
     }
+    if (outSumPartial) store_partials<1, 1>(osum, batch, 1, outSumPartial, sm);
 }
 
 template <int W, typename TB, typename H>
@@ -1841,13 +1867,13 @@ static int cg_grid(const Csr<T> &A, const T *b, T *x, int B, int maxit, const do
             }
             PP_TIMED(PP_K_CG_DIRECTION, n, strm,
                 hipLaunchKernelGGL((k_cg_direction<T, NB, PM_NONE>), grid, dim3(BLOCK), 0, strm, A, r, (const T *)nullptr,
-                                   pIn, pOut, q, st, B, partial, ctrl));
+                                   (const double *)nullptr, pIn, pOut, q, st, B, partial, ctrl));
             hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl);
             PP_TIMED(PP_K_CG_UPDATE, n, strm,
                 hipLaunchKernelGGL((k_cg_update<T, NB, PM_NONE>), grid, dim3(BLOCK), 0, strm, n, A.rpm, r, pOut, q, x, xbest,
-                                   st, B, partial, ctrl, normScale, returnBest));
+                                   st, B, partial, ctrl, normScale, returnBest, (double *)nullptr));
             hipLaunchKernelGGL((k_fin_cg_update<PM_NONE>), dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl,
-                               normScale, returnBest));
+                               normScale, returnBest, (const double *)nullptr, n, (double *)nullptr));
         parity ^= 1;
     };
     int fin = 0;
@@ -1951,7 +1977,9 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
     const H omega = H(omegaD);
     const int G = grid_width(n, B);
     enum { R = 0, Q, P0, P1, X_UNUSED, BW, XB, U, NUMV };
-    if (int e = prepare_workspace(ws, NUMV, n, B, sizeof(T), (size_t)B * G * 3 + 2 * B, tols, ntols, s)) return e;
+    // partial sums: [phase partials | sums of r | sums of u] regions of B * GP each, then the means
+    const int GP = std::max(G, std::max(1, num_cus() * 8 / B));
+    if (int e = prepare_workspace(ws, NUMV, n, B, sizeof(T), (size_t)B * GP * 3 + 2 * B, tols, ntols, s)) return e;
     // hierarchy vectors: x, res, y, b per level + coarsest b and x
     std::vector<size_t> offs(L + 1);
     size_t total = 0;
@@ -1971,7 +1999,8 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
     BatchState *st = state_of(ws);
     Control *ctrl = ctrl_of(ws);
     double *partial = ws.partial;
-    double *meanR = partial + (size_t)B * G * 3, *meanU = meanR + B;
+    double *sumR = partial + (size_t)B * GP, *sumU = partial + (size_t)2 * B * GP;
+    double *meanR = partial + (size_t)3 * B * GP, *meanU = meanR + B;
     const dim3 grid(G, B);
     hipLaunchKernelGGL(k_init_state, dim3(1), dim3(64), 0, s, st, B, ntols == 1 ? tols[0] : 0.0, maxit, ctrl,
                        ntols > 1 ? ws.tols : nullptr);
@@ -1983,6 +2012,12 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
     }
     hipLaunchKernelGGL((k_residual<T, 1>), grid, dim3(BLOCK), 0, s, Af, bw, x, r, (T *)nullptr, st, B, partial, ctrl, normScale, 0);
     hipLaunchKernelGGL(k_fin_residual, dim3(1), dim3(BLOCK), 0, s, st, B, partial, G, ctrl, normScale, 0, 0);
+    if (project) {
+        // mean of the initial residual for the first projected V-cycle; later ones come with
+        // the residual updates (k_cg_update / k_fin_cg_update)
+        hipLaunchKernelGGL((k_mean<T>), grid, dim3(BLOCK), 0, s, r, n, B, meanR, partial, ctrl, st, 1);
+        hipLaunchKernelGGL(k_fin_mean, dim3(1), dim3(BLOCK), 0, s, partial, G, n, B, meanR, ctrl, 1);
+    }
     PP_LAUNCH_CHECK();
     const int cus = num_cus();
     auto lgrid = [&](int rows, int W) {
@@ -2040,14 +2075,13 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
         PP_HIP(hipMemcpyAsync(ws.tailDev, ws.tailHost, sizeof(tail), hipMemcpyHostToDevice, s));
     }
     const TailArgs<H> *tailDev = reinterpret_cast<const TailArgs<H> *>(ws.tailDev);
+    // With the constant-mode projection the V-cycle runs on r - mean(r) (mean from the last
+    // residual update) and its output u is used as z = u - mean(u): the last finest-level sweep
+    // leaves per-workgroup sums of u, one finish kernel makes the mean, and the dot and
+    // direction kernels subtract it as they read u (the same T-arithmetic subtraction as
+    // storing u - mean(u) first).
+    const double *shift = project ? meanR : nullptr;
     auto vcycle = [&](hipStream_t strm) {
-        const double *shift = nullptr;
-        if (project) {
-            PP_TIMED(PP_K_MEAN, n, strm,
-                hipLaunchKernelGGL((k_mean<T>), grid, dim3(BLOCK), 0, strm, r, n, B, meanR, partial, ctrl, st, 1));
-            hipLaunchKernelGGL(k_fin_mean, dim3(1), dim3(BLOCK), 0, strm, partial, G, n, B, meanR, ctrl, 1);
-            shift = meanR;
-        }
         std::vector<H *> cur(L), oth(L);
         for (int l = 0; l < L; ++l) {
             if (l == lt) break; // levels lt.. run in the one-workgroup tail
@@ -2060,8 +2094,8 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
                 if (l == 0) PP_TIMED(PP_K_AMG_PRESMOOTH, lv[l].n, strm, hipLaunchKernelGGL((k_presmooth_residual<W, T, H>), g, dim3(BLOCK), 0, strm, LA[l], dinv, r, shift, cur[l], resl, omega, st));
                 else PP_TIMED(PP_K_AMG_PRESMOOTH, lv[l].n, strm, hipLaunchKernelGGL((k_presmooth_residual<W, H, H>), g, dim3(BLOCK), 0, strm, LA[l], dinv, lb(l), (const double *)nullptr, cur[l], resl, omega, st));
                 for (int k = 1; k < pre; ++k) {
-                    if (l == 0) hipLaunchKernelGGL((k_jacobi_sweep<W, T, H, H>), g, dim3(BLOCK), 0, strm, LA[l], dinv, r, shift, cur[l], oth[l], omega, st);
-                    else hipLaunchKernelGGL((k_jacobi_sweep<W, H, H, H>), g, dim3(BLOCK), 0, strm, LA[l], dinv, lb(l), (const double *)nullptr, cur[l], oth[l], omega, st);
+                    if (l == 0) hipLaunchKernelGGL((k_jacobi_sweep<W, T, H, H>), g, dim3(BLOCK), 0, strm, LA[l], dinv, r, shift, cur[l], oth[l], omega, st, (double *)nullptr);
+                    else hipLaunchKernelGGL((k_jacobi_sweep<W, H, H, H>), g, dim3(BLOCK), 0, strm, LA[l], dinv, lb(l), (const double *)nullptr, cur[l], oth[l], omega, st, (double *)nullptr);
                     std::swap(cur[l], oth[l]);
                 }
                 if (pre > 1) {
@@ -2093,38 +2127,37 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
             W_DISPATCH(wa[l],
                 for (int k = 0; k < post; ++k) {
                     if (l == 0) {
-                        if (k == post - 1) PP_TIMED(PP_K_AMG_POST_JACOBI, lv[l].n, strm, hipLaunchKernelGGL((k_jacobi_sweep<W, T, H, T>), g, dim3(BLOCK), 0, strm, LA[l], dinv, r, shift, cur[l], u, omega, st));
-                        else PP_TIMED(PP_K_AMG_JACOBI, lv[l].n, strm, hipLaunchKernelGGL((k_jacobi_sweep<W, T, H, H>), g, dim3(BLOCK), 0, strm, LA[l], dinv, r, shift, cur[l], oth[l], omega, st));
+                        if (k == post - 1) PP_TIMED(PP_K_AMG_POST_JACOBI, lv[l].n, strm, hipLaunchKernelGGL((k_jacobi_sweep<W, T, H, T>), g, dim3(BLOCK), 0, strm, LA[l], dinv, r, shift, cur[l], u, omega, st, project ? sumU : (double *)nullptr));
+                        else PP_TIMED(PP_K_AMG_JACOBI, lv[l].n, strm, hipLaunchKernelGGL((k_jacobi_sweep<W, T, H, H>), g, dim3(BLOCK), 0, strm, LA[l], dinv, r, shift, cur[l], oth[l], omega, st, (double *)nullptr));
                     } else {
-                        PP_TIMED(PP_K_AMG_POST_JACOBI, lv[l].n, strm, hipLaunchKernelGGL((k_jacobi_sweep<W, H, H, H>), g, dim3(BLOCK), 0, strm, LA[l], dinv, lb(l), (const double *)nullptr, cur[l], oth[l], omega, st));
+                        PP_TIMED(PP_K_AMG_POST_JACOBI, lv[l].n, strm, hipLaunchKernelGGL((k_jacobi_sweep<W, H, H, H>), g, dim3(BLOCK), 0, strm, LA[l], dinv, lb(l), (const double *)nullptr, cur[l], oth[l], omega, st, (double *)nullptr));
                     }
                     std::swap(cur[l], oth[l]);
                 });
         }
-        if (project) {
-            PP_TIMED(PP_K_MEAN, n, strm,
-                hipLaunchKernelGGL((k_mean<T>), grid, dim3(BLOCK), 0, strm, u, n, B, meanU, partial, ctrl, st, 1));
-            hipLaunchKernelGGL(k_fin_mean, dim3(1), dim3(BLOCK), 0, strm, partial, G, n, B, meanU, ctrl, 1);
-            PP_TIMED(PP_K_SUBTRACT_MEAN, n, strm,
-                hipLaunchKernelGGL((k_subtract_mean<T>), grid, dim3(BLOCK), 0, strm, u, n, meanU, st, 1));
-        }
+        if (project)
+            hipLaunchKernelGGL(k_fin_mean, dim3(1), dim3(BLOCK), 0, strm, sumU, (int)lgrid(n, wa[0]).x, n, B, meanU, ctrl, 1);
     };
     int parity = 0;
     auto body = [&](hipStream_t strm) {
         T *pIn = parity ? p1 : p0, *pOut = parity ? p0 : p1;
         vcycle(strm);
         PP_TIMED(PP_K_PCG_DOTS, n, strm,
-            hipLaunchKernelGGL((k_cg_dots<T>), grid, dim3(BLOCK), 0, strm, n, r, u, st, B, partial, ctrl));
+            if (project) hipLaunchKernelGGL((k_cg_dots<T, true>), grid, dim3(BLOCK), 0, strm, n, r, u, meanU, st, B, partial, ctrl);
+            else hipLaunchKernelGGL((k_cg_dots<T, false>), grid, dim3(BLOCK), 0, strm, n, r, u, (const double *)nullptr, st, B,
+                                    partial, ctrl));
         hipLaunchKernelGGL(k_fin_cg_dots, dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl);
         PP_TIMED(PP_K_PCG_DIRECTION, n, strm,
-            hipLaunchKernelGGL((k_cg_direction<T, 1, PM_VECTOR>), grid, dim3(BLOCK), 0, strm, Af, r, u, pIn, pOut, q, st, B,
-                               partial, ctrl));
+            if (project) hipLaunchKernelGGL((k_cg_direction<T, 1, PM_VECTOR_SHIFTED>), grid, dim3(BLOCK), 0, strm, Af, r, u,
+                                            meanU, pIn, pOut, q, st, B, partial, ctrl);
+            else hipLaunchKernelGGL((k_cg_direction<T, 1, PM_VECTOR>), grid, dim3(BLOCK), 0, strm, Af, r, u,
+                                    (const double *)nullptr, pIn, pOut, q, st, B, partial, ctrl));
         hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl);
         PP_TIMED(PP_K_PCG_UPDATE, n, strm,
             hipLaunchKernelGGL((k_cg_update<T, 1, PM_VECTOR>), grid, dim3(BLOCK), 0, strm, n, 1, r, pOut, q, x, xbest, st, B,
-                               partial, ctrl, normScale, returnBest));
+                               partial, ctrl, normScale, returnBest, project ? sumR : (double *)nullptr));
         hipLaunchKernelGGL((k_fin_cg_update<PM_VECTOR>), dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl, normScale,
-                           returnBest);
+                           returnBest, project ? sumR : (const double *)nullptr, n, meanR);
         parity ^= 1;
     };
     int fin = 0;
@@ -2300,6 +2333,22 @@ __global__ void k_inverse_diagonal(const int32_t *__restrict__ row, const int32_
     T d = T(0);
     for (int k = row[i]; k < row[i + 1]; ++k)
         if (col[k] == i) d += v[k];
+    TO dd = TO(d);
+    const double ad = fabs(double(dd));
+    dinv[idx] = ad > 1e-30 ? TO(1) / dd : TO(1);
+}
+
+// The same through the ELL copy (entry k of row i at [k][i]; padding adds zeros on the diagonal)
+template <typename T, typename TO>
+__global__ void k_inverse_diagonal_ell(const int32_t *__restrict__ ellCol, const T *__restrict__ ellVal, int n,
+                                       int width, int nmat, TO *__restrict__ dinv) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)n * nmat) return;
+    const int m = idx / n, i = idx % n;
+    const T *v = ellVal + (size_t)m * width * n;
+    T d = T(0);
+    for (int k = 0; k < width; ++k)
+        if (ellCol[(size_t)k * n + i] == i) d += v[(size_t)k * n + i];
     TO dd = TO(d);
     const double ad = fabs(double(dd));
     dinv[idx] = ad > 1e-30 ? TO(1) / dd : TO(1);
@@ -2599,7 +2648,16 @@ int pp_inverse_diagonal(const pp_csr *A, int32_t dtype, void *dinv, int32_t dtyp
     const size_t total = (size_t)A->n * A->num_matrices;
     const dim3 g((total + 255) / 256);
     hipStream_t s = as_stream(stream);
-    if (dtype == PP_F64 && dtype_out == PP_F64)
+    const bool ell = A->ell_width > 0 && A->ell_col && A->ell_val;
+    if (ell && dtype == PP_F64 && dtype_out == PP_F64)
+        hipLaunchKernelGGL((k_inverse_diagonal_ell<double, double>), g, dim3(256), 0, s, A->ell_col, (const double *)A->ell_val, A->n, A->ell_width, A->num_matrices, (double *)dinv);
+    else if (ell && dtype == PP_F64 && dtype_out == PP_F32)
+        hipLaunchKernelGGL((k_inverse_diagonal_ell<double, float>), g, dim3(256), 0, s, A->ell_col, (const double *)A->ell_val, A->n, A->ell_width, A->num_matrices, (float *)dinv);
+    else if (ell && dtype == PP_F32 && dtype_out == PP_F32)
+        hipLaunchKernelGGL((k_inverse_diagonal_ell<float, float>), g, dim3(256), 0, s, A->ell_col, (const float *)A->ell_val, A->n, A->ell_width, A->num_matrices, (float *)dinv);
+    else if (ell && dtype == PP_F32 && dtype_out == PP_F64)
+        hipLaunchKernelGGL((k_inverse_diagonal_ell<float, double>), g, dim3(256), 0, s, A->ell_col, (const float *)A->ell_val, A->n, A->ell_width, A->num_matrices, (double *)dinv);
+    else if (dtype == PP_F64 && dtype_out == PP_F64)
         hipLaunchKernelGGL((k_inverse_diagonal<double, double>), g, dim3(256), 0, s, A->row, A->col, (const double *)A->val, A->n, A->nnz, A->num_matrices, (double *)dinv);
     else if (dtype == PP_F64 && dtype_out == PP_F32)
         hipLaunchKernelGGL((k_inverse_diagonal<double, float>), g, dim3(256), 0, s, A->row, A->col, (const double *)A->val, A->n, A->nnz, A->num_matrices, (float *)dinv);

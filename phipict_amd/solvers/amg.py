@@ -57,7 +57,7 @@ class Level:
 # Levels and restrictions with at least this many rows get a sliced ELL copy read with one lane
 # per row (coalesced); with fewer rows one lane per row leaves too few wavefronts per SIMD to
 # hide memory latency and the CSR kernels with lane groups per row are used instead.
-SELL_MIN_ROWS = 262144
+SELL_MIN_ROWS = 131072
 
 
 class SellLayout:
