@@ -17,6 +17,8 @@ import torch
 
 from . import _lib
 
+_p = _lib.ptr
+
 
 class BoundaryType(IntEnum):
     """Face types (csrc/phipict/domain/domain_structs_gpu.h:146-161)."""
@@ -721,7 +723,10 @@ class Domain:
         self._blocks: list[Block] = []
         self._batch = 1
         self._initialized = False
+        self._desc_generation = 0
         self._changed = True
+        self._atlas_key = None
+        self._dom_key = None
         self.advectionScheme = AdvectionScheme.CENTRAL
         self.totalSize = 0
         self.C: Optional[CSRmatrix] = None
@@ -1000,15 +1005,37 @@ class Domain:
                     bd.conn_axes[i] = bound.axes[i] if i < len(bound.axes) else 0
         return d
 
+    def _block_key(self, b: Block) -> tuple:
+        """What a block descriptor depends on beyond the descriptor generation: the tensors it
+        points to (replaced tensors change the pointer, in-place writes to the transform its
+        version)."""
+        t, s, v = b.transform, b.velocitySource, b.viscosity
+        return (_p(b.velocity), _p(b.pressure), _p(b.epot), _p(s), -1 if s is None else s.dim(),
+                -1 if s is None else s.size(0), _p(v), -1 if v is None else v.numel(), _p(t),
+                -1 if t is None else t._version)
+
     def UpdateDomainData(self) -> None:
-        """Repack the device descriptors if any tensor of the domain changed."""
+        """Repack the device descriptors if any tensor of the domain changed (a no-op when the
+        descriptor generation and every referenced tensor are unchanged)."""
         if not self._initialized:
             raise RuntimeError("Domain is not initialized.")
-        arr = (_lib.BlockDesc * len(self._blocks))(*[self._block_desc(b) for b in self._blocks])
-        raw = bytes(arr)
-        if raw != self._atlas_bytes:
-            self._upload_atlas(raw)
-            self._atlas_bytes = raw
+        gen = self._desc_generation
+        akey = (gen, tuple(self._block_key(b) for b in self._blocks))
+        if akey != self._atlas_key:
+            arr = (_lib.BlockDesc * len(self._blocks))(*[self._block_desc(b) for b in self._blocks])
+            raw = bytes(arr)
+            if raw != self._atlas_bytes:
+                self._upload_atlas(raw)
+                self._atlas_bytes = raw
+            self._atlas_key = akey
+        dkey = (gen, _p(self._atlas), _p(self._time_step), _p(self.A), _p(self.velocityRHS), _p(self.velocityResult),
+                _p(self.pressureRHS), _p(self.pressureRHSdiv), _p(self.pressureResult), _p(self.epotRHS),
+                _p(self.epotResult), float(self.viscosity[0]),
+                *(None if M is None else (_p(M.value), _p(M.ell_val), M.ell_valid()) for M in (self.C, self.P, self.Epot)))
+        if dkey == self._dom_key:
+            self._changed = False
+            return
+        self._dom_key = dkey
         d = self._desc
         d.dims = self._dims
         d.num_blocks = len(self._blocks)
@@ -1068,6 +1095,18 @@ class Domain:
 
     def IsTensorChanged(self) -> bool:
         return self._changed
+
+    # Every mutator of a block, boundary or domain tensor sets _changed; setting it also bumps
+    # the descriptor generation that the cached descriptors are keyed on.
+    @property
+    def _changed(self) -> bool:
+        return self._changed_flag
+
+    @_changed.setter
+    def _changed(self, value: bool) -> None:
+        if value:
+            self._desc_generation = getattr(self, "_desc_generation", 0) + 1
+        self._changed_flag = bool(value)
 
     def desc(self) -> _lib.DomainDesc:
         d = self._desc

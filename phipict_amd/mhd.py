@@ -100,7 +100,13 @@ class MHDSimulation(Simulation):
         return self._e_b
 
     def _stuart(self, like: torch.Tensor) -> torch.Tensor:
-        return self._stuart_number.to(device=like.device, dtype=like.dtype)
+        """N on the device of ``like`` (cached: a host-to-device copy waits for the stream)."""
+        key = (like.device, like.dtype, self._stuart_number.data_ptr(), self._stuart_number._version)
+        cached = self.__dict__.get("_stuart_cache")
+        if cached is None or cached[0] != key:
+            cached = (key, self._stuart_number.to(device=like.device, dtype=like.dtype))
+            self._stuart_cache = cached
+        return cached[1]
 
     def _add_lorentz_force(self, domain: Domain, **kwargs: Any) -> None:
         """F_L = N (J x e_b) added to the velocity source of every block (mhd.py:543-594)."""
