@@ -608,6 +608,31 @@ constexpr int BLOCK = 256;
 // Infinity Cache serves.
 #define FOR_BLOCK_ROWS(i, n) for (int i = blockIdx.x * BLOCK + (int)threadIdx.x; i < (n); i += gridDim.x * BLOCK)
 
+// Entries (column, value) of row i of A, from its ELL copy or its CSR arrays; with EW > 0 the
+// ELL width is a compile-time constant and all column and value loads are issued up front.
+template <int EW, typename T, typename F>
+__device__ __forceinline__ void for_row_entries(const Csr<T> &A, const T *__restrict__ ellv, const T *__restrict__ val,
+                                                int n, int i, F f) {
+    if constexpr (EW > 0) {
+        int cs[EW];
+        T as[EW];
+#pragma unroll
+        for (int j = 0; j < EW; ++j) {
+            cs[j] = A.ell_col[(size_t)j * n + i];
+            as[j] = ellv[(size_t)j * n + i];
+        }
+#pragma unroll
+        for (int j = 0; j < EW; ++j) f(cs[j], as[j]);
+    } else {
+        const int ew = A.ell_w;
+        const int jb = ew ? 0 : A.row[i], je = ew ? ew : A.row[i + 1];
+        for (int j = jb; j < je; ++j) {
+            const size_t eo = (size_t)j * n + i;
+            f(ew ? A.ell_col[eo] : A.col[j], ew ? ellv[eo] : val[j]);
+        }
+    }
+}
+
 // Per-workgroup partial sums, written with plain stores. The one-workgroup finish kernel that
 // follows on the same stream combines them in a fixed order (deterministic); the kernel
 // boundary makes them visible across XCDs. (A last-workgroup-done scheme would need a
@@ -673,7 +698,7 @@ __global__ void k_init_state(BatchState *st, int B, double tol, int maxit, Contr
 }
 
 // r = b - A x (+ r0 = r for BiCGStab); reset mode only acts at multiples of resetEvery.
-template <typename T, int NB>
+template <typename T, int NB, int EW = 0>
 __global__ void __launch_bounds__(BLOCK) k_residual(Csr<T> A, const T *__restrict__ b, const T *__restrict__ x,
                                                     T *__restrict__ r, T *__restrict__ r0, BatchState *st, int B,
                                                     double *partial, Control *ctrl, double normScale, int resetEvery) {
@@ -693,20 +718,15 @@ __global__ void __launch_bounds__(BLOCK) k_residual(Csr<T> A, const T *__restric
     if (any) {
         const T *val = A.val + (size_t)grp.m * A.valStride;
         const T *ellv = A.ell_val + (A.valStride ? (size_t)grp.m * A.ell_w * n : 0);
-        const int ew = A.ell_w;
         FOR_BLOCK_ROWS(i, n) {
             T sum[NB];
 #pragma unroll
             for (int k = 0; k < NB; ++k) sum[k] = T(0);
-            const int jb = ew ? 0 : A.row[i], je = ew ? ew : A.row[i + 1];
-            for (int j = jb; j < je; ++j) {
-                const size_t eo = (size_t)j * n + i;
-                const int c = ew ? A.ell_col[eo] : A.col[j];
-                const T a = ew ? ellv[eo] : val[j];
+            for_row_entries<EW>(A, ellv, val, n, i, [&](int c, T a) {
 #pragma unroll
                 for (int k = 0; k < NB; ++k)
                     if (active[k]) sum[k] += a * x[(size_t)(grp.b0 + k) * n + c];
-            }
+            });
 #pragma unroll
             for (int k = 0; k < NB; ++k)
                 if (active[k]) {
@@ -781,7 +801,7 @@ __global__ void __launch_bounds__(BLOCK) k_fin_residual(BatchState *st, int B, c
 enum PrecondMode { PM_NONE = 0, PM_VECTOR = 2, PM_VECTOR_SHIFTED = 3 };
 
 // CG direction: p' = z + beta p (z = r or the preconditioned vector u); q = A p'; (p', q).
-template <typename T, int NB, int PM>
+template <typename T, int NB, int PM, int EW = 0>
 __global__ void __launch_bounds__(BLOCK) k_cg_direction(Csr<T> A, const T *__restrict__ r, const T *__restrict__ z,
                                                         const double *__restrict__ zShift, const T *__restrict__ pIn,
                                                         T *__restrict__ pOut, T *__restrict__ q, BatchState *st, int B,
@@ -814,11 +834,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_direction(Csr<T> A, const T *__res
             T sum[NB];
 #pragma unroll
             for (int k = 0; k < NB; ++k) sum[k] = T(0);
-            const int jb = ew ? 0 : A.row[i], je = ew ? ew : A.row[i + 1];
-            for (int j = jb; j < je; ++j) {
-                const size_t eo = (size_t)j * n + i;
-                const int c = ew ? A.ell_col[eo] : A.col[j];
-                const T a = ew ? ellv[eo] : val[j];
+            auto entry = [&](int c, T a) {
 #pragma unroll
                 for (int k = 0; k < NB; ++k)
                     if (active[k]) {
@@ -826,6 +842,24 @@ __global__ void __launch_bounds__(BLOCK) k_cg_direction(Csr<T> A, const T *__res
                         const T zc = PM == PM_VECTOR_SHIFTED ? zz[o] - shift[k] : zz[o];
                         sum[k] += a * (restart[k] ? zc : zc + beta[k] * pIn[o]);
                     }
+            };
+            if constexpr (EW > 0) {
+                // ELL rows of a compile-time width: all column and value loads issued up front
+                int cs[EW];
+                T as[EW];
+#pragma unroll
+                for (int j = 0; j < EW; ++j) {
+                    cs[j] = A.ell_col[(size_t)j * n + i];
+                    as[j] = ellv[(size_t)j * n + i];
+                }
+#pragma unroll
+                for (int j = 0; j < EW; ++j) entry(cs[j], as[j]);
+            } else {
+                const int jb = ew ? 0 : A.row[i], je = ew ? ew : A.row[i + 1];
+                for (int j = jb; j < je; ++j) {
+                    const size_t eo = (size_t)j * n + i;
+                    entry(ew ? A.ell_col[eo] : A.col[j], ew ? ellv[eo] : val[j]);
+                }
             }
 #pragma unroll
             for (int k = 0; k < NB; ++k)
@@ -1005,7 +1039,7 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_p(int n, const T *__restrict__ r
     }
 }
 
-template <typename T, int NB>
+template <typename T, int NB, int EW = 0>
 __global__ void __launch_bounds__(BLOCK) k_bicg_v(Csr<T> A, const T *__restrict__ p, T *__restrict__ v,
                                                   const T *__restrict__ r0, BatchState *st, int B, double *partial,
                                                   Control *ctrl) {
@@ -1022,20 +1056,15 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_v(Csr<T> A, const T *__restrict_
     if (any) {
         const T *val = A.val + (size_t)grp.m * A.valStride;
         const T *ellv = A.ell_val + (A.valStride ? (size_t)grp.m * A.ell_w * n : 0);
-        const int ew = A.ell_w;
         FOR_BLOCK_ROWS(i, n) {
             T sum[NB];
 #pragma unroll
             for (int k = 0; k < NB; ++k) sum[k] = T(0);
-            const int jb = ew ? 0 : A.row[i], je = ew ? ew : A.row[i + 1];
-            for (int j = jb; j < je; ++j) {
-                const size_t eo = (size_t)j * n + i;
-                const int c = ew ? A.ell_col[eo] : A.col[j];
-                const T a = ew ? ellv[eo] : val[j];
+            for_row_entries<EW>(A, ellv, val, n, i, [&](int c, T a) {
 #pragma unroll
                 for (int k = 0; k < NB; ++k)
                     if (active[k]) sum[k] += a * p[(size_t)(grp.b0 + k) * n + c];
-            }
+            });
 #pragma unroll
             for (int k = 0; k < NB; ++k)
                 if (active[k]) {
@@ -1048,7 +1077,7 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_v(Csr<T> A, const T *__restrict_
     store_partials<NB, 1>(acc, grp.b0, grp.nb, partial, sm);
 }
 
-template <typename T, int NB>
+template <typename T, int NB, int EW = 0>
 __global__ void __launch_bounds__(BLOCK) k_bicg_t(Csr<T> A, const T *__restrict__ r, const T *__restrict__ v,
                                                   T *__restrict__ t, BatchState *st, int B, double *partial,
                                                   Control *ctrl, double normScale) {
@@ -1070,23 +1099,18 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_t(Csr<T> A, const T *__restrict_
     if (any) {
         const T *val = A.val + (size_t)grp.m * A.valStride;
         const T *ellv = A.ell_val + (A.valStride ? (size_t)grp.m * A.ell_w * n : 0);
-        const int ew = A.ell_w;
         FOR_BLOCK_ROWS(i, n) {
             T sum[NB];
 #pragma unroll
             for (int k = 0; k < NB; ++k) sum[k] = T(0);
-            const int jb = ew ? 0 : A.row[i], je = ew ? ew : A.row[i + 1];
-            for (int j = jb; j < je; ++j) {
-                const size_t eo = (size_t)j * n + i;
-                const int c = ew ? A.ell_col[eo] : A.col[j];
-                const T a = ew ? ellv[eo] : val[j];
+            for_row_entries<EW>(A, ellv, val, n, i, [&](int c, T a) {
 #pragma unroll
                 for (int k = 0; k < NB; ++k)
                     if (active[k]) {
                         const size_t o = (size_t)(grp.b0 + k) * n + c;
                         sum[k] += a * (r[o] - alpha[k] * v[o]);
                     }
-            }
+            });
 #pragma unroll
             for (int k = 0; k < NB; ++k)
                 if (active[k]) {
@@ -1277,18 +1301,36 @@ template <int W, typename H, typename G>
 __device__ __forceinline__ H level_row_sum(const Csr<H> &A, const H *__restrict__ val, const H *__restrict__ ellv,
                                            int i, bool valid, int lane, G gather) {
     if (A.sell_off) {
+        // W lanes per row split the row's entries (entry k on lane k % W)
         H sum = H(0);
         if (valid) {
             const int sl = i >> 6;
             const int w = A.sell_w[sl];
             const int base = A.sell_off[sl] + (i & 63);
 #pragma unroll 4
-            for (int k = 0; k < w; ++k) sum += A.sell_val[base + 64 * k] * gather(A.sell_col[base + 64 * k]);
+            for (int k = lane; k < w; k += W) sum += A.sell_val[base + 64 * k] * gather(A.sell_col[base + 64 * k]);
         }
+#pragma unroll
+        for (int o = W / 2; o > 0; o >>= 1) sum += __shfl_down(sum, o, W);
         return sum;
     }
     if (!A.ell_w) return group_row_sum<W, H>(A.row, A.col, val, i, valid, lane, gather);
     H sum = H(0);
+    if (W == 1 && A.ell_w == 7) {
+        // the 3D finest level: all column and value loads of the row issued up front
+        if (valid) {
+            int cs[7];
+            H as[7];
+#pragma unroll
+            for (int k = 0; k < 7; ++k) {
+                cs[k] = A.ell_col[(size_t)k * A.n + i];
+                as[k] = ellv[(size_t)k * A.n + i];
+            }
+#pragma unroll
+            for (int k = 0; k < 7; ++k) sum += as[k] * gather(cs[k]);
+        }
+        return sum;
+    }
     if (valid)
         for (int k = lane; k < A.ell_w; k += W) {
             const size_t o = (size_t)k * A.n + i;
@@ -1584,6 +1626,14 @@ static int row_group_width(double nnzPerRow) {
     return 32;
 }
 
+// ELL rows of width 7 (3D) and 5 (2D) get kernels with the width fixed at compile time
+#define EW_DISPATCH(EWV, ...)                                               \
+    switch (EWV) {                                                          \
+        case 7: { constexpr int EW = 7; __VA_ARGS__; break; }               \
+        case 5: { constexpr int EW = 5; __VA_ARGS__; break; }               \
+        default: { constexpr int EW = 0; __VA_ARGS__; break; }              \
+    }
+
 #define W_DISPATCH(WV, ...)                                   \
     switch (WV) {                                             \
         case 1: { constexpr int W = 1; __VA_ARGS__; break; }  \
@@ -1851,8 +1901,8 @@ static int cg_grid(const Csr<T> &A, const T *b, T *x, int B, int maxit, const do
         hipLaunchKernelGGL((k_subtract_mean<T>), dim3(Gm, B), dim3(BLOCK), 0, s, bw, n, means, st, 0);
     }
     NB_DISPATCH(NBv,
-        hipLaunchKernelGGL((k_residual<T, NB>), grid, dim3(BLOCK), 0, s, A, bw, x, r, (T *)nullptr, st, B, partial, ctrl,
-                           normScale, 0));
+        EW_DISPATCH(A.ell_w, hipLaunchKernelGGL((k_residual<T, NB, EW>), grid, dim3(BLOCK), 0, s, A, bw, x, r, (T *)nullptr, st, B, partial, ctrl,
+                           normScale, 0)));
     hipLaunchKernelGGL(k_fin_residual, dim3(1), dim3(BLOCK), 0, s, st, B, partial, G, ctrl, normScale, 0, 0, 1);
     PP_LAUNCH_CHECK();
     int parity = 0;
@@ -1860,14 +1910,15 @@ static int cg_grid(const Csr<T> &A, const T *b, T *x, int B, int maxit, const do
         T *pIn = parity ? p1 : p0, *pOut = parity ? p0 : p1;
         NB_DISPATCH(NBv,
             if (resetEvery > 0) {
-                hipLaunchKernelGGL((k_residual<T, NB>), grid, dim3(BLOCK), 0, strm, A, bw, x, r, (T *)nullptr, st, B, partial,
-                                   ctrl, normScale, resetEvery);
+                EW_DISPATCH(A.ell_w, hipLaunchKernelGGL((k_residual<T, NB, EW>), grid, dim3(BLOCK), 0, strm, A, bw, x, r, (T *)nullptr, st, B, partial,
+                                   ctrl, normScale, resetEvery));
                 hipLaunchKernelGGL(k_fin_residual, dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl, normScale,
                                    resetEvery, 0);
             }
             PP_TIMED(PP_K_CG_DIRECTION, n, strm,
-                hipLaunchKernelGGL((k_cg_direction<T, NB, PM_NONE>), grid, dim3(BLOCK), 0, strm, A, r, (const T *)nullptr,
-                                   (const double *)nullptr, pIn, pOut, q, st, B, partial, ctrl));
+                EW_DISPATCH(A.ell_w,
+                    hipLaunchKernelGGL((k_cg_direction<T, NB, PM_NONE, EW>), grid, dim3(BLOCK), 0, strm, A, r,
+                                       (const T *)nullptr, (const double *)nullptr, pIn, pOut, q, st, B, partial, ctrl)));
             hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl);
             PP_TIMED(PP_K_CG_UPDATE, n, strm,
                 hipLaunchKernelGGL((k_cg_update<T, NB, PM_NONE>), grid, dim3(BLOCK), 0, strm, n, A.rpm, r, pOut, q, x, xbest,
@@ -1911,18 +1962,18 @@ static int bicg_grid(const Csr<T> &A, const T *b, T *x, int B, int maxit, const 
     PP_HIP(hipMemsetAsync(p, 0, sizeof(T) * (size_t)n * B, s));
     PP_HIP(hipMemsetAsync(v, 0, sizeof(T) * (size_t)n * B, s));
     NB_DISPATCH(NBv,
-        hipLaunchKernelGGL((k_residual<T, NB>), grid, dim3(BLOCK), 0, s, A, b, x, r, r0, st, B, partial, ctrl, normScale, 0));
+        EW_DISPATCH(A.ell_w, hipLaunchKernelGGL((k_residual<T, NB, EW>), grid, dim3(BLOCK), 0, s, A, b, x, r, r0, st, B, partial, ctrl, normScale, 0)));
     hipLaunchKernelGGL(k_fin_residual, dim3(1), dim3(BLOCK), 0, s, st, B, partial, G, ctrl, normScale, 0, 1, 1);
     PP_LAUNCH_CHECK();
     auto body = [&](hipStream_t strm) {
         PP_TIMED(PP_K_BICG_P, n, strm, hipLaunchKernelGGL((k_bicg_p<T>), gridP, dim3(BLOCK), 0, strm, n, r, p, v, st));
         NB_DISPATCH(NBv,
-            PP_TIMED(PP_K_BICG_V, n, strm,
-                hipLaunchKernelGGL((k_bicg_v<T, NB>), grid, dim3(BLOCK), 0, strm, A, p, v, r0, st, B, partial, ctrl));
+            EW_DISPATCH(A.ell_w, PP_TIMED(PP_K_BICG_V, n, strm,
+                hipLaunchKernelGGL((k_bicg_v<T, NB, EW>), grid, dim3(BLOCK), 0, strm, A, p, v, r0, st, B, partial, ctrl)));
             hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl);
-            PP_TIMED(PP_K_BICG_T, n, strm,
-                hipLaunchKernelGGL((k_bicg_t<T, NB>), grid, dim3(BLOCK), 0, strm, A, r, v, t, st, B, partial, ctrl,
-                                   normScale));
+            EW_DISPATCH(A.ell_w, PP_TIMED(PP_K_BICG_T, n, strm,
+                hipLaunchKernelGGL((k_bicg_t<T, NB, EW>), grid, dim3(BLOCK), 0, strm, A, r, v, t, st, B, partial, ctrl,
+                                   normScale)));
             hipLaunchKernelGGL(k_fin_bicg_t, dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl, normScale);
             PP_TIMED(PP_K_BICG_UPDATE, n, strm,
                 hipLaunchKernelGGL((k_bicg_update<T, NB>), grid, dim3(BLOCK), 0, strm, n, A.rpm, r, r0, p, v, t, x, st, B,
@@ -2010,7 +2061,7 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
         hipLaunchKernelGGL(k_fin_mean, dim3(1), dim3(BLOCK), 0, s, partial, G, n, B, meanR, ctrl, 1);
         hipLaunchKernelGGL((k_subtract_mean<T>), grid, dim3(BLOCK), 0, s, bw, n, meanR, st, 1);
     }
-    hipLaunchKernelGGL((k_residual<T, 1>), grid, dim3(BLOCK), 0, s, Af, bw, x, r, (T *)nullptr, st, B, partial, ctrl, normScale, 0);
+    EW_DISPATCH(Af.ell_w, hipLaunchKernelGGL((k_residual<T, 1, EW>), grid, dim3(BLOCK), 0, s, Af, bw, x, r, (T *)nullptr, st, B, partial, ctrl, normScale, 0));
     hipLaunchKernelGGL(k_fin_residual, dim3(1), dim3(BLOCK), 0, s, st, B, partial, G, ctrl, normScale, 0, 0);
     if (project) {
         // mean of the initial residual for the first projected V-cycle; later ones come with
@@ -2044,7 +2095,8 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
         };
         attach(LA[l], lv[l].a_sell);
         attach(LR[l], lv[l].r_sell);
-        // ELL and sliced ELL copies are read with one lane per row (coalesced across the wavefront)
+        // ELL and sliced ELL copies are read with one lane per row (coalesced across the
+        // wavefront; 2, 4 or 8 lanes per sliced row measured slower on the duct hierarchy)
         wa[l] = (LA[l].ell_w || LA[l].sell_off) ? 1 : row_group_width(double(lv[l].nnz_a) / std::max(1, lv[l].n));
         wr[l] = LR[l].sell_off ? 1 : row_group_width(double(lv[l].nnz_r) / std::max(1, lv[l].n_coarse));
         wp[l] = row_group_width(double(lv[l].nnz_p) / std::max(1, lv[l].n));
@@ -2148,10 +2200,11 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
                                     partial, ctrl));
         hipLaunchKernelGGL(k_fin_cg_dots, dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl);
         PP_TIMED(PP_K_PCG_DIRECTION, n, strm,
-            if (project) hipLaunchKernelGGL((k_cg_direction<T, 1, PM_VECTOR_SHIFTED>), grid, dim3(BLOCK), 0, strm, Af, r, u,
-                                            meanU, pIn, pOut, q, st, B, partial, ctrl);
-            else hipLaunchKernelGGL((k_cg_direction<T, 1, PM_VECTOR>), grid, dim3(BLOCK), 0, strm, Af, r, u,
-                                    (const double *)nullptr, pIn, pOut, q, st, B, partial, ctrl));
+            EW_DISPATCH(Af.ell_w,
+                if (project) hipLaunchKernelGGL((k_cg_direction<T, 1, PM_VECTOR_SHIFTED, EW>), grid, dim3(BLOCK), 0, strm, Af,
+                                                r, u, meanU, pIn, pOut, q, st, B, partial, ctrl);
+                else hipLaunchKernelGGL((k_cg_direction<T, 1, PM_VECTOR, EW>), grid, dim3(BLOCK), 0, strm, Af, r, u,
+                                        (const double *)nullptr, pIn, pOut, q, st, B, partial, ctrl)));
         hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(BLOCK), 0, strm, st, B, partial, G, ctrl);
         PP_TIMED(PP_K_PCG_UPDATE, n, strm,
             hipLaunchKernelGGL((k_cg_update<T, 1, PM_VECTOR>), grid, dim3(BLOCK), 0, strm, n, 1, r, pOut, q, x, xbest, st, B,
