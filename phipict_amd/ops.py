@@ -135,7 +135,7 @@ def ComputeCurrentDensityFaceBased(domain: Domain, epotField: torch.Tensor, uCro
     if dims == 2 and uxb.numel() != n * 3 * B:
         raise ValueError("2D domains need the 3-component u x B field")
     phi = epotField.contiguous()
-    out = torch.zeros(n * 3 * B, dtype=domain.getDtype(), device=domain.getDevice())
+    out = torch.empty(n * 3 * B, dtype=domain.getDtype(), device=domain.getDevice())  # every entry is written
     d = _prep(domain)
     _lib.call("pp_compute_current_density", C.byref(d), _lib.ptr(phi), _lib.ptr(uxb), _lib.ptr(out), _stream(domain))
     return out
@@ -279,7 +279,10 @@ def pack_amg_levels(levels: Sequence[Sequence[torch.Tensor]], hd, extras: Option
     for i, t in enumerate(levels):
         if len(t) != 10:
             raise ValueError("each AMG level needs (A row, col, val, dinv, R row, col, val, P row, col, val)")
-        if t[2].dtype != hd or t[3].dtype != hd or t[6].dtype != hd or t[9].dtype != hd:
+        ex = extras[i] if extras is not None else {}
+        if t[2] is None and (i != 0 or "ell" not in ex):
+            raise ValueError("only the finest level may omit its CSR values, and only with an ELL copy")
+        if (t[2] is not None and t[2].dtype != hd) or t[3].dtype != hd or t[6].dtype != hd or t[9].dtype != hd:
             raise TypeError("all hierarchy values must have the dtype of the coarse pseudo-inverse")
         L = lv[i]
         L.n = t[0].numel() - 1
@@ -287,12 +290,13 @@ def pack_amg_levels(levels: Sequence[Sequence[torch.Tensor]], hd, extras: Option
         L.n_coarse = t[4].numel() - 1
         L.nnz_r = t[5].numel()
         L.nnz_p = t[8].numel()
-        L.a_batched = 1 if t[2].numel() != t[1].numel() else 0
+        L.a_batched = 1 if t[3].numel() != L.n else 0
         for name, tt in zip(("a_row", "a_col", "a_val", "dinv", "r_row", "r_col", "r_val", "p_row", "p_col", "p_val"), t):
+            if tt is None:
+                continue
             tt = tt.contiguous()
             keep.append(tt)
             setattr(L, name, _lib.ptr(tt))
-        ex = extras[i] if extras is not None else {}
         if "ell" in ex:
             col, val, width = ex["ell"]
             if val.dtype != hd or col.numel() != width * L.n or val.numel() % (width * L.n):

@@ -49,8 +49,9 @@ class Level:
     # map from this level's operator values to the next level's (seg, src, w): Galerkin refresh
     gmap: Optional[tuple] = None
     native: dict = field(default_factory=dict)
-    # sliced ELL layouts of the operator ("A") and of R ("R"), per sparsity pattern: SellLayout
-    sell: dict = field(default_factory=dict)
+    # per sparsity pattern, shared by the refreshes of a hierarchy: sliced ELL layouts of the
+    # operator ("A") and of R ("R") (SellLayout), and R / P values per V-cycle dtype ("RP", dtype)
+    shared: dict = field(default_factory=dict)
     native_extra: dict = field(default_factory=dict)
 
 
@@ -141,13 +142,19 @@ class AMGHierarchy:
             nat = lvl.native.get(dtype)
             if nat is None:
                 A = lvl.A
-                aval = A.value if A.value.dtype == dtype else A.value.to(dtype)
+                if lvl is self.levels[0] and A.ell_col is not None and A.ell_width > 0:
+                    aval = None  # the V-cycle reads the finest operator through its ELL copy only
+                else:
+                    aval = A.value if A.value.dtype == dtype else A.value.to(dtype)
                 dinv = torch.empty(A.getRows() * A.getBatchSize(), dtype=dtype, device=A.value.device)
                 d = A.desc()
                 _lib.call("pp_inverse_diagonal", C.byref(d), _lib.dtype_code(A.getDtype()), _lib.ptr(dinv),
                           _lib.dtype_code(dtype), _lib.stream_ptr(A.value.device))
-                nat = (A.row, A.index, aval, dinv, lvl.R_row, lvl.R_col, lvl.R_val.to(dtype), lvl.P_row, lvl.P_col,
-                       lvl.P_val.to(dtype))
+                # R and P never change: their copies in the V-cycle dtype are shared by refreshes
+                rp = lvl.shared.get(("RP", dtype))
+                if rp is None:
+                    rp = lvl.shared[("RP", dtype)] = (lvl.R_val.to(dtype), lvl.P_val.to(dtype))
+                nat = (A.row, A.index, aval, dinv, lvl.R_row, lvl.R_col, rp[0], lvl.P_row, lvl.P_col, rp[1])
                 lvl.native[dtype] = nat
             out.append(nat)
         return out
@@ -170,10 +177,10 @@ class AMGHierarchy:
                               A.getNnz(), A.getBatchSize(), A.ell_width, _lib.dtype_code(A.getDtype()), None,
                               _lib.ptr(ev), _lib.dtype_code(dtype), _lib.stream_ptr(dev))
                     ex["ell"] = (A.ell_col, ev, A.ell_width)
-                layout = lvl.sell.get("A")
+                layout = lvl.shared.get("A")
                 if i > 0 and layout is not None and A.getBatchSize() == 1:
                     ex["a_sell"] = layout.desc(layout.fill(A.value.reshape(-1)[: A.getNnz()], dtype))
-                layout = lvl.sell.get("R")
+                layout = lvl.shared.get("R")
                 if layout is not None:
                     ex["r_sell"] = layout.desc(layout.cached_values(lvl.R_val, dtype))
                 lvl.native_extra[dtype] = ex
@@ -192,7 +199,7 @@ class AMGHierarchy:
                 arr, keep = pcache[dtype]
                 arr = type(arr).from_buffer_copy(arr)
                 nat = self.native_levels(dtype)[0]
-                arr[0].a_val = nat[2].data_ptr()
+                arr[0].a_val = _lib.ptr(nat[2])  # null when the finest level runs on its ELL copy
                 arr[0].dinv = nat[3].data_ptr()
                 ex = self.native_extras(dtype)[0]
                 if "ell" in ex:
@@ -328,9 +335,9 @@ def build_amg_hierarchy(csr_mat: CSRmatrix, *, method: str = "ruge_stuben", max_
             seg, src, w = galerkin_map_host(fine_row, fine_col, lv["P"], lv["R"], host[i + 1]["A"][0], host[i + 1]["A"][1])
             level.gmap = (_dev(seg, device), _dev(src, device), _dev(w, device, torch.float64))
             if rr.size - 1 >= SELL_MIN_ROWS:
-                level.sell["R"] = SellLayout(rr, level.R_row, level.R_col)
+                level.shared["R"] = SellLayout(rr, level.R_row, level.R_col)
         if i > 0 and lv["n"] >= SELL_MIN_ROWS and "P" in lv:
-            level.sell["A"] = SellLayout(lv["A"][0], A.row, A.index)
+            level.shared["A"] = SellLayout(lv["A"][0], A.row, A.index)
         levels.append(level)
     return AMGHierarchy(levels, _dense(levels[-1].A), project_constant, num_pre_smooth, num_post_smooth, native_dtype,
                         time.perf_counter() - t0)
@@ -366,7 +373,7 @@ def galerkin_refresh(hierarchy: AMGHierarchy, csr_mat: CSRmatrix, *, project_con
     A = CSRmatrix(csr_mat.value.reshape(-1)[: csr_mat.getNnz()], csr_mat.index, csr_mat.row)
     A.share_ell_from(csr_mat)
     levels = [Level(n=fine.n, A=A, R_row=fine.R_row, R_col=fine.R_col, R_val=fine.R_val, P_row=fine.P_row,
-                    P_col=fine.P_col, P_val=fine.P_val, sell=fine.sell)]
+                    P_col=fine.P_col, P_val=fine.P_val, shared=fine.shared)]
     levels[0].gmap = fine.gmap
     if coarse:
         for i in range(1, len(hierarchy.levels)):
@@ -387,7 +394,7 @@ def galerkin_refresh(hierarchy: AMGHierarchy, csr_mat: CSRmatrix, *, project_con
                           old.A.getNnz(), _lib.dtype_code(A.getDtype()), stream)
             levels.append(Level(n=old.n, A=CSRmatrix(cval, old.A.index, old.A.row), R_row=old.R_row, R_col=old.R_col,
                                 R_val=old.R_val, P_row=old.P_row, P_col=old.P_col, P_val=old.P_val, gmap=old.gmap,
-                                sell=old.sell))
+                                shared=old.shared))
         coarse_A = _dense(levels[-1].A)
     else:
         for old in hierarchy.levels[1:]:

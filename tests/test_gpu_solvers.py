@@ -399,7 +399,7 @@ def test_amg_pcg_sliced_ell_matches_oracle(monkeypatch, combo, project):
     row, col, val = poisson(96, 80, shift=0.0 if project else 1e-3)
     n = row.size - 1
     A, h, hs = _hier_pair(row, col, val, dt_s, project, dt_h)
-    assert all("R" in lv.sell for lv in h.levels[:-1]) and all("A" in lv.sell for lv in h.levels[1:-1])
+    assert all("R" in lv.shared for lv in h.levels[:-1]) and all("A" in lv.shared for lv in h.levels[1:-1])
     b = np.random.default_rng(12).standard_normal(n)
     if project:
         b -= b.mean()
