@@ -196,6 +196,7 @@ class OracleDomain:
         off = 0
         self.offsets = []
         self.velocity, self.pressure, self.vsource = [], [], []
+        self.face_vel, self.face_tr = {}, {}  # (block, face) -> face velocity / face transforms
         for bi, s in enumerate(specs):
             b = self.blocks[bi]
             for i in range(3):
@@ -259,6 +260,8 @@ class OracleDomain:
                     ob.vel = _p(vel_b)
                     ob.has_transform = 1
                     ob.transform = _p(bt)
+                    self.face_vel[(bi, f)] = vel_b
+                    self.face_tr[(bi, f)] = bt
                     ob.pot_type = fs.pot
                     if fs.pot_types is not None:
                         pt = np.ascontiguousarray(fs.pot_types.reshape(-1), np.int8)
@@ -275,6 +278,32 @@ class OracleDomain:
         rc = lib().o_csr_rows_f64(C.byref(self.dom), _p(self.row))
         assert rc == 0
         self.nnz = int(self.row[-1])
+
+    def face_cells(self, bi: int, f: int) -> int:
+        size = list(self.specs[bi].size_xyz[:self.dims])
+        size[f >> 1] = 1
+        return int(np.prod(size))
+
+    def face_velocity(self, bi: int, f: int) -> np.ndarray:
+        """Velocity of a fixed face: [d] (static) or [d, face cells] (varying)."""
+        v = self.face_vel[(bi, f)]
+        return v if self.blocks[bi].b[f].vel_static else v.reshape(self.dims, -1)
+
+    def set_face_velocity(self, bi: int, f: int, vel: np.ndarray) -> None:
+        """Replace a fixed face's velocity: [d] (static) or [d, face cells] (varying)."""
+        v = np.ascontiguousarray(vel, self.dtype)
+        static = v.ndim == 1
+        if not static and v.reshape(self.dims, -1).shape[1] != self.face_cells(bi, f):
+            raise ValueError("face velocity must have one value per face cell")
+        v = v.reshape(-1)
+        ob = self.blocks[bi].b[f]
+        ob.vel_static = 1 if static else 0
+        ob.vel = _p(v)
+        self.face_vel[(bi, f)] = v
+
+    def face_transform(self, bi: int, f: int) -> np.ndarray:
+        """Face transforms of a fixed face, [face cells, 2d^2+1]."""
+        return self.face_tr[(bi, f)].reshape(-1, 2 * self.dims * self.dims + 1)
 
     def set_dt(self, dt: float) -> None:
         self.dom.dt = float(dt)

@@ -43,6 +43,70 @@ def cross(v: np.ndarray, e: np.ndarray) -> np.ndarray:
     return np.stack([v[1] * e[2] - v[2] * e[1], v[2] * e[0] - v[0] * e[2], v[0] * e[1] - v[1] * e[0]])
 
 
+# ---- boundary driving (piso_simulation.py:308-838), restated on the oracle's face arrays
+def face_flux(od: O.OracleDomain, bi: int, f: int) -> np.ndarray:
+    """Contravariant outward-axis flux det * Minv[axis] . u per face cell of a fixed face
+    (FixedBoundary::GetFluxes, boundaries.cpp; zero for a non-Dirichlet face)."""
+    d = od.dims
+    axis = f >> 1
+    fc = od.face_cells(bi, f)
+    v = od.face_velocity(bi, f)
+    vel = np.broadcast_to(v.reshape(d, 1), (d, fc)) if v.ndim == 1 else v
+    if od.specs[bi].faces[f].neumann:
+        return np.zeros(fc, od.dtype)
+    t = od.face_transform(bi, f)
+    minv = t[:, d * d:2 * d * d].reshape(fc, d, d)[:, axis, :]
+    det = t[:, 2 * d * d]
+    return det * (minv * vel.T).sum(axis=1)
+
+
+def net_fixed_flux(od: O.OracleDomain, faces) -> float:
+    """Net outward flux of (block, face) pairs (get_fixed_boundary_fluxes :344-382)."""
+    total = 0.0
+    for bi, f in faces:
+        fl = float(np.sum(face_flux(od, bi, f), dtype=np.float64))
+        total = total - fl if f % 2 == 0 else total + fl
+    return total
+
+
+def balance_boundary_fluxes(od: O.OracleDomain, free, tol: Optional[float] = None) -> None:
+    """Rescale the free faces' velocities so the net flux vanishes (:533-610)."""
+    fixed_faces = [(bi, f) for bi, s in enumerate(od.specs) for f, fs in enumerate(s.faces)
+                   if fs.kind == "fixed" and (bi, f) not in free]
+    fix = net_fixed_flux(od, fixed_faces)
+    var = net_fixed_flux(od, free)
+    btol = tol if tol is not None else (1e-8 if od.dtype == np.float64 else 1e-5)
+    if abs(fix + var) <= btol * 0.01:
+        return
+    scale = -fix / var
+    for bi, f in free:
+        od.set_face_velocity(bi, f, od.face_velocity(bi, f) * od.dtype.type(scale))
+
+
+def update_advective_boundaries(od: O.OracleDomain, bounds, velm: np.ndarray, dt: float,
+                                tol: Optional[float] = None) -> None:
+    """u_b <- u_b - t (u_b - u_cell), t = 1 - 1/(1 + 2 dt Minv[axis].velm) per face cell, then
+    the flux balance (:611-838, get_advective_velocity :455-530)."""
+    d = od.dims
+    for bi, f in bounds:
+        axis, upper = f >> 1, f & 1
+        v = od.face_velocity(bi, f)
+        if v.ndim == 1:
+            raise ValueError("outflow faces must be varying")
+        t = od.face_transform(bi, f)
+        t_inv_row = t[:, d * d + axis * d: d * d + (axis + 1) * d]  # [fc, d]
+        adv = (t_inv_row * np.asarray(velm, od.dtype)[None, :]).sum(axis=1)  # [fc]
+        alpha = od.dtype.type(dt) * 2 * adv
+        w = 1 - 1 / (1 + alpha)
+        shape = list(reversed(od.specs[bi].size_xyz[:d]))  # (z), y, x
+        cells = od.velocity[bi].reshape([d] + shape)
+        idx = [slice(None)] * (d + 1)
+        idx[1 + d - 1 - axis] = slice(-1, None) if upper else slice(0, 1)
+        layer = cells[tuple(idx)].reshape(d, -1)
+        od.set_face_velocity(bi, f, v - w[None, :] * (v - layer))
+    balance_boundary_fluxes(od, list(bounds), tol)
+
+
 class OracleHierarchyState:
     """AMG hierarchy for the oracle's PCG: RS setup once per pattern, full Galerkin refresh
     for a new fine operator, hierarchy values in ``hdtype`` (amg.py:450-598, 749-847)."""

@@ -15,7 +15,7 @@ from typing import Any, Optional
 import torch
 
 from . import ops
-from .domain import Domain
+from .domain import Domain, transform_size
 from .hooks import Hook, Hooks
 from .solvers import amg as amg_mod
 from .solvers.tolerance import SolverTolerance, default_tolerance, resolve_tolerance
@@ -50,6 +50,141 @@ def check_solver_return_infos(infos, tol, max_iter, return_best_result: bool, ta
             _LOG.warning(msg)
         else:
             raise LinsolveError(msg + f" Solver infos: {[str(i) for i in infos]}")
+
+
+# ---- boundary driving (piso_simulation.py:308-838)
+# The reference computes these with tensor operations on the (small) face tensors; the same is
+# done here, on the device, and without host synchronisation: the balance's "already balanced"
+# test selects a scale of exactly 1 instead of branching on the host.
+
+def _face_slice(block, face: int) -> torch.Tensor:
+    """Block velocity in the cell layer next to ``face`` (piso_simulation.py:735-775)."""
+    dims = block.getSpatialDims()
+    axis, upper = face >> 1, face & 1
+    idx = [slice(None), slice(None)] + [slice(None)] * dims
+    idx[2 + dims - 1 - axis] = slice(-1, None) if upper else slice(0, 1)
+    return block.velocity[tuple(idx)]
+
+
+def _per_env_sum(flux: torch.Tensor) -> torch.Tensor:
+    """Sum over the face cells, one value per environment ([N])."""
+    return flux.reshape(flux.size(0), -1).sum(dim=1)
+
+
+def get_fixed_boundary_flux(bound, bound_idx: int) -> torch.Tensor:
+    """Total boundary-normal flux through a Dirichlet velocity face, per environment
+    (piso_simulation.py:308-341)."""
+    from .domain import BoundaryConditionType, FixedBoundary
+
+    if not isinstance(bound, FixedBoundary):
+        raise TypeError("boundary type not supported.")
+    if bound.velocityType != BoundaryConditionType.DIRICHLET:
+        raise ValueError("Only DIRICHLET boundaries are supported")
+    return _per_env_sum(bound.GetFluxes())
+
+
+def get_fixed_boundary_fluxes(list_idx_bound) -> torch.Tensor:
+    """Net outward flux through (face index, boundary) pairs: [1], or [B] when a boundary has
+    per-environment velocities (piso_simulation.py:344-382)."""
+    if len(list_idx_bound) == 0:
+        return None
+    dom = list_idx_bound[0][1].getParentDomain()
+    total = torch.zeros([1], dtype=dom.getDtype(), device=dom.getDevice())
+    for face, bound in list_idx_bound:
+        flux = get_fixed_boundary_flux(bound, face)
+        total = total - flux if face % 2 == 0 else total + flux
+    return total
+
+
+def balance_boundary_fluxes(domain: Domain, free_bounds, tol=None) -> None:
+    """Rescale the free boundaries so the net boundary flux vanishes, per environment
+    (piso_simulation.py:533-610); a free boundary shared by all environments becomes
+    per-environment when the scales differ."""
+    pairs = []
+    for block in domain.getBlocks():
+        pairs.extend(block.getFixedBoundaries())
+    fixed = [pb for pb in pairs if not any(pb[1] is fb for fb in free_bounds)]
+    free = [pb for pb in pairs if any(pb[1] is fb for fb in free_bounds)]
+    if not free:
+        return
+    var_flux = get_fixed_boundary_fluxes(free)
+    fix_flux = get_fixed_boundary_fluxes(fixed)
+    if fix_flux is None:
+        fix_flux = torch.zeros_like(var_flux)
+    # an explicit tolerance passes through, otherwise the dtype default (piso_diff.py:397-421)
+    balance_tol = tol if tol is not None else default_tolerance(domain.getDtype())
+    # torch.allclose(fixed + variable, 0, atol=0.01 tol) decides on the host in the reference;
+    # a scale of exactly 1 where balanced gives the same velocities without a synchronisation
+    balanced = (fix_flux + var_flux).abs() <= float(balance_tol) * 0.01
+    scale = torch.where(balanced.all(), torch.ones_like(var_flux), -fix_flux / var_flux)
+    for face, bound in free:
+        v = bound.velocity
+        env_scale = scale.reshape([-1] + [1] * (v.dim() - 1)) if scale.numel() > 1 else scale
+        bound.setVelocity(v * env_scale)
+
+
+def get_advective_velocity(velms, velm_idx: int, bound, bound_idx: int) -> torch.Tensor:
+    """Contravariant boundary-normal advection velocity of an outflow face, N1(D)HW
+    (piso_simulation.py:455-530): the face's Minv row applied to ``velm``."""
+    velm = velms if isinstance(velms, torch.Tensor) else velms[velm_idx]
+    dom = bound.getParentDomain()
+    dims = dom.getSpatialDims()
+    axis = bound_idx >> 1
+    shape = bound.face_shape()
+    static = velm.dim() == 2
+    velm = velm.to(device=dom.getDevice(), dtype=dom.getDtype())
+    if not static and list(velm.shape[2:]) != shape:
+        raise ValueError("varying velm size must match boundary")
+    if bound.transform is not None:
+        n_env = velm.size(0)
+        if static:
+            velm = velm.reshape([n_env, dims] + [1] * dims).expand(n_env, dims, *shape)
+        t = bound.transform.reshape(-1, transform_size(dims))
+        t_inv_row = t[:, dims * dims + axis * dims: dims * dims + (axis + 1) * dims]  # [fc, d]
+        v = velm.reshape(n_env, dims, -1)  # [N, d, fc]
+        adv = (t_inv_row.t()[None] * v).sum(dim=1)
+        return adv.reshape([n_env, 1] + shape)
+    if static:
+        return velm[:, axis:axis + 1].reshape([velm.size(0), 1] + [1] * dims)
+    return velm[:, axis:axis + 1]
+
+
+def update_advective_boundaries(domain: Domain, bounds, velms, dt, tol=None) -> None:
+    """Advect the velocity of outflow faces with ``velms`` over ``dt`` and balance the boundary
+    fluxes afterwards (piso_simulation.py:611-838):
+    ``u_b <- u_b - t (u_b - u_cell)``, ``t = 1 - 1/(1 + 2 dt v_n)``."""
+    from .domain import FixedBoundary
+
+    if not isinstance(bounds, list) or not bounds:
+        raise ValueError("bounds must be a non-empty list of boundaries")
+    dims, B = domain.getSpatialDims(), domain.getBatchSize()
+    if isinstance(velms, torch.Tensor):
+        if velms.dim() != 2 or velms.size(0) not in (1, B) or velms.size(1) != dims:
+            raise ValueError("velms tensors have wrong shape")
+    elif not (isinstance(velms, list) and len(velms) == len(bounds)):
+        raise ValueError("velms tensors have wrong shape")
+    if isinstance(dt, torch.Tensor) and dt.numel() > 1:
+        dt = dt.to(device=domain.getDevice(), dtype=domain.getDtype()).view(-1, *([1] * (dims + 1)))
+    found = []
+    for block in domain.getBlocks():
+        for face in range(2 * dims):
+            bound = block.getBoundary(face)
+            if isinstance(bound, FixedBoundary) and any(bound is b for b in bounds):
+                found.append((block, face, bound))
+    if len(found) != len(bounds):
+        raise ValueError("Bounds passed as advective outflow are not (all) part of the domain.")
+    for block, face, bound in found:
+        if getattr(block, "passiveScalar", None) is not None:
+            raise NotImplementedError("passive scalars are not implemented")
+        if bound.isVelocityStatic():
+            raise ValueError("outflow boundaries must be varying. Use FixedBoundary.makeVelocityVarying() to make "
+                             "the velocity varying.")
+        idx = next(i for i, b in enumerate(bounds) if b is bound)
+        alpha = dt * 2 * get_advective_velocity(velms, idx, bound, face)
+        t = 1 - 1 / (1 + alpha)
+        v = bound.velocity
+        bound.setVelocity(v - t * (v - _face_slice(block, face)))
+    balance_boundary_fluxes(domain, bounds, tol=tol)
 
 
 class Simulation:

@@ -111,6 +111,25 @@ def inflow2d(nx: int = 20, ny: int = 10, stretch: float = 1.5, seed: int = 2, U:
     return [BlockSpec(v, faces, velocity=vel, pressure=np.zeros(cz))]
 
 
+def outflow2d(nx: int = 24, ny: int = 12, stretch: float = 1.5, seed: int = 5, U: float = 1.0) -> list:
+    """A 2D channel with a parabolic Dirichlet inflow at -x, no-slip walls in y and an outflow
+    at +x whose per-face-cell velocity (initially the plug U) is advected out of the domain
+    and rescaled to a zero net flux every step (update_advective_boundaries)."""
+    rng = np.random.default_rng(seed)
+    v = vertices([np.linspace(0.0, 4.0, nx + 1), tanh_coords(ny, -1.0, 1.0, stretch)])
+    cz = cell_shape(v)
+    yc = 0.5 * (v[1, 1:, 0] + v[1, :-1, 0])
+    prof = np.zeros([2] + face_shape(cz, 0))
+    prof[0, :, 0] = 1.5 * U * (1 - yc ** 2)
+    out = np.zeros([2] + face_shape(cz, 1))
+    out[0] = U
+    faces = [FaceSpec("fixed", velocity=prof, pot=POT_OPEN), FaceSpec("fixed", velocity=out, pot=POT_OPEN),
+             FaceSpec("fixed"), FaceSpec("fixed")]
+    vel = _noise(rng, [2] + cz, 1e-3)
+    vel[0] += U
+    return [BlockSpec(v, faces, velocity=vel, pressure=np.zeros(cz))]
+
+
 def split_channel2d(nx: int = 16, ny: int = 12, split: int = 6, seed: int = 0, stretch: float = 1.5,
                     gradient: float = 1.0) -> tuple:
     """The channel of ``channel2d`` as one block and as two blocks joined along x (the second

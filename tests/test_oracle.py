@@ -494,3 +494,27 @@ def test_oracle_hartmann_profile():
     exact = G / N * (1 - np.cosh(Ha * y) / math.cosh(Ha))
     err = np.abs(u - exact).max() / np.abs(exact).max()
     assert err < 5e-3, err
+
+
+def test_advective_outflow_is_implicit_euler():
+    """The outflow update is the implicit-Euler step of du/dt + U du/dn = 0 over the half cell
+    between the last cell centre and the face: u' = (u + a c) / (1 + a), a = 2 U dt / dx; and
+    the flux balance leaves a zero net boundary flux (and an already balanced domain unchanged)."""
+    from oracle.piso_oracle import balance_boundary_fluxes, net_fixed_flux, update_advective_boundaries
+
+    nx, ny, U, dt = 8, 6, 0.7, 3e-2
+    specs = cases.outflow2d(nx=nx, ny=ny)
+    od = O.OracleDomain(specs, 0.02, dt)
+    rng = np.random.default_rng(4)
+    od.velocity[0][...] = rng.standard_normal(od.velocity[0].shape)
+    v0 = rng.standard_normal((2, ny))
+    od.set_face_velocity(0, 1, v0)
+    layer = od.velocity[0].reshape(2, ny, nx)[:, :, -1]
+    update_advective_boundaries(od, [(0, 1)], np.array([U, 0.0]), dt, tol=1e30)  # balance off
+    a = 2 * U * dt / (4.0 / nx)
+    np.testing.assert_allclose(od.face_velocity(0, 1), (v0 + a * layer) / (1 + a), rtol=1e-13, atol=1e-14)
+    balance_boundary_fluxes(od, [(0, 1)])
+    assert abs(net_fixed_flux(od, [(0, 0), (0, 1), (0, 2), (0, 3)])) < 1e-13
+    before = od.face_velocity(0, 1).copy()
+    balance_boundary_fluxes(od, [(0, 1)])
+    np.testing.assert_array_equal(od.face_velocity(0, 1), before)
