@@ -212,6 +212,11 @@ int pp_coords_to_transforms(int32_t dims, const int32_t *cells, const void *coor
                             int32_t dtype, void *stream);
 int pp_coords_to_face_transforms(int32_t dims, const int32_t *cells, const void *coords, void *out,
                                  int32_t dtype, void *stream);
+/* replaces TransformVectors (csrc/phipict/grid/transform_vectors.cu:84-136): vectors
+ * [batch][dims][cells], transforms [1|batch][cells][2d^2+1] -> out[n][i][c] =
+ * sum_k T[i][k] vectors[n][k][c] with T = Minv of cell c (inverse) or M. */
+int pp_transform_vectors(int32_t dims, int32_t batch, int32_t cells, const void *vectors, const void *transforms,
+                         int32_t batched_transforms, int32_t inverse, void *out, int32_t dtype, void *stream);
 
 /* ---- PISO assembly (non-orthogonal correction flags accepted; the corner terms are
  * zero on the rectilinear grids these entry points are validated on) ---- */
@@ -260,6 +265,15 @@ int pp_compute_current_density(const pp_domain *dom, const void *phi, const void
 /* replaces Copy{Velocity,Pressure,Epot}Result{To,From}Blocks (csrc/phipict/piso/result_copy.cu:19-475).
  * field: 0 velocity, 1 pressure, 2 epot; to_blocks: 1 flat->blocks, 0 blocks->flat. */
 int pp_copy_result(const pp_domain *dom, int32_t field, int32_t to_blocks, void *stream);
+/* replaces Domain::getMaxVelocity / getMaxVelocityPerEnv / getMaxVelocityMagnitude
+ * (csrc/phipict/domain/domain.cpp:212-245) and the Block methods of the same names
+ * (csrc/phipict/domain/block.cpp:505-600): block -1 = every block, else that block only;
+ * with_bounds adds the face cells of the fixed faces; computational measures Minv u (cells
+ * of blocks with transforms, faces with face transforms); norm 0 = largest absolute
+ * component, 1 = Euclidean magnitude. out[batch + 1] (dtype): out[b] for environment b,
+ * out[batch] = the maximum over all environments; a NaN velocity gives NaN. */
+int pp_max_velocity(const pp_domain *dom, int32_t block, int32_t with_bounds, int32_t computational, int32_t norm,
+                    void *out, void *stream);
 
 /* ---- linear solvers ---- */
 

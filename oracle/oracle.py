@@ -197,6 +197,7 @@ class OracleDomain:
         self.offsets = []
         self.velocity, self.pressure, self.vsource = [], [], []
         self.face_vel, self.face_tr = {}, {}  # (block, face) -> face velocity / face transforms
+        self.cell_tr = []  # per block: cell transforms [cells][2d^2+1]
         for bi, s in enumerate(specs):
             b = self.blocks[bi]
             for i in range(3):
@@ -207,6 +208,7 @@ class OracleDomain:
             T = transforms(s.vertices, self.dtype)
             FT = face_transforms(s.vertices, self.dtype)
             self.keep += [T, FT]
+            self.cell_tr.append(T)
             b.has_transform = 1
             b.transform = _p(T)
             vel = np.zeros((self.dims, s.ncells), self.dtype) if s.velocity is None else \
@@ -304,6 +306,40 @@ class OracleDomain:
     def face_transform(self, bi: int, f: int) -> np.ndarray:
         """Face transforms of a fixed face, [face cells, 2d^2+1]."""
         return self.face_tr[(bi, f)].reshape(-1, 2 * self.dims * self.dims + 1)
+
+    def max_velocity(self, with_bounds: bool = True, computational: bool = False, magnitude: bool = False,
+                     block: Optional[int] = None) -> float:
+        """Largest absolute velocity component (or magnitude) over the blocks' cells and, with
+        bounds, their fixed faces' face cells; computational: Minv u with the cell transforms
+        and the face transforms (Block::getMaxVelocity[Magnitude], block.cpp:267-273, 505-600;
+        FixedBoundary::getVelocity, boundaries.cpp:302-318; Domain::getMaxVelocity,
+        domain.cpp:212-243). A NaN velocity gives NaN."""
+        d = self.dims
+        ts = 2 * d * d + 1
+
+        def measure(v, tr):
+            if computational:
+                minv = tr.reshape(-1, ts)[:, d * d:2 * d * d].reshape(-1, d, d)
+                v = np.einsum("cik,kc->ic", minv, v)
+            m = np.sqrt((v * v).sum(axis=0)) if magnitude else np.abs(v)
+            return float(np.max(m)) if m.size else 0.0
+
+        best = 0.0
+        for bi, s in enumerate(self.specs):
+            if block is not None and bi != block:
+                continue
+            vals = [measure(self.velocity[bi], self.cell_tr[bi])]
+            if with_bounds:
+                for f in range(2 * d):
+                    if (bi, f) not in self.face_vel:
+                        continue
+                    v = self.face_velocity(bi, f)
+                    fc = self.face_cells(bi, f)
+                    v = np.repeat(v.reshape(d, 1), fc, axis=1) if v.ndim == 1 else v
+                    vals.append(measure(v, self.face_tr[(bi, f)]))
+            for x in vals:
+                best = x if (x > best or np.isnan(x)) and not np.isnan(best) else best
+        return best
 
     def set_dt(self, dt: float) -> None:
         self.dom.dt = float(dt)

@@ -4,7 +4,7 @@ The PISO (+ inductionless MHD) step of the reference, driven on the CPU with the
 kernels and solvers: the step order of ``_PISO_split_step`` (src/phipict/core/piso_simulation.py:
 3525-4169, orthogonal and non-orthogonal branches), the magnetic step and Lorentz force of
 ``MHDSimulation`` (src/phipict/simulation/mhd.py:309-355, 543-594, 877-1002). One environment,
-absolute tolerances, fixed substeps.
+absolute tolerances, fixed or adaptive CFL substeps.
 """
 from __future__ import annotations
 
@@ -105,6 +105,19 @@ def update_advective_boundaries(od: O.OracleDomain, bounds, velm: np.ndarray, dt
         layer = cells[tuple(idx)].reshape(d, -1)
         od.set_face_velocity(bi, f, v - w[None, :] * (v - layer))
     balance_boundary_fluxes(od, list(bounds), tol)
+
+
+def cfl_substep(remaining: float, max_vel: float, cfl: float) -> tuple:
+    """(next substep size, substeps of the remaining time) at the maximum velocity
+    (_cfl_substep, src/phipict/core/piso_simulation.py:186-215)."""
+    if np.isclose(max_vel, 0):
+        max_ts = remaining
+    else:
+        max_ts = cfl / max_vel
+    if max_ts >= remaining:
+        return remaining, 1
+    n = int(np.ceil(remaining / max_ts))
+    return remaining / n, n
 
 
 class OracleHierarchyState:
@@ -324,6 +337,22 @@ class OraclePISO:
             # the Lorentz force stays in the velocity source, as in the reference: a caller that
             # drives the flow with a fixed source resets it in ``pre_velocity_setup``
             self._magnetic_step()
+
+    def adaptive_step(self, time_step: float, cfl: float = 0.8) -> list:
+        """One step of ``time_step`` in adaptive CFL substeps, each sized from the maximum
+        computational velocity (cells and fixed faces) at its start
+        (_PISO_adaptive_step, piso_simulation.py:4171-4240). Returns the substep sizes."""
+        remaining = time_step
+        sizes = []
+        while remaining > 0 and not np.isclose(remaining, 0):
+            vmax = np.asarray(self.od.max_velocity(True, True), self.od.dtype)
+            ts, _ = cfl_substep(remaining, vmax, cfl)
+            remaining -= ts
+            self.od.set_dt(float(ts))
+            self.dt = float(ts)
+            self.step()
+            sizes.append(float(ts))
+        return sizes
 
     def _p_singular(self, pval):
         """Pure-Neumann pressure operator: every row sum vanishes to within the rounding of the

@@ -169,6 +169,8 @@ SIGNATURES = {
     "pp_compute_epot_rhs": [_P(DomainDesc), _vp, _vp, _vp],
     "pp_compute_current_density": [_P(DomainDesc), _vp, _vp, _vp, _vp],
     "pp_copy_result": [_P(DomainDesc), _i32, _i32, _vp],
+    "pp_max_velocity": [_P(DomainDesc), _i32, _i32, _i32, _i32, _vp, _vp],
+    "pp_transform_vectors": [_i32, _i32, _i32, _vp, _vp, _i32, _i32, _vp, _i32, _vp],
     "pp_solve_linear": [_P(CsrDesc), _vp, _vp, _i32, _i32, _i32, _P(_dbl), _i32, _i32, _i32, _i32, _i32, _i32, _i32,
                         _vp, _P(SolveResult)],
     "pp_amg_pcg_solve": [_P(CsrDesc), _P(AmgLevel), _i32, _vp, _i32, _i32, _i32, _vp, _vp, _i32, _i32, _i32, _dbl,

@@ -811,6 +811,151 @@ __global__ void __launch_bounds__(256) k_transpose_2d(const T *__restrict__ in, 
     }
 }
 
+// ---- maximum velocity (CFL substeps) and vector transforms
+
+// Unsigned integers that order non-negative values (and one canonical NaN above +inf) like
+// the values themselves: the running maxima are combined with integer atomic max.
+__device__ __forceinline__ unsigned long long order_bits(double v) {
+    return v != v ? 0x7ff8000000000000ull : static_cast<unsigned long long>(__double_as_longlong(v));
+}
+__device__ __forceinline__ unsigned int order_bits(float v) { return v != v ? 0x7fc00000u : __float_as_uint(v); }
+
+__device__ __forceinline__ void fold_max(double *p, double v) {
+    atomicMax(reinterpret_cast<unsigned long long *>(p), order_bits(v));
+}
+__device__ __forceinline__ void fold_max(float *p, float v) { atomicMax(reinterpret_cast<unsigned int *>(p), order_bits(v)); }
+
+// larger of a and b; a NaN on either side wins, as torch::max propagates NaN
+template <typename T>
+__device__ __forceinline__ T nan_max(T a, T b) { return (b > a || b != b) ? b : a; }
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill(T *__restrict__ p, int n, T v) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = v;
+}
+
+// v <- Minv v with the strided transform t (TransformVectors with inverse = true,
+// transform_vectors.cu:43-72: rows of Minv at D*D + row*D)
+template <typename T, int D>
+__device__ __forceinline__ void to_computational(const TView<T> &t, T *v) {
+    T r[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        T s = 0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) s += t[D * D + i * D + k] * v[k];
+        r[i] = s;
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) v[i] = r[i];
+}
+
+template <typename T, int D>
+__device__ __forceinline__ T velocity_measure(const T *v, int norm) {
+    if (norm) {
+        T s = 0;
+#pragma unroll
+        for (int i = 0; i < D; ++i) s += v[i] * v[i];
+        return dsqrt<T>(s);
+    }
+    T m = v[0] < T(0) ? -v[0] : v[0];
+#pragma unroll
+    for (int i = 1; i < D; ++i) m = nan_max<T>(m, v[i] < T(0) ? -v[i] : v[i]);
+    return m;
+}
+
+// Block::getMaxVelocity / getMaxVelocityPerEnv / getMaxVelocityMagnitude, block.cpp:505-600:
+// max over the block's cells and, with bounds, the face cells of its fixed faces of the
+// largest absolute component (norm 0) or magnitude (norm 1) of the velocity, optionally in
+// computational space (Minv u: block.cpp:267-273, boundaries.cpp:302-318). A block's
+// workgroups sweep its cells followed by its fixed faces' face cells; every workgroup folds
+// its maximum into out[env] and out[batch] (the maximum over environments) with one atomic
+// max each. out must hold zeros (k_fill) beforehand.
+template <typename T, int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_max_velocity(pp_domain dom, int firstBlock, int withBounds,
+                                                            int computational, int norm, T *__restrict__ out) {
+    const int wpb = (dom.max_block_cells + ASM_BLOCK - 1) / ASM_BLOCK;
+    const int bi = firstBlock + __builtin_amdgcn_readfirstlane(blockIdx.x / wpb);
+    const int tile = blockIdx.x - (bi - firstBlock) * wpb;
+    const pp_block_desc &b = dom.blocks[bi];
+    const int env = blockIdx.y;
+    const int cells = b.cells;
+    int faceStart[2 * D + 1];
+    int units = cells;
+#pragma unroll
+    for (int f = 0; f < 2 * D; ++f) {
+        faceStart[f] = units;
+        if (withBounds && b.bnd[f].type == PP_BOUND_FIXED) units += cells / b.size[f >> 1];
+    }
+    faceStart[2 * D] = units;
+    const T *vel = reinterpret_cast<const T *>(b.velocity) + (size_t)env * D * cells;
+    const bool cellTransform = computational && b.has_transform;
+    T m = 0;
+    for (int u = tile * ASM_BLOCK + threadIdx.x; u < units; u += wpb * ASM_BLOCK) {
+        T v[D];
+        if (u < cells) {
+#pragma unroll
+            for (int c = 0; c < D; ++c) v[c] = vel[(size_t)c * cells + u];
+            if (cellTransform) to_computational<T, D>(Metric<T, D>::at(b, u), v);
+        } else {
+            int f = 0;
+#pragma unroll
+            for (int k = 1; k < 2 * D; ++k)
+                if (u >= faceStart[k]) f = k;
+            const pp_boundary_desc &fb = b.bnd[f];
+            const int ff = u - faceStart[f];
+            const int fc = faceStart[f + 1] - faceStart[f];
+            const int vb = fb.vel_batched ? env : 0;
+            const T *fv = reinterpret_cast<const T *>(fb.vel);
+#pragma unroll
+            for (int c = 0; c < D; ++c)
+                v[c] = fb.vel_static ? fv[(size_t)vb * D + c] : fv[((size_t)vb * D + c) * fc + ff];
+            if (computational && fb.has_transform) to_computational<T, D>(Metric<T, D>::face(fb, ff), v);
+        }
+        m = nan_max<T>(m, velocity_measure<T, D>(v, norm));
+    }
+    // wavefront, then workgroup maximum
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = nan_max<T>(m, __shfl_xor(m, o, 64));
+    __shared__ T wmax[ASM_BLOCK / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) wmax[wave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T r = wmax[0];
+#pragma unroll
+        for (int w = 1; w < ASM_BLOCK / 64; ++w) r = nan_max<T>(r, wmax[w]);
+        fold_max(out + env, r);
+        fold_max(out + dom.batch, r);
+    }
+}
+
+// TransformVectors, transform_vectors.cu:43-136: out[n][i][c] = sum_k T[i][k] v[n][k][c] with
+// T = Minv (inverse) or M of cell c; vectors [N][D][cells], transforms [1|N][cells][2D^2+1]
+template <typename T, int D>
+__global__ void __launch_bounds__(256) k_transform_vectors(const T *__restrict__ v, const T *__restrict__ tr,
+                                                           int cells, int batchedTransforms, int inverse,
+                                                           T *__restrict__ out) {
+    const int n = blockIdx.y;
+    const T *vn = v + (size_t)n * D * cells;
+    T *on = out + (size_t)n * D * cells;
+    const T *tn = tr + (batchedTransforms ? (size_t)n * cells * (2 * D * D + 1) : 0);
+    const int off = inverse ? D * D : 0;
+    for (int c = blockIdx.x * 256 + threadIdx.x; c < cells; c += gridDim.x * 256) {
+        const T *t = tn + (size_t)c * (2 * D * D + 1) + off;
+        T a[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) a[k] = vn[(size_t)k * cells + c];
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            T s = 0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) s += t[i * D + k] * a[k];
+            on[(size_t)i * cells + c] = s;
+        }
+    }
+}
+
 static int check_domain(const pp_domain *dom, const char *fn) {
     if (!dom || !dom->blocks || dom->num_blocks <= 0 || dom->num_cells <= 0 || dom->batch <= 0 ||
         dom->max_block_cells <= 0 ||
@@ -967,6 +1112,41 @@ int pp_copy_result(const pp_domain *dom, int32_t field, int32_t to_blocks, void 
     PP_DISPATCH(dom->dtype,
         PP_TIMED(PP_K_COPY_RESULT, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_copy_result<T>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom, (int)field, ch,
                            (int)to_blocks, reinterpret_cast<const T *>(flat), reinterpret_cast<T *>(flat))));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_max_velocity(const pp_domain *dom, int32_t block, int32_t with_bounds, int32_t computational, int32_t norm,
+                    void *out, void *stream) {
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(out, "out must not be null");
+    PP_CHECK_ARG(block >= -1 && block < dom->num_blocks, "block index out of range");
+    PP_CHECK_ARG(norm == 0 || norm == 1, "norm must be 0 or 1");
+    const int first = block < 0 ? 0 : block;
+    const int nblocks = block < 0 ? dom->num_blocks : 1;
+    const int wpb = (dom->max_block_cells + ASM_BLOCK - 1) / ASM_BLOCK;
+    PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
+        hipLaunchKernelGGL((k_fill<T>), dim3(1), dim3(256), 0, as_stream(stream), static_cast<T *>(out),
+                           dom->batch + 1, T(0));
+        hipLaunchKernelGGL((k_max_velocity<T, D>), dim3(wpb * nblocks, dom->batch), dim3(ASM_BLOCK), 0,
+                           as_stream(stream), *dom, first, (int)(with_bounds != 0), (int)(computational != 0),
+                           (int)norm, static_cast<T *>(out))));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_transform_vectors(int32_t dims, int32_t batch, int32_t cells, const void *vectors, const void *transforms,
+                         int32_t batched_transforms, int32_t inverse, void *out, int32_t dtype, void *stream) {
+    PP_CHECK_ARG(dims == 2 || dims == 3, "dims must be 2 or 3");
+    PP_CHECK_ARG(batch >= 1 && batch <= 65535 && cells >= 0, "bad sizes");
+    PP_CHECK_ARG(vectors && transforms && out, "null argument");
+    PP_CHECK_ARG(out != vectors, "out must not alias the vectors");
+    if (cells == 0) return PP_OK;
+    const dim3 g(std::min((cells + 255) / 256, 4096), batch);
+    PP_DISPATCH(dtype, DIMS_DISPATCH(dims,
+        hipLaunchKernelGGL((k_transform_vectors<T, D>), g, dim3(256), 0, as_stream(stream),
+                           static_cast<const T *>(vectors), static_cast<const T *>(transforms), (int)cells,
+                           (int)(batched_transforms != 0), (int)(inverse != 0), static_cast<T *>(out))));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }

@@ -75,6 +75,36 @@ def transform_size(dims: int) -> int:
     return 2 * dims * dims + 1
 
 
+def transform_vectors(vectors: torch.Tensor, transforms: torch.Tensor, inverse: bool) -> torch.Tensor:
+    """Vectors ``[N, d, (z), y, x]`` mapped by the cell transforms ``[1|N, (z), y, x, 2d^2+1]``:
+    ``Minv v`` (inverse, physical to computational) or ``M v`` (TransformVectors,
+    csrc/phipict/grid/transform_vectors.cu:84-136)."""
+    if not 2 < vectors.dim() < 6 or vectors.size(0) < 1:
+        raise ValueError("vectors must have batch and channel dimension and be 1-3D.")
+    dims = vectors.dim() - 2
+    if vectors.size(1) != dims:
+        raise ValueError("vectors channel dimension must match spatial dimensionality.")
+    if transforms.dim() != vectors.dim():
+        raise ValueError("dimensionality of transforms must match vectors.")
+    if transforms.size(0) not in (1, vectors.size(0)):
+        raise ValueError("transforms batch dimension must be 1 or match the vectors.")
+    if transforms.size(-1) != transform_size(dims):
+        raise ValueError("transforms channels must match spatial dimensions.")
+    if list(transforms.shape[1:-1]) != list(vectors.shape[2:]):
+        raise ValueError("spatial dimensions of transforms must match vectors.")
+    if transforms.dtype != vectors.dtype or transforms.device != vectors.device:
+        raise ValueError("vectors and transforms must share dtype and device.")
+    v = vectors.contiguous()
+    t = transforms.contiguous()
+    out = torch.empty_like(v)
+    cells = 1
+    for n in vectors.shape[2:]:
+        cells *= n
+    _lib.call("pp_transform_vectors", dims, v.size(0), cells, _p(v), _p(t), int(t.size(0) > 1), int(bool(inverse)),
+              _p(out), _lib.dtype_code(v.dtype), _lib.stream_ptr(v.device))
+    return out
+
+
 # ------------------------------------------------------------------------------------------
 class CSRmatrix:
     """CSR matrix with int32 ``row``/``index`` and one or more value arrays of one pattern
@@ -255,7 +285,18 @@ class FixedBoundary(Boundary):
             v = self.velocity.reshape(*self.velocity.shape, *([1] * (len(shape) - 2))).expand(*shape)
             self.setVelocity(v.contiguous())
 
+    def getVelocityVarying(self) -> torch.Tensor:
+        """The velocity as ``[vb, d, (z), y, x]``; a static velocity expanded over the face."""
+        if not self.isVelocityStatic():
+            return self.velocity
+        shape = [self.velocity.size(0), self.velocity.size(1)] + self.face_shape()
+        return self.velocity.reshape(*self.velocity.shape, *([1] * (len(shape) - 2))).expand(*shape).contiguous()
+
     def getVelocity(self, computational: bool = False) -> torch.Tensor:
+        """The velocity; with ``computational`` and a face transform, ``Minv u`` per face cell
+        (FixedBoundary::getVelocity, boundaries.cpp:302-318)."""
+        if computational and self.hasTransform():
+            return transform_vectors(self.getVelocityVarying(), self.transform, True)
         return self.velocity
 
     def face_shape(self) -> list:
@@ -530,6 +571,30 @@ class Block:
 
     def hasTransform(self) -> bool:
         return self.transform is not None
+
+    def getVelocity(self, computational: bool = False) -> torch.Tensor:
+        """The velocity; with ``computational``, ``Minv u`` per cell (Block::getVelocity,
+        block.cpp:267-273)."""
+        if computational:
+            if not self.hasTransform():
+                raise RuntimeError("Coordinates or Transformations are required to compute computational velocities.")
+            return transform_vectors(self.velocity, self.transform, True)
+        return self.velocity
+
+    def getMaxVelocity(self, withBounds: bool = True, computational: bool = False) -> torch.Tensor:
+        """Largest absolute velocity component over the block (and its fixed faces), a 0-d
+        tensor (Block::getMaxVelocity, block.cpp:505-535)."""
+        return self._domain._max_velocity(self, withBounds, computational, 0)[-1]
+
+    def getMaxVelocityPerEnv(self, withBounds: bool = True, computational: bool = False) -> torch.Tensor:
+        """Largest absolute velocity component of every environment, ``[B]``
+        (Block::getMaxVelocityPerEnv, block.cpp:537-568)."""
+        return self._domain._max_velocity(self, withBounds, computational, 0)[:-1]
+
+    def getMaxVelocityMagnitude(self, withBounds: bool = True, computational: bool = False) -> torch.Tensor:
+        """Largest velocity magnitude over the block (and its fixed faces), a 0-d tensor
+        (Block::getMaxVelocityMagnitude, block.cpp:575-600)."""
+        return self._domain._max_velocity(self, withBounds, computational, 1)[-1]
 
     def _transform_component_major(self) -> Optional[torch.Tensor]:
         """The cell transforms as [2d^2+1][cells] for the library's descriptors; rebuilt when
@@ -1115,6 +1180,37 @@ class Domain:
                 or (self.Epot is not None and d.E.val != self.Epot.value.data_ptr())):
             self.UpdateDomainData()
         return self._desc
+
+    # --- maximum velocity (CFL substeps)
+    def _max_velocity(self, block: Optional[Block], withBounds: bool, computational: bool, norm: int) -> torch.Tensor:
+        """``[B + 1]`` maxima from one reduction launch: every environment, then all of them."""
+        if not self._blocks:
+            raise RuntimeError("Domain does not contain any blocks.")
+        if not self._initialized:
+            raise RuntimeError("Domain is not initialized. Run domain.PrepareSolve() first.")
+        blocks = self._blocks if block is None else [block]
+        if computational and not all(b.hasTransform() for b in blocks):
+            raise RuntimeError("Coordinates or Transformations are required to compute computational velocities.")
+        idx = -1 if block is None else self._blocks.index(block)
+        out = torch.empty(self._batch + 1, dtype=self._dtype, device=self._device)
+        _lib.call("pp_max_velocity", C.byref(self.desc()), idx, int(bool(withBounds)), int(bool(computational)), int(norm),
+                  _p(out), _lib.stream_ptr(self._device))
+        return out
+
+    def getMaxVelocity(self, withBounds: bool = True, computational: bool = False) -> torch.Tensor:
+        """Largest absolute velocity component over all blocks (and fixed faces) and
+        environments, a 0-d tensor (Domain::getMaxVelocity, domain.cpp:212-221)."""
+        return self._max_velocity(None, withBounds, computational, 0)[-1]
+
+    def getMaxVelocityPerEnv(self, withBounds: bool = True, computational: bool = False) -> torch.Tensor:
+        """Largest absolute velocity component of every environment, ``[B]``
+        (Domain::getMaxVelocityPerEnv, domain.cpp:223-232)."""
+        return self._max_velocity(None, withBounds, computational, 0)[:-1]
+
+    def getMaxVelocityMagnitude(self, withBounds: bool = True, computational: bool = False) -> torch.Tensor:
+        """Largest velocity magnitude over all blocks (and fixed faces), a 0-d tensor
+        (Domain::getMaxVelocityMagnitude, domain.cpp:234-243)."""
+        return self._max_velocity(None, withBounds, computational, 1)[-1]
 
     # --- diagnostics
     def GetBoundaryFluxBalance(self) -> torch.Tensor:

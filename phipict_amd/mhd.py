@@ -157,6 +157,11 @@ class MHDSimulation(Simulation):
         self._epot_amg_matrix = mat
         return self._epot_amg_hierarchy
 
+    def _env_state_extras(self) -> list:
+        """Adds ``u x e_b`` of the last magnetic step, which the next substep's Lorentz force
+        uses (mhd.py:864-875)."""
+        return [*super()._env_state_extras(), lambda: self._u_cross_eb_flat]
+
     def _magnetic_step_callback(self, domain: Domain, time_step=None, max_iter: Optional[int] = None, **kwargs) -> bool:
         """u x e_b, Poisson solve for phi, copy to blocks (mhd.py:877-1002)."""
         total, batch = domain.getTotalSize(), domain.getBatchSize()

@@ -1,8 +1,9 @@
 """PISO time step driver.
 
 Restates the control flow of the reference's ``Simulation._PISO_split_step``
-(src/phipict/core/piso_simulation.py:3525-4169) and ``single_step``
-(src/phipict/simulation/simulation.py:262-333): hooks, momentum prediction (BiCGStab),
+(src/phipict/core/piso_simulation.py:3525-4169), ``single_step``
+(src/phipict/simulation/simulation.py:262-333) and the adaptive CFL substeps
+(piso_simulation.py:149-215, 4171-4340): hooks, momentum prediction (BiCGStab),
 pressure correctors (CG or AMG-PCG), velocity correction, mean projection of the pressure,
 tolerance resolution and the pressure AMG refresh policy (:2222-2392). Every kernel and solve
 runs in the gfx950 library through :mod:`phipict_amd.ops`.
@@ -10,8 +11,10 @@ runs in the gfx950 library through :mod:`phipict_amd.ops`.
 from __future__ import annotations
 
 import logging
-from typing import Any, Optional
+import numbers
+from typing import Any, Callable, Optional
 
+import numpy as np
 import torch
 
 from . import ops
@@ -187,6 +190,31 @@ def update_advective_boundaries(domain: Domain, bounds, velms, dt, tol=None) -> 
     balance_boundary_fluxes(domain, bounds, tol=tol)
 
 
+def get_max_time_step(domain: Domain, time_step_target: float, CFL_cond: float = 0.8,
+                      with_transformations: bool = True) -> tuple:
+    """Split ``time_step_target`` into equal substeps whose size meets ``CFL_cond`` at the
+    current maximum velocity: ``(substep size, substeps)`` (piso_simulation.py:149-183)."""
+    max_vel = domain.getMaxVelocity(True, with_transformations).cpu().numpy()
+    max_time_step = CFL_cond / max_vel
+    if max_time_step >= time_step_target:
+        return time_step_target, 1
+    substeps = int(np.ceil(time_step_target / max_time_step))
+    return time_step_target / substeps, substeps
+
+
+def _cfl_substep(time_step_target, max_vel, CFL_cond: float) -> tuple:
+    """Size of the next substep of the remaining time ``time_step_target`` and the number of
+    substeps it splits into at the maximum velocity ``max_vel`` (piso_simulation.py:186-215)."""
+    if np.isclose(max_vel, 0):
+        max_time_step = time_step_target
+    else:
+        max_time_step = CFL_cond / max_vel
+    if max_time_step >= time_step_target:
+        return time_step_target, 1
+    substeps = int(np.ceil(time_step_target / max_time_step))
+    return time_step_target / substeps, substeps
+
+
 class Simulation:
     """PISO simulation of an incompressible flow on a multi-block domain (the reference's
     ``phipict.Simulation``; constructor arguments as simulation.py:166-248)."""
@@ -207,9 +235,10 @@ class Simulation:
                  normalize_pressure_result: bool = True, pressure_return_best_result: bool = False,
                  pressure_time_step_normalized: bool = False, velocity_corrector: str = "FD",
                  non_orthogonal: bool = True, differentiable: bool = False, hooks=None, prep_fn=None,
-                 time_step: Optional[float] = None, **unsupported: Any) -> None:
+                 time_step: Optional[float] = None, adaptive_CFL: float = 0.8, adaptive_CFL_per_env: bool = True,
+                 stop_fn: Optional[Callable[[], bool]] = None, **unsupported: Any) -> None:
         for k, v in unsupported.items():
-            if k in ("adaptive_CFL", "adaptive_CFL_per_env", "density_viscosity", "scipy_solve_advection",
+            if k in ("density_viscosity", "scipy_solve_advection",
                      "scipy_solve_pressure", "preconditionBiCG", "BiCG_precondition_fallback", "convergence_tol",
                      "solver_double_fallback", "advect_passive_scalar", "exclude_advection_solve_gradients",
                      "exclude_pressure_solve_gradients", "output_resampling_shape", "output_resampling_fill_max_steps"):
@@ -221,12 +250,15 @@ class Simulation:
             raise ValueError("Pass hooks or its former name prep_fn, not both.")
         if differentiable:
             raise NotImplementedError("the differentiable (adjoint) PISO step is not implemented")
-        if substeps == "ADAPTIVE" or (isinstance(substeps, int) and substeps < 1):
-            raise NotImplementedError("adaptive CFL substeps are not implemented; use a fixed number of substeps")
         self.domain = domain
         self.hooks = Hooks.coerce(hooks if hooks is not None else prep_fn)
         self.time_step = float(time_step if time_step is not None else dt)
-        self.substeps = int(substeps)
+        self.substeps = substeps
+        self.adaptive_CFL = float(adaptive_CFL)
+        self.adaptive_CFL_per_env = bool(adaptive_CFL_per_env)
+        self.print_adaptive_step_info = False
+        self.stop_fn = stop_fn
+        self.differentiable = False
         self.corrector_steps = int(corrector_steps)
         self.advection_use_BiCG = advection_use_BiCG
         self.pressure_use_BiCG = pressure_use_BiCG
@@ -260,6 +292,26 @@ class Simulation:
         self._pressure_rank_deficient_cache = None
         self.solver_log: list = []
         self.record_solver_stats = False
+
+    @property
+    def substeps(self) -> int:
+        """Substeps per step; -1 for adaptive CFL substeps."""
+        return self._substeps
+
+    @substeps.setter
+    def substeps(self, substeps) -> None:
+        """An integer > 0, or ``"ADAPTIVE"`` (stored as -1) (piso_simulation.py:1596-1619)."""
+        if isinstance(substeps, str) and substeps.upper() == "ADAPTIVE":
+            substeps = -1
+        else:
+            if not isinstance(substeps, numbers.Integral):
+                raise TypeError("substeps must be integral type or 'ADAPTIVE'.")
+            if not substeps > 0:
+                raise ValueError("substeps must be positive.")
+        self._substeps = int(substeps)
+
+    def _check_stop(self) -> bool:
+        return self.stop_fn is not None and bool(self.stop_fn())
 
     # ---- helpers
     def _time_step_tensor(self) -> torch.Tensor:
@@ -403,13 +455,29 @@ class Simulation:
 
     # ---- the step
     def single_step(self, static: bool = False) -> bool:
-        """One step: flux-balance check, then ``substeps`` PISO substeps (simulation.py:262-333)."""
+        """One step: flux-balance check, then ``substeps`` PISO substeps, adaptive CFL substeps
+        (-1) or substeps fixed from the current velocity (-2) (simulation.py:262-333)."""
         balance = self.domain.GetBoundaryFluxBalance()
         if float(balance.abs().max()) > self.flux_balance_tol:
             raise RuntimeError(f"Domain boundary fluxes not balanced, cannot proceed with simulation step. "
                                f"Flux balance: {balance}, flux_balance_tol: {self.flux_balance_tol}")
+        substeps = self.substeps
+        time_step = None
+        adaptive = False
+        if substeps > 0:
+            pass
+        elif substeps == -1:
+            adaptive = True
+        elif substeps == -2:
+            # substeps from the current velocity, kept for the whole step
+            ts, substeps = get_max_time_step(self.domain, self.time_step, 0.8, with_transformations=True)
+            time_step = torch.tensor([float(ts)], dtype=self.domain.getDtype())
+        else:
+            raise ValueError("Invalid substeps")
         try:
-            return self._PISO_split_step(self.substeps)
+            if adaptive:
+                return self._PISO_adaptive_step()
+            return self._PISO_split_step(substeps, time_step=time_step)
         except LinsolveError:
             _LOG.exception("Simulation failed in step (total step %d):", self.total_step)
             return False
@@ -512,6 +580,92 @@ class Simulation:
                 break
             self.end_step(time_step)
         return solve_ok
+
+    # ---- adaptive CFL substeps
+    def _PISO_adaptive_step(self, CFL_cond: Optional[float] = None, max_subsetps: int = 1000) -> bool:
+        """Advance by ``time_step`` in substeps re-derived before each one from the current
+        maximum computational velocity (piso_simulation.py:4171-4240): one reduction and one
+        4-byte read-back per substep."""
+        domain = self.domain
+        if domain is None or not domain.IsInitialized():
+            raise RuntimeError("the simulation needs a prepared domain (domain.PrepareSolve())")
+        CFL_cond = CFL_cond if CFL_cond is not None else self.adaptive_CFL
+        if domain.getBatchSize() > 1 and self.adaptive_CFL_per_env and not self.differentiable:
+            return self._PISO_adaptive_step_per_env(CFL_cond, max_subsetps)
+        remaining = self.time_step
+        substep = 0
+        warned = False
+        while remaining > 0 and not np.isclose(remaining, 0):
+            max_vel = domain.getMaxVelocity(True, True).cpu().numpy()
+            ts, substeps = _cfl_substep(remaining, max_vel, CFL_cond)
+            remaining -= ts
+            ts_t = torch.tensor([float(ts)], dtype=domain.getDtype())
+            if substeps > max_subsetps and not warned:
+                _LOG.warning("adaptive step (CFL=%.02f) results in more than %d substeps (%d).", CFL_cond,
+                             max_subsetps, substeps)
+                warned = True
+            elif substep == 0 and self.print_adaptive_step_info:
+                _LOG.debug("Adaptive step %d substeps: %d. From CFL = %.02f, max vel = %.03e, time step = %.03e.",
+                           substep, substeps, CFL_cond, float(max_vel), remaining)
+            ok = self._PISO_split_step(1, time_step=ts_t)
+            substep += 1
+            if not ok or self._check_stop():
+                return False
+        _LOG.debug("Adaptive time step %.03e (CFL=%.02f) used %d substeps.", self.time_step, CFL_cond, substep)
+        return True
+
+    def _PISO_adaptive_step_per_env(self, CFL_cond: float, max_subsetps: int) -> bool:
+        """Adaptive substeps per batched environment (piso_simulation.py:4242-4340): every
+        environment splits ``time_step`` by its own maximum velocity; the batch runs until the
+        one with the most substeps is done, and an environment that is already done runs the
+        remaining substeps at its last substep size with its state restored afterwards
+        (:class:`phipict_amd.batching.EnvStateSnapshot`)."""
+        from .batching import EnvStateSnapshot
+
+        domain = self.domain
+        B = domain.getBatchSize()
+        start_time = self.total_time
+        remaining = [self.time_step] * B
+        time_steps = [self.time_step] * B
+        taken = [0] * B
+        substep = 0
+        warned = False
+        while True:
+            active = [r > 0 and not np.isclose(r, 0) for r in remaining]
+            if not any(active):
+                break
+            max_vel = domain.getMaxVelocityPerEnv(True, True).cpu().numpy()
+            most = 0
+            for b in range(B):
+                if not active[b]:
+                    continue  # keeps its last substep size
+                time_steps[b], n = _cfl_substep(remaining[b], max_vel[b], CFL_cond)
+                remaining[b] -= time_steps[b]
+                taken[b] += 1
+                most = max(most, n)
+            if most > max_subsetps and not warned:
+                _LOG.warning("adaptive step (CFL=%.02f) results in more than %d substeps (%d).", CFL_cond,
+                             max_subsetps, most)
+                warned = True
+            done = [b for b in range(B) if not active[b]]
+            snapshot = EnvStateSnapshot(domain, torch.tensor(done), extra=self._env_state_extras()) if done else None
+            ts_t = torch.tensor([float(t) for t in time_steps], dtype=domain.getDtype())
+            ok = self._PISO_split_step(1, time_step=ts_t)
+            if snapshot is not None:
+                snapshot.restore()
+            substep += 1
+            self.total_time = start_time + self.time_step - float(min(remaining))
+            if not ok or self._check_stop():
+                return False
+        self.total_time = start_time + self.time_step
+        _LOG.debug("Adaptive time step %.03e (CFL=%.02f) used %d batched substeps, per environment %s.",
+                   self.time_step, CFL_cond, substep, taken)
+        return True
+
+    def _env_state_extras(self) -> list:
+        """Getters of flat per-environment state outside the domain, restored with it for
+        environments that sit out a substep (piso_simulation.py:4342-4353)."""
+        return []
 
     def make_divergence_free(self, max_iter: int = 1000) -> bool:
         """Project the current velocity onto a divergence-free field: one pressure solve and

@@ -36,9 +36,11 @@ def _tdt(dtype):
     return torch.float64 if dtype == np.float64 else torch.float32
 
 
-def run_pair(specs, dtype, steps, nu, dt, mhd=None, amg=False, tol=None, pot_amg=False, outflow=None, velm=None):
+def run_pair(specs, dtype, steps, nu, dt, mhd=None, amg=False, tol=None, pot_amg=False, outflow=None, velm=None,
+             cfl=None):
     """Run ``steps`` steps on the GPU and in the oracle; return both final states. ``outflow``:
-    (block, face) pairs advected with the static velocity ``velm`` before every step."""
+    (block, face) pairs advected with the static velocity ``velm`` before every step; ``cfl``:
+    adaptive CFL substeps (the oracle's substep sizes are kept in ``osim.substep_sizes``)."""
     import phipict_amd as pa
 
     rtol = tol if tol is not None else RTOL[dtype]
@@ -69,6 +71,8 @@ def run_pair(specs, dtype, steps, nu, dt, mhd=None, amg=False, tol=None, pot_amg
         hooks["PRE"] = advect
     common = dict(domain=dom, dt=dt, non_orthogonal=False, advection_tol=tol, pressure_tol=tol,
                   pressure_use_amg=amg, hooks=hooks)
+    if cfl is not None:
+        common.update(substeps="ADAPTIVE", adaptive_CFL=cfl)
     if epot:
         sim = pa.MHDSimulation(stuart_number=torch.tensor(mhd.stuart), e_b=torch.tensor(mhd.e_b, dtype=_tdt(dtype)),
                                potential_tol=tol, potential_use_preconditioner=pot_amg, **common)
@@ -91,9 +95,13 @@ def run_pair(specs, dtype, steps, nu, dt, mhd=None, amg=False, tol=None, pot_amg
         # the step refuses an unbalanced domain: balance the outflow once before the first step
         pa.simulation.balance_boundary_fluxes(dom, out_bounds)
         balance_boundary_fluxes(od, outflow)
+    osim.substep_sizes = []
     for s in range(steps):
         assert sim.single_step(), f"step {s} failed"
-        osim.step()
+        if cfl is None:
+            osim.step()
+        else:
+            osim.substep_sizes.append(osim.adaptive_step(dt, cfl))
     return dom, sim, od, osim
 
 
@@ -241,3 +249,125 @@ def test_advective_outflow_matches_oracle(dtype, mhd):
     assert np.abs(out[0] - 1.0).max() > 1e-3
     assert abs(float(dom.GetBoundaryFluxBalance()[0])) <= (1e-9 if dtype == np.float64 else 1e-4)
     assert abs(net_fixed_flux(od, [(0, 0), (0, 1)])) <= (1e-9 if dtype == np.float64 else 1e-4)
+
+
+def _max_velocity_cases():
+    return {"outflow2d": cases.outflow2d(nx=12, ny=8), "inflow2d": cases.inflow2d(nx=10, ny=6),
+            "duct3d": cases.duct3d(nx=8, ny=6, nz=5, noise=0.3), "two_blocks": cases.split_channel2d(nx=16, ny=12)[1],
+            "rotated_pair3d": cases.rotated_pair3d()}
+
+
+@pytest.mark.parametrize("case", ["outflow2d", "inflow2d", "duct3d", "two_blocks", "rotated_pair3d"])
+def test_max_velocity_matches_oracle(dtype, case):
+    """Domain/Block getMaxVelocity, getMaxVelocityPerEnv and getMaxVelocityMagnitude (one
+    reduction launch over cells and fixed faces) against the oracle, with and without bounds,
+    in physical and computational space; per environment on a batch of two whose second
+    environment runs 3x faster."""
+    print(f"[max velocity] {case} dtype={np.dtype(dtype).name}", flush=True)
+    specs = _max_velocity_cases()[case]
+    dom = build_domain(specs, 0.05, 1e-2, _tdt(dtype))
+    od = O.OracleDomain(specs, 0.05, 1e-2, dtype)
+    rtol = 1e-13 if dtype == np.float64 else 2e-6
+    for wb in (False, True):
+        for comp in (False, True):
+            want = od.max_velocity(wb, comp)
+            got = float(dom.getMaxVelocity(wb, comp))
+            assert got == pytest.approx(want, rel=rtol), (wb, comp)
+            got_m = float(dom.getMaxVelocityMagnitude(wb, comp))
+            assert got_m == pytest.approx(od.max_velocity(wb, comp, magnitude=True), rel=rtol), (wb, comp)
+            for bi, blk in enumerate(dom.getBlocks()):
+                assert float(blk.getMaxVelocity(wb, comp)) == pytest.approx(od.max_velocity(wb, comp, block=bi),
+                                                                             rel=rtol)
+    # physical vs computational through the transform operator
+    blk = dom.getBlocks()[0]
+    vc = blk.getVelocity(True)
+    assert float(vc.abs().max()) == pytest.approx(od.max_velocity(False, True, block=0), rel=rtol)
+    # per environment
+    dom2 = build_domain(specs, 0.05, 1e-2, _tdt(dtype), batch=2)
+    for b in dom2.getBlocks():
+        v = b.velocity.clone()
+        v[1] *= 3.0
+        b.setVelocity(v)
+    per_env = to_np(dom2.getMaxVelocityPerEnv(False, True))
+    assert per_env.shape == (2,)
+    assert per_env[0] == pytest.approx(od.max_velocity(False, True), rel=rtol)
+    assert per_env[1] == pytest.approx(3.0 * od.max_velocity(False, True), rel=rtol)
+    assert float(dom2.getMaxVelocity(False, True)) == pytest.approx(per_env[1], rel=rtol)
+    # a NaN anywhere gives NaN, as torch.max does
+    v = dom.getBlocks()[-1].velocity.clone()
+    v.view(-1)[3] = float("nan")
+    dom.getBlocks()[-1].setVelocity(v)
+    assert math.isnan(float(dom.getMaxVelocity(True, True)))
+
+
+@pytest.mark.parametrize("mhd", [False, True], ids=["hydro", "mhd"])
+def test_adaptive_substeps_match_oracle(dtype, mhd):
+    """substeps="ADAPTIVE": before every substep the size is re-derived from the maximum
+    computational velocity (cells and fixed faces); the GPU and the oracle take the same
+    substeps and reach the same fields."""
+    print(f"[piso] adaptive CFL dtype={np.dtype(dtype).name} mhd={mhd}", flush=True)
+    specs = cases.channel2d(nx=16, ny=12, u0=1.0, gradient=4.0, noise=1e-2)
+    dom, sim, od, osim = run_pair(specs, dtype, 2, 0.05, 0.25, mhd=MHDSpec(4.0, e_b=(0.0, 0.6, 0.8)) if mhd else None,
+                                  cfl=0.8)
+    # The pressure of this near-uniform flow is 1.5e-4 of the velocity's scale, so FP32 fixes it
+    # only to about 1e-3 (the oracle's own FP32 and FP64 runs differ by 1.6e-3 there).
+    compare(dom, od, osim, dtype, mhd, 5e-3 if dtype == np.float32 else None)
+    sizes = [t for step in osim.substep_sizes for t in step]
+    assert len(sizes) >= 6  # at least 3 substeps per step at 8 cells per unit time
+    assert sim.total_step == len(sizes)
+    # the time advances by the substep sizes in the domain's precision
+    assert sim.total_time == pytest.approx(0.5, rel=1e-12 if dtype == np.float64 else 1e-6)
+
+
+@pytest.mark.parametrize("mhd", [False, True], ids=["hydro", "mhd"])
+def test_adaptive_per_env_matches_single(mhd):
+    """Per-environment adaptive substeps of a batch of two (the second flows 2.5x faster and
+    takes more substeps): each environment ends exactly where it ends on its own, the one done
+    early restored after the extra substeps it sat out."""
+    import phipict_amd as pa
+
+    print(f"[piso] adaptive per env mhd={mhd}", flush=True)
+    dt, steps = 0.25, 2
+    scales = (1.0, 2.5)
+
+    def make(batch):
+        specs = cases.channel2d(nx=16, ny=12, u0=1.0, gradient=4.0, noise=1e-2)
+        dom = build_domain(specs, 0.05, dt, torch.float64, batch=batch)
+        kw = dict(domain=dom, dt=dt, non_orthogonal=False, advection_tol=1e-12, pressure_tol=1e-12,
+                  substeps="ADAPTIVE", adaptive_CFL=0.8)
+        src = dom.getBlocks()[0].velocitySource.clone()
+
+        def reset(domain, **k):
+            domain.getBlocks()[0].setVelocitySource(src.clone())
+            domain.UpdateDomainData()
+
+        kw["hooks"] = {"PRE_VELOCITY_SETUP": reset}
+        if mhd:
+            sim = pa.MHDSimulation(stuart_number=torch.tensor(4.0), e_b=torch.tensor([0.0, 1.0, 0.0], dtype=torch.float64),
+                                   potential_tol=1e-12, **kw)
+        else:
+            sim = pa.Simulation(**kw)
+        return dom, sim
+
+    singles = []
+    for sc in scales:
+        dom, sim = make(1)
+        blk = dom.getBlocks()[0]
+        blk.setVelocity(blk.velocity * sc)
+        for _ in range(steps):
+            assert sim.single_step()
+        singles.append((to_np(blk.velocity)[0], to_np(blk.pressure)[0], sim.total_step))
+    dom, sim = make(2)
+    blk = dom.getBlocks()[0]
+    v = blk.velocity.clone()
+    for e, sc in enumerate(scales):
+        v[e] *= sc
+    blk.setVelocity(v)
+    for _ in range(steps):
+        assert sim.single_step()
+    assert singles[0][2] < singles[1][2]
+    assert sim.total_step == singles[1][2]
+    assert sim.total_time == pytest.approx(steps * dt, rel=1e-12)
+    for e in range(2):
+        assert rel_l2(to_np(blk.velocity)[e], singles[e][0]) < 1e-10
+        assert rel_l2(to_np(blk.pressure)[e], singles[e][1]) < 1e-9

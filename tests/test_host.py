@@ -216,8 +216,6 @@ def test_simulation_rejects_unimplemented_options():
     dom = _cpu_domain(2)
     with pytest.raises(NotImplementedError):
         pa.Simulation(domain=dom, dt=0.1, differentiable=True)
-    with pytest.raises(NotImplementedError):
-        pa.Simulation(domain=dom, dt=0.1, substeps="ADAPTIVE")
     with pytest.raises(TypeError):
         pa.Simulation(domain=dom, dt=0.1, no_such_option=1)
     with pytest.raises(ValueError):
@@ -292,3 +290,28 @@ def test_counter_traffic_table(tmp_path):
     assert r["dispatches"] == 3 and r["early_exits"] == 1
     assert r["fetch_bytes"] == pytest.approx(500 * 1024 / 2)
     assert r["write_bytes"] == pytest.approx(150 * 1024 / 1.5)
+
+
+def test_substeps_setter_and_cfl_split():
+    """substeps takes a positive integer or "ADAPTIVE" (-1), anything else raises like the
+    reference's setter (piso_simulation.py:1596-1619); the CFL split of the product and the
+    oracle's agree on the same inputs, including a float32 maximum velocity."""
+    import phipict_amd as pa
+    from phipict_amd.simulation import _cfl_substep
+    from oracle.piso_oracle import cfl_substep
+
+    sim = pa.Simulation(domain=None, dt=0.1)
+    assert sim.substeps == 1
+    sim.substeps = "adaptive"
+    assert sim.substeps == -1
+    sim.substeps = 3
+    assert sim.substeps == 3
+    with pytest.raises(ValueError):
+        sim.substeps = 0
+    with pytest.raises(TypeError):
+        sim.substeps = 2.0
+    with pytest.raises(TypeError):
+        pa.Simulation(domain=None, substeps="fast")
+    for t, v in [(0.25, np.asarray(8.0, np.float32)), (0.25, np.asarray(7.77, np.float64)), (1.0, np.float32(0.0)),
+                 (0.1, np.asarray(3.3, np.float32))]:
+        assert _cfl_substep(t, v, 0.8) == cfl_substep(t, v, 0.8)

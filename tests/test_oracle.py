@@ -518,3 +518,73 @@ def test_advective_outflow_is_implicit_euler():
     before = od.face_velocity(0, 1).copy()
     balance_boundary_fluxes(od, [(0, 1)])
     np.testing.assert_array_equal(od.face_velocity(0, 1), before)
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_max_velocity_affine_closed_form(d):
+    """On vertices x = L diag(h) xi every cell and face has Minv = (L diag(h))^-1, so the
+    computational velocity is a linear solve per cell: the maximum over cells and fixed faces
+    (static and per-face-cell velocities) of |Minv u| and |Minv u|_2, and of |u| in physical
+    space; a face velocity larger than every cell's decides the maximum only with bounds."""
+    rng = np.random.default_rng(11 + d)
+    n = [5, 4, 3][:d]
+    h = np.array([0.3, 0.7, 0.45][:d])
+    L = np.eye(d) + 0.3 * rng.standard_normal((d, d))
+    M = L @ np.diag(h)
+    base = cases.vertices([np.arange(k + 1, dtype=np.float64) for k in n])
+    v = np.einsum("ij,j...->i...", M, base)
+    cz = cases.cell_shape(v)
+    vel = rng.standard_normal([d] + cz)
+    static = 0.5 * rng.standard_normal(d)
+    varying = rng.standard_normal([d] + cases.face_shape(cz, 3))
+    faces = [O.FaceSpec("periodic"), O.FaceSpec("periodic"), O.FaceSpec("fixed", velocity=static),
+             O.FaceSpec("fixed", velocity=varying)] + [O.FaceSpec("periodic")] * (2 * d - 4)
+    od = O.OracleDomain([O.BlockSpec(v, faces, velocity=vel)], 0.1, 1e-2)
+    Minv = np.linalg.inv(M)
+    cell_c = np.einsum("ik,k...->i...", Minv, vel).reshape(d, -1)
+    face_c = [Minv @ static, np.einsum("ik,k...->i...", Minv, varying).reshape(d, -1)]
+    assert od.max_velocity(False, False) == pytest.approx(np.abs(vel).max(), rel=1e-15)
+    assert od.max_velocity(False, True) == pytest.approx(np.abs(cell_c).max(), rel=1e-12)
+    assert od.max_velocity(False, True, magnitude=True) == pytest.approx(np.linalg.norm(cell_c, axis=0).max(), rel=1e-12)
+    with_faces = max(np.abs(cell_c).max(), np.abs(face_c[0]).max(), np.abs(face_c[1]).max())
+    assert od.max_velocity(True, True) == pytest.approx(with_faces, rel=1e-12)
+    mags = max(np.linalg.norm(cell_c, axis=0).max(), np.linalg.norm(face_c[0]), np.linalg.norm(face_c[1], axis=0).max())
+    assert od.max_velocity(True, True, magnitude=True) == pytest.approx(mags, rel=1e-12)
+    # a fast static face decides the maximum with bounds only
+    od.set_face_velocity(0, 2, 100.0 * np.eye(d)[0])
+    assert od.max_velocity(True, False) == pytest.approx(100.0)
+    assert od.max_velocity(True, True) == pytest.approx(np.abs(Minv[:, 0]).max() * 100.0, rel=1e-12)
+    assert od.max_velocity(False, True) == pytest.approx(np.abs(cell_c).max(), rel=1e-12)
+    od.velocity[0][0, 1] = np.nan
+    assert np.isnan(od.max_velocity(True, True))
+
+
+def test_cfl_substep_splits():
+    """The substeps of the remaining time cover it exactly, each within the CFL bound, and are
+    the fewest that do; a resting or slow flow takes the whole time in one substep."""
+    from oracle.piso_oracle import cfl_substep
+
+    assert cfl_substep(0.3, 0.0, 0.8) == (0.3, 1)
+    assert cfl_substep(0.3, 1.0, 0.8) == (0.3, 1)
+    for t, v, c in [(1.0, 7.3, 0.8), (0.25, 8.0, 0.8), (2.0, 0.41, 0.5), (0.1, 123.4, 0.3)]:
+        ts, n = cfl_substep(t, v, c)
+        assert ts * n == pytest.approx(t, rel=1e-15)
+        assert ts * v <= c * (1 + 1e-15)
+        assert n == 1 or (t / (n - 1)) * v > c
+
+
+def test_oracle_adaptive_step_substeps():
+    """An adaptive step of a channel flow at 1.0 (computational velocity 1/dx = 8 in x) and
+    CFL 0.8 splits 0.25 into ceil(0.25 * 8 / 0.8) = 3 equal substeps at first, and the substeps
+    of a step always add up to it."""
+    from oracle.piso_oracle import OraclePISO
+
+    specs = cases.channel2d(nx=16, ny=12, u0=1.0, noise=0.0, gradient=None)
+    od = O.OracleDomain(specs, 0.05, 0.25)
+    assert od.max_velocity(True, True) == pytest.approx(8.0, rel=1e-12)
+    osim = OraclePISO(od, 0.25, advection_tol=1e-12, pressure_tol=1e-12)
+    sizes = osim.adaptive_step(0.25, 0.8)
+    assert len(sizes) == 3 and sizes[0] == pytest.approx(0.25 / 3, rel=1e-15)
+    assert sum(sizes) == pytest.approx(0.25, rel=1e-14)
+    sizes2 = osim.adaptive_step(0.25, 0.8)
+    assert sum(sizes2) == pytest.approx(0.25, rel=1e-14)
