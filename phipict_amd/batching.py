@@ -1,5 +1,8 @@
-"""Per-environment state of a batched domain, saved before a substep and written back
-afterwards for the environments that sit it out (src/phipict/batching.py:178-320)."""
+"""Batched environments: B copies of one domain stepped together, sharing the grid, the block
+and boundary layout and the sparsity patterns, each with its own state (``[B, ...]`` block
+fields) -- stacking single-environment domains into one, copying an environment's state, and
+saving the state of some environments across a substep they sit out
+(src/phipict/batching.py:19-320)."""
 from __future__ import annotations
 
 from typing import Callable, Optional, Sequence
@@ -7,6 +10,73 @@ from typing import Callable, Optional, Sequence
 import torch
 
 from .domain import Domain, FixedBoundary
+
+__all__ = ["EnvStateSnapshot", "copy_env_state", "stack_domains"]
+
+
+def _copy_boundary_data(dst: torch.Tensor, src: torch.Tensor, env_idx: int, batch_size: int,
+                        setter: Callable[[torch.Tensor], None]) -> None:
+    """Write boundary data ``src`` (batch 1) into environment ``env_idx``; shared data equal to
+    ``src`` stays shared, otherwise it becomes per-environment first (batching.py:129-150)."""
+    if dst.shape[1:] != src.shape[1:]:
+        raise ValueError(f"Boundary data shapes differ: {tuple(dst.shape[1:])} vs {tuple(src.shape[1:])}.")
+    if dst.size(0) == 1:
+        if torch.equal(dst, src):
+            return
+        dst = dst.expand(batch_size, *dst.shape[1:]).clone()
+        dst[env_idx].copy_(src[0])
+        setter(dst)
+    else:
+        dst[env_idx].copy_(src[0])
+
+
+def copy_env_state(dst: Domain, env_idx: int, src: Domain) -> None:
+    """Copy the state of a single-environment domain into environment ``env_idx`` of a batch
+    (batching.py:19-127): block velocity, pressure and potential in place, and the velocity of
+    every fixed face that differs from ``dst``'s (making it per-environment). The faces'
+    Dirichlet potential values are one array shared by all environments here: values that
+    differ between the domains raise."""
+    if src.getBatchSize() != 1:
+        raise ValueError(f"src must have batch size 1, got {src.getBatchSize()}.")
+    if not 0 <= env_idx < dst.getBatchSize():
+        raise ValueError(f"env_idx {env_idx} out of range for batch size {dst.getBatchSize()}.")
+    if src.getNumBlocks() != dst.getNumBlocks():
+        raise ValueError("src and dst have a different number of blocks.")
+    with torch.no_grad():
+        for db, sb in zip(dst.getBlocks(), src.getBlocks()):
+            fields = [(db.velocity, sb.velocity), (db.pressure, sb.pressure)]
+            if db.epot is not None and sb.epot is not None:
+                fields.append((db.epot, sb.epot))
+            for df, sf in fields:
+                if df.shape[1:] != sf.shape[1:]:
+                    raise ValueError(f"Block field shapes differ: {tuple(df.shape[1:])} vs {tuple(sf.shape[1:])}.")
+                df[env_idx].copy_(sf[0])
+            for face in range(2 * dst.getSpatialDims()):
+                dbound, sbound = db.getBoundary(face), sb.getBoundary(face)
+                if not isinstance(dbound, FixedBoundary) or not isinstance(sbound, FixedBoundary):
+                    continue
+                _copy_boundary_data(dbound.velocity, sbound.velocity, env_idx, dst.getBatchSize(), dbound.setVelocity)
+                dv, sv = dbound.potentialValues, sbound.potentialValues
+                if (dv is None) != (sv is None) or (dv is not None and not torch.equal(dv, sv)):
+                    raise NotImplementedError("per-environment Dirichlet potential values are not supported")
+            db._domain._changed = True
+    dst.UpdateDomainData()
+
+
+def stack_domains(domains: Sequence[Domain]) -> Domain:
+    """Combine single-environment domains of one setup into a batched domain: the first
+    becomes the batch (in place, prepared for solving) and environment ``b`` gets the state of
+    ``domains[b]`` (batching.py:152-177)."""
+    if len(domains) == 0:
+        raise ValueError("stack_domains needs at least one domain.")
+    batched = domains[0]
+    if batched.getBatchSize() != 1:
+        raise ValueError(f"Domains must have batch size 1, got {batched.getBatchSize()}.")
+    batched.setBatchSize(len(domains))
+    batched.PrepareSolve()
+    for env_idx, domain in enumerate(domains[1:], start=1):
+        copy_env_state(batched, env_idx, domain)
+    return batched
 
 
 class EnvStateSnapshot:

@@ -225,6 +225,7 @@ class Simulation:
     NON_ORTHO_DIAGONAL_RHS = 8
     NON_ORTHO_CENTER_MATRIX = 16
     PRESSURE_AMG_AUTO_MIN_ROWS = 200_000
+    PRESSURE_AMG_AUTO_MIN_ROWS_BATCHED = 400_000
     _NON_ORTHO_MODE = NON_ORTHO_CENTER_MATRIX | NON_ORTHO_DIRECT_MATRIX | NON_ORTHO_DIAGONAL_RHS
 
     def __init__(self, domain: Optional[Domain] = None, dt: float = 1.0, substeps: int = 1, corrector_steps: int = 2,
@@ -370,9 +371,7 @@ class Simulation:
         if self.pressure_use_BiCG:
             return None
         if P.getBatchSize() > 1:
-            if self.pressure_use_amg:
-                raise NotImplementedError("batched pressure AMG is not implemented")
-            return None
+            return self._get_batched_pressure_amg_hierarchy(P, self.pressure_use_amg)
         use_amg = self.pressure_use_amg
         if use_amg is None:
             if P.getRows() < self.PRESSURE_AMG_AUTO_MIN_ROWS:
@@ -397,6 +396,35 @@ class Simulation:
             self._pressure_amg_baseline_iters = None
             refreshes = 0
         self._pressure_amg_refreshes = refreshes + 1
+        self._pressure_amg_version = version
+        return self._pressure_amg_hierarchy
+
+    def _get_batched_pressure_amg_hierarchy(self, P, use_amg):
+        """Batched counterpart (:2294-2392): the environments share the interpolation and their
+        coarse operators are refreshed together; in auto mode AMG is used from
+        ``PRESSURE_AMG_AUTO_MIN_ROWS_BATCHED`` unknowns over all environments."""
+        batch = P.getBatchSize()
+        if use_amg is None:
+            use_amg = P.getRows() * batch >= self.PRESSURE_AMG_AUTO_MIN_ROWS_BATCHED
+        if not use_amg:
+            return None
+        version = self.domain._a_version
+        previous = self._pressure_amg_hierarchy
+        batched = previous is not None and previous.batch_size == batch
+        if batched and self._pressure_amg_version == version:
+            return previous
+        options = dict(self._pressure_amg_options)
+        interval = int(options.pop("coarse_refresh_interval", 4))
+        coarse = (not batched or interval <= 1 or self._pressure_amg_refreshes % interval == 0
+                  or self._pressure_amg_stale)
+        self._pressure_amg_hierarchy = amg_mod.batched_hierarchy_for(
+            P, batch, project_constant=self._pressure_rank_deficient(P), previous=previous if batched else None,
+            coarse=coarse, **options)
+        if coarse:
+            self._pressure_amg_refreshes = 0
+            self._pressure_amg_stale = False
+            self._pressure_amg_baseline_iters = None
+        self._pressure_amg_refreshes += 1
         self._pressure_amg_version = version
         return self._pressure_amg_hierarchy
 

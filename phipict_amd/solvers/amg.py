@@ -120,7 +120,8 @@ class AMGHierarchy:
         self.set_coarse(coarse_A)
 
     def set_coarse(self, coarse_A: torch.Tensor) -> None:
-        """Dense pseudo-inverse of the coarsest operator (numpy, float64; amg.py:583-587)."""
+        """Dense pseudo-inverse of the coarsest operator, ``[nc, nc]`` or ``[B, nc, nc]`` for
+        batched operators (numpy, float64; amg.py:583-587, 1354-1362)."""
         dense = coarse_A.detach().to(torch.float64).cpu().numpy()
         self.coarse_pinv64 = torch.from_numpy(np.linalg.pinv(dense)).to(coarse_A.device)
         self._pinv_native = {}
@@ -128,6 +129,11 @@ class AMGHierarchy:
     @property
     def dtype(self):
         return self.levels[0].A.getDtype()
+
+    @property
+    def batch_size(self) -> int:
+        """Operators per level: 1, or one per batched environment (shared interpolation)."""
+        return self.levels[0].A.getBatchSize()
 
     def coarse_pinv(self, dtype) -> torch.Tensor:
         if dtype not in self._pinv_native:
@@ -347,6 +353,7 @@ _DENSE_ROWS: dict = {}
 
 
 def _dense(A: CSRmatrix) -> torch.Tensor:
+    """The operator as a dense float64 ``[n, n]``, or ``[B, n, n]`` for B batched operators."""
     n = A.getRows()
     key = (A.row.data_ptr(), n, A.getNnz())
     rows = _DENSE_ROWS.get(key)
@@ -357,20 +364,31 @@ def _dense(A: CSRmatrix) -> torch.Tensor:
         rows = (A.row, torch.repeat_interleave(torch.arange(n, device=row.device), counts))
         _DENSE_ROWS[key] = rows
     rows = rows[1]
+    B, nnz = A.getBatchSize(), A.getNnz()
+    if B > 1:
+        D = torch.zeros(B, n * n, dtype=torch.float64, device=A.value.device)
+        D.index_add_(1, rows * n + A.index.to(torch.int64), A.value.reshape(-1)[: B * nnz].reshape(B, nnz).to(torch.float64))
+        return D.reshape(B, n, n)
     D = torch.zeros(n, n, dtype=torch.float64, device=A.value.device)
-    D.index_put_((rows, A.index.to(torch.int64)), A.value.reshape(-1)[: A.getNnz()].to(torch.float64), accumulate=True)
+    D.index_put_((rows, A.index.to(torch.int64)), A.value.reshape(-1)[:nnz].to(torch.float64), accumulate=True)
     return D
 
 
 def galerkin_refresh(hierarchy: AMGHierarchy, csr_mat: CSRmatrix, *, project_constant: Optional[bool] = None,
                      coarse: bool = True) -> AMGHierarchy:
     """Keep P and R, replace the finest operator by ``csr_mat`` and (with ``coarse``) recompute
-    the coarse operators as device Galerkin products (amg.py:749-847)."""
+    the coarse operators as device Galerkin products (amg.py:749-847). A batched ``csr_mat``
+    (B operators of one pattern) gives B operators on every level from the same interpolation,
+    refreshed together (BatchedAMGHierarchy.refresh, amg.py:1325-1368); a finest-only refresh
+    needs a hierarchy of the same batch size."""
     t0 = time.perf_counter()
     fine = hierarchy.levels[0]
     if csr_mat.getRows() != fine.n:
         raise ValueError("operator does not match the finest AMG level")
-    A = CSRmatrix(csr_mat.value.reshape(-1)[: csr_mat.getNnz()], csr_mat.index, csr_mat.row)
+    B = csr_mat.getBatchSize()
+    if not coarse and B != hierarchy.batch_size:
+        raise ValueError("a finest-level refresh needs a hierarchy of the same batch size")
+    A = CSRmatrix(csr_mat.value.reshape(-1)[: B * csr_mat.getNnz()], csr_mat.index, csr_mat.row)
     A.share_ell_from(csr_mat)
     levels = [Level(n=fine.n, A=A, R_row=fine.R_row, R_col=fine.R_col, R_val=fine.R_val, P_row=fine.P_row,
                     P_col=fine.P_col, P_val=fine.P_val, shared=fine.shared)]
@@ -379,19 +397,26 @@ def galerkin_refresh(hierarchy: AMGHierarchy, csr_mat: CSRmatrix, *, project_con
         for i in range(1, len(hierarchy.levels)):
             prev = levels[-1]
             old = hierarchy.levels[i]
-            cval = torch.empty(old.A.getNnz(), dtype=A.getDtype(), device=A.value.device)
+            nnz_c, nnz_f = old.A.getNnz(), prev.A.getNnz()
+            cval = torch.empty(B * nnz_c, dtype=A.getDtype(), device=A.value.device)
             stream = _lib.stream_ptr(A.value.device)
             if hierarchy.levels[i - 1].gmap is not None:
+                # one pass of the static map over all B operators
                 seg, src, w = hierarchy.levels[i - 1].gmap
-                _lib.call("pp_galerkin_apply", _lib.ptr(seg), _lib.ptr(src), _lib.ptr(w), old.A.getNnz(), src.numel(),
+                _lib.call("pp_galerkin_apply", _lib.ptr(seg), _lib.ptr(src), _lib.ptr(w), nnz_c, src.numel(),
                           _lib.ptr(prev.A.value), _lib.dtype_code(prev.A.getDtype()), _lib.ptr(cval),
-                          _lib.dtype_code(A.getDtype()), 1, 0, 0, stream)
+                          _lib.dtype_code(A.getDtype()), B, nnz_f, nnz_c, stream)
             else:
-                d = prev.A.desc()
-                _lib.call("pp_galerkin_rap", _lib.ptr(prev.R_row), _lib.ptr(prev.R_col), _lib.ptr(prev.R_val),
-                          C.byref(d), _lib.dtype_code(prev.A.getDtype()), _lib.ptr(prev.P_row), _lib.ptr(prev.P_col),
-                          _lib.ptr(prev.P_val), _lib.ptr(old.A.row), _lib.ptr(old.A.index), _lib.ptr(cval), old.n,
-                          old.A.getNnz(), _lib.dtype_code(A.getDtype()), stream)
+                esize = prev.A.value.element_size()
+                for b in range(B):
+                    d = prev.A.desc()
+                    d.val = _lib.ptr(prev.A.value) + b * nnz_f * esize
+                    d.num_matrices, d.ell_width, d.ell_col, d.ell_val = 1, 0, None, None
+                    _lib.call("pp_galerkin_rap", _lib.ptr(prev.R_row), _lib.ptr(prev.R_col), _lib.ptr(prev.R_val),
+                              C.byref(d), _lib.dtype_code(prev.A.getDtype()), _lib.ptr(prev.P_row),
+                              _lib.ptr(prev.P_col), _lib.ptr(prev.P_val), _lib.ptr(old.A.row), _lib.ptr(old.A.index),
+                              _lib.ptr(cval) + b * nnz_c * cval.element_size(), old.n, nnz_c,
+                              _lib.dtype_code(A.getDtype()), stream)
             levels.append(Level(n=old.n, A=CSRmatrix(cval, old.A.index, old.A.row), R_row=old.R_row, R_col=old.R_col,
                                 R_val=old.R_val, P_row=old.P_row, P_col=old.P_col, P_val=old.P_val, gmap=old.gmap,
                                 shared=old.shared))
@@ -440,6 +465,32 @@ def hierarchy_for(csr_mat: CSRmatrix, *, project_constant: bool = False, reuse_i
         _INTERPOLATION_CACHE[key] = h
         return h
     h = galerkin_refresh(base, csr_mat, project_constant=project_constant)
+    h.native_dtype = native_dtype
+    return h
+
+
+def batched_hierarchy_for(csr_mat: CSRmatrix, batch_size: int, *, project_constant: bool = False,
+                          previous: Optional[AMGHierarchy] = None, coarse: bool = True, native_dtype=torch.float32,
+                          **build_kwargs) -> AMGHierarchy:
+    """Hierarchy of the B operators of a batched ``csr_mat`` (amg.py:1414-1482): the cached
+    interpolation of the sparsity pattern (set up from the first operator if the pattern is
+    new) shared by all environments, every level holding B operators refreshed together.
+    ``previous`` (same pattern and batch size) allows a finest-only refresh (``coarse=False``)."""
+    nnz = csr_mat.getNnz()
+    if csr_mat.getBatchSize() != batch_size:
+        raise ValueError(f"matrix holds {csr_mat.getBatchSize()} operators, not {batch_size}")
+    if (previous is None or previous.batch_size != batch_size or previous.levels[0].A.getNnz() != nnz
+            or previous.levels[0].A.row is not csr_mat.row):
+        key = _sparsity_key(csr_mat)
+        base = _INTERPOLATION_CACHE.get(key)
+        if base is None or base.dtype != csr_mat.getDtype():
+            first = CSRmatrix(csr_mat.value.reshape(-1)[:nnz], csr_mat.index, csr_mat.row)
+            base = build_amg_hierarchy(first, project_constant=project_constant, native_dtype=native_dtype,
+                                       **build_kwargs)
+            _INTERPOLATION_CACHE[key] = base
+        previous = base
+        coarse = True
+    h = galerkin_refresh(previous, csr_mat, project_constant=project_constant, coarse=coarse)
     h.native_dtype = native_dtype
     return h
 

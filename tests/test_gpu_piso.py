@@ -197,6 +197,74 @@ def test_batched_environments_match_single():
         assert rel_l2(got[e], runs[e]) < 1e-10
 
 
+def test_batched_pressure_amg_matches_single():
+    """Two environments of a 3D duct in one batched domain with AMG-PCG pressure solves (shared
+    interpolation, per-environment Galerkin operators refreshed together) step like two
+    separate domains with their own AMG hierarchies, to the solver tolerance."""
+    import phipict_amd as pa
+
+    print("[piso] batched pressure AMG", flush=True)
+    specs_a = cases.duct3d(nx=12, ny=8, nz=10, noise=1e-2, gradient=20.0, seed=1)
+    specs_b = cases.duct3d(nx=12, ny=8, nz=10, noise=1e-2, gradient=20.0, seed=2)
+    dt = 2e-3
+    kw = dict(dt=dt, non_orthogonal=False, advection_tol=1e-12, pressure_tol=1e-12, pressure_use_amg=True)
+    runs = []
+    for specs in (specs_a, specs_b):
+        dom = build_domain(specs, 0.02, dt, torch.float64)
+        sim = pa.Simulation(domain=dom, **kw)
+        for _ in range(3):
+            assert sim.single_step()
+        runs.append((to_np(dom.getBlocks()[0].velocity)[0], to_np(dom.getBlocks()[0].pressure)[0]))
+    dom = build_domain(specs_a, 0.02, dt, torch.float64, batch=2)
+    blk = dom.getBlocks()[0]
+    v = blk.velocity.clone()
+    v[1] = torch.as_tensor(specs_b[0].velocity, dtype=torch.float64)
+    blk.setVelocity(v)
+    sim = pa.Simulation(domain=dom, **kw)
+    for _ in range(3):
+        assert sim.single_step()
+    assert sim._pressure_amg_hierarchy is not None and sim._pressure_amg_hierarchy.batch_size == 2
+    for e in range(2):
+        assert rel_l2(to_np(blk.velocity)[e], runs[e][0]) < 1e-9
+        assert rel_l2(to_np(blk.pressure)[e], runs[e][1]) < 1e-7
+
+
+def test_stack_domains_matches_single():
+    """stack_domains of two channels with different inflow velocities (the fixed faces become
+    per-environment) steps both like separate domains; copy_env_state writes one environment."""
+    import phipict_amd as pa
+    from phipict_amd.batching import copy_env_state, stack_domains
+
+    print("[piso] stack_domains", flush=True)
+    dt = 1e-2
+    kw = dict(dt=dt, non_orthogonal=False, advection_tol=1e-12, pressure_tol=1e-12)
+    specs = [cases.inflow2d(nx=20, ny=10, U=1.0, seed=3), cases.inflow2d(nx=20, ny=10, U=1.6, seed=4)]
+    runs = []
+    for sp in specs:
+        dom = build_domain(sp, 0.02, dt, torch.float64)
+        sim = pa.Simulation(domain=dom, **kw)
+        for _ in range(3):
+            assert sim.single_step()
+        runs.append(to_np(dom.getBlocks()[0].velocity)[0])
+    doms = [build_domain(sp, 0.02, dt, torch.float64) for sp in specs]
+    batched = stack_domains(doms)
+    batched.setTimeStep(dt)
+    assert batched.getBatchSize() == 2
+    assert batched.getBlocks()[0].getBoundary(0).velocity.size(0) == 2
+    sim = pa.Simulation(domain=batched, **kw)
+    for _ in range(3):
+        assert sim.single_step()
+    got = to_np(batched.getBlocks()[0].velocity)
+    for e in range(2):
+        assert rel_l2(got[e], runs[e]) < 1e-10
+    # copy_env_state: environment 0 takes the second domain's (unstepped) state
+    fresh = build_domain(specs[1], 0.02, dt, torch.float64)
+    copy_env_state(batched, 0, fresh)
+    np.testing.assert_array_equal(to_np(batched.getBlocks()[0].velocity)[0], to_np(fresh.getBlocks()[0].velocity)[0])
+    np.testing.assert_array_equal(to_np(batched.getBlocks()[0].getBoundary(0).velocity)[0],
+                                  to_np(fresh.getBlocks()[0].getBoundary(0).velocity)[0])
+
+
 def test_hartmann_profile_error():
     """Steady Hartmann flow: the computed profile against u(y) = G/N (1 - cosh(Ha y)/cosh Ha)
     (examples/hartmann.py:85); the maximum relative error is printed and bounded."""

@@ -444,6 +444,52 @@ def test_galerkin_refresh_matches_oracle():
     assert h3.coarse_pinv64 is h.coarse_pinv64
 
 
+@pytest.mark.parametrize("combo", ["f64/f64", "f64/f32"])
+def test_batched_amg_pcg_matches_oracle(combo):
+    """Two operators of one pattern (batched environments) on the interpolation of the first
+    (batched_hierarchy_for): each environment's solve equals the oracle's AMG-PCG with the
+    first operator's interpolation and its own Galerkin operators and coarse pseudo-inverse;
+    a finest-only refresh keeps the coarse levels of the batch."""
+    print(f"[amg] batched combo={combo}", flush=True)
+    import phipict_amd as pa
+    from phipict_amd.solvers import amg
+
+    dt_h = np.float64 if combo == "f64/f64" else np.float32
+    row, col, val0 = poisson(48, 40, shift=0.0, seed=1)
+    _, _, val1 = poisson(48, 40, shift=0.0, seed=2)
+    val1 = 1.7 * val1
+    n, nnz = row.size - 1, col.size
+    amg.clear_interpolation_cache()
+    A = csr(row, col, np.concatenate([val0, val1]), np.float64)
+    assert A.getBatchSize() == 2
+    h = amg.batched_hierarchy_for(A, 2, project_constant=True, native_dtype=_tdt(dt_h))
+    assert h.batch_size == 2 and h.coarse_pinv64.dim() == 3
+    hs0 = OracleHierarchyState(row, col, val0, True, hdtype=dt_h)
+    hs1 = OracleHierarchyState(row, col, val1, True, hdtype=dt_h, base=hs0)
+    b = np.random.default_rng(5).standard_normal(2 * n)
+    b[:n] -= b[:n].mean()
+    b[n:] -= b[n:].mean()
+    x = torch.zeros(2 * n, dtype=torch.float64, device=DEV)
+    infos = amg.amg_pcg_solve(torch.as_tensor(b, device=DEV), x, h, tol=1e-9, max_iter=500)
+    xs = to_np(x).reshape(2, n)
+    exact_h = dt_h == np.float64
+    for r, (hs, v) in enumerate(((hs0, val0), (hs1, val1))):
+        xo = np.zeros(n)
+        ref = O.amg_pcg((row, col, v), hs.native, hs.pinv, b[r * n:(r + 1) * n].copy(), xo, 1e-9, maxit=500,
+                        project=True, hdtype=dt_h)
+        assert ref.converged
+        check_info(infos[r], ref, np.float64, f"env {r}", 0 if exact_h else 2)
+        a, o = xs[r] - xs[r].mean(), xo - xo.mean()
+        assert rel_l2(a, o) < (1e-9 if exact_h else 1e-4), rel_l2(a, o)
+    # the finest-only refresh of the batch keeps its coarse operators
+    A2 = pa.CSRmatrix(torch.as_tensor(np.concatenate([val0, val1 * 1.01]), device=DEV), A.index, A.row)
+    h2 = amg.batched_hierarchy_for(A2, 2, project_constant=True, previous=h, coarse=False, native_dtype=_tdt(dt_h))
+    assert h2.levels[1] is h.levels[1] and h2.coarse_pinv64 is h.coarse_pinv64
+    x2 = torch.zeros(2 * n, dtype=torch.float64, device=DEV)
+    infos2 = amg.amg_pcg_solve(torch.as_tensor(b, device=DEV), x2, h2, tol=1e-9, max_iter=500)
+    assert all(i.converged for i in infos2)
+
+
 @pytest.mark.parametrize("use_bicg", [False, True])
 def test_ell_copy_gives_identical_solves(use_bicg):
     """The domain matrices carry an ELL copy written by the assembly kernels; solves through
