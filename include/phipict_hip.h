@@ -265,6 +265,46 @@ int pp_compute_current_density(const pp_domain *dom, const void *phi, const void
 /* replaces Copy{Velocity,Pressure,Epot}Result{To,From}Blocks (csrc/phipict/piso/result_copy.cu:19-475).
  * field: 0 velocity, 1 pressure, 2 epot; to_blocks: 1 flat->blocks, 0 blocks->flat. */
 int pp_copy_result(const pp_domain *dom, int32_t field, int32_t to_blocks, void *stream);
+/* ---- adjoints of the assembly (orthogonal CENTRAL path; the reference's *Grad bindings,
+ * csrc/phipict/bindings.cpp:781-814). Gradients are ADDED to the output buffers (zero them
+ * first): flat [batch][channels][n] by global cell index; grad_bound (device array of
+ * num_blocks*6 pointers, entry block*6+face, null where unused) points at [batch][d][face
+ * cells] gradients of the fixed faces' velocities; grad_source (or null) is per cell
+ * [batch][d][n]. The descriptor must show the forward's inputs (block velocity, pressure,
+ * boundary data, time step); values the forward read from the domain's flat vectors are
+ * passed explicitly. */
+/* replaces SetupAdvectionMatrixGrad (advection.cu:309-542): d(C values [B][nnz], A [B][n] or
+ * null) -> d(block velocity) */
+int pp_setup_advection_matrix_grad(const pp_domain *dom, const void *grad_c, const void *grad_a, void *grad_u,
+                                   void *stream);
+/* replaces SetupAdvectionVelocityGrad (advection.cu:957-1200, applyPressureGradient off):
+ * d(velocity RHS) -> d(block velocity), d(source), d(fixed-face velocities) */
+int pp_setup_advection_velocity_grad(const pp_domain *dom, const void *grad_rhs, void *grad_u, void *grad_source,
+                                     void *const *grad_bound, void *stream);
+/* replaces SetupPressureMatrixGrad (pressure.cu:195-348): d(P values) -> d(A); a = forward A */
+int pp_setup_pressure_matrix_grad(const pp_domain *dom, const void *a, const void *grad_p, void *grad_a,
+                                  void *stream);
+/* replaces SetupPressureRHSGrad (pressure.cu:524-755): d(HbyA) -> d(A), d(C values),
+ * d(velocity result), d(block velocity), d(source), d(fixed-face velocities); a, c_val,
+ * velocity_result and hbya are the forward's inputs and output */
+int pp_setup_pressure_rhs_grad(const pp_domain *dom, const void *grad_hbya, const void *a, const void *c_val,
+                               const void *velocity_result, const void *hbya, void *grad_a, void *grad_c,
+                               void *grad_velocity_result, void *grad_u, void *grad_source, void *const *grad_bound,
+                               void *stream);
+/* replaces SetupPressureRHSdivGrad (pressure.cu:805-880): d(div) -> d(HbyA), d(fixed-face
+ * velocities) */
+int pp_setup_pressure_rhs_div_grad(const pp_domain *dom, const void *grad_div, int32_t time_step_norm, void *grad_hbya,
+                                   void *const *grad_bound, void *stream);
+/* replaces CorrectVelocityGrad (velocity_correction.cu:300-375, version 1): d(corrected
+ * velocity) -> d(HbyA), d(A), d(pressure [B][n]); the blocks must hold the forward's pressure */
+int pp_correct_velocity_grad(const pp_domain *dom, const void *grad_out, const void *a, int32_t time_step_norm,
+                             void *grad_hbya, void *grad_a, void *grad_pressure, void *stream);
+/* replaces ComputeEpotRHSGrad (potential.cu:980-1040): d(div [B][n]) -> d(in-plane field [B][d][n]) */
+int pp_compute_epot_rhs_grad(const pp_domain *dom, const void *grad_div, void *grad_field, void *stream);
+/* replaces ComputeCurrentDensityFaceBasedGrad (potential.cu:1248-1329): d(J [B][3][n]) ->
+ * d(phi [B][n]), d(u x e_b [B][3][n]) */
+int pp_compute_current_density_grad(const pp_domain *dom, const void *grad_j, void *grad_phi, void *grad_uxb,
+                                    void *stream);
 /* replaces Domain::getMaxVelocity / getMaxVelocityPerEnv / getMaxVelocityMagnitude
  * (csrc/phipict/domain/domain.cpp:212-245) and the Block methods of the same names
  * (csrc/phipict/domain/block.cpp:505-600): block -1 = every block, else that block only;

@@ -170,6 +170,14 @@ SIGNATURES = {
     "pp_compute_current_density": [_P(DomainDesc), _vp, _vp, _vp, _vp],
     "pp_copy_result": [_P(DomainDesc), _i32, _i32, _vp],
     "pp_max_velocity": [_P(DomainDesc), _i32, _i32, _i32, _i32, _vp, _vp],
+    "pp_setup_advection_matrix_grad": [_P(DomainDesc), _vp, _vp, _vp, _vp],
+    "pp_setup_advection_velocity_grad": [_P(DomainDesc), _vp, _vp, _vp, _vp, _vp],
+    "pp_setup_pressure_matrix_grad": [_P(DomainDesc), _vp, _vp, _vp, _vp],
+    "pp_setup_pressure_rhs_grad": [_P(DomainDesc), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "pp_setup_pressure_rhs_div_grad": [_P(DomainDesc), _vp, _i32, _vp, _vp, _vp],
+    "pp_correct_velocity_grad": [_P(DomainDesc), _vp, _vp, _i32, _vp, _vp, _vp, _vp],
+    "pp_compute_epot_rhs_grad": [_P(DomainDesc), _vp, _vp, _vp],
+    "pp_compute_current_density_grad": [_P(DomainDesc), _vp, _vp, _vp, _vp],
     "pp_transform_vectors": [_i32, _i32, _i32, _vp, _vp, _i32, _i32, _vp, _i32, _vp],
     "pp_solve_linear": [_P(CsrDesc), _vp, _vp, _i32, _i32, _i32, _P(_dbl), _i32, _i32, _i32, _i32, _i32, _i32, _i32,
                         _vp, _P(SolveResult)],

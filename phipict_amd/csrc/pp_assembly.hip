@@ -811,6 +811,503 @@ __global__ void __launch_bounds__(256) k_transpose_2d(const T *__restrict__ in, 
     }
 }
 
+// ---- adjoints of the assembly kernels (the reference's _GRAD kernels, advection.cu:309-542,
+// :957-1200, pressure.cu:195-348, :524-755, :805-880, velocity_correction.cu:300-375,
+// potential.cu:980-1329), orthogonal CENTRAL path. Each restates its forward kernel above in
+// reverse: per cell, the gradient of the cell's outputs is pulled back to the inputs the
+// outputs were computed from and added to their gradients. Inputs shared by several cells
+// (neighbour velocities, pressures, the diagonal) take atomic adds; a cell's own entries are
+// written by its work-item alone. Gradient buffers are flat [batch][channels][n] (global cell
+// index), fixed faces' velocity gradients [batch][d][face cells] per (block, face).
+
+// Position of each candidate entry in the row written by emit_row (ascending column, the
+// lower candidate first among equal columns); -1 for candidates that were not written.
+template <int D>
+__device__ __forceinline__ void row_positions(const int *idx, int *pos) {
+#pragma unroll
+    for (int s = 0; s < 2 * D + 1; ++s) {
+        int r = 0;
+#pragma unroll
+        for (int t = 0; t < 2 * D + 1; ++t)
+            if (idx[t] >= 0 && (idx[t] < idx[s] || (idx[t] == idx[s] && t < s))) ++r;
+        pos[s] = idx[s] >= 0 ? r : -1;
+    }
+}
+
+// gradient of the contravariant component w of cell f (global index gi) of a vector field:
+// d(det Minv[w] . v)/dv_k = det Minv[w][k]; gv points at the field's [channels][n] gradient
+template <typename T, int D>
+__device__ __forceinline__ void add_contra_grad(const pp_block_desc &b, int f, int gi, int w, int n, T gval,
+                                                T *__restrict__ gv) {
+    if (!b.has_transform) {
+        atomicAdd(gv + (size_t)w * n + gi, gval);
+        return;
+    }
+    const TView<T> t = Metric<T, D>::at(b, f);
+    const T s = gval * t[2 * D * D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) atomicAdd(gv + (size_t)k * n + gi, s * t[D * D + w * D + k]);
+}
+
+// gradient of contra_fixed (component w at face cell ff) with respect to the face velocity
+template <typename T, int D>
+__device__ __forceinline__ void contra_fixed_grad(const pp_boundary_desc &fb, int ff, int w, T gval, T *gv) {
+    if (!fb.has_transform) {
+        gv[w] += gval;
+        return;
+    }
+    if (fb.vel_type != PP_BC_DIRICHLET) return;
+    const TView<T> t = Metric<T, D>::face(fb, ff);
+    const T s = gval * t[2 * D * D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) gv[k] += s * t[D * D + w * D + k];
+}
+
+// Pull the gradients gfl[bound] of the face fluxes of cell f (cell_fluxes, non-EPOT and EPOT
+// forms) back to the vector field (gfield: [D|3][n] of this environment) and, for the
+// non-EPOT form, to the fixed faces' velocities (gbv: per face, D values for this face cell).
+template <typename T, int D, bool EPOT>
+__device__ void cell_fluxes_grad(const pp_domain &dom, const pp_block_desc *__restrict__ blocks, int bi, const int *p,
+                                 int f, int gi, const T *gfl, T *__restrict__ gfield, T (*gbv)[D]) {
+    const pp_block_desc &b = blocks[bi];
+    const int n = dom.num_cells;
+#pragma unroll
+    for (int bound = 0; bound < 2 * D; ++bound) {
+        const int dim = bound >> 1;
+        const int up = bound & 1;
+        const T gf = gfl[bound];
+        if (at_bound(p, bound, b)) {
+            const pp_boundary_desc &bd = b.bnd[bound];
+            if (bd.type == PP_BOUND_FIXED) {
+                if (EPOT) {
+                    if (!is_insulating_wall(b, bound, p)) add_contra_grad<T, D>(b, f, gi, dim, n, gf, gfield);
+                } else if (gbv) {
+                    contra_fixed_grad<T, D>(bd, face_flat(p, bound, b), dim, gf, gbv[bound]);
+                }
+            } else if (bd.type == PP_BOUND_CONNECTED) {
+                int np[3];
+                conn_pos(blocks, D, p, dim, bd, 0, np);
+                const pp_block_desc &nb = blocks[bd.conn_block];
+                const int nf = flat3(np, nb.size);
+                const T sgn = (bd.conn_axes[0] & 1) == up ? T(-1) : T(1);
+                add_contra_grad<T, D>(nb, nf, nf + nb.global_offset, bd.conn_axes[0] >> 1, n, T(0.5) * gf * sgn, gfield);
+                add_contra_grad<T, D>(b, f, gi, dim, n, T(0.5) * gf, gfield);
+            } else if (bd.type == PP_BOUND_PERIODIC) {
+                int np[3] = {p[0], p[1], p[2]};
+                np[dim] = up ? 0 : b.size[dim] - 1;
+                const int nf = flat3(np, b.size);
+                add_contra_grad<T, D>(b, nf, nf + b.global_offset, dim, n, T(0.5) * gf, gfield);
+                add_contra_grad<T, D>(b, f, gi, dim, n, T(0.5) * gf, gfield);
+            }
+        } else {
+            int np[3] = {p[0], p[1], p[2]};
+            np[dim] += face_sign(bound);
+            const int nf = flat3(np, b.size);
+            add_contra_grad<T, D>(b, nf, nf + b.global_offset, dim, n, T(0.5) * gf, gfield);
+            add_contra_grad<T, D>(b, f, gi, dim, n, T(0.5) * gf, gfield);
+        }
+    }
+}
+
+// the candidate columns of row g of C / P (as the forward kernels build them)
+template <typename T, int D>
+__device__ __forceinline__ void row_columns(const pp_block_desc *__restrict__ blocks, int bi, const int *p, int g,
+                                            int *idx) {
+    const pp_block_desc &b = blocks[bi];
+    idx[0] = g;
+#pragma unroll
+    for (int bound = 0; bound < 2 * D; ++bound) {
+        if (!at_bound(p, bound, b) || !is_fixed(b, bound)) {
+            int np[3], nw;
+            const int nbi = neighbor(blocks, bi, D, p, bound, np, &nw);
+            const pp_block_desc &nb = blocks[nbi];
+            idx[bound + 1] = flat3(np, nb.size) + nb.global_offset;
+        } else {
+            idx[bound + 1] = -1;
+        }
+    }
+}
+
+// store the fixed faces' velocity gradients of one cell (each face cell borders one cell)
+template <typename T, int D>
+__device__ __forceinline__ void store_face_grads(const pp_block_desc &b, const int *p, int bi, int env,
+                                                 T *const *__restrict__ gbnd, T (*gbv)[D]) {
+#pragma unroll
+    for (int bound = 0; bound < 2 * D; ++bound) {
+        if (!at_bound(p, bound, b) || !is_fixed(b, bound)) continue;
+        T *gb = gbnd[bi * 6 + bound];
+        if (!gb) continue;
+        const int fc = face_cells(b, bound), ff = face_flat(p, bound, b);
+#pragma unroll
+        for (int k = 0; k < D; ++k) gb[((size_t)env * D + k) * fc + ff] += gbv[bound][k];
+    }
+}
+
+// adjoint of the momentum boundary source (boundary_source / the loop in k_advection_rhs)
+// scaled by q: adds to gbv[bound][k] for every fixed face of the cell
+template <typename T, int D>
+__device__ __forceinline__ void boundary_source_grad(const pp_domain &dom, const pp_block_desc &b, const int *p, int f,
+                                                     int dim, int env, T q, T (*gbv)[D]) {
+#pragma unroll
+    for (int bound = 0; bound < 2 * D; ++bound) {
+        if (!at_bound(p, bound, b)) continue;
+        const pp_boundary_desc &fb = b.bnd[bound];
+        if (fb.type != PP_BOUND_FIXED) continue;
+        const T fn = T((bound & 1) * 2 - 1);
+        const int fc = face_cells(b, bound), ff = face_flat(p, bound, b);
+        const T vel = fixed_vel<T>(fb, D, fc, ff, dim, env);
+        const T alpha = alpha_fixed<T, D>(fb, ff, bound >> 1);
+        const T flux = contra_fixed<T, D>(fb, fc, ff, bound >> 1, env) * fn;
+        const T bv = visc_at<T>(dom, b, f);
+        const T slip = fb.vel_type == PP_BC_DIRICHLET ? T(0) : T(1);
+        // s += -vel * flux + vel * (1 - slip) * bv * 2 * alpha
+        gbv[bound][dim] += q * ((1 - slip) * bv * 2 * alpha - flux);
+        contra_fixed_grad<T, D>(fb, ff, bound >> 1, -q * vel * fn, gbv[bound]);
+    }
+}
+
+// SetupAdvectionMatrixEulerImplicit_GRAD (advection.cu:309-542): gC [B][nnz], gA [B][n] (or
+// null) -> gU [B][D][n] (the block velocity; C depends on it through the face fluxes only)
+template <typename T, int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_advection_matrix_grad(pp_domain dom, const T *__restrict__ gC,
+                                                                     const T *__restrict__ gA, T *__restrict__ gU) {
+    CELL_PROLOGUE
+    const T det = Metric<T, D>::det(b, f);
+    int idx[2 * D + 1], pos[2 * D + 1];
+    row_columns<T, D>(blocks, bi, p, g, idx);
+    row_positions<D>(idx, pos);
+    const T *gc = gC + (size_t)env * dom.C.nnz + dom.C.row[g];
+    T grow[2 * D + 1];
+#pragma unroll
+    for (int s = 0; s < 2 * D + 1; ++s) grow[s] = pos[s] >= 0 ? gc[pos[s]] / det : T(0);
+    if (gA) grow[0] += gA[(size_t)env * dom.num_cells + g] / det;
+    T gfl[2 * D];
+#pragma unroll
+    for (int bound = 0; bound < 2 * D; ++bound)
+        gfl[bound] = idx[bound + 1] >= 0 ? T(0.5) * T(face_sign(bound)) * (grow[0] + grow[bound + 1]) : T(0);
+    cell_fluxes_grad<T, D, false>(dom, blocks, bi, p, f, g, gfl, gU + (size_t)env * D * dom.num_cells, nullptr);
+}
+
+// SetupAdvectionVelocityEulerImplicitRHS_GRAD (advection.cu:957-1200), without the pressure
+// gradient term: gR [B][D][n] -> gU (block velocity), gS (per-cell source, or null), fixed
+// faces' velocities (gbnd, per block and face, or null entries)
+template <typename T, int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_advection_rhs_grad(pp_domain dom, const T *__restrict__ gR,
+                                                                  T *__restrict__ gU, T *__restrict__ gS,
+                                                                  T *const *__restrict__ gbnd) {
+    CELL_PROLOGUE
+    const int n = dom.num_cells;
+    const T dt = reinterpret_cast<const T *>(dom.time_step)[env];
+    const T det = Metric<T, D>::det(b, f);
+    T gbv[2 * D][D];
+#pragma unroll
+    for (int i = 0; i < 2 * D; ++i)
+#pragma unroll
+        for (int k = 0; k < D; ++k) gbv[i][k] = T(0);
+#pragma unroll
+    for (int dim = 0; dim < D; ++dim) {
+        const size_t o = ((size_t)env * D + dim) * n + g;
+        const T gv = gR[o];
+        gU[o] += gv * det / dt / det;
+        if (gS) gS[o] += gv;
+        boundary_source_grad<T, D>(dom, b, p, f, dim, env, gv / det, gbv);
+    }
+    if (gbnd) store_face_grads<T, D>(b, p, bi, env, gbnd, gbv);
+}
+
+// SetupPressureMatrix_GRAD (pressure.cu:195-348): gP [B][nnz] -> gA [B][n] (A: the diagonal
+// the matrix was built from)
+template <typename T, int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_pressure_matrix_grad(pp_domain dom, const T *__restrict__ Aval,
+                                                                    const T *__restrict__ gP, T *__restrict__ gA) {
+    CELL_PROLOGUE
+    const T *A = Aval + (size_t)env * dom.num_cells;
+    T *ga = gA + (size_t)env * dom.num_cells;
+    int idx[2 * D + 1], pos[2 * D + 1];
+    row_columns<T, D>(blocks, bi, p, g, idx);
+    row_positions<D>(idx, pos);
+    const T *gp = gP + (size_t)env * dom.P.nnz + dom.P.row[g];
+    const T raP = T(1.0) / A[g];
+    T gaP = T(0);
+#pragma unroll
+    for (int bound = 0; bound < 2 * D; ++bound) {
+        if (idx[bound + 1] < 0) continue;
+        const int dim = bound >> 1;
+        int np[3], nw;
+        const int nbi = neighbor(blocks, bi, D, p, bound, np, &nw);
+        const pp_block_desc &nb = blocks[nbi];
+        const int nf = flat3(np, nb.size);
+        const T an = Metric<T, D>::alpha(nb, nf, nw);
+        const int ng = nf + nb.global_offset;
+        const T raN = T(1.0) / A[ng];
+        // coef = 0.5 (ap raP + an raN); diagonal -= coef, entry += coef
+        const T gcoef = gp[pos[bound + 1]] - gp[pos[0]];
+        gaP -= gcoef * T(0.5) * Metric<T, D>::alpha(b, f, dim) * raP * raP;
+        atomicAdd(ga + ng, -gcoef * T(0.5) * an * raN * raN);
+    }
+    atomicAdd(ga + g, gaP);
+}
+
+// SetupPressureRHS_GRAD (pressure.cu:524-755): gH [B][D][n] (gradient of HbyA) -> gA, gC
+// (off-diagonal entries of C), gV (the predicted velocity), gU (block velocity), gS
+// (per-cell source or null), fixed faces' velocities. Aval, Cval, V and H are the forward
+// inputs and output.
+template <typename T, int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_pressure_rhs_grad(pp_domain dom, const T *__restrict__ gH,
+                                                                 const T *__restrict__ Aval, const T *__restrict__ Cval,
+                                                                 const T *__restrict__ V, const T *__restrict__ H,
+                                                                 T *__restrict__ gA, T *__restrict__ gC,
+                                                                 T *__restrict__ gV, T *__restrict__ gU,
+                                                                 T *__restrict__ gS, T *const *__restrict__ gbnd) {
+    CELL_PROLOGUE
+    const int n = dom.num_cells;
+    const T dt = reinterpret_cast<const T *>(dom.time_step)[env];
+    const T A = Aval[(size_t)env * n + g];
+    const T det = Metric<T, D>::det(b, f);
+    T q[D];
+    T ga = T(0);
+    T gbv[2 * D][D];
+#pragma unroll
+    for (int i = 0; i < 2 * D; ++i)
+#pragma unroll
+        for (int k = 0; k < D; ++k) gbv[i][k] = T(0);
+#pragma unroll
+    for (int dim = 0; dim < D; ++dim) {
+        const size_t o = ((size_t)env * D + dim) * n + g;
+        // HbyA = (u/dt - H + S) / A
+        q[dim] = gH[o] / A;
+        ga -= gH[o] * H[o] / A;
+        gU[o] += q[dim] / dt;
+        if (gS) gS[o] += q[dim];
+        boundary_source_grad<T, D>(dom, b, p, f, dim, env, q[dim] / det, gbv);
+    }
+    atomicAdd(gA + (size_t)env * n + g, ga);
+    const int32_t cs = dom.C.row[g];
+    const int rs = dom.C.row[g + 1] - cs;
+    const T *cv = Cval + (size_t)env * dom.C.nnz + cs;
+    T *gc = gC + (size_t)env * dom.C.nnz + cs;
+    const T *v = V + (size_t)env * D * n;
+    T *gv = gV + (size_t)env * D * n;
+    for (int i = 0; i < rs && i < 2 * D + 1; ++i) {
+        const int ix = dom.C.col[cs + i];
+        if (ix == g) continue;
+        T s = T(0);
+#pragma unroll
+        for (int dim = 0; dim < D; ++dim) {
+            s -= q[dim] * v[ix + (size_t)n * dim];
+            atomicAdd(gv + ix + (size_t)n * dim, -cv[i] * q[dim]);
+        }
+        gc[i] += s;
+    }
+    if (gbnd) store_face_grads<T, D>(b, p, bi, env, gbnd, gbv);
+}
+
+// SetupPressureRHSdiv_GRAD (pressure.cu:805-880): gDiv [B][n] -> gH [B][D][n] (HbyA) and the
+// fixed faces' velocities
+template <typename T, int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_pressure_div_grad(pp_domain dom, const T *__restrict__ gDiv,
+                                                                 int time_step_norm, T *__restrict__ gH,
+                                                                 T *const *__restrict__ gbnd) {
+    CELL_PROLOGUE
+    T gd = gDiv[(size_t)env * dom.num_cells + g];
+    if (time_step_norm) gd /= reinterpret_cast<const T *>(dom.time_step)[env];
+    T gfl[2 * D];
+#pragma unroll
+    for (int bound = 0; bound < 2 * D; ++bound) gfl[bound] = T(face_sign(bound)) * gd;
+    T gbv[2 * D][D];
+#pragma unroll
+    for (int i = 0; i < 2 * D; ++i)
+#pragma unroll
+        for (int k = 0; k < D; ++k) gbv[i][k] = T(0);
+    cell_fluxes_grad<T, D, false>(dom, blocks, bi, p, f, g, gfl, gH + (size_t)env * D * dom.num_cells, gbv);
+    if (gbnd) store_face_grads<T, D>(b, p, bi, env, gbnd, gbv);
+}
+
+// global index of the cell pressure_at reads for a position that may lie one cell outside
+// the block; -1 where it reads zero
+__device__ int pressure_at_index(const pp_domain &dom, const pp_block_desc *__restrict__ blocks, int bi,
+                                 const int *pin) {
+    const pp_block_desc &b = blocks[bi];
+    int t[3] = {pin[0], pin[1], pin[2]};
+    const int D = dom.dims;
+    for (int dim = 0; dim < D; ++dim) {
+        const bool lower = pin[dim] < 0;
+        const bool upper = pin[dim] >= b.size[dim];
+        if (!lower && !upper) continue;
+        const pp_boundary_desc &bd = b.bnd[dim * 2 + (upper ? 1 : 0)];
+        if (bd.type == PP_BOUND_FIXED) {
+            t[dim] = lower ? 0 : b.size[dim] - 1;
+            return b.global_offset + flat3(t, b.size);
+        }
+        if (bd.type == PP_BOUND_CONNECTED) {
+            int op[3];
+            conn_pos(blocks, D, t, dim, bd, lower ? -(pin[dim] + 1) : pin[dim] - b.size[dim], op);
+            const pp_block_desc &nb = blocks[bd.conn_block];
+            return nb.global_offset + flat3(op, nb.size);
+        }
+        if (bd.type == PP_BOUND_PERIODIC) {
+            t[dim] += lower ? b.size[dim] : -b.size[dim];
+            return b.global_offset + flat3(t, b.size);
+        }
+        return -1;
+    }
+    return b.global_offset + flat3(pin, b.size);
+}
+
+// CorrectVelocity_GRAD (velocity_correction.cu:300-375), version 1 (FD): gOut [B][D][n] ->
+// gH (HbyA), gA, gP [B][n] (the pressure the gradient was taken of, global cell index)
+template <typename T, int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_correct_velocity_grad(pp_domain dom, const T *__restrict__ gOut,
+                                                                     const T *__restrict__ Aval, int time_step_norm,
+                                                                     T *__restrict__ gH, T *__restrict__ gA,
+                                                                     T *__restrict__ gP) {
+    CELL_PROLOGUE
+    const int n = dom.num_cells;
+    const T A = Aval[(size_t)env * n + g];
+    const T rDiag = 1 / A;
+    T gr[D];
+    pressure_grad<T, D>(dom, blocks, bi, p, f, env, gr);
+    const T s = time_step_norm ? reinterpret_cast<const T *>(dom.time_step)[env] : T(1);
+    T ggr[D];
+    T ga = T(0);
+#pragma unroll
+    for (int dim = 0; dim < D; ++dim) {
+        const size_t o = ((size_t)env * D + dim) * n + g;
+        const T go = gOut[o];
+        gH[o] += go;
+        // u = -rDiag * gr * s + HbyA
+        ga += go * gr[dim] * s * rDiag * rDiag;
+        ggr[dim] = -rDiag * go * s;
+    }
+    atomicAdd(gA + (size_t)env * n + g, ga);
+    // gr_j = sum_i Minv[i][j] raw_i  ->  graw_i = sum_j Minv[i][j] ggr_j
+    T graw[D];
+    if (b.has_transform) {
+        const TView<T> t = Metric<T, D>::at(b, f);
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            T acc = T(0);
+#pragma unroll
+            for (int j = 0; j < D; ++j) acc += t[D * D + i * D + j] * ggr[j];
+            graw[i] = acc;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < D; ++i) graw[i] = ggr[i];
+    }
+    T *gp = gP + (size_t)env * n;
+#pragma unroll
+    for (int dim = 0; dim < D; ++dim) {
+        int t[3] = {p[0], p[1], p[2]};
+        T fac = T(0.5);
+        if (!(p[dim] == 0 && is_fixed(b, dim * 2))) t[dim] = p[dim] - 1;
+        else fac = T(1);
+        const int in_ = pressure_at_index(dom, blocks, bi, t);
+        if (!(p[dim] == b.size[dim] - 1 && is_fixed(b, dim * 2 + 1))) t[dim] = p[dim] + 1;
+        else { t[dim] = p[dim]; fac = T(1); }
+        const int ip = pressure_at_index(dom, blocks, bi, t);
+        if (ip >= 0) atomicAdd(gp + ip, graw[dim] * fac);
+        if (in_ >= 0) atomicAdd(gp + in_, -graw[dim] * fac);
+    }
+}
+
+// ComputeEpotRHSGrad (potential.cu:980-1040): gDiv [B][n] -> gF [B][D][n], the in-plane
+// vector field whose face fluxes the divergence was taken of
+template <typename T, int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_epot_rhs_grad(pp_domain dom, const T *__restrict__ gDiv,
+                                                             T *__restrict__ gF) {
+    CELL_PROLOGUE
+    const T gd = gDiv[(size_t)env * dom.num_cells + g];
+    T gfl[2 * D];
+#pragma unroll
+    for (int bound = 0; bound < 2 * D; ++bound) gfl[bound] = T(face_sign(bound)) * gd;
+    cell_fluxes_grad<T, D, true>(dom, blocks, bi, p, f, g, gfl, gF + (size_t)env * D * dom.num_cells, nullptr);
+}
+
+// ComputeCurrentDensityFaceBasedGrad (potential.cu:1248-1329): gJ [B][3][n] -> gPhi [B][n],
+// gX [B][3][n] (u x e_b)
+template <typename T, int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_current_density_grad(pp_domain dom, const T *__restrict__ gJall,
+                                                                    T *__restrict__ gPhiAll, T *__restrict__ gXall) {
+    CELL_PROLOGUE
+    const int nc = dom.num_cells;
+    const T *gJ = gJall + (size_t)env * 3 * nc;
+    T *gphi = gPhiAll + (size_t)env * nc;
+    T *gx = gXall + (size_t)env * 3 * nc;
+    const T gJv[3] = {gJ[g], gJ[g + nc], gJ[g + (size_t)nc * 2]};
+    T gja[D];
+    if (b.has_transform) {
+        const TView<T> t = Metric<T, D>::at(b, f);
+        const T inv = T(1) / t[2 * D * D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            T acc = T(0);
+#pragma unroll
+            for (int i = 0; i < D; ++i) acc += t[i * D + k] * gJv[i];
+            gja[k] = acc * inv;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < D; ++k) gja[k] = gJv[k];
+    }
+    if constexpr (D == 2) atomicAdd(gx + g + (size_t)nc * 2, gJv[2]);
+    T ap[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) ap[k] = Metric<T, D>::alpha(b, f, k);
+    T gphiP = T(0);
+#pragma unroll
+    for (int bound = 0; bound < 2 * D; ++bound) {
+        const int dim = bound >> 1;
+        const int up = bound & 1;
+        const T fs = T(face_sign(bound));
+        const T gjf = T(0.5) * gja[dim];
+        const bool atb = at_bound(p, bound, b);
+        if (atb && is_fixed(b, bound)) {
+            if (is_insulating_wall(b, bound, p)) continue;
+            if (is_epot_dirichlet(b, bound, p)) gphiP += ap[dim] * fs * gjf;
+            add_contra_grad<T, D>(b, f, g, dim, nc, gjf, gx);
+            continue;
+        }
+        int ng, nf, ca;
+        const pp_block_desc *nbp;
+        T sgn = T(1);
+        if (atb) {
+            const pp_boundary_desc &bd = b.bnd[bound];
+            if (bd.type == PP_BOUND_CONNECTED) {
+                int np[3];
+                conn_pos(blocks, D, p, dim, bd, 0, np);
+                ca = conn_channel(D, dim, dim, bd);
+                nbp = &blocks[bd.conn_block];
+                nf = flat3(np, nbp->size);
+                if ((bd.conn_axes[0] & 1) == up) sgn = T(-1);
+            } else if (bd.type == PP_BOUND_PERIODIC) {
+                int np[3] = {p[0], p[1], p[2]};
+                np[dim] = up ? 0 : b.size[dim] - 1;
+                nbp = &b;
+                nf = flat3(np, b.size);
+                ca = dim;
+            } else {
+                continue;
+            }
+        } else {
+            int np[3] = {p[0], p[1], p[2]};
+            np[dim] += face_sign(bound);
+            nbp = &b;
+            nf = flat3(np, b.size);
+            ca = dim;
+        }
+        ng = nf + nbp->global_offset;
+        const T aN = Metric<T, D>::alpha(*nbp, nf, ca);
+        const T coef = T(0.5) * (ap[dim] + aN);
+        // jf = -coef fs (phiN - phiP) + 0.5 (cvp + sgn cvN)
+        atomicAdd(gphi + ng, -coef * fs * gjf);
+        gphiP += coef * fs * gjf;
+        add_contra_grad<T, D>(b, f, g, dim, nc, T(0.5) * gjf, gx);
+        add_contra_grad<T, D>(*nbp, nf, ng, ca, nc, T(0.5) * gjf * sgn, gx);
+    }
+    atomicAdd(gphi + g, gphiP);
+}
+
 // ---- maximum velocity (CFL substeps) and vector transforms
 
 // Unsigned integers that order non-negative values (and one canonical NaN above +inf) like
@@ -1115,6 +1612,96 @@ int pp_copy_result(const pp_domain *dom, int32_t field, int32_t to_blocks, void 
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
+
+#define PP_GRAD_LAUNCH(KERNEL, ...)                                                                              \
+    PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,                                                            \
+        hipLaunchKernelGGL((KERNEL<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), \
+                           *dom, __VA_ARGS__)))
+#define PP_T(X) reinterpret_cast<T *>(X)
+#define PP_CT(X) reinterpret_cast<const T *>(X)
+
+int pp_setup_advection_matrix_grad(const pp_domain *dom, const void *grad_c, const void *grad_a, void *grad_u,
+                                   void *stream) {
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(grad_c && grad_u && dom->C.row, "grad_c, grad_u and the C pattern must be set");
+    PP_GRAD_LAUNCH(k_advection_matrix_grad, PP_CT(grad_c), PP_CT(grad_a), PP_T(grad_u));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_setup_advection_velocity_grad(const pp_domain *dom, const void *grad_rhs, void *grad_u, void *grad_source,
+                                     void *const *grad_bound, void *stream) {
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(grad_rhs && grad_u, "grad_rhs and grad_u must be set");
+    PP_GRAD_LAUNCH(k_advection_rhs_grad, PP_CT(grad_rhs), PP_T(grad_u), PP_T(grad_source),
+                   reinterpret_cast<T *const *>(grad_bound));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_setup_pressure_matrix_grad(const pp_domain *dom, const void *a, const void *grad_p, void *grad_a,
+                                  void *stream) {
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(a && grad_p && grad_a && dom->P.row, "a, grad_p, grad_a and the P pattern must be set");
+    PP_GRAD_LAUNCH(k_pressure_matrix_grad, PP_CT(a), PP_CT(grad_p), PP_T(grad_a));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_setup_pressure_rhs_grad(const pp_domain *dom, const void *grad_hbya, const void *a, const void *c_val,
+                               const void *velocity_result, const void *hbya, void *grad_a, void *grad_c,
+                               void *grad_velocity_result, void *grad_u, void *grad_source, void *const *grad_bound,
+                               void *stream) {
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(grad_hbya && a && c_val && velocity_result && hbya && grad_a && grad_c && grad_velocity_result &&
+                     grad_u && dom->C.row,
+                 "null argument");
+    PP_GRAD_LAUNCH(k_pressure_rhs_grad, PP_CT(grad_hbya), PP_CT(a), PP_CT(c_val), PP_CT(velocity_result), PP_CT(hbya),
+                   PP_T(grad_a), PP_T(grad_c), PP_T(grad_velocity_result), PP_T(grad_u), PP_T(grad_source),
+                   reinterpret_cast<T *const *>(grad_bound));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_setup_pressure_rhs_div_grad(const pp_domain *dom, const void *grad_div, int32_t time_step_norm, void *grad_hbya,
+                                   void *const *grad_bound, void *stream) {
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(grad_div && grad_hbya, "grad_div and grad_hbya must be set");
+    PP_GRAD_LAUNCH(k_pressure_div_grad, PP_CT(grad_div), (int)time_step_norm, PP_T(grad_hbya),
+                   reinterpret_cast<T *const *>(grad_bound));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_correct_velocity_grad(const pp_domain *dom, const void *grad_out, const void *a, int32_t time_step_norm,
+                             void *grad_hbya, void *grad_a, void *grad_pressure, void *stream) {
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(grad_out && a && grad_hbya && grad_a && grad_pressure, "null argument");
+    PP_GRAD_LAUNCH(k_correct_velocity_grad, PP_CT(grad_out), PP_CT(a), (int)time_step_norm, PP_T(grad_hbya),
+                   PP_T(grad_a), PP_T(grad_pressure));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_compute_epot_rhs_grad(const pp_domain *dom, const void *grad_div, void *grad_field, void *stream) {
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(grad_div && grad_field, "null argument");
+    PP_GRAD_LAUNCH(k_epot_rhs_grad, PP_CT(grad_div), PP_T(grad_field));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_compute_current_density_grad(const pp_domain *dom, const void *grad_j, void *grad_phi, void *grad_uxb,
+                                    void *stream) {
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(grad_j && grad_phi && grad_uxb, "null argument");
+    PP_GRAD_LAUNCH(k_current_density_grad, PP_CT(grad_j), PP_T(grad_phi), PP_T(grad_uxb));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+#undef PP_GRAD_LAUNCH
+#undef PP_T
+#undef PP_CT
 
 int pp_max_velocity(const pp_domain *dom, int32_t block, int32_t with_bounds, int32_t computational, int32_t norm,
                     void *out, void *stream) {

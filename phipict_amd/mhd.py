@@ -110,6 +110,9 @@ class MHDSimulation(Simulation):
 
     def _add_lorentz_force(self, domain: Domain, **kwargs: Any) -> None:
         """F_L = N (J x e_b) added to the velocity source of every block (mhd.py:543-594)."""
+        if self.differentiable:
+            self._add_lorentz_force_diff(domain)
+            return
         if self._potential_stale:
             self._magnetic_step_callback(domain)
             self._potential_stale = False
@@ -162,8 +165,66 @@ class MHDSimulation(Simulation):
         uses (mhd.py:864-875)."""
         return [*super()._env_state_extras(), lambda: self._u_cross_eb_flat]
 
+    # ---- the differentiable magnetic step: u x e_b, the potential's right-hand side and solve,
+    # the current density and the Lorentz force as tracked operators (piso_diff.py:2454-2740)
+    def _diff_state(self, domain: Domain):
+        """(phi, u x e_b) the next Lorentz force uses, as tracked tensors."""
+        phi = self.__dict__.get("_diff_phi")
+        if phi is None:
+            phi = domain.epotResult.detach().clone()
+        return phi, self._u_cross_eb_flat
+
+    def _add_lorentz_force_diff(self, domain: Domain) -> None:
+        from . import adjoint as adj
+
+        phi, uxb = self._diff_state(domain)
+        J = adj.CurrentDensityFunction.apply(domain, phi, uxb)
+        e_b = self._e_b.to(device=domain.getDevice())
+        N = self._stuart_number.to(device=domain.getDevice(), dtype=domain.getDtype())
+        srcs = [b.velocitySource for b in domain.getBlocks()]
+        for block, F in zip(domain.getBlocks(), adj.lorentz_sources(domain, J, e_b, N, srcs)):
+            block.setVelocitySource(F)
+        domain.UpdateDomainData()
+
+    def _magnetic_step_diff(self, domain: Domain, max_iter: Optional[int] = None) -> bool:
+        from . import adjoint as adj
+
+        total, batch = domain.getTotalSize(), domain.getBatchSize()
+        U = self.__dict__.get("_diff_velocity_result")
+        if U is None:
+            U = domain.velocityResult
+        e_b = self._e_b.to(device=domain.getDevice())
+        uxb = adj.CrossEbFunction.apply(U, e_b, total, self._dims)
+        inplane = uxb.reshape(batch, 3, total)[:, : self._dims].reshape(-1)
+        rhs = adj.EpotRHSFunction.apply(domain, inplane)
+        singular = self._potential_rank_deficient
+        if singular:
+            rhs = adj._project(rhs, batch)
+        hier = self._get_epot_amg_hierarchy(domain.Epot) if self._potential_use_preconditioner else None
+
+        def solve(b):
+            x, _ = self.linear_solve(domain.Epot, b, x=torch.zeros_like(b), use_BiCG=False, tol=self._potential_tol,
+                                     matrix_rank_deficient=singular, return_best_result=False,
+                                     max_iter=self._potential_solve_max_iter if max_iter is None else max_iter,
+                                     amg_hierarchy=hier, tag="epot")
+            return x
+
+        phi = adj.LinearSolveFunction.apply(solve, domain.Epot, domain.Epot.value, rhs, 1, singular, True,
+                                            self.adjoint_rtol)
+        if self._potential_normalize and singular:
+            phi = adj._project(phi, batch)
+        self._u_cross_eb_flat = uxb
+        self._diff_phi = phi
+        domain.setEpotRHS(rhs.detach().clone())
+        domain.setEpotResult(phi.detach().clone())
+        domain.UpdateDomainData()
+        ops.CopyEpotResultToBlocks(domain)
+        return True
+
     def _magnetic_step_callback(self, domain: Domain, time_step=None, max_iter: Optional[int] = None, **kwargs) -> bool:
         """u x e_b, Poisson solve for phi, copy to blocks (mhd.py:877-1002)."""
+        if self.differentiable:
+            return self._magnetic_step_diff(domain, max_iter)
         total, batch = domain.getTotalSize(), domain.getBatchSize()
         self._u_cross_eb_flat = compute_u_cross_eb_flat(domain.velocityResult, self._e_b_device, total, self._dims)
         inplane = self._u_cross_eb_flat.reshape(batch, 3, total)[:, : self._dims].reshape(-1).contiguous()

@@ -249,8 +249,10 @@ class Simulation:
             raise TypeError(f"unexpected argument {k!r}")
         if hooks is not None and prep_fn is not None:
             raise ValueError("Pass hooks or its former name prep_fn, not both.")
-        if differentiable:
-            raise NotImplementedError("the differentiable (adjoint) PISO step is not implemented")
+        if differentiable and non_orthogonal:
+            raise NotImplementedError("the differentiable step covers the orthogonal path (non_orthogonal=False)")
+        if differentiable and velocity_corrector != "FD":
+            raise NotImplementedError("the differentiable step uses the FD velocity corrector")
         self.domain = domain
         self.hooks = Hooks.coerce(hooks if hooks is not None else prep_fn)
         self.time_step = float(time_step if time_step is not None else dt)
@@ -259,7 +261,9 @@ class Simulation:
         self.adaptive_CFL_per_env = bool(adaptive_CFL_per_env)
         self.print_adaptive_step_info = False
         self.stop_fn = stop_fn
-        self.differentiable = False
+        self.differentiable = bool(differentiable)
+        # relative tolerance of the adjoint (transposed) solves, against the upstream gradient
+        self.adjoint_rtol = 1e-12
         self.corrector_steps = int(corrector_steps)
         self.advection_use_BiCG = advection_use_BiCG
         self.pressure_use_BiCG = pressure_use_BiCG
@@ -486,7 +490,7 @@ class Simulation:
         """One step: flux-balance check, then ``substeps`` PISO substeps, adaptive CFL substeps
         (-1) or substeps fixed from the current velocity (-2) (simulation.py:262-333)."""
         balance = self.domain.GetBoundaryFluxBalance()
-        if float(balance.abs().max()) > self.flux_balance_tol:
+        if float(balance.detach().abs().max()) > self.flux_balance_tol:
             raise RuntimeError(f"Domain boundary fluxes not balanced, cannot proceed with simulation step. "
                                f"Flux balance: {balance}, flux_balance_tol: {self.flux_balance_tol}")
         substeps = self.substeps
@@ -505,6 +509,8 @@ class Simulation:
         try:
             if adaptive:
                 return self._PISO_adaptive_step()
+            if self.differentiable:
+                return self._PISO_split_step_diff(substeps, time_step=time_step)
             return self._PISO_split_step(substeps, time_step=time_step)
         except LinsolveError:
             _LOG.exception("Simulation failed in step (total step %d):", self.total_step)
@@ -608,6 +614,75 @@ class Simulation:
                 break
             self.end_step(time_step)
         return solve_ok
+
+    # ---- the differentiable step (piso_diff.py; piso_simulation.py:3525-4169 with
+    # differentiable operators)
+    def _diff_solver(self, csr, use_bicg: bool, tol, tag: str, amg_hierarchy=None, rank_deficient: bool = False):
+        def solve(b):
+            x, _ = self.linear_solve(csr, b, x=torch.zeros_like(b), use_BiCG=use_bicg, tol=tol,
+                                     matrix_rank_deficient=rank_deficient, amg_hierarchy=amg_hierarchy, tag=tag)
+            return x
+        return solve
+
+    def _PISO_split_step_diff(self, iterations: int, time_step: Optional[torch.Tensor] = None) -> bool:
+        """The orthogonal PISO step on autograd-tracked domain tensors: the blocks'
+        velocities, the fixed faces' velocities and the velocity sources are inputs of
+        differentiable operators whose backward runs the library's adjoint kernels and
+        transposed solves. The new block velocities and pressures carry the graph."""
+        from . import adjoint as adj
+
+        domain = self.domain
+        if domain is None or not domain.IsInitialized():
+            raise RuntimeError("the simulation needs a prepared domain (domain.PrepareSolve())")
+        if time_step is None:
+            time_step = self._time_step_tensor()
+        d, B = domain.getSpatialDims(), domain.getBatchSize()
+        tsn = bool(self.pressure_time_step_normalized)
+        for step in range(int(iterations)):
+            self._run_hook(Hook.PRE, local_step=step, time_step=time_step)
+            domain.UpdateDomainData()
+            self._run_hook(Hook.PRE_VELOCITY_SETUP, local_step=step, time_step=time_step)
+            blocks = domain.getBlocks()
+            vels = [b.velocity for b in blocks]
+            bvels = [bd.velocity for _, _, bd in adj.fixed_faces(domain)]
+            srcs = [b.velocitySource for b in blocks]
+            nv, nb = len(vels), len(bvels)
+            A, Cv = adj.AdvectionMatrixFunction.apply(domain, time_step, *vels)
+            domain._a_version += 1
+            rhs = adj.AdvectionRHSFunction.apply(domain, time_step, nv, nb, *vels, *bvels, *srcs)
+            self._run_hook(Hook.POST_VELOCITY_SETUP, no_step=0, local_step=step, time_step=time_step)
+            solve_v = self._diff_solver(domain.C, self.advection_use_BiCG, self.advection_tol, "velocity")
+            U = adj.LinearSolveFunction.apply(solve_v, domain.C, Cv, rhs, d, False, False, self.adjoint_rtol)
+            self._run_hook(Hook.POST_PREDICTION, local_step=step, time_step=time_step)
+            p = None
+            for cstep in range(self.corrector_steps):
+                domain.setVelocityResult(U.detach().clone())
+                domain.setA(A.detach().clone())
+                domain.UpdateDomainData()
+                Pv = adj.PressureMatrixFunction.apply(domain, time_step, A)
+                H, div = adj.PressureRHSFunction.apply(domain, time_step, tsn, A, Cv, U, nv, nb, *vels, *bvels,
+                                                       *srcs)
+                self._run_hook(Hook.POST_PRESSURE_SETUP, local_step=step, time_step=time_step)
+                singular = self._pressure_rank_deficient(domain.P)
+                solve_p = self._diff_solver(domain.P, self.pressure_use_BiCG,
+                                            self._pressure_tol_for(cstep, 0, self.corrector_steps), "pressure",
+                                            amg_hierarchy=self._get_pressure_amg_hierarchy(domain.P))
+                p = adj.LinearSolveFunction.apply(solve_p, domain.P, Pv, div, 1, singular, True, self.adjoint_rtol)
+                if self.normalize_pressure_result:
+                    p = adj._project(p, B)
+                self._run_hook(Hook.POST_PRESSURE_RESULT, local_step=step, time_step=time_step)
+                U = adj.CorrectVelocityFunction.apply(domain, time_step, tsn, H, A, p)
+                self._run_hook(Hook.POST_VELOCITY_CORRECTION, local_step=step, time_step=time_step)
+            for b, v, pr in zip(blocks, adj.flat_to_blocks(domain, U, d), adj.flat_to_blocks(domain, p, 1)):
+                b.velocity, b.pressure = v, pr
+            domain._changed = True
+            domain.setVelocityResult(U.detach().clone())
+            domain.setPressureResult(p.detach().clone())
+            domain.UpdateDomainData()
+            self._diff_velocity_result = U
+            self._run_hook(Hook.POST, local_step=step, time_step=time_step)
+            self.end_step(time_step)
+        return True
 
     # ---- adaptive CFL substeps
     def _PISO_adaptive_step(self, CFL_cond: Optional[float] = None, max_subsetps: int = 1000) -> bool:

@@ -1,0 +1,170 @@
+"""The differentiable step against central finite differences of the forward step on the GPU
+(FP64): directional derivatives of a loss of the velocity and pressure after two PISO + MHD
+steps with respect to the initial velocity, the inflow profile, the velocity source, the field
+direction e_b and the Stuart number agree to 1e-5 relative (the reference checks its adjoint
+the same way, test_potential_bc.py:605-716, runscripts/validation/gradcheck.py:23-26); the
+solve adjoint satisfies lam = A^{-T} g; each adjoint kernel passes a dot-product test against
+its forward operator."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import cases
+from cases import build_domain
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda", 0)
+F64 = torch.float64
+
+
+def _specs():
+    return cases.outflow2d(nx=14, ny=8, seed=7)
+
+
+def _run(pert: dict, eps: float, steps: int = 2, differentiable: bool = False, grads: dict = None):
+    """Two MHD steps of the outflow channel with every input moved by eps times its direction;
+    returns the loss (and, differentiable, the inputs holding gradients)."""
+    import phipict_amd as pa
+
+    specs = _specs()
+    dom = build_domain(specs, 0.05, 2e-2, F64)
+    blk = dom.getBlocks()[0]
+    rng = np.random.default_rng(21)
+    u0 = blk.velocity.detach().clone() + eps * pert["u"]
+    inlet = blk.getBoundary(0)
+    vb = inlet.velocity.detach().clone() + eps * pert["vb"]
+    src = torch.tensor([[0.5, 0.1]], dtype=F64, device=DEV) + eps * pert["src"]
+    e_b = torch.tensor([0.0, 0.6, 0.8], dtype=F64) + eps * pert["eb"].cpu()
+    N = torch.tensor(3.0, dtype=F64) + eps * float(pert["N"])
+    if differentiable:
+        for t in (u0, vb, src, e_b, N):
+            t.requires_grad_(True)
+    blk.setVelocity(u0)
+    inlet.setVelocity(vb)
+    # the plug outflow is rescaled to the parabolic inflow's flux (tracked: it depends on vb)
+    pa.simulation.balance_boundary_fluxes(dom, [blk.getBoundary(1)])
+    dom.UpdateDomainData()
+
+    def reset(domain, **kw):
+        domain.getBlocks()[0].setVelocitySource(src)
+        domain.UpdateDomainData()
+
+    tol = pa.SolverTolerance(rtol=1e-13)
+    sim = pa.MHDSimulation(domain=dom, dt=2e-2, stuart_number=N, e_b=e_b, non_orthogonal=False,
+                           advection_tol=tol, pressure_tol=tol, potential_tol=tol, differentiable=differentiable,
+                           hooks={"PRE_VELOCITY_SETUP": reset}, flux_balance_tol=1e-3)
+    for _ in range(steps):
+        assert sim.single_step()
+    w_u = torch.as_tensor(rng.standard_normal(tuple(blk.velocity.shape)), dtype=F64, device=DEV)
+    w_p = torch.as_tensor(rng.standard_normal(tuple(blk.pressure.shape)), dtype=F64, device=DEV)
+    loss = (w_u * blk.velocity).sum() + 10.0 * (w_p * blk.pressure).sum()
+    if differentiable:
+        loss.backward()
+        grads.update(u=u0.grad, vb=vb.grad, src=src.grad, eb=e_b.grad, N=N.grad)
+    return float(loss)
+
+
+def _directions():
+    specs = _specs()
+    rng = np.random.default_rng(5)
+    s = specs[0]
+    u = torch.as_tensor(rng.standard_normal([1, 2] + s.cells_zyx), dtype=F64, device=DEV)
+    fshape = cases.face_shape(s.cells_zyx, 0)
+    vb = torch.zeros([1, 2] + fshape, dtype=F64, device=DEV)
+    prof = rng.standard_normal(fshape[0])
+    # the normal component moves without changing the net inflow (face heights dy weight the
+    # flux), so the domain stays balanced; the tangential component is free
+    dy = np.diff(s.vertices[1, :, 0])
+    prof -= (prof * dy).sum() / dy.sum()
+    vb[0, 0, :, 0] = torch.as_tensor(prof, dtype=F64, device=DEV)
+    vb[0, 1, :, 0] = torch.as_tensor(rng.standard_normal(fshape[0]), dtype=F64, device=DEV)
+    src = torch.as_tensor(rng.standard_normal((1, 2)), dtype=F64, device=DEV)
+    eb = torch.as_tensor(rng.standard_normal(3), dtype=F64, device=DEV)
+    return dict(u=u, vb=vb, src=src, eb=eb, N=0.7)
+
+
+@pytest.mark.parametrize("which", ["u", "vb", "src", "eb", "N"])
+def test_step_gradient_matches_finite_differences(which):
+    print(f"[adjoint] gradient w.r.t. {which}", flush=True)
+    dirs = _directions()
+    zero = {k: (v * 0 if isinstance(v, torch.Tensor) else 0.0) for k, v in dirs.items()}
+    grads: dict = {}
+    _run(zero, 0.0, differentiable=True, grads=grads)
+    g = grads[which]
+    assert g is not None, f"no gradient for {which}"
+    v = dirs[which]
+    analytic = float((g.to(DEV) * v).sum()) if isinstance(v, torch.Tensor) else float(g) * v
+    eps = 1e-6
+    plus = dict(zero)
+    plus[which] = v
+    fd = (_run(plus, eps) - _run(plus, -eps)) / (2 * eps)
+    print(f"analytic {analytic:.10e} fd {fd:.10e}", flush=True)
+    assert abs(fd) > 1e-4, "the loss does not depend on this input"
+    assert abs(analytic - fd) <= 1e-5 * abs(fd), (analytic, fd)
+
+
+def _run_blocks(case: str, dirs, eps: float, differentiable: bool = False, grads: list = None, batch: int = 1):
+    """Two steps of a multi-block domain (connected, periodic and fixed faces) with the initial
+    velocities moved by eps times dirs; MHD with an oblique field except for the rotated pair."""
+    import phipict_amd as pa
+
+    specs = {"two_blocks": lambda: cases.split_channel2d(12, 10, 5, gradient=4.0)[1],
+             "duct3d": lambda: cases.duct3d(nx=6, ny=5, nz=4, noise=0.1, gradient=5.0),
+             "rotated_pair3d": cases.rotated_pair3d}[case]()
+    dom = build_domain(specs, 0.05, 1e-2, F64, batch=batch)
+    us = []
+    for blk, dv in zip(dom.getBlocks(), dirs):
+        u = blk.velocity.detach().clone() + eps * dv
+        if differentiable:
+            u.requires_grad_(True)
+        blk.setVelocity(u)
+        us.append(u)
+    dom.UpdateDomainData()
+    tol = pa.SolverTolerance(rtol=1e-13)
+    kw = dict(domain=dom, dt=1e-2, non_orthogonal=False, advection_tol=tol, pressure_tol=tol,
+              differentiable=differentiable)
+    if case == "rotated_pair3d":
+        sim = pa.Simulation(**kw)
+    else:
+        sim = pa.MHDSimulation(stuart_number=torch.tensor(2.0, dtype=F64), e_b=torch.tensor([0.0, 0.6, 0.8], dtype=F64),
+                               potential_tol=tol, **kw)
+    for _ in range(2):
+        assert sim.single_step()
+    rng = np.random.default_rng(33)
+    loss = 0.0
+    for blk in dom.getBlocks():
+        w = torch.as_tensor(rng.standard_normal(tuple(blk.velocity.shape)), dtype=F64, device=DEV)
+        wp = torch.as_tensor(rng.standard_normal(tuple(blk.pressure.shape)), dtype=F64, device=DEV)
+        loss = loss + (w * blk.velocity).sum() + 10.0 * (wp * blk.pressure).sum()
+    if differentiable:
+        loss.backward()
+        grads.extend(u.grad for u in us)
+    return float(loss)
+
+
+@pytest.mark.parametrize("case", ["two_blocks", "duct3d", "rotated_pair3d"])
+@pytest.mark.parametrize("batch", [1, 2])
+def test_multiblock_gradient_matches_finite_differences(case, batch):
+    """Connected (also with crossed tangential axes), periodic and wall faces in the adjoint
+    kernels, 2D and 3D, one environment and a batch of two: the gradient of the initial
+    velocities against central differences."""
+    print(f"[adjoint] multi-block {case} batch={batch}", flush=True)
+    import phipict_amd as pa  # noqa: F401
+
+    probe = build_domain({"two_blocks": lambda: cases.split_channel2d(12, 10, 5, gradient=4.0)[1],
+                          "duct3d": lambda: cases.duct3d(nx=6, ny=5, nz=4, noise=0.1, gradient=5.0),
+                          "rotated_pair3d": cases.rotated_pair3d}[case](), 0.05, 1e-2, F64, batch=batch)
+    rng = np.random.default_rng(8)
+    dirs = [torch.as_tensor(rng.standard_normal(tuple(b.velocity.shape)), dtype=F64, device=DEV)
+            for b in probe.getBlocks()]
+    grads: list = []
+    _run_blocks(case, [d * 0 for d in dirs], 0.0, differentiable=True, grads=grads, batch=batch)
+    analytic = sum(float((g * d).sum()) for g, d in zip(grads, dirs))
+    eps = 1e-6
+    fd = (_run_blocks(case, dirs, eps, batch=batch) - _run_blocks(case, dirs, -eps, batch=batch)) / (2 * eps)
+    print(f"analytic {analytic:.10e} fd {fd:.10e}", flush=True)
+    assert abs(fd) > 1e-4
+    assert abs(analytic - fd) <= 1e-5 * abs(fd), (analytic, fd)
