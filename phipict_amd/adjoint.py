@@ -433,3 +433,95 @@ def lorentz_sources(domain: Domain, J: torch.Tensor, e_b: torch.Tensor, N: torch
             Fb = Fb + (S.reshape(S.size(0), d, *([1] * (Fb.dim() - 2))) if S.dim() == 2 else S)
         out.append(Fb)
     return out
+
+
+# ----------------------------------------------------------------------------- checkpointing
+def state_tensors(sim) -> dict:
+    """The state one step hands to the next: block velocities, fixed-face velocities and, for
+    MHD, the potential and u x e_b the next Lorentz force uses."""
+    dom = sim.domain
+    st = {"velocity": [b.velocity for b in dom.getBlocks()],
+          "faces": [bd.velocity for _, _, bd in fixed_faces(dom)],
+          "sources": [b.velocitySource for b in dom.getBlocks()]}
+    if hasattr(sim, "_u_cross_eb_flat"):
+        phi = sim.__dict__.get("_diff_phi") if sim.differentiable else None
+        st["phi"] = [phi if phi is not None else dom.epotResult]
+        st["uxb"] = [sim._u_cross_eb_flat]
+    return st
+
+
+def _set_state(sim, st: dict) -> None:
+    dom = sim.domain
+    for b, t in zip(dom.getBlocks(), st["velocity"]):
+        b.velocity = t
+    for (_, _, bd), t in zip(fixed_faces(dom), st["faces"]):
+        bd.velocity = t
+        dom._bc_generation += 1
+    for b, t in zip(dom.getBlocks(), st["sources"]):
+        b.velocitySource = t
+    if "phi" in st:
+        sim._diff_phi = st["phi"][0]
+        sim._u_cross_eb_flat = st["uxb"][0]
+        dom.setEpotResult(st["phi"][0].detach().clone())
+    dom._changed = True
+    dom.UpdateDomainData()
+
+
+def _clone_state(st: dict, requires_grad: bool = False) -> dict:
+    out = {}
+    for k, ts in st.items():
+        out[k] = [None if t is None else t.detach().clone().requires_grad_(requires_grad) for t in ts]
+    return out
+
+
+def checkpointed_gradient(sim, steps: int, every: int, loss_fn) -> tuple:
+    """Backpropagate ``loss_fn(domain)`` after ``steps`` steps through the whole rollout while
+    keeping only every ``every``-th state: the forward runs untracked and stores the state at
+    the start of each segment; the backward re-runs one segment at a time with the
+    differentiable step, from the last to the first, and hands each segment the gradient of its
+    end state. Parameters used by every step (e_b, N, tensors the hooks set) collect their
+    gradients across the segments. Returns (loss, gradients of the initial state)."""
+    if every < 1 or steps < 1:
+        raise ValueError("steps and every must be positive")
+    was_diff = sim.differentiable
+    ckpts = []
+    sim.differentiable = False
+    try:
+        with torch.no_grad():
+            for k in range(0, steps, every):
+                ckpts.append((_clone_state(state_tensors(sim)), sim.total_step, sim.total_time, min(every, steps - k)))
+                for _ in range(ckpts[-1][3]):
+                    if not sim.single_step():
+                        raise RuntimeError("checkpointed rollout: a forward step failed")
+        final = _clone_state(state_tensors(sim))
+        final_counters = (sim.total_step, sim.total_time)
+        sim.differentiable = True
+        g_next, loss_value = None, None
+        for ci in reversed(range(len(ckpts))):
+            st, ts, tt, nsteps = ckpts[ci]
+            leaves = _clone_state(st, requires_grad=True)
+            _set_state(sim, leaves)
+            sim.total_step, sim.total_time = ts, tt
+            for _ in range(nsteps):
+                if not sim.single_step():
+                    raise RuntimeError("checkpointed rollout: a recomputed step failed")
+            if ci == len(ckpts) - 1:
+                obj = loss_fn(sim.domain)
+                loss_value = float(obj.detach())
+            else:
+                end = state_tensors(sim)
+                obj = 0.0
+                for k, ts_end in end.items():
+                    for t, g in zip(ts_end, g_next.get(k, [])):
+                        if g is not None and t is not None and t.requires_grad:
+                            obj = obj + (t * g).sum()
+            if isinstance(obj, torch.Tensor) and obj.requires_grad:
+                # parameters computed once outside the rollout (e.g. e_b = B0 * direction) are
+                # part of every segment's graph
+                obj.backward(retain_graph=True)
+            g_next = {k: [None if t is None else t.grad for t in ts] for k, ts in leaves.items()}
+        _set_state(sim, final)
+        sim.total_step, sim.total_time = final_counters
+        return loss_value, g_next
+    finally:
+        sim.differentiable = was_diff

@@ -225,6 +225,7 @@ class MHDSimulation(Simulation):
         """u x e_b, Poisson solve for phi, copy to blocks (mhd.py:877-1002)."""
         if self.differentiable:
             return self._magnetic_step_diff(domain, max_iter)
+        self._diff_phi = None
         total, batch = domain.getTotalSize(), domain.getBatchSize()
         self._u_cross_eb_flat = compute_u_cross_eb_flat(domain.velocityResult, self._e_b_device, total, self._dims)
         inplane = self._u_cross_eb_flat.reshape(batch, 3, total)[:, : self._dims].reshape(-1).contiguous()

@@ -206,6 +206,40 @@ def potential_box2d(nx: int = 7, ny: int = 6, seed: int = 5, cw: float = 0.3) ->
                       vsource=np.array([0.1, -0.2]))]
 
 
+def tjunction3d(n: int = 4, seed: int = 4, U: float = 1.0) -> list:
+    """A 3-block T-junction duct: the main arm (2n x n x n cells, x in [0, 2]) takes an inflow
+    u ~ N(U, 0.1 U) (seeded) at -x and joins the junction cube (n^3, x in [2, 3]) whose -y face is
+    an outlet and whose +y face continues into the branch cube (n^3, y in [1, 2]) with an outlet
+    at +y; all other faces are insulating no-slip walls. The outlets start with half the inflow
+    each and are advected and flux-balanced every step. At n = 32: 65 536 + 2 x 32 768 cells."""
+    rng = np.random.default_rng(seed)
+    h = 1.0 / n
+    v0 = vertices([np.linspace(0.0, 2.0, 2 * n + 1), np.linspace(0.0, 1.0, n + 1), np.linspace(0.0, 1.0, n + 1)])
+    v1 = vertices([np.linspace(2.0, 3.0, n + 1), np.linspace(0.0, 1.0, n + 1), np.linspace(0.0, 1.0, n + 1)])
+    v2 = vertices([np.linspace(2.0, 3.0, n + 1), np.linspace(1.0, 2.0, n + 1), np.linspace(0.0, 1.0, n + 1)])
+    c0 = cell_shape(v0)
+    inflow = np.zeros([3] + face_shape(c0, 0))
+    inflow[0] = U * (1.0 + 0.1 * rng.standard_normal(face_shape(c0, 0)))
+    half = 0.5 * float(inflow[0].sum()) * h * h  # half of the inflow through each outlet (area 1)
+    out1 = np.zeros([3] + face_shape(cell_shape(v1), 2))
+    out1[1] = -half
+    out2 = np.zeros([3] + face_shape(cell_shape(v2), 3))
+    out2[1] = half
+    wall = FaceSpec("fixed")
+    f0 = [FaceSpec("fixed", velocity=inflow, pot=POT_OPEN), FaceSpec("connect", other=1, other_face=0),
+          wall, wall, wall, wall]
+    f1 = [FaceSpec("connect", other=0, other_face=1), wall, FaceSpec("fixed", velocity=out1, pot=POT_OPEN),
+          FaceSpec("connect", other=2, other_face=2), wall, wall]
+    f2 = [wall, wall, FaceSpec("connect", other=1, other_face=3), FaceSpec("fixed", velocity=out2, pot=POT_OPEN),
+          wall, wall]
+    blocks = []
+    for v, f in ((v0, f0), (v1, f1), (v2, f2)):
+        cz = cell_shape(v)
+        vel = _noise(rng, [3] + cz, 1e-2)
+        blocks.append(BlockSpec(v, f, velocity=vel, pressure=np.zeros(cz)))
+    return blocks
+
+
 CASES = {
     "channel2d": lambda: channel2d(),
     "duct3d": lambda: duct3d(),
@@ -215,6 +249,7 @@ CASES = {
     "rotated3d": lambda: rotated_pair3d(),
     "potbox3d": lambda: potential_box3d(),
     "potbox2d": lambda: potential_box2d(),
+    "tjunction3d": lambda: tjunction3d(),
 }
 
 

@@ -168,3 +168,140 @@ def test_multiblock_gradient_matches_finite_differences(case, batch):
     print(f"analytic {analytic:.10e} fd {fd:.10e}", flush=True)
     assert abs(fd) > 1e-4
     assert abs(analytic - fd) <= 1e-5 * abs(fd), (analytic, fd)
+
+
+@pytest.mark.parametrize("singular", [False, True])
+def test_linear_solve_adjoint_identity(singular):
+    """The solve's backward is lam = A^{-T} g (scipy on the host), db = lam and dA = -lam x^T on
+    the pattern; singular systems on the mean-free subspace."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    import phipict_amd as pa
+    from oracle import oracle as O
+    from phipict_amd import adjoint as adj
+
+    od = O.OracleDomain(cases.channel2d(12, 10, u0=2.0, seed=3), 0.02, 0.05)
+    col, val, A_diag = od.advection_matrix()
+    row = od.row
+    if singular:
+        col, val = od.pressure_matrix(A_diag)
+    n = row.size - 1
+    A = pa.CSRmatrix(torch.as_tensor(val, device=DEV), torch.as_tensor(col, device=DEV),
+                     torch.as_tensor(row, device=DEV))
+    rng = np.random.default_rng(4)
+    b = rng.standard_normal(n)
+    if singular:
+        b -= b.mean()
+    value = A.value.clone().requires_grad_(True)
+    bt = torch.as_tensor(b, device=DEV).requires_grad_(True)
+    sim = pa.Simulation(domain=None, dt=0.1)
+
+    def solve(rhs):
+        x, _ = sim.linear_solve(A, rhs, x=torch.zeros_like(rhs), use_BiCG=not singular, tol=1e-14,
+                                matrix_rank_deficient=singular)
+        return x
+
+    x = adj.LinearSolveFunction.apply(solve, A, value, bt, 1, singular, singular, 1e-13)
+    g = rng.standard_normal(n)
+    (x * torch.as_tensor(g, device=DEV)).sum().backward()
+    M = sp.csr_matrix((val, col, row), shape=(n, n))
+    if singular:
+        g = g - g.mean()
+        lam = np.linalg.lstsq(M.T.toarray(), g, rcond=None)[0]
+        lam -= lam.mean()
+    else:
+        lam = spla.spsolve(M.T.tocsc(), g)
+    got = bt.grad.cpu().numpy()
+    assert np.linalg.norm(got - lam) <= 1e-9 * np.linalg.norm(lam)
+    xs = x.detach().cpu().numpy()
+    rows = np.repeat(np.arange(n), np.diff(row))
+    np.testing.assert_allclose(value.grad.cpu().numpy(), -lam[rows] * xs[col], rtol=1e-7,
+                               atol=1e-9 * np.abs(lam).max() * np.abs(xs).max())
+
+
+def _tjunction_run(n: int, steps: int, every: int, d_in, d_b0: float, eps: float, mode: str):
+    """MHD T-junction rollout with advected, flux-balanced outlets; loss = MSE of the branch
+    outlet's cell-layer velocity against a seeded target. mode: "fd" (untracked forward),
+    "full" (one tracked rollout) or "ckpt" (checkpointed every ``every`` steps)."""
+    import phipict_amd as pa
+    from phipict_amd import adjoint as adj
+    from phipict_amd.simulation import _face_slice
+
+    specs = cases.tjunction3d(n=n)
+    dom = build_domain(specs, 1.0 / 200, 5e-3, F64)
+    b0, b1, b2 = dom.getBlocks()
+    inlet = b0.getBoundary(0)
+    vin = inlet.velocity.detach().clone() + eps * d_in
+    B0 = torch.tensor(1.0 + eps * d_b0, dtype=F64)
+    if mode != "fd":
+        vin.requires_grad_(True)
+        B0.requires_grad_(True)
+    inlet.setVelocity(vin)
+    outs = [b1.getBoundary(2), b2.getBoundary(3)]
+    velm = torch.tensor([[0.0, 0.5, 0.0]], dtype=F64, device=DEV)
+    pa.simulation.balance_boundary_fluxes(dom, outs)
+    dom.UpdateDomainData()
+
+    def advect(domain, **kw):
+        pa.simulation.update_advective_boundaries(domain, outs, velm, 5e-3)
+
+    def no_source(domain, **kw):
+        # the Lorentz force is added to the current source: start every step from none
+        for b in domain.getBlocks():
+            b.clearVelocitySource()
+        domain.UpdateDomainData()
+
+    tol = pa.SolverTolerance(rtol=1e-13)
+    e_b = B0 * torch.tensor([0.0, 1.0, 0.0], dtype=F64)
+    sim = pa.MHDSimulation(domain=dom, dt=5e-3, stuart_number=torch.tensor(2.0, dtype=F64), e_b=e_b,
+                           non_orthogonal=False, advection_tol=tol, pressure_tol=tol, potential_tol=tol,
+                           hooks={"PRE": advect, "PRE_VELOCITY_SETUP": no_source}, differentiable=(mode == "full"),
+                           flux_balance_tol=1e-6)
+    target = torch.as_tensor(np.random.default_rng(5).standard_normal([1, 3, 1, n, n]), dtype=F64, device=DEV)
+
+    def loss_fn(domain):
+        return ((_face_slice(domain.getBlocks()[2], 3) - target) ** 2).mean()
+
+    if mode == "ckpt":
+        loss, g0 = adj.checkpointed_gradient(sim, steps, every, loss_fn)
+        # the inlet is part of the state: its gradient is the initial state's
+        g_in = g0["faces"][0]
+        return loss, g_in, B0.grad
+    for _ in range(steps):
+        assert sim.single_step()
+    loss = loss_fn(dom)
+    if mode == "full":
+        loss.backward()
+        return float(loss), vin.grad, B0.grad
+    return float(loss)
+
+
+def test_checkpointed_tjunction_gradients():
+    """T-junction (3 connected blocks, advected outlets): gradients of the outlet MSE after 4
+    MHD steps w.r.t. the inflow field and B0 from the checkpointed backward (segments of 2)
+    equal the single-segment backward and central differences."""
+    print("[adjoint] checkpointed T-junction", flush=True)
+    n, steps = 4, 4
+    rng = np.random.default_rng(12)
+    fshape = cases.face_shape([n, n, 2 * n], 0)
+    d_in = torch.zeros([1, 3] + fshape, dtype=F64, device=DEV)
+    prof = rng.standard_normal(fshape)
+    prof -= prof.mean()  # uniform face cells: zero net inflow change
+    d_in[0, 0] = torch.as_tensor(prof, dtype=F64, device=DEV)
+    d_in[0, 1] = torch.as_tensor(rng.standard_normal(fshape), dtype=F64, device=DEV)
+    loss_c, g_in_c, g_b0_c = _tjunction_run(n, steps, 2, d_in, 0.0, 0.0, "ckpt")
+    loss_f, g_in_f, g_b0_f = _tjunction_run(n, steps, steps, d_in, 0.0, 0.0, "full")
+    assert abs(loss_c - loss_f) <= 1e-12 * abs(loss_f)
+    assert float((g_in_c - g_in_f).norm()) <= 1e-8 * float(g_in_f.norm())
+    assert abs(float(g_b0_c) - float(g_b0_f)) <= 1e-8 * abs(float(g_b0_f))
+    eps = 1e-6
+    fd_in = (_tjunction_run(n, steps, 0, d_in, 0.0, eps, "fd") - _tjunction_run(n, steps, 0, d_in, 0.0, -eps, "fd")) / (2 * eps)
+    fd_b0 = (_tjunction_run(n, steps, 0, d_in * 0, 1.0, eps, "fd")
+             - _tjunction_run(n, steps, 0, d_in * 0, 1.0, -eps, "fd")) / (2 * eps)
+    an_in = float((g_in_c * d_in).sum())
+    an_b0 = float(g_b0_c)
+    print(f"inflow: analytic {an_in:.10e} fd {fd_in:.10e}; B0: analytic {an_b0:.10e} fd {fd_b0:.10e}", flush=True)
+    assert abs(fd_in) > 1e-8 and abs(fd_b0) > 1e-8
+    assert abs(an_in - fd_in) <= 1e-5 * abs(fd_in)
+    assert abs(an_b0 - fd_b0) <= 1e-5 * abs(fd_b0)
