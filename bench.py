@@ -44,6 +44,9 @@ def parse_args(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=["duct", "tjunction-adjoint"], default="duct",
+                    help="duct: the PISO + MHD step throughput (default); tjunction-adjoint: the checkpointed "
+                         "backward through a 3-block T-junction rollout and its finite-difference check")
     ap.add_argument("--dtype", choices=["f64", "f32"], default="f64")
     ap.add_argument("--size", default="192x96x96", help="cells nx x ny x nz")
     ap.add_argument("--timer-every", type=int, default=4, help="time a pseudo-random 1/k of the launches of each kernel")
@@ -108,6 +111,155 @@ def make_sim(nx, ny, nz, dtype, device):
                            potential_tol=TOL, hooks={"PRE_VELOCITY_SETUP": forcing})
     sim.record_solver_stats = True
     return dom, sim
+
+
+# ----------------------------------------------------------------------------- adjoint workload
+TJ_RE, TJ_HA, TJ_DT = 200.0, 20.0, 5e-3
+
+
+def make_tjunction(n, device, B0, inflow, seed=4):
+    """3-block T-junction duct: main arm 2n x n x n (x in [0, 2], inflow at -x), junction cube n^3
+    (x in [2, 3], outlet at -y) and branch cube n^3 (y in [1, 2], outlet at +y), insulating
+    no-slip walls, Re = 200, Ha = 20 (N = 2), e_b = B0 (0, 1, 0); outlets advected and
+    flux-balanced every step. ``inflow``: the inflow face velocity [1, 3, n, n, 1]."""
+    import numpy as np
+    import torch
+
+    import phipict_amd as pa
+
+    dt = torch.float64
+
+    def verts(x0, x1, nx, y0, y1):
+        xs, ys, zs = np.linspace(x0, x1, nx + 1), np.linspace(y0, y1, n + 1), np.linspace(0.0, 1.0, n + 1)
+        Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+        return torch.as_tensor(np.stack([X, Y, Z]), dtype=dt).unsqueeze(0)
+
+    gen = torch.Generator("cpu").manual_seed(seed)
+    dom = pa.Domain(3, torch.tensor([1.0 / TJ_RE], dtype=dt), dtype=dt, device=device)
+    shapes = [(2 * n, 0.0, 2.0, 0.0, 1.0), (n, 2.0, 3.0, 0.0, 1.0), (n, 2.0, 3.0, 1.0, 2.0)]
+    blocks = []
+    for nx, x0, x1, y0, y1 in shapes:
+        vel = 1e-2 * torch.randn(1, 3, n, n, nx, generator=gen, dtype=dt)
+        blocks.append(dom.CreateBlock(velocity=vel, vertexCoordinates=verts(x0, x1, nx, y0, y1)))
+    b0, b1, b2 = blocks
+    b0.ConnectBlock(1, b1, 0)
+    b1.ConnectBlock(3, b2, 2)
+    for blk, faces in ((b0, (0, 2, 3, 4, 5)), (b1, (1, 2, 4, 5)), (b2, (0, 1, 3, 4, 5))):
+        for f in faces:
+            blk.CloseBoundary(f)
+    inlet = b0.getBoundary(0)
+    inlet.setVelocity(inflow)
+    inlet.setPotentialBC(pa.PotentialBC.OPEN)
+    flux = float(inflow[0, 0].detach().sum()) / (n * n)
+    outs = []
+    for blk, f, sign in ((b1, 2, -1.0), (b2, 3, 1.0)):
+        bd = blk.getBoundary(f)
+        v = torch.zeros(1, 3, n, 1, n, dtype=dt, device=device)
+        # 45% of the inflow each: the balance below rescales them, a smooth function of the inflow
+        v[0, 1] = 0.45 * sign * flux
+        bd.setVelocity(v)
+        bd.setPotentialBC(pa.PotentialBC.OPEN)
+        outs.append(bd)
+    dom.PrepareSolve()
+    pa.simulation.balance_boundary_fluxes(dom, outs)
+    velm = torch.tensor([[0.0, 0.5, 0.0]], dtype=dt, device=device)
+
+    def advect(domain, **kw):
+        pa.simulation.update_advective_boundaries(domain, outs, velm, TJ_DT)
+
+    def no_source(domain, **kw):
+        for b in domain.getBlocks():
+            b.clearVelocitySource()
+        domain.UpdateDomainData()
+
+    tol = pa.SolverTolerance(rtol=1e-12)
+    e_b = B0 * torch.tensor([0.0, 1.0, 0.0], dtype=dt)
+    sim = pa.MHDSimulation(domain=dom, dt=TJ_DT, stuart_number=torch.tensor(TJ_HA ** 2 / TJ_RE, dtype=dt), e_b=e_b,
+                           non_orthogonal=False, advection_tol=tol, pressure_tol=tol, potential_tol=tol,
+                           hooks={"PRE": advect, "PRE_VELOCITY_SETUP": no_source}, flux_balance_tol=1e-6)
+    return dom, sim
+
+
+def tjunction_adjoint(args) -> int:
+    """Gradients of the outlet MSE after ``steps`` steps w.r.t. the inflow field and B0 by the
+    checkpointed backward; timed, and checked against central differences of the forward."""
+    import numpy as np
+    import torch
+
+    from phipict_amd import adjoint as adj
+    from phipict_amd.simulation import _face_slice
+
+    device = torch.device("cuda", 0)
+    n, steps, every = 32, args.steps if args.steps != 10 else 20, 5
+    rng = np.random.default_rng(4)
+    inflow = torch.zeros(1, 3, n, n, 1, dtype=torch.float64, device=device)
+    inflow[0, 0] = torch.as_tensor(rng.normal(1.0, 0.1, (n, n, 1)), device=device)
+    target = torch.as_tensor(np.random.default_rng(5).standard_normal((1, 3, n, 1, n)), device=device)
+
+    def loss_fn(domain):
+        return ((_face_slice(domain.getBlocks()[2], 3) - target) ** 2).mean()
+
+    vin_leaf = [None]
+
+    def rollout(b0, vin, grad):
+        B0 = torch.tensor(b0, dtype=torch.float64, requires_grad=grad)
+        v = vin.clone().requires_grad_(grad)
+        vin_leaf[0] = v
+        dom, sim = make_tjunction(n, device, B0, v)
+        if grad:
+            return adj.checkpointed_gradient(sim, steps, every, loss_fn), B0
+        for _ in range(steps):
+            if not sim.single_step():
+                raise RuntimeError("forward step failed")
+        return float(loss_fn(dom)), B0
+
+    print("[bench] T-junction adjoint warm-up", file=sys.stderr, flush=True)
+    rollout(1.0, inflow, True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    (loss, g0), B0 = rollout(1.0, inflow, True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    g_in, g_b0 = vin_leaf[0].grad, float(B0.grad)
+    print(f"[bench] forward + checkpointed backward of {steps} steps: {elapsed:.3f} s", file=sys.stderr, flush=True)
+    # central differences: B0, and a random inflow direction (its net flux change rescales the
+    # outlets, through the balance before and during the rollout)
+    d = torch.as_tensor(np.random.default_rng(12).standard_normal((n, n, 1)), device=device)
+    d_in = torch.zeros_like(inflow)
+    d_in[0, 0] = d
+    eps, eps_in = 1e-6, 1e-6
+    t1 = time.perf_counter()
+    fd_b0 = (rollout(1.0 + eps, inflow, False)[0] - rollout(1.0 - eps, inflow, False)[0]) / (2 * eps)
+    fd_in = (rollout(1.0, inflow + eps_in * d_in, False)[0]
+             - rollout(1.0, inflow - eps_in * d_in, False)[0]) / (2 * eps_in)
+    fwd_time = (time.perf_counter() - t1) / 4
+    an_in = float((g_in * d_in).sum())
+    line = {
+        "metric": "adjoint PISO steps/s (forward + checkpointed backward)",
+        "value": round(steps / elapsed, 4),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 1,
+        "ms_per_step": round(1e3 * elapsed / steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"3-block MHD T-junction 2x{n}^3 + 2x{n}^3 cells ({4 * n ** 3} cells), Re=200, Ha=20, "
+                               f"dt=5e-3, {steps} steps, checkpoint every {every}, CG/BiCGStab rtol 1e-12, "
+                               "loss = outlet MSE vs a seeded target; gradients w.r.t. the inflow field and B0",
+                   "cells": 4 * n ** 3},
+        "forward_only_s": round(fwd_time, 3),
+        "loss": loss,
+        "gradcheck": {"b0": {"adjoint": g_b0, "fd": fd_b0, "rel_err": abs(g_b0 - fd_b0) / max(abs(fd_b0), 1e-300)},
+                      "inflow": {"adjoint": an_in, "fd": fd_in,
+                                 "rel_err": abs(an_in - fd_in) / max(abs(fd_in), 1e-300), "eps": eps_in},
+                      "eps": eps},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
 
 
 # ----------------------------------------------------------------------------- algorithmic bytes
@@ -372,6 +524,8 @@ def main(argv=None) -> int:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (the PISO path would "
               "shard by blocks across GPUs, which is not built)", file=sys.stderr)
         return 2
+    if args.workload == "tjunction-adjoint":
+        return tjunction_adjoint(args)
     import numpy as np
     import torch
 

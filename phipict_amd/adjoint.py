@@ -480,10 +480,13 @@ def checkpointed_gradient(sim, steps: int, every: int, loss_fn) -> tuple:
     the start of each segment; the backward re-runs one segment at a time with the
     differentiable step, from the last to the first, and hands each segment the gradient of its
     end state. Parameters used by every step (e_b, N, tensors the hooks set) collect their
-    gradients across the segments. Returns (loss, gradients of the initial state)."""
+    gradients across the segments; so do tracked tensors the initial state was computed from
+    (the initial state's gradients are pushed into their graph at the end). Returns (loss,
+    gradients of the initial state)."""
     if every < 1 or steps < 1:
         raise ValueError("steps and every must be positive")
     was_diff = sim.differentiable
+    initial = state_tensors(sim)
     ckpts = []
     sim.differentiable = False
     try:
@@ -520,6 +523,16 @@ def checkpointed_gradient(sim, steps: int, every: int, loss_fn) -> tuple:
                 # part of every segment's graph
                 obj.backward(retain_graph=True)
             g_next = {k: [None if t is None else t.grad for t in ts] for k, ts in leaves.items()}
+        # continue into whatever the initial state was computed from (a tracked inflow, the
+        # outflow balanced against it before the rollout)
+        outs, grads = [], []
+        for k, ts in initial.items():
+            for t, g in zip(ts, g_next.get(k, [])):
+                if t is not None and g is not None and t.requires_grad:
+                    outs.append(t)
+                    grads.append(g)
+        if outs:
+            torch.autograd.backward(outs, grads, retain_graph=True)
         _set_state(sim, final)
         sim.total_step, sim.total_time = final_counters
         return loss_value, g_next
