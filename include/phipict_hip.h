@@ -48,7 +48,7 @@ enum {
 
 /* ---- domain descriptors ---- */
 
-/* One face of a block, 88 bytes. Face order [-x,+x,-y,+y,-z,+z]. */
+/* One face of a block, 104 bytes. Face order [-x,+x,-y,+y,-z,+z]. */
 typedef struct pp_boundary_desc {
     int32_t type;          /* PP_BOUND_* */
     int32_t vel_type;      /* FIXED: PP_BC_DIRICHLET or PP_BC_NEUMANN (velocity) */
@@ -58,15 +58,18 @@ typedef struct pp_boundary_desc {
     int32_t pot_type;      /* FIXED: PP_POT_* (used when pot_types == 0) */
     int32_t conn_block;    /* CONNECTED: index of the connected block */
     int32_t conn_axes[3];  /* CONNECTED: face-format axes of the connection */
-    int32_t pad0, pad1;
+    int32_t scalar_static; /* FIXED: passive scalar [sb][C] (1) or [sb][C][face cells] (0) */
+    int32_t scalar_batched;/* FIXED: passive scalar has one slice per environment */
     uint64_t vel;          /* device pointer (dtype) */
     uint64_t transform;    /* device pointer (dtype) or 0 */
     uint64_t pot_types;    /* device pointer int8 [face cells] or 0 */
     uint64_t pot_values;   /* device pointer (dtype) [face cells] or 0 */
     double   pot_cw;       /* thin-wall conductance ratio */
+    uint64_t scalar;       /* FIXED: passive scalar face data (Dirichlet value or Neumann gradient) or 0 */
+    uint64_t scalar_types; /* FIXED: int8 [C] PP_BC_* per channel, or 0 (all Dirichlet) */
 } pp_boundary_desc;
 
-/* One block, 96 + 6*88 = 624 bytes. */
+/* One block, 104 + 6*104 = 728 bytes. */
 typedef struct pp_block_desc {
     int32_t size[3];        /* cells in x, y, z (1 for unused dimensions) */
     int32_t cells;          /* size[0]*size[1]*size[2] */
@@ -83,6 +86,7 @@ typedef struct pp_block_desc {
     uint64_t epot;          /* [batch][cells] or 0 */
     uint64_t vsource;       /* see vsource_mode */
     uint64_t viscosity;     /* see visc_mode */
+    uint64_t scalar;        /* passive scalar [batch][C][cells] or 0 */
     pp_boundary_desc bnd[6];
 } pp_block_desc;
 
@@ -123,6 +127,11 @@ typedef struct pp_domain {
     pp_csr C;               /* advection matrix, values [batch][nnz] */
     pp_csr P;               /* pressure matrix, values [batch][nnz] */
     pp_csr E;               /* electric potential matrix, values [nnz] */
+    int32_t scalar_channels;  /* passive scalar channels C (0: none) */
+    int32_t scalar_visc_count;/* 0: the domain viscosity; 1: one value for all channels; C */
+    const void *scalar_viscosity; /* DEVICE pointer [scalar_visc_count] (dtype) */
+    void *scalar_rhs;       /* [batch][C][n] */
+    void *scalar_result;    /* [batch][C][n] */
 } pp_domain;
 
 /* The reference's LinearSolverResultInfo (csrc/phipict/solvers/linear_solvers.h:180-185). */
@@ -234,6 +243,11 @@ int pp_csr_pattern(const pp_domain *dom, int32_t *row, int32_t *col, void *strea
 /* replaces SetupAdvectionMatrixEulerImplicit (csrc/phipict/piso/advection.cu:1230-1250),
  * kernel PISO_build_matrix advection.cu:26-306: writes C (row, col, values) and A. */
 int pp_setup_advection_matrix(const pp_domain *dom, int32_t non_ortho_flags, void *stream);
+/* SetupAdvectionMatrix with forPassiveScalar (advection.cu:26-306): the matrix of passive
+ * scalar channel `channel` (the channel's diffusivity and its boundary conditions) into C, A */
+int pp_setup_advection_matrix_scalar(const pp_domain *dom, int32_t non_ortho_flags, int32_t channel, void *stream);
+/* replaces SetupAdvectionScalar (advection.cu:544-700, 1301-1373): scalar_rhs [B][C][n] */
+int pp_setup_advection_scalar(const pp_domain *dom, int32_t non_ortho_flags, void *stream);
 /* replaces SetupAdvectionVelocityEulerImplicitRHS (advection.cu:1375-1406),
  * kernel kPISO_build_advection_RHS advection.cu:799-955: writes velocity_rhs. */
 int pp_setup_advection_velocity(const pp_domain *dom, int32_t non_ortho_flags,
@@ -263,7 +277,7 @@ int pp_compute_epot_rhs(const pp_domain *dom, const void *vector_field, void *di
 int pp_compute_current_density(const pp_domain *dom, const void *phi, const void *uxb, void *j_out,
                                void *stream);
 /* replaces Copy{Velocity,Pressure,Epot}Result{To,From}Blocks (csrc/phipict/piso/result_copy.cu:19-475).
- * field: 0 velocity, 1 pressure, 2 epot; to_blocks: 1 flat->blocks, 0 blocks->flat. */
+ * field: 0 velocity, 1 pressure, 2 epot, 3 passive scalar; to_blocks: 1 flat->blocks, 0 blocks->flat. */
 int pp_copy_result(const pp_domain *dom, int32_t field, int32_t to_blocks, void *stream);
 /* ---- adjoints of the assembly (orthogonal CENTRAL path; the reference's *Grad bindings,
  * csrc/phipict/bindings.cpp:781-814). Gradients are ADDED to the output buffers (zero them

@@ -98,6 +98,8 @@ class FaceSpec:
     other_face: int = -1
     axis1: int = -1
     axis2: int = -1
+    scalar: Optional[np.ndarray] = None        # passive scalar face data: [C] or [C, (z), y, x]
+    scalar_neumann: Optional[list] = None      # per channel; default: Neumann on 'neumann' faces
 
 
 @dataclass
@@ -108,6 +110,7 @@ class BlockSpec:
     pressure: Optional[np.ndarray] = None      # [(z), y, x]
     vsource: Optional[np.ndarray] = None       # [d] or [d, (z), y, x]
     viscosity: Optional[np.ndarray] = None     # [1] or [(z), y, x]
+    scalar: Optional[np.ndarray] = None        # passive scalar [C, (z), y, x]
 
     @property
     def dims(self) -> int:
@@ -187,7 +190,9 @@ def boundary_slice(ft: np.ndarray, face: int) -> np.ndarray:
 class OracleDomain:
     """Host domain for the oracle kernels, built from BlockSpecs in one dtype."""
 
-    def __init__(self, specs: Sequence[BlockSpec], viscosity: float, dt: float, dtype=np.float64):
+    def __init__(self, specs: Sequence[BlockSpec], viscosity: float, dt: float, dtype=np.float64,
+                 scalar_visc: Optional[Sequence[float]] = None):
+        self.scalar_visc = None if scalar_visc is None else [float(v) for v in np.atleast_1d(scalar_visc)]
         self.dtype = np.dtype(dtype)
         self.specs = list(specs)
         self.dims = specs[0].dims
@@ -275,6 +280,10 @@ class OracleDomain:
                         ob.pot_values = _p(pv)
                     ob.pot_cw = fs.cw
         self.ncells = off
+        self.viscosity = float(viscosity)
+        self.scalar = [None if s.scalar is None else np.ascontiguousarray(s.scalar.reshape(s.scalar.shape[0], -1),
+                                                                           self.dtype) for s in specs]
+        self.channels = 0 if self.scalar[0] is None else self.scalar[0].shape[0]
         self.dom = ODomain(self.dims, len(specs), off, 0, float(viscosity), float(dt), self.blocks)
         self.row = np.zeros(off + 1, np.int32)
         rc = lib().o_csr_rows_f64(C.byref(self.dom), _p(self.row))
@@ -340,6 +349,86 @@ class OracleDomain:
             for x in vals:
                 best = x if (x > best or np.isnan(x)) and not np.isnan(best) else best
         return best
+
+    # ---- passive scalar (restated in numpy; the matrix through the C advection matrix)
+    def scalar_is_neumann(self, bi: int, f: int, ch: int) -> bool:
+        fs = self.specs[bi].faces[f]
+        if fs.scalar_neumann is not None:
+            return bool(fs.scalar_neumann[ch])
+        return bool(fs.neumann) and fs.scalar is None
+
+    def scalar_diffusivity(self, ch: int) -> float:
+        """getViscosity with forPassiveScalar (discretization.cuh:1141-1151)."""
+        if self.scalar_visc is None:
+            return self.viscosity
+        return self.scalar_visc[0] if len(self.scalar_visc) == 1 else self.scalar_visc[ch]
+
+    def scalar_matrix(self, ch: int):
+        """The advection-diffusion matrix of passive-scalar channel ch (PISO_build_matrix with
+        forPassiveScalar, advection.cu:26-306): the same fluxes with the channel's diffusivity
+        for every cell and its boundary condition at the prescribed faces -- the C matrix of
+        the oracle on a copy of the domain whose viscosity and wall conditions are replaced."""
+        blocks = (OBlock * len(self.specs))()
+        C.memmove(blocks, self.blocks, C.sizeof(blocks))
+        for bi in range(len(self.specs)):
+            blocks[bi].visc_mode = 0
+            for f in range(2 * self.dims):
+                if blocks[bi].b[f].type == OB_FIXED:
+                    blocks[bi].b[f].vel_neumann = 1 if self.scalar_is_neumann(bi, f, ch) else 0
+        dom = ODomain(self.dims, len(self.specs), self.ncells, 0, self.scalar_diffusivity(ch), self.dom.dt, blocks)
+        col = np.zeros(self.nnz, np.int32)
+        val = np.zeros(self.nnz, self.dtype)
+        A = np.zeros(self.ncells, self.dtype)
+        assert self._fn("o_advection_matrix")(C.byref(dom), _p(self.row), _p(col), _p(val), _p(A)) == 0
+        return col, val, A
+
+    def face_cell_index(self, bi: int, f: int) -> np.ndarray:
+        """Block cell (flat) next to every face cell of face f, in face-cell order."""
+        sz = list(self.specs[bi].size_xyz)
+        axis, upper = f >> 1, f & 1
+        fs = list(sz)
+        fs[axis] = 1
+        q = np.indices((fs[2], fs[1], fs[0])).reshape(3, -1)  # z, y, x in face-flat order
+        pos = {0: q[2], 1: q[1], 2: q[0]}
+        pos[axis] = np.full(q.shape[1], sz[axis] - 1 if upper else 0)
+        return pos[0] + sz[0] * (pos[1] + sz[1] * pos[2])
+
+    def scalar_rhs(self) -> np.ndarray:
+        """kPISO_build_scalar_advection_RHS, advection.cu:544-700 (CENTRAL, orthogonal): [C][n]."""
+        d, ts = self.dims, 2 * self.dims * self.dims + 1
+        C_ = self.channels
+        dt = self.dtype.type(self.dom.dt)
+        out = np.zeros((C_, self.ncells), self.dtype)
+        for bi, spec in enumerate(self.specs):
+            T = self.cell_tr[bi].reshape(-1, ts)
+            det = T[:, -1]
+            phi = self.scalar[bi]
+            o = self.offsets[bi]
+            for ch in range(C_):
+                t = det * phi[ch] / dt
+                for f in range(2 * d):
+                    fsp = spec.faces[f]
+                    if fsp.kind != "fixed":
+                        continue
+                    cells = self.face_cell_index(bi, f)
+                    fc = cells.size
+                    ft = self.face_transform(bi, f)
+                    minv = ft[:, d * d:2 * d * d].reshape(fc, d, d)[:, f >> 1, :]
+                    detf = ft[:, -1]
+                    fn = self.dtype.type((f & 1) * 2 - 1)
+                    from .piso_oracle import face_flux
+                    flux = face_flux(self, bi, f) * fn
+                    visc = self.dtype.type(self.scalar_diffusivity(ch))
+                    sv = np.zeros(fc, self.dtype) if fsp.scalar is None else \
+                        np.broadcast_to(np.asarray(fsp.scalar, self.dtype).reshape(C_, -1)[ch], (fc,))
+                    if not self.scalar_is_neumann(bi, f, ch):
+                        alpha = detf * (minv * minv).sum(axis=1)
+                        np.add.at(t, cells, -sv * flux + sv * visc * 2 * alpha)
+                    else:
+                        area = detf * np.sqrt((minv * minv).sum(axis=1))
+                        np.add.at(t, cells, -phi[ch][cells] * flux + sv * visc * area)
+                out[ch, o:o + spec.ncells] = t / det
+        return out
 
     def set_dt(self, dt: float) -> None:
         self.dom.dt = float(dt)

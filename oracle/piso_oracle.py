@@ -164,8 +164,9 @@ class OraclePISO:
                  advection_tol: Optional[float] = None, pressure_tol: Optional[float] = None,
                  pressure_use_amg: bool = False, mhd: Optional[MHDSpec] = None,
                  pre_velocity_setup: Optional[Callable] = None, normalize_pressure: bool = True,
-                 max_iter: int = 5000):
+                 max_iter: int = 5000, advect_scalar: bool = True):
         self.od = od
+        self.advect_scalar = advect_scalar
         od.set_dt(dt)
         self.dt = dt
         self.corrector_steps = corrector_steps
@@ -270,14 +271,42 @@ class OraclePISO:
         self.epot_result = np.ascontiguousarray(x, od.dtype)
 
     # ---- step
+    def _advect_scalar(self):
+        """Passive-scalar advection-diffusion before the momentum step (piso_simulation.py:
+        3588-3775): one matrix for all channels or one per channel when their diffusivities
+        differ; the scalar matrix doubles as the momentum matrix when neither the scalar nor a
+        block has its own viscosity. Returns that matrix or None."""
+        od = self.od
+        n, C_ = od.ncells, od.channels
+        split = od.scalar_visc is not None and len(od.scalar_visc) > 1
+        for_velocity = not (od.scalar_visc is not None or any(sp.viscosity is not None for sp in od.specs) or split)
+        rhs = od.scalar_rhs()
+        tol = resolve_tol(self.adv_tol, rhs.reshape(-1))
+        mats = [od.scalar_matrix(ch) for ch in range(C_)] if split else [od.scalar_matrix(0)] * C_
+        result = np.zeros_like(rhs)
+        for ch in range(C_):
+            col, val, _ = mats[ch]
+            x = np.zeros(n, od.dtype)
+            if np.any(rhs[ch] != 0):
+                r = O.bicgstab(od.row, col, val, np.ascontiguousarray(rhs[ch]), x, tol, self.max_iter)
+                self.iters.append(("bicg-scalar", r.used_iterations, r.final_residual, r.converged))
+                if not r.converged:
+                    raise RuntimeError("oracle scalar solve did not converge")
+            result[ch] = x
+        for bi, spec in enumerate(od.specs):
+            o = od.offsets[bi]
+            od.scalar[bi][...] = result[:, o:o + spec.ncells]
+        return mats[0] if for_velocity else None
+
     def step(self):
         od = self.od
         n, d = od.ncells, od.dims
+        scalar_matrix = self._advect_scalar() if (self.advect_scalar and od.channels) else None
         if self.pre_velocity_setup is not None:
             self.pre_velocity_setup(self)
         if self.mhd is not None:
             self._lorentz()
-        ccol, cval, A = od.advection_matrix()
+        ccol, cval, A = scalar_matrix if scalar_matrix is not None else od.advection_matrix()
         if not self.non_orthogonal:
             rhs = od.advection_rhs()
             x = self.vel_result

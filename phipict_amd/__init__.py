@@ -10,9 +10,11 @@ from .domain import (AdvectionScheme, Block, Boundary, BoundaryConditionType, Bo
                      ConvergenceCriterion, CSRmatrix, Domain, FixedBoundary, PeriodicBoundary, PotentialBC)
 from .ops import (AMGPCGSolve, ComputeCurrentDensityFaceBased, ComputeEpotRHS, CoordsToFaceTransforms,
                   CoordsToTransforms, CopyEpotResultFromBlocks, CopyEpotResultToBlocks, CopyPressureResultFromBlocks,
-                  CopyPressureResultToBlocks, CopyVelocityResultFromBlocks, CopyVelocityResultToBlocks, CorrectVelocity,
+                  CopyPressureResultToBlocks, CopyScalarResultFromBlocks, CopyScalarResultToBlocks,
+                  CopyVelocityResultFromBlocks, CopyVelocityResultToBlocks, CorrectVelocity,
                   GetKrylovSettings, LinearSolverResultInfo, ReleaseKrylovWorkspaces, SetKrylovSettings,
-                  SetupAdvectionMatrix, SetupAdvectionVelocity, SetupEpotMatrix, SetupPressureCorrection,
+                  SetupAdvectionMatrix, SetupAdvectionScalar, SetupAdvectionVelocity, SetupEpotMatrix,
+                  SetupPressureCorrection,
                   SetupPressureMatrix, SetupPressureRHS, SetupPressureRHSdiv, SolveLinear, SparseOuterProduct)
 from .hooks import Hook, Hooks
 from .simulation import LinsolveError, Simulation

@@ -28,13 +28,15 @@ class BoundaryDesc(C.Structure):
         ("pot_type", C.c_int32),
         ("conn_block", C.c_int32),
         ("conn_axes", C.c_int32 * 3),
-        ("pad0", C.c_int32),
-        ("pad1", C.c_int32),
+        ("scalar_static", C.c_int32),
+        ("scalar_batched", C.c_int32),
         ("vel", C.c_uint64),
         ("transform", C.c_uint64),
         ("pot_types", C.c_uint64),
         ("pot_values", C.c_uint64),
         ("pot_cw", C.c_double),
+        ("scalar", C.c_uint64),
+        ("scalar_types", C.c_uint64),
     ]
 
 
@@ -56,6 +58,7 @@ class BlockDesc(C.Structure):
         ("epot", C.c_uint64),
         ("vsource", C.c_uint64),
         ("viscosity", C.c_uint64),
+        ("scalar", C.c_uint64),
         ("bnd", BoundaryDesc * 6),
     ]
 
@@ -96,6 +99,11 @@ class DomainDesc(C.Structure):
         ("C", CsrDesc),
         ("P", CsrDesc),
         ("E", CsrDesc),
+        ("scalar_channels", C.c_int32),
+        ("scalar_visc_count", C.c_int32),
+        ("scalar_viscosity", C.c_void_p),
+        ("scalar_rhs", C.c_void_p),
+        ("scalar_result", C.c_void_p),
     ]
 
 
@@ -145,8 +153,8 @@ class AmgLevel(C.Structure):
     ]
 
 
-assert C.sizeof(BoundaryDesc) == 88, C.sizeof(BoundaryDesc)
-assert C.sizeof(BlockDesc) == 624, C.sizeof(BlockDesc)
+assert C.sizeof(BoundaryDesc) == 104, C.sizeof(BoundaryDesc)
+assert C.sizeof(BlockDesc) == 728, C.sizeof(BlockDesc)
 
 _i32, _i64, _vp, _dbl = C.c_int32, C.c_int64, C.c_void_p, C.c_double
 _P = C.POINTER
@@ -170,6 +178,8 @@ SIGNATURES = {
     "pp_compute_current_density": [_P(DomainDesc), _vp, _vp, _vp, _vp],
     "pp_copy_result": [_P(DomainDesc), _i32, _i32, _vp],
     "pp_max_velocity": [_P(DomainDesc), _i32, _i32, _i32, _i32, _vp, _vp],
+    "pp_setup_advection_matrix_scalar": [_P(DomainDesc), _i32, _i32, _vp],
+    "pp_setup_advection_scalar": [_P(DomainDesc), _i32, _vp],
     "pp_setup_advection_matrix_grad": [_P(DomainDesc), _vp, _vp, _vp, _vp],
     "pp_setup_advection_velocity_grad": [_P(DomainDesc), _vp, _vp, _vp, _vp, _vp],
     "pp_setup_pressure_matrix_grad": [_P(DomainDesc), _vp, _vp, _vp, _vp],

@@ -147,9 +147,11 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_csr_pattern(pp_domain dom, int32_
     row[g + 1] = start + w;
 }
 
-// PISO_build_matrix, advection.cu:26-306 (CENTRAL): values of C and the diagonal A.
+// PISO_build_matrix, advection.cu:26-306 (CENTRAL): values of C and the diagonal A. With
+// sc >= 0 the matrix of passive-scalar channel sc (forPassiveScalar): the channel's
+// diffusivity and its boundary condition at the prescribed faces.
 template <typename T, int D>
-__global__ void __launch_bounds__(ASM_BLOCK) k_advection_matrix(pp_domain dom) {
+__global__ void __launch_bounds__(ASM_BLOCK) k_advection_matrix(pp_domain dom, int sc) {
     CELL_PROLOGUE
     const T dt = reinterpret_cast<const T *>(dom.time_step)[env];
     T fl[2 * D];
@@ -161,7 +163,7 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_advection_matrix(pp_domain dom) {
     const T det = Metric<T, D>::det(b, f);
     T diag = det / dt;
     idx[0] = g;
-    const T vp = visc_at<T>(dom, b, f);
+    const T vp = sc < 0 ? visc_at<T>(dom, b, f) : scalar_visc<T>(dom, sc);
     T ap[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) ap[k] = Metric<T, D>::alpha(b, f, k);
@@ -178,13 +180,14 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_advection_matrix(pp_domain dom) {
             const pp_block_desc &nb = blocks[nbi];
             const int nf = flat3(np, nb.size);
             const T an = Metric<T, D>::alpha(nb, nf, nw);
-            const T vn = visc_at<T>(dom, nb, nf);
+            const T vn = sc < 0 ? visc_at<T>(dom, nb, nf) : vp;
             idx[bound + 1] = nf + nb.global_offset;
             const T vc = (ap[dim] * vp + an * vn) * T(0.5);
             diag += vc;
             rv[bound + 1] -= vc;
         } else {
-            const int slip = b.bnd[bound].vel_type == PP_BC_DIRICHLET ? 0 : 1;
+            const int slip = sc < 0 ? (b.bnd[bound].vel_type == PP_BC_DIRICHLET ? 0 : 1)
+                                    : (scalar_neumann(b.bnd[bound], sc) ? 1 : 0);
             diag += T(1 - slip) * 2 * vp * ap[dim];
             idx[bound + 1] = -1;
         }
@@ -321,6 +324,50 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_advection_rhs(pp_domain dom, int 
         t += vsource<T>(dom, b, f, dim, env);
         if (apply_pg) t -= pg[dim] * dt;
         rhs[(size_t)dim * dom.num_cells + g] = t;
+    }
+}
+
+// kPISO_build_scalar_advection_RHS, advection.cu:544-700 (CENTRAL): every passive-scalar
+// channel; prescribed faces give the advected face value and the diffusive flux (Dirichlet:
+// the face value with the ghost-cell coefficient 2 alpha; Neumann: the cell value advected and
+// the prescribed normal gradient times the face area)
+template <typename T, int D>
+__global__ void __launch_bounds__(ASM_BLOCK) k_scalar_rhs(pp_domain dom) {
+    CELL_PROLOGUE
+    const int C = dom.scalar_channels;
+    const T dt = reinterpret_cast<const T *>(dom.time_step)[env];
+    const T det = Metric<T, D>::det(b, f);
+    const T *phi = reinterpret_cast<const T *>(b.scalar) + (size_t)env * C * b.cells;
+    T *rhs = reinterpret_cast<T *>(dom.scalar_rhs) + (size_t)env * C * dom.num_cells;
+    for (int ch = 0; ch < C; ++ch) {
+        const T phiP = phi[(size_t)ch * b.cells + f];
+        T t = det * phiP / dt;
+#pragma unroll
+        for (int bound = 0; bound < 2 * D; ++bound) {
+            if (!at_bound(p, bound, b) || !is_fixed(b, bound)) continue;
+            const pp_boundary_desc &fb = b.bnd[bound];
+            const T fn = T((bound & 1) * 2 - 1);
+            const int fc = face_cells(b, bound), ff = face_flat(p, bound, b), axis = bound >> 1;
+            const T s = fixed_scalar<T>(fb, C, fc, ff, ch, env);
+            const T flux = contra_fixed<T, D>(fb, fc, ff, axis, env) * fn;
+            const T visc = scalar_visc<T>(dom, ch);
+            if (!scalar_neumann(fb, ch)) {
+                t -= s * flux;
+                t += s * visc * 2 * alpha_fixed<T, D>(fb, ff, axis);
+            } else {
+                t -= phiP * flux;
+                T area = T(1);
+                if (fb.has_transform) {
+                    const TView<T> tf = Metric<T, D>::face(fb, ff);
+                    T r = T(0);
+#pragma unroll
+                    for (int k = 0; k < D; ++k) r += tf[D * D + axis * D + k] * tf[D * D + axis * D + k];
+                    area = tf[2 * D * D] * dsqrt<T>(r);
+                }
+                t += s * visc * area;
+            }
+        }
+        rhs[(size_t)ch * dom.num_cells + g] = t / det;
     }
 }
 
@@ -641,7 +688,8 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_copy_result(pp_domain dom, int fi
     if (f >= b.cells) return;
     const int g = b.global_offset + f;
     const int env = blockIdx.y;
-    T *bf = reinterpret_cast<T *>(field == 0 ? b.velocity : (field == 1 ? b.pressure : b.epot));
+    T *bf = reinterpret_cast<T *>(field == 0 ? b.velocity : (field == 1 ? b.pressure : (field == 2 ? b.epot : b.scalar)));
+    if (!bf) return;
     for (int c = 0; c < ch; ++c) {
         const size_t fo = ((size_t)env * ch + c) * dom.num_cells + g;
         const size_t bo = ((size_t)env * ch + c) * b.cells + f;
@@ -1511,7 +1559,30 @@ int pp_setup_advection_matrix(const pp_domain *dom, int32_t non_ortho_flags, voi
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(dom->C.val && dom->C.row && dom->A && dom->time_step, "C, A and time_step must be set");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        PP_TIMED(PP_K_ADVECTION_MATRIX, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_advection_matrix<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom))));
+        PP_TIMED(PP_K_ADVECTION_MATRIX, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_advection_matrix<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom, -1))));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_setup_advection_matrix_scalar(const pp_domain *dom, int32_t non_ortho_flags, int32_t channel, void *stream) {
+    (void)non_ortho_flags;
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(dom->C.val && dom->C.row && dom->A && dom->time_step, "C, A and time_step must be set");
+    PP_CHECK_ARG(channel >= 0 && channel < dom->scalar_channels, "passive scalar channel out of range");
+    PP_CHECK_ARG(dom->scalar_visc_count == 0 || dom->scalar_viscosity, "scalar viscosity not set");
+    PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
+        hipLaunchKernelGGL((k_advection_matrix<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom, (int)channel)));
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_setup_advection_scalar(const pp_domain *dom, int32_t non_ortho_flags, void *stream) {
+    (void)non_ortho_flags;
+    if (int e = check_domain(dom, __func__)) return e;
+    PP_CHECK_ARG(dom->scalar_channels > 0 && dom->scalar_rhs && dom->time_step, "passive scalar fields must be set");
+    PP_CHECK_ARG(dom->scalar_visc_count == 0 || dom->scalar_viscosity, "scalar viscosity not set");
+    PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
+        hipLaunchKernelGGL((k_scalar_rhs<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom)));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
@@ -1602,10 +1673,12 @@ int pp_compute_current_density(const pp_domain *dom, const void *phi, const void
 
 int pp_copy_result(const pp_domain *dom, int32_t field, int32_t to_blocks, void *stream) {
     if (int e = check_domain(dom, __func__)) return e;
-    PP_CHECK_ARG(field >= 0 && field <= 2, "field must be 0, 1 or 2");
-    void *flat = field == 0 ? dom->velocity_result : (field == 1 ? dom->pressure_result : dom->epot_result);
+    PP_CHECK_ARG(field >= 0 && field <= 3, "field must be 0, 1, 2 or 3");
+    void *flat = field == 0 ? dom->velocity_result
+                            : (field == 1 ? dom->pressure_result : (field == 2 ? dom->epot_result : dom->scalar_result));
     PP_CHECK_ARG(flat != nullptr, "result vector not set");
-    const int ch = field == 0 ? dom->dims : 1;
+    PP_CHECK_ARG(field != 3 || dom->scalar_channels > 0, "the domain has no passive scalar");
+    const int ch = field == 0 ? dom->dims : (field == 3 ? dom->scalar_channels : 1);
     PP_DISPATCH(dom->dtype,
         PP_TIMED(PP_K_COPY_RESULT, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_copy_result<T>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom, (int)field, ch,
                            (int)to_blocks, reinterpret_cast<const T *>(flat), reinterpret_cast<T *>(flat))));

@@ -217,6 +217,28 @@ __device__ __forceinline__ T visc_at(const pp_domain &dom, const pp_block_desc &
     return b.visc_mode == 1 ? v[0] : v[f];
 }
 
+// Passive-scalar diffusivity of channel ch (getViscosity with forPassiveScalar,
+// discretization.cuh:1141-1151): the domain's scalar viscosity (one value or one per channel),
+// else the domain viscosity
+template <typename T>
+__device__ __forceinline__ T scalar_visc(const pp_domain &dom, int ch) {
+    if (dom.scalar_visc_count == 0) return T(dom.viscosity);
+    return reinterpret_cast<const T *>(dom.scalar_viscosity)[dom.scalar_visc_count == 1 ? 0 : ch];
+}
+// passive-scalar condition of a fixed face for channel ch (Dirichlet when no types are given)
+__device__ __forceinline__ bool scalar_neumann(const pp_boundary_desc &fb, int ch) {
+    return fb.scalar_types && reinterpret_cast<const int8_t *>(fb.scalar_types)[ch] == PP_BC_NEUMANN;
+}
+// passive-scalar face data of channel ch at face cell ff (0 when the face has none)
+template <typename T>
+__device__ __forceinline__ T fixed_scalar(const pp_boundary_desc &fb, int C, int fc, int ff, int ch, int env) {
+    if (!fb.scalar) return T(0);
+    const T *data = reinterpret_cast<const T *>(fb.scalar);
+    const int e = fb.scalar_batched ? env : 0;
+    if (fb.scalar_static) return data[e * C + ch];
+    return data[((size_t)e * C + ch) * fc + ff];
+}
+
 // Velocity component k of cell f of block b (env e): from the block field (glob == nullptr)
 // or from a flat [batch][d][n] vector whose env slice starts at glob.
 template <typename T>

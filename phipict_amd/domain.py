@@ -262,6 +262,12 @@ class FixedBoundary(Boundary):
         self._potentialCw = 0.0
         self.potentialTypes: Optional[torch.Tensor] = None
         self.potentialValues: Optional[torch.Tensor] = None
+        # passive scalar: face data (Dirichlet value or Neumann normal gradient), static
+        # [1|B, C] or varying [1|B, C, (z), y, x], and the condition of every channel
+        self.passiveScalar: Optional[torch.Tensor] = None
+        self._scalarTypes: list = []
+        self._scalarDefault = BoundaryConditionType.DIRICHLET
+        self._scalarTypesDevice: Optional[torch.Tensor] = None
 
     # velocity
     def setVelocity(self, velocity: torch.Tensor) -> None:
@@ -298,6 +304,68 @@ class FixedBoundary(Boundary):
         if computational and self.hasTransform():
             return transform_vectors(self.getVelocityVarying(), self.transform, True)
         return self.velocity
+
+    # passive scalar (FixedBoundary::setPassiveScalar / setPassiveScalarType, boundaries.cpp)
+    def setPassiveScalar(self, t: torch.Tensor) -> None:
+        """Static ``[1|B, C]`` or varying ``[1|B, C, (z), y, x]`` face data of the passive scalar."""
+        dims = self._domain.getSpatialDims()
+        C = self._domain.getPassiveScalarChannels()
+        if t.dim() not in (2, dims + 2) or t.size(1) != C:
+            raise ValueError(f"passive scalar must be [1|B, {C}] or [1|B, {C}, (z), y, x]")
+        if t.dim() == dims + 2 and list(t.shape[2:]) != self.face_shape():
+            raise ValueError(f"varying passive scalar must have spatial shape {self.face_shape()}")
+        if t.size(0) not in (1, self._domain.getBatchSize()):
+            raise ValueError("passive scalar batch must be 1 or the batch size")
+        self.passiveScalar = t.to(device=self._domain.getDevice(), dtype=self._domain.getDtype()).contiguous()
+        self._domain._changed = True
+
+    def CreatePassiveScalar(self) -> None:
+        C = self._domain.getPassiveScalarChannels()
+        self.setPassiveScalar(torch.zeros(1, C, dtype=self._domain.getDtype(), device=self._domain.getDevice()))
+
+    def hasPassiveScalar(self) -> bool:
+        return self.passiveScalar is not None
+
+    def isPassiveScalarStatic(self) -> bool:
+        return self.passiveScalar is None or self.passiveScalar.dim() == 2
+
+    def getPassiveScalarChannels(self) -> int:
+        return 0 if self.passiveScalar is None else self.passiveScalar.size(1)
+
+    def setPassiveScalarType(self, types) -> None:
+        """One condition for every channel, or one per channel."""
+        if isinstance(types, (list, tuple)):
+            self._scalarTypes = [BoundaryConditionType(t) for t in types]
+        else:
+            self._scalarDefault = BoundaryConditionType(types)
+            self._scalarTypes = []
+        self._scalarTypesDevice = None
+        self._domain._changed = True
+
+    def getPassiveScalarTypes(self) -> list:
+        C = self._domain.getPassiveScalarChannels()
+        if self._scalarTypes:
+            if len(self._scalarTypes) != C:
+                raise ValueError(f"{len(self._scalarTypes)} passive scalar boundary types for {C} channels")
+            return list(self._scalarTypes)
+        return [self._scalarDefault] * C
+
+    @property
+    def passiveScalarTypes(self) -> list:
+        return self.getPassiveScalarTypes()
+
+    def isPassiveScalarBoundaryTypeStatic(self) -> bool:
+        return True
+
+    def _scalar_types_device(self) -> Optional[torch.Tensor]:
+        if self._domain.getPassiveScalarChannels() == 0:
+            return None
+        types = self.getPassiveScalarTypes()
+        t = self._scalarTypesDevice
+        if t is None or t.numel() != len(types):
+            t = torch.tensor([int(x) for x in types], dtype=torch.int8).to(self._domain.getDevice())
+            self._scalarTypesDevice = t
+        return t
 
     def face_shape(self) -> list:
         """Spatial shape [(z), y, x] of the face (size 1 along the normal)."""
@@ -420,6 +488,7 @@ class Block:
             pressure = torch.zeros(self.velocity.size(0), 1, *spatial, dtype=domain.getDtype(), device=domain.getDevice())
         self.pressure = pressure.to(device=domain.getDevice(), dtype=domain.getDtype()).contiguous()
         self.epot: Optional[torch.Tensor] = None
+        self.passiveScalar: Optional[torch.Tensor] = None
         self.velocitySource: Optional[torch.Tensor] = None
         self.viscosity: Optional[torch.Tensor] = None
         self.vertexCoordinates: Optional[torch.Tensor] = None
@@ -482,6 +551,31 @@ class Block:
 
     def hasEpot(self) -> bool:
         return self.epot is not None
+
+    # passive scalar [B, C, (z), y, x] (Block::setPassiveScalar, block.cpp)
+    def setPassiveScalar(self, t: torch.Tensor) -> None:
+        C = self._domain.getPassiveScalarChannels()
+        if list(t.shape) != [self.getBatchSize(), C] + self.getSizesZYX():
+            raise ValueError(f"passive scalar must be [{self.getBatchSize()}, {C}, (z), y, x]")
+        self.passiveScalar = t.to(device=self._domain.getDevice(), dtype=self._domain.getDtype()).contiguous()
+        self._domain._changed = True
+
+    def CreatePassiveScalar(self) -> None:
+        C = self._domain.getPassiveScalarChannels()
+        self.setPassiveScalar(torch.zeros(self.getBatchSize(), C, *self.getSizesZYX(), dtype=self._domain.getDtype(),
+                                          device=self._domain.getDevice()))
+
+    def hasPassiveScalar(self) -> bool:
+        return self.passiveScalar is not None
+
+    def getPassiveScalarChannels(self) -> int:
+        return 0 if self.passiveScalar is None else self.passiveScalar.size(1)
+
+    def hasPassiveScalarViscosity(self) -> bool:
+        return False
+
+    def isAllFixedBoundariesPassiveScalarTypeStatic(self) -> bool:
+        return True
 
     def setVelocitySource(self, s: torch.Tensor) -> None:
         """Static ``[1|B, d]`` or per-cell ``[1|B, d, (z), y, x]`` source (block.cpp:275-300)."""
@@ -684,16 +778,28 @@ class Block:
                 other._make_fixed(of, None, BoundaryConditionType.DIRICHLET)
 
     def CloseBoundary(self, face, velocity: Optional[torch.Tensor] = None, passiveScalar=None) -> FixedBoundary:
-        """No-slip (or prescribed-velocity) Dirichlet face (Block::CloseBoundary, block.cpp:960-964)."""
+        """No-slip (or prescribed-velocity) Dirichlet face; the passive scalar is Dirichlet, zero
+        or ``passiveScalar`` (Block::CloseBoundary, block.cpp:960-964)."""
         f = face_index(face)
         self._disconnect(f)
-        return self._make_fixed(f, velocity, BoundaryConditionType.DIRICHLET)
+        bound = self._make_fixed(f, velocity, BoundaryConditionType.DIRICHLET)
+        bound.setPassiveScalarType(BoundaryConditionType.DIRICHLET)
+        if passiveScalar is not None:
+            bound.setPassiveScalar(passiveScalar)
+        return bound
 
     def OpenBoundary(self, face, passiveScalar=None) -> FixedBoundary:
-        """Zero-gradient (Neumann) velocity face (Block::OpenBoundary, block.cpp:969-977)."""
+        """Zero-gradient (Neumann) velocity face; the passive scalar is zero-gradient unless
+        ``passiveScalar`` gives Dirichlet values (Block::OpenBoundary, block.cpp:969-977)."""
         f = face_index(face)
         self._disconnect(f)
-        return self._make_fixed(f, None, BoundaryConditionType.NEUMANN)
+        bound = self._make_fixed(f, None, BoundaryConditionType.NEUMANN)
+        if passiveScalar is not None:
+            bound.setPassiveScalarType(BoundaryConditionType.DIRICHLET)
+            bound.setPassiveScalar(passiveScalar)
+        else:
+            bound.setPassiveScalarType(BoundaryConditionType.NEUMANN)
+        return bound
 
     def CloseAllBoundaries(self) -> None:
         for f in range(2 * self.getSpatialDims()):
@@ -771,7 +877,7 @@ class Domain:
     ``[B][channel][totalSize]`` and the matrices C (advection), P (pressure) and Epot."""
 
     def __init__(self, spatialDims: int, viscosity, name: str = "Domain", dtype=torch.float32, device=None,
-                 passiveScalarChannels: int = 0):
+                 passiveScalarChannels: int = 1, scalarViscosity: Optional[torch.Tensor] = None):
         if spatialDims not in (2, 3):
             raise ValueError("spatialDims must be 2 or 3")
         self._dims = int(spatialDims)
@@ -818,6 +924,15 @@ class Domain:
         # bumped by every assembly of the advection matrix: the pressure operator depends on
         # the advection diagonal only, so solvers key their pressure caches on it
         self._a_version = 0
+        if passiveScalarChannels < 0:
+            raise ValueError("passiveScalarChannels must not be negative")
+        self._scalar_channels = int(passiveScalarChannels)
+        self.passiveScalarViscosity: Optional[torch.Tensor] = None
+        self._scalar_visc_device: Optional[torch.Tensor] = None
+        self.scalarRHS: Optional[torch.Tensor] = None
+        self.scalarResult: Optional[torch.Tensor] = None
+        if scalarViscosity is not None:
+            self.setScalarViscosity(scalarViscosity)
 
     # --- basic accessors
     def getSpatialDims(self) -> int:
@@ -850,6 +965,58 @@ class Domain:
     def hasEpot(self) -> bool:
         return self.Epot is not None and self.epotRHS is not None and self.epotResult is not None
 
+    # --- passive scalar (domain.cpp; bindings.cpp:498-538)
+    def getPassiveScalarChannels(self) -> int:
+        return self._scalar_channels
+
+    def hasPassiveScalar(self) -> bool:
+        return bool(self._blocks) and self._scalar_channels > 0 and all(b.hasPassiveScalar() for b in self._blocks)
+
+    def CreatePassiveScalarOnBlocks(self) -> None:
+        for b in self._blocks:
+            if not b.hasPassiveScalar():
+                b.CreatePassiveScalar()
+        self._initialized = False
+
+    def setScalarViscosity(self, viscosity: torch.Tensor) -> None:
+        """Diffusivity of the passive scalar: one value for every channel or one per channel."""
+        v = torch.as_tensor(viscosity, dtype=self._dtype).detach().cpu().reshape(-1)
+        if v.numel() not in (1, self._scalar_channels):
+            raise ValueError("scalar viscosity must have 1 or passiveScalarChannels entries")
+        self.passiveScalarViscosity = v
+        self._scalar_visc_device = v.to(self._device)
+        self._changed = True
+
+    def clearScalarViscosity(self) -> None:
+        self.passiveScalarViscosity = None
+        self._scalar_visc_device = None
+        self._changed = True
+
+    def hasPassiveScalarViscosity(self) -> bool:
+        return self.passiveScalarViscosity is not None
+
+    def isPassiveScalarViscosityStatic(self) -> bool:
+        return self.passiveScalarViscosity is None or self.passiveScalarViscosity.numel() == 1
+
+    def hasPassiveScalarBlockViscosity(self) -> bool:
+        return False
+
+    def isAllFixedBoundariesPassiveScalarTypeStatic(self) -> bool:
+        return True
+
+    def setScalarRHS(self, t): self.scalarRHS = self._check_flat(t, self._scalar_channels, "scalarRHS")
+    def setScalarResult(self, t): self.scalarResult = self._check_flat(t, self._scalar_channels, "scalarResult")
+
+    def CreateScalarRHS(self) -> None:
+        self.scalarRHS = torch.zeros(self._batch * self._scalar_channels * self.totalSize, dtype=self._dtype,
+                                     device=self._device)
+        self._changed = True
+
+    def CreateScalarResult(self) -> None:
+        self.scalarResult = torch.zeros(self._batch * self._scalar_channels * self.totalSize, dtype=self._dtype,
+                                        device=self._device)
+        self._changed = True
+
     def setAdvectionScheme(self, scheme) -> None:
         if AdvectionScheme(scheme) != AdvectionScheme.CENTRAL:
             raise NotImplementedError("only the CENTRAL advection scheme is implemented")
@@ -869,6 +1036,8 @@ class Domain:
         block = Block(self, velocity, pressure, name)
         if vertexCoordinates is not None:
             block.setVertexCoordinates(vertexCoordinates)
+        if passiveScalar is not None:
+            block.setPassiveScalar(passiveScalar)
         self._blocks.append(block)
         self._initialized = False
         self._changed = True
@@ -893,6 +1062,8 @@ class Domain:
             b.pressure = b.pressure[:1].repeat(batchSize, *([1] * (b.pressure.dim() - 1))).contiguous()
             if b.epot is not None:
                 b.epot = b.epot[:1].repeat(batchSize, *([1] * (b.epot.dim() - 1))).contiguous()
+            if b.passiveScalar is not None:
+                b.passiveScalar = b.passiveScalar[:1].repeat(batchSize, *([1] * (b.passiveScalar.dim() - 1))).contiguous()
         self._batch = batchSize
         self._initialized = False
         self._changed = True
@@ -938,6 +1109,9 @@ class Domain:
         self.pressureRHS = torch.zeros(B * total * dims, dtype=dt, device=dev)
         self.pressureRHSdiv = torch.zeros(B * total, dtype=dt, device=dev)
         self.pressureResult = torch.zeros(B * total, dtype=dt, device=dev)
+        if self.hasPassiveScalar():
+            self.CreateScalarRHS()
+            self.CreateScalarResult()
         if self._time_step is None or self._time_step.numel() != B:
             self.setTimeStep(torch.ones(B, dtype=dt))
         self._initialized = True
@@ -1038,6 +1212,7 @@ class Domain:
         d.velocity = _lib.ptr(b.velocity)
         d.pressure = _lib.ptr(b.pressure)
         d.epot = _lib.ptr(b.epot)
+        d.scalar = _lib.ptr(b.passiveScalar) if self.hasPassiveScalar() else 0
         if b.velocitySource is None:
             d.vsource_mode = 0
         else:
@@ -1064,6 +1239,12 @@ class Domain:
                 bd.pot_types = _lib.ptr(bound.potentialTypes)
                 bd.pot_values = _lib.ptr(bound.potentialValues)
                 bd.pot_cw = bound.getPotentialCw()
+                if self._scalar_channels > 0:
+                    ps = bound.passiveScalar
+                    bd.scalar = _lib.ptr(ps) if ps is not None else 0
+                    bd.scalar_static = 1 if (ps is None or ps.dim() == 2) else 0
+                    bd.scalar_batched = 1 if (ps is not None and ps.size(0) > 1) else 0
+                    bd.scalar_types = _lib.ptr(bound._scalar_types_device())
             elif isinstance(bound, ConnectedBoundary):
                 bd.conn_block = self._blocks.index(bound.getConnectedBlock())
                 for i in range(3):
@@ -1077,7 +1258,7 @@ class Domain:
         t, s, v = b.transform, b.velocitySource, b.viscosity
         return (_p(b.velocity), _p(b.pressure), _p(b.epot), _p(s), -1 if s is None else s.dim(),
                 -1 if s is None else s.size(0), _p(v), -1 if v is None else v.numel(), _p(t),
-                -1 if t is None else t._version)
+                -1 if t is None else t._version, _p(b.passiveScalar))
 
     def UpdateDomainData(self) -> None:
         """Repack the device descriptors if any tensor of the domain changed (a no-op when the
@@ -1095,7 +1276,8 @@ class Domain:
             self._atlas_key = akey
         dkey = (gen, _p(self._atlas), _p(self._time_step), _p(self.A), _p(self.velocityRHS), _p(self.velocityResult),
                 _p(self.pressureRHS), _p(self.pressureRHSdiv), _p(self.pressureResult), _p(self.epotRHS),
-                _p(self.epotResult), float(self.viscosity[0]),
+                _p(self.epotResult), float(self.viscosity[0]), _p(self.scalarRHS), _p(self.scalarResult),
+                _p(self._scalar_visc_device), self.hasPassiveScalar(),
                 *(None if M is None else (_p(M.value), _p(M.ell_val), M.ell_valid()) for M in (self.C, self.P, self.Epot)))
         if dkey == self._dom_key:
             self._changed = False
@@ -1122,6 +1304,13 @@ class Domain:
         d.C = self.C.desc(ell_output=True) if self.C is not None else _lib.CsrDesc()
         d.P = self.P.desc(ell_output=True) if self.P is not None else _lib.CsrDesc()
         d.E = self.Epot.desc(ell_output=True) if self.Epot is not None else _lib.CsrDesc()
+        scalar = self.hasPassiveScalar()
+        d.scalar_channels = self._scalar_channels if scalar else 0
+        vs = self._scalar_visc_device
+        d.scalar_visc_count = 0 if (vs is None or not scalar) else vs.numel()
+        d.scalar_viscosity = _lib.ptr(vs) if scalar else None
+        d.scalar_rhs = _lib.ptr(self.scalarRHS) if scalar else None
+        d.scalar_result = _lib.ptr(self.scalarResult) if scalar else None
         self._changed = False
 
     ATLAS_RING = 16

@@ -45,11 +45,24 @@ def _stream(domain: Domain) -> int:
 def SetupAdvectionMatrix(domain: Domain, timeStep, nonOrthoFlags: int = 0, forPassiveScalar: bool = False,
                          passiveScalarChannel: int = 0) -> None:
     """Implicit advection-diffusion matrix C and its diagonal A (bindings.cpp:639-641)."""
-    if forPassiveScalar:
-        raise NotImplementedError("passive-scalar advection is not implemented")
     d = _prep(domain, timeStep, nonOrthoFlags)
-    _lib.call("pp_setup_advection_matrix", C.byref(d), int(nonOrthoFlags), _stream(domain))
+    if forPassiveScalar:
+        if not domain.hasPassiveScalar():
+            raise RuntimeError("the domain has no passive scalar")
+        _lib.call("pp_setup_advection_matrix_scalar", C.byref(d), int(nonOrthoFlags), int(passiveScalarChannel),
+                  _stream(domain))
+    else:
+        _lib.call("pp_setup_advection_matrix", C.byref(d), int(nonOrthoFlags), _stream(domain))
     domain.C.mark_ell_valid()
+
+
+def SetupAdvectionScalar(domain: Domain, timeStep, nonOrthoFlags: int = 0) -> None:
+    """Passive-scalar right-hand side of every channel into ``domain.scalarRHS``
+    (bindings.cpp:642-644 -> advection.cu:1301-1373)."""
+    if not domain.hasPassiveScalar():
+        raise RuntimeError("the domain has no passive scalar")
+    d = _prep(domain, timeStep, nonOrthoFlags)
+    _lib.call("pp_setup_advection_scalar", C.byref(d), int(nonOrthoFlags), _stream(domain))
 
 
 def SetupAdvectionVelocity(domain: Domain, timeStep, nonOrthoFlags: int = 0, applyPressureGradient: bool = False) -> None:
@@ -151,6 +164,8 @@ def CopyVelocityResultFromBlocks(domain: Domain) -> None: _copy(domain, 0, 0)
 def CopyPressureResultToBlocks(domain: Domain) -> None: _copy(domain, 1, 1)
 def CopyPressureResultFromBlocks(domain: Domain) -> None: _copy(domain, 1, 0)
 def CopyEpotResultToBlocks(domain: Domain) -> None: _copy(domain, 2, 1)
+def CopyScalarResultToBlocks(domain: Domain) -> None: _copy(domain, 3, 1)
+def CopyScalarResultFromBlocks(domain: Domain) -> None: _copy(domain, 3, 0)
 def CopyEpotResultFromBlocks(domain: Domain) -> None: _copy(domain, 2, 0)
 
 

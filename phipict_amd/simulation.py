@@ -18,7 +18,7 @@ import numpy as np
 import torch
 
 from . import ops
-from .domain import Domain, transform_size
+from .domain import CSRmatrix, Domain, transform_size
 from .hooks import Hook, Hooks
 from .solvers import amg as amg_mod
 from .solvers.tolerance import SolverTolerance, default_tolerance, resolve_tolerance
@@ -237,11 +237,12 @@ class Simulation:
                  pressure_time_step_normalized: bool = False, velocity_corrector: str = "FD",
                  non_orthogonal: bool = True, differentiable: bool = False, hooks=None, prep_fn=None,
                  time_step: Optional[float] = None, adaptive_CFL: float = 0.8, adaptive_CFL_per_env: bool = True,
-                 stop_fn: Optional[Callable[[], bool]] = None, **unsupported: Any) -> None:
+                 stop_fn: Optional[Callable[[], bool]] = None, advect_passive_scalar: bool = True,
+                 **unsupported: Any) -> None:
         for k, v in unsupported.items():
             if k in ("density_viscosity", "scipy_solve_advection",
                      "scipy_solve_pressure", "preconditionBiCG", "BiCG_precondition_fallback", "convergence_tol",
-                     "solver_double_fallback", "advect_passive_scalar", "exclude_advection_solve_gradients",
+                     "solver_double_fallback", "exclude_advection_solve_gradients",
                      "exclude_pressure_solve_gradients", "output_resampling_shape", "output_resampling_fill_max_steps"):
                 if v not in (None, False, 0, 0.8, True) or k in ("scipy_solve_advection", "scipy_solve_pressure") and v:
                     raise NotImplementedError(f"option {k}={v!r} is not implemented")
@@ -261,6 +262,7 @@ class Simulation:
         self.adaptive_CFL_per_env = bool(adaptive_CFL_per_env)
         self.print_adaptive_step_info = False
         self.stop_fn = stop_fn
+        self.advect_passive_scalar = bool(advect_passive_scalar)
         self.differentiable = bool(differentiable)
         # relative tolerance of the adjoint (transposed) solves, against the upstream gradient
         self.adjoint_rtol = 1e-12
@@ -530,9 +532,15 @@ class Simulation:
         for step in range(int(iterations)):
             self._run_hook(Hook.PRE, local_step=step, time_step=time_step)
             domain.UpdateDomainData()
+            matrix_for_velocity = False
+            if self.advect_passive_scalar and domain.hasPassiveScalar():
+                matrix_for_velocity, solve_ok = self._advect_scalar(step, time_step, flags)
+                if not solve_ok:
+                    return solve_ok
             # momentum prediction
             self._run_hook(Hook.PRE_VELOCITY_SETUP, local_step=step, time_step=time_step)
-            ops.SetupAdvectionMatrix(domain, time_step, flags)
+            if not matrix_for_velocity:
+                ops.SetupAdvectionMatrix(domain, time_step, flags)
             domain._a_version += 1
             if not self.non_orthogonal:
                 ops.SetupAdvectionVelocity(domain, time_step, 0, False)
@@ -615,6 +623,51 @@ class Simulation:
             self.end_step(time_step)
         return solve_ok
 
+    # ---- passive scalar (piso_simulation.py:3588-3775)
+    def _advect_scalar(self, step: int, time_step, flags: int) -> tuple:
+        """Implicit advection-diffusion of every passive-scalar channel: one matrix for all
+        channels, or one per channel when their diffusivities differ; the scalar matrix is
+        also the momentum matrix when neither the scalar nor a block has its own viscosity.
+        Returns (the scalar matrix serves the momentum step, solves converged)."""
+        domain = self.domain
+        B, C, n = domain.getBatchSize(), domain.getPassiveScalarChannels(), domain.getTotalSize()
+        split = not domain.isPassiveScalarViscosityStatic()
+        for_velocity = not (domain.hasPassiveScalarViscosity() or any(b.viscosity is not None for b in domain.getBlocks())
+                            or domain.hasPassiveScalarBlockViscosity() or split)
+        matrices = []
+        if split:
+            for ch in range(C):
+                ops.SetupAdvectionMatrix(domain, time_step, flags, forPassiveScalar=True, passiveScalarChannel=ch)
+                matrices.append(CSRmatrix(domain.C.value.clone(), domain.C.index, domain.C.row))
+        else:
+            ops.SetupAdvectionMatrix(domain, time_step, flags, forPassiveScalar=True, passiveScalarChannel=0)
+        if self.non_orthogonal:
+            ops.CopyScalarResultFromBlocks(domain)
+        ok = True
+        for no_step in range(self.advect_non_ortho_steps if self.non_orthogonal else 1):
+            ops.SetupAdvectionScalar(domain, time_step, flags)
+            self._run_hook(Hook.POST_SCALAR_SETUP, no_step=no_step, local_step=step, time_step=time_step)
+            if split:
+                rhs = domain.scalarRHS.view(B, C, n)
+                x0 = domain.scalarResult.view(B, C, n) if (self.non_orthogonal and no_step > 0) else None
+                out = torch.empty_like(rhs)
+                for ch in range(C):
+                    r, ok = self.linear_solve(matrices[ch], rhs[:, ch].contiguous().view(-1),
+                                              x=None if x0 is None else x0[:, ch].contiguous().view(-1),
+                                              use_BiCG=self.advection_use_BiCG, tol=self.advection_tol, tag="scalar")
+                    out[:, ch] = r.view(B, n)
+                result = out.view(-1)
+            else:
+                x0 = domain.scalarResult.clone() if (self.non_orthogonal and no_step > 0) else None
+                result, ok = self.linear_solve(domain.C, domain.scalarRHS, x=x0, use_BiCG=self.advection_use_BiCG,
+                                               tol=self.advection_tol, tag="scalar")
+            domain.setScalarResult(result)
+            domain.UpdateDomainData()
+            if not ok:
+                return for_velocity, ok
+        ops.CopyScalarResultToBlocks(domain)
+        return for_velocity, ok
+
     # ---- the differentiable step (piso_diff.py; piso_simulation.py:3525-4169 with
     # differentiable operators)
     def _diff_solver(self, csr, use_bicg: bool, tol, tag: str, amg_hierarchy=None, rank_deficient: bool = False):
@@ -637,6 +690,8 @@ class Simulation:
         if time_step is None:
             time_step = self._time_step_tensor()
         d, B = domain.getSpatialDims(), domain.getBatchSize()
+        if self.advect_passive_scalar and domain.hasPassiveScalar():
+            raise NotImplementedError("the differentiable step does not advect passive scalars")
         tsn = bool(self.pressure_time_step_normalized)
         for step in range(int(iterations)):
             self._run_hook(Hook.PRE, local_step=step, time_step=time_step)

@@ -206,6 +206,26 @@ def potential_box2d(nx: int = 7, ny: int = 6, seed: int = 5, cw: float = 0.3) ->
                       vsource=np.array([0.1, -0.2]))]
 
 
+def heated_channel2d(nx: int = 20, ny: int = 10, channels: int = 2, seed: int = 6, U: float = 1.0,
+                     stretch: float = 1.5) -> list:
+    """A 2D channel carrying ``channels`` passive scalars: plug inflow U at -x with a per-face-cell
+    scalar profile, the same plug leaving at +x (the scalar zero-gradient there), a hot bottom
+    wall (Dirichlet scalar) and a top wall with a prescribed scalar flux (Neumann)."""
+    rng = np.random.default_rng(seed)
+    v = vertices([np.linspace(0.0, 4.0, nx + 1), tanh_coords(ny, -1.0, 1.0, stretch)])
+    cz = cell_shape(v)
+    C = channels
+    inflow_s = 0.2 + 0.1 * rng.standard_normal([C] + face_shape(cz, 0))
+    faces = [FaceSpec("fixed", velocity=np.array([U, 0.0]), pot=POT_OPEN, scalar=inflow_s),
+             FaceSpec("fixed", velocity=np.array([U, 0.0]), pot=POT_OPEN, scalar_neumann=[True] * C),
+             FaceSpec("fixed", scalar=np.linspace(1.0, 0.5, C)),
+             FaceSpec("fixed", scalar=np.linspace(0.3, -0.2, C), scalar_neumann=[True] * C)]
+    vel = _noise(rng, [2] + cz, 1e-2)
+    vel[0] += U
+    scal = 0.5 + 0.1 * rng.standard_normal([C] + cz)
+    return [BlockSpec(v, faces, velocity=vel, pressure=np.zeros(cz), scalar=scal)]
+
+
 def tjunction3d(n: int = 4, seed: int = 4, U: float = 1.0) -> list:
     """A 3-block T-junction duct: the main arm (2n x n x n cells, x in [0, 2]) takes an inflow
     u ~ N(U, 0.1 U) (seeded) at -x and joins the junction cube (n^3, x in [2, 3]) whose -y face is
@@ -251,12 +271,13 @@ CASES = {
     "potbox3d": lambda: potential_box3d(),
     "potbox2d": lambda: potential_box2d(),
     "tjunction3d": lambda: tjunction3d(),
+    "heated2d": lambda: heated_channel2d(),
 }
 
 
 # ----------------------------------------------------------------------------- product domain
 def build_domain(specs: Sequence[BlockSpec], viscosity: float, dt: float, dtype=None, device=None, batch: int = 1,
-                 epot: bool = False):
+                 epot: bool = False, scalar_visc=None):
     """``phipict_amd.Domain`` with the blocks, boundaries and fields of ``specs``."""
     import torch
 
@@ -265,7 +286,10 @@ def build_domain(specs: Sequence[BlockSpec], viscosity: float, dt: float, dtype=
     dtype = dtype or torch.float64
     device = device or torch.device("cuda", 0)
     dims = specs[0].dims
-    dom = pa.Domain(dims, torch.tensor([viscosity], dtype=dtype), dtype=dtype, device=device)
+    channels = specs[0].scalar.shape[0] if specs[0].scalar is not None else 1
+    dom = pa.Domain(dims, torch.tensor([viscosity], dtype=dtype), dtype=dtype, device=device,
+                    passiveScalarChannels=channels,
+                    scalarViscosity=None if scalar_visc is None else torch.tensor(np.atleast_1d(scalar_visc), dtype=dtype))
 
     def T(a):
         return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=device)
@@ -274,7 +298,8 @@ def build_domain(specs: Sequence[BlockSpec], viscosity: float, dt: float, dtype=
     for s in specs:
         vel = T(s.velocity if s.velocity is not None else np.zeros([dims] + s.cells_zyx)).unsqueeze(0)
         prs = T(s.pressure if s.pressure is not None else np.zeros(s.cells_zyx)).reshape(1, 1, *s.cells_zyx)
-        blk = dom.CreateBlock(velocity=vel, pressure=prs, vertexCoordinates=T(s.vertices).unsqueeze(0))
+        blk = dom.CreateBlock(velocity=vel, pressure=prs, vertexCoordinates=T(s.vertices).unsqueeze(0),
+                              passiveScalar=None if s.scalar is None else T(s.scalar).unsqueeze(0))
         if s.vsource is not None:
             vs = np.asarray(s.vsource)
             blk.setVelocitySource(T(vs).reshape(1, dims) if vs.ndim == 1 else T(vs).unsqueeze(0))
@@ -293,14 +318,21 @@ def build_domain(specs: Sequence[BlockSpec], viscosity: float, dt: float, dtype=
                     a2 = "" if fs.axis2 < 0 else fs.axis2
                     blk.ConnectBlock(f, blocks[fs.other], fs.other_face, a1, a2)
                 continue
+            sc = None
+            if fs.scalar is not None:
+                sv = np.asarray(fs.scalar)
+                sc = T(sv).reshape(1, -1) if sv.ndim == 1 else T(sv).unsqueeze(0)
             if fs.neumann:
-                bound = blk.OpenBoundary(f)
+                bound = blk.OpenBoundary(f, passiveScalar=sc)
             else:
                 vel = None
                 if fs.velocity is not None:
                     fv = np.asarray(fs.velocity)
                     vel = T(fv).reshape(1, dims) if fv.ndim == 1 else T(fv).unsqueeze(0)
-                bound = blk.CloseBoundary(f, velocity=vel)
+                bound = blk.CloseBoundary(f, velocity=vel, passiveScalar=sc)
+            if fs.scalar_neumann is not None:
+                bound.setPassiveScalarType([pa.BoundaryConditionType.NEUMANN if nm else pa.BoundaryConditionType.DIRICHLET
+                                            for nm in fs.scalar_neumann])
             if fs.pot_types is not None:
                 bound.setPotentialTypes(torch.as_tensor(fs.pot_types.reshape(-1), dtype=torch.int8),
                                         cw=fs.cw if (fs.pot_types == POT_THIN_WALL).any() else None)

@@ -588,3 +588,54 @@ def test_oracle_adaptive_step_substeps():
     assert sum(sizes) == pytest.approx(0.25, rel=1e-14)
     sizes2 = osim.adaptive_step(0.25, 0.8)
     assert sum(sizes2) == pytest.approx(0.25, rel=1e-14)
+
+
+def _scalar_domain(ny, U, top, bottom, dt, nx=4, visc=0.1):
+    """A 2D slab (uniform grid on [0, 1]^2, periodic x unless U != 0) with one passive scalar."""
+    v = cases.vertices([np.linspace(0.0, 1.0, nx + 1), np.linspace(0.0, 1.0, ny + 1)])
+    cz = cases.cell_shape(v)
+    if U:
+        side = [O.FaceSpec("fixed", velocity=np.array([U, 0.0]), scalar=np.array([1.5])),
+                O.FaceSpec("fixed", velocity=np.array([U, 0.0]), scalar_neumann=[True])]
+    else:
+        side = [O.FaceSpec("periodic"), O.FaceSpec("periodic")]
+    faces = side + [bottom, top]
+    vel = np.zeros([2] + cz)
+    vel[0] = U
+    spec = O.BlockSpec(v, faces, velocity=vel, pressure=np.zeros(cz),
+                       scalar=np.random.default_rng(3).standard_normal([1] + cz))
+    return O.OracleDomain([spec], visc, dt, scalar_visc=[visc]), v
+
+
+def test_scalar_steady_linear_profile():
+    """Pure diffusion between a Dirichlet wall (value a at y = 0) and a prescribed outward
+    gradient g at y = 1 has the steady profile a + g y, which the finite-volume scheme represents
+    exactly: one implicit step with dt = 1e12 lands on it (ghost-cell coefficient 2 alpha at the
+    Dirichlet face, g times the face area at the Neumann face, their signs)."""
+    a, g = 2.0, 0.5
+    od, v = _scalar_domain(16, 0.0, O.FaceSpec("fixed", scalar=np.array([g]), scalar_neumann=[True]),
+                           O.FaceSpec("fixed", scalar=np.array([a])), 1e12)
+    col, val, _ = od.scalar_matrix(0)
+    rhs = od.scalar_rhs()[0]
+    n = od.ncells
+    phi = spla.spsolve(sp.csr_matrix((val, col, od.row), shape=(n, n)).tocsc(), rhs)
+    yc = 0.5 * (v[1, 1:, 1:] + v[1, :-1, :-1]).reshape(-1)
+    np.testing.assert_allclose(phi, a + g * yc, rtol=0, atol=1e-9)
+
+
+def test_scalar_steady_advected_inflow_value():
+    """Advection-diffusion through a slab with a Dirichlet inflow value s, a zero-gradient
+    outflow (the cell value carried out, explicitly) and zero-flux walls keeps phi = s: a step
+    from phi = s returns phi = s (a wrong sign of the inflow or outflow term leaves a gradient)."""
+    od, _ = _scalar_domain(6, 1.0, O.FaceSpec("fixed", scalar_neumann=[True]),
+                           O.FaceSpec("fixed", scalar_neumann=[True]), 0.1, nx=10, visc=0.05)
+    od.scalar[0][...] = 1.5
+    col, val, _ = od.scalar_matrix(0)
+    rhs = od.scalar_rhs()[0]
+    n = od.ncells
+    phi = spla.spsolve(sp.csr_matrix((val, col, od.row), shape=(n, n)).tocsc(), rhs)
+    np.testing.assert_allclose(phi, 1.5, rtol=0, atol=1e-12)
+    # and a different inflow value moves the field towards it
+    od.specs[0].faces[0].scalar = np.array([2.5])
+    phi2 = spla.spsolve(sp.csr_matrix((val, col, od.row), shape=(n, n)).tocsc(), od.scalar_rhs()[0])
+    assert phi2.max() > 1.5 + 1e-3 and phi2.min() >= 1.5 - 1e-12
