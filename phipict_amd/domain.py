@@ -445,6 +445,12 @@ class FixedBoundary(Boundary):
             return bool((self.potentialTypes == int(PotentialBC.DIRICHLET)).any())
         return self._potentialType == PotentialBC.DIRICHLET
 
+    def hasPotentialTypes(self) -> bool:
+        return self.potentialTypes is not None
+
+    def hasPotentialValues(self) -> bool:
+        return self.potentialValues is not None
+
     def hasPotentialThinWall(self) -> bool:
         if self.potentialTypes is not None:
             return bool((self.potentialTypes == int(PotentialBC.THIN_WALL)).any())
@@ -619,6 +625,9 @@ class Block:
     def hasViscosity(self) -> bool:
         return self.viscosity is not None
 
+    def hasVertexCoordinates(self) -> bool:
+        return self.vertexCoordinates is not None
+
     def isViscosityStatic(self) -> bool:
         return self.viscosity is not None and self.viscosity.numel() == 1
 
@@ -786,6 +795,8 @@ class Block:
         bound.setPassiveScalarType(BoundaryConditionType.DIRICHLET)
         if passiveScalar is not None:
             bound.setPassiveScalar(passiveScalar)
+        elif self._domain.getPassiveScalarChannels() > 0:
+            bound.CreatePassiveScalar()
         return bound
 
     def OpenBoundary(self, face, passiveScalar=None) -> FixedBoundary:
@@ -799,6 +810,8 @@ class Block:
             bound.setPassiveScalar(passiveScalar)
         else:
             bound.setPassiveScalarType(BoundaryConditionType.NEUMANN)
+            if self._domain.getPassiveScalarChannels() > 0:
+                bound.CreatePassiveScalar()
         return bound
 
     def CloseAllBoundaries(self) -> None:

@@ -459,3 +459,30 @@ def test_passive_scalar_matches_oracle(dtype, split):
     dom, sim, od, osim = run_pair(specs, dtype, 4, 0.01, 1e-2, scalar_visc=visc, tol=tol)
     compare(dom, od, osim, dtype, False, NOISE_PRESSURE_TOL_F32 if dtype == np.float32 else None)
     assert dom.hasPassiveScalar() and any(t == "scalar" for _, t, _ in sim.solver_log)
+
+
+def test_saved_domain_steps_identically(tmp_path):
+    """A heated MHD channel saved and loaded again (vertex coordinates, fixed faces with inflow
+    profiles, scalar conditions and potential settings) steps bit-identically to the original."""
+    import phipict_amd as pa
+    from phipict_amd.domain_io import load_domain, save_domain
+
+    print("[piso] save/load round trip", flush=True)
+    specs = cases.heated_channel2d(nx=16, ny=8, channels=2)
+    runs = []
+    for reload in (False, True):
+        dom = build_domain(specs, 0.02, 1e-2, torch.float64, scalar_visc=[0.02, 0.01])
+        if reload:
+            save_domain(dom, tmp_path / "heated")
+            dom = load_domain(tmp_path / "heated")
+            dom.PrepareSolve()
+            dom.setTimeStep(1e-2)
+        sim = pa.MHDSimulation(domain=dom, dt=1e-2, stuart_number=torch.tensor(2.0),
+                               e_b=torch.tensor([0.0, 0.6, 0.8], dtype=torch.float64), non_orthogonal=False,
+                               advection_tol=1e-12, pressure_tol=1e-12, potential_tol=1e-12)
+        for _ in range(2):
+            assert sim.single_step()
+        b = dom.getBlocks()[0]
+        runs.append([to_np(t) for t in (b.velocity, b.pressure, b.passiveScalar, dom.epotResult)])
+    for a, c in zip(*runs):
+        np.testing.assert_array_equal(a, c)

@@ -315,3 +315,77 @@ def test_substeps_setter_and_cfl_split():
     for t, v in [(0.25, np.asarray(8.0, np.float32)), (0.25, np.asarray(7.77, np.float64)), (1.0, np.float32(0.0)),
                  (0.1, np.asarray(3.3, np.float32))]:
         assert _cfl_substep(t, v, 0.8) == cfl_substep(t, v, 0.8)
+
+
+def test_domain_save_load_round_trip(tmp_path):
+    """The JSON + NPZ pair (format version 2) restores blocks, fields, connections, periodic
+    faces, face velocities, passive scalars with per-channel conditions and potential settings;
+    shared tensors are stored once; a pair from different saves, a missing file and a newer
+    format are refused."""
+    import json
+
+    import phipict_amd as pa
+    from phipict_amd.domain_io import load_domain, save_domain
+
+    dom = pa.Domain(2, torch.tensor([0.02], dtype=torch.float64), name="case", dtype=torch.float64, device="cpu",
+                    passiveScalarChannels=2, scalarViscosity=torch.tensor([0.1, 0.2], dtype=torch.float64))
+    b0 = dom.CreateBlockWithSize([6, 4], name="left")
+    b1 = dom.CreateBlockWithSize([5, 4], name="right")
+    g = torch.Generator().manual_seed(3)
+    for b in (b0, b1):
+        b.setVelocity(torch.randn(b.velocity.shape, generator=g, dtype=torch.float64))
+        b.setPassiveScalar(torch.randn(1, 2, *b.getSizesZYX(), generator=g, dtype=torch.float64))
+    b0.ConnectBlock("+x", b1, "-x")
+    inlet = b0.CloseBoundary("-x", velocity=torch.tensor([[1.0, 0.0]], dtype=torch.float64),
+                             passiveScalar=torch.tensor([[0.5, 0.25]], dtype=torch.float64))
+    inlet.setPotentialBC(pa.PotentialBC.OPEN)
+    out = b1.OpenBoundary("+x")
+    wall = b1.CloseBoundary("-y")
+    wall.setPassiveScalarType([pa.BoundaryConditionType.DIRICHLET, pa.BoundaryConditionType.NEUMANN])
+    wall.setPotentialTypes(torch.tensor([3, 3, 1, 0, 2], dtype=torch.int8), cw=0.4)
+    wall.setPotentialValues(torch.linspace(0, 1, 5, dtype=torch.float64))
+    for b in (b0, b1):
+        b.CloseBoundary("+y")
+    b0.CloseBoundary("-y")
+    save_domain(dom, tmp_path / "case")
+    meta = json.loads((tmp_path / "case.json").read_text())
+    assert meta["format_version"] == 2 and meta["passiveScalarChannels"] == 2
+    back = load_domain(tmp_path / "case.json")
+    assert back.name == "case" and back.getDtype() == torch.float64 and back.getPassiveScalarChannels() == 2
+    torch.testing.assert_close(back.passiveScalarViscosity, dom.passiveScalarViscosity)
+    for a, b in zip(dom.getBlocks(), back.getBlocks()):
+        assert a.name == b.name
+        for name in ("velocity", "pressure", "passiveScalar"):
+            torch.testing.assert_close(getattr(b, name), getattr(a, name), rtol=0, atol=0)
+        for f in range(4):
+            ba, bb = a.getBoundary(f), b.getBoundary(f)
+            assert type(ba) is type(bb), (a.name, f)
+            if isinstance(ba, pa.FixedBoundary):
+                assert ba.velocityType == bb.velocityType
+                torch.testing.assert_close(bb.velocity, ba.velocity, rtol=0, atol=0)
+                if ba.hasPassiveScalar():
+                    torch.testing.assert_close(bb.passiveScalar, ba.passiveScalar, rtol=0, atol=0)
+                assert bb.passiveScalarTypes == ba.passiveScalarTypes
+                assert bb.getPotentialBC() == ba.getPotentialBC()
+                assert bb.getPotentialCw() == ba.getPotentialCw()
+                assert (bb.potentialTypes is None) == (ba.potentialTypes is None)
+                if ba.potentialTypes is not None:
+                    assert torch.equal(bb.potentialTypes, ba.potentialTypes)
+                    torch.testing.assert_close(bb.potentialValues, ba.potentialValues, rtol=0, atol=0)
+            elif isinstance(ba, pa.ConnectedBoundary):
+                assert list(bb.axes) == list(ba.axes)
+                assert dom.getBlocks().index(ba.getConnectedBlock()) == back.getBlocks().index(bb.getConnectedBlock())
+    # a pair from two saves is refused
+    other = tmp_path / "other"
+    save_domain(dom, other)
+    (tmp_path / "case.npz").write_bytes((tmp_path / "other.npz").read_bytes())
+    with pytest.raises(ValueError, match="different saves"):
+        load_domain(tmp_path / "case")
+    (tmp_path / "other.npz").unlink()
+    with pytest.raises(FileNotFoundError):
+        load_domain(other)
+    meta["format_version"] = 3
+    (tmp_path / "v3.json").write_text(json.dumps(meta))
+    (tmp_path / "v3.npz").write_bytes((tmp_path / "case.npz").read_bytes())
+    with pytest.raises(ValueError, match="format version"):
+        load_domain(tmp_path / "v3")
