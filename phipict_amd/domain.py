@@ -1142,8 +1142,12 @@ class Domain:
             raise ValueError(f"{name} must be flat with {self._batch * self.totalSize * channels} entries")
         if t.dtype != self._dtype:
             raise TypeError(f"{name} dtype mismatch")
-        self._changed = True
-        return t.contiguous()
+        t = t.contiguous()
+        cur = getattr(self, name, None)
+        # the descriptors hold pointers only: the same storage again changes nothing
+        if cur is None or cur.data_ptr() != t.data_ptr() or cur.numel() != t.numel():
+            self._changed = True
+        return t
 
     def setVelocityRHS(self, t): self.velocityRHS = self._check_flat(t, self._dims, "velocityRHS")
     def setVelocityResult(self, t): self.velocityResult = self._check_flat(t, self._dims, "velocityResult")

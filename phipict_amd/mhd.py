@@ -15,7 +15,7 @@ import torch
 from . import _lib, ops
 from .domain import Domain
 from .hooks import Hook, Hooks
-from .simulation import Simulation, subtract_env_mean
+from .simulation import Simulation, subtract_env_mean, subtract_env_mean_
 from .solvers import amg as amg_mod
 
 _LOG = logging.getLogger("phipict_amd.mhd")
@@ -146,9 +146,12 @@ class MHDSimulation(Simulation):
         prev = domain.epotResult
         if prev is None or prev.numel() != rhs.numel():
             return None
-        x = prev.detach().reshape(rhs.shape).to(dtype=rhs.dtype).clone()
+        if prev.is_contiguous() and prev.dtype == rhs.dtype and not prev.requires_grad:
+            x = prev.view(rhs.shape)  # solved in place: the result replaces it anyway
+        else:
+            x = prev.detach().reshape(rhs.shape).to(dtype=rhs.dtype).clone()
         if self._potential_rank_deficient:
-            x = subtract_env_mean(x, domain.getBatchSize())
+            x = subtract_env_mean_(x, domain.getBatchSize())
         return x
 
     def _get_epot_amg_hierarchy(self, mat):
@@ -229,9 +232,14 @@ class MHDSimulation(Simulation):
         total, batch = domain.getTotalSize(), domain.getBatchSize()
         self._u_cross_eb_flat = compute_u_cross_eb_flat(domain.velocityResult, self._e_b_device, total, self._dims)
         inplane = self._u_cross_eb_flat.reshape(batch, 3, total)[:, : self._dims].reshape(-1).contiguous()
-        rhs = ops.ComputeEpotRHS(domain, inplane)
-        domain.setEpotRHS(rhs)
-        domain.UpdateDomainData()
+        # written into the domain's vectors in place: their descriptors stay valid
+        prev = domain.epotRHS
+        if prev is not None and prev.numel() == total * batch and prev.is_contiguous():
+            rhs = ops.ComputeEpotRHS(domain, inplane, out=prev)
+        else:
+            rhs = ops.ComputeEpotRHS(domain, inplane)
+            domain.setEpotRHS(rhs)
+            domain.UpdateDomainData()
         rhs = domain.epotRHS
         if self._potential_rank_deficient:
             rhs = subtract_env_mean(rhs, batch)
@@ -244,8 +252,12 @@ class MHDSimulation(Simulation):
                                                    if self._potential_use_preconditioner else None),
                                     tag="epot")
         if self._potential_normalize and self._potential_rank_deficient:
-            phi = subtract_env_mean(phi, batch)
-        domain.setEpotResult(phi)
+            phi = subtract_env_mean_(phi, batch)
+        cur = domain.epotResult
+        if cur is not None and cur.numel() == phi.numel() and cur.is_contiguous() and cur.data_ptr() != phi.data_ptr():
+            cur.copy_(phi)
+        else:
+            domain.setEpotResult(phi)
         domain.UpdateDomainData()
         ops.CopyEpotResultToBlocks(domain)
         return bool(ok)

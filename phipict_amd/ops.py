@@ -119,15 +119,20 @@ def SetupEpotMatrix(domain: Domain, nonOrthoFlags: int = 0, useFaceTransform: bo
     domain.Epot.mark_ell_valid()
 
 
-def ComputeEpotRHS(domain: Domain, vectorField: torch.Tensor, useFaceTransform: bool = False) -> torch.Tensor:
-    """div(u x B) per cell, ``[B * n]`` (potential.cu:325-370); ``vectorField`` is ``[B][d][n]``."""
+def ComputeEpotRHS(domain: Domain, vectorField: torch.Tensor, useFaceTransform: bool = False,
+                   out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """div(u x B) per cell, ``[B * n]`` (potential.cu:325-370); ``vectorField`` is ``[B][d][n]``.
+    ``out``: a contiguous ``[B * n]`` vector to write instead of a new one."""
     n, B, dims = domain.getTotalSize(), domain.getBatchSize(), domain.getSpatialDims()
     if vectorField.dim() != 1 or vectorField.numel() != n * dims * B:
         raise ValueError("vectorField must be flat [batch][dims][totalSize]")
     if vectorField.dtype != domain.getDtype():
         raise TypeError("vectorField dtype must match the domain dtype")
     vf = vectorField.contiguous()
-    out = torch.empty(n * B, dtype=domain.getDtype(), device=domain.getDevice())
+    if out is None:
+        out = torch.empty(n * B, dtype=domain.getDtype(), device=domain.getDevice())
+    elif out.numel() != n * B or not out.is_contiguous() or out.dtype != domain.getDtype():
+        raise ValueError("out must be a contiguous [batch * totalSize] vector of the domain dtype")
     d = _prep(domain)
     _lib.call("pp_compute_epot_rhs", C.byref(d), _lib.ptr(vf), _lib.ptr(out), _stream(domain))
     return out

@@ -39,6 +39,14 @@ def subtract_env_mean(flat: torch.Tensor, batch: int) -> torch.Tensor:
     return (per_env - per_env.mean(dim=1, keepdim=True)).reshape(flat.shape)
 
 
+def subtract_env_mean_(flat: torch.Tensor, batch: int) -> torch.Tensor:
+    """In-place :func:`subtract_env_mean` of a contiguous vector (the storage the domain's
+    descriptors point at stays the same)."""
+    per_env = flat.view(batch, -1)
+    per_env.sub_(per_env.mean(dim=1, keepdim=True))
+    return flat
+
+
 def check_solver_return_infos(infos, tol, max_iter, return_best_result: bool, tag: str = "") -> None:
     """Raise LinsolveError like the reference (piso_diff.py:659-791): always for a non-finite
     residual, for non-convergence unless the best iterate is returned."""
@@ -348,7 +356,12 @@ class Simulation:
 
     def _store_pressure_guess(self, cstep: int, result: torch.Tensor) -> None:
         if self.pressure_warm_start:
-            self._pressure_guess[cstep] = result.detach().clone()
+            g = self._pressure_guess.get(cstep)
+            r = result.detach()
+            if g is not None and g.shape == r.shape and g.dtype == r.dtype and g.device == r.device:
+                g.copy_(r)  # the stored buffer is reused
+            else:
+                self._pressure_guess[cstep] = r.clone()
 
     def _pressure_rank_deficient(self, P) -> bool:
         """Pure-Neumann pressure operator: every row sum vanishes (:2165-2220)."""
@@ -569,8 +582,14 @@ class Simulation:
                 if not self.non_orthogonal:
                     ops.SetupPressureCorrection(domain, time_step, 0, False, timeStepNorm=self.pressure_time_step_normalized)
                     self._run_hook(Hook.POST_PRESSURE_SETUP, local_step=step, time_step=time_step)
-                    p, solve_ok = self.linear_solve(domain.P, domain.pressureRHSdiv,
-                                                    x=self._pressure_initial_guess(cstep, domain.pressureRHSdiv),
+                    # solved in place in the domain's pressure vector (zero start, or the warm start)
+                    x = domain.pressureResult
+                    g = self._pressure_guess.get(cstep) if self.pressure_warm_start else None
+                    if g is None or g.shape != x.shape or not bool(torch.isfinite(g).all()):
+                        x.zero_()
+                    else:
+                        x.copy_(g)
+                    p, solve_ok = self.linear_solve(domain.P, domain.pressureRHSdiv, x=x,
                                                     use_BiCG=self.pressure_use_BiCG,
                                                     tol=self._pressure_tol_for(cstep, 0, self.corrector_steps),
                                                     return_best_result=self.pressure_return_best_result,
@@ -579,7 +598,7 @@ class Simulation:
                     if not solve_ok:
                         return solve_ok
                     if self.normalize_pressure_result:
-                        p = subtract_env_mean(p, domain.getBatchSize())
+                        p = subtract_env_mean_(p, domain.getBatchSize())
                     self._store_pressure_guess(cstep, p)
                     domain.setPressureResult(p)
                     domain.UpdateDomainData()
