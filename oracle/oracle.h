@@ -57,6 +57,19 @@ typedef struct ob_domain {
     const ob_block *blocks;
 } ob_domain;
 
+/* non-orthogonal flag bits (non_ortho_flags.h) and what a corner term reads: no values
+ * (matrices), velocity result component w, the pressure result, or passive-scalar result
+ * channel w */
+enum { ONO_DIRECT_MATRIX = 1, ONO_DIRECT_RHS = 2, ONO_DIAGONAL_RHS = 8, ONO_CENTER_MATRIX = 16 };
+enum { OK_TYPES = 0, OK_VEL = 1, OK_PRES = 2, OK_SCAL = 3 };
+
+/* Passive-scalar data of one fixed face for one channel (non-orthogonal right-hand sides). */
+typedef struct ob_face_scalar {
+    const void *values;    /* [1] (is_static) or [face cells]; NULL: zero */
+    int32_t is_static;
+    int32_t neumann;       /* the channel's condition on this face is Neumann */
+} ob_face_scalar;
+
 typedef struct ob_result {
     double final_residual;
     int32_t used_iterations;

@@ -48,6 +48,16 @@ class ODomain(C.Structure):
                 ("viscosity", C.c_double), ("dt", C.c_double), ("blocks", C.POINTER(OBlock))]
 
 
+class OFaceScalar(C.Structure):
+    _fields_ = [("values", C.c_void_p), ("is_static", C.c_int32), ("neumann", C.c_int32)]
+
+
+# non-orthogonal flag bits (non_ortho_flags.h) and the reference's default mode
+NO_DIRECT_MATRIX, NO_DIRECT_RHS, NO_DIAGONAL_RHS, NO_CENTER_MATRIX = 1, 2, 8, 16
+NO_DEFAULT_MODE = NO_CENTER_MATRIX | NO_DIRECT_MATRIX | NO_DIAGONAL_RHS
+OK_VEL, OK_PRES, OK_SCAL = 1, 2, 3
+
+
 class OResult(C.Structure):
     _fields_ = [("final_residual", C.c_double), ("used_iterations", C.c_int32), ("converged", C.c_int32),
                 ("is_finite", C.c_int32), ("pad", C.c_int32)]
@@ -363,7 +373,39 @@ class OracleDomain:
             return self.viscosity
         return self.scalar_visc[0] if len(self.scalar_visc) == 1 else self.scalar_visc[ch]
 
-    def scalar_matrix(self, ch: int):
+    def _face_scalar_table(self, ch: int):
+        """Per block face (block * 6 + face) the channel's fixed-face values and condition."""
+        table = (OFaceScalar * (6 * len(self.specs)))()
+        keep = []
+        for bi, spec in enumerate(self.specs):
+            for f in range(2 * self.dims):
+                fsp = spec.faces[f]
+                if fsp.kind != "fixed":
+                    continue
+                e = table[bi * 6 + f]
+                e.neumann = 1 if self.scalar_is_neumann(bi, f, ch) else 0
+                if fsp.scalar is not None:
+                    a = np.asarray(fsp.scalar, self.dtype)
+                    e.is_static = 1 if a.ndim == 1 else 0
+                    v = np.ascontiguousarray(a.reshape(self.channels, -1)[ch])
+                    keep.append(v)
+                    e.values = v.ctypes.data
+        return table, keep
+
+    def nonortho_rhs(self, kind: int, flags: int, w: int, data: np.ndarray, A: Optional[np.ndarray] = None) -> np.ndarray:
+        """Explicit non-orthogonal corner terms S of every cell (o_nonortho_rhs): kind OK_VEL
+        (component w of a [d][n] velocity result), OK_PRES (pressure result, needs A) or OK_SCAL
+        (channel w of a [C][n] scalar result)."""
+        out = np.zeros(self.ncells, self.dtype)
+        table, keep = self._face_scalar_table(w) if kind == OK_SCAL else (None, [])
+        data = np.ascontiguousarray(data, self.dtype)
+        A = None if A is None else np.ascontiguousarray(A, self.dtype)
+        assert self._fn("o_nonortho_rhs")(C.byref(self.dom), int(kind), int(flags), int(w), _p(data), _p(A),
+                                          None if table is None else C.cast(table, C.c_void_p), _p(out)) == 0
+        del keep
+        return out
+
+    def scalar_matrix(self, ch: int, flags: int = 0):
         """The advection-diffusion matrix of passive-scalar channel ch (PISO_build_matrix with
         forPassiveScalar, advection.cu:26-306): the same fluxes with the channel's diffusivity
         for every cell and its boundary condition at the prescribed faces -- the C matrix of
@@ -379,7 +421,7 @@ class OracleDomain:
         col = np.zeros(self.nnz, np.int32)
         val = np.zeros(self.nnz, self.dtype)
         A = np.zeros(self.ncells, self.dtype)
-        assert self._fn("o_advection_matrix")(C.byref(dom), _p(self.row), _p(col), _p(val), _p(A)) == 0
+        assert self._fn("o_advection_matrix")(C.byref(dom), int(flags), 1, _p(self.row), _p(col), _p(val), _p(A)) == 0
         return col, val, A
 
     def face_cell_index(self, bi: int, f: int) -> np.ndarray:
@@ -393,8 +435,12 @@ class OracleDomain:
         pos[axis] = np.full(q.shape[1], sz[axis] - 1 if upper else 0)
         return pos[0] + sz[0] * (pos[1] + sz[1] * pos[2])
 
-    def scalar_rhs(self) -> np.ndarray:
-        """kPISO_build_scalar_advection_RHS, advection.cu:544-700 (CENTRAL, orthogonal): [C][n]."""
+    def scalar_rhs(self, flags: int = 0, scalar_result: Optional[np.ndarray] = None) -> np.ndarray:
+        """kPISO_build_scalar_advection_RHS, advection.cu:544-700 (CENTRAL): [C][n]; with the
+        flags' right-hand-side bits the corner terms of ``scalar_result`` [C][n] times the
+        diffusivity are subtracted before the division by det."""
+        use_no = bool(flags & (NO_DIRECT_RHS | NO_DIAGONAL_RHS)) and self.dims > 1
+        S = [self.nonortho_rhs(OK_SCAL, flags, ch, scalar_result) for ch in range(self.channels)] if use_no else None
         d, ts = self.dims, 2 * self.dims * self.dims + 1
         C_ = self.channels
         dt = self.dtype.type(self.dom.dt)
@@ -427,6 +473,8 @@ class OracleDomain:
                     else:
                         area = detf * np.sqrt((minv * minv).sum(axis=1))
                         np.add.at(t, cells, -phi[ch][cells] * flux + sv * visc * area)
+                if S is not None:
+                    t = t - S[ch][o:o + spec.ncells] * self.dtype.type(self.scalar_diffusivity(ch))
                 out[ch, o:o + spec.ncells] = t / det
         return out
 
@@ -466,22 +514,29 @@ class OracleDomain:
     def _fn(self, name):
         return getattr(lib(), name + sfx(self.dtype))
 
-    def advection_matrix(self):
+    def advection_matrix(self, flags: int = 0):
         col = np.zeros(self.nnz, np.int32)
         val = np.zeros(self.nnz, self.dtype)
         A = np.zeros(self.ncells, self.dtype)
-        assert self._fn("o_advection_matrix")(C.byref(self.dom), _p(self.row), _p(col), _p(val), _p(A)) == 0
+        assert self._fn("o_advection_matrix")(C.byref(self.dom), int(flags), 0, _p(self.row), _p(col), _p(val),
+                                              _p(A)) == 0
         return col, val, A
 
-    def advection_rhs(self, apply_pressure_gradient: bool = False) -> np.ndarray:
+    def advection_rhs(self, apply_pressure_gradient: bool = False, flags: int = 0,
+                      vel_result: Optional[np.ndarray] = None) -> np.ndarray:
+        """Momentum right-hand side [d][n]; with the flags' right-hand-side bits the corner terms
+        of ``vel_result`` [d][n] are subtracted before the division by det."""
         rhs = np.zeros(self.dims * self.ncells, self.dtype)
-        assert self._fn("o_advection_rhs")(C.byref(self.dom), int(apply_pressure_gradient), _p(rhs)) == 0
+        S = None
+        if flags & (NO_DIRECT_RHS | NO_DIAGONAL_RHS) and self.dims > 1:
+            S = np.concatenate([self.nonortho_rhs(OK_VEL, flags, k, vel_result) for k in range(self.dims)])
+        assert self._fn("o_advection_rhs")(C.byref(self.dom), int(apply_pressure_gradient), _p(S), _p(rhs)) == 0
         return rhs
 
-    def pressure_matrix(self, A: np.ndarray):
+    def pressure_matrix(self, A: np.ndarray, flags: int = 0):
         col = np.zeros(self.nnz, np.int32)
         val = np.zeros(self.nnz, self.dtype)
-        assert self._fn("o_pressure_matrix")(C.byref(self.dom), _p(A), _p(self.row), _p(col), _p(val)) == 0
+        assert self._fn("o_pressure_matrix")(C.byref(self.dom), _p(A), int(flags), _p(self.row), _p(col), _p(val)) == 0
         return col, val
 
     def pressure_rhs(self, A, ccol, cval, vel_result) -> np.ndarray:

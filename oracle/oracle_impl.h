@@ -63,6 +63,7 @@ static void FN(conn_pos)(const FN(octx) *c, const int *pos, int bdim, const ob_b
     const ob_block *ob = &c->dom->blocks[cb->conn_block];
     out[0] = pos[0]; out[1] = pos[1]; out[2] = pos[2];
     int ca = cb->conn_axes[0] >> 1;
+    if (border > ob->size[ca] - 1) border = ob->size[ca] - 1; /* one cell thick: stay inside */
     out[ca] = (cb->conn_axes[0] & 1) ? ob->size[ca] - 1 - border : border;
     if (c->d > 1) {
         int axis = (bdim + 1) % c->d;
@@ -261,6 +262,309 @@ static int FN(emit_row)(int *idx, REAL *vals, int n, int32_t *col, REAL *val, in
 
 static int FN(domain_ok)(const ob_domain *dom) { return dom && dom->dims >= 2 && dom->dims <= 3 && dom->nblocks > 0; }
 
+/* ---- non-orthogonal corner terms of skewed grids ----------------------------------------
+ * nonortho_laplace.cuh:53-131 (face coefficients of the matrices), block_data.cuh:280-420
+ * (neighbour and diagonal data), :477-616 (corner walk), nonortho_rhs.cuh:28-200 (explicit
+ * corner terms), with useFaceTransform false. Flag bits of non_ortho_flags.h. */
+
+typedef struct {
+    const FN(octx) *c;
+    int kind;
+    int w;
+    const REAL *data;            /* VEL, SCAL: [..][ncells] (component/channel w); PRES: [ncells] */
+    const ob_face_scalar *fscal; /* SCAL: [nblocks*6] face data of channel w */
+    int scalar_types;            /* TYPES: the face conditions are the scalar channel's (fscal) */
+} FN(ndata);
+
+/* getLaplaceCoefficientsFull / SingleAxis, discretization.cuh:419-470: det Minv[c1].Minv[c2] */
+static REAL FN(lapc_t)(const FN(octx) *c, const REAL *t, int c1, int c2) {
+    REAL r = 0;
+    for (int i = 0; i < c->d; ++i) r += t[c->d * c->d + c1 * c->d + i] * t[c->d * c->d + c2 * c->d + i];
+    return t[2 * c->d * c->d] * r;
+}
+static REAL FN(lapc)(const FN(octx) *c, const ob_block *b, const int *p, int c1, int c2) {
+    if (!b->has_transform) return c1 == c2 ? (REAL)1 : (REAL)0;
+    return FN(lapc_t)(c, FN(T)(b, p, c->ts), c1, c2);
+}
+/* computeConnectedDir, topology.cuh:236-244 */
+static int FN(conn_dir)(const FN(octx) *c, int dir, int bdim, const ob_bound *cb) {
+    return cb->conn_axes[FN(pos_mod)((dir >> 1) - bdim, c->d)] ^ (dir & 1);
+}
+static int FN(block_index)(const FN(octx) *c, const ob_block *b) { return (int)(b - c->dom->blocks); }
+
+static REAL FN(ncell)(const FN(ndata) *nd, const ob_block *b, const int *p) {
+    const size_t g = (size_t)b->global_offset + FN(flat)(p, b->size);
+    const size_t n = (size_t)nd->c->dom->ncells;
+    if (nd->kind == OK_VEL || nd->kind == OK_SCAL) return nd->data[(size_t)nd->w * n + g];
+    if (nd->kind == OK_PRES) return nd->data[g];
+    return 0;
+}
+/* getFixedBoundaryData (block_data.cuh:136-199); no pressure values live on fixed faces */
+static REAL FN(nfixed)(const FN(ndata) *nd, const ob_block *b, int bound, const int *p) {
+    if (nd->kind == OK_VEL) return FN(fixed_vel)(nd->c, b, bound, p, nd->w);
+    if (nd->kind == OK_SCAL) {
+        const ob_face_scalar *fs = &nd->fscal[FN(block_index)(nd->c, b) * 6 + bound];
+        if (!fs->values) return 0;
+        return ((const REAL *)fs->values)[fs->is_static ? 0 : FN(face_flat)(p, bound, b)];
+    }
+    return 0;
+}
+/* getFixedBoundaryType (block_data.cuh:110-134) */
+static int FN(nneumann)(const FN(ndata) *nd, const ob_block *b, int bound) {
+    if (nd->kind == OK_SCAL || (nd->kind == OK_TYPES && nd->scalar_types))
+        return nd->fscal[FN(block_index)(nd->c, b) * 6 + bound].neumann;
+    if (nd->kind == OK_VEL || nd->kind == OK_TYPES) return b->b[bound].vel_neumann;
+    return 0;
+}
+
+/* getCornerValue, block_data.cuh:501-616 (includeDepth0 false, maxDepth 2). Returns the value;
+ * *cells = cells averaged (0: from a fixed face), *neu = that face's condition is Neumann. */
+static REAL FN(corner)(const FN(ndata) *nd, const ob_block *b, const int *p, int dir1, int dir2, int depth1,
+                       int *cells, int *neu) {
+    const FN(octx) *c = nd->c;
+    REAL data = 0;
+    int num = 1;
+    *neu = 0;
+    const ob_block *cb[2] = {b, b};
+    int pos[2][3] = {{p[0], p[1], p[2]}, {p[0], p[1], p[2]}};
+    int dir[2][2] = {{dir1, dir2}, {dir2, dir1}};
+    for (int depth = 1; depth <= 2; ++depth) {
+        for (int d = 0; d < 2; ++d) {
+            int da = dir[d][0], db = dir[d][1], axis = da >> 1;
+            if (FN(at_bound)(pos[d], da, cb[d])) {
+                const ob_bound *bd = &cb[d]->b[da];
+                if (bd->type == OB_FIXED) {
+                    *neu = FN(nneumann)(nd, cb[d], da);
+                    *cells = 0;
+                    if (FN(at_bound)(pos[d], db, cb[d])) return FN(nfixed)(nd, cb[d], da, pos[d]);
+                    int q[3] = {pos[d][0], pos[d][1], pos[d][2]};
+                    q[db >> 1] += FN(face_sign)(db);
+                    return (FN(nfixed)(nd, cb[d], da, pos[d]) + FN(nfixed)(nd, cb[d], da, q)) * (REAL)0.5;
+                } else if (bd->type == OB_CONNECTED) {
+                    int q[3];
+                    FN(conn_pos)(c, pos[d], axis, bd, 1, q);
+                    cb[d] = &c->dom->blocks[bd->conn_block];
+                    pos[d][0] = q[0]; pos[d][1] = q[1]; pos[d][2] = q[2];
+                    dir[d][0] = FN(conn_dir)(c, db, axis, bd);
+                    dir[d][1] = FN(conn_dir)(c, da, axis, bd) ^ 1;
+                } else if (bd->type == OB_PERIODIC) {
+                    pos[d][axis] = (da & 1) ? 0 : cb[d]->size[axis] - 1;
+                    dir[d][0] = db;
+                    dir[d][1] = da ^ 1;
+                }
+            } else {
+                pos[d][axis] += FN(face_sign)(da);
+                dir[d][0] = db;
+                dir[d][1] = da ^ 1;
+            }
+            int o = d ^ 1;
+            if (cb[d] == cb[o] && pos[d][0] == pos[o][0] && pos[d][1] == pos[o][1] && pos[d][2] == pos[o][2]) {
+                *cells = num;
+                return data / (REAL)num;
+            }
+            if (depth > 1 || depth1) data += FN(ncell)(nd, cb[d], pos[d]);
+            ++num;
+        }
+    }
+    *cells = num;
+    return data / (REAL)num;
+}
+
+/* getBlockDataNeighbor (zeroBound false), block_data.cuh:276-320 */
+static REAL FN(nb_data)(const FN(ndata) *nd, const ob_block *b, const int *p, int dir) {
+    int axis = dir >> 1;
+    int q[3] = {p[0], p[1], p[2]};
+    if (FN(at_bound)(p, dir, b)) {
+        const ob_bound *bd = &b->b[dir];
+        if (bd->type == OB_FIXED) return FN(nfixed)(nd, b, dir, p);
+        if (bd->type == OB_CONNECTED) {
+            FN(conn_pos)(nd->c, p, axis, bd, 1, q);
+            return FN(ncell)(nd, &nd->c->dom->blocks[bd->conn_block], q);
+        }
+        if (bd->type == OB_PERIODIC) {
+            q[axis] = (dir & 1) ? 0 : b->size[axis] - 1;
+            return FN(ncell)(nd, b, q);
+        }
+        return 0;
+    }
+    q[axis] += FN(face_sign)(dir);
+    return FN(ncell)(nd, b, q);
+}
+
+/* getBlockDataNeighborDiagonal (zeroBound false), block_data.cuh:372-420 */
+static REAL FN(diag_data)(const FN(ndata) *nd, const ob_block *b, const int *p, int dir1, int dir2) {
+    int first_fixed = FN(is_empty)(b, dir1);
+    int dirs[2] = {first_fixed ? dir2 : dir1, first_fixed ? dir1 : dir2};
+    const ob_block *cb = b;
+    int q[3] = {p[0], p[1], p[2]};
+    for (int i = 0; i < 2; ++i) {
+        int dir = dirs[i], axis = dir >> 1;
+        if ((dir & 1) ? q[axis] == cb->size[axis] - 1 : q[axis] == 0) {
+            const ob_bound *bd = &cb->b[dir];
+            if (bd->type == OB_FIXED) return FN(nfixed)(nd, cb, dir, q);
+            if (bd->type == OB_CONNECTED) {
+                int r[3];
+                FN(conn_pos)(nd->c, q, axis, bd, 1, r);
+                cb = &nd->c->dom->blocks[bd->conn_block];
+                q[0] = r[0]; q[1] = r[1]; q[2] = r[2];
+            } else if (bd->type == OB_PERIODIC) {
+                q[axis] = (dir & 1) ? 0 : cb->size[axis] - 1;
+            } else {
+                return 0;
+            }
+        } else {
+            q[axis] += FN(face_sign)(dir);
+        }
+    }
+    return FN(ncell)(nd, cb, q);
+}
+
+/* interpolateNonOrthoLaplaceComponents, nonortho_laplace.cuh:53-131: the coefficient of face
+ * `face` towards tAxis, (alpha_P[axis,tAxis] raP + alpha_N[tAxis] raN) / 2; 0 for every face
+ * of a cell at either end of the face's axis when that face is prescribed (as the reference's
+ * test reads) */
+static REAL FN(face_alpha_m)(const FN(octx) *c, const ob_block *b, const int *p, int face, int tAxis, REAL raP,
+                             int with_visc, const REAL *A) {
+    int axis = face >> 1;
+    if (!((0 < p[axis] && p[axis] < b->size[axis] - 1) || !FN(is_empty)(b, face))) return 0;
+    int np[3], nw;
+    const ob_block *nb = FN(neighbor)(c, b, p, face, np, &nw);
+    REAL raN = A ? (REAL)1 / A[nb->global_offset + FN(flat)(np, nb->size)] : (REAL)1;
+    if (with_visc) raN = raN * FN(visc)(c, nb, np);
+    REAL aP = FN(lapc)(c, b, p, axis, tAxis);
+    REAL aN = FN(lapc)(c, nb, np, tAxis, axis);
+    return (aP * raP + aN * raN) * (REAL)0.5;
+}
+
+/* corner terms of face `bound` of a matrix row: PISO_build_matrix, advection.cu:177-240 (sign
+ * +1, diag = the diagonal accumulator); PISO_build_pressure_matrix, pressure.cu:104-160 (sign
+ * -1, diag = rv[0]) */
+static void FN(matrix_corner)(const FN(ndata) *nd, const ob_block *b, const int *p, int bound, int flags, REAL raP,
+                              int with_visc, const REAL *A, REAL vscale, int always_gradient, REAL sign, REAL *diag,
+                              REAL *rv) {
+    const FN(octx) *c = nd->c;
+    int nbr = flags & ONO_DIRECT_MATRIX, ctr = flags & ONO_CENTER_MATRIX;
+    if (c->d < 2 || !(nbr || ctr)) return;
+    int dim = bound >> 1, fs = FN(face_sign)(bound);
+    for (int i = 1; i < c->d; ++i) {
+        int tAxis = (dim + i) % c->d;
+        REAL alpha = FN(face_alpha_m)(c, b, p, bound, tAxis, raP, with_visc, A) * vscale;
+        if (alpha == 0) continue;
+        for (int tUp = 0; tUp < 2; ++tUp) {
+            int tFace = tAxis * 2 + tUp, tfs = FN(face_sign)(tFace);
+            int cells, neu;
+            FN(corner)(nd, b, p, bound, tFace, 0, &cells, &neu);
+            if (cells < 1) {
+                if (always_gradient || neu) {
+                    REAL k = (REAL)(fs * tfs) * alpha * (REAL)0.25;
+                    if (ctr) *diag -= sign * (3 * k);
+                    if (nbr) {
+                        rv[bound + 1] -= sign * (3 * k);
+                        rv[(tFace ^ 1) + 1] += sign * k;
+                    }
+                }
+            } else {
+                REAL k = (REAL)(fs * tfs) * alpha * ((REAL)1.0 / (REAL)cells);
+                if (ctr) *diag -= sign * k;
+                if (nbr) {
+                    rv[bound + 1] -= sign * k;
+                    rv[tFace + 1] -= sign * k;
+                }
+            }
+        }
+    }
+}
+
+/* getNonOrthoLaplaceRHS_v2, nonortho_rhs.cuh:28-200 (useFaceTransform false) */
+static REAL FN(nonortho_rhs)(const FN(ndata) *nd, const ob_block *b, const int *p, int flags, int with_visc,
+                             const REAL *A) {
+    const FN(octx) *c = nd->c;
+    if (c->d < 2 || !(flags & (ONO_DIRECT_RHS | ONO_DIAGONAL_RHS))) return 0;
+    REAL S = 0;
+    int f = FN(flat)(p, b->size);
+    REAL viscP = with_visc ? FN(visc)(c, b, p) : (REAL)1;
+    REAL raP = (A ? (REAL)1 / A[b->global_offset + f] : (REAL)1) * viscP;
+    for (int face = 0; face < 2 * c->d; ++face) {
+        int axis = face >> 1, fs = FN(face_sign)(face);
+        int atb = FN(at_bound)(p, face, b);
+        if (atb && FN(is_empty)(b, face)) {
+            if (nd->kind == OK_PRES || FN(nneumann)(nd, b, face)) continue;
+            const ob_bound *fb = &b->b[face];
+            for (int dd = 1; dd < c->d; ++dd) {
+                int tAxis = (axis + dd) % c->d;
+                REAL bA = fb->has_transform
+                              ? FN(lapc_t)(c, (const REAL *)fb->transform + (size_t)FN(face_flat)(p, face, b) * c->ts,
+                                           tAxis, axis)
+                              : (tAxis == axis ? (REAL)1 : (REAL)0);
+                int tLo = p[tAxis] == 0, tHi = p[tAxis] == b->size[tAxis] - 1;
+                int lo[3] = {p[0], p[1], p[2]}, hi[3] = {p[0], p[1], p[2]};
+                if (!tLo) lo[tAxis] -= 1;
+                if (!tHi) hi[tAxis] += 1;
+                REAL df = (tLo || tHi) ? (REAL)1.0 : (REAL)0.5;
+                REAL g = df * (FN(nfixed)(nd, b, face, hi) - FN(nfixed)(nd, b, face, lo));
+                S -= (REAL)fs * bA * g * viscP;
+            }
+            continue;
+        }
+        /* resolveNeighborCell, topology.cuh:352-398 */
+        int np[3], nw;
+        const ob_block *nb = FN(neighbor)(c, b, p, face, np, &nw);
+        int map[3] = {0, 2, 4};
+        if (atb && b->b[face].type == OB_CONNECTED)
+            for (int k = 0; k < c->d; ++k) map[k] = FN(conn_dir)(c, map[k], axis, &b->b[face]);
+        REAL raN = A ? (REAL)1 / A[nb->global_offset + FN(flat)(np, nb->size)] : (REAL)1;
+        if (with_visc) raN = raN * FN(visc)(c, nb, np);
+        for (int dd = 1; dd < c->d; ++dd) {
+            int tAxis = (axis + dd) % c->d;
+            REAL aN = nb->has_transform ? FN(lapc)(c, nb, np, map[tAxis] >> 1, map[axis] >> 1)
+                                        : (tAxis == axis ? (REAL)1 : (REAL)0);
+            REAL faceAlpha = (FN(lapc)(c, b, p, axis, tAxis) * raP + aN * raN) * (REAL)0.5;
+            REAL tg = 0;
+            for (int tUp = 0; tUp < 2; ++tUp) {
+                int tFace = tAxis * 2 + tUp, tfs = FN(face_sign)(tFace);
+                int cells, neu;
+                REAL v = FN(corner)(nd, b, p, face, tFace, (flags & ONO_DIRECT_RHS) != 0, &cells, &neu);
+                if (cells == 0 && (neu || nd->kind == OK_PRES)) {
+                    int tOther = tFace ^ 1;
+                    if (flags & ONO_DIRECT_RHS) {
+                        tg += (REAL)tfs * FN(nb_data)(nd, b, p, face) * (REAL)0.75;
+                        tg += (REAL)(-tfs) * FN(nb_data)(nd, b, p, tOther) * (REAL)0.25;
+                    }
+                    if (flags & ONO_DIAGONAL_RHS) tg += (REAL)(-tfs) * FN(diag_data)(nd, b, p, face, tOther) * (REAL)0.25;
+                } else {
+                    tg += (REAL)tfs * v;
+                }
+            }
+            S -= (REAL)fs * faceAlpha * tg;
+        }
+    }
+    return S;
+}
+
+/* S of every cell (the explicit corner terms; k_pressureRHSaddNonOrthoComponents adds them to
+ * the pressure divergence, kPISO_build_advection_RHS and the scalar right-hand side subtract
+ * them before dividing by det). kind OK_VEL (component w of vel_result [d][n], viscosity in
+ * the coefficients), OK_PRES (pressure result, 1/A in the coefficients), OK_SCAL (channel w of
+ * [C][n]; the caller multiplies by the diffusivity). */
+int FN(o_nonortho_rhs)(const ob_domain *dom, int kind, int flags, int w, const REAL *data, const REAL *adiag,
+                       const ob_face_scalar *fscal, REAL *out) {
+    if (!FN(domain_ok)(dom)) return -1;
+    if (kind == OK_PRES && !adiag) return -1;
+    if (kind == OK_SCAL && !fscal) return -1;
+    FN(octx) c = {dom, dom->dims, 2 * dom->dims * dom->dims + 1};
+    FN(ndata) nd = {&c, kind, w, data, fscal, 0};
+    for (int bi = 0; bi < dom->nblocks; ++bi) {
+        const ob_block *b = &dom->blocks[bi];
+        int n = FN(ncells)(b);
+        for (int f = 0; f < n; ++f) {
+            int p[3];
+            FN(unflat)(f, b->size, p);
+            out[b->global_offset + f] = FN(nonortho_rhs)(&nd, b, p, flags, kind == OK_VEL, kind == OK_PRES ? adiag : NULL);
+        }
+    }
+    return 0;
+}
+
 /* CSR pattern row offsets: rows in global cell order, row sizes 2d+1 minus the prescribed
  * faces of the cell (getCSRMatrixRowEndOffsetFromBlockBoundaries3D, topology.cuh:168-233). */
 int FN(o_csr_rows)(const ob_domain *dom, int32_t *row) {
@@ -284,9 +588,13 @@ int FN(o_csr_rows)(const ob_domain *dom, int32_t *row) {
 }
 
 /* PISO_build_matrix, advection.cu:26-306 (CENTRAL scheme, orthogonal part) */
-int FN(o_advection_matrix)(const ob_domain *dom, const int32_t *row, int32_t *col, REAL *val, REAL *adiag) {
+/* flags: non-orthogonal flag bits; scalar_mode: the passive-scalar variant, whose corner
+ * coefficients carry no viscosity and are multiplied by the (domain) diffusivity */
+int FN(o_advection_matrix)(const ob_domain *dom, int flags, int scalar_mode, const int32_t *row, int32_t *col,
+                           REAL *val, REAL *adiag) {
     if (!FN(domain_ok)(dom)) return -1;
     FN(octx) c = {dom, dom->dims, 2 * dom->dims * dom->dims + 1};
+    FN(ndata) nd = {&c, OK_TYPES, 0, NULL, NULL, 0};
     const REAL dt = (REAL)dom->dt;
     for (int bi = 0; bi < dom->nblocks; ++bi) {
         const ob_block *b = &dom->blocks[bi];
@@ -321,6 +629,9 @@ int FN(o_advection_matrix)(const ob_domain *dom, const int32_t *row, int32_t *co
                     REAL vc = (ap[dim] * vp + an * vn) * (REAL)0.5;
                     diag += vc;
                     rv[bound + 1] -= vc;
+                    if (flags)
+                        FN(matrix_corner)(&nd, b, p, bound, flags, scalar_mode ? (REAL)1 : vp, !scalar_mode, NULL,
+                                          scalar_mode ? vp : (REAL)1, 0, (REAL)1, &diag, rv);
                 } else {
                     int slip = b->b[bound].vel_neumann ? 1 : 0;
                     diag += (REAL)(1 - slip) * 2 * vp * ap[dim];
@@ -339,7 +650,7 @@ int FN(o_advection_matrix)(const ob_domain *dom, const int32_t *row, int32_t *co
 }
 
 /* kPISO_build_advection_RHS, advection.cu:799-955 (CENTRAL, orthogonal part) */
-int FN(o_advection_rhs)(const ob_domain *dom, int apply_pressure_gradient, REAL *rhs);
+int FN(o_advection_rhs)(const ob_domain *dom, int apply_pressure_gradient, const REAL *S, REAL *rhs);
 /* getPressureGradient + getPressureAtWithBounds, velocity.cuh:347-382, nonortho_laplace.cuh:302-378 */
 static REAL FN(pressure_at)(const FN(octx) *c, const ob_block *b, const int *pin) {
     int t[3] = {pin[0], pin[1], pin[2]};
@@ -396,7 +707,8 @@ static void FN(pressure_grad)(const FN(octx) *c, const ob_block *b, const int *p
     for (int j = 0; j < c->d; ++j) g[j] = gr[j];
 }
 
-int FN(o_advection_rhs)(const ob_domain *dom, int apply_pressure_gradient, REAL *rhs) {
+/* S: the explicit non-orthogonal corner terms [d][n] (o_nonortho_rhs with OK_VEL) or NULL */
+int FN(o_advection_rhs)(const ob_domain *dom, int apply_pressure_gradient, const REAL *S, REAL *rhs) {
     if (!FN(domain_ok)(dom)) return -1;
     FN(octx) c = {dom, dom->dims, 2 * dom->dims * dom->dims + 1};
     const REAL dt = (REAL)dom->dt;
@@ -421,6 +733,7 @@ int FN(o_advection_rhs)(const ob_domain *dom, int apply_pressure_gradient, REAL 
                     REAL slip = b->b[bound].vel_neumann ? 1 : 0;
                     t += vel * (1 - slip) * bv * 2 * alpha;
                 }
+                if (S) t -= S[(size_t)dim * dom->ncells + b->global_offset + f];
                 t /= det;
                 t += FN(source)(b, p, dim);
                 if (apply_pressure_gradient) {
@@ -436,9 +749,11 @@ int FN(o_advection_rhs)(const ob_domain *dom, int apply_pressure_gradient, REAL 
 }
 
 /* PISO_build_pressure_matrix, pressure.cu:25-193 (orthogonal part, useFaceTransform=false) */
-int FN(o_pressure_matrix)(const ob_domain *dom, const REAL *adiag, const int32_t *row, int32_t *col, REAL *val) {
+int FN(o_pressure_matrix)(const ob_domain *dom, const REAL *adiag, int flags, const int32_t *row, int32_t *col,
+                          REAL *val) {
     if (!FN(domain_ok)(dom)) return -1;
     FN(octx) c = {dom, dom->dims, 2 * dom->dims * dom->dims + 1};
+    FN(ndata) nd = {&c, OK_TYPES, 0, NULL, NULL, 0};
     for (int bi = 0; bi < dom->nblocks; ++bi) {
         const ob_block *b = &dom->blocks[bi];
         int n = FN(ncells)(b);
@@ -464,6 +779,7 @@ int FN(o_pressure_matrix)(const ob_domain *dom, const REAL *adiag, const int32_t
                     rv[0] -= coef;
                     rv[bound + 1] += coef;
                     idx[bound + 1] = ng;
+                    if (flags) FN(matrix_corner)(&nd, b, p, bound, flags, raP, 0, adiag, (REAL)1, 1, (REAL)-1, &rv[0], rv);
                 } else {
                     idx[bound + 1] = -1;
                 }

@@ -171,6 +171,9 @@ class OraclePISO:
         self.dt = dt
         self.corrector_steps = corrector_steps
         self.non_orthogonal = non_orthogonal
+        # the reference's default mode (piso_simulation.py:1112-1126): corner terms of the
+        # matrices' center and direct neighbours, diagonal neighbours on the right-hand side
+        self.flags = O.NO_DEFAULT_MODE if non_orthogonal else 0
         default = 1e-8 if od.dtype == np.float64 else 1e-5
         self.adv_tol = advection_tol if advection_tol is not None else default
         self.p_tol = pressure_tol if pressure_tol is not None else default
@@ -280,9 +283,15 @@ class OraclePISO:
         n, C_ = od.ncells, od.channels
         split = od.scalar_visc is not None and len(od.scalar_visc) > 1
         for_velocity = not (od.scalar_visc is not None or any(sp.viscosity is not None for sp in od.specs) or split)
-        rhs = od.scalar_rhs()
+        sres = None
+        if self.flags:
+            sres = np.zeros((C_, n), od.dtype)
+            for bi, spec in enumerate(od.specs):
+                sres[:, od.offsets[bi]:od.offsets[bi] + spec.ncells] = od.scalar[bi]
+        rhs = od.scalar_rhs(self.flags, sres)
         tol = resolve_tol(self.adv_tol, rhs.reshape(-1))
-        mats = [od.scalar_matrix(ch) for ch in range(C_)] if split else [od.scalar_matrix(0)] * C_
+        mats = ([od.scalar_matrix(ch, self.flags) for ch in range(C_)] if split
+                else [od.scalar_matrix(0, self.flags)] * C_)
         result = np.zeros_like(rhs)
         for ch in range(C_):
             col, val, _ = mats[ch]
@@ -306,13 +315,13 @@ class OraclePISO:
             self.pre_velocity_setup(self)
         if self.mhd is not None:
             self._lorentz()
-        ccol, cval, A = scalar_matrix if scalar_matrix is not None else od.advection_matrix()
+        ccol, cval, A = scalar_matrix if scalar_matrix is not None else od.advection_matrix(self.flags)
         if not self.non_orthogonal:
             rhs = od.advection_rhs()
             x = self.vel_result
         else:
             self.vel_result = self._blocks_velocity_flat()
-            rhs = od.advection_rhs()
+            rhs = od.advection_rhs(flags=self.flags, vel_result=self.vel_result)
             x = np.zeros_like(self.vel_result)
         if np.any(rhs != 0):
             adv_tol = resolve_tol(self.adv_tol, rhs)
@@ -328,9 +337,12 @@ class OraclePISO:
             x[...] = 0
         self.vel_result = x
         for cstep in range(self.corrector_steps):
-            pcol, pval = od.pressure_matrix(A)
+            pcol, pval = od.pressure_matrix(A, self.flags)
             prhs = od.pressure_rhs(A, ccol, cval, self.vel_result)
             div = od.pressure_div(prhs)
+            if self.flags and od.dims > 1:
+                # k_pressureRHSaddNonOrthoComponents, pressure.cu:838-860
+                div = div + od.nonortho_rhs(O.OK_PRES, self.flags, 0, self.pressure_result, A)
             p = np.zeros(n, od.dtype)
             hier = None
             if self.pressure_use_amg:

@@ -1,11 +1,12 @@
 // PISO assembly kernels for gfx950: one work-item per (cell, environment), the grid's y
 // dimension enumerates batched environments. Each kernel restates one reference kernel
-// (cited at its definition) for the CENTRAL advection scheme on grids whose
-// non-orthogonal corner terms vanish.
+// (cited at its definition) for the CENTRAL advection scheme; the non-orthogonal corner terms
+// of skewed grids are added where the flags ask for them.
 //
 // Matrix kernels write the values of rows whose pattern (row offsets, ascending columns)
 // is fixed per domain; k_csr_pattern writes that pattern once per domain.
 #include "pp_common.hpp"
+#include "pp_nonortho.hpp"
 
 #include <vector>
 
@@ -150,8 +151,8 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_csr_pattern(pp_domain dom, int32_
 // PISO_build_matrix, advection.cu:26-306 (CENTRAL): values of C and the diagonal A. With
 // sc >= 0 the matrix of passive-scalar channel sc (forPassiveScalar): the channel's
 // diffusivity and its boundary condition at the prescribed faces.
-template <typename T, int D>
-__global__ void __launch_bounds__(ASM_BLOCK) k_advection_matrix(pp_domain dom, int sc) {
+template <typename T, int D, bool NO>
+__global__ void __launch_bounds__(ASM_BLOCK) k_advection_matrix(pp_domain dom, int sc, int flags) {
     CELL_PROLOGUE
     const T dt = reinterpret_cast<const T *>(dom.time_step)[env];
     T fl[2 * D];
@@ -185,6 +186,18 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_advection_matrix(pp_domain dom, i
             const T vc = (ap[dim] * vp + an * vn) * T(0.5);
             diag += vc;
             rv[bound + 1] -= vc;
+            if (NO && flags) {
+                if (sc < 0) {
+                    const CornerData<T, D, NK_VELOCITY_TYPES> cd{dom, blocks, 0, env};
+                    matrix_corner_terms<T, D, NK_VELOCITY_TYPES>(cd, bi, p, f, bound, flags, vp, true, nullptr, T(1),
+                                                                  false, T(1), diag, rv);
+                } else {
+                    // scalar: coefficients without viscosity, times the channel's diffusivity
+                    const CornerData<T, D, NK_SCALAR_TYPES> cd{dom, blocks, sc, env};
+                    matrix_corner_terms<T, D, NK_SCALAR_TYPES>(cd, bi, p, f, bound, flags, T(1), false, nullptr, vp,
+                                                                false, T(1), diag, rv);
+                }
+            }
         } else {
             const int slip = sc < 0 ? (b.bnd[bound].vel_type == PP_BC_DIRICHLET ? 0 : 1)
                                     : (scalar_neumann(b.bnd[bound], sc) ? 1 : 0);
@@ -292,8 +305,8 @@ __device__ __forceinline__ T boundary_source(const pp_domain &dom, const pp_bloc
 }
 
 // kPISO_build_advection_RHS, advection.cu:799-955 (CENTRAL)
-template <typename T, int D>
-__global__ void __launch_bounds__(ASM_BLOCK) k_advection_rhs(pp_domain dom, int apply_pg) {
+template <typename T, int D, bool NO>
+__global__ void __launch_bounds__(ASM_BLOCK) k_advection_rhs(pp_domain dom, int apply_pg, int flags) {
     CELL_PROLOGUE
     const T dt = reinterpret_cast<const T *>(dom.time_step)[env];
     const T det = Metric<T, D>::det(b, f);
@@ -320,6 +333,10 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_advection_rhs(pp_domain dom, int 
             const T slip = fb.vel_type == PP_BC_DIRICHLET ? T(0) : T(1);
             t += v * (1 - slip) * bv * 2 * alpha;
         }
+        if (NO && flags) {
+            const CornerData<T, D, NK_VELOCITY> cd{dom, blocks, dim, env};
+            t -= nonortho_rhs<T, D, NK_VELOCITY>(cd, bi, p, f, flags, true, nullptr);
+        }
         t /= det;
         t += vsource<T>(dom, b, f, dim, env);
         if (apply_pg) t -= pg[dim] * dt;
@@ -331,8 +348,8 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_advection_rhs(pp_domain dom, int 
 // channel; prescribed faces give the advected face value and the diffusive flux (Dirichlet:
 // the face value with the ghost-cell coefficient 2 alpha; Neumann: the cell value advected and
 // the prescribed normal gradient times the face area)
-template <typename T, int D>
-__global__ void __launch_bounds__(ASM_BLOCK) k_scalar_rhs(pp_domain dom) {
+template <typename T, int D, bool NO>
+__global__ void __launch_bounds__(ASM_BLOCK) k_scalar_rhs(pp_domain dom, int flags) {
     CELL_PROLOGUE
     const int C = dom.scalar_channels;
     const T dt = reinterpret_cast<const T *>(dom.time_step)[env];
@@ -367,13 +384,17 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_scalar_rhs(pp_domain dom) {
                 t += s * visc * area;
             }
         }
+        if (NO && flags) {
+            const CornerData<T, D, NK_SCALAR> cd{dom, blocks, ch, env};
+            t -= nonortho_rhs<T, D, NK_SCALAR>(cd, bi, p, f, flags, false, nullptr) * scalar_visc<T>(dom, ch);
+        }
         rhs[(size_t)ch * dom.num_cells + g] = t / det;
     }
 }
 
 // PISO_build_pressure_matrix, pressure.cu:25-193 (useFaceTransform = false)
-template <typename T, int D>
-__global__ void __launch_bounds__(ASM_BLOCK) k_pressure_matrix(pp_domain dom) {
+template <typename T, int D, bool NO>
+__global__ void __launch_bounds__(ASM_BLOCK) k_pressure_matrix(pp_domain dom, int flags) {
     CELL_PROLOGUE
     const T *A = reinterpret_cast<const T *>(dom.A) + (size_t)env * dom.num_cells;
     int idx[2 * D + 1];
@@ -400,6 +421,11 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_pressure_matrix(pp_domain dom) {
             rv[0] -= coef;
             rv[bound + 1] += coef;
             idx[bound + 1] = ng;
+            if (NO && flags) {
+                const CornerData<T, D, NK_FIXED_ONLY> cd{dom, blocks, 0, env};
+                matrix_corner_terms<T, D, NK_FIXED_ONLY>(cd, bi, p, f, bound, flags, raP, false, A, T(1), true, T(-1),
+                                                         rv[0], rv);
+            }
         } else {
             idx[bound + 1] = -1;
         }
@@ -461,8 +487,10 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_pressure_rhs(pp_domain dom) {
 }
 
 // k_computePressureRHSdivergenceFromFlux, pressure.cu:755-803 (useFaceTransform = false)
-template <typename T, int D>
-__global__ void __launch_bounds__(ASM_BLOCK) k_pressure_div(pp_domain dom, int time_step_norm) {
+template <typename T, int D, bool NO>
+// With the flags' right-hand-side bits it also adds the pressure's explicit corner terms
+// (k_pressureRHSaddNonOrthoComponents, pressure.cu:838-860).
+__global__ void __launch_bounds__(ASM_BLOCK) k_pressure_div(pp_domain dom, int time_step_norm, int flags) {
     CELL_PROLOGUE
     const T *prhs = reinterpret_cast<const T *>(dom.pressure_rhs) + (size_t)env * D * dom.num_cells;
     T fl[2 * D];
@@ -471,6 +499,11 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_pressure_div(pp_domain dom, int t
 #pragma unroll
     for (int dim = 0; dim < D; ++dim) dv += fl[dim * 2 + 1] - fl[dim * 2];
     if (time_step_norm) dv /= reinterpret_cast<const T *>(dom.time_step)[env];
+    if (NO && (flags & (NO_DIRECT_RHS | NO_DIAGONAL_RHS))) {
+        const CornerData<T, D, NK_PRESSURE> cd{dom, blocks, 0, env};
+        dv += nonortho_rhs<T, D, NK_PRESSURE>(cd, bi, p, f, flags, false,
+                                              reinterpret_cast<const T *>(dom.A) + (size_t)env * dom.num_cells);
+    }
     reinterpret_cast<T *>(dom.pressure_rhs_div)[(size_t)env * dom.num_cells + g] = dv;
 }
 
@@ -1554,79 +1587,88 @@ int pp_csr_pattern(const pp_domain *dom, int32_t *row, int32_t *col, void *strea
     return PP_OK;
 }
 
+// Flags without a corner term on either side (0, or only NON_ORTHO_DIAGONAL_MATRIX, which the
+// reference does not implement) take the orthogonal path.
+static inline int matrix_flags(int32_t f) { return f & (NO_DIRECT_MATRIX | NO_CENTER_MATRIX); }
+static inline int rhs_flags(int32_t f) { return f & (NO_DIRECT_RHS | NO_DIAGONAL_RHS); }
+
 int pp_setup_advection_matrix(const pp_domain *dom, int32_t non_ortho_flags, void *stream) {
-    (void)non_ortho_flags;
     if (int e = check_domain(dom, __func__)) return e;
+    const int flags = matrix_flags(non_ortho_flags);
     PP_CHECK_ARG(dom->C.val && dom->C.row && dom->A && dom->time_step, "C, A and time_step must be set");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        PP_TIMED(PP_K_ADVECTION_MATRIX, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_advection_matrix<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom, -1))));
+        PP_TIMED(PP_K_ADVECTION_MATRIX, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((flags ? k_advection_matrix<T, D, true> : k_advection_matrix<T, D, false>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom, -1, flags))));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
 
 int pp_setup_advection_matrix_scalar(const pp_domain *dom, int32_t non_ortho_flags, int32_t channel, void *stream) {
-    (void)non_ortho_flags;
     if (int e = check_domain(dom, __func__)) return e;
+    const int flags = matrix_flags(non_ortho_flags);
     PP_CHECK_ARG(dom->C.val && dom->C.row && dom->A && dom->time_step, "C, A and time_step must be set");
     PP_CHECK_ARG(channel >= 0 && channel < dom->scalar_channels, "passive scalar channel out of range");
     PP_CHECK_ARG(dom->scalar_visc_count == 0 || dom->scalar_viscosity, "scalar viscosity not set");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        hipLaunchKernelGGL((k_advection_matrix<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom, (int)channel)));
+        hipLaunchKernelGGL((flags ? k_advection_matrix<T, D, true> : k_advection_matrix<T, D, false>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom, (int)channel, flags)));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
 
 int pp_setup_advection_scalar(const pp_domain *dom, int32_t non_ortho_flags, void *stream) {
-    (void)non_ortho_flags;
     if (int e = check_domain(dom, __func__)) return e;
+    const int flags = rhs_flags(non_ortho_flags);
     PP_CHECK_ARG(dom->scalar_channels > 0 && dom->scalar_rhs && dom->time_step, "passive scalar fields must be set");
+    PP_CHECK_ARG(!flags || dom->scalar_result, "the non-orthogonal terms read scalar_result");
     PP_CHECK_ARG(dom->scalar_visc_count == 0 || dom->scalar_viscosity, "scalar viscosity not set");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        hipLaunchKernelGGL((k_scalar_rhs<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom)));
+        hipLaunchKernelGGL((flags ? k_scalar_rhs<T, D, true> : k_scalar_rhs<T, D, false>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom, flags)));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
 
 int pp_setup_advection_velocity(const pp_domain *dom, int32_t non_ortho_flags, int32_t apply_pressure_gradient, void *stream) {
-    (void)non_ortho_flags;
     if (int e = check_domain(dom, __func__)) return e;
+    const int flags = rhs_flags(non_ortho_flags);
     PP_CHECK_ARG(dom->velocity_rhs && dom->time_step, "velocity_rhs and time_step must be set");
+    PP_CHECK_ARG(!flags || dom->velocity_result, "the non-orthogonal terms read velocity_result");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        PP_TIMED(PP_K_ADVECTION_RHS, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_advection_rhs<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
-                           (int)apply_pressure_gradient))));
+        PP_TIMED(PP_K_ADVECTION_RHS, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((flags ? k_advection_rhs<T, D, true> : k_advection_rhs<T, D, false>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+                           (int)apply_pressure_gradient, flags))));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
 
 int pp_setup_pressure_matrix(const pp_domain *dom, int32_t non_ortho_flags, void *stream) {
-    (void)non_ortho_flags;
     if (int e = check_domain(dom, __func__)) return e;
+    const int flags = matrix_flags(non_ortho_flags);
     PP_CHECK_ARG(dom->P.val && dom->P.row && dom->A, "P and A must be set");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        PP_TIMED(PP_K_PRESSURE_MATRIX, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_pressure_matrix<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom))));
+        PP_TIMED(PP_K_PRESSURE_MATRIX, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((flags ? k_pressure_matrix<T, D, true> : k_pressure_matrix<T, D, false>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom, flags))));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
 
 int pp_setup_pressure_rhs(const pp_domain *dom, int32_t non_ortho_flags, int32_t time_step_norm, void *stream) {
-    (void)non_ortho_flags;
     if (int e = check_domain(dom, __func__)) return e;
+    const int flags = rhs_flags(non_ortho_flags);
     PP_CHECK_ARG(dom->pressure_rhs && dom->pressure_rhs_div && dom->C.val && dom->velocity_result, "pressure fields must be set");
+    PP_CHECK_ARG(!flags || (dom->pressure_result && dom->A), "the non-orthogonal terms read pressure_result and A");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
         PP_TIMED(PP_K_PRESSURE_RHS, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_pressure_rhs<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom));
-        PP_TIMED(PP_K_PRESSURE_DIV, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_pressure_div<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
-                           (int)time_step_norm))));
+        PP_TIMED(PP_K_PRESSURE_DIV, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((flags ? k_pressure_div<T, D, true> : k_pressure_div<T, D, false>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+                           (int)time_step_norm, flags))));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
 
 int pp_setup_pressure_rhs_div(const pp_domain *dom, int32_t non_ortho_flags, int32_t time_step_norm, void *stream) {
-    (void)non_ortho_flags;
     if (int e = check_domain(dom, __func__)) return e;
+    const int flags = rhs_flags(non_ortho_flags);
     PP_CHECK_ARG(dom->pressure_rhs && dom->pressure_rhs_div, "pressure fields must be set");
+    PP_CHECK_ARG(!flags || (dom->pressure_result && dom->A), "the non-orthogonal terms read pressure_result and A");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        PP_TIMED(PP_K_PRESSURE_DIV, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((k_pressure_div<T, D>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
-                           (int)time_step_norm))));
+        PP_TIMED(PP_K_PRESSURE_DIV, dom->num_cells, as_stream(stream), hipLaunchKernelGGL((flags ? k_pressure_div<T, D, true> : k_pressure_div<T, D, false>), block_grid(dom, dom->batch), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+                           (int)time_step_norm, flags))));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }

@@ -112,11 +112,14 @@ __device__ __forceinline__ int face_flat(const int *p, int bound, const pp_block
 __device__ __forceinline__ int pos_mod(int a, int m) { int r = a % m; return r < 0 ? r + m : r; }
 
 // Position in the connected block across face `bdim` (computeConnectedPos, topology.cuh:246-266).
+// `border` cells in from the connected face; a block one cell thick there clamps it to 0 (the
+// reference would read outside the block).
 __device__ __forceinline__ void conn_pos(const pp_block_desc *__restrict__ blocks, int d, const int *pos, int bdim,
                                          const pp_boundary_desc &cb, int border, int *out) {
     const pp_block_desc &ob = blocks[cb.conn_block];
     out[0] = pos[0]; out[1] = pos[1]; out[2] = pos[2];
     int ca = cb.conn_axes[0] >> 1;
+    if (border > ob.size[ca] - 1) border = ob.size[ca] - 1;
     out[ca] = (cb.conn_axes[0] & 1) ? ob.size[ca] - 1 - border : border;
     if (d > 1) {
         int axis = (bdim + 1) % d;

@@ -75,6 +75,22 @@ def transform_size(dims: int) -> int:
     return 2 * dims * dims + 1
 
 
+def _metric_orthogonality(t: torch.Tensor, dims: int) -> tuple:
+    """(off-diagonal ``Minv Minv^T`` within 1e-10 relative, exactly zero) of transforms
+    ``[..., 2d^2+1]``. The exact test uses the products ``Minv[i][k] Minv[j][k]`` the assembly
+    sums, so an exact zero here means the non-orthogonal terms are exactly zero."""
+    minv = t.reshape(-1, transform_size(dims))[:, dims * dims:2 * dims * dims].reshape(-1, dims, dims)
+    g = (minv.to(torch.float64) @ minv.to(torch.float64).transpose(1, 2))
+    diag = torch.diagonal(g, dim1=1, dim2=2)
+    off = g - torch.diag_embed(diag)
+    scale = diag.abs().amax(dim=1, keepdim=True).clamp_min(1e-300)
+    near = bool(((off.abs().amax(dim=2) / scale) <= 1e-10).all())
+    prod = minv.unsqueeze(2) * minv.unsqueeze(1)  # [cells, i, j, k]
+    eye = torch.eye(dims, dtype=torch.bool, device=t.device)
+    exact = bool((prod[:, ~eye] == 0).all())
+    return near, exact
+
+
 def transform_vectors(vectors: torch.Tensor, transforms: torch.Tensor, inverse: bool) -> torch.Tensor:
     """Vectors ``[N, d, (z), y, x]`` mapped by the cell transforms ``[1|N, (z), y, x, 2d^2+1]``:
     ``Minv v`` (inverse, physical to computational) or ``M v`` (TransformVectors,
@@ -721,23 +737,30 @@ class Block:
         return self.faceTransform is not None
 
     def isOrthogonal(self) -> bool:
-        """True when the metric ``Minv Minv^T`` of every cell is diagonal, i.e. the
-        non-orthogonal correction terms of the reference's assembly vanish (one device sync per
-        new transform)."""
+        """True when the metric ``Minv Minv^T`` of every cell is diagonal to 1e-10 relative, i.e.
+        the non-orthogonal correction terms of the reference's assembly are negligible (one device
+        sync per new transform)."""
+        return self._orthogonality()[0]
+
+    def nonOrthoTermsVanish(self) -> bool:
+        """True when the non-orthogonal correction terms are exactly zero: the off-diagonal
+        products ``Minv[i] . Minv[j]`` of every cell and of every fixed face's transform are 0."""
+        if not self._orthogonality()[1]:
+            return False
+        for bound in self.getBoundaries():
+            if isinstance(bound, FixedBoundary) and bound.transform is not None:
+                if not _metric_orthogonality(bound.transform, self.getSpatialDims())[1]:
+                    return False
+        return True
+
+    def _orthogonality(self) -> tuple:
         t = self.transform
         if t is None:
-            return True
+            return True, True
         key = (t.data_ptr(), t._version)
-        if getattr(self, "_ortho_key", None) == key:
-            return self._ortho
-        dims = self.getSpatialDims()
-        minv = t.reshape(-1, transform_size(dims))[:, dims * dims:2 * dims * dims].reshape(-1, dims, dims)
-        g = (minv.to(torch.float64) @ minv.to(torch.float64).transpose(1, 2))
-        diag = torch.diagonal(g, dim1=1, dim2=2)
-        off = g - torch.diag_embed(diag)
-        scale = diag.abs().amax(dim=1, keepdim=True).clamp_min(1e-300)
-        self._ortho = bool(((off.abs().amax(dim=2) / scale) <= 1e-10).all())
-        self._ortho_key = key
+        if getattr(self, "_ortho_key", None) != key:
+            self._ortho = _metric_orthogonality(t, self.getSpatialDims())
+            self._ortho_key = key
         return self._ortho
 
     def faceTransformBoundarySlice(self, face: int) -> torch.Tensor:
@@ -1442,6 +1465,11 @@ class Domain:
 
     def isOrthogonal(self) -> bool:
         return all(b.isOrthogonal() for b in self._blocks)
+
+    def nonOrthoTermsVanish(self) -> bool:
+        """True when every block's non-orthogonal correction terms are exactly zero (the
+        assembly then skips them)."""
+        return all(b.nonOrthoTermsVanish() for b in self._blocks)
 
     def getVertexCoordinates(self) -> list:
         return [b.vertexCoordinates for b in self._blocks]

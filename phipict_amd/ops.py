@@ -24,18 +24,23 @@ NON_ORTHO_DIAGONAL_RHS = 8
 NON_ORTHO_CENTER_MATRIX = 16
 
 
-def _prep(domain: Domain, timeStep=None, nonOrthoFlags: int = 0) -> _lib.DomainDesc:
+def _prep(domain: Domain, timeStep=None) -> _lib.DomainDesc:
     if not domain.IsInitialized():
         raise RuntimeError("Domain is not initialized.")
     if domain.getNumBlocks() == 0:
         raise RuntimeError("Domain does not contain blocks.")
-    if nonOrthoFlags and not domain.isOrthogonal():
-        # the corner/diagonal correction terms of skewed grids (piso/device/nonortho_*.cuh) are
-        # not assembled; on grids whose metric is diagonal they vanish and the flags are no-ops
-        raise NotImplementedError("non-orthogonal correction terms on skewed grids are not implemented")
     if timeStep is not None:
         domain.setTimeStep(timeStep)
     return domain.desc()
+
+
+def _flags(domain: Domain, nonOrthoFlags: int) -> int:
+    """The non-orthogonal flags the kernels get: 0 when the terms are exactly zero on this
+    grid (every metric diagonal), so rectilinear grids take the kernels' orthogonal path."""
+    nonOrthoFlags = int(nonOrthoFlags)
+    if nonOrthoFlags and domain.nonOrthoTermsVanish():
+        return 0
+    return nonOrthoFlags
 
 
 def _stream(domain: Domain) -> int:
@@ -45,14 +50,15 @@ def _stream(domain: Domain) -> int:
 def SetupAdvectionMatrix(domain: Domain, timeStep, nonOrthoFlags: int = 0, forPassiveScalar: bool = False,
                          passiveScalarChannel: int = 0) -> None:
     """Implicit advection-diffusion matrix C and its diagonal A (bindings.cpp:639-641)."""
-    d = _prep(domain, timeStep, nonOrthoFlags)
+    d = _prep(domain, timeStep)
+    flags = _flags(domain, nonOrthoFlags)
     if forPassiveScalar:
         if not domain.hasPassiveScalar():
             raise RuntimeError("the domain has no passive scalar")
-        _lib.call("pp_setup_advection_matrix_scalar", C.byref(d), int(nonOrthoFlags), int(passiveScalarChannel),
+        _lib.call("pp_setup_advection_matrix_scalar", C.byref(d), flags, int(passiveScalarChannel),
                   _stream(domain))
     else:
-        _lib.call("pp_setup_advection_matrix", C.byref(d), int(nonOrthoFlags), _stream(domain))
+        _lib.call("pp_setup_advection_matrix", C.byref(d), flags, _stream(domain))
     domain.C.mark_ell_valid()
 
 
@@ -61,21 +67,22 @@ def SetupAdvectionScalar(domain: Domain, timeStep, nonOrthoFlags: int = 0) -> No
     (bindings.cpp:642-644 -> advection.cu:1301-1373)."""
     if not domain.hasPassiveScalar():
         raise RuntimeError("the domain has no passive scalar")
-    d = _prep(domain, timeStep, nonOrthoFlags)
-    _lib.call("pp_setup_advection_scalar", C.byref(d), int(nonOrthoFlags), _stream(domain))
+    d = _prep(domain, timeStep)
+    _lib.call("pp_setup_advection_scalar", C.byref(d), _flags(domain, nonOrthoFlags), _stream(domain))
 
 
 def SetupAdvectionVelocity(domain: Domain, timeStep, nonOrthoFlags: int = 0, applyPressureGradient: bool = False) -> None:
     """Momentum right-hand side into ``domain.velocityRHS`` (bindings.cpp:645-647)."""
-    d = _prep(domain, timeStep, nonOrthoFlags)
-    _lib.call("pp_setup_advection_velocity", C.byref(d), int(nonOrthoFlags), int(bool(applyPressureGradient)), _stream(domain))
+    d = _prep(domain, timeStep)
+    _lib.call("pp_setup_advection_velocity", C.byref(d), _flags(domain, nonOrthoFlags), int(bool(applyPressureGradient)),
+              _stream(domain))
 
 
 def SetupPressureMatrix(domain: Domain, timeStep, nonOrthoMode: int = 0, useFaceTransform: bool = False) -> None:
     if useFaceTransform:
         raise NotImplementedError("face-transform pressure Laplacians are not implemented")
-    d = _prep(domain, timeStep, nonOrthoMode)
-    _lib.call("pp_setup_pressure_matrix", C.byref(d), int(nonOrthoMode), _stream(domain))
+    d = _prep(domain, timeStep)
+    _lib.call("pp_setup_pressure_matrix", C.byref(d), _flags(domain, nonOrthoMode), _stream(domain))
     domain.P.mark_ell_valid()
 
 
@@ -83,16 +90,17 @@ def SetupPressureRHS(domain: Domain, timeStep, nonOrthoMode: int = 0, useFaceTra
                      timeStepNorm: bool = False) -> None:
     if useFaceTransform:
         raise NotImplementedError("face-transform fluxes are not implemented")
-    d = _prep(domain, timeStep, nonOrthoMode)
-    _lib.call("pp_setup_pressure_rhs", C.byref(d), int(nonOrthoMode), int(bool(timeStepNorm)), _stream(domain))
+    d = _prep(domain, timeStep)
+    _lib.call("pp_setup_pressure_rhs", C.byref(d), _flags(domain, nonOrthoMode), int(bool(timeStepNorm)), _stream(domain))
 
 
 def SetupPressureRHSdiv(domain: Domain, timeStep, nonOrthoMode: int = 0, useFaceTransform: bool = False,
                         timeStepNorm: bool = False) -> None:
     if useFaceTransform:
         raise NotImplementedError("face-transform fluxes are not implemented")
-    d = _prep(domain, timeStep, nonOrthoMode)
-    _lib.call("pp_setup_pressure_rhs_div", C.byref(d), int(nonOrthoMode), int(bool(timeStepNorm)), _stream(domain))
+    d = _prep(domain, timeStep)
+    _lib.call("pp_setup_pressure_rhs_div", C.byref(d), _flags(domain, nonOrthoMode), int(bool(timeStepNorm)),
+              _stream(domain))
 
 
 def SetupPressureCorrection(domain: Domain, timeStep, nonOrthoMode: int = 0, useFaceTransform: bool = False,
@@ -114,8 +122,12 @@ def CorrectVelocity(domain: Domain, timeStep, version: int = 0, timeStepNorm: bo
 def SetupEpotMatrix(domain: Domain, nonOrthoFlags: int = 0, useFaceTransform: bool = False) -> None:
     if domain.Epot is None:
         raise RuntimeError("domain.Epot is not allocated. Call domain.SetupEpotOnDomain() first.")
-    d = _prep(domain, None, nonOrthoFlags)
-    _lib.call("pp_setup_epot_matrix", C.byref(d), int(nonOrthoFlags), _stream(domain))
+    flags = _flags(domain, nonOrthoFlags)
+    if flags:
+        # the potential's corner terms (potential.cu:103-150) are not assembled here
+        raise NotImplementedError("non-orthogonal correction terms of the potential matrix are not implemented")
+    d = _prep(domain, None)
+    _lib.call("pp_setup_epot_matrix", C.byref(d), 0, _stream(domain))
     domain.Epot.mark_ell_valid()
 
 

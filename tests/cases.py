@@ -40,6 +40,20 @@ def skewed_vertices(axes: Sequence[np.ndarray], amount: float, seed: int) -> np.
     return v
 
 
+def skew(specs: Sequence[BlockSpec], amount: float = 0.25, seed: int = 0) -> list:
+    """The blocks of ``specs`` with seeded jitter on every vertex off the block faces (the block
+    interfaces stay matched): a non-orthogonal grid of the same topology."""
+    rng = np.random.default_rng(seed)
+    for spec in specs:
+        v = spec.vertices.copy()
+        d = v.shape[0]
+        inner = tuple([slice(None)] + [slice(1, -1)] * d)
+        spacing = min(float(np.min(np.abs(np.diff(v[a], axis=d - 1 - a)))) for a in range(d))
+        v[inner] += amount * spacing * rng.uniform(-1, 1, v[inner].shape)
+        spec.vertices = v
+    return list(specs)
+
+
 def cell_shape(v: np.ndarray) -> list:
     return [s - 1 for s in v.shape[1:]]
 
@@ -261,6 +275,19 @@ def tjunction3d(n: int = 4, seed: int = 4, U: float = 1.0) -> list:
     return blocks
 
 
+def heated_periodic2d(channels: int = 2, seed: int = 8) -> list:
+    """``channel2d`` carrying passive scalars: a hot bottom wall (Dirichlet) and a top wall with
+    a Dirichlet first channel and a prescribed flux (Neumann) on the others."""
+    specs = channel2d(16, 12, seed=seed, u0=0.5)
+    rng = np.random.default_rng(seed)
+    cz = cell_shape(specs[0].vertices)
+    specs[0].faces[2].scalar = np.linspace(1.0, 0.5, channels)
+    specs[0].faces[3].scalar = np.linspace(0.0, 0.2, channels)
+    specs[0].faces[3].scalar_neumann = [False] + [True] * (channels - 1)
+    specs[0].scalar = 0.5 + 0.1 * rng.standard_normal([channels] + cz)
+    return specs
+
+
 CASES = {
     "channel2d": lambda: channel2d(),
     "duct3d": lambda: duct3d(),
@@ -272,6 +299,7 @@ CASES = {
     "potbox2d": lambda: potential_box2d(),
     "tjunction3d": lambda: tjunction3d(),
     "heated2d": lambda: heated_channel2d(),
+    "heated_periodic2d": lambda: heated_periodic2d(),
 }
 
 

@@ -164,19 +164,77 @@ def test_batched_assembly_per_env(dtype):
         close(to_np(dom.pressureRHSdiv)[e * n:(e + 1) * n], od.pressure_div(prhs), dtype, f"env {e} divergence")
 
 
-def test_skewed_grid_rejects_non_orthogonal_flags():
+NONORTHO_CASES = ["channel2d", "duct3d", "inflow2d_varying", "split2d", "rotated3d", "tjunction3d", "heated2d"]
+
+
+@pytest.mark.parametrize("flags", [O.NO_DEFAULT_MODE, O.NO_DEFAULT_MODE | O.NO_DIRECT_RHS, O.NO_DIRECT_MATRIX])
+@pytest.mark.parametrize("name", NONORTHO_CASES)
+def test_nonortho_assembly_matches_oracle(name, flags, dtype):
+    """Skewed grids (jittered interior vertices): the corner terms of the advection, scalar and
+    pressure matrices and the explicit corner terms of the momentum, scalar and pressure
+    right-hand sides (walks through periodic seams, block connections with crossed axes and
+    fixed faces) against the oracle, on random result vectors."""
+    print(f"[nonortho] case={name} flags={flags} dtype={np.dtype(dtype).name}", flush=True)
     import phipict_amd as pa
 
-    specs = cases.channel2d(8, 6)
-    axes = [np.linspace(0, 2, 9), cases.tanh_coords(6, -1, 1, 1.5)]
-    specs[0].vertices = cases.skewed_vertices(axes, 0.2, seed=1)
-    dom = build_domain(specs, NU, DT, torch.float64)
+    specs = cases.skew(cases.CASES[name](), 0.25, seed=7)
+    od = O.OracleDomain(specs, NU, DT, dtype)
+    dom = build_domain(specs, NU, DT, _tdt(dtype))
     assert not dom.isOrthogonal()
+    ts = torch.tensor([DT], dtype=_tdt(dtype))
+    n, d = od.ncells, od.dims
+    dev = dom.getDevice()
+    rng = np.random.default_rng(5)
+    pa.SetupAdvectionMatrix(dom, ts, flags)
+    ccol, cval, A = od.advection_matrix(flags)
+    close(to_np(dom.C.value), cval, dtype, "advection matrix")
+    close(to_np(dom.A), A, dtype, "advection diagonal")
+    vres = rng.standard_normal(d * n).astype(dtype)
+    dom.setVelocityResult(torch.as_tensor(vres, device=dev))
+    pa.SetupAdvectionVelocity(dom, ts, flags, False)
+    close(to_np(dom.velocityRHS), od.advection_rhs(False, flags, vres), dtype, "momentum rhs")
+    pa.SetupPressureMatrix(dom, ts, flags)
+    pcol, pval = od.pressure_matrix(A, flags)
+    close(to_np(dom.P.value), pval, dtype, "pressure matrix")
+    pres = rng.standard_normal(n).astype(dtype)
+    dom.setPressureResult(torch.as_tensor(pres, device=dev))
+    pa.SetupPressureRHS(dom, ts, flags)
+    prhs = od.pressure_rhs(A, ccol, cval, vres)
+    div = od.pressure_div(prhs)
+    if flags & (O.NO_DIRECT_RHS | O.NO_DIAGONAL_RHS):
+        div = div + od.nonortho_rhs(O.OK_PRES, flags, 0, pres, A)
+    close(to_np(dom.pressureRHSdiv), div, dtype, "pressure rhs divergence with corner terms")
+    pa.SetupPressureRHSdiv(dom, ts, flags)
+    close(to_np(dom.pressureRHSdiv), div, dtype, "divergence with corner terms")
+    if specs[0].scalar is not None:
+        C_ = od.channels
+        for ch in range(C_):
+            pa.SetupAdvectionMatrix(dom, ts, flags, forPassiveScalar=True, passiveScalarChannel=ch)
+            _, sval, sA = od.scalar_matrix(ch, flags)
+            close(to_np(dom.C.value), sval, dtype, f"scalar matrix {ch}")
+        sres = rng.standard_normal(C_ * n).astype(dtype)
+        dom.setScalarResult(torch.as_tensor(sres, device=dev))
+        pa.SetupAdvectionScalar(dom, ts, flags)
+        close(to_np(dom.scalarRHS), od.scalar_rhs(flags, sres.reshape(C_, n)).reshape(-1), dtype, "scalar rhs")
+
+
+def test_nonortho_flags_skip_exactly_orthogonal_grids():
+    """Rectilinear grids take the orthogonal kernels (their corner terms are exactly zero), a
+    skewed grid does not, and the potential matrix's corner terms are refused."""
+    import phipict_amd as pa
+
+    rect = build_domain(cases.channel2d(8, 6), NU, DT, torch.float64, epot=True)
+    assert rect.isOrthogonal() and rect.nonOrthoTermsVanish()
+    ts = torch.tensor([DT], dtype=torch.float64)
+    pa.SetupAdvectionMatrix(rect, ts, 0)
+    c0 = rect.C.value.clone()
+    pa.SetupAdvectionMatrix(rect, ts, O.NO_DEFAULT_MODE)
+    assert torch.equal(c0, rect.C.value)
+    pa.SetupEpotMatrix(rect, O.NO_DEFAULT_MODE)
+    skewed = build_domain(cases.skew(cases.channel2d(8, 6), 0.2, seed=1), NU, DT, torch.float64, epot=True)
+    assert not skewed.nonOrthoTermsVanish()
     with pytest.raises(NotImplementedError):
-        pa.SetupAdvectionMatrix(dom, torch.tensor([DT], dtype=torch.float64), 1)
-    pa.SetupAdvectionMatrix(dom, torch.tensor([DT], dtype=torch.float64), 0)
-    rect = build_domain(cases.channel2d(8, 6), NU, DT, torch.float64)
-    assert rect.isOrthogonal()
+        pa.SetupEpotMatrix(skewed, O.NO_DEFAULT_MODE)
 
 
 @pytest.mark.parametrize("dims", [2, 3])

@@ -639,3 +639,88 @@ def test_scalar_steady_advected_inflow_value():
     od.specs[0].faces[0].scalar = np.array([2.5])
     phi2 = spla.spsolve(sp.csr_matrix((val, col, od.row), shape=(n, n)).tocsc(), od.scalar_rhs()[0])
     assert phi2.max() > 1.5 + 1e-3 and phi2.min() >= 1.5 - 1e-12
+
+
+# ----------------------------------------------------------------------------- non-orthogonal terms
+def _sheared(d: int, n: int, shear):
+    """An affine sheared block (x = xi + s0 eta (+ s1 zeta), y = eta (+ s2 zeta), z = zeta; cell
+    size 1/n): periodic along x, fixed faces elsewhere, zero velocity."""
+    h = 1.0 / n
+    sizes = [n + 2, n, n - 1][:d]
+    grids = np.meshgrid(*[np.arange(s + 1) * h for s in reversed(sizes)], indexing="ij")  # z, y, x order
+    xi = grids[-1]
+    eta = grids[-2]
+    if d == 2:
+        v = np.stack([xi + shear[0] * eta, eta])
+        J = np.array([[1.0, shear[0]], [0.0, 1.0]])
+    else:
+        zeta = grids[0]
+        v = np.stack([xi + shear[0] * eta + shear[1] * zeta, eta + shear[2] * zeta, zeta])
+        J = np.array([[1.0, shear[0], shear[1]], [0.0, 1.0, shear[2]], [0.0, 0.0, 1.0]])
+    faces = [cases.FaceSpec("periodic"), cases.FaceSpec("periodic")] + [
+        cases.FaceSpec("fixed", velocity=np.zeros(d)) for _ in range(2 * d - 2)]
+    spec = cases.BlockSpec(v, faces, velocity=np.zeros([d] + cases.cell_shape(v)))
+    return spec, J, h
+
+
+def _cell_centers(spec) -> np.ndarray:
+    v = spec.vertices
+    d = v.shape[0]
+    c = v
+    for ax in range(1, d + 1):
+        sl0 = [slice(None)] * (d + 1)
+        sl1 = [slice(None)] * (d + 1)
+        sl0[ax] = slice(0, -1)
+        sl1[ax] = slice(1, None)
+        c = 0.5 * (c[tuple(sl0)] + c[tuple(sl1)])
+    return c.reshape(d, -1)  # [d][cells], x fastest
+
+
+@pytest.mark.parametrize("d", [2, 3])
+@pytest.mark.parametrize("field", ["quadratic", "wall-normal"])
+def test_nonortho_laplacian_exact_for_quadratics(d, field):
+    """On an affine sheared grid the implicit corner terms of the pressure and diffusion matrices
+    plus the explicit diagonal-corner terms of the right-hand sides (the reference's default
+    mode) form a Laplacian that is exact for quadratic fields: L(p) = det * (tr(H) p) in every
+    cell whose corner stencil stays off the fixed faces. The wall-normal field depends only on
+    (y, z), so it is periodic along x and the check includes the cells at the periodic seam."""
+    spec, J, h = _sheared(d, 8, [0.45, -0.3, 0.25])
+    od = O.OracleDomain([spec], 0.37, 0.05, np.float64)
+    n = od.ncells
+    X = _cell_centers(spec)
+    rng = np.random.default_rng(3)
+    Hm = rng.standard_normal((d, d))
+    Hm = Hm + Hm.T
+    g = rng.standard_normal(d)
+    if field == "wall-normal":
+        Hm[0, :] = 0
+        Hm[:, 0] = 0
+        g[0] = 0
+    p = 0.5 * np.einsum("ic,ij,jc->c", X, Hm, X) + g @ X
+    lap = np.trace(Hm) * h ** d  # det * laplacian; the shear keeps the cell volume h^d
+    idx = np.indices(list(reversed(spec.size_xyz[:d]))).reshape(d, -1)[::-1]  # x, y, (z) per cell
+    size = spec.size_xyz
+    ok = np.ones(n, bool)
+    for ax in range(1, d):
+        ok &= (idx[ax] >= 2) & (idx[ax] <= size[ax] - 3)
+    if field == "quadratic":
+        ok &= (idx[0] >= 2) & (idx[0] <= size[0] - 3)
+    flags = O.NO_DEFAULT_MODE
+    # pressure: P p - S(p), with A = 1
+    ones = np.ones(n)
+    col, val = od.pressure_matrix(ones, flags)
+    Lp = O.spmv(od.row, col, val, p) - od.nonortho_rhs(O.OK_PRES, flags, 0, p, ones)
+    np.testing.assert_allclose(Lp[ok], lap, rtol=1e-9, atol=1e-12)
+    if field == "quadratic":
+        # without the corner terms the operator is not exact on the sheared grid (for the
+        # wall-normal field the cross terms cancel between opposite faces)
+        col0, val0 = od.pressure_matrix(ones, 0)
+        assert np.abs(O.spmv(od.row, col0, val0, p)[ok] - lap).max() > 1e-3 * abs(lap)
+    # diffusion of each velocity component: det/dt u - det (C u) - S_vel(u) = nu L(u)
+    det = h ** d
+    u = np.concatenate([p * (k + 1) for k in range(d)])
+    ccol, cval, _ = od.advection_matrix(flags)
+    for k in range(d):
+        uk = u[k * n:(k + 1) * n]
+        Lu = det / od.dom.dt * uk - det * O.spmv(od.row, ccol, cval, uk) - od.nonortho_rhs(O.OK_VEL, flags, k, u)
+        np.testing.assert_allclose(Lu[ok], 0.37 * lap * (k + 1), rtol=1e-9, atol=1e-12)
