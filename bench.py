@@ -50,6 +50,9 @@ def parse_args(argv=None):
                          "backward through a 3-block T-junction rollout and its finite-difference check")
     ap.add_argument("--dtype", choices=["f64", "f32"], default="f64")
     ap.add_argument("--size", default="192x96x96", help="cells nx x ny x nz")
+    ap.add_argument("--skew", type=float, default=0.0,
+                    help="jitter the duct's interior vertices by this fraction of the smallest spacing and step in "
+                         "the non-orthogonal mode (corner terms; CPU baseline off)")
     ap.add_argument("--timer-every", type=int, default=4, help="time a pseudo-random 1/k of the launches of each kernel")
     ap.add_argument("--timer-in-timed-region", action="store_true",
                     help="run the kernel timer during the timed steps (default: during as many further steps)")
@@ -86,7 +89,8 @@ def duct_vertices(nx, ny, nz):
     return np.stack([X, Y, Z])
 
 
-def make_sim(nx, ny, nz, dtype, device):
+def make_sim(nx, ny, nz, dtype, device, skew=0.0):
+    import numpy as np
     import torch
 
     import phipict_amd as pa
@@ -94,6 +98,11 @@ def make_sim(nx, ny, nz, dtype, device):
     N = HA ** 2 / RE
     G = N  # core velocity of order one
     v = duct_vertices(nx, ny, nz)
+    if skew > 0:
+        # seeded jitter of every vertex off the block faces: a non-orthogonal grid
+        rng = np.random.default_rng(3)
+        spacing = min(float(np.min(np.abs(np.diff(v[a], axis=2 - a)))) for a in range(3))
+        v[:, 1:-1, 1:-1, 1:-1] += skew * spacing * rng.uniform(-1, 1, v[:, 1:-1, 1:-1, 1:-1].shape)
     gen = torch.Generator("cpu").manual_seed(1)
     vel = 1e-3 * torch.randn(1, 3, nz, ny, nx, generator=gen, dtype=torch.float64)
     dom = pa.Domain(3, torch.tensor([1.0 / RE], dtype=dtype), dtype=dtype, device=device)
@@ -109,7 +118,7 @@ def make_sim(nx, ny, nz, dtype, device):
         domain.UpdateDomainData()
 
     sim = pa.MHDSimulation(domain=dom, dt=DT, stuart_number=torch.tensor(N),
-                           e_b=torch.tensor([0.0, 1.0, 0.0], dtype=dtype), non_orthogonal=False,
+                           e_b=torch.tensor([0.0, 1.0, 0.0], dtype=dtype), non_orthogonal=skew > 0,
                            pressure_use_amg=True, potential_use_preconditioner=True, pressure_tol=TOL,
                            potential_tol=TOL, hooks={"PRE_VELOCITY_SETUP": forcing})
     sim.record_solver_stats = True
@@ -539,7 +548,7 @@ def main(argv=None) -> int:
     dtype = torch.float64 if args.dtype == "f64" else torch.float32
     device = torch.device("cuda", 0)
     print(f"[bench] building the {nx}x{ny}x{nz} duct ({args.dtype})", file=sys.stderr, flush=True)
-    dom, sim = make_sim(nx, ny, nz, dtype, device)
+    dom, sim = make_sim(nx, ny, nz, dtype, device, skew=args.skew)
     import ctypes
 
     from phipict_amd import _lib
@@ -652,7 +661,7 @@ def main(argv=None) -> int:
     for _, tag, infos in sim.solver_log[log_start:log_start + len(sim.solver_log[log_start:]) // (1 if args.timer_in_timed_region else 2)]:
         iters.setdefault(tag, []).append(max(i[0] for i in infos))
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.skew == 0:
         cx, cy, cz = (int(v) for v in args.cpu_size.lower().split("x"))
         print(f"[bench] CPU baseline sample {cx}x{cy}x{cz}", file=sys.stderr, flush=True)
         cpu = cpu_baseline((cx, cy, cz), args.cpu_steps, n, args.dtype)
@@ -671,7 +680,9 @@ def main(argv=None) -> int:
         "dtype": args.dtype,
         "data": "synthetic",
         "config": {"workload": f"3D MHD duct {nx}x{ny}x{nz} cells (periodic x, pressure-gradient source), Re=500, "
-                               f"Ha=100, dt=5e-4, AMG-PCG pressure+potential tol 1e-8, BiCGStab momentum",
+                               f"Ha=100, dt=5e-4, AMG-PCG pressure+potential tol 1e-8, BiCGStab momentum"
+                               + (f", skewed grid (interior vertices jittered by {args.skew} of the spacing), "
+                                  f"non-orthogonal corner terms" if args.skew > 0 else ""),
                    "cells": n, "nnz": dom.C.getNnz(), "correctors": 2,
                    "host": "Python driver; gc.collect() + gc.freeze() after the warm-up"},
         "cell_updates_per_s": round(sps * n, 1),

@@ -314,6 +314,12 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_advection_rhs(pp_domain dom, int 
     T *rhs = reinterpret_cast<T *>(dom.velocity_rhs) + (size_t)env * D * dom.num_cells;
     T pg[D];
     if (apply_pg) pressure_grad<T, D>(dom, blocks, bi, p, f, env, pg);
+    // explicit corner terms of every component (one walk per corner)
+    T Sv[D];
+    if (NO && flags) {
+        const CornerData<T, D, NK_VELOCITY> cd{dom, blocks, 0, env};
+        nonortho_rhs<T, D, NK_VELOCITY>(cd, bi, p, f, flags, true, nullptr, Sv);
+    }
 #pragma unroll
     for (int dim = 0; dim < D; ++dim) {
         T t = det * vel[(size_t)dim * b.cells + f] / dt;
@@ -333,10 +339,7 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_advection_rhs(pp_domain dom, int 
             const T slip = fb.vel_type == PP_BC_DIRICHLET ? T(0) : T(1);
             t += v * (1 - slip) * bv * 2 * alpha;
         }
-        if (NO && flags) {
-            const CornerData<T, D, NK_VELOCITY> cd{dom, blocks, dim, env};
-            t -= nonortho_rhs<T, D, NK_VELOCITY>(cd, bi, p, f, flags, true, nullptr);
-        }
+        if (NO && flags) t -= Sv[dim];
         t /= det;
         t += vsource<T>(dom, b, f, dim, env);
         if (apply_pg) t -= pg[dim] * dt;
@@ -386,7 +389,9 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_scalar_rhs(pp_domain dom, int fla
         }
         if (NO && flags) {
             const CornerData<T, D, NK_SCALAR> cd{dom, blocks, ch, env};
-            t -= nonortho_rhs<T, D, NK_SCALAR>(cd, bi, p, f, flags, false, nullptr) * scalar_visc<T>(dom, ch);
+            T S[1];
+            nonortho_rhs<T, D, NK_SCALAR>(cd, bi, p, f, flags, false, nullptr, S);
+            t -= S[0] * scalar_visc<T>(dom, ch);
         }
         rhs[(size_t)ch * dom.num_cells + g] = t / det;
     }
@@ -501,8 +506,10 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_pressure_div(pp_domain dom, int t
     if (time_step_norm) dv /= reinterpret_cast<const T *>(dom.time_step)[env];
     if (NO && (flags & (NO_DIRECT_RHS | NO_DIAGONAL_RHS))) {
         const CornerData<T, D, NK_PRESSURE> cd{dom, blocks, 0, env};
-        dv += nonortho_rhs<T, D, NK_PRESSURE>(cd, bi, p, f, flags, false,
-                                              reinterpret_cast<const T *>(dom.A) + (size_t)env * dom.num_cells);
+        T S[1];
+        nonortho_rhs<T, D, NK_PRESSURE>(cd, bi, p, f, flags, false,
+                                        reinterpret_cast<const T *>(dom.A) + (size_t)env * dom.num_cells, S);
+        dv += S[0];
     }
     reinterpret_cast<T *>(dom.pressure_rhs_div)[(size_t)env * dom.num_cells + g] = dv;
 }

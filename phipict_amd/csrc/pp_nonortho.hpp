@@ -4,6 +4,10 @@
 // nonortho_rhs.cuh:28-200). Only the flag bits of the reference's default mode and
 // NON_ORTHO_DIRECT_RHS are implemented; NON_ORTHO_DIAGONAL_MATRIX is not implemented there
 // either.
+//
+// Positions are three scalars updated through selects (a cell index along a run-time axis
+// would otherwise keep them in scratch memory), and the velocity's corner terms carry all
+// components through one walk.
 #pragma once
 #include "pp_common.hpp"
 
@@ -14,6 +18,55 @@ enum { NO_DIRECT_MATRIX = 1, NO_DIRECT_RHS = 2, NO_DIAGONAL_MATRIX = 4, NO_DIAGO
 // The data a corner term reads (GridDataType of the reference). The matrix kinds read only
 // the boundary condition types of fixed faces (velocity, or scalar channel w), no values.
 enum { NK_FIXED_ONLY = 0, NK_VELOCITY = 1, NK_PRESSURE = 2, NK_SCALAR = 3, NK_VELOCITY_TYPES = 4, NK_SCALAR_TYPES = 5 };
+
+struct P3 {
+    int x, y, z;
+    __device__ __forceinline__ int get(int a) const { return a == 0 ? x : (a == 1 ? y : z); }
+    __device__ __forceinline__ void set(int a, int v) {
+        x = a == 0 ? v : x;
+        y = a == 1 ? v : y;
+        z = a == 2 ? v : z;
+    }
+    __device__ __forceinline__ void add(int a, int v) { set(a, get(a) + v); }
+    __device__ __forceinline__ bool operator==(const P3 &o) const { return x == o.x && y == o.y && z == o.z; }
+};
+
+__device__ __forceinline__ int bsize(const pp_block_desc &b, int a) { return a == 0 ? b.size[0] : (a == 1 ? b.size[1] : b.size[2]); }
+__device__ __forceinline__ bool at_bnd(const P3 &p, int dir, const pp_block_desc &b) {
+    const int a = dir >> 1;
+    return (dir & 1) ? p.get(a) == bsize(b, a) - 1 : p.get(a) == 0;
+}
+__device__ __forceinline__ int flat_p(const P3 &p, const pp_block_desc &b) { return p.x + b.size[0] * (p.y + b.size[1] * p.z); }
+// face cell of position p on face `bound` (the face axis dropped)
+__device__ __forceinline__ int face_flat_p(const P3 &p, int bound, const pp_block_desc &b) {
+    const int a = bound >> 1;
+    const int sx = a == 0 ? 1 : b.size[0], sy = a == 1 ? 1 : b.size[1];
+    const int qx = a == 0 ? 0 : p.x, qy = a == 1 ? 0 : p.y, qz = a == 2 ? 0 : p.z;
+    return qx + sx * (qy + sy * qz);
+}
+// computeConnectedPos (topology.cuh:246-266), `border` cells in from the connected face; a
+// block one cell thick there clamps it to 0 (the reference would read outside the block)
+template <int D>
+__device__ __forceinline__ P3 conn_p(const pp_block_desc *__restrict__ blocks, const P3 &pos, int bdim,
+                                     const pp_boundary_desc &cb, int border) {
+    const pp_block_desc &ob = blocks[cb.conn_block];
+    P3 out = pos;
+    int ca = cb.conn_axes[0] >> 1;
+    const int sz = bsize(ob, ca);
+    if (border > sz - 1) border = sz - 1;
+    out.set(ca, (cb.conn_axes[0] & 1) ? sz - 1 - border : border);
+    if (D > 1) {
+        int axis = (bdim + 1) % D;
+        ca = cb.conn_axes[1] >> 1;
+        out.set(ca, (cb.conn_axes[1] & 1) ? bsize(ob, ca) - 1 - pos.get(axis) : pos.get(axis));
+        if (D > 2) {
+            axis = (bdim + 2) % D;
+            ca = cb.conn_axes[2] >> 1;
+            out.set(ca, (cb.conn_axes[2] & 1) ? bsize(ob, ca) - 1 - pos.get(axis) : pos.get(axis));
+        }
+    }
+    return out;
+}
 
 // alpha_{c1 c2} = det Minv[c1].Minv[c2] (getLaplaceCoefficientsFull / SingleAxis,
 // discretization.cuh:419-470).
@@ -33,46 +86,52 @@ __device__ __forceinline__ T lap_coef(const pp_block_desc &b, int f, int c1, int
 // computeConnectedDir, topology.cuh:236-244: the face-format direction `dir` seen from the
 // block across connection cb of face axis bdim.
 __device__ __forceinline__ int conn_dir(int d, int dir, int bdim, const pp_boundary_desc &cb) {
-    return cb.conn_axes[pos_mod((dir >> 1) - bdim, d)] ^ (dir & 1);
+    const int k = pos_mod((dir >> 1) - bdim, d);
+    return (k == 0 ? cb.conn_axes[0] : (k == 1 ? cb.conn_axes[1] : cb.conn_axes[2])) ^ (dir & 1);
 }
 
-// Where the walk is: block and cell.
-struct WalkPos {
-    int bi;
-    int p[3];
-};
-
-// Reads of one data kind: cell values (result vectors), values on fixed faces, and the kind's
-// boundary condition type on a fixed face.
+// Reads of one data kind: NC values per cell (velocity: all components; otherwise component
+// or channel w), values on fixed faces, and the kind's boundary condition type on a fixed face.
 template <typename T, int D, int KIND>
 struct CornerData {
+    static constexpr int NC = KIND == NK_VELOCITY ? D : 1;
     const pp_domain &dom;
     const pp_block_desc *__restrict__ blocks;
-    int w;   // velocity component or scalar channel
+    int w;   // scalar channel
     int env;
-    __device__ T cell(int bi, const int *p) const {
+    __device__ __forceinline__ void cell(int bi, const P3 &p, T *out) const {
         const pp_block_desc &b = blocks[bi];
-        const size_t g = (size_t)b.global_offset + flat3(p, b.size);
+        const size_t g = (size_t)b.global_offset + flat_p(p, b);
         const size_t n = dom.num_cells;
-        if (KIND == NK_VELOCITY)
-            return reinterpret_cast<const T *>(dom.velocity_result)[((size_t)env * D + w) * n + g];
-        if (KIND == NK_PRESSURE) return reinterpret_cast<const T *>(dom.pressure_result)[(size_t)env * n + g];
-        if (KIND == NK_SCALAR)
-            return reinterpret_cast<const T *>(dom.scalar_result)[((size_t)env * dom.scalar_channels + w) * n + g];
-        return T(0);
+        if (KIND == NK_VELOCITY) {
+#pragma unroll
+            for (int k = 0; k < NC; ++k)
+                out[k] = reinterpret_cast<const T *>(dom.velocity_result)[((size_t)env * D + k) * n + g];
+        } else if (KIND == NK_PRESSURE) {
+            out[0] = reinterpret_cast<const T *>(dom.pressure_result)[(size_t)env * n + g];
+        } else if (KIND == NK_SCALAR) {
+            out[0] = reinterpret_cast<const T *>(dom.scalar_result)[((size_t)env * dom.scalar_channels + w) * n + g];
+        } else {
+            out[0] = T(0);
+        }
     }
     // getFixedBoundaryData (block_data.cuh:136-199): the reference keeps no pressure values on
     // fixed faces, so the pressure kind reads 0 there.
-    __device__ T fixed(int bi, int bound, const int *p) const {
+    __device__ __forceinline__ void fixed(int bi, int bound, const P3 &p, T *out) const {
         const pp_block_desc &b = blocks[bi];
         const pp_boundary_desc &fb = b.bnd[bound];
-        const int fc = face_cells(b, bound), ff = face_flat(p, bound, b);
-        if (KIND == NK_VELOCITY) return fixed_vel<T>(fb, D, fc, ff, w, env);
-        if (KIND == NK_SCALAR) return fixed_scalar<T>(fb, dom.scalar_channels, fc, ff, w, env);
-        return T(0);
+        const int fc = face_cells(b, bound), ff = face_flat_p(p, bound, b);
+        if (KIND == NK_VELOCITY) {
+#pragma unroll
+            for (int k = 0; k < NC; ++k) out[k] = fixed_vel<T>(fb, D, fc, ff, k, env);
+        } else if (KIND == NK_SCALAR) {
+            out[0] = fixed_scalar<T>(fb, dom.scalar_channels, fc, ff, w, env);
+        } else {
+            out[0] = T(0);
+        }
     }
     // getFixedBoundaryType (block_data.cuh:110-134)
-    __device__ bool neumann(int bi, int bound) const {
+    __device__ __forceinline__ bool neumann(int bi, int bound) const {
         const pp_boundary_desc &fb = blocks[bi].bnd[bound];
         if (KIND == NK_VELOCITY || KIND == NK_VELOCITY_TYPES) return fb.vel_type != PP_BC_DIRICHLET;
         if (KIND == NK_SCALAR || KIND == NK_SCALAR_TYPES) return scalar_neumann(fb, w);
@@ -80,9 +139,9 @@ struct CornerData {
     }
 };
 
-template <typename T>
+template <typename T, int NC>
 struct CornerValue {
-    T data;
+    T data[NC];
     int cells;    // cells the value averages; 0 when it comes from a fixed face
     bool neumann; // boundary condition type of that fixed face
 };
@@ -93,114 +152,130 @@ struct CornerValue {
 // cells is summed only with `depth1`. A walk across a block connection lands on the second
 // cell layer of the connected block (computeConnectedPos with border offset 1), as in the
 // reference.
-template <typename T, int D, int KIND>
-__device__ CornerValue<T> corner_value(const CornerData<T, D, KIND> &cd, int bi, const int *p, int dir1, int dir2,
-                                       bool depth1) {
+template <typename T, int D, int KIND, int NC = CornerData<T, D, KIND>::NC>
+__device__ __forceinline__ CornerValue<T, NC> corner_value(const CornerData<T, D, KIND> &cd, int bi, const P3 &p, int dir1, int dir2,
+                                           bool depth1) {
     const pp_block_desc *__restrict__ blocks = cd.blocks;
-    T data = T(0);
-    int num = 1;
-    bool neu = false;
-    WalkPos cyc[2];
-    int cdir[2][2] = {{dir1, dir2}, {dir2, dir1}};
+    CornerValue<T, NC> r;
 #pragma unroll
-    for (int d = 0; d < 2; ++d) {
-        cyc[d].bi = bi;
-        cyc[d].p[0] = p[0]; cyc[d].p[1] = p[1]; cyc[d].p[2] = p[2];
-    }
+    for (int k = 0; k < NC; ++k) r.data[k] = T(0);
+    r.neumann = false;
+    int num = 1;
+    int cb[2] = {bi, bi};
+    P3 cp[2] = {p, p};
+    int da_[2] = {dir1, dir2}, db_[2] = {dir2, dir1};
+#pragma unroll
     for (int depth = 1; depth <= 2; ++depth) {
+#pragma unroll
         for (int d = 0; d < 2; ++d) {
-            const int da = cdir[d][0], db = cdir[d][1];
+            const int da = da_[d], db = db_[d];
             const int axis = da >> 1;
-            const pp_block_desc &cb = blocks[cyc[d].bi];
-            if (at_bound(cyc[d].p, da, cb)) {
-                const pp_boundary_desc &bd = cb.bnd[da];
+            const pp_block_desc &b = blocks[cb[d]];
+            if (at_bnd(cp[d], da, b)) {
+                const pp_boundary_desc &bd = b.bnd[da];
                 if (bd.type == PP_BOUND_FIXED) {
-                    neu = cd.neumann(cyc[d].bi, da);
-                    if (at_bound(cyc[d].p, db, cb)) return {cd.fixed(cyc[d].bi, da, cyc[d].p), 0, neu};
-                    int q[3] = {cyc[d].p[0], cyc[d].p[1], cyc[d].p[2]};
-                    q[db >> 1] += face_sign(db);
-                    return {(cd.fixed(cyc[d].bi, da, cyc[d].p) + cd.fixed(cyc[d].bi, da, q)) * T(0.5), 0, neu};
+                    r.neumann = cd.neumann(cb[d], da);
+                    r.cells = 0;
+                    cd.fixed(cb[d], da, cp[d], r.data);
+                    if (!at_bnd(cp[d], db, b)) {
+                        P3 q = cp[d];
+                        q.add(db >> 1, face_sign(db));
+                        T o[NC];
+                        cd.fixed(cb[d], da, q, o);
+#pragma unroll
+                        for (int k = 0; k < NC; ++k) r.data[k] = (r.data[k] + o[k]) * T(0.5);
+                    }
+                    return r;
                 } else if (bd.type == PP_BOUND_CONNECTED) {
-                    int q[3];
-                    conn_pos(blocks, D, cyc[d].p, axis, bd, 1, q);
-                    cyc[d].bi = bd.conn_block;
-                    cyc[d].p[0] = q[0]; cyc[d].p[1] = q[1]; cyc[d].p[2] = q[2];
-                    cdir[d][0] = conn_dir(D, db, axis, bd);
-                    cdir[d][1] = conn_dir(D, da, axis, bd) ^ 1;
+                    cp[d] = conn_p<D>(blocks, cp[d], axis, bd, 1);
+                    cb[d] = bd.conn_block;
+                    da_[d] = conn_dir(D, db, axis, bd);
+                    db_[d] = conn_dir(D, da, axis, bd) ^ 1;
                 } else if (bd.type == PP_BOUND_PERIODIC) {
-                    cyc[d].p[axis] = (da & 1) ? 0 : cb.size[axis] - 1;
-                    cdir[d][0] = db;
-                    cdir[d][1] = da ^ 1;
+                    cp[d].set(axis, (da & 1) ? 0 : bsize(b, axis) - 1);
+                    da_[d] = db;
+                    db_[d] = da ^ 1;
                 }
             } else {
-                cyc[d].p[axis] += face_sign(da);
-                cdir[d][0] = db;
-                cdir[d][1] = da ^ 1;
+                cp[d].add(axis, face_sign(da));
+                da_[d] = db;
+                db_[d] = da ^ 1;
             }
-            const int o = d ^ 1;
-            if (cyc[d].bi == cyc[o].bi && cyc[d].p[0] == cyc[o].p[0] && cyc[d].p[1] == cyc[o].p[1] &&
-                cyc[d].p[2] == cyc[o].p[2])
-                return {data / T(num), num, neu};
-            if (depth > 1 || depth1) data += cd.cell(cyc[d].bi, cyc[d].p);
+            if (cb[d] == cb[d ^ 1] && cp[d] == cp[d ^ 1]) {
+                r.cells = num;
+#pragma unroll
+                for (int k = 0; k < NC; ++k) r.data[k] = r.data[k] / T(num);
+                return r;
+            }
+            if (depth > 1 || depth1) {
+                T v[NC];
+                cd.cell(cb[d], cp[d], v);
+#pragma unroll
+                for (int k = 0; k < NC; ++k) r.data[k] += v[k];
+            }
             ++num;
         }
     }
-    return {data / T(num), num, neu};
+    r.cells = num;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) r.data[k] = r.data[k] / T(num);
+    return r;
 }
 
 // getBlockDataNeighbor (zeroBound false), block_data.cuh:280-320
-template <typename T, int D, int KIND>
-__device__ T neighbor_data(const CornerData<T, D, KIND> &cd, int bi, const int *p, int dir) {
+template <typename T, int D, int KIND, int NC = CornerData<T, D, KIND>::NC>
+__device__ __forceinline__ void neighbor_data(const CornerData<T, D, KIND> &cd, int bi, const P3 &p, int dir, T *out) {
     const pp_block_desc &b = cd.blocks[bi];
     const int axis = dir >> 1;
-    int q[3] = {p[0], p[1], p[2]};
-    if (at_bound(p, dir, b)) {
+    P3 q = p;
+    if (at_bnd(p, dir, b)) {
         const pp_boundary_desc &bd = b.bnd[dir];
-        if (bd.type == PP_BOUND_FIXED) return cd.fixed(bi, dir, p);
-        if (bd.type == PP_BOUND_CONNECTED) {
-            conn_pos(cd.blocks, D, p, axis, bd, 1, q);
-            return cd.cell(bd.conn_block, q);
-        }
+        if (bd.type == PP_BOUND_FIXED) { cd.fixed(bi, dir, p, out); return; }
+        if (bd.type == PP_BOUND_CONNECTED) { cd.cell(bd.conn_block, conn_p<D>(cd.blocks, p, axis, bd, 1), out); return; }
         if (bd.type == PP_BOUND_PERIODIC) {
-            q[axis] = (dir & 1) ? 0 : b.size[axis] - 1;
-            return cd.cell(bi, q);
+            q.set(axis, (dir & 1) ? 0 : bsize(b, axis) - 1);
+            cd.cell(bi, q, out);
+            return;
         }
-        return T(0);
+#pragma unroll
+        for (int k = 0; k < NC; ++k) out[k] = T(0);
+        return;
     }
-    q[axis] += face_sign(dir);
-    return cd.cell(bi, q);
+    q.add(axis, face_sign(dir));
+    cd.cell(bi, q, out);
 }
 
 // getBlockDataNeighborDiagonal (zeroBound false), block_data.cuh:372-420: two steps, the
 // prescribed direction second.
-template <typename T, int D, int KIND>
-__device__ T diagonal_data(const CornerData<T, D, KIND> &cd, int bi, const int *p, int dir1, int dir2) {
+template <typename T, int D, int KIND, int NC = CornerData<T, D, KIND>::NC>
+__device__ __forceinline__ void diagonal_data(const CornerData<T, D, KIND> &cd, int bi, const P3 &p, int dir1, int dir2, T *out) {
     const bool first_fixed = is_fixed(cd.blocks[bi], dir1);
     const int dirs[2] = {first_fixed ? dir2 : dir1, first_fixed ? dir1 : dir2};
     int cbi = bi;
-    int q[3] = {p[0], p[1], p[2]};
+    P3 q = p;
+#pragma unroll
     for (int i = 0; i < 2; ++i) {
         const int dir = dirs[i];
         const int axis = dir >> 1;
         const pp_block_desc &b = cd.blocks[cbi];
-        if ((dir & 1) ? q[axis] == b.size[axis] - 1 : q[axis] == 0) {
+        if (at_bnd(q, dir, b)) {
             const pp_boundary_desc &bd = b.bnd[dir];
-            if (bd.type == PP_BOUND_FIXED) return cd.fixed(cbi, dir, q);
+            if (bd.type == PP_BOUND_FIXED) { cd.fixed(cbi, dir, q, out); return; }
             if (bd.type == PP_BOUND_CONNECTED) {
-                int r[3];
-                conn_pos(cd.blocks, D, q, axis, bd, 1, r);
+                q = conn_p<D>(cd.blocks, q, axis, bd, 1);
                 cbi = bd.conn_block;
-                q[0] = r[0]; q[1] = r[1]; q[2] = r[2];
             } else if (bd.type == PP_BOUND_PERIODIC) {
-                q[axis] = (dir & 1) ? 0 : b.size[axis] - 1;
+                q.set(axis, (dir & 1) ? 0 : bsize(b, axis) - 1);
             } else {
-                return T(0);
+#pragma unroll
+                for (int k = 0; k < NC; ++k) out[k] = T(0);
+                return;
             }
         } else {
-            q[axis] += face_sign(dir);
+            q.add(axis, face_sign(dir));
         }
     }
-    return cd.cell(cbi, q);
+    cd.cell(cbi, q, out);
 }
 
 // Matrix variant of the tangential coefficients of face `face` towards axis tAxis
@@ -209,7 +284,7 @@ __device__ T diagonal_data(const CornerData<T, D, KIND> &cd, int bi, const int *
 // As in the reference it is 0 for every face of a cell at either end of the face's axis when
 // that face is prescribed, and the neighbour's coefficients use unmapped axes.
 template <typename T, int D>
-__device__ T face_alpha_matrix(const pp_domain &dom, const pp_block_desc *__restrict__ blocks, int bi, const int *p,
+__device__ __forceinline__ T face_alpha_matrix(const pp_domain &dom, const pp_block_desc *__restrict__ blocks, int bi, const int *p,
                                int f, int face, int tAxis, T raP, bool withVisc, const T *A) {
     const pp_block_desc &b = blocks[bi];
     const int axis = face >> 1;
@@ -230,13 +305,14 @@ __device__ T face_alpha_matrix(const pp_domain &dom, const pp_block_desc *__rest
 // does (diffusion positive on the diagonal), -1 as the pressure matrix (Laplacian). vscale
 // multiplies the coefficient (the scalar diffusivity of a passive-scalar matrix).
 template <typename T, int D, int KIND>
-__device__ void matrix_corner_terms(const CornerData<T, D, KIND> &cd, int bi, const int *p, int f, int bound,
+__device__ __forceinline__ void matrix_corner_terms(const CornerData<T, D, KIND> &cd, int bi, const int *p, int f, int bound,
                                     int flags, T raP, bool withVisc, const T *A, T vscale, bool always_gradient,
                                     T sign, T &diag, T *rv) {
     const bool nbr = flags & NO_DIRECT_MATRIX, ctr = flags & NO_CENTER_MATRIX;
     if (D < 2 || !(nbr || ctr)) return;
     const int dim = bound >> 1;
     const int fs = face_sign(bound);
+    const P3 pp{p[0], p[1], p[2]};
 #pragma unroll
     for (int i = 1; i < D; ++i) {
         const int tAxis = (dim + i) % D;
@@ -246,7 +322,7 @@ __device__ void matrix_corner_terms(const CornerData<T, D, KIND> &cd, int bi, co
         for (int tUp = 0; tUp < 2; ++tUp) {
             const int tFace = tAxis * 2 + tUp;
             const int tfs = face_sign(tFace);
-            const CornerValue<T> cv = corner_value<T, D, KIND>(cd, bi, p, bound, tFace, false);
+            const auto cv = corner_value<T, D, KIND>(cd, bi, pp, bound, tFace, false);
             if (cv.cells < 1) {
                 if (always_gradient || cv.neumann) {
                     const T c = T(fs * tfs) * alpha * T(0.25);
@@ -269,83 +345,103 @@ __device__ void matrix_corner_terms(const CornerData<T, D, KIND> &cd, int bi, co
 }
 
 // Explicit corner terms of a right-hand side (getNonOrthoLaplaceRHS_v2, nonortho_rhs.cuh:28-200,
-// useFaceTransform false), for the Laplace term positive on the left-hand side. withVisc: ra
-// includes the viscosity (velocity); A: ra includes 1/A (pressure).
-template <typename T, int D, int KIND>
-__device__ T nonortho_rhs(const CornerData<T, D, KIND> &cd, int bi, const int *p, int f, int flags, bool withVisc,
-                          const T *A) {
-    if (D < 2 || !(flags & (NO_DIRECT_RHS | NO_DIAGONAL_RHS))) return T(0);
+// useFaceTransform false), for the Laplace term positive on the left-hand side, into S[NC].
+// withVisc: ra includes the viscosity (velocity); A: ra includes 1/A (pressure).
+template <typename T, int D, int KIND, int NC = CornerData<T, D, KIND>::NC>
+__device__ __forceinline__ void nonortho_rhs(const CornerData<T, D, KIND> &cd, int bi, const int *pin, int f, int flags, bool withVisc,
+                             const T *A, T *S) {
+#pragma unroll
+    for (int k = 0; k < NC; ++k) S[k] = T(0);
+    if (D < 2 || !(flags & (NO_DIRECT_RHS | NO_DIAGONAL_RHS))) return;
     const pp_block_desc *__restrict__ blocks = cd.blocks;
     const pp_block_desc &b = blocks[bi];
-    T S = T(0);
+    const P3 p{pin[0], pin[1], pin[2]};
     const T viscP = withVisc ? visc_at<T>(cd.dom, b, f) : T(1);
     const T raP = (A ? T(1) / A[b.global_offset + f] : T(1)) * viscP;
 #pragma unroll
     for (int face = 0; face < 2 * D; ++face) {
         const int axis = face >> 1;
         const int fs = face_sign(face);
-        const bool atb = at_bound(p, face, b);
+        const bool atb = at_bound(pin, face, b);
         if (atb && is_fixed(b, face)) {
             if (KIND == NK_PRESSURE || cd.neumann(bi, face)) continue;
             // tangential derivative of the prescribed values along the face
             const pp_boundary_desc &fb = b.bnd[face];
-            const int ff = face_flat(p, face, b);
+            const int ff = face_flat(pin, face, b);
 #pragma unroll
             for (int dd = 1; dd < D; ++dd) {
                 const int tAxis = (axis + dd) % D;
                 const T bA = fb.has_transform ? lap_coef_t<T, D>(Metric<T, D>::face(fb, ff), tAxis, axis)
                                               : (tAxis == axis ? T(1) : T(0));
-                const bool tLo = p[tAxis] == 0, tHi = p[tAxis] == b.size[tAxis] - 1;
-                int lo[3] = {p[0], p[1], p[2]}, hi[3] = {p[0], p[1], p[2]};
-                if (!tLo) lo[tAxis] -= 1;
-                if (!tHi) hi[tAxis] += 1;
+                const bool tLo = pin[tAxis] == 0, tHi = pin[tAxis] == b.size[tAxis] - 1;
+                P3 lo = p, hi = p;
+                if (!tLo) lo.add(tAxis, -1);
+                if (!tHi) hi.add(tAxis, 1);
                 const T df = (tLo || tHi) ? T(1.0) : T(0.5);
-                const T g = df * (cd.fixed(bi, face, hi) - cd.fixed(bi, face, lo));
-                S -= T(fs) * bA * g * viscP;
+                T vh[NC], vl[NC];
+                cd.fixed(bi, face, hi, vh);
+                cd.fixed(bi, face, lo, vl);
+#pragma unroll
+                for (int k = 0; k < NC; ++k) S[k] -= T(fs) * bA * (df * (vh[k] - vl[k])) * viscP;
             }
             continue;
         }
         // neighbour across the face (resolveNeighborCell, topology.cuh:352-398: border 0, axes
         // mapped through a connection)
         int np[3], nw;
-        const int nbi = neighbor(blocks, bi, D, p, face, np, &nw);
+        const int nbi = neighbor(blocks, bi, D, pin, face, np, &nw);
         const pp_block_desc &nb = blocks[nbi];
         const int nf = flat3(np, nb.size);
-        int map[3] = {0, 2, 4};
-        if (atb && b.bnd[face].type == PP_BOUND_CONNECTED)
-#pragma unroll
-            for (int k = 0; k < D; ++k) map[k] = conn_dir(D, map[k], axis, b.bnd[face]);
+        const bool conn = atb && b.bnd[face].type == PP_BOUND_CONNECTED;
         T raN = A ? T(1) / A[nb.global_offset + nf] : T(1);
         if (withVisc) raN = raN * visc_at<T>(cd.dom, nb, nf);
 #pragma unroll
         for (int dd = 1; dd < D; ++dd) {
             const int tAxis = (axis + dd) % D;
             T aN;
-            if (nb.has_transform) aN = lap_coef<T, D>(nb, nf, map[tAxis] >> 1, map[axis] >> 1);
-            else aN = tAxis == axis ? T(1) : T(0);
+            if (nb.has_transform) {
+                const int mt = conn ? conn_dir(D, tAxis * 2, axis, b.bnd[face]) >> 1 : tAxis;
+                const int ma = conn ? conn_dir(D, axis * 2, axis, b.bnd[face]) >> 1 : axis;
+                aN = lap_coef<T, D>(nb, nf, mt, ma);
+            } else {
+                aN = tAxis == axis ? T(1) : T(0);
+            }
             const T faceAlpha = (lap_coef<T, D>(b, f, axis, tAxis) * raP + aN * raN) * T(0.5);
-            T tg = T(0);
+            T tg[NC];
+#pragma unroll
+            for (int k = 0; k < NC; ++k) tg[k] = T(0);
 #pragma unroll
             for (int tUp = 0; tUp < 2; ++tUp) {
                 const int tFace = tAxis * 2 + tUp;
                 const int tfs = face_sign(tFace);
-                const CornerValue<T> cv = corner_value<T, D, KIND>(cd, bi, p, face, tFace, flags & NO_DIRECT_RHS);
+                const auto cv = corner_value<T, D, KIND>(cd, bi, p, face, tFace, flags & NO_DIRECT_RHS);
                 if (cv.cells == 0 && (cv.neumann || KIND == NK_PRESSURE)) {
                     const int tOther = tFace ^ 1;
                     if (flags & NO_DIRECT_RHS) {
-                        tg += T(tfs) * neighbor_data<T, D, KIND>(cd, bi, p, face) * T(0.75);
-                        tg += T(-tfs) * neighbor_data<T, D, KIND>(cd, bi, p, tOther) * T(0.25);
+                        T a[NC], c[NC];
+                        neighbor_data<T, D, KIND>(cd, bi, p, face, a);
+                        neighbor_data<T, D, KIND>(cd, bi, p, tOther, c);
+#pragma unroll
+                        for (int k = 0; k < NC; ++k) {
+                            tg[k] += T(tfs) * a[k] * T(0.75);
+                            tg[k] += T(-tfs) * c[k] * T(0.25);
+                        }
                     }
-                    if (flags & NO_DIAGONAL_RHS)
-                        tg += T(-tfs) * diagonal_data<T, D, KIND>(cd, bi, p, face, tOther) * T(0.25);
+                    if (flags & NO_DIAGONAL_RHS) {
+                        T e[NC];
+                        diagonal_data<T, D, KIND>(cd, bi, p, face, tOther, e);
+#pragma unroll
+                        for (int k = 0; k < NC; ++k) tg[k] += T(-tfs) * e[k] * T(0.25);
+                    }
                 } else {
-                    tg += T(tfs) * cv.data;
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) tg[k] += T(tfs) * cv.data[k];
                 }
             }
-            S -= T(fs) * faceAlpha * tg;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) S[k] -= T(fs) * faceAlpha * tg[k];
         }
     }
-    return S;
 }
 
 } // namespace pp
