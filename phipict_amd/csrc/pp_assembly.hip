@@ -18,7 +18,7 @@ constexpr int ASM_BLOCK = 256;
 // discretization.cuh:758-852; with EPOT the prescribed faces follow
 // computeEpotFluxesNDLoop :856-912 (insulating wall 0, open face velC).
 template <typename T, int D, bool EPOT>
-__device__ void cell_fluxes(const pp_domain &dom, const pp_block_desc *__restrict__ blocks, int bi, const int *p,
+__device__ __forceinline__ void cell_fluxes(const pp_domain &dom, const pp_block_desc *__restrict__ blocks, int bi, const int *p,
                             int f, const T *glob, int env, T *fl) {
     const pp_block_desc &b = blocks[bi];
     T velCd[D];

@@ -68,6 +68,24 @@ __device__ __forceinline__ P3 conn_p(const pp_block_desc *__restrict__ blocks, c
     return out;
 }
 
+// Neighbour across face `bound` (interior, periodic wrap or connected block, border 0): the
+// `neighbor` of the orthogonal kernels on P3 positions. Returns the neighbour's block.
+template <int D>
+__device__ __forceinline__ int neighbor_p(const pp_block_desc *__restrict__ blocks, int bi, const P3 &p, int bound,
+                                          P3 &np) {
+    const pp_block_desc &b = blocks[bi];
+    const int dim = bound >> 1;
+    np = p;
+    const bool atb = at_bnd(p, bound, b);
+    if (atb && b.bnd[bound].type == PP_BOUND_CONNECTED) {
+        np = conn_p<D>(blocks, p, dim, b.bnd[bound], 0);
+        return b.bnd[bound].conn_block;
+    }
+    if (atb && b.bnd[bound].type == PP_BOUND_PERIODIC) np.set(dim, (bound & 1) ? 0 : bsize(b, dim) - 1);
+    else np.add(dim, face_sign(bound));
+    return bi;
+}
+
 // alpha_{c1 c2} = det Minv[c1].Minv[c2] (getLaplaceCoefficientsFull / SingleAxis,
 // discretization.cuh:419-470).
 template <typename T, int D>
@@ -289,10 +307,10 @@ __device__ __forceinline__ T face_alpha_matrix(const pp_domain &dom, const pp_bl
     const pp_block_desc &b = blocks[bi];
     const int axis = face >> 1;
     if (!((0 < p[axis] && p[axis] < b.size[axis] - 1) || !is_fixed(b, face))) return T(0);
-    int np[3], nw;
-    const int nbi = neighbor(blocks, bi, D, p, face, np, &nw);
+    P3 np;
+    const int nbi = neighbor_p<D>(blocks, bi, P3{p[0], p[1], p[2]}, face, np);
     const pp_block_desc &nb = blocks[nbi];
-    const int nf = flat3(np, nb.size);
+    const int nf = flat_p(np, nb);
     T raN = A ? T(1) / A[nb.global_offset + nf] : T(1);
     if (withVisc) raN = raN * visc_at<T>(dom, nb, nf);
     const T aP = lap_coef<T, D>(b, f, axis, tAxis);
@@ -388,10 +406,10 @@ __device__ __forceinline__ void nonortho_rhs(const CornerData<T, D, KIND> &cd, i
         }
         // neighbour across the face (resolveNeighborCell, topology.cuh:352-398: border 0, axes
         // mapped through a connection)
-        int np[3], nw;
-        const int nbi = neighbor(blocks, bi, D, pin, face, np, &nw);
+        P3 np;
+        const int nbi = neighbor_p<D>(blocks, bi, p, face, np);
         const pp_block_desc &nb = blocks[nbi];
-        const int nf = flat3(np, nb.size);
+        const int nf = flat_p(np, nb);
         const bool conn = atb && b.bnd[face].type == PP_BOUND_CONNECTED;
         T raN = A ? T(1) / A[nb.global_offset + nf] : T(1);
         if (withVisc) raN = raN * visc_at<T>(cd.dom, nb, nf);
