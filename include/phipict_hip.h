@@ -240,6 +240,15 @@ int pp_transpose_2d(const void *in, int32_t rows, int32_t cols, int32_t dtype, v
  * Needs dom->blocks with csr_offset set; written once per domain. */
 int pp_csr_pattern(const pp_domain *dom, int32_t *row, int32_t *col, void *stream);
 
+/* non_ortho_flags of the setup entry points below: the bits of csrc/phipict/piso/
+ * non_ortho_flags.h. NON_ORTHO_DIRECT_MATRIX (1) and NON_ORTHO_CENTER_MATRIX (16) add the
+ * corner terms to the matrices (nonortho_laplace.cuh:53-131, advection.cu:177-240,
+ * pressure.cu:104-160); NON_ORTHO_DIRECT_RHS (2) and NON_ORTHO_DIAGONAL_RHS (8) add the explicit
+ * corner terms of velocity_result / scalar_result / pressure_result to the right-hand sides
+ * (nonortho_rhs.cuh:28-200), which then must be set; NON_ORTHO_DIAGONAL_MATRIX (4) is ignored,
+ * as in the reference. pp_setup_epot_matrix ignores the flags (its corner terms are not
+ * assembled; the package refuses them on skewed grids). */
+
 /* replaces SetupAdvectionMatrixEulerImplicit (csrc/phipict/piso/advection.cu:1230-1250),
  * kernel PISO_build_matrix advection.cu:26-306: writes C (row, col, values) and A. */
 int pp_setup_advection_matrix(const pp_domain *dom, int32_t non_ortho_flags, void *stream);
@@ -259,7 +268,9 @@ int pp_setup_pressure_matrix(const pp_domain *dom, int32_t non_ortho_flags, void
  * pressure.cu:349-522 and k_computePressureRHSdivergenceFromFlux :755-803. */
 int pp_setup_pressure_rhs(const pp_domain *dom, int32_t non_ortho_flags, int32_t time_step_norm,
                           void *stream);
-/* replaces SetupPressureRHSdiv (pressure.cu:1030-1070): divergence only. */
+/* replaces SetupPressureRHSdiv (pressure.cu:1030-1070): divergence only; with the flags'
+ * right-hand-side bits plus k_pressureRHSaddNonOrthoComponents (pressure.cu:838-860), which
+ * pp_setup_pressure_rhs applies too. */
 int pp_setup_pressure_rhs_div(const pp_domain *dom, int32_t non_ortho_flags, int32_t time_step_norm,
                               void *stream);
 /* replaces CorrectVelocity version 1 (csrc/phipict/piso/velocity_correction.cu:216-271),
