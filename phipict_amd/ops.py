@@ -60,6 +60,7 @@ def SetupAdvectionMatrix(domain: Domain, timeStep, nonOrthoFlags: int = 0, forPa
     else:
         _lib.call("pp_setup_advection_matrix", C.byref(d), flags, _stream(domain))
     domain.C.mark_ell_valid()
+    domain._a_version += 1  # A changed: a pressure matrix built from the old one is stale
 
 
 def SetupAdvectionScalar(domain: Domain, timeStep, nonOrthoFlags: int = 0) -> None:
@@ -84,6 +85,15 @@ def SetupPressureMatrix(domain: Domain, timeStep, nonOrthoMode: int = 0, useFace
     d = _prep(domain, timeStep)
     _lib.call("pp_setup_pressure_matrix", C.byref(d), _flags(domain, nonOrthoMode), _stream(domain))
     domain.P.mark_ell_valid()
+    domain._p_matrix_key = pressure_matrix_key(domain, nonOrthoMode)
+
+
+def pressure_matrix_key(domain: Domain, nonOrthoMode: int) -> tuple:
+    """What the pressure matrix's values depend on besides the (static) metrics: the advection
+    diagonal's version, the effective flags, and the value tensor (replaced or written since)."""
+    P, A = domain.P, domain.A
+    return (domain._a_version, A.data_ptr(), A._version, _flags(domain, nonOrthoMode), P.value.data_ptr(),
+            P.value._version, domain.getBatchSize())
 
 
 def SetupPressureRHS(domain: Domain, timeStep, nonOrthoMode: int = 0, useFaceTransform: bool = False,

@@ -346,6 +346,15 @@ class Simulation:
         final = cstep == corrector_steps - 1 and pstep == self.pressure_non_ortho_steps - 1
         return self.pressure_tol if final else self.pressure_tol_intermediate
 
+    def _setup_pressure_matrix(self, domain, time_step, flags: int) -> None:
+        """The pressure matrix of this corrector (pressure.cu:947-975). It depends on the
+        advection diagonal, the metrics and the flags only, so a corrector after the first of a
+        step finds it assembled already and skips the launch (unless the values were written
+        since)."""
+        if domain.P is not None and getattr(domain, "_p_matrix_key", None) == ops.pressure_matrix_key(domain, flags):
+            return
+        ops.SetupPressureMatrix(domain, time_step, flags, False)
+
     def _pressure_initial_guess(self, cstep: int, reference: torch.Tensor):
         if not self.pressure_warm_start:
             return None
@@ -580,7 +589,8 @@ class Simulation:
             # pressure correctors
             for cstep in range(self.corrector_steps):
                 if not self.non_orthogonal:
-                    ops.SetupPressureCorrection(domain, time_step, 0, False, timeStepNorm=self.pressure_time_step_normalized)
+                    self._setup_pressure_matrix(domain, time_step, 0)
+                    ops.SetupPressureRHS(domain, time_step, 0, False, timeStepNorm=self.pressure_time_step_normalized)
                     self._run_hook(Hook.POST_PRESSURE_SETUP, local_step=step, time_step=time_step)
                     # solved in place in the domain's pressure vector (zero start, or the warm start)
                     x = domain.pressureResult
@@ -604,7 +614,7 @@ class Simulation:
                     domain.UpdateDomainData()
                     self._run_hook(Hook.POST_PRESSURE_RESULT, local_step=step, time_step=time_step)
                 else:
-                    ops.SetupPressureMatrix(domain, time_step, flags, False)
+                    self._setup_pressure_matrix(domain, time_step, flags)
                     for pstep in range(self.pressure_non_ortho_steps):
                         if pstep == 0:
                             ops.SetupPressureRHS(domain, time_step, flags, False,
