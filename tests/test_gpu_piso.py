@@ -500,3 +500,28 @@ def test_saved_domain_steps_identically(tmp_path):
         runs.append([to_np(t) for t in (b.velocity, b.pressure, b.passiveScalar, dom.epotResult)])
     for a, c in zip(*runs):
         np.testing.assert_array_equal(a, c)
+
+
+def test_pressure_matrix_assembled_once_per_step(monkeypatch):
+    """The second corrector of a step reuses the pressure matrix (it depends on the advection
+    diagonal only); a caller's write to its values between correctors forces a rebuild."""
+    import phipict_amd as pa
+
+    dom = build_domain(cases.channel2d(), 0.01, 1e-2, torch.float64)
+    calls = []
+    orig = pa.ops.SetupPressureMatrix
+
+    def counting(*a, **k):
+        calls.append(1)
+        return orig(*a, **k)
+
+    monkeypatch.setattr(pa.ops, "SetupPressureMatrix", counting)
+    sim = pa.Simulation(domain=dom, dt=1e-2, non_orthogonal=False, corrector_steps=2)
+    assert sim.single_step() and len(calls) == 1
+    assert sim.single_step() and len(calls) == 2
+
+    def touch(domain, **kw):
+        domain.P.value.mul_(1.0)
+
+    sim.hooks.append(pa.Hook.POST_PRESSURE_RESULT, touch)
+    assert sim.single_step() and len(calls) == 4
