@@ -9,6 +9,11 @@ CG pressure and potential solves to the absolute tolerance 1e-8; initial velocit
 (seed 1). A step is one full PISO + magnetic step of the library (``MHDSimulation.single_step``)
 with every field resident in HBM.
 
+``--skew F`` jitters the duct's interior vertices by F times the smallest spacing (seeded) and
+steps in the reference's default non-orthogonal mode (corner terms in the advection and pressure
+matrices and on their right-hand sides); the CPU baseline is skipped then.
+``--workload tjunction-adjoint`` times a checkpointed adjoint rollout instead.
+
 Printed: one JSON line with steps/s of the uninstrumented timed steps, the roofline of the
 kernel with the largest share of the kernel time (algorithmic bytes per launch / average launch
 time, from HIP events recorded on the kernel's own stream for a pseudo-random quarter of the
