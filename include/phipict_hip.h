@@ -161,9 +161,9 @@ typedef struct pp_amg_level {
     /* optional ELL copy of A (layout of pp_csr.ell_col/ell_val; no reference counterpart) */
     int32_t a_ell_width, pad;
     const int32_t *a_ell_col; const void *a_ell_val;
-    /* optional sliced ELL copies of A and R (values in the hierarchy dtype; A only when shared
-     * by all right-hand sides) */
-    pp_sell a_sell, r_sell;
+    /* optional sliced ELL copies of A, R and P (values in the hierarchy dtype; A only when
+     * shared by all right-hand sides) */
+    pp_sell a_sell, r_sell, p_sell;
 } pp_amg_level;
 
 /* ---- library ---- */

@@ -150,6 +150,7 @@ class AmgLevel(C.Structure):
         ("a_ell_val", C.c_void_p),
         ("a_sell", SellDesc),
         ("r_sell", SellDesc),
+        ("p_sell", SellDesc),
     ]
 
 

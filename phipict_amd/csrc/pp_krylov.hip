@@ -2097,11 +2097,12 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
         };
         attach(LA[l], lv[l].a_sell);
         attach(LR[l], lv[l].r_sell);
+        attach(LP[l], lv[l].p_sell);
         // ELL and sliced ELL copies are read with one lane per row (coalesced across the
         // wavefront; 2, 4 or 8 lanes per sliced row measured slower on the duct hierarchy)
         wa[l] = (LA[l].ell_w || LA[l].sell_off) ? 1 : row_group_width(double(lv[l].nnz_a) / std::max(1, lv[l].n));
         wr[l] = LR[l].sell_off ? 1 : row_group_width(double(lv[l].nnz_r) / std::max(1, lv[l].n_coarse));
-        wp[l] = row_group_width(double(lv[l].nnz_p) / std::max(1, lv[l].n));
+        wp[l] = LP[l].sell_off ? 1 : row_group_width(double(lv[l].nnz_p) / std::max(1, lv[l].n));
     }
     // first level handled by the one-workgroup tail (never the finest level)
     int lt = L;

@@ -345,7 +345,7 @@ def pack_amg_levels(levels: Sequence[Sequence[torch.Tensor]], hd, extras: Option
                 raise ValueError("ELL copy does not match the level operator")
             keep.extend((col, val))
             L.a_ell_width, L.a_ell_col, L.a_ell_val = int(width), _lib.ptr(col), _lib.ptr(val)
-        for key, field_ in (("a_sell", "a_sell"), ("r_sell", "r_sell")):
+        for key, field_ in (("a_sell", "a_sell"), ("r_sell", "r_sell"), ("p_sell", "p_sell")):
             if key in ex:
                 off, width, col, val = ex[key]
                 if val.dtype != hd or col.numel() != val.numel():

@@ -167,9 +167,9 @@ class AMGHierarchy:
 
     def native_extras(self, dtype) -> list:
         """Per level above the coarsest: coalesced copies for the V-cycle kernels in ``dtype``
-        -- "ell": (columns, values, width) of the finest operator, "a_sell" / "r_sell":
-        sliced ELL copies (slice offsets, widths, columns, values) of large operators and
-        restrictions."""
+        -- "ell": (columns, values, width) of the finest operator, "a_sell" / "r_sell" /
+        "p_sell": sliced ELL copies (slice offsets, widths, columns, values) of large operators,
+        restrictions and interpolations."""
         out = []
         for i, lvl in enumerate(self.levels[:-1]):
             ex = lvl.native_extra.get(dtype)
@@ -189,6 +189,9 @@ class AMGHierarchy:
                 layout = lvl.shared.get("R")
                 if layout is not None:
                     ex["r_sell"] = layout.desc(layout.cached_values(lvl.R_val, dtype))
+                layout = lvl.shared.get("P")
+                if layout is not None:
+                    ex["p_sell"] = layout.desc(layout.cached_values(lvl.P_val, dtype))
                 lvl.native_extra[dtype] = ex
             out.append(ex)
         return out
@@ -342,6 +345,8 @@ def build_amg_hierarchy(csr_mat: CSRmatrix, *, method: str = "ruge_stuben", max_
             level.gmap = (_dev(seg, device), _dev(src, device), _dev(w, device, torch.float64))
             if rr.size - 1 >= SELL_MIN_ROWS:
                 level.shared["R"] = SellLayout(rr, level.R_row, level.R_col)
+            if pr.size - 1 >= SELL_MIN_ROWS:
+                level.shared["P"] = SellLayout(pr, level.P_row, level.P_col)
         if i > 0 and lv["n"] >= SELL_MIN_ROWS and "P" in lv:
             level.shared["A"] = SellLayout(lv["A"][0], A.row, A.index)
         levels.append(level)
