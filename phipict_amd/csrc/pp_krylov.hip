@@ -1311,7 +1311,8 @@ __device__ __forceinline__ H level_row_sum(const Csr<H> &A, const H *__restrict_
             const int sl = i >> 6;
             const int w = A.sell_w[sl];
             const int base = A.sell_off[sl] + (i & 63);
-#pragma unroll 4
+            // eight entries' loads in flight per round trip (the sum keeps its order)
+#pragma unroll 8
             for (int k = lane; k < w; k += W) sum += A.sell_val[base + 64 * k] * gather(A.sell_col[base + 64 * k]);
         }
 #pragma unroll
