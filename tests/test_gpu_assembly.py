@@ -286,3 +286,44 @@ def test_mhd_cross_products(dtype, dims, per_env):
     with pytest.raises(RuntimeError):
         _lib.call("pp_lorentz_source", _lib.ptr(Jt), total, B, dims, _lib.ptr(ebt), 0, _lib.ptr(Nt), 0, g0, total,
                   None, 0, 0, _lib.ptr(out), _lib.dtype_code(td), s)
+
+
+def test_nonortho_batched_per_env(dtype):
+    """Two environments of a skewed channel with different velocity and pressure results: the
+    corner terms of each environment's matrices and right-hand sides equal those of a single
+    oracle domain with that environment's state."""
+    import phipict_amd as pa
+
+    flags = O.NO_DEFAULT_MODE | O.NO_DIRECT_RHS
+    specs = cases.skew(cases.channel2d(12, 9), 0.25, seed=2)
+    specs2 = cases.skew(cases.channel2d(12, 9, seed=5), 0.25, seed=2)
+    dom = build_domain(specs, NU, DT, _tdt(dtype), batch=2)
+    blk = dom.getBlocks()[0]
+    v = blk.velocity.clone()
+    v[1] = torch.as_tensor(specs2[0].velocity, dtype=v.dtype)
+    blk.setVelocity(v)
+    dom.UpdateDomainData()
+    ts = torch.tensor([DT, DT], dtype=_tdt(dtype))
+    n = dom.getTotalSize()
+    rng = np.random.default_rng(31)
+    vres = rng.standard_normal(2 * 2 * n).astype(dtype)
+    pres = rng.standard_normal(2 * n).astype(dtype)
+    pa.SetupAdvectionMatrix(dom, ts, flags)
+    dom.setVelocityResult(torch.as_tensor(vres, device=dom.getDevice()))
+    pa.SetupAdvectionVelocity(dom, ts, flags, False)
+    pa.SetupPressureMatrix(dom, ts, flags)
+    dom.setPressureResult(torch.as_tensor(pres, device=dom.getDevice()))
+    pa.SetupPressureRHS(dom, ts, flags)
+    nnz = dom.C.getNnz()
+    for e, sp_ in enumerate([specs, specs2]):
+        od = O.OracleDomain(sp_, NU, DT, dtype)
+        ccol, cval, A = od.advection_matrix(flags)
+        close(to_np(dom.C.value)[e * nnz:(e + 1) * nnz], cval, dtype, f"env {e} advection matrix")
+        vr = vres[e * 2 * n:(e + 1) * 2 * n]
+        close(to_np(dom.velocityRHS)[e * 2 * n:(e + 1) * 2 * n], od.advection_rhs(False, flags, vr), dtype,
+              f"env {e} momentum rhs")
+        pcol, pval = od.pressure_matrix(A, flags)
+        close(to_np(dom.P.value)[e * nnz:(e + 1) * nnz], pval, dtype, f"env {e} pressure matrix")
+        div = od.pressure_div(od.pressure_rhs(A, ccol, cval, vr)) + od.nonortho_rhs(O.OK_PRES, flags, 0,
+                                                                                     pres[e * n:(e + 1) * n], A)
+        close(to_np(dom.pressureRHSdiv)[e * n:(e + 1) * n], div, dtype, f"env {e} divergence with corner terms")
