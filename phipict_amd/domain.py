@@ -747,11 +747,14 @@ class Block:
         products ``Minv[i] . Minv[j]`` of every cell and of every fixed face's transform are 0."""
         if not self._orthogonality()[1]:
             return False
-        for bound in self.getBoundaries():
-            if isinstance(bound, FixedBoundary) and bound.transform is not None:
-                if not _metric_orthogonality(bound.transform, self.getSpatialDims())[1]:
-                    return False
-        return True
+        faces = [b.transform for b in self.getBoundaries() if isinstance(b, FixedBoundary) and b.transform is not None]
+        key = tuple((t.data_ptr(), t._version) for t in faces)
+        cached = getattr(self, "_face_ortho", None)
+        if cached is None or cached[0] != key:
+            # one device sync per new set of face transforms
+            cached = (key, all(_metric_orthogonality(t, self.getSpatialDims())[1] for t in faces))
+            self._face_ortho = cached
+        return cached[1]
 
     def _orthogonality(self) -> tuple:
         t = self.transform
