@@ -2,7 +2,7 @@
 and boundary layout and the sparsity patterns, each with its own state (``[B, ...]`` block
 fields) -- stacking single-environment domains into one, copying an environment's state, and
 saving the state of some environments across a substep they sit out
-(src/phipict/batching.py:19-320)."""
+(src/phipict/batching.py:19-325)."""
 from __future__ import annotations
 
 from typing import Callable, Optional, Sequence
@@ -79,17 +79,47 @@ def stack_domains(domains: Sequence[Domain]) -> Domain:
     return batched
 
 
-class EnvStateSnapshot:
-    """State of some environments of a batched domain, to undo a substep for them.
+class _SavedRows:
+    """Per-environment rows of one tensor that a step may write in place or replace: the
+    tensor is looked up again when restoring (``lookup``), viewed as ``[B, -1]`` (block fields
+    are ``[B, ...]``, flat solve vectors hold B equal parts) and the saved rows written back."""
 
-    Saved: the block fields (velocity, pressure, potential), the domain's solution vectors
-    (the next solve's initial guess and the potential the Lorentz force uses), ``extra`` flat
-    per-environment tensors, the velocity sources and the fixed faces' velocities. Fields are
-    read through getters when restoring, since a step may replace a tensor rather than write
-    it; sources and face velocities are restored only if the step replaced or wrote them, and
-    restoring some environments of a shared one (batch 1) makes it per-environment.
-    Potential values of fixed faces are one array shared by all environments here and are not
-    part of the per-environment state.
+    __slots__ = ("lookup", "rows")
+
+    def __init__(self, lookup: Callable[[], Optional[torch.Tensor]], rows: torch.Tensor) -> None:
+        self.lookup = lookup
+        self.rows = rows
+
+
+class _SavedInput:
+    """A per-block input a step may replace through its setter (a velocity source or a fixed
+    face's velocity): kept as the tensor identity (pointer and version) seen when saving plus a
+    copy, so that restoring touches only inputs the step changed."""
+
+    __slots__ = ("lookup", "assign", "ptr", "version", "copy")
+
+    def __init__(self, lookup, assign, t: Optional[torch.Tensor]) -> None:
+        self.lookup, self.assign = lookup, assign
+        self.ptr = 0 if t is None else t.data_ptr()
+        self.version = 0 if t is None else t._version
+        self.copy = None if t is None else t.clone()
+
+    def unchanged(self, t: Optional[torch.Tensor]) -> bool:
+        return t is None or (t.data_ptr() == self.ptr and t._version == self.version)
+
+
+class EnvStateSnapshot:
+    """The state of a subset of a batch's environments, saved before a substep those
+    environments sit out and written back after it (batching.py:178-325).
+
+    Saved per environment: every block's velocity, pressure and potential, the domain's
+    solution vectors (the next solve's warm start and the potential the Lorentz force reads),
+    any ``extra`` per-environment tensors (given as zero-argument lookups), and -- only if the
+    step replaces or writes them -- the blocks' velocity sources and the fixed faces'
+    velocities. Tensors are looked up again when restoring, since a step may swap a tensor for
+    a new one. A shared input (batch 1) the step changed comes back per-environment: the other
+    environments keep the new value. Dirichlet potential values of fixed faces are one array
+    for all environments here and are not saved.
     """
 
     def __init__(self, domain: Domain, env_idx: torch.Tensor,
@@ -97,60 +127,54 @@ class EnvStateSnapshot:
         self._domain = domain
         self._batch = domain.getBatchSize()
         self._idx = env_idx.to(device=domain.getDevice(), dtype=torch.long)
-        getters: list = []
-        for block in domain.getBlocks():
-            getters += [lambda block=block: block.velocity, lambda block=block: block.pressure,
-                        lambda block=block: block.epot]
-        getters += [lambda: domain.velocityResult, lambda: domain.pressureResult, lambda: domain.epotResult, *extra]
-        self._fields: list = []
-        self._bounds: list = []
         with torch.no_grad():
-            for getter in getters:
-                t = getter()
-                if t is None or t.numel() == 0:
-                    continue
-                self._fields.append((getter, self._env_rows(t).index_select(0, self._idx)))
-            for block in domain.getBlocks():
-                src = block.velocitySource
-                self._bounds.append((lambda block=block: block.velocitySource, block.setVelocitySource,
-                                     0 if src is None else src.data_ptr(), 0 if src is None else src._version,
-                                     None if src is None else src.clone()))
-                for bound in block.getBoundaries():
-                    if not isinstance(bound, FixedBoundary):
-                        continue
-                    t = bound.velocity
-                    self._bounds.append((lambda bound=bound: bound.velocity, bound.setVelocity, t.data_ptr(),
-                                         t._version, t.clone()))
+            self._rows = [_SavedRows(lookup, self._per_env(t).index_select(0, self._idx))
+                          for lookup in self._state_lookups(domain, extra)
+                          for t in (lookup(),) if t is not None and t.numel() > 0]
+            self._inputs = list(self._input_records(domain))
 
-    def _env_rows(self, t: torch.Tensor) -> torch.Tensor:
-        """``t`` as ``[B, -1]``: block fields are ``[B, ...]``, flat vectors ``B`` parts."""
-        if t.numel() % self._batch != 0:
-            raise ValueError(f"Tensor of {t.numel()} elements has no slice per environment "
-                             f"(batch size {self._batch}).")
+    @staticmethod
+    def _state_lookups(domain: Domain, extra) -> list:
+        lookups = []
+        for blk in domain.getBlocks():
+            for name in ("velocity", "pressure", "epot"):
+                lookups.append(lambda blk=blk, name=name: getattr(blk, name))
+        for name in ("velocityResult", "pressureResult", "epotResult"):
+            lookups.append(lambda name=name: getattr(domain, name))
+        return lookups + list(extra)
+
+    @staticmethod
+    def _input_records(domain: Domain):
+        for blk in domain.getBlocks():
+            yield _SavedInput(lambda blk=blk: blk.velocitySource, blk.setVelocitySource, blk.velocitySource)
+            for face in blk.getBoundaries():
+                if isinstance(face, FixedBoundary):
+                    yield _SavedInput(lambda face=face: face.velocity, face.setVelocity, face.velocity)
+
+    def _per_env(self, t: torch.Tensor) -> torch.Tensor:
+        if t.numel() % self._batch:
+            raise ValueError(f"A tensor of {t.numel()} elements cannot be split into {self._batch} environments.")
         return t.view(self._batch, -1)
 
     def restore(self) -> None:
         """Write the saved state back into the saved environments."""
-        B = self._batch
         with torch.no_grad():
-            for getter, rows in self._fields:
-                t = getter()
-                if t is None or t.numel() == 0:
-                    continue
-                self._env_rows(t).index_copy_(0, self._idx, rows)
-            changed = False
-            for getter, setter, ptr, version, saved in self._bounds:
-                t = getter()
-                if t is None or (t.data_ptr() == ptr and t._version == version):
-                    continue  # untouched by the step
-                if saved is None:
-                    saved = torch.zeros_like(t[:1])  # there was no source
-                if t.shape[1:] != saved.shape[1:]:
-                    raise ValueError(f"Boundary data changed shape during the step: "
-                                     f"{tuple(saved.shape)} to {tuple(t.shape)}.")
-                restored = t.expand(B, *t.shape[1:]).clone()
-                restored[self._idx] = saved.expand(B, *saved.shape[1:])[self._idx]
-                setter(restored)
-                changed = True
-        if changed:
+            for rec in self._rows:
+                t = rec.lookup()
+                if t is not None and t.numel() > 0:
+                    self._per_env(t).index_copy_(0, self._idx, rec.rows)
+            replaced = [rec for rec in self._inputs if not rec.unchanged(rec.lookup())]
+            for rec in replaced:
+                self._put_back(rec, rec.lookup())
+        if replaced:
             self._domain.UpdateDomainData()
+
+    def _put_back(self, rec: _SavedInput, now: torch.Tensor) -> None:
+        """The changed input ``now`` with the saved environments' rows restored (an input that
+        did not exist when saving restores as zeros)."""
+        old = rec.copy if rec.copy is not None else torch.zeros_like(now[:1])
+        if old.shape[1:] != now.shape[1:]:
+            raise ValueError(f"An input changed its shape during the step: {tuple(old.shape)} -> {tuple(now.shape)}.")
+        full = now.expand(self._batch, *now.shape[1:]).clone()
+        full.index_copy_(0, self._idx, old.expand(self._batch, *old.shape[1:]).index_select(0, self._idx))
+        rec.assign(full)
