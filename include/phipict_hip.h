@@ -143,12 +143,16 @@ typedef struct pp_solve_result {
     int32_t pad;
 } pp_solve_result;
 
-/* Sliced ELL copy of a CSR matrix (no reference counterpart): rows in slices of 64, slice s
- * padded to slice_width[s] entries per row; entry k of row 64 s + j at slice_off[s] + 64 k + j
- * (col, val), padding with column 0 and value 0. All four pointers set, or all null. */
+/* Wave-interleaved sliced ELL copy of a CSR matrix (no reference counterpart): rows in slices
+ * of 64 / lanes; slice s holds slice_steps[s] columns of 64 entries, entry k * lanes + j of row
+ * r of the slice at slice_off[s] + 64 k + r * lanes + j (col, val), padding with column 0 and
+ * value 0. A wavefront reads one slice with `lanes` lanes per row. total: entries per operator
+ * (the value stride of batched operators). The four pointers set, or all null. */
 typedef struct pp_sell {
-    const int32_t *slice_off, *slice_width, *col;
+    const int32_t *slice_off, *slice_steps, *col;
     const void *val;
+    int32_t lanes, pad;
+    int64_t total;
 } pp_sell;
 
 /* One AMG level above the coarsest (reference layout of _C.AMGPCGSolve levels,
@@ -412,12 +416,17 @@ int pp_galerkin_rap(const int32_t *r_row, const int32_t *r_col, const double *r_
  * output may be NULL; no reference counterpart. */
 int pp_ell_from_csr(const int32_t *row, const int32_t *col, const void *val, int32_t n, int32_t nnz, int32_t nmat,
                     int32_t width, int32_t dtype, int32_t *ell_col, void *ell_val, int32_t ell_dtype, void *stream);
-/* Fill a sliced ELL copy (pp_sell layout) of the CSR matrix (row, col, val) with n rows, whose
- * slice offsets and widths the caller computed from the row lengths: columns (sell_col, may be
- * null) and values converted from dtype to sell_dtype (sell_val, may be null). */
-int pp_sell_fill(const int32_t *row, const int32_t *col, const void *val, int32_t n, const int32_t *slice_off,
-                 const int32_t *slice_width, int32_t dtype, int32_t *sell_col, void *sell_val, int32_t sell_dtype,
-                 void *stream);
+/* Fill a sliced ELL copy (pp_sell layout with `lanes` lanes per row) of the CSR matrix (row,
+ * col, val) with n rows and nnz entries, whose slice offsets and steps the caller computed from
+ * the row lengths: columns (sell_col, may be null) and the values of nmat operators (stride nnz
+ * in, total out) converted from dtype to sell_dtype (sell_val, may be null). */
+int pp_sell_fill(const int32_t *row, const int32_t *col, const void *val, int32_t n, int32_t nnz, int32_t nmat,
+                 int32_t lanes, const int32_t *slice_off, const int32_t *slice_steps, int64_t total, int32_t dtype,
+                 int32_t *sell_col, void *sell_val, int32_t sell_dtype, void *stream);
+
+/* Autotuner probe (no reference counterpart): mean duration in microseconds of y = M x through
+ * the sliced copy M (rows x cols, values in hdtype), over reps launches after a warm-up. */
+int pp_sell_probe(const pp_sell *M, int32_t rows, int32_t cols, int32_t hdtype, int32_t reps, void *stream, double *us);
 
 /* Galerkin map (the precomputed product map of _galerkin_map, src/phipict/solvers/amg.py:
  * 1114-1180, used here for every refresh): host construction for fixed R, P, fine pattern A

@@ -370,6 +370,8 @@ class OraclePISO:
                     self.p_stale = True
             if self.normalize_pressure:
                 p = p - np.mean(p)
+            # the last corrector's pressure system, for residual checks of other solutions
+            self.last_pressure_system = (pcol, pval, div)
             self.pressure_result = np.ascontiguousarray(p, od.dtype)
             od.blocks_from_flat(self.pressure_result, od.pressure, 1)
             self.vel_result = od.correct_velocity(A, prhs)

@@ -120,9 +120,12 @@ class SolveResult(C.Structure):
 class SellDesc(C.Structure):
     _fields_ = [
         ("slice_off", C.c_void_p),
-        ("slice_width", C.c_void_p),
+        ("slice_steps", C.c_void_p),
         ("col", C.c_void_p),
         ("val", C.c_void_p),
+        ("lanes", C.c_int32),
+        ("pad", C.c_int32),
+        ("total", C.c_int64),
     ]
 
 
@@ -217,7 +220,8 @@ SIGNATURES = {
     "pp_kernel_timer_stop": [],
     "pp_kernel_timer_read": [_i32, _vp, _vp, _vp, _vp, _vp, _P(_i32)],
     "pp_copy_bytes": [_vp, _vp, _i64, _vp],
-    "pp_sell_fill": [_vp, _vp, _vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _vp],
+    "pp_sell_fill": [_vp, _vp, _vp, _i32, _i32, _i32, _i32, _vp, _vp, _i64, _i32, _vp, _vp, _i32, _vp],
+    "pp_sell_probe": [_P(SellDesc), _i32, _i32, _i32, _i32, _vp, _P(_dbl)],
     "pp_cross_eb": [_vp, _i32, _i32, _i32, _vp, _i32, _vp, _i32, _vp],
     "pp_lorentz_source": [_vp, _i32, _i32, _i32, _vp, _i32, _vp, _i32, _i32, _i32, _vp, _i32, _i32, _vp, _i32, _vp],
 }

@@ -314,16 +314,17 @@ def SolveLinear(A: CSRmatrix, RHS: torch.Tensor, x: torch.Tensor, maxIterations=
 def pack_amg_levels(levels: Sequence[Sequence[torch.Tensor]], hd, extras: Optional[Sequence[dict]] = None) -> tuple:
     """ctypes level descriptors for ``levels`` (per level: A row, col, val, dinv, R row, col,
     val, P row, col, val, all values in ``hd``) and optional per-level coalesced copies
-    (``extras[i]``: "ell" = (columns, values, width) of A, "a_sell" / "r_sell" = (slice
-    offsets, slice widths, columns, values) of A / R); returns (array, tensors kept alive)."""
+    (``extras[i]``: "ell" = (columns, values, width) of the finest A, "a_sell" / "r_sell" /
+    "p_sell" = (slice offsets, slice steps, columns, values, lanes, entries per operator) of
+    A / R / P in the wave-interleaved sliced layout); returns (array, tensors kept alive)."""
     lv = (_lib.AmgLevel * len(levels))()
     keep = []
     for i, t in enumerate(levels):
         if len(t) != 10:
             raise ValueError("each AMG level needs (A row, col, val, dinv, R row, col, val, P row, col, val)")
         ex = extras[i] if extras is not None else {}
-        if t[2] is None and (i != 0 or "ell" not in ex):
-            raise ValueError("only the finest level may omit its CSR values, and only with an ELL copy")
+        if t[2] is None and not ((i == 0 and "ell" in ex) or "a_sell" in ex):
+            raise ValueError("a level may omit its CSR values only with a sliced copy (or, the finest, an ELL copy)")
         if (t[2] is not None and t[2].dtype != hd) or t[3].dtype != hd or t[6].dtype != hd or t[9].dtype != hd:
             raise TypeError("all hierarchy values must have the dtype of the coarse pseudo-inverse")
         L = lv[i]
@@ -347,11 +348,12 @@ def pack_amg_levels(levels: Sequence[Sequence[torch.Tensor]], hd, extras: Option
             L.a_ell_width, L.a_ell_col, L.a_ell_val = int(width), _lib.ptr(col), _lib.ptr(val)
         for key, field_ in (("a_sell", "a_sell"), ("r_sell", "r_sell"), ("p_sell", "p_sell")):
             if key in ex:
-                off, width, col, val = ex[key]
-                if val.dtype != hd or col.numel() != val.numel():
+                off, steps, col, val, lanes, total = ex[key]
+                if val.dtype != hd or col.numel() != total or val.numel() % total:
                     raise ValueError("sliced ELL copy does not match the level")
-                keep.extend((off, width, col, val))
-                setattr(L, field_, _lib.SellDesc(_lib.ptr(off), _lib.ptr(width), _lib.ptr(col), _lib.ptr(val)))
+                keep.extend((off, steps, col, val))
+                setattr(L, field_, _lib.SellDesc(_lib.ptr(off), _lib.ptr(steps), _lib.ptr(col), _lib.ptr(val), int(lanes), 0,
+                                                 int(total)))
     return lv, keep
 
 

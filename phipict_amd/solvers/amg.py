@@ -10,6 +10,7 @@ hierarchy's native dtype (float32 by default, :956).
 from __future__ import annotations
 
 import ctypes as C
+import os
 import time
 from dataclasses import dataclass, field
 from typing import Any, Optional, Sequence
@@ -55,45 +56,57 @@ class Level:
     native_extra: dict = field(default_factory=dict)
 
 
-# Levels and restrictions with at least this many rows get a sliced ELL copy read with one lane
-# per row (coalesced); with fewer rows one lane per row leaves too few wavefronts per SIMD to
-# hide memory latency and the CSR kernels with lane groups per row are used instead.
-SELL_MIN_ROWS = 131072
+#: lanes per row the autotuner tries for a sliced copy (a wavefront reads 64 / lanes rows)
+SELL_LANES = (1, 2, 4, 8, 16)
+#: levels of at most this many rows run in the library's one-workgroup coarse tail, on CSR
+TAIL_ROWS = 128
+#: time the candidate lanes per row on the device (PHIPICT_AMD_AUTOTUNE=0: row-length rule)
+AUTOTUNE = os.environ.get("PHIPICT_AMD_AUTOTUNE", "1") != "0"
+#: fixed lanes per row for every sliced copy (tests), or None
+FORCE_LANES: Optional[int] = None
+#: autotuner decisions: (rows, entries, {lanes: microseconds}, chosen lanes)
+TUNING_LOG: list = []
 
 
 class SellLayout:
-    """Slices of 64 rows of a CSR pattern padded to the slice's longest row: device slice
-    offsets, widths and columns (the pp_sell layout), plus value copies per dtype for
-    matrices whose values never change (R)."""
+    """Wave-interleaved sliced ELL layout of a CSR pattern (the pp_sell layout): slices of
+    64 / lanes rows, each padded to its longest row rounded up to a multiple of ``lanes``;
+    entry k * lanes + j of row r of slice s at slice_off[s] + 64 k + r * lanes + j. Holds the
+    device slice offsets, steps and columns, plus value copies per dtype for matrices whose
+    values never change (R, P)."""
 
-    def __init__(self, row_host: np.ndarray, row_dev: torch.Tensor, col_dev: torch.Tensor):
+    def __init__(self, row_host: np.ndarray, row_dev: torch.Tensor, col_dev: torch.Tensor, lanes: int = 1):
+        if lanes not in (1, 2, 4, 8, 16, 32):
+            raise ValueError(f"lanes must be a power of two <= 32, got {lanes}")
         n = row_host.size - 1
+        per = 64 // lanes
         lengths = np.diff(row_host.astype(np.int64))
-        slices = (n + 63) // 64
-        padded = np.zeros(slices * 64, np.int64)
+        slices = (n + per - 1) // per
+        padded = np.zeros(slices * per, np.int64)
         padded[:n] = lengths
-        width = padded.reshape(slices, 64).max(axis=1)
+        steps = -(-padded.reshape(slices, per).max(axis=1) // lanes)
         off = np.zeros(slices, np.int64)
-        off[1:] = np.cumsum(64 * width[:-1])
-        self.total = int(64 * width.sum())
+        off[1:] = np.cumsum(64 * steps[:-1])
+        self.total = max(int(64 * steps.sum()), 64)
         if self.total >= 2 ** 31:
             raise ValueError("sliced ELL copy too large for 32-bit offsets")
         dev = row_dev.device
-        self.n = n
+        self.n, self.nnz, self.lanes = n, int(lengths.sum()), lanes
         self.slice_off = _dev(off.astype(np.int32), dev)
-        self.slice_width = _dev(width.astype(np.int32), dev)
-        self.col = torch.empty(max(self.total, 1), dtype=torch.int32, device=dev)
+        self.slice_steps = _dev(steps.astype(np.int32), dev)
+        self.col = torch.zeros(self.total, dtype=torch.int32, device=dev)
         self.row, self.src_col = row_dev, col_dev
-        _lib.call("pp_sell_fill", _lib.ptr(row_dev), _lib.ptr(col_dev), None, n, _lib.ptr(self.slice_off),
-                  _lib.ptr(self.slice_width), _lib.PP_F32, _lib.ptr(self.col), None, _lib.PP_F32,
-                  _lib.stream_ptr(dev))
+        _lib.call("pp_sell_fill", _lib.ptr(row_dev), _lib.ptr(col_dev), None, n, self.nnz, 1, lanes,
+                  _lib.ptr(self.slice_off), _lib.ptr(self.slice_steps), self.total, _lib.PP_F32, _lib.ptr(self.col), None,
+                  _lib.PP_F32, _lib.stream_ptr(dev))
         self.values: dict = {}
 
-    def fill(self, val: torch.Tensor, dtype) -> torch.Tensor:
-        out = torch.empty(max(self.total, 1), dtype=dtype, device=val.device)
-        _lib.call("pp_sell_fill", _lib.ptr(self.row), _lib.ptr(self.src_col), _lib.ptr(val), self.n,
-                  _lib.ptr(self.slice_off), _lib.ptr(self.slice_width), _lib.dtype_code(val.dtype), None, _lib.ptr(out),
-                  _lib.dtype_code(dtype), _lib.stream_ptr(val.device))
+    def fill(self, val: torch.Tensor, dtype, nmat: int = 1) -> torch.Tensor:
+        """Values of ``nmat`` operators of this pattern (``val``: nmat x nnz) in the layout."""
+        out = torch.empty(nmat * self.total, dtype=dtype, device=val.device)
+        _lib.call("pp_sell_fill", _lib.ptr(self.row), _lib.ptr(self.src_col), _lib.ptr(val), self.n, self.nnz, nmat,
+                  self.lanes, _lib.ptr(self.slice_off), _lib.ptr(self.slice_steps), self.total,
+                  _lib.dtype_code(val.dtype), None, _lib.ptr(out), _lib.dtype_code(dtype), _lib.stream_ptr(val.device))
         return out
 
     def cached_values(self, val: torch.Tensor, dtype) -> torch.Tensor:
@@ -102,7 +115,47 @@ class SellLayout:
         return self.values[dtype]
 
     def desc(self, val: torch.Tensor) -> tuple:
-        return (self.slice_off, self.slice_width, self.col, val)
+        return (self.slice_off, self.slice_steps, self.col, val, self.lanes, self.total)
+
+
+def _rule_lanes(avg: float) -> int:
+    """Lanes per row without timing: about eight entries per lane."""
+    lanes = 1
+    while lanes < 16 and avg > 8.0 * lanes:
+        lanes *= 2
+    return lanes
+
+
+def tuned_layout(row_host: np.ndarray, row_dev: torch.Tensor, col_dev: torch.Tensor, val_dev: torch.Tensor,
+                 n_cols: int, dtype) -> SellLayout:
+    """The sliced layout of a level matrix with the lanes per row that runs its product fastest.
+
+    The autotuner builds the layout for every candidate in SELL_LANES up to twice the mean row
+    length, times y = M x through it on the device (``pp_sell_probe``, 20 launches after a
+    warm-up) and keeps the fastest; decisions are appended to TUNING_LOG. Without a GPU, with
+    AUTOTUNE off or FORCE_LANES set, one layout is built by rule."""
+    n = row_host.size - 1
+    avg = float(row_host[-1] - row_host[0]) / max(1, n)
+    if FORCE_LANES is not None:
+        return SellLayout(row_host, row_dev, col_dev, FORCE_LANES)
+    if not AUTOTUNE or not val_dev.is_cuda:
+        return SellLayout(row_host, row_dev, col_dev, _rule_lanes(avg))
+    timings, best = {}, None
+    for lanes in SELL_LANES:
+        if lanes > 1 and lanes > 2 * avg:
+            break
+        lay = SellLayout(row_host, row_dev, col_dev, lanes)
+        vals = lay.fill(val_dev, dtype)
+        desc = _lib.SellDesc(_lib.ptr(lay.slice_off), _lib.ptr(lay.slice_steps), _lib.ptr(lay.col), _lib.ptr(vals), lanes,
+                             0, lay.total)
+        us = C.c_double()
+        _lib.call("pp_sell_probe", C.byref(desc), n, int(n_cols), _lib.dtype_code(dtype), 20,
+                  _lib.stream_ptr(val_dev.device), C.byref(us))
+        timings[lanes] = us.value
+        if best is None or us.value < timings[best.lanes]:
+            best = lay
+    TUNING_LOG.append((n, int(row_host[-1]), timings, best.lanes))
+    return best
 
 
 class AMGHierarchy:
@@ -150,6 +203,8 @@ class AMGHierarchy:
                 A = lvl.A
                 if lvl is self.levels[0] and A.ell_col is not None and A.ell_width > 0:
                     aval = None  # the V-cycle reads the finest operator through its ELL copy only
+                elif lvl is not self.levels[0] and "A" in lvl.shared:
+                    aval = None  # ... and the other levels outside the tail through their sliced copies
                 else:
                     aval = A.value if A.value.dtype == dtype else A.value.to(dtype)
                 dinv = torch.empty(A.getRows() * A.getBatchSize(), dtype=dtype, device=A.value.device)
@@ -168,8 +223,9 @@ class AMGHierarchy:
     def native_extras(self, dtype) -> list:
         """Per level above the coarsest: coalesced copies for the V-cycle kernels in ``dtype``
         -- "ell": (columns, values, width) of the finest operator, "a_sell" / "r_sell" /
-        "p_sell": sliced ELL copies (slice offsets, widths, columns, values) of large operators,
-        restrictions and interpolations."""
+        "p_sell": wave-interleaved sliced copies (slice offsets, steps, columns, values, lanes,
+        entries per operator) of the operators, restrictions and interpolations outside the
+        coarse tail."""
         out = []
         for i, lvl in enumerate(self.levels[:-1]):
             ex = lvl.native_extra.get(dtype)
@@ -184,8 +240,9 @@ class AMGHierarchy:
                               _lib.ptr(ev), _lib.dtype_code(dtype), _lib.stream_ptr(dev))
                     ex["ell"] = (A.ell_col, ev, A.ell_width)
                 layout = lvl.shared.get("A")
-                if i > 0 and layout is not None and A.getBatchSize() == 1:
-                    ex["a_sell"] = layout.desc(layout.fill(A.value.reshape(-1)[: A.getNnz()], dtype))
+                if i > 0 and layout is not None:
+                    nmat = A.getBatchSize()
+                    ex["a_sell"] = layout.desc(layout.fill(A.value.reshape(-1)[: nmat * A.getNnz()], dtype, nmat))
                 layout = lvl.shared.get("R")
                 if layout is not None:
                     ex["r_sell"] = layout.desc(layout.cached_values(lvl.R_val, dtype))
@@ -343,12 +400,14 @@ def build_amg_hierarchy(csr_mat: CSRmatrix, *, method: str = "ruge_stuben", max_
             fine_row, fine_col = (row, col) if i == 0 else lv["A"][:2]
             seg, src, w = galerkin_map_host(fine_row, fine_col, lv["P"], lv["R"], host[i + 1]["A"][0], host[i + 1]["A"][1])
             level.gmap = (_dev(seg, device), _dev(src, device), _dev(w, device, torch.float64))
-            if rr.size - 1 >= SELL_MIN_ROWS:
-                level.shared["R"] = SellLayout(rr, level.R_row, level.R_col)
-            if pr.size - 1 >= SELL_MIN_ROWS:
-                level.shared["P"] = SellLayout(pr, level.P_row, level.P_col)
-        if i > 0 and lv["n"] >= SELL_MIN_ROWS and "P" in lv:
-            level.shared["A"] = SellLayout(lv["A"][0], A.row, A.index)
+            if lv["n"] > TAIL_ROWS:
+                # transfers out of and into a level outside the coarse tail run through
+                # sliced copies with autotuned lanes per row
+                nc = host[i + 1]["n"]
+                level.shared["R"] = tuned_layout(rr, level.R_row, level.R_col, level.R_val, lv["n"], native_dtype)
+                level.shared["P"] = tuned_layout(pr, level.P_row, level.P_col, level.P_val, nc, native_dtype)
+        if i > 0 and lv["n"] > TAIL_ROWS and "P" in lv:
+            level.shared["A"] = tuned_layout(lv["A"][0], A.row, A.index, A.value, lv["n"], native_dtype)
         levels.append(level)
     return AMGHierarchy(levels, _dense(levels[-1].A), project_constant, num_pre_smooth, num_post_smooth, native_dtype,
                         time.perf_counter() - t0)

@@ -144,6 +144,38 @@ def outflow2d(nx: int = 24, ny: int = 12, stretch: float = 1.5, seed: int = 5, U
     return [BlockSpec(v, faces, velocity=vel, pressure=np.zeros(cz))]
 
 
+def hartmann_c1(nx: int = 64, ny: int = 64, stretch: float = 1.5, length: float = 4.0, seed: int = 0,
+                U: float = 1.0) -> list:
+    """The 2D Hartmann MHD channel of the CPU-reference workload: x uniform on [0, length], y in
+    [-1, 1] tanh-stretched toward both walls; uniform inflow U at -x and an advected
+    (zero-gradient) outflow at +x, both with an open potential; insulating no-slip walls in y;
+    initial velocity 0 plus seeded N(0, 1e-3) noise."""
+    rng = np.random.default_rng(seed)
+    v = vertices([np.linspace(0.0, length, nx + 1), tanh_coords(ny, -1.0, 1.0, stretch)])
+    cz = cell_shape(v)
+    out = np.zeros([2] + face_shape(cz, 1))
+    out[0] = U
+    faces = [FaceSpec("fixed", velocity=np.array([U, 0.0]), pot=POT_OPEN), FaceSpec("fixed", velocity=out, pot=POT_OPEN),
+             FaceSpec("fixed"), FaceSpec("fixed")]
+    return [BlockSpec(v, faces, velocity=_noise(rng, [2] + cz, 1e-3), pressure=np.zeros(cz))]
+
+
+def inflow_duct3d(nx: int = 48, ny: int = 32, nz: int = 24, stretch: float = 1.5, seed: int = 9, U: float = 1.0) -> list:
+    """A 3D duct with a uniform inflow U at -x and an advected outflow at +x (open potential at
+    both), insulating no-slip walls in y and z (wall-refined); initial plug U plus N(0, 1e-3)."""
+    rng = np.random.default_rng(seed)
+    v = vertices([np.linspace(0.0, 4.0, nx + 1), tanh_coords(ny, -1.0, 1.0, stretch),
+                  tanh_coords(nz, -1.0, 1.0, stretch)])
+    cz = cell_shape(v)
+    out = np.zeros([3] + face_shape(cz, 1))
+    out[0] = U
+    faces = [FaceSpec("fixed", velocity=np.array([U, 0.0, 0.0]), pot=POT_OPEN),
+             FaceSpec("fixed", velocity=out, pot=POT_OPEN)] + [FaceSpec("fixed") for _ in range(4)]
+    vel = _noise(rng, [3] + cz, 1e-3)
+    vel[0] += U
+    return [BlockSpec(v, faces, velocity=vel, pressure=np.zeros(cz))]
+
+
 def split_channel2d(nx: int = 16, ny: int = 12, split: int = 6, seed: int = 0, stretch: float = 1.5,
                     gradient: float = 1.0) -> tuple:
     """The channel of ``channel2d`` as one block and as two blocks joined along x (the second

@@ -3,8 +3,7 @@
 steps with respect to the initial velocity, the inflow profile, the velocity source, the field
 direction e_b and the Stuart number agree to 1e-5 relative (the reference checks its adjoint
 the same way, test_potential_bc.py:605-716, runscripts/validation/gradcheck.py:23-26); the
-solve adjoint satisfies lam = A^{-T} g; each adjoint kernel passes a dot-product test against
-its forward operator."""
+solve adjoint satisfies lam = A^{-T} g."""
 from __future__ import annotations
 
 import numpy as np
@@ -277,12 +276,14 @@ def _tjunction_run(n: int, steps: int, every: int, d_in, d_b0: float, eps: float
     return float(loss)
 
 
-def test_checkpointed_tjunction_gradients():
+@pytest.mark.parametrize("n", [4, 24])
+def test_checkpointed_tjunction_gradients(n):
     """T-junction (3 connected blocks, advected outlets): gradients of the outlet MSE after 4
     MHD steps w.r.t. the inflow field and B0 from the checkpointed backward (segments of 2)
-    equal the single-segment backward and central differences."""
-    print("[adjoint] checkpointed T-junction", flush=True)
-    n, steps = 4, 4
+    equal the single-segment backward and central differences. At n = 24 the domain has
+    55296 cells, so every forward and transposed solve runs the multi-workgroup engine."""
+    print(f"[adjoint] checkpointed T-junction n={n}", flush=True)
+    steps = 4
     rng = np.random.default_rng(12)
     fshape = cases.face_shape([n, n, 2 * n], 0)
     d_in = torch.zeros([1, 3] + fshape, dtype=F64, device=DEV)

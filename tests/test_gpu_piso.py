@@ -37,15 +37,18 @@ def _tdt(dtype):
 
 
 def run_pair(specs, dtype, steps, nu, dt, mhd=None, amg=False, tol=None, pot_amg=False, outflow=None, velm=None,
-             cfl=None, scalar_visc=None, non_orthogonal=False):
+             cfl=None, scalar_visc=None, non_orthogonal=False, abs_tol=None):
     """Run ``steps`` steps on the GPU and in the oracle; return both final states. ``outflow``:
     (block, face) pairs advected with the static velocity ``velm`` before every step; ``cfl``:
-    adaptive CFL substeps (the oracle's substep sizes are kept in ``osim.substep_sizes``)."""
+    adaptive CFL substeps (the oracle's substep sizes are kept in ``osim.substep_sizes``);
+    ``abs_tol``: one absolute tolerance for every solve instead of the relative one."""
     import phipict_amd as pa
 
     rtol = tol if tol is not None else RTOL[dtype]
     tol = pa.SolverTolerance(rtol=rtol)
     otol = ("rtol", rtol)
+    if abs_tol is not None:
+        tol = otol = float(abs_tol)
     epot = mhd is not None
     dom = build_domain(specs, nu, dt, _tdt(dtype), scalar_visc=scalar_visc)
     base = [None if b.velocitySource is None else b.velocitySource.clone() for b in dom.getBlocks()]

@@ -359,9 +359,11 @@ def test_amg_pcg_matches_oracle(combo, project):
         assert rel_l2(a, o) < (1e-9 if exact_h else (1e-4 if dt_s == np.float64 else 1e-3)), rel_l2(a, o)
 
 
-def test_sliced_ell_layout_round_trip():
-    """pp_sell_fill: every CSR entry lands at slice_off[s] + 64 k + j of row 64 s + j, padding
-    holds column 0 and value 0, and the slice widths are the slices' longest rows."""
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16])
+def test_sliced_ell_layout_round_trip(lanes):
+    """pp_sell_fill: entry k * lanes + j of row r of slice s lands at slice_off[s] + 64 k +
+    r * lanes + j, padding holds column 0 and value 0, and the slice steps are the slices'
+    longest rows divided by lanes, rounded up; two operators of one pattern go to stride total."""
     from phipict_amd.solvers import amg
 
     row, col, val = poisson(23, 17, shift=1e-3)
@@ -371,35 +373,45 @@ def test_sliced_ell_layout_round_trip():
     col, val, rows = col[keep], val[keep], rows[keep]
     row = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=row.size - 1))]).astype(np.int32)
     n = row.size - 1
-    lay = amg.SellLayout(row, torch.as_tensor(row, device=DEV), torch.as_tensor(col.astype(np.int32), device=DEV))
-    sv = to_np(lay.fill(torch.as_tensor(val, device=DEV), torch.float64))
-    sc, off, wid = to_np(lay.col), to_np(lay.slice_off), to_np(lay.slice_width)
+    lay = amg.SellLayout(row, torch.as_tensor(row, device=DEV), torch.as_tensor(col.astype(np.int32), device=DEV), lanes)
+    two = np.concatenate([val, 2.0 * val])
+    sv = to_np(lay.fill(torch.as_tensor(two, device=DEV), torch.float64, nmat=2))
+    sc, off, steps = to_np(lay.col), to_np(lay.slice_off), to_np(lay.slice_steps)
     lengths = np.diff(row)
-    for s in range((n + 63) // 64):
-        rs = range(64 * s, min(n, 64 * s + 64))
-        assert wid[s] == max(lengths[i] for i in rs)
+    per = 64 // lanes
+    for s in range((n + per - 1) // per):
+        rs = range(per * s, min(n, per * s + per))
+        assert steps[s] == -(-max(lengths[i] for i in rs) // lanes)
         for i in rs:
-            j = i - 64 * s
-            pos = off[s] + j + 64 * np.arange(wid[s])
-            np.testing.assert_array_equal(sc[pos[: lengths[i]]], col[row[i]:row[i + 1]])
-            np.testing.assert_array_equal(sv[pos[: lengths[i]]], val[row[i]:row[i + 1]])
-            assert (sc[pos[lengths[i]:]] == 0).all() and (sv[pos[lengths[i]:]] == 0).all()
+            r = i - per * s
+            e = np.arange(steps[s] * lanes)
+            pos = off[s] + 64 * (e // lanes) + r * lanes + e % lanes
+            L = lengths[i]
+            np.testing.assert_array_equal(sc[pos[:L]], col[row[i]:row[i + 1]])
+            np.testing.assert_array_equal(sv[pos[:L]], val[row[i]:row[i + 1]])
+            np.testing.assert_array_equal(sv[lay.total + pos[:L]], 2.0 * val[row[i]:row[i + 1]])
+            assert (sc[pos[L:]] == 0).all() and (sv[pos[L:]] == 0).all()
 
 
+@pytest.mark.parametrize("lanes", [1, 2, 4, 16])
 @pytest.mark.parametrize("combo", ["f64/f32", "f64/f64"])
 @pytest.mark.parametrize("project", [False, True])
-def test_amg_pcg_sliced_ell_matches_oracle(monkeypatch, combo, project):
-    """The V-cycle on sliced ELL copies of the coarse operators and restrictions (used for
-    large levels; here forced on every level) gives the oracle's solves."""
-    print(f"[amg sell] combo={combo} project={project}", flush=True)
+def test_amg_pcg_sliced_ell_matches_oracle(monkeypatch, combo, project, lanes):
+    """The V-cycle on wave-interleaved sliced copies of the operators, restrictions and
+    interpolations outside the coarse tail, with every lanes-per-row choice, gives the
+    oracle's solves."""
+    print(f"[amg sell] combo={combo} project={project} lanes={lanes}", flush=True)
     from phipict_amd.solvers import amg
 
-    monkeypatch.setattr(amg, "SELL_MIN_ROWS", 1)
+    monkeypatch.setattr(amg, "FORCE_LANES", lanes)
     dt_s, dt_h = {"f64/f32": (np.float64, np.float32), "f64/f64": (np.float64, np.float64)}[combo]
     row, col, val = poisson(96, 80, shift=0.0 if project else 1e-3)
     n = row.size - 1
     A, h, hs = _hier_pair(row, col, val, dt_s, project, dt_h)
-    assert all("R" in lv.shared for lv in h.levels[:-1]) and all("A" in lv.shared for lv in h.levels[1:-1])
+    outside = [lv for lv in h.levels[:-1] if lv.n > amg.TAIL_ROWS]
+    assert len(outside) >= 3
+    assert all("R" in lv.shared and "P" in lv.shared for lv in outside)
+    assert all("A" in lv.shared and lv.shared["A"].lanes == lanes for lv in outside[1:])
     b = np.random.default_rng(12).standard_normal(n)
     if project:
         b -= b.mean()
