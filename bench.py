@@ -13,6 +13,15 @@ with every field resident in HBM.
 steps in the reference's default non-orthogonal mode (corner terms in the advection and pressure
 matrices and on their right-hand sides); the CPU baseline is skipped then.
 ``--workload tjunction-adjoint`` times a checkpointed adjoint rollout instead.
+``--workload uneven-batch`` times 64 independent 2D Hartmann channels of uneven sizes
+(Nx, Ny log-uniform in [32, 256], Ha ~ U[5, 50], Re ~ U[50, 500], stretching ~ U[1, 2]; inflow
+U = 1 with an advected outflow, insulating walls, dt = 1e-3, CG pressure and potential and
+BiCGStab momentum to the absolute tolerance 1e-10) stepped as one domain of independent blocks
+with segmented solves, against looping the plain path over the 64 instances one at a time.
+``--workload serving`` serves Poisson arrivals of 2D and 3D MHD duct requests (cells
+log-uniform in [16k, 2M], Ha ~ U[5, 200], Re ~ U[50, 1000], 50 PISO steps each, AMG-CG to 1e-8 in
+FP32) admitted against an HBM budget into concurrent slots, and reports throughput and p50/p99
+latency, against serving the same requests one at a time.
 
 Printed: one JSON line with steps/s of the uninstrumented timed steps, the roofline of the
 kernel with the largest share of the kernel time (algorithmic bytes per launch / average launch
@@ -50,7 +59,7 @@ def parse_args(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["duct", "tjunction-adjoint"], default="duct",
+    ap.add_argument("--workload", choices=["duct", "tjunction-adjoint", "uneven-batch", "serving"], default="duct",
                     help="duct: the PISO + MHD step throughput (default); tjunction-adjoint: the checkpointed "
                          "backward through a 3-block T-junction rollout and its finite-difference check")
     ap.add_argument("--dtype", choices=["f64", "f32"], default="f64")
@@ -274,6 +283,234 @@ def tjunction_adjoint(args) -> int:
                       "inflow": {"adjoint": an_in, "fd": fd_in,
                                  "rel_err": abs(an_in - fd_in) / max(abs(fd_in), 1e-300), "eps": eps_in},
                       "eps": eps},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+# ----------------------------------------------------------------------------- uneven batch
+UB_COUNT, UB_DT, UB_TOL = 64, 1e-3, 1e-10
+
+
+def uneven_instances(count=UB_COUNT):
+    """(nx, ny, stretch, Re, Ha) of the uneven-batch workload: sizes log-uniform in [32, 256]
+    (seed 2), Ha ~ U[5, 50], Re ~ U[50, 500], stretching ~ U[1, 2] (seed 3)."""
+    import numpy as np
+
+    rs = np.random.default_rng(2)
+    sizes = np.rint(np.exp(rs.uniform(np.log(32), np.log(256), (count, 2)))).astype(int)
+    rp = np.random.default_rng(3)
+    ha, re, st = rp.uniform(5, 50, count), rp.uniform(50, 500, count), rp.uniform(1, 2, count)
+    return [(int(sizes[i, 0]), int(sizes[i, 1]), float(st[i]), float(re[i]), float(ha[i])) for i in range(count)]
+
+
+def make_uneven(instances, dtype, device, independent):
+    """One domain holding the instances as independent blocks (independent=True), or one domain
+    per instance (a list of (domain, simulation) pairs)."""
+    import numpy as np
+    import torch
+
+    import phipict_amd as pa
+    from phipict_amd import grid
+
+    def add_block(dom, i, nx, ny, stretch, re):
+        xs = np.linspace(0.0, 4.0, nx + 1)
+        ys = np.array(grid.axis_coords(-1.0, 1.0, grid.make_weights_tanh(ny, stretch, "BOTH")))
+        Y, X = np.meshgrid(ys, xs, indexing="ij")
+        v = torch.as_tensor(np.stack([X, Y]), dtype=dtype).unsqueeze(0)
+        gen = torch.Generator("cpu").manual_seed(100 + i)
+        vel = (1e-3 * torch.randn(1, 2, ny, nx, generator=gen, dtype=torch.float64)).to(dtype)
+        blk = dom.CreateBlock(velocity=vel, vertexCoordinates=v)
+        blk.setViscosity(torch.tensor([1.0 / re], dtype=dtype, device=device))
+        inlet = blk.CloseBoundary(0, velocity=torch.tensor([[1.0, 0.0]], dtype=dtype, device=device))
+        inlet.setPotentialBC(pa.PotentialBC.OPEN)
+        out = blk.CloseBoundary(1, velocity=torch.ones(1, 2, ny, 1, dtype=dtype, device=device)
+                                * torch.tensor([1.0, 0.0], dtype=dtype, device=device).view(1, 2, 1, 1))
+        out.setPotentialBC(pa.PotentialBC.OPEN)
+        blk.CloseBoundary(2)
+        blk.CloseBoundary(3)
+        return out
+
+    def finish(dom, outs, stuart):
+        dom.PrepareSolve()
+        pa.simulation.balance_boundary_fluxes(dom, outs)
+        dom.UpdateDomainData()
+        velm = torch.tensor([[1.0, 0.0]], dtype=dtype, device=device)
+
+        def advect(domain, **kw):
+            pa.simulation.update_advective_boundaries(domain, outs, velm, UB_DT)
+
+        def no_source(domain, **kw):
+            for b in domain.getBlocks():
+                b.clearVelocitySource()
+            domain.UpdateDomainData()
+
+        return pa.MHDSimulation(domain=dom, dt=UB_DT, stuart_number=torch.tensor(stuart, dtype=dtype),
+                                e_b=torch.tensor([0.0, 1.0, 0.0], dtype=dtype), non_orthogonal=False,
+                                advection_tol=UB_TOL, pressure_tol=UB_TOL, potential_tol=UB_TOL,
+                                hooks={"PRE": advect, "PRE_VELOCITY_SETUP": no_source})
+
+    if independent:
+        dom = pa.Domain(2, torch.tensor([0.01], dtype=dtype), dtype=dtype, device=device)
+        outs = [add_block(dom, i, nx, ny, st, re) for i, (nx, ny, st, re, _ha) in enumerate(instances)]
+        dom.setIndependentBlocks()
+        sim = finish(dom, outs, [ha * ha / re for (_nx, _ny, _st, re, ha) in instances])
+        return dom, sim
+    pairs = []
+    for i, (nx, ny, st, re, ha) in enumerate(instances):
+        dom = pa.Domain(2, torch.tensor([0.01], dtype=dtype), dtype=dtype, device=device)
+        out = add_block(dom, i, nx, ny, st, re)
+        pairs.append((dom, finish(dom, [out], [ha * ha / re])))
+    return pairs
+
+
+def uneven_batch(args) -> int:
+    import torch
+
+    device = torch.device("cuda", 0)
+    dtype = torch.float64 if args.dtype == "f64" else torch.float32
+    inst = uneven_instances()
+    cells = sum(nx * ny for nx, ny, *_ in inst)
+    print(f"[bench] uneven batch: {len(inst)} instances, {cells} cells", file=sys.stderr, flush=True)
+    dom, sim = make_uneven(inst, dtype, device, True)
+    for _ in range(args.warmup):
+        if not sim.single_step():
+            raise RuntimeError("batched step failed")
+    torch.cuda.synchronize()
+    gc.collect()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        if not sim.single_step():
+            raise RuntimeError("batched step failed")
+    torch.cuda.synchronize()
+    batched = time.perf_counter() - t0
+    print(f"[bench] batched: {args.steps} steps in {batched:.3f} s", file=sys.stderr, flush=True)
+    del dom, sim
+    pairs = make_uneven(inst, dtype, device, False)
+    for _ in range(max(1, args.warmup // 2)):
+        for _d, s_ in pairs:
+            s_.single_step()
+    torch.cuda.synchronize()
+    loop_steps = max(1, args.steps // 4)
+    t1 = time.perf_counter()
+    for _ in range(loop_steps):
+        for _d, s_ in pairs:
+            if not s_.single_step():
+                raise RuntimeError("solo step failed")
+    torch.cuda.synchronize()
+    looped = time.perf_counter() - t1
+    print(f"[bench] looped: {loop_steps} sweeps over {len(pairs)} instances in {looped:.3f} s", file=sys.stderr,
+          flush=True)
+    env_batched = len(inst) * args.steps / batched
+    env_looped = len(inst) * loop_steps / looped
+    line = {
+        "metric": "uneven-batch env steps/s (batched, segmented solves)",
+        "value": round(env_batched, 2),
+        "unit": "env steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * batched / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic",
+        "config": {"workload": f"{len(inst)} independent 2D Hartmann channels, Nx,Ny log-uniform [32,256], "
+                               "Ha~U[5,50], Re~U[50,500], stretch~U[1,2], inflow + advected outflow, dt=1e-3, "
+                               "CG pressure/potential + BiCGStab momentum, abs tol 1e-10",
+                   "cells": cells},
+        "looped_env_steps_per_s": round(env_looped, 2),
+        "batched_over_looped": round(env_batched / env_looped, 2),
+        "cell_updates_per_s": round(cells * args.steps / batched, 1),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+# ----------------------------------------------------------------------------- serving
+SV_COUNT, SV_RATE, SV_SLOTS, SV_BUDGET_GIB = 16, 2.0, 4, 96
+
+
+def serving_requests(count=SV_COUNT, rate=SV_RATE, steps=50):
+    """Mixed 2D / 3D duct requests: cells log-uniform in [16k, 2M] (seed 6), 3D ducts with
+    nx = 2 ny = 2 nz, 2D channels with nx = 2 ny; Ha ~ U[5, 200], Re ~ U[50, 1000] (seed 7);
+    Poisson arrivals of the given rate (seed 8)."""
+    import numpy as np
+    import torch
+
+    from phipict_amd.serving import DuctRequest, poisson_arrivals
+
+    rg = np.random.default_rng(6)
+    cells = np.exp(rg.uniform(np.log(16e3), np.log(2e6), count))
+    three = rg.random(count) < 0.5
+    rp = np.random.default_rng(7)
+    ha, re = rp.uniform(5, 200, count), rp.uniform(50, 1000, count)
+    arrivals = poisson_arrivals(rate, count, seed=8)
+    reqs = []
+    for i in range(count):
+        if three[i]:
+            n = max(8, int(round((cells[i] / 2) ** (1 / 3))))
+            shape = (2 * n, n, n)
+        else:
+            n = max(16, int(round((cells[i] / 2) ** 0.5)))
+            shape = (2 * n, n)
+        reqs.append(DuctRequest(rid=i, cells=shape, re=float(re[i]), ha=float(ha[i]), steps=steps, dt=5e-4, tol=1e-8,
+                                dtype=torch.float32, seed=30 + i, arrival=float(arrivals[i])))
+    return reqs
+
+
+def serving_bench(args) -> int:
+    from phipict_amd import serving
+
+    reqs = serving_requests(steps=args.steps if args.steps != 10 else 50)
+    cells = sum(r.ncells for r in reqs)
+    budget = SV_BUDGET_GIB << 30
+    print(f"[bench] serving {len(reqs)} requests ({cells} cells in total), {SV_SLOTS} slots, budget {SV_BUDGET_GIB} GiB",
+          file=sys.stderr, flush=True)
+    # warm-up: one small request of each dimension (library loading, first compilations)
+    serving.ServingLoop(budget, SV_SLOTS).run([serving.DuctRequest(**{**reqs[0].__dict__, "cells": c, "steps": 2,
+                                                                      "arrival": 0.0, "rid": 900 + k})
+                                               for k, c in enumerate([(64, 32), (16, 8, 8)])])
+    loop = serving.ServingLoop(budget, SV_SLOTS)
+    t0 = time.perf_counter()
+    recs = loop.run(reqs)
+    served = time.perf_counter() - t0
+    errors = [r.error for r in recs if r.error]
+    print(f"[bench] served in {served:.2f} s, errors: {errors}", file=sys.stderr, flush=True)
+    # the same requests one at a time, all available at once: the plain path looped
+    one = serving.ServingLoop(budget, 1)
+    t1 = time.perf_counter()
+    solo = one.run([serving.DuctRequest(**{**r.__dict__, "arrival": 0.0}) for r in reqs])
+    looped = time.perf_counter() - t1
+    print(f"[bench] one at a time in {looped:.2f} s", file=sys.stderr, flush=True)
+    lat = serving.latency_percentiles(recs)
+    steps = sum(r.steps for r in reqs)
+    line = {
+        "metric": "served MHD solve requests: throughput and p50/p99 latency",
+        "value": round(len(reqs) / served, 4),
+        "unit": "requests/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 2,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{len(reqs)} Poisson arrivals (rate {SV_RATE}/s) of 2D/3D MHD duct requests, cells "
+                               "log-uniform [16k, 2M], Ha~U[5,200], Re~U[50,1000], 50 PISO steps each, AMG-CG tol 1e-8 "
+                               f"FP32 (potential solved in FP64), {SV_SLOTS} slots, HBM budget {SV_BUDGET_GIB} GiB",
+                   "cells": cells},
+        "latency_s": {k: round(v, 3) for k, v in lat.items()},
+        "cell_updates_per_s": round(sum(r.ncells * r.steps for r in reqs) / served, 1),
+        "makespan_s": round(served, 3),
+        "one_at_a_time_s": round(looped, 3),
+        "served_over_one_at_a_time": round(looped / served, 3),
+        "peak_running": loop.peak_running,
+        "peak_budget_in_use_gib": round(loop.peak_in_use / 2 ** 30, 2),
+        "errors": errors + [r.error for r in solo if r.error],
     }
     print(json.dumps(line), flush=True)
     return 0
@@ -543,6 +780,10 @@ def main(argv=None) -> int:
         return 2
     if args.workload == "tjunction-adjoint":
         return tjunction_adjoint(args)
+    if args.workload == "uneven-batch":
+        return uneven_batch(args)
+    if args.workload == "serving":
+        return serving_bench(args)
     import numpy as np
     import torch
 

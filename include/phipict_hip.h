@@ -207,7 +207,8 @@ enum pp_kernel_id {
     PP_K_AMG_JACOBI = 34, PP_K_AMG_LEVEL_RESIDUAL = 35, PP_K_AMG_RESTRICT = 36, PP_K_AMG_PROLONG = 37,
     PP_K_AMG_COARSE = 38, PP_K_MEAN = 39, PP_K_SUBTRACT_MEAN = 40, PP_K_AMG_POST_JACOBI = 41,
     PP_K_AMG_TAIL = 42,
-    PP_K_SPMV = 50, PP_K_GALERKIN = 51
+    PP_K_SPMV = 50, PP_K_GALERKIN = 51,
+    PP_K_SEG_CG_DIRECTION = 60, PP_K_SEG_CG_UPDATE = 61, PP_K_SEG_BICG = 62
 };
 int pp_kernel_timer_start(int32_t sample_every);
 int pp_kernel_timer_stop(void);
@@ -367,8 +368,31 @@ int pp_amg_pcg_solve(const pp_csr *A, const pp_amg_level *levels, int32_t num_le
                      int32_t pre_sweeps, int32_t post_sweeps, double omega,
                      const double *tols, int32_t ntols, void *stream, pp_solve_result *results);
 
+/* Segmented solve (no reference counterpart; the uneven-size batching form): nseg independent
+ * systems of different sizes, the diagonal blocks of one CSR matrix A (instance s owns rows
+ * seg_rows[s] .. seg_rows[s+1]; its rows reference only its own columns), with ncomp
+ * right-hand sides each (rhs, x: [ncomp][n]; the velocity components of every instance share
+ * its block). CG or BiCGStab with the reference's recurrences and stopping rules per instance
+ * and component (NORM2_NORMALIZED over the instance's rows), one launch per phase for all;
+ * project: each instance's right-hand side made mean-free first (rank-deficient CG).
+ * tols: 1 or ncomp * nseg values (component-major); tile_rows: rows per workgroup tile, a
+ * multiple of 256 (0: 1024). results: ncomp * nseg entries, component-major. */
+int pp_solve_segmented(const pp_csr *A, const int32_t *seg_rows, int32_t nseg, int32_t ncomp, const void *rhs, void *x,
+                       int32_t dtype, int32_t maxit, const double *tols, int32_t ntols, int32_t use_bicg,
+                       int32_t project, int32_t tile_rows, void *stream, pp_solve_result *results);
+
 /* replaces ReleaseKrylovWorkspaces (bindings.cpp:777, krylov.cu:921-924). */
 int pp_release_workspaces(void);
+
+/* replace SetKrylovSettings / GetKrylovSettings (bindings.cpp:739-763, krylov.h:68-74):
+ * check_interval = iterations per launch chunk at most (the host's pacing of the multi-
+ * workgroup engines); use_graphs: replay captured chunks of iterations from HIP graphs (the
+ * segmented engine; the reference replays its iteration chunks from CUDA graphs,
+ * krylov.cu:984-1042); cg_jacobi / bicg_jacobi select the Jacobi preconditioner of the fused
+ * CG (z = diag(A)^-1 r, krylov.cu:414-425) and of the fused BiCGStab (right preconditioning,
+ * krylov.cu:243-247) that pp_solve_linear runs. */
+int pp_set_krylov_settings(int32_t check_interval, int32_t use_graphs, int32_t cg_jacobi, int32_t bicg_jacobi);
+int pp_get_krylov_settings(int32_t *check_interval, int32_t *use_graphs, int32_t *cg_jacobi, int32_t *bicg_jacobi);
 
 /* replaces SparseOuterProduct (linear_solve.cu:166-190, SDDMM legacy_cg.cu:888-927):
  * out.val[k] = a[row(k)] * b[col(k)] on the pattern of out. */

@@ -8,10 +8,27 @@
  * Every function cites the reference file:line it restates (paths relative to the
  * phiPICT source tree). Descriptors here are the oracle's own (host pointers, one
  * environment), independent of the product's include/phipict_hip.h.
+ *
+ * The same sources also build a multi-threaded copy for timing the CPU baseline
+ * (_build/liboracle_omp.so, compiled with -fopenmp): the cell and row loops below are
+ * marked O_PAR / O_PAR_SUM, which share them among threads there and expand to nothing in
+ * the checker build. Threaded sums add in a thread-dependent order, so only the
+ * single-threaded build checks results.
  */
 #ifndef PHIPICT_ORACLE_H
 #define PHIPICT_ORACLE_H
 #include <stdint.h>
+
+#ifdef _OPENMP
+#define O_PRAGMA(x) _Pragma(#x)
+#define O_PAR O_PRAGMA(omp parallel for schedule(static))
+#define O_PAR_SUM(...) O_PRAGMA(omp parallel for schedule(static) reduction(+ : __VA_ARGS__))
+#define O_PAR_OR(v) O_PRAGMA(omp parallel for schedule(static) reduction(| : v))
+#else
+#define O_PAR
+#define O_PAR_SUM(...)
+#define O_PAR_OR(v)
+#endif
 
 enum { OB_CONNECTED = 20, OB_PERIODIC = 21, OB_FIXED = 30 };
 enum { OB_POT_INSULATING = 0, OB_POT_OPEN = 1, OB_POT_DIRICHLET = 2, OB_POT_THIN_WALL = 3 };

@@ -18,6 +18,9 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "_build" / "liboracle.so"
+#: the OpenMP build of the same sources: timing of the CPU baseline only (its sums add in a
+#: thread-dependent order, so the single-threaded library is the checker)
+LIB_PATH_OMP = HERE / "_build" / "liboracle_omp.so"
 
 OB_CONNECTED, OB_PERIODIC, OB_FIXED = 20, 21, 30
 POT_INSULATING, POT_OPEN, POT_DIRICHLET, POT_THIN_WALL = 0, 1, 2, 3
@@ -63,15 +66,24 @@ class OResult(C.Structure):
                 ("is_finite", C.c_int32), ("pad", C.c_int32)]
 
 
-_LIB = None
+_LIBS: dict = {}
+_THREADED = False
+
+
+def use_threads(on: bool = True) -> None:
+    """Route later oracle calls to the OpenMP build (True) or back to the single-threaded
+    checker (False). The thread count is OpenMP's (OMP_NUM_THREADS)."""
+    global _THREADED
+    _THREADED = bool(on)
 
 
 def lib() -> C.CDLL:
-    global _LIB
-    if _LIB is None:
-        if not LIB_PATH.exists():
+    path = LIB_PATH_OMP if _THREADED else LIB_PATH
+    L = _LIBS.get(path)
+    if L is None:
+        if not path.exists():
             build()
-        L = C.CDLL(str(LIB_PATH))
+        L = C.CDLL(str(path))
         L.o_amg_setup.restype = C.c_void_p
         L.o_amg_setup.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int]
         L.o_amg_num_levels.argtypes = [C.c_void_p]
@@ -79,8 +91,8 @@ def lib() -> C.CDLL:
         L.o_amg_level_copy.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 10
         L.o_amg_free.argtypes = [C.c_void_p]
         L.o_galerkin.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int] + [C.c_void_p] * 9 + [C.c_int]
-        _LIB = L
-    return _LIB
+        _LIBS[path] = L
+    return L
 
 
 def _p(a: Optional[np.ndarray]) -> Optional[C.c_void_p]:
@@ -581,19 +593,22 @@ def spmv(row, col, val, x):
     return y
 
 
-def cg(row, col, val, b, x, tol, maxit=5000, normalized=True, reset_every=0, return_best=False, project=False):
-    """Fused CG of the reference (returns the OResult; x updated in place)."""
+def cg(row, col, val, b, x, tol, maxit=5000, normalized=True, reset_every=0, return_best=False, project=False,
+       jacobi=False):
+    """Fused CG of the reference, optionally Jacobi-preconditioned (returns the OResult; x
+    updated in place)."""
     r = OResult()
     getattr(lib(), "o_cg" + sfx(val.dtype))(row.size - 1, _p(row), _p(col), _p(val), _p(b), _p(x), int(maxit),
                                             C.c_double(tol), int(normalized), int(reset_every), int(return_best),
-                                            int(project), C.byref(r))
+                                            int(project), int(jacobi), C.byref(r))
     return r
 
 
-def bicgstab(row, col, val, b, x, tol, maxit=5000, normalized=True):
+def bicgstab(row, col, val, b, x, tol, maxit=5000, normalized=True, jacobi=False):
+    """Fused BiCGStab of the reference, optionally right-preconditioned with Jacobi."""
     r = OResult()
     getattr(lib(), "o_bicgstab" + sfx(val.dtype))(row.size - 1, _p(row), _p(col), _p(val), _p(b), _p(x), int(maxit),
-                                                  C.c_double(tol), int(normalized), C.byref(r))
+                                                  C.c_double(tol), int(normalized), int(jacobi), C.byref(r))
     return r
 
 
@@ -692,10 +707,11 @@ def amg_pcg(A_fine, levels_h, pinv_h, b, x, tol, maxit=5000, return_best=False, 
 
 
 def inverse_diagonal(row, col, val):
+    """1 / (sum of the row's diagonal entries), 1 where that sum vanishes (the V-cycle's Jacobi
+    weights, krylov.cu:272-283); the entries are added in row order."""
     n = row.size - 1
+    rows = np.repeat(np.arange(n), np.diff(row))
+    on = col == rows
     d = np.zeros(n, val.dtype)
-    for i in range(n):
-        for k in range(row[i], row[i + 1]):
-            if col[k] == i:
-                d[i] += val[k]
+    np.add.at(d, rows[on], val[on])
     return np.where(np.abs(d) > 1e-30, 1.0 / np.where(d == 0, 1, d), 1.0).astype(val.dtype)

@@ -556,6 +556,7 @@ int FN(o_nonortho_rhs)(const ob_domain *dom, int kind, int flags, int w, const R
     for (int bi = 0; bi < dom->nblocks; ++bi) {
         const ob_block *b = &dom->blocks[bi];
         int n = FN(ncells)(b);
+        O_PAR
         for (int f = 0; f < n; ++f) {
             int p[3];
             FN(unflat)(f, b->size, p);
@@ -596,9 +597,11 @@ int FN(o_advection_matrix)(const ob_domain *dom, int flags, int scalar_mode, con
     FN(octx) c = {dom, dom->dims, 2 * dom->dims * dom->dims + 1};
     FN(ndata) nd = {&c, OK_TYPES, 0, NULL, NULL, 0};
     const REAL dt = (REAL)dom->dt;
+    int bad = 0;
     for (int bi = 0; bi < dom->nblocks; ++bi) {
         const ob_block *b = &dom->blocks[bi];
         int n = FN(ncells)(b);
+        O_PAR_OR(bad)
         for (int f = 0; f < n; ++f) {
             int p[3];
             FN(unflat)(f, b->size, p);
@@ -642,11 +645,11 @@ int FN(o_advection_matrix)(const ob_domain *dom, int flags, int scalar_mode, con
             rv[0] = diag;
             int g = b->global_offset + f;
             int w = FN(emit_row)(idx, rv, 7, col + row[g], val + row[g], 1, det);
-            if (w != row[g + 1] - row[g]) return -2;
+            if (w != row[g + 1] - row[g]) bad = 1;
             adiag[g] = rv[0];
         }
     }
-    return 0;
+    return bad ? -2 : 0;
 }
 
 /* kPISO_build_advection_RHS, advection.cu:799-955 (CENTRAL, orthogonal part) */
@@ -715,6 +718,7 @@ int FN(o_advection_rhs)(const ob_domain *dom, int apply_pressure_gradient, const
     for (int bi = 0; bi < dom->nblocks; ++bi) {
         const ob_block *b = &dom->blocks[bi];
         int n = FN(ncells)(b);
+        O_PAR
         for (int f = 0; f < n; ++f) {
             int p[3];
             FN(unflat)(f, b->size, p);
@@ -754,9 +758,11 @@ int FN(o_pressure_matrix)(const ob_domain *dom, const REAL *adiag, int flags, co
     if (!FN(domain_ok)(dom)) return -1;
     FN(octx) c = {dom, dom->dims, 2 * dom->dims * dom->dims + 1};
     FN(ndata) nd = {&c, OK_TYPES, 0, NULL, NULL, 0};
+    int bad = 0;
     for (int bi = 0; bi < dom->nblocks; ++bi) {
         const ob_block *b = &dom->blocks[bi];
         int n = FN(ncells)(b);
+        O_PAR_OR(bad)
         for (int f = 0; f < n; ++f) {
             int p[3];
             FN(unflat)(f, b->size, p);
@@ -786,10 +792,10 @@ int FN(o_pressure_matrix)(const ob_domain *dom, const REAL *adiag, int flags, co
             }
             for (int dd = c.d; dd < 3; ++dd) { idx[dd * 2 + 1] = -1; idx[dd * 2 + 2] = -1; }
             int w = FN(emit_row)(idx, rv, 7, col + row[g], val + row[g], 0, 1);
-            if (w != row[g + 1] - row[g]) return -2;
+            if (w != row[g + 1] - row[g]) bad = 1;
         }
     }
-    return 0;
+    return bad ? -2 : 0;
 }
 
 /* PISO_build_pressure_rhs, pressure.cu:349-522 (CENTRAL, boundary sources on) */
@@ -801,6 +807,7 @@ int FN(o_pressure_rhs)(const ob_domain *dom, const REAL *adiag, const int32_t *c
     for (int bi = 0; bi < dom->nblocks; ++bi) {
         const ob_block *b = &dom->blocks[bi];
         int n = FN(ncells)(b);
+        O_PAR
         for (int f = 0; f < n; ++f) {
             int p[3];
             FN(unflat)(f, b->size, p);
@@ -844,6 +851,7 @@ int FN(o_pressure_div)(const ob_domain *dom, const REAL *prhs, int time_step_nor
     for (int bi = 0; bi < dom->nblocks; ++bi) {
         const ob_block *b = &dom->blocks[bi];
         int n = FN(ncells)(b);
+        O_PAR
         for (int f = 0; f < n; ++f) {
             int p[3];
             FN(unflat)(f, b->size, p);
@@ -865,6 +873,7 @@ int FN(o_correct_velocity)(const ob_domain *dom, const REAL *adiag, const REAL *
     for (int bi = 0; bi < dom->nblocks; ++bi) {
         const ob_block *b = &dom->blocks[bi];
         int n = FN(ncells)(b);
+        O_PAR
         for (int f = 0; f < n; ++f) {
             int p[3];
             FN(unflat)(f, b->size, p);
@@ -888,9 +897,11 @@ int FN(o_correct_velocity)(const ob_domain *dom, const REAL *adiag, const REAL *
 int FN(o_epot_matrix)(const ob_domain *dom, const int32_t *row, int32_t *col, REAL *val) {
     if (!FN(domain_ok)(dom)) return -1;
     FN(octx) c = {dom, dom->dims, 2 * dom->dims * dom->dims + 1};
+    int bad = 0;
     for (int bi = 0; bi < dom->nblocks; ++bi) {
         const ob_block *b = &dom->blocks[bi];
         int n = FN(ncells)(b);
+        O_PAR_OR(bad)
         for (int f = 0; f < n; ++f) {
             int p[3];
             FN(unflat)(f, b->size, p);
@@ -949,10 +960,10 @@ int FN(o_epot_matrix)(const ob_domain *dom, const int32_t *row, int32_t *col, RE
                 }
             }
             int w = FN(emit_row)(idx, rv, 7, col + row[g], val + row[g], 0, 1);
-            if (w != row[g + 1] - row[g]) return -2;
+            if (w != row[g + 1] - row[g]) bad = 1;
         }
     }
-    return 0;
+    return bad ? -2 : 0;
 }
 
 /* k_computeEpotRHS, potential.cu:272-322 (useFaceTransform=false). vf: [d][n] */
@@ -962,6 +973,7 @@ int FN(o_epot_rhs)(const ob_domain *dom, const REAL *vf, REAL *div) {
     for (int bi = 0; bi < dom->nblocks; ++bi) {
         const ob_block *b = &dom->blocks[bi];
         int n = FN(ncells)(b);
+        O_PAR
         for (int f = 0; f < n; ++f) {
             int p[3];
             FN(unflat)(f, b->size, p);
@@ -990,6 +1002,7 @@ int FN(o_current_density)(const ob_domain *dom, const REAL *phi, const REAL *uxb
     for (int bi = 0; bi < dom->nblocks; ++bi) {
         const ob_block *b = &dom->blocks[bi];
         int n = FN(ncells)(b);
+        O_PAR
         for (int f = 0; f < n; ++f) {
             int p[3];
             FN(unflat)(f, b->size, p);
@@ -1191,6 +1204,7 @@ int FN(o_face_transforms)(int d, const int32_t *cells, const REAL *coords, REAL 
 /* ---------------- sparse linear algebra ---------------- */
 
 int FN(o_spmv)(int n, const int32_t *row, const int32_t *col, const REAL *val, const REAL *x, REAL *y) {
+    O_PAR
     for (int i = 0; i < n; ++i) {
         REAL s = 0;
         for (int j = row[i]; j < row[i + 1]; ++j) s += val[j] * x[col[j]];
@@ -1204,15 +1218,31 @@ int FN(o_spmv)(int n, const int32_t *row, const int32_t *col, const REAL *val, c
 enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_NONFINITE = 2, ST_MAXIT = 3, ST_RISING = 4 };
 #endif
 
+/* Inverse diagonal of the Jacobi preconditioner (kInverseDiagonal, krylov.cu:272-283): the sum
+ * of the row's diagonal entries, 1 where it is zero. */
+static void FN(o_jacobi_dinv)(int n, const int32_t *row, const int32_t *col, const REAL *val, REAL *dinv) {
+    for (int i = 0; i < n; ++i) {
+        REAL d = 0;
+        for (int j = row[i]; j < row[i + 1]; ++j)
+            if (col[j] == i) d += val[j];
+        dinv[i] = d != (REAL)0 ? (REAL)1 / d : (REAL)1;
+    }
+}
+
 /* Fused CG of krylov::cgSolve, krylov.cu:1058-1125 (kResidual :291-366, kCgDirection
- * :427-498, kCgUpdate :501-568, kRestoreBest :369-375, collectResults :958-980), one RHS,
- * no preconditioner. project: subtract the mean of b first (singular consistent systems). */
+ * :427-498, kCgUpdate :501-568, kRestoreBest :369-375, collectResults :958-980), one RHS.
+ * jacobi: z = diag(A)^-1 r (PM_JACOBI, krylov.cu:414-425), rho = (r, z); otherwise z = r.
+ * project: subtract the mean of b first (singular consistent systems). */
 int FN(o_cg)(int n, const int32_t *row, const int32_t *col, const REAL *val, const REAL *b_in, REAL *x,
-             int maxit, double tol, int normalized, int reset_every, int return_best, int project, ob_result *res) {
+             int maxit, double tol, int normalized, int reset_every, int return_best, int project, int jacobi,
+             ob_result *res) {
     const double ns = normalized ? 1.0 / sqrt((double)n) : 1.0;
     REAL *r = (REAL *)malloc(sizeof(REAL) * n), *pa = (REAL *)malloc(sizeof(REAL) * n);
     REAL *pb = (REAL *)malloc(sizeof(REAL) * n), *q = (REAL *)malloc(sizeof(REAL) * n);
     REAL *bw = (REAL *)malloc(sizeof(REAL) * n), *xbest = (REAL *)malloc(sizeof(REAL) * n);
+    REAL *dv = (REAL *)malloc(sizeof(REAL) * n);
+    if (jacobi) FN(o_jacobi_dinv)(n, row, col, val, dv);
+    else for (int i = 0; i < n; ++i) dv[i] = (REAL)1;
     memcpy(bw, b_in, sizeof(REAL) * n);
     if (project) {
         double m = 0;
@@ -1224,15 +1254,17 @@ int FN(o_cg)(int n, const int32_t *row, const int32_t *col, const REAL *val, con
     int it = 0, bestIt = -1, rising = 0, done = ST_RUNNING, restart = 1, saveBest = 0, usedIt = 0;
     /* initial residual */
     {
-        double acc = 0;
+        double acc = 0, accz = 0;
+        O_PAR_SUM(acc, accz)
         for (int i = 0; i < n; ++i) {
             REAL s = 0;
             for (int j = row[i]; j < row[i + 1]; ++j) s += val[j] * x[col[j]];
             REAL ri = bw[i] - s;
             r[i] = ri;
             acc += (double)ri * (double)ri;
+            accz += (double)ri * (double)(ri * dv[i]);
         }
-        rho = acc;
+        rho = jacobi ? accz : acc;
         restart = 1;
         crit = sqrt(acc) * ns;
         if (!isfinite(crit)) { done = ST_NONFINITE; usedIt = 0; }
@@ -1242,21 +1274,28 @@ int FN(o_cg)(int n, const int32_t *row, const int32_t *col, const REAL *val, con
     long guard = 0;
     while (done == ST_RUNNING && guard++ < (long)maxit + 4) {
         if (reset_every > 0 && it > 0 && (it % reset_every) == 0) {
-            double acc = 0;
+            double acc = 0, accz = 0;
+            O_PAR_SUM(acc, accz)
             for (int i = 0; i < n; ++i) {
                 REAL s = 0;
                 for (int j = row[i]; j < row[i + 1]; ++j) s += val[j] * x[col[j]];
                 REAL ri = bw[i] - s;
                 r[i] = ri;
                 acc += (double)ri * (double)ri;
+                accz += (double)ri * (double)(ri * dv[i]);
             }
-            rho = acc;
+            rho = jacobi ? accz : acc;
             restart = 1;
         }
         /* direction */
         double pq = 0;
         const REAL bt = (REAL)beta;
-        for (int i = 0; i < n; ++i) pout[i] = restart ? r[i] : r[i] + bt * pin[i];
+        O_PAR
+        for (int i = 0; i < n; ++i) {
+            const REAL zi = jacobi ? r[i] * dv[i] : r[i];
+            pout[i] = restart ? zi : zi + bt * pin[i];
+        }
+        O_PAR_SUM(pq)
         for (int i = 0; i < n; ++i) {
             REAL s = 0;
             for (int j = row[i]; j < row[i + 1]; ++j) s += val[j] * pout[col[j]];
@@ -1270,8 +1309,9 @@ int FN(o_cg)(int n, const int32_t *row, const int32_t *col, const REAL *val, con
             alpha = a;
         }
         /* update */
-        double rr = 0;
+        double rr = 0, rz = 0;
         const REAL al = (REAL)alpha;
+        O_PAR_SUM(rr, rz)
         for (int i = 0; i < n; ++i) {
             REAL xi = x[i];
             if (saveBest && return_best) xbest[i] = xi;
@@ -1279,6 +1319,7 @@ int FN(o_cg)(int n, const int32_t *row, const int32_t *col, const REAL *val, con
             REAL ri = r[i] - al * q[i];
             r[i] = ri;
             rr += (double)ri * (double)ri;
+            rz += (double)ri * (double)(ri * dv[i]);
         }
         crit = sqrt(rr) * ns;
         ++it;
@@ -1295,7 +1336,7 @@ int FN(o_cg)(int n, const int32_t *row, const int32_t *col, const REAL *val, con
             else if (it >= maxit) done = ST_MAXIT;
             else if (return_best && rising >= 100) done = ST_RISING;
             else {
-                double gamma = rr;
+                double gamma = jacobi ? rz : rr;
                 beta = gamma / rho;
                 rho = gamma;
                 if (!isfinite(beta)) done = ST_NONFINITE;
@@ -1313,23 +1354,28 @@ int FN(o_cg)(int n, const int32_t *row, const int32_t *col, const REAL *val, con
         res->used_iterations = bestIt;
         res->final_residual = best;
     }
-    free(r); free(pa); free(pb); free(q); free(bw); free(xbest);
+    free(r); free(pa); free(pb); free(q); free(bw); free(xbest); free(dv);
     return 0;
 }
 
 /* Fused BiCGStab of krylov::bicgstabSolve, krylov.cu:1129-1186 (kResidual with r0,
- * kBicgP :598-610, kBicgV :613-670, kBicgT :674-750, kBicgUpdate :754-824), one RHS,
- * no preconditioner. */
+ * kBicgP :598-610, kBicgV :613-670, kBicgT :674-750, kBicgUpdate :754-824), one RHS.
+ * jacobi: right preconditioning with diag(A)^-1 -- v = A D^-1 p, t = A D^-1 s and the
+ * solution updated with D^-1 (alpha p + omega s) (loadPre, krylov.cu:243-247). */
 int FN(o_bicgstab)(int n, const int32_t *row, const int32_t *col, const REAL *val, const REAL *b, REAL *x,
-                   int maxit, double tol, int normalized, ob_result *res) {
+                   int maxit, double tol, int normalized, int jacobi, ob_result *res) {
     const double ns = normalized ? 1.0 / sqrt((double)n) : 1.0;
     REAL *r = (REAL *)malloc(sizeof(REAL) * n), *r0 = (REAL *)malloc(sizeof(REAL) * n);
     REAL *p = (REAL *)calloc(n, sizeof(REAL)), *v = (REAL *)calloc(n, sizeof(REAL));
     REAL *t = (REAL *)malloc(sizeof(REAL) * n), *s = (REAL *)malloc(sizeof(REAL) * n);
+    REAL *dv = (REAL *)malloc(sizeof(REAL) * n);
+    if (jacobi) FN(o_jacobi_dinv)(n, row, col, val, dv);
+    else for (int i = 0; i < n; ++i) dv[i] = (REAL)1;
     double rho = 0, alpha = 0, beta = 0, omega = 0, crit = 0, halfCrit = 0;
     int it = 0, done = ST_RUNNING, restart = 1, halfDone = 0, usedIt = 0;
     {
         double acc = 0;
+        O_PAR_SUM(acc)
         for (int i = 0; i < n; ++i) {
             REAL sm = 0;
             for (int j = row[i]; j < row[i + 1]; ++j) sm += val[j] * x[col[j]];
@@ -1348,13 +1394,15 @@ int FN(o_bicgstab)(int n, const int32_t *row, const int32_t *col, const REAL *va
     while (done == ST_RUNNING && guard++ < (long)maxit + 4) {
         {
             const REAL bt = (REAL)beta, om = (REAL)omega;
+            O_PAR
             for (int i = 0; i < n; ++i) p[i] = restart ? r[i] : r[i] + bt * (p[i] - om * v[i]);
         }
         {
             double acc = 0;
+            O_PAR_SUM(acc)
             for (int i = 0; i < n; ++i) {
                 REAL sm = 0;
-                for (int j = row[i]; j < row[i + 1]; ++j) sm += val[j] * p[col[j]];
+                for (int j = row[i]; j < row[i + 1]; ++j) sm += val[j] * (jacobi ? p[col[j]] * dv[col[j]] : p[col[j]]);
                 v[i] = sm;
                 acc += (double)r0[i] * (double)sm;
             }
@@ -1365,11 +1413,13 @@ int FN(o_bicgstab)(int n, const int32_t *row, const int32_t *col, const REAL *va
         }
         {
             const REAL al = (REAL)alpha;
+            O_PAR
             for (int i = 0; i < n; ++i) s[i] = r[i] - al * v[i];
             double ts_ = 0, tt = 0, ss = 0;
+            O_PAR_SUM(ts_, tt, ss)
             for (int i = 0; i < n; ++i) {
                 REAL sm = 0;
-                for (int j = row[i]; j < row[i + 1]; ++j) sm += val[j] * s[col[j]];
+                for (int j = row[i]; j < row[i + 1]; ++j) sm += val[j] * (jacobi ? s[col[j]] * dv[col[j]] : s[col[j]]);
                 t[i] = sm;
                 ts_ += (double)sm * (double)s[i];
                 tt += (double)sm * (double)sm;
@@ -1388,13 +1438,14 @@ int FN(o_bicgstab)(int n, const int32_t *row, const int32_t *col, const REAL *va
         {
             const REAL al = (REAL)alpha, om = (REAL)omega;
             double rr0 = 0, rr = 0;
+            O_PAR_SUM(rr0, rr)
             for (int i = 0; i < n; ++i) {
                 REAL si = r[i] - al * v[i];
                 if (halfDone) {
-                    x[i] += al * p[i] * (REAL)1;
+                    x[i] += al * p[i] * dv[i];
                     r[i] = si;
                 } else {
-                    x[i] += (al * p[i] + om * si) * (REAL)1;
+                    x[i] += (al * p[i] + om * si) * dv[i];
                     REAL ri = si - om * t[i];
                     r[i] = ri;
                     rr0 += (double)r0[i] * (double)ri;
@@ -1422,7 +1473,7 @@ int FN(o_bicgstab)(int n, const int32_t *row, const int32_t *col, const REAL *va
     res->used_iterations = usedIt;
     res->converged = done == ST_CONVERGED;
     res->is_finite = isfinite(crit) ? 1 : 0;
-    free(r); free(r0); free(p); free(v); free(t); free(s);
+    free(r); free(r0); free(p); free(v); free(t); free(s); free(dv);
     return 0;
 }
 

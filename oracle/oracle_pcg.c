@@ -48,8 +48,9 @@ int NAME(int n, const int32_t *frow, const int32_t *fcol, const T *fval, int L, 
     double rho = 0, alpha = 0, beta = 0, crit = 0, best = INFINITY, last = 0;                         \
     int it = 0, bestIt = -1, rising = 0, done = 0, restart = 1, saveBest = 0, usedIt = 0;             \
     {                                                                                                 \
-        double acc = 0;                                                                               \
-        for (int i = 0; i < n; ++i) {                                                                 \
+        double acc = 0;                                                                           \
+        O_PAR_SUM(acc)                                                                            \
+        for (int i = 0; i < n; ++i) {                                                             \
             T s = 0;                                                                                  \
             for (int j = frow[i]; j < frow[i + 1]; ++j) s += fval[j] * x[fcol[j]];                    \
             T ri = bw[i] - s; r[i] = ri; acc += (double)ri * (double)ri;                              \
@@ -63,14 +64,20 @@ int NAME(int n, const int32_t *frow, const int32_t *fcol, const T *fval, int L, 
     while (done == 0 && guard++ < (long)maxit + 4) {                                                  \
         /* V-cycle: u = M r */                                                                        \
         double shift = 0;                                                                             \
-        if (project) { double m = 0; for (int i = 0; i < n; ++i) m += (double)r[i]; shift = m / (double)n; } \
+        if (project) {                                                                            \
+            double m = 0;                                                                         \
+            O_PAR_SUM(m)                                                                          \
+            for (int i = 0; i < n; ++i) m += (double)r[i];                                        \
+            shift = m / (double)n;                                                                \
+        }                                                                                         \
         H *cur[32], *oth[32];                                                                         \
         for (int l = 0; l < L; ++l) {                                                                 \
             int nl = ln[l];                                                                           \
             cur[l] = lx[l]; oth[l] = ly[l];                                                           \
             const H *bl = lb[l];                                                                      \
             /* fused first pre-sweep + residual (kPresmoothResidual) */                               \
-            for (int i = 0; i < nl; ++i) {                                                            \
+            O_PAR                                                                                 \
+            for (int i = 0; i < nl; ++i) {                                                        \
                 H s = 0;                                                                              \
                 for (int j = larow[l][i]; j < larow[l][i + 1]; ++j) {                                 \
                     int c = lacol[l][j];                                                              \
@@ -82,7 +89,8 @@ int NAME(int n, const int32_t *frow, const int32_t *fcol, const T *fval, int L, 
                 lres[l][i] = bi - omega * s;                                                          \
             }                                                                                         \
             for (int k = 1; k < pre; ++k) {                                                           \
-                for (int i = 0; i < nl; ++i) {                                                        \
+                O_PAR                                                                             \
+                for (int i = 0; i < nl; ++i) {                                                    \
                     H s = 0;                                                                          \
                     for (int j = larow[l][i]; j < larow[l][i + 1]; ++j) s += laval[l][j] * cur[l][lacol[l][j]]; \
                     H bi = l == 0 ? (project ? (H)((double)r[i] - shift) : (H)r[i]) : bl[i];          \
@@ -91,7 +99,8 @@ int NAME(int n, const int32_t *frow, const int32_t *fcol, const T *fval, int L, 
                 H *t_ = cur[l]; cur[l] = oth[l]; oth[l] = t_;                                         \
             }                                                                                         \
             if (pre > 1) {                                                                            \
-                for (int i = 0; i < nl; ++i) {                                                        \
+                O_PAR                                                                             \
+                for (int i = 0; i < nl; ++i) {                                                    \
                     H s = 0;                                                                          \
                     for (int j = larow[l][i]; j < larow[l][i + 1]; ++j) s += laval[l][j] * cur[l][lacol[l][j]]; \
                     H bi = l == 0 ? (project ? (H)((double)r[i] - shift) : (H)r[i]) : bl[i];          \
@@ -99,28 +108,32 @@ int NAME(int n, const int32_t *frow, const int32_t *fcol, const T *fval, int L, 
                 }                                                                                     \
             }                                                                                         \
             int ncl = l + 1 < L ? ln[l + 1] : nc;                                                     \
-            for (int i = 0; i < ncl; ++i) {                                                           \
+            O_PAR                                                                                 \
+            for (int i = 0; i < ncl; ++i) {                                                       \
                 H s = 0;                                                                              \
                 for (int j = lrrow[l][i]; j < lrrow[l][i + 1]; ++j) s += lrval[l][j] * lres[l][lrcol[l][j]]; \
                 lb[l + 1][i] = s;                                                                     \
             }                                                                                         \
         }                                                                                             \
-        for (int i = 0; i < nc; ++i) {                                                                \
-            double s = 0;                                                                             \
+        O_PAR                                                                                     \
+        for (int i = 0; i < nc; ++i) {                                                            \
+            double s = 0;                                                                         \
             for (int j = 0; j < nc; ++j) s += (double)pinv[(size_t)i * nc + j] * (double)lb[L][j];    \
             cx[i] = (H)s;                                                                             \
         }                                                                                             \
         for (int l = L - 1; l >= 0; --l) {                                                            \
             int nl = ln[l];                                                                           \
             const H *xc = l + 1 < L ? cur[l + 1] : cx;                                                \
-            for (int i = 0; i < nl; ++i) {                                                            \
+            O_PAR                                                                                 \
+            for (int i = 0; i < nl; ++i) {                                                        \
                 H s = 0;                                                                              \
                 for (int j = lprow[l][i]; j < lprow[l][i + 1]; ++j) s += lpval[l][j] * xc[lpcol[l][j]]; \
                 cur[l][i] = cur[l][i] + s;                                                            \
             }                                                                                         \
             const H *bl = lb[l];                                                                      \
             for (int k = 0; k < post; ++k) {                                                          \
-                for (int i = 0; i < nl; ++i) {                                                        \
+                O_PAR                                                                             \
+                for (int i = 0; i < nl; ++i) {                                                    \
                     H s = 0;                                                                          \
                     for (int j = larow[l][i]; j < larow[l][i + 1]; ++j) s += laval[l][j] * cur[l][lacol[l][j]]; \
                     H bi = l == 0 ? (project ? (H)((double)r[i] - shift) : (H)r[i]) : bl[i];          \
@@ -132,14 +145,16 @@ int NAME(int n, const int32_t *frow, const int32_t *fcol, const T *fval, int L, 
         }                                                                                             \
         if (project) {                                                                                \
             double m = 0;                                                                             \
-            for (int i = 0; i < n; ++i) m += (double)u[i];                                            \
+            O_PAR_SUM(m)                                                                          \
+            for (int i = 0; i < n; ++i) m += (double)u[i];                                        \
             T mm = (T)(m / (double)n);                                                                \
             for (int i = 0; i < n; ++i) u[i] -= mm;                                                   \
         }                                                                                             \
         /* kCgDots */                                                                                 \
         {                                                                                             \
             double rz = 0;                                                                            \
-            for (int i = 0; i < n; ++i) rz += (double)r[i] * (double)u[i];                            \
+            O_PAR_SUM(rz)                                                                         \
+            for (int i = 0; i < n; ++i) rz += (double)r[i] * (double)u[i];                        \
             if (!restart) beta = rz / rho;                                                            \
             rho = rz;                                                                                 \
             if (!isfinite(rho) || !isfinite(beta)) { done = 2; usedIt = it; break; }                  \
@@ -147,9 +162,11 @@ int NAME(int n, const int32_t *frow, const int32_t *fcol, const T *fval, int L, 
         /* kCgDirection<PM_VECTOR> */                                                                 \
         {                                                                                             \
             const T bt = (T)beta;                                                                     \
-            for (int i = 0; i < n; ++i) pout[i] = restart ? u[i] : u[i] + bt * pin[i];                \
-            double pq = 0;                                                                            \
-            for (int i = 0; i < n; ++i) {                                                             \
+            O_PAR                                                                                 \
+            for (int i = 0; i < n; ++i) pout[i] = restart ? u[i] : u[i] + bt * pin[i];            \
+            double pq = 0;                                                                        \
+            O_PAR_SUM(pq)                                                                         \
+            for (int i = 0; i < n; ++i) {                                                         \
                 T s = 0;                                                                              \
                 for (int j = frow[i]; j < frow[i + 1]; ++j) s += fval[j] * pout[fcol[j]];             \
                 q[i] = s; pq += (double)pout[i] * (double)s;                                          \
@@ -162,8 +179,9 @@ int NAME(int n, const int32_t *frow, const int32_t *fcol, const T *fval, int L, 
         /* kCgUpdate<PM_VECTOR> */                                                                    \
         {                                                                                             \
             double rr = 0;                                                                            \
-            const T al = (T)alpha;                                                                    \
-            for (int i = 0; i < n; ++i) {                                                             \
+            const T al = (T)alpha;                                                                \
+            O_PAR_SUM(rr)                                                                         \
+            for (int i = 0; i < n; ++i) {                                                         \
                 T xi = x[i];                                                                          \
                 if (saveBest && return_best) xbest[i] = xi;                                           \
                 x[i] = xi + al * pout[i];                                                             \

@@ -15,7 +15,8 @@ from .ops import (AMGPCGSolve, ComputeCurrentDensityFaceBased, ComputeEpotRHS, C
                   GetKrylovSettings, LinearSolverResultInfo, ReleaseKrylovWorkspaces, SetKrylovSettings,
                   SetupAdvectionMatrix, SetupAdvectionScalar, SetupAdvectionVelocity, SetupEpotMatrix,
                   SetupPressureCorrection,
-                  SetupPressureMatrix, SetupPressureRHS, SetupPressureRHSdiv, SolveLinear, SparseOuterProduct)
+                  SetupPressureMatrix, SetupPressureRHS, SetupPressureRHSdiv, SolveLinear, SolveSegmented,
+                  SparseOuterProduct)
 from .hooks import Hook, Hooks
 from .simulation import LinsolveError, Simulation
 from .mhd import MHDSimulation

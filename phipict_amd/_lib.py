@@ -222,6 +222,10 @@ SIGNATURES = {
     "pp_copy_bytes": [_vp, _vp, _i64, _vp],
     "pp_sell_fill": [_vp, _vp, _vp, _i32, _i32, _i32, _i32, _vp, _vp, _i64, _i32, _vp, _vp, _i32, _vp],
     "pp_sell_probe": [_P(SellDesc), _i32, _i32, _i32, _i32, _vp, _P(_dbl)],
+    "pp_set_krylov_settings": [_i32, _i32, _i32],
+    "pp_solve_segmented": [_P(CsrDesc), _vp, _i32, _i32, _vp, _vp, _i32, _i32, _P(_dbl), _i32, _i32, _i32, _i32, _vp,
+                           _vp],
+    "pp_get_krylov_settings": [_P(_i32), _P(_i32), _P(_i32)],
     "pp_cross_eb": [_vp, _i32, _i32, _i32, _vp, _i32, _vp, _i32, _vp],
     "pp_lorentz_source": [_vp, _i32, _i32, _i32, _vp, _i32, _vp, _i32, _i32, _i32, _vp, _i32, _i32, _vp, _i32, _vp],
 }

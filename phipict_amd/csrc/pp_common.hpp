@@ -55,6 +55,7 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 // bracketed by two HIP events on its own stream; durations are summed per (kernel, rows).
 struct TimerSlot;
 TimerSlot *timer_begin(int kernel_id, int rows, hipStream_t s);
+bool timer_enabled(); // launches are being timed (their events must not be captured in graphs)
 void timer_end(TimerSlot *slot, hipStream_t s);
 
 #define PP_TIMED(ID, ROWS, STREAM, ...)                                       \

@@ -70,16 +70,21 @@ __device__ __forceinline__ void small_allsum(double (&acc)[NV], double *red) {
     __syncthreads();
 }
 
+// CG; with dinv the Jacobi-preconditioned variant (z = D^-1 r, rho = (r, z)).
 template <typename T, int NB>
 __global__ void __launch_bounds__(SMALL_BLOCK) k_small_cg(Csr<T> A, const T *__restrict__ b, T *__restrict__ x,
-                                                          T *__restrict__ work, SolveRec *rec, SmallArgs a) {
+                                                          T *__restrict__ work, SolveRec *rec, const T *__restrict__ dinv,
+                                                          SmallArgs a) {
     __shared__ SolveRec R[NB];
     __shared__ double red[16 * 2 * NB];
     __shared__ double beta[NB], alpha[NB];
+    static_assert(2 * NB <= 16, "reduction width");
     const SmallSpan sp = small_span(blockIdx.x, NB, a.rpm, a.B);
     if (sp.count <= 0) return;
     const int n = a.n, tid = threadIdx.x;
     const T *val = A.val + (size_t)sp.m * A.valStride;
+    const T *dv = dinv ? dinv + (size_t)sp.m * a.dinvStride : nullptr;
+    auto zv = [&](const T *rk, int i) { return dv ? rk[i] * dv[i] : rk[i]; }; // preconditioned residual
     T *r[NB], *pv[2][NB], *q[NB], *xb[NB], *bw[NB], *xs[NB];
     bool on[NB];
 #pragma unroll
@@ -114,10 +119,10 @@ __global__ void __launch_bounds__(SMALL_BLOCK) k_small_cg(Csr<T> A, const T *__r
                 bw[k][i] = a.project ? bi - T(mean[k] / double(n)) : bi;
             }
     __syncthreads();
-    // r = bw - A x; (r, r) of the selected right-hand sides
-    auto residual = [&](const bool (&sel)[NB], double (&rr)[NB]) {
+    // r = bw - A x; (r, r) and (r, z) of the selected right-hand sides
+    auto residual = [&](const bool (&sel)[NB], double (&rr)[2 * NB]) {
 #pragma unroll
-        for (int k = 0; k < NB; ++k) rr[k] = 0.0;
+        for (int k = 0; k < 2 * NB; ++k) rr[k] = 0.0;
         for (int i = tid; i < n; i += SMALL_BLOCK) {
             T ax[NB];
             small_row<T, NB>(A, val, i, ax, [&](int k, int c) { return sel[k] ? xs[k][c] : T(0); });
@@ -126,16 +131,20 @@ __global__ void __launch_bounds__(SMALL_BLOCK) k_small_cg(Csr<T> A, const T *__r
                 if (sel[k]) {
                     const T ri = bw[k][i] - ax[k];
                     r[k][i] = ri;
-                    rr[k] += double(ri) * double(ri);
+                    rr[2 * k] += double(ri) * double(ri);
+                    rr[2 * k + 1] += double(ri) * double(zv(r[k], i));
                 }
         }
-        small_allsum<NB>(rr, red);
+        small_allsum<2 * NB>(rr, red);
     };
     {
-        double rr[NB];
+        double rr[2 * NB];
         residual(on, rr);
         if (tid == 0)
-            for (int k = 0; k < sp.count; ++k) rule_open(R[k], rr[k], a.normScale, false);
+            for (int k = 0; k < sp.count; ++k) {
+                rule_open(R[k], rr[2 * k], a.normScale, false);
+                if (dv) R[k].rho = rr[2 * k + 1];
+            }
         __syncthreads();
     }
     for (int loop = 0, cur = 0; loop < a.maxit + 2; ++loop, cur ^= 1) {
@@ -152,11 +161,11 @@ __global__ void __launch_bounds__(SMALL_BLOCK) k_small_cg(Csr<T> A, const T *__r
                 anyDue |= due[k];
             }
             if (anyDue) {
-                double rr[NB];
+                double rr[2 * NB];
                 residual(due, rr);
                 if (tid == 0)
                     for (int k = 0; k < sp.count; ++k)
-                        if (due[k]) { R[k].rho = rr[k]; R[k].flags |= R_FRESH; }
+                        if (due[k]) { R[k].rho = dv ? rr[2 * k + 1] : rr[2 * k]; R[k].flags |= R_FRESH; }
                 __syncthreads();
             }
         }
@@ -171,7 +180,7 @@ __global__ void __launch_bounds__(SMALL_BLOCK) k_small_cg(Csr<T> A, const T *__r
             double pq[NB];
 #pragma unroll
             for (int k = 0; k < NB; ++k) pq[k] = 0.0;
-            auto dir = [&](int k, int c) { return fresh[k] ? r[k][c] : r[k][c] + bt[k] * pin[k][c]; };
+            auto dir = [&](int k, int c) { return fresh[k] ? zv(r[k], c) : zv(r[k], c) + bt[k] * pin[k][c]; };
             for (int i = tid; i < n; i += SMALL_BLOCK) {
                 T ap[NB];
                 small_row<T, NB>(A, val, i, ap, [&](int k, int c) { return act[k] ? dir(k, c) : T(0); });
@@ -204,9 +213,9 @@ __global__ void __launch_bounds__(SMALL_BLOCK) k_small_cg(Csr<T> A, const T *__r
                 al[k] = T(alpha[k]);
             }
             T *const *p = pv[cur ^ 1];
-            double rr[NB];
+            double rr[2 * NB];
 #pragma unroll
-            for (int k = 0; k < NB; ++k) rr[k] = 0.0;
+            for (int k = 0; k < 2 * NB; ++k) rr[k] = 0.0;
             for (int i = tid; i < n; i += SMALL_BLOCK)
 #pragma unroll
                 for (int k = 0; k < NB; ++k)
@@ -216,14 +225,15 @@ __global__ void __launch_bounds__(SMALL_BLOCK) k_small_cg(Csr<T> A, const T *__r
                         xs[k][i] = xi + al[k] * p[k][i];
                         const T ri = r[k][i] - al[k] * q[k][i];
                         r[k][i] = ri;
-                        rr[k] += double(ri) * double(ri);
+                        rr[2 * k] += double(ri) * double(ri);
+                        rr[2 * k + 1] += double(ri) * double(zv(r[k], i));
                     }
-            small_allsum<NB>(rr, red);
+            small_allsum<2 * NB>(rr, red);
             if (tid == 0)
                 for (int k = 0; k < sp.count; ++k)
                     if (go[k]) {
-                        rule_step(R[k], rr[k], a.normScale, a.returnBest != 0);
-                        if (rec_active(R[k])) beta[k] = rule_cg_beta(R[k], rr[k]);
+                        rule_step(R[k], rr[2 * k], a.normScale, a.returnBest != 0);
+                        if (rec_active(R[k])) beta[k] = rule_cg_beta(R[k], dv ? rr[2 * k + 1] : rr[2 * k]);
                     }
             __syncthreads();
         }
@@ -239,9 +249,12 @@ __global__ void __launch_bounds__(SMALL_BLOCK) k_small_cg(Csr<T> A, const T *__r
     if (tid < NB && on[tid]) rec[sp.first + tid] = R[tid];
 }
 
+// BiCGStab; with dinv right-preconditioned by Jacobi (v = A D^-1 p, t = A D^-1 s, the
+// solution updated with D^-1 (alpha p + omega s)).
 template <typename T, int NB>
 __global__ void __launch_bounds__(SMALL_BLOCK) k_small_bicg(Csr<T> A, const T *__restrict__ b, T *__restrict__ x,
-                                                            T *__restrict__ work, SolveRec *rec, SmallArgs a) {
+                                                            T *__restrict__ work, SolveRec *rec, const T *__restrict__ dinv,
+                                                            SmallArgs a) {
     __shared__ SolveRec R[NB];
     __shared__ double red[16 * 3 * NB];
     __shared__ double beta[NB];
@@ -249,6 +262,8 @@ __global__ void __launch_bounds__(SMALL_BLOCK) k_small_bicg(Csr<T> A, const T *_
     if (sp.count <= 0) return;
     const int n = a.n, tid = threadIdx.x;
     const T *val = A.val + (size_t)sp.m * A.valStride;
+    const T *dv = dinv ? dinv + (size_t)sp.m * a.dinvStride : nullptr;
+    auto pre = [&](T v_, int i) { return dv ? v_ * dv[i] : v_; };
     T *r[NB], *r0[NB], *p[NB], *v[NB], *t[NB], *xs[NB];
     bool on[NB];
 #pragma unroll
@@ -312,7 +327,7 @@ __global__ void __launch_bounds__(SMALL_BLOCK) k_small_bicg(Csr<T> A, const T *_
             for (int k = 0; k < NB; ++k) r0v[k] = 0.0;
             for (int i = tid; i < n; i += SMALL_BLOCK) {
                 T ap[NB];
-                small_row<T, NB>(A, val, i, ap, [&](int k, int c) { return act[k] ? p[k][c] : T(0); });
+                small_row<T, NB>(A, val, i, ap, [&](int k, int c) { return act[k] ? pre(p[k][c], c) : T(0); });
 #pragma unroll
                 for (int k = 0; k < NB; ++k)
                     if (act[k]) {
@@ -340,7 +355,7 @@ __global__ void __launch_bounds__(SMALL_BLOCK) k_small_bicg(Csr<T> A, const T *_
             for (int k = 0; k < 3 * NB; ++k) acc[k] = 0.0;
             for (int i = tid; i < n; i += SMALL_BLOCK) {
                 T as[NB];
-                small_row<T, NB>(A, val, i, as, [&](int k, int c) { return go[k] ? r[k][c] - al[k] * v[k][c] : T(0); });
+                small_row<T, NB>(A, val, i, as, [&](int k, int c) { return go[k] ? pre(r[k][c] - al[k] * v[k][c], c) : T(0); });
 #pragma unroll
                 for (int k = 0; k < NB; ++k)
                     if (go[k]) {
@@ -377,10 +392,10 @@ __global__ void __launch_bounds__(SMALL_BLOCK) k_small_bicg(Csr<T> A, const T *_
                     if (fin[k]) {
                         const T si = r[k][i] - al[k] * v[k][i];
                         if (half[k]) {
-                            xs[k][i] += al[k] * p[k][i];
+                            xs[k][i] += pre(al[k] * p[k][i], i);
                             r[k][i] = si;
                         } else {
-                            xs[k][i] += al[k] * p[k][i] + om[k] * si;
+                            xs[k][i] += pre(al[k] * p[k][i] + om[k] * si, i);
                             const T ri = si - om[k] * t[k][i];
                             r[k][i] = ri;
                             acc[2 * k] += double(r0[k][i]) * double(ri);
@@ -401,14 +416,14 @@ __global__ void __launch_bounds__(SMALL_BLOCK) k_small_bicg(Csr<T> A, const T *_
 }
 
 template <typename T>
-void launch_small(const Csr<T> &A, const T *b, T *x, void *work, SolveRec *rec, const SmallArgs &a, int groups,
-                  hipStream_t s) {
+void launch_small(const Csr<T> &A, const T *b, T *x, void *work, SolveRec *rec, const T *dinv, const SmallArgs &a,
+                  int groups, hipStream_t s) {
     T *w = reinterpret_cast<T *>(work);
     switch (a.nb_per_group) {
-#define PP_SMALL_CASE(NBV)                                                                                       \
-    case NBV:                                                                                                    \
-        if (a.bicg) hipLaunchKernelGGL((k_small_bicg<T, NBV>), dim3(groups), dim3(SMALL_BLOCK), 0, s, A, b, x, w, rec, a); \
-        else hipLaunchKernelGGL((k_small_cg<T, NBV>), dim3(groups), dim3(SMALL_BLOCK), 0, s, A, b, x, w, rec, a);    \
+#define PP_SMALL_CASE(NBV)                                                                                                \
+    case NBV:                                                                                                             \
+        if (a.bicg) hipLaunchKernelGGL((k_small_bicg<T, NBV>), dim3(groups), dim3(SMALL_BLOCK), 0, s, A, b, x, w, rec, dinv, a); \
+        else hipLaunchKernelGGL((k_small_cg<T, NBV>), dim3(groups), dim3(SMALL_BLOCK), 0, s, A, b, x, w, rec, dinv, a);    \
         break;
         PP_SMALL_CASE(1)
         PP_SMALL_CASE(2)
@@ -417,10 +432,10 @@ void launch_small(const Csr<T> &A, const T *b, T *x, void *work, SolveRec *rec, 
 #undef PP_SMALL_CASE
     }
 }
-template void launch_small<float>(const Csr<float> &, const float *, float *, void *, SolveRec *, const SmallArgs &, int,
-                                  hipStream_t);
-template void launch_small<double>(const Csr<double> &, const double *, double *, void *, SolveRec *, const SmallArgs &,
-                                   int, hipStream_t);
+template void launch_small<float>(const Csr<float> &, const float *, float *, void *, SolveRec *, const float *,
+                                  const SmallArgs &, int, hipStream_t);
+template void launch_small<double>(const Csr<double> &, const double *, double *, void *, SolveRec *, const double *,
+                                   const SmallArgs &, int, hipStream_t);
 
 // ======================================================================================
 // Sparse utility kernels
@@ -782,7 +797,7 @@ int pp_galerkin_rap(const int32_t *r_row, const int32_t *r_col, const double *r_
     PP_CHECK_ARG(r_row && r_col && r_val && A && p_row && p_col && p_val && c_row && c_col && c_val, "null argument");
     if (c_n == 0) return PP_OK;
     hipStream_t s = as_stream(stream);
-    static int *bad = nullptr;
+    static thread_local int *bad = nullptr; // one flag per host thread
     if (!bad) PP_HIP(hipMalloc((void **)&bad, sizeof(int)));
     PP_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
     const dim3 g((c_n + 127) / 128, A->num_matrices);

@@ -206,13 +206,15 @@ constexpr int SMALL_SLOTS = 9; // work vectors per right-hand side
 struct SmallArgs {
     int n, B, nb_per_group, rpm, maxit, normalized, returnBest, project, resetEvery, bicg;
     double normScale;
+    int dinvStride; // Jacobi preconditioner: n per matrix of a batch, 0 for a shared matrix
 };
 
 // Launch the on-chip engine: `groups` workgroups, records rec[B] initialised by the caller,
-// work vectors (9 per right-hand side, n each) at work.
+// work vectors (9 per right-hand side, n each) at work; dinv (may be null): inverse diagonals
+// of the Jacobi preconditioner.
 template <typename T>
-void launch_small(const Csr<T> &A, const T *b, T *x, void *work, SolveRec *rec, const SmallArgs &a, int groups,
-                  hipStream_t s);
+void launch_small(const Csr<T> &A, const T *b, T *x, void *work, SolveRec *rec, const T *dinv, const SmallArgs &a,
+                  int groups, hipStream_t s);
 
 // Values of A^T on A's (structurally symmetric) pattern into tval; *bad set if a mirror entry
 // is missing.

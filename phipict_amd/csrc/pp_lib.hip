@@ -31,6 +31,8 @@ static struct {
     std::vector<TimerSlot *> pending, pool;
 } g_timer;
 
+bool timer_enabled() { return g_timer.enabled; }
+
 TimerSlot *timer_begin(int kernel_id, int rows, hipStream_t s) {
     if (!g_timer.enabled) return nullptr;
     std::lock_guard<std::mutex> lock(g_timer.mu);

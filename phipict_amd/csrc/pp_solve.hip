@@ -159,6 +159,15 @@ struct Phase {
     int keepBest;       // return-best bookkeeping
 };
 
+// Jacobi preconditioner of the fused CG / BiCGStab (krylov.cu:414-425, 243-247): the inverse
+// diagonal of each matrix, d[m * stride + i]; with JAC false it is never read.
+template <typename T>
+struct Jac {
+    const T *d;
+    int stride; // n per matrix of a batch, 0 when the matrix is shared
+    __device__ __forceinline__ const T *of(int m) const { return d + (size_t)m * stride; }
+};
+
 // ======================================================================================
 // Opening: partial sums of b, and the initial residual r = b - A x
 // ======================================================================================
@@ -173,11 +182,12 @@ __global__ void __launch_bounds__(WG) k_sum(const T *__restrict__ v, int n, Part
 }
 
 // r = bw - A x with bw = b - mean(b) when bSum is given (bw written to bwOut for later
-// residual resets), r0 = r for BiCGStab. Partials: (r, r), (bw, bw), sum(r).
-template <typename T, int NB, int EW>
+// residual resets), r0 = r for BiCGStab. Partials: (r, r), (bw, bw) and sum(r), or with the
+// Jacobi preconditioner (r, D^-1 r) in the last place.
+template <typename T, int NB, int EW, bool JAC>
 __global__ void __launch_bounds__(WG) k_open(Csr<T> A, const T *__restrict__ b, Parts bSum, T *__restrict__ bwOut,
                                              const T *__restrict__ x, T *__restrict__ r, T *__restrict__ r0, int B,
-                                             Parts out) {
+                                             Parts out, Jac<T> jac) {
     __shared__ double lds[16 * NB * 3];
     const int n = A.n;
     const Span sp = span_of<NB>(blockIdx.y, A.rpm, B);
@@ -212,7 +222,8 @@ __global__ void __launch_bounds__(WG) k_open(Csr<T> A, const T *__restrict__ b, 
                 if (r0) r0[o] = ri;
                 acc[k][0] += double(ri) * double(ri);
                 acc[k][1] += double(bi) * double(bi);
-                acc[k][2] += double(ri);
+                if constexpr (JAC) acc[k][2] += double(ri) * double(ri * jac.of(sp.m)[i]);
+                else acc[k][2] += double(ri);
             }
     }
     put_parts<NB, 3>(out, acc, sp.first, sp.count, lds);
@@ -234,11 +245,12 @@ enum TurnMode { TURN_OPEN = 0, TURN_STEP = 1, TURN_STEP_RESET = 2, TURN_AFTER_RE
 // TURN_STEP_RESET decides like TURN_STEP and then, instead of a direction, replaces the
 // residual by bw - A x (partials of its (r, r) into resetOut, records marked fresh and
 // reset); TURN_AFTER_RESET only forms the fresh direction from the records already decided.
-template <typename T, int NB, int EW>
+// JAC: z = D^-1 r in place of r, rho = (r, z) (the last quantity of rrIn).
+template <typename T, int NB, int EW, bool JAC>
 __global__ void __launch_bounds__(WG) k_cg_turn(Csr<T> A, T *__restrict__ r, const T *__restrict__ pIn,
                                                 T *__restrict__ pOut, T *__restrict__ q, const T *__restrict__ x,
                                                 const T *__restrict__ bw, Parts rrIn, Parts pqOut, Parts resetOut,
-                                                Phase ph, int mode) {
+                                                Phase ph, int mode, Jac<T> jac) {
     __shared__ double lds[16 * NB * 3];
     const int n = A.n;
     const Span sp = span_of<NB>(blockIdx.y, A.rpm, ph.B);
@@ -249,8 +261,8 @@ __global__ void __launch_bounds__(WG) k_cg_turn(Csr<T> A, T *__restrict__ r, con
 #pragma unroll
     for (int k = 0; k < NB; ++k) beta[k] = T(0);
     if (mode != TURN_AFTER_RESET) {
-        double tot[NB][2];
-        fold<NB, 2>(rrIn, 0, sp.first, sp.count, tot, lds);
+        double tot[NB][3];
+        fold<NB, 3>(rrIn, 0, sp.first, sp.count, tot, lds);
         any = false;
 #pragma unroll
         for (int k = 0; k < NB; ++k) {
@@ -258,9 +270,10 @@ __global__ void __launch_bounds__(WG) k_cg_turn(Csr<T> A, T *__restrict__ r, con
             if (mode == TURN_OPEN) {
                 rule_open(R[k], tot[k][0], ph.scale, false);
                 R[k].zero = tot[k][1] == 0.0;
+                if (JAC) R[k].rho = tot[k][2];
             } else {
                 rule_step(R[k], tot[k][0], ph.scale, ph.keepBest != 0);
-                if (rec_active(R[k])) beta[k] = T(rule_cg_beta(R[k], tot[k][0]));
+                if (rec_active(R[k])) beta[k] = T(rule_cg_beta(R[k], JAC ? tot[k][2] : tot[k][0]));
             }
             act[k] = rec_active(R[k]);
             any |= act[k];
@@ -273,10 +286,11 @@ __global__ void __launch_bounds__(WG) k_cg_turn(Csr<T> A, T *__restrict__ r, con
     }
     const T *val = A.val + (size_t)sp.m * A.valStride;
     const T *ellv = A.ell_val + (A.valStride ? (size_t)sp.m * A.ell_w * n : 0);
+    const T *dv = JAC ? jac.of(sp.m) : nullptr;
     if (mode == TURN_STEP_RESET) {
-        double acc[NB][1];
+        double acc[NB][2];
 #pragma unroll
-        for (int k = 0; k < NB; ++k) acc[k][0] = 0.0;
+        for (int k = 0; k < NB; ++k) acc[k][0] = acc[k][1] = 0.0;
         if (any)
             PP_ROWS(i, n) {
                 T ax[NB];
@@ -294,9 +308,10 @@ __global__ void __launch_bounds__(WG) k_cg_turn(Csr<T> A, T *__restrict__ r, con
                         const T ri = bw[o] - ax[k];
                         r[o] = ri;
                         acc[k][0] += double(ri) * double(ri);
+                        if constexpr (JAC) acc[k][1] += double(ri) * double(ri * dv[i]);
                     }
             }
-        put_parts<NB, 1>(resetOut, acc, sp.first, sp.count, lds);
+        put_parts<NB, 2>(resetOut, acc, sp.first, sp.count, lds);
         return;
     }
     bool fresh[NB];
@@ -315,14 +330,16 @@ __global__ void __launch_bounds__(WG) k_cg_turn(Csr<T> A, T *__restrict__ r, con
                 for (int k = 0; k < NB; ++k)
                     if (act[k]) {
                         const size_t o = (size_t)(sp.first + k) * n + c;
-                        ap[k] += a * (fresh[k] ? r[o] : r[o] + beta[k] * pIn[o]);
+                        const T zc = JAC ? r[o] * dv[c] : r[o];
+                        ap[k] += a * (fresh[k] ? zc : zc + beta[k] * pIn[o]);
                     }
             });
 #pragma unroll
             for (int k = 0; k < NB; ++k)
                 if (act[k]) {
                     const size_t o = (size_t)(sp.first + k) * n + i;
-                    const T pi = fresh[k] ? r[o] : r[o] + beta[k] * pIn[o];
+                    const T zi = JAC ? r[o] * dv[i] : r[o];
+                    const T pi = fresh[k] ? zi : zi + beta[k] * pIn[o];
                     pOut[o] = pi;
                     q[o] = ap[k];
                     acc[k][0] += double(pi) * double(ap[k]);
@@ -332,12 +349,13 @@ __global__ void __launch_bounds__(WG) k_cg_turn(Csr<T> A, T *__restrict__ r, con
 }
 
 // Advance: alpha = rho / (p', q) from the folded partials; x += alpha p', r -= alpha q
-// (saving x first when it is the best iterate); partials (r, r), 0, sum(r). Workgroup 0
-// commits the records (fresh and reset marks cleared; a reset's rho from resetIn).
-template <typename T, int NB>
+// (saving x first when it is the best iterate); partials (r, r), 0 and sum(r), or (r, D^-1 r)
+// with JAC. Workgroup 0 commits the records (fresh and reset marks cleared; a reset's rho from
+// resetIn).
+template <typename T, int NB, bool JAC>
 __global__ void __launch_bounds__(WG) k_cg_step(int n, int rpm, T *__restrict__ r, const T *__restrict__ p,
                                                 const T *__restrict__ q, T *__restrict__ x, T *__restrict__ xbest,
-                                                Parts pqIn, Parts resetIn, Parts rrOut, Phase ph) {
+                                                Parts pqIn, Parts resetIn, Parts rrOut, Phase ph, Jac<T> jac) {
     __shared__ double lds[16 * NB * 3];
     const Span sp = span_of<NB>(blockIdx.y, rpm, ph.B);
     SolveRec R[NB];
@@ -347,11 +365,11 @@ __global__ void __launch_bounds__(WG) k_cg_step(int n, int rpm, T *__restrict__ 
 #pragma unroll
     for (int k = 0; k < NB; ++k) reset |= act[k] && (R[k].flags & R_RESET);
     if (reset) {
-        double rr[NB][1];
-        fold<NB, 1>(resetIn, 0, sp.first, sp.count, rr, lds);
+        double rr[NB][2];
+        fold<NB, 2>(resetIn, 0, sp.first, sp.count, rr, lds);
 #pragma unroll
         for (int k = 0; k < NB; ++k)
-            if (act[k] && (R[k].flags & R_RESET)) R[k].rho = rr[k][0];
+            if (act[k] && (R[k].flags & R_RESET)) R[k].rho = JAC ? rr[k][1] : rr[k][0];
     }
     double pq[NB][1];
     fold<NB, 1>(pqIn, 0, sp.first, sp.count, pq, lds);
@@ -383,7 +401,8 @@ __global__ void __launch_bounds__(WG) k_cg_step(int n, int rpm, T *__restrict__ 
             const T ri = r[o] - alpha[k] * q[o];
             r[o] = ri;
             acc[k][0] += double(ri) * double(ri);
-            acc[k][2] += double(ri);
+            if constexpr (JAC) acc[k][2] += double(ri) * double(ri * jac.of(sp.m)[i]);
+            else acc[k][2] += double(ri);
         }
     }
     put_parts<NB, 3>(rrOut, acc, sp.first, sp.count, lds);
@@ -422,10 +441,10 @@ __global__ void __launch_bounds__(WG) k_bicg_dir(int n, const T *__restrict__ r,
     else PP_ROWS(i, n) p[base + i] = r[base + i] + beta * (p[base + i] - omega * v[base + i]);
 }
 
-// v = A p; partials (r0, v).
-template <typename T, int NB, int EW>
+// v = A p (A D^-1 p with JAC); partials (r0, v).
+template <typename T, int NB, int EW, bool JAC>
 __global__ void __launch_bounds__(WG) k_bicg_apply(Csr<T> A, const T *__restrict__ p, T *__restrict__ v,
-                                                   const T *__restrict__ r0, Parts r0vOut, Phase ph) {
+                                                   const T *__restrict__ r0, Parts r0vOut, Phase ph, Jac<T> jac) {
     __shared__ double lds[16 * NB];
     const int n = A.n;
     const Span sp = span_of<NB>(blockIdx.y, A.rpm, ph.B);
@@ -445,7 +464,10 @@ __global__ void __launch_bounds__(WG) k_bicg_apply(Csr<T> A, const T *__restrict
             row_each<EW>(A, val, ellv, i, [&](int c, T a) {
 #pragma unroll
                 for (int k = 0; k < NB; ++k)
-                    if (act[k]) ap[k] += a * p[(size_t)(sp.first + k) * n + c];
+                    if (act[k]) {
+                        const T pc = p[(size_t)(sp.first + k) * n + c];
+                        ap[k] += a * (JAC ? pc * jac.of(sp.m)[c] : pc);
+                    }
             });
 #pragma unroll
             for (int k = 0; k < NB; ++k)
@@ -459,10 +481,10 @@ __global__ void __launch_bounds__(WG) k_bicg_apply(Csr<T> A, const T *__restrict
 }
 
 // alpha = rho / (r0, v) folded; s = r - alpha v (formed at every neighbour as it is read),
-// t = A s; partials (t, s), (t, t), (s, s).
-template <typename T, int NB, int EW>
+// t = A s (A D^-1 s with JAC); partials (t, s), (t, t), (s, s).
+template <typename T, int NB, int EW, bool JAC>
 __global__ void __launch_bounds__(WG) k_bicg_stab(Csr<T> A, const T *__restrict__ r, const T *__restrict__ v,
-                                                  T *__restrict__ t, Parts r0vIn, Parts tsOut, Phase ph) {
+                                                  T *__restrict__ t, Parts r0vIn, Parts tsOut, Phase ph, Jac<T> jac) {
     __shared__ double lds[16 * NB * 3];
     const int n = A.n;
     const Span sp = span_of<NB>(blockIdx.y, A.rpm, ph.B);
@@ -496,7 +518,8 @@ __global__ void __launch_bounds__(WG) k_bicg_stab(Csr<T> A, const T *__restrict_
                 for (int k = 0; k < NB; ++k)
                     if (act[k]) {
                         const size_t o = (size_t)(sp.first + k) * n + c;
-                        as[k] += a * (r[o] - alpha[k] * v[o]);
+                        const T sc = r[o] - alpha[k] * v[o];
+                        as[k] += a * (JAC ? sc * jac.of(sp.m)[c] : sc);
                     }
             });
 #pragma unroll
@@ -514,13 +537,13 @@ __global__ void __launch_bounds__(WG) k_bicg_stab(Csr<T> A, const T *__restrict_
 }
 
 // alpha and omega from the folded partials (half-step test on (s, s)); x += alpha p + omega s,
-// r = s - omega t (half step: x += alpha p, r = s); partials (r0, r), (r, r). Workgroup 0
-// commits the records.
-template <typename T>
-__global__ void __launch_bounds__(WG) k_bicg_step(int n, T *__restrict__ r, const T *__restrict__ r0,
+// r = s - omega t (half step: x += alpha p, r = s), the x increment scaled by D^-1 with JAC;
+// partials (r0, r), (r, r). Workgroup 0 commits the records.
+template <typename T, bool JAC>
+__global__ void __launch_bounds__(WG) k_bicg_step(int n, int rpm, T *__restrict__ r, const T *__restrict__ r0,
                                                   const T *__restrict__ p, const T *__restrict__ v,
                                                   const T *__restrict__ t, T *__restrict__ x, Parts r0vIn, Parts tsIn,
-                                                  Parts rrOut, Phase ph) {
+                                                  Parts rrOut, Phase ph, Jac<T> jac) {
     __shared__ double lds[16 * 3];
     const int b = blockIdx.y;
     SolveRec R = ph.in[b];
@@ -542,14 +565,15 @@ __global__ void __launch_bounds__(WG) k_bicg_step(int n, T *__restrict__ r, cons
     if (act) {
         const T alpha = T(R.alpha), omega = T(R.omega);
         const size_t base = (size_t)b * n;
+        const T *dv = JAC ? jac.of(b / rpm) : nullptr;
         PP_ROWS(i, n) {
             const size_t o = base + i;
             const T si = r[o] - alpha * v[o];
             if (half) {
-                x[o] += alpha * p[o];
+                x[o] += JAC ? alpha * p[o] * dv[i] : alpha * p[o];
                 r[o] = si;
             } else {
-                x[o] += alpha * p[o] + omega * si;
+                x[o] += JAC ? (alpha * p[o] + omega * si) * dv[i] : alpha * p[o] + omega * si;
                 const T ri = si - omega * t[o];
                 r[o] = ri;
                 acc[0][0] += double(r0[o]) * double(ri);
@@ -558,6 +582,20 @@ __global__ void __launch_bounds__(WG) k_bicg_step(int n, T *__restrict__ r, cons
         }
     }
     put_parts<1, 2>(rrOut, acc, b, 1, lds);
+}
+
+// Inverse diagonal of each matrix for the Jacobi preconditioner (kInverseDiagonal,
+// krylov.cu:272-283): the row's diagonal entries summed, 1 where the sum is zero.
+template <typename T>
+__global__ void k_jacobi_dinv(Csr<T> A, int nmat, T *__restrict__ dinv) {
+    const int m = blockIdx.y;
+    const T *val = A.val + (size_t)m * A.valStride;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+        T d = T(0);
+        for (int e = A.row[i]; e < A.row[i + 1]; ++e)
+            if (A.col[e] == i) d += val[e];
+        dinv[(size_t)m * A.n + i] = d != T(0) ? T(1) / d : T(1);
+    }
 }
 
 // x = 0 for every right-hand side (all right-hand sides were exactly zero).
@@ -1008,8 +1046,35 @@ struct Space {
     void *tailHost = nullptr;
     char *snap = nullptr;  size_t snapCap = 0;  // pinned snapshots: two slots of B records
 };
+// Workspaces belong to the calling host thread: solves issued from different threads (each
+// on its own stream) run concurrently without sharing buffers. g_lock guards what is shared
+// (settings, the graph cache, the capture streams, the registry of workspaces).
 static std::mutex g_lock;
-static Space g_space[3]; // CG, BiCGStab, AMG-PCG
+
+// The workspaces of one host thread: CG, BiCGStab, AMG-PCG, the segmented engine (with its
+// uploaded tile layout) and the transposed-matrix buffer.
+struct SegSpace {
+    Space sp;
+    int32_t *layoutDev = nullptr; size_t layoutCap = 0;
+    int32_t *layoutHost = nullptr; size_t layoutHostCap = 0;
+};
+struct ThreadSpaces {
+    Space cg, bicg, amg;
+    SegSpace seg;
+    void *tbuf = nullptr; size_t tcap = 0;
+    int *tbad = nullptr;
+};
+static std::vector<ThreadSpaces *> g_all_spaces; // every thread's, for pp_release_workspaces
+
+static ThreadSpaces &my_spaces() {
+    static thread_local ThreadSpaces *mine = nullptr;
+    if (!mine) {
+        mine = new ThreadSpaces();
+        std::lock_guard<std::mutex> lock(g_lock);
+        g_all_spaces.push_back(mine);
+    }
+    return *mine;
+}
 
 static int grow(void **ptr, size_t *cap, size_t bytes) {
     bytes = std::max<size_t>(bytes, 256);
@@ -1111,9 +1176,76 @@ struct Pace {
     bool eager;
 };
 
+// Captured chunks of iterations (the useGraphs setting): a chunk of k iterations starting at
+// an even or odd iteration is captured once into a HIP graph, keyed by everything its kernels
+// are given (pointers, sizes, grids), and replayed with one graph launch instead of 2k-4k
+// kernel launches. The cache keeps the most recent entries.
+static int g_use_graphs = 1;
+static std::mutex g_graph_lock; // the graph cache and the capture streams
+struct ChunkGraph {
+    std::vector<uintptr_t> key;
+    int k, parity;
+    hipGraphExec_t exec;
+};
+static std::vector<ChunkGraph> g_graphs;
+constexpr size_t GRAPH_CACHE = 96;
+
+static void drop_graphs() {
+    for (auto &g : g_graphs) (void)hipGraphExecDestroy(g.exec);
+    g_graphs.clear();
+}
+
+template <typename Body>
+static int launch_chunk(hipStream_t s, int first, int k, const std::vector<uintptr_t> *key, Body &body) {
+    if (!key || !g_use_graphs || timer_enabled()) {
+        for (int j = 0; j < k; ++j) body(s, first + j);
+        return PP_OK;
+    }
+    const int parity = first & 1;
+    std::lock_guard<std::mutex> glock(g_graph_lock);
+    for (size_t i = 0; i < g_graphs.size(); ++i)
+        if (g_graphs[i].k == k && g_graphs[i].parity == parity && g_graphs[i].key == *key) {
+            PP_HIP(hipGraphLaunch(g_graphs[i].exec, s));
+            return PP_OK;
+        }
+    // capture on a private stream of this device (the caller's may be the legacy default
+    // stream, which cannot capture); the graph is then launched on the caller's stream
+    static hipStream_t capture[16] = {};
+    int dev = 0;
+    PP_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 16) {
+        for (int j = 0; j < k; ++j) body(s, first + j);
+        return PP_OK;
+    }
+    if (!capture[dev]) PP_HIP(hipStreamCreateWithFlags(&capture[dev], hipStreamNonBlocking));
+    hipStream_t cs = capture[dev];
+    hipGraph_t graph = nullptr;
+    PP_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    for (int j = 0; j < k; ++j) body(cs, first + j);
+    const hipError_t ce = hipStreamEndCapture(cs, &graph);
+    if (ce != hipSuccess || !graph) {
+        set_error(std::string("launch_chunk: capture failed: ") + hipGetErrorString(ce));
+        return PP_ERR_HIP;
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+        set_error(std::string("launch_chunk: instantiate failed: ") + hipGetErrorString(ie));
+        return PP_ERR_HIP;
+    }
+    if (g_graphs.size() >= GRAPH_CACHE) {
+        (void)hipGraphExecDestroy(g_graphs.front().exec);
+        g_graphs.erase(g_graphs.begin());
+    }
+    g_graphs.push_back(ChunkGraph{*key, k, parity, exec});
+    PP_HIP(hipGraphLaunch(exec, s));
+    return PP_OK;
+}
+
 template <typename Open, typename Body>
 static int pace_solve(Space &sp, hipStream_t s, const SolveRec *decided, int B, int loopsMax, Pace pace, int *lastSlot,
-                      Open stopAfterOpening, Body body) {
+                      Open stopAfterOpening, Body body, const std::vector<uintptr_t> *graphKey = nullptr) {
     const size_t bytes = sizeof(SolveRec) * (size_t)B;
     const SolveRec *slots[2] = {reinterpret_cast<SolveRec *>(sp.snap), reinterpret_cast<SolveRec *>(sp.snap + bytes)};
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -1189,7 +1321,7 @@ static int pace_solve(Space &sp, hipStream_t s, const SolveRec *decided, int B, 
             if (next <= 0) next = pending == 0 ? 1 : 0;
             next = std::min(next, loopsMax - done);
             if (next > 0) {
-                for (int k = 0; k < next; ++k) body(s);
+                if ((rc = launch_chunk(s, done, next, graphKey, body))) break;
                 done += next;
                 if (hipGetLastError() != hipSuccess) {
                     set_error("pace_solve: kernel launch failed");
@@ -1245,60 +1377,75 @@ static bool all_zero_rhs(const SolveRec *r, int B) {
         default: { constexpr int EW = 0; __VA_ARGS__; break; } \
     }
 
+// Settings of the fused solvers (SetKrylovSettings, bindings.cpp:739-752): iterations per
+// launch chunk at most, and the Jacobi preconditioner of CG and of BiCGStab.
 static int g_chunk_cap = 8;
+static int g_cg_jacobi = 0, g_bicg_jacobi = 0;
+
+#define PP_JAC_CASE(ON, ...)                                   \
+    if (ON) { constexpr bool JAC = true; __VA_ARGS__; }        \
+    else { constexpr bool JAC = false; __VA_ARGS__; }
+
+// Inverse diagonals of the Jacobi preconditioner into dv (numMat x n).
+template <typename T>
+static void launch_jacobi(const Csr<T> &A, int numMat, T *dv, hipStream_t s) {
+    hipLaunchKernelGGL((k_jacobi_dinv<T>), dim3(std::min(1024, (A.n + 255) / 256), numMat), dim3(256), 0, s, A, numMat, dv);
+}
 
 // ---------------- CG (large systems) ----------------
 template <typename T>
 static int cg_large(const Csr<T> &A, const T *b, T *x, int B, int maxit, const double *tols, int ntols, double scale,
-                    int resetEvery, int returnBest, int project, hipStream_t s, pp_solve_result *res) {
-    Space &sp = g_space[0];
+                    int resetEvery, int returnBest, int project, int jacobi, Space &sp, hipStream_t s,
+                    pp_solve_result *res) {
     const int n = A.n;
     const int NBv = std::min(A.rpm, 4);
-    const int rows = ((B + A.rpm - 1) / A.rpm) * ((A.rpm + NBv - 1) / NBv);
+    const int numMat = (B + A.rpm - 1) / A.rpm;
+    const int rows = numMat * ((A.rpm + NBv - 1) / NBv);
     const int G = phase_width(n, rows);
     enum { V_R = 0, V_Q, V_P0, V_P1, V_XB, V_BW, V_COUNT };
-    const size_t partDoubles = (size_t)B * G * (1 + 3 + 1 + 1);
-    if (int e = prepare(sp, sizeof(T) * (size_t)V_COUNT * n * B, partDoubles, B, tols, ntols, s)) return e;
+    const size_t vecBytes = sizeof(T) * ((size_t)V_COUNT * n * B + (size_t)numMat * n);
+    const size_t partDoubles = (size_t)B * G * (1 + 3 + 1 + 2);
+    if (int e = prepare(sp, vecBytes, partDoubles, B, tols, ntols, s)) return e;
     T *vec = reinterpret_cast<T *>(sp.vec);
     auto V = [&](int k) { return vec + (size_t)k * n * B; };
     T *r = V(V_R), *q = V(V_Q), *p[2] = {V(V_P0), V(V_P1)};
     T *xbest = returnBest ? V(V_XB) : nullptr;
     T *bw = project ? V(V_BW) : const_cast<T *>(b);
+    const Jac<T> jac{V(V_COUNT), A.valStride ? n : 0};
     PartsCarver pc{reinterpret_cast<double *>(sp.parts)};
-    const Parts sSum = pc.take(B, 1, G), sRR = pc.take(B, 3, G), sPQ = pc.take(B, 1, G), sRst = pc.take(B, 1, G);
+    const Parts sSum = pc.take(B, 1, G), sRR = pc.take(B, 3, G), sPQ = pc.take(B, 1, G), sRst = pc.take(B, 2, G);
     SolveRec *ra = recs_a(sp), *rb = recs_b(sp, B);
     const dim3 grid(G, rows), block(WG);
     hipLaunchKernelGGL(k_init_recs, dim3(1), dim3(64), 0, s, ra, B, tols[0], ntols > 1 ? (const double *)sp.tols : nullptr, maxit);
+    if (jacobi) launch_jacobi<T>(A, numMat, V(V_COUNT), s);
     if (project) hipLaunchKernelGGL((k_sum<T>), dim3(G, B), block, 0, s, b, n, sSum);
     const Phase openPh{ra, rb, B, scale, returnBest};
-    PP_NB_CASE(NBv, PP_EW_CASE(A.ell_w,
-        hipLaunchKernelGGL((k_open<T, NB, EW>), grid, block, 0, s, A, b, project ? sSum : Parts{nullptr, 1, G},
-                           project ? bw : (T *)nullptr, x, r, (T *)nullptr, B, sRR);
-        hipLaunchKernelGGL((k_cg_turn<T, NB, EW>), grid, block, 0, s, A, r, p[0], p[1], q, x, bw, sRR, sPQ, sRst, openPh,
-                           (int)TURN_OPEN)));
+    PP_JAC_CASE(jacobi, PP_NB_CASE(NBv, PP_EW_CASE(A.ell_w,
+        hipLaunchKernelGGL((k_open<T, NB, EW, JAC>), grid, block, 0, s, A, b, project ? sSum : Parts{nullptr, 1, G},
+                           project ? bw : (T *)nullptr, x, r, (T *)nullptr, B, sRR, jac);
+        hipLaunchKernelGGL((k_cg_turn<T, NB, EW, JAC>), grid, block, 0, s, A, r, p[0], p[1], q, x, bw, sRR, sPQ, sRst,
+                           openPh, (int)TURN_OPEN, jac))));
     PP_LAUNCH_CHECK();
-    int cur = 1, steps = 0;
     const Phase stepPh{rb, ra, B, scale, returnBest}, turnPh{ra, rb, B, scale, returnBest};
     const Phase afterPh{rb, nullptr, B, scale, returnBest};
-    auto body = [&](hipStream_t st) {
-        ++steps;
-        const bool reset = resetEvery > 0 && steps % resetEvery == 0;
-        PP_NB_CASE(NBv,
+    auto body = [&](hipStream_t st, int j) {
+        const int cur = 1 ^ (j & 1); // the opening turn wrote p[1]
+        const bool reset = resetEvery > 0 && (j + 1) % resetEvery == 0;
+        PP_JAC_CASE(jacobi, PP_NB_CASE(NBv,
             PP_TIMED(PP_K_CG_UPDATE, n, st,
-                hipLaunchKernelGGL((k_cg_step<T, NB>), grid, block, 0, st, n, A.rpm, r, p[cur], q, x, xbest, sPQ, sRst,
-                                   sRR, stepPh));
+                hipLaunchKernelGGL((k_cg_step<T, NB, JAC>), grid, block, 0, st, n, A.rpm, r, p[cur], q, x, xbest, sPQ,
+                                   sRst, sRR, stepPh, jac));
             PP_EW_CASE(A.ell_w,
                 if (reset) {
-                    hipLaunchKernelGGL((k_cg_turn<T, NB, EW>), grid, block, 0, st, A, r, p[cur], p[cur ^ 1], q, x, bw, sRR,
-                                       sPQ, sRst, turnPh, (int)TURN_STEP_RESET);
-                    hipLaunchKernelGGL((k_cg_turn<T, NB, EW>), grid, block, 0, st, A, r, p[cur], p[cur ^ 1], q, x, bw, sRR,
-                                       sPQ, sRst, afterPh, (int)TURN_AFTER_RESET);
+                    hipLaunchKernelGGL((k_cg_turn<T, NB, EW, JAC>), grid, block, 0, st, A, r, p[cur], p[cur ^ 1], q, x, bw,
+                                       sRR, sPQ, sRst, turnPh, (int)TURN_STEP_RESET, jac);
+                    hipLaunchKernelGGL((k_cg_turn<T, NB, EW, JAC>), grid, block, 0, st, A, r, p[cur], p[cur ^ 1], q, x, bw,
+                                       sRR, sPQ, sRst, afterPh, (int)TURN_AFTER_RESET, jac);
                 } else {
                     PP_TIMED(PP_K_CG_DIRECTION, n, st,
-                        hipLaunchKernelGGL((k_cg_turn<T, NB, EW>), grid, block, 0, st, A, r, p[cur], p[cur ^ 1], q, x, bw,
-                                           sRR, sPQ, sRst, turnPh, (int)TURN_STEP));
-                }));
-        cur ^= 1;
+                        hipLaunchKernelGGL((k_cg_turn<T, NB, EW, JAC>), grid, block, 0, st, A, r, p[cur], p[cur ^ 1], q, x,
+                                           bw, sRR, sPQ, sRst, turnPh, (int)TURN_STEP, jac));
+                })));
     };
     int last = 0;
     bool zero = false;
@@ -1319,36 +1466,42 @@ static int cg_large(const Csr<T> &A, const T *b, T *x, int B, int maxit, const d
 // ---------------- BiCGStab (large systems) ----------------
 template <typename T>
 static int bicg_large(const Csr<T> &A, const T *b, T *x, int B, int maxit, const double *tols, int ntols, double scale,
-                      hipStream_t s, pp_solve_result *res) {
-    Space &sp = g_space[1];
+                      int jacobi, Space &sp, hipStream_t s, pp_solve_result *res) {
     const int n = A.n;
     const int NBv = std::min(A.rpm, 4);
-    const int rows = ((B + A.rpm - 1) / A.rpm) * ((A.rpm + NBv - 1) / NBv);
+    const int numMat = (B + A.rpm - 1) / A.rpm;
+    const int rows = numMat * ((A.rpm + NBv - 1) / NBv);
     const int G = phase_width(n, std::max(rows, B));
     enum { V_R = 0, V_R0, V_P, V_V, V_T, V_COUNT };
+    const size_t vecBytes = sizeof(T) * ((size_t)V_COUNT * n * B + (size_t)numMat * n);
     const size_t partDoubles = (size_t)B * G * (3 + 1 + 3);
-    if (int e = prepare(sp, sizeof(T) * (size_t)V_COUNT * n * B, partDoubles, B, tols, ntols, s)) return e;
+    if (int e = prepare(sp, vecBytes, partDoubles, B, tols, ntols, s)) return e;
     T *vec = reinterpret_cast<T *>(sp.vec);
     auto V = [&](int k) { return vec + (size_t)k * n * B; };
     T *r = V(V_R), *r0 = V(V_R0), *p = V(V_P), *v = V(V_V), *t = V(V_T);
+    const Jac<T> jac{V(V_COUNT), A.valStride ? n : 0};
     PartsCarver pc{reinterpret_cast<double *>(sp.parts)};
     const Parts sRR = pc.take(B, 3, G), sV = pc.take(B, 1, G), sT = pc.take(B, 3, G);
     SolveRec *ra = recs_a(sp), *rb = recs_b(sp, B);
     const dim3 grid(G, rows), gridB(G, B), block(WG);
     hipLaunchKernelGGL(k_init_recs, dim3(1), dim3(64), 0, s, ra, B, tols[0], ntols > 1 ? (const double *)sp.tols : nullptr, maxit);
+    if (jacobi) launch_jacobi<T>(A, numMat, V(V_COUNT), s);
     PP_NB_CASE(NBv, PP_EW_CASE(A.ell_w,
-        hipLaunchKernelGGL((k_open<T, NB, EW>), grid, block, 0, s, A, b, Parts{nullptr, 1, G}, (T *)nullptr, x, r, r0, B, sRR)));
+        hipLaunchKernelGGL((k_open<T, NB, EW, false>), grid, block, 0, s, A, b, Parts{nullptr, 1, G}, (T *)nullptr, x, r, r0, B,
+                           sRR, jac)));
     hipLaunchKernelGGL((k_bicg_dir<T>), gridB, block, 0, s, n, r, p, v, sRR, Phase{ra, rb, B, scale, 0}, 1);
     PP_LAUNCH_CHECK();
     const Phase useB{rb, nullptr, B, scale, 0}, commit{rb, ra, B, scale, 0}, decide{ra, rb, B, scale, 0};
-    auto body = [&](hipStream_t st) {
-        PP_NB_CASE(NBv, PP_EW_CASE(A.ell_w,
-            PP_TIMED(PP_K_BICG_V, n, st,
-                hipLaunchKernelGGL((k_bicg_apply<T, NB, EW>), grid, block, 0, st, A, p, v, r0, sV, useB));
-            PP_TIMED(PP_K_BICG_T, n, st,
-                hipLaunchKernelGGL((k_bicg_stab<T, NB, EW>), grid, block, 0, st, A, r, v, t, sV, sT, useB))));
-        PP_TIMED(PP_K_BICG_UPDATE, n, st,
-            hipLaunchKernelGGL((k_bicg_step<T>), gridB, block, 0, st, n, r, r0, p, v, t, x, sV, sT, sRR, commit));
+    auto body = [&](hipStream_t st, int) {
+        PP_JAC_CASE(jacobi,
+            PP_NB_CASE(NBv, PP_EW_CASE(A.ell_w,
+                PP_TIMED(PP_K_BICG_V, n, st,
+                    hipLaunchKernelGGL((k_bicg_apply<T, NB, EW, JAC>), grid, block, 0, st, A, p, v, r0, sV, useB, jac));
+                PP_TIMED(PP_K_BICG_T, n, st,
+                    hipLaunchKernelGGL((k_bicg_stab<T, NB, EW, JAC>), grid, block, 0, st, A, r, v, t, sV, sT, useB, jac))));
+            PP_TIMED(PP_K_BICG_UPDATE, n, st,
+                hipLaunchKernelGGL((k_bicg_step<T, JAC>), gridB, block, 0, st, n, A.rpm, r, r0, p, v, t, x, sV, sT, sRR,
+                                   commit, jac)));
         PP_TIMED(PP_K_BICG_P, n, st, hipLaunchKernelGGL((k_bicg_dir<T>), gridB, block, 0, st, n, r, p, v, sRR, decide, 0));
     };
     int last = 0;
@@ -1368,18 +1521,22 @@ static int bicg_large(const Csr<T> &A, const T *b, T *x, int B, int maxit, const
 // ---------------- small systems ----------------
 template <typename T>
 static int small_solve(const Csr<T> &A, const T *b, T *x, int B, int maxit, const double *tols, int ntols, double scale,
-                       int resetEvery, int returnBest, int project, int bicg, hipStream_t s, pp_solve_result *res) {
-    Space &sp = g_space[bicg ? 1 : 0];
+                       int resetEvery, int returnBest, int project, int bicg, int jacobi, Space &sp, hipStream_t s,
+                       pp_solve_result *res) {
     const int n = A.n;
-    if (int e = prepare(sp, sizeof(T) * (size_t)SMALL_SLOTS * n * B, 8, B, tols, ntols, s)) return e;
+    const int numMat = (B + A.rpm - 1) / A.rpm;
+    if (int e = prepare(sp, sizeof(T) * ((size_t)SMALL_SLOTS * n * B + (size_t)numMat * n), 8, B, tols, ntols, s)) return e;
     SolveRec *ra = recs_a(sp);
+    T *dinv = jacobi ? reinterpret_cast<T *>(sp.vec) + (size_t)SMALL_SLOTS * n * B : nullptr;
     hipLaunchKernelGGL(k_init_recs, dim3(1), dim3(64), 0, s, ra, B, tols[0], ntols > 1 ? (const double *)sp.tols : nullptr, maxit);
+    if (jacobi) launch_jacobi<T>(A, numMat, dinv, s);
     SmallArgs a;
     a.n = n; a.B = B; a.rpm = A.rpm; a.maxit = maxit; a.normalized = 1; a.returnBest = returnBest; a.project = project;
     a.resetEvery = resetEvery; a.bicg = bicg; a.normScale = scale;
+    a.dinvStride = A.valStride ? n : 0;
     a.nb_per_group = std::min(A.rpm, 4);
-    const int groups = ((B + A.rpm - 1) / A.rpm) * ((A.rpm + a.nb_per_group - 1) / a.nb_per_group);
-    PP_TIMED(bicg ? PP_K_BICG_SMALL : PP_K_CG_SMALL, n, s, launch_small<T>(A, b, x, sp.vec, ra, a, groups, s));
+    const int groups = numMat * ((A.rpm + a.nb_per_group - 1) / a.nb_per_group);
+    PP_TIMED(bicg ? PP_K_BICG_SMALL : PP_K_CG_SMALL, n, s, launch_small<T>(A, b, x, sp.vec, ra, dinv, a, groups, s));
     PP_LAUNCH_CHECK();
     SolveRec *host = reinterpret_cast<SolveRec *>(sp.snap);
     PP_HIP(hipMemcpyAsync(host, ra, sizeof(SolveRec) * B, hipMemcpyDeviceToHost, s));
@@ -1450,7 +1607,7 @@ template <typename T, typename H>
 static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pinv, int nc, int coarseBatched, const T *b,
                    T *x, int B, int maxit, const double *tols, int ntols, double scale, int returnBest, int project,
                    int pre, int post, double omegaD, hipStream_t s, pp_solve_result *res) {
-    Space &sp = g_space[2];
+    Space &sp = my_spaces().amg;
     const int n = Af.n;
     pre = std::max(1, pre);
     post = std::max(1, post);
@@ -1662,24 +1819,23 @@ static int amg_pcg(const Csr<T> &Af, const pp_amg_level *lv, int L, const H *pin
     hipLaunchKernelGGL(k_init_recs, dim3(1), dim3(64), 0, s, ra, B, tols[0], ntols > 1 ? (const double *)sp.tols : nullptr, maxit);
     if (project) hipLaunchKernelGGL((k_sum<T>), dim3(G, B), dim3(WG), 0, s, b, n, sSum);
     PP_EW_CASE(Af.ell_w,
-        hipLaunchKernelGGL((k_open<T, 1, EW>), dim3(G, B), dim3(WG), 0, s, Af, b, project ? sSum : Parts{nullptr, 1, G},
-                           (T *)nullptr, x, r, (T *)nullptr, B, sRR));
+        hipLaunchKernelGGL((k_open<T, 1, EW, false>), dim3(G, B), dim3(WG), 0, s, Af, b, project ? sSum : Parts{nullptr, 1, G},
+                           (T *)nullptr, x, r, (T *)nullptr, B, sRR, Jac<T>{nullptr, 0}));
     decide(s, Phase{ra, rb, B, scale, returnBest}, 1);
     PP_LAUNCH_CHECK();
-    int cur2 = 0;
     const Phase dirPh{rb, ra, B, scale, returnBest}, updPh{ra, nullptr, B, scale, returnBest},
         decPh{ra, rb, B, scale, returnBest};
-    auto body = [&](hipStream_t st) {
+    auto body = [&](hipStream_t st, int j) {
+        const int cur2 = j & 1;
         vcycle_rest(st);
         PP_TIMED(PP_K_PCG_DIRECTION, n, st,
             PP_EW_CASE(Af.ell_w,
                 hipLaunchKernelGGL((k_pcg_dir<T, EW>), dim3(G, B), dim3(WG), 0, st, Af, u, p[cur2], p[cur2 ^ 1], q, sPost, sRR,
                                    sPQ, dirPh, project)));
         PP_TIMED(PP_K_PCG_UPDATE, n, st,
-            hipLaunchKernelGGL((k_cg_step<T, 1>), dim3(G, B), dim3(WG), 0, st, n, 1, r, p[cur2 ^ 1], q, x, xbest, sPQ,
-                               Parts{nullptr, 1, G}, sRR, updPh));
+            hipLaunchKernelGGL((k_cg_step<T, 1, false>), dim3(G, B), dim3(WG), 0, st, n, 1, r, p[cur2 ^ 1], q, x, xbest,
+                               sPQ, Parts{nullptr, 2, G}, sRR, updPh, Jac<T>{nullptr, 0}));
         decide(st, decPh, 0);
-        cur2 ^= 1;
     };
     int last = 0;
     auto opening = [&](const SolveRec *) { return false; };
@@ -1725,16 +1881,472 @@ static int probe_sell(const pp_sell &sd, int rows, int cols, int reps, hipStream
     return PP_OK;
 }
 
-} // namespace pp
+// ======================================================================================
+// Segmented solves: independent systems of different sizes in one launch per phase
+// ======================================================================================
+// The instances' operators are the diagonal blocks of one CSR matrix (instance s owns matrix
+// rows seg[s] .. seg[s+1]); a solve of ncomp right-hand sides per instance has K = ncomp * S
+// pieces, piece k = c * S + s with its rows at vector offset base[k] = c * N + seg[s]. Tiles of
+// tileRows rows map workgroups to pieces (a tile never straddles two pieces); partial sums are
+// written per tile, [quantity][tile], and every tile folds the tiles of its own piece, so each
+// piece runs its own recurrence, stopping rules and record, while one launch per phase covers
+// all of them. The first tile of a piece writes its record.
+namespace seg {
 
-using namespace pp;
+struct Pieces {
+    const int32_t *base, *len, *mrow; // [K]
+    const int32_t *tilePiece, *tileFirst; // [T]
+    const int32_t *tileOff;           // [K + 1]
+    int K, T, tileRows, N;
+};
 
-extern "C" {
+struct SegParts {
+    double *p; // [nq][T]
+    int nq, T;
+    __device__ __forceinline__ double *at(int q) const { return p + (size_t)q * T; }
+};
 
-int pp_solve_linear(const pp_csr *A, const void *rhs, void *x, int32_t nrhs, int32_t dtype, int32_t maxit,
-                    const double *tols, int32_t ntols, int32_t criterion, int32_t use_bicg, int32_t matrix_rank_deficient,
-                    int32_t residual_reset_steps, int32_t transpose_a, int32_t return_best, void *stream,
-                    pp_solve_result *results) {
+template <int NQ>
+__device__ __forceinline__ void put(const SegParts &P, double (&acc)[NQ], double *lds) {
+    block_sum<NQ>(acc, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) P.at(q)[blockIdx.x] = acc[q];
+}
+
+// Totals over the tiles t0 .. t1-1 of quantities q0.., the same bits in every tile.
+template <int NQ>
+__device__ __forceinline__ void fold(const SegParts &P, int q0, int t0, int t1, double (&out)[NQ], double *lds) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double s = 0.0;
+        const double *src = P.at(q0 + q);
+        for (int t = t0 + threadIdx.x; t < t1; t += WG) s += src[t];
+        out[q] = s;
+    }
+    block_sum<NQ>(out, lds);
+    __shared__ double share[8];
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) share[q] = out[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) out[q] = share[q];
+    __syncthreads();
+}
+
+// This tile: its piece, the piece's tile range, the rows it covers and the vector offset.
+struct Tile {
+    int k, t0, t1, first, last, base, mrow, len;
+    bool lead;
+};
+__device__ __forceinline__ Tile tile_of(const Pieces &P) {
+    Tile t;
+    t.k = P.tilePiece[blockIdx.x];
+    t.t0 = P.tileOff[t.k];
+    t.t1 = P.tileOff[t.k + 1];
+    t.first = P.tileFirst[blockIdx.x];
+    t.len = P.len[t.k];
+    t.last = min(t.first + P.tileRows, t.len);
+    t.base = P.base[t.k];
+    t.mrow = P.mrow[t.k];
+    t.lead = t.first == 0;
+    return t;
+}
+
+#define PP_TILE_ROWS(i, t) for (int i = (t).first + (int)threadIdx.x; i < (t).last; i += WG)
+
+// Entries of matrix row gr (ELL copy when present) with their vector columns.
+template <typename T, typename F>
+__device__ __forceinline__ void seg_row(const Csr<T> &A, int N, int gr, F f) {
+    if (A.ell_w > 0) {
+        for (int j = 0; j < A.ell_w; ++j) f(A.ell_col[(size_t)j * N + gr], A.ell_val[(size_t)j * N + gr]);
+    } else {
+        const int end = A.row[gr + 1];
+        for (int e = A.row[gr]; e < end; ++e) f(A.col[e], A.val[e]);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(WG) k_sum(const T *__restrict__ v, Pieces P, SegParts out) {
+    __shared__ double lds[16];
+    const Tile t = tile_of(P);
+    double acc[1] = {0.0};
+    PP_TILE_ROWS(i, t) acc[0] += double(v[t.base + i]);
+    put<1>(out, acc, lds);
+}
+
+// r = bw - A x (bw = b minus the piece's mean with bSum), r0 = r for BiCGStab; partials
+// (r, r), (bw, bw), sum(r).
+template <typename T>
+__global__ void __launch_bounds__(WG) k_open(Csr<T> A, const T *__restrict__ b, SegParts bSum, T *__restrict__ bwOut,
+                                             const T *__restrict__ x, T *__restrict__ r, T *__restrict__ r0, Pieces P,
+                                             SegParts out) {
+    __shared__ double lds[16 * 3];
+    const Tile t = tile_of(P);
+    double mean[1] = {0.0};
+    if (bSum.p) {
+        fold<1>(bSum, 0, t.t0, t.t1, mean, lds);
+        mean[0] /= double(t.len);
+    }
+    double acc[3] = {0.0, 0.0, 0.0};
+    PP_TILE_ROWS(i, t) {
+        T ax = T(0);
+        seg_row(A, P.N, t.mrow + i, [&](int c, T a) { ax += a * x[t.base + (c - t.mrow)]; });
+        const int o = t.base + i;
+        const T bi = bSum.p ? b[o] - T(mean[0]) : b[o];
+        if (bwOut) bwOut[o] = bi;
+        const T ri = bi - ax;
+        r[o] = ri;
+        if (r0) r0[o] = ri;
+        acc[0] += double(ri) * double(ri);
+        acc[1] += double(bi) * double(bi);
+        acc[2] += double(ri);
+    }
+    put<3>(out, acc, lds);
+}
+
+// CG: decide on the folded (r, r) (open: the initial residual), then p' = r + beta pIn (r when
+// fresh) formed at the neighbours, q = A p', partials (p', q).
+template <typename T>
+__global__ void __launch_bounds__(WG) k_cg_turn(Csr<T> A, const T *__restrict__ r, const T *__restrict__ pIn,
+                                                T *__restrict__ pOut, T *__restrict__ q, Pieces P, SegParts rrIn,
+                                                SegParts pqOut, const SolveRec *in, SolveRec *out, int open) {
+    __shared__ double lds[16 * 3];
+    const Tile t = tile_of(P);
+    SolveRec R = in[t.k];
+    bool act = rec_active(R);
+    double tot[2];
+    fold<2>(rrIn, 0, t.t0, t.t1, tot, lds);
+    T beta = T(0);
+    const double scale = 1.0 / sqrt(double(t.len));
+    if (act) {
+        if (open) {
+            rule_open(R, tot[0], scale, false);
+            R.zero = tot[1] == 0.0;
+        } else {
+            rule_step(R, tot[0], scale, false);
+            if (rec_active(R)) beta = T(rule_cg_beta(R, tot[0]));
+        }
+        act = rec_active(R);
+    }
+    if (t.lead && threadIdx.x == 0) out[t.k] = R;
+    double acc[1] = {0.0};
+    if (act) {
+        const bool fresh = R.flags & R_FRESH;
+        PP_TILE_ROWS(i, t) {
+            T ap = T(0);
+            seg_row(A, P.N, t.mrow + i, [&](int c, T a) {
+                const int o = t.base + (c - t.mrow);
+                ap += a * (fresh ? r[o] : r[o] + beta * pIn[o]);
+            });
+            const int o = t.base + i;
+            const T pi = fresh ? r[o] : r[o] + beta * pIn[o];
+            pOut[o] = pi;
+            q[o] = ap;
+            acc[0] += double(pi) * double(ap);
+        }
+    }
+    put<1>(pqOut, acc, lds);
+}
+
+// CG: alpha from the folded (p', q); x += alpha p', r -= alpha q; partials (r, r), 0, sum(r);
+// the lead tile commits the record.
+template <typename T>
+__global__ void __launch_bounds__(WG) k_cg_step(T *__restrict__ r, const T *__restrict__ p, const T *__restrict__ q,
+                                                T *__restrict__ x, Pieces P, SegParts pqIn, SegParts rrOut,
+                                                const SolveRec *in, SolveRec *out) {
+    __shared__ double lds[16 * 3];
+    const Tile t = tile_of(P);
+    SolveRec R = in[t.k];
+    bool act = rec_active(R);
+    double pq[1];
+    fold<1>(pqIn, 0, t.t0, t.t1, pq, lds);
+    T alpha = T(0);
+    if (act) {
+        alpha = T(rule_alpha(R, pq[0]));
+        R.flags &= ~R_FRESH;
+        act = rec_active(R);
+    }
+    if (t.lead && threadIdx.x == 0) out[t.k] = R;
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (act)
+        PP_TILE_ROWS(i, t) {
+            const int o = t.base + i;
+            x[o] += alpha * p[o];
+            const T ri = r[o] - alpha * q[o];
+            r[o] = ri;
+            acc[0] += double(ri) * double(ri);
+            acc[2] += double(ri);
+        }
+    put<3>(rrOut, acc, lds);
+}
+
+// BiCGStab: decide (open: initial residual; otherwise the full step's (r0, r), (r, r)), then
+// p = r + beta (p - omega v) row by row.
+template <typename T>
+__global__ void __launch_bounds__(WG) k_bi_dir(const T *__restrict__ r, T *__restrict__ p, const T *__restrict__ v,
+                                               Pieces P, SegParts rrIn, const SolveRec *in, SolveRec *out, int open) {
+    __shared__ double lds[16 * 2];
+    const Tile t = tile_of(P);
+    SolveRec R = in[t.k];
+    bool act = rec_active(R);
+    double tot[2];
+    fold<2>(rrIn, 0, t.t0, t.t1, tot, lds);
+    const double scale = 1.0 / sqrt(double(t.len));
+    T beta = T(0);
+    if (act) {
+        if (open) {
+            rule_open(R, tot[0], scale, true);
+            R.zero = tot[1] == 0.0;
+        } else {
+            beta = T(rule_bicg_full(R, tot[0], tot[1], scale));
+        }
+        act = rec_active(R);
+    }
+    if (t.lead && threadIdx.x == 0) out[t.k] = R;
+    if (!act) return;
+    const bool fresh = R.flags & R_FRESH;
+    const T omega = T(R.omega);
+    PP_TILE_ROWS(i, t) {
+        const int o = t.base + i;
+        p[o] = fresh ? r[o] : r[o] + beta * (p[o] - omega * v[o]);
+    }
+}
+
+// v = A p; partials (r0, v).
+template <typename T>
+__global__ void __launch_bounds__(WG) k_bi_apply(Csr<T> A, const T *__restrict__ p, T *__restrict__ v,
+                                                 const T *__restrict__ r0, Pieces P, SegParts out, const SolveRec *in) {
+    __shared__ double lds[16];
+    const Tile t = tile_of(P);
+    double acc[1] = {0.0};
+    if (rec_active(in[t.k]))
+        PP_TILE_ROWS(i, t) {
+            T ap = T(0);
+            seg_row(A, P.N, t.mrow + i, [&](int c, T a) { ap += a * p[t.base + (c - t.mrow)]; });
+            const int o = t.base + i;
+            v[o] = ap;
+            acc[0] += double(r0[o]) * double(ap);
+        }
+    put<1>(out, acc, lds);
+}
+
+// alpha from the folded (r0, v); s = r - alpha v at the neighbours, t = A s; partials
+// (t, s), (t, t), (s, s).
+template <typename T>
+__global__ void __launch_bounds__(WG) k_bi_stab(Csr<T> A, const T *__restrict__ r, const T *__restrict__ v,
+                                                T *__restrict__ tv, Pieces P, SegParts r0vIn, SegParts out,
+                                                const SolveRec *in) {
+    __shared__ double lds[16 * 3];
+    const Tile t = tile_of(P);
+    SolveRec R = in[t.k];
+    double r0v[1];
+    fold<1>(r0vIn, 0, t.t0, t.t1, r0v, lds);
+    T alpha = T(0);
+    bool act = rec_active(R);
+    if (act) {
+        alpha = T(rule_alpha(R, r0v[0]));
+        act = rec_active(R);
+    }
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (act)
+        PP_TILE_ROWS(i, t) {
+            T as = T(0);
+            seg_row(A, P.N, t.mrow + i, [&](int c, T a) {
+                const int o = t.base + (c - t.mrow);
+                as += a * (r[o] - alpha * v[o]);
+            });
+            const int o = t.base + i;
+            const T si = r[o] - alpha * v[o];
+            tv[o] = as;
+            acc[0] += double(as) * double(si);
+            acc[1] += double(as) * double(as);
+            acc[2] += double(si) * double(si);
+        }
+    put<3>(out, acc, lds);
+}
+
+// alpha, omega from the folded partials; x and r updated; partials (r0, r), (r, r); the lead
+// tile commits the record.
+template <typename T>
+__global__ void __launch_bounds__(WG) k_bi_step(T *__restrict__ r, const T *__restrict__ r0, const T *__restrict__ p,
+                                                const T *__restrict__ v, const T *__restrict__ tv, T *__restrict__ x,
+                                                Pieces P, SegParts r0vIn, SegParts tsIn, SegParts rrOut,
+                                                const SolveRec *in, SolveRec *out) {
+    __shared__ double lds[16 * 3];
+    const Tile t = tile_of(P);
+    SolveRec R = in[t.k];
+    bool act = rec_active(R);
+    double r0v[1], ts[3];
+    fold<1>(r0vIn, 0, t.t0, t.t1, r0v, lds);
+    fold<3>(tsIn, 0, t.t0, t.t1, ts, lds);
+    bool half = false;
+    if (act) {
+        R.alpha = rule_alpha(R, r0v[0]);
+        if (rec_active(R)) R.omega = rule_bicg_half(R, ts[0], ts[1], ts[2], 1.0 / sqrt(double(t.len)));
+        act = rec_active(R);
+        half = act && (R.flags & R_HALF);
+        if (half) rule_bicg_half_met(R);
+        R.flags &= ~R_FRESH;
+    }
+    if (t.lead && threadIdx.x == 0) out[t.k] = R;
+    double acc[2] = {0.0, 0.0};
+    if (act) {
+        const T alpha = T(R.alpha), omega = T(R.omega);
+        PP_TILE_ROWS(i, t) {
+            const int o = t.base + i;
+            const T si = r[o] - alpha * v[o];
+            if (half) {
+                x[o] += alpha * p[o];
+                r[o] = si;
+            } else {
+                x[o] += alpha * p[o] + omega * si;
+                const T ri = si - omega * tv[o];
+                r[o] = ri;
+                acc[0] += double(r0[o]) * double(ri);
+                acc[1] += double(ri) * double(ri);
+            }
+        }
+    }
+    put<2>(rrOut, acc, lds);
+}
+
+template <typename T>
+__global__ void k_fill_pieces_zero(T *x, Pieces P, const SolveRec *rec) {
+    const Tile t = tile_of(P);
+    if (!rec[t.k].zero) return;
+    PP_TILE_ROWS(i, t) x[t.base + i] = T(0);
+}
+
+} // namespace seg
+
+// Host side of the segmented solve: tiles built from the pieces, uploaded with one copy.
+struct SegLayout {
+    std::vector<int32_t> host; // base[K] len[K] mrow[K] tileOff[K+1] tilePiece[T] tileFirst[T]
+    int K = 0, T = 0;
+};
+
+static SegLayout seg_layout(const int32_t *segRows, int S, int ncomp, int N, int tileRows) {
+    SegLayout L;
+    L.K = S * ncomp;
+    std::vector<int32_t> base(L.K), len(L.K), mrow(L.K), off(L.K + 1, 0), piece, first;
+    for (int c = 0; c < ncomp; ++c)
+        for (int s = 0; s < S; ++s) {
+            const int k = c * S + s;
+            base[k] = c * N + segRows[s];
+            len[k] = segRows[s + 1] - segRows[s];
+            mrow[k] = segRows[s];
+            const int tiles = std::max(1, (len[k] + tileRows - 1) / tileRows);
+            off[k + 1] = off[k] + tiles;
+            for (int j = 0; j < tiles; ++j) {
+                piece.push_back(k);
+                first.push_back(j * tileRows);
+            }
+        }
+    L.T = (int)piece.size();
+    for (auto *v : {&base, &len, &mrow, &off, &piece, &first}) L.host.insert(L.host.end(), v->begin(), v->end());
+    return L;
+}
+
+
+template <typename T>
+static int seg_solve(const Csr<T> &A, const int32_t *segRows, int S, int ncomp, const T *b, T *x, int maxit,
+                     const double *tols, int ntols, int useBicg, int project, int tileRows, hipStream_t s,
+                     pp_solve_result *res) {
+    const int N = A.n;
+    SegLayout L = seg_layout(segRows, S, ncomp, N, tileRows);
+    const int K = L.K, Tt = L.T;
+    SegSpace &g_seg = my_spaces().seg;
+    Space &sp = g_seg.sp;
+    const size_t len = (size_t)ncomp * N;
+    const int nvec = useBicg ? 5 : 5;
+    const size_t partDoubles = (size_t)Tt * (useBicg ? (3 + 1 + 3) : (1 + 3 + 1));
+    if (int e = prepare(sp, sizeof(T) * nvec * len, partDoubles, K, tols, ntols, s)) return e;
+    // layout upload through a pinned staging copy (rewritten only after this solve's final wait)
+    const size_t lbytes = sizeof(int32_t) * L.host.size();
+    if (g_seg.layoutHostCap < lbytes) {
+        if (g_seg.layoutHost) (void)hipHostFree(g_seg.layoutHost);
+        g_seg.layoutHost = nullptr;
+        PP_HIP(hipHostMalloc((void **)&g_seg.layoutHost, lbytes, hipHostMallocDefault));
+        g_seg.layoutHostCap = lbytes;
+    }
+    if (int e = grow((void **)&g_seg.layoutDev, &g_seg.layoutCap, lbytes)) return e;
+    std::memcpy(g_seg.layoutHost, L.host.data(), lbytes);
+    PP_HIP(hipMemcpyAsync(g_seg.layoutDev, g_seg.layoutHost, lbytes, hipMemcpyHostToDevice, s));
+    const int32_t *ld = g_seg.layoutDev;
+    seg::Pieces P{ld, ld + K, ld + 2 * K, ld + 4 * K + 1, ld + 4 * K + 1 + Tt, ld + 3 * K, K, Tt, tileRows, N};
+    // everything the iteration kernels are given, for the captured chunks
+    uint64_t layoutHash = 1469598103934665603ull;
+    for (int32_t v : L.host) layoutHash = (layoutHash ^ (uint32_t)v) * 1099511628211ull;
+    const std::vector<uintptr_t> key = {(uintptr_t)A.row, (uintptr_t)A.col, (uintptr_t)A.val, (uintptr_t)A.ell_col,
+                                        (uintptr_t)A.ell_val, (uintptr_t)A.ell_w, (uintptr_t)b, (uintptr_t)x,
+                                        (uintptr_t)sp.vec, (uintptr_t)sp.parts, (uintptr_t)sp.recs, (uintptr_t)ld,
+                                        (uintptr_t)layoutHash, (uintptr_t)K, (uintptr_t)Tt, (uintptr_t)N,
+                                        (uintptr_t)tileRows, (uintptr_t)useBicg, (uintptr_t)project, sizeof(T)};
+    T *vec = reinterpret_cast<T *>(sp.vec);
+    auto V = [&](int k) { return vec + (size_t)k * len; };
+    double *pd = reinterpret_cast<double *>(sp.parts);
+    size_t used = 0;
+    auto take = [&](int nq) { seg::SegParts q{pd + used, nq, Tt}; used += (size_t)nq * Tt; return q; };
+    SolveRec *ra = recs_a(sp), *rb = recs_b(sp, K);
+    hipLaunchKernelGGL(k_init_recs, dim3(1), dim3(256), 0, s, ra, K, tols[0], ntols > 1 ? (const double *)sp.tols : nullptr, maxit);
+    const dim3 grid(Tt), block(WG);
+    int last = 0;
+    bool zero = false;
+    auto opening = [&](const SolveRec *recs) {
+        zero = false;
+        for (int k = 0; k < K; ++k) zero |= recs[k].zero != 0;
+        return false;
+    };
+    if (!useBicg) {
+        T *r = V(0), *q = V(1), *p[2] = {V(2), V(3)}, *bw = project ? V(4) : const_cast<T *>(b);
+        const seg::SegParts sSum = take(1), sRR = take(3), sPQ = take(1);
+        if (project) hipLaunchKernelGGL((seg::k_sum<T>), grid, block, 0, s, b, P, sSum);
+        hipLaunchKernelGGL((seg::k_open<T>), grid, block, 0, s, A, b, project ? sSum : seg::SegParts{nullptr, 1, Tt},
+                           project ? bw : (T *)nullptr, x, r, (T *)nullptr, P, sRR);
+        hipLaunchKernelGGL((seg::k_cg_turn<T>), grid, block, 0, s, A, r, p[0], p[1], q, P, sRR, sPQ, ra, rb, 1);
+        PP_LAUNCH_CHECK();
+        auto body = [&](hipStream_t st, int j) {
+            const int cur = 1 ^ (j & 1); // the opening turn wrote p[1]
+            PP_TIMED(PP_K_SEG_CG_UPDATE, N, st,
+                     hipLaunchKernelGGL((seg::k_cg_step<T>), grid, block, 0, st, r, p[cur], q, x, P, sPQ, sRR, rb, ra));
+            PP_TIMED(PP_K_SEG_CG_DIRECTION, N, st,
+                     hipLaunchKernelGGL((seg::k_cg_turn<T>), grid, block, 0, st, A, r, p[cur], p[cur ^ 1], q, P, sRR, sPQ, ra,
+                                        rb, 0));
+        };
+        if (int e = pace_solve(sp, s, rb, K, maxit + 2, Pace{g_chunk_cap, 0, true}, &last, opening, body, &key)) return e;
+    } else {
+        T *r = V(0), *r0 = V(1), *p = V(2), *v = V(3), *tv = V(4);
+        const seg::SegParts sRR = take(3), sV = take(1), sT = take(3);
+        hipLaunchKernelGGL((seg::k_open<T>), grid, block, 0, s, A, b, seg::SegParts{nullptr, 1, Tt}, (T *)nullptr, x, r, r0,
+                           P, sRR);
+        hipLaunchKernelGGL((seg::k_bi_dir<T>), grid, block, 0, s, r, p, v, P, sRR, ra, rb, 1);
+        PP_LAUNCH_CHECK();
+        auto body = [&](hipStream_t st, int) {
+            PP_TIMED(PP_K_SEG_BICG, N, st,
+                hipLaunchKernelGGL((seg::k_bi_apply<T>), grid, block, 0, st, A, p, v, r0, P, sV, rb);
+                hipLaunchKernelGGL((seg::k_bi_stab<T>), grid, block, 0, st, A, r, v, tv, P, sV, sT, rb);
+                hipLaunchKernelGGL((seg::k_bi_step<T>), grid, block, 0, st, r, r0, p, v, tv, x, P, sV, sT, sRR, rb, ra);
+                hipLaunchKernelGGL((seg::k_bi_dir<T>), grid, block, 0, st, r, p, v, P, sRR, ra, rb, 0));
+        };
+        if (int e = pace_solve(sp, s, rb, K, maxit + 2, Pace{g_chunk_cap, 0, true}, &last, opening, body, &key)) return e;
+    }
+    // pieces whose right-hand side is exactly zero get x = 0 (converged, no iterations)
+    const SolveRec *recs = slot_recs(sp, last, K);
+    if (zero) hipLaunchKernelGGL((seg::k_fill_pieces_zero<T>), grid, block, 0, s, x, P, rb);
+    PP_LAUNCH_CHECK();
+    report(recs, K, 0, res);
+    for (int k = 0; k < K; ++k)
+        if (recs[k].zero) res[k] = pp_solve_result{0.0, 0, 1, 1, F_MET};
+    return PP_OK;
+}
+
+// The body of pp_solve_linear (spaces: the CG and the BiCGStab workspaces).
+static int solve_linear(const pp_csr *A, const void *rhs, void *x, int32_t nrhs, int32_t dtype, int32_t maxit,
+                        const double *tols, int32_t ntols, int32_t criterion, int32_t use_bicg,
+                        int32_t matrix_rank_deficient, int32_t residual_reset_steps, int32_t transpose_a,
+                        int32_t return_best, void *stream, pp_solve_result *results, Space &cgSpace, Space &bicgSpace) {
     PP_CHECK_ARG(A && A->row && A->col && A->val && rhs && x && tols && results, "null argument");
     PP_CHECK_ARG(A->n > 0 && nrhs > 0, "empty system");
     PP_CHECK_ARG(A->num_matrices >= 1 && nrhs % A->num_matrices == 0,
@@ -1743,14 +2355,15 @@ int pp_solve_linear(const pp_csr *A, const void *rhs, void *x, int32_t nrhs, int
     PP_CHECK_ARG(criterion == PP_CONV_NORM2 || criterion == PP_CONV_NORM2_NORMALIZED, "unsupported convergence criterion");
     PP_CHECK_ARG(maxit >= 0, "maxit must be non-negative");
     PP_CHECK_ARG(!(use_bicg && matrix_rank_deficient), "rank-deficient mode is a CG mode");
-    std::lock_guard<std::mutex> lock(g_lock);
     hipStream_t s = as_stream(stream);
     const int rpm = nrhs / A->num_matrices;
     const double scale = criterion == PP_CONV_NORM2_NORMALIZED ? 1.0 / std::sqrt(double(A->n)) : 1.0;
     if (use_bicg) return_best = 0;
-    static void *tbuf = nullptr;
-    static size_t tcap = 0;
-    static int *tbad = nullptr;
+    const int jacobi = use_bicg ? g_bicg_jacobi : g_cg_jacobi;
+    ThreadSpaces &ts = my_spaces();
+    void *&tbuf = ts.tbuf;
+    size_t &tcap = ts.tcap;
+    int *&tbad = ts.tbad;
     PP_DISPATCH(dtype, {
         Csr<T> C = csr_view<T>(A, rpm);
         if (transpose_a && use_bicg) {
@@ -1768,12 +2381,73 @@ int pp_solve_linear(const pp_csr *A, const void *rhs, void *x, int32_t nrhs, int
         }
         const T *b = reinterpret_cast<const T *>(rhs);
         T *xx = reinterpret_cast<T *>(x);
+        Space &sp = use_bicg ? bicgSpace : cgSpace;
         if (A->n <= SMALL_N)
             return small_solve<T>(C, b, xx, nrhs, maxit, tols, ntols, scale, use_bicg ? 0 : residual_reset_steps,
-                                  return_best, matrix_rank_deficient, use_bicg, s, results);
-        if (use_bicg) return bicg_large<T>(C, b, xx, nrhs, maxit, tols, ntols, scale, s, results);
+                                  return_best, matrix_rank_deficient, use_bicg, jacobi, sp, s, results);
+        if (use_bicg) return bicg_large<T>(C, b, xx, nrhs, maxit, tols, ntols, scale, jacobi, sp, s, results);
         return cg_large<T>(C, b, xx, nrhs, maxit, tols, ntols, scale, residual_reset_steps, return_best,
-                           matrix_rank_deficient, s, results);
+                           matrix_rank_deficient, jacobi, sp, s, results);
+    });
+    return PP_OK;
+}
+
+} // namespace pp
+
+using namespace pp;
+
+extern "C" {
+
+int pp_solve_linear(const pp_csr *A, const void *rhs, void *x, int32_t nrhs, int32_t dtype, int32_t maxit,
+                    const double *tols, int32_t ntols, int32_t criterion, int32_t use_bicg, int32_t matrix_rank_deficient,
+                    int32_t residual_reset_steps, int32_t transpose_a, int32_t return_best, void *stream,
+                    pp_solve_result *results) {
+    ThreadSpaces &ts = my_spaces();
+    return solve_linear(A, rhs, x, nrhs, dtype, maxit, tols, ntols, criterion, use_bicg, matrix_rank_deficient,
+                        residual_reset_steps, transpose_a, return_best, stream, results, ts.cg, ts.bicg);
+}
+
+int pp_set_krylov_settings(int32_t check_interval, int32_t use_graphs, int32_t cg_jacobi, int32_t bicg_jacobi) {
+    PP_CHECK_ARG(check_interval >= 1, "checkInterval must be positive");
+    std::lock_guard<std::mutex> lock(g_lock);
+    g_chunk_cap = check_interval;
+    if (!use_graphs && g_use_graphs) {
+        std::lock_guard<std::mutex> glock(g_graph_lock);
+        drop_graphs();
+    }
+    g_use_graphs = use_graphs ? 1 : 0;
+    g_cg_jacobi = cg_jacobi ? 1 : 0;
+    g_bicg_jacobi = bicg_jacobi ? 1 : 0;
+    return PP_OK;
+}
+
+int pp_get_krylov_settings(int32_t *check_interval, int32_t *use_graphs, int32_t *cg_jacobi, int32_t *bicg_jacobi) {
+    std::lock_guard<std::mutex> lock(g_lock);
+    if (check_interval) *check_interval = g_chunk_cap;
+    if (use_graphs) *use_graphs = g_use_graphs;
+    if (cg_jacobi) *cg_jacobi = g_cg_jacobi;
+    if (bicg_jacobi) *bicg_jacobi = g_bicg_jacobi;
+    return PP_OK;
+}
+
+int pp_solve_segmented(const pp_csr *A, const int32_t *seg_rows, int32_t nseg, int32_t ncomp, const void *rhs, void *x,
+                       int32_t dtype, int32_t maxit, const double *tols, int32_t ntols, int32_t use_bicg,
+                       int32_t project, int32_t tile_rows, void *stream, pp_solve_result *results) {
+    PP_CHECK_ARG(A && A->row && A->col && A->val && seg_rows && rhs && x && tols && results, "null argument");
+    PP_CHECK_ARG(A->num_matrices == 1, "a segmented solve takes one block-diagonal matrix");
+    PP_CHECK_ARG(nseg >= 1 && ncomp >= 1, "need at least one instance and one component");
+    PP_CHECK_ARG(seg_rows[0] == 0 && seg_rows[nseg] == A->n, "instance rows must cover the matrix rows");
+    for (int s = 0; s < nseg; ++s) PP_CHECK_ARG(seg_rows[s + 1] > seg_rows[s], "instance row ranges must be non-empty and ascending");
+    PP_CHECK_ARG(ntols == 1 || ntols == nseg * ncomp, "tolerance must be a scalar or one value per instance and component");
+    PP_CHECK_ARG(maxit >= 0, "maxit must be non-negative");
+    PP_CHECK_ARG(!(use_bicg && project), "the projected (rank-deficient) mode is a CG mode");
+    if (tile_rows <= 0) tile_rows = 1024;
+    PP_CHECK_ARG(tile_rows % WG == 0 && tile_rows <= (1 << 16), "tile_rows must be a multiple of 256, at most 65536");
+    
+    PP_DISPATCH(dtype, {
+        const Csr<T> C = csr_view<T>(A, 1);
+        return seg_solve<T>(C, seg_rows, nseg, ncomp, reinterpret_cast<const T *>(rhs), reinterpret_cast<T *>(x), maxit,
+                            tols, ntols, use_bicg, project, tile_rows, as_stream(stream), results);
     });
     return PP_OK;
 }
@@ -1809,7 +2483,7 @@ int pp_amg_pcg_solve(const pp_csr *A, const pp_amg_level *levels, int32_t num_le
     PP_CHECK_ARG(!levels[0].a_sell.slice_off, "the finest level runs on its ELL copy or its CSR arrays, not a sliced copy");
     PP_CHECK_ARG(levels[num_levels - 1].n_coarse == nc, "coarsest level size mismatch");
     PP_CHECK_ARG(ntols == 0 || ntols == 1 || ntols == nrhs, "tolerance count");
-    std::lock_guard<std::mutex> lock(g_lock);
+    
     hipStream_t s = as_stream(stream);
     const double scale = normalized ? 1.0 / std::sqrt(double(A->n)) : 1.0;
     double t1 = tol;
@@ -1828,13 +2502,27 @@ int pp_amg_pcg_solve(const pp_csr *A, const pp_amg_level *levels, int32_t num_le
 }
 
 int pp_release_workspaces(void) {
+    // frees every thread's workspaces: no solve may be running
+    {
+        std::lock_guard<std::mutex> glock(g_graph_lock);
+        drop_graphs();
+    }
     std::lock_guard<std::mutex> lock(g_lock);
-    for (auto &sp : g_space) {
-        for (void *p : {sp.vec, sp.parts, sp.recs, sp.aux, sp.tols, sp.tailDev})
-            if (p) (void)hipFree(p);
-        if (sp.tailHost) (void)hipHostFree(sp.tailHost);
-        if (sp.snap) (void)hipHostFree(sp.snap);
-        sp = Space();
+    for (ThreadSpaces *ts : g_all_spaces) {
+        if (ts->seg.layoutDev) (void)hipFree(ts->seg.layoutDev);
+        if (ts->seg.layoutHost) (void)hipHostFree(ts->seg.layoutHost);
+        ts->seg.layoutDev = nullptr; ts->seg.layoutHost = nullptr; ts->seg.layoutCap = ts->seg.layoutHostCap = 0;
+        if (ts->tbuf) (void)hipFree(ts->tbuf);
+        if (ts->tbad) (void)hipFree(ts->tbad);
+        ts->tbuf = nullptr; ts->tbad = nullptr; ts->tcap = 0;
+        for (Space *spp : {&ts->cg, &ts->bicg, &ts->amg, &ts->seg.sp}) {
+            Space &sp = *spp;
+            for (void *p : {sp.vec, sp.parts, sp.recs, sp.aux, sp.tols, sp.tailDev})
+                if (p) (void)hipFree(p);
+            if (sp.tailHost) (void)hipHostFree(sp.tailHost);
+            if (sp.snap) (void)hipHostFree(sp.snap);
+            sp = Space();
+        }
     }
     return PP_OK;
 }

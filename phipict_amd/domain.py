@@ -1107,6 +1107,47 @@ class Domain:
         self._initialized = False
         self._changed = True
 
+    # --- independent blocks: uneven-size batching (no reference counterpart)
+    def setIndependentBlocks(self, independent: bool = True) -> None:
+        """Treat every block as an independent instance: the domain then holds instances of
+        different grid sizes side by side (no block may be connected to another), whose linear
+        systems are the diagonal blocks of the domain's matrices and are solved segmented --
+        each instance with its own recurrence, stopping rules, tolerance and mean projection,
+        all of them in one launch per solver phase."""
+        if independent:
+            for b in self._blocks:
+                if any(isinstance(bd, ConnectedBoundary) for bd in b.boundaries):
+                    raise ValueError("independent blocks cannot be connected to each other")
+        self._independent = bool(independent)
+        self._segment_cache = None
+
+    def hasIndependentBlocks(self) -> bool:
+        return bool(getattr(self, "_independent", False))
+
+    def getSegments(self):
+        """Instance row offsets (the blocks' global offsets and the total size) of a domain of
+        independent blocks, else None."""
+        if not self.hasIndependentBlocks():
+            return None
+        if not self._initialized:
+            self.PrepareSolve()
+        return tuple(b.globalOffset for b in self._blocks) + (self.totalSize,)
+
+    def segmentIds(self) -> torch.Tensor:
+        """Instance index of every cell (device, int64), cached."""
+        segs = self.getSegments()
+        cache = getattr(self, "_segment_cache", None)
+        if cache is None or cache[0] != segs:
+            lengths = torch.tensor([segs[i + 1] - segs[i] for i in range(len(segs) - 1)])
+            ids = torch.repeat_interleave(torch.arange(len(segs) - 1), lengths).to(self._device)
+            cache = (segs, ids, lengths.to(device=self._device, dtype=self._dtype))
+            self._segment_cache = cache
+        return cache[1]
+
+    def segmentLengths(self) -> torch.Tensor:
+        self.segmentIds()
+        return self._segment_cache[2]
+
     # --- setup (Domain::PrepareSolve, domain.cpp:426-561)
     def PrepareSolve(self) -> None:
         if not self._blocks:

@@ -15,7 +15,7 @@ import torch
 from . import _lib, ops
 from .domain import Domain
 from .hooks import Hook, Hooks
-from .simulation import Simulation, subtract_env_mean, subtract_env_mean_
+from .simulation import Simulation, center_solution, subtract_env_mean, subtract_env_mean_
 from .solvers import amg as amg_mod
 
 _LOG = logging.getLogger("phipict_amd.mhd")
@@ -60,12 +60,20 @@ class MHDSimulation(Simulation):
             raise ValueError("Pass hooks or its former name prep_fn, not both.")
         hooks = Hooks(hooks if hooks is not None else prep_fn)
         super().__init__(domain=domain, dt=dt, hooks=hooks, **kwargs)
-        if not isinstance(stuart_number, torch.Tensor) or stuart_number.numel() not in (1, domain.getBatchSize()):
-            raise ValueError("stuart_number must be a scalar tensor (or one value per environment).")
-        if potential_use_BiCG:
-            raise NotImplementedError("the potential solve uses CG")
-        if potential_solve_dtype is not None and potential_solve_dtype != domain.getDtype():
-            raise NotImplementedError("a potential solve dtype different from the domain's is not implemented")
+        per_instance = domain.hasIndependentBlocks() and isinstance(stuart_number, torch.Tensor) and \
+            stuart_number.numel() == domain.getNumBlocks() and domain.getBatchSize() == 1
+        if not isinstance(stuart_number, torch.Tensor) or not (stuart_number.numel() in (1, domain.getBatchSize())
+                                                               or per_instance):
+            raise ValueError("stuart_number must be a scalar tensor (or one value per environment, or per instance "
+                             "of a domain of independent blocks).")
+        self._stuart_per_block = per_instance and stuart_number.numel() > 1
+        if potential_solve_dtype is not None and potential_solve_dtype not in (torch.float32, torch.float64):
+            raise ValueError("potential_solve_dtype must be torch.float32 or torch.float64")
+        self._potential_use_BiCG = bool(potential_use_BiCG)
+        # the Poisson system may be solved in another precision than the domain's: up-cast
+        # matrix and right-hand side, solve, cast back (mhd.py:940-949)
+        self._potential_solve_dtype = (potential_solve_dtype if potential_solve_dtype != domain.getDtype() else None)
+        self._epot_cast = None  # (source matrix, its values' version, the cast matrix)
         self._stuart_number = stuart_number
         e_b = torch.as_tensor(e_b, dtype=domain.getDtype())
         if e_b.dim() == 1 and e_b.numel() != 3:
@@ -120,9 +128,10 @@ class MHDSimulation(Simulation):
         J = ops.ComputeCurrentDensityFaceBased(domain, domain.epotResult, self._u_cross_eb_flat)
         dtype, dev = J.dtype, J.device
         eb = self._e_b_device.to(dtype).contiguous()
-        N = self._stuart(J).reshape(-1).contiguous()
+        N_all = self._stuart(J).reshape(-1).contiguous()
         code, stream = _lib.dtype_code(dtype), _lib.stream_ptr(dev)
-        for block in domain.getBlocks():
+        for bi, block in enumerate(domain.getBlocks()):
+            N = N_all[bi:bi + 1] if self._stuart_per_block else N_all
             # F_L = N (J x e_b)[:d] over the block's cells plus its current source, one launch
             cells = block.getCells()
             out = torch.empty(batch, self._dims, *block.getSizesZYX(), dtype=dtype, device=dev)
@@ -140,6 +149,13 @@ class MHDSimulation(Simulation):
             block.setVelocitySource(out)
         domain.UpdateDomainData()
 
+    def _epot_matrix_cast(self, mat):
+        """The potential matrix in the solve dtype, cast once per set of matrix values."""
+        key = (mat.value.data_ptr(), mat.value._version)
+        if self._epot_cast is None or self._epot_cast[0] != key:
+            self._epot_cast = (key, mat.toType(self._potential_solve_dtype))
+        return self._epot_cast[1]
+
     def _epot_initial_guess(self, domain: Domain, rhs: torch.Tensor):
         if not self._potential_reuse_result:
             return None
@@ -151,7 +167,7 @@ class MHDSimulation(Simulation):
         else:
             x = prev.detach().reshape(rhs.shape).to(dtype=rhs.dtype).clone()
         if self._potential_rank_deficient:
-            x = subtract_env_mean_(x, domain.getBatchSize())
+            x = center_solution(domain, x, inplace=True)
         return x
 
     def _get_epot_amg_hierarchy(self, mat):
@@ -241,18 +257,25 @@ class MHDSimulation(Simulation):
             domain.setEpotRHS(rhs)
             domain.UpdateDomainData()
         rhs = domain.epotRHS
+        mat = domain.Epot
+        if self._potential_solve_dtype is not None:
+            mat = self._epot_matrix_cast(mat)
+            rhs = rhs.to(self._potential_solve_dtype)
         if self._potential_rank_deficient:
-            rhs = subtract_env_mean(rhs, batch)
+            rhs = center_solution(domain, rhs)
         x = self._epot_initial_guess(domain, rhs)
-        phi, ok = self.linear_solve(domain.Epot, rhs, x=x, use_BiCG=False, tol=self._potential_tol,
-                                    matrix_rank_deficient=self._potential_rank_deficient,
+        # BiCGStab runs on the projected right-hand side without the CG's projected mode
+        phi, ok = self.linear_solve(mat, rhs, x=x, use_BiCG=self._potential_use_BiCG, tol=self._potential_tol,
+                                    matrix_rank_deficient=self._potential_rank_deficient and not self._potential_use_BiCG,
                                     return_best_result=self._potential_return_best_result,
                                     max_iter=self._potential_solve_max_iter if max_iter is None else max_iter,
-                                    amg_hierarchy=(self._get_epot_amg_hierarchy(domain.Epot)
+                                    amg_hierarchy=(self._get_epot_amg_hierarchy(mat)
                                                    if self._potential_use_preconditioner else None),
                                     tag="epot")
+        if phi.dtype != domain.getDtype():
+            phi = phi.to(domain.getDtype())
         if self._potential_normalize and self._potential_rank_deficient:
-            phi = subtract_env_mean_(phi, batch)
+            phi = center_solution(domain, phi, inplace=True)
         cur = domain.epotResult
         if cur is not None and cur.numel() == phi.numel() and cur.is_contiguous() and cur.data_ptr() != phi.data_ptr():
             cur.copy_(phi)

@@ -238,22 +238,41 @@ class KrylovSettings:
     enabled: bool = True
     checkInterval: int = 8
     useGraphs: bool = True
-    cgPreconditioner: int = 0
-    bicgPreconditioner: int = 0
+    cgPreconditioner: str = "none"
+    bicgPreconditioner: str = "none"
 
 
 _KRYLOV = KrylovSettings()
+_PRECONDITIONERS = {"none": 0, "jacobi": 1}
 
 
-def SetKrylovSettings(enabled: bool = True, checkInterval: int = 8, useGraphs: bool = True, cgPreconditioner: int = 0,
-                      bicgPreconditioner: int = 0) -> None:
-    if cgPreconditioner or bicgPreconditioner:
-        raise NotImplementedError("Jacobi-preconditioned fused solvers are not implemented")
-    _KRYLOV.enabled, _KRYLOV.checkInterval, _KRYLOV.useGraphs = bool(enabled), int(checkInterval), bool(useGraphs)
+def _precond_code(name) -> int:
+    if isinstance(name, str) and name in _PRECONDITIONERS:
+        return _PRECONDITIONERS[name]
+    raise ValueError(f"Unknown preconditioner '{name}', expected 'none' or 'jacobi'.")
 
 
-def GetKrylovSettings() -> KrylovSettings:
-    return KrylovSettings(**_KRYLOV.__dict__)
+def SetKrylovSettings(enabled: bool = True, checkInterval: int = 8, useGraphs: bool = True, cgPreconditioner: str = "none",
+                      bicgPreconditioner: str = "none") -> None:
+    """Configure the fused solvers (bindings.cpp:739-752): ``checkInterval`` caps the iterations
+    the host enqueues per chunk, ``useGraphs`` replays captured chunks of iterations from HIP
+    graphs (segmented solves), ``cgPreconditioner`` / ``bicgPreconditioner`` are "none" or
+    "jacobi". ``enabled=False`` (the reference's legacy cuBLAS solvers) is recorded; the library
+    has one solver path."""
+    cg, bicg = _precond_code(cgPreconditioner), _precond_code(bicgPreconditioner)
+    if int(checkInterval) < 1:
+        raise ValueError("checkInterval must be positive")
+    _lib.call("pp_set_krylov_settings", int(checkInterval), int(bool(useGraphs)), cg, bicg)
+    _KRYLOV.enabled = bool(enabled)
+
+
+def GetKrylovSettings() -> dict:
+    """The current settings as the reference returns them (bindings.cpp:753-763)."""
+    ci, ug, cg, bicg = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+    _lib.call("pp_get_krylov_settings", C.byref(ci), C.byref(ug), C.byref(cg), C.byref(bicg))
+    names = {v: k for k, v in _PRECONDITIONERS.items()}
+    return {"enabled": _KRYLOV.enabled, "checkInterval": int(ci.value), "useGraphs": bool(ug.value),
+            "cgPreconditioner": names[cg.value], "bicgPreconditioner": names[bicg.value]}
 
 
 def _results(res, n) -> list:
@@ -309,6 +328,39 @@ def SolveLinear(A: CSRmatrix, RHS: torch.Tensor, x: torch.Tensor, maxIterations=
               int(residualResetSteps), int(bool(transposeA)), int(bool(returnBestResult)),
               _lib.stream_ptr(x.device), res)
     return _results(res, nrhs)
+
+
+def SolveSegmented(A: CSRmatrix, segments: Sequence[int], RHS: torch.Tensor, x: torch.Tensor, maxIterations=1000,
+                   tolerance=1e-8, useBiCG: bool = False, matrixRankDeficient: bool = False, tileRows: int = 0) -> list:
+    """Solve the independent instances of a block-diagonal system in one launch per phase
+    (``pp_solve_segmented``): instance s owns rows ``segments[s] .. segments[s+1]``; ``RHS`` and
+    ``x`` hold ``k`` right-hand sides of ``A.getRows()`` entries each (component-major), every
+    instance solved with the reference's CG or BiCGStab recurrences and stopping rules on its
+    own rows (NORM2_NORMALIZED over them). ``tolerance``: one value or one per (component,
+    instance), component-major; ``matrixRankDeficient``: each instance's right-hand side made
+    mean-free first. Returns one LinearSolverResultInfo per (component, instance)."""
+    n = A.getRows()
+    if A.getBatchSize() != 1:
+        raise ValueError("a segmented solve takes one block-diagonal matrix")
+    if RHS.numel() % n or x.numel() != RHS.numel() or not x.is_contiguous():
+        raise ValueError("RHS and x must hold whole right-hand sides and x must be contiguous")
+    if RHS.dtype != A.getDtype() or x.dtype != A.getDtype():
+        raise TypeError("Data type of RHS and x must match A.")
+    ncomp = RHS.numel() // n
+    segs = [int(v) for v in segments]
+    S = len(segs) - 1
+    tols = _host_scalar_list(tolerance, A.getDtype())
+    if len(tols) not in (1, ncomp * S):
+        raise ValueError("tolerance must be a scalar or one value per instance and component")
+    rhs = RHS.contiguous()
+    res = (_lib.SolveResult * (ncomp * S))()
+    seg_arr = (C.c_int32 * len(segs))(*segs)
+    tarr = (C.c_double * len(tols))(*tols)
+    desc = A.desc()
+    _lib.call("pp_solve_segmented", C.byref(desc), seg_arr, S, ncomp, _lib.ptr(rhs), _lib.ptr(x),
+              _lib.dtype_code(A.getDtype()), int(_host_scalar_list(maxIterations, torch.int32)[0]), tarr, len(tols),
+              int(bool(useBiCG)), int(bool(matrixRankDeficient)), int(tileRows), _lib.stream_ptr(x.device), res)
+    return _results(res, ncomp * S)
 
 
 def pack_amg_levels(levels: Sequence[Sequence[torch.Tensor]], hd, extras: Optional[Sequence[dict]] = None) -> tuple:
@@ -392,15 +444,15 @@ def amg_pcg_packed(aRow, aCol, aVal, packed: tuple, coarsePinv: torch.Tensor, rh
 
 
 def AMGPCGSolve(aRow, aCol, aVal, levels: Sequence[Sequence[torch.Tensor]], coarsePinv: torch.Tensor, rhs: torch.Tensor,
-                x: torch.Tensor, maxIterations: int, tolerance: float, normalized: bool = True, returnBest: bool = False,
-                projectConstant: bool = False, preSweeps: int = 1, postSweeps: int = 1, omega: float = 2.0 / 3.0,
-                tolerances: Sequence[float] = ()) -> list:
+                x: torch.Tensor, maxIterations: int, tolerance: float, normalized: bool = True,
+                returnBestResult: bool = False, projectConstant: bool = False, preSweeps: int = 1, postSweeps: int = 1,
+                omega: float = 2.0 / 3.0, tolerances: Sequence[float] = ()) -> list:
     """AMG-preconditioned CG (bindings.cpp:764-776 -> krylov.cu:1565-1612). ``levels`` holds
     per level ``(A row, col, val, dinv, R row, col, val, P row, col, val)``; the hierarchy
     dtype is that of ``coarsePinv``; ``x`` is updated in place."""
     packed = pack_amg_levels(levels, coarsePinv.dtype)
     return amg_pcg_packed(aRow, aCol, aVal, packed, coarsePinv, rhs, x, maxIterations, tolerance, normalized,
-                          returnBest, projectConstant, preSweeps, postSweeps, omega, tolerances)
+                          returnBestResult, projectConstant, preSweeps, postSweeps, omega, tolerances)
 
 
 def ReleaseKrylovWorkspaces() -> None:

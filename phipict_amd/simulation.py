@@ -47,6 +47,78 @@ def subtract_env_mean_(flat: torch.Tensor, batch: int) -> torch.Tensor:
     return flat
 
 
+def center_solution(domain: Domain, flat: torch.Tensor, inplace: bool = False) -> torch.Tensor:
+    """Subtract the mean of every environment (piso_simulation.py:68-89), or of every instance
+    of a domain of independent blocks (each instance's singular system fixes its own
+    constant)."""
+    if not domain.hasIndependentBlocks():
+        return subtract_env_mean_(flat, domain.getBatchSize()) if inplace else subtract_env_mean(flat, domain.getBatchSize())
+    ids, lengths = domain.segmentIds(), domain.segmentLengths()
+    parts = flat.numel() // domain.getTotalSize()
+    view = flat.view(parts, -1) if inplace else flat.reshape(parts, -1)
+    means = torch.zeros(parts, lengths.numel(), dtype=flat.dtype, device=flat.device)
+    means.index_add_(1, ids, view)
+    means /= lengths
+    if inplace:
+        view.sub_(means[:, ids])
+        return flat
+    return (view - means[:, ids]).reshape(flat.shape)
+
+
+def segmented_tolerances(tol, rhs: torch.Tensor, domain: Domain) -> list:
+    """Absolute tolerance of every (component, instance) piece of a segmented solve: a
+    SolverTolerance resolves against each piece's own right-hand side (||b||/sqrt(n) over the
+    instance's rows), a number applies to all."""
+    segs = domain.getSegments()
+    S, N = len(segs) - 1, domain.getTotalSize()
+    parts = rhs.numel() // N
+    if not isinstance(tol, SolverTolerance):
+        t = tol if tol is not None else default_tolerance(rhs.dtype)
+        vals = [float(v) for v in t] if isinstance(t, (list, tuple)) else [float(t)]
+        return vals if len(vals) in (1, parts * S) else [vals[0]]
+    ids, lengths = domain.segmentIds(), domain.segmentLengths()
+    sq = torch.zeros(parts, S, dtype=torch.float64, device=rhs.device)
+    sq.index_add_(1, ids, rhs.detach().reshape(parts, N).to(torch.float64) ** 2)
+    scales = (sq.sqrt() / lengths.to(torch.float64).sqrt()).reshape(-1).cpu().numpy()
+    return tol._bound(scales, rhs.dtype)
+
+
+#: rows per workgroup tile the segmented engine's autotuner tries
+SEGMENT_TILE_CANDIDATES = (256, 512, 1024, 2048, 4096)
+#: autotuner decisions: (instances, rows, {tile rows: microseconds per iteration}, chosen)
+SEGMENT_TUNING_LOG: list = []
+
+
+def tune_segment_tiles(A: CSRmatrix, segments, rhs: torch.Tensor, use_BiCG: bool = False, iterations: int = 24) -> int:
+    """Rows per workgroup tile of the segmented engine for this set of instances: times a fixed
+    number of iterations of the call's solver (tolerance 0, zero start) on the first right-hand
+    side component for each
+    candidate tile size with HIP events on the current stream and keeps the fastest. Larger
+    tiles mean fewer workgroups and shorter folds, smaller ones more parallelism on small
+    instances."""
+    n = A.getRows()
+    b = rhs.reshape(-1)[:n].contiguous()
+    if not b.is_cuda:
+        return 1024
+    x = torch.zeros_like(b)
+    timings = {}
+    for tile in SEGMENT_TILE_CANDIDATES:
+        if tile > max(256, n):
+            break
+        ops.SolveSegmented(A, segments, b, x.zero_(), maxIterations=2, tolerance=0.0, useBiCG=use_BiCG,
+                           tileRows=tile)  # warm-up
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        ops.SolveSegmented(A, segments, b, x.zero_(), maxIterations=iterations, tolerance=0.0, useBiCG=use_BiCG,
+                           tileRows=tile)
+        stop.record()
+        stop.synchronize()
+        timings[tile] = 1e3 * start.elapsed_time(stop) / iterations
+    best = min(timings, key=timings.get)
+    SEGMENT_TUNING_LOG.append((len(segments) - 1, n, timings, best))
+    return best
+
+
 def check_solver_return_infos(infos, tol, max_iter, return_best_result: bool, tag: str = "") -> None:
     """Raise LinsolveError like the reference (piso_diff.py:659-791): always for a non-finite
     residual, for non-convergence unless the best iterate is returned."""
@@ -111,9 +183,20 @@ def balance_boundary_fluxes(domain: Domain, free_bounds, tol=None) -> None:
     """Rescale the free boundaries so the net boundary flux vanishes, per environment
     (piso_simulation.py:533-610); a free boundary shared by all environments becomes
     per-environment when the scales differ."""
+    if domain.hasIndependentBlocks() and domain.getNumBlocks() > 1:
+        # every instance balances its own outflow against its own inflow
+        for block in domain.getBlocks():
+            own = [fb for fb in free_bounds if any(fb is b for _f, b in block.getFixedBoundaries())]
+            if own:
+                _balance(domain, block.getFixedBoundaries(), own, tol)
+        return
     pairs = []
     for block in domain.getBlocks():
         pairs.extend(block.getFixedBoundaries())
+    _balance(domain, pairs, free_bounds, tol)
+
+
+def _balance(domain: Domain, pairs, free_bounds, tol) -> None:
     fixed = [pb for pb in pairs if not any(pb[1] is fb for fb in free_bounds)]
     free = [pb for pb in pairs if any(pb[1] is fb for fb in free_bounds)]
     if not free:
@@ -475,6 +558,9 @@ class Simulation:
                      tag: str = "unknown"):
         if max_iter is None:
             max_iter = self.linear_solve_max_iterations
+        if self.domain is not None and self.domain.hasIndependentBlocks():
+            return self._segmented_solve(csrMat, rhs, x, use_BiCG, tol, max_iter, matrix_rank_deficient,
+                                         amg_hierarchy, tag)
         batch = self.domain.getBatchSize() if self.domain is not None else 1
         abs_tol = resolve_tolerance(tol, rhs, batch_size=batch)
         if isinstance(abs_tol, list):
@@ -507,6 +593,23 @@ class Simulation:
             check_solver_return_infos(infos, tv, max_iter, return_best_result, tag)
         else:
             x.zero_()
+        return x, True
+
+    def _segmented_solve(self, csrMat, rhs, x, use_BiCG, tol, max_iter, matrix_rank_deficient, amg_hierarchy, tag):
+        """A solve over a domain of independent blocks: every instance (and velocity component)
+        with its own recurrence, tolerance and stopping rules, one launch per solver phase."""
+        if amg_hierarchy is not None:
+            raise NotImplementedError("AMG-preconditioned solves of independent blocks are not implemented: use CG")
+        if x is None:
+            x = torch.zeros_like(rhs)
+        tols = segmented_tolerances(tol, rhs, self.domain)
+        tile = self.__dict__.get("segment_tile_rows")
+        if tile is None:
+            tile = self.segment_tile_rows = tune_segment_tiles(csrMat, self.domain.getSegments(), rhs, use_BiCG)
+        infos = ops.SolveSegmented(csrMat, self.domain.getSegments(), rhs, x, maxIterations=max_iter, tolerance=tols,
+                                   useBiCG=use_BiCG, matrixRankDeficient=matrix_rank_deficient, tileRows=tile)
+        self._record(tag, infos)
+        check_solver_return_infos(infos, max(tols), max_iter, False, tag)
         return x, True
 
     # ---- the step
@@ -608,7 +711,7 @@ class Simulation:
                     if not solve_ok:
                         return solve_ok
                     if self.normalize_pressure_result:
-                        p = subtract_env_mean_(p, domain.getBatchSize())
+                        p = center_solution(domain, p, inplace=True)
                     self._store_pressure_guess(cstep, p)
                     domain.setPressureResult(p)
                     domain.UpdateDomainData()
@@ -633,7 +736,7 @@ class Simulation:
                                                         amg_hierarchy=self._get_pressure_amg_hierarchy(domain.P),
                                                         tag="pressure")
                         if self.normalize_pressure_result:
-                            p = subtract_env_mean(p, domain.getBatchSize())
+                            p = center_solution(domain, p)
                         if pstep == self.pressure_non_ortho_steps - 1:
                             self._store_pressure_guess(cstep, p)
                         domain.setPressureResult(p)
@@ -868,7 +971,7 @@ class Simulation:
         ops.SetupPressureRHSdiv(domain, ts, 0)
         p, ok = self.linear_solve(domain.P, domain.pressureRHSdiv, use_BiCG=self.pressure_use_BiCG, tol=self.pressure_tol,
                                   max_iter=max_iter, tag="pressure")
-        p = subtract_env_mean(p, domain.getBatchSize())
+        p = center_solution(domain, p)
         domain.setPressureResult(p)
         ops.CopyPressureResultToBlocks(domain)
         ops.CorrectVelocity(domain, ts, version=1)

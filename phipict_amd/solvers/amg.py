@@ -516,15 +516,28 @@ def clear_interpolation_cache() -> None:
     _INTERPOLATION_CACHE.clear()
 
 
+def release_pattern(csr_mat: CSRmatrix) -> bool:
+    """Drop the cached interpolation (and the hierarchy holding it) of ``csr_mat``'s sparsity
+    pattern, freeing its device memory once no hierarchy refreshed from it is alive; True if
+    an entry was cached."""
+    key = _sparsity_key(csr_mat)
+    found = False
+    for dtype in (torch.float32, torch.float64):
+        found |= _INTERPOLATION_CACHE.pop((key, dtype), None) is not None
+    return found
+
+
 def hierarchy_for(csr_mat: CSRmatrix, *, project_constant: bool = False, reuse_interpolation: bool = True,
                   native_dtype=torch.float32, **build_kwargs) -> AMGHierarchy:
     """Hierarchy for ``csr_mat``: a new host setup for the first operator of a pattern, a
     device Galerkin refresh with the cached interpolation for every further one."""
     if not reuse_interpolation:
         return build_amg_hierarchy(csr_mat, project_constant=project_constant, native_dtype=native_dtype, **build_kwargs)
-    key = _sparsity_key(csr_mat)
+    # one interpolation per pattern and operator dtype (a float32 pressure and a potential solved
+    # in float64 on the same grid keep one each)
+    key = (_sparsity_key(csr_mat), csr_mat.getDtype())
     base = _INTERPOLATION_CACHE.get(key)
-    if base is None or base.dtype != csr_mat.getDtype():
+    if base is None:
         h = build_amg_hierarchy(csr_mat, project_constant=project_constant, native_dtype=native_dtype, **build_kwargs)
         _INTERPOLATION_CACHE[key] = h
         return h
@@ -545,9 +558,9 @@ def batched_hierarchy_for(csr_mat: CSRmatrix, batch_size: int, *, project_consta
         raise ValueError(f"matrix holds {csr_mat.getBatchSize()} operators, not {batch_size}")
     if (previous is None or previous.batch_size != batch_size or previous.levels[0].A.getNnz() != nnz
             or previous.levels[0].A.row is not csr_mat.row):
-        key = _sparsity_key(csr_mat)
+        key = (_sparsity_key(csr_mat), csr_mat.getDtype())
         base = _INTERPOLATION_CACHE.get(key)
-        if base is None or base.dtype != csr_mat.getDtype():
+        if base is None:
             first = CSRmatrix(csr_mat.value.reshape(-1)[:nnz], csr_mat.index, csr_mat.row)
             base = build_amg_hierarchy(first, project_constant=project_constant, native_dtype=native_dtype,
                                        **build_kwargs)

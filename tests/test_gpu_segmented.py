@@ -1,0 +1,224 @@
+"""Uneven-size batching: independent instances of different grid sizes in one domain, solved
+with the segmented engine (one launch per solver phase for all instances).
+
+* The segmented CG and BiCGStab against the oracle on every instance of a block-diagonal
+  system (six 2D operators of different sizes, several right-hand-side components, per-piece
+  tolerances, an exactly zero piece, the projected rank-deficient mode), for several tile
+  sizes.
+* Whole MHD steps: six Hartmann channels of different sizes, stretching, Reynolds and Hartmann
+  numbers, each with an inflow and an advected outflow balanced on its own, stepped together;
+  every instance equals its own solo run (velocity, pressure, potential to 1e-10 in FP64).
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+import pytest
+import torch
+
+import cases
+from cases import build_domain, rel_l2, to_np
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda", 0)
+
+SIZES = [(24, 16), (40, 20), (32, 32), (48, 24), (20, 28), (36, 18)]
+
+
+def _tdt(dtype):
+    return torch.float64 if dtype == np.float64 else torch.float32
+
+
+def _block_diag(parts):
+    rows, cols, vals, off = [np.zeros(1, np.int32)], [], [], 0
+    nnz = 0
+    for row, col, val in parts:
+        rows.append((row[1:] + nnz).astype(np.int32))
+        cols.append((col + off).astype(np.int32))
+        vals.append(val)
+        off += row.size - 1
+        nnz += row[-1]
+    return np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+
+
+def _operators(use_bicg):
+    out = []
+    for i, (nx, ny) in enumerate(SIZES):
+        od = O.OracleDomain(cases.channel2d(nx, ny, seed=i, u0=1.0 + 0.3 * i), 0.02 + 0.01 * i, 0.02)
+        if use_bicg:
+            col, val, _ = od.advection_matrix()
+        else:
+            _, _, A = od.advection_matrix()
+            col, val = od.pressure_matrix(A)
+            val = val.copy()
+            diag = np.flatnonzero(col == np.repeat(np.arange(od.ncells), np.diff(od.row)))
+            val[diag] -= 0.01 * (i + 1)
+        out.append((od.row, col, val))
+    return out
+
+
+@pytest.mark.parametrize("tile_rows", [256, 1024])
+@pytest.mark.parametrize("use_bicg", [False, True], ids=["cg", "bicgstab"])
+def test_segmented_solver_matches_oracle(dtype_f64_f32, use_bicg, tile_rows):
+    import phipict_amd as pa
+
+    dtype = dtype_f64_f32
+    print(f"[segmented] {'bicg' if use_bicg else 'cg'} tile_rows={tile_rows} {np.dtype(dtype).name}", flush=True)
+    parts = _operators(use_bicg)
+    row, col, val = _block_diag(parts)
+    segs = np.concatenate([[0], np.cumsum([p[0].size - 1 for p in parts])])
+    N, S, ncomp = row.size - 1, len(parts), 2
+    rng = np.random.default_rng(4)
+    b = rng.standard_normal(ncomp * N).astype(dtype)
+    b[N + segs[2]:N + segs[3]] = 0.0  # one piece with an exactly zero right-hand side
+    base_tol = 1e-10 if dtype == np.float64 else 1e-5
+    tols = [base_tol * (1 + 0.5 * (k % 3)) for k in range(ncomp * S)]
+    A = pa.CSRmatrix(torch.as_tensor(val.astype(dtype), device=DEV), torch.as_tensor(col, device=DEV),
+                     torch.as_tensor(row, device=DEV))
+    x0 = (0.1 * rng.standard_normal(ncomp * N)).astype(dtype)
+    x = torch.as_tensor(x0.copy(), device=DEV)
+    infos = pa.SolveSegmented(A, segs, torch.as_tensor(b, device=DEV), x, maxIterations=4000, tolerance=tols,
+                              useBiCG=use_bicg, tileRows=tile_rows)
+    xs = to_np(x)
+    slack = 0 if dtype == np.float64 else 3
+    for c in range(ncomp):
+        for s, (r_, c_, v_) in enumerate(parts):
+            k = c * S + s
+            lo, hi = c * N + segs[s], c * N + segs[s + 1]
+            bk = b[lo:hi].copy()
+            info = infos[k]
+            if not bk.any():
+                assert info.converged and info.usedIterations == 0 and not xs[lo:hi].any()
+                continue
+            xo = x0[lo:hi].copy()
+            solve = O.bicgstab if use_bicg else O.cg
+            ref = solve(r_, c_, v_.astype(dtype), bk, xo, tols[k])
+            assert ref.converged and info.converged, (k, info)
+            assert abs(info.usedIterations - ref.used_iterations) <= slack, (k, info.usedIterations, ref.used_iterations)
+            assert rel_l2(xs[lo:hi], xo) < (1e-8 if dtype == np.float64 else 1e-3), k
+
+
+def test_segmented_projected_cg():
+    """The projected mode: each instance's singular pressure operator with a right-hand side of
+    nonzero mean; every piece equals the oracle's rank-deficient CG up to the constant."""
+    import phipict_amd as pa
+
+    parts = []
+    for i, (nx, ny) in enumerate(SIZES[:4]):
+        od = O.OracleDomain(cases.channel2d(nx, ny, seed=i), 0.02, 0.02)
+        _, _, Am = od.advection_matrix()
+        col, val = od.pressure_matrix(Am)
+        parts.append((od.row, col, val))
+    row, col, val = _block_diag(parts)
+    segs = np.concatenate([[0], np.cumsum([p[0].size - 1 for p in parts])])
+    N = row.size - 1
+    b = np.random.default_rng(7).standard_normal(N) + 0.3
+    A = pa.CSRmatrix(torch.as_tensor(val, device=DEV), torch.as_tensor(col, device=DEV), torch.as_tensor(row, device=DEV))
+    x = torch.zeros(N, dtype=torch.float64, device=DEV)
+    infos = pa.SolveSegmented(A, segs, torch.as_tensor(b, device=DEV), x, maxIterations=4000, tolerance=1e-10,
+                              matrixRankDeficient=True)
+    xs = to_np(x)
+    for s, (r_, c_, v_) in enumerate(parts):
+        lo, hi = segs[s], segs[s + 1]
+        xo = np.zeros(hi - lo)
+        ref = O.cg(r_, c_, v_, b[lo:hi].copy(), xo, 1e-10, project=True)
+        # a singular system near its stopping point: one iteration apart between two summation orders
+        assert infos[s].converged and abs(infos[s].usedIterations - ref.used_iterations) <= 1, (s, infos[s],
+                                                                                             ref.used_iterations)
+        a, o = xs[lo:hi] - xs[lo:hi].mean(), xo - xo.mean()
+        assert rel_l2(a, o) < 1e-8
+
+
+@pytest.fixture(params=[np.float64, np.float32], ids=["f64", "f32"])
+def dtype_f64_f32(request):
+    return request.param
+
+
+INSTANCES = [  # (nx, ny, stretch, Re, Ha)
+    (24, 16, 1.0, 60.0, 5.0), (40, 20, 1.5, 120.0, 12.0), (32, 32, 2.0, 300.0, 30.0),
+    (48, 24, 1.2, 80.0, 8.0), (20, 28, 1.8, 200.0, 20.0), (36, 18, 1.3, 450.0, 45.0),
+]
+
+
+def _instance_spec(i, nx, ny, stretch, re):
+    spec = cases.hartmann_c1(nx=nx, ny=ny, stretch=stretch, seed=10 + i)[0]
+    return dataclasses.replace(spec, viscosity=np.array([1.0 / re]))
+
+
+def _run(specs, stuart, dt, steps, independent):
+    import phipict_amd as pa
+
+    dom = build_domain(specs, 0.01, dt, torch.float64)
+    if independent:
+        dom.setIndependentBlocks()
+    outs = [b.getBoundary(1) for b in dom.getBlocks()]
+    velm = torch.tensor([[1.0, 0.0]], dtype=torch.float64, device=DEV)
+    pa.simulation.balance_boundary_fluxes(dom, outs)
+    dom.UpdateDomainData()
+
+    def advect(domain, **kw):
+        pa.simulation.update_advective_boundaries(domain, outs, velm, dt)
+
+    def no_source(domain, **kw):
+        for b in domain.getBlocks():
+            b.clearVelocitySource()
+        domain.UpdateDomainData()
+
+    tol = pa.SolverTolerance(rtol=1e-12)
+    sim = pa.MHDSimulation(domain=dom, dt=dt, stuart_number=torch.tensor(stuart, dtype=torch.float64),
+                           e_b=torch.tensor([0.0, 1.0, 0.0], dtype=torch.float64), non_orthogonal=False,
+                           advection_tol=tol, pressure_tol=tol, potential_tol=tol,
+                           hooks={"PRE": advect, "PRE_VELOCITY_SETUP": no_source})
+    for _ in range(steps):
+        assert sim.single_step()
+    return dom
+
+
+def test_uneven_instances_step_like_solo_runs():
+    """Six Hartmann channels of different sizes, Re and Ha stepped as one domain of independent
+    blocks equal six solo runs."""
+    print("[segmented] six uneven MHD channels, 3 steps", flush=True)
+    dt, steps = 1e-3, 3
+    specs = [_instance_spec(i, *inst[:4]) for i, inst in enumerate(INSTANCES)]
+    stuart = [ha * ha / re for (_nx, _ny, _st, re, ha) in INSTANCES]
+    batched = _run(specs, stuart, dt, steps, True)
+    segs = batched.getSegments()
+    for i, spec in enumerate(specs):
+        solo = _run([spec], [stuart[i]], dt, steps, False)
+        bb, sb = batched.getBlocks()[i], solo.getBlocks()[0]
+        errs = {"velocity": rel_l2(to_np(bb.velocity), to_np(sb.velocity)),
+                "pressure": rel_l2(to_np(bb.pressure), to_np(sb.pressure)),
+                "potential": rel_l2(to_np(batched.epotResult)[segs[i]:segs[i + 1]], to_np(solo.epotResult))}
+        print(f"instance {i} {INSTANCES[i][:2]}: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+        assert all(v <= 1e-10 for v in errs.values()), (i, errs)
+
+
+def test_captured_chunks_equal_direct_launches():
+    """useGraphs: the segmented solves replayed from captured HIP graphs give bit-identical
+    solutions and results to direct launches, over repeated calls (replays of cached graphs)."""
+    import phipict_amd as pa
+
+    parts = _operators(False)
+    row, col, val = _block_diag(parts)
+    segs = np.concatenate([[0], np.cumsum([p[0].size - 1 for p in parts])])
+    N = row.size - 1
+    A = pa.CSRmatrix(torch.as_tensor(val, device=DEV), torch.as_tensor(col, device=DEV), torch.as_tensor(row, device=DEV))
+    b = torch.as_tensor(np.random.default_rng(9).standard_normal(N), device=DEV)
+    runs = {}
+    try:
+        for graphs in (False, True):
+            pa.SetKrylovSettings(useGraphs=graphs)
+            outs = []
+            for _ in range(3):
+                x = torch.zeros(N, dtype=torch.float64, device=DEV)
+                infos = pa.SolveSegmented(A, segs, b, x, maxIterations=4000, tolerance=1e-10)
+                outs.append((x.clone(), [(i.usedIterations, i.finalResidual, i.converged) for i in infos]))
+            runs[graphs] = outs
+    finally:
+        pa.SetKrylovSettings()
+    for (xa, ia), (xb, ib) in zip(runs[False], runs[True]):
+        assert ia == ib
+        assert torch.equal(xa, xb)

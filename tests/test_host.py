@@ -389,3 +389,31 @@ def test_domain_save_load_round_trip(tmp_path):
     (tmp_path / "v3.npz").write_bytes((tmp_path / "case.npz").read_bytes())
     with pytest.raises(ValueError, match="format version"):
         load_domain(tmp_path / "v3")
+
+
+def test_krylov_settings_keywords():
+    """SetKrylovSettings / GetKrylovSettings with the reference's keywords and preconditioner
+    names (bindings.cpp:739-763); host-only calls of the library."""
+    import phipict_amd as pa
+
+    pa.SetKrylovSettings(enabled=True, checkInterval=4, useGraphs=False, cgPreconditioner="jacobi",
+                         bicgPreconditioner="none")
+    got = pa.GetKrylovSettings()
+    assert got == {"enabled": True, "checkInterval": 4, "useGraphs": False, "cgPreconditioner": "jacobi",
+                   "bicgPreconditioner": "none"}
+    with pytest.raises(ValueError):
+        pa.SetKrylovSettings(cgPreconditioner="ilu")
+    pa.SetKrylovSettings()
+    assert pa.GetKrylovSettings()["cgPreconditioner"] == "none"
+
+
+def test_amg_pcg_solve_reference_keywords():
+    """AMGPCGSolve takes the reference's keyword names (bindings.cpp:764-776), including
+    returnBestResult."""
+    import inspect
+
+    import phipict_amd as pa
+
+    names = list(inspect.signature(pa.AMGPCGSolve).parameters)
+    assert names == ["aRow", "aCol", "aVal", "levels", "coarsePinv", "rhs", "x", "maxIterations", "tolerance", "normalized",
+                     "returnBestResult", "projectConstant", "preSweeps", "postSweeps", "omega", "tolerances"]
