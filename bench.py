@@ -59,11 +59,14 @@ def parse_args(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["duct", "tjunction-adjoint", "uneven-batch", "serving"], default="duct",
+    ap.add_argument("--workload", choices=["duct", "hartmann-c1", "tjunction-adjoint", "uneven-batch", "serving"],
+                    default="duct",
                     help="duct: the PISO + MHD step throughput (default); tjunction-adjoint: the checkpointed "
                          "backward through a 3-block T-junction rollout and its finite-difference check")
     ap.add_argument("--dtype", choices=["f64", "f32"], default="f64")
     ap.add_argument("--size", default="192x96x96", help="cells nx x ny x nz")
+    ap.add_argument("--inflow", action="store_true",
+                    help="uniform inflow and advected outflow instead of the periodic, source-driven duct")
     ap.add_argument("--skew", type=float, default=0.0,
                     help="jitter the duct's interior vertices by this fraction of the smallest spacing and step in "
                          "the non-orthogonal mode (corner terms; CPU baseline off)")
@@ -71,8 +74,7 @@ def parse_args(argv=None):
     ap.add_argument("--timer-in-timed-region", action="store_true",
                     help="run the kernel timer during the timed steps (default: during as many further steps)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-size", default="96x48x48", help="cells of the CPU sample")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="timed oracle steps of the CPU baseline")
     ap.add_argument("--details", action="store_true", help="also print the per-kernel table to stderr")
     ap.add_argument("--host-profile", action="store_true",
                     help="profile the host (Python) side of the timed steps with cProfile; the result goes to stderr")
@@ -103,12 +105,17 @@ def duct_vertices(nx, ny, nz):
     return np.stack([X, Y, Z])
 
 
-def make_sim(nx, ny, nz, dtype, device, skew=0.0):
+def make_sim(nx, ny, nz, dtype, device, skew=0.0, inflow=False):
+    """The duct: periodic in x and driven by a pressure-gradient source, or (inflow) with a
+    uniform inflow U = 1 at -x and an advected outflow at +x, both with an open potential
+    (the pressure then carries the flow's physical pressure drop and its solve iterates)."""
     import numpy as np
     import torch
 
     import phipict_amd as pa
 
+    if inflow:
+        return make_inflow_sim(nx, ny, nz, dtype, device)
     N = HA ** 2 / RE
     G = N  # core velocity of order one
     v = duct_vertices(nx, ny, nz)
@@ -135,6 +142,45 @@ def make_sim(nx, ny, nz, dtype, device, skew=0.0):
                            e_b=torch.tensor([0.0, 1.0, 0.0], dtype=dtype), non_orthogonal=skew > 0,
                            pressure_use_amg=True, potential_use_preconditioner=True, pressure_tol=TOL,
                            potential_tol=TOL, hooks={"PRE_VELOCITY_SETUP": forcing})
+    sim.record_solver_stats = True
+    return dom, sim
+
+
+def make_inflow_sim(nx, ny, nz, dtype, device, U=1.0):
+    import torch
+
+    import phipict_amd as pa
+
+    N = HA ** 2 / RE
+    v = duct_vertices(nx, ny, nz)
+    gen = torch.Generator("cpu").manual_seed(1)
+    vel = 1e-3 * torch.randn(1, 3, nz, ny, nx, generator=gen, dtype=torch.float64)
+    vel[:, 0] += U
+    dom = pa.Domain(3, torch.tensor([1.0 / RE], dtype=dtype), dtype=dtype, device=device)
+    blk = dom.CreateBlock(velocity=vel.to(dtype), vertexCoordinates=torch.as_tensor(v, dtype=dtype).unsqueeze(0))
+    plug = torch.tensor([U, 0.0, 0.0], dtype=dtype, device=device)
+    inlet = blk.CloseBoundary(0, velocity=plug.view(1, 3))
+    inlet.setPotentialBC(pa.PotentialBC.OPEN)
+    out = blk.CloseBoundary(1, velocity=torch.ones(1, 3, nz, ny, 1, dtype=dtype, device=device) * plug.view(1, 3, 1, 1, 1))
+    out.setPotentialBC(pa.PotentialBC.OPEN)
+    for f in (2, 3, 4, 5):
+        blk.CloseBoundary(f)
+    dom.PrepareSolve()
+    pa.simulation.balance_boundary_fluxes(dom, [out])
+    dom.UpdateDomainData()
+    velm = torch.tensor([[U, 0.0, 0.0]], dtype=dtype, device=device)
+
+    def advect(domain, **kw):
+        pa.simulation.update_advective_boundaries(domain, [out], velm, DT)
+
+    def no_source(domain, **kw):
+        domain.getBlocks()[0].clearVelocitySource()
+        domain.UpdateDomainData()
+
+    sim = pa.MHDSimulation(domain=dom, dt=DT, stuart_number=torch.tensor(N),
+                           e_b=torch.tensor([0.0, 1.0, 0.0], dtype=dtype), non_orthogonal=False,
+                           pressure_use_amg=True, potential_use_preconditioner=True, pressure_tol=TOL,
+                           potential_tol=TOL, hooks={"PRE": advect, "PRE_VELOCITY_SETUP": no_source})
     sim.record_solver_stats = True
     return dom, sim
 
@@ -288,8 +334,117 @@ def tjunction_adjoint(args) -> int:
     return 0
 
 
+# ----------------------------------------------------------------------------- C1 in full
+C1_STEPS, C1_DT, C1_TOL, C1_RE = 200, 1e-3, 1e-10, 100.0
+
+
+def hartmann_c1(args) -> int:
+    """The 2D Hartmann channel of the CPU-reference workload as stated: 64 x 64 cells, tanh
+    stretching 1.5 toward the walls, Re = 100, Ha = 10, e_b = (0, 1, 0), uniform inflow U = 1 and
+    an advected (zero-gradient) outflow, insulating no-slip walls, initial N(0, 1e-3) noise,
+    dt = 1e-3, 200 steps with two correctors, BiCGStab momentum and CG pressure / potential to
+    the absolute tolerance 1e-10, FP64. The whole 200 steps on the GPU, and in full in the
+    oracle (one thread and the host's threads; the faster is the baseline), with the relative
+    L2 differences of the final fields."""
+    import numpy as np
+    import torch
+
+    import phipict_amd as pa
+
+    sys.path.insert(0, str(ROOT / "tests"))
+    import cases
+    from oracle import oracle as O
+    from oracle.piso_oracle import MHDSpec, OraclePISO, balance_boundary_fluxes, update_advective_boundaries
+
+    device = torch.device("cuda", 0)
+    specs = cases.hartmann_c1()
+    N = (10.0 ** 2) / C1_RE
+
+    def gpu_run(steps):
+        dom = cases.build_domain(specs, 1.0 / C1_RE, C1_DT, torch.float64)
+        outs = [dom.getBlocks()[0].getBoundary(1)]
+        velm = torch.tensor([[1.0, 0.0]], dtype=torch.float64, device=device)
+
+        def advect(domain, **kw):
+            pa.simulation.update_advective_boundaries(domain, outs, velm, C1_DT)
+
+        def no_source(domain, **kw):
+            domain.getBlocks()[0].clearVelocitySource()
+            domain.UpdateDomainData()
+
+        sim = pa.MHDSimulation(domain=dom, dt=C1_DT, stuart_number=torch.tensor(N),
+                               e_b=torch.tensor([0.0, 1.0, 0.0], dtype=torch.float64), non_orthogonal=False,
+                               advection_tol=C1_TOL, pressure_tol=C1_TOL, potential_tol=C1_TOL,
+                               hooks={"PRE": advect, "PRE_VELOCITY_SETUP": no_source})
+        pa.simulation.balance_boundary_fluxes(dom, outs)
+        dom.UpdateDomainData()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            if not sim.single_step():
+                raise RuntimeError(f"C1 step {i} failed")
+        torch.cuda.synchronize()
+        return dom, time.perf_counter() - t0
+
+    gpu_run(args.warmup)
+    print(f"[bench] C1: {C1_STEPS} GPU steps", file=sys.stderr, flush=True)
+    dom, gpu_s = gpu_run(C1_STEPS)
+
+    def oracle_run(threaded):
+        O.use_threads(threaded)
+        try:
+            od = O.OracleDomain(specs, 1.0 / C1_RE, C1_DT, np.float64)
+            osim = OraclePISO(od, C1_DT, advection_tol=C1_TOL, pressure_tol=C1_TOL,
+                              mhd=MHDSpec(N, (0.0, 1.0, 0.0), potential_tol=C1_TOL),
+                              pre_velocity_setup=lambda o: (update_advective_boundaries(o.od, [(0, 1)], np.array([1.0, 0.0]),
+                                                                                        C1_DT), o.od.set_source(0, None)))
+            balance_boundary_fluxes(od, [(0, 1)])
+            t0 = time.perf_counter()
+            for _ in range(C1_STEPS):
+                osim.step()
+            return od, osim, time.perf_counter() - t0
+        finally:
+            O.use_threads(False)
+
+    threads = cpu_threads()
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    print(f"[bench] C1: {C1_STEPS} oracle steps, 1 and {threads} threads", file=sys.stderr, flush=True)
+    od, osim, cpu1 = oracle_run(False)
+    _, _, cpun = oracle_run(True)
+    blk = dom.getBlocks()[0]
+    errs = {"velocity": cases.rel_l2(cases.to_np(blk.velocity)[0], od.velocity[0]),
+            "pressure": cases.rel_l2(cases.to_np(blk.pressure)[0], od.pressure[0]),
+            "potential": cases.rel_l2(cases.to_np(dom.epotResult), osim.epot_result)}
+    best_cores, best_s = (1, cpu1) if cpu1 <= cpun else (threads, cpun)
+    cells = int(dom.getTotalSize())
+    line = {
+        "metric": "PISO steps/s",
+        "value": round(C1_STEPS / gpu_s, 2),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": C1_STEPS,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * gpu_s / C1_STEPS, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "2D Hartmann MHD channel 64x64 cells, tanh 1.5 in y, Re=100, Ha=10, inflow U=1 + "
+                               "advected outflow, dt=1e-3, 200 steps, 2 correctors, BiCGStab momentum + CG pressure "
+                               "and potential to abs tol 1e-10", "cells": cells},
+        "cell_updates_per_s": round(cells * C1_STEPS / gpu_s, 1),
+        "rel_l2_vs_oracle": {k: float(f"{v:.3e}") for k, v in errs.items()},
+        "cpu_baseline": {"value": round(C1_STEPS / best_s, 3), "unit": "steps/s", "cores": best_cores, "kind": "port",
+                         "sample": (f"the oracle on the same case, all {C1_STEPS} steps: {cpu1:.2f} s on 1 thread, "
+                                    f"{cpun:.2f} s on {threads} threads (OpenMP build)")},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 # ----------------------------------------------------------------------------- uneven batch
-UB_COUNT, UB_DT, UB_TOL = 64, 1e-3, 1e-10
+UB_COUNT, UB_DT, UB_TOL = 64, 1e-3, 1e-8
 
 
 def uneven_instances(count=UB_COUNT):
@@ -372,20 +527,33 @@ def uneven_batch(args) -> int:
     inst = uneven_instances()
     cells = sum(nx * ny for nx, ny, *_ in inst)
     print(f"[bench] uneven batch: {len(inst)} instances, {cells} cells", file=sys.stderr, flush=True)
-    dom, sim = make_uneven(inst, dtype, device, True)
-    for _ in range(args.warmup):
-        if not sim.single_step():
-            raise RuntimeError("batched step failed")
-    torch.cuda.synchronize()
-    gc.collect()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        if not sim.single_step():
-            raise RuntimeError("batched step failed")
-    torch.cuda.synchronize()
-    batched = time.perf_counter() - t0
-    print(f"[bench] batched: {args.steps} steps in {batched:.3f} s", file=sys.stderr, flush=True)
-    del dom, sim
+    import phipict_amd as pa
+
+    runs = {}
+    for graphs in (False, True):
+        # the segmented solves' iteration chunks replayed from captured HIP graphs (useGraphs),
+        # and launched directly for comparison
+        pa.SetKrylovSettings(useGraphs=graphs)
+        try:
+            dom, sim = make_uneven(inst, dtype, device, True)
+            for _ in range(args.warmup):
+                if not sim.single_step():
+                    raise RuntimeError("batched step failed")
+            torch.cuda.synchronize()
+            gc.collect()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                if not sim.single_step():
+                    raise RuntimeError("batched step failed")
+            torch.cuda.synchronize()
+            runs[graphs] = time.perf_counter() - t0
+            tile = sim.__dict__.get("segment_tile_rows")
+            del dom, sim
+        finally:
+            pa.SetKrylovSettings()
+        print(f"[bench] batched (graphs {graphs}): {args.steps} steps in {runs[graphs]:.3f} s, tile rows {tile}",
+              file=sys.stderr, flush=True)
+    batched = runs[True]
     pairs = make_uneven(inst, dtype, device, False)
     for _ in range(max(1, args.warmup // 2)):
         for _d, s_ in pairs:
@@ -418,8 +586,12 @@ def uneven_batch(args) -> int:
         "data": "synthetic",
         "config": {"workload": f"{len(inst)} independent 2D Hartmann channels, Nx,Ny log-uniform [32,256], "
                                "Ha~U[5,50], Re~U[50,500], stretch~U[1,2], inflow + advected outflow, dt=1e-3, "
-                               "CG pressure/potential + BiCGStab momentum, abs tol 1e-10",
+                               "CG pressure/potential + BiCGStab momentum, abs tol 1e-8, segmented solves from "
+                               "captured HIP graphs",
                    "cells": cells},
+        "graphs": True,
+        "direct_launch_env_steps_per_s": round(len(inst) * args.steps / runs[False], 2),
+        "segment_tile_rows": tile,
         "looped_env_steps_per_s": round(env_looped, 2),
         "batched_over_looped": round(env_batched / env_looped, 2),
         "cell_updates_per_s": round(cells * args.steps / batched, 1),
@@ -578,13 +750,16 @@ def kernel_bytes(name, rows, s, sh, fine, levels, m=3):
 
 
 def hierarchy_levels(sim):
+    """Level rows -> (nnz_A, n_coarse, nnz_R, nnz_P, lanes_A, lanes_R, lanes_P) of the pressure
+    and potential hierarchies (lanes of the sliced copies; 1 without one)."""
     out = {}
     for h in (getattr(sim, "_pressure_amg_hierarchy", None), getattr(sim, "_epot_amg_hierarchy", None)):
         if h is None:
             continue
         for i, lvl in enumerate(h.levels[:-1]):
             nxt = h.levels[i + 1]
-            out[lvl.n] = (lvl.A.getNnz(), nxt.n, int(lvl.R_col.numel()), int(lvl.P_col.numel()))
+            lanes = [lvl.shared[k].lanes if k in lvl.shared else 1 for k in ("A", "R", "P")]
+            out[lvl.n] = (lvl.A.getNnz(), nxt.n, int(lvl.R_col.numel()), int(lvl.P_col.numel()), *lanes)
     return out
 
 
@@ -678,41 +853,48 @@ def load_traffic(path):
     return json.loads(body)
 
 
-def row_group_width(nnz_per_row):
-    """Lanes per row of the AMG level kernels (the library's rule for CSR levels)."""
-    for w, lim in ((1, 3.0), (2, 6.0), (4, 12.0), (8, 24.0), (16, 48.0)):
-        if nnz_per_row <= lim:
-            return w
-    return 32
+def level_grid_threads(rows, lanes, cus, ell=False):
+    """Threads of a V-cycle level launch: 64 / lanes rows per wavefront (64 on the ELL finest
+    level), four wavefronts per workgroup, at most 8 workgroups per CU."""
+    per = 64 if ell else 64 // lanes
+    waves = -(-rows // per)
+    return min(max(1, -(-waves // 4)), 8 * cus) * 256
 
 
 def traffic_for(table, name, rows, n, dtype_name, levels, cus):
     """Measured bytes per launch of the timer's kernel ``name`` at ``rows`` rows, or None.
-    AMG level kernels are told apart by their lanes per row (1 on the finest level, which runs
-    on its ELL copy) and their grid (min(rows * lanes / 256, 8 * CUs) workgroups of 256)."""
+    Level passes are told apart by their pass (k_level's first template argument), and their
+    grid, which follows from the rows and the lanes per row of the matrix they read (``levels``
+    maps level rows to (nnz_A, n_coarse, nnz_R, nnz_P, lanes_A, lanes_R, lanes_P))."""
     from phipict_amd import profiling
 
     if table is None:
         return None
     base = profiling.TIMER_TO_KERNEL.get(name)
-    cands = [e for e in table["kernels"] if e["kernel"] == base]
     if name.startswith("amg_"):
-        if name not in ("amg_presmooth", "amg_jacobi", "amg_post_jacobi"):
+        op = profiling.AMG_TIMER_OP.get(name)
+        if op is None:
             return None
-        if rows == n:
-            w = 1
+        if rows == n and name == "amg_presmooth":
+            base, grid, targ0 = "k_pcg_decide", level_grid_threads(rows, 1, cus, ell=True), None
+        elif rows == n and name in ("amg_jacobi", "amg_post_jacobi"):
+            grid, targ0 = level_grid_threads(rows, 1, cus, ell=True), op
+        elif name == "amg_restrict":
+            fine = next((nf for nf, lv in levels.items() if lv[1] == rows), None)
+            if fine is None:
+                return None
+            grid, targ0 = level_grid_threads(rows, levels[fine][5], cus), op
         elif rows in levels:
-            w = row_group_width(levels[rows][0] / rows)
+            lanes = levels[rows][6] if name == "amg_prolong" else levels[rows][4]
+            grid, targ0 = level_grid_threads(rows, lanes, cus), op
         else:
             return None
-        grid = min(-(-rows * w // 256), 8 * cus) * 256
-        cands = [e for e in cands if e["targs"] and e["targs"][0] == str(w) and e["grid"] == grid]
-        sys_t = "double" if dtype_name == "f64" else "float"
-        if rows == n:
-            want_out = sys_t if name == "amg_post_jacobi" else "float"
-            cands = [e for e in cands if e["kernel"] != "k_jacobi_sweep" or e["targs"][-1] == want_out]
-    elif rows != n:
-        return None
+        cands = [e for e in table["kernels"] if e["kernel"] == base and e["grid"] == grid
+                 and (targ0 is None or (e["targs"] and e["targs"][0] == targ0))]
+    else:
+        if rows != n:
+            return None
+        cands = [e for e in table["kernels"] if e["kernel"] == base]
     if not cands:
         return None
     e = max(cands, key=lambda e: e["fetch_bytes"])
@@ -722,9 +904,20 @@ def traffic_for(table, name, rows, n, dtype_name, levels, cus):
 
 
 # ----------------------------------------------------------------------------- CPU baseline
-def cpu_baseline(size, steps, full_cells, dtype_name):
-    """The oracle (single-threaded C restatement) on a bounded sample: the same physics and
-    solver settings on a coarser grid; steps/s scaled to the full grid by the cell ratio."""
+def cpu_threads() -> int:
+    """Threads of the CPU baseline: OMP_NUM_THREADS when set (the host share given to this
+    process), else every host core."""
+    try:
+        return max(1, int(os.environ.get("OMP_NUM_THREADS", "")))
+    except ValueError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(size, steps, dtype_name):
+    """The oracle (the C restatement, built with OpenMP: cell and row loops shared among the
+    host threads) on the same duct, grid and solver settings as the GPU run, for a reduced
+    number of steps after one untimed step (AMG setup and first assembly); steps/s of the
+    timed steps, no extrapolation."""
     import numpy as np
 
     from oracle import oracle as O
@@ -732,34 +925,43 @@ def cpu_baseline(size, steps, full_cells, dtype_name):
 
     nx, ny, nz = size
     dtype = np.float64 if dtype_name == "f64" else np.float32
-    v = duct_vertices(nx, ny, nz)
-    rng = np.random.default_rng(1)
-    cz = [nz, ny, nx]
-    faces = [O.FaceSpec("periodic"), O.FaceSpec("periodic")] + [O.FaceSpec("fixed") for _ in range(4)]
-    N = HA ** 2 / RE
-    src = np.array([N, 0.0, 0.0])
-    spec = O.BlockSpec(v, faces, velocity=1e-3 * rng.standard_normal([3] + cz), pressure=np.zeros(cz), vsource=src)
-    od = O.OracleDomain([spec], 1.0 / RE, DT, dtype)
-    atol = 1e-8 if dtype == np.float64 else 1e-5
-    sim = OraclePISO(od, DT, advection_tol=atol, pressure_tol=TOL, pressure_use_amg=True,
-                     mhd=MHDSpec(N, potential_tol=TOL, use_amg=True),
-                     pre_velocity_setup=lambda o: o.od.set_source(0, src.astype(dtype)))
-    sim.step()  # AMG setup and first assembly
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        sim.step()
-    el = time.perf_counter() - t0
+    threads = cpu_threads()
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    O.use_threads(True)
+    try:
+        v = duct_vertices(nx, ny, nz)
+        rng = np.random.default_rng(1)
+        cz = [nz, ny, nx]
+        faces = [O.FaceSpec("periodic"), O.FaceSpec("periodic")] + [O.FaceSpec("fixed") for _ in range(4)]
+        N = HA ** 2 / RE
+        src = np.array([N, 0.0, 0.0])
+        spec = O.BlockSpec(v, faces, velocity=1e-3 * rng.standard_normal([3] + cz), pressure=np.zeros(cz),
+                           vsource=src)
+        od = O.OracleDomain([spec], 1.0 / RE, DT, dtype)
+        atol = 1e-8 if dtype == np.float64 else 1e-5
+        sim = OraclePISO(od, DT, advection_tol=atol, pressure_tol=TOL, pressure_use_amg=True,
+                         mhd=MHDSpec(N, potential_tol=TOL, use_amg=True),
+                         pre_velocity_setup=lambda o: o.od.set_source(0, src.astype(dtype)))
+        t0 = time.perf_counter()
+        sim.step()  # AMG setup and first assembly
+        setup = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sim.step()
+        el = time.perf_counter() - t0
+    finally:
+        O.use_threads(False)
+    sps = steps / el
     cells = nx * ny * nz
-    sps_sample = steps / el
     return {
-        "value": sps_sample * cells / full_cells,
+        "value": round(sps, 5),
         "unit": "steps/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": (f"oracle (C, 1 thread) on the same duct physics at {nx}x{ny}x{nz} cells, {steps} steps after one "
-                   f"warm-up step ({el:.2f} s, {sps_sample:.3f} steps/s, {sps_sample * cells:.3e} cell-updates/s); "
-                   f"value scaled to {full_cells} cells by the cell ratio"),
-        "cell_updates_per_s": sps_sample * cells,
+        "sample": (f"oracle (C restatement, OpenMP, {threads} threads) on the same {nx}x{ny}x{nz} duct "
+                   f"({cells} cells), {steps} timed steps after one untimed step with the AMG setup "
+                   f"({setup:.1f} s); {el:.2f} s timed, {sps * cells:.3e} cell-updates/s"),
+        "cell_updates_per_s": round(sps * cells, 1),
     }
 
 
@@ -782,6 +984,8 @@ def main(argv=None) -> int:
         return tjunction_adjoint(args)
     if args.workload == "uneven-batch":
         return uneven_batch(args)
+    if args.workload == "hartmann-c1":
+        return hartmann_c1(args)
     if args.workload == "serving":
         return serving_bench(args)
     import numpy as np
@@ -794,7 +998,7 @@ def main(argv=None) -> int:
     dtype = torch.float64 if args.dtype == "f64" else torch.float32
     device = torch.device("cuda", 0)
     print(f"[bench] building the {nx}x{ny}x{nz} duct ({args.dtype})", file=sys.stderr, flush=True)
-    dom, sim = make_sim(nx, ny, nz, dtype, device, skew=args.skew)
+    dom, sim = make_sim(nx, ny, nz, dtype, device, skew=args.skew, inflow=args.inflow)
     import ctypes
 
     from phipict_amd import _lib
@@ -907,10 +1111,10 @@ def main(argv=None) -> int:
     for _, tag, infos in sim.solver_log[log_start:log_start + len(sim.solver_log[log_start:]) // (1 if args.timer_in_timed_region else 2)]:
         iters.setdefault(tag, []).append(max(i[0] for i in infos))
     cpu = None
-    if not args.no_cpu_baseline and args.skew == 0:
-        cx, cy, cz = (int(v) for v in args.cpu_size.lower().split("x"))
-        print(f"[bench] CPU baseline sample {cx}x{cy}x{cz}", file=sys.stderr, flush=True)
-        cpu = cpu_baseline((cx, cy, cz), args.cpu_steps, n, args.dtype)
+    if not args.no_cpu_baseline and args.skew == 0 and not args.inflow:
+        print(f"[bench] CPU baseline: the oracle on the {nx}x{ny}x{nz} duct, {args.cpu_steps} steps, "
+              f"{cpu_threads()} threads", file=sys.stderr, flush=True)
+        cpu = cpu_baseline((nx, ny, nz), args.cpu_steps, args.dtype)
     sps = args.steps / elapsed
     line = {
         "metric": "PISO steps/s",
@@ -925,7 +1129,9 @@ def main(argv=None) -> int:
         "vs_baseline": None,
         "dtype": args.dtype,
         "data": "synthetic",
-        "config": {"workload": f"3D MHD duct {nx}x{ny}x{nz} cells (periodic x, pressure-gradient source), Re=500, "
+        "config": {"workload": f"3D MHD duct {nx}x{ny}x{nz} cells ("
+                               + ("uniform inflow U=1, advected outflow, open potential at both ends" if args.inflow
+                                  else "periodic x, pressure-gradient source") + "), Re=500, "
                                f"Ha=100, dt=5e-4, AMG-PCG pressure+potential tol 1e-8, BiCGStab momentum"
                                + (f", skewed grid (interior vertices jittered by {args.skew} of the spacing), "
                                   f"non-orthogonal corner terms" if args.skew > 0 else ""),

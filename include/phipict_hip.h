@@ -106,6 +106,12 @@ typedef struct pp_csr {
 } pp_csr;
 
 /* Host-side view of a domain (the reference's DomainGPU, domain_structs_gpu.h:334-445). */
+/* Convective schemes of the momentum and passive-scalar equations (AdvectionScheme,
+ * csrc/phipict/domain/domain_structs_gpu.h:65-68): CENTRAL assembles the face value into the
+ * matrix; LINEAR_UPWIND puts first-order upwind into the matrix and the second-order upwind
+ * remainder on the right-hand side (deferred correction). */
+enum { PP_ADV_CENTRAL = 0, PP_ADV_LINEAR_UPWIND = 1 };
+
 typedef struct pp_domain {
     int32_t dims;           /* 2 or 3 */
     int32_t num_blocks;
@@ -132,6 +138,8 @@ typedef struct pp_domain {
     const void *scalar_viscosity; /* DEVICE pointer [scalar_visc_count] (dtype) */
     void *scalar_rhs;       /* [batch][C][n] */
     void *scalar_result;    /* [batch][C][n] */
+    int32_t advection_scheme; /* PP_ADV_CENTRAL / PP_ADV_LINEAR_UPWIND (Domain::setAdvectionScheme) */
+    int32_t reserved;
 } pp_domain;
 
 /* The reference's LinearSolverResultInfo (csrc/phipict/solvers/linear_solvers.h:180-185). */

@@ -64,11 +64,14 @@ typedef struct ob_block {
     ob_bound b[6];
 } ob_block;
 
+/* convective schemes (AdvectionScheme, domain_structs_gpu.h:65-68) */
+enum { OB_ADV_CENTRAL = 0, OB_ADV_LINEAR_UPWIND = 1 };
+
 typedef struct ob_domain {
     int32_t dims;
     int32_t nblocks;
     int32_t ncells;
-    int32_t pad;
+    int32_t scheme;        /* OB_ADV_* */
     double viscosity;
     double dt;
     const ob_block *blocks;

@@ -23,6 +23,7 @@ LIB_PATH = HERE / "_build" / "liboracle.so"
 LIB_PATH_OMP = HERE / "_build" / "liboracle_omp.so"
 
 OB_CONNECTED, OB_PERIODIC, OB_FIXED = 20, 21, 30
+ADV_CENTRAL, ADV_LINEAR_UPWIND = 0, 1  # AdvectionScheme (domain_structs_gpu.h:65-68)
 POT_INSULATING, POT_OPEN, POT_DIRICHLET, POT_THIN_WALL = 0, 1, 2, 3
 
 
@@ -47,7 +48,7 @@ class OBlock(C.Structure):
 
 
 class ODomain(C.Structure):
-    _fields_ = [("dims", C.c_int32), ("nblocks", C.c_int32), ("ncells", C.c_int32), ("pad", C.c_int32),
+    _fields_ = [("dims", C.c_int32), ("nblocks", C.c_int32), ("ncells", C.c_int32), ("scheme", C.c_int32),
                 ("viscosity", C.c_double), ("dt", C.c_double), ("blocks", C.POINTER(OBlock))]
 
 
@@ -213,8 +214,9 @@ class OracleDomain:
     """Host domain for the oracle kernels, built from BlockSpecs in one dtype."""
 
     def __init__(self, specs: Sequence[BlockSpec], viscosity: float, dt: float, dtype=np.float64,
-                 scalar_visc: Optional[Sequence[float]] = None):
+                 scalar_visc: Optional[Sequence[float]] = None, advection_scheme: int = ADV_CENTRAL):
         self.scalar_visc = None if scalar_visc is None else [float(v) for v in np.atleast_1d(scalar_visc)]
+        self.advection_scheme = int(advection_scheme)
         self.dtype = np.dtype(dtype)
         self.specs = list(specs)
         self.dims = specs[0].dims
@@ -306,7 +308,7 @@ class OracleDomain:
         self.scalar = [None if s.scalar is None else np.ascontiguousarray(s.scalar.reshape(s.scalar.shape[0], -1),
                                                                            self.dtype) for s in specs]
         self.channels = 0 if self.scalar[0] is None else self.scalar[0].shape[0]
-        self.dom = ODomain(self.dims, len(specs), off, 0, float(viscosity), float(dt), self.blocks)
+        self.dom = ODomain(self.dims, len(specs), off, self.advection_scheme, float(viscosity), float(dt), self.blocks)
         self.row = np.zeros(off + 1, np.int32)
         rc = lib().o_csr_rows_f64(C.byref(self.dom), _p(self.row))
         assert rc == 0
@@ -429,7 +431,8 @@ class OracleDomain:
             for f in range(2 * self.dims):
                 if blocks[bi].b[f].type == OB_FIXED:
                     blocks[bi].b[f].vel_neumann = 1 if self.scalar_is_neumann(bi, f, ch) else 0
-        dom = ODomain(self.dims, len(self.specs), self.ncells, 0, self.scalar_diffusivity(ch), self.dom.dt, blocks)
+        dom = ODomain(self.dims, len(self.specs), self.ncells, self.advection_scheme, self.scalar_diffusivity(ch),
+                      self.dom.dt, blocks)
         col = np.zeros(self.nnz, np.int32)
         val = np.zeros(self.nnz, self.dtype)
         A = np.zeros(self.ncells, self.dtype)
@@ -448,15 +451,23 @@ class OracleDomain:
         return pos[0] + sz[0] * (pos[1] + sz[1] * pos[2])
 
     def scalar_rhs(self, flags: int = 0, scalar_result: Optional[np.ndarray] = None) -> np.ndarray:
-        """kPISO_build_scalar_advection_RHS, advection.cu:544-700 (CENTRAL): [C][n]; with the
-        flags' right-hand-side bits the corner terms of ``scalar_result`` [C][n] times the
-        diffusivity are subtracted before the division by det."""
+        """kPISO_build_scalar_advection_RHS, advection.cu:544-700: [C][n]; with LINEAR_UPWIND the
+        deferred correction of each channel (``o_deferred_field``) is added after the prescribed
+        faces' terms; with the flags' right-hand-side bits the corner terms of ``scalar_result``
+        [C][n] times the diffusivity are subtracted before the division by det."""
         use_no = bool(flags & (NO_DIRECT_RHS | NO_DIAGONAL_RHS)) and self.dims > 1
         S = [self.nonortho_rhs(OK_SCAL, flags, ch, scalar_result) for ch in range(self.channels)] if use_no else None
         d, ts = self.dims, 2 * self.dims * self.dims + 1
         C_ = self.channels
         dt = self.dtype.type(self.dom.dt)
         out = np.zeros((C_, self.ncells), self.dtype)
+        corr = None
+        if self.advection_scheme == ADV_LINEAR_UPWIND:
+            glob = np.concatenate([self.scalar[bi] for bi in range(len(self.specs))], axis=1)
+            corr = np.zeros((C_, self.ncells), self.dtype)
+            for ch in range(C_):
+                g = np.ascontiguousarray(glob[ch])
+                assert self._fn("o_deferred_field")(C.byref(self.dom), _p(g), _p(corr[ch])) == 0
         for bi, spec in enumerate(self.specs):
             T = self.cell_tr[bi].reshape(-1, ts)
             det = T[:, -1]
@@ -485,6 +496,8 @@ class OracleDomain:
                     else:
                         area = detf * np.sqrt((minv * minv).sum(axis=1))
                         np.add.at(t, cells, -phi[ch][cells] * flux + sv * visc * area)
+                if corr is not None:
+                    t = t + corr[ch][o:o + spec.ncells]
                 if S is not None:
                     t = t - S[ch][o:o + spec.ncells] * self.dtype.type(self.scalar_diffusivity(ch))
                 out[ch, o:o + spec.ncells] = t / det

@@ -260,6 +260,43 @@ static int FN(emit_row)(int *idx, REAL *vals, int n, int32_t *col, REAL *val, in
     return written;
 }
 
+/* hasFarUpwindCell, shared_stencils.cuh:35-43: the face's neighbour and the far-upwind cell
+ * (one further against the flow along the face normal) both lie inside the block. */
+static int FN(far_upwind)(const int *p, int bound, int outflow, const ob_block *b) {
+    const int axis = bound >> 1, fs = FN(face_sign)(bound), last = b->size[axis] - 1;
+    const int iD = p[axis] + fs;
+    if (iD < 0 || iD > last) return 0;
+    const int iUU = outflow ? p[axis] - fs : p[axis] + 2 * fs;
+    return iUU >= 0 && iUU <= last;
+}
+
+/* The LINEAR_UPWIND deferred correction of one field component at cell p (advection.cu:893-937
+ * for the velocity, :650-684 for the passive scalar): for every face whose far-upwind cell
+ * exists, t -= F 0.5 psi (phi_D - phi_U) with psi = r = (phi_U - phi_UU) / (phi_D - phi_U),
+ * skipped on flat faces (|phi_D - phi_U| < 1e-20). fl: the cell's face fluxes (not multiplied
+ * by the face sign); phi: the component's values over the block. */
+static void FN(deferred_correction)(const ob_block *b, int d, const int *p, const REAL *fl, const REAL *phi, REAL *t) {
+    for (int bound = 0; bound < 2 * d; ++bound) {
+        const int axis = bound >> 1, fs = FN(face_sign)(bound);
+        const REAL ffo = (REAL)fs * fl[bound];
+        const int outflow = ffo >= 0;
+        if (!FN(far_upwind)(p, bound, outflow, b)) continue;
+        int q[3] = {p[0], p[1], p[2]};
+        const REAL phiP = phi[FN(flat)(q, b->size)];
+        q[axis] = p[axis] + fs;
+        const REAL phiN = phi[FN(flat)(q, b->size)];
+        q[axis] = outflow ? p[axis] - fs : p[axis] + 2 * fs;
+        const REAL phiUU = phi[FN(flat)(q, b->size)];
+        const REAL phiU = outflow ? phiP : phiN;
+        const REAL phiD = outflow ? phiN : phiP;
+        const REAL dD = phiD - phiU;
+        const REAL absDD = dD < 0 ? -dD : dD;
+        if (absDD < (REAL)1e-20) continue;
+        const REAL psi = (phiU - phiUU) / dD;
+        *t -= ffo * (REAL)0.5 * psi * dD;
+    }
+}
+
 static int FN(domain_ok)(const ob_domain *dom) { return dom && dom->dims >= 2 && dom->dims <= 3 && dom->nblocks > 0; }
 
 /* ---- non-orthogonal corner terms of skewed grids ----------------------------------------
@@ -621,9 +658,16 @@ int FN(o_advection_matrix)(const ob_domain *dom, int flags, int scalar_mode, con
                 int atb = FN(at_bound)(p, bound, b);
                 if (!atb || !FN(is_empty)(b, bound)) {
                     REAL ffo = fs * fl[bound];
-                    REAL ff = (REAL)0.5 * ffo;
-                    diag += ff;
-                    rv[bound + 1] += ff;
+                    if (dom->scheme == OB_ADV_LINEAR_UPWIND && FN(far_upwind)(p, bound, ffo >= 0, b)) {
+                        /* first-order upwind implicitly (advection.cu:129-140); the rest of the
+                         * face value is the deferred correction of the right-hand side */
+                        if (ffo >= 0) diag += ffo;
+                        else rv[bound + 1] += ffo;
+                    } else {
+                        REAL ff = (REAL)0.5 * ffo;
+                        diag += ff;
+                        rv[bound + 1] += ff;
+                    }
                     int np[3], nw;
                     const ob_block *nb = FN(neighbor)(&c, b, p, bound, np, &nw);
                     REAL an = FN(alpha)(&c, nb, np, nw);
@@ -723,6 +767,8 @@ int FN(o_advection_rhs)(const ob_domain *dom, int apply_pressure_gradient, const
             int p[3];
             FN(unflat)(f, b->size, p);
             const REAL det = b->has_transform ? FN(det)(&c, b, p) : (REAL)1;
+            REAL fl[6];
+            if (dom->scheme == OB_ADV_LINEAR_UPWIND) FN(fluxes)(&c, b, p, NULL, 0, fl);
             for (int dim = 0; dim < c.d; ++dim) {
                 REAL t = det * ((const REAL *)b->velocity)[(size_t)dim * n + f] / dt;
                 for (int bound = 0; bound < 2 * c.d; ++bound) {
@@ -737,6 +783,8 @@ int FN(o_advection_rhs)(const ob_domain *dom, int apply_pressure_gradient, const
                     REAL slip = b->b[bound].vel_neumann ? 1 : 0;
                     t += vel * (1 - slip) * bv * 2 * alpha;
                 }
+                if (dom->scheme == OB_ADV_LINEAR_UPWIND)
+                    FN(deferred_correction)(b, c.d, p, fl, (const REAL *)b->velocity + (size_t)dim * n, &t);
                 if (S) t -= S[(size_t)dim * dom->ncells + b->global_offset + f];
                 t /= det;
                 t += FN(source)(b, p, dim);
@@ -747,6 +795,29 @@ int FN(o_advection_rhs)(const ob_domain *dom, int apply_pressure_gradient, const
                 }
                 rhs[(size_t)dim * dom->ncells + b->global_offset + f] = t;
             }
+        }
+    }
+    return 0;
+}
+
+/* The LINEAR_UPWIND deferred correction of a passive-scalar channel (advection.cu:650-684)
+ * alone, for the oracle's numpy restatement of the scalar right-hand side: out[g] = the sum
+ * the kernel subtracts face by face (before the division by det), phi in global cell order. */
+int FN(o_deferred_field)(const ob_domain *dom, const REAL *phi, REAL *out) {
+    if (!FN(domain_ok)(dom)) return -1;
+    FN(octx) c = {dom, dom->dims, 2 * dom->dims * dom->dims + 1};
+    for (int bi = 0; bi < dom->nblocks; ++bi) {
+        const ob_block *b = &dom->blocks[bi];
+        int n = FN(ncells)(b);
+        O_PAR
+        for (int f = 0; f < n; ++f) {
+            int p[3];
+            FN(unflat)(f, b->size, p);
+            REAL fl[6];
+            FN(fluxes)(&c, b, p, NULL, 0, fl);
+            REAL t = 0;
+            FN(deferred_correction)(b, c.d, p, fl, phi + b->global_offset, &t);
+            out[b->global_offset + f] = t;
         }
     }
     return 0;

@@ -104,6 +104,8 @@ class DomainDesc(C.Structure):
         ("scalar_viscosity", C.c_void_p),
         ("scalar_rhs", C.c_void_p),
         ("scalar_result", C.c_void_p),
+        ("advection_scheme", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

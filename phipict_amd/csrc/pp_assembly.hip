@@ -88,6 +88,45 @@ __device__ __forceinline__ int emit_row(int *idx, T *vals, bool divide, T det, i
     return written;
 }
 
+// hasFarUpwindCell, shared_stencils.cuh:35-43: the face's neighbour and the far-upwind cell
+// (one further against the flow along the face normal) both lie inside the block.
+__device__ __forceinline__ bool far_upwind(const int *p, int bound, bool outflow, const pp_block_desc &b) {
+    const int axis = bound >> 1, fs = face_sign(bound), last = b.size[axis] - 1;
+    const int iD = p[axis] + fs;
+    if (iD < 0 || iD > last) return false;
+    const int iUU = outflow ? p[axis] - fs : p[axis] + 2 * fs;
+    return iUU >= 0 && iUU <= last;
+}
+
+// LINEAR_UPWIND's deferred correction of one field component at cell p (advection.cu:893-937
+// for the velocity, :650-684 for the passive scalar): the matrix holds first-order upwind, so
+// -F 0.5 psi (phi_D - phi_U) with psi = r = (phi_U - phi_UU) / (phi_D - phi_U) is subtracted
+// for every face whose far-upwind cell exists, skipping flat faces. fl: the cell's face fluxes
+// (before the face sign), phi: the component over the block.
+template <typename T, int D>
+__device__ __forceinline__ void deferred_correction(const pp_block_desc &b, const int *p, const T *fl,
+                                                    const T *__restrict__ phi, T &t) {
+#pragma unroll
+    for (int bound = 0; bound < 2 * D; ++bound) {
+        const int axis = bound >> 1, fs = face_sign(bound);
+        const T ffo = T(fs) * fl[bound];
+        const bool outflow = ffo >= T(0);
+        if (!far_upwind(p, bound, outflow, b)) continue;
+        int q[3] = {p[0], p[1], p[2]};
+        const T phiP = phi[flat3(q, b.size)];
+        q[axis] = p[axis] + fs;
+        const T phiN = phi[flat3(q, b.size)];
+        q[axis] = outflow ? p[axis] - fs : p[axis] + 2 * fs;
+        const T phiUU = phi[flat3(q, b.size)];
+        const T phiU = outflow ? phiP : phiN;
+        const T phiD = outflow ? phiN : phiP;
+        const T dD = phiD - phiU;
+        if ((dD < T(0) ? -dD : dD) < T(1e-20)) continue;
+        const T psi = (phiU - phiUU) / dD;
+        t -= ffo * T(0.5) * psi * dD;
+    }
+}
+
 // Workgroups never straddle blocks: blockIdx.x = block * wpb + tile, so the block index and
 // its descriptor are uniform over the workgroup (scalar loads).
 #define CELL_PROLOGUE                                                          \
@@ -148,9 +187,10 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_csr_pattern(pp_domain dom, int32_
     row[g + 1] = start + w;
 }
 
-// PISO_build_matrix, advection.cu:26-306 (CENTRAL): values of C and the diagonal A. With
-// sc >= 0 the matrix of passive-scalar channel sc (forPassiveScalar): the channel's
-// diffusivity and its boundary condition at the prescribed faces.
+// PISO_build_matrix, advection.cu:26-306: values of C and the diagonal A. With sc >= 0 the
+// matrix of passive-scalar channel sc (forPassiveScalar): the channel's diffusivity and its
+// boundary condition at the prescribed faces. LINEAR_UPWIND puts first-order upwind on the
+// faces whose far-upwind cell lies in the block (advection.cu:129-140), CENTRAL elsewhere.
 template <typename T, int D, bool NO>
 __global__ void __launch_bounds__(ASM_BLOCK) k_advection_matrix(pp_domain dom, int sc, int flags) {
     CELL_PROLOGUE
@@ -173,9 +213,14 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_advection_matrix(pp_domain dom, i
         const int dim = bound >> 1;
         if (!at_bound(p, bound, b) || !is_fixed(b, bound)) {
             const T ffo = T(face_sign(bound)) * fl[bound];
-            const T ff = T(0.5) * ffo;
-            diag += ff;
-            rv[bound + 1] += ff;
+            if (dom.advection_scheme == PP_ADV_LINEAR_UPWIND && far_upwind(p, bound, ffo >= T(0), b)) {
+                if (ffo >= T(0)) diag += ffo;
+                else rv[bound + 1] += ffo;
+            } else {
+                const T ff = T(0.5) * ffo;
+                diag += ff;
+                rv[bound + 1] += ff;
+            }
             int np[3], nw;
             const int nbi = neighbor(blocks, bi, D, p, bound, np, &nw);
             const pp_block_desc &nb = blocks[nbi];
@@ -304,13 +349,17 @@ __device__ __forceinline__ T boundary_source(const pp_domain &dom, const pp_bloc
     return s;
 }
 
-// kPISO_build_advection_RHS, advection.cu:799-955 (CENTRAL)
+// kPISO_build_advection_RHS, advection.cu:799-955 (with LINEAR_UPWIND the deferred correction
+// of the block velocity, lagged to the start of the step)
 template <typename T, int D, bool NO>
 __global__ void __launch_bounds__(ASM_BLOCK) k_advection_rhs(pp_domain dom, int apply_pg, int flags) {
     CELL_PROLOGUE
     const T dt = reinterpret_cast<const T *>(dom.time_step)[env];
     const T det = Metric<T, D>::det(b, f);
     const T *vel = reinterpret_cast<const T *>(b.velocity) + (size_t)env * D * b.cells;
+    const bool deferred = dom.advection_scheme == PP_ADV_LINEAR_UPWIND;
+    T fl[2 * D];
+    if (deferred) cell_fluxes<T, D, false>(dom, blocks, bi, p, f, nullptr, env, fl);
     T *rhs = reinterpret_cast<T *>(dom.velocity_rhs) + (size_t)env * D * dom.num_cells;
     T pg[D];
     if (apply_pg) pressure_grad<T, D>(dom, blocks, bi, p, f, env, pg);
@@ -339,6 +388,7 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_advection_rhs(pp_domain dom, int 
             const T slip = fb.vel_type == PP_BC_DIRICHLET ? T(0) : T(1);
             t += v * (1 - slip) * bv * 2 * alpha;
         }
+        if (deferred) deferred_correction<T, D>(b, p, fl, vel + (size_t)dim * b.cells, t);
         if (NO && flags) t -= Sv[dim];
         t /= det;
         t += vsource<T>(dom, b, f, dim, env);
@@ -347,10 +397,10 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_advection_rhs(pp_domain dom, int 
     }
 }
 
-// kPISO_build_scalar_advection_RHS, advection.cu:544-700 (CENTRAL): every passive-scalar
-// channel; prescribed faces give the advected face value and the diffusive flux (Dirichlet:
-// the face value with the ghost-cell coefficient 2 alpha; Neumann: the cell value advected and
-// the prescribed normal gradient times the face area)
+// kPISO_build_scalar_advection_RHS, advection.cu:544-700: every passive-scalar channel;
+// prescribed faces give the advected face value and the diffusive flux (Dirichlet: the face
+// value with the ghost-cell coefficient 2 alpha; Neumann: the cell value advected and the
+// prescribed normal gradient times the face area); with LINEAR_UPWIND the deferred correction
 template <typename T, int D, bool NO>
 __global__ void __launch_bounds__(ASM_BLOCK) k_scalar_rhs(pp_domain dom, int flags) {
     CELL_PROLOGUE
@@ -359,6 +409,9 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_scalar_rhs(pp_domain dom, int fla
     const T det = Metric<T, D>::det(b, f);
     const T *phi = reinterpret_cast<const T *>(b.scalar) + (size_t)env * C * b.cells;
     T *rhs = reinterpret_cast<T *>(dom.scalar_rhs) + (size_t)env * C * dom.num_cells;
+    const bool deferred = dom.advection_scheme == PP_ADV_LINEAR_UPWIND;
+    T fl[2 * D];
+    if (deferred) cell_fluxes<T, D, false>(dom, blocks, bi, p, f, nullptr, env, fl);
     for (int ch = 0; ch < C; ++ch) {
         const T phiP = phi[(size_t)ch * b.cells + f];
         T t = det * phiP / dt;
@@ -387,6 +440,7 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_scalar_rhs(pp_domain dom, int fla
                 t += s * visc * area;
             }
         }
+        if (deferred) deferred_correction<T, D>(b, p, fl, phi + (size_t)ch * b.cells, t);
         if (NO && flags) {
             const CornerData<T, D, NK_SCALAR> cd{dom, blocks, ch, env};
             T S[1];

@@ -159,6 +159,18 @@ class CSRmatrix:
     def ell_valid(self) -> bool:
         return self.ell_val is not None and self._ell_key == self._value_key()
 
+    def refresh_ell(self) -> None:
+        """Rewrite the ELL copy's values from the CSR values when they were changed outside the
+        library's assembly (kernels that read the matrix through its ELL copy then see them)."""
+        if self.ell_val is None or self.ell_valid():
+            return
+        nnz = self.index.numel()
+        nmat = self.value.numel() // max(nnz, 1)
+        _lib.call("pp_ell_from_csr", _lib.ptr(self.row), _lib.ptr(self.index), _lib.ptr(self.value), self.getRows(),
+                  nnz, nmat, self.ell_width, _lib.dtype_code(self.value.dtype), None, _lib.ptr(self.ell_val),
+                  _lib.dtype_code(self.ell_val.dtype), _lib.stream_ptr(self.value.device))
+        self.mark_ell_valid()
+
     def share_ell_from(self, other: "CSRmatrix") -> None:
         """Use ``other``'s ELL copy for this matrix (same pattern and value storage)."""
         if other.ell_valid() and other.value.data_ptr() == self.value.data_ptr():
@@ -401,6 +413,7 @@ class FixedBoundary(Boundary):
         self.transform = t
         self._domain._changed = True
         self._domain._bc_generation += 1
+        self._domain._transform_generation += 1
 
     def hasTransform(self) -> bool:
         return self.transform is not None
@@ -675,6 +688,7 @@ class Block:
             if isinstance(b, FixedBoundary):
                 b.setTransform(self.faceTransformBoundarySlice(b.face))
         self._domain._changed = True
+        self._domain._transform_generation += 1
 
     def setTransform(self, transform: torch.Tensor, faceTransform: Optional[torch.Tensor] = None) -> None:
         dims = self.getSpatialDims()
@@ -687,6 +701,7 @@ class Block:
             if isinstance(b, FixedBoundary):
                 b.setTransform(self.faceTransformBoundarySlice(b.face) if self.faceTransform is not None else None)
         self._domain._changed = True
+        self._domain._transform_generation += 1
 
     def hasTransform(self) -> bool:
         return self.transform is not None
@@ -748,7 +763,7 @@ class Block:
         if not self._orthogonality()[1]:
             return False
         faces = [b.transform for b in self.getBoundaries() if isinstance(b, FixedBoundary) and b.transform is not None]
-        key = tuple((t.data_ptr(), t._version) for t in faces)
+        key = (self._domain._transform_generation,) + tuple((t.data_ptr(), t._version) for t in faces)
         cached = getattr(self, "_face_ortho", None)
         if cached is None or cached[0] != key:
             # one device sync per new set of face transforms
@@ -760,7 +775,7 @@ class Block:
         t = self.transform
         if t is None:
             return True, True
-        key = (t.data_ptr(), t._version)
+        key = (self._domain._transform_generation, t.data_ptr(), t._version)
         if getattr(self, "_ortho_key", None) != key:
             self._ortho = _metric_orthogonality(t, self.getSpatialDims())
             self._ortho_key = key
@@ -959,6 +974,9 @@ class Domain:
         self._atlas_slot = -1
         self._flux_cache: Optional[tuple] = None
         self._bc_generation = 0  # bumped by every change of a boundary object, velocity or transform
+        # bumped by every new cell or face transform: keys the orthogonality caches, which a
+        # tensor's address and version alone cannot (a new tensor may reuse a freed address)
+        self._transform_generation = 0
         self._desc = _lib.DomainDesc()
         # bumped by every assembly of the advection matrix: the pressure operator depends on
         # the advection diagonal only, so solvers key their pressure caches on it
@@ -1057,9 +1075,11 @@ class Domain:
         self._changed = True
 
     def setAdvectionScheme(self, scheme) -> None:
-        if AdvectionScheme(scheme) != AdvectionScheme.CENTRAL:
-            raise NotImplementedError("only the CENTRAL advection scheme is implemented")
+        """Convective scheme of the momentum and passive-scalar equations (Domain::setAdvectionScheme,
+        bindings.cpp:518-521): CENTRAL, or LINEAR_UPWIND (implicit first-order upwind plus an
+        explicit second-order upwind correction on the right-hand side)."""
         self.advectionScheme = AdvectionScheme(scheme)
+        self._changed = True
 
     def getAdvectionScheme(self) -> AdvectionScheme:
         return self.advectionScheme
@@ -1395,6 +1415,7 @@ class Domain:
         d.scalar_viscosity = _lib.ptr(vs) if scalar else None
         d.scalar_rhs = _lib.ptr(self.scalarRHS) if scalar else None
         d.scalar_result = _lib.ptr(self.scalarResult) if scalar else None
+        d.advection_scheme = int(self.advectionScheme)
         self._changed = False
 
     ATLAS_RING = 16

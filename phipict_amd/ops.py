@@ -100,6 +100,9 @@ def SetupPressureRHS(domain: Domain, timeStep, nonOrthoMode: int = 0, useFaceTra
                      timeStepNorm: bool = False) -> None:
     if useFaceTransform:
         raise NotImplementedError("face-transform fluxes are not implemented")
+    if domain.C is not None:
+        # HbyA reads C through its ELL copy: values written since the assembly are copied over
+        domain.C.refresh_ell()
     d = _prep(domain, timeStep)
     _lib.call("pp_setup_pressure_rhs", C.byref(d), _flags(domain, nonOrthoMode), int(bool(timeStepNorm)), _stream(domain))
 

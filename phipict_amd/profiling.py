@@ -18,6 +18,7 @@ KERNEL_NAMES = {
     30: "pcg_dots", 31: "pcg_direction", 32: "pcg_update", 33: "amg_presmooth", 34: "amg_jacobi",
     35: "amg_level_residual", 36: "amg_restrict", 37: "amg_prolong", 38: "amg_coarse", 39: "mean", 40: "subtract_mean",
     41: "amg_post_jacobi", 42: "amg_tail", 50: "spmv", 51: "galerkin",
+    60: "seg_cg_direction", 61: "seg_cg_update", 62: "seg_bicg",
 }
 
 
@@ -69,12 +70,16 @@ TIMER_TO_KERNEL = {
     "advection_matrix": "k_advection_matrix", "advection_rhs": "k_advection_rhs", "pressure_matrix": "k_pressure_matrix",
     "pressure_rhs": "k_pressure_rhs", "pressure_div": "k_pressure_div", "correct_velocity": "k_correct_velocity",
     "epot_rhs": "k_epot_rhs", "current_density": "k_current_density", "copy_result": "k_copy_result",
-    "residual": "k_residual", "cg_direction": "k_cg_direction", "cg_update": "k_cg_update", "bicg_p": "k_bicg_p",
-    "bicg_v": "k_bicg_v", "bicg_t": "k_bicg_t", "bicg_update": "k_bicg_update", "pcg_dots": "k_cg_dots",
-    "pcg_direction": "k_cg_direction", "pcg_update": "k_cg_update", "amg_presmooth": "k_presmooth_residual",
-    "amg_jacobi": "k_jacobi_sweep", "amg_post_jacobi": "k_jacobi_sweep", "amg_tail": "k_vcycle_tail",
-    "mean": "k_mean", "subtract_mean": "k_subtract_mean", "galerkin": "k_galerkin_apply",
+    "cg_direction": "k_cg_turn", "cg_update": "k_cg_step", "bicg_p": "k_bicg_dir", "bicg_v": "k_bicg_apply",
+    "bicg_t": "k_bicg_stab", "bicg_update": "k_bicg_step", "pcg_direction": "k_pcg_dir", "pcg_update": "k_cg_step",
+    "amg_presmooth": "k_level", "amg_jacobi": "k_level", "amg_post_jacobi": "k_level", "amg_restrict": "k_level",
+    "amg_prolong": "k_level", "amg_tail": "k_vcycle_tail", "galerkin": "k_galerkin_apply",
+    "seg_cg_direction": "k_cg_turn", "seg_cg_update": "k_cg_step",
 }
+#: the level pass of each AMG timer: k_level's first template argument (OP_FIRST, OP_SWEEP,
+#: OP_APPLY); the finest level's first sweep runs inside k_pcg_decide
+AMG_TIMER_OP = {"amg_presmooth": "0", "amg_jacobi": "1", "amg_post_jacobi": "1", "amg_restrict": "3",
+                "amg_prolong": "3"}
 
 
 def split_kernel_name(name: str) -> tuple:

@@ -80,9 +80,11 @@ def build_duct(req: DuctRequest, device: torch.device):
     """The request's domain and MHD simulation: tanh wall refinement 1.5 on [-1, 1], length 4,
     nu = 1/Re, N = Ha^2/Re, source N (core velocity of order one), e_b = (0, 1, 0), seeded
     N(0, 1e-3) initial velocity; AMG-preconditioned CG for pressure and potential at the
-    request's absolute tolerance, the pressure's raised to the dtype's rounding floor of its
-    right-hand side (8 eps ||b|| / sqrt(n): a float32 solve stalls near 1e-8 on small grids),
-    the potential solved in float64; BiCGStab momentum."""
+    request's absolute tolerance, the pressure's raised to 64 eps ||b|| / sqrt(n) of the
+    request's dtype, the potential solved in float64; BiCGStab momentum. A float32 pressure solve
+    can stall above 1e-8 (its rounding floor follows the pressure, not the right-hand side, whose
+    scale is small in a periodic duct): float32 requests keep the best iterate there, with the
+    reference's warning, instead of failing (pressure_return_best_result)."""
     import phipict_amd as pa
 
     dt = req.dtype
@@ -112,7 +114,8 @@ def build_duct(req: DuctRequest, device: torch.device):
     sim = pa.MHDSimulation(domain=dom, dt=req.dt, stuart_number=torch.tensor(N, dtype=dt),
                            e_b=torch.tensor([0.0, 1.0, 0.0], dtype=dt), non_orthogonal=False,
                            pressure_use_amg=True, potential_use_preconditioner=True,
-                           pressure_tol=pa.SolverTolerance(atol=req.tol),
+                           pressure_tol=pa.SolverTolerance(rtol=64.0 * torch.finfo(dt).eps, atol=req.tol),
+                           pressure_return_best_result=dt == torch.float32,
                            potential_tol=req.tol, potential_solve_dtype=torch.float64,
                            hooks={"PRE_VELOCITY_SETUP": forcing})
     return dom, sim

@@ -341,8 +341,6 @@ class Simulation:
             raise TypeError(f"unexpected argument {k!r}")
         if hooks is not None and prep_fn is not None:
             raise ValueError("Pass hooks or its former name prep_fn, not both.")
-        if differentiable and non_orthogonal:
-            raise NotImplementedError("the differentiable step covers the orthogonal path (non_orthogonal=False)")
         if differentiable and velocity_corrector != "FD":
             raise NotImplementedError("the differentiable step uses the FD velocity corrector")
         self.domain = domain
@@ -824,6 +822,15 @@ class Simulation:
         d, B = domain.getSpatialDims(), domain.getBatchSize()
         if self.advect_passive_scalar and domain.hasPassiveScalar():
             raise NotImplementedError("the differentiable step does not advect passive scalars")
+        if int(domain.getAdvectionScheme()) != 0:
+            raise NotImplementedError("the differentiable step covers the CENTRAL advection scheme")
+        if self.non_orthogonal and not domain.isOrthogonal():
+            # with non_orthogonal (the reference's default, piso_simulation.py:1469) the step is
+            # the orthogonal one on grids whose metric is orthogonal to 1e-10 relative
+            # (rectilinear grids: the face transforms of fixed faces carry off-diagonal terms of
+            # rounding size only, whose corner terms are negligible)
+            raise NotImplementedError("the differentiable step covers grids whose non-orthogonal corner terms vanish "
+                                      "(rectilinear grids); this grid is skewed")
         tsn = bool(self.pressure_time_step_normalized)
         for step in range(int(iterations)):
             self._run_hook(Hook.PRE, local_step=step, time_step=time_step)

@@ -23,7 +23,8 @@ def _specs():
     return cases.outflow2d(nx=14, ny=8, seed=7)
 
 
-def _run(pert: dict, eps: float, steps: int = 2, differentiable: bool = False, grads: dict = None):
+def _run(pert: dict, eps: float, steps: int = 2, differentiable: bool = False, grads: dict = None,
+         non_orthogonal: bool = False):
     """Two MHD steps of the outflow channel with every input moved by eps times its direction;
     returns the loss (and, differentiable, the inputs holding gradients)."""
     import phipict_amd as pa
@@ -52,7 +53,7 @@ def _run(pert: dict, eps: float, steps: int = 2, differentiable: bool = False, g
         domain.UpdateDomainData()
 
     tol = pa.SolverTolerance(rtol=1e-13)
-    sim = pa.MHDSimulation(domain=dom, dt=2e-2, stuart_number=N, e_b=e_b, non_orthogonal=False,
+    sim = pa.MHDSimulation(domain=dom, dt=2e-2, stuart_number=N, e_b=e_b, non_orthogonal=non_orthogonal,
                            advection_tol=tol, pressure_tol=tol, potential_tol=tol, differentiable=differentiable,
                            hooks={"PRE_VELOCITY_SETUP": reset}, flux_balance_tol=1e-3)
     for _ in range(steps):
@@ -103,6 +104,35 @@ def test_step_gradient_matches_finite_differences(which):
     print(f"analytic {analytic:.10e} fd {fd:.10e}", flush=True)
     assert abs(fd) > 1e-4, "the loss does not depend on this input"
     assert abs(analytic - fd) <= 1e-5 * abs(fd), (analytic, fd)
+
+
+def test_default_non_orthogonal_mode_on_a_rectilinear_grid():
+    """differentiable=True with the reference's default non_orthogonal=True
+    (piso_simulation.py:1469): on a rectilinear grid the loss equals that of
+    non_orthogonal=False bit for bit and every gradient to the rounding of the adjoint kernels'
+    atomic sums; a skewed grid is refused."""
+    import phipict_amd as pa
+
+    print("[adjoint] non_orthogonal=True on a rectilinear grid", flush=True)
+    zero = {k: (v * 0 if isinstance(v, torch.Tensor) else 0.0) for k, v in _directions().items()}
+    runs = []
+    for non_ortho in (False, True):
+        grads: dict = {}
+        loss = _run(zero, 0.0, differentiable=True, grads=grads, non_orthogonal=non_ortho)
+        runs.append((loss, grads))
+    assert runs[0][0] == runs[1][0]
+    for k in runs[0][1]:
+        # the adjoint kernels scatter with atomic adds: equal up to the order of those sums
+        a, b = runs[0][1][k].double(), runs[1][1][k].double()
+        assert float((a - b).norm()) <= 1e-12 * float(a.norm()), k
+    specs = cases.skew(_specs(), 0.2, seed=3)
+    dom = build_domain(specs, 0.05, 2e-2, F64)
+    pa.simulation.balance_boundary_fluxes(dom, [dom.getBlocks()[0].getBoundary(1)])
+    dom.UpdateDomainData()
+    sim = pa.MHDSimulation(domain=dom, dt=2e-2, stuart_number=torch.tensor(3.0, dtype=F64),
+                           e_b=torch.tensor([0.0, 0.6, 0.8], dtype=F64), differentiable=True, flux_balance_tol=1e-3)
+    with pytest.raises(NotImplementedError, match="skewed"):
+        sim.single_step()
 
 
 def _run_blocks(case: str, dirs, eps: float, differentiable: bool = False, grads: list = None, batch: int = 1):

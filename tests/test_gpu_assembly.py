@@ -167,6 +167,59 @@ def test_batched_assembly_per_env(dtype):
 NONORTHO_CASES = ["channel2d", "duct3d", "inflow2d_varying", "split2d", "rotated3d", "tjunction3d", "heated2d"]
 
 
+def test_hbya_reads_values_written_after_the_assembly(dtype):
+    """HbyA reads C through the ELL copy the assembly writes; values written into C afterwards
+    (a hook adding terms, an adjoint's perturbation) reach it as they reach the reference's CSR
+    read (pressure.cu:349-522)."""
+    import phipict_amd as pa
+
+    specs = cases.inflow2d()
+    od = O.OracleDomain(specs, NU, DT, dtype)
+    dom = build_domain(specs, NU, DT, _tdt(dtype))
+    ts = torch.tensor([DT], dtype=_tdt(dtype))
+    pa.SetupAdvectionMatrix(dom, ts, 0)
+    ccol, cval, A = od.advection_matrix()
+    cval = (cval * (1.0 + 0.1 * np.random.default_rng(2).standard_normal(cval.size))).astype(dtype)
+    dom.C.value.copy_(torch.as_tensor(cval, device=dom.getDevice()))
+    vres = np.random.default_rng(3).standard_normal(od.dims * od.ncells).astype(dtype)
+    dom.setVelocityResult(torch.as_tensor(vres, device=dom.getDevice()))
+    pa.SetupPressureRHS(dom, ts, 0)
+    close(to_np(dom.pressureRHS), od.pressure_rhs(A, ccol, cval, vres), dtype, "HbyA with modified C")
+
+
+@pytest.mark.parametrize("name", list(cases.CASES))
+def test_linear_upwind_assembly_matches_oracle(name, dtype):
+    """Domain.setAdvectionScheme(LINEAR_UPWIND) (advection.cu:129-140, 893-937, 650-684): the
+    upwinded advection matrix and diagonal, the momentum right-hand side with the deferred
+    correction, and for passive scalars their matrices and right-hand sides, against the
+    oracle."""
+    print(f"[assembly] LINEAR_UPWIND case={name} dtype={np.dtype(dtype).name}", flush=True)
+    import phipict_amd as pa
+
+    specs = cases.CASES[name]()
+    od = O.OracleDomain(specs, NU, DT, dtype, advection_scheme=O.ADV_LINEAR_UPWIND)
+    dom = build_domain(specs, NU, DT, _tdt(dtype))
+    dom.setAdvectionScheme(pa.AdvectionScheme.LINEAR_UPWIND)
+    assert dom.getAdvectionScheme() == pa.AdvectionScheme.LINEAR_UPWIND
+    ts = torch.tensor([DT], dtype=_tdt(dtype))
+    pa.SetupAdvectionMatrix(dom, ts, 0)
+    ccol, cval, A = od.advection_matrix()
+    close(to_np(dom.C.value), cval, dtype, "advection matrix")
+    close(to_np(dom.A), A, dtype, "advection diagonal")
+    pa.SetupAdvectionVelocity(dom, ts, 0, False)
+    close(to_np(dom.velocityRHS), od.advection_rhs(False), dtype, "momentum rhs")
+    central = O.OracleDomain(specs, NU, DT, dtype)
+    assert rel_l2(od.advection_rhs(False), central.advection_rhs(False)) > 0
+    if specs[0].scalar is not None:
+        C_ = od.channels
+        for ch in range(C_):
+            pa.SetupAdvectionMatrix(dom, ts, 0, forPassiveScalar=True, passiveScalarChannel=ch)
+            _, sval, _ = od.scalar_matrix(ch)
+            close(to_np(dom.C.value), sval, dtype, f"scalar matrix {ch}")
+        pa.SetupAdvectionScalar(dom, ts, 0)
+        close(to_np(dom.scalarRHS), od.scalar_rhs().reshape(-1), dtype, "scalar rhs")
+
+
 @pytest.mark.parametrize("flags", [O.NO_DEFAULT_MODE, O.NO_DEFAULT_MODE | O.NO_DIRECT_RHS, O.NO_DIRECT_MATRIX])
 @pytest.mark.parametrize("name", NONORTHO_CASES)
 def test_nonortho_assembly_matches_oracle(name, flags, dtype):

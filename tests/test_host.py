@@ -215,7 +215,10 @@ def test_simulation_rejects_unimplemented_options():
 
     dom = _cpu_domain(2)
     with pytest.raises(NotImplementedError):
-        pa.Simulation(domain=dom, dt=0.1, differentiable=True)
+        pa.Simulation(domain=dom, dt=0.1, differentiable=True, velocity_corrector="FVM")
+    # the reference's default non_orthogonal=True is accepted with differentiable=True: the step
+    # refuses skewed grids when it runs
+    pa.Simulation(domain=dom, dt=0.1, differentiable=True)
     with pytest.raises(TypeError):
         pa.Simulation(domain=dom, dt=0.1, no_such_option=1)
     with pytest.raises(ValueError):

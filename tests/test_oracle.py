@@ -153,6 +153,48 @@ def test_advection_rhs_and_hbya_closed_form():
     np.testing.assert_allclose(prhs, expect, rtol=1e-12)
 
 
+def test_linear_upwind_closed_form():
+    """LINEAR_UPWIND (advection.cu:129-140, 893-937): first-order upwind in the matrix plus the
+    deferred correction -F 0.5 psi (phi_D - phi_U), psi = (phi_U - phi_UU) / (phi_D - phi_U), on
+    the faces whose far-upwind cell lies in the block. On a uniform grid with flow along +x,
+    R = C u - rhs then differs from CENTRAL's by (-0.5 F_e D2u_i - 0.5 F_w' D2u_(i-1)) / det
+    (F_e the east face's outward flux, F_w' the west face's, D2u_i = u_(i+1) - 2 u_i + u_(i-1))
+    on the cells whose faces are all corrected: zero for a field linear in x, the second
+    difference for a quadratic one. The scheme's advective off-diagonals are never positive."""
+    nx, ny, lx, ly = 16, 4, 2.0, 1.0
+    dx, dy = lx / nx, ly / ny
+    x = (np.arange(nx) + 0.5) * dx
+    for coef in ((1.0, 0.3, 0.0), (1.0, 0.3, 0.8)):
+        ux = coef[0] + coef[1] * x + coef[2] * x * x
+        specs = periodic_box([nx, ny], [lx, ly])
+        specs[0].velocity = np.stack([np.broadcast_to(ux, (ny, nx)), np.zeros((ny, nx))]).copy()
+        R = {}
+        for scheme in (O.ADV_CENTRAL, O.ADV_LINEAR_UPWIND):
+            od = O.OracleDomain(specs, 1e-4, 0.01, advection_scheme=scheme)
+            col, val, A = od.advection_matrix()
+            u = specs[0].velocity.reshape(2, -1)
+            Cm = dense(od.row, col, val)
+            R[scheme] = np.stack([Cm @ u[k] for k in range(2)]) - od.advection_rhs().reshape(2, -1)
+            if scheme == O.ADV_LINEAR_UPWIND:
+                rows = np.repeat(np.arange(od.ncells), np.diff(od.row))
+                off = col != rows
+                ix = rows % nx
+                inner = (ix >= 2) & (ix <= nx - 3)
+                # viscous off-diagonals are -nu alpha; no advective part makes them positive
+                assert (val[off & inner] <= 0).all()
+        diff = (R[O.ADV_LINEAR_UPWIND] - R[O.ADV_CENTRAL]).reshape(2, ny, nx)
+        Fe = dy * 0.5 * (ux[2:-2] + ux[3:-1])          # outward flux through the east face
+        Fw = -dy * 0.5 * (ux[1:-3] + ux[2:-2])         # outward flux through the west face
+        d2 = ux[2:] - 2 * ux[1:-1] + ux[:-2]           # D2u at cells 1..nx-2
+        expect = (-0.5 * Fe * d2[1:-1] - 0.5 * Fw * d2[:-2]) / (dx * dy)
+        np.testing.assert_allclose(diff[0][:, 2:-2], np.broadcast_to(expect, (ny, nx - 4)), rtol=1e-9, atol=1e-11)
+        np.testing.assert_allclose(diff[1], 0.0, atol=1e-11)
+        if coef[2] == 0.0:
+            assert np.abs(diff[0][:, 2:-2]).max() < 1e-10
+        else:
+            assert np.abs(expect).min() > 1e-3
+
+
 def test_divergence_gradient_current_closed_form():
     """Face-interpolated divergence, FD gradient correction and face-based current density of
     Fourier modes on a uniform periodic grid."""
@@ -213,6 +255,49 @@ def test_epot_row_sums_only_at_anchored_faces():
     od = O.OracleDomain(cases.duct3d(), 0.01, 0.02)
     ec, ev = od.epot_matrix()
     np.testing.assert_allclose(dense(od.row, ec, ev).sum(axis=1), 0, atol=1e-12)
+
+
+@pytest.mark.parametrize("mixed", [False, True], ids=["whole_face", "mixed_face"])
+def test_thin_wall_surface_laplacian_closed_form(mixed):
+    """Thin-wall condition dphi/dn = C_w lap_tau(phi) at a wall (potential.cu:164-218): on a
+    rectilinear box, periodic in x with unequal cell widths dx_i, the potential matrix with a
+    thin wall on -y differs from the insulating one only in the wall cells' rows, by the 1D
+    surface Laplacian C_w / 2 (1/dx_i + 1/dx_j) (phi_j - phi_i) over the tangential neighbours j
+    (alpha_x |Minv_y| = (dx dy / dx^2) / dy = 1 / dx); on a mixed face only between two
+    thin-wall face cells."""
+    rng = np.random.default_rng(3)
+    nx, ny, cw = 10, 5, 0.4
+    xs = np.concatenate([[0.0], np.cumsum(rng.uniform(0.5, 1.5, nx))])
+    ys = np.array([-1.0, -0.7, -0.2, 0.3, 0.6, 1.0])
+    v = cases.vertices([xs, ys])
+    cz = cases.cell_shape(v)
+    thin = np.zeros(nx, bool)
+    thin[:6 if mixed else nx] = True
+
+    def domain(with_thin):
+        if with_thin and mixed:
+            types = np.where(thin, O.POT_THIN_WALL, O.POT_INSULATING).astype(np.int8)
+            bottom = O.FaceSpec("fixed", pot_types=types, cw=cw)
+        elif with_thin:
+            bottom = O.FaceSpec("fixed", pot=O.POT_THIN_WALL, cw=cw)
+        else:
+            bottom = O.FaceSpec("fixed")
+        faces = [O.FaceSpec("periodic"), O.FaceSpec("periodic"), bottom, O.FaceSpec("fixed")]
+        od = O.OracleDomain([O.BlockSpec(v, faces, velocity=np.zeros([2] + cz), pressure=np.zeros(cz))], 0.01, 0.02)
+        ec, ev = od.epot_matrix()
+        return dense(od.row, ec, ev)
+
+    D = domain(True) - domain(False)
+    dx = np.diff(xs)
+    expect = np.zeros_like(D)
+    for i in range(nx):
+        for j in ((i - 1) % nx, (i + 1) % nx):
+            if thin[i] and thin[j]:
+                c = cw * 0.5 * (1.0 / dx[i] + 1.0 / dx[j])
+                expect[i, j] += c
+                expect[i, i] -= c
+    np.testing.assert_allclose(D, expect, atol=1e-13)
+    assert np.abs(expect).max() > 0.1
 
 
 def test_constant_potential_with_dirichlet_face():
