@@ -389,6 +389,20 @@ int pp_solve_segmented(const pp_csr *A, const int32_t *seg_rows, int32_t nseg, i
                        int32_t dtype, int32_t maxit, const double *tols, int32_t ntols, int32_t use_bicg,
                        int32_t project, int32_t tile_rows, void *stream, pp_solve_result *results);
 
+/* No counterpart in the reference: pp_solve_linear for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, results and kernels as pp_solve_linear, but every work buffer (vectors, partial
+ * sums, solve records, tolerances) comes from one static device array of 63.1 MiB per GPU (56
+ * MiB of work vectors: e.g. seven vectors of 1.0 M doubles), the host snapshots from a static
+ * host array (at most 2048 right-hand sides) and the snapshot events are created at the first
+ * call on a device; it calls no hipMalloc, hipFree or other allocating routine. Larger solves
+ * and transposed BiCGStab solves are refused with an error. Calls on one GPU share the array:
+ * calls that overlap on one GPU (from other streams or threads) race. */
+int pp_solve_linear_noalloc(const pp_csr *A, const void *rhs, void *x, int32_t nrhs, int32_t dtype,
+                            int32_t maxit, const double *tols, int32_t ntols, int32_t criterion,
+                            int32_t use_bicg, int32_t matrix_rank_deficient, int32_t residual_reset_steps,
+                            int32_t transpose_a, int32_t return_best, void *stream, pp_solve_result *results);
+
 /* replaces ReleaseKrylovWorkspaces (bindings.cpp:777, krylov.cu:921-924). */
 int pp_release_workspaces(void);
 

@@ -197,6 +197,8 @@ SIGNATURES = {
     "pp_transform_vectors": [_i32, _i32, _i32, _vp, _vp, _i32, _i32, _vp, _i32, _vp],
     "pp_solve_linear": [_P(CsrDesc), _vp, _vp, _i32, _i32, _i32, _P(_dbl), _i32, _i32, _i32, _i32, _i32, _i32, _i32,
                         _vp, _P(SolveResult)],
+    "pp_solve_linear_noalloc": [_P(CsrDesc), _vp, _vp, _i32, _i32, _i32, _P(_dbl), _i32, _i32, _i32, _i32, _i32, _i32, _i32,
+                        _vp, _P(SolveResult)],
     "pp_amg_pcg_solve": [_P(CsrDesc), _P(AmgLevel), _i32, _vp, _i32, _i32, _i32, _vp, _vp, _i32, _i32, _i32, _dbl,
                          _i32, _i32, _i32, _i32, _i32, _dbl, _P(_dbl), _i32, _vp, _P(SolveResult)],
     "pp_release_workspaces": [],

@@ -1045,10 +1045,13 @@ struct Space {
     void *tailDev = nullptr; size_t tailCap = 0;
     void *tailHost = nullptr;
     char *snap = nullptr;  size_t snapCap = 0;  // pinned snapshots: two slots of B records
+    bool fixed = false;    // carved from the static scratch array: never grows, never freed
+    hipEvent_t ev[2] = {nullptr, nullptr}; // snapshot events created once (fixed spaces)
 };
 // Workspaces belong to the calling host thread: solves issued from different threads (each
 // on its own stream) run concurrently without sharing buffers. g_lock guards what is shared
-// (settings, the graph cache, the capture streams, the registry of workspaces).
+// (settings, the graph cache, the capture streams, the registry of workspaces, the static
+// scratch of the allocation-free entry point).
 static std::mutex g_lock;
 
 // The workspaces of one host thread: CG, BiCGStab, AMG-PCG, the segmented engine (with its
@@ -1118,6 +1121,18 @@ struct PartsCarver {
 };
 
 static int prepare(Space &sp, size_t vecBytes, size_t partDoubles, int B, const double *tols, int ntols, hipStream_t s) {
+    if (sp.fixed) {
+        const size_t recBytes = 2 * sizeof(SolveRec) * (size_t)B + sizeof(double) * B;
+        if (vecBytes > sp.vecCap || partDoubles * sizeof(double) > sp.partsCap || recBytes > sp.recsCap ||
+            2 * sizeof(SolveRec) * (size_t)B > sp.snapCap || sizeof(double) * (size_t)ntols > sp.tolsCap) {
+            set_error("solve exceeds the static scratch maximum (" + std::to_string(sp.vecCap >> 20) +
+                      " MiB of work vectors, " + std::to_string(sp.snapCap / (2 * sizeof(SolveRec))) +
+                      " right-hand sides): use pp_solve_linear");
+            return PP_ERR_ARG;
+        }
+        if (ntols > 1) PP_HIP(hipMemcpyAsync(sp.tols, tols, sizeof(double) * ntols, hipMemcpyHostToDevice, s));
+        return PP_OK;
+    }
     if (int e = grow(&sp.vec, &sp.vecCap, vecBytes)) return e;
     if (int e = grow(&sp.parts, &sp.partsCap, partDoubles * sizeof(double))) return e;
     if (int e = grow(&sp.recs, &sp.recsCap, 2 * sizeof(SolveRec) * B + sizeof(double) * B)) return e;
@@ -1248,18 +1263,21 @@ static int pace_solve(Space &sp, hipStream_t s, const SolveRec *decided, int B, 
                       Open stopAfterOpening, Body body, const std::vector<uintptr_t> *graphKey = nullptr) {
     const size_t bytes = sizeof(SolveRec) * (size_t)B;
     const SolveRec *slots[2] = {reinterpret_cast<SolveRec *>(sp.snap), reinterpret_cast<SolveRec *>(sp.snap + bytes)};
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {sp.ev[0], sp.ev[1]};
+    const bool ownEvents = !(ev[0] && ev[1]); // fixed spaces bring events created at set-up
     int rc = PP_OK;
     auto cleanup = [&]() {
-        for (auto &e : ev)
-            if (e) (void)hipEventDestroy(e);
+        if (ownEvents)
+            for (auto &e : ev)
+                if (e) (void)hipEventDestroy(e);
     };
-    for (auto &e : ev)
-        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
-            cleanup();
-            set_error("pace_solve: event creation failed");
-            return PP_ERR_HIP;
-        }
+    if (ownEvents)
+        for (auto &e : ev)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+                cleanup();
+                set_error("pace_solve: event creation failed");
+                return PP_ERR_HIP;
+            }
     auto snap = [&](int slot) -> int {
         if (hipMemcpyAsync((void *)slots[slot], decided, bytes, hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipEventRecord(ev[slot], s) != hipSuccess) {
@@ -2342,11 +2360,13 @@ static int seg_solve(const Csr<T> &A, const int32_t *segRows, int S, int ncomp, 
     return PP_OK;
 }
 
-// The body of pp_solve_linear (spaces: the CG and the BiCGStab workspaces).
+// The body of pp_solve_linear and of its allocation-free variant (spaces: the CG and the
+// BiCGStab workspaces; allowTranspose: the transposed-operator buffer may be allocated).
 static int solve_linear(const pp_csr *A, const void *rhs, void *x, int32_t nrhs, int32_t dtype, int32_t maxit,
                         const double *tols, int32_t ntols, int32_t criterion, int32_t use_bicg,
                         int32_t matrix_rank_deficient, int32_t residual_reset_steps, int32_t transpose_a,
-                        int32_t return_best, void *stream, pp_solve_result *results, Space &cgSpace, Space &bicgSpace) {
+                        int32_t return_best, void *stream, pp_solve_result *results, Space &cgSpace, Space &bicgSpace,
+                        bool allowTranspose) {
     PP_CHECK_ARG(A && A->row && A->col && A->val && rhs && x && tols && results, "null argument");
     PP_CHECK_ARG(A->n > 0 && nrhs > 0, "empty system");
     PP_CHECK_ARG(A->num_matrices >= 1 && nrhs % A->num_matrices == 0,
@@ -2355,6 +2375,8 @@ static int solve_linear(const pp_csr *A, const void *rhs, void *x, int32_t nrhs,
     PP_CHECK_ARG(criterion == PP_CONV_NORM2 || criterion == PP_CONV_NORM2_NORMALIZED, "unsupported convergence criterion");
     PP_CHECK_ARG(maxit >= 0, "maxit must be non-negative");
     PP_CHECK_ARG(!(use_bicg && matrix_rank_deficient), "rank-deficient mode is a CG mode");
+    PP_CHECK_ARG(allowTranspose || !(transpose_a && use_bicg),
+                 "transposed BiCGStab solves need a transposed copy of the matrix: use pp_solve_linear");
     hipStream_t s = as_stream(stream);
     const int rpm = nrhs / A->num_matrices;
     const double scale = criterion == PP_CONV_NORM2_NORMALIZED ? 1.0 / std::sqrt(double(A->n)) : 1.0;
@@ -2392,6 +2414,41 @@ static int solve_linear(const pp_csr *A, const void *rhs, void *x, int32_t nrhs,
     return PP_OK;
 }
 
+// ---------------- the allocation-free variant ----------------
+// Work vectors, partial sets, records and tolerances of pp_solve_linear_noalloc live in one
+// static device array per GPU, carved into fixed regions at the first call on a device (its
+// address from hipGetSymbolAddress, kept per device); host snapshots live in a static host
+// array and the snapshot events are created at that set-up too.
+constexpr size_t SCR_VEC = size_t(56) << 20, SCR_PARTS = size_t(6) << 20, SCR_RECS = size_t(1) << 20,
+                 SCR_TOLS = size_t(64) << 10;
+constexpr size_t SCRATCH_BYTES = SCR_VEC + SCR_PARTS + SCR_RECS + SCR_TOLS;
+constexpr int SCRATCH_RHS = 2048;     // right-hand sides per call at most
+constexpr int SCRATCH_DEVICES = 16;   // devices of one process with a scratch table entry
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+static Space g_fixed[SCRATCH_DEVICES];
+static SolveRec g_fixed_snap[2 * SCRATCH_RHS];
+
+static int fixed_space(Space **out) {
+    int dev = 0;
+    PP_HIP(hipGetDevice(&dev));
+    PP_CHECK_ARG(dev >= 0 && dev < SCRATCH_DEVICES, "device index beyond the scratch table");
+    Space &sp = g_fixed[dev];
+    if (!sp.fixed) {
+        unsigned char *base = nullptr;
+        PP_HIP(hipGetSymbolAddress((void **)&base, HIP_SYMBOL(scratch)));
+        PP_HIP(hipEventCreateWithFlags(&sp.ev[0], hipEventDisableTiming));
+        PP_HIP(hipEventCreateWithFlags(&sp.ev[1], hipEventDisableTiming));
+        sp.vec = base;                                    sp.vecCap = SCR_VEC;
+        sp.parts = base + SCR_VEC;                        sp.partsCap = SCR_PARTS;
+        sp.recs = base + SCR_VEC + SCR_PARTS;             sp.recsCap = SCR_RECS;
+        sp.tols = base + SCR_VEC + SCR_PARTS + SCR_RECS;  sp.tolsCap = SCR_TOLS;
+        sp.snap = reinterpret_cast<char *>(g_fixed_snap); sp.snapCap = sizeof(g_fixed_snap);
+        sp.fixed = true;
+    }
+    *out = &sp;
+    return PP_OK;
+}
+
 } // namespace pp
 
 using namespace pp;
@@ -2404,7 +2461,18 @@ int pp_solve_linear(const pp_csr *A, const void *rhs, void *x, int32_t nrhs, int
                     pp_solve_result *results) {
     ThreadSpaces &ts = my_spaces();
     return solve_linear(A, rhs, x, nrhs, dtype, maxit, tols, ntols, criterion, use_bicg, matrix_rank_deficient,
-                        residual_reset_steps, transpose_a, return_best, stream, results, ts.cg, ts.bicg);
+                        residual_reset_steps, transpose_a, return_best, stream, results, ts.cg, ts.bicg, true);
+}
+
+int pp_solve_linear_noalloc(const pp_csr *A, const void *rhs, void *x, int32_t nrhs, int32_t dtype, int32_t maxit,
+                            const double *tols, int32_t ntols, int32_t criterion, int32_t use_bicg,
+                            int32_t matrix_rank_deficient, int32_t residual_reset_steps, int32_t transpose_a,
+                            int32_t return_best, void *stream, pp_solve_result *results) {
+    std::lock_guard<std::mutex> lock(g_lock);
+    Space *sp = nullptr;
+    if (int e = fixed_space(&sp)) return e;
+    return solve_linear(A, rhs, x, nrhs, dtype, maxit, tols, ntols, criterion, use_bicg, matrix_rank_deficient,
+                        residual_reset_steps, transpose_a, return_best, stream, results, *sp, *sp, false);
 }
 
 int pp_set_krylov_settings(int32_t check_interval, int32_t use_graphs, int32_t cg_jacobi, int32_t bicg_jacobi) {
