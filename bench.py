@@ -725,7 +725,7 @@ def kernel_bytes(name, rows, s, sh, fine, levels, m=3):
         out = s if (rows == n and name == "amg_post_jacobi") else sh
         return M(lv[0], rows, sh) + rows * (sh + vb + sh + out)
     if name == "amg_restrict":
-        for nf, (na, nc, nr, npp) in levels.items():
+        for nf, (na, nc, nr, npp, *_lanes) in levels.items():
             if nc == rows:
                 return M(nr, nc, sh) + nf * sh + nc * sh
     if name == "amg_prolong" and lv:
@@ -1107,6 +1107,19 @@ def main(argv=None) -> int:
         if tr is not None:
             roofline["traffic"] = tr["bytes"]
             roofline["traffic_detail"] = {**tr, "source": os.path.relpath(tfile, ROOT)}
+    # the whole step against the roofline: the algorithmic bytes of every timed kernel with a
+    # byte model (launches over the instrumented steps) per step, over the un-instrumented step
+    # time; the BiCGStab phases are counted with all right-hand sides active (an upper bound)
+    if roofline is not None:
+        covered = [(t, b) for t, b in rows if b is not None and t.sampled > 0]
+        step_bytes = sum(b * t.launches for t, b in covered) / args.steps
+        step_s = elapsed / args.steps
+        roofline["step"] = {
+            "algorithmic_bytes": int(step_bytes), "ms": round(1e3 * step_s, 3),
+            "achieved": round(step_bytes / step_s / 1e9, 1),
+            "frac": round(step_bytes / step_s / 1e9 / HBM_PEAK_GBS, 4),
+            "kernel_time_covered": round(sum(t.est_total_ms for t, _ in covered)
+                                         / max(sum(t.est_total_ms for t, _ in rows), 1e-12), 3)}
     iters = {}
     for _, tag, infos in sim.solver_log[log_start:log_start + len(sim.solver_log[log_start:]) // (1 if args.timer_in_timed_region else 2)]:
         iters.setdefault(tag, []).append(max(i[0] for i in infos))

@@ -48,7 +48,7 @@ enum {
 
 /* ---- domain descriptors ---- */
 
-/* One face of a block, 104 bytes. Face order [-x,+x,-y,+y,-z,+z]. */
+/* One face of a block, 112 bytes. Face order [-x,+x,-y,+y,-z,+z]. */
 typedef struct pp_boundary_desc {
     int32_t type;          /* PP_BOUND_* */
     int32_t vel_type;      /* FIXED: PP_BC_DIRICHLET or PP_BC_NEUMANN (velocity) */
@@ -60,16 +60,18 @@ typedef struct pp_boundary_desc {
     int32_t conn_axes[3];  /* CONNECTED: face-format axes of the connection */
     int32_t scalar_static; /* FIXED: passive scalar [sb][C] (1) or [sb][C][face cells] (0) */
     int32_t scalar_batched;/* FIXED: passive scalar has one slice per environment */
+    int32_t pot_values_batched; /* FIXED: pot_values has one slice per environment */
+    int32_t reserved;      /* zero */
     uint64_t vel;          /* device pointer (dtype) */
     uint64_t transform;    /* device pointer (dtype) or 0 */
     uint64_t pot_types;    /* device pointer int8 [face cells] or 0 */
-    uint64_t pot_values;   /* device pointer (dtype) [face cells] or 0 */
+    uint64_t pot_values;   /* device pointer (dtype) [face cells] ([batch][face cells]) or 0 */
     double   pot_cw;       /* thin-wall conductance ratio */
     uint64_t scalar;       /* FIXED: passive scalar face data (Dirichlet value or Neumann gradient) or 0 */
     uint64_t scalar_types; /* FIXED: int8 [C] PP_BC_* per channel, or 0 (all Dirichlet) */
 } pp_boundary_desc;
 
-/* One block, 104 + 6*104 = 728 bytes. */
+/* One block, 104 + 6*112 = 776 bytes. */
 typedef struct pp_block_desc {
     int32_t size[3];        /* cells in x, y, z (1 for unused dimensions) */
     int32_t cells;          /* size[0]*size[1]*size[2] */
@@ -215,7 +217,7 @@ enum pp_kernel_id {
     PP_K_AMG_JACOBI = 34, PP_K_AMG_LEVEL_RESIDUAL = 35, PP_K_AMG_RESTRICT = 36, PP_K_AMG_PROLONG = 37,
     PP_K_AMG_COARSE = 38, PP_K_MEAN = 39, PP_K_SUBTRACT_MEAN = 40, PP_K_AMG_POST_JACOBI = 41,
     PP_K_AMG_TAIL = 42,
-    PP_K_SPMV = 50, PP_K_GALERKIN = 51,
+    PP_K_SPMV = 50, PP_K_GALERKIN = 51, PP_K_GALERKIN_AP = 52,
     PP_K_SEG_CG_DIRECTION = 60, PP_K_SEG_CG_UPDATE = 61, PP_K_SEG_BICG = 62
 };
 int pp_kernel_timer_start(int32_t sample_every);
@@ -310,25 +312,27 @@ int pp_copy_result(const pp_domain *dom, int32_t field, int32_t to_blocks, void 
  * cells] gradients of the fixed faces' velocities; grad_source (or null) is per cell
  * [batch][d][n]. The descriptor must show the forward's inputs (block velocity, pressure,
  * boundary data, time step); values the forward read from the domain's flat vectors are
- * passed explicitly. */
+ * passed explicitly. grad_visc (or null) is [batch][n]: the gradient with respect to the
+ * viscosity as a per-cell field (the reference's viscosityGrad and block viscosityGrad), which
+ * the caller sums over the cells that use the domain's or a block's viscosity. */
 /* replaces SetupAdvectionMatrixGrad (advection.cu:309-542): d(C values [B][nnz], A [B][n] or
- * null) -> d(block velocity) */
+ * null) -> d(block velocity), d(viscosity) */
 int pp_setup_advection_matrix_grad(const pp_domain *dom, const void *grad_c, const void *grad_a, void *grad_u,
-                                   void *stream);
+                                   void *grad_visc, void *stream);
 /* replaces SetupAdvectionVelocityGrad (advection.cu:957-1200, applyPressureGradient off):
- * d(velocity RHS) -> d(block velocity), d(source), d(fixed-face velocities) */
+ * d(velocity RHS) -> d(block velocity), d(source), d(fixed-face velocities), d(viscosity) */
 int pp_setup_advection_velocity_grad(const pp_domain *dom, const void *grad_rhs, void *grad_u, void *grad_source,
-                                     void *const *grad_bound, void *stream);
+                                     void *const *grad_bound, void *grad_visc, void *stream);
 /* replaces SetupPressureMatrixGrad (pressure.cu:195-348): d(P values) -> d(A); a = forward A */
 int pp_setup_pressure_matrix_grad(const pp_domain *dom, const void *a, const void *grad_p, void *grad_a,
                                   void *stream);
 /* replaces SetupPressureRHSGrad (pressure.cu:524-755): d(HbyA) -> d(A), d(C values),
- * d(velocity result), d(block velocity), d(source), d(fixed-face velocities); a, c_val,
- * velocity_result and hbya are the forward's inputs and output */
+ * d(velocity result), d(block velocity), d(source), d(fixed-face velocities), d(viscosity); a,
+ * c_val, velocity_result and hbya are the forward's inputs and output */
 int pp_setup_pressure_rhs_grad(const pp_domain *dom, const void *grad_hbya, const void *a, const void *c_val,
                                const void *velocity_result, const void *hbya, void *grad_a, void *grad_c,
                                void *grad_velocity_result, void *grad_u, void *grad_source, void *const *grad_bound,
-                               void *stream);
+                               void *grad_visc, void *stream);
 /* replaces SetupPressureRHSdivGrad (pressure.cu:805-880): d(div) -> d(HbyA), d(fixed-face
  * velocities) */
 int pp_setup_pressure_rhs_div_grad(const pp_domain *dom, const void *grad_div, int32_t time_step_norm, void *grad_hbya,
@@ -337,12 +341,16 @@ int pp_setup_pressure_rhs_div_grad(const pp_domain *dom, const void *grad_div, i
  * velocity) -> d(HbyA), d(A), d(pressure [B][n]); the blocks must hold the forward's pressure */
 int pp_correct_velocity_grad(const pp_domain *dom, const void *grad_out, const void *a, int32_t time_step_norm,
                              void *grad_hbya, void *grad_a, void *grad_pressure, void *stream);
-/* replaces ComputeEpotRHSGrad (potential.cu:980-1040): d(div [B][n]) -> d(in-plane field [B][d][n]) */
-int pp_compute_epot_rhs_grad(const pp_domain *dom, const void *grad_div, void *grad_field, void *stream);
+/* replaces ComputeEpotRHSGrad (potential.cu:980-1040): d(div [B][n]) -> d(in-plane field [B][d][n])
+ * and, with grad_values (a device table, entry block * 6 + face, of [B][face cells] buffers or
+ * null entries; the table itself may be null), d(Dirichlet potential values) of the faces
+ * (the reference's potentialValuesGrad, potential.cu:1000-1040) */
+int pp_compute_epot_rhs_grad(const pp_domain *dom, const void *grad_div, void *grad_field, void *const *grad_values,
+                             void *stream);
 /* replaces ComputeCurrentDensityFaceBasedGrad (potential.cu:1248-1329): d(J [B][3][n]) ->
- * d(phi [B][n]), d(u x e_b [B][3][n]) */
+ * d(phi [B][n]), d(u x e_b [B][3][n]), d(Dirichlet potential values) as above */
 int pp_compute_current_density_grad(const pp_domain *dom, const void *grad_j, void *grad_phi, void *grad_uxb,
-                                    void *stream);
+                                    void *const *grad_values, void *stream);
 /* replaces Domain::getMaxVelocity / getMaxVelocityPerEnv / getMaxVelocityMagnitude
  * (csrc/phipict/domain/domain.cpp:212-245) and the Block methods of the same names
  * (csrc/phipict/domain/block.cpp:505-600): block -1 = every block, else that block only;
@@ -444,19 +452,19 @@ int pp_amg_level_copy(void *handle, int32_t level, int32_t *a_row, int32_t *a_co
                       double *r_val, int32_t *splitting);
 int pp_amg_free(void *handle);
 
-/* Galerkin refresh on the device (replaces galerkin_refresh, amg.py:749-847):
- * a precomputed product map turns fine operator values into coarse operator values,
- * out[e] = sum_{t in seg(e)} w[t] * in[src[t]]. seg: [n_out+1]. */
-int pp_segmented_gather(const int32_t *seg, const int32_t *src, const void *w, int32_t n_out,
-                        const void *in, void *out, int32_t nmat, int32_t in_stride,
-                        int32_t out_stride, int32_t dtype, void *stream);
-/* Galerkin product C = R A P into the fixed pattern (c_row, c_col) of C, accumulated in
- * double (galerkin_refresh's _galerkin_product, src/phipict/solvers/amg.py:662-696, 749-847).
- * R and P values are double; A holds A->num_matrices value arrays (a_dtype); C gets one
- * value array per A matrix (c_dtype). */
-int pp_galerkin_rap(const int32_t *r_row, const int32_t *r_col, const double *r_val, const pp_csr *A, int32_t a_dtype,
-                    const int32_t *p_row, const int32_t *p_col, const double *p_val, const int32_t *c_row,
-                    const int32_t *c_col, void *c_val, int32_t c_n, int32_t c_nnz, int32_t c_dtype, void *stream);
+/* Galerkin refresh on the device (replaces the products of galerkin_refresh,
+ * src/phipict/solvers/amg.py:749-847, and the product map of _galerkin_map, :1114-1180):
+ * C = R (A P) as two row products, first A P into the pattern (ap_row, ap_col) of
+ * pp_ap_pattern (ap_val: a double workspace of A->num_matrices x ap_nnz values, overwritten),
+ * then R (A P) into the coarse pattern (c_row, c_col) with c_n rows; every entry accumulated in
+ * double in a fixed order. R and P values are double; A holds A->num_matrices value arrays
+ * (a_dtype); C gets one value array per A matrix (c_dtype). max_rows (host): the longest rows
+ * of A, P, A P, R and C. No memory beyond the arguments. */
+int pp_galerkin_rows(const pp_csr *A, int32_t a_dtype, const int32_t *p_row, const int32_t *p_col, const double *p_val,
+                     const int32_t *ap_row, const int32_t *ap_col, int32_t ap_nnz, double *ap_val,
+                     const int32_t *r_row, const int32_t *r_col, const double *r_val, const int32_t *c_row,
+                     const int32_t *c_col, void *c_val, int32_t c_n, int32_t c_nnz, int32_t c_dtype,
+                     const int32_t *max_rows, void *stream);
 /* ELL copy of a CSR matrix (the layout of pp_csr.ell_col / ell_val): entry k of row i at
  * [k][i], rows padded to `width` entries with the row's own column and zero values. Either
  * output may be NULL; no reference counterpart. */
@@ -474,20 +482,15 @@ int pp_sell_fill(const int32_t *row, const int32_t *col, const void *val, int32_
  * the sliced copy M (rows x cols, values in hdtype), over reps launches after a warm-up. */
 int pp_sell_probe(const pp_sell *M, int32_t rows, int32_t cols, int32_t hdtype, int32_t reps, void *stream, double *us);
 
-/* Galerkin map (the precomputed product map of _galerkin_map, src/phipict/solvers/amg.py:
- * 1114-1180, used here for every refresh): host construction for fixed R, P, fine pattern A
- * and coarse pattern C -- c[e] = sum_{t in seg(e)} w[t] * a[src[t]], seg [nnz_c+1] -- and its
- * device application in double accumulation (a: nmat value arrays of stride a_stride). */
-int pp_galerkin_map_build(int32_t n, const int32_t *a_row, const int32_t *a_col, int32_t nc, const int32_t *p_row,
-                          const int32_t *p_col, const double *p_val, const int32_t *r_row, const int32_t *r_col,
-                          const double *r_val, const int32_t *c_row, const int32_t *c_col, void **handle);
-int pp_galerkin_map_size(void *handle, int64_t *nterms);
-int pp_galerkin_map_copy(void *handle, int32_t *seg, int32_t *src, double *w);
-int pp_galerkin_map_free(void *handle);
-int pp_galerkin_apply(const int32_t *seg, const int32_t *src, const double *w, int32_t n_out, int64_t nterms,
-                      const void *a_val,
-                      int32_t a_dtype, void *c_val, int32_t c_dtype, int32_t nmat, int32_t a_stride, int32_t c_stride,
-                      void *stream);
+/* Symbolic pattern of A P for the refresh (host; the pattern side of _galerkin_map,
+ * src/phipict/solvers/amg.py:1114-1180): fine pattern (a_row, a_col) with n rows, interpolation
+ * pattern (p_row, p_col) with nc columns. Row i holds the union of the interpolation rows of row
+ * i's columns, sorted. First call with ap_col NULL: writes ap_row[n+1]; second call with ap_col
+ * of ap_row[n] entries: writes the columns and, when R's (r_row, r_col) and the coarse pattern
+ * (c_row, c_col) are given, checks that the coarse pattern holds every entry of R (A P). */
+int pp_ap_pattern(int32_t n, const int32_t *a_row, const int32_t *a_col, int32_t nc, const int32_t *p_row,
+                  const int32_t *p_col, const int32_t *r_row, const int32_t *r_col, const int32_t *c_row,
+                  const int32_t *c_col, int32_t *ap_row, int32_t *ap_col);
 /* inverse diagonal of a CSR matrix (rows without a usable diagonal get 1):
  * dinv [num_matrices][n] in dtype_out from values in dtype. */
 int pp_inverse_diagonal(const pp_csr *A, int32_t dtype, void *dinv, int32_t dtype_out, void *stream);

@@ -240,10 +240,11 @@ class OracleDomain:
             self.cell_tr.append(T)
             b.has_transform = 1
             b.transform = _p(T)
+            # the state arrays are the oracle's own copies: a step never writes into the case
             vel = np.zeros((self.dims, s.ncells), self.dtype) if s.velocity is None else \
-                np.ascontiguousarray(s.velocity.reshape(self.dims, -1), self.dtype)
+                np.array(s.velocity.reshape(self.dims, -1), self.dtype)
             prs = np.zeros(s.ncells, self.dtype) if s.pressure is None else \
-                np.ascontiguousarray(s.pressure.reshape(-1), self.dtype)
+                np.array(s.pressure.reshape(-1), self.dtype)
             self.velocity.append(vel)
             self.pressure.append(prs)
             b.velocity, b.pressure = _p(vel), _p(prs)
@@ -251,7 +252,7 @@ class OracleDomain:
                 b.vsource_mode = 0
                 self.vsource.append(None)
             else:
-                vs = np.ascontiguousarray(s.vsource, self.dtype)
+                vs = np.array(s.vsource, self.dtype)
                 b.vsource_mode = 1 if vs.ndim == 1 else 2
                 vs = vs.reshape(-1)
                 self.vsource.append(vs)
@@ -283,7 +284,7 @@ class OracleDomain:
                     ob.type = OB_FIXED
                     ob.vel_neumann = 1 if fs.neumann else 0
                     vel_b = np.zeros(self.dims, self.dtype) if fs.velocity is None else \
-                        np.ascontiguousarray(fs.velocity, self.dtype)
+                        np.array(fs.velocity, self.dtype)
                     ob.vel_static = 1 if vel_b.ndim == 1 else 0
                     vel_b = vel_b.reshape(-1)
                     bt = boundary_slice(FT, f).reshape(-1)
@@ -305,8 +306,8 @@ class OracleDomain:
                     ob.pot_cw = fs.cw
         self.ncells = off
         self.viscosity = float(viscosity)
-        self.scalar = [None if s.scalar is None else np.ascontiguousarray(s.scalar.reshape(s.scalar.shape[0], -1),
-                                                                           self.dtype) for s in specs]
+        self.scalar = [None if s.scalar is None else np.array(s.scalar.reshape(s.scalar.shape[0], -1), self.dtype)
+                       for s in specs]
         self.channels = 0 if self.scalar[0] is None else self.scalar[0].shape[0]
         self.dom = ODomain(self.dims, len(specs), off, self.advection_scheme, float(viscosity), float(dt), self.blocks)
         self.row = np.zeros(off + 1, np.int32)

@@ -30,6 +30,8 @@ class BoundaryDesc(C.Structure):
         ("conn_axes", C.c_int32 * 3),
         ("scalar_static", C.c_int32),
         ("scalar_batched", C.c_int32),
+        ("pot_values_batched", C.c_int32),
+        ("reserved", C.c_int32),
         ("vel", C.c_uint64),
         ("transform", C.c_uint64),
         ("pot_types", C.c_uint64),
@@ -159,8 +161,8 @@ class AmgLevel(C.Structure):
     ]
 
 
-assert C.sizeof(BoundaryDesc) == 104, C.sizeof(BoundaryDesc)
-assert C.sizeof(BlockDesc) == 728, C.sizeof(BlockDesc)
+assert C.sizeof(BoundaryDesc) == 112, C.sizeof(BoundaryDesc)
+assert C.sizeof(BlockDesc) == 776, C.sizeof(BlockDesc)
 
 _i32, _i64, _vp, _dbl = C.c_int32, C.c_int64, C.c_void_p, C.c_double
 _P = C.POINTER
@@ -186,14 +188,14 @@ SIGNATURES = {
     "pp_max_velocity": [_P(DomainDesc), _i32, _i32, _i32, _i32, _vp, _vp],
     "pp_setup_advection_matrix_scalar": [_P(DomainDesc), _i32, _i32, _vp],
     "pp_setup_advection_scalar": [_P(DomainDesc), _i32, _vp],
-    "pp_setup_advection_matrix_grad": [_P(DomainDesc), _vp, _vp, _vp, _vp],
-    "pp_setup_advection_velocity_grad": [_P(DomainDesc), _vp, _vp, _vp, _vp, _vp],
+    "pp_setup_advection_matrix_grad": [_P(DomainDesc), _vp, _vp, _vp, _vp, _vp],
+    "pp_setup_advection_velocity_grad": [_P(DomainDesc), _vp, _vp, _vp, _vp, _vp, _vp],
     "pp_setup_pressure_matrix_grad": [_P(DomainDesc), _vp, _vp, _vp, _vp],
-    "pp_setup_pressure_rhs_grad": [_P(DomainDesc), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "pp_setup_pressure_rhs_grad": [_P(DomainDesc), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "pp_setup_pressure_rhs_div_grad": [_P(DomainDesc), _vp, _i32, _vp, _vp, _vp],
     "pp_correct_velocity_grad": [_P(DomainDesc), _vp, _vp, _i32, _vp, _vp, _vp, _vp],
-    "pp_compute_epot_rhs_grad": [_P(DomainDesc), _vp, _vp, _vp],
-    "pp_compute_current_density_grad": [_P(DomainDesc), _vp, _vp, _vp, _vp],
+    "pp_compute_epot_rhs_grad": [_P(DomainDesc), _vp, _vp, _vp, _vp],
+    "pp_compute_current_density_grad": [_P(DomainDesc), _vp, _vp, _vp, _vp, _vp],
     "pp_transform_vectors": [_i32, _i32, _i32, _vp, _vp, _i32, _i32, _vp, _i32, _vp],
     "pp_solve_linear": [_P(CsrDesc), _vp, _vp, _i32, _i32, _i32, _P(_dbl), _i32, _i32, _i32, _i32, _i32, _i32, _i32,
                         _vp, _P(SolveResult)],
@@ -210,16 +212,12 @@ SIGNATURES = {
     "pp_amg_level_sizes": [_vp, _i32, _P(_i32), _P(_i32), _P(_i32), _P(_i32)],
     "pp_amg_level_copy": [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "pp_amg_free": [_vp],
-    "pp_segmented_gather": [_vp, _vp, _vp, _i32, _vp, _vp, _i32, _i32, _i32, _i32, _vp],
-    "pp_galerkin_rap": [_vp, _vp, _vp, _P(CsrDesc), _i32, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _vp],
+    "pp_galerkin_rows": [_P(CsrDesc), _i32, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32,
+                         _i32, _i32, _vp, _vp],
     "pp_inverse_diagonal": [_P(CsrDesc), _i32, _vp, _i32, _vp],
     "pp_cast": [_vp, _i32, _vp, _i32, _i64, _vp],
     "pp_ell_from_csr": [_vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32, _vp],
-    "pp_galerkin_map_build": [_i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _P(_vp)],
-    "pp_galerkin_map_size": [_vp, _P(_i64)],
-    "pp_galerkin_map_copy": [_vp, _vp, _vp, _vp],
-    "pp_galerkin_map_free": [_vp],
-    "pp_galerkin_apply": [_vp, _vp, _vp, _i32, _i64, _vp, _i32, _vp, _i32, _i32, _i32, _i32, _vp],
+    "pp_ap_pattern": [_i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "pp_kernel_timer_start": [_i32],
     "pp_kernel_timer_stop": [],
     "pp_kernel_timer_read": [_i32, _vp, _vp, _vp, _vp, _vp, _P(_i32)],

@@ -5,8 +5,9 @@
 Functions).
 
 Tracked inputs: the blocks' velocities, the fixed faces' velocities, the blocks' velocity
-sources, and for MHD the field direction ``e_b`` and the Stuart number. Viscosities, grid
-metrics and potential boundary values are constants here. Gradients of the linear solves:
+sources, the domain's viscosity and the blocks' own viscosities (one value or one per cell), and
+for MHD the field direction ``e_b``, the Stuart number and the Dirichlet potential values of the
+fixed faces (shared by the environments or one slice each). Grid metrics are constants. Gradients of the linear solves:
 for ``A x = b`` with an upstream gradient ``g``, ``A^T lam = g``, ``db = lam`` and
 ``dA = -lam (x)^T`` on the sparsity pattern; singular (pure-Neumann) systems work on the
 mean-free subspace (``_project_out_constant``, piso_diff.py:1000-1026).
@@ -136,6 +137,47 @@ def _reduce_source_grads(domain: Domain, sources: Sequence[Optional[torch.Tensor
     return out
 
 
+def viscosity_inputs(domain: Domain) -> list:
+    """The viscosities the momentum assembly reads, as tracked inputs: the domain's, then each
+    block's own (None for a block that uses the domain's)."""
+    return [domain.viscosity] + [b.viscosity for b in domain.getBlocks()]
+
+
+def _visc_grad_buffer(domain: Domain, viscs: Sequence[Optional[torch.Tensor]], needs: Sequence[bool]):
+    """A zeroed [B][n] buffer for the gradient with respect to the viscosity as a per-cell field,
+    or None when no viscosity input needs a gradient."""
+    if not viscs or not any(needs):
+        return None
+    return _zeros_flat(domain, 1)
+
+
+def _reduce_visc_grads(domain: Domain, viscs: Sequence[Optional[torch.Tensor]], g: Optional[torch.Tensor]) -> list:
+    """The per-cell viscosity gradient [B][n] to the inputs of ``viscosity_inputs``: the domain's
+    value collects the cells of the blocks without their own viscosity, a block's static value
+    its cells, a per-cell viscosity its cells one by one; the viscosities are shared by the
+    environments, so their gradients sum over them."""
+    if not viscs:
+        return []
+    if g is None:
+        return [None] * len(viscs)
+    blocks = domain.getBlocks()
+    per_cell = g.reshape(domain.getBatchSize(), -1).sum(dim=0)
+    parts = torch.split(per_cell, [b.getCells() for b in blocks])
+    out = []
+    dom_v = viscs[0]
+    if dom_v is not None:
+        total = sum((p.sum() for b, p in zip(blocks, parts) if b.viscosity is None), per_cell.new_zeros(()))
+        out.append(total.reshape(dom_v.shape).to(device=dom_v.device, dtype=dom_v.dtype))
+    else:
+        out.append(None)
+    for v, p in zip(viscs[1:], parts):
+        if v is None:
+            out.append(None)
+        else:
+            out.append((p.sum() if v.numel() == 1 else p).reshape(v.shape).to(v.dtype))
+    return out
+
+
 def _zeros_flat(domain: Domain, channels: int) -> torch.Tensor:
     return torch.zeros(domain.getBatchSize() * channels * domain.getTotalSize(), dtype=domain.getDtype(),
                        device=domain.getDevice())
@@ -147,28 +189,32 @@ def _stream(domain: Domain) -> int:
 
 # ----------------------------------------------------------------------------- assembly
 class AdvectionMatrixFunction(torch.autograd.Function):
-    """(A, C values) of the implicit advection-diffusion matrix from the block velocities
-    (piso_diff.py:1427-1596)."""
+    """(A, C values) of the implicit advection-diffusion matrix from the block velocities and,
+    when given after them, the ``viscosity_inputs`` (piso_diff.py:1427-1596)."""
 
     @staticmethod
-    def forward(ctx, domain: Domain, dt: torch.Tensor, *vels):
+    def forward(ctx, domain: Domain, dt: torch.Tensor, *inputs):
         ops.SetupAdvectionMatrix(domain, dt, 0)
+        nv = domain.getNumBlocks()
         ctx.domain, ctx.dt = domain, dt.clone()
-        ctx.n_vels = len(vels)
+        ctx.n_vels, ctx.viscs = nv, [None if v is None else v.detach() for v in inputs[nv:]]
         return domain.A.clone(), domain.C.value.clone()
 
     @staticmethod
     def backward(ctx, gA, gC):
         dom = ctx.domain
+        nv = ctx.n_vels
         if not any(ctx.needs_input_grad[2:]):
-            return (None, None) + (None,) * ctx.n_vels
+            return (None, None) + (None,) * (nv + len(ctx.viscs))
         gU = _zeros_flat(dom, dom.getSpatialDims())
+        gNu = _visc_grad_buffer(dom, ctx.viscs, ctx.needs_input_grad[2 + nv:])
         if gC is None:
             gC = torch.zeros_like(dom.C.value)
         with _Bound(dom, ctx.dt) as d:
             _lib.call("pp_setup_advection_matrix_grad", C.byref(d), _lib.ptr(gC.contiguous()),
-                      _lib.ptr(gA.contiguous()) if gA is not None else None, _lib.ptr(gU), _stream(dom))
-        return (None, None, *flat_to_blocks(dom, gU, dom.getSpatialDims()))
+                      _lib.ptr(gA.contiguous()) if gA is not None else None, _lib.ptr(gU),
+                      _lib.ptr(gNu) if gNu is not None else None, _stream(dom))
+        return (None, None, *flat_to_blocks(dom, gU, dom.getSpatialDims()), *_reduce_visc_grads(dom, ctx.viscs, gNu))
 
 
 class AdvectionRHSFunction(torch.autograd.Function):
@@ -178,11 +224,13 @@ class AdvectionRHSFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, domain: Domain, dt: torch.Tensor, nv: int, nb: int, *inputs):
         ops.SetupAdvectionVelocity(domain, dt, 0, False)
+        ns = domain.getNumBlocks()
         ctx.domain, ctx.dt, ctx.nv, ctx.nb = domain, dt.clone(), nv, nb
         bvels = inputs[nv:nv + nb]
-        srcs = inputs[nv + nb:]
+        srcs = inputs[nv + nb:nv + nb + ns]
         ctx.bvels = [v.detach() for v in bvels]
         ctx.src_meta = [None if s is None else s.detach() for s in srcs]
+        ctx.viscs = [None if v is None else v.detach() for v in inputs[nv + nb + ns:]]
         return domain.velocityRHS.clone()
 
     @staticmethod
@@ -192,14 +240,16 @@ class AdvectionRHSFunction(torch.autograd.Function):
         gU = _zeros_flat(dom, d)
         want_s = any(s is not None for s in ctx.src_meta)
         gS = _zeros_flat(dom, d) if want_s else None
+        gNu = _visc_grad_buffer(dom, ctx.viscs, ctx.needs_input_grad[4 + ctx.nv + ctx.nb + len(ctx.src_meta):])
         bufs, table = _bound_grad_table(dom, ctx.nb > 0)
         with _Bound(dom, ctx.dt, bvels=ctx.bvels) as desc:
             _lib.call("pp_setup_advection_velocity_grad", C.byref(desc), _lib.ptr(gR.contiguous()), _lib.ptr(gU),
-                      _lib.ptr(gS) if gS is not None else None, _lib.ptr(table) if ctx.nb else None, _stream(dom))
+                      _lib.ptr(gS) if gS is not None else None, _lib.ptr(table) if ctx.nb else None,
+                      _lib.ptr(gNu) if gNu is not None else None, _stream(dom))
         gv = flat_to_blocks(dom, gU, d)
         gb = [_reduce_bound_grad(v, g) for v, g in zip(ctx.bvels, bufs)] if ctx.nb else []
         gs = _reduce_source_grads(dom, ctx.src_meta, gS) if want_s else [None] * len(ctx.src_meta)
-        return (None, None, None, None, *gv, *gb, *gs)
+        return (None, None, None, None, *gv, *gb, *gs, *_reduce_visc_grads(dom, ctx.viscs, gNu))
 
 
 class PressureMatrixFunction(torch.autograd.Function):
@@ -231,9 +281,11 @@ class PressureRHSFunction(torch.autograd.Function):
     def forward(ctx, domain: Domain, dt: torch.Tensor, tsn: bool, A, Cv, V, nv: int, nb: int, *inputs):
         ops.SetupPressureRHS(domain, dt, 0, False, timeStepNorm=tsn)
         H, div = domain.pressureRHS.clone(), domain.pressureRHSdiv.clone()
+        ns = domain.getNumBlocks()
         ctx.domain, ctx.dt, ctx.tsn, ctx.nv, ctx.nb = domain, dt.clone(), tsn, nv, nb
         ctx.bvels = [v.detach() for v in inputs[nv:nv + nb]]
-        ctx.src_meta = [None if s is None else s.detach() for s in inputs[nv + nb:]]
+        ctx.src_meta = [None if s is None else s.detach() for s in inputs[nv + nb:nv + nb + ns]]
+        ctx.viscs = [None if v is None else v.detach() for v in inputs[nv + nb + ns:]]
         ctx.save_for_backward(A, Cv, V, H)
         return H, div
 
@@ -247,6 +299,8 @@ class PressureRHSFunction(torch.autograd.Function):
         gA, gC, gV, gU = torch.zeros_like(A), torch.zeros_like(Cv), torch.zeros_like(V), _zeros_flat(dom, d)
         want_s = any(s is not None for s in ctx.src_meta)
         gS = _zeros_flat(dom, d) if want_s else None
+        # inputs after (domain, dt, tsn, A, Cv, V, nv, nb): velocities, face velocities, sources, viscosities
+        gNu = _visc_grad_buffer(dom, ctx.viscs, ctx.needs_input_grad[8 + ctx.nv + ctx.nb + len(ctx.src_meta):])
         with _Bound(dom, ctx.dt, bvels=ctx.bvels) as desc:
             tp = _lib.ptr(table) if ctx.nb else None
             if gDiv is not None:
@@ -254,11 +308,12 @@ class PressureRHSFunction(torch.autograd.Function):
                           _lib.ptr(gHt), tp, _stream(dom))
             _lib.call("pp_setup_pressure_rhs_grad", C.byref(desc), _lib.ptr(gHt), _lib.ptr(A), _lib.ptr(Cv),
                       _lib.ptr(V), _lib.ptr(H), _lib.ptr(gA), _lib.ptr(gC), _lib.ptr(gV), _lib.ptr(gU),
-                      _lib.ptr(gS) if gS is not None else None, tp, _stream(dom))
+                      _lib.ptr(gS) if gS is not None else None, tp, _lib.ptr(gNu) if gNu is not None else None,
+                      _stream(dom))
         gv = flat_to_blocks(dom, gU, d)
         gb = [_reduce_bound_grad(v, g) for v, g in zip(ctx.bvels, bufs)] if ctx.nb else []
         gs = _reduce_source_grads(dom, ctx.src_meta, gS) if want_s else [None] * len(ctx.src_meta)
-        return (None, None, None, gA, gC, gV, None, None, *gv, *gb, *gs)
+        return (None, None, None, gA, gC, gV, None, None, *gv, *gb, *gs, *_reduce_visc_grads(dom, ctx.viscs, gNu))
 
 
 class CorrectVelocityFunction(torch.autograd.Function):
@@ -382,39 +437,94 @@ class CrossEbFunction(torch.autograd.Function):
         return gv, ge, None, None
 
 
+def potential_value_faces(domain: Domain) -> list:
+    """(block index, face, boundary) of the fixed faces whose Dirichlet potential values reach the
+    potential kernels (_potential_value_bounds, piso_diff.py:2455-2472): the values enter the
+    potential's right-hand side and the current density linearly, never the matrix."""
+    return [(bi, f, bd) for bi, f, bd in fixed_faces(domain) if bd.hasPotentialValues() and bd.hasPotentialDirichlet()]
+
+
+def potential_values(domain: Domain) -> list:
+    """The value tensors of ``potential_value_faces``, in the same order: the inputs the
+    potential operators track."""
+    return [bd.potentialValues for _, _, bd in potential_value_faces(domain)]
+
+
+def _value_grad_table(domain: Domain, faces: list, needs: Sequence[bool]):
+    """Zeroed [B][face cells] gradient buffers of the faces whose values need a gradient and the
+    device table of their pointers (entry block * 6 + face; None when no face needs one)."""
+    if not any(needs):
+        return [None] * len(faces), None
+    B = domain.getBatchSize()
+    bufs = [torch.zeros(B, bd.face_cells(), dtype=domain.getDtype(), device=domain.getDevice()) if need else None
+            for (_, _, bd), need in zip(faces, needs)]
+    table = torch.zeros(domain.getNumBlocks() * 6, dtype=torch.int64)
+    for (bi, f, _), buf in zip(faces, bufs):
+        if buf is not None:
+            table[bi * 6 + f] = buf.data_ptr()
+    return bufs, table.to(domain.getDevice())
+
+
+def _reduce_value_grads(values: Sequence[torch.Tensor], bufs: Sequence[Optional[torch.Tensor]]) -> list:
+    """Per-environment value gradients [B][face cells] to the shape of each value input: values
+    shared by the environments get the sum over them (_potential_values_grad, piso_diff.py:2474-2530)."""
+    out = []
+    for v, g in zip(values, bufs):
+        if g is None:
+            out.append(None)
+            continue
+        if v.numel() == g.size(1):
+            g = g.sum(dim=0)
+        out.append(g.reshape(v.shape).to(v.dtype))
+    return out
+
+
 class EpotRHSFunction(torch.autograd.Function):
-    """Face divergence of the in-plane u x e_b for the potential equation (piso_diff.py:2541-2635)."""
+    """Face divergence of the in-plane u x e_b for the potential equation, with the Dirichlet
+    values of the faces ``potential_value_faces`` (piso_diff.py:2541-2635)."""
 
     @staticmethod
-    def forward(ctx, domain: Domain, field: torch.Tensor):
+    def forward(ctx, domain: Domain, field: torch.Tensor, *values):
         ctx.domain = domain
+        ctx.faces = potential_value_faces(domain)
+        if len(ctx.faces) != len(values):
+            raise ValueError("EpotRHSFunction takes the values of every potential_value_faces face")
+        ctx.values = [v.detach() for v in values]
         return ops.ComputeEpotRHS(domain, field.detach().contiguous())
 
     @staticmethod
     def backward(ctx, g):
         dom = ctx.domain
         gF = _zeros_flat(dom, dom.getSpatialDims())
+        bufs, table = _value_grad_table(dom, ctx.faces, ctx.needs_input_grad[2:])
         d = dom.desc()
-        _lib.call("pp_compute_epot_rhs_grad", C.byref(d), _lib.ptr(g.contiguous()), _lib.ptr(gF), _stream(dom))
-        return None, gF
+        _lib.call("pp_compute_epot_rhs_grad", C.byref(d), _lib.ptr(g.contiguous()), _lib.ptr(gF),
+                  _lib.ptr(table) if table is not None else None, _stream(dom))
+        return (None, gF, *_reduce_value_grads(ctx.values, bufs))
 
 
 class CurrentDensityFunction(torch.autograd.Function):
-    """Face-based current density from phi and u x e_b (piso_diff.py:2637-2740)."""
+    """Face-based current density from phi and u x e_b, with the Dirichlet values of the faces
+    ``potential_value_faces`` (piso_diff.py:2637-2740)."""
 
     @staticmethod
-    def forward(ctx, domain: Domain, phi: torch.Tensor, uxb: torch.Tensor):
+    def forward(ctx, domain: Domain, phi: torch.Tensor, uxb: torch.Tensor, *values):
         ctx.domain = domain
+        ctx.faces = potential_value_faces(domain)
+        if len(ctx.faces) != len(values):
+            raise ValueError("CurrentDensityFunction takes the values of every potential_value_faces face")
+        ctx.values = [v.detach() for v in values]
         return ops.ComputeCurrentDensityFaceBased(domain, phi.detach().contiguous(), uxb.detach().contiguous())
 
     @staticmethod
     def backward(ctx, gJ):
         dom = ctx.domain
         gphi, gx = _zeros_flat(dom, 1), _zeros_flat(dom, 3)
+        bufs, table = _value_grad_table(dom, ctx.faces, ctx.needs_input_grad[3:])
         d = dom.desc()
         _lib.call("pp_compute_current_density_grad", C.byref(d), _lib.ptr(gJ.contiguous()), _lib.ptr(gphi),
-                  _lib.ptr(gx), _stream(dom))
-        return None, gphi, gx
+                  _lib.ptr(gx), _lib.ptr(table) if table is not None else None, _stream(dom))
+        return (None, gphi, gx, *_reduce_value_grads(ctx.values, bufs))
 
 
 def lorentz_sources(domain: Domain, J: torch.Tensor, e_b: torch.Tensor, N: torch.Tensor,

@@ -33,9 +33,8 @@ def _copy_boundary_data(dst: torch.Tensor, src: torch.Tensor, env_idx: int, batc
 def copy_env_state(dst: Domain, env_idx: int, src: Domain) -> None:
     """Copy the state of a single-environment domain into environment ``env_idx`` of a batch
     (batching.py:19-127): block velocity, pressure and potential in place, and the velocity of
-    every fixed face that differs from ``dst``'s (making it per-environment). The faces'
-    Dirichlet potential values are one array shared by all environments here: values that
-    differ between the domains raise."""
+    every fixed face that differs from ``dst``'s (making it per-environment), and likewise the
+    faces' Dirichlet potential values (one slice per environment once they differ)."""
     if src.getBatchSize() != 1:
         raise ValueError(f"src must have batch size 1, got {src.getBatchSize()}.")
     if not 0 <= env_idx < dst.getBatchSize():
@@ -57,8 +56,16 @@ def copy_env_state(dst: Domain, env_idx: int, src: Domain) -> None:
                     continue
                 _copy_boundary_data(dbound.velocity, sbound.velocity, env_idx, dst.getBatchSize(), dbound.setVelocity)
                 dv, sv = dbound.potentialValues, sbound.potentialValues
-                if (dv is None) != (sv is None) or (dv is not None and not torch.equal(dv, sv)):
-                    raise NotImplementedError("per-environment Dirichlet potential values are not supported")
+                if (dv is None) != (sv is None):
+                    raise NotImplementedError("Dirichlet potential values on one side only")
+                if dv is not None and not torch.equal(dv, sv):
+                    # the environment gets its own slice of the values
+                    fc, nb = dbound.face_cells(), dst.getBatchSize()
+                    if sv.numel() != fc:
+                        raise ValueError("the source environment's potential values must be one slice")
+                    full = (dv.reshape(nb, fc) if dbound.potentialValuesBatched() else dv.reshape(1, fc).expand(nb, fc)).clone()
+                    full[env_idx] = sv
+                    dbound.setPotentialValues(full)
             db._domain._changed = True
     dst.UpdateDomainData()
 

@@ -357,95 +357,60 @@ int pp_amg_level_copy(void *handle, int32_t level, int32_t *a_row, int32_t *a_co
     return PP_OK;
 }
 
-// ---- Galerkin product map ----------------------------------------------------------
-// For fixed R, P and fine/coarse patterns, C = R A P is linear in A's values:
-// C[e] = sum_t w[t] * A[src[t]] with w = R(I,i) P(k,J) for every path I -> i -> k -> J
-// (A(i,k) at position src). Terms of an entry are kept in traversal order (i ascending in
-// R's row, then A's row, then P's row), so every refresh sums them in the same order.
-struct GalerkinMap {
-    std::vector<int32_t> seg, src;
-    std::vector<double> w;
-};
-
-int pp_galerkin_map_build(int32_t n, const int32_t *a_row, const int32_t *a_col, int32_t nc, const int32_t *p_row,
-                          const int32_t *p_col, const double *p_val, const int32_t *r_row, const int32_t *r_col,
-                          const double *r_val, const int32_t *c_row, const int32_t *c_col, void **handle) {
-    if (!a_row || !a_col || !p_row || !p_col || !p_val || !r_row || !r_col || !r_val || !c_row || !c_col || !handle ||
-        n < 0 || nc < 0) {
-        pp::set_error("pp_galerkin_map_build: null argument");
+// ---- pattern of A P ------------------------------------------------------------------
+// The device refresh computes C = R (A P) as two row products (pp_galerkin_rows), the first into
+// the symbolic pattern of A P: row i holds the union of the interpolation rows of row i's
+// columns, sorted. Built once per interpolation, with a dense marker over the coarse columns.
+// The second call (ap_col given) also checks that every entry of R (A P) lies in the coarse
+// pattern, so the refresh needs no check of its own.
+int pp_ap_pattern(int32_t n, const int32_t *a_row, const int32_t *a_col, int32_t nc, const int32_t *p_row,
+                  const int32_t *p_col, const int32_t *r_row, const int32_t *r_col, const int32_t *c_row,
+                  const int32_t *c_col, int32_t *ap_row, int32_t *ap_col) {
+    if (!a_row || !a_col || !p_row || !p_col || !ap_row || n < 0 || nc < 0) {
+        pp::set_error("pp_ap_pattern: null argument");
         return PP_ERR_ARG;
     }
-    auto m = std::make_unique<GalerkinMap>();
-    const int32_t nnzc = c_row[nc];
-    m->seg.assign((size_t)nnzc + 1, 0);
-    std::vector<int32_t> pos((size_t)nc, -1);
-    // pass 1: terms per coarse entry
-    for (int I = 0; I < nc; ++I) {
-        for (int e = c_row[I]; e < c_row[I + 1]; ++e) pos[c_col[e]] = e;
-        for (int ri = r_row[I]; ri < r_row[I + 1]; ++ri) {
-            const int i = r_col[ri];
-            if (i < 0 || i >= n) { pp::set_error("pp_galerkin_map_build: R column out of range"); return PP_ERR_ARG; }
-            for (int a = a_row[i]; a < a_row[i + 1]; ++a) {
-                const int k = a_col[a];
-                for (int pk = p_row[k]; pk < p_row[k + 1]; ++pk) {
-                    const int e = pos[p_col[pk]];
-                    if (e < 0) {
-                        pp::set_error("pp_galerkin_map_build: coarse pattern misses an entry of R A P");
-                        return PP_ERR_ARG;
-                    }
-                    ++m->seg[(size_t)e + 1];
-                }
+    std::vector<int32_t> mark((size_t)nc, -1), cols;
+    int64_t total = 0;
+    if (!ap_col) ap_row[0] = 0;
+    for (int i = 0; i < n; ++i) {
+        cols.clear();
+        for (int a = a_row[i]; a < a_row[i + 1]; ++a) {
+            const int k = a_col[a];
+            if (k < 0 || k >= n) { pp::set_error("pp_ap_pattern: column out of range"); return PP_ERR_ARG; }
+            for (int q = p_row[k]; q < p_row[k + 1]; ++q) {
+                const int J = p_col[q];
+                if (J < 0 || J >= nc) { pp::set_error("pp_ap_pattern: interpolation column out of range"); return PP_ERR_ARG; }
+                if (mark[J] != i) { mark[J] = i; cols.push_back(J); }
             }
         }
-        for (int e = c_row[I]; e < c_row[I + 1]; ++e) pos[c_col[e]] = -1;
-    }
-    for (size_t e = 0; e < (size_t)nnzc; ++e) {
-        m->seg[e + 1] += m->seg[e];
-        if (m->seg[e + 1] < 0) { pp::set_error("pp_galerkin_map_build: more than 2^31 terms"); return PP_ERR_ARG; }
-    }
-    const size_t nt = (size_t)m->seg[nnzc];
-    m->src.resize(nt);
-    m->w.resize(nt);
-    std::vector<int32_t> fill(m->seg.begin(), m->seg.end() - 1);
-    // pass 2: the terms
-    for (int I = 0; I < nc; ++I) {
-        for (int e = c_row[I]; e < c_row[I + 1]; ++e) pos[c_col[e]] = e;
-        for (int ri = r_row[I]; ri < r_row[I + 1]; ++ri) {
-            const int i = r_col[ri];
-            const double rv = r_val[ri];
-            for (int a = a_row[i]; a < a_row[i + 1]; ++a) {
-                const int k = a_col[a];
-                for (int pk = p_row[k]; pk < p_row[k + 1]; ++pk) {
-                    const int e = pos[p_col[pk]];
-                    const int t = fill[e]++;
-                    m->src[t] = a;
-                    m->w[t] = rv * p_val[pk];
-                }
+        total += (int64_t)cols.size();
+        if (total > INT32_MAX) { pp::set_error("pp_ap_pattern: more than 2^31 entries"); return PP_ERR_ARG; }
+        if (!ap_col) {
+            ap_row[i + 1] = (int32_t)total;
+        } else {
+            if (ap_row[i + 1] - ap_row[i] != (int32_t)cols.size()) {
+                pp::set_error("pp_ap_pattern: row offsets of another pattern");
+                return PP_ERR_ARG;
             }
+            std::sort(cols.begin(), cols.end());
+            std::copy(cols.begin(), cols.end(), ap_col + ap_row[i]);
         }
-        for (int e = c_row[I]; e < c_row[I + 1]; ++e) pos[c_col[e]] = -1;
     }
-    *handle = m.release();
-    return PP_OK;
-}
-
-int pp_galerkin_map_size(void *handle, int64_t *nterms) {
-    if (!handle || !nterms) { pp::set_error("pp_galerkin_map_size: null argument"); return PP_ERR_ARG; }
-    *nterms = (int64_t)static_cast<GalerkinMap *>(handle)->src.size();
-    return PP_OK;
-}
-
-int pp_galerkin_map_copy(void *handle, int32_t *seg, int32_t *src, double *w) {
-    if (!handle || !seg || !src || !w) { pp::set_error("pp_galerkin_map_copy: null argument"); return PP_ERR_ARG; }
-    const GalerkinMap *m = static_cast<GalerkinMap *>(handle);
-    std::memcpy(seg, m->seg.data(), m->seg.size() * sizeof(int32_t));
-    std::memcpy(src, m->src.data(), m->src.size() * sizeof(int32_t));
-    std::memcpy(w, m->w.data(), m->w.size() * sizeof(double));
-    return PP_OK;
-}
-
-int pp_galerkin_map_free(void *handle) {
-    delete static_cast<GalerkinMap *>(handle);
+    if (!ap_col || !r_row || !r_col || !c_row || !c_col) return PP_OK;
+    std::fill(mark.begin(), mark.end(), -1);
+    for (int I = 0; I < nc; ++I) {
+        for (int e = c_row[I]; e < c_row[I + 1]; ++e) mark[c_col[e]] = I;
+        for (int t = r_row[I]; t < r_row[I + 1]; ++t) {
+            const int i = r_col[t];
+            if (i < 0 || i >= n) { pp::set_error("pp_ap_pattern: R column out of range"); return PP_ERR_ARG; }
+            for (int e = ap_row[i]; e < ap_row[i + 1]; ++e)
+                if (mark[ap_col[e]] != I) {
+                    pp::set_error("pp_ap_pattern: coarse pattern misses an entry of R A P");
+                    return PP_ERR_ARG;
+                }
+        }
+    }
     return PP_OK;
 }
 

@@ -127,12 +127,29 @@ __device__ __forceinline__ void deferred_correction(const pp_block_desc &b, cons
     }
 }
 
-// Workgroups never straddle blocks: blockIdx.x = block * wpb + tile, so the block index and
+#ifndef PP_ASM_XCD_SPAN
+#define PP_ASM_XCD_SPAN 0
+#endif
+constexpr int ASM_XCDS = 8;
+
+// Workgroup index in cell order. The dispatcher deals workgroups to the XCDs in turn
+// (workgroup w to XCD w mod 8); with PP_ASM_XCD_SPAN each XCD instead takes one contiguous
+// span of tiles, so the neighbour rows and planes a stencil reads were mostly fetched into
+// the same XCD's L2 by its own earlier workgroups.
+__device__ __forceinline__ int asm_tile() {
+    if (!PP_ASM_XCD_SPAN) return blockIdx.x;
+    const int n = gridDim.x, x = blockIdx.x % ASM_XCDS, slot = blockIdx.x / ASM_XCDS;
+    const int per = n / ASM_XCDS, extra = n % ASM_XCDS;
+    return x * per + (x < extra ? x : extra) + slot;
+}
+
+// Workgroups never straddle blocks: tile = block * wpb + tile in block, so the block index and
 // its descriptor are uniform over the workgroup (scalar loads).
 #define CELL_PROLOGUE                                                          \
     const int wpb_ = (dom.max_block_cells + ASM_BLOCK - 1) / ASM_BLOCK;        \
-    const int bi = __builtin_amdgcn_readfirstlane(blockIdx.x / wpb_);           \
-    const int f = (blockIdx.x - bi * wpb_) * ASM_BLOCK + threadIdx.x;           \
+    const int tile_ = asm_tile();                                              \
+    const int bi = __builtin_amdgcn_readfirstlane(tile_ / wpb_);               \
+    const int f = (tile_ - bi * wpb_) * ASM_BLOCK + threadIdx.x;               \
     const pp_block_desc *__restrict__ blocks = dom.blocks;                     \
     const pp_block_desc &b = blocks[bi];                                       \
     if (f >= b.cells) return;                                                  \
@@ -670,7 +687,7 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_epot_rhs(pp_domain dom, const T *
     for (int bound = 0; bound < 2 * D; ++bound) {
         if (!at_bound(p, bound, b) || !is_fixed(b, bound)) continue;
         if (!is_epot_dirichlet(b, bound, p)) continue;
-        const T gv = pot_value<T>(b, bound, p);
+        const T gv = pot_value<T>(b, bound, p, env);
         if (gv == T(0)) continue;
         dv -= Metric<T, D>::alpha(b, f, bound >> 1) * gv;
     }
@@ -703,7 +720,7 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_current_density(pp_domain dom, co
         if (atb && is_fixed(b, bound)) {
             if (is_insulating_wall(b, bound, p)) jf[bound] = T(0);
             else if (is_epot_dirichlet(b, bound, p)) {
-                const T gv = pot_value<T>(b, bound, p);
+                const T gv = pot_value<T>(b, bound, p, env);
                 jf[bound] = ap[dim] * fs * (phiP - gv) + cvp[dim];
             } else jf[bound] = cvp[dim];
             continue;
@@ -1089,7 +1106,7 @@ __device__ __forceinline__ void store_face_grads(const pp_block_desc &b, const i
 // scaled by q: adds to gbv[bound][k] for every fixed face of the cell
 template <typename T, int D>
 __device__ __forceinline__ void boundary_source_grad(const pp_domain &dom, const pp_block_desc &b, const int *p, int f,
-                                                     int dim, int env, T q, T (*gbv)[D]) {
+                                                     int dim, int env, T q, T (*gbv)[D], T &gnu) {
 #pragma unroll
     for (int bound = 0; bound < 2 * D; ++bound) {
         if (!at_bound(p, bound, b)) continue;
@@ -1104,15 +1121,20 @@ __device__ __forceinline__ void boundary_source_grad(const pp_domain &dom, const
         const T slip = fb.vel_type == PP_BC_DIRICHLET ? T(0) : T(1);
         // s += -vel * flux + vel * (1 - slip) * bv * 2 * alpha
         gbv[bound][dim] += q * ((1 - slip) * bv * 2 * alpha - flux);
+        gnu += q * vel * (1 - slip) * 2 * alpha;
         contra_fixed_grad<T, D>(fb, ff, bound >> 1, -q * vel * fn, gbv[bound]);
     }
 }
 
 // SetupAdvectionMatrixEulerImplicit_GRAD (advection.cu:309-542): gC [B][nnz], gA [B][n] (or
 // null) -> gU [B][D][n] (the block velocity; C depends on it through the face fluxes only)
+// gNu [B][n] (or null): the gradient with respect to the viscosity as a per-cell field (the
+// viscous coefficient of a face is (alpha_P nu_P + alpha_N nu_N) / 2, of a no-slip wall
+// 2 alpha_P nu_P); the caller reduces it to the domain's, a block's or a cell's viscosity.
 template <typename T, int D>
 __global__ void __launch_bounds__(ASM_BLOCK) k_advection_matrix_grad(pp_domain dom, const T *__restrict__ gC,
-                                                                     const T *__restrict__ gA, T *__restrict__ gU) {
+                                                                     const T *__restrict__ gA, T *__restrict__ gU,
+                                                                     T *__restrict__ gNu) {
     CELL_PROLOGUE
     const T det = Metric<T, D>::det(b, f);
     int idx[2 * D + 1], pos[2 * D + 1];
@@ -1128,6 +1150,29 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_advection_matrix_grad(pp_domain d
     for (int bound = 0; bound < 2 * D; ++bound)
         gfl[bound] = idx[bound + 1] >= 0 ? T(0.5) * T(face_sign(bound)) * (grow[0] + grow[bound + 1]) : T(0);
     cell_fluxes_grad<T, D, false>(dom, blocks, bi, p, f, g, gfl, gU + (size_t)env * D * dom.num_cells, nullptr);
+    if (gNu) {
+        // diag += vc, off-diagonal -= vc with vc = (ap nu_P + an nu_N) / 2; walls: diag += (1 - slip) 2 ap nu_P
+        T *gn = gNu + (size_t)env * dom.num_cells;
+        T own = T(0);
+#pragma unroll
+        for (int bound = 0; bound < 2 * D; ++bound) {
+            const int dim = bound >> 1;
+            const T ap = Metric<T, D>::alpha(b, f, dim);
+            if (idx[bound + 1] >= 0) {
+                int np[3], nw;
+                const int nbi = neighbor(blocks, bi, D, p, bound, np, &nw);
+                const pp_block_desc &nb = blocks[nbi];
+                const int nf = flat3(np, nb.size);
+                const T dvc = grow[0] - grow[bound + 1];
+                own += dvc * ap * T(0.5);
+                atomicAdd(gn + nf + nb.global_offset, dvc * Metric<T, D>::alpha(nb, nf, nw) * T(0.5));
+            } else {
+                const T slip = b.bnd[bound].vel_type == PP_BC_DIRICHLET ? T(0) : T(1);
+                own += grow[0] * (1 - slip) * 2 * ap;
+            }
+        }
+        atomicAdd(gn + g, own);
+    }
 }
 
 // SetupAdvectionVelocityEulerImplicitRHS_GRAD (advection.cu:957-1200), without the pressure
@@ -1136,12 +1181,13 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_advection_matrix_grad(pp_domain d
 template <typename T, int D>
 __global__ void __launch_bounds__(ASM_BLOCK) k_advection_rhs_grad(pp_domain dom, const T *__restrict__ gR,
                                                                   T *__restrict__ gU, T *__restrict__ gS,
-                                                                  T *const *__restrict__ gbnd) {
+                                                                  T *const *__restrict__ gbnd, T *__restrict__ gNu) {
     CELL_PROLOGUE
     const int n = dom.num_cells;
     const T dt = reinterpret_cast<const T *>(dom.time_step)[env];
     const T det = Metric<T, D>::det(b, f);
     T gbv[2 * D][D];
+    T gnu = T(0);
 #pragma unroll
     for (int i = 0; i < 2 * D; ++i)
 #pragma unroll
@@ -1152,9 +1198,10 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_advection_rhs_grad(pp_domain dom,
         const T gv = gR[o];
         gU[o] += gv * det / dt / det;
         if (gS) gS[o] += gv;
-        boundary_source_grad<T, D>(dom, b, p, f, dim, env, gv / det, gbv);
+        boundary_source_grad<T, D>(dom, b, p, f, dim, env, gv / det, gbv, gnu);
     }
     if (gbnd) store_face_grads<T, D>(b, p, bi, env, gbnd, gbv);
+    if (gNu && gnu != T(0)) atomicAdd(gNu + (size_t)env * n + g, gnu);
 }
 
 // SetupPressureMatrix_GRAD (pressure.cu:195-348): gP [B][nnz] -> gA [B][n] (A: the diagonal
@@ -1200,14 +1247,15 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_pressure_rhs_grad(pp_domain dom, 
                                                                  const T *__restrict__ V, const T *__restrict__ H,
                                                                  T *__restrict__ gA, T *__restrict__ gC,
                                                                  T *__restrict__ gV, T *__restrict__ gU,
-                                                                 T *__restrict__ gS, T *const *__restrict__ gbnd) {
+                                                                 T *__restrict__ gS, T *const *__restrict__ gbnd,
+                                                                 T *__restrict__ gNu) {
     CELL_PROLOGUE
     const int n = dom.num_cells;
     const T dt = reinterpret_cast<const T *>(dom.time_step)[env];
     const T A = Aval[(size_t)env * n + g];
     const T det = Metric<T, D>::det(b, f);
     T q[D];
-    T ga = T(0);
+    T ga = T(0), gnu = T(0);
     T gbv[2 * D][D];
 #pragma unroll
     for (int i = 0; i < 2 * D; ++i)
@@ -1221,9 +1269,10 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_pressure_rhs_grad(pp_domain dom, 
         ga -= gH[o] * H[o] / A;
         gU[o] += q[dim] / dt;
         if (gS) gS[o] += q[dim];
-        boundary_source_grad<T, D>(dom, b, p, f, dim, env, q[dim] / det, gbv);
+        boundary_source_grad<T, D>(dom, b, p, f, dim, env, q[dim] / det, gbv, gnu);
     }
     atomicAdd(gA + (size_t)env * n + g, ga);
+    if (gNu && gnu != T(0)) atomicAdd(gNu + (size_t)env * n + g, gnu);
     const int32_t cs = dom.C.row[g];
     const int rs = dom.C.row[g + 1] - cs;
     const T *cv = Cval + (size_t)env * dom.C.nnz + cs;
@@ -1355,22 +1404,44 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_correct_velocity_grad(pp_domain d
 
 // ComputeEpotRHSGrad (potential.cu:980-1040): gDiv [B][n] -> gF [B][D][n], the in-plane
 // vector field whose face fluxes the divergence was taken of
+// Gradient of a Dirichlet potential value at the face cell next to cell p: gv holds one
+// [batch][face cells] buffer per (block, face) (entry block * 6 + face, null: not wanted); the
+// face cell has this one cell as its only neighbour, so no atomics are needed.
+template <typename T>
+__device__ __forceinline__ void put_value_grad(T *const *gv, int bi, int bound, const pp_block_desc &b, const int *p,
+                                               int env, T v) {
+    if (!gv) return;
+    T *buf = gv[bi * 6 + bound];
+    if (buf) buf[(size_t)env * face_cells(b, bound) + face_flat(p, bound, b)] += v;
+}
+
+// ComputeEpotRHSGrad (potential.cu:980-1040): gDiv [B][n] -> gF [B][d][n] (in-plane u x e_b) and
+// the Dirichlet values' gradients (div -= alpha g at a Dirichlet face, potential.cu:272-322).
 template <typename T, int D>
 __global__ void __launch_bounds__(ASM_BLOCK) k_epot_rhs_grad(pp_domain dom, const T *__restrict__ gDiv,
-                                                             T *__restrict__ gF) {
+                                                             T *__restrict__ gF, T *const *gv) {
     CELL_PROLOGUE
     const T gd = gDiv[(size_t)env * dom.num_cells + g];
     T gfl[2 * D];
 #pragma unroll
     for (int bound = 0; bound < 2 * D; ++bound) gfl[bound] = T(face_sign(bound)) * gd;
     cell_fluxes_grad<T, D, true>(dom, blocks, bi, p, f, g, gfl, gF + (size_t)env * D * dom.num_cells, nullptr);
+    if (gv) {
+#pragma unroll
+        for (int bound = 0; bound < 2 * D; ++bound) {
+            if (!at_bound(p, bound, b) || !is_fixed(b, bound) || !is_epot_dirichlet(b, bound, p)) continue;
+            put_value_grad<T>(gv, bi, bound, b, p, env, -Metric<T, D>::alpha(b, f, bound >> 1) * gd);
+        }
+    }
 }
 
 // ComputeCurrentDensityFaceBasedGrad (potential.cu:1248-1329): gJ [B][3][n] -> gPhi [B][n],
 // gX [B][3][n] (u x e_b)
+// and the Dirichlet values' gradients (gv as in k_epot_rhs_grad)
 template <typename T, int D>
 __global__ void __launch_bounds__(ASM_BLOCK) k_current_density_grad(pp_domain dom, const T *__restrict__ gJall,
-                                                                    T *__restrict__ gPhiAll, T *__restrict__ gXall) {
+                                                                    T *__restrict__ gPhiAll, T *__restrict__ gXall,
+                                                                    T *const *gv) {
     CELL_PROLOGUE
     const int nc = dom.num_cells;
     const T *gJ = gJall + (size_t)env * 3 * nc;
@@ -1406,7 +1477,11 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_current_density_grad(pp_domain do
         const bool atb = at_bound(p, bound, b);
         if (atb && is_fixed(b, bound)) {
             if (is_insulating_wall(b, bound, p)) continue;
-            if (is_epot_dirichlet(b, bound, p)) gphiP += ap[dim] * fs * gjf;
+            if (is_epot_dirichlet(b, bound, p)) {
+                // jf = ap fs (phiP - value) + cvp
+                gphiP += ap[dim] * fs * gjf;
+                put_value_grad<T>(gv, bi, bound, b, p, env, -ap[dim] * fs * gjf);
+            }
             add_contra_grad<T, D>(b, f, g, dim, nc, gjf, gx);
             continue;
         }
@@ -1797,20 +1872,20 @@ int pp_copy_result(const pp_domain *dom, int32_t field, int32_t to_blocks, void 
 #define PP_CT(X) reinterpret_cast<const T *>(X)
 
 int pp_setup_advection_matrix_grad(const pp_domain *dom, const void *grad_c, const void *grad_a, void *grad_u,
-                                   void *stream) {
+                                   void *grad_visc, void *stream) {
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(grad_c && grad_u && dom->C.row, "grad_c, grad_u and the C pattern must be set");
-    PP_GRAD_LAUNCH(k_advection_matrix_grad, PP_CT(grad_c), PP_CT(grad_a), PP_T(grad_u));
+    PP_GRAD_LAUNCH(k_advection_matrix_grad, PP_CT(grad_c), PP_CT(grad_a), PP_T(grad_u), PP_T(grad_visc));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
 
 int pp_setup_advection_velocity_grad(const pp_domain *dom, const void *grad_rhs, void *grad_u, void *grad_source,
-                                     void *const *grad_bound, void *stream) {
+                                     void *const *grad_bound, void *grad_visc, void *stream) {
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(grad_rhs && grad_u, "grad_rhs and grad_u must be set");
     PP_GRAD_LAUNCH(k_advection_rhs_grad, PP_CT(grad_rhs), PP_T(grad_u), PP_T(grad_source),
-                   reinterpret_cast<T *const *>(grad_bound));
+                   reinterpret_cast<T *const *>(grad_bound), PP_T(grad_visc));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
@@ -1827,14 +1902,14 @@ int pp_setup_pressure_matrix_grad(const pp_domain *dom, const void *a, const voi
 int pp_setup_pressure_rhs_grad(const pp_domain *dom, const void *grad_hbya, const void *a, const void *c_val,
                                const void *velocity_result, const void *hbya, void *grad_a, void *grad_c,
                                void *grad_velocity_result, void *grad_u, void *grad_source, void *const *grad_bound,
-                               void *stream) {
+                               void *grad_visc, void *stream) {
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(grad_hbya && a && c_val && velocity_result && hbya && grad_a && grad_c && grad_velocity_result &&
                      grad_u && dom->C.row,
                  "null argument");
     PP_GRAD_LAUNCH(k_pressure_rhs_grad, PP_CT(grad_hbya), PP_CT(a), PP_CT(c_val), PP_CT(velocity_result), PP_CT(hbya),
                    PP_T(grad_a), PP_T(grad_c), PP_T(grad_velocity_result), PP_T(grad_u), PP_T(grad_source),
-                   reinterpret_cast<T *const *>(grad_bound));
+                   reinterpret_cast<T *const *>(grad_bound), PP_T(grad_visc));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
@@ -1859,19 +1934,21 @@ int pp_correct_velocity_grad(const pp_domain *dom, const void *grad_out, const v
     return PP_OK;
 }
 
-int pp_compute_epot_rhs_grad(const pp_domain *dom, const void *grad_div, void *grad_field, void *stream) {
+int pp_compute_epot_rhs_grad(const pp_domain *dom, const void *grad_div, void *grad_field, void *const *grad_values,
+                             void *stream) {
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(grad_div && grad_field, "null argument");
-    PP_GRAD_LAUNCH(k_epot_rhs_grad, PP_CT(grad_div), PP_T(grad_field));
+    PP_GRAD_LAUNCH(k_epot_rhs_grad, PP_CT(grad_div), PP_T(grad_field), reinterpret_cast<T *const *>(grad_values));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }
 
 int pp_compute_current_density_grad(const pp_domain *dom, const void *grad_j, void *grad_phi, void *grad_uxb,
-                                    void *stream) {
+                                    void *const *grad_values, void *stream) {
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(grad_j && grad_phi && grad_uxb, "null argument");
-    PP_GRAD_LAUNCH(k_current_density_grad, PP_CT(grad_j), PP_T(grad_phi), PP_T(grad_uxb));
+    PP_GRAD_LAUNCH(k_current_density_grad, PP_CT(grad_j), PP_T(grad_phi), PP_T(grad_uxb),
+                   reinterpret_cast<T *const *>(grad_values));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }

@@ -314,10 +314,11 @@ __device__ __forceinline__ int pot_bc(const pp_block_desc &b, int bound, const i
     return reinterpret_cast<const int8_t *>(fb.pot_types)[face_flat(p, bound, b)];
 }
 template <typename T>
-__device__ __forceinline__ T pot_value(const pp_block_desc &b, int bound, const int *p) {
+__device__ __forceinline__ T pot_value(const pp_block_desc &b, int bound, const int *p, int env) {
     const pp_boundary_desc &fb = b.bnd[bound];
     if (!fb.pot_values) return T(0);
-    return reinterpret_cast<const T *>(fb.pot_values)[face_flat(p, bound, b)];
+    const size_t e = fb.pot_values_batched ? (size_t)env * face_cells(b, bound) : 0;
+    return reinterpret_cast<const T *>(fb.pot_values)[e + face_flat(p, bound, b)];
 }
 __device__ __forceinline__ bool is_epot_dirichlet(const pp_block_desc &b, int bound, const int *p) {
     return b.bnd[bound].type == PP_BOUND_FIXED && pot_bc(b, bound, p) == PP_POT_DIRICHLET;

@@ -536,83 +536,251 @@ __global__ void k_outer_product(const T *__restrict__ a, const T *__restrict__ b
     for (int k = row[i]; k < row[i + 1]; ++k) out[k] = ai * b[col[k]];
 }
 
-template <typename T>
-__global__ void k_segmented_gather(const int32_t *__restrict__ seg, const int32_t *__restrict__ src,
-                                   const T *__restrict__ w, int nOut, const T *__restrict__ in, T *__restrict__ out,
-                                   int inStride, int outStride) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nOut) return;
-    const int m = blockIdx.y;
-    const T *vin = in + (size_t)m * inStride;
-    T s = T(0);
-    for (int t = seg[e]; t < seg[e + 1]; ++t) s += w[t] * vin[src[t]];
-    out[(size_t)m * outStride + e] = s;
+// Galerkin refresh C = R (A P) as two row products, both by k_rap_rows: O = X Y into the fixed
+// pattern of O, O(i, J) = sum_t X(i, t) Y(t, J) over row i of X in column order, every entry
+// accumulated in double by one lane (no atomics, the same bits on every run). A group of G lanes
+// (one wavefront or a part of one) takes an output row; per chunk of the row of X it stages the
+// X entries and the rows of Y they select in LDS with coalesced loads, and each lane then sums
+// its own output entries (e = lane + G r) over the staged rows, finding J by binary search in
+// each (sorted) row of Y. First product X = A, Y = P into the pattern of A P (double
+// workspace); second X = R, Y = A P into the coarse pattern.
+constexpr int RAP_WG = 256;
+constexpr int RAP_OWN = 4;  // output entries per lane and chunk, X entries per lane and chunk
+constexpr int RAP_TPL = 4;  // staged Y entries per lane
+
+// Orders the LDS accesses of a wavefront's lanes around this point (groups never span
+// wavefronts, so no workgroup barrier is needed).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-
-// out[e] = sum_{t in seg(e)} w[t] * in[src[t]] accumulated in double (Galerkin map). W lanes
-// share an entry: lane j takes terms j, j+W, ... (neighbouring lanes read neighbouring terms),
-// then a fixed shuffle tree combines the W partial sums.
-template <int W, typename TA, typename TO>
-__global__ void __launch_bounds__(256) k_galerkin_apply(const int32_t *__restrict__ seg, const int32_t *__restrict__ src,
-                                                        const double *__restrict__ w, int nOut,
-                                                        const TA *__restrict__ in, TO *__restrict__ out, int inStride,
-                                                        int outStride) {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int e = gid / W;
-    const int lane = gid % W;
-    const TA *vin = in + (size_t)blockIdx.y * inStride;
-    double acc = 0.0;
-    if (e < nOut) {
-        const int end = seg[e + 1];
-        for (int t = seg[e] + lane; t < end; t += W) acc += w[t] * double(vin[src[t]]);
-    }
+template <int G, typename TX, typename TO>
+__global__ void __launch_bounds__(RAP_WG) k_rap_rows(int nOut, const int32_t *__restrict__ xrow,
+                                                     const int32_t *__restrict__ xcol, const TX *__restrict__ xval,
+                                                     int64_t xStride, const int32_t *__restrict__ yrow,
+                                                     const int32_t *__restrict__ ycol, const double *__restrict__ yval,
+                                                     int64_t yStride, const int32_t *__restrict__ orow,
+                                                     const int32_t *__restrict__ ocol, TO *__restrict__ oval,
+                                                     int64_t oStride) {
+    constexpr int GPW = RAP_WG / G, CAP = RAP_OWN * G, TCAP = RAP_TPL * G;
+    __shared__ int sYb[GPW][CAP];
+    __shared__ int sPre[GPW][CAP + 1];
+    __shared__ double sXv[GPW][CAP];
+    __shared__ int sYc[GPW][TCAP];
+    __shared__ double sYv[GPW][TCAP];
+    const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+    const int i = blockIdx.x * GPW + grp;
+    if (i >= nOut) return;
+    int *yb = sYb[grp], *pre = sPre[grp], *yc = sYc[grp];
+    double *xv = sXv[grp], *yv = sYv[grp];
+    const TX *X = xval + (int64_t)blockIdx.y * xStride;
+    const double *Y = yval + (int64_t)blockIdx.y * yStride;
+    TO *O = oval + (int64_t)blockIdx.y * oStride;
+    const int ob = orow[i], m = orow[i + 1] - ob;
+    const int xb = xrow[i], nx = xrow[i + 1] - xb;
+    for (int oc = 0; oc < m; oc += CAP) {
+        int J[RAP_OWN];
+        double acc[RAP_OWN];
 #pragma unroll
-    for (int o = W / 2; o > 0; o >>= 1) acc += __shfl_down(acc, o, W);
-    if (e < nOut && lane == 0) out[(size_t)blockIdx.y * outStride + e] = TO(acc);
-}
-
-// Galerkin product C = R A P into the fixed pattern of C (one thread per coarse row,
-// accumulation in double; A may hold nmat value arrays of one pattern).
-template <typename TA, typename TO>
-__global__ void k_galerkin_rap(const int32_t *__restrict__ rrow, const int32_t *__restrict__ rcol, const double *__restrict__ rval,
-                               const int32_t *__restrict__ arow, const int32_t *__restrict__ acol, const TA *__restrict__ aval,
-                               int annz, const int32_t *__restrict__ prow, const int32_t *__restrict__ pcol,
-                               const double *__restrict__ pval, const int32_t *__restrict__ crow, const int32_t *__restrict__ ccol,
-                               TO *__restrict__ cval, int cnnz, int nc, int *bad) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nc) return;
-    const int m = blockIdx.y;
-    const TA *av = aval + (size_t)m * annz;
-    const int cb = crow[i], ce = crow[i + 1];
-    double acc[64];
-    const bool inreg = ce - cb <= 64;
-    TO *out = cval + (size_t)m * cnnz;
-    if (inreg)
-        for (int t = 0; t < ce - cb; ++t) acc[t] = 0.0;
-    else
-        for (int t = cb; t < ce; ++t) out[t] = TO(0);
-    for (int kk = rrow[i]; kk < rrow[i + 1]; ++kk) {
-        const int k = rcol[kk];
-        const double rv = rval[kk];
-        for (int ll = arow[k]; ll < arow[k + 1]; ++ll) {
-            const int l = acol[ll];
-            const double ra = rv * double(av[ll]);
-            for (int jj = prow[l]; jj < prow[l + 1]; ++jj) {
-                const int j = pcol[jj];
-                int lo = cb, hi = ce - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (ccol[mid] < j) lo = mid + 1; else hi = mid;
+        for (int r = 0; r < RAP_OWN; ++r) {
+            const int e = oc + lane + G * r;
+            J[r] = e < m ? ocol[ob + e] : -1;
+            acc[r] = 0.0;
+        }
+        for (int xc = 0; xc < nx; xc += CAP) {
+            const int nxc = nx - xc < CAP ? nx - xc : CAP;
+            for (int t = lane; t < nxc; t += G) {
+                const int k = xcol[xb + xc + t];
+                const int b0 = yrow[k];
+                yb[t] = b0;
+                pre[t + 1] = yrow[k + 1] - b0;
+                xv[t] = double(X[xb + xc + t]);
+            }
+            wave_sync();
+            // offsets of the staged rows: a blocked scan over the group's lanes
+            int len[RAP_OWN], tot = 0;
+#pragma unroll
+            for (int r = 0; r < RAP_OWN; ++r) {
+                const int t = lane * RAP_OWN + r;
+                len[r] = t < nxc ? pre[t + 1] : 0;
+                tot += len[r];
+            }
+            int inc = tot;
+#pragma unroll
+            for (int o = 1; o < G; o <<= 1) {
+                const int v = __shfl_up(inc, o, G);
+                if (lane >= o) inc += v;
+            }
+            wave_sync();
+            int run = inc - tot;
+#pragma unroll
+            for (int r = 0; r < RAP_OWN; ++r) {
+                const int t = lane * RAP_OWN + r;
+                if (t < nxc) {
+                    run += len[r];
+                    pre[t + 1] = run;
                 }
-                if (lo >= ce || ccol[lo] != j) { *bad = 1; continue; }
-                if (inreg) acc[lo - cb] += ra * pval[jj];
-                else out[lo] = TO(double(out[lo]) + ra * pval[jj]);
+            }
+            if (lane == 0) pre[0] = 0;
+            wave_sync();
+            for (int t0 = 0; t0 < nxc;) {
+                // the staged rows t0..t1-1 fit in TCAP entries (a row of Y is at most TCAP long)
+                const int base = pre[t0];
+                int lo = t0 + 1, hi = nxc;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (pre[mid] - base <= TCAP) lo = mid;
+                    else hi = mid - 1;
+                }
+                const int t1 = lo, cnt = pre[t1] - base;
+                for (int j = lane; j < cnt; j += G) {
+                    int a = t0, z = t1 - 1; // the row holding entry j: the last t with pre[t] - base <= j
+                    while (a < z) {
+                        const int mid = (a + z + 1) >> 1;
+                        if (pre[mid] - base <= j) a = mid;
+                        else z = mid - 1;
+                    }
+                    const int q = yb[a] + (j + base - pre[a]);
+                    yc[j] = ycol[q];
+                    yv[j] = Y[q];
+                }
+                wave_sync();
+#pragma unroll
+                for (int r = 0; r < RAP_OWN; ++r) {
+                    if (J[r] < 0) continue;
+                    double s = acc[r];
+                    for (int t = t0; t < t1; ++t) {
+                        int a = pre[t] - base, z = pre[t + 1] - base;
+                        while (a < z) {
+                            const int mid = (a + z) >> 1;
+                            if (yc[mid] < J[r]) a = mid + 1;
+                            else z = mid;
+                        }
+                        if (a < pre[t + 1] - base && yc[a] == J[r]) s += xv[t] * yv[a];
+                    }
+                    acc[r] = s;
+                }
+                wave_sync();
+                t0 = t1;
             }
         }
+#pragma unroll
+        for (int r = 0; r < RAP_OWN; ++r)
+            if (J[r] >= 0) O[ob + oc + lane + G * r] = TO(acc[r]);
     }
-    if (inreg)
-        for (int t = 0; t < ce - cb; ++t) out[cb + t] = TO(acc[t]);
+}
+
+// The same product with one lane per output row, for output rows of at most MAXR entries: the
+// row's columns and sums stay in registers, and each term finds its entry by an unrolled compare
+// against all MAXR columns (no indexed register access). Every X entry, then every entry of
+// its Y row, in column order: the summation order of k_rap_rows.
+template <int MAXR, typename TX, typename TO>
+__global__ void __launch_bounds__(RAP_WG) k_rap_reg(int nOut, const int32_t *__restrict__ xrow,
+                                                    const int32_t *__restrict__ xcol, const TX *__restrict__ xval,
+                                                    int64_t xStride, const int32_t *__restrict__ yrow,
+                                                    const int32_t *__restrict__ ycol, const double *__restrict__ yval,
+                                                    int64_t yStride, const int32_t *__restrict__ orow,
+                                                    const int32_t *__restrict__ ocol, TO *__restrict__ oval,
+                                                    int64_t oStride) {
+    const int i = blockIdx.x * RAP_WG + threadIdx.x;
+    if (i >= nOut) return;
+    const TX *X = xval + (int64_t)blockIdx.y * xStride;
+    const double *Y = yval + (int64_t)blockIdx.y * yStride;
+    TO *O = oval + (int64_t)blockIdx.y * oStride;
+    const int ob = orow[i], m = orow[i + 1] - ob;
+    int cols[MAXR];
+    double acc[MAXR];
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r) {
+        cols[r] = r < m ? ocol[ob + r] : -1;
+        acc[r] = 0.0;
+    }
+    for (int t = xrow[i], te = xrow[i + 1]; t < te; ++t) {
+        const int k = xcol[t];
+        const double xv = double(X[t]);
+        for (int q = yrow[k], qe = yrow[k + 1]; q < qe; ++q) {
+            const int c = ycol[q];
+            const double v = xv * Y[q];
+#pragma unroll
+            for (int r = 0; r < MAXR; ++r)
+                if (cols[r] == c) acc[r] += v;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r)
+        if (r < m) O[ob + r] = TO(acc[r]);
+}
+
+// The same product for rows of Y longer than the staging area: one lane per output row, J found
+// by binary search in the rows of Y in global memory (slow; no such rows arise in Ruge-Stueben
+// hierarchies of the PISO systems).
+template <typename TX, typename TO>
+__global__ void __launch_bounds__(RAP_WG) k_rap_rows_long(int nOut, const int32_t *__restrict__ xrow,
+                                                          const int32_t *__restrict__ xcol, const TX *__restrict__ xval,
+                                                          int64_t xStride, const int32_t *__restrict__ yrow,
+                                                          const int32_t *__restrict__ ycol,
+                                                          const double *__restrict__ yval, int64_t yStride,
+                                                          const int32_t *__restrict__ orow,
+                                                          const int32_t *__restrict__ ocol, TO *__restrict__ oval,
+                                                          int64_t oStride) {
+    const int i = blockIdx.x * RAP_WG + threadIdx.x;
+    if (i >= nOut) return;
+    const TX *X = xval + (int64_t)blockIdx.y * xStride;
+    const double *Y = yval + (int64_t)blockIdx.y * yStride;
+    TO *O = oval + (int64_t)blockIdx.y * oStride;
+    for (int e = orow[i]; e < orow[i + 1]; ++e) {
+        const int J = ocol[e];
+        double s = 0.0;
+        for (int t = xrow[i]; t < xrow[i + 1]; ++t) {
+            const int k = xcol[t];
+            int a = yrow[k], z = yrow[k + 1];
+            while (a < z) {
+                const int mid = (a + z) >> 1;
+                if (ycol[mid] < J) a = mid + 1;
+                else z = mid;
+            }
+            if (a < yrow[k + 1] && ycol[a] == J) s += double(X[t]) * Y[a];
+        }
+        O[e] = TO(s);
+    }
+}
+
+// Output rows of at most 32 entries: one lane per row (k_rap_reg); otherwise the lanes per row of
+// k_rap_rows: the fewest whose chunks hold a row of Y (TCAP entries) and cover the longer rows in
+// a few chunks
+template <typename TX, typename TO>
+static void launch_rap(int nOut, int maxY, int maxRow, int maxOut, int nmat, const int32_t *xrow, const int32_t *xcol,
+                       const TX *xval, int64_t xStride, const int32_t *yrow, const int32_t *ycol, const double *yval,
+                       int64_t yStride, const int32_t *orow, const int32_t *ocol, TO *oval, int64_t oStride,
+                       hipStream_t s) {
+    if (maxOut <= 32) {
+#define RAP_REG(MR)                                                                                               \
+    hipLaunchKernelGGL((k_rap_reg<MR, TX, TO>), dim3((nOut + RAP_WG - 1) / RAP_WG, nmat), dim3(RAP_WG), 0, s, nOut, \
+                       xrow, xcol, xval, xStride, yrow, ycol, yval, yStride, orow, ocol, oval, oStride)
+        if (maxOut <= 16) RAP_REG(16);
+        else RAP_REG(32);
+#undef RAP_REG
+        return;
+    }
+    if (maxY > RAP_TPL * 64) {
+        hipLaunchKernelGGL((k_rap_rows_long<TX, TO>), dim3((nOut + RAP_WG - 1) / RAP_WG, nmat), dim3(RAP_WG), 0, s,
+                           nOut, xrow, xcol, xval, xStride, yrow, ycol, yval, yStride, orow, ocol, oval, oStride);
+        return;
+    }
+    int G = 8;
+    while (G < 64 && (RAP_TPL * G < maxY || RAP_OWN * G < maxRow)) G <<= 1;
+#define RAP_GO(GV)                                                                                                \
+    hipLaunchKernelGGL((k_rap_rows<GV, TX, TO>), dim3((nOut + RAP_WG / GV - 1) / (RAP_WG / GV), nmat), dim3(RAP_WG), \
+                       0, s, nOut, xrow, xcol, xval, xStride, yrow, ycol, yval, yStride, orow, ocol, oval, oStride)
+    switch (G) {
+    case 8: RAP_GO(8); break;
+    case 16: RAP_GO(16); break;
+    case 32: RAP_GO(32); break;
+    default: RAP_GO(64); break;
+    }
+#undef RAP_GO
 }
 
 template <typename T, typename TO>
@@ -699,19 +867,6 @@ int pp_csr_transpose(const pp_csr *A, int32_t dtype, int32_t *t_row, int32_t *t_
     return PP_OK;
 }
 
-int pp_segmented_gather(const int32_t *seg, const int32_t *src, const void *w, int32_t n_out, const void *in, void *out,
-                        int32_t nmat, int32_t in_stride, int32_t out_stride, int32_t dtype, void *stream) {
-    PP_CHECK_ARG(seg && src && w && in && out && n_out >= 0 && nmat >= 1, "null argument");
-    if (n_out == 0) return PP_OK;
-    PP_DISPATCH(dtype,
-        hipLaunchKernelGGL((k_segmented_gather<T>), dim3((n_out + 255) / 256, nmat), dim3(256), 0, as_stream(stream), seg, src,
-                           reinterpret_cast<const T *>(w), n_out, reinterpret_cast<const T *>(in), reinterpret_cast<T *>(out),
-                           in_stride, out_stride));
-    PP_LAUNCH_CHECK();
-    return PP_OK;
-}
-
-
 int pp_ell_from_csr(const int32_t *row, const int32_t *col, const void *val, int32_t n, int32_t nnz, int32_t nmat,
                     int32_t width, int32_t dtype, int32_t *ell_col, void *ell_val, int32_t ell_dtype, void *stream) {
     PP_CHECK_ARG(row && n >= 0 && width > 0 && nmat >= 1, "bad arguments");
@@ -757,59 +912,39 @@ int pp_sell_fill(const int32_t *row, const int32_t *col, const void *val, int32_
     return PP_OK;
 }
 
-int pp_galerkin_apply(const int32_t *seg, const int32_t *src, const double *w, int32_t n_out, int64_t nterms,
-                      const void *a_val,
-                      int32_t a_dtype, void *c_val, int32_t c_dtype, int32_t nmat, int32_t a_stride, int32_t c_stride,
-                      void *stream) {
-    PP_CHECK_ARG(seg && src && w && a_val && c_val && n_out >= 0 && nmat >= 1, "null argument");
-    if (n_out == 0) return PP_OK;
+int pp_galerkin_rows(const pp_csr *A, int32_t a_dtype, const int32_t *p_row, const int32_t *p_col, const double *p_val,
+                     const int32_t *ap_row, const int32_t *ap_col, int32_t ap_nnz, double *ap_val,
+                     const int32_t *r_row, const int32_t *r_col, const double *r_val, const int32_t *c_row,
+                     const int32_t *c_col, void *c_val, int32_t c_n, int32_t c_nnz, int32_t c_dtype,
+                     const int32_t *max_rows, void *stream) {
+    PP_CHECK_ARG(A && A->row && A->col && A->val && p_row && p_col && p_val && ap_row && ap_col && ap_val && r_row &&
+                 r_col && r_val && c_row && c_col && c_val && max_rows, "null argument");
+    PP_CHECK_ARG(A->n >= 0 && c_n >= 0 && ap_nnz >= 0 && c_nnz >= 0 && A->num_matrices >= 1, "sizes");
+    PP_CHECK_ARG(a_dtype == PP_F64 || a_dtype == PP_F32, "a_dtype");
+    PP_CHECK_ARG(c_dtype == PP_F64 || c_dtype == PP_F32, "c_dtype");
+    if (c_n == 0 || A->n == 0) return PP_OK;
     hipStream_t s = as_stream(stream);
-    // lanes per entry from the mean number of terms
-    const double mean = double(nterms) / double(n_out);
-    const int W = mean > 24 ? 16 : (mean > 12 ? 8 : (mean > 6 ? 4 : (mean > 3 ? 2 : 1)));
-    const dim3 g((unsigned)(((size_t)n_out * W + 255) / 256), nmat);
-#define GAL_LAUNCH(WV, TA, TO)                                                                                  \
-    PP_TIMED(PP_K_GALERKIN, n_out, s,                                                                          \
-             hipLaunchKernelGGL((k_galerkin_apply<WV, TA, TO>), g, dim3(256), 0, s, seg, src, w, n_out,          \
-                                (const TA *)a_val, (TO *)c_val, a_stride, c_stride))
-#define GAL_W(TA, TO)                                                                                           \
-    switch (W) {                                                                                                \
-    case 1: GAL_LAUNCH(1, TA, TO); break;                                                                      \
-    case 2: GAL_LAUNCH(2, TA, TO); break;                                                                      \
-    case 4: GAL_LAUNCH(4, TA, TO); break;                                                                      \
-    case 8: GAL_LAUNCH(8, TA, TO); break;                                                                      \
-    default: GAL_LAUNCH(16, TA, TO); break;                                                                    \
-    }
-    if (a_dtype == PP_F64 && c_dtype == PP_F64) { GAL_W(double, double) }
-    else if (a_dtype == PP_F64 && c_dtype == PP_F32) { GAL_W(double, float) }
-    else if (a_dtype == PP_F32 && c_dtype == PP_F32) { GAL_W(float, float) }
-    else if (a_dtype == PP_F32 && c_dtype == PP_F64) { GAL_W(float, double) }
-    else { set_error("pp_galerkin_apply: dtype"); return PP_ERR_ARG; }
-#undef GAL_W
-#undef GAL_LAUNCH
-    PP_LAUNCH_CHECK();
-    return PP_OK;
-}
-
-int pp_galerkin_rap(const int32_t *r_row, const int32_t *r_col, const double *r_val, const pp_csr *A, int32_t a_dtype,
-                    const int32_t *p_row, const int32_t *p_col, const double *p_val, const int32_t *c_row,
-                    const int32_t *c_col, void *c_val, int32_t c_n, int32_t c_nnz, int32_t c_dtype, void *stream) {
-    PP_CHECK_ARG(r_row && r_col && r_val && A && p_row && p_col && p_val && c_row && c_col && c_val, "null argument");
-    if (c_n == 0) return PP_OK;
-    hipStream_t s = as_stream(stream);
-    static thread_local int *bad = nullptr; // one flag per host thread
-    if (!bad) PP_HIP(hipMalloc((void **)&bad, sizeof(int)));
-    PP_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
-    const dim3 g((c_n + 127) / 128, A->num_matrices);
-    if (a_dtype == PP_F64 && c_dtype == PP_F64)
-        hipLaunchKernelGGL((k_galerkin_rap<double, double>), g, dim3(128), 0, s, r_row, r_col, r_val, A->row, A->col, (const double *)A->val, A->nnz, p_row, p_col, p_val, c_row, c_col, (double *)c_val, c_nnz, c_n, bad);
-    else if (a_dtype == PP_F64 && c_dtype == PP_F32)
-        hipLaunchKernelGGL((k_galerkin_rap<double, float>), g, dim3(128), 0, s, r_row, r_col, r_val, A->row, A->col, (const double *)A->val, A->nnz, p_row, p_col, p_val, c_row, c_col, (float *)c_val, c_nnz, c_n, bad);
-    else if (a_dtype == PP_F32 && c_dtype == PP_F32)
-        hipLaunchKernelGGL((k_galerkin_rap<float, float>), g, dim3(128), 0, s, r_row, r_col, r_val, A->row, A->col, (const float *)A->val, A->nnz, p_row, p_col, p_val, c_row, c_col, (float *)c_val, c_nnz, c_n, bad);
-    else if (a_dtype == PP_F32 && c_dtype == PP_F64)
-        hipLaunchKernelGGL((k_galerkin_rap<float, double>), g, dim3(128), 0, s, r_row, r_col, r_val, A->row, A->col, (const float *)A->val, A->nnz, p_row, p_col, p_val, c_row, c_col, (double *)c_val, c_nnz, c_n, bad);
-    else { set_error("pp_galerkin_rap: dtype"); return PP_ERR_ARG; }
+    const int nmat = A->num_matrices;
+    // max_rows: rows of A, P, A P, R, C
+    const int row1 = std::max(max_rows[0], max_rows[2]), row2 = std::max(max_rows[3], max_rows[4]);
+    PP_TIMED(PP_K_GALERKIN_AP, A->n, s, {
+        if (a_dtype == PP_F64)
+            launch_rap<double, double>(A->n, max_rows[1], row1, max_rows[2], nmat, A->row, A->col, (const double *)A->val,
+                                       (int64_t)A->nnz, p_row, p_col, p_val, 0, ap_row, ap_col, ap_val,
+                                       (int64_t)ap_nnz, s);
+        else
+            launch_rap<float, double>(A->n, max_rows[1], row1, max_rows[2], nmat, A->row, A->col, (const float *)A->val,
+                                      (int64_t)A->nnz, p_row, p_col, p_val, 0, ap_row, ap_col, ap_val,
+                                      (int64_t)ap_nnz, s);
+    });
+    PP_TIMED(PP_K_GALERKIN, c_n, s, {
+        if (c_dtype == PP_F64)
+            launch_rap<double, double>(c_n, max_rows[2], row2, max_rows[4], nmat, r_row, r_col, r_val, 0, ap_row, ap_col, ap_val,
+                                       (int64_t)ap_nnz, c_row, c_col, (double *)c_val, (int64_t)c_nnz, s);
+        else
+            launch_rap<double, float>(c_n, max_rows[2], row2, max_rows[4], nmat, r_row, r_col, r_val, 0, ap_row, ap_col, ap_val,
+                                      (int64_t)ap_nnz, c_row, c_col, (float *)c_val, (int64_t)c_nnz, s);
+    });
     PP_LAUNCH_CHECK();
     return PP_OK;
 }

@@ -29,6 +29,14 @@
 namespace pp {
 
 constexpr int WG = 256; // threads per workgroup of every phase and level kernel
+// Workgroups per CU of the phase kernels (the width every workgroup folds at the start of the
+// next phase) and of the level passes.
+#ifndef PP_PHASE_WG_PER_CU
+#define PP_PHASE_WG_PER_CU 8
+#endif
+#ifndef PP_LEVEL_WG_PER_CU
+#define PP_LEVEL_WG_PER_CU 8
+#endif
 
 // ======================================================================================
 // Partial sets and the replicated fold
@@ -1105,7 +1113,7 @@ static int cu_count() {
 // width also bounds the fold).
 static int phase_width(long rows, int gridRows) {
     const long want = std::max<long>(1, (rows + WG - 1) / WG);
-    const long cap = std::max<long>(1, (long)cu_count() * 8 / std::max(1, gridRows));
+    const long cap = std::max<long>(1, (long)cu_count() * PP_PHASE_WG_PER_CU / std::max(1, gridRows));
     return (int)std::min(want, cap);
 }
 
@@ -1595,7 +1603,7 @@ static dim3 level_grid(const LvMat<H> &M, int lay, int B) {
     const int per = lay == LAY_ELL ? 64 : 64 / M.lanes;
     const long waves = (M.n + per - 1) / per;
     const long wgs = std::max<long>(1, (waves + WG / 64 - 1) / (WG / 64));
-    return dim3((unsigned)std::min<long>(wgs, std::max(1, cu_count() * 8 / B)), B);
+    return dim3((unsigned)std::min<long>(wgs, std::max(1, cu_count() * PP_LEVEL_WG_PER_CU / B)), B);
 }
 
 template <int OP, typename TB, typename H, typename TO>

@@ -460,10 +460,19 @@ class FixedBoundary(Boundary):
         self._domain._changed = True
 
     def setPotentialValues(self, values: torch.Tensor) -> None:
-        if values.numel() != self.face_cells():
-            raise ValueError("Potential values must have one entry per face cell")
+        """Dirichlet potential values of the face: one per face cell, shared by the
+        environments, or one slice per environment (``[batch, face...]``). A tensor that
+        requires grad stays in its graph: the differentiable magnetic step returns gradients
+        for it (bc.set_potential_values, src/phipict/bc.py; potentialValuesGrad)."""
+        fc, batch = self.face_cells(), self._domain.getBatchSize()
+        if values.numel() != fc and not (batch > 1 and values.numel() == batch * fc):
+            raise ValueError("Potential values must have one entry per face cell (or one slice per environment)")
         self.potentialValues = values.to(device=self._domain.getDevice(), dtype=self._domain.getDtype()).contiguous().reshape(-1)
         self._domain._changed = True
+
+    def potentialValuesBatched(self) -> bool:
+        """The values hold one slice per environment."""
+        return self.potentialValues is not None and self.potentialValues.numel() != self.face_cells()
 
     def clearPotentialValues(self) -> None:
         self.potentialValues = None
@@ -943,7 +952,9 @@ class Domain:
             viscosity = torch.tensor([float(viscosity)], dtype=dtype)
         if viscosity.dtype != dtype:
             raise TypeError("Data type of viscosity does not match.")
-        self.viscosity = viscosity.detach().cpu().reshape(-1)
+        # a viscosity that requires grad stays in its graph (the differentiable step returns its
+        # gradient); the kernels read its value through the descriptor
+        self.viscosity = viscosity.cpu().reshape(-1)
         self.name = name
         self._blocks: list[Block] = []
         self._batch = 1
@@ -1034,6 +1045,17 @@ class Domain:
             if not b.hasPassiveScalar():
                 b.CreatePassiveScalar()
         self._initialized = False
+
+    def setViscosity(self, viscosity: torch.Tensor) -> None:
+        """The domain's kinematic viscosity (one value; blocks may carry their own). A tensor that
+        requires grad stays in its graph: the differentiable step returns its gradient."""
+        v = viscosity if isinstance(viscosity, torch.Tensor) else torch.tensor([float(viscosity)], dtype=self._dtype)
+        if v.dtype != self._dtype:
+            raise TypeError("Data type of viscosity does not match.")
+        if v.numel() != 1:
+            raise ValueError("the domain viscosity is one value")
+        self.viscosity = v.cpu().reshape(-1)
+        self._changed = True
 
     def setScalarViscosity(self, viscosity: torch.Tensor) -> None:
         """Diffusivity of the passive scalar: one value for every channel or one per channel."""
@@ -1342,6 +1364,9 @@ class Domain:
                 bd.pot_type = int(bound.getPotentialBC())
                 bd.pot_types = _lib.ptr(bound.potentialTypes)
                 bd.pot_values = _lib.ptr(bound.potentialValues)
+                bd.pot_values_batched = 1 if bound.potentialValuesBatched() else 0
+                if bd.pot_values_batched and bound.potentialValues.numel() != self.getBatchSize() * bound.face_cells():
+                    raise ValueError("per-environment potential values do not match the batch size")
                 bd.pot_cw = bound.getPotentialCw()
                 if self._scalar_channels > 0:
                     ps = bound.passiveScalar

@@ -197,7 +197,7 @@ class MHDSimulation(Simulation):
         from . import adjoint as adj
 
         phi, uxb = self._diff_state(domain)
-        J = adj.CurrentDensityFunction.apply(domain, phi, uxb)
+        J = adj.CurrentDensityFunction.apply(domain, phi, uxb, *adj.potential_values(domain))
         e_b = self._e_b.to(device=domain.getDevice())
         N = self._stuart_number.to(device=domain.getDevice(), dtype=domain.getDtype())
         srcs = [b.velocitySource for b in domain.getBlocks()]
@@ -215,7 +215,7 @@ class MHDSimulation(Simulation):
         e_b = self._e_b.to(device=domain.getDevice())
         uxb = adj.CrossEbFunction.apply(U, e_b, total, self._dims)
         inplane = uxb.reshape(batch, 3, total)[:, : self._dims].reshape(-1)
-        rhs = adj.EpotRHSFunction.apply(domain, inplane)
+        rhs = adj.EpotRHSFunction.apply(domain, inplane, *adj.potential_values(domain))
         singular = self._potential_rank_deficient
         if singular:
             rhs = adj._project(rhs, batch)
@@ -236,8 +236,12 @@ class MHDSimulation(Simulation):
         self._diff_phi = phi
         domain.setEpotRHS(rhs.detach().clone())
         domain.setEpotResult(phi.detach().clone())
+        # the blocks' potential: the copy of the result, as tracked tensors of their own (a loss of
+        # block.epot reaches the inputs; later steps replace them instead of writing into them)
+        for blk, e in zip(domain.getBlocks(), adj.flat_to_blocks(domain, phi, 1)):
+            if blk.hasEpot():
+                blk.setEpot(e.reshape(blk.epot.shape))
         domain.UpdateDomainData()
-        ops.CopyEpotResultToBlocks(domain)
         return True
 
     def _magnetic_step_callback(self, domain: Domain, time_step=None, max_iter: Optional[int] = None, **kwargs) -> bool:

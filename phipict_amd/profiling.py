@@ -17,7 +17,7 @@ KERNEL_NAMES = {
     27: "cg_small", 28: "bicg_small",
     30: "pcg_dots", 31: "pcg_direction", 32: "pcg_update", 33: "amg_presmooth", 34: "amg_jacobi",
     35: "amg_level_residual", 36: "amg_restrict", 37: "amg_prolong", 38: "amg_coarse", 39: "mean", 40: "subtract_mean",
-    41: "amg_post_jacobi", 42: "amg_tail", 50: "spmv", 51: "galerkin",
+    41: "amg_post_jacobi", 42: "amg_tail", 50: "spmv", 51: "galerkin", 52: "galerkin_ap",
     60: "seg_cg_direction", 61: "seg_cg_update", 62: "seg_bicg",
 }
 
@@ -73,7 +73,7 @@ TIMER_TO_KERNEL = {
     "cg_direction": "k_cg_turn", "cg_update": "k_cg_step", "bicg_p": "k_bicg_dir", "bicg_v": "k_bicg_apply",
     "bicg_t": "k_bicg_stab", "bicg_update": "k_bicg_step", "pcg_direction": "k_pcg_dir", "pcg_update": "k_cg_step",
     "amg_presmooth": "k_level", "amg_jacobi": "k_level", "amg_post_jacobi": "k_level", "amg_restrict": "k_level",
-    "amg_prolong": "k_level", "amg_tail": "k_vcycle_tail", "galerkin": "k_galerkin_apply",
+    "amg_prolong": "k_level", "amg_tail": "k_vcycle_tail", "galerkin": "k_rap_rc", "galerkin_ap": "k_rap_ap",
     "seg_cg_direction": "k_cg_turn", "seg_cg_update": "k_cg_step",
 }
 #: the level pass of each AMG timer: k_level's first template argument (OP_FIRST, OP_SWEEP,

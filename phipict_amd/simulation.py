@@ -483,7 +483,9 @@ class Simulation:
             return self._get_batched_pressure_amg_hierarchy(P, self.pressure_use_amg)
         use_amg = self.pressure_use_amg
         if use_amg is None:
-            if P.getRows() < self.PRESSURE_AMG_AUTO_MIN_ROWS:
+            # the automatic choice never asks for AMG over independent instances (their
+            # segmented solves are Krylov-only)
+            if P.getRows() < self.PRESSURE_AMG_AUTO_MIN_ROWS or self.domain.hasIndependentBlocks():
                 return None
         elif not use_amg:
             return None
@@ -514,7 +516,8 @@ class Simulation:
         ``PRESSURE_AMG_AUTO_MIN_ROWS_BATCHED`` unknowns over all environments."""
         batch = P.getBatchSize()
         if use_amg is None:
-            use_amg = P.getRows() * batch >= self.PRESSURE_AMG_AUTO_MIN_ROWS_BATCHED
+            use_amg = (P.getRows() * batch >= self.PRESSURE_AMG_AUTO_MIN_ROWS_BATCHED
+                       and not self.domain.hasIndependentBlocks())
         if not use_amg:
             return None
         version = self.domain._a_version
@@ -840,10 +843,11 @@ class Simulation:
             vels = [b.velocity for b in blocks]
             bvels = [bd.velocity for _, _, bd in adj.fixed_faces(domain)]
             srcs = [b.velocitySource for b in blocks]
+            viscs = adj.viscosity_inputs(domain)
             nv, nb = len(vels), len(bvels)
-            A, Cv = adj.AdvectionMatrixFunction.apply(domain, time_step, *vels)
+            A, Cv = adj.AdvectionMatrixFunction.apply(domain, time_step, *vels, *viscs)
             domain._a_version += 1
-            rhs = adj.AdvectionRHSFunction.apply(domain, time_step, nv, nb, *vels, *bvels, *srcs)
+            rhs = adj.AdvectionRHSFunction.apply(domain, time_step, nv, nb, *vels, *bvels, *srcs, *viscs)
             self._run_hook(Hook.POST_VELOCITY_SETUP, no_step=0, local_step=step, time_step=time_step)
             solve_v = self._diff_solver(domain.C, self.advection_use_BiCG, self.advection_tol, "velocity")
             U = adj.LinearSolveFunction.apply(solve_v, domain.C, Cv, rhs, d, False, False, self.adjoint_rtol)
@@ -855,7 +859,7 @@ class Simulation:
                 domain.UpdateDomainData()
                 Pv = adj.PressureMatrixFunction.apply(domain, time_step, A)
                 H, div = adj.PressureRHSFunction.apply(domain, time_step, tsn, A, Cv, U, nv, nb, *vels, *bvels,
-                                                       *srcs)
+                                                       *srcs, *viscs)
                 self._run_hook(Hook.POST_PRESSURE_SETUP, local_step=step, time_step=time_step)
                 singular = self._pressure_rank_deficient(domain.P)
                 solve_p = self._diff_solver(domain.P, self.pressure_use_BiCG,

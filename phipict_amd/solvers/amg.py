@@ -47,8 +47,9 @@ class Level:
     P_row: Optional[torch.Tensor] = None
     P_col: Optional[torch.Tensor] = None
     P_val: Optional[torch.Tensor] = None  # float64
-    # map from this level's operator values to the next level's (seg, src, w): Galerkin refresh
-    gmap: Optional[tuple] = None
+    # pattern of this level's A P (row offsets, columns) for the Galerkin refresh of the next
+    # level, with the longest rows of A, P, A P, R and the next level's operator (int32 [5], host)
+    ap: Optional[tuple] = None
     native: dict = field(default_factory=dict)
     # per sparsity pattern, shared by the refreshes of a hierarchy: sliced ELL layouts of the
     # operator ("A") and of R ("R") (SellLayout), and R / P values per V-cycle dtype ("RP", dtype)
@@ -66,6 +67,8 @@ AUTOTUNE = os.environ.get("PHIPICT_AMD_AUTOTUNE", "1") != "0"
 FORCE_LANES: Optional[int] = None
 #: autotuner decisions: (rows, entries, {lanes: microseconds}, chosen lanes)
 TUNING_LOG: list = []
+#: probe timings within this fraction of the fastest count as ties
+TUNE_MARGIN = 0.1
 
 
 class SellLayout:
@@ -140,7 +143,7 @@ def tuned_layout(row_host: np.ndarray, row_dev: torch.Tensor, col_dev: torch.Ten
         return SellLayout(row_host, row_dev, col_dev, FORCE_LANES)
     if not AUTOTUNE or not val_dev.is_cuda:
         return SellLayout(row_host, row_dev, col_dev, _rule_lanes(avg))
-    timings, best = {}, None
+    timings, layouts = {}, {}
     for lanes in SELL_LANES:
         if lanes > 1 and lanes > 2 * avg:
             break
@@ -151,11 +154,16 @@ def tuned_layout(row_host: np.ndarray, row_dev: torch.Tensor, col_dev: torch.Ten
         us = C.c_double()
         _lib.call("pp_sell_probe", C.byref(desc), n, int(n_cols), _lib.dtype_code(dtype), 20,
                   _lib.stream_ptr(val_dev.device), C.byref(us))
-        timings[lanes] = us.value
-        if best is None or us.value < timings[best.lanes]:
-            best = lay
-    TUNING_LOG.append((n, int(row_host[-1]), timings, best.lanes))
-    return best
+        timings[lanes], layouts[lanes] = us.value, lay
+    # Timings within TUNE_MARGIN of the fastest do not tell the candidates apart (small matrices,
+    # whose probe is launch-bound, or a profiler's per-dispatch cost): among those, the lanes
+    # closest to the rule's, whose lanes walk about eight entries each
+    fastest = min(timings.values())
+    close = [l for l, t in timings.items() if t <= fastest * (1.0 + TUNE_MARGIN)]
+    rule = _rule_lanes(avg)
+    chosen = min(close, key=lambda l: (abs(np.log2(l) - np.log2(rule)), -l))
+    TUNING_LOG.append((n, int(row_host[-1]), timings, chosen))
+    return layouts[chosen]
 
 
 class AMGHierarchy:
@@ -345,26 +353,20 @@ def rs_setup_host(row: np.ndarray, col: np.ndarray, val: np.ndarray, theta: floa
                         np.ascontiguousarray(val, np.float64), theta, max_levels, max_coarse).levels()
 
 
-def galerkin_map_host(fine_row, fine_col, P, R, coarse_row, coarse_col) -> tuple:
-    """Host Galerkin map of one level: (seg, src, w) with C[e] = sum_t w[t] A[src[t]]."""
+def ap_pattern_host(fine_row, fine_col, P, R, coarse_row, coarse_col) -> tuple:
+    """Host pattern of A P for one level, (row offsets, sorted columns); checks that the coarse
+    pattern holds every entry of R (A P)."""
     pr, pc, rr, rc = (np.ascontiguousarray(a, np.int32) for a in (P[0], P[1], R[0], R[1]))
-    pv, rv = np.ascontiguousarray(P[2], np.float64), np.ascontiguousarray(R[2], np.float64)
     fr, fc = np.ascontiguousarray(fine_row, np.int32), np.ascontiguousarray(fine_col, np.int32)
     cr, cc = np.ascontiguousarray(coarse_row, np.int32), np.ascontiguousarray(coarse_col, np.int32)
-    h = C.c_void_p()
-    _lib.call("pp_galerkin_map_build", fr.size - 1, fr.ctypes.data, fc.ctypes.data, cr.size - 1, pr.ctypes.data,
-              pc.ctypes.data, pv.ctypes.data, rr.ctypes.data, rc.ctypes.data, rv.ctypes.data, cr.ctypes.data,
-              cc.ctypes.data, C.byref(h))
-    try:
-        nt = C.c_int64()
-        _lib.call("pp_galerkin_map_size", h, C.byref(nt))
-        seg = np.empty(cc.size + 1, np.int32)
-        src = np.empty(max(nt.value, 1), np.int32)
-        w = np.empty(max(nt.value, 1), np.float64)
-        _lib.call("pp_galerkin_map_copy", h, seg.ctypes.data, src.ctypes.data, w.ctypes.data)
-    finally:
-        _lib.LIB.pp_galerkin_map_free(h)
-    return seg, src[: nt.value], w[: nt.value]
+    n, nc = fr.size - 1, cr.size - 1
+    ap_row = np.empty(n + 1, np.int32)
+    _lib.call("pp_ap_pattern", n, fr.ctypes.data, fc.ctypes.data, nc, pr.ctypes.data, pc.ctypes.data, None, None,
+              None, None, ap_row.ctypes.data, None)
+    ap_col = np.empty(max(int(ap_row[-1]), 1), np.int32)
+    _lib.call("pp_ap_pattern", n, fr.ctypes.data, fc.ctypes.data, nc, pr.ctypes.data, pc.ctypes.data, rr.ctypes.data,
+              rc.ctypes.data, cr.ctypes.data, cc.ctypes.data, ap_row.ctypes.data, ap_col.ctypes.data)
+    return ap_row, ap_col[: int(ap_row[-1])]
 
 
 def _dev(a: np.ndarray, device, dtype=None) -> torch.Tensor:
@@ -398,8 +400,11 @@ def build_amg_hierarchy(csr_mat: CSRmatrix, *, method: str = "ruge_stuben", max_
             level.P_row, level.P_col, level.P_val = _dev(pr, device), _dev(pc, device), _dev(pv, device, torch.float64)
             level.R_row, level.R_col, level.R_val = _dev(rr, device), _dev(rc, device), _dev(rv, device, torch.float64)
             fine_row, fine_col = (row, col) if i == 0 else lv["A"][:2]
-            seg, src, w = galerkin_map_host(fine_row, fine_col, lv["P"], lv["R"], host[i + 1]["A"][0], host[i + 1]["A"][1])
-            level.gmap = (_dev(seg, device), _dev(src, device), _dev(w, device, torch.float64))
+            ap_row, ap_col = ap_pattern_host(fine_row, fine_col, lv["P"], lv["R"], host[i + 1]["A"][0],
+                                             host[i + 1]["A"][1])
+            longest = np.array([np.diff(r).max(initial=0) for r in (fine_row, pr, ap_row, rr, host[i + 1]["A"][0])],
+                               np.int32)
+            level.ap = (_dev(ap_row, device), _dev(ap_col, device), longest)
             if lv["n"] > TAIL_ROWS:
                 # transfers out of and into a level outside the coarse tail run through
                 # sliced copies with autotuned lanes per row
@@ -456,7 +461,7 @@ def galerkin_refresh(hierarchy: AMGHierarchy, csr_mat: CSRmatrix, *, project_con
     A.share_ell_from(csr_mat)
     levels = [Level(n=fine.n, A=A, R_row=fine.R_row, R_col=fine.R_col, R_val=fine.R_val, P_row=fine.P_row,
                     P_col=fine.P_col, P_val=fine.P_val, shared=fine.shared)]
-    levels[0].gmap = fine.gmap
+    levels[0].ap = fine.ap
     if coarse:
         for i in range(1, len(hierarchy.levels)):
             prev = levels[-1]
@@ -464,25 +469,19 @@ def galerkin_refresh(hierarchy: AMGHierarchy, csr_mat: CSRmatrix, *, project_con
             nnz_c, nnz_f = old.A.getNnz(), prev.A.getNnz()
             cval = torch.empty(B * nnz_c, dtype=A.getDtype(), device=A.value.device)
             stream = _lib.stream_ptr(A.value.device)
-            if hierarchy.levels[i - 1].gmap is not None:
-                # one pass of the static map over all B operators
-                seg, src, w = hierarchy.levels[i - 1].gmap
-                _lib.call("pp_galerkin_apply", _lib.ptr(seg), _lib.ptr(src), _lib.ptr(w), nnz_c, src.numel(),
-                          _lib.ptr(prev.A.value), _lib.dtype_code(prev.A.getDtype()), _lib.ptr(cval),
-                          _lib.dtype_code(A.getDtype()), B, nnz_f, nnz_c, stream)
-            else:
-                esize = prev.A.value.element_size()
-                for b in range(B):
-                    d = prev.A.desc()
-                    d.val = _lib.ptr(prev.A.value) + b * nnz_f * esize
-                    d.num_matrices, d.ell_width, d.ell_col, d.ell_val = 1, 0, None, None
-                    _lib.call("pp_galerkin_rap", _lib.ptr(prev.R_row), _lib.ptr(prev.R_col), _lib.ptr(prev.R_val),
-                              C.byref(d), _lib.dtype_code(prev.A.getDtype()), _lib.ptr(prev.P_row),
-                              _lib.ptr(prev.P_col), _lib.ptr(prev.P_val), _lib.ptr(old.A.row), _lib.ptr(old.A.index),
-                              _lib.ptr(cval) + b * nnz_c * cval.element_size(), old.n, nnz_c,
-                              _lib.dtype_code(A.getDtype()), stream)
+            ap_row, ap_col, longest = hierarchy.levels[i - 1].ap
+            # A P of all B operators (double), then R (A P) into the coarse pattern
+            ap_val = torch.empty(B * max(ap_col.numel(), 1), dtype=torch.float64, device=A.value.device)
+            d = prev.A.desc()
+            d.num_matrices, d.ell_width, d.ell_col, d.ell_val = B, 0, None, None
+            _lib.call("pp_galerkin_rows", C.byref(d), _lib.dtype_code(prev.A.getDtype()), _lib.ptr(prev.P_row),
+                      _lib.ptr(prev.P_col), _lib.ptr(prev.P_val), _lib.ptr(ap_row), _lib.ptr(ap_col), ap_col.numel(),
+                      _lib.ptr(ap_val), _lib.ptr(prev.R_row), _lib.ptr(prev.R_col), _lib.ptr(prev.R_val),
+                      _lib.ptr(old.A.row), _lib.ptr(old.A.index), _lib.ptr(cval), old.n, nnz_c,
+                      _lib.dtype_code(A.getDtype()), longest.ctypes.data, stream)
+            del ap_val
             levels.append(Level(n=old.n, A=CSRmatrix(cval, old.A.index, old.A.row), R_row=old.R_row, R_col=old.R_col,
-                                R_val=old.R_val, P_row=old.P_row, P_col=old.P_col, P_val=old.P_val, gmap=old.gmap,
+                                R_val=old.R_val, P_row=old.P_row, P_col=old.P_col, P_val=old.P_val, ap=old.ap,
                                 shared=old.shared))
         coarse_A = _dense(levels[-1].A)
     else:

@@ -196,6 +196,37 @@ def test_uneven_instances_step_like_solo_runs():
         assert all(v <= 1e-10 for v in errs.values()), (i, errs)
 
 
+def test_64_uneven_instances_step_like_solo_runs():
+    """64 Hartmann channels with cell counts log-uniform in [8, 40] per axis, stretching
+    U[1, 2], Re U[50, 500], Ha U[5, 50] (seeded), stepped as one domain of independent blocks
+    with the segmented solves replayed from captured HIP graphs: every instance equals its own
+    solo run (FP64, 1e-10)."""
+    import phipict_amd as pa
+
+    rng = np.random.default_rng(64)
+    sizes = np.rint(np.exp(rng.uniform(np.log(8), np.log(40), (64, 2)))).astype(int)
+    st, re, ha = rng.uniform(1, 2, 64), rng.uniform(50, 500, 64), rng.uniform(5, 50, 64)
+    print("[segmented] 64 uneven MHD channels, 2 steps, graphs", flush=True)
+    dt, steps = 1e-3, 2
+    specs = [_instance_spec(i, int(sizes[i, 0]), int(sizes[i, 1]), float(st[i]), float(re[i])) for i in range(64)]
+    stuart = [float(ha[i] ** 2 / re[i]) for i in range(64)]
+    try:
+        pa.SetKrylovSettings(useGraphs=True)
+        batched = _run(specs, stuart, dt, steps, True)
+    finally:
+        pa.SetKrylovSettings()
+    segs = batched.getSegments()
+    worst = 0.0
+    for i, spec in enumerate(specs):
+        solo = _run([spec], [stuart[i]], dt, steps, False)
+        bb, sb = batched.getBlocks()[i], solo.getBlocks()[0]
+        errs = [rel_l2(to_np(bb.velocity), to_np(sb.velocity)), rel_l2(to_np(bb.pressure), to_np(sb.pressure)),
+                rel_l2(to_np(batched.epotResult)[segs[i]:segs[i + 1]], to_np(solo.epotResult))]
+        worst = max(worst, max(errs))
+        assert max(errs) <= 1e-10, (i, tuple(sizes[i]), errs)
+    print(f"  worst rel L2 over 64 instances: {worst:.2e}", flush=True)
+
+
 def test_captured_chunks_equal_direct_launches():
     """useGraphs: the segmented solves replayed from captured HIP graphs give bit-identical
     solutions and results to direct launches, over repeated calls (replays of cached graphs)."""

@@ -234,10 +234,15 @@ def test_product_does_not_import_oracle():
 
 
 @pytest.mark.parametrize("which", ["pressure2d", "epot3d"])
-def test_galerkin_map_reproduces_rap(which):
-    """The host Galerkin map applied to new fine values gives R A P of those values (the
-    oracle's double-precision product) on every level of the hierarchy."""
-    from phipict_amd.solvers.amg import galerkin_map_host, rs_setup_host
+def test_ap_pattern_is_the_symbolic_product(which):
+    """The host pattern of A P (pp_ap_pattern), on every level of the setup, is the symbolic
+    product of the fine pattern and the interpolation (scipy's product of 0/1 matrices, so no
+    entry cancels), sorted per row; the oracle's Galerkin product R A P fits in the coarse pattern
+    (its pattern is the setup's), and a coarse pattern with one entry of R (A P) removed is
+    refused."""
+    import scipy.sparse as sp
+
+    from phipict_amd.solvers.amg import ap_pattern_host, rs_setup_host
 
     if which == "pressure2d":
         od = O.OracleDomain(cases.channel2d(40, 30), 0.02, 0.01)
@@ -248,19 +253,34 @@ def test_galerkin_map_reproduces_rap(which):
         col, val = od.epot_matrix()
     row = od.row
     levels = rs_setup_host(row, col, val)
-    rng = np.random.default_rng(0)
-    fine = (row, col, val * (1 + 0.3 * rng.uniform(size=val.size)))
+    fine = (row, col, val)
     for l in range(len(levels) - 1):
         lv, nxt = levels[l], levels[l + 1]
-        frow, fcol = (row, col) if l == 0 else lv["A"][:2]
-        seg, src, w = galerkin_map_host(frow, fcol, lv["P"], lv["R"], nxt["A"][0], nxt["A"][1])
-        got = np.add.reduceat(w * fine[2][src], seg[:-1]) if src.size else np.zeros(seg.size - 1)
-        got[np.diff(seg) == 0] = 0.0
-        ref = O.galerkin(fine, lv["P"], lv["R"], nxt["n"])
-        np.testing.assert_array_equal(ref[0], nxt["A"][0])
-        np.testing.assert_array_equal(ref[1], nxt["A"][1])
-        np.testing.assert_allclose(got, ref[2], rtol=1e-12, atol=1e-14 * np.abs(ref[2]).max())
-        fine = ref
+        frow, fcol = fine[0], fine[1]
+        n, nc = frow.size - 1, nxt["n"]
+        ap_row, ap_col = ap_pattern_host(frow, fcol, lv["P"], lv["R"], nxt["A"][0], nxt["A"][1])
+        ones = lambda c: np.ones(c.size)  # noqa: E731
+        Ab = sp.csr_matrix((ones(fcol), fcol, frow), shape=(n, n))
+        Pb = sp.csr_matrix((ones(lv["P"][1]), lv["P"][1], lv["P"][0]), shape=(n, nc))
+        ref = (Ab @ Pb).tocsr()
+        ref.sort_indices()
+        np.testing.assert_array_equal(ap_row, ref.indptr)
+        np.testing.assert_array_equal(ap_col, ref.indices)
+        rap = O.galerkin(fine, lv["P"], lv["R"], nc)
+        np.testing.assert_array_equal(rap[0], nxt["A"][0])
+        np.testing.assert_array_equal(rap[1], nxt["A"][1])
+        fine = rap
+    # drop the first off-diagonal entry of coarse row 0 of the first level
+    lv, nxt = levels[0], levels[1]
+    cr, cc = nxt["A"][0].copy(), nxt["A"][1].copy()
+    drop = next(e for e in range(cr[0], cr[1]) if cc[e] != 0)
+    cc2 = np.delete(cc, drop)
+    cr2 = cr.copy()
+    cr2[1:] -= 1
+    from phipict_amd._lib import PhipictHipError
+
+    with pytest.raises(PhipictHipError, match="misses an entry"):
+        ap_pattern_host(row, col, lv["P"], lv["R"], cr2, cc2)
 
 
 def test_counter_traffic_table(tmp_path):
