@@ -689,17 +689,21 @@ def serving_bench(args) -> int:
 
 
 # ----------------------------------------------------------------------------- algorithmic bytes
-def kernel_bytes(name, rows, s, sh, fine, levels, m=3):
+def kernel_bytes(name, rows, s, sh, fine, levels, m=3, diag=False):
     """Algorithmic HBM bytes of one launch (neighbour reads counted once): ``fine`` = (n, nnz)
     of the system matrices, ``levels`` maps level rows -> (nnz_A, n_coarse, nnz_R, nnz_P).
     s / sh: bytes per value of the system / the AMG hierarchy; m: right-hand sides of the
-    BiCGStab launches (the momentum solve: one per velocity component, sharing the matrix)."""
+    BiCGStab launches (the momentum solve: one per velocity component, sharing the matrix);
+    diag: the grid's metrics are diagonal, so the assembly needs d + 1 metric values per cell
+    (the diagonal of Minv and det; 2d + 1 where M is read too) instead of the full matrices."""
     n, nnz = fine
+    d = 3
+    minv = (d + 1) if diag else (d * d + 1)      # Minv rows and det
+    full = (2 * d + 1) if diag else (2 * d * d + 1)  # M, Minv and det
 
     def M(nz, rws, val):
         return nz * (val + 4) + (rws + 1) * 4
 
-    d = 3
     if name in ("cg_direction", "pcg_direction"):
         return M(nnz, n, s) + 4 * n * s
     if name in ("cg_update", "pcg_update"):
@@ -731,21 +735,21 @@ def kernel_bytes(name, rows, s, sh, fine, levels, m=3):
     if name == "amg_prolong" and lv:
         return M(lv[3], rows, sh) + lv[1] * sh + 2 * rows * sh
     if name == "advection_matrix":
-        return n * (d * s + (2 * d * d + 1) * s + (2 * d + 1) * s + s)
+        return n * (d * s + minv * s + (2 * d + 1) * s + s)
     if name == "advection_rhs":
         return n * ((2 * d + 2) * s)
     if name == "pressure_matrix":
-        return n * (s + (d * d + 1) * s + (2 * d + 1) * s)
+        return n * (s + minv * s + (2 * d + 1) * s)
     if name == "pressure_rhs":
         return n * ((2 * d + 1) * (s + 4) + 4 + 3 * d * s + s)
     if name == "pressure_div":
-        return n * (d * s + d * d * s + s)
+        return n * (d * s + minv * s + s)
     if name == "correct_velocity":
         return n * (2 * s + 2 * d * s)
     if name == "epot_rhs":
-        return n * (3 * s + d * d * s + s)
+        return n * (3 * s + minv * s + s)
     if name == "current_density":
-        return n * (s + 3 * s + (2 * d * d + 1) * s + 3 * s)
+        return n * (s + 3 * s + full * s + 3 * s)
     return None
 
 
@@ -1078,8 +1082,9 @@ def main(argv=None) -> int:
     fine = (n, dom.C.getNnz())
     levels = hierarchy_levels(sim)
     rows = []
+    diag = all(b.transform is None or b.isTransformDiagonal() for b in dom.getBlocks())
     for t in timings:
-        b = kernel_bytes(t.name, t.rows, s, sh, fine, levels)
+        b = kernel_bytes(t.name, t.rows, s, sh, fine, levels, diag=diag)
         rows.append((t, b))
     rows.sort(key=lambda tb: -tb[0].est_total_ms)
     if args.details:

@@ -81,7 +81,8 @@ typedef struct pp_block_desc {
     int32_t vsource_mode;   /* 0 none, 1 static [vb][d], 2 per cell [vb][d][cells] */
     int32_t vsource_batched;/* source has one slice per environment */
     int32_t visc_mode;      /* 0 domain viscosity, 1 static [1], 2 per cell [cells] */
-    int32_t pad0, pad1;
+    int32_t transform_diagonal; /* 1: the off-diagonal entries of every cell's M and Minv are 0 */
+    int32_t pad1;
     uint64_t transform;     /* component-major [2d^2+1][cells]: Block.transform transposed */
     uint64_t velocity;      /* [batch][d][cells] */
     uint64_t pressure;      /* [batch][cells] */

@@ -52,7 +52,7 @@ class BlockDesc(C.Structure):
         ("vsource_mode", C.c_int32),
         ("vsource_batched", C.c_int32),
         ("visc_mode", C.c_int32),
-        ("pad0", C.c_int32),
+        ("transform_diagonal", C.c_int32),
         ("pad1", C.c_int32),
         ("transform", C.c_uint64),
         ("velocity", C.c_uint64),

@@ -326,7 +326,11 @@ __device__ void pressure_grad(const pp_domain &dom, const pp_block_desc *__restr
         const T vp = pressure_at<T>(dom, blocks, bi, t, env);
         gr[dim] = (vp - vn) * fac;
     }
-    if (b.has_transform) {
+    if (b.has_transform && b.transform_diagonal) {
+        const TView<T> t = Metric<T, D>::at(b, f);
+#pragma unroll
+        for (int j = 0; j < D; ++j) gr[j] = t[D * D + j * D + j] * gr[j];
+    } else if (b.has_transform) {
         const TView<T> t = Metric<T, D>::at(b, f);
         T r[D];
 #pragma unroll
@@ -765,7 +769,13 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_current_density(pp_domain dom, co
 #pragma unroll
     for (int k = 0; k < D; ++k) ja[k] = T(0.5) * (jf[k * 2] + jf[k * 2 + 1]);
     T Jv[3] = {T(0), T(0), T(0)};
-    if (b.has_transform) {
+    if (b.has_transform && b.transform_diagonal) {
+        const TView<T> t = Metric<T, D>::at(b, f);
+        const T inv = T(1) / t[2 * D * D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) Jv[k] = (t[k * D + k] * ja[k]) * inv;
+        if constexpr (D == 2) Jv[2] = uxb[g + (size_t)nc * 2];
+    } else if (b.has_transform) {
         const TView<T> t = Metric<T, D>::at(b, f);
         const T inv = T(1) / t[2 * D * D];
         if constexpr (D == 2) {
@@ -1004,6 +1014,10 @@ __device__ __forceinline__ void add_contra_grad(const pp_block_desc &b, int f, i
     }
     const TView<T> t = Metric<T, D>::at(b, f);
     const T s = gval * t[2 * D * D];
+    if (b.transform_diagonal) {
+        atomicAdd(gv + (size_t)w * n + gi, s * t[D * D + w * D + w]);
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < D; ++k) atomicAdd(gv + (size_t)k * n + gi, s * t[D * D + w * D + k]);
 }

@@ -190,9 +190,17 @@ struct Metric {
         }
         return t[2 * D * D] * r;
     }
+    // Blocks whose metrics are diagonal (transform_diagonal: every off-diagonal entry of M and
+    // Minv is 0, e.g. rectilinear grids) read only the diagonal and det: the same values as the
+    // full rows, whose other products are exact zeros, from a third of the loads.
     __device__ static __forceinline__ T alpha(const pp_block_desc &b, int f, int w) {
         if (!b.has_transform) return T(1);
-        return alpha_t(at(b, f), w);
+        const TView<T> t = at(b, f);
+        if (b.transform_diagonal) {
+            const T m = t[D * D + w * D + w];
+            return t[2 * D * D] * (m * m);
+        }
+        return alpha_t(t, w);
     }
     // det * Minv[w] . v (VelocityToContravariantComponent, velocity.cuh:24-39)
     __device__ static __forceinline__ T contra_t(const TView<T> &t, int w, const T *v) {
@@ -204,6 +212,10 @@ struct Metric {
     __device__ static __forceinline__ T minv_norm(const pp_block_desc &b, int f, int axis) {
         if (!b.has_transform) return T(1);
         const TView<T> t = at(b, f);
+        if (b.transform_diagonal) {
+            const T m = t[D * D + axis * D + axis];
+            return dsqrt<T>(m * m);
+        }
         T r = 0;
 #pragma unroll
         for (int i = 0; i < D; ++i) {
@@ -255,6 +267,10 @@ __device__ __forceinline__ T vel_at(const pp_domain &dom, const pp_block_desc &b
 template <typename T, int D>
 __device__ __forceinline__ T contra(const pp_domain &dom, const pp_block_desc &b, int f, int w, const T *glob, int env) {
     if (!b.has_transform) return vel_at<T>(dom, b, f, w, glob, env);
+    if (b.transform_diagonal) {
+        const TView<T> t = Metric<T, D>::at(b, f);
+        return t[2 * D * D] * (t[D * D + w * D + w] * vel_at<T>(dom, b, f, w, glob, env));
+    }
     T v[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) v[k] = vel_at<T>(dom, b, f, k, glob, env);
@@ -274,6 +290,12 @@ __device__ __forceinline__ void contra_all(const pp_domain &dom, const pp_block_
         return;
     }
     const TView<T> t = Metric<T, D>::at(b, f);
+    if (b.transform_diagonal) {
+        const T det = t[2 * D * D];
+#pragma unroll
+        for (int w = 0; w < D; ++w) out[w] = det * (t[D * D + w * D + w] * v[w]);
+        return;
+    }
 #pragma unroll
     for (int w = 0; w < D; ++w) out[w] = Metric<T, D>::contra_t(t, w, v);
 }

@@ -674,8 +674,11 @@ __global__ void __launch_bounds__(RAP_WG) k_rap_rows(int nOut, const int32_t *__
 
 // The same product with one lane per output row, for output rows of at most MAXR entries: the
 // row's columns and sums stay in registers, and each term finds its entry by an unrolled compare
-// against all MAXR columns (no indexed register access). Every X entry, then every entry of
-// its Y row, in column order: the summation order of k_rap_rows.
+// against all MAXR columns (no indexed register access). The row of X is taken RAP_RU entries at
+// a time: their columns, values and Y row bounds are loaded together, then the Y rows are walked
+// side by side (entry s of each), so every round of dependent loads serves RAP_RU terms. A sum's
+// terms arrive in this fixed (chunk, s, u) order on every run.
+constexpr int RAP_RU = 4;
 template <int MAXR, typename TX, typename TO>
 __global__ void __launch_bounds__(RAP_WG) k_rap_reg(int nOut, const int32_t *__restrict__ xrow,
                                                     const int32_t *__restrict__ xcol, const TX *__restrict__ xval,
@@ -697,15 +700,37 @@ __global__ void __launch_bounds__(RAP_WG) k_rap_reg(int nOut, const int32_t *__r
         cols[r] = r < m ? ocol[ob + r] : -1;
         acc[r] = 0.0;
     }
-    for (int t = xrow[i], te = xrow[i + 1]; t < te; ++t) {
-        const int k = xcol[t];
-        const double xv = double(X[t]);
-        for (int q = yrow[k], qe = yrow[k + 1]; q < qe; ++q) {
-            const int c = ycol[q];
-            const double v = xv * Y[q];
+    const int xe = xrow[i + 1];
+    for (int t0 = xrow[i]; t0 < xe; t0 += RAP_RU) {
+        int yb[RAP_RU], len[RAP_RU], k[RAP_RU];
+        double xv[RAP_RU];
 #pragma unroll
-            for (int r = 0; r < MAXR; ++r)
-                if (cols[r] == c) acc[r] += v;
+        for (int u = 0; u < RAP_RU; ++u) {
+            const bool ok = t0 + u < xe;
+            k[u] = ok ? xcol[t0 + u] : -1;
+            xv[u] = ok ? double(X[t0 + u]) : 0.0;
+        }
+        int mlen = 0;
+#pragma unroll
+        for (int u = 0; u < RAP_RU; ++u) {
+            yb[u] = k[u] >= 0 ? yrow[k[u]] : 0;
+            len[u] = k[u] >= 0 ? yrow[k[u] + 1] - yb[u] : 0;
+            mlen = len[u] > mlen ? len[u] : mlen;
+        }
+        for (int s = 0; s < mlen; ++s) {
+            int c[RAP_RU];
+            double v[RAP_RU];
+#pragma unroll
+            for (int u = 0; u < RAP_RU; ++u) {
+                const bool ok = s < len[u];
+                c[u] = ok ? ycol[yb[u] + s] : -1;
+                v[u] = ok ? xv[u] * Y[yb[u] + s] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < RAP_RU; ++u)
+#pragma unroll
+                for (int r = 0; r < MAXR; ++r)
+                    if (cols[r] == c[u]) acc[r] += v[u];
         }
     }
 #pragma unroll

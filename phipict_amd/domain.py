@@ -91,6 +91,14 @@ def _metric_orthogonality(t: torch.Tensor, dims: int) -> tuple:
     return near, exact
 
 
+def _metric_diagonal(t: torch.Tensor, dims: int) -> bool:
+    """Every off-diagonal entry of M and Minv is exactly 0 in the transforms ``[..., 2d^2+1]``
+    (rectilinear grids): the kernels may then read the diagonal alone, with the same results."""
+    m = t.reshape(-1, transform_size(dims))[:, :2 * dims * dims].reshape(-1, 2, dims, dims)
+    off = ~torch.eye(dims, dtype=torch.bool, device=t.device)
+    return bool((m[:, :, off] == 0).all())
+
+
 def transform_vectors(vectors: torch.Tensor, transforms: torch.Tensor, inverse: bool) -> torch.Tensor:
     """Vectors ``[N, d, (z), y, x]`` mapped by the cell transforms ``[1|N, (z), y, x, 2d^2+1]``:
     ``Minv v`` (inverse, physical to computational) or ``M v`` (TransformVectors,
@@ -780,13 +788,21 @@ class Block:
             self._face_ortho = cached
         return cached[1]
 
+    def isTransformDiagonal(self) -> bool:
+        """True when every cell's M and Minv are diagonal matrices (off-diagonal entries exactly
+        0): the library's kernels then read the diagonal of the metric alone."""
+        return self._orthogonality()[2]
+
     def _orthogonality(self) -> tuple:
+        """(near-orthogonal, corner terms exactly zero, diagonal metric), cached per transform
+        (one device sync per new transform)."""
         t = self.transform
         if t is None:
-            return True, True
+            return True, True, True
         key = (self._domain._transform_generation, t.data_ptr(), t._version)
         if getattr(self, "_ortho_key", None) != key:
-            self._ortho = _metric_orthogonality(t, self.getSpatialDims())
+            self._ortho = _metric_orthogonality(t, self.getSpatialDims()) + (
+                _metric_diagonal(t, self.getSpatialDims()),)
             self._ortho_key = key
         return self._ortho
 
@@ -1334,6 +1350,7 @@ class Domain:
         d.global_offset = b.globalOffset
         d.csr_offset = b.csrOffset
         d.has_transform = 1 if b.transform is not None else 0
+        d.transform_diagonal = 1 if (b.transform is not None and b.isTransformDiagonal()) else 0
         d.transform = _lib.ptr(b._transform_component_major())
         d.velocity = _lib.ptr(b.velocity)
         d.pressure = _lib.ptr(b.pressure)
