@@ -1088,6 +1088,11 @@ def main(argv=None) -> int:
         rows.append((t, b))
     rows.sort(key=lambda tb: -tb[0].est_total_ms)
     if args.details:
+        from phipict_amd.solvers import amg as amg_mod
+
+        for entry in amg_mod.RAP_TUNING_LOG:
+            print(f"[bench] refresh kernel tuning: rows {entry[0]} {entry[1]}: {entry[2]} -> {entry[3]}",
+                  file=sys.stderr)
         tot = sum(t.est_total_ms for t, _ in rows)
         print(f"{'kernel':22s} {'rows':>9s} {'launches':>9s} {'avg us':>9s} {'share':>7s} {'GB/s':>8s}", file=sys.stderr)
         for t, b in rows:

@@ -460,12 +460,16 @@ int pp_amg_free(void *handle);
  * then R (A P) into the coarse pattern (c_row, c_col) with c_n rows; every entry accumulated in
  * double in a fixed order. R and P values are double; A holds A->num_matrices value arrays
  * (a_dtype); C gets one value array per A matrix (c_dtype). max_rows (host): the longest rows
- * of A, P, A P, R and C. No memory beyond the arguments. */
+ * of A, P, A P, R and C. plan (host, [2], or NULL for the built-in rule): the kernel of each
+ * product, PP_RAP_* (a kernel that does not fit the rows is refused); stages: bit 0 runs A P,
+ * bit 1 R (A P) (the autotuner times them one at a time). No memory beyond the arguments. */
+enum { PP_RAP_AUTO = 0, PP_RAP_REG16, PP_RAP_REG32, PP_RAP_ROWS8, PP_RAP_ROWS16, PP_RAP_ROWS32, PP_RAP_ROWS64,
+       PP_RAP_LONG };
 int pp_galerkin_rows(const pp_csr *A, int32_t a_dtype, const int32_t *p_row, const int32_t *p_col, const double *p_val,
                      const int32_t *ap_row, const int32_t *ap_col, int32_t ap_nnz, double *ap_val,
                      const int32_t *r_row, const int32_t *r_col, const double *r_val, const int32_t *c_row,
                      const int32_t *c_col, void *c_val, int32_t c_n, int32_t c_nnz, int32_t c_dtype,
-                     const int32_t *max_rows, void *stream);
+                     const int32_t *max_rows, const int32_t *plan, int32_t stages, void *stream);
 /* ELL copy of a CSR matrix (the layout of pp_csr.ell_col / ell_val): entry k of row i at
  * [k][i], rows padded to `width` entries with the row's own column and zero values. Either
  * output may be NULL; no reference counterpart. */

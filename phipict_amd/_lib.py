@@ -213,7 +213,7 @@ SIGNATURES = {
     "pp_amg_level_copy": [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "pp_amg_free": [_vp],
     "pp_galerkin_rows": [_P(CsrDesc), _i32, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32,
-                         _i32, _i32, _vp, _vp],
+                         _i32, _i32, _vp, _vp, _i32, _vp],
     "pp_inverse_diagonal": [_P(CsrDesc), _i32, _vp, _i32, _vp],
     "pp_cast": [_vp, _i32, _vp, _i32, _i64, _vp],
     "pp_ell_from_csr": [_vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32, _vp],

@@ -772,40 +772,69 @@ __global__ void __launch_bounds__(RAP_WG) k_rap_rows_long(int nOut, const int32_
     }
 }
 
-// Output rows of at most 32 entries: one lane per row (k_rap_reg); otherwise the lanes per row of
-// k_rap_rows: the fewest whose chunks hold a row of Y (TCAP entries) and cover the longer rows in
-// a few chunks
+// Kernels of one row product: RAP_AUTO chooses by the longest rows; the others name a kernel
+// (the host autotuner times the ones that fit and keeps the fastest per level and product).
+enum RapKernel { RAP_AUTO = 0, RAP_REG16, RAP_REG32, RAP_ROWS8, RAP_ROWS16, RAP_ROWS32, RAP_ROWS64, RAP_LONG, RAP_KINDS };
+
+// Whether kernel `kind` fits rows of X up to maxX, of Y up to maxY and outputs up to maxOut
+// entries (k_rap_rows stages whole rows of Y: at most RAP_TPL G entries).
+static bool rap_fits(int kind, int maxY, int maxOut) {
+    switch (kind) {
+    case RAP_REG16: return maxOut <= 16;
+    case RAP_REG32: return maxOut <= 32;
+    case RAP_ROWS8: return maxY <= RAP_TPL * 8;
+    case RAP_ROWS16: return maxY <= RAP_TPL * 16;
+    case RAP_ROWS32: return maxY <= RAP_TPL * 32;
+    case RAP_ROWS64: return maxY <= RAP_TPL * 64;
+    case RAP_LONG: return true;
+    default: return false;
+    }
+}
+
+// The rule without tuning: one lane per row for short rows of X and outputs of up to 32
+// entries; otherwise the fewest lanes per row whose chunks hold a row of Y and cover the longer
+// rows in a few chunks; the one-lane fallback for rows of Y beyond the staging area.
+static int rap_rule(int maxX, int maxY, int maxOut) {
+    if (maxOut <= 32 && maxX <= 8) return maxOut <= 16 ? RAP_REG16 : RAP_REG32;
+    if (maxY > RAP_TPL * 64) return RAP_LONG;
+    const int row = std::max(maxX, maxOut);
+    int G = 8, kind = RAP_ROWS8;
+    while (G < 64 && (RAP_TPL * G < maxY || RAP_OWN * G < row)) {
+        G <<= 1;
+        ++kind;
+    }
+    return kind;
+}
+
 template <typename TX, typename TO>
-static void launch_rap(int nOut, int maxY, int maxRow, int maxOut, int nmat, const int32_t *xrow, const int32_t *xcol,
-                       const TX *xval, int64_t xStride, const int32_t *yrow, const int32_t *ycol, const double *yval,
-                       int64_t yStride, const int32_t *orow, const int32_t *ocol, TO *oval, int64_t oStride,
-                       hipStream_t s) {
-    if (maxOut <= 32) {
+static int launch_rap(int kind, int nOut, int maxX, int maxY, int maxOut, int nmat, const int32_t *xrow,
+                      const int32_t *xcol, const TX *xval, int64_t xStride, const int32_t *yrow, const int32_t *ycol,
+                      const double *yval, int64_t yStride, const int32_t *orow, const int32_t *ocol, TO *oval,
+                      int64_t oStride, hipStream_t s) {
+    if (kind == RAP_AUTO) kind = rap_rule(maxX, maxY, maxOut);
+    PP_CHECK_ARG(kind > RAP_AUTO && kind < RAP_KINDS, "unknown row-product kernel");
+    PP_CHECK_ARG(rap_fits(kind, maxY, maxOut), "the row-product kernel does not fit these rows");
 #define RAP_REG(MR)                                                                                               \
     hipLaunchKernelGGL((k_rap_reg<MR, TX, TO>), dim3((nOut + RAP_WG - 1) / RAP_WG, nmat), dim3(RAP_WG), 0, s, nOut, \
                        xrow, xcol, xval, xStride, yrow, ycol, yval, yStride, orow, ocol, oval, oStride)
-        if (maxOut <= 16) RAP_REG(16);
-        else RAP_REG(32);
-#undef RAP_REG
-        return;
-    }
-    if (maxY > RAP_TPL * 64) {
-        hipLaunchKernelGGL((k_rap_rows_long<TX, TO>), dim3((nOut + RAP_WG - 1) / RAP_WG, nmat), dim3(RAP_WG), 0, s,
-                           nOut, xrow, xcol, xval, xStride, yrow, ycol, yval, yStride, orow, ocol, oval, oStride);
-        return;
-    }
-    int G = 8;
-    while (G < 64 && (RAP_TPL * G < maxY || RAP_OWN * G < maxRow)) G <<= 1;
 #define RAP_GO(GV)                                                                                                \
     hipLaunchKernelGGL((k_rap_rows<GV, TX, TO>), dim3((nOut + RAP_WG / GV - 1) / (RAP_WG / GV), nmat), dim3(RAP_WG), \
                        0, s, nOut, xrow, xcol, xval, xStride, yrow, ycol, yval, yStride, orow, ocol, oval, oStride)
-    switch (G) {
-    case 8: RAP_GO(8); break;
-    case 16: RAP_GO(16); break;
-    case 32: RAP_GO(32); break;
-    default: RAP_GO(64); break;
+    switch (kind) {
+    case RAP_REG16: RAP_REG(16); break;
+    case RAP_REG32: RAP_REG(32); break;
+    case RAP_ROWS8: RAP_GO(8); break;
+    case RAP_ROWS16: RAP_GO(16); break;
+    case RAP_ROWS32: RAP_GO(32); break;
+    case RAP_ROWS64: RAP_GO(64); break;
+    default:
+        hipLaunchKernelGGL((k_rap_rows_long<TX, TO>), dim3((nOut + RAP_WG - 1) / RAP_WG, nmat), dim3(RAP_WG), 0, s,
+                           nOut, xrow, xcol, xval, xStride, yrow, ycol, yval, yStride, orow, ocol, oval, oStride);
+        break;
     }
 #undef RAP_GO
+#undef RAP_REG
+    return PP_OK;
 }
 
 template <typename T, typename TO>
@@ -941,35 +970,43 @@ int pp_galerkin_rows(const pp_csr *A, int32_t a_dtype, const int32_t *p_row, con
                      const int32_t *ap_row, const int32_t *ap_col, int32_t ap_nnz, double *ap_val,
                      const int32_t *r_row, const int32_t *r_col, const double *r_val, const int32_t *c_row,
                      const int32_t *c_col, void *c_val, int32_t c_n, int32_t c_nnz, int32_t c_dtype,
-                     const int32_t *max_rows, void *stream) {
+                     const int32_t *max_rows, const int32_t *plan, int32_t stages, void *stream) {
     PP_CHECK_ARG(A && A->row && A->col && A->val && p_row && p_col && p_val && ap_row && ap_col && ap_val && r_row &&
                  r_col && r_val && c_row && c_col && c_val && max_rows, "null argument");
     PP_CHECK_ARG(A->n >= 0 && c_n >= 0 && ap_nnz >= 0 && c_nnz >= 0 && A->num_matrices >= 1, "sizes");
     PP_CHECK_ARG(a_dtype == PP_F64 || a_dtype == PP_F32, "a_dtype");
     PP_CHECK_ARG(c_dtype == PP_F64 || c_dtype == PP_F32, "c_dtype");
+    PP_CHECK_ARG(stages >= 1 && stages <= 3, "stages");
     if (c_n == 0 || A->n == 0) return PP_OK;
     hipStream_t s = as_stream(stream);
     const int nmat = A->num_matrices;
-    // max_rows: rows of A, P, A P, R, C
-    const int row1 = std::max(max_rows[0], max_rows[2]), row2 = std::max(max_rows[3], max_rows[4]);
-    PP_TIMED(PP_K_GALERKIN_AP, A->n, s, {
-        if (a_dtype == PP_F64)
-            launch_rap<double, double>(A->n, max_rows[1], row1, max_rows[2], nmat, A->row, A->col, (const double *)A->val,
-                                       (int64_t)A->nnz, p_row, p_col, p_val, 0, ap_row, ap_col, ap_val,
-                                       (int64_t)ap_nnz, s);
-        else
-            launch_rap<float, double>(A->n, max_rows[1], row1, max_rows[2], nmat, A->row, A->col, (const float *)A->val,
-                                      (int64_t)A->nnz, p_row, p_col, p_val, 0, ap_row, ap_col, ap_val,
-                                      (int64_t)ap_nnz, s);
-    });
-    PP_TIMED(PP_K_GALERKIN, c_n, s, {
-        if (c_dtype == PP_F64)
-            launch_rap<double, double>(c_n, max_rows[2], row2, max_rows[4], nmat, r_row, r_col, r_val, 0, ap_row, ap_col, ap_val,
-                                       (int64_t)ap_nnz, c_row, c_col, (double *)c_val, (int64_t)c_nnz, s);
-        else
-            launch_rap<double, float>(c_n, max_rows[2], row2, max_rows[4], nmat, r_row, r_col, r_val, 0, ap_row, ap_col, ap_val,
-                                      (int64_t)ap_nnz, c_row, c_col, (float *)c_val, (int64_t)c_nnz, s);
-    });
+    // max_rows: the longest rows of A, P, A P, R and C
+    const int k1 = plan ? plan[0] : RAP_AUTO, k2 = plan ? plan[1] : RAP_AUTO;
+    int rc = PP_OK;
+    if (stages & 1)
+        PP_TIMED(PP_K_GALERKIN_AP, A->n, s, {
+            if (a_dtype == PP_F64)
+                rc = launch_rap<double, double>(k1, A->n, max_rows[0], max_rows[1], max_rows[2], nmat, A->row, A->col,
+                                                (const double *)A->val, (int64_t)A->nnz, p_row, p_col, p_val, 0,
+                                                ap_row, ap_col, ap_val, (int64_t)ap_nnz, s);
+            else
+                rc = launch_rap<float, double>(k1, A->n, max_rows[0], max_rows[1], max_rows[2], nmat, A->row, A->col,
+                                               (const float *)A->val, (int64_t)A->nnz, p_row, p_col, p_val, 0, ap_row,
+                                               ap_col, ap_val, (int64_t)ap_nnz, s);
+        });
+    if (rc != PP_OK) return rc;
+    if (stages & 2)
+        PP_TIMED(PP_K_GALERKIN, c_n, s, {
+            if (c_dtype == PP_F64)
+                rc = launch_rap<double, double>(k2, c_n, max_rows[3], max_rows[2], max_rows[4], nmat, r_row, r_col,
+                                                r_val, 0, ap_row, ap_col, ap_val, (int64_t)ap_nnz, c_row, c_col,
+                                                (double *)c_val, (int64_t)c_nnz, s);
+            else
+                rc = launch_rap<double, float>(k2, c_n, max_rows[3], max_rows[2], max_rows[4], nmat, r_row, r_col,
+                                               r_val, 0, ap_row, ap_col, ap_val, (int64_t)ap_nnz, c_row, c_col,
+                                               (float *)c_val, (int64_t)c_nnz, s);
+        });
+    if (rc != PP_OK) return rc;
     PP_LAUNCH_CHECK();
     return PP_OK;
 }

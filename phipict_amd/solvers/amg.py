@@ -369,6 +369,71 @@ def ap_pattern_host(fine_row, fine_col, P, R, coarse_row, coarse_col) -> tuple:
     return ap_row, ap_col[: int(ap_row[-1])]
 
 
+#: kernels of the two row products of a refresh (PP_RAP_* of the library) and their fit:
+#: (name, largest output row, largest Y row) -- None means no limit
+RAP_KERNELS = {1: ("reg16", 16, None), 2: ("reg32", 32, None), 3: ("rows8", None, 32), 4: ("rows16", None, 64),
+               5: ("rows32", None, 128), 6: ("rows64", None, 256), 7: ("long", None, None)}
+#: refresh kernel decisions: (level rows, product, {kernel: microseconds}, chosen)
+RAP_TUNING_LOG: list = []
+
+
+def _rap_fits(kind: int, max_y: int, max_out: int) -> bool:
+    _, out_cap, y_cap = RAP_KERNELS[kind]
+    return (out_cap is None or max_out <= out_cap) and (y_cap is None or max_y <= y_cap)
+
+
+def galerkin_rows(prev_A: CSRmatrix, level: "Level", coarse_A: CSRmatrix, c_val: torch.Tensor, ap_val: torch.Tensor,
+                  batch: int, plan=None, stages: int = 3) -> None:
+    """C = R (A P) of ``level`` into ``c_val`` (the pattern of ``coarse_A``) for ``batch`` operators
+    of ``prev_A``'s pattern: the library's two row products, with the level's kernel plan."""
+    ap_row, ap_col, longest = level.ap[:3]
+    plan_arr = np.asarray(plan if plan is not None else level.ap[3], np.int32)
+    d = prev_A.desc()
+    d.num_matrices, d.ell_width, d.ell_col, d.ell_val = batch, 0, None, None
+    _lib.call("pp_galerkin_rows", C.byref(d), _lib.dtype_code(prev_A.getDtype()), _lib.ptr(level.P_row),
+              _lib.ptr(level.P_col), _lib.ptr(level.P_val), _lib.ptr(ap_row), _lib.ptr(ap_col), ap_col.numel(),
+              _lib.ptr(ap_val), _lib.ptr(level.R_row), _lib.ptr(level.R_col), _lib.ptr(level.R_val),
+              _lib.ptr(coarse_A.row), _lib.ptr(coarse_A.index), _lib.ptr(c_val), coarse_A.getRows(), coarse_A.getNnz(),
+              _lib.dtype_code(c_val.dtype), longest.ctypes.data, plan_arr.ctypes.data, int(stages),
+              _lib.stream_ptr(c_val.device))
+
+
+def tune_rap_plan(prev_A: CSRmatrix, level: "Level", coarse_A: CSRmatrix) -> np.ndarray:
+    """Kernel of each row product of this level's refresh: every kernel that fits the rows is
+    timed on the device (one warm-up, then two launches) on the level's own operator, the fastest
+    kept; decisions go to RAP_TUNING_LOG. Without a GPU or with AUTOTUNE off: the library's rule."""
+    plan = np.zeros(2, np.int32)
+    if not AUTOTUNE or not prev_A.value.is_cuda:
+        return plan
+    _, ap_col, longest = level.ap[:3]
+    ap_val = torch.empty(max(ap_col.numel(), 1), dtype=torch.float64, device=prev_A.value.device)
+    c_val = torch.empty(max(coarse_A.getNnz(), 1), dtype=prev_A.getDtype(), device=prev_A.value.device)
+    for stage, (max_y, max_out) in enumerate(((int(longest[1]), int(longest[2])), (int(longest[2]), int(longest[4])))):
+        cands = [k for k in RAP_KERNELS if _rap_fits(k, max_y, max_out)]
+        if len(cands) > 1 and 7 in cands:
+            cands.remove(7)  # the one-lane fallback only where nothing else fits
+        timings = {}
+        for k in cands:
+            trial = plan.copy()
+            trial[stage] = k
+            galerkin_rows(prev_A, level, coarse_A, c_val, ap_val, 1, trial, 1 << stage)
+            start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            start.record()
+            for _ in range(2):
+                galerkin_rows(prev_A, level, coarse_A, c_val, ap_val, 1, trial, 1 << stage)
+            stop.record()
+            stop.synchronize()
+            timings[k] = 1e3 * start.elapsed_time(stop) / 2
+        best = min(timings, key=timings.get)
+        plan[stage] = best
+        if stage == 0:
+            # the second product reads A P: leave the fastest kernel's values in the workspace
+            galerkin_rows(prev_A, level, coarse_A, c_val, ap_val, 1, plan, 1)
+        RAP_TUNING_LOG.append((level.n, ("A P", "R (A P)")[stage],
+                               {RAP_KERNELS[k][0]: round(t, 1) for k, t in timings.items()}, RAP_KERNELS[best][0]))
+    return plan
+
+
 def _dev(a: np.ndarray, device, dtype=None) -> torch.Tensor:
     t = torch.from_numpy(np.ascontiguousarray(a))
     return t.to(device=device, dtype=dtype) if dtype is not None else t.to(device)
@@ -404,7 +469,7 @@ def build_amg_hierarchy(csr_mat: CSRmatrix, *, method: str = "ruge_stuben", max_
                                              host[i + 1]["A"][1])
             longest = np.array([np.diff(r).max(initial=0) for r in (fine_row, pr, ap_row, rr, host[i + 1]["A"][0])],
                                np.int32)
-            level.ap = (_dev(ap_row, device), _dev(ap_col, device), longest)
+            level.ap = (_dev(ap_row, device), _dev(ap_col, device), longest, np.zeros(2, np.int32))
             if lv["n"] > TAIL_ROWS:
                 # transfers out of and into a level outside the coarse tail run through
                 # sliced copies with autotuned lanes per row
@@ -414,6 +479,9 @@ def build_amg_hierarchy(csr_mat: CSRmatrix, *, method: str = "ruge_stuben", max_
         if i > 0 and lv["n"] > TAIL_ROWS and "P" in lv:
             level.shared["A"] = tuned_layout(lv["A"][0], A.row, A.index, A.value, lv["n"], native_dtype)
         levels.append(level)
+    for lvl, nxt in zip(levels[:-1], levels[1:]):
+        # the refresh kernels of every level, timed on the level's own operator
+        lvl.ap = lvl.ap[:3] + (tune_rap_plan(lvl.A, lvl, nxt.A),)
     return AMGHierarchy(levels, _dense(levels[-1].A), project_constant, num_pre_smooth, num_post_smooth, native_dtype,
                         time.perf_counter() - t0)
 
@@ -466,19 +534,11 @@ def galerkin_refresh(hierarchy: AMGHierarchy, csr_mat: CSRmatrix, *, project_con
         for i in range(1, len(hierarchy.levels)):
             prev = levels[-1]
             old = hierarchy.levels[i]
-            nnz_c, nnz_f = old.A.getNnz(), prev.A.getNnz()
-            cval = torch.empty(B * nnz_c, dtype=A.getDtype(), device=A.value.device)
-            stream = _lib.stream_ptr(A.value.device)
-            ap_row, ap_col, longest = hierarchy.levels[i - 1].ap
+            cval = torch.empty(B * old.A.getNnz(), dtype=A.getDtype(), device=A.value.device)
             # A P of all B operators (double), then R (A P) into the coarse pattern
-            ap_val = torch.empty(B * max(ap_col.numel(), 1), dtype=torch.float64, device=A.value.device)
-            d = prev.A.desc()
-            d.num_matrices, d.ell_width, d.ell_col, d.ell_val = B, 0, None, None
-            _lib.call("pp_galerkin_rows", C.byref(d), _lib.dtype_code(prev.A.getDtype()), _lib.ptr(prev.P_row),
-                      _lib.ptr(prev.P_col), _lib.ptr(prev.P_val), _lib.ptr(ap_row), _lib.ptr(ap_col), ap_col.numel(),
-                      _lib.ptr(ap_val), _lib.ptr(prev.R_row), _lib.ptr(prev.R_col), _lib.ptr(prev.R_val),
-                      _lib.ptr(old.A.row), _lib.ptr(old.A.index), _lib.ptr(cval), old.n, nnz_c,
-                      _lib.dtype_code(A.getDtype()), longest.ctypes.data, stream)
+            ap_val = torch.empty(B * max(hierarchy.levels[i - 1].ap[1].numel(), 1), dtype=torch.float64,
+                                 device=A.value.device)
+            galerkin_rows(prev.A, hierarchy.levels[i - 1], old.A, cval, ap_val, B)
             del ap_val
             levels.append(Level(n=old.n, A=CSRmatrix(cval, old.A.index, old.A.row), R_row=old.R_row, R_col=old.R_col,
                                 R_val=old.R_val, P_row=old.P_row, P_col=old.P_col, P_val=old.P_val, ap=old.ap,
