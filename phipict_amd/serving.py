@@ -123,16 +123,16 @@ def build_duct(req: DuctRequest, device: torch.device):
 
 def estimate_footprint(req: DuctRequest) -> int:
     """Device bytes a request holds while it runs: fields, the three operators with their ELL
-    copies, solver work vectors and the AMG hierarchies (with their Galerkin maps), per cell
-    for the dtype and dimension; a conservative model (measured peaks of the duct stay below
-    it), used for admission."""
+    copies, solver work vectors and the AMG hierarchies (levels, sliced copies, the patterns of
+    A P), per cell for the dtype and dimension; a conservative model (measured peaks of the
+    duct stay below it), used for admission."""
     s = 4 if req.dtype == torch.float32 else 8
     d = req.dims
     per_cell = (
         (4 * d + 8) * s             # fields, flat solve vectors, right-hand sides
         + 3 * (2 * d + 1) * (2 * s + 8)  # C, P, Epot: CSR values, ELL values and columns
         + 14 * s + 8 * 8            # solver work vectors (momentum, pressure, potential)
-        + 2 * (900 if d == 3 else 500)   # two AMG hierarchies: levels, sliced copies, Galerkin maps
+        + 2 * (900 if d == 3 else 500)   # two AMG hierarchies: levels, sliced copies, A P patterns
     )
     return int(per_cell * req.ncells + (64 << 20))
 

@@ -809,3 +809,48 @@ def test_nonortho_laplacian_exact_for_quadratics(d, field):
         uk = u[k * n:(k + 1) * n]
         Lu = det / od.dom.dt * uk - det * O.spmv(od.row, ccol, cval, uk) - od.nonortho_rhs(O.OK_VEL, flags, k, u)
         np.testing.assert_allclose(Lu[ok], 0.37 * lap * (k + 1), rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("setup", ["oracle", "library"])
+def test_ruge_stueben_1d_laplacian_closed_form(setup):
+    """Classical Ruge-Stueben on the 1D Dirichlet Laplacian tridiag(-1, 2, -1) with 2^8 - 1
+    points: every off-diagonal is strong, so the first pass alternates C and F points (F at both
+    ends); an F point interpolates 1/2 from each C neighbour (-a_ij / a_ii times the ratio of its
+    off-diagonal sum to the C part, both -2 inside and -1 at the ends); so every coarse
+    operator R A P is again the Laplacian, halved per level: 2^-l tridiag(-1, 2, -1) on 2^(8-l) - 1
+    points, until at most 64 remain. Independent of both restatements of the setup; a wrong
+    weight, a dropped normalisation or a transposed R fails it."""
+    n = 2 ** 8 - 1
+    i = np.arange(n)
+    rows = np.concatenate([i, i[1:], i[:-1]])
+    cols = np.concatenate([i, i[1:] - 1, i[:-1] + 1])
+    vals = np.concatenate([np.full(n, 2.0), np.full(n - 1, -1.0), np.full(n - 1, -1.0)])
+    order = np.lexsort((cols, rows))
+    row = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=n))]).astype(np.int32)
+    col, val = cols[order].astype(np.int32), vals[order]
+    if setup == "oracle":
+        levels = O.OracleHierarchy(row, col, val).levels
+    else:
+        from phipict_amd.solvers.amg import rs_setup_host
+
+        levels = rs_setup_host(row, col, val)
+    assert [lv["n"] for lv in levels] == [255, 127, 63]
+    for l, lv in enumerate(levels):
+        m = lv["n"]
+        A = _as_sp(lv["A"], (m, m)).toarray()
+        lap = 2.0 ** -l * (2 * np.eye(m) - np.eye(m, k=1) - np.eye(m, k=-1))
+        np.testing.assert_allclose(A, lap, rtol=0, atol=1e-14)
+        if l == len(levels) - 1:
+            break
+        split = np.asarray(lv["split"])
+        np.testing.assert_array_equal(split, np.arange(m) % 2)  # C at the odd points
+        mc = levels[l + 1]["n"]
+        P = _as_sp(lv["P"], (m, mc)).toarray()
+        expect = np.zeros((m, mc))
+        for j in range(mc):
+            c = 2 * j + 1
+            expect[c, j] = 1.0
+            expect[c - 1, j] = 0.5
+            expect[c + 1, j] = 0.5
+        np.testing.assert_allclose(P, expect, rtol=0, atol=1e-15)
+        np.testing.assert_array_equal(_as_sp(lv["R"], (mc, m)).toarray(), P.T)
