@@ -502,6 +502,43 @@ class FixedBoundary(Boundary):
             return bool((self.potentialTypes == int(PotentialBC.THIN_WALL)).any())
         return self._potentialType == PotentialBC.THIN_WALL
 
+    # legacy flag interface, thin wrappers over the PotentialBC above
+    # (domain_structs.h:242-249, boundaries.cpp:968-991)
+    def setEpotCw(self, cw: float) -> None:
+        """cw > 0 makes the face THIN_WALL; 0 turns a THIN_WALL face back to INSULATING."""
+        if not cw >= 0.0:
+            raise ValueError("Epot wall conductance Cw must be non-negative.")
+        if cw > 0.0:
+            self.setPotentialBC(PotentialBC.THIN_WALL, cw)
+        elif self._potentialType == PotentialBC.THIN_WALL:
+            self.setPotentialBC(PotentialBC.INSULATING)
+
+    def hasEpotCw(self) -> bool:
+        return self.hasPotentialThinWall() and self._potentialCw > 0.0
+
+    def getEpotCw(self) -> float:
+        return self._potentialCw if self.hasPotentialThinWall() else 0.0
+
+    def setEpotDirichlet(self, dirichlet: bool) -> None:
+        """True makes the face DIRICHLET; False turns a DIRICHLET face back to INSULATING."""
+        if dirichlet:
+            self.setPotentialBC(PotentialBC.DIRICHLET)
+        elif self._potentialType == PotentialBC.DIRICHLET:
+            self.setPotentialBC(PotentialBC.INSULATING)
+
+    def hasEpotDirichlet(self) -> bool:
+        return self.hasPotentialDirichlet()
+
+    def setEpotInsulating(self, insulating: bool) -> None:
+        """False makes the face OPEN; True turns an OPEN face back to INSULATING."""
+        if not insulating:
+            self.setPotentialBC(PotentialBC.OPEN)
+        elif self._potentialType == PotentialBC.OPEN:
+            self.setPotentialBC(PotentialBC.INSULATING)
+
+    def isEpotInsulating(self) -> bool:
+        return self._potentialType != PotentialBC.OPEN
+
     def GetFluxes(self) -> torch.Tensor:
         """Contravariant boundary fluxes det*Minv[axis].u per face cell, [vb, face cells]."""
         dims = self._domain.getSpatialDims()

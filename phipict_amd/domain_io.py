@@ -258,3 +258,11 @@ def _load_fixed(block, f: int, b: dict, get, with_scalar: bool) -> None:
         bound.setPotentialTypes(types.to(torch.int8), cw=cw)
     elif "potentialBC" in b:
         bound.setPotentialBC(PotentialBC[b["potentialBC"]], cw=cw)
+    else:
+        # files written before the PotentialBC type held independent flags (domain_io.py:202-209)
+        if "epotCw" in b:
+            bound.setEpotCw(b["epotCw"])
+        if "epotInsulating" in b:
+            bound.setEpotInsulating(b["epotInsulating"])
+        if b.get("epotDirichlet", False):
+            bound.setEpotDirichlet(True)

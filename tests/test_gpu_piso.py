@@ -10,6 +10,7 @@ far below the default absolute tolerances."""
 from __future__ import annotations
 
 import math
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -527,6 +528,41 @@ def test_saved_domain_steps_identically(tmp_path):
         runs.append([to_np(t) for t in (b.velocity, b.pressure, b.passiveScalar, dom.epotResult)])
     for a, c in zip(*runs):
         np.testing.assert_array_equal(a, c)
+
+
+def test_reference_schema_fixture_loads_and_steps():
+    """The pair laid out as the reference's save_domain writes it (tests/golden/domain_v2_case,
+    domain_io.py:298-452) loads onto the GPU: the stretched vertex grid of the first block gives
+    cell volumes dx*dy, and the two-block channel (y-periodic, inflow with an OPEN potential,
+    outflow with per-cell THIN_WALL/DIRICHLET/OPEN potential types and values) prepares and
+    takes MHD steps whose solves converge to finite fields."""
+    import phipict_amd as pa
+    from phipict_amd.domain_io import load_domain
+
+    print("[piso] reference-schema fixture", flush=True)
+    golden = Path(__file__).resolve().parent / "golden"
+    dom = load_domain(golden / "domain_v2_case.json", device="cuda")
+    assert dom.getDtype() == torch.float32
+    left, right = dom.getBlocks()
+    assert left.hasVertexCoordinates() and left.vertexCoordinates.is_cuda
+    xs = np.array([0.0, 0.2, 0.45, 0.75, 1.1])
+    ys = np.array([0.0, 0.3, 0.7, 1.0])
+    det = to_np(left.transform)[0, ..., -1]
+    np.testing.assert_allclose(det, np.outer(np.diff(ys), np.diff(xs)), rtol=1e-5)
+    dom = load_domain(golden / "domain_v2_case.json", dtype=torch.float64, device="cuda")
+    dom.PrepareSolve()
+    dom.setTimeStep(1e-2)
+    # the saved outflow velocity is zero: balance it against the inflow before the first step
+    pa.simulation.balance_boundary_fluxes(dom, [dom.getBlocks()[1].getBoundary(1)])
+    sim = pa.MHDSimulation(domain=dom, dt=1e-2, stuart_number=torch.tensor(1.0),
+                           e_b=torch.tensor([0.0, 0.0, 1.0], dtype=torch.float64), non_orthogonal=False,
+                           advection_tol=1e-12, pressure_tol=1e-12, potential_tol=1e-12)
+    for _ in range(3):
+        assert sim.single_step()
+    for b in dom.getBlocks():
+        for t in (b.velocity, b.pressure, b.passiveScalar):
+            assert np.isfinite(to_np(t)).all()
+    assert np.isfinite(to_np(dom.epotResult)).all()
 
 
 def test_pressure_matrix_assembled_once_per_step(monkeypatch):

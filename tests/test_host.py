@@ -414,6 +414,113 @@ def test_domain_save_load_round_trip(tmp_path):
         load_domain(tmp_path / "v3")
 
 
+def test_load_domain_reference_schema_fixture(tmp_path):
+    """A saved pair laid out field by field as the reference's save_domain writes it
+    (domain_io.py:298-452; written by tests/golden/make_domain_v2_fixture.py from numpy and json
+    alone) loads with every field in place: domain and per-channel scalar viscosity, a block with
+    viscosity, source and potential, the connection axes, periodic faces,
+    an inflow with Dirichlet scalars and an OPEN potential, an outflow whose velocity and scalar
+    share one stored tensor, per-cell potential types with cw and Dirichlet values; the legacy
+    potential flags of older files (domain_io.py:202-209) map to the same PotentialBC values."""
+    import json
+    import shutil
+
+    import phipict_amd as pa
+    from phipict_amd.domain_io import load_domain
+
+    golden = Path(__file__).resolve().parent / "golden"
+    meta = json.loads((golden / "domain_v2_case.json").read_text())
+    with np.load(golden / "domain_v2_case.npz") as npz:
+        arr = {k: npz[k] for k in npz.files}
+    # the metrics of vertex coordinates are computed by the library on the GPU (the GPU test
+    # loads the pair as it is); here the first block falls back to the unit grid
+    assert meta["blocks"][0]["vertexCoordinates"] in meta["data_info"]
+    host_meta = json.loads(json.dumps(meta))
+    host_meta["blocks"][0].pop("vertexCoordinates")
+    (tmp_path / "host.json").write_text(json.dumps(host_meta))
+    shutil.copy(golden / "domain_v2_case.npz", tmp_path / "host.npz")
+    dom = load_domain(tmp_path / "host.json")
+    assert dom.name == "two_block_channel" and dom.getSpatialDims() == 2
+    assert dom.getDtype() == torch.float32 and dom.getPassiveScalarChannels() == 2
+    assert dom.getDevice().type == "cpu"
+    torch.testing.assert_close(dom.viscosity.cpu().reshape(-1), torch.tensor([0.015]), rtol=0, atol=0)
+    torch.testing.assert_close(dom.passiveScalarViscosity.cpu(), torch.tensor([0.1, 0.05]), rtol=0, atol=0)
+    left, right = dom.getBlocks()
+    assert (left.name, right.name) == ("inlet_block", "outlet_block")
+    assert left.getSizesZYX() == [3, 4] and right.getSizesZYX() == [3, 3]
+
+    def stored(d, key):
+        return torch.from_numpy(arr[d[key]])
+
+    lb, rb = meta["blocks"]
+    for block, bd in ((left, lb), (right, rb)):
+        for field, key in (("velocity", "velocity"), ("pressure", "pressure"), ("passiveScalar", "scalar")):
+            torch.testing.assert_close(getattr(block, field).cpu(), stored(bd, key), rtol=0, atol=0)
+    torch.testing.assert_close(left.viscosity.cpu(), stored(lb, "viscosity"), rtol=0, atol=0)
+    torch.testing.assert_close(left.velocitySource.cpu(), stored(lb, "velocitySource"), rtol=0, atol=0)
+    torch.testing.assert_close(left.epot.cpu(), stored(lb, "epot"), rtol=0, atol=0)
+    assert not right.hasViscosity() and not right.hasEpot() and not right.hasVelocitySource()
+
+    inflow, to_right, ly0, ly1 = (left.getBoundary(f) for f in range(4))
+    to_left, outflow, ry0, ry1 = (right.getBoundary(f) for f in range(4))
+    assert all(isinstance(b, pa.PeriodicBoundary) for b in (ly0, ly1, ry0, ry1))
+    assert isinstance(to_right, pa.ConnectedBoundary) and to_right.getConnectedBlock() is right
+    assert list(to_right.axes) == [0, 2] and list(to_left.axes) == [1, 2]
+    assert isinstance(inflow, pa.FixedBoundary) and inflow.velocityType == pa.BoundaryConditionType.DIRICHLET
+    torch.testing.assert_close(inflow.velocity.cpu().reshape(-1), torch.tensor([1.0, 0.0]), rtol=0, atol=0)
+    torch.testing.assert_close(inflow.passiveScalar.cpu().reshape(-1), torch.tensor([1.0, 0.25]), rtol=0, atol=0)
+    assert list(inflow.passiveScalarTypes) == [pa.BoundaryConditionType.DIRICHLET] * 2
+    assert inflow.getPotentialBC() == pa.PotentialBC.OPEN and not inflow.hasPotentialTypes()
+    assert isinstance(outflow, pa.FixedBoundary) and outflow.velocityType == pa.BoundaryConditionType.NEUMANN
+    assert list(outflow.passiveScalarTypes) == [pa.BoundaryConditionType.NEUMANN] * 2
+    assert meta["blocks"][1]["boundaries"][1]["velocity"] == meta["blocks"][1]["boundaries"][1]["scalar"]
+    assert outflow.potentialTypes.tolist() == [3, 2, 1] and outflow.getPotentialCw() == 0.25
+    assert outflow.hasPotentialThinWall() and outflow.hasPotentialDirichlet()
+    torch.testing.assert_close(outflow.potentialValues.cpu(), torch.tensor([0.0, 0.5, 0.0]), rtol=0, atol=0)
+
+    # the same pair with the pre-PotentialBC flags of older files
+    legacy = json.loads(json.dumps(host_meta))
+    out_d = legacy["blocks"][1]["boundaries"][1]
+    for key in ("potentialTypes", "potentialBC", "potentialCw", "potentialValues"):
+        out_d.pop(key)
+    out_d["epotCw"] = 0.5
+    in_d = legacy["blocks"][0]["boundaries"][0]
+    in_d.pop("potentialBC")
+    in_d["epotInsulating"] = False
+    (tmp_path / "legacy.json").write_text(json.dumps(legacy))
+    shutil.copy(golden / "domain_v2_case.npz", tmp_path / "legacy.npz")
+    old = load_domain(tmp_path / "legacy", dtype=torch.float64)
+    assert old.getDtype() == torch.float64
+    assert old.getBlocks()[1].getBoundary(1).getPotentialBC() == pa.PotentialBC.THIN_WALL
+    assert old.getBlocks()[1].getBoundary(1).getEpotCw() == 0.5
+    assert old.getBlocks()[0].getBoundary(0).getPotentialBC() == pa.PotentialBC.OPEN
+    out_d.pop("epotCw")
+    out_d["epotDirichlet"] = True
+    (tmp_path / "legacy.json").write_text(json.dumps(legacy))
+    assert load_domain(tmp_path / "legacy").getBlocks()[1].getBoundary(1).getPotentialBC() == pa.PotentialBC.DIRICHLET
+
+
+def test_legacy_potential_flags_map_to_potential_bc():
+    """setEpotCw / setEpotInsulating / setEpotDirichlet set and undo THIN_WALL, OPEN and
+    DIRICHLET (boundaries.cpp:968-991; the reference's test_potential_bc.py:360-370)."""
+    import phipict_amd as pa
+
+    dom = pa.Domain(2, torch.tensor([0.02]), name="flags", device="cpu")
+    bound = dom.CreateBlockWithSize([4, 3]).CloseBoundary("-y")
+    bound.setEpotCw(0.3)
+    assert bound.getPotentialBC() == pa.PotentialBC.THIN_WALL and bound.getEpotCw() == 0.3 and bound.hasEpotCw()
+    bound.setEpotCw(0.0)
+    assert bound.getPotentialBC() == pa.PotentialBC.INSULATING and not bound.hasEpotCw()
+    bound.setEpotInsulating(False)
+    assert bound.getPotentialBC() == pa.PotentialBC.OPEN and not bound.isEpotInsulating()
+    bound.setEpotDirichlet(True)
+    assert bound.getPotentialBC() == pa.PotentialBC.DIRICHLET and bound.hasEpotDirichlet()
+    bound.setEpotDirichlet(False)
+    assert bound.getPotentialBC() == pa.PotentialBC.INSULATING
+    with pytest.raises(ValueError):
+        bound.setEpotCw(-1.0)
+
+
 def test_krylov_settings_keywords():
     """SetKrylovSettings / GetKrylovSettings with the reference's keywords and preconditioner
     names (bindings.cpp:739-763); host-only calls of the library."""
