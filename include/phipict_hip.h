@@ -262,8 +262,9 @@ int pp_csr_pattern(const pp_domain *dom, int32_t *row, int32_t *col, void *strea
  * pressure.cu:104-160); NON_ORTHO_DIRECT_RHS (2) and NON_ORTHO_DIAGONAL_RHS (8) add the explicit
  * corner terms of velocity_result / scalar_result / pressure_result to the right-hand sides
  * (nonortho_rhs.cuh:28-200), which then must be set; NON_ORTHO_DIAGONAL_MATRIX (4) is ignored,
- * as in the reference. pp_setup_epot_matrix ignores the flags (its corner terms are not
- * assembled; the package refuses them on skewed grids). */
+ * as in the reference. pp_setup_epot_matrix takes the matrix bits (potential.cu:103-150); its
+ * corner-term face coefficients weigh both cells with 1/A of the domain's advection diagonal
+ * (first environment), as the reference's interpolation with withA set, so A must be set. */
 
 /* replaces SetupAdvectionMatrixEulerImplicit (csrc/phipict/piso/advection.cu:1230-1250),
  * kernel PISO_build_matrix advection.cu:26-306: writes C (row, col, values) and A. */
@@ -496,6 +497,15 @@ int pp_sell_probe(const pp_sell *M, int32_t rows, int32_t cols, int32_t hdtype, 
 int pp_ap_pattern(int32_t n, const int32_t *a_row, const int32_t *a_col, int32_t nc, const int32_t *p_row,
                   const int32_t *p_col, const int32_t *r_row, const int32_t *r_col, const int32_t *c_row,
                   const int32_t *c_col, int32_t *ap_row, int32_t *ap_col);
+
+/* Inverse of the dense coarsest AMG operator on the device, for a refresh that should not wait
+ * for the host (replaces np.linalg.pinv of the coarsest operator, src/phipict/solvers/amg.py:
+ * 583-587 and 1354-1362, where the operator allows it): C and out are batch row-major n x n
+ * double matrices, n <= 128, one workgroup each, Gauss-Jordan with partial pivoting. rank_one
+ * = 0: out = inv(C). rank_one = 1: out = inv(C + (s/n) 1 1^T) - 1/(s n) 1 1^T with
+ * s = trace(C)/n, which is pinv(C) when C is symmetric with the constant vector spanning its
+ * null space. No memory beyond the arguments. */
+int pp_coarse_inverse(const double *C, double *out, int32_t n, int32_t batch, int32_t rank_one, void *stream);
 /* inverse diagonal of a CSR matrix (rows without a usable diagonal get 1):
  * dinv [num_matrices][n] in dtype_out from values in dtype. */
 int pp_inverse_diagonal(const pp_csr *A, int32_t dtype, void *dinv, int32_t dtype_out, void *stream);

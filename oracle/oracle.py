@@ -581,10 +581,14 @@ class OracleDomain:
         assert self._fn("o_correct_velocity")(C.byref(self.dom), _p(A), _p(prhs), int(time_step_norm), _p(vel)) == 0
         return vel
 
-    def epot_matrix(self):
+    def epot_matrix(self, flags: int = 0, A: np.ndarray = None):
+        """The potential's Laplacian; with non-orthogonal ``flags`` also its corner terms,
+        whose face coefficients weigh the cells with 1/A (A: the advection diagonal)."""
         col = np.zeros(self.nnz, np.int32)
         val = np.zeros(self.nnz, self.dtype)
-        assert self._fn("o_epot_matrix")(C.byref(self.dom), _p(self.row), _p(col), _p(val)) == 0
+        a = None if A is None else np.ascontiguousarray(A, self.dtype)
+        assert self._fn("o_epot_matrix")(C.byref(self.dom), int(flags), None if a is None else _p(a), _p(self.row),
+                                         _p(col), _p(val)) == 0
         return col, val
 
     def epot_rhs(self, vf: np.ndarray) -> np.ndarray:

@@ -964,10 +964,16 @@ int FN(o_correct_velocity)(const ob_domain *dom, const REAL *adiag, const REAL *
     return 0;
 }
 
-/* PISO_build_epot_matrix, potential.cu:29-233 (orthogonal part, useFaceTransform=false) */
-int FN(o_epot_matrix)(const ob_domain *dom, const int32_t *row, int32_t *col, REAL *val) {
+/* PISO_build_epot_matrix, potential.cu:29-233 (useFaceTransform=false). flags: the
+ * non-orthogonal corner terms (potential.cu:103-150), whose face coefficients weigh both cells
+ * with 1/adiag (interpolateNonOrthoLaplaceComponents with withA set, potential.cu:108) although
+ * the orthogonal part uses 1 */
+int FN(o_epot_matrix)(const ob_domain *dom, int flags, const REAL *adiag, const int32_t *row, int32_t *col,
+                      REAL *val) {
     if (!FN(domain_ok)(dom)) return -1;
+    if (flags && !adiag) return -1;
     FN(octx) c = {dom, dom->dims, 2 * dom->dims * dom->dims + 1};
+    FN(ndata) nd = {&c, OK_TYPES, 0, NULL, NULL, 0};
     int bad = 0;
     for (int bi = 0; bi < dom->nblocks; ++bi) {
         const ob_block *b = &dom->blocks[bi];
@@ -995,6 +1001,9 @@ int FN(o_epot_matrix)(const ob_domain *dom, const int32_t *row, int32_t *col, RE
                     rv[0] -= coef;
                     rv[bound + 1] += coef;
                     idx[bound + 1] = ng;
+                    if (flags)
+                        FN(matrix_corner)(&nd, b, p, bound, flags, (REAL)1.0 / adiag[g], 0, adiag, (REAL)1, 1, (REAL)-1,
+                                          &rv[0], rv);
                 } else {
                     idx[bound + 1] = -1;
                     if (FN(is_epot_dirichlet)(b, bound, p)) rv[0] -= ap[dim] * raP;

@@ -218,6 +218,7 @@ SIGNATURES = {
     "pp_cast": [_vp, _i32, _vp, _i32, _i64, _vp],
     "pp_ell_from_csr": [_vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32, _vp],
     "pp_ap_pattern": [_i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "pp_coarse_inverse": [_vp, _vp, _i32, _i32, _i32, _vp],
     "pp_kernel_timer_start": [_i32],
     "pp_kernel_timer_stop": [],
     "pp_kernel_timer_read": [_i32, _vp, _vp, _vp, _vp, _vp, _P(_i32)],

@@ -611,9 +611,13 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_correct_velocity(pp_domain dom, i
 
 // PISO_build_epot_matrix, potential.cu:29-233 (useFaceTransform = false)
 template <typename T, int D>
-__global__ void __launch_bounds__(ASM_BLOCK) k_epot_matrix(pp_domain dom) {
+__global__ void __launch_bounds__(ASM_BLOCK) k_epot_matrix(pp_domain dom, int flags) {
     CELL_PROLOGUE
     if (env != 0) return; // one matrix shared by all environments
+    // the corner terms weigh both cells with 1/A, the advection diagonal of the first
+    // environment, although the orthogonal part uses 1 (interpolateNonOrthoLaplaceComponents
+    // with withA set, potential.cu:103-150)
+    const T *A = reinterpret_cast<const T *>(dom.A);
     int idx[2 * D + 1];
     T rv[2 * D + 1];
 #pragma unroll
@@ -637,6 +641,11 @@ __global__ void __launch_bounds__(ASM_BLOCK) k_epot_matrix(pp_domain dom) {
             rv[0] -= coef;
             rv[bound + 1] += coef;
             idx[bound + 1] = nf + nb.global_offset;
+            if (flags) {
+                const CornerData<T, D, NK_FIXED_ONLY> cd{dom, blocks, 0, 0};
+                matrix_corner_terms<T, D, NK_FIXED_ONLY>(cd, bi, p, f, bound, flags, T(1.0) / A[g], false, A, T(1),
+                                                         true, T(-1), rv[0], rv);
+            }
         } else {
             idx[bound + 1] = -1;
             if (is_epot_dirichlet(b, bound, p)) rv[0] -= ap[dim] * raP;
@@ -1834,11 +1843,13 @@ int pp_correct_velocity(const pp_domain *dom, int32_t time_step_norm, void *stre
 }
 
 int pp_setup_epot_matrix(const pp_domain *dom, int32_t non_ortho_flags, void *stream) {
-    (void)non_ortho_flags;
     if (int e = check_domain(dom, __func__)) return e;
     PP_CHECK_ARG(dom->E.val && dom->E.row, "E must be set");
+    const int flags = non_ortho_flags & (NO_DIRECT_MATRIX | NO_CENTER_MATRIX);
+    PP_CHECK_ARG(!flags || dom->A, "the potential's corner terms need the advection diagonal A");
     PP_DISPATCH(dom->dtype, DIMS_DISPATCH(dom->dims,
-        hipLaunchKernelGGL((k_epot_matrix<T, D>), block_grid(dom, 1), dim3(ASM_BLOCK), 0, as_stream(stream), *dom)));
+        hipLaunchKernelGGL((k_epot_matrix<T, D>), block_grid(dom, 1), dim3(ASM_BLOCK), 0, as_stream(stream), *dom,
+                           flags)));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }

@@ -874,6 +874,117 @@ __global__ void k_cast(const TI *__restrict__ in, TO *__restrict__ out, int64_t 
         out[i] = TO(in[i]);
 }
 
+// Inverse of a dense coarsest operator, one workgroup per operator: Gauss-Jordan elimination
+// with partial pivoting on [M | I] held in registers. Thread t owns column t % 256 of the 2n
+// augmented columns and the rows t / 256 + 4 j (j < 32), so n <= 128. Pivot rows are not
+// swapped: a pivot row is marked used, and row r of the right half ends as row iperm[r] of
+// the inverse. With rank_one, M = C + (s / n) 1 1^T with s = trace(C) / n and the result is
+// inv(M) - 1 / (s n) 1 1^T: the pseudo-inverse of a symmetric C whose null space is the
+// constant vector (for C + s e e^T with unit e spanning both null spaces of C,
+// inv(C + s e e^T) = pinv(C) + e e^T / s).
+constexpr int CINV_MAX = 128;
+constexpr int CINV_THREADS = 1024;
+constexpr int CINV_ROWS = CINV_MAX / (CINV_THREADS / (2 * CINV_MAX)); // rows per thread: 32
+
+__global__ void __launch_bounds__(CINV_THREADS) k_coarse_inverse(const double *__restrict__ Cm,
+                                                                 double *__restrict__ out, int n, int rank_one) {
+    __shared__ double colk[CINV_MAX];
+    __shared__ double prow[2 * CINV_MAX];
+    __shared__ int iperm[CINV_MAX];
+    __shared__ int used[CINV_MAX];
+    __shared__ int piv_s;
+    __shared__ double shift_s;
+    const int t = threadIdx.x, c = t & (2 * CINV_MAX - 1), rg = t / (2 * CINV_MAX);
+    const int groups = CINV_THREADS / (2 * CINV_MAX);
+    const size_t nn = (size_t)n * n;
+    const double *C = Cm + (size_t)blockIdx.x * nn;
+    double *o = out + (size_t)blockIdx.x * nn;
+    if (t < n) {
+        colk[t] = C[(size_t)t * n + t];
+        used[t] = 0;
+    }
+    __syncthreads();
+    if (t == 0) {
+        double tr = 0.0;
+        for (int i = 0; i < n; ++i) tr += colk[i];
+        shift_s = rank_one ? tr / n : 0.0;
+    }
+    __syncthreads();
+    const double add = shift_s / n;
+    double a[CINV_ROWS];
+#pragma unroll
+    for (int j = 0; j < CINV_ROWS; ++j) {
+        const int r = rg + groups * j;
+        double v = 0.0;
+        if (r < n && c < 2 * n) v = c < n ? C[(size_t)r * n + c] + add : (c - n == r ? 1.0 : 0.0);
+        a[j] = v;
+    }
+    for (int k = 0; k < n; ++k) {
+        __syncthreads(); // the previous step's reads of colk and prow are done
+        if (c == k) {
+#pragma unroll
+            for (int j = 0; j < CINV_ROWS; ++j) {
+                const int r = rg + groups * j;
+                if (r < n) colk[r] = a[j];
+            }
+        }
+        __syncthreads();
+        if (t < 64) {
+            // pivot: the unused row of largest |entry| in column k, the lowest index on ties
+            // (a NaN entry counts as 0, so every step finds an unused row: bi < n)
+            double best = -1.0;
+            int bi = CINV_MAX;
+            for (int r = t; r < n; r += 64) {
+                const double v = fabs(colk[r]);
+                const double key = v == v ? v : 0.0;
+                if (!used[r] && key > best) {
+                    best = key;
+                    bi = r;
+                }
+            }
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                const double ob = __shfl_xor(best, off, 64);
+                const int oi = __shfl_xor(bi, off, 64);
+                if (ob > best || (ob == best && oi < bi)) {
+                    best = ob;
+                    bi = oi;
+                }
+            }
+            if (t == 0) {
+                piv_s = bi;
+                used[bi] = 1;
+                iperm[bi] = k;
+            }
+        }
+        __syncthreads();
+        const int piv = piv_s;
+        const double rp = 1.0 / colk[piv];
+#pragma unroll
+        for (int j = 0; j < CINV_ROWS; ++j)
+            if (rg + groups * j == piv) {
+                a[j] *= rp;
+                if (c < 2 * n) prow[c] = a[j];
+            }
+        __syncthreads();
+        const double pc = c < 2 * n ? prow[c] : 0.0;
+#pragma unroll
+        for (int j = 0; j < CINV_ROWS; ++j) {
+            const int r = rg + groups * j;
+            if (r < n && r != piv) a[j] -= colk[r] * pc;
+        }
+    }
+    __syncthreads();
+    if (c >= n && c < 2 * n) {
+        const double corr = rank_one ? 1.0 / (shift_s * n) : 0.0;
+#pragma unroll
+        for (int j = 0; j < CINV_ROWS; ++j) {
+            const int r = rg + groups * j;
+            if (r < n) o[(size_t)iperm[r] * n + (c - n)] = a[j] - corr;
+        }
+    }
+}
+
 } // namespace pp
 
 using namespace pp;
@@ -1007,6 +1118,15 @@ int pp_galerkin_rows(const pp_csr *A, int32_t a_dtype, const int32_t *p_row, con
                                                (float *)c_val, (int64_t)c_nnz, s);
         });
     if (rc != PP_OK) return rc;
+    PP_LAUNCH_CHECK();
+    return PP_OK;
+}
+
+int pp_coarse_inverse(const double *C, double *out, int32_t n, int32_t batch, int32_t rank_one, void *stream) {
+    PP_CHECK_ARG(C && out, "null argument");
+    PP_CHECK_ARG(n >= 1 && n <= CINV_MAX && batch >= 1, "n must lie in [1, 128] and batch be positive");
+    hipLaunchKernelGGL(k_coarse_inverse, dim3(batch), dim3(CINV_THREADS), 0, as_stream(stream), C, out, (int)n,
+                       (int)(rank_one != 0));
     PP_LAUNCH_CHECK();
     return PP_OK;
 }

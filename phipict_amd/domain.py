@@ -1358,8 +1358,11 @@ class Domain:
             raise RuntimeError("Domain is not initialized. Run domain.PrepareSolve() first.")
         if useFaceTransform:
             raise NotImplementedError("face-transform Laplacians are not implemented")
-        if nonOrthoFlags and not self.isOrthogonal():
-            raise NotImplementedError("non-orthogonal correction terms on skewed grids are not implemented")
+        if nonOrthoFlags and not self.nonOrthoTermsVanish() and not bool((self.A != 0).all()):
+            # the corner terms weigh the cells with 1/A (potential.cu:103-150); on an A not yet
+            # assembled the reference's matrix would hold infinities
+            raise ValueError("the potential's non-orthogonal corner terms weigh the cells with 1/A: assemble the "
+                             "advection matrix (domain.A) before SetupEpotOnDomain with non-orthogonal flags")
         self._epot_enabled = True
         self._epot_flags = (nonOrthoFlags, useFaceTransform)
         if self.epotRHS is None or self.epotRHS.numel() != self._batch * self.totalSize:
@@ -1374,7 +1377,8 @@ class Domain:
                 b.CreateEpot()
         self._changed = True
         self.UpdateDomainData()
-        _lib.call("pp_setup_epot_matrix", C.byref(self._desc), int(nonOrthoFlags), _lib.stream_ptr(self._device))
+        flags = 0 if self.nonOrthoTermsVanish() else int(nonOrthoFlags)
+        _lib.call("pp_setup_epot_matrix", C.byref(self._desc), flags, _lib.stream_ptr(self._device))
         self.Epot.mark_ell_valid()
 
     # --- descriptors

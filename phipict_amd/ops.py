@@ -136,11 +136,12 @@ def SetupEpotMatrix(domain: Domain, nonOrthoFlags: int = 0, useFaceTransform: bo
     if domain.Epot is None:
         raise RuntimeError("domain.Epot is not allocated. Call domain.SetupEpotOnDomain() first.")
     flags = _flags(domain, nonOrthoFlags)
-    if flags:
-        # the potential's corner terms (potential.cu:103-150) are not assembled here
-        raise NotImplementedError("non-orthogonal correction terms of the potential matrix are not implemented")
+    if flags and (domain.A is None or not bool((domain.A != 0).all())):
+        # on an A not yet assembled the reference's matrix would hold infinities
+        raise ValueError("the potential's non-orthogonal corner terms weigh the cells with 1/A (potential.cu:"
+                         "103-150): assemble the advection matrix (domain.A) first")
     d = _prep(domain, None)
-    _lib.call("pp_setup_epot_matrix", C.byref(d), 0, _stream(domain))
+    _lib.call("pp_setup_epot_matrix", C.byref(d), int(flags), _stream(domain))
     domain.Epot.mark_ell_valid()
 
 

@@ -69,6 +69,13 @@ FORCE_LANES: Optional[int] = None
 TUNING_LOG: list = []
 #: probe timings within this fraction of the fastest count as ties
 TUNE_MARGIN = 0.1
+# The refreshes invert the coarsest operator on the device (pp_coarse_inverse) when it has at
+# most this many rows and the device inverse of the first operator of the pattern matches the
+# host pseudo-inverse to DEVICE_PINV_RTOL of its largest entry; PHIPICT_AMD_DEVICE_PINV=0 keeps
+# every refresh on the host.
+DEVICE_PINV_MAX = 128
+DEVICE_PINV_RTOL = 1e-9
+DEVICE_PINV = os.environ.get("PHIPICT_AMD_DEVICE_PINV", "1") != "0"
 
 
 class SellLayout:
@@ -179,13 +186,32 @@ class AMGHierarchy:
         self.native_dtype = native_dtype
         self.setup_seconds = setup_seconds
         self.set_coarse(coarse_A)
+        self.device_pinv = self._device_pinv_matches(coarse_A)
 
-    def set_coarse(self, coarse_A: torch.Tensor) -> None:
+    def set_coarse(self, coarse_A: torch.Tensor, device: bool = False) -> None:
         """Dense pseudo-inverse of the coarsest operator, ``[nc, nc]`` or ``[B, nc, nc]`` for
-        batched operators (numpy, float64; amg.py:583-587, 1354-1362)."""
-        dense = coarse_A.detach().to(torch.float64).cpu().numpy()
-        self.coarse_pinv64 = torch.from_numpy(np.linalg.pinv(dense)).to(coarse_A.device)
+        batched operators, in float64: numpy's on the host (amg.py:583-587, 1354-1362), or with
+        ``device`` the device inverse, which the host does not wait for."""
+        if device:
+            self.coarse_pinv64 = device_coarse_inverse(coarse_A, self.project_constant)
+        else:
+            dense = coarse_A.detach().to(torch.float64).cpu().numpy()
+            self.coarse_pinv64 = torch.from_numpy(np.linalg.pinv(dense)).to(coarse_A.device)
         self._pinv_native = {}
+
+    def _device_pinv_matches(self, coarse_A: torch.Tensor) -> bool:
+        """Whether refreshes of this pattern may use the device inverse: the coarsest operator
+        is small enough and, for this first operator, the device inverse (plain, or with the
+        constant null space removed when the constant mode is projected out) equals numpy's
+        pseudo-inverse to DEVICE_PINV_RTOL. A refresh has the same pattern and the same assembly,
+        so it keeps the symmetry and null space this check relies on."""
+        n = coarse_A.shape[-1]
+        if not DEVICE_PINV or coarse_A.device.type != "cuda" or n > DEVICE_PINV_MAX:
+            return False
+        dev = device_coarse_inverse(coarse_A, self.project_constant)
+        scale = float(self.coarse_pinv64.abs().max())
+        err = float((dev - self.coarse_pinv64).abs().max())
+        return scale > 0.0 and np.isfinite(err) and err <= DEVICE_PINV_RTOL * scale
 
     @property
     def dtype(self):
@@ -489,6 +515,18 @@ def build_amg_hierarchy(csr_mat: CSRmatrix, *, method: str = "ruge_stuben", max_
 _DENSE_ROWS: dict = {}
 
 
+def device_coarse_inverse(coarse_A: torch.Tensor, rank_one: bool) -> torch.Tensor:
+    """pp_coarse_inverse of a dense ``[n, n]`` or ``[B, n, n]`` operator on the device: its
+    inverse, or with ``rank_one`` its pseudo-inverse for a symmetric operator whose null space
+    is the constant vector; float64, queued on the current stream."""
+    C_ = coarse_A.detach().to(torch.float64).contiguous()
+    n = C_.shape[-1]
+    out = torch.empty_like(C_)
+    _lib.call("pp_coarse_inverse", _lib.ptr(C_), _lib.ptr(out), n, C_.numel() // (n * n), int(bool(rank_one)),
+              _lib.stream_ptr(C_.device))
+    return out
+
+
 def _dense(A: CSRmatrix) -> torch.Tensor:
     """The operator as a dense float64 ``[n, n]``, or ``[B, n, n]`` for B batched operators."""
     n = A.getRows()
@@ -554,8 +592,9 @@ def galerkin_refresh(hierarchy: AMGHierarchy, csr_mat: CSRmatrix, *, project_con
     h.num_pre_smooth, h.num_post_smooth = hierarchy.num_pre_smooth, hierarchy.num_post_smooth
     h.native_dtype = hierarchy.native_dtype
     h.setup_seconds = time.perf_counter() - t0
+    h.device_pinv = hierarchy.device_pinv and h.project_constant == hierarchy.project_constant
     if coarse_A is not None:
-        h.set_coarse(coarse_A)
+        h.set_coarse(coarse_A, device=h.device_pinv)
     else:
         h.coarse_pinv64 = hierarchy.coarse_pinv64
         h._pinv_native = hierarchy._pinv_native

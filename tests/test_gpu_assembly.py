@@ -273,7 +273,8 @@ def test_nonortho_assembly_matches_oracle(name, flags, dtype):
 
 def test_nonortho_flags_skip_exactly_orthogonal_grids():
     """Rectilinear grids take the orthogonal kernels (their corner terms are exactly zero), a
-    skewed grid does not, and the potential matrix's corner terms are refused."""
+    skewed grid does not, and the potential matrix's corner terms are refused while the
+    advection diagonal they weigh the cells with is not assembled."""
     import phipict_amd as pa
 
     rect = build_domain(cases.channel2d(8, 6), NU, DT, torch.float64, epot=True)
@@ -286,8 +287,33 @@ def test_nonortho_flags_skip_exactly_orthogonal_grids():
     pa.SetupEpotMatrix(rect, O.NO_DEFAULT_MODE)
     skewed = build_domain(cases.skew(cases.channel2d(8, 6), 0.2, seed=1), NU, DT, torch.float64, epot=True)
     assert not skewed.nonOrthoTermsVanish()
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(ValueError):
         pa.SetupEpotMatrix(skewed, O.NO_DEFAULT_MODE)
+
+
+@pytest.mark.parametrize("name", ["potbox2d", "potbox3d", "tjunction3d", "rotated3d"])
+def test_epot_matrix_corner_terms(dtype, name):
+    """The potential matrix with its non-orthogonal corner terms on skewed grids (potential.cu:
+    103-150: the face coefficients weigh both cells with 1/A of the advection diagonal, the
+    orthogonal part with 1; walks through connections, periodic seams and fixed faces, thin-wall
+    and Dirichlet potential faces) against the oracle, for each matrix flag."""
+    print(f"[nonortho] epot case={name} dtype={np.dtype(dtype).name}", flush=True)
+    import phipict_amd as pa
+
+    specs = cases.skew(cases.CASES[name](), 0.25, seed=3)
+    od = O.OracleDomain(specs, NU, DT, dtype)
+    dom = build_domain(specs, NU, DT, _tdt(dtype), epot=True)
+    assert not dom.nonOrthoTermsVanish()
+    ts = torch.tensor([DT], dtype=_tdt(dtype))
+    for flags in (O.NO_DIRECT_MATRIX, O.NO_CENTER_MATRIX, O.NO_DEFAULT_MODE):
+        pa.SetupAdvectionMatrix(dom, ts, flags)
+        _, _, A = od.advection_matrix(flags)
+        pa.SetupEpotMatrix(dom, flags)
+        ecol, evalv = od.epot_matrix(flags, A)
+        np.testing.assert_array_equal(to_np(dom.Epot.index), ecol)
+        close(to_np(dom.Epot.value), evalv, dtype, f"Epot matrix with corner terms (flags {flags})")
+        plain = od.epot_matrix()[1]
+        assert np.abs(evalv - plain).max() > 1e-6 * np.abs(plain).max()
 
 
 @pytest.mark.parametrize("dims", [2, 3])

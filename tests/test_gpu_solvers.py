@@ -460,6 +460,56 @@ def test_galerkin_refresh_matches_oracle():
     assert h3.coarse_pinv64 is h.coarse_pinv64
 
 
+def test_coarse_inverse_matches_numpy():
+    """pp_coarse_inverse: the inverse of nonsymmetric operators that need row pivoting (a zero
+    leading diagonal), batched, at 1 and 128 rows, equals numpy's inverse; with the rank-one
+    term, the pseudo-inverse of a singular periodic Laplacian (null space: the constants)
+    equals numpy's pinv; more than 128 rows is refused. A refreshed hierarchy whose first
+    operator passed the check inverts its coarsest operator on the device, and that equals
+    numpy's pinv of the refreshed operator."""
+    print("[amg] coarse inverse", flush=True)
+    from phipict_amd import _lib
+    from phipict_amd.solvers import amg
+
+    rng = np.random.default_rng(11)
+    for n, batch in ((1, 1), (37, 3), (128, 2)):
+        M = rng.standard_normal((batch, n, n)) + 3.0 * np.eye(n)
+        if n > 1:
+            M[:, 0, 0] = 0.0  # the first column needs a row swap
+        got = to_np(amg.device_coarse_inverse(torch.as_tensor(M, device=DEV), False))
+        ref = np.linalg.inv(M)
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-11 * np.abs(ref).max())
+    # periodic 10 x 12 Laplacian with varying symmetric face coefficients: singular, constant null space
+    nx, ny = 10, 12
+    L = np.zeros((nx * ny, nx * ny))
+    for j in range(ny):
+        for i in range(nx):
+            a = j * nx + i
+            for b in (j * nx + (i + 1) % nx, ((j + 1) % ny) * nx + i):
+                w = 1.0 + rng.random()
+                L[a, b] -= w
+                L[b, a] -= w
+                L[a, a] += w
+                L[b, b] += w
+    got = to_np(amg.device_coarse_inverse(torch.as_tensor(L, device=DEV), True))
+    ref = np.linalg.pinv(L)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-11 * np.abs(ref).max())
+    np.testing.assert_allclose(got, got.T, rtol=0, atol=1e-11 * np.abs(ref).max())
+    big = torch.zeros(129, 129, dtype=torch.float64, device=DEV)
+    with pytest.raises(RuntimeError):
+        _lib.call("pp_coarse_inverse", _lib.ptr(big), _lib.ptr(big), 129, 1, 0, _lib.stream_ptr(big.device))
+    # the refresh path
+    row, col, val = poisson(64, 48, shift=0.0)
+    _, _, val2 = poisson(64, 48, shift=0.0, seed=9)
+    amg.clear_interpolation_cache()
+    h = amg.build_amg_hierarchy(csr(row, col, val, np.float64), project_constant=True)
+    assert h.device_pinv, "the device inverse of the first coarsest operator differs from numpy's pinv"
+    h2 = amg.galerkin_refresh(h, csr(row, col, 1.3 * val2, np.float64))
+    coarse = to_np(amg._dense(h2.levels[-1].A))
+    ref = np.linalg.pinv(coarse)
+    np.testing.assert_allclose(to_np(h2.coarse_pinv64), ref, rtol=0, atol=1e-10 * np.abs(ref).max())
+
+
 @pytest.mark.parametrize("combo", ["f64/f64", "f64/f32"])
 def test_batched_amg_pcg_matches_oracle(combo):
     """Two operators of one pattern (batched environments) on the interpolation of the first
