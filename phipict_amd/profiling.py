@@ -65,7 +65,8 @@ def read() -> list:
 
 
 # ----------------------------------------------------------------------------- memory counters
-# rocprofv3 names of the library kernels behind the timer's kernel names
+# rocprofv3 names of the library kernels behind the timer's kernel names (the two refresh
+# products run whichever of k_rap_reg, k_rap_rows and k_rap_rows_long the level's plan chose)
 TIMER_TO_KERNEL = {
     "advection_matrix": "k_advection_matrix", "advection_rhs": "k_advection_rhs", "pressure_matrix": "k_pressure_matrix",
     "pressure_rhs": "k_pressure_rhs", "pressure_div": "k_pressure_div", "correct_velocity": "k_correct_velocity",
@@ -73,7 +74,7 @@ TIMER_TO_KERNEL = {
     "cg_direction": "k_cg_turn", "cg_update": "k_cg_step", "bicg_p": "k_bicg_dir", "bicg_v": "k_bicg_apply",
     "bicg_t": "k_bicg_stab", "bicg_update": "k_bicg_step", "pcg_direction": "k_pcg_dir", "pcg_update": "k_cg_step",
     "amg_presmooth": "k_level", "amg_jacobi": "k_level", "amg_post_jacobi": "k_level", "amg_restrict": "k_level",
-    "amg_prolong": "k_level", "amg_tail": "k_vcycle_tail", "galerkin": "k_rap_rc", "galerkin_ap": "k_rap_ap",
+    "amg_prolong": "k_level", "amg_tail": "k_vcycle_tail", "galerkin": "k_rap_", "galerkin_ap": "k_rap_",
     "seg_cg_direction": "k_cg_turn", "seg_cg_update": "k_cg_step",
 }
 #: the level pass of each AMG timer: k_level's first template argument (OP_FIRST, OP_SWEEP,
